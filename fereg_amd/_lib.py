@@ -1,0 +1,144 @@
+"""ctypes declarations of the C ABI (include/fereg_hip.h)."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(HERE, "libfereg_hip.so")
+
+FH_OK = 0
+FH_INTERRUPTED = 1
+FH_INVALID = 198
+FH_CONVERGENCE = 430
+FH_OOM = 909
+FH_TOO_FEW = 2001
+FH_INTERNAL = 9999
+FH_PLAIN_FP64 = 0
+FH_MIXED = 1
+FH_VCOV = {"iid": 0, "hetero": 1, "cluster": 2}
+FH_KFIXEF = {"none": 0, "full": 1, "nonnested": 2}
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+u64p = ctypes.POINTER(ctypes.c_uint64)
+i32p = ctypes.POINTER(ctypes.c_int32)
+i64p = ctypes.POINTER(ctypes.c_int64)
+f64p = ctypes.POINTER(ctypes.c_double)
+vp = ctypes.c_void_p
+
+
+class FeregError(RuntimeError):
+    """An error code of the reference's convention (error.rs:6-37) with its message."""
+
+    def __init__(self, rc, msg):
+        super().__init__(f"[{rc}] {msg}")
+        self.rc = rc
+
+
+class Term(ctypes.Structure):
+    _fields_ = [("codes", u32p), ("g", ctypes.c_uint64), ("intercept", ctypes.c_int32)]
+
+
+class SolverConfig(ctypes.Structure):
+    _fields_ = [("small_limit", ctypes.c_uint64), ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class SolveOptsC(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("maxiter", ctypes.c_uint64), ("want_coefs", ctypes.c_int32),
+                ("precision", ctypes.c_int32)]
+
+
+class SolveInfoC(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_uint64), ("escalated", ctypes.c_int32), ("exact", ctypes.c_int32),
+                ("refinements", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+
+
+class Ctx(ctypes.Structure):
+    _fields_ = [("interrupt", ctypes.CFUNCTYPE(ctypes.c_int, vp)), ("log", ctypes.CFUNCTYPE(None, vp, ctypes.c_char_p)),
+                ("user", vp), ("verbose", ctypes.c_int32)]
+
+
+class StructureInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("nterms", ctypes.c_uint64), ("g1", ctypes.c_uint64),
+                ("ncells", ctypes.c_uint64), ("nb", ctypes.c_uint64), ("m", ctypes.c_uint64),
+                ("m_small", ctypes.c_uint64), ("small_work", ctypes.c_uint64), ("fe1", ctypes.c_int64),
+                ("norm_fe1", ctypes.c_int64), ("has_cell_obs_off", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Ssc(ctypes.Structure):
+    _fields_ = [("k_adj", ctypes.c_int32), ("k_fixef", ctypes.c_int32), ("g_adj", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class FitStats(ctypes.Structure):
+    _fields_ = [("ssr", ctypes.c_double), ("df_k", ctypes.c_double), ("nparams", ctypes.c_double),
+                ("deviance", ctypes.c_double), ("iterations", ctypes.c_uint64), ("pcg_iterations", ctypes.c_uint64),
+                ("n_clusters", ctypes.c_uint64), ("converged", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+# (name, restype, argtypes) of every exported entry point of include/fereg_hip.h
+SIGNATURES = [
+    ("fh_last_error", ctypes.c_char_p, []),
+    ("fh_config_default", None, [ctypes.POINTER(SolverConfig)]),
+    ("fh_absorber_create", ctypes.c_int, [ctypes.POINTER(Term), ctypes.c_size_t, f64p, ctypes.c_size_t,
+                                          ctypes.POINTER(SolverConfig), ctypes.POINTER(Ctx), ctypes.POINTER(vp)]),
+    ("fh_absorber_create_dev", ctypes.c_int, [ctypes.POINTER(Term), ctypes.c_size_t, vp, ctypes.c_size_t,
+                                              ctypes.POINTER(SolverConfig), ctypes.POINTER(Ctx), ctypes.POINTER(vp)]),
+    ("fh_absorber_destroy", None, [vp]),
+    ("fh_absorber_n", ctypes.c_size_t, [vp]),
+    ("fh_absorber_row_order", u32p, [vp]),
+    ("fh_absorber_row_order_dev", vp, [vp]),
+    ("fh_absorber_set_weights", ctypes.c_int, [vp, f64p, ctypes.POINTER(Ctx)]),
+    ("fh_absorber_set_weights_dev", ctypes.c_int, [vp, vp, ctypes.POINTER(Ctx)]),
+    ("fh_absorber_residualize", ctypes.c_int, [vp, f64p, ctypes.c_size_t, ctypes.POINTER(SolveOptsC), f64p, f64p,
+                                               ctypes.POINTER(SolveInfoC), ctypes.POINTER(Ctx)]),
+    ("fh_absorber_residualize_dev", ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(SolveOptsC), vp, vp,
+                                                   ctypes.POINTER(SolveInfoC), ctypes.POINTER(Ctx)]),
+    ("fh_absorber_coef_rows", ctypes.c_size_t, [vp]),
+    ("fh_absorber_fit_rows", ctypes.c_int, [vp, f64p, ctypes.c_size_t, ctypes.c_size_t, f64p]),
+    ("fh_absorber_term_values", ctypes.c_int, [vp, f64p, ctypes.c_size_t, ctypes.c_size_t, f64p, u64p]),
+    ("fh_absorber_info", ctypes.c_int, [vp, ctypes.POINTER(StructureInfo)]),
+    ("fh_absorber_copy_structure", ctypes.c_int, [vp, u32p, u32p, u32p, u32p]),
+    ("fh_absorber_copy_blocks", ctypes.c_int, [vp, i64p, u64p, u64p, i32p]),
+    ("fh_absorber_copy_level_cells", ctypes.c_int, [vp, ctypes.c_size_t, u32p, u32p]),
+    ("fh_absorber_copy_weight_state", ctypes.c_int, [vp, f64p, f64p, f64p, ctypes.POINTER(ctypes.c_uint8), f64p]),
+    ("fh_absorber_apply", ctypes.c_int, [vp, f64p, ctypes.c_size_t, f64p, ctypes.c_int32]),
+    ("fh_absorber_time_spmv", ctypes.c_int, [vp, ctypes.c_size_t, ctypes.c_int32, ctypes.c_uint32, f64p, f64p]),
+    ("fh_ols_fit", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, ctypes.c_int32, u32p, i32p, ctypes.POINTER(Ssc),
+                                  ctypes.POINTER(SolveOptsC), f64p, f64p, i32p, ctypes.POINTER(FitStats)]),
+    ("fh_ols_fit_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, ctypes.c_int32, vp, i32p, ctypes.POINTER(Ssc),
+                                      ctypes.POINTER(SolveOptsC), f64p, f64p, i32p, ctypes.POINTER(FitStats)]),
+    ("fh_ppml_fit", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, ctypes.c_int32, u32p, i32p, ctypes.POINTER(Ssc),
+                                   ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
+                                   ctypes.POINTER(FitStats)]),
+    ("fh_ppml_fit_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, ctypes.c_int32, vp, i32p, ctypes.POINTER(Ssc),
+                                       ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
+                                       ctypes.POINTER(FitStats)]),
+    ("fh_device_alloc", ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),
+    ("fh_device_free", ctypes.c_int, [vp]),
+    ("fh_memcpy_h2d", ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    ("fh_memcpy_d2h", ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    ("fh_device_synchronize", ctypes.c_int, []),
+]
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises when it has not been built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise ImportError(f"{library_path} is missing: build it with `python -m fereg_amd.build` "
+                              "(or __graft_entry__.build()); fereg_amd has no CPU fallback")
+        L = ctypes.CDLL(library_path)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != FH_OK:
+        msg = lib().fh_last_error()
+        raise FeregError(rc, msg.decode() if msg else "")

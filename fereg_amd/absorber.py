@@ -1,0 +1,269 @@
+"""The Absorber boundary (crates/fereg-core/src/absorber.rs:62-97) over the HIP library.
+
+Blocks are n x R row-major in `row_order` (the solver's observation order). Host blocks are
+numpy float64 arrays; device blocks are torch tensors on the absorber's GPU (HBM-resident, no
+copies across PCIe).
+"""
+import ctypes
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib as L
+
+
+@dataclass
+class SolveOpts:
+    """absorber.rs:18-24 (+ the precision of this solve: None = the absorber's)."""
+    tol: float = 1e-8
+    maxiter: int = 16000
+    want_coefs: bool = False
+    precision: Optional[int] = None
+
+    def c(self):
+        return L.SolveOptsC(self.tol, self.maxiter, int(self.want_coefs),
+                            -1 if self.precision is None else int(self.precision))
+
+
+@dataclass
+class FeCoefs:
+    """absorber.rs:39-60: p x r row-major, p = m + g1 ([x; alpha1], solve.rs:7-9)."""
+    p: int
+    r: int
+    data: np.ndarray
+
+    def col(self, j):
+        return self.data.reshape(self.p, self.r)[:, j].copy()
+
+    def combine(self, c):
+        c = np.asarray(c, float)
+        assert len(c) == self.r, "combine: one weight per column"
+        return FeCoefs(self.p, 1, self.data.reshape(self.p, self.r) @ c)
+
+
+@dataclass
+class SolveInfo:
+    """absorber.rs:27-37."""
+    iterations: int
+    escalated: bool
+    coefs: Optional[FeCoefs]
+    exact: bool
+    refinements: int = 0
+    seconds: float = 0.0
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Absorber:
+    """solve::build / FeSolver::new (solve.rs:338-465) on the GPU.
+
+    codes: list of per-term kept-order codes (uint32 numpy arrays, or torch uint32/int32 device
+    tensors), g: levels per term, weights: kept-order weights or None.
+    """
+
+    def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, verbose=0):
+        lib = L.lib()
+        self._keep = []
+        n = None
+        dev = any(_is_torch(c) for c in codes) or (weights is not None and _is_torch(weights))
+        terms = (L.Term * max(1, len(codes)))()
+        for q, (c, gq) in enumerate(zip(codes, g)):
+            if dev:
+                c = c.contiguous()
+                self._keep.append(c)
+                terms[q] = L.Term(ctypes.cast(c.data_ptr(), L.u32p), int(gq), 1)
+                nq = c.numel()
+            else:
+                c = np.ascontiguousarray(c, dtype=np.uint32)
+                self._keep.append(c)
+                terms[q] = L.Term(_ptr(c, L.u32p), int(gq), 1)
+                nq = len(c)
+            if n is None:
+                n = nq
+            elif n != nq:
+                raise L.FeregError(L.FH_INVALID, "term codes length differs from the kept sample")
+        if weights is not None:
+            if n is None:
+                n = len(weights)
+            elif len(weights) != n:
+                raise L.FeregError(L.FH_INVALID, "weight length differs from the kept sample")
+        if n is None:
+            raise L.FeregError(L.FH_INVALID, "no terms and no weights: the sample size is unknown")
+        cfg = L.SolverConfig(small_limit, precision, 0)
+        h = ctypes.c_void_p()
+        if dev:
+            wptr = None
+            if weights is not None:
+                weights = weights.contiguous().double()
+                self._keep.append(weights)
+                wptr = weights.data_ptr()
+            L.check(lib.fh_absorber_create_dev(terms, len(codes), wptr, n, ctypes.byref(cfg), None, ctypes.byref(h)))
+        else:
+            w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+            L.check(lib.fh_absorber_create(terms, len(codes), None if w is None else _ptr(w, L.f64p), n,
+                                           ctypes.byref(cfg), None, ctypes.byref(h)))
+        self._h = h
+        self._keep = []
+        self._n = n
+        self._weights = None if weights is None else (weights if dev else np.asarray(weights, float))
+        info = L.StructureInfo()
+        L.check(lib.fh_absorber_info(h, ctypes.byref(info)))
+        self.info = info
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().fh_absorber_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- trait
+    def n(self):
+        return self._n
+
+    def row_order(self):
+        p = L.lib().fh_absorber_row_order(self._h)
+        if not p:
+            L.check(L.FH_INTERNAL)
+        return np.ctypeslib.as_array(p, shape=(self._n,)).copy()
+
+    def weights(self):
+        """Current weights in row order (None = all ones)."""
+        return self._weights_rows
+
+    _weights_rows = None
+
+    def set_weights(self, w_rows):
+        lib = L.lib()
+        if w_rows is None:
+            L.check(lib.fh_absorber_set_weights(self._h, None, None))
+        elif _is_torch(w_rows):
+            w_rows = w_rows.contiguous().double()
+            L.check(lib.fh_absorber_set_weights_dev(self._h, w_rows.data_ptr(), None))
+        else:
+            w = np.ascontiguousarray(w_rows, dtype=np.float64)
+            if len(w) != self._n:
+                raise L.FeregError(L.FH_INVALID, "weights must be positive and finite")
+            L.check(lib.fh_absorber_set_weights(self._h, _ptr(w, L.f64p), None))
+        self._weights_rows = w_rows
+
+    def coef_rows(self):
+        return L.lib().fh_absorber_coef_rows(self._h)
+
+    def residualize(self, z, r, opts=None, warm=None):
+        """Residualizes the n x r row-major block z in place (solve.rs:1141-1311)."""
+        o = opts or SolveOpts()
+        lib = L.lib()
+        p = self.coef_rows()
+        info = L.SolveInfoC()
+        coefs = None
+        if _is_torch(z):
+            import torch
+            assert z.dtype == torch.float64 and z.is_contiguous() and z.numel() == self._n * r
+            cf = torch.zeros(p * r, dtype=torch.float64, device=z.device) if o.want_coefs else None
+            wm = None
+            if warm is not None:
+                wm = warm.data if _is_torch(warm.data) else torch.as_tensor(warm.data, device=z.device)
+                if warm.r != r or warm.p != p:
+                    raise L.FeregError(L.FH_INVALID, "residualize: warm start has a different shape")
+            L.check(lib.fh_absorber_residualize_dev(self._h, z.data_ptr(), r, ctypes.byref(o.c()),
+                                                    None if wm is None else wm.data_ptr(),
+                                                    None if cf is None else cf.data_ptr(), ctypes.byref(info), None))
+            if cf is not None:
+                coefs = FeCoefs(p, r, cf)
+        else:
+            assert z.dtype == np.float64 and z.flags.c_contiguous
+            if z.size != self._n * r:
+                raise L.FeregError(L.FH_INVALID, "residualize: block size differs from n*R")
+            cf = np.zeros(p * r) if o.want_coefs else None
+            wm = None
+            if warm is not None:
+                if warm.r != r or warm.p != p:
+                    raise L.FeregError(L.FH_INVALID, "residualize: warm start has a different shape")
+                wm = np.ascontiguousarray(warm.data, dtype=np.float64)
+            L.check(lib.fh_absorber_residualize(self._h, _ptr(z, L.f64p), r, ctypes.byref(o.c()),
+                                                None if wm is None else _ptr(wm, L.f64p),
+                                                None if cf is None else _ptr(cf, L.f64p), ctypes.byref(info), None))
+            if cf is not None:
+                coefs = FeCoefs(p, r, cf)
+        return SolveInfo(info.iterations, bool(info.escalated), coefs, bool(info.exact), info.refinements,
+                         info.seconds)
+
+    def fit_rows(self, c: FeCoefs, col: int):
+        out = np.zeros(self._n)
+        data = np.ascontiguousarray(c.data, dtype=np.float64)
+        L.check(L.lib().fh_absorber_fit_rows(self._h, _ptr(data, L.f64p), c.r, col, _ptr(out, L.f64p)))
+        return out
+
+    def term_values(self, c: FeCoefs, col: int) -> List[np.ndarray]:
+        nt = self.info.nterms
+        lens = np.zeros(max(nt, 1), np.uint64)
+        tot = self.info.g1 + self.info.m
+        out = np.zeros(max(tot, 1))
+        data = np.ascontiguousarray(c.data, dtype=np.float64)
+        L.check(L.lib().fh_absorber_term_values(self._h, _ptr(data, L.f64p), c.r, col, _ptr(out, L.f64p),
+                                                _ptr(lens, L.u64p)))
+        res, pos = [], 0
+        for q in range(nt):
+            k = int(lens[q])
+            res.append(out[pos:pos + k].copy())
+            pos += k
+        return res
+
+    # ---------------------------------------------------------------- introspection
+    def blocks(self):
+        nb = self.info.nb
+        term = np.zeros(max(nb, 1), np.int64)
+        g = np.zeros(max(nb, 1), np.uint64)
+        off = np.zeros(max(nb, 1), np.uint64)
+        small = np.zeros(max(nb, 1), np.int32)
+        L.check(L.lib().fh_absorber_copy_blocks(self._h, _ptr(term, L.i64p), _ptr(g, L.u64p), _ptr(off, L.u64p),
+                                                _ptr(small, L.i32p)))
+        return term[:nb], g[:nb], off[:nb], small[:nb]
+
+    def structure(self):
+        i = self.info
+        pi = np.zeros(i.n, np.uint32)
+        seg = np.zeros(i.g1 + 1, np.uint32)
+        cc = np.zeros(max(1, i.ncells * i.nb), np.uint32)
+        coo = np.zeros(i.ncells + 1, np.uint32) if i.has_cell_obs_off else None
+        L.check(L.lib().fh_absorber_copy_structure(self._h, _ptr(pi, L.u32p), _ptr(seg, L.u32p), _ptr(cc, L.u32p),
+                                                   None if coo is None else _ptr(coo, L.u32p)))
+        return {"pi": pi, "seg_off": seg, "cell_codes": cc[: i.ncells * i.nb], "cell_obs_off": coo}
+
+    def level_cells(self, b):
+        _, g, _, _ = self.blocks()
+        perm = np.zeros(max(1, self.info.ncells), np.uint32)
+        off = np.zeros(int(g[b]) + 1, np.uint32)
+        L.check(L.lib().fh_absorber_copy_level_cells(self._h, b, _ptr(perm, L.u32p), _ptr(off, L.u32p)))
+        return perm[: self.info.ncells], off
+
+    def weight_state(self):
+        i = self.info
+        n1 = np.zeros(i.g1)
+        nr = np.zeros(max(i.m, 1))
+        dg = np.zeros(max(i.m, 1))
+        red = np.zeros(max(i.m, 1), np.uint8)
+        sw = ctypes.c_double()
+        L.check(L.lib().fh_absorber_copy_weight_state(self._h, _ptr(n1, L.f64p), _ptr(nr, L.f64p), _ptr(dg, L.f64p),
+                                                      red.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                                      ctypes.byref(sw)))
+        return {"n1": n1, "nr": nr[: i.m], "diag": dg[: i.m], "redundant": red[: i.m].astype(bool), "sw": sw.value}
+
+    def apply(self, p, r, precision=L.FH_PLAIN_FP64):
+        """y = S p on the GPU (operator.rs:1375), host buffers m x r."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        y = np.zeros(self.info.m * r)
+        L.check(L.lib().fh_absorber_apply(self._h, _ptr(p, L.f64p), r, _ptr(y, L.f64p), precision))
+        return y

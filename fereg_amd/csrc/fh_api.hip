@@ -1,0 +1,1151 @@
+// C-ABI entry points of fereg_hip (include/fereg_hip.h) and the host orchestration of a
+// residualization (solve.rs:1141-1311), an OLS fit (fit/ols.rs:213-330, fit/vcov.rs:63-382) and
+// a Poisson PML fit by IRLS (glm/irls.rs:367-913, fixest flavour).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+
+#include "fh_internal.hpp"
+#include "fh_sort.hpp"
+
+namespace fh {
+
+// fh_estimate.hip
+void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
+               double* G_host);
+double residual_pass(Absorber& a, const double* z, size_t n, int ld, const double* c_host, const double* w, double* e);
+void hetero_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
+                 double* meat);
+uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
+                      const uint32_t* cl_kept_dev, uint64_t G, double* meat);
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+void ensure_w32(Absorber&) {}  // refresh_weights fills both precisions
+
+namespace {
+
+constexpr int TPB = 256;
+inline unsigned vgrid(size_t n) { return grid_for(n, TPB, 8192); }
+
+__global__ void k_gather_rows(const double* __restrict__ y, const double* __restrict__ X, size_t K,
+                              const uint32_t* __restrict__ pi, size_t n, double* __restrict__ rows) {
+    const size_t R = K + 1;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t o = pi[i];
+        if (y) rows[i * R] = y[o];
+        for (size_t k = 0; k < K; k++) rows[i * R + 1 + k] = X[o * K + k];
+    }
+}
+
+__global__ void k_gather_x(const double* __restrict__ X, size_t K, const uint32_t* __restrict__ pi, size_t n,
+                           double* __restrict__ out) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * K; e += (size_t)gridDim.x * blockDim.x)
+        out[e] = X[(size_t)pi[e / K] * K + e % K];
+}
+
+// flag = 1 when some code >= g (structure.rs:483-489) or some weight is not positive and finite.
+__global__ void k_check_codes(const uint32_t* __restrict__ c, size_t n, uint64_t g, uint32_t* flag) {
+    bool bad = false;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        bad |= c[i] >= g;
+    if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1u);
+}
+__global__ void k_check_weights(const double* __restrict__ w, size_t n, uint32_t* flag) {
+    bool bad = false;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        bad |= !(w[i] > 0.0) || !isfinite(w[i]);
+    if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1u);
+}
+__global__ void k_max_u32(const uint32_t* __restrict__ c, size_t n, uint32_t* out) {
+    uint32_t m = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = max(m, c[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, WAVE));
+    if (lane_id() == 0) atomicMax(out, m);
+}
+__global__ void k_check_y(const double* __restrict__ y, size_t n, uint32_t* flag, double* part) {
+    bool bad = false;
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        bad |= !(y[i] >= 0.0) || !isfinite(y[i]);
+        s += y[i];
+    }
+    if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1u);
+    __shared__ double red[TPB / WAVE];
+    s = wave_sum(s);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < TPB / WAVE; q++) t += red[q];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_gather_vec(const double* __restrict__ v, const uint32_t* __restrict__ pi, size_t n,
+                             double* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = v[pi[i]];
+}
+
+// Columns [c0, c0+rc) of an n x ld block <-> a contiguous n x rc block.
+__global__ void k_cols_out(const double* __restrict__ z, size_t n, int ld, int c0, int rc, double* __restrict__ out) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * rc; e += (size_t)gridDim.x * blockDim.x)
+        out[e] = z[(e / rc) * ld + c0 + e % rc];
+}
+__global__ void k_cols_in(double* __restrict__ z, size_t n, int ld, int c0, int rc, const double* __restrict__ in) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * rc; e += (size_t)gridDim.x * blockDim.x)
+        z[(e / rc) * ld + c0 + e % rc] = in[e];
+}
+
+__global__ void k_zero_cols(double* __restrict__ x, size_t m, int r, const int32_t* __restrict__ zero) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < m * r; e += (size_t)gridDim.x * blockDim.x)
+        if (zero[e % r]) x[e] = 0.0;
+}
+
+__global__ void k_wsq_b1(const double* __restrict__ b1, const double* __restrict__ n1, size_t g1, int r,
+                         double* __restrict__ part) {
+    // part[blockIdx.x * r + j] = sum_s b1^2 / n1 over the block's segments (fixed grid)
+    __shared__ double red[TPB / WAVE][16];
+    double acc[16];
+    for (int j = 0; j < 16; j++) acc[j] = 0.0;
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x)
+        for (int j = 0; j < r; j++) {
+            const double v = b1[s * r + j];
+            acc[j] += v * v / n1[s];
+        }
+    for (int j = 0; j < 16; j++)
+        if (j < r) {
+            double v = wave_sum(acc[j]);
+            if (lane_id() == 0) red[threadIdx.x / WAVE][j] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)r) {
+        double v = 0.0;
+        for (int q = 0; q < TPB / WAVE; q++) v += red[q][threadIdx.x];
+        part[(size_t)blockIdx.x * r + threadIdx.x] = v;
+    }
+}
+
+// IRLS fill (glm/irls.rs:467-493): w = mu, working variable z = eta - off + (y - mu)/mu in column 0
+// of the block, raw X (row order) in columns 1..K.
+__global__ void k_irls_fill(const double* __restrict__ y, const double* __restrict__ mu, const double* __restrict__ eta,
+                            const double* __restrict__ off, const double* __restrict__ tw, const double* __restrict__ xr,
+                            size_t n, size_t K, double* __restrict__ rows, double* __restrict__ zraw,
+                            double* __restrict__ w) {
+    const size_t R = K + 1;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double m = mu[i];
+        const double z = eta[i] - (off ? off[i] : 0.0) + (y[i] - m) / m;
+        rows[i * R] = z;
+        zraw[i] = z;
+        for (size_t k = 0; k < K; k++) rows[i * R + 1 + k] = xr[i * K + k];
+        w[i] = (tw ? tw[i] : 1.0) * m;
+    }
+}
+
+// eta = (z - e) + off (step s in (0,1] toward it from eta_old), mu = max(exp eta, 100 eps),
+// deviance partials (irls.rs:254-270; family.rs:103-112).
+__global__ __launch_bounds__(TPB) void k_irls_eta(const double* __restrict__ zraw, const double* __restrict__ e,
+                                                  const double* __restrict__ off, const double* __restrict__ eta_old,
+                                                  double step, const double* __restrict__ y,
+                                                  const double* __restrict__ tw, size_t n, double* __restrict__ eta,
+                                                  double* __restrict__ mu, double* __restrict__ part) {
+    double acc = 0.0;
+    const double mu_min = 100.0 * 2.220446049250313e-16;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double et = zraw[i] - e[i] + (off ? off[i] : 0.0);
+        if (step < 1.0) et = eta_old[i] + step * (et - eta_old[i]);
+        double m = exp(et);
+        if (!(m >= mu_min)) m = mu_min;
+        eta[i] = et;
+        mu[i] = m;
+        const double yi = y[i], wi = tw ? tw[i] : 1.0;
+        acc += wi * (m - yi) + (yi > 0.0 ? wi * yi * (log(yi) - et) : 0.0);
+    }
+    __shared__ double red[TPB / WAVE];
+    acc = wave_sum(acc);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < TPB / WAVE; q++) t += red[q];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_init_mu(const double* __restrict__ y, double ybar, size_t n, double* __restrict__ mu,
+                          double* __restrict__ eta) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double m = 0.5 * (y[i] + ybar);
+        mu[i] = m;
+        eta[i] = log(m);
+    }
+}
+
+template <class T>
+void h2d(T* dst, const T* src, size_t count, hipStream_t st) {
+    if (count) FH_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+}
+template <class T>
+void d2h(T* dst, const T* src, size_t count, hipStream_t st) {
+    if (count) FH_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, st));
+}
+
+double sum_parts(const double* dev, size_t count, hipStream_t st) {
+    std::vector<double> h(count);
+    d2h(h.data(), dev, count, st);
+    FH_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double v : h) s += v;
+    return s;
+}
+
+int guard(const std::function<void()>& f) {
+    try {
+        f();
+        return FH_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.rc;
+    } catch (const std::bad_alloc&) {
+        set_last_error("out of host memory");
+        return FH_OOM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return FH_INTERNAL;
+    }
+}
+
+void check_interrupt(const fh_ctx* ctx) {
+    if (ctx && ctx->interrupt && ctx->interrupt(ctx->user)) throw Error(FH_INTERRUPTED, "interrupted");
+}
+
+void log_line(const fh_ctx* ctx, int level, const std::string& s) {
+    if (ctx && ctx->log && ctx->verbose >= level) ctx->log(ctx->user, s.c_str());
+}
+
+bool weights_ok(Absorber& a, const double* w) {
+    DBuf<uint32_t> bad(1);
+    FH_HIP(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), a.st));
+    hipLaunchKernelGGL(k_check_weights, dim3(1024), dim3(TPB), 0, a.st, w, a.n, bad.p);
+    FH_CHECK_LAUNCH();
+    uint32_t h = 0;
+    d2h(&h, bad.p, 1, a.st);
+    FH_HIP(hipStreamSynchronize(a.st));
+    return h == 0;
+}
+
+// ------------------------------------------------------------------ residualize
+
+struct ResidOut {
+    uint64_t iterations = 1;
+    bool exact = true;
+    uint64_t refinements = 0;
+};
+
+// One residualization of a device block of r <= 16 columns (solve.rs:1141-1311).
+ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t maxiter, const double* warm_dev,
+                           double* coefs_dev, int precision, const fh_ctx* ctx) {
+    hipStream_t st = a.st;
+    const size_t m = a.m, g1 = a.g1;
+    ResidOut out;
+    a.b1.reserve(g1 * r);
+    a.s64.rhs.reserve(std::max<size_t>(m * r, 1));
+    a.s64.x.reserve(std::max<size_t>(m * r, 1));
+    std::vector<double> ws(2 * r);
+    rhs_pass(a, z, r, a.s64.rhs.p, a.b1.p, ws.data());
+    check_interrupt(ctx);
+    // ss0 = wsq - wsum^2/sw (0 if <= 1e-15 wsq), ss1 = wsq - sum_s b1^2/n1 (solve.rs:1210-1217)
+    const unsigned pg = 256;
+    DBuf<double> part((size_t)pg * r);
+    hipLaunchKernelGGL(k_wsq_b1, dim3(pg), dim3(TPB), 0, st, a.b1.p, a.n1.p, g1, r, part.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> hp((size_t)pg * r);
+    d2h(hp.data(), part.p, hp.size(), st);
+    FH_HIP(hipStreamSynchronize(st));
+    std::vector<double> ss0(r), ss1(r);
+    std::vector<int32_t> zero(r);
+    for (int j = 0; j < r; j++) {
+        double b1sq = 0.0;
+        for (unsigned q = 0; q < pg; q++) b1sq += hp[(size_t)q * r + j];
+        const double wsq = ws[j], wsum = ws[r + j];
+        const double v = wsq - wsum * wsum / a.sw;
+        ss0[j] = v > 1e-15 * wsq ? v : 0.0;
+        ss1[j] = wsq - b1sq;
+        zero[j] = ss0[j] == 0.0;
+    }
+    double* x = a.s64.x.p;
+    if (m > 0) {
+        if (warm_dev) FH_HIP(hipMemcpyAsync(x, warm_dev, m * r * sizeof(double), hipMemcpyDeviceToDevice, st));
+        else FH_HIP(hipMemsetAsync(x, 0, m * r * sizeof(double), st));
+        DBuf<int32_t> dz(r);
+        h2d(dz.p, zero.data(), r, st);
+        hipLaunchKernelGGL(k_zero_cols, dim3(vgrid(m * r)), dim3(TPB), 0, st, x, m, r, dz.p);
+        FH_CHECK_LAUNCH();
+        PcgResult pr = precision == FH_MIXED ? pcg_mixed(a, a.s64.rhs.p, x, r, ss1.data(), ss0.data(), tol, maxiter, ctx)
+                                             : pcg_fp64(a, a.s64.rhs.p, x, r, ss1.data(), ss0.data(), tol, maxiter, ctx);
+        if (pr.rc == FH_CONVERGENCE)
+            throw Error(FH_CONVERGENCE, "the FE solver did not converge within " + std::to_string(maxiter) + " iterations");
+        out.iterations = std::max<uint64_t>(pr.iterations, 1);
+        bool allconst = true;
+        for (int j = 0; j < r; j++) allconst &= ss0[j] == 0.0;
+        out.exact = pr.exact || allconst;
+        out.refinements = pr.refinements;
+    }
+    residualize_rows(a, z, r, x, a.b1.p, coefs_dev ? coefs_dev + m * r : nullptr);
+    if (coefs_dev && m > 0) FH_HIP(hipMemcpyAsync(coefs_dev, x, m * r * sizeof(double), hipMemcpyDeviceToDevice, st));
+    FH_HIP(hipStreamSynchronize(st));
+    return out;
+}
+
+// Any r: columns in chunks of <= 16 (every column is an independent system).
+ResidOut residualize_any(Absorber& a, double* z, size_t r, double tol, uint64_t maxiter, const double* warm_dev,
+                         double* coefs_dev, int precision, const fh_ctx* ctx) {
+    if (r <= 16) return residualize_block(a, z, (int)r, tol, maxiter, warm_dev, coefs_dev, precision, ctx);
+    const size_t n = a.n, p = a.m + a.g1;
+    ResidOut tot;
+    tot.iterations = 0;
+    DBuf<double> zc(n * 16), wc(p * 16), cc(p * 16);
+    for (size_t c0 = 0; c0 < r; c0 += 16) {
+        const int rc = (int)std::min<size_t>(16, r - c0);
+        hipLaunchKernelGGL(k_cols_out, dim3(vgrid(n * rc)), dim3(TPB), 0, a.st, z, n, (int)r, (int)c0, rc, zc.p);
+        if (warm_dev)
+            hipLaunchKernelGGL(k_cols_out, dim3(vgrid(p * rc)), dim3(TPB), 0, a.st, warm_dev, p, (int)r, (int)c0, rc, wc.p);
+        FH_CHECK_LAUNCH();
+        ResidOut o = residualize_block(a, zc.p, rc, tol, maxiter, warm_dev ? wc.p : nullptr, coefs_dev ? cc.p : nullptr,
+                                       precision, ctx);
+        hipLaunchKernelGGL(k_cols_in, dim3(vgrid(n * rc)), dim3(TPB), 0, a.st, z, n, (int)r, (int)c0, rc, zc.p);
+        if (coefs_dev)
+            hipLaunchKernelGGL(k_cols_in, dim3(vgrid(p * rc)), dim3(TPB), 0, a.st, coefs_dev, p, (int)r, (int)c0, rc, cc.p);
+        FH_CHECK_LAUNCH();
+        tot.iterations = std::max(tot.iterations, o.iterations);
+        tot.exact = tot.exact && o.exact;
+        tot.refinements = std::max(tot.refinements, o.refinements);
+    }
+    FH_HIP(hipStreamSynchronize(a.st));
+    return tot;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        FH_HIP(hipGetDevice(&prev));
+        if (prev != dev) FH_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// ------------------------------------------------------------------ estimation helpers (host, K x K)
+
+// Column-order Cholesky with exclusion of pivots below tol and the inverse over the kept
+// columns (vcov.rs:124-128, 163-211).
+void chol_collin(const std::vector<double>& A, int k, double tol, std::vector<int>& excluded, std::vector<double>& inv) {
+    std::vector<double> R((size_t)k * k, 0.0);
+    excluded.assign(k, 0);
+    for (int j = 0; j < k; j++) {
+        double rjj = A[(size_t)j * k + j];
+        for (int i = 0; i < j; i++)
+            if (!excluded[i]) rjj -= R[(size_t)i * k + j] * R[(size_t)i * k + j];
+        if (!(rjj >= tol)) {
+            excluded[j] = 1;
+            continue;
+        }
+        const double d = std::sqrt(rjj);
+        R[(size_t)j * k + j] = d;
+        for (int c = j + 1; c < k; c++) {
+            double v = A[(size_t)c * k + j];
+            for (int i = 0; i < j; i++)
+                if (!excluded[i]) v -= R[(size_t)i * k + c] * R[(size_t)i * k + j];
+            R[(size_t)j * k + c] = v / d;
+        }
+    }
+    std::vector<int> kept;
+    for (int j = 0; j < k; j++)
+        if (!excluded[j]) kept.push_back(j);
+    const int mk = (int)kept.size();
+    std::vector<double> ri((size_t)mk * mk, 0.0);
+    for (int c = 0; c < mk; c++) {
+        ri[(size_t)c * mk + c] = 1.0 / R[(size_t)kept[c] * k + kept[c]];
+        for (int row = c - 1; row >= 0; row--) {
+            double s = 0.0;
+            for (int l = row + 1; l <= c; l++) s += R[(size_t)kept[row] * k + kept[l]] * ri[(size_t)l * mk + c];
+            ri[(size_t)row * mk + c] = -s / R[(size_t)kept[row] * k + kept[row]];
+        }
+    }
+    inv.assign((size_t)mk * mk, 0.0);
+    for (int i = 0; i < mk; i++)
+        for (int j = 0; j < mk; j++) {
+            double s = 0.0;
+            for (int l = std::max(i, j); l < mk; l++) s += ri[(size_t)i * mk + l] * ri[(size_t)j * mk + l];
+            inv[(size_t)i * mk + j] = s;
+        }
+}
+
+// fixest df.K (fit/vcov.rs:63-112, K.exact = FALSE).
+double df_k(const Absorber& a, int kind, const fh_ssc& ssc, const int32_t* nested, double np, int kdesign) {
+    const size_t nt = a.nterms;
+    double fe_params = 0.0;
+    if (nt > 0) {
+        for (uint64_t g : a.term_g) fe_params += (double)g;
+        fe_params -= (double)nt - 1.0;
+    }
+    int kf = ssc.k_fixef;
+    if (kind != FH_VCOV_CLUSTER && kf == FH_KFIXEF_NONNESTED) kf = FH_KFIXEF_FULL;
+    if (kf == FH_KFIXEF_NONE) return np - fe_params;
+    if (kf == FH_KFIXEF_FULL || nt == 0) return np;
+    int n_nested = 0;
+    for (size_t q = 0; q < nt; q++) n_nested += nested && nested[q] ? 1 : 0;
+    double k = np;
+    if ((size_t)n_nested == nt) {
+        k = k - fe_params + 1.0;
+    } else {
+        double s = 0.0, c = 0.0;
+        for (size_t q = 0; q < nt; q++)
+            if (nested && nested[q]) {
+                s += (double)a.term_g[q];
+                c += a.term_g[q] > 0 ? 1.0 : 0.0;
+            }
+        k -= s - c;
+    }
+    return std::max(k, (double)kdesign + 1.0);
+}
+
+struct FitCommon {
+    std::vector<double> beta, vcov;
+    std::vector<int32_t> kept;
+};
+
+// V for a residualized block rows (n x R: [target, X...]) with e the residual combo applied
+// (fit/vcov.rs:316-382). bread over kept columns; scores x_i e_i w_i.
+void vcov_from(Absorber& a, double* rows, int R, const std::vector<int>& excluded, const std::vector<double>& bread,
+               const double* e, const double* w, int kind, const uint32_t* cl_dev, uint64_t G, const fh_ssc& ssc,
+               double dfk, double iid_scale, std::vector<double>& V, uint64_t* ncl) {
+    const int K = R - 1;
+    std::vector<int> kept;
+    for (int j = 0; j < K; j++)
+        if (!excluded[j]) kept.push_back(j);
+    const int mk = (int)kept.size();
+    const double n = (double)a.n;
+    V.assign((size_t)K * K, 0.0);
+    std::vector<double> vk((size_t)mk * mk, 0.0);
+    if (mk == 0) return;
+    if (kind == FH_VCOV_IID) {
+        const double factor = ssc.k_adj ? (n - 1.0) / (n - dfk) : 1.0;
+        for (size_t i = 0; i < vk.size(); i++) vk[i] = bread[i] * iid_scale * factor;
+    } else {
+        std::vector<double> meat((size_t)K * K);
+        double factor = 1.0;
+        if (kind == FH_VCOV_HETERO) {
+            hetero_meat(a, rows, a.n, R, 1, K, e, w, meat.data());
+            if (ssc.k_adj) factor = n / (n - dfk);
+        } else {
+            require(cl_dev != nullptr, "a clustered variance matrix needs cluster codes");
+            uint64_t g = cluster_meat(a, rows, a.n, R, 1, K, e, w, cl_dev, G, meat.data());
+            if (ncl) *ncl = g;
+            if (ssc.g_adj) factor *= (double)g / ((double)g - 1.0);
+            if (ssc.k_adj) factor *= (n - 1.0) / (n - dfk);
+        }
+        std::vector<double> mkk((size_t)mk * mk), t((size_t)mk * mk, 0.0);
+        for (int i = 0; i < mk; i++)
+            for (int j = 0; j < mk; j++) mkk[(size_t)i * mk + j] = meat[(size_t)kept[i] * K + kept[j]];
+        for (int i = 0; i < mk; i++)
+            for (int j = 0; j < mk; j++) {
+                double s = 0.0;
+                for (int l = 0; l < mk; l++) s += bread[(size_t)i * mk + l] * mkk[(size_t)l * mk + j];
+                t[(size_t)i * mk + j] = s;
+            }
+        for (int i = 0; i < mk; i++)
+            for (int j = 0; j < mk; j++) {
+                double s = 0.0;
+                for (int l = 0; l < mk; l++) s += t[(size_t)i * mk + l] * bread[(size_t)l * mk + j];
+                vk[(size_t)i * mk + j] = s * factor;
+            }
+    }
+    for (int i = 0; i < mk; i++)
+        for (int j = 0; j < mk; j++) V[(size_t)kept[i] * K + kept[j]] = vk[(size_t)i * mk + j];
+}
+
+// Cluster levels: max code + 1.
+uint64_t cluster_levels(Absorber& a, const uint32_t* cl_dev) {
+    require(cl_dev != nullptr, "a clustered variance matrix needs cluster codes");
+    DBuf<uint32_t> mx(1);
+    FH_HIP(hipMemsetAsync(mx.p, 0, sizeof(uint32_t), a.st));
+    hipLaunchKernelGGL(k_max_u32, dim3(1024), dim3(TPB), 0, a.st, cl_dev, a.n, mx.p);
+    FH_CHECK_LAUNCH();
+    uint32_t h = 0;
+    d2h(&h, mx.p, 1, a.st);
+    FH_HIP(hipStreamSynchronize(a.st));
+    return (uint64_t)h + 1;
+}
+
+// OLS on device inputs (kept order). rows scratch: n x (1+K) in row order.
+void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, const uint32_t* cl_dev,
+             const int32_t* nested, const fh_ssc& ssc, const fh_solve_opts& o, double* beta, double* vcov,
+             int32_t* kept_out, fh_fit_stats* stats, const fh_ctx* ctx) {
+    hipStream_t st = a.st;
+    const size_t n = a.n, R = K + 1;
+    require(K >= 1 && K <= 21, "1..21 regressors are supported");
+    a.rows_tmp.reserve(n * R);
+    double* rows = a.rows_tmp.p;
+    hipLaunchKernelGGL(k_gather_rows, dim3(vgrid(n)), dim3(TPB), 0, st, y, X, K, a.pi.p, n, rows);
+    FH_CHECK_LAUNCH();
+    const int prec = o.precision < 0 ? a.precision : o.precision;
+    ResidOut ro = residualize_any(a, rows, R, o.tol, o.maxiter, nullptr, nullptr, prec, ctx);
+    const double* w = a.weighted ? a.w_rows.p : nullptr;
+    std::vector<double> G(R * R);
+    gram_cols(a, rows, n, (int)R, 0, (int)R, w, nullptr, G.data());
+    // regress column 0 on 1..K (fit/design.rs:64-95)
+    std::vector<double> Gxx(K * K);
+    for (size_t i = 0; i < K; i++)
+        for (size_t j = 0; j < K; j++) Gxx[i * K + j] = G[(i + 1) * R + (j + 1)];
+    std::vector<int> excluded;
+    std::vector<double> inv;
+    chol_collin(Gxx, (int)K, 1e-9, excluded, inv);
+    std::vector<int> kept;
+    for (size_t j = 0; j < K; j++)
+        if (!excluded[j]) kept.push_back((int)j);
+    const int mk = (int)kept.size();
+    require(mk > 0, "all the regressors are collinear (with the fixed effects)");
+    std::vector<double> coef(mk, 0.0);
+    for (int i = 0; i < mk; i++) {
+        double s = 0.0;
+        for (int j = 0; j < mk; j++) s += inv[(size_t)i * mk + j] * G[(size_t)(kept[j] + 1) * R];
+        coef[i] = s;
+    }
+    std::vector<double> c(R, 0.0);
+    c[0] = 1.0;
+    for (int i = 0; i < mk; i++) c[kept[i] + 1] = -coef[i];
+    DBuf<double> e(n);
+    const double ssr = residual_pass(a, rows, n, (int)R, c.data(), w, e.p);
+    const double np = (double)mk + [&] {
+        double s = 0.0;
+        for (uint64_t g : a.term_g) s += (double)g;
+        return s - (a.nterms > 1 ? (double)a.nterms - 1.0 : 0.0);
+    }();
+    const double dfk = df_k(a, kind, ssc, nested, np, mk);
+    std::vector<double> V;
+    uint64_t ncl = 0;
+    const uint64_t G_cl = kind == FH_VCOV_CLUSTER ? cluster_levels(a, cl_dev) : 0;
+    vcov_from(a, rows, (int)R, excluded, inv, e.p, w, kind, cl_dev, G_cl, ssc, dfk, ssr / ((double)n - 1.0), V, &ncl);
+    for (size_t j = 0; j < K; j++) {
+        beta[j] = 0.0;
+        kept_out[j] = !excluded[j];
+    }
+    for (int i = 0; i < mk; i++) beta[kept[i]] = coef[i];
+    std::memcpy(vcov, V.data(), K * K * sizeof(double));
+    if (stats) {
+        stats->ssr = ssr;
+        stats->df_k = dfk;
+        stats->nparams = np;
+        stats->deviance = 0.0;
+        stats->iterations = ro.iterations;
+        stats->pcg_iterations = ro.iterations;
+        stats->n_clusters = ncl;
+        stats->converged = 1;
+    }
+}
+
+void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, const uint32_t* cl_dev,
+              const int32_t* nested, const fh_ssc& ssc, double tol, uint64_t maxiter, const fh_solve_opts& o,
+              double* beta, double* vcov, int32_t* kept_out, fh_fit_stats* stats, const fh_ctx* ctx) {
+    hipStream_t st = a.st;
+    const size_t n = a.n, R = K + 1;
+    require(K >= 1 && K <= 21, "1..21 regressors are supported");
+    require(!a.weighted, "prior weights of a Poisson fit are not supported here");
+    // row-order copies of y and X
+    DBuf<double> yr(n), xr(n * K), mu(n), eta(n), eta_old(n), zraw(n), w(n), e(n), part(1024);
+    hipLaunchKernelGGL(k_gather_vec, dim3(vgrid(n)), dim3(TPB), 0, st, y, a.pi.p, n, yr.p);
+    hipLaunchKernelGGL(k_gather_x, dim3(vgrid(n * K)), dim3(TPB), 0, st, X, K, a.pi.p, n, xr.p);
+    FH_CHECK_LAUNCH();
+    // y must be >= 0 and finite
+    double ybar = 0.0;
+    {
+        DBuf<uint32_t> flag(1);
+        FH_HIP(hipMemsetAsync(flag.p, 0, sizeof(uint32_t), st));
+        hipLaunchKernelGGL(k_check_y, dim3(1024), dim3(TPB), 0, st, yr.p, n, flag.p, part.p);
+        FH_CHECK_LAUNCH();
+        uint32_t bad = 0;
+        d2h(&bad, flag.p, 1, st);
+        ybar = sum_parts(part.p, 1024, st) / (double)n;
+        require(!bad, "Poisson outcomes must be non-negative and finite");
+    }
+    hipLaunchKernelGGL(k_init_mu, dim3(vgrid(n)), dim3(TPB), 0, st, yr.p, ybar, n, mu.p, eta.p);
+    FH_CHECK_LAUNCH();
+    a.rows_tmp.reserve(n * R);
+    double* rows = a.rows_tmp.p;
+    const size_t p = a.m + a.g1;
+    DBuf<double> warm(p * R), coefs(p * R);
+    bool have_warm = false;
+    double dev_old = NAN;
+    std::vector<int> excluded;
+    std::vector<double> inv, coef;
+    std::vector<int> kept;
+    uint64_t it = 0, pcg_total = 0;
+    bool converged = false;
+    const int prec = o.precision < 0 ? a.precision : o.precision;
+    for (it = 1; it <= maxiter; it++) {
+        check_interrupt(ctx);
+        hipLaunchKernelGGL(k_irls_fill, dim3(vgrid(n)), dim3(TPB), 0, st, yr.p, mu.p, eta.p, (const double*)nullptr,
+                           (const double*)nullptr, xr.p, n, K, rows, zraw.p, w.p);
+        FH_CHECK_LAUNCH();
+        refresh_weights(a, w.p);
+        // inner tolerance: loose while the deviance still moves, tight at the end
+        double tol_in = o.tol;
+        ResidOut ro = residualize_any(a, rows, R, tol_in, o.maxiter, have_warm ? warm.p : nullptr, coefs.p, prec, ctx);
+        pcg_total += ro.iterations;
+        FH_HIP(hipMemcpyAsync(warm.p, coefs.p, p * R * sizeof(double), hipMemcpyDeviceToDevice, st));
+        have_warm = true;
+        std::vector<double> G(R * R);
+        gram_cols(a, rows, n, (int)R, 0, (int)R, w.p, nullptr, G.data());
+        std::vector<double> Gxx(K * K);
+        for (size_t i = 0; i < K; i++)
+            for (size_t j = 0; j < K; j++) Gxx[i * K + j] = G[(i + 1) * R + (j + 1)];
+        chol_collin(Gxx, (int)K, 1e-9, excluded, inv);
+        kept.clear();
+        for (size_t j = 0; j < K; j++)
+            if (!excluded[j]) kept.push_back((int)j);
+        const int mk = (int)kept.size();
+        require(mk > 0, "all the regressors are collinear (with the fixed effects)");
+        coef.assign(mk, 0.0);
+        for (int i = 0; i < mk; i++) {
+            double s = 0.0;
+            for (int j = 0; j < mk; j++) s += inv[(size_t)i * mk + j] * G[(size_t)(kept[j] + 1) * R];
+            coef[i] = s;
+        }
+        std::vector<double> c(R, 0.0);
+        c[0] = 1.0;
+        for (int i = 0; i < mk; i++) c[kept[i] + 1] = -coef[i];
+        residual_pass(a, rows, n, (int)R, c.data(), nullptr, e.p);
+        FH_HIP(hipMemcpyAsync(eta_old.p, eta.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        double step = 1.0, dev = NAN;
+        for (int half = 0; half < 30; half++) {
+            hipLaunchKernelGGL(k_irls_eta, dim3(1024), dim3(TPB), 0, st, zraw.p, e.p, (const double*)nullptr, eta_old.p,
+                               step, yr.p, (const double*)nullptr, n, eta.p, mu.p, part.p);
+            FH_CHECK_LAUNCH();
+            dev = 2.0 * sum_parts(part.p, 1024, st);
+            if (std::isfinite(dev) && (std::isnan(dev_old) || dev <= dev_old * (1.0 + 1e-12) + 1e-300)) break;
+            step *= 0.5;
+        }
+        const double crit = std::isnan(dev_old) ? INFINITY : std::fabs(dev - dev_old) / (0.1 + std::fabs(dev));
+        log_line(ctx, 2, "fereg_hip: irls iteration " + std::to_string(it) + " deviance " + std::to_string(dev));
+        dev_old = dev;
+        if (crit < tol) {
+            converged = true;
+            break;
+        }
+    }
+    if (!converged) throw Error(FH_CONVERGENCE, "IRLS did not converge within " + std::to_string(maxiter) + " iterations");
+    // variance at the last IRLS weights: bread (X~'WX~)^-1, scores x~ e w (fit/glm.rs:560-600)
+    const double np = (double)kept.size() + [&] {
+        double s = 0.0;
+        for (uint64_t g : a.term_g) s += (double)g;
+        return s - (a.nterms > 1 ? (double)a.nterms - 1.0 : 0.0);
+    }();
+    const double dfk = df_k(a, kind, ssc, nested, np, (int)kept.size());
+    uint64_t ncl = 0;
+    const uint64_t G_cl = kind == FH_VCOV_CLUSTER ? cluster_levels(a, cl_dev) : 0;
+    std::vector<double> V;
+    vcov_from(a, rows, (int)R, excluded, inv, e.p, w.p, kind, cl_dev, G_cl, ssc, dfk, 1.0, V, &ncl);
+    for (size_t j = 0; j < K; j++) {
+        beta[j] = 0.0;
+        kept_out[j] = !excluded[j];
+    }
+    for (size_t i = 0; i < kept.size(); i++) beta[kept[i]] = coef[i];
+    std::memcpy(vcov, V.data(), K * K * sizeof(double));
+    if (stats) {
+        stats->ssr = 0.0;
+        stats->df_k = dfk;
+        stats->nparams = np;
+        stats->deviance = dev_old;
+        stats->iterations = it;
+        stats->pcg_iterations = pcg_total;
+        stats->n_clusters = ncl;
+        stats->converged = 1;
+    }
+}
+
+}  // namespace
+}  // namespace fh
+
+using namespace fh;
+
+struct fh_absorber {
+    Absorber a;
+};
+
+extern "C" {
+
+const char* fh_last_error(void) { return g_last_error.c_str(); }
+
+void fh_config_default(fh_solver_config* cfg) {
+    cfg->small_limit = 4096;
+    cfg->precision = FH_PLAIN_FP64;
+    cfg->reserved = 0;
+}
+
+static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept, size_t n, const fh_solver_config* cfg,
+                       const fh_ctx* ctx, fh_absorber** out, bool dev) {
+    return guard([&] {
+        require(out != nullptr, "null output handle");
+        *out = nullptr;
+        require(nterms == 0 || terms != nullptr, "null terms");
+        for (size_t q = 0; q < nterms; q++) {
+            require(terms[q].intercept == 1, "slope terms are not supported by this solver");
+            require(terms[q].codes != nullptr || n == 0, "null term codes");
+        }
+        require(n >= 1, "no observations", FH_TOO_FEW);
+        auto h = std::make_unique<fh_absorber>();
+        Absorber& a = h->a;
+        FH_HIP(hipGetDevice(&a.device));
+        FH_HIP(hipStreamCreateWithFlags(&a.st, hipStreamNonBlocking));
+        a.n = n;
+        a.nterms = nterms;
+        fh_solver_config c;
+        fh_config_default(&c);
+        if (cfg) c = *cfg;
+        a.small_limit = c.small_limit;
+        a.precision = c.precision;
+        std::vector<DBuf<uint32_t>> up(nterms);
+        std::vector<const uint32_t*> dc(nterms);
+        for (size_t q = 0; q < nterms; q++) {
+            a.term_g.push_back(terms[q].g);
+            if (dev) {
+                dc[q] = terms[q].codes;
+            } else {
+                up[q].alloc(n);
+                h2d(up[q].p, terms[q].codes, n, a.st);
+                dc[q] = up[q].p;
+            }
+        }
+        // code range check (structure.rs:483-489): every code below its level count
+        {
+            DBuf<uint32_t> bad(1);
+            FH_HIP(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), a.st));
+            for (size_t q = 0; q < nterms; q++)
+                hipLaunchKernelGGL(k_check_codes, dim3(1024), dim3(TPB), 0, a.st, dc[q], n, terms[q].g, bad.p);
+            FH_CHECK_LAUNCH();
+            uint32_t hb = 0;
+            d2h(&hb, bad.p, 1, a.st);
+            FH_HIP(hipStreamSynchronize(a.st));
+            require(!hb, "a term code is not below its level count");
+        }
+        DBuf<double> wd;
+        const double* wk = nullptr;
+        if (w_kept) {
+            wd.alloc(n);
+            if (dev) FH_HIP(hipMemcpyAsync(wd.p, w_kept, n * sizeof(double), hipMemcpyDeviceToDevice, a.st));
+            else h2d(wd.p, w_kept, n, a.st);
+            require(weights_ok(a, wd.p), "weights must be positive and finite");
+            wk = wd.p;
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        build_structure(a, dc);
+        log_line(ctx, 2, "fereg_hip: structure " +
+                             std::to_string(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()) + "s");
+        DBuf<double> wrow;
+        if (wk) {
+            wrow.alloc(n);
+            hipLaunchKernelGGL(k_gather_vec, dim3(vgrid(n)), dim3(TPB), 0, a.st, wk, a.pi.p, n, wrow.p);
+            FH_CHECK_LAUNCH();
+        }
+        refresh_weights(a, wk ? wrow.p : nullptr);
+        *out = h.release();
+    });
+}
+
+int fh_absorber_create(const fh_term* terms, size_t nterms, const double* w_kept, size_t n, const fh_solver_config* cfg,
+                       const fh_ctx* ctx, fh_absorber** out) {
+    return create_impl(terms, nterms, w_kept, n, cfg, ctx, out, false);
+}
+int fh_absorber_create_dev(const fh_term* terms, size_t nterms, const double* w_kept, size_t n,
+                           const fh_solver_config* cfg, const fh_ctx* ctx, fh_absorber** out) {
+    return create_impl(terms, nterms, w_kept, n, cfg, ctx, out, true);
+}
+
+void fh_absorber_destroy(fh_absorber* ab) {
+    if (!ab) return;
+    hipStream_t st = ab->a.st;
+    {
+        DeviceGuard g(ab->a.device);
+        (void)hipStreamSynchronize(st);
+        delete ab;
+        (void)hipStreamDestroy(st);
+    }
+}
+
+size_t fh_absorber_n(const fh_absorber* ab) { return ab ? ab->a.n : 0; }
+
+const uint32_t* fh_absorber_row_order(fh_absorber* ab) {
+    if (!ab) return nullptr;
+    Absorber& a = ab->a;
+    if (!a.host_pi_valid) {
+        int rc = guard([&] {
+            DeviceGuard g(a.device);
+            a.host_pi.resize(a.n);
+            d2h(a.host_pi.data(), a.pi.p, a.n, a.st);
+            FH_HIP(hipStreamSynchronize(a.st));
+            a.host_pi_valid = true;
+        });
+        if (rc) return nullptr;
+    }
+    return a.host_pi.data();
+}
+const uint32_t* fh_absorber_row_order_dev(const fh_absorber* ab) { return ab ? ab->a.pi.p : nullptr; }
+
+static int set_weights_impl(fh_absorber* ab, const double* w, const fh_ctx* ctx, bool dev) {
+    return guard([&] {
+        require(ab != nullptr, "null absorber");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        check_interrupt(ctx);
+        if (!w) {
+            refresh_weights(a, nullptr);
+            return;
+        }
+        DBuf<double> d(a.n);
+        if (dev) FH_HIP(hipMemcpyAsync(d.p, w, a.n * sizeof(double), hipMemcpyDeviceToDevice, a.st));
+        else h2d(d.p, w, a.n, a.st);
+        require(weights_ok(a, d.p), "weights must be positive and finite");
+        refresh_weights(a, d.p);
+    });
+}
+int fh_absorber_set_weights(fh_absorber* ab, const double* w, const fh_ctx* ctx) { return set_weights_impl(ab, w, ctx, false); }
+int fh_absorber_set_weights_dev(fh_absorber* ab, const double* w, const fh_ctx* ctx) { return set_weights_impl(ab, w, ctx, true); }
+
+static int residualize_impl(fh_absorber* ab, double* z, size_t r, const fh_solve_opts* o, const double* warm,
+                            double* coefs_out, fh_solve_info* info, const fh_ctx* ctx, bool dev) {
+    return guard([&] {
+        require(ab != nullptr && o != nullptr, "null argument");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        auto t0 = std::chrono::steady_clock::now();
+        const size_t n = a.n, p = a.m + a.g1;
+        if (info) *info = fh_solve_info{1, 0, 1, 0, 0.0};
+        if (r == 0) return;
+        require(z != nullptr, "null block");
+        check_interrupt(ctx);
+        const int prec = o->precision < 0 ? a.precision : o->precision;
+        ResidOut ro;
+        if (dev) {
+            ro = residualize_any(a, z, r, o->tol, o->maxiter, warm, coefs_out, prec, ctx);
+        } else {
+            DBuf<double> dz(n * r), dw, dcf;
+            h2d(dz.p, z, n * r, a.st);
+            if (warm) {
+                dw.alloc(p * r);
+                h2d(dw.p, warm, p * r, a.st);
+            }
+            if (coefs_out) dcf.alloc(p * r);
+            ro = residualize_any(a, dz.p, r, o->tol, o->maxiter, warm ? dw.p : nullptr, coefs_out ? dcf.p : nullptr, prec, ctx);
+            d2h(z, dz.p, n * r, a.st);
+            if (coefs_out) d2h(coefs_out, dcf.p, p * r, a.st);
+            FH_HIP(hipStreamSynchronize(a.st));
+        }
+        if (info) {
+            info->iterations = ro.iterations;
+            info->exact = ro.exact;
+            info->escalated = 0;
+            info->refinements = ro.refinements;
+            info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+    });
+}
+int fh_absorber_residualize(fh_absorber* ab, double* z, size_t r, const fh_solve_opts* o, const double* warm,
+                            double* coefs_out, fh_solve_info* info, const fh_ctx* ctx) {
+    return residualize_impl(ab, z, r, o, warm, coefs_out, info, ctx, false);
+}
+int fh_absorber_residualize_dev(fh_absorber* ab, double* z, size_t r, const fh_solve_opts* o, const double* warm,
+                                double* coefs_out, fh_solve_info* info, const fh_ctx* ctx) {
+    return residualize_impl(ab, z, r, o, warm, coefs_out, info, ctx, true);
+}
+
+size_t fh_absorber_coef_rows(const fh_absorber* ab) { return ab ? ab->a.m + ab->a.g1 : 0; }
+
+int fh_absorber_info(const fh_absorber* ab, fh_structure_info* o) {
+    return guard([&] {
+        require(ab && o, "null argument");
+        const Absorber& a = ab->a;
+        *o = fh_structure_info{a.n, a.nterms, a.g1, a.M, a.nb, a.m, a.ms, a.small_work, a.fe1, a.norm_fe1,
+                               (int32_t)!a.trivial(), 0};
+    });
+}
+
+int fh_absorber_copy_structure(fh_absorber* ab, uint32_t* pi, uint32_t* seg_off, uint32_t* cc, uint32_t* coo) {
+    return guard([&] {
+        require(ab != nullptr, "null absorber");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        if (pi) d2h(pi, a.pi.p, a.n, a.st);
+        if (seg_off) d2h(seg_off, a.seg_off.p, a.g1 + 1, a.st);
+        if (cc) d2h(cc, a.cell_codes.p, a.M * a.nb, a.st);
+        if (coo && !a.trivial()) d2h(coo, a.cell_obs_off.p, a.M + 1, a.st);
+        FH_HIP(hipStreamSynchronize(a.st));
+    });
+}
+
+int fh_absorber_copy_blocks(const fh_absorber* ab, int64_t* term, uint64_t* g, uint64_t* off, int32_t* small) {
+    return guard([&] {
+        require(ab != nullptr, "null absorber");
+        const Absorber& a = ab->a;
+        for (size_t b = 0; b < a.nb; b++) {
+            term[b] = a.blocks[b].term;
+            g[b] = a.blocks[b].g;
+            off[b] = a.blocks[b].off;
+            small[b] = a.blocks[b].small;
+        }
+    });
+}
+
+int fh_absorber_copy_level_cells(fh_absorber* ab, size_t b, uint32_t* perm, uint32_t* off) {
+    return guard([&] {
+        require(ab != nullptr, "null absorber");
+        Absorber& a = ab->a;
+        require(b < a.nb, "block index out of range");
+        DeviceGuard g(a.device);
+        ensure_level_cells(a, b);
+        d2h(perm, a.lperm[b].p, a.M, a.st);
+        d2h(off, a.loff[b].p, a.blocks[b].g + 1, a.st);
+        FH_HIP(hipStreamSynchronize(a.st));
+    });
+}
+
+int fh_absorber_copy_weight_state(fh_absorber* ab, double* n1, double* nr, double* diag, uint8_t* red, double* sw) {
+    return guard([&] {
+        require(ab != nullptr, "null absorber");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        if (n1) d2h(n1, a.n1.p, a.g1, a.st);
+        if (nr) d2h(nr, a.nr.p, a.m, a.st);
+        if (diag) d2h(diag, a.diag.p, a.m, a.st);
+        if (red) d2h(red, a.red.p, a.m, a.st);
+        if (sw) *sw = a.sw;
+        FH_HIP(hipStreamSynchronize(a.st));
+    });
+}
+
+int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int32_t precision) {
+    return guard([&] {
+        require(ab != nullptr && p && y, "null argument");
+        require(r >= 1 && r <= 16, "1..16 columns");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        const size_t m = a.m;
+        if (m == 0) return;
+        if (precision == FH_MIXED) {
+            std::vector<float> hp(p, p + m * r), hy(m * r);
+            DBuf<float> dp(m * r), dy(m * r);
+            h2d(dp.p, hp.data(), m * r, a.st);
+            spmv<float>(a, dp.p, dy.p, (int)r, a.s32, nullptr, a.st);
+            d2h(hy.data(), dy.p, m * r, a.st);
+            FH_HIP(hipStreamSynchronize(a.st));
+            for (size_t i = 0; i < m * r; i++) y[i] = hy[i];
+        } else {
+            DBuf<double> dp(m * r), dy(m * r);
+            h2d(dp.p, p, m * r, a.st);
+            spmv<double>(a, dp.p, dy.p, (int)r, a.s64, nullptr, a.st);
+            d2h(y, dy.p, m * r, a.st);
+            FH_HIP(hipStreamSynchronize(a.st));
+        }
+    });
+}
+
+int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t reps, double* ms, double* bytes) {
+    return guard([&] {
+        require(ab && ms && bytes && r >= 1 && r <= 16 && reps >= 1, "bad argument");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        const size_t m = a.m;
+        const double s = precision == FH_MIXED ? 4.0 : 8.0;
+        *bytes = 4.0 * a.nb * a.M + 12.0 * a.g1 + 2.0 * s * m * r + (a.unit() ? 0.0 : s * a.M);
+        *ms = 0.0;
+        if (m == 0) return;
+        hipEvent_t e0, e1;
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        auto run = [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            DBuf<T> dp(m * r), dy(m * r);
+            std::vector<T> hp(m * r);
+            for (size_t i = 0; i < m * r; i++) hp[i] = T(0.001 * (double)(i % 997));
+            h2d(dp.p, hp.data(), m * r, a.st);
+            Scratch<T>& sc = precision == FH_MIXED ? reinterpret_cast<Scratch<T>&>(a.s32) : reinterpret_cast<Scratch<T>&>(a.s64);
+            spmv<T>(a, dp.p, dy.p, (int)r, sc, nullptr, a.st);  // warm-up (scratch allocation)
+            FH_HIP(hipEventRecord(e0, a.st));
+            for (uint32_t k = 0; k < reps; k++) spmv<T>(a, dp.p, dy.p, (int)r, sc, nullptr, a.st);
+            FH_HIP(hipEventRecord(e1, a.st));
+            FH_HIP(hipEventSynchronize(e1));
+            float t = 0.f;
+            FH_HIP(hipEventElapsedTime(&t, e0, e1));
+            *ms = (double)t / reps;
+        };
+        if (precision == FH_MIXED) run((float*)nullptr);
+        else run((double*)nullptr);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
+// fit_rows (solve.rs:1313-1317; operator.rs fit_rows): alpha1[s(i)] + sum_b x[off_b + code_b(c(i))].
+int fh_absorber_fit_rows(fh_absorber* ab, const double* coefs, size_t r, size_t col, double* out) {
+    return guard([&] {
+        require(ab && coefs && out && col < r, "bad argument");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        std::vector<uint32_t> seg(a.g1 + 1), cc(a.M * a.nb), coo;
+        d2h(seg.data(), a.seg_off.p, a.g1 + 1, a.st);
+        d2h(cc.data(), a.cell_codes.p, a.M * a.nb, a.st);
+        if (!a.trivial()) {
+            coo.resize(a.M + 1);
+            d2h(coo.data(), a.cell_obs_off.p, a.M + 1, a.st);
+        }
+        FH_HIP(hipStreamSynchronize(a.st));
+        const size_t m = a.m;
+        for (size_t s = 0; s < a.g1; s++)
+            for (uint32_t c = seg[s]; c < seg[s + 1]; c++) {
+                double v = coefs[(m + s) * r + col];
+                for (size_t b = 0; b < a.nb; b++) v += coefs[(a.blocks[b].off + cc[(size_t)c * a.nb + b]) * r + col];
+                const size_t r0 = coo.empty() ? c : coo[c], r1 = coo.empty() ? c + 1 : coo[c + 1];
+                for (size_t i = r0; i < r1; i++) out[i] = v;
+            }
+    });
+}
+
+// term_values (solve.rs:1319-1343): per term levels, normalized so every term except FE1
+// (fe1_index) has weighted mean 0 and FE1 absorbs the shifts.
+int fh_absorber_term_values(fh_absorber* ab, const double* coefs, size_t r, size_t col, double* out, uint64_t* lens) {
+    return guard([&] {
+        require(ab && coefs && out && lens && col < r, "bad argument");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        const size_t nt = a.nterms;
+        if (a.fe1 < 0 || a.norm_fe1 < 0) {
+            for (size_t q = 0; q < nt; q++) lens[q] = 0;
+            return;
+        }
+        std::vector<double> n1(a.g1), nr(a.m);
+        d2h(n1.data(), a.n1.p, a.g1, a.st);
+        if (a.m) d2h(nr.data(), a.nr.p, a.m, a.st);
+        FH_HIP(hipStreamSynchronize(a.st));
+        std::vector<std::vector<double>> vals(nt), lw(nt);
+        vals[a.fe1].resize(a.g1);
+        lw[a.fe1] = n1;
+        for (size_t s = 0; s < a.g1; s++) vals[a.fe1][s] = coefs[(a.m + s) * r + col];
+        for (auto& b : a.blocks) {
+            vals[b.term].resize(b.g);
+            lw[b.term].assign(nr.begin() + b.off, nr.begin() + b.off + b.g);
+            for (size_t l = 0; l < b.g; l++) vals[b.term][l] = coefs[(b.off + l) * r + col];
+        }
+        double shift = 0.0;
+        for (size_t t = 0; t < nt; t++) {
+            if ((int64_t)t == a.norm_fe1) continue;
+            double mean = 0.0;
+            for (size_t l = 0; l < vals[t].size(); l++) mean += vals[t][l] * lw[t][l];
+            mean /= a.sw;
+            for (double& v : vals[t]) v -= mean;
+            shift += mean;
+        }
+        for (double& v : vals[a.norm_fe1]) v += shift;
+        size_t pos = 0;
+        for (size_t t = 0; t < nt; t++) {
+            lens[t] = vals[t].size();
+            std::memcpy(out + pos, vals[t].data(), vals[t].size() * sizeof(double));
+            pos += vals[t].size();
+        }
+    });
+}
+
+static int ols_impl(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
+                    const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, double* beta, double* vcov,
+                    int32_t* kept, fh_fit_stats* stats, bool dev) {
+    return guard([&] {
+        require(ab && y && X && ssc && o && beta && vcov && kept, "null argument");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        const size_t n = a.n;
+        if (dev) {
+            ols_fit(a, y, X, K, kind, cluster, nested, *ssc, *o, beta, vcov, kept, stats, nullptr);
+        } else {
+            DBuf<double> dy(n), dx(n * K);
+            DBuf<uint32_t> dc;
+            h2d(dy.p, y, n, a.st);
+            h2d(dx.p, X, n * K, a.st);
+            if (cluster) {
+                dc.alloc(n);
+                h2d(dc.p, cluster, n, a.st);
+            }
+            ols_fit(a, dy.p, dx.p, K, kind, cluster ? dc.p : nullptr, nested, *ssc, *o, beta, vcov, kept, stats, nullptr);
+        }
+    });
+}
+int fh_ols_fit(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
+               const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, double* beta, double* vcov,
+               int32_t* kept, fh_fit_stats* stats) {
+    return ols_impl(ab, y, X, K, kind, cluster, nested, ssc, o, beta, vcov, kept, stats, false);
+}
+int fh_ols_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
+                   const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, double* beta, double* vcov,
+                   int32_t* kept, fh_fit_stats* stats) {
+    return ols_impl(ab, y, X, K, kind, cluster, nested, ssc, o, beta, vcov, kept, stats, true);
+}
+
+static int ppml_impl(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
+                     const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter, const fh_solve_opts* o,
+                     double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats, bool dev) {
+    return guard([&] {
+        require(ab && y && X && ssc && o && beta && vcov && kept, "null argument");
+        Absorber& a = ab->a;
+        DeviceGuard g(a.device);
+        const size_t n = a.n;
+        if (dev) {
+            ppml_fit(a, y, X, K, kind, cluster, nested, *ssc, tol, maxiter, *o, beta, vcov, kept, stats, nullptr);
+        } else {
+            DBuf<double> dy(n), dx(n * K);
+            DBuf<uint32_t> dc;
+            h2d(dy.p, y, n, a.st);
+            h2d(dx.p, X, n * K, a.st);
+            if (cluster) {
+                dc.alloc(n);
+                h2d(dc.p, cluster, n, a.st);
+            }
+            ppml_fit(a, dy.p, dx.p, K, kind, cluster ? dc.p : nullptr, nested, *ssc, tol, maxiter, *o, beta, vcov, kept,
+                     stats, nullptr);
+        }
+    });
+}
+int fh_ppml_fit(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
+                const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter, const fh_solve_opts* o,
+                double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats) {
+    return ppml_impl(ab, y, X, K, kind, cluster, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, false);
+}
+int fh_ppml_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
+                    const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter, const fh_solve_opts* o,
+                    double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats) {
+    return ppml_impl(ab, y, X, K, kind, cluster, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, true);
+}
+
+int fh_device_alloc(void** p, size_t bytes) {
+    return guard([&] { FH_HIP(hipMalloc(p, bytes)); });
+}
+int fh_device_free(void* p) {
+    return guard([&] { FH_HIP(hipFree(p)); });
+}
+int fh_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    return guard([&] { FH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int fh_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    return guard([&] { FH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int fh_device_synchronize(void) {
+    return guard([&] { FH_HIP(hipDeviceSynchronize()); });
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+// Estimation kernels on the residualized block (FP64): the weighted Gram matrix
+// G = sum_i w_i z_i z_i' (stats.rs:195-251), the residual pass e = Z c with sum w e^2, and the
+// per-cluster score sums s_g = sum_{i in g} x_i e_i w_i of the CRV1 meat (vcov.rs:320-448), with
+// the rows sorted by cluster once (radix sort) so every sum is in a fixed order.
+#include <algorithm>
+
+#include "fh_internal.hpp"
+#include "fh_sort.hpp"
+
+namespace fh {
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int GRAM_TILE = 256;
+constexpr int GRAM_KMAX = 22;  // k(k+1)/2 <= 256 pairs, one per thread
+constexpr int GRID = 1024;
+
+// Per-workgroup partial Gram of columns [c0, c0+k) of the n x ld row-major block z, weights
+// w (optional) times v (optional): part[wg][a*k+b] for a <= b.
+__global__ __launch_bounds__(TPB) void k_gram(const double* __restrict__ z, size_t n, int ld, int c0, int k,
+                                              const double* __restrict__ w, const double* __restrict__ v,
+                                              double* __restrict__ part) {
+    __shared__ double tile[GRAM_TILE][GRAM_KMAX + 1];
+    __shared__ double wt[GRAM_TILE];
+    const int npair = k * (k + 1) / 2;
+    int pa = 0, pb = 0;
+    {
+        int t = threadIdx.x, a = 0;
+        while (a < k && t >= k - a) { t -= k - a; a++; }
+        pa = a;
+        pb = a + t;
+    }
+    double acc = 0.0;
+    for (size_t base = (size_t)blockIdx.x * GRAM_TILE; base < n; base += (size_t)gridDim.x * GRAM_TILE) {
+        const size_t rows = min((size_t)GRAM_TILE, n - base);
+        for (size_t e = threadIdx.x; e < rows * k; e += TPB) {
+            const size_t i = e / k;
+            const int c = (int)(e % k);
+            tile[i][c] = z[(base + i) * ld + c0 + c];
+        }
+        for (size_t i = threadIdx.x; i < rows; i += TPB) {
+            double x = w ? w[base + i] : 1.0;
+            if (v) x *= v[base + i];
+            wt[i] = x;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < npair)
+            for (size_t i = 0; i < rows; i++) acc += wt[i] * tile[i][pa] * tile[i][pb];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < npair) part[(size_t)blockIdx.x * npair + threadIdx.x] = acc;
+}
+
+// e_i = sum_j c_j z_ij; partial sums of w e^2 per workgroup.
+__global__ __launch_bounds__(TPB) void k_resid(const double* __restrict__ z, size_t n, int ld,
+                                               const double* __restrict__ c, const double* __restrict__ w,
+                                               double* __restrict__ e, double* __restrict__ part) {
+    __shared__ double cc[32];
+    if (threadIdx.x < (unsigned)ld) cc[threadIdx.x] = c[threadIdx.x];
+    __syncthreads();
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
+        double v = 0.0;
+        for (int j = 0; j < ld; j++) v += cc[j] * z[i * ld + j];
+        e[i] = v;
+        s += (w ? w[i] : 1.0) * v * v;
+    }
+    __shared__ double red[TPB / WAVE];
+    s = wave_sum(s);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < TPB / WAVE; q++) t += red[q];
+        part[blockIdx.x] = t;
+    }
+}
+
+// v_i = (w_i e_i)^2 (hetero meat weights, fit/vcov.rs:322-332).
+__global__ void k_sq_we(const double* __restrict__ e, const double* __restrict__ w, size_t n, double* __restrict__ v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double s = e[i] * (w ? w[i] : 1.0);
+        v[i] = s * s;
+    }
+}
+
+__global__ void k_gather_cluster(const uint32_t* __restrict__ cl_kept, const uint32_t* __restrict__ pi, size_t n,
+                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        keys[i] = cl_kept[pi[i]];
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_cl_offsets(const uint64_t* __restrict__ key, size_t n, size_t g, uint32_t* __restrict__ off) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        uint64_t s = key[c];
+        int64_t prev = c ? (int64_t)key[c - 1] : -1;
+        for (int64_t k = prev + 1; k <= (int64_t)s; k++) off[k] = (uint32_t)c;
+        if (c == n - 1)
+            for (size_t k = (size_t)s + 1; k <= g; k++) off[k] = (uint32_t)n;
+    }
+}
+
+// s_g = sum over the rows of cluster g (sorted order) of x_i e_i w_i; one thread per cluster.
+__global__ void k_cluster_scores(const uint32_t* __restrict__ off, const uint32_t* __restrict__ rows_sorted, size_t g,
+                                 const double* __restrict__ z, int ld, int c0, int k, const double* __restrict__ e,
+                                 const double* __restrict__ w, double* __restrict__ sg) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < g; q += (size_t)gridDim.x * blockDim.x) {
+        double acc[GRAM_KMAX];
+        for (int a = 0; a < k; a++) acc[a] = 0.0;
+        for (uint32_t j = off[q]; j < off[q + 1]; j++) {
+            const size_t i = rows_sorted[j];
+            const double s = e[i] * (w ? w[i] : 1.0);
+            for (int a = 0; a < k; a++) acc[a] += z[i * ld + c0 + a] * s;
+        }
+        for (int a = 0; a < k; a++) sg[q * k + a] = acc[a];
+    }
+}
+
+__global__ void k_count_nonempty(const uint32_t* __restrict__ off, size_t g, unsigned long long* cnt) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < g; q += (size_t)gridDim.x * blockDim.x)
+        if (off[q + 1] > off[q]) atomicAdd(cnt, 1ull);
+}
+
+}  // namespace
+
+void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
+               double* G_host) {
+    require(k >= 1 && k <= GRAM_KMAX, "at most 21 regressors are supported");
+    const int npair = k * (k + 1) / 2;
+    const unsigned grid = std::max(1u, std::min<unsigned>(GRID, (unsigned)((n + GRAM_TILE - 1) / GRAM_TILE)));
+    DBuf<double> part((size_t)grid * npair);
+    hipLaunchKernelGGL(k_gram, dim3(grid), dim3(TPB), 0, a.st, z, n, ld, c0, k, w, v, part.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> h((size_t)grid * npair);
+    FH_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, a.st));
+    FH_HIP(hipStreamSynchronize(a.st));
+    std::vector<double> s(npair, 0.0);
+    for (unsigned q = 0; q < grid; q++)
+        for (int p = 0; p < npair; p++) s[p] += h[(size_t)q * npair + p];
+    int p = 0;
+    for (int x = 0; x < k; x++)
+        for (int y = x; y < k; y++, p++) {
+            G_host[x * k + y] = s[p];
+            G_host[y * k + x] = s[p];
+        }
+}
+
+void gram(Absorber& a, const double* z, const double* w, size_t n, int r, double* G_host) {
+    gram_cols(a, z, n, r, 0, r, w, nullptr, G_host);
+}
+
+double residual_pass(Absorber& a, const double* z, size_t n, int ld, const double* c_host, const double* w, double* e) {
+    DBuf<double> c(ld), part(GRID);
+    FH_HIP(hipMemcpyAsync(c.p, c_host, ld * sizeof(double), hipMemcpyHostToDevice, a.st));
+    hipLaunchKernelGGL(k_resid, dim3(GRID), dim3(TPB), 0, a.st, z, n, ld, c.p, w, e, part.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> h(GRID);
+    FH_HIP(hipMemcpyAsync(h.data(), part.p, GRID * sizeof(double), hipMemcpyDeviceToHost, a.st));
+    FH_HIP(hipStreamSynchronize(a.st));
+    double s = 0.0;
+    for (double x : h) s += x;
+    return s;
+}
+
+void hetero_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
+                 double* meat) {
+    DBuf<double> v(n);
+    hipLaunchKernelGGL(k_sq_we, dim3(grid_for(n, TPB, 4096)), dim3(TPB), 0, a.st, e, w, n, v.p);
+    FH_CHECK_LAUNCH();
+    gram_cols(a, z, n, ld, c0, k, nullptr, v.p, meat);
+}
+
+// Returns the number of non-empty clusters.
+uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
+                      const uint32_t* cl_kept_dev, uint64_t G, double* meat) {
+    hipStream_t st = a.st;
+    DBuf<uint64_t> keys(n), kalt(n);
+    DBuf<uint32_t> vals(n), valt(n), off(G + 1);
+    hipLaunchKernelGGL(k_gather_cluster, dim3(grid_for(n, TPB, 4096)), dim3(TPB), 0, st, cl_kept_dev, a.pi.p, n, keys.p,
+                       vals.p);
+    FH_CHECK_LAUNCH();
+    radix_sort_pairs(keys.p, vals.p, kalt.p, valt.p, n, (int)bits_for(G), st);
+    FH_HIP(hipMemsetAsync(off.p, 0, off.bytes(), st));
+    hipLaunchKernelGGL(k_cl_offsets, dim3(grid_for(n, TPB, 4096)), dim3(TPB), 0, st, keys.p, n, (size_t)G, off.p);
+    DBuf<double> sg(G * k);
+    hipLaunchKernelGGL(k_cluster_scores, dim3(grid_for(G, TPB, 4096)), dim3(TPB), 0, st, off.p, vals.p, (size_t)G, z,
+                       ld, c0, k, e, w, sg.p);
+    DBuf<unsigned long long> cnt(1);
+    FH_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_count_nonempty, dim3(grid_for(G, TPB, 4096)), dim3(TPB), 0, st, off.p, (size_t)G, cnt.p);
+    FH_CHECK_LAUNCH();
+    gram_cols(a, sg.p, G, k, 0, k, nullptr, nullptr, meat);
+    unsigned long long c = 0;
+    FH_HIP(hipMemcpyAsync(&c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    return c;
+}
+
+}  // namespace fh

@@ -1,0 +1,138 @@
+// Internal state of an absorber and the host launchers of its kernels.
+//
+// Data layout in HBM (one absorber, n rows, M cells, g1 segments, m reduced coordinates):
+//   pi            u32[n]      kept-observation index of every row (row order = sorted order)
+//   seg_off       u32[g1+1]   first cell of every FE1 level
+//   cell_codes    u32[M*nb]   block codes per cell, row-major (cell, block)
+//   cell_obs_off  u32[M+1]    first row of every cell (absent when every cell is one row)
+//   cell_seg      u32[M]      FE1 level of every cell
+//   per large block b (level-major transpose of the cells, built once):
+//     loff        u32[g_b+1]  level offsets;  lperm u32[M] cells in (level, cell) order
+//     lseg        u32[M]      FE1 level of the cell at each level position
+//     lother      u32[M*(nb-1)] codes of the other blocks at each level position
+//     lw          T[M]        cell weight at each level position (weighted designs)
+//   weight state: cw T[M] (null = unit), inv_n1 T[g1], inv_diag T[m], minv T[ms*ms] (dense
+//   grounded pseudo-inverse of the small super-block).
+#pragma once
+
+#include <memory>
+
+#include "fh_common.hpp"
+
+namespace fh {
+
+constexpr int NB_MAX = 8;  // non-FE1 blocks supported by the kernels
+
+struct BlockInfo {
+    int64_t term;
+    uint64_t g, off;
+    bool small;
+};
+
+// Weight-dependent arrays in one precision.
+template <class T>
+struct WeightArrays {
+    DBuf<T> cw;        // M (empty = every cell weight is 1)
+    DBuf<T> inv_n1;    // g1
+    DBuf<T> inv_diag;  // m (0 on redundant coordinates)
+    std::vector<DBuf<T>> lw;  // per large block, M (empty when unit)
+    DBuf<T> minv;      // ms*ms
+};
+
+// Per-call scratch of the PCG and operator kernels (grown on demand).
+template <class T>
+struct Scratch {
+    DBuf<T> r, z, p, q, x, rhs, t, zc, part;
+    DBuf<double> dpart;  // FP64 partial sums (dots)
+};
+
+struct Absorber {
+    int device = 0;
+    hipStream_t st = nullptr;
+    // structure
+    uint64_t n = 0, nterms = 0, g1 = 0, M = 0, nb = 0, m = 0, ms = 0, small_work = 0;
+    int64_t fe1 = -1, norm_fe1 = -1;
+    std::vector<uint64_t> term_g;
+    std::vector<BlockInfo> blocks;
+    uint64_t small_limit = 4096;
+    int precision = FH_PLAIN_FP64;
+    DBuf<uint32_t> pi, seg_off, cell_codes, cell_obs_off, cell_seg;
+    std::vector<DBuf<uint32_t>> lperm, loff, lseg, lother;  // indexed by block (large only built)
+    std::vector<uint32_t> host_pi;
+    bool host_pi_valid = false;
+    // weights
+    bool weighted = false;
+    DBuf<double> w_rows;  // n (row order) when weighted
+    double sw = 0.0;
+    DBuf<double> n1, nr, diag;
+    DBuf<uint8_t> red;
+    uint64_t nkeep = 0;
+    WeightArrays<double> w64;
+    WeightArrays<float> w32;
+    bool w32_valid = false;
+    // scratch
+    Scratch<double> s64;
+    Scratch<float> s32;
+    DBuf<double> b1, rows_tmp;
+    DBuf<double> status;  // PCG scalars
+    DBuf<uint32_t> u32tmp;
+
+    bool trivial() const { return cell_obs_off.n == 0; }
+    bool unit() const { return !weighted && trivial(); }
+    uint32_t seg_lanes = 8, lvl_lanes = 8;  // lanes per segment / per level in the kernels
+};
+
+// ---------------------------------------------------------------- structure (fh_structure.hip)
+// Builds the cell structure from device term codes (kept order).
+void build_structure(Absorber& a, const std::vector<const uint32_t*>& dcodes);
+// Level-sorted cells of block b (built on demand for small blocks).
+void ensure_level_cells(Absorber& a, size_t b);
+
+// ---------------------------------------------------------------- weights (fh_weights.hip)
+void refresh_weights(Absorber& a, const double* w_rows_dev /* null = unit */);
+void ensure_w32(Absorber& a);
+
+// ---------------------------------------------------------------- operator (fh_operator.hip)
+template <class T>
+struct OpArgs {
+    const uint32_t* cell_codes;
+    const uint32_t* seg_off;
+    const uint32_t* cell_obs_off;
+    const T* cw;
+    const T* inv_n1;
+    uint64_t g1, M, m, ms;
+    int nb, nsb;  // blocks, small blocks (the first nsb blocks)
+    uint32_t boff[NB_MAX];
+};
+
+template <class T>
+OpArgs<T> op_args(const Absorber& a);
+
+// y = S p (m x r). t_buf >= g1*r, part >= spmv_part_len(a, r).
+template <class T>
+void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st);
+
+// rhs pass over rows (n x r, FP64): rhs (m x r, T), b1 (g1 x r), wsq/wsum partials -> out2 (2r).
+void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, double* wsq_wsum);
+// rows -= alpha_s + u_c; alpha (g1 x r) optional.
+void residualize_rows(Absorber& a, double* rows, int r, const double* x, const double* b1, double* alpha);
+
+// ---------------------------------------------------------------- PCG (fh_pcg.hip)
+struct PcgResult {
+    uint64_t iterations = 0;
+    bool exact = false;
+    int rc = FH_OK;
+    uint64_t refinements = 0;
+};
+// Solves S x = rhs for r columns; x holds the start. FP64 throughout.
+PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, int r, const double* ss1, const double* ss0,
+                   double tol, uint64_t maxiter, const fh_ctx* ctx);
+// Mixed form: FP32 inner PCG, FP64 outer residual and update.
+PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, int r, const double* ss1, const double* ss0,
+                    double tol, uint64_t maxiter, const fh_ctx* ctx);
+
+// ---------------------------------------------------------------- estimation (fh_estimate.hip)
+// G = sum_i w_i z_i z_i' (r x r, FP64 on the host), z n x r row-major; w may be null.
+void gram(Absorber& a, const double* z, const double* w, size_t n, int r, double* G_host);
+
+}  // namespace fh

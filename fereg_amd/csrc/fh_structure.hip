@@ -1,0 +1,441 @@
+// Cell structure of the intercept-only FE design on the GPU (bit-exact with
+// crates/fereg-core/src/structure.rs:356-527):
+//   rows sorted by (FE1 code, block codes in block order, obs) with a stable LSD radix sort of
+//   packed keys (obs order is the stable tie-break), cells = runs of equal tuples found by a
+//   flag + exclusive-scan pass, seg_off from the cells' FE1 codes, sum_seg s^2 of the small
+//   coordinates with a per-wave LDS bitmap, and the level-major transpose of every large block
+//   (a second radix sort of the cells by code, structure.rs:144-153).
+#include <algorithm>
+#include <numeric>
+
+#include "fh_internal.hpp"
+#include "fh_sort.hpp"
+
+namespace fh {
+
+namespace {
+
+struct CodeSet {
+    const uint32_t* c[NB_MAX + 1];
+    uint32_t shift[NB_MAX + 1];
+    int nq;
+};
+
+__global__ void k_hist(const uint32_t* __restrict__ codes, size_t n, uint32_t* __restrict__ cnt) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[codes[i]], 1u);
+}
+
+__global__ void k_sumsq(const uint32_t* __restrict__ v, size_t g, unsigned long long* out) {
+    unsigned long long s = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < g; i += (size_t)gridDim.x * blockDim.x)
+        s += (unsigned long long)v[i] * v[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, WAVE);
+    if (lane_id() == 0) atomicAdd(out, s);
+}
+
+// key[i] = packed fields of row vals[i] (fields of the word: most significant first).
+__global__ void k_pack(CodeSet cs, const uint32_t* __restrict__ vals, size_t n, uint64_t* __restrict__ keys) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t o = vals ? vals[i] : (uint32_t)i;
+        uint64_t k = 0;
+        for (int q = 0; q < cs.nq; q++) k |= (uint64_t)cs.c[q][o] << cs.shift[q];
+        keys[i] = k;
+    }
+}
+
+__global__ void k_iota(uint32_t* __restrict__ v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        v[i] = (uint32_t)i;
+}
+
+// Row i starts a cell when its (FE1, block codes) tuple differs from row i-1's.
+__global__ void k_cell_flags(CodeSet cs, const uint32_t* __restrict__ pi, size_t n, uint32_t* __restrict__ flags) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t f = 1;
+        if (i > 0) {
+            uint32_t o = pi[i], p = pi[i - 1];
+            f = 0;
+            for (int q = 0; q < cs.nq; q++) f |= cs.c[q][o] != cs.c[q][p];
+        }
+        flags[i] = f;
+    }
+}
+
+__global__ void k_cell_start(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ scan, size_t n,
+                             uint32_t* __restrict__ start) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        if (flags[i]) start[scan[i]] = (uint32_t)i;
+}
+
+// Per cell: FE1 level and block codes of its first row.
+__global__ void k_cell_codes(CodeSet cs, int has_fe1, const uint32_t* __restrict__ start,
+                             const uint32_t* __restrict__ pi, size_t M, uint32_t* __restrict__ cell_seg,
+                             uint32_t* __restrict__ cell_codes) {
+    const int nb = cs.nq - 1;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
+        uint32_t o = pi[start[c]];
+        cell_seg[c] = has_fe1 ? cs.c[0][o] : 0u;
+        for (int b = 0; b < nb; b++) cell_codes[c * nb + b] = cs.c[1 + b][o];
+    }
+}
+
+// off[k] = first position whose (ascending) key is >= k, for k in 0..=g.
+__global__ void k_offsets(const uint32_t* __restrict__ key, size_t M, size_t g, uint32_t* __restrict__ off) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
+        uint32_t s = key[c];
+        int64_t prev = c ? (int64_t)key[c - 1] : -1;
+        for (int64_t k = prev + 1; k <= (int64_t)s; k++) off[k] = (uint32_t)c;
+        if (c == M - 1)
+            for (size_t k = (size_t)s + 1; k <= g; k++) off[k] = (uint32_t)M;
+    }
+}
+
+__global__ void k_offsets_u64(const uint64_t* __restrict__ key, size_t M, size_t g, uint32_t* __restrict__ off) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
+        uint64_t s = key[c];
+        int64_t prev = c ? (int64_t)key[c - 1] : -1;
+        for (int64_t k = prev + 1; k <= (int64_t)s; k++) off[k] = (uint32_t)c;
+        if (c == M - 1)
+            for (size_t k = (size_t)s + 1; k <= g; k++) off[k] = (uint32_t)M;
+    }
+}
+
+// sum over segments of (distinct small coordinates)^2: one wave per segment, LDS bitmap.
+constexpr int SW_WG = 256;
+__global__ __launch_bounds__(SW_WG) void k_small_work(const uint32_t* __restrict__ seg_off,
+                                                      const uint32_t* __restrict__ cell_codes, size_t g1, int nb,
+                                                      int nsb, const uint32_t* __restrict__ boff_small, int words,
+                                                      unsigned long long* out) {
+    extern __shared__ uint32_t bm[];
+    const int w = threadIdx.x / WAVE, l = lane_id();
+    uint32_t* my = bm + (size_t)w * words;
+    unsigned long long acc = 0;
+    const size_t nwaves = (size_t)gridDim.x * (SW_WG / WAVE);
+    for (size_t s = (size_t)blockIdx.x * (SW_WG / WAVE) + w; s < g1; s += nwaves) {
+        for (int i = l; i < words; i += WAVE) my[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        uint32_t c0 = seg_off[s], c1 = seg_off[s + 1];
+        for (uint32_t c = c0 + l; c < c1; c += WAVE)
+            for (int b = 0; b < nsb; b++) {
+                uint32_t k = boff_small[b] + cell_codes[(size_t)c * nb + b];
+                atomicOr(&my[k >> 5], 1u << (k & 31));
+            }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t d = 0;
+        for (int i = l; i < words; i += WAVE) d += __popc(my[i]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, WAVE);
+        acc += (unsigned long long)d * d;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (l == 0) atomicAdd(out, acc);
+}
+
+__global__ void k_level_keys(const uint32_t* __restrict__ cell_codes, size_t M, int nb, int b,
+                             uint64_t* __restrict__ keys) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x)
+        keys[c] = cell_codes[c * nb + b];
+}
+
+__global__ void k_level_records(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ cell_seg,
+                                const uint32_t* __restrict__ cell_codes, size_t M, int nb, int b,
+                                uint32_t* __restrict__ lseg, uint32_t* __restrict__ lother) {
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (size_t)gridDim.x * blockDim.x) {
+        uint32_t c = perm[j];
+        lseg[j] = cell_seg[c];
+        int k = 0;
+        for (int bb = 0; bb < nb; bb++)
+            if (bb != b) lother[j * (nb - 1) + k++] = cell_codes[(size_t)c * nb + bb];
+    }
+}
+
+// E of eliminating term f against one other term o: sum_s d_{s,o}^2 (structure.rs:325-354),
+// from the sorted (code_f, code_o) pairs: d_s = distinct pairs of level s.
+__global__ void k_pair_keys(const uint32_t* __restrict__ cf, const uint32_t* __restrict__ co, int bits_o, size_t n,
+                            uint64_t* __restrict__ keys) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        keys[i] = ((uint64_t)cf[i] << bits_o) | co[i];
+}
+__global__ void k_distinct(const uint64_t* __restrict__ keys, size_t n, int bits_o, uint32_t* __restrict__ d) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        if (i == 0 || keys[i] != keys[i - 1]) atomicAdd(&d[keys[i] >> bits_o], 1u);
+}
+
+constexpr unsigned TPB = 256;
+inline unsigned blocks_for(size_t n) { return grid_for(n, TPB, 8192); }
+
+template <class T>
+T read_scalar(const T* dev, hipStream_t st) {
+    T v;
+    FH_HIP(hipMemcpyAsync(&v, dev, sizeof(T), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    return v;
+}
+
+}  // namespace
+
+// choose_elimination (structure.rs:256-313), with the histogram and E sums on the GPU.
+static int64_t choose_fe1(Absorber& a, const std::vector<const uint32_t*>& dc) {
+    const auto& g = a.term_g;
+    const size_t nt = g.size(), n = a.n;
+    int64_t f0 = -1;
+    for (size_t q = 0; q < nt; q++)
+        if (f0 < 0 || g[q] > g[f0]) f0 = (int64_t)q;  // fe1_index, absorber.rs:99-108
+    a.norm_fe1 = f0;
+    if (f0 < 0 || nt < 3) return f0;
+    std::vector<size_t> cands;
+    for (size_t q = 0; q < nt; q++)
+        if (g[q] * 4 >= g[f0]) cands.push_back(q);
+    if (cands.size() < 2) return f0;
+    hipStream_t st = a.st;
+    DBuf<uint32_t> cnt(std::max<uint64_t>(g[f0], 1));
+    DBuf<unsigned long long> acc(1);
+    FH_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), st));
+    FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_hist, dim3(blocks_for(n)), dim3(TPB), 0, st, dc[f0], n, cnt.p);
+    hipLaunchKernelGGL(k_sumsq, dim3(blocks_for(g[f0])), dim3(TPB), 0, st, cnt.p, (size_t)g[f0], acc.p);
+    FH_CHECK_LAUNCH();
+    unsigned long long ssq = read_scalar(acc.p, st);
+    if ((nt - 1) * ssq <= 64ull * n) return f0;
+    DBuf<uint64_t> keys(n), kalt(n);
+    DBuf<uint32_t> vals(n), valt(n);
+    int64_t best = -1;
+    unsigned long long bestE = 0;
+    for (size_t f : cands) {
+        unsigned long long e = 0;
+        DBuf<uint32_t> d(std::max<uint64_t>(g[f], 1));
+        for (size_t o = 0; o < nt; o++) {
+            if (o == f) continue;
+            int bo = (int)bits_for(g[o]);
+            hipLaunchKernelGGL(k_pair_keys, dim3(blocks_for(n)), dim3(TPB), 0, st, dc[f], dc[o], bo, n, keys.p);
+            hipLaunchKernelGGL(k_iota, dim3(blocks_for(n)), dim3(TPB), 0, st, vals.p, n);
+            FH_CHECK_LAUNCH();
+            radix_sort_pairs(keys.p, vals.p, kalt.p, valt.p, n, (int)(bits_for(g[f]) + bo), st);
+            FH_HIP(hipMemsetAsync(d.p, 0, d.bytes(), st));
+            FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
+            hipLaunchKernelGGL(k_distinct, dim3(blocks_for(n)), dim3(TPB), 0, st, keys.p, n, bo, d.p);
+            hipLaunchKernelGGL(k_sumsq, dim3(blocks_for(g[f])), dim3(TPB), 0, st, d.p, (size_t)g[f], acc.p);
+            FH_CHECK_LAUNCH();
+            e += read_scalar(acc.p, st);
+        }
+        // key (E, Reverse(g), f): smaller E, then more levels, then lower index
+        if (best < 0 || e < bestE || (e == bestE && g[f] > g[best])) {
+            best = (int64_t)f;
+            bestE = e;
+        }
+    }
+    return best;
+}
+
+void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
+    hipStream_t st = a.st;
+    const size_t n = a.n, nt = a.nterms;
+    a.fe1 = choose_fe1(a, dc);
+    a.g1 = a.fe1 >= 0 ? a.term_g[a.fe1] : 1;
+
+    // small/large classification: ascending (G, q) while the running total fits (structure.rs:366-385)
+    std::vector<int64_t> others;
+    for (size_t q = 0; q < nt; q++)
+        if ((int64_t)q != a.fe1) others.push_back((int64_t)q);
+    require(others.size() <= (size_t)NB_MAX, "at most 9 absorbed terms are supported");
+    std::vector<int64_t> by_g = others;
+    std::stable_sort(by_g.begin(), by_g.end(), [&](int64_t x, int64_t y) {
+        return a.term_g[x] != a.term_g[y] ? a.term_g[x] < a.term_g[y] : x < y;
+    });
+    std::vector<bool> small_set(nt, false);
+    uint64_t total = 0;
+    for (int64_t q : by_g) {
+        if (total + a.term_g[q] > a.small_limit) break;
+        total += a.term_g[q];
+        small_set[q] = true;
+    }
+    a.blocks.clear();
+    uint64_t off = 0;
+    for (int pass = 0; pass < 2; pass++)
+        for (int64_t q : others) {
+            if ((pass == 0) != small_set[q]) continue;
+            a.blocks.push_back(BlockInfo{q, a.term_g[q], off, small_set[q]});
+            off += a.term_g[q];
+        }
+    a.nb = a.blocks.size();
+    a.m = off;
+    a.ms = 0;
+    for (auto& b : a.blocks)
+        if (b.small) a.ms += b.g;
+    const int nb = (int)a.nb;
+    require(a.m < (1ull << 32) && n < (1ull << 32), "sizes beyond 2^32 rows or levels are not supported");
+
+    // field list: FE1 then blocks (most significant first)
+    std::vector<const uint32_t*> fields;
+    std::vector<unsigned> fbits;
+    DBuf<uint32_t> zeros;
+    if (a.fe1 >= 0) {
+        fields.push_back(dc[a.fe1]);
+        fbits.push_back(bits_for(a.g1));
+    }
+    for (auto& b : a.blocks) {
+        fields.push_back(dc[b.term]);
+        fbits.push_back(bits_for(b.g));
+    }
+
+    // rows sorted by the fields, obs as the stable tie-break: words of <= 64 bits from the least
+    // significant end, each sorted stably in turn (LSD over words).
+    a.pi.alloc(std::max<size_t>(n, 1));
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(n)), dim3(TPB), 0, st, a.pi.p, n);
+    FH_CHECK_LAUNCH();
+    if (n > 1 && !fields.empty()) {
+        // group fields into words
+        std::vector<std::pair<size_t, size_t>> words;  // [begin, end) field ranges
+        size_t e = fields.size();
+        while (e > 0) {
+            size_t b = e;
+            unsigned w = 0;
+            while (b > 0 && w + fbits[b - 1] <= 64) { w += fbits[b - 1]; b--; }
+            require(b < e, "a code wider than 64 bits");
+            words.push_back({b, e});
+            e = b;
+        }
+        DBuf<uint64_t> keys(n), kalt(n);
+        DBuf<uint32_t> valt(n);
+        DBuf<uint32_t> flag(1);
+        bool first = true;
+        for (auto [b, e2] : words) {
+            CodeSet cs{};
+            unsigned sh = 0, width = 0;
+            cs.nq = (int)(e2 - b);
+            for (size_t k = e2; k-- > b;) {
+                cs.c[k - b] = fields[k];
+                cs.shift[k - b] = sh;
+                sh += fbits[k];
+            }
+            width = sh;
+            if (width == 0) continue;
+            hipLaunchKernelGGL(k_pack, dim3(blocks_for(n)), dim3(TPB), 0, st, cs, first ? nullptr : a.pi.p, n, keys.p);
+            FH_CHECK_LAUNCH();
+            bool sorted = false;
+            if (first && words.size() == 1) {
+                check_sorted_u64(keys.p, n, flag.p, st);
+                sorted = read_scalar(flag.p, st) == 0;
+            }
+            if (!sorted) radix_sort_pairs(keys.p, a.pi.p, kalt.p, valt.p, n, (int)width, st);
+            first = false;
+        }
+    }
+
+    // cells: flags, exclusive scan, starts, codes, segments
+    CodeSet all{};
+    all.nq = 0;
+    if (a.fe1 >= 0) all.c[all.nq++] = dc[a.fe1];
+    else {
+        zeros.alloc(std::max<size_t>(n, 1));
+        FH_HIP(hipMemsetAsync(zeros.p, 0, zeros.bytes(), st));
+        all.c[all.nq++] = zeros.p;
+    }
+    for (auto& b : a.blocks) all.c[all.nq++] = dc[b.term];
+    DBuf<uint32_t> flags(std::max<size_t>(n, 1)), scan(std::max<size_t>(n, 1)), mdev(1);
+    hipLaunchKernelGGL(k_cell_flags, dim3(blocks_for(n)), dim3(TPB), 0, st, all, a.pi.p, n, flags.p);
+    FH_CHECK_LAUNCH();
+    exclusive_scan_u32(flags.p, scan.p, n, mdev.p, st);
+    a.M = n ? read_scalar(mdev.p, st) : 0;
+    const size_t M = a.M;
+    DBuf<uint32_t> start(M + 1);
+    hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(n)), dim3(TPB), 0, st, flags.p, scan.p, n, start.p);
+    FH_CHECK_LAUNCH();
+    a.cell_seg.alloc(std::max<size_t>(M, 1));
+    a.cell_codes.alloc(std::max<size_t>(M * nb, 1));
+    hipLaunchKernelGGL(k_cell_codes, dim3(blocks_for(M)), dim3(TPB), 0, st, all, (int)(a.fe1 >= 0), start.p, a.pi.p,
+                       M, a.cell_seg.p, a.cell_codes.p);
+    FH_CHECK_LAUNCH();
+    a.seg_off.alloc(a.g1 + 1);
+    FH_HIP(hipMemsetAsync(a.seg_off.p, 0, a.seg_off.bytes(), st));
+    if (M > 0) {
+        hipLaunchKernelGGL(k_offsets, dim3(blocks_for(M)), dim3(TPB), 0, st, a.cell_seg.p, M, (size_t)a.g1, a.seg_off.p);
+        FH_CHECK_LAUNCH();
+    }
+    if (M != n) {
+        uint32_t nn = (uint32_t)n;
+        FH_HIP(hipMemcpyAsync(start.p + M, &nn, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        a.cell_obs_off = std::move(start);
+    } else {
+        a.cell_obs_off.release();
+    }
+
+    // sum_seg s^2 of the small coordinates, then the demotion rule (structure.rs:521-525)
+    a.small_work = 0;
+    int nsb = 0;
+    for (auto& b : a.blocks) nsb += b.small ? 1 : 0;
+    if (a.ms > 0 && M > 0) {
+        int words = (int)((a.ms + 31) / 32);
+        require(words * 4 * (SW_WG / WAVE) <= 64 * 1024, "small_limit above 131072 levels is not supported");
+        std::vector<uint32_t> boff(nsb);
+        for (int b = 0; b < nsb; b++) boff[b] = (uint32_t)a.blocks[b].off;
+        DBuf<uint32_t> dboff(nsb);
+        FH_HIP(hipMemcpyAsync(dboff.p, boff.data(), nsb * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        DBuf<unsigned long long> acc(1);
+        FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
+        unsigned grid = grid_for(a.g1, SW_WG / WAVE, 4096);
+        hipLaunchKernelGGL(k_small_work, dim3(grid), dim3(SW_WG), words * 4 * (SW_WG / WAVE), st, a.seg_off.p,
+                           a.cell_codes.p, (size_t)a.g1, nb, nsb, dboff.p, words, acc.p);
+        FH_CHECK_LAUNCH();
+        a.small_work = read_scalar(acc.p, st);
+        if (a.small_work > 64ull * M) {
+            for (auto& b : a.blocks) b.small = false;
+            a.ms = 0;
+        }
+    }
+
+    // level-major transpose of every large block
+    a.lperm.clear(); a.loff.clear(); a.lseg.clear(); a.lother.clear();
+    a.lperm.resize(nb); a.loff.resize(nb); a.lseg.resize(nb); a.lother.resize(nb);
+    for (int b = 0; b < nb; b++) {
+        if (a.blocks[b].small) continue;
+        ensure_level_cells(a, b);
+        a.lseg[b].alloc(std::max<size_t>(M, 1));
+        a.lother[b].alloc(std::max<size_t>(M * (nb - 1), 1));
+        if (M > 0) {
+            hipLaunchKernelGGL(k_level_records, dim3(blocks_for(M)), dim3(TPB), 0, st, a.lperm[b].p, a.cell_seg.p,
+                               a.cell_codes.p, M, nb, b, a.lseg[b].p, a.lother[b].p);
+            FH_CHECK_LAUNCH();
+        }
+    }
+    // lanes per segment / per level: the next power of two of the mean length, within 1..64
+    auto lanes = [](double mean) {
+        uint32_t L = 1;
+        while (L < 64 && L < mean) L <<= 1;
+        return L;
+    };
+    a.seg_lanes = lanes(a.g1 ? double(M) / double(a.g1) : 1.0);
+    uint64_t glarge = 0;
+    int nlarge = 0;
+    for (auto& b : a.blocks)
+        if (!b.small) { glarge += b.g; nlarge++; }
+    a.lvl_lanes = lanes(glarge ? double(M) * nlarge / double(glarge) : 1.0);
+    FH_HIP(hipStreamSynchronize(st));
+}
+
+void ensure_level_cells(Absorber& a, size_t b) {
+    if (a.lperm.size() < a.nb) { a.lperm.resize(a.nb); a.loff.resize(a.nb); }
+    if (a.lperm[b].n) return;
+    hipStream_t st = a.st;
+    const size_t M = a.M;
+    const int nb = (int)a.nb;
+    const uint64_t g = a.blocks[b].g;
+    a.lperm[b].alloc(std::max<size_t>(M, 1));
+    a.loff[b].alloc(g + 1);
+    FH_HIP(hipMemsetAsync(a.loff[b].p, 0, a.loff[b].bytes(), st));
+    if (M == 0) return;
+    DBuf<uint64_t> keys(M), kalt(M);
+    DBuf<uint32_t> valt(M), flag(1);
+    hipLaunchKernelGGL(k_level_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, a.cell_codes.p, M, nb, (int)b, keys.p);
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.lperm[b].p, M);
+    FH_CHECK_LAUNCH();
+    check_sorted_u64(keys.p, M, flag.p, st);
+    if (read_scalar(flag.p, st) != 0) radix_sort_pairs(keys.p, a.lperm[b].p, kalt.p, valt.p, M, (int)bits_for(g), st);
+    hipLaunchKernelGGL(k_offsets_u64, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, (size_t)g, a.loff[b].p);
+    FH_CHECK_LAUNCH();
+    FH_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace fh

@@ -1,0 +1,315 @@
+// Weight state on the GPU (operator.rs:211-338, 344-570): cell weights W_c, FE1 level weights
+// n1 (and sum w), level weights n_r, diag(S) = n_r - sum_s c_{s,k}^2/n1_s, redundant flags,
+// the Jacobi inverse diagonal (precond/jacobi.rs:7-9), and the dense small super-block S_SS
+// whose grounded LDL' factor (precond/dense.rs:25-37, 79-115) is turned into an explicit
+// pseudo-inverse on the host (an m_s x m_s FP64 factorization) and applied on the GPU.
+#include <algorithm>
+#include <cmath>
+
+#include "fh_internal.hpp"
+
+namespace fh {
+
+namespace {
+
+constexpr unsigned TPB = 256;
+inline unsigned blocks_for(size_t n) { return grid_for(n, TPB, 8192); }
+
+__global__ void k_cell_weights(const uint32_t* __restrict__ coo, const double* __restrict__ w, size_t M,
+                               double* __restrict__ cw, float* __restrict__ cw32) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
+        size_t r0 = coo ? coo[c] : c, r1 = coo ? coo[c + 1] : c + 1;
+        double s = 0.0;
+        for (size_t i = r0; i < r1; i++) s += w ? w[i] : 1.0;
+        cw[c] = s;
+        cw32[c] = (float)s;
+    }
+}
+
+__global__ void k_n1(const uint32_t* __restrict__ seg_off, const double* __restrict__ cw, size_t g1,
+                     double* __restrict__ n1, double* __restrict__ inv64, float* __restrict__ inv32) {
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
+        uint32_t c0 = seg_off[s], c1 = seg_off[s + 1];
+        double v = 0.0;
+        if (cw)
+            for (uint32_t c = c0; c < c1; c++) v += cw[c];
+        else
+            v = double(c1 - c0);
+        n1[s] = v;
+        inv64[s] = 1.0 / v;
+        inv32[s] = (float)(1.0 / v);
+    }
+}
+
+// Deterministic partial sums (fixed grid, fixed order): out[blockIdx.x].
+__global__ __launch_bounds__(TPB) void k_partial_sum(const double* __restrict__ v, size_t n, double* __restrict__ out) {
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) s += v[i];
+    __shared__ double ws[TPB / WAVE];
+    s = wave_sum(s);
+    if (lane_id() == 0) ws[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(TPB / WAVE); i++) t += ws[i];
+        out[blockIdx.x] = t;
+    }
+}
+
+// Large block: one thread per level walks its cells in (level, cell) order; cells of one FE1
+// level are adjacent there, so c_{s,k} are run sums.
+__global__ void k_level_weights(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lperm,
+                                const uint32_t* __restrict__ lseg, const double* __restrict__ cw,
+                                const double* __restrict__ n1, size_t g, size_t off, double* __restrict__ nr,
+                                double* __restrict__ diag, uint8_t* __restrict__ red, double* __restrict__ inv64,
+                                float* __restrict__ inv32, double* __restrict__ lw64, float* __restrict__ lw32) {
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
+        uint32_t j0 = loff[l], j1 = loff[l + 1];
+        double tot = 0.0, dq = 0.0, run = 0.0;
+        uint32_t cur = 0xFFFFFFFFu;
+        for (uint32_t j = j0; j < j1; j++) {
+            double w = cw ? cw[lperm[j]] : 1.0;
+            if (lw64) { lw64[j] = w; lw32[j] = (float)w; }
+            uint32_t s = lseg[j];
+            if (s != cur) {
+                if (cur != 0xFFFFFFFFu) dq += run * run / n1[cur];
+                cur = s;
+                run = 0.0;
+            }
+            run += w;
+            tot += w;
+        }
+        if (cur != 0xFFFFFFFFu) dq += run * run / n1[cur];
+        double d = tot - dq;
+        bool r = d <= 1e-12 * tot;  // operator.rs:21, 319-333
+        nr[off + l] = tot;
+        diag[off + l] = d;
+        red[off + l] = r;
+        double inv = (r || !(d > 0.0)) ? 0.0 : 1.0 / d;  // jacobi.rs:7-9
+        inv64[off + l] = inv;
+        inv32[off + l] = (float)inv;
+    }
+}
+
+struct SmallOffs {
+    uint32_t off[NB_MAX];
+    int nsb, nb;
+};
+
+// S_SS (+ n_r of the small coordinates in row ms) per workgroup: one thread per segment adds
+// sum_c W_c [k][l] and subtracts sum_{c,c'} W_c W_c' / n1_s [k(c)][l(c')] (operator.rs:1457-1560).
+constexpr int SA_LDS_MAX = 64;  // small coordinates accumulated in LDS (ms*ms + ms doubles)
+__global__ __launch_bounds__(TPB) void k_small_assemble(const uint32_t* __restrict__ seg_off,
+                                                        const uint32_t* __restrict__ cell_codes,
+                                                        const double* __restrict__ cw, const double* __restrict__ n1,
+                                                        size_t g1, size_t ms, SmallOffs so, double* __restrict__ out) {
+    extern __shared__ double acc[];
+    const size_t len = ms * ms + ms;
+    const bool lds = ms <= SA_LDS_MAX;
+    double* dst = lds ? acc : out;
+    if (lds) {
+        for (size_t i = threadIdx.x; i < len; i += blockDim.x) acc[i] = 0.0;
+        __syncthreads();
+    }
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
+        uint32_t c0 = seg_off[s], c1 = seg_off[s + 1];
+        double inv = 1.0 / n1[s];
+        for (uint32_t c = c0; c < c1; c++) {
+            double w = cw ? cw[c] : 1.0;
+            for (int b = 0; b < so.nsb; b++) {
+                uint32_t k = so.off[b] + cell_codes[(size_t)c * so.nb + b];
+                atomicAdd(&dst[ms * ms + k], w);
+                for (int b2 = 0; b2 < so.nsb; b2++) {
+                    uint32_t l = so.off[b2] + cell_codes[(size_t)c * so.nb + b2];
+                    atomicAdd(&dst[(size_t)k * ms + l], w);
+                }
+            }
+            for (uint32_t c2 = c0; c2 < c1; c2++) {
+                double w2 = w * (cw ? cw[c2] : 1.0) * inv;
+                for (int b = 0; b < so.nsb; b++) {
+                    uint32_t k = so.off[b] + cell_codes[(size_t)c * so.nb + b];
+                    for (int b2 = 0; b2 < so.nsb; b2++) {
+                        uint32_t l = so.off[b2] + cell_codes[(size_t)c2 * so.nb + b2];
+                        atomicAdd(&dst[(size_t)k * ms + l], -w2);
+                    }
+                }
+            }
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        double* part = out + (size_t)blockIdx.x * len;
+        for (size_t i = threadIdx.x; i < len; i += blockDim.x) part[i] = acc[i];
+    }
+}
+
+__global__ void k_reduce_parts(const double* __restrict__ parts, size_t nparts, size_t len, double* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (size_t p = 0; p < nparts; p++) s += parts[p * len + i];
+        out[i] = s;
+    }
+}
+
+double sum_dev(const double* v, size_t n, hipStream_t st) {
+    const unsigned nb = 256;
+    DBuf<double> part(nb);
+    hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(TPB), 0, st, v, n, part.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> h(nb);
+    FH_HIP(hipMemcpyAsync(h.data(), part.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double x : h) s += x;
+    return s;
+}
+
+// Grounded LDL' of S_SS over the kept coordinates and its explicit inverse
+// (precond/dense.rs:25-37, 79-160): unit-diagonal scaling, pivots <= 1e-10 replaced by 1e30.
+std::vector<double> grounded_inverse(const std::vector<double>& a, size_t ms, const std::vector<uint8_t>& red,
+                                     uint64_t& nkeep) {
+    std::vector<uint32_t> keep;
+    for (size_t i = 0; i < ms; i++)
+        if (!red[i] && a[i * ms + i] > 0.0) keep.push_back((uint32_t)i);
+    const size_t k = keep.size();
+    nkeep = k;
+    std::vector<double> inv(ms * ms, 0.0);
+    if (k == 0) return inv;
+    std::vector<double> sc(k), L(k * k, 0.0);
+    for (size_t i = 0; i < k; i++) sc[i] = 1.0 / std::sqrt(a[(size_t)keep[i] * (ms + 1)]);
+    for (size_t i = 0; i < k; i++)
+        for (size_t j = 0; j <= i; j++) L[i * k + j] = a[(size_t)keep[i] * ms + keep[j]] * sc[i] * sc[j];
+    for (size_t j = 0; j < k; j++) {
+        double d = L[j * k + j];
+        for (size_t p = 0; p < j; p++) d -= L[j * k + p] * L[j * k + p] * L[p * k + p];
+        if (d <= 1e-10) d = 1e30;
+        L[j * k + j] = d;
+        for (size_t i = j + 1; i < k; i++) {
+            double v = L[i * k + j];
+            for (size_t p = 0; p < j; p++) v -= L[i * k + p] * L[j * k + p] * L[p * k + p];
+            L[i * k + j] = v / d;
+        }
+    }
+    std::vector<double> t(k);
+    for (size_t col = 0; col < k; col++) {
+        std::fill(t.begin(), t.end(), 0.0);
+        t[col] = sc[col];
+        for (size_t i = 0; i < k; i++)
+            for (size_t p = 0; p < i; p++) t[i] -= L[i * k + p] * t[p];
+        for (size_t i = 0; i < k; i++) t[i] /= L[i * k + i];
+        for (size_t ii = k; ii-- > 0;)
+            for (size_t p = ii + 1; p < k; p++) t[ii] -= L[p * k + ii] * t[p];
+        for (size_t i = 0; i < k; i++) inv[(size_t)keep[i] * ms + keep[col]] = t[i] * sc[i];
+    }
+    return inv;
+}
+
+}  // namespace
+
+void refresh_weights(Absorber& a, const double* w_rows_dev) {
+    hipStream_t st = a.st;
+    const size_t M = a.M, g1 = a.g1, m = a.m, ms = a.ms;
+    a.weighted = w_rows_dev != nullptr;
+    if (a.weighted && w_rows_dev != a.w_rows.p) {
+        a.w_rows.reserve(a.n);
+        FH_HIP(hipMemcpyAsync(a.w_rows.p, w_rows_dev, a.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    const bool unit = a.unit();
+    // cell weights
+    if (!unit) {
+        a.w64.cw.reserve(M);
+        a.w32.cw.reserve(M);
+        hipLaunchKernelGGL(k_cell_weights, dim3(blocks_for(M)), dim3(TPB), 0, st, a.trivial() ? nullptr : a.cell_obs_off.p,
+                           a.weighted ? a.w_rows.p : nullptr, M, a.w64.cw.p, a.w32.cw.p);
+        FH_CHECK_LAUNCH();
+    } else {
+        a.w64.cw.release();
+        a.w32.cw.release();
+    }
+    const double* cw = unit ? nullptr : a.w64.cw.p;
+    a.n1.reserve(g1);
+    a.w64.inv_n1.reserve(g1);
+    a.w32.inv_n1.reserve(g1);
+    hipLaunchKernelGGL(k_n1, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p, cw, g1, a.n1.p, a.w64.inv_n1.p,
+                       a.w32.inv_n1.p);
+    FH_CHECK_LAUNCH();
+    a.sw = sum_dev(a.n1.p, g1, st);
+    a.nr.reserve(std::max<size_t>(m, 1));
+    a.diag.reserve(std::max<size_t>(m, 1));
+    a.red.reserve(std::max<size_t>(m, 1));
+    a.w64.inv_diag.reserve(std::max<size_t>(m, 1));
+    a.w32.inv_diag.reserve(std::max<size_t>(m, 1));
+    const int nb = (int)a.nb;
+    a.w64.lw.resize(nb);
+    a.w32.lw.resize(nb);
+    for (int b = 0; b < nb; b++) {
+        const BlockInfo& bi = a.blocks[b];
+        if (bi.small) continue;
+        if (!unit) {
+            a.w64.lw[b].reserve(M);
+            a.w32.lw[b].reserve(M);
+        } else {
+            a.w64.lw[b].release();
+            a.w32.lw[b].release();
+        }
+        hipLaunchKernelGGL(k_level_weights, dim3(blocks_for(bi.g)), dim3(TPB), 0, st, a.loff[b].p, a.lperm[b].p,
+                           a.lseg[b].p, cw, a.n1.p, (size_t)bi.g, (size_t)bi.off, a.nr.p, a.diag.p, a.red.p,
+                           a.w64.inv_diag.p, a.w32.inv_diag.p, unit ? nullptr : a.w64.lw[b].p,
+                           unit ? nullptr : a.w32.lw[b].p);
+        FH_CHECK_LAUNCH();
+    }
+    // small super-block
+    a.nkeep = 0;
+    if (ms > 0) {
+        SmallOffs so{};
+        so.nb = nb;
+        so.nsb = 0;
+        for (int b = 0; b < nb; b++)
+            if (a.blocks[b].small) so.off[so.nsb++] = (uint32_t)a.blocks[b].off;
+        const size_t len = ms * ms + ms;
+        DBuf<double> sfull(len);
+        if (ms <= (size_t)SA_LDS_MAX) {
+            const unsigned grid = std::min<unsigned>(blocks_for(g1), 512);
+            DBuf<double> parts((size_t)grid * len);
+            hipLaunchKernelGGL(k_small_assemble, dim3(grid), dim3(TPB), len * sizeof(double), st, a.seg_off.p,
+                               a.cell_codes.p, cw, a.n1.p, g1, ms, so, parts.p);
+            FH_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(len)), dim3(TPB), 0, st, parts.p, (size_t)grid, len,
+                               sfull.p);
+            FH_CHECK_LAUNCH();
+            FH_HIP(hipStreamSynchronize(st));
+        } else {
+            FH_HIP(hipMemsetAsync(sfull.p, 0, sfull.bytes(), st));
+            hipLaunchKernelGGL(k_small_assemble, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p, a.cell_codes.p,
+                               cw, a.n1.p, g1, ms, so, sfull.p);
+            FH_CHECK_LAUNCH();
+        }
+        std::vector<double> h(len);
+        FH_HIP(hipMemcpyAsync(h.data(), sfull.p, len * sizeof(double), hipMemcpyDeviceToHost, st));
+        FH_HIP(hipStreamSynchronize(st));
+        std::vector<double> nrs(ms), dg(ms);
+        std::vector<uint8_t> rd(ms);
+        std::vector<double> S(h.begin(), h.begin() + ms * ms);
+        for (size_t k = 0; k < ms; k++) {
+            nrs[k] = h[ms * ms + k];
+            dg[k] = S[k * ms + k];
+            rd[k] = dg[k] <= 1e-12 * nrs[k];
+        }
+        std::vector<double> inv = grounded_inverse(S, ms, rd, a.nkeep);
+        std::vector<float> inv32(inv.begin(), inv.end());
+        std::vector<double> zeros(ms, 0.0);
+        std::vector<float> zeros32(ms, 0.0f);
+        FH_HIP(hipMemcpyAsync(a.nr.p, nrs.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
+        FH_HIP(hipMemcpyAsync(a.diag.p, dg.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
+        FH_HIP(hipMemcpyAsync(a.red.p, rd.data(), ms, hipMemcpyHostToDevice, st));
+        FH_HIP(hipMemcpyAsync(a.w64.inv_diag.p, zeros.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
+        FH_HIP(hipMemcpyAsync(a.w32.inv_diag.p, zeros32.data(), ms * sizeof(float), hipMemcpyHostToDevice, st));
+        a.w64.minv.reserve(ms * ms);
+        a.w32.minv.reserve(ms * ms);
+        FH_HIP(hipMemcpyAsync(a.w64.minv.p, inv.data(), ms * ms * sizeof(double), hipMemcpyHostToDevice, st));
+        FH_HIP(hipMemcpyAsync(a.w32.minv.p, inv32.data(), ms * ms * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+    FH_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace fh

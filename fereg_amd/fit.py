@@ -1,0 +1,103 @@
+"""Estimation entry points over the C ABI (fit/mod.rs:591; fit/ols.rs:213; fit/glm.rs:362).
+
+`feols` and `fepois` take kept-sample arrays (sample accounting -- missing values, singleton
+removal, factor encoding -- happens upstream, as in the reference's fit/input.rs), FE codes per
+term, and a variance request in fixest's conventions (fit/vcov.rs:278-382).
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib as L
+from .absorber import Absorber, SolveOpts, _is_torch, _ptr
+
+
+@dataclass
+class FitResult:
+    coef: np.ndarray
+    vcov: np.ndarray
+    kept: np.ndarray
+    df_k: float
+    nparams: float
+    ssr: float
+    deviance: float
+    iterations: int
+    pcg_iterations: int
+    n_clusters: int
+    se: np.ndarray = field(init=False)
+
+    def __post_init__(self):
+        self.se = np.sqrt(np.clip(np.diag(self.vcov), 0.0, None))
+
+
+def nested_fixest(fe_codes, cluster_codes, fe_names=None, cluster_names=None):
+    """fixest's K.fixef = "nonnested" test (fit/vcov.rs:115-136): by name first (a term that is a
+    cluster variable, or an interaction containing one), then by value unless every cluster
+    variable is itself an FE term."""
+    nt = len(fe_codes)
+    fe_names = fe_names or [None] * nt
+    cl_names = cluster_names or []
+    nested = [nm is not None and (nm in cl_names or ("^" in nm and any(f in cl_names for f in nm.split("^"))))
+              for nm in fe_names]
+    if not all(nested) and not (cl_names and all(c in fe_names for c in cl_names)):
+        for q in range(nt):
+            if not nested[q] and cluster_codes is not None:
+                c = np.asarray(fe_codes[q])
+                k = np.asarray(cluster_codes)
+                g = int(c.max()) + 1
+                first = np.full(g, -1, np.int64)
+                order = np.arange(len(c))
+                first[c[::-1]] = k[::-1]  # some cluster of each level
+                nested[q] = bool(np.all(first[c] == k))
+    return nested
+
+
+def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, extra):
+    lib = L.lib()
+    X = np.asarray(X, float) if not _is_torch(X) else X
+    if not _is_torch(X) and X.ndim == 1:
+        X = X[:, None]
+    K = X.shape[1]
+    ab = absorber or Absorber(codes, g, weights=weights)
+    kind = L.FH_VCOV[vcov]
+    ssc = ssc or {}
+    sscc = L.Ssc(int(ssc.get("K.adj", True)), L.FH_KFIXEF[ssc.get("K.fixef", "nonnested")], int(ssc.get("G.adj", True)), 0)
+    nt = ab.info.nterms
+    nest = np.zeros(max(nt, 1), np.int32)
+    if nested is not None:
+        nest[:nt] = np.asarray(nested, np.int32)
+    beta = np.zeros(K)
+    V = np.zeros(K * K)
+    kept = np.zeros(K, np.int32)
+    st = L.FitStats()
+    o = (opts or SolveOpts(tol=1e-10)).c()
+    if _is_torch(y):
+        cl = None if cluster is None else cluster.contiguous().data_ptr()
+        fn = getattr(lib, kind_fn + "_dev")
+        args = [ab._h, y.contiguous().data_ptr(), X.contiguous().data_ptr(), K, kind, cl, _ptr(nest, L.i32p),
+                ctypes.byref(sscc)]
+    else:
+        yv = np.ascontiguousarray(y, dtype=np.float64)
+        Xv = np.ascontiguousarray(X, dtype=np.float64)
+        cl = None if cluster is None else np.ascontiguousarray(cluster, dtype=np.uint32)
+        fn = getattr(lib, kind_fn)
+        args = [ab._h, _ptr(yv, L.f64p), _ptr(Xv, L.f64p), K, kind, None if cl is None else _ptr(cl, L.u32p),
+                _ptr(nest, L.i32p), ctypes.byref(sscc)]
+    args += extra + [ctypes.byref(o), _ptr(beta, L.f64p), _ptr(V, L.f64p), _ptr(kept, L.i32p), ctypes.byref(st)]
+    L.check(fn(*args))
+    return FitResult(beta, V.reshape(K, K), kept.astype(bool), st.df_k, st.nparams, st.ssr, st.deviance,
+                     st.iterations, st.pcg_iterations, st.n_clusters)
+
+
+def feols(y, X, codes, g, weights=None, vcov="iid", cluster=None, nested=None, ssc=None, opts=None, absorber=None):
+    """OLS with absorbed FEs (fit/ols.rs:213-330) on the GPU."""
+    return _fit("fh_ols_fit", y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, [])
+
+
+def fepois(y, X, codes, g, vcov="hetero", cluster=None, nested=None, ssc=None, tol=1e-8, maxiter=100, opts=None,
+           absorber=None):
+    """Poisson PML by IRLS (glm/irls.rs:367-913) on the GPU."""
+    return _fit("fh_ppml_fit", y, X, codes, g, None, vcov, cluster, nested, ssc, opts, absorber,
+                [ctypes.c_double(tol), ctypes.c_uint64(maxiter)])

@@ -1,0 +1,205 @@
+/* fereg_hip: MI355X (gfx950) fixed-effect demeaning and estimation, C ABI.
+ *
+ * Drop-in for fereg-core's `Absorber` boundary (crates/fereg-core/src/absorber.rs:62-97) and the
+ * estimation steps its callers run on the residualized block (ols.rs:234-449, fit/ols.rs:213-330,
+ * fit/vcov.rs:278-382, glm/irls.rs:367-913). Plain pointers and sizes only. Every function that
+ * can fail returns one of the reference's error codes (error.rs:6-37):
+ *   FH_OK 0, FH_INTERRUPTED 1, FH_INVALID 198, FH_CONVERGENCE 430, FH_OOM 909,
+ *   FH_TOO_FEW 2001, FH_INTERNAL 9999 (HIP errors are reported as FH_INTERNAL),
+ * and fh_last_error() returns the message of the last failure on the calling thread.
+ *
+ * Threading: an absorber is Send-not-Sync (absorber.rs:62): one caller at a time, movable
+ * between threads. Each absorber owns one HIP stream of default priority on the device that was
+ * current when it was created; every call is synchronous with respect to the host.
+ *
+ * Memory: functions ending in _dev take device pointers (HBM-resident inputs/outputs on the
+ * absorber's device); the others take host pointers.
+ */
+#ifndef FEREG_HIP_H
+#define FEREG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FH_OK 0
+#define FH_INTERRUPTED 1
+#define FH_INVALID 198
+#define FH_CONVERGENCE 430
+#define FH_OOM 909
+#define FH_TOO_FEW 2001
+#define FH_INTERNAL 9999
+
+/* Precision of the reduced-system solve. */
+#define FH_PLAIN_FP64 0 /* the reference's method in FP64 */
+#define FH_MIXED 1      /* FP32 PCG with FP64 iterative refinement */
+
+typedef struct fh_absorber fh_absorber;
+
+/* One absorbed term (session::Term, intercept only): dense codes 0..g-1 in kept order.
+ * Replaces `Term { codes, g, intercept, .. }` (crates/fereg-core/src/session.rs). */
+typedef struct {
+    const uint32_t* codes;
+    uint64_t g;
+    int32_t intercept; /* must be 1: slope terms are out of scope */
+} fh_term;
+
+/* SolverConfig (solve.rs:53-80): the knobs this solver honours. */
+typedef struct {
+    uint64_t small_limit; /* levels factored jointly and exactly (default 4096) */
+    int32_t precision;    /* FH_PLAIN_FP64 or FH_MIXED */
+    int32_t reserved;
+} fh_solver_config;
+
+/* SolveOpts (absorber.rs:18-24) plus the precision of this solve. */
+typedef struct {
+    double tol;
+    uint64_t maxiter;
+    int32_t want_coefs;
+    int32_t precision; /* -1 = the absorber's configured precision */
+} fh_solve_opts;
+
+/* SolveInfo (absorber.rs:27-37); refinements = FP64 outer sweeps of the mixed form. */
+typedef struct {
+    uint64_t iterations;
+    int32_t escalated;
+    int32_t exact;
+    uint64_t refinements;
+    double seconds;
+} fh_solve_info;
+
+/* Ctx (api.rs:198-229): cooperative interrupt polled between kernels, log sink. */
+typedef struct {
+    int (*interrupt)(void* user);
+    void (*log)(void* user, const char* line);
+    void* user;
+    int32_t verbose;
+} fh_ctx;
+
+/* Structure summary (structure.rs:55-92). */
+typedef struct {
+    uint64_t n, nterms, g1, ncells, nb, m, m_small, small_work;
+    int64_t fe1, norm_fe1;
+    int32_t has_cell_obs_off;
+    int32_t reserved;
+} fh_structure_info;
+
+const char* fh_last_error(void);
+void fh_config_default(fh_solver_config* cfg);
+
+/* solve::build / FeSolver::new (solve.rs:338-349, 396-465). w_kept: kept-order weights or NULL. */
+int fh_absorber_create(const fh_term* terms, size_t nterms, const double* w_kept, size_t n,
+                       const fh_solver_config* cfg, const fh_ctx* ctx, fh_absorber** out);
+int fh_absorber_create_dev(const fh_term* terms, size_t nterms, const double* w_kept, size_t n,
+                           const fh_solver_config* cfg, const fh_ctx* ctx, fh_absorber** out);
+void fh_absorber_destroy(fh_absorber* ab);
+
+/* Absorber::n, row_order (absorber.rs:64-69). row_order is borrowed, n entries, host memory. */
+size_t fh_absorber_n(const fh_absorber* ab);
+const uint32_t* fh_absorber_row_order(fh_absorber* ab);
+const uint32_t* fh_absorber_row_order_dev(const fh_absorber* ab);
+
+/* Absorber::set_weights (absorber.rs:74-78; solve.rs:1010-1139): row order, NULL = unit. */
+int fh_absorber_set_weights(fh_absorber* ab, const double* w_rows, const fh_ctx* ctx);
+int fh_absorber_set_weights_dev(fh_absorber* ab, const double* w_rows, const fh_ctx* ctx);
+
+/* Absorber::residualize (absorber.rs:80-90; solve.rs:1141-1311): z is n x r row-major in row
+ * order, residualized in place. warm / coefs_out: (m + g1) x r row-major or NULL. */
+int fh_absorber_residualize(fh_absorber* ab, double* z, size_t r, const fh_solve_opts* o, const double* warm,
+                            double* coefs_out, fh_solve_info* info, const fh_ctx* ctx);
+int fh_absorber_residualize_dev(fh_absorber* ab, double* z, size_t r, const fh_solve_opts* o,
+                                const double* warm, double* coefs_out, fh_solve_info* info,
+                                const fh_ctx* ctx);
+
+/* FeCoefs rows p = m + g1 (solve.rs:7-9). */
+size_t fh_absorber_coef_rows(const fh_absorber* ab);
+/* Absorber::fit_rows (absorber.rs:92; solve.rs:1313-1317): FE fit of column col per row. */
+int fh_absorber_fit_rows(fh_absorber* ab, const double* coefs, size_t r, size_t col, double* out_rows);
+/* Absorber::term_values (absorber.rs:94-96; solve.rs:1319-1343): out holds sum_q g_q values in
+ * term order; lens (nterms) receives g_q. */
+int fh_absorber_term_values(fh_absorber* ab, const double* coefs, size_t r, size_t col, double* out,
+                            uint64_t* lens);
+
+/* Structure and weight-state introspection (the bit-exact and parity checks). */
+int fh_absorber_info(const fh_absorber* ab, fh_structure_info* out);
+int fh_absorber_copy_structure(fh_absorber* ab, uint32_t* pi, uint32_t* seg_off, uint32_t* cell_codes,
+                               uint32_t* cell_obs_off);
+int fh_absorber_copy_blocks(const fh_absorber* ab, int64_t* term, uint64_t* g, uint64_t* off, int32_t* small);
+int fh_absorber_copy_level_cells(fh_absorber* ab, size_t b, uint32_t* perm, uint32_t* off);
+int fh_absorber_copy_weight_state(fh_absorber* ab, double* n1, double* nr, double* diag, uint8_t* redundant,
+                                  double* sw);
+/* Reduced operator y = S p (operator.rs:1375), host buffers m x r: the SpMV parity check. */
+int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int32_t precision);
+
+/* SpMV timing probe (no counterpart in the reference; measurement only): runs `reps` products
+ * y = S p on the absorber's stream and returns the mean milliseconds per product (HIP events on
+ * that stream) and the compulsory HBM bytes of one product,
+ * 4 nb M + 12 g1 + 2 s m r + s M [weighted] (s = 8 FP64, 4 FP32). */
+int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t reps, double* ms_per_spmv,
+                          double* bytes_per_spmv);
+
+/* ------------------------------------------------------------------ estimation */
+
+#define FH_VCOV_IID 0
+#define FH_VCOV_HETERO 1
+#define FH_VCOV_CLUSTER 2
+
+/* fixest small-sample rules (fit/vcov.rs:63-112, ssc of fit/mod.rs). */
+#define FH_KFIXEF_NONE 0
+#define FH_KFIXEF_FULL 1
+#define FH_KFIXEF_NONNESTED 2
+typedef struct {
+    int32_t k_adj;
+    int32_t k_fixef;
+    int32_t g_adj;
+    int32_t reserved;
+} fh_ssc;
+
+/* Result of an estimation on the absorber's sample. beta (K), vcov (K x K), kept (K: 1 = kept,
+ * 0 = dropped as collinear; dropped coefficients are 0 with zero variance). */
+typedef struct {
+    double ssr;
+    double df_k;
+    double nparams;
+    double deviance;    /* Poisson */
+    uint64_t iterations; /* solver iterations (OLS) or IRLS iterations (Poisson) */
+    uint64_t pcg_iterations;
+    uint64_t n_clusters;
+    int32_t converged;
+    int32_t reserved;
+} fh_fit_stats;
+
+/* OLS of y on X (K columns) with the absorber's FEs (fit/ols.rs:213-330; fit/vcov.rs:316-382).
+ * y (n), X (n x K row-major) in KEPT order; cluster: kept-order codes (n) or NULL; nested: per
+ * absorbed term, whether it is nested in the cluster (fixest K.fixef = "nonnested"). */
+int fh_ols_fit(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t vcov_kind,
+               const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o,
+               double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
+int fh_ols_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t vcov_kind,
+                   const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o,
+                   double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
+
+/* Poisson PML by IRLS with warm-started weighted residualization (glm/irls.rs:367-913, fixest
+ * flavour; family.rs:103-145), robust (hetero) or iid or cluster variance. y, X kept order. */
+int fh_ppml_fit(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t vcov_kind,
+                const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
+                const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
+int fh_ppml_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t vcov_kind,
+                    const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol,
+                    uint64_t maxiter, const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept,
+                    fh_fit_stats* stats);
+
+/* Device memory helpers for callers without another allocator. */
+int fh_device_alloc(void** p, size_t bytes);
+int fh_device_free(void* p);
+int fh_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int fh_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int fh_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

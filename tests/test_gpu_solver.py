@@ -1,0 +1,148 @@
+"""GPU reduced operator, PCG and residualization against the oracle and the dense projection."""
+import numpy as np
+import pytest
+
+from fixtures import dense_residualize
+from gpu_util import announce
+from oracle.oracle import OracleSolver
+from test_oracle import residualize_designs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_apply_matches_oracle(weighted):
+    import fereg_amd as F
+    rng = np.random.default_rng(3)
+    n = 30_000
+    gs = [3000, 9, 400]
+    codes = [rng.integers(0, g, n).astype(np.uint32) for g in gs]
+    codes.append(codes[0] // 3)
+    gs.append(1000)
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    o = OracleSolver(codes, gs, weights=w)
+    a = F.Absorber(codes, gs, weights=w)
+    for r in (1, 2, 3, 5, 11, 16):
+        p = rng.standard_normal(o.info.m * r)
+        announce("apply", weighted, r)
+        yo = o.apply(p, r)
+        ya = a.apply(p, r)
+        scale = np.abs(yo).max()
+        np.testing.assert_allclose(ya, yo, rtol=0, atol=1e-12 * scale)
+        y32 = a.apply(p, r, precision=F.FH_MIXED)
+        np.testing.assert_allclose(y32, yo, rtol=0, atol=3e-5 * scale)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("di", range(8))
+def test_residualize_matches_dense_projection(di, weighted, precision):
+    import fereg_amd as F
+    codes, gs = residualize_designs()[di]
+    n = len(codes[0]) if codes else 2000
+    rng = np.random.default_rng(100 + di)
+    w = rng.uniform(0.2, 3.0, n) if weighted else None
+    if not codes and w is None:
+        w = np.ones(n)
+    a = F.Absorber(codes, gs, weights=w, precision=precision)
+    r = 3
+    v = rng.standard_normal((n, r))
+    v[:, 2] = 1.0
+    pi = a.row_order()
+    rows = np.ascontiguousarray(v[pi])
+    announce("residualize design", di, "weighted", weighted, "precision", precision)
+    info = a.residualize(rows, r, F.SolveOpts(tol=1e-12))
+    ref = dense_residualize(v, codes, gs, w)[pi]
+    scale = np.sqrt((v ** 2).sum(0))
+    assert np.max(np.abs(rows - ref) / scale) <= 1e-9, info
+
+
+def test_matches_oracle_iterations_and_coefs():
+    import fereg_amd as F
+    codes, gs = residualize_designs()[1]
+    n = len(codes[0])
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal((n, 2))
+    o = OracleSolver(codes, gs)
+    a = F.Absorber(codes, gs)
+    pi = a.row_order()
+    ro = np.ascontiguousarray(v[pi])
+    ra = ro.copy()
+    it_o, _, co = o.residualize(ro, 2, tol=1e-10, want_coefs=True)
+    announce("oracle iterations")
+    info = a.residualize(ra, 2, F.SolveOpts(tol=1e-10, want_coefs=True))
+    np.testing.assert_allclose(ra, ro, atol=1e-9)
+    # same algorithm and rules: the iteration counts agree up to rounding of the stopping test
+    assert abs(info.iterations - it_o) <= 1
+    # the fit per row (unique, unlike the FE coefficients) agrees
+    fa = a.fit_rows(info.coefs, 0)
+    resid_from_fit = v[pi, 0] - fa
+    np.testing.assert_allclose(resid_from_fit, ra[:, 0], atol=1e-9)
+
+
+def test_warm_start_and_weights_refresh():
+    import fereg_amd as F
+    codes, gs = residualize_designs()[2]
+    n = len(codes[0])
+    rng = np.random.default_rng(12)
+    a = F.Absorber(codes, gs)
+    pi = a.row_order()
+    v = rng.standard_normal((n, 2))
+    prev = None
+    for rnd in range(3):
+        w = rng.uniform(0.5, 2.0, n)
+        a.set_weights(w[pi])
+        rows = np.ascontiguousarray(v[pi])
+        announce("warm round", rnd)
+        info = a.residualize(rows, 2, F.SolveOpts(tol=1e-12, want_coefs=True), warm=prev)
+        prev = info.coefs
+        np.testing.assert_allclose(rows, dense_residualize(v, codes, gs, w)[pi], atol=1e-9)
+
+
+def test_many_columns_are_chunked():
+    import fereg_amd as F
+    codes, gs = residualize_designs()[1]
+    n = len(codes[0])
+    rng = np.random.default_rng(2)
+    a = F.Absorber(codes, gs)
+    r = 21
+    v = rng.standard_normal((n, r))
+    pi = a.row_order()
+    rows = np.ascontiguousarray(v[pi])
+    announce("21 columns")
+    a.residualize(rows, r, F.SolveOpts(tol=1e-12))
+    np.testing.assert_allclose(rows, dense_residualize(v, codes, gs)[pi], atol=1e-9)
+
+
+def test_maxiter_is_enforced():
+    """tests/solver.rs:489-500: maxiter exceeded -> Convergence (430)."""
+    import fereg_amd as F
+    n = 4000
+    i = np.arange(n)
+    # ring: slow Jacobi convergence
+    a_code = (i // 2).astype(np.uint32)
+    b_code = (((i // 2) + (i % 2)) % 2000).astype(np.uint32)
+    c_code = (i % 7).astype(np.uint32)
+    a = F.Absorber([a_code, b_code, c_code], [2000, 2000, 7])
+    rows = np.ascontiguousarray(np.random.default_rng(0).standard_normal((n, 1))[a.row_order()])
+    announce("maxiter")
+    with pytest.raises(F.FeregError) as e:
+        a.residualize(rows, 1, F.SolveOpts(tol=1e-14, maxiter=5))
+    assert e.value.rc == 430
+
+
+def test_device_resident_block():
+    import torch
+    import fereg_amd as F
+    codes, gs = residualize_designs()[1]
+    n = len(codes[0])
+    rng = np.random.default_rng(21)
+    v = rng.standard_normal((n, 2))
+    dcodes = [torch.as_tensor(c.astype(np.int32)).cuda() for c in codes]
+    a = F.Absorber(dcodes, gs)
+    pi = a.row_order()
+    z = torch.as_tensor(np.ascontiguousarray(v[pi])).cuda()
+    announce("device block")
+    a.residualize(z, 2, F.SolveOpts(tol=1e-12))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(z.cpu().numpy(), dense_residualize(v, codes, gs)[pi], atol=1e-9)
