@@ -129,6 +129,13 @@ def lib():
         if not os.path.exists(library_path):
             raise ImportError(f"{library_path} is missing: build it with `python -m fereg_amd.build` "
                               "(or __graft_entry__.build()); fereg_amd has no CPU fallback")
+        # PyTorch ships its own HIP runtime under the same soname (libamdhip64.so.7): the first one
+        # loaded serves the whole process. Load torch's first so that device tensors and this
+        # library share one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(library_path)
         for name, res, args in SIGNATURES:
             f = getattr(L, name)

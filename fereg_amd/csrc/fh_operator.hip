@@ -145,14 +145,26 @@ __global__ __launch_bounds__(WG) void k_seg_spmv(OpArgs<T> A, const T* __restric
     }
 }
 
+// y[i] = sum over the workgroup partials part[q*len + i]: one workgroup per entry, strided
+// per-thread sums and a fixed tree (deterministic, no serial chain of loads).
 template <class T>
-__global__ void k_reduce_small(const T* __restrict__ part, size_t nparts, size_t len, T* __restrict__ y,
-                               const uint32_t* __restrict__ done) {
+__global__ __launch_bounds__(WG) void k_reduce_small(const T* __restrict__ part, size_t nparts, size_t len,
+                                                     T* __restrict__ y, const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+    __shared__ T red[WAVES];
+    for (size_t i = blockIdx.x; i < len; i += gridDim.x) {
         T s = T(0);
-        for (size_t q = 0; q < nparts; q++) s += part[q * len + i];
-        y[i] = s;
+        for (size_t q = threadIdx.x; q < nparts; q += WG) s += part[q * len + i];
+        s = wave_sum(s);
+        if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            T t = T(0);
+#pragma unroll
+            for (int w = 0; w < WAVES; w++) t += red[w];
+            y[i] = t;
+        }
+        __syncthreads();
     }
 }
 
@@ -476,8 +488,8 @@ void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint3
     FH_CHECK_LAUNCH();
     if (accl) {
         if (lds) {
-            hipLaunchKernelGGL(k_reduce_small<T>, dim3(grid_for(accl, 256, 1024)), dim3(256), 0, st, s.part.p,
-                               (size_t)grid, accl, y, done);
+            hipLaunchKernelGGL(k_reduce_small<T>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
+                               s.part.p, (size_t)grid, accl, y, done);
         } else {
             FH_HIP(hipMemcpyAsync(y, s.part.p, accl * sizeof(T), hipMemcpyDeviceToDevice, st));
         }
@@ -527,8 +539,8 @@ void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, d
     FH_CHECK_LAUNCH();
     if (accl) {
         if (lds)
-            hipLaunchKernelGGL(k_reduce_small<double>, dim3(grid_for(accl, 256, 1024)), dim3(256), 0, st, s.part.p,
-                               (size_t)grid, accl, rhs, nullptr);
+            hipLaunchKernelGGL(k_reduce_small<double>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
+                               s.part.p, (size_t)grid, accl, rhs, nullptr);
         else
             FH_HIP(hipMemcpyAsync(rhs, s.part.p, accl * sizeof(double), hipMemcpyDeviceToDevice, st));
         FH_CHECK_LAUNCH();

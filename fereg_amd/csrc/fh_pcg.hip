@@ -32,6 +32,7 @@ struct PcgState {
 };
 
 enum Slot { S_PQ = 0, S_XB = 1, S_XR = 2, S_RZL = 3, S_RZS = 4, NSLOT = 5 };
+constexpr int NSLOT_MAX = 4;
 
 // Per-workgroup partial dots: dpart[(slot*DOTG + wg)*RMAX + j] over rows [row0, row1).
 template <class T>
@@ -68,10 +69,27 @@ __global__ __launch_bounds__(TPB) void k_dot(const T* __restrict__ a, const T* _
     }
 }
 
-__device__ __forceinline__ double slot_sum(const double* dpart, int slot, int j) {
-    double s = 0.0;
-    for (int q = 0; q < DOTG; q++) s += dpart[((size_t)slot * DOTG + q) * RMAX + j];
-    return s;
+// sums[slot][j] = sum over the DOTG workgroup partials of `nslot` slots (first, ...), computed
+// by the whole 256-thread block: thread t holds partial t, a wave sum, then 4 wave sums (fixed
+// order). Every thread must call it.
+__device__ void slot_sums(const double* __restrict__ dpart, const int* slots, int nslot, int r,
+                          double (*sums)[RMAX]) {
+    __shared__ double red[TPB / WAVE][NSLOT_MAX][RMAX];
+    const int t = threadIdx.x, w = t / WAVE;
+    for (int a = 0; a < nslot; a++)
+        for (int j = 0; j < r; j++) {
+            double v = dpart[((size_t)slots[a] * DOTG + t) * RMAX + j];
+            v = wave_sum(v);
+            if (lane_id() == 0) red[w][a][j] = v;
+        }
+    __syncthreads();
+    if (t < nslot * r) {
+        const int a = t / r, j = t % r;
+        double v = 0.0;
+        for (int ww = 0; ww < TPB / WAVE; ww++) v += red[ww][a][j];
+        sums[a][j] = v;
+    }
+    __syncthreads();
 }
 
 __global__ void k_zero_slots(double* dpart, int slot, int nslots) {
@@ -126,10 +144,13 @@ __global__ void k_add_to64(double* __restrict__ x, const TI* __restrict__ d, siz
 }
 
 // Start of a (re)start: rho = r'z, inactive when rho <= 0 (cg.rs:199-205).
-__global__ void k_start(PcgState* S, const double* __restrict__ dpart) {
+__global__ __launch_bounds__(TPB) void k_start(PcgState* S, const double* __restrict__ dpart) {
+    __shared__ double sums[NSLOT_MAX][RMAX];
+    const int sl[2] = {S_RZL, S_RZS};
+    slot_sums(dpart, sl, 2, S->r, sums);
     const int j = threadIdx.x;
     if (j < S->r) {
-        double rho = slot_sum(dpart, S_RZL, j) + slot_sum(dpart, S_RZS, j);
+        double rho = sums[0][j] + sums[1][j];
         S->rho[j] = rho;
         S->rho0[j] = rho;
         if (!(rho > 0.0)) S->active[j] = 0;
@@ -151,13 +172,16 @@ __global__ void k_mask_copy(const T* __restrict__ z, T* __restrict__ p, size_t n
 }
 
 // gamma = rho / p'q (cg.rs:224-234).
-__global__ void k_gamma(PcgState* S, const double* __restrict__ dpart) {
+__global__ __launch_bounds__(TPB) void k_gamma(PcgState* S, const double* __restrict__ dpart) {
     if (S->done) return;
+    __shared__ double sums[NSLOT_MAX][RMAX];
+    const int sl[1] = {S_PQ};
+    slot_sums(dpart, sl, 1, S->r, sums);
     const int j = threadIdx.x;
     if (j < S->r) {
         double g = 0.0;
         if (S->active[j]) {
-            const double pq = slot_sum(dpart, S_PQ, j);
+            const double pq = sums[0][j];
             if (pq > 0.0 && isfinite(pq)) g = S->rho[j] / pq;
             else S->active[j] = 0;
         }
@@ -225,18 +249,21 @@ __global__ __launch_bounds__(TPB) void k_update(T* __restrict__ x, T* __restrict
 }
 
 // Stopping rule and beta (cg.rs:237-289).
-__global__ void k_stop(PcgState* S, const double* __restrict__ dpart) {
+__global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restrict__ dpart) {
     if (S->done) return;
+    __shared__ double sums[NSLOT_MAX][RMAX];
+    const int sl[4] = {S_RZL, S_RZS, S_XB, S_XR};
+    slot_sums(dpart, sl, 4, S->r, sums);
     const int j = threadIdx.x;
     __shared__ int any;
     if (j == 0) any = 0;
     __syncthreads();
     const uint64_t k = S->k + 1;
     if (j < S->r) {
-        const double rn = slot_sum(dpart, S_RZL, j) + slot_sum(dpart, S_RZS, j);
+        const double rn = sums[0][j] + sums[1][j];
         double beta = 0.0;
         if (S->active[j]) {
-            const double xb = slot_sum(dpart, S_XB, j), xr = slot_sum(dpart, S_XR, j);
+            const double xb = sums[2][j], xr = sums[3][j];
             S->xb[j] = xb;
             S->xr[j] = xr;
             S->ring[j][(k - 1) % WINDOW] = S->gamma[j] * S->rho[j];
@@ -360,7 +387,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
         out.exact = true;
         return out;
     }
-    hipLaunchKernelGGL(k_start, dim3(1), dim3(64), 0, st, S, s.dpart.p);
+    hipLaunchKernelGGL(k_start, dim3(1), dim3(TPB), 0, st, S, s.dpart.p);
     hipLaunchKernelGGL(k_mask_copy<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, s.z.p, s.p.p, nv, r, S);
     FH_CHECK_LAUNCH();
     WeightArrays<T>& W = warr<T>(a);
@@ -370,7 +397,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
             spmv<T>(a, s.p.p, s.q.p, r, s, done, st);
             hipLaunchKernelGGL(k_dot<T>, dim3(DOTG), dim3(TPB), 0, st, s.p.p, s.q.p, (size_t)0, m, r, s.dpart.p,
                                (int)S_PQ, done);
-            hipLaunchKernelGGL(k_gamma, dim3(1), dim3(64), 0, st, S, s.dpart.p);
+            hipLaunchKernelGGL(k_gamma, dim3(1), dim3(TPB), 0, st, S, s.dpart.p);
             hipLaunchKernelGGL(k_update<T>, dim3(DOTG), dim3(TPB), 0, st, x, s.r.p, s.p.p, s.q.p, rhs, W.inv_diag.p,
                                s.z.p, ms, m, r, S, s.dpart.p);
             if (ms > 0) {
@@ -379,7 +406,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
                 hipLaunchKernelGGL(k_dot<T>, dim3(DOTG), dim3(TPB), 0, st, s.r.p, s.z.p, (size_t)0, ms, r, s.dpart.p,
                                    (int)S_RZS, done);
             }
-            hipLaunchKernelGGL(k_stop, dim3(1), dim3(64), 0, st, S, s.dpart.p);
+            hipLaunchKernelGGL(k_stop, dim3(1), dim3(TPB), 0, st, S, s.dpart.p);
             hipLaunchKernelGGL(k_update_p<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, s.p.p, s.z.p, nv, r, S);
             FH_CHECK_LAUNCH();
         }

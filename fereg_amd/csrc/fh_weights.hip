@@ -143,11 +143,21 @@ __global__ __launch_bounds__(TPB) void k_small_assemble(const uint32_t* __restri
     }
 }
 
-__global__ void k_reduce_parts(const double* __restrict__ parts, size_t nparts, size_t len, double* __restrict__ out) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(TPB) void k_reduce_parts(const double* __restrict__ parts, size_t nparts, size_t len,
+                                                     double* __restrict__ out) {
+    __shared__ double red[TPB / WAVE];
+    for (size_t i = blockIdx.x; i < len; i += gridDim.x) {
         double s = 0.0;
-        for (size_t p = 0; p < nparts; p++) s += parts[p * len + i];
-        out[i] = s;
+        for (size_t q = threadIdx.x; q < nparts; q += TPB) s += parts[q * len + i];
+        s = wave_sum(s);
+        if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < (int)(TPB / WAVE); w++) t += red[w];
+            out[i] = t;
+        }
+        __syncthreads();
     }
 }
 
@@ -274,8 +284,8 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             hipLaunchKernelGGL(k_small_assemble, dim3(grid), dim3(TPB), len * sizeof(double), st, a.seg_off.p,
                                a.cell_codes.p, cw, a.n1.p, g1, ms, so, parts.p);
             FH_CHECK_LAUNCH();
-            hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(len)), dim3(TPB), 0, st, parts.p, (size_t)grid, len,
-                               sfull.p);
+            hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)std::min<size_t>(len, 8192)), dim3(TPB), 0, st, parts.p,
+                               (size_t)grid, len, sfull.p);
             FH_CHECK_LAUNCH();
             FH_HIP(hipStreamSynchronize(st));
         } else {

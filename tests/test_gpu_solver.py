@@ -117,13 +117,14 @@ def test_many_columns_are_chunked():
 def test_maxiter_is_enforced():
     """tests/solver.rs:489-500: maxiter exceeded -> Convergence (430)."""
     import fereg_amd as F
-    n = 4000
+    n = 10000
     i = np.arange(n)
-    # ring: slow Jacobi convergence
+    # ring of 5000 firm levels (beyond the dense small limit): slow Jacobi convergence
     a_code = (i // 2).astype(np.uint32)
-    b_code = (((i // 2) + (i % 2)) % 2000).astype(np.uint32)
+    b_code = (((i // 2) + (i % 2)) % 5000).astype(np.uint32)
     c_code = (i % 7).astype(np.uint32)
-    a = F.Absorber([a_code, b_code, c_code], [2000, 2000, 7])
+    a = F.Absorber([a_code, b_code, c_code], [5000, 5000, 7])
+    assert a.info.m_small == 7
     rows = np.ascontiguousarray(np.random.default_rng(0).standard_normal((n, 1))[a.row_order()])
     announce("maxiter")
     with pytest.raises(F.FeregError) as e:
