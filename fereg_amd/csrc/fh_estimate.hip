@@ -17,39 +17,60 @@ constexpr int GRAM_KMAX = 22;  // k(k+1)/2 <= 256 pairs, one per thread
 constexpr int GRID = 1024;
 
 // Per-workgroup partial Gram of columns [c0, c0+k) of the n x ld row-major block z, weights
-// w (optional) times v (optional): part[wg][a*k+b] for a <= b.
+// w (optional) times v (optional): part[wg][p] over the k(k+1)/2 pairs p = (a <= b). The pairs
+// are split into groups of GP; thread t takes pair group t % ngroups and every
+// (256/ngroups)-th row, keeps its row's k values and GP sums in registers; the block then adds
+// the threads of each group in a fixed order.
+constexpr int GP = 8;
 __global__ __launch_bounds__(TPB) void k_gram(const double* __restrict__ z, size_t n, int ld, int c0, int k,
                                               const double* __restrict__ w, const double* __restrict__ v,
                                               double* __restrict__ part) {
-    __shared__ double tile[GRAM_TILE][GRAM_KMAX + 1];
-    __shared__ double wt[GRAM_TILE];
+    __shared__ double red[TPB][GP];
     const int npair = k * (k + 1) / 2;
-    int pa = 0, pb = 0;
-    {
-        int t = threadIdx.x, a = 0;
-        while (a < k && t >= k - a) { t -= k - a; a++; }
-        pa = a;
-        pb = a + t;
+    const int ngroups = (npair + GP - 1) / GP;
+    const int grp = threadIdx.x % ngroups;
+    const int rl = threadIdx.x / ngroups, nrl = TPB / ngroups;
+    int pa[GP], pb[GP];
+#pragma unroll
+    for (int q = 0; q < GP; q++) {
+        int t = grp * GP + q, a = 0;
+        if (t >= npair) { pa[q] = 0; pb[q] = 0; continue; }
+        while (t >= k - a) { t -= k - a; a++; }
+        pa[q] = a;
+        pb[q] = a + t;
     }
-    double acc = 0.0;
-    for (size_t base = (size_t)blockIdx.x * GRAM_TILE; base < n; base += (size_t)gridDim.x * GRAM_TILE) {
-        const size_t rows = min((size_t)GRAM_TILE, n - base);
-        for (size_t e = threadIdx.x; e < rows * k; e += TPB) {
-            const size_t i = e / k;
-            const int c = (int)(e % k);
-            tile[i][c] = z[(base + i) * ld + c0 + c];
+    double acc[GP];
+#pragma unroll
+    for (int q = 0; q < GP; q++) acc[q] = 0.0;
+    if (rl < nrl) {
+        for (size_t i = (size_t)blockIdx.x * nrl + rl; i < n; i += (size_t)gridDim.x * nrl) {
+            double x[GRAM_KMAX];
+            const double* row = z + i * ld + c0;
+#pragma unroll
+            for (int c = 0; c < GRAM_KMAX; c++) x[c] = c < k ? row[c] : 0.0;
+            double wt = w ? w[i] : 1.0;
+            if (v) wt *= v[i];
+#pragma unroll
+            for (int q = 0; q < GP; q++) {
+                double xa = 0.0, xb = 0.0;
+#pragma unroll
+                for (int c = 0; c < GRAM_KMAX; c++) {
+                    if (c == pa[q]) xa = x[c];
+                    if (c == pb[q]) xb = x[c];
+                }
+                acc[q] += wt * xa * xb;
+            }
         }
-        for (size_t i = threadIdx.x; i < rows; i += TPB) {
-            double x = w ? w[base + i] : 1.0;
-            if (v) x *= v[base + i];
-            wt[i] = x;
-        }
-        __syncthreads();
-        if ((int)threadIdx.x < npair)
-            for (size_t i = 0; i < rows; i++) acc += wt[i] * tile[i][pa] * tile[i][pb];
-        __syncthreads();
     }
-    if ((int)threadIdx.x < npair) part[(size_t)blockIdx.x * npair + threadIdx.x] = acc;
+#pragma unroll
+    for (int q = 0; q < GP; q++) red[threadIdx.x][q] = acc[q];
+    __syncthreads();
+    for (int p = threadIdx.x; p < npair; p += TPB) {
+        const int g = p / GP, q = p % GP;
+        double s = 0.0;
+        for (int t = g; t < nrl * ngroups; t += ngroups) s += red[t][q];
+        part[(size_t)blockIdx.x * npair + p] = s;
+    }
 }
 
 // e_i = sum_j c_j z_ij; partial sums of w e^2 per workgroup.
@@ -130,7 +151,7 @@ void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, co
                double* G_host) {
     require(k >= 1 && k <= GRAM_KMAX, "at most 21 regressors are supported");
     const int npair = k * (k + 1) / 2;
-    const unsigned grid = std::max(1u, std::min<unsigned>(GRID, (unsigned)((n + GRAM_TILE - 1) / GRAM_TILE)));
+    const unsigned grid = std::max(1u, std::min<unsigned>(GRID, (unsigned)((n + 255) / 256)));
     DBuf<double> part((size_t)grid * npair);
     hipLaunchKernelGGL(k_gram, dim3(grid), dim3(TPB), 0, a.st, z, n, ld, c0, k, w, v, part.p);
     FH_CHECK_LAUNCH();

@@ -61,6 +61,48 @@ __device__ __forceinline__ void multisplit_add(uint32_t key, bool valid, const T
     }
 }
 
+// Per-lane register accumulators of the small-coordinate scatter: a[k][j] += z[j] for key k,
+// keys < NK. One predicated add per (key, column) and cell; the wave reduction happens once,
+// at the end of the kernel (flush), not per cell.
+template <class T, int RM, int NK>
+struct SmallRegs {
+    T a[NK > 0 ? NK : 1][RM];
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int k = 0; k < (NK > 0 ? NK : 1); k++)
+#pragma unroll
+            for (int j = 0; j < RM; j++) a[k][j] = T(0);
+    }
+    __device__ __forceinline__ void add(uint32_t key, bool valid, const T (&z)[RM]) {
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const bool hit = valid && key == (uint32_t)k;
+#pragma unroll
+            for (int j = 0; j < RM; j++) a[k][j] += hit ? z[j] : T(0);
+        }
+    }
+    // part[k*r + j] (k < ms, j < r) = sum over the workgroup (fixed order). All threads call it.
+    __device__ void flush(T* __restrict__ part, size_t ms, int r) {
+        __shared__ T red[WAVES][NK > 0 ? NK : 1][RM];
+        const int w = threadIdx.x / WAVE;
+#pragma unroll
+        for (int k = 0; k < NK; k++)
+#pragma unroll
+            for (int j = 0; j < RM; j++) {
+                const T v = wave_sum(a[k][j]);
+                if (lane_id() == 0) red[w][k][j] = v;
+            }
+        __syncthreads();
+        for (int e = threadIdx.x; e < (int)ms * r; e += WG) {
+            const int k = e / r, j = e % r;
+            T v = T(0);
+#pragma unroll
+            for (int ww = 0; ww < WAVES; ww++) v += red[ww][k][j];
+            part[e] = v;
+        }
+    }
+};
+
 template <class T, int RM>
 __device__ __forceinline__ void gather_u(const OpArgs<T>& A, const T* __restrict__ p, uint32_t c, int r, T (&u)[RM]) {
 #pragma unroll
@@ -96,15 +138,26 @@ __global__ __launch_bounds__(WG) void k_seg_spmv(OpArgs<T> A, const T* __restric
         T tv[RM];
 #pragma unroll
         for (int j = 0; j < RM; j++) tv[j] = T(0);
-        for (uint32_t it = 0; it < tmax; it++) {
-            const uint32_t c = c0 + it * L + li;
-            if (c < c1) {
-                T u[RM];
-                gather_u<T, RM>(A, p, c, r, u);
-                const T wc = A.cw ? A.cw[c] : T(1);
+        for (uint32_t it = 0; it < tmax; it += 2) {
+            const uint32_t ca = c0 + it * L + li, cb = ca + L;
+            const bool va = ca < c1, vb = cb < c1;
+            T ua[RM], ub[RM];
 #pragma unroll
-                for (int j = 0; j < RM; j++) tv[j] += wc * u[j];
+            for (int j = 0; j < RM; j++) { ua[j] = T(0); ub[j] = T(0); }
+            for (int b = 0; b < A.nb; b++) {
+                const size_t ka = va ? (size_t)(A.boff[b] + A.cell_codes[(size_t)ca * A.nb + b]) * r : 0;
+                const size_t kb = vb ? (size_t)(A.boff[b] + A.cell_codes[(size_t)cb * A.nb + b]) * r : 0;
+#pragma unroll
+                for (int j = 0; j < RM; j++)
+                    if (j < r) {
+                        if (va) ua[j] += p[ka + j];
+                        if (vb) ub[j] += p[kb + j];
+                    }
             }
+            const T wa = va ? (A.cw ? A.cw[ca] : T(1)) : T(0);
+            const T wb = vb ? (A.cw ? A.cw[cb] : T(1)) : T(0);
+#pragma unroll
+            for (int j = 0; j < RM; j++) tv[j] += wa * ua[j] + wb * ub[j];
         }
         const T inv = sv ? A.inv_n1[s] : T(0);
 #pragma unroll
@@ -173,15 +226,21 @@ struct OtherOffs {
 };
 
 // ---------------------------------------------------------------- S*p, level pass of one large block
-template <class T, int RM>
+// y_l = sum_{c in l} W_c (u_c - t_{s(c)}) over the level-major records; with NK > 0 the same
+// W_c (u_c - t_s) is also scattered into the nsb small blocks (keys < NK) through per-lane
+// registers, and each workgroup writes its partial small vector to part[blockIdx.x][ms*r].
+template <class T, int RM, int NK>
 __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lseg,
                                                  const uint32_t* __restrict__ lother, const T* __restrict__ lw,
-                                                 size_t g, uint32_t off_b, int nother, OtherOffs oo,
+                                                 size_t g, uint32_t off_b, int nother, OtherOffs oo, int nsb, size_t ms,
                                                  const T* __restrict__ p, const T* __restrict__ t, int r, int L,
-                                                 T* __restrict__ y, const uint32_t* __restrict__ done) {
+                                                 T* __restrict__ y, T* __restrict__ part,
+                                                 const uint32_t* __restrict__ done) {
     if (done && *done) return;
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
+    SmallRegs<T, RM, NK> sm;
+    if (NK > 0) sm.zero();
     const size_t nwt = (size_t)gridDim.x * WAVES;
     for (size_t base = ((size_t)blockIdx.x * WAVES + w) * gpw; base < g; base += nwt * gpw) {
         const size_t lv = base + gl;
@@ -194,23 +253,45 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
             pb[j] = (valid && j < r) ? p[(size_t)(off_b + lv) * r + j] : T(0);
             acc[j] = T(0);
         }
-        for (uint32_t it = 0; it < tmax; it++) {
-            const uint32_t pos = j0 + it * L + li;
-            if (pos < j1) {
-                const T wc = lw ? lw[pos] : T(1);
-                const size_t s = lseg[pos];
-                T u[RM];
+        for (uint32_t it = 0; it < tmax; it += 2) {
+            // two positions per lane: indices first, then the gathers (independent loads in flight)
+            uint32_t pos[2], sg[2];
+            bool pv[2];
+            T wc[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                pos[h] = j0 + (it + h) * L + li;
+                pv[h] = pos[h] < j1;
+                sg[h] = pv[h] ? lseg[pos[h]] : 0u;
+                wc[h] = pv[h] ? (lw ? lw[pos[h]] : T(1)) : T(0);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                T u[RM], z[RM];
 #pragma unroll
                 for (int j = 0; j < RM; j++) u[j] = pb[j];
+                uint32_t key0 = 0;
                 for (int o = 0; o < nother; o++) {
-                    const size_t k = (size_t)(oo.off[o] + lother[(size_t)pos * nother + o]) * r;
+                    const uint32_t code = pv[h] ? lother[(size_t)pos[h] * nother + o] : 0u;
+                    if (o == 0) key0 = oo.off[0] + code;
+                    const size_t k = (size_t)(oo.off[o] + code) * r;
 #pragma unroll
                     for (int j = 0; j < RM; j++)
-                        if (j < r) u[j] += p[k + j];
+                        if (j < r && pv[h]) u[j] += p[k + j];
                 }
 #pragma unroll
-                for (int j = 0; j < RM; j++)
-                    if (j < r) acc[j] += wc * (u[j] - t[s * r + j]);
+                for (int j = 0; j < RM; j++) {
+                    z[j] = (pv[h] && j < r) ? wc[h] * (u[j] - t[(size_t)sg[h] * r + j]) : T(0);
+                    acc[j] += z[j];
+                }
+                if (NK > 0) {
+                    // small blocks come first among the other blocks
+                    sm.add(key0, pv[h] && nsb > 0, z);
+                    for (int sb = 1; sb < nsb; sb++) {
+                        const uint32_t code = pv[h] ? lother[(size_t)pos[h] * nother + sb] : 0u;
+                        sm.add(oo.off[sb] + code, pv[h], z);
+                    }
+                }
             }
         }
 #pragma unroll
@@ -220,6 +301,7 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
             for (int j = 0; j < RM; j++)
                 if (j < r) y[(size_t)(off_b + lv) * r + j] = acc[j];
     }
+    if (NK > 0) sm.flush(part + (size_t)blockIdx.x * ms * r, ms, r);
 }
 
 // ---------------------------------------------------------------- rhs pass (FP64)
@@ -334,13 +416,17 @@ __global__ __launch_bounds__(WG) void k_rhs_seg(OpArgs<double> A, const double* 
     }
 }
 
-// rhs of a large block: pull of z_c along the level lists.
-template <int RM>
+// rhs of a large block: pull of z_c along the level lists; with NK > 0 z_c is also scattered
+// into the small blocks (keys from the level-major records) as in k_lvl_spmv.
+template <int RM, int NK>
 __global__ __launch_bounds__(WG) void k_lvl_pull(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lperm,
-                                                 size_t g, const double* __restrict__ zc, int r, int L,
-                                                 double* __restrict__ out) {
+                                                 const uint32_t* __restrict__ lother, int nother, OtherOffs oo, int nsb,
+                                                 size_t ms, size_t g, const double* __restrict__ zc, int r, int L,
+                                                 double* __restrict__ out, double* __restrict__ part) {
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
+    SmallRegs<double, RM, NK> sm;
+    if (NK > 0) sm.zero();
     const size_t nwt = (size_t)gridDim.x * WAVES;
     for (size_t base = ((size_t)blockIdx.x * WAVES + w) * gpw; base < g; base += nwt * gpw) {
         const size_t lv = base + gl;
@@ -352,12 +438,19 @@ __global__ __launch_bounds__(WG) void k_lvl_pull(const uint32_t* __restrict__ lo
         for (int j = 0; j < RM; j++) acc[j] = 0.0;
         for (uint32_t it = 0; it < tmax; it++) {
             const uint32_t pos = j0 + it * L + li;
-            if (pos < j1) {
-                const size_t c = lperm[pos];
+            const bool pv = pos < j1;
+            double z[RM];
+            const size_t c = pv ? lperm[pos] : 0;
 #pragma unroll
-                for (int j = 0; j < RM; j++)
-                    if (j < r) acc[j] += zc[c * r + j];
+            for (int j = 0; j < RM; j++) {
+                z[j] = (pv && j < r) ? zc[c * r + j] : 0.0;
+                acc[j] += z[j];
             }
+            if (NK > 0)
+                for (int sb = 0; sb < nsb; sb++) {
+                    const uint32_t code = pv ? lother[(size_t)pos * nother + sb] : 0u;
+                    sm.add(oo.off[sb] + code, pv, z);
+                }
         }
 #pragma unroll
         for (int j = 0; j < RM; j++) acc[j] = gsum(acc[j], L);
@@ -366,6 +459,7 @@ __global__ __launch_bounds__(WG) void k_lvl_pull(const uint32_t* __restrict__ lo
             for (int j = 0; j < RM; j++)
                 if (j < r) out[lv * r + j] = acc[j];
     }
+    if (NK > 0) sm.flush(part + (size_t)blockIdx.x * ms * r, ms, r);
 }
 
 // ---------------------------------------------------------------- back-substitution (FP64)
@@ -432,7 +526,7 @@ constexpr size_t LDS_ACC_MAX = 96 * 1024;
 
 inline unsigned seg_grid(const Absorber& a) {
     const size_t spw = WAVE / a.seg_lanes;
-    return grid_for(a.g1, (unsigned)(spw * WAVES), 2048);
+    return grid_for(a.g1, (unsigned)(spw * WAVES), 1u << 16);
 }
 
 }  // namespace
@@ -462,55 +556,115 @@ OpArgs<T> op_args(const Absorber& a) {
 template OpArgs<double> op_args<double>(const Absorber&);
 template OpArgs<float> op_args<float>(const Absorber&);
 
-#define FH_DISPATCH_R(r, KERNEL_CALL)                            \
-    do {                                                         \
-        if ((r) <= 1) { constexpr int RM = 1; KERNEL_CALL; }     \
-        else if ((r) <= 2) { constexpr int RM = 2; KERNEL_CALL; } \
-        else if ((r) <= 4) { constexpr int RM = 4; KERNEL_CALL; } \
-        else if ((r) <= 8) { constexpr int RM = 8; KERNEL_CALL; } \
-        else { constexpr int RM = 16; KERNEL_CALL; }             \
+#define FH_DISPATCH_R(r, ...)                                      \
+    do {                                                           \
+        if ((r) <= 1) { constexpr int RM = 1; __VA_ARGS__; }       \
+        else if ((r) <= 2) { constexpr int RM = 2; __VA_ARGS__; }   \
+        else if ((r) <= 4) { constexpr int RM = 4; __VA_ARGS__; }   \
+        else if ((r) <= 8) { constexpr int RM = 8; __VA_ARGS__; }   \
+        else if ((r) <= 12) { constexpr int RM = 12; __VA_ARGS__; } \
+        else { constexpr int RM = 16; __VA_ARGS__; }               \
     } while (0)
+
+// Register budget of the small-coordinate accumulators (bytes per lane).
+constexpr int SMALL_REG_BYTES = 576;
+template <class T, int RM, int NK>
+constexpr bool small_fits() { return NK * RM * (int)sizeof(T) <= SMALL_REG_BYTES; }
+
+// Calls f(std::integral_constant<int, NK>) with the smallest NK in {8, 16, 32} >= ms whose
+// accumulators fit the register budget; returns false when none does.
+template <class T, int RM, class F>
+bool with_small_nk(size_t ms, F&& f) {
+    if (ms <= 8 && small_fits<T, RM, 8>()) { f(std::integral_constant<int, 8>{}); return true; }
+    if constexpr (small_fits<T, RM, 16>()) if (ms <= 16) { f(std::integral_constant<int, 16>{}); return true; }
+    if constexpr (small_fits<T, RM, 32>()) if (ms <= 32) { f(std::integral_constant<int, 32>{}); return true; }
+    return false;
+}
+
+// Index of the first large block (-1 when every block is small).
+static int first_large(const Absorber& a) {
+    for (size_t b = 0; b < a.nb; b++)
+        if (!a.blocks[b].small) return (int)b;
+    return -1;
+}
+
+static OtherOffs other_offs(const Absorber& a, int b) {
+    OtherOffs oo{};
+    int k = 0;
+    for (int bb = 0; bb < (int)a.nb; bb++)
+        if (bb != b) oo.off[k++] = (uint32_t)a.blocks[bb].off;
+    return oo;
+}
+
+static unsigned lvl_grid(const Absorber& a, size_t g) {
+    const int Ll = (int)a.lvl_lanes;
+    return grid_for(g, (unsigned)((WAVE / Ll) * WAVES), 1u << 20);
+}
 
 template <class T>
 void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st) {
     require(r >= 1 && r <= 16, "spmv: 1..16 columns");
     OpArgs<T> A = op_args<T>(a);
-    const unsigned grid = seg_grid(a);
     const size_t accl = a.ms * (size_t)r;
-    const bool lds = lds_bytes<T>(a, r) <= LDS_ACC_MAX;
-    s.t.reserve(std::max<size_t>(a.g1 * r, 1));
-    s.part.reserve(std::max<size_t>(lds ? (size_t)grid * accl : accl, 1));
-    if (!lds && accl) FH_HIP(hipMemsetAsync(s.part.p, 0, accl * sizeof(T), st));
-    const int L = (int)a.seg_lanes;
-    const size_t shm = lds ? lds_bytes<T>(a, r) : 0;
-    FH_DISPATCH_R(r, hipLaunchKernelGGL((k_seg_spmv<T, RM>), dim3(grid), dim3(WG), shm, st, A, p, r, L, s.t.p, s.part.p,
-                                        (int)lds, done));
-    FH_CHECK_LAUNCH();
-    if (accl) {
-        if (lds) {
-            hipLaunchKernelGGL(k_reduce_small<T>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
-                               s.part.p, (size_t)grid, accl, y, done);
-        } else {
-            FH_HIP(hipMemcpyAsync(y, s.part.p, accl * sizeof(T), hipMemcpyDeviceToDevice, st));
-        }
-        FH_CHECK_LAUNCH();
-    }
+    const int fl = first_large(a);
+    const int nb = (int)a.nb;
     const WeightArrays<T>* W;
     if constexpr (std::is_same<T, double>::value) W = &a.w64; else W = &a.w32;
-    const int nb = (int)a.nb;
+    s.t.reserve(std::max<size_t>(a.g1 * r, 1));
+    const int L = (int)a.seg_lanes;
+    const unsigned grid = seg_grid(a);
+    // Small scatter inside the first large block's level pass (per-lane registers) when the
+    // accumulators fit; otherwise in the segment pass (wave multisplit).
+    bool small_in_lvl = false;
+    if (accl && fl >= 0) {
+        FH_DISPATCH_R(r, small_in_lvl = with_small_nk<T, RM>(a.ms, [](auto) {}));
+    }
+    if (accl && !small_in_lvl) {
+        const bool lds = lds_bytes<T>(a, r) <= LDS_ACC_MAX;
+        s.part.reserve(std::max<size_t>(lds ? (size_t)grid * accl : accl, 1));
+        if (!lds) FH_HIP(hipMemsetAsync(s.part.p, 0, accl * sizeof(T), st));
+        const size_t shm = lds ? lds_bytes<T>(a, r) : 0;
+        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_seg_spmv<T, RM>), dim3(grid), dim3(WG), shm, st, A, p, r, L, s.t.p,
+                                            s.part.p, (int)lds, done));
+        FH_CHECK_LAUNCH();
+        if (lds)
+            hipLaunchKernelGGL(k_reduce_small<T>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
+                               s.part.p, (size_t)grid, accl, y, done);
+        else
+            FH_HIP(hipMemcpyAsync(y, s.part.p, accl * sizeof(T), hipMemcpyDeviceToDevice, st));
+        FH_CHECK_LAUNCH();
+    } else {
+        OpArgs<T> At = A;
+        At.nsb = 0;  // t only
+        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_seg_spmv<T, RM>), dim3(grid), dim3(WG), 0, st, At, p, r, L, s.t.p,
+                                            (T*)nullptr, 0, done));
+        FH_CHECK_LAUNCH();
+    }
+    int nsb = 0;
+    for (auto& bi : a.blocks) nsb += bi.small ? 1 : 0;
     for (int b = 0; b < nb; b++) {
         const BlockInfo& bi = a.blocks[b];
         if (bi.small) continue;
-        OtherOffs oo{};
-        int k = 0;
-        for (int bb = 0; bb < nb; bb++)
-            if (bb != b) oo.off[k++] = (uint32_t)a.blocks[bb].off;
+        const OtherOffs oo = other_offs(a, b);
         const int Ll = (int)a.lvl_lanes;
-        const unsigned lg = grid_for(bi.g, (unsigned)((WAVE / Ll) * WAVES), 4096);
+        const unsigned lg = lvl_grid(a, bi.g);
         const T* lw = a.unit() ? nullptr : W->lw[b].p;
-        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_lvl_spmv<T, RM>), dim3(lg), dim3(WG), 0, st, a.loff[b].p, a.lseg[b].p,
-                                            a.lother[b].p, lw, (size_t)bi.g, (uint32_t)bi.off, nb - 1, oo, p, s.t.p, r,
-                                            Ll, y, done));
+        if (b == fl && small_in_lvl) {
+            s.part.reserve((size_t)lg * accl);
+            FH_DISPATCH_R(r, with_small_nk<T, RM>(a.ms, [&](auto nk) {
+                constexpr int NK = decltype(nk)::value;
+                hipLaunchKernelGGL((k_lvl_spmv<T, RM, NK>), dim3(lg), dim3(WG), 0, st, a.loff[b].p, a.lseg[b].p,
+                                   a.lother[b].p, lw, (size_t)bi.g, (uint32_t)bi.off, nb - 1, oo, nsb, (size_t)a.ms, p,
+                                   s.t.p, r, Ll, y, s.part.p, done);
+            }));
+            FH_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_reduce_small<T>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
+                               s.part.p, (size_t)lg, accl, y, done);
+        } else {
+            FH_DISPATCH_R(r, hipLaunchKernelGGL((k_lvl_spmv<T, RM, 0>), dim3(lg), dim3(WG), 0, st, a.loff[b].p,
+                                                a.lseg[b].p, a.lother[b].p, lw, (size_t)bi.g, (uint32_t)bi.off, nb - 1,
+                                                oo, 0, (size_t)0, p, s.t.p, r, Ll, y, (T*)nullptr, done));
+        }
         FH_CHECK_LAUNCH();
     }
 }
@@ -523,21 +677,28 @@ void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, d
     OpArgs<double> A = op_args<double>(a);
     const unsigned grid = seg_grid(a);
     const size_t accl = a.ms * (size_t)r;
-    const bool lds = lds_bytes<double>(a, r) <= LDS_ACC_MAX;
     Scratch<double>& s = a.s64;
-    s.part.reserve(std::max<size_t>(lds ? (size_t)grid * accl : accl, 1));
-    if (!lds && accl) FH_HIP(hipMemsetAsync(s.part.p, 0, accl * sizeof(double), st));
+    const int fl = first_large(a);
+    bool small_in_lvl = false;
+    if (accl && fl >= 0) {
+        FH_DISPATCH_R(r, small_in_lvl = with_small_nk<double, RM>(a.ms, [](auto) {}));
+    }
+    const bool lds = lds_bytes<double>(a, r) <= LDS_ACC_MAX;
+    if (accl && !small_in_lvl) {
+        s.part.reserve(std::max<size_t>(lds ? (size_t)grid * accl : accl, 1));
+        if (!lds) FH_HIP(hipMemsetAsync(s.part.p, 0, accl * sizeof(double), st));
+    }
     s.dpart.reserve((size_t)grid * 2 * r);
-    bool has_large = false;
-    for (auto& bi : a.blocks) has_large |= !bi.small;
-    if (has_large) s.zc.reserve(a.M * r);
+    if (fl >= 0) s.zc.reserve(a.M * r);
     const int L = (int)a.seg_lanes;
-    const size_t shm = lds ? lds_bytes<double>(a, r) : 0;
     const double* wr = a.weighted ? a.w_rows.p : nullptr;
-    FH_DISPATCH_R(r, hipLaunchKernelGGL((k_rhs_seg<RM>), dim3(grid), dim3(WG), shm, st, A, rows, wr, r, L, b1,
-                                        has_large ? s.zc.p : nullptr, s.part.p, s.dpart.p, (int)lds));
+    OpArgs<double> As = A;
+    if (small_in_lvl) As.nsb = 0;  // the scatter happens in the level pull
+    const size_t shm = (accl && !small_in_lvl && lds) ? lds_bytes<double>(a, r) : 0;
+    FH_DISPATCH_R(r, hipLaunchKernelGGL((k_rhs_seg<RM>), dim3(grid), dim3(WG), shm, st, As, rows, wr, r, L, b1,
+                                        fl >= 0 ? s.zc.p : nullptr, s.part.p, s.dpart.p, (int)(lds && shm > 0)));
     FH_CHECK_LAUNCH();
-    if (accl) {
+    if (accl && !small_in_lvl) {
         if (lds)
             hipLaunchKernelGGL(k_reduce_small<double>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
                                s.part.p, (size_t)grid, accl, rhs, nullptr);
@@ -545,13 +706,30 @@ void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, d
             FH_HIP(hipMemcpyAsync(rhs, s.part.p, accl * sizeof(double), hipMemcpyDeviceToDevice, st));
         FH_CHECK_LAUNCH();
     }
+    int nsb = 0;
+    for (auto& bi : a.blocks) nsb += bi.small ? 1 : 0;
     for (size_t b = 0; b < a.nb; b++) {
         const BlockInfo& bi = a.blocks[b];
         if (bi.small) continue;
+        const OtherOffs oo = other_offs(a, (int)b);
         const int Ll = (int)a.lvl_lanes;
-        const unsigned lg = grid_for(bi.g, (unsigned)((WAVE / Ll) * WAVES), 4096);
-        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_lvl_pull<RM>), dim3(lg), dim3(WG), 0, st, a.loff[b].p, a.lperm[b].p,
-                                            (size_t)bi.g, s.zc.p, r, Ll, rhs + bi.off * r));
+        const unsigned lg = lvl_grid(a, bi.g);
+        if ((int)b == fl && small_in_lvl) {
+            s.part.reserve((size_t)lg * accl);
+            FH_DISPATCH_R(r, with_small_nk<double, RM>(a.ms, [&](auto nk) {
+                constexpr int NK = decltype(nk)::value;
+                hipLaunchKernelGGL((k_lvl_pull<RM, NK>), dim3(lg), dim3(WG), 0, st, a.loff[b].p, a.lperm[b].p,
+                                   a.lother[b].p, (int)a.nb - 1, oo, nsb, (size_t)a.ms, (size_t)bi.g, s.zc.p, r, Ll,
+                                   rhs + bi.off * r, s.part.p);
+            }));
+            FH_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_reduce_small<double>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
+                               s.part.p, (size_t)lg, accl, rhs, nullptr);
+        } else {
+            FH_DISPATCH_R(r, hipLaunchKernelGGL((k_lvl_pull<RM, 0>), dim3(lg), dim3(WG), 0, st, a.loff[b].p,
+                                                a.lperm[b].p, a.lother[b].p, (int)a.nb - 1, oo, 0, (size_t)0,
+                                                (size_t)bi.g, s.zc.p, r, Ll, rhs + bi.off * r, (double*)nullptr));
+        }
         FH_CHECK_LAUNCH();
     }
     // wsq / wsum: fixed-order sum of the workgroup partials on the host

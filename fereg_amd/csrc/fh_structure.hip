@@ -133,6 +133,22 @@ __global__ __launch_bounds__(SW_WG) void k_small_work(const uint32_t* __restrict
     if (l == 0) atomicAdd(out, acc);
 }
 
+// One small block (block 0, codes ascending within a segment): distinct = runs of equal codes.
+__global__ void k_small_work1(const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cell_codes, size_t g1,
+                              int nb, unsigned long long* out) {
+    unsigned long long acc = 0;
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t c0 = seg_off[s], c1 = seg_off[s + 1];
+        unsigned long long d = 0;
+        for (uint32_t c = c0; c < c1; c++)
+            d += (c == c0 || cell_codes[(size_t)c * nb] != cell_codes[(size_t)(c - 1) * nb]) ? 1 : 0;
+        acc += d * d;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, WAVE);
+    if (lane_id() == 0) atomicAdd(out, acc);
+}
+
 __global__ void k_level_keys(const uint32_t* __restrict__ cell_codes, size_t M, int nb, int b,
                              uint64_t* __restrict__ keys) {
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x)
@@ -375,9 +391,14 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         FH_HIP(hipMemcpyAsync(dboff.p, boff.data(), nsb * sizeof(uint32_t), hipMemcpyHostToDevice, st));
         DBuf<unsigned long long> acc(1);
         FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
-        unsigned grid = grid_for(a.g1, SW_WG / WAVE, 4096);
-        hipLaunchKernelGGL(k_small_work, dim3(grid), dim3(SW_WG), words * 4 * (SW_WG / WAVE), st, a.seg_off.p,
-                           a.cell_codes.p, (size_t)a.g1, nb, nsb, dboff.p, words, acc.p);
+        if (nsb == 1) {
+            hipLaunchKernelGGL(k_small_work1, dim3(blocks_for(a.g1)), dim3(TPB), 0, st, a.seg_off.p, a.cell_codes.p,
+                               (size_t)a.g1, nb, acc.p);
+        } else {
+            unsigned grid = grid_for(a.g1, SW_WG / WAVE, 4096);
+            hipLaunchKernelGGL(k_small_work, dim3(grid), dim3(SW_WG), words * 4 * (SW_WG / WAVE), st, a.seg_off.p,
+                               a.cell_codes.p, (size_t)a.g1, nb, nsb, dboff.p, words, acc.p);
+        }
         FH_CHECK_LAUNCH();
         a.small_work = read_scalar(acc.p, st);
         if (a.small_work > 64ull * M) {
@@ -406,12 +427,12 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         while (L < 64 && L < mean) L <<= 1;
         return L;
     };
-    a.seg_lanes = lanes(a.g1 ? double(M) / double(a.g1) : 1.0);
+    a.seg_lanes = lanes(a.g1 ? double(M) / double(a.g1) / 2.0 : 1.0);
     uint64_t glarge = 0;
     int nlarge = 0;
     for (auto& b : a.blocks)
         if (!b.small) { glarge += b.g; nlarge++; }
-    a.lvl_lanes = lanes(glarge ? double(M) * nlarge / double(glarge) : 1.0);
+    a.lvl_lanes = lanes(glarge ? double(M) * nlarge / double(glarge) / 4.0 : 1.0);
     FH_HIP(hipStreamSynchronize(st));
 }
 

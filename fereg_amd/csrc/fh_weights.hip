@@ -10,6 +10,9 @@
 
 namespace fh {
 
+void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
+               double* G_host);
+
 namespace {
 
 constexpr unsigned TPB = 256;
@@ -140,6 +143,23 @@ __global__ __launch_bounds__(TPB) void k_small_assemble(const uint32_t* __restri
         __syncthreads();
         double* part = out + (size_t)blockIdx.x * len;
         for (size_t i = threadIdx.x; i < len; i += blockDim.x) part[i] = acc[i];
+    }
+}
+
+// Row s of C = [c_{s,0..ms-1} / sqrt(n1_s), sqrt(n1_s)] for the single small block (block 0,
+// whose codes ascend within a segment). One thread per segment writes its own row.
+__global__ void k_seg_level_sums(const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cell_codes,
+                                 const double* __restrict__ cw, const double* __restrict__ n1, size_t g1, int nb,
+                                 uint32_t off0, int ms, double* __restrict__ C) {
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
+        double* row = C + s * (ms + 1);
+        for (int k = 0; k <= ms; k++) row[k] = 0.0;
+        const uint32_t c0 = seg_off[s], c1 = seg_off[s + 1];
+        for (uint32_t c = c0; c < c1; c++) row[off0 + cell_codes[(size_t)c * nb]] += cw ? cw[c] : 1.0;
+        const double sq = sqrt(n1[s]);
+        const double f = sq > 0.0 ? 1.0 / sq : 0.0;
+        for (int k = 0; k < ms; k++) row[k] *= f;
+        row[ms] = sq;
     }
 }
 
@@ -278,7 +298,21 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             if (a.blocks[b].small) so.off[so.nsb++] = (uint32_t)a.blocks[b].off;
         const size_t len = ms * ms + ms;
         DBuf<double> sfull(len);
-        if (ms <= (size_t)SA_LDS_MAX) {
+        std::vector<double> h(len);
+        if (so.nsb == 1 && ms <= 21) {
+            // S_SS = diag(n_r) - C'C over the scaled level sums; n_r from the extra column
+            DBuf<double> C(g1 * (ms + 1));
+            hipLaunchKernelGGL(k_seg_level_sums, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p, a.cell_codes.p,
+                               cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p);
+            FH_CHECK_LAUNCH();
+            std::vector<double> G((ms + 1) * (ms + 1));
+            gram_cols(a, C.p, g1, (int)ms + 1, 0, (int)ms + 1, nullptr, nullptr, G.data());
+            for (size_t k = 0; k < ms; k++) {
+                h[ms * ms + k] = G[k * (ms + 1) + ms];
+                for (size_t l = 0; l < ms; l++) h[k * ms + l] = -G[k * (ms + 1) + l];
+                h[k * ms + k] += h[ms * ms + k];
+            }
+        } else if (ms <= (size_t)SA_LDS_MAX) {
             const unsigned grid = std::min<unsigned>(blocks_for(g1), 512);
             DBuf<double> parts((size_t)grid * len);
             hipLaunchKernelGGL(k_small_assemble, dim3(grid), dim3(TPB), len * sizeof(double), st, a.seg_off.p,
@@ -294,9 +328,10 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
                                cw, a.n1.p, g1, ms, so, sfull.p);
             FH_CHECK_LAUNCH();
         }
-        std::vector<double> h(len);
-        FH_HIP(hipMemcpyAsync(h.data(), sfull.p, len * sizeof(double), hipMemcpyDeviceToHost, st));
-        FH_HIP(hipStreamSynchronize(st));
+        if (!(so.nsb == 1 && ms <= 21)) {
+            FH_HIP(hipMemcpyAsync(h.data(), sfull.p, len * sizeof(double), hipMemcpyDeviceToHost, st));
+            FH_HIP(hipStreamSynchronize(st));
+        }
         std::vector<double> nrs(ms), dg(ms);
         std::vector<uint8_t> rd(ms);
         std::vector<double> S(h.begin(), h.begin() + ms * ms);
