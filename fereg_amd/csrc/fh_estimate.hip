@@ -3,6 +3,7 @@
 // per-cluster score sums s_g = sum_{i in g} x_i e_i w_i of the CRV1 meat (vcov.rs:320-448), with
 // the rows sorted by cluster once (radix sort) so every sum is in a fixed order.
 #include <algorithm>
+#include <utility>
 
 #include "fh_internal.hpp"
 #include "fh_sort.hpp"
@@ -16,61 +17,61 @@ constexpr int GRAM_TILE = 256;
 constexpr int GRAM_KMAX = 22;  // k(k+1)/2 <= 256 pairs, one per thread
 constexpr int GRID = 1024;
 
-// Per-workgroup partial Gram of columns [c0, c0+k) of the n x ld row-major block z, weights
-// w (optional) times v (optional): part[wg][p] over the k(k+1)/2 pairs p = (a <= b). The pairs
-// are split into groups of GP; thread t takes pair group t % ngroups and every
-// (256/ngroups)-th row, keeps its row's k values and GP sums in registers; the block then adds
+// Per-workgroup partial Gram of columns [c0, c0+K) of the n x ld row-major block z, weights
+// w (optional) times v (optional): part[wg][p] over the K(K+1)/2 pairs p = (a <= b) in row-major
+// upper order. K is a compile-time constant: the pair loops unroll to straight FMAs. Pairs are
+// split into NG groups (thread t takes group t % NG and every (256/NG)-th row); the block adds
 // the threads of each group in a fixed order.
-constexpr int GP = 8;
-__global__ __launch_bounds__(TPB) void k_gram(const double* __restrict__ z, size_t n, int ld, int c0, int k,
+template <int K>
+__global__ __launch_bounds__(TPB) void k_gram(const double* __restrict__ z, size_t n, int ld, int c0,
                                               const double* __restrict__ w, const double* __restrict__ v,
                                               double* __restrict__ part) {
-    __shared__ double red[TPB][GP];
-    const int npair = k * (k + 1) / 2;
-    const int ngroups = (npair + GP - 1) / GP;
-    const int grp = threadIdx.x % ngroups;
-    const int rl = threadIdx.x / ngroups, nrl = TPB / ngroups;
-    int pa[GP], pb[GP];
+    constexpr int NP = K * (K + 1) / 2;
+    constexpr int NG = NP <= 36 ? 1 : (NP + 35) / 36;
+    constexpr int GPS = (NP + NG - 1) / NG;  // pairs per group
+    __shared__ double red[TPB][GPS];
+    const int grp = threadIdx.x % NG;
+    const int rl = threadIdx.x / NG, nrl = TPB / NG;
+    double acc[GPS];
 #pragma unroll
-    for (int q = 0; q < GP; q++) {
-        int t = grp * GP + q, a = 0;
-        if (t >= npair) { pa[q] = 0; pb[q] = 0; continue; }
-        while (t >= k - a) { t -= k - a; a++; }
-        pa[q] = a;
-        pb[q] = a + t;
-    }
-    double acc[GP];
-#pragma unroll
-    for (int q = 0; q < GP; q++) acc[q] = 0.0;
+    for (int q = 0; q < GPS; q++) acc[q] = 0.0;
     if (rl < nrl) {
         for (size_t i = (size_t)blockIdx.x * nrl + rl; i < n; i += (size_t)gridDim.x * nrl) {
-            double x[GRAM_KMAX];
+            double x[K];
             const double* row = z + i * ld + c0;
 #pragma unroll
-            for (int c = 0; c < GRAM_KMAX; c++) x[c] = c < k ? row[c] : 0.0;
+            for (int c = 0; c < K; c++) x[c] = row[c];
             double wt = w ? w[i] : 1.0;
             if (v) wt *= v[i];
+            int p = 0;
 #pragma unroll
-            for (int q = 0; q < GP; q++) {
-                double xa = 0.0, xb = 0.0;
+            for (int a = 0; a < K; a++)
 #pragma unroll
-                for (int c = 0; c < GRAM_KMAX; c++) {
-                    if (c == pa[q]) xa = x[c];
-                    if (c == pb[q]) xb = x[c];
-                }
-                acc[q] += wt * xa * xb;
-            }
+                for (int b = a; b < K; b++, p++)
+                    if (NG == 1 || p / GPS == grp) acc[p % GPS] += wt * x[a] * x[b];
         }
     }
 #pragma unroll
-    for (int q = 0; q < GP; q++) red[threadIdx.x][q] = acc[q];
+    for (int q = 0; q < GPS; q++) red[threadIdx.x][q] = acc[q];
     __syncthreads();
-    for (int p = threadIdx.x; p < npair; p += TPB) {
-        const int g = p / GP, q = p % GP;
+    for (int p = threadIdx.x; p < NP; p += TPB) {
+        const int g = p / GPS, q = p % GPS;
         double s = 0.0;
-        for (int t = g; t < nrl * ngroups; t += ngroups) s += red[t][q];
-        part[(size_t)blockIdx.x * npair + p] = s;
+        for (int t = g; t < nrl * NG; t += NG) s += red[t][q];
+        part[(size_t)blockIdx.x * NP + p] = s;
     }
+}
+
+template <int K>
+void launch_gram(unsigned grid, hipStream_t st, const double* z, size_t n, int ld, int c0, const double* w,
+                 const double* v, double* part) {
+    hipLaunchKernelGGL((k_gram<K>), dim3(grid), dim3(TPB), 0, st, z, n, ld, c0, w, v, part);
+}
+
+template <int... Ks>
+void gram_dispatch(int k, unsigned grid, hipStream_t st, const double* z, size_t n, int ld, int c0, const double* w,
+                   const double* v, double* part, std::integer_sequence<int, Ks...>) {
+    ((k == Ks + 1 ? launch_gram<Ks + 1>(grid, st, z, n, ld, c0, w, v, part) : void()), ...);
 }
 
 // e_i = sum_j c_j z_ij; partial sums of w e^2 per workgroup.
@@ -153,7 +154,7 @@ void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, co
     const int npair = k * (k + 1) / 2;
     const unsigned grid = std::max(1u, std::min<unsigned>(GRID, (unsigned)((n + 255) / 256)));
     DBuf<double> part((size_t)grid * npair);
-    hipLaunchKernelGGL(k_gram, dim3(grid), dim3(TPB), 0, a.st, z, n, ld, c0, k, w, v, part.p);
+    gram_dispatch(k, grid, a.st, z, n, ld, c0, w, v, part.p, std::make_integer_sequence<int, GRAM_KMAX>{});
     FH_CHECK_LAUNCH();
     std::vector<double> h((size_t)grid * npair);
     FH_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, a.st));

@@ -1,0 +1,12 @@
+// Reduced operator, FP32 instantiation (the inner solve of the mixed form).
+#include "fh_operator_impl.hpp"
+
+namespace fh {
+
+template <class T>
+void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st) {
+    spmv_impl<T>(a, p, y, r, s, done, st);
+}
+template void spmv<float>(const Absorber&, const float*, float*, int, Scratch<float>&, const uint32_t*, hipStream_t);
+
+}  // namespace fh

@@ -427,12 +427,12 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         while (L < 64 && L < mean) L <<= 1;
         return L;
     };
-    a.seg_lanes = lanes(a.g1 ? double(M) / double(a.g1) / 2.0 : 1.0);
+    a.seg_lanes = lanes(a.g1 ? double(M) / double(a.g1) / 4.0 : 1.0);
     uint64_t glarge = 0;
     int nlarge = 0;
     for (auto& b : a.blocks)
         if (!b.small) { glarge += b.g; nlarge++; }
-    a.lvl_lanes = lanes(glarge ? double(M) * nlarge / double(glarge) / 4.0 : 1.0);
+    a.lvl_lanes = lanes(glarge ? double(M) * nlarge / double(glarge) / 8.0 : 1.0);
     FH_HIP(hipStreamSynchronize(st));
 }
 

@@ -147,19 +147,29 @@ __global__ __launch_bounds__(TPB) void k_small_assemble(const uint32_t* __restri
 }
 
 // Row s of C = [c_{s,0..ms-1} / sqrt(n1_s), sqrt(n1_s)] for the single small block (block 0,
-// whose codes ascend within a segment). One thread per segment writes its own row.
+// whose codes ascend within a segment). One thread per segment, the row in registers.
+template <int MS>
 __global__ void k_seg_level_sums(const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cell_codes,
                                  const double* __restrict__ cw, const double* __restrict__ n1, size_t g1, int nb,
                                  uint32_t off0, int ms, double* __restrict__ C) {
     for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
-        double* row = C + s * (ms + 1);
-        for (int k = 0; k <= ms; k++) row[k] = 0.0;
+        double row[MS];
+#pragma unroll
+        for (int k = 0; k < MS; k++) row[k] = 0.0;
         const uint32_t c0 = seg_off[s], c1 = seg_off[s + 1];
-        for (uint32_t c = c0; c < c1; c++) row[off0 + cell_codes[(size_t)c * nb]] += cw ? cw[c] : 1.0;
+        for (uint32_t c = c0; c < c1; c++) {
+            const uint32_t key = off0 + cell_codes[(size_t)c * nb];
+            const double w = cw ? cw[c] : 1.0;
+#pragma unroll
+            for (int k = 0; k < MS; k++) row[k] += key == (uint32_t)k ? w : 0.0;
+        }
         const double sq = sqrt(n1[s]);
         const double f = sq > 0.0 ? 1.0 / sq : 0.0;
-        for (int k = 0; k < ms; k++) row[k] *= f;
-        row[ms] = sq;
+        double* out = C + s * (ms + 1);
+#pragma unroll
+        for (int k = 0; k < MS; k++)
+            if (k < ms) out[k] = row[k] * f;
+        out[ms] = sq;
     }
 }
 
@@ -302,8 +312,12 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         if (so.nsb == 1 && ms <= 21) {
             // S_SS = diag(n_r) - C'C over the scaled level sums; n_r from the extra column
             DBuf<double> C(g1 * (ms + 1));
-            hipLaunchKernelGGL(k_seg_level_sums, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p, a.cell_codes.p,
-                               cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p);
+            if (ms <= 8)
+                hipLaunchKernelGGL(k_seg_level_sums<8>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
+                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p);
+            else
+                hipLaunchKernelGGL(k_seg_level_sums<21>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
+                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p);
             FH_CHECK_LAUNCH();
             std::vector<double> G((ms + 1) * (ms + 1));
             gram_cols(a, C.p, g1, (int)ms + 1, 0, (int)ms + 1, nullptr, nullptr, G.data());
