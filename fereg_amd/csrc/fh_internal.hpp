@@ -6,11 +6,15 @@
 //   cell_codes    u32[M*nb]   block codes per cell, row-major (cell, block)
 //   cell_obs_off  u32[M+1]    first row of every cell (absent when every cell is one row)
 //   cell_seg      u32[M]      FE1 level of every cell
-//   per large block b (level-major transpose of the cells, built once):
-//     loff        u32[g_b+1]  level offsets;  lperm u32[M] cells in (level, cell) order
-//     lseg        u32[M]      FE1 level of the cell at each level position
-//     lother      u32[M*(nb-1)] codes of the other blocks at each level position
-//     lw          T[M]        cell weight at each level position (weighted designs)
+//   per large block b, the level-major transpose of the cells in segment buckets (built once):
+//     the FE1 levels are cut into buckets of 2^17 consecutive segments; cells are ordered by
+//     (bucket, level, cell), so a pass over one bucket gathers an L2-sized slice of per-segment
+//     vectors. nbk[b] buckets; item i = bucket * g_b + level:
+//     boff        u32[nbk*g_b+1] item offsets;  bperm u32[M] cells in (bucket, level, cell) order
+//     lseg        u32[M]      FE1 level of the cell at each position
+//     lother      u32[M*(nb-1)] codes of the other blocks at each position
+//     lw          T[M]        cell weight at each position (weighted designs)
+//   lperm/loff: the plain (level, cell) lists of structure.rs:144-153, built on demand.
 //   weight state: cw T[M] (null = unit), inv_n1 T[g1], inv_diag T[m], minv T[ms*ms] (dense
 //   grounded pseudo-inverse of the small super-block).
 #pragma once
@@ -42,7 +46,7 @@ struct WeightArrays {
 // Per-call scratch of the PCG and operator kernels (grown on demand).
 template <class T>
 struct Scratch {
-    DBuf<T> r, z, p, q, x, rhs, t, zc, part;
+    DBuf<T> r, z, p, q, x, rhs, t, zc, part, ypart;
     DBuf<double> dpart;  // FP64 partial sums (dots)
 };
 
@@ -57,7 +61,10 @@ struct Absorber {
     uint64_t small_limit = 4096;
     int precision = FH_PLAIN_FP64;
     DBuf<uint32_t> pi, seg_off, cell_codes, cell_obs_off, cell_seg;
-    std::vector<DBuf<uint32_t>> lperm, loff, lseg, lother;  // indexed by block (large only built)
+    std::vector<DBuf<uint32_t>> lperm, loff;                // plain level lists, on demand
+    std::vector<DBuf<uint32_t>> bperm, boff, lseg, lother;  // bucketed transpose (large blocks)
+    std::vector<uint32_t> nbk;                              // buckets per block
+    static constexpr int BUCKET_SHIFT = 17;
     std::vector<uint32_t> host_pi;
     bool host_pi_valid = false;
     // weights

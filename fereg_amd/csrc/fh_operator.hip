@@ -58,24 +58,37 @@ void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, d
         if (bi.small) continue;
         const OtherOffs oo = other_offs(a, (int)b);
         const int Ll = (int)a.lvl_lanes;
-        const unsigned lg = lvl_grid(a, bi.g);
+        const uint32_t nbk = a.nbk[b];
+        const size_t items = (size_t)nbk * bi.g;
+        const unsigned lg = lvl_grid(a, items);
+        double* ypart = nullptr;
+        if (nbk > 1) {
+            s.ypart.reserve(items * r);
+            ypart = s.ypart.p;
+        }
         if ((int)b == fl && small_in_lvl) {
             s.part.reserve((size_t)lg * accl);
             FH_DISPATCH_R(r, with_small_nk<double, RM>(a.ms, [&](auto nk) {
                 constexpr int NK = decltype(nk)::value;
-                hipLaunchKernelGGL((k_lvl_pull<RM, NK>), dim3(lg), dim3(WG), 0, st, a.loff[b].p, a.lperm[b].p,
-                                   a.lother[b].p, (int)a.nb - 1, oo, nsb, (size_t)a.ms, (size_t)bi.g, s.zc.p, r, Ll,
-                                   rhs + bi.off * r, s.part.p);
+                hipLaunchKernelGGL((k_lvl_pull<RM, NK>), dim3(lg), dim3(WG), 0, st, a.boff[b].p, a.bperm[b].p,
+                                   a.lother[b].p, (int)a.nb - 1, oo, nsb, (size_t)a.ms, items, (uint32_t)bi.g, s.zc.p,
+                                   r, Ll, rhs + bi.off * r, ypart, s.part.p);
             }));
             FH_CHECK_LAUNCH();
             hipLaunchKernelGGL(k_reduce_small<double>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
                                s.part.p, (size_t)lg, accl, rhs, nullptr);
         } else {
-            FH_DISPATCH_R(r, hipLaunchKernelGGL((k_lvl_pull<RM, 0>), dim3(lg), dim3(WG), 0, st, a.loff[b].p,
-                                                a.lperm[b].p, a.lother[b].p, (int)a.nb - 1, oo, 0, (size_t)0,
-                                                (size_t)bi.g, s.zc.p, r, Ll, rhs + bi.off * r, (double*)nullptr));
+            FH_DISPATCH_R(r, hipLaunchKernelGGL((k_lvl_pull<RM, 0>), dim3(lg), dim3(WG), 0, st, a.boff[b].p,
+                                                a.bperm[b].p, a.lother[b].p, (int)a.nb - 1, oo, 0, (size_t)0, items,
+                                                (uint32_t)bi.g, s.zc.p, r, Ll, rhs + bi.off * r, ypart,
+                                                (double*)nullptr));
         }
         FH_CHECK_LAUNCH();
+        if (ypart) {
+            hipLaunchKernelGGL(k_bucket_reduce<double>, dim3(grid_for(bi.g * r, WG, 8192)), dim3(WG), 0, st, ypart,
+                               nbk, (size_t)bi.g, r, rhs + bi.off * r, (const uint32_t*)nullptr);
+            FH_CHECK_LAUNCH();
+        }
     }
     // wsq / wsum: fixed-order sum of the workgroup partials on the host
     std::vector<double> h((size_t)grid * 2 * r);

@@ -24,6 +24,11 @@ constexpr int WAVES = WG / WAVE;
 template <class T>
 __device__ __forceinline__ T shfl_xor_t(T v, int o) { return __shfl_xor(v, o, WAVE); }
 
+// Streamed (read-once) loads: non-temporal, so the index streams do not evict the gathered
+// vectors (p, t) from L2.
+template <class T>
+__device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, WAVE));
@@ -229,8 +234,8 @@ __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__
                 const uint32_t c = c0 + (it + h) * L + li;
                 v[h] = c < c1;
 #pragma unroll
-                for (int b = 0; b < NB; b++) code[h][b] = v[h] ? A.cell_codes[(size_t)c * NB + b] : 0u;
-                wc[h] = v[h] ? (A.cw ? A.cw[c] : T(1)) : T(0);
+                for (int b = 0; b < NB; b++) code[h][b] = v[h] ? ldnt(&A.cell_codes[(size_t)c * NB + b]) : 0u;
+                wc[h] = v[h] ? (A.cw ? ldnt(&A.cw[c]) : T(1)) : T(0);
             }
 #pragma unroll
             for (int h = 0; h < U; h++) {
@@ -291,9 +296,9 @@ struct OtherOffs {
 template <class T, int RM, int NK, int NO>
 __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lseg,
                                                  const uint32_t* __restrict__ lother, const T* __restrict__ lw,
-                                                 size_t g, uint32_t off_b, OtherOffs oo, int nother, int nsb, size_t ms,
-                                                 const T* __restrict__ p, const T* __restrict__ t, int r, int L,
-                                                 T* __restrict__ y, T* __restrict__ part,
+                                                 size_t g, uint32_t gb, uint32_t off_b, OtherOffs oo, int nother, int nsb,
+                                                 size_t ms, const T* __restrict__ p, const T* __restrict__ t, int r,
+                                                 int L, T* __restrict__ y, T* __restrict__ ypart, T* __restrict__ part,
                                                  const uint32_t* __restrict__ done) {
     if (done && *done) return;
     constexpr int U = 4;
@@ -303,9 +308,10 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
     if (NK > 0) sm.zero();
     const size_t nwt = (size_t)gridDim.x * WAVES;
     for (size_t base = ((size_t)blockIdx.x * WAVES + w) * gpw; base < g; base += nwt * gpw) {
-        const size_t lv = base + gl;
-        const bool valid = lv < g;
-        const uint32_t j0 = valid ? loff[lv] : 0, j1 = valid ? loff[lv + 1] : 0;
+        const size_t it_ = base + gl;  // item = bucket * gb + level
+        const bool valid = it_ < g;
+        const size_t lv = valid ? it_ % gb : 0;
+        const uint32_t j0 = valid ? loff[it_] : 0, j1 = valid ? loff[it_ + 1] : 0;
         const uint32_t tmax = wave_max_u32((j1 - j0 + L - 1) / L);
         T pb[RM], acc[RM];
 #pragma unroll
@@ -322,10 +328,11 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
             for (int h = 0; h < U; h++) {
                 const uint32_t pos = j0 + (it + h) * L + li;
                 pv[h] = pos < j1;
-                sg[h] = pv[h] ? lseg[pos] : 0u;
+                sg[h] = pv[h] ? ldnt(&lseg[pos]) : 0u;
 #pragma unroll
-                for (int o = 0; o < NO; o++) oc[h][o] = (pv[h] && o < nother) ? lother[(size_t)pos * nother + o] : 0u;
-                wc[h] = pv[h] ? (lw ? lw[pos] : T(1)) : T(0);
+                for (int o = 0; o < NO; o++)
+                    oc[h][o] = (pv[h] && o < nother) ? ldnt(&lother[(size_t)pos * nother + o]) : 0u;
+                wc[h] = pv[h] ? (lw ? ldnt(&lw[pos]) : T(1)) : T(0);
             }
 #pragma unroll
             for (int h = 0; h < U; h++) {
@@ -354,12 +361,26 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
         }
 #pragma unroll
         for (int j = 0; j < RM; j++) acc[j] = gsum(acc[j], L);
-        if (valid && li == 0)
+        if (valid && li == 0) {
+            T* dst = ypart ? ypart + it_ * r : y + (size_t)(off_b + lv) * r;
 #pragma unroll
             for (int j = 0; j < RM; j++)
-                if (j < r) y[(size_t)(off_b + lv) * r + j] = acc[j];
+                if (j < r) dst[j] = acc[j];
+        }
     }
     if (NK > 0) sm.flush(part + (size_t)blockIdx.x * ms * r, ms, r);
+}
+
+// y[l] = sum over the buckets of ypart[k*gb + l] (fixed order).
+template <class T>
+__global__ void k_bucket_reduce(const T* __restrict__ ypart, uint32_t nbk, size_t gb, int r, T* __restrict__ y,
+                                const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < gb * r; e += (size_t)gridDim.x * blockDim.x) {
+        T s = T(0);
+        for (uint32_t k = 0; k < nbk; k++) s += ypart[(size_t)k * gb * r + e];
+        y[e] = s;
+    }
 }
 
 // ---------------------------------------------------------------- rhs pass (FP64)
@@ -479,15 +500,16 @@ __global__ __launch_bounds__(WG) void k_rhs_seg(OpArgs<double> A, const double* 
 template <int RM, int NK>
 __global__ __launch_bounds__(WG) void k_lvl_pull(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lperm,
                                                  const uint32_t* __restrict__ lother, int nother, OtherOffs oo, int nsb,
-                                                 size_t ms, size_t g, const double* __restrict__ zc, int r, int L,
-                                                 double* __restrict__ out, double* __restrict__ part) {
+                                                 size_t ms, size_t g, uint32_t gb, const double* __restrict__ zc, int r,
+                                                 int L, double* __restrict__ out, double* __restrict__ ypart,
+                                                 double* __restrict__ part) {
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
     SmallRegs<double, RM, NK> sm;
     if (NK > 0) sm.zero();
     const size_t nwt = (size_t)gridDim.x * WAVES;
     for (size_t base = ((size_t)blockIdx.x * WAVES + w) * gpw; base < g; base += nwt * gpw) {
-        const size_t lv = base + gl;
+        const size_t lv = base + gl;  // item = bucket * gb + level
         const bool valid = lv < g;
         const uint32_t j0 = valid ? loff[lv] : 0, j1 = valid ? loff[lv + 1] : 0;
         const uint32_t tmax = wave_max_u32((j1 - j0 + L - 1) / L);
@@ -512,10 +534,12 @@ __global__ __launch_bounds__(WG) void k_lvl_pull(const uint32_t* __restrict__ lo
         }
 #pragma unroll
         for (int j = 0; j < RM; j++) acc[j] = gsum(acc[j], L);
-        if (valid && li == 0)
+        if (valid && li == 0) {
+            double* dst = ypart ? ypart + lv * r : out + (lv % gb) * r;
 #pragma unroll
             for (int j = 0; j < RM; j++)
-                if (j < r) out[lv * r + j] = acc[j];
+                if (j < r) dst[j] = acc[j];
+        }
     }
     if (NK > 0) sm.flush(part + (size_t)blockIdx.x * ms * r, ms, r);
 }
@@ -655,6 +679,9 @@ void with_no(int no, F&& f) {
     }
 }
 
+template <class T>
+DBuf<T>& ypart_buf(Scratch<T>& s) { return s.ypart; }
+
 // Index of the first large block (-1 when every block is small).
 inline int first_large(const Absorber& a) {
     for (size_t b = 0; b < a.nb; b++)
@@ -727,17 +754,24 @@ void spmv_impl(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const 
         if (bi.small) continue;
         const OtherOffs oo = other_offs(a, b);
         const int Ll = (int)a.lvl_lanes;
-        const unsigned lg = lvl_grid(a, bi.g);
+        const uint32_t nbk = a.nbk[b];
+        const size_t items = (size_t)nbk * bi.g;
+        const unsigned lg = lvl_grid(a, items);
         const T* lw = a.unit() ? nullptr : W->lw[b].p;
+        T* ypart = nullptr;
+        if (nbk > 1) {
+            ypart_buf<T>(s).reserve(items * r);
+            ypart = ypart_buf<T>(s).p;
+        }
         if (b == fl && small_in_lvl) {
             s.part.reserve((size_t)lg * accl);
             FH_DISPATCH_R(r, with_small_nk<T, RM>(a.ms, [&](auto nk) {
                 constexpr int NK = decltype(nk)::value;
                 with_no(nb - 1, [&](auto noc) {
                     constexpr int NO = decltype(noc)::value;
-                    hipLaunchKernelGGL((k_lvl_spmv<T, RM, NK, NO>), dim3(lg), dim3(WG), 0, st, a.loff[b].p,
-                                       a.lseg[b].p, a.lother[b].p, lw, (size_t)bi.g, (uint32_t)bi.off, oo, nb - 1, nsb,
-                                       (size_t)a.ms, p, s.t.p, r, Ll, y, s.part.p, done);
+                    hipLaunchKernelGGL((k_lvl_spmv<T, RM, NK, NO>), dim3(lg), dim3(WG), 0, st, a.boff[b].p,
+                                       a.lseg[b].p, a.lother[b].p, lw, items, (uint32_t)bi.g, (uint32_t)bi.off, oo,
+                                       nb - 1, nsb, (size_t)a.ms, p, s.t.p, r, Ll, y, ypart, s.part.p, done);
                 });
             }));
             FH_CHECK_LAUNCH();
@@ -746,12 +780,17 @@ void spmv_impl(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const 
         } else {
             FH_DISPATCH_R(r, with_no(nb - 1, [&](auto noc) {
                 constexpr int NO = decltype(noc)::value;
-                hipLaunchKernelGGL((k_lvl_spmv<T, RM, 0, NO>), dim3(lg), dim3(WG), 0, st, a.loff[b].p, a.lseg[b].p,
-                                   a.lother[b].p, lw, (size_t)bi.g, (uint32_t)bi.off, oo, nb - 1, 0, (size_t)0, p, s.t.p, r,
-                                   Ll, y, (T*)nullptr, done);
+                hipLaunchKernelGGL((k_lvl_spmv<T, RM, 0, NO>), dim3(lg), dim3(WG), 0, st, a.boff[b].p, a.lseg[b].p,
+                                   a.lother[b].p, lw, items, (uint32_t)bi.g, (uint32_t)bi.off, oo, nb - 1, 0, (size_t)0,
+                                   p, s.t.p, r, Ll, y, ypart, (T*)nullptr, done);
             }));
         }
         FH_CHECK_LAUNCH();
+        if (ypart) {
+            hipLaunchKernelGGL(k_bucket_reduce<T>, dim3(grid_for(bi.g * r, WG, 8192)), dim3(WG), 0, st, ypart, nbk,
+                               (size_t)bi.g, r, y + bi.off * r, done);
+            FH_CHECK_LAUNCH();
+        }
     }
 }
 }  // namespace fh

@@ -155,6 +155,13 @@ __global__ void k_level_keys(const uint32_t* __restrict__ cell_codes, size_t M, 
         keys[c] = cell_codes[c * nb + b];
 }
 
+// keys[c] = bucket(s(c)) * g + code_b(c): the (bucket, level) item of every cell.
+__global__ void k_bucket_keys(const uint32_t* __restrict__ cell_codes, const uint32_t* __restrict__ cell_seg, size_t M,
+                              int nb, int b, int shift, uint64_t g, uint64_t* __restrict__ keys) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x)
+        keys[c] = (uint64_t)(cell_seg[c] >> shift) * g + cell_codes[c * nb + b];
+}
+
 __global__ void k_level_records(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ cell_seg,
                                 const uint32_t* __restrict__ cell_codes, size_t M, int nb, int b,
                                 uint32_t* __restrict__ lseg, uint32_t* __restrict__ lother) {
@@ -407,18 +414,39 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         }
     }
 
-    // level-major transpose of every large block
-    a.lperm.clear(); a.loff.clear(); a.lseg.clear(); a.lother.clear();
-    a.lperm.resize(nb); a.loff.resize(nb); a.lseg.resize(nb); a.lother.resize(nb);
+    // bucketed level-major transpose of every large block
+    a.lperm.clear(); a.loff.clear(); a.lseg.clear(); a.lother.clear(); a.bperm.clear(); a.boff.clear();
+    a.lperm.resize(nb); a.loff.resize(nb); a.lseg.resize(nb); a.lother.resize(nb); a.bperm.resize(nb);
+    a.boff.resize(nb);
+    a.nbk.assign(nb, 1);
     for (int b = 0; b < nb; b++) {
         if (a.blocks[b].small) continue;
-        ensure_level_cells(a, b);
+        const uint64_t g = a.blocks[b].g;
+        const uint32_t nbk = (uint32_t)((a.g1 + (1ull << Absorber::BUCKET_SHIFT) - 1) >> Absorber::BUCKET_SHIFT);
+        a.nbk[b] = std::max<uint32_t>(nbk, 1);
+        const uint64_t items = (uint64_t)a.nbk[b] * g;
+        require(items < (1ull << 32), "too many (bucket, level) items");
+        a.bperm[b].alloc(std::max<size_t>(M, 1));
+        a.boff[b].alloc(items + 1);
+        FH_HIP(hipMemsetAsync(a.boff[b].p, 0, a.boff[b].bytes(), st));
         a.lseg[b].alloc(std::max<size_t>(M, 1));
         a.lother[b].alloc(std::max<size_t>(M * (nb - 1), 1));
         if (M > 0) {
-            hipLaunchKernelGGL(k_level_records, dim3(blocks_for(M)), dim3(TPB), 0, st, a.lperm[b].p, a.cell_seg.p,
+            DBuf<uint64_t> keys(M), kalt(M);
+            DBuf<uint32_t> valt(M), flag(1);
+            hipLaunchKernelGGL(k_bucket_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, a.cell_codes.p, a.cell_seg.p, M, nb,
+                               b, Absorber::BUCKET_SHIFT, g, keys.p);
+            hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, M);
+            FH_CHECK_LAUNCH();
+            check_sorted_u64(keys.p, M, flag.p, st);
+            if (read_scalar(flag.p, st) != 0)
+                radix_sort_pairs(keys.p, a.bperm[b].p, kalt.p, valt.p, M, (int)bits_for(items), st);
+            hipLaunchKernelGGL(k_offsets_u64, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, (size_t)items,
+                               a.boff[b].p);
+            hipLaunchKernelGGL(k_level_records, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, a.cell_seg.p,
                                a.cell_codes.p, M, nb, b, a.lseg[b].p, a.lother[b].p);
             FH_CHECK_LAUNCH();
+            FH_HIP(hipStreamSynchronize(st));
         }
     }
     // lanes per segment / per level: the next power of two of the mean length, within 1..64
@@ -438,6 +466,7 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
 
 void ensure_level_cells(Absorber& a, size_t b) {
     if (a.lperm.size() < a.nb) { a.lperm.resize(a.nb); a.loff.resize(a.nb); }
+    if (b >= a.nb) return;
     if (a.lperm[b].n) return;
     hipStream_t st = a.st;
     const size_t M = a.M;

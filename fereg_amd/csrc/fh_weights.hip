@@ -59,36 +59,39 @@ __global__ __launch_bounds__(TPB) void k_partial_sum(const double* __restrict__ 
     }
 }
 
-// Large block: one thread per level walks its cells in (level, cell) order; cells of one FE1
-// level are adjacent there, so c_{s,k} are run sums.
-__global__ void k_level_weights(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lperm,
-                                const uint32_t* __restrict__ lseg, const double* __restrict__ cw,
+// Large block: one thread per level walks its cells bucket by bucket ((bucket, level, cell)
+// order); the cells of one FE1 level are adjacent within a bucket, so c_{s,k} are run sums.
+__global__ void k_level_weights(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bperm,
+                                const uint32_t* __restrict__ lseg, uint32_t nbk, const double* __restrict__ cw,
                                 const double* __restrict__ n1, size_t g, size_t off, double* __restrict__ nr,
                                 double* __restrict__ diag, uint8_t* __restrict__ red, double* __restrict__ inv64,
                                 float* __restrict__ inv32, double* __restrict__ lw64, float* __restrict__ lw32) {
     for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
-        uint32_t j0 = loff[l], j1 = loff[l + 1];
-        double tot = 0.0, dq = 0.0, run = 0.0;
-        uint32_t cur = 0xFFFFFFFFu;
-        for (uint32_t j = j0; j < j1; j++) {
-            double w = cw ? cw[lperm[j]] : 1.0;
-            if (lw64) { lw64[j] = w; lw32[j] = (float)w; }
-            uint32_t s = lseg[j];
-            if (s != cur) {
-                if (cur != 0xFFFFFFFFu) dq += run * run / n1[cur];
-                cur = s;
-                run = 0.0;
+        double tot = 0.0, dq = 0.0;
+        for (uint32_t k = 0; k < nbk; k++) {
+            const uint32_t j0 = boff[(size_t)k * g + l], j1 = boff[(size_t)k * g + l + 1];
+            double run = 0.0;
+            uint32_t cur = 0xFFFFFFFFu;
+            for (uint32_t j = j0; j < j1; j++) {
+                const double w = cw ? cw[bperm[j]] : 1.0;
+                if (lw64) { lw64[j] = w; lw32[j] = (float)w; }
+                const uint32_t s = lseg[j];
+                if (s != cur) {
+                    if (cur != 0xFFFFFFFFu) dq += run * run / n1[cur];
+                    cur = s;
+                    run = 0.0;
+                }
+                run += w;
+                tot += w;
             }
-            run += w;
-            tot += w;
+            if (cur != 0xFFFFFFFFu) dq += run * run / n1[cur];
         }
-        if (cur != 0xFFFFFFFFu) dq += run * run / n1[cur];
-        double d = tot - dq;
-        bool r = d <= 1e-12 * tot;  // operator.rs:21, 319-333
+        const double d = tot - dq;
+        const bool r = d <= 1e-12 * tot;  // operator.rs:21, 319-333
         nr[off + l] = tot;
         diag[off + l] = d;
         red[off + l] = r;
-        double inv = (r || !(d > 0.0)) ? 0.0 : 1.0 / d;  // jacobi.rs:7-9
+        const double inv = (r || !(d > 0.0)) ? 0.0 : 1.0 / d;  // jacobi.rs:7-9
         inv64[off + l] = inv;
         inv32[off + l] = (float)inv;
     }
@@ -292,8 +295,8 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             a.w64.lw[b].release();
             a.w32.lw[b].release();
         }
-        hipLaunchKernelGGL(k_level_weights, dim3(blocks_for(bi.g)), dim3(TPB), 0, st, a.loff[b].p, a.lperm[b].p,
-                           a.lseg[b].p, cw, a.n1.p, (size_t)bi.g, (size_t)bi.off, a.nr.p, a.diag.p, a.red.p,
+        hipLaunchKernelGGL(k_level_weights, dim3(blocks_for(bi.g)), dim3(TPB), 0, st, a.boff[b].p, a.bperm[b].p,
+                           a.lseg[b].p, a.nbk[b], cw, a.n1.p, (size_t)bi.g, (size_t)bi.off, a.nr.p, a.diag.p, a.red.p,
                            a.w64.inv_diag.p, a.w32.inv_diag.p, unit ? nullptr : a.w64.lw[b].p,
                            unit ? nullptr : a.w32.lw[b].p);
         FH_CHECK_LAUNCH();

@@ -147,3 +147,32 @@ def test_device_resident_block():
     a.residualize(z, 2, F.SolveOpts(tol=1e-12))
     torch.cuda.synchronize()
     np.testing.assert_allclose(z.cpu().numpy(), dense_residualize(v, codes, gs)[pi], atol=1e-9)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_bucketed_level_lists_match_oracle(weighted):
+    """FE1 with more than 2^17 levels: the level passes run over several segment buckets."""
+    import fereg_amd as F
+    rng = np.random.default_rng(77)
+    n = 600_000
+    i = np.arange(n)
+    codes = [(i // 2).astype(np.uint32), rng.integers(0, 5000, n).astype(np.uint32), (i % 2).astype(np.uint32),
+             rng.integers(0, 3, n).astype(np.uint32)]
+    gs = [300_000, 5000, 2, 3]
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    o = OracleSolver(codes, gs, weights=w)
+    a = F.Absorber(codes, gs, weights=w)
+    assert a.info.g1 == 300_000 and a.info.m_small == 5
+    for r in (1, 3):
+        p = rng.standard_normal(o.info.m * r)
+        announce("bucketed apply", weighted, r)
+        yo = o.apply(p, r)
+        np.testing.assert_allclose(a.apply(p, r), yo, rtol=0, atol=1e-12 * np.abs(yo).max())
+    v = rng.standard_normal((n, 2))
+    pi = a.row_order()
+    ro = np.ascontiguousarray(v[pi])
+    ra = ro.copy()
+    o.residualize(ro, 2, tol=1e-11)
+    announce("bucketed residualize", weighted)
+    a.residualize(ra, 2, F.SolveOpts(tol=1e-11))
+    np.testing.assert_allclose(ra, ro, atol=1e-8 * np.abs(ro).max())
