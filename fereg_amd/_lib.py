@@ -38,7 +38,7 @@ class Term(ctypes.Structure):
 
 
 class SolverConfig(ctypes.Structure):
-    _fields_ = [("small_limit", ctypes.c_uint64), ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+    _fields_ = [("small_limit", ctypes.c_uint64), ("precision", ctypes.c_int32), ("bucket_log2", ctypes.c_int32)]
 
 
 class SolveOptsC(ctypes.Structure):

@@ -68,7 +68,7 @@ class Absorber:
     tensors), g: levels per term, weights: kept-order weights or None.
     """
 
-    def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, verbose=0):
+    def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, bucket_log2=0, verbose=0):
         lib = L.lib()
         self._keep = []
         n = None
@@ -96,7 +96,7 @@ class Absorber:
                 raise L.FeregError(L.FH_INVALID, "weight length differs from the kept sample")
         if n is None:
             raise L.FeregError(L.FH_INVALID, "no terms and no weights: the sample size is unknown")
-        cfg = L.SolverConfig(small_limit, precision, 0)
+        cfg = L.SolverConfig(small_limit, precision, bucket_log2)
         h = ctypes.c_void_p()
         if dev:
             wptr = None

@@ -689,7 +689,7 @@ const char* fh_last_error(void) { return g_last_error.c_str(); }
 void fh_config_default(fh_solver_config* cfg) {
     cfg->small_limit = 4096;
     cfg->precision = FH_PLAIN_FP64;
-    cfg->reserved = 0;
+    cfg->bucket_log2 = 0;
 }
 
 static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept, size_t n, const fh_solver_config* cfg,
@@ -714,6 +714,8 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         if (cfg) c = *cfg;
         a.small_limit = c.small_limit;
         a.precision = c.precision;
+        require(c.bucket_log2 >= 0 && c.bucket_log2 <= 31, "bucket_log2 must be in 0..31");
+        a.bucket_shift = c.bucket_log2 > 0 ? c.bucket_log2 : 31;
         std::vector<DBuf<uint32_t>> up(nterms);
         std::vector<const uint32_t*> dc(nterms);
         for (size_t q = 0; q < nterms; q++) {

@@ -7,7 +7,7 @@
 //   cell_obs_off  u32[M+1]    first row of every cell (absent when every cell is one row)
 //   cell_seg      u32[M]      FE1 level of every cell
 //   per large block b, the level-major transpose of the cells in segment buckets (built once):
-//     the FE1 levels are cut into buckets of 2^17 consecutive segments; cells are ordered by
+//     the FE1 levels are cut into buckets of 2^k consecutive segments; cells are ordered by
 //     (bucket, level, cell), so a pass over one bucket gathers an L2-sized slice of per-segment
 //     vectors. nbk[b] buckets; item i = bucket * g_b + level:
 //     boff        u32[nbk*g_b+1] item offsets;  bperm u32[M] cells in (bucket, level, cell) order
@@ -64,7 +64,7 @@ struct Absorber {
     std::vector<DBuf<uint32_t>> lperm, loff;                // plain level lists, on demand
     std::vector<DBuf<uint32_t>> bperm, boff, lseg, lother;  // bucketed transpose (large blocks)
     std::vector<uint32_t> nbk;                              // buckets per block
-    static constexpr int BUCKET_SHIFT = 17;
+    int bucket_shift = 31;                                  // FE1 levels per bucket = 2^shift
     std::vector<uint32_t> host_pi;
     bool host_pi_valid = false;
     // weights

@@ -206,13 +206,14 @@ __global__ __launch_bounds__(WG) void k_seg_spmv(OpArgs<T> A, const T* __restric
 }
 
 // t_s = sum_{c in s} W_c u_c / n1_s only (the small scatter runs in the level pass): NB
-// (compile-time block count) and U = 4 trips per lane with every index load issued before the
-// gathers, so a lane keeps ~4 x NB independent loads in flight.
+// (compile-time block count) and U = 8 trips per lane with every index load issued before the
+// gathers, so a lane keeps ~8 x NB independent loads in flight (the passes are latency-bound
+// with fewer: 32 resident waves per CU at most).
 template <class T, int RM, int NB>
 __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__ p, int r, int L, T* __restrict__ t,
                                               const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    constexpr int U = 4;
+    constexpr int U = 8;
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int spw = WAVE / L, gl = l / L, li = l % L;
     const size_t nwt = (size_t)gridDim.x * WAVES;
@@ -301,7 +302,7 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
                                                  int L, T* __restrict__ y, T* __restrict__ ypart, T* __restrict__ part,
                                                  const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    constexpr int U = 4;
+    constexpr int U = 8;
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
     SmallRegs<T, RM, NK> sm;

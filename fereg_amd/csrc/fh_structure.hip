@@ -422,7 +422,7 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
     for (int b = 0; b < nb; b++) {
         if (a.blocks[b].small) continue;
         const uint64_t g = a.blocks[b].g;
-        const uint32_t nbk = (uint32_t)((a.g1 + (1ull << Absorber::BUCKET_SHIFT) - 1) >> Absorber::BUCKET_SHIFT);
+        const uint32_t nbk = (uint32_t)((a.g1 + (1ull << a.bucket_shift) - 1) >> a.bucket_shift);
         a.nbk[b] = std::max<uint32_t>(nbk, 1);
         const uint64_t items = (uint64_t)a.nbk[b] * g;
         require(items < (1ull << 32), "too many (bucket, level) items");
@@ -435,7 +435,7 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
             DBuf<uint64_t> keys(M), kalt(M);
             DBuf<uint32_t> valt(M), flag(1);
             hipLaunchKernelGGL(k_bucket_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, a.cell_codes.p, a.cell_seg.p, M, nb,
-                               b, Absorber::BUCKET_SHIFT, g, keys.p);
+                               b, a.bucket_shift, g, keys.p);
             hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, M);
             FH_CHECK_LAUNCH();
             check_sorted_u64(keys.p, M, flag.p, st);
@@ -455,12 +455,13 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         while (L < 64 && L < mean) L <<= 1;
         return L;
     };
-    a.seg_lanes = lanes(a.g1 ? double(M) / double(a.g1) / 4.0 : 1.0);
-    uint64_t glarge = 0;
+    a.seg_lanes = lanes(a.g1 ? double(M) / double(a.g1) / 8.0 : 1.0);
+    uint64_t items = 0;
     int nlarge = 0;
-    for (auto& b : a.blocks)
-        if (!b.small) { glarge += b.g; nlarge++; }
-    a.lvl_lanes = lanes(glarge ? double(M) * nlarge / double(glarge) / 8.0 : 1.0);
+    for (size_t b = 0; b < a.blocks.size(); b++)
+        if (!a.blocks[b].small) { items += (uint64_t)a.nbk[b] * a.blocks[b].g; nlarge++; }
+    // about 8 positions per lane in the level passes
+    a.lvl_lanes = lanes(items ? double(M) * nlarge / double(items) / 8.0 : 1.0);
     FH_HIP(hipStreamSynchronize(st));
 }
 

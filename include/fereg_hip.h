@@ -51,7 +51,7 @@ typedef struct {
 typedef struct {
     uint64_t small_limit; /* levels factored jointly and exactly (default 4096) */
     int32_t precision;    /* FH_PLAIN_FP64 or FH_MIXED */
-    int32_t reserved;
+    int32_t bucket_log2;  /* level passes in buckets of 2^k FE1 levels; 0 = one bucket */
 } fh_solver_config;
 
 /* SolveOpts (absorber.rs:18-24) plus the precision of this solve. */

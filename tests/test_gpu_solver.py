@@ -161,7 +161,7 @@ def test_bucketed_level_lists_match_oracle(weighted):
     gs = [300_000, 5000, 2, 3]
     w = rng.uniform(0.5, 2.0, n) if weighted else None
     o = OracleSolver(codes, gs, weights=w)
-    a = F.Absorber(codes, gs, weights=w)
+    a = F.Absorber(codes, gs, weights=w, bucket_log2=17)
     assert a.info.g1 == 300_000 and a.info.m_small == 5
     for r in (1, 3):
         p = rng.standard_normal(o.info.m * r)
