@@ -8,10 +8,11 @@ python/fereg/_binding.py:129). There is no CPU fallback: without the built libra
 raises.
 """
 from ._lib import (FH_CONVERGENCE, FH_INTERNAL, FH_INTERRUPTED, FH_INVALID, FH_MIXED, FH_OK, FH_OOM,
-                   FH_PLAIN_FP64, FH_TOO_FEW, FeregError, lib, library_path)
+                   FH_PLAIN_FP64, FH_TOO_FEW, FeregError, lib, library_path, memory_stats,
+                   release_cached_memory)
 from .absorber import Absorber, FeCoefs, SolveInfo, SolveOpts
 from .fit import FitResult, feols, fepois
 
 __all__ = ["Absorber", "FeCoefs", "SolveInfo", "SolveOpts", "FitResult", "feols", "fepois", "FeregError", "lib",
            "library_path", "FH_OK", "FH_INTERRUPTED", "FH_INVALID", "FH_CONVERGENCE", "FH_OOM", "FH_TOO_FEW",
-           "FH_INTERNAL", "FH_PLAIN_FP64", "FH_MIXED"]
+           "FH_INTERNAL", "FH_PLAIN_FP64", "FH_MIXED", "memory_stats", "release_cached_memory"]

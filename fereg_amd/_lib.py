@@ -117,6 +117,8 @@ SIGNATURES = [
     ("fh_memcpy_h2d", ctypes.c_int, [vp, vp, ctypes.c_size_t]),
     ("fh_memcpy_d2h", ctypes.c_int, [vp, vp, ctypes.c_size_t]),
     ("fh_device_synchronize", ctypes.c_int, []),
+    ("fh_release_cached_memory", ctypes.c_int, []),
+    ("fh_memory_stats", ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
 ]
 
 _lib = None
@@ -149,3 +151,15 @@ def check(rc):
     if rc != FH_OK:
         msg = lib().fh_last_error()
         raise FeregError(rc, msg.decode() if msg else "")
+
+
+def memory_stats():
+    """(bytes in use, bytes cached) of the library's stream-ordered device pool."""
+    a, b = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    check(lib().fh_memory_stats(ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def release_cached_memory():
+    """Returns the pool's completed cached blocks to the device."""
+    check(lib().fh_release_cached_memory())

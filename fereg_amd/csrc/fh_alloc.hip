@@ -1,0 +1,150 @@
+// Stream-ordered caching pool for the library's device buffers.
+//
+// hipMalloc is slow and hipFree waits for the whole device, so a fit that allocates and frees
+// tens of scratch arrays would spend most of its time in the allocator. Freed blocks are kept
+// per device in a size-ordered free list together with the stream they were last used on and
+// an event recorded on that stream at the free:
+//   - a request on the same stream reuses a block at once (stream order makes it safe);
+//   - a request on another stream reuses a block only once its event has completed.
+// Blocks are rounded to 256 B (below 1 MiB: to the next power of two) or 2 MiB multiples and a
+// block is reused for requests of at least half its size. Memory goes back to HIP on
+// fh_release_cached_memory() or when hipMalloc fails (completed blocks are released and the
+// allocation is tried once more).
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "fh_common.hpp"
+
+namespace fh {
+
+namespace {
+
+struct Block {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+};
+
+struct Pool {
+    std::mutex mu;
+    std::unordered_map<void*, Block> live;
+    std::map<int, std::multimap<size_t, Block>> free_by_dev;
+    size_t cached = 0, in_use = 0;
+};
+
+Pool& pool() {
+    static Pool* p = new Pool();  // never destroyed: HIP may be gone at process exit
+    return *p;
+}
+
+thread_local hipStream_t tls_stream = nullptr;
+
+size_t round_bytes(size_t b) {
+    if (b <= 256) return 256;
+    if (b < (1u << 20)) {
+        size_t r = 256;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    const size_t q = 2u << 20;
+    return (b + q - 1) / q * q;
+}
+
+// Releases every cached block whose last use has completed (caller holds the lock).
+void release_completed(Pool& P) {
+    for (auto& [dev, fl] : P.free_by_dev) {
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        if (prev != dev) (void)hipSetDevice(dev);
+        for (auto it = fl.begin(); it != fl.end();) {
+            Block& b = it->second;
+            if (b.ev && hipEventQuery(b.ev) != hipSuccess) { ++it; continue; }
+            if (b.ev) (void)hipEventDestroy(b.ev);
+            (void)hipFree(b.p);
+            P.cached -= b.bytes;
+            it = fl.erase(it);
+        }
+        if (prev != dev && prev >= 0) (void)hipSetDevice(prev);
+    }
+}
+
+}  // namespace
+
+hipStream_t current_stream() { return tls_stream; }
+
+StreamScope::StreamScope(hipStream_t s) : prev(tls_stream) { tls_stream = s; }
+StreamScope::~StreamScope() { tls_stream = prev; }
+
+void* pool_alloc(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    Pool& P = pool();
+    const size_t rb = round_bytes(bytes);
+    int dev = 0;
+    FH_HIP(hipGetDevice(&dev));
+    const hipStream_t st = tls_stream;
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto& fl = P.free_by_dev[dev];
+    for (auto it = fl.lower_bound(rb); it != fl.end() && it->first <= 2 * rb; ++it) {
+        Block& b = it->second;
+        if (b.stream != st && b.ev && hipEventQuery(b.ev) != hipSuccess) continue;
+        Block blk = b;
+        fl.erase(it);
+        blk.stream = st;
+        P.cached -= blk.bytes;
+        P.in_use += blk.bytes;
+        P.live.emplace(blk.p, blk);
+        return blk.p;
+    }
+    Block blk;
+    blk.bytes = rb;
+    blk.device = dev;
+    blk.stream = st;
+    hipError_t e = hipMalloc(&blk.p, rb);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        release_completed(P);
+        e = hipMalloc(&blk.p, rb);
+    }
+    FH_HIP(e);
+    P.in_use += rb;
+    P.live.emplace(blk.p, blk);
+    return blk.p;
+}
+
+void pool_free(void* p) {
+    if (!p) return;
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.live.find(p);
+    if (it == P.live.end()) return;
+    Block b = it->second;
+    P.live.erase(it);
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (prev != b.device) (void)hipSetDevice(b.device);
+    if (!b.ev) (void)hipEventCreateWithFlags(&b.ev, hipEventDisableTiming);
+    if (b.ev) (void)hipEventRecord(b.ev, b.stream);
+    if (prev != b.device && prev >= 0) (void)hipSetDevice(prev);
+    P.in_use -= b.bytes;
+    P.cached += b.bytes;
+    P.free_by_dev[b.device].emplace(b.bytes, b);
+}
+
+void pool_release_cached() {
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    (void)hipDeviceSynchronize();
+    release_completed(P);
+}
+
+void pool_stats(size_t* in_use, size_t* cached) {
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    if (in_use) *in_use = P.in_use;
+    if (cached) *cached = P.cached;
+}
+
+}  // namespace fh

@@ -344,6 +344,13 @@ struct DeviceGuard {
     }
 };
 
+// The absorber's device is current and pool buffers are ordered on its stream.
+struct AbsScope {
+    DeviceGuard dg;
+    StreamScope ss;
+    explicit AbsScope(const Absorber& a) : dg(a.device), ss(a.st) {}
+};
+
 // ------------------------------------------------------------------ estimation helpers (host, K x K)
 
 // Column-order Cholesky with exclusion of pivots below tol and the inverse over the kept
@@ -707,6 +714,7 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         Absorber& a = h->a;
         FH_HIP(hipGetDevice(&a.device));
         FH_HIP(hipStreamCreateWithFlags(&a.st, hipStreamNonBlocking));
+        StreamScope ss(a.st);
         a.n = n;
         a.nterms = nterms;
         fh_solver_config c;
@@ -791,7 +799,7 @@ const uint32_t* fh_absorber_row_order(fh_absorber* ab) {
     Absorber& a = ab->a;
     if (!a.host_pi_valid) {
         int rc = guard([&] {
-            DeviceGuard g(a.device);
+            AbsScope g(a);
             a.host_pi.resize(a.n);
             d2h(a.host_pi.data(), a.pi.p, a.n, a.st);
             FH_HIP(hipStreamSynchronize(a.st));
@@ -807,7 +815,7 @@ static int set_weights_impl(fh_absorber* ab, const double* w, const fh_ctx* ctx,
     return guard([&] {
         require(ab != nullptr, "null absorber");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         check_interrupt(ctx);
         if (!w) {
             refresh_weights(a, nullptr);
@@ -828,7 +836,7 @@ static int residualize_impl(fh_absorber* ab, double* z, size_t r, const fh_solve
     return guard([&] {
         require(ab != nullptr && o != nullptr, "null argument");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         auto t0 = std::chrono::steady_clock::now();
         const size_t n = a.n, p = a.m + a.g1;
         if (info) *info = fh_solve_info{1, 0, 1, 0, 0.0};
@@ -885,7 +893,7 @@ int fh_absorber_copy_structure(fh_absorber* ab, uint32_t* pi, uint32_t* seg_off,
     return guard([&] {
         require(ab != nullptr, "null absorber");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         if (pi) d2h(pi, a.pi.p, a.n, a.st);
         if (seg_off) d2h(seg_off, a.seg_off.p, a.g1 + 1, a.st);
         if (cc) d2h(cc, a.cell_codes.p, a.M * a.nb, a.st);
@@ -912,7 +920,7 @@ int fh_absorber_copy_level_cells(fh_absorber* ab, size_t b, uint32_t* perm, uint
         require(ab != nullptr, "null absorber");
         Absorber& a = ab->a;
         require(b < a.nb, "block index out of range");
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         ensure_level_cells(a, b);
         d2h(perm, a.lperm[b].p, a.M, a.st);
         d2h(off, a.loff[b].p, a.blocks[b].g + 1, a.st);
@@ -924,7 +932,7 @@ int fh_absorber_copy_weight_state(fh_absorber* ab, double* n1, double* nr, doubl
     return guard([&] {
         require(ab != nullptr, "null absorber");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         if (n1) d2h(n1, a.n1.p, a.g1, a.st);
         if (nr) d2h(nr, a.nr.p, a.m, a.st);
         if (diag) d2h(diag, a.diag.p, a.m, a.st);
@@ -939,7 +947,7 @@ int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int
         require(ab != nullptr && p && y, "null argument");
         require(r >= 1 && r <= 16, "1..16 columns");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         const size_t m = a.m;
         if (m == 0) return;
         if (precision == FH_MIXED) {
@@ -964,7 +972,7 @@ int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t
     return guard([&] {
         require(ab && ms && bytes && r >= 1 && r <= 16 && reps >= 1, "bad argument");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         const size_t m = a.m;
         const double s = precision == FH_MIXED ? 4.0 : 8.0;
         *bytes = 4.0 * a.nb * a.M + 12.0 * a.g1 + 2.0 * s * m * r + (a.unit() ? 0.0 : s * a.M);
@@ -1001,7 +1009,7 @@ int fh_absorber_fit_rows(fh_absorber* ab, const double* coefs, size_t r, size_t 
     return guard([&] {
         require(ab && coefs && out && col < r, "bad argument");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         std::vector<uint32_t> seg(a.g1 + 1), cc(a.M * a.nb), coo;
         d2h(seg.data(), a.seg_off.p, a.g1 + 1, a.st);
         d2h(cc.data(), a.cell_codes.p, a.M * a.nb, a.st);
@@ -1027,7 +1035,7 @@ int fh_absorber_term_values(fh_absorber* ab, const double* coefs, size_t r, size
     return guard([&] {
         require(ab && coefs && out && lens && col < r, "bad argument");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         const size_t nt = a.nterms;
         if (a.fe1 < 0 || a.norm_fe1 < 0) {
             for (size_t q = 0; q < nt; q++) lens[q] = 0;
@@ -1071,7 +1079,7 @@ static int ols_impl(fh_absorber* ab, const double* y, const double* X, size_t K,
     return guard([&] {
         require(ab && y && X && ssc && o && beta && vcov && kept, "null argument");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         const size_t n = a.n;
         if (dev) {
             ols_fit(a, y, X, K, kind, cluster, nested, *ssc, *o, beta, vcov, kept, stats, nullptr);
@@ -1105,7 +1113,7 @@ static int ppml_impl(fh_absorber* ab, const double* y, const double* X, size_t K
     return guard([&] {
         require(ab && y && X && ssc && o && beta && vcov && kept, "null argument");
         Absorber& a = ab->a;
-        DeviceGuard g(a.device);
+        AbsScope g(a);
         const size_t n = a.n;
         if (dev) {
             ppml_fit(a, y, X, K, kind, cluster, nested, *ssc, tol, maxiter, *o, beta, vcov, kept, stats, nullptr);
@@ -1148,6 +1156,12 @@ int fh_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 }
 int fh_device_synchronize(void) {
     return guard([&] { FH_HIP(hipDeviceSynchronize()); });
+}
+int fh_release_cached_memory(void) {
+    return guard([&] { pool_release_cached(); });
+}
+int fh_memory_stats(size_t* in_use, size_t* cached) {
+    return guard([&] { pool_stats(in_use, cached); });
 }
 
 }  // extern "C"

@@ -41,7 +41,23 @@ inline void require(bool ok, const std::string& msg, int rc = FH_INVALID) {
 
 // ------------------------------------------------------------------ device buffers
 
-// Owning device allocation (hipMalloc on the current device).
+// Stream-ordered caching pool (fh_alloc.hip): blocks freed on one stream are reused at once on
+// that stream and on other streams once their last use has completed.
+void* pool_alloc(size_t bytes);  // on the current device, for use on current_stream()
+void pool_free(void* p);
+void pool_release_cached();
+void pool_stats(size_t* in_use, size_t* cached);
+hipStream_t current_stream();
+// Sets the stream that pool allocations of this thread are used on, for the scope's lifetime.
+struct StreamScope {
+    hipStream_t prev;
+    explicit StreamScope(hipStream_t s);
+    ~StreamScope();
+    StreamScope(const StreamScope&) = delete;
+    StreamScope& operator=(const StreamScope&) = delete;
+};
+
+// Owning device allocation from the pool (current device, current_stream()).
 template <class T>
 struct DBuf {
     T* p = nullptr;
@@ -59,14 +75,14 @@ struct DBuf {
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) FH_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        if (count) p = static_cast<T*>(pool_alloc(count * sizeof(T)));
     }
     // Grows to at least `count` elements (contents not preserved).
     void reserve(size_t count) {
         if (count > n) alloc(count);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) pool_free(p);
         p = nullptr;
         n = 0;
     }

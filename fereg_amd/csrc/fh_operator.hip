@@ -39,9 +39,14 @@ void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, d
     const double* wr = a.weighted ? a.w_rows.p : nullptr;
     OpArgs<double> As = A;
     if (small_in_lvl) As.nsb = 0;  // the scatter happens in the level pull
-    const size_t shm = (accl && !small_in_lvl && lds) ? lds_bytes<double>(a, r) : 0;
-    FH_DISPATCH_R(r, hipLaunchKernelGGL((k_rhs_seg<RM>), dim3(grid), dim3(WG), shm, st, As, rows, wr, r, L, b1,
-                                        fl >= 0 ? s.zc.p : nullptr, s.part.p, s.dpart.p, (int)(lds && shm > 0)));
+    if (As.nsb == 0) {
+        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_rhs_seg_reg<RM>), dim3(grid), dim3(WG), 0, st, As, rows, wr, r, L, b1,
+                                            fl >= 0 ? s.zc.p : nullptr, s.dpart.p));
+    } else {
+        const size_t shm = lds ? lds_bytes<double>(a, r) : 0;
+        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_rhs_seg<RM>), dim3(grid), dim3(WG), shm, st, As, rows, wr, r, L, b1,
+                                            fl >= 0 ? s.zc.p : nullptr, s.part.p, s.dpart.p, (int)(lds && shm > 0)));
+    }
     FH_CHECK_LAUNCH();
     if (accl && !small_in_lvl) {
         if (lds)
@@ -104,7 +109,15 @@ void residualize_rows(Absorber& a, double* rows, int r, const double* x, const d
     OpArgs<double> A = op_args_impl<double>(a);
     const unsigned grid = seg_grid(a);
     const int L = (int)a.seg_lanes;
-    FH_DISPATCH_R(r, hipLaunchKernelGGL((k_resid_seg<RM>), dim3(grid), dim3(WG), 0, a.st, A, rows, r, L, x, b1, alpha));
+    if (a.nb <= 4) {
+        FH_DISPATCH_R(r, with_nb0((int)a.nb, [&](auto nbc) {
+            constexpr int NB = decltype(nbc)::value;
+            hipLaunchKernelGGL((k_resid_seg<RM, NB>), dim3(grid), dim3(WG), 0, a.st, A, rows, r, L, x, b1, alpha);
+        }));
+    } else {
+        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_resid_seg_any<RM>), dim3(grid), dim3(WG), 0, a.st, A, rows, r, L, x, b1,
+                                            alpha));
+    }
     FH_CHECK_LAUNCH();
 }
 

@@ -546,8 +546,9 @@ __global__ __launch_bounds__(WG) void k_lvl_pull(const uint32_t* __restrict__ lo
 }
 
 // ---------------------------------------------------------------- back-substitution (FP64)
+// Any block count (runtime loop over the blocks, u gathered twice).
 template <int RM>
-__global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __restrict__ rows, int r, int L,
+__global__ __launch_bounds__(WG) void k_resid_seg_any(OpArgs<double> A, double* __restrict__ rows, int r, int L,
                                                   const double* __restrict__ x, const double* __restrict__ b1,
                                                   double* __restrict__ alpha) {
     const int w = threadIdx.x / WAVE, l = lane_id();
@@ -598,6 +599,163 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
                         if (j < r) rows[i * r + j] -= u[j];
             }
         }
+    }
+}
+
+// Back-substitution with a compile-time block count: U trips per lane with the index loads
+// issued before the gathers; u_c stays in registers between the alpha sum and the row update
+// when a segment fits one trip (the common case), so x is gathered once per cell.
+template <int RM>
+constexpr int resid_trips() { return RM <= 2 ? 8 : (RM <= 4 ? 4 : 2); }
+
+template <int RM, int NB>
+__global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __restrict__ rows, int r, int L,
+                                                  const double* __restrict__ x, const double* __restrict__ b1,
+                                                  double* __restrict__ alpha) {
+    constexpr int U = resid_trips<RM>();
+    const int w = threadIdx.x / WAVE, l = lane_id();
+    const int spw = WAVE / L, gl = l / L, li = l % L;
+    const size_t nwt = (size_t)gridDim.x * WAVES;
+    for (size_t base = ((size_t)blockIdx.x * WAVES + w) * spw; base < A.g1; base += nwt * spw) {
+        const size_t s = base + gl;
+        const bool sv = s < A.g1;
+        const uint32_t c0 = sv ? A.seg_off[s] : 0, c1 = sv ? A.seg_off[s + 1] : 0;
+        const uint32_t tmax = wave_max_u32((c1 - c0 + L - 1) / L);
+        double av[RM], uk[U][RM];
+#pragma unroll
+        for (int j = 0; j < RM; j++) av[j] = 0.0;
+        auto gather = [&](uint32_t it, double (&u)[U][RM], double (&wc)[U]) {
+            uint32_t code[U][NB > 0 ? NB : 1];
+#pragma unroll
+            for (int h = 0; h < U; h++) {
+                const uint32_t c = c0 + (it + h) * L + li;
+                const bool v = c < c1;
+#pragma unroll
+                for (int b = 0; b < NB; b++) code[h][b] = v ? ldnt(&A.cell_codes[(size_t)c * NB + b]) : 0u;
+                wc[h] = v ? (A.cw ? A.cw[c] : 1.0) : 0.0;
+            }
+#pragma unroll
+            for (int h = 0; h < U; h++) {
+                const bool v = c0 + (it + h) * L + li < c1;
+#pragma unroll
+                for (int j = 0; j < RM; j++) u[h][j] = 0.0;
+#pragma unroll
+                for (int b = 0; b < NB; b++) {
+                    const size_t k = (size_t)(A.boff[b] + code[h][b]) * r;
+#pragma unroll
+                    for (int j = 0; j < RM; j++)
+                        if (j < r && v) u[h][j] += x[k + j];
+                }
+            }
+        };
+        for (uint32_t it = 0; it < tmax; it += U) {
+            double wc[U];
+            gather(it, uk, wc);
+#pragma unroll
+            for (int h = 0; h < U; h++)
+#pragma unroll
+                for (int j = 0; j < RM; j++) av[j] += wc[h] * uk[h][j];
+        }
+#pragma unroll
+        for (int j = 0; j < RM; j++) av[j] = gsum(av[j], L);
+        const double inv = sv ? A.inv_n1[s] : 0.0;
+#pragma unroll
+        for (int j = 0; j < RM; j++) av[j] = sv && j < r ? (b1[s * r + j] - av[j]) * inv : 0.0;
+        if (alpha && sv && li == 0)
+#pragma unroll
+            for (int j = 0; j < RM; j++)
+                if (j < r) alpha[s * r + j] = av[j];
+        for (uint32_t it = 0; it < tmax; it += U) {
+            if (tmax > U) {
+                double wc[U];
+                gather(it, uk, wc);
+            }
+#pragma unroll
+            for (int h = 0; h < U; h++) {
+                const uint32_t c = c0 + (it + h) * L + li;
+                if (c < c1) {
+                    const size_t r0 = A.cell_obs_off ? A.cell_obs_off[c] : c;
+                    const size_t r1 = A.cell_obs_off ? A.cell_obs_off[c + 1] : c + 1;
+                    for (size_t i = r0; i < r1; i++)
+#pragma unroll
+                        for (int j = 0; j < RM; j++)
+                            if (j < r) rows[i * r + j] -= uk[h][j] + av[j];
+                }
+            }
+        }
+    }
+}
+
+// rhs segment pass without the small scatter (the scatter runs in the level pull, or there are
+// no small blocks): U trips per lane, cell sums kept in registers for z_c when a segment fits
+// one trip, so every row is read once.
+template <int RM>
+__global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const double* __restrict__ rows,
+                                                    const double* __restrict__ wr, int r, int L, double* __restrict__ b1,
+                                                    double* __restrict__ zc, double* __restrict__ spart) {
+    constexpr int U = resid_trips<RM>();
+    __shared__ double red[WAVES][2 * 16];
+    const int w = threadIdx.x / WAVE, l = lane_id();
+    const int spw = WAVE / L, gl = l / L, li = l % L;
+    double wsq[RM], wsum[RM];
+#pragma unroll
+    for (int j = 0; j < RM; j++) { wsq[j] = 0.0; wsum[j] = 0.0; }
+    const size_t nwt = (size_t)gridDim.x * WAVES;
+    for (size_t base = ((size_t)blockIdx.x * WAVES + w) * spw; base < A.g1; base += nwt * spw) {
+        const size_t s = base + gl;
+        const bool sv = s < A.g1;
+        const uint32_t c0 = sv ? A.seg_off[s] : 0, c1 = sv ? A.seg_off[s + 1] : 0;
+        const uint32_t tmax = wave_max_u32((c1 - c0 + L - 1) / L);
+        double bv[RM], sk[U][RM];
+#pragma unroll
+        for (int j = 0; j < RM; j++) bv[j] = 0.0;
+        for (uint32_t it = 0; it < tmax; it += U) {
+#pragma unroll
+            for (int h = 0; h < U; h++) {
+                const uint32_t c = c0 + (it + h) * L + li;
+#pragma unroll
+                for (int j = 0; j < RM; j++) sk[h][j] = 0.0;
+                if (c < c1) cell_sums<RM>(A, rows, wr, c, r, sk[h], wsq, wsum, true);
+#pragma unroll
+                for (int j = 0; j < RM; j++) bv[j] += sk[h][j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RM; j++) bv[j] = gsum(bv[j], L);
+        if (sv && li == 0)
+#pragma unroll
+            for (int j = 0; j < RM; j++)
+                if (j < r) b1[s * r + j] = bv[j];
+        if (!zc) continue;
+        const double inv = sv ? A.inv_n1[s] : 0.0;
+        for (uint32_t it = 0; it < tmax; it += U) {
+#pragma unroll
+            for (int h = 0; h < U; h++) {
+                const uint32_t c = c0 + (it + h) * L + li;
+                if (c < c1) {
+                    if (tmax > U) {
+                        double d0[RM], d1[RM];
+                        cell_sums<RM>(A, rows, wr, c, r, sk[h], d0, d1, false);
+                    }
+                    const double f = (A.cw ? A.cw[c] : 1.0) * inv;
+#pragma unroll
+                    for (int j = 0; j < RM; j++)
+                        if (j < r) zc[(size_t)c * r + j] = sk[h][j] - f * bv[j];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < RM; j++) {
+        double a = wave_sum(wsq[j]), b = wave_sum(wsum[j]);
+        if (l == 0 && j < r) { red[w][j] = a; red[w][16 + j] = b; }
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)r) {
+        double a = 0.0, b = 0.0;
+        for (int ww = 0; ww < WAVES; ww++) { a += red[ww][threadIdx.x]; b += red[ww][16 + threadIdx.x]; }
+        spart[(size_t)blockIdx.x * 2 * r + threadIdx.x] = a;
+        spart[(size_t)blockIdx.x * 2 * r + r + threadIdx.x] = b;
     }
 }
 
@@ -668,6 +826,11 @@ void with_nb(int nb, F&& f) {
         case 3: f(std::integral_constant<int, 3>{}); break;
         default: f(std::integral_constant<int, 4>{}); break;
     }
+}
+template <class F>
+void with_nb0(int nb, F&& f) {
+    if (nb == 0) f(std::integral_constant<int, 0>{});
+    else with_nb(nb, f);
 }
 template <class F>
 void with_no(int no, F&& f) {

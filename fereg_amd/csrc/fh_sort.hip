@@ -193,6 +193,7 @@ __global__ void sorted_check(const uint64_t* __restrict__ keys, size_t n, uint32
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
                       int end_bit, hipStream_t st) {
     if (n <= 1 || end_bit <= 0) return;
+    StreamScope ss(st);  // scratch is stream-ordered on st
     const uint32_t ntiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
     DBuf<uint32_t> hist((size_t)256 * ntiles);
     uint64_t *ka = keys, *kb = keys_alt;
@@ -210,8 +211,6 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
         FH_HIP(hipMemcpyAsync(keys, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
         FH_HIP(hipMemcpyAsync(vals, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
     }
-    // hist is freed at scope exit: wait for the kernels that read it.
-    FH_HIP(hipStreamSynchronize(st));
 }
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total_dev, hipStream_t st) {
@@ -220,6 +219,7 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* t
         return;
     }
     const size_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
+    StreamScope ss(st);
     DBuf<uint32_t> bsum(nb);
     hipLaunchKernelGGL(scan_reduce, dim3((unsigned)nb), dim3(SC_WG), 0, st, in, n, bsum.p);
     FH_CHECK_LAUNCH();
@@ -227,7 +227,6 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* t
     FH_CHECK_LAUNCH();
     hipLaunchKernelGGL(scan_down, dim3((unsigned)nb), dim3(SC_WG), 0, st, in, out, n, bsum.p);
     FH_CHECK_LAUNCH();
-    FH_HIP(hipStreamSynchronize(st));
 }
 
 void check_sorted_u64(const uint64_t* keys, size_t n, uint32_t* flag_dev, hipStream_t st) {

@@ -446,7 +446,6 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
             hipLaunchKernelGGL(k_level_records, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, a.cell_seg.p,
                                a.cell_codes.p, M, nb, b, a.lseg[b].p, a.lother[b].p);
             FH_CHECK_LAUNCH();
-            FH_HIP(hipStreamSynchronize(st));
         }
     }
     // lanes per segment / per level: the next power of two of the mean length, within 1..64
@@ -462,7 +461,6 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         if (!a.blocks[b].small) { items += (uint64_t)a.nbk[b] * a.blocks[b].g; nlarge++; }
     // about 8 positions per lane in the level passes
     a.lvl_lanes = lanes(items ? double(M) * nlarge / double(items) / 8.0 : 1.0);
-    FH_HIP(hipStreamSynchronize(st));
 }
 
 void ensure_level_cells(Absorber& a, size_t b) {
@@ -486,7 +484,6 @@ void ensure_level_cells(Absorber& a, size_t b) {
     if (read_scalar(flag.p, st) != 0) radix_sort_pairs(keys.p, a.lperm[b].p, kalt.p, valt.p, M, (int)bits_for(g), st);
     hipLaunchKernelGGL(k_offsets_u64, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, (size_t)g, a.loff[b].p);
     FH_CHECK_LAUNCH();
-    FH_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace fh

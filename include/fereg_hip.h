@@ -198,6 +198,11 @@ int fh_device_free(void* p);
 int fh_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int fh_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int fh_device_synchronize(void);
+/* The library keeps freed scratch blocks in a stream-ordered cache (no reference counterpart:
+ * the CPU solver allocates with the system allocator). Returns every cached block whose last
+ * use has completed to the device; in_use/cached report the pool's bytes (either may be null). */
+int fh_release_cached_memory(void);
+int fh_memory_stats(size_t* in_use, size_t* cached);
 
 #ifdef __cplusplus
 }

@@ -176,3 +176,30 @@ def test_bucketed_level_lists_match_oracle(weighted):
     announce("bucketed residualize", weighted)
     a.residualize(ra, 2, F.SolveOpts(tol=1e-11))
     np.testing.assert_allclose(ra, ro, atol=1e-8 * np.abs(ro).max())
+
+
+def test_pool_reuses_and_releases_scratch():
+    """Scratch goes back to the stream-ordered pool when an absorber is closed, is reused by the
+    next absorber (no growth over repeated fits) and is returned to the device on request."""
+    import fereg_amd as F
+    codes, gs = residualize_designs()[2]
+    n = len(codes[0])
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal((n, 2))
+    ref = dense_residualize(v, codes, gs)
+    base_use, _ = F.memory_stats()
+    peaks = []
+    for rep in range(3):
+        a = F.Absorber(codes, gs, precision=F.FH_MIXED)
+        pi = a.row_order()
+        z = np.ascontiguousarray(v[pi])
+        announce("pool rep", rep)
+        a.residualize(z, 2, F.SolveOpts(tol=1e-12))
+        np.testing.assert_allclose(z, ref[pi], atol=1e-9)
+        use, cached = F.memory_stats()
+        peaks.append(use + cached)
+        a.close()
+        assert F.memory_stats()[0] == base_use
+    assert peaks[2] == peaks[1]
+    F.release_cached_memory()
+    assert F.memory_stats()[1] == 0
