@@ -29,6 +29,44 @@ __device__ __forceinline__ T shfl_xor_t(T v, int o) { return __shfl_xor(v, o, WA
 template <class T>
 __device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_load(p); }
 
+// row[j] = src[j] (j < r) for a valid lane, else 0. When r == RM the row is read with 16-byte
+// (or 8-byte) vector loads: one gather instruction per row instead of one per column, which
+// matters because random gathers are bounded by instructions x distinct lines, not by bytes.
+template <class T, int RM>
+__device__ __forceinline__ void ld_row(const T* __restrict__ src, int r, bool v, T (&row)[RM]) {
+#pragma unroll
+    for (int j = 0; j < RM; j++) row[j] = T(0);
+    if (!v) return;
+    if constexpr (std::is_same<T, float>::value && RM % 4 == 0) {
+        if (r == RM) {
+#pragma unroll
+            for (int q = 0; q < RM / 4; q++) {
+                const float4 x = reinterpret_cast<const float4*>(src)[q];
+                row[4 * q] = x.x; row[4 * q + 1] = x.y; row[4 * q + 2] = x.z; row[4 * q + 3] = x.w;
+            }
+            return;
+        }
+    } else if constexpr (std::is_same<T, float>::value && RM == 2) {
+        if (r == 2) {
+            const float2 x = *reinterpret_cast<const float2*>(src);
+            row[0] = x.x; row[1] = x.y;
+            return;
+        }
+    } else if constexpr (std::is_same<T, double>::value && RM % 2 == 0) {
+        if (r == RM) {
+#pragma unroll
+            for (int q = 0; q < RM / 2; q++) {
+                const double2 x = reinterpret_cast<const double2*>(src)[q];
+                row[2 * q] = x.x; row[2 * q + 1] = x.y;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < RM; j++)
+        if (j < r) row[j] = src[j];
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, WAVE));
@@ -245,10 +283,10 @@ __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__
                 for (int j = 0; j < RM; j++) u[j] = T(0);
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
-                    const size_t k = (size_t)(A.boff[b] + code[h][b]) * r;
+                    T row[RM];
+                    ld_row<T, RM>(p + (size_t)(A.boff[b] + code[h][b]) * r, r, v[h], row);
 #pragma unroll
-                    for (int j = 0; j < RM; j++)
-                        if (j < r && v[h]) u[j] += p[k + j];
+                    for (int j = 0; j < RM; j++) u[j] += row[j];
                 }
 #pragma unroll
                 for (int j = 0; j < RM; j++) tv[j] += wc[h] * u[j];
@@ -342,14 +380,16 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
                 for (int j = 0; j < RM; j++) u[j] = pb[j];
 #pragma unroll
                 for (int o = 0; o < NO; o++) {
-                    const size_t k = (size_t)(oo.off[o] + oc[h][o]) * r;
+                    T row[RM];
+                    ld_row<T, RM>(p + (size_t)(oo.off[o] + oc[h][o]) * r, r, pv[h] && o < nother, row);
 #pragma unroll
-                    for (int j = 0; j < RM; j++)
-                        if (j < r && pv[h] && o < nother) u[j] += p[k + j];
+                    for (int j = 0; j < RM; j++) u[j] += row[j];
                 }
+                T tr[RM];
+                ld_row<T, RM>(t + (size_t)sg[h] * r, r, pv[h], tr);
 #pragma unroll
                 for (int j = 0; j < RM; j++) {
-                    z[j] = (pv[h] && j < r) ? wc[h] * (u[j] - t[(size_t)sg[h] * r + j]) : T(0);
+                    z[j] = (pv[h] && j < r) ? wc[h] * (u[j] - tr[j]) : T(0);
                     acc[j] += z[j];
                 }
                 if (NK > 0) {
@@ -641,10 +681,10 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
                 for (int j = 0; j < RM; j++) u[h][j] = 0.0;
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
-                    const size_t k = (size_t)(A.boff[b] + code[h][b]) * r;
+                    double row[RM];
+                    ld_row<double, RM>(x + (size_t)(A.boff[b] + code[h][b]) * r, r, v, row);
 #pragma unroll
-                    for (int j = 0; j < RM; j++)
-                        if (j < r && v) u[h][j] += x[k + j];
+                    for (int j = 0; j < RM; j++) u[h][j] += row[j];
                 }
             }
         };
