@@ -6,17 +6,18 @@
 //   cell_codes    u32[M*nb]   block codes per cell, row-major (cell, block)
 //   cell_obs_off  u32[M+1]    first row of every cell (absent when every cell is one row)
 //   cell_seg      u32[M]      FE1 level of every cell
-//   per large block b, the level-major transpose of the cells in segment buckets (built once):
-//     the FE1 levels are cut into buckets of 2^k consecutive segments; cells are ordered by
-//     (bucket, level, cell), so a pass over one bucket gathers an L2-sized slice of per-segment
-//     vectors. nbk[b] buckets; item i = bucket * g_b + level:
+//   per block b, the level-major transpose of the cells in segment buckets (built once):
+//     the FE1 levels are cut into nbk[b] buckets of 2^bshift[b] consecutive segments (one
+//     bucket unless the block's levels are long: ~2048 cells per item); cells are ordered by
+//     (bucket, level, cell). Item i = bucket * g_b + level:
 //     boff        u32[nbk*g_b+1] item offsets;  bperm u32[M] cells in (bucket, level, cell) order
 //     lseg        u32[M]      FE1 level of the cell at each position
 //     lother      u32[M*(nb-1)] codes of the other blocks at each position
 //     lw          T[M]        cell weight at each position (weighted designs)
 //   lperm/loff: the plain (level, cell) lists of structure.rs:144-153, built on demand.
 //   weight state: cw T[M] (null = unit), inv_n1 T[g1], inv_diag T[m], minv T[ms*ms] (dense
-//   grounded pseudo-inverse of the small super-block).
+//   grounded pseudo-inverse of the small super-block); with one small block of <= 32 levels,
+//   the dense small-row operator D T[ms*g1], C T[ms*(m-ms)], nrs T[ms].
 #pragma once
 
 #include <memory>
@@ -39,8 +40,12 @@ struct WeightArrays {
     DBuf<T> cw;        // M (empty = every cell weight is 1)
     DBuf<T> inv_n1;    // g1
     DBuf<T> inv_diag;  // m (0 on redundant coordinates)
-    std::vector<DBuf<T>> lw;  // per large block, M (empty when unit)
+    std::vector<DBuf<T>> lw;  // per block, M (empty when unit)
     DBuf<T> minv;      // ms*ms
+    // dense small-block operator (Absorber::small_dense): y_S = nr_S p_S + C p_L - D t
+    DBuf<T> dsm;       // D = ms x g1, D[k][s] = sum of W_c over the cells of segment s with small level k
+    DBuf<T> csm;       // C = ms x (m - ms), C[k][l] = sum of W_c over the cells with small level k, large coord l
+    DBuf<T> nrs;       // ms, n_r of the small coordinates
 };
 
 // Per-call scratch of the PCG and operator kernels (grown on demand).
@@ -64,7 +69,9 @@ struct Absorber {
     std::vector<DBuf<uint32_t>> lperm, loff;                // plain level lists, on demand
     std::vector<DBuf<uint32_t>> bperm, boff, lseg, lother;  // bucketed transpose (large blocks)
     std::vector<uint32_t> nbk;                              // buckets per block
-    int bucket_shift = 31;                                  // FE1 levels per bucket = 2^shift
+    std::vector<int> bshift;                                // FE1 levels per bucket = 2^bshift
+    std::vector<uint32_t> blanes;                           // lanes per item in the level passes
+    int bucket_shift = 31;  // configured bucket size of blocks with short levels (31 = one bucket)
     std::vector<uint32_t> host_pi;
     bool host_pi_valid = false;
     // weights
@@ -84,9 +91,13 @@ struct Absorber {
     DBuf<double> status;  // PCG scalars
     DBuf<uint32_t> u32tmp;
 
+    // one small block with few levels: its part of S p is dense products with D and C (fewer
+    // bytes than a level pass over the cells), set by refresh_weights
+    bool small_dense = false;
+
     bool trivial() const { return cell_obs_off.n == 0; }
     bool unit() const { return !weighted && trivial(); }
-    uint32_t seg_lanes = 8, lvl_lanes = 8;  // lanes per segment / per level in the kernels
+    uint32_t seg_lanes = 8;  // lanes per segment in the segment passes
 };
 
 // ---------------------------------------------------------------- structure (fh_structure.hip)

@@ -21,79 +21,29 @@ void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, d
     hipStream_t st = a.st;
     OpArgs<double> A = op_args_impl<double>(a);
     const unsigned grid = seg_grid(a);
-    const size_t accl = a.ms * (size_t)r;
     Scratch<double>& s = a.s64;
-    const int fl = first_large(a);
-    bool small_in_lvl = false;
-    if (accl && fl >= 0) {
-        FH_DISPATCH_R(r, small_in_lvl = with_small_nk<double, RM>(a.ms, [](auto) {}));
-    }
-    const bool lds = lds_bytes<double>(a, r) <= LDS_ACC_MAX;
-    if (accl && !small_in_lvl) {
-        s.part.reserve(std::max<size_t>(lds ? (size_t)grid * accl : accl, 1));
-        if (!lds) FH_HIP(hipMemsetAsync(s.part.p, 0, accl * sizeof(double), st));
-    }
     s.dpart.reserve((size_t)grid * 2 * r);
-    if (fl >= 0) s.zc.reserve(a.M * r);
+    if (a.nb > 0) s.zc.reserve(a.M * r);
     const int L = (int)a.seg_lanes;
     const double* wr = a.weighted ? a.w_rows.p : nullptr;
-    OpArgs<double> As = A;
-    if (small_in_lvl) As.nsb = 0;  // the scatter happens in the level pull
-    if (As.nsb == 0) {
-        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_rhs_seg_reg<RM>), dim3(grid), dim3(WG), 0, st, As, rows, wr, r, L, b1,
-                                            fl >= 0 ? s.zc.p : nullptr, s.dpart.p));
-    } else {
-        const size_t shm = lds ? lds_bytes<double>(a, r) : 0;
-        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_rhs_seg<RM>), dim3(grid), dim3(WG), shm, st, As, rows, wr, r, L, b1,
-                                            fl >= 0 ? s.zc.p : nullptr, s.part.p, s.dpart.p, (int)(lds && shm > 0)));
-    }
+    FH_DISPATCH_R(r, hipLaunchKernelGGL((k_rhs_seg_reg<RM>), dim3(grid), dim3(WG), 0, st, A, rows, wr, r, L, b1,
+                                        a.nb > 0 ? s.zc.p : nullptr, s.dpart.p));
     FH_CHECK_LAUNCH();
-    if (accl && !small_in_lvl) {
-        if (lds)
-            hipLaunchKernelGGL(k_reduce_small<double>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
-                               s.part.p, (size_t)grid, accl, rhs, nullptr);
-        else
-            FH_HIP(hipMemcpyAsync(rhs, s.part.p, accl * sizeof(double), hipMemcpyDeviceToDevice, st));
-        FH_CHECK_LAUNCH();
-    }
-    int nsb = 0;
-    for (auto& bi : a.blocks) nsb += bi.small ? 1 : 0;
     for (size_t b = 0; b < a.nb; b++) {
         const BlockInfo& bi = a.blocks[b];
-        if (bi.small) continue;
-        const OtherOffs oo = other_offs(a, (int)b);
-        const int Ll = (int)a.lvl_lanes;
+        const int Ll = (int)a.blanes[b];
         const uint32_t nbk = a.nbk[b];
         const size_t items = (size_t)nbk * bi.g;
-        const unsigned lg = lvl_grid(a, items);
         double* ypart = nullptr;
         if (nbk > 1) {
             s.ypart.reserve(items * r);
             ypart = s.ypart.p;
         }
-        if ((int)b == fl && small_in_lvl) {
-            s.part.reserve((size_t)lg * accl);
-            FH_DISPATCH_R(r, with_small_nk<double, RM>(a.ms, [&](auto nk) {
-                constexpr int NK = decltype(nk)::value;
-                hipLaunchKernelGGL((k_lvl_pull<RM, NK>), dim3(lg), dim3(WG), 0, st, a.boff[b].p, a.bperm[b].p,
-                                   a.lother[b].p, (int)a.nb - 1, oo, nsb, (size_t)a.ms, items, (uint32_t)bi.g, s.zc.p,
-                                   r, Ll, rhs + bi.off * r, ypart, s.part.p);
-            }));
-            FH_CHECK_LAUNCH();
-            hipLaunchKernelGGL(k_reduce_small<double>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
-                               s.part.p, (size_t)lg, accl, rhs, nullptr);
-        } else {
-            FH_DISPATCH_R(r, hipLaunchKernelGGL((k_lvl_pull<RM, 0>), dim3(lg), dim3(WG), 0, st, a.boff[b].p,
-                                                a.bperm[b].p, a.lother[b].p, (int)a.nb - 1, oo, 0, (size_t)0, items,
-                                                (uint32_t)bi.g, s.zc.p, r, Ll, rhs + bi.off * r, ypart,
-                                                (double*)nullptr));
-        }
+        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_lvl_pull<RM>), dim3(lvl_grid(Ll, items)), dim3(WG), 0, st, a.boff[b].p,
+                                            a.bperm[b].p, items, (uint32_t)bi.g, s.zc.p, r, Ll, rhs + bi.off * r,
+                                            ypart));
         FH_CHECK_LAUNCH();
-        if (ypart) {
-            hipLaunchKernelGGL(k_bucket_reduce<double>, dim3(grid_for(bi.g * r, WG, 8192)), dim3(WG), 0, st, ypart,
-                               nbk, (size_t)bi.g, r, rhs + bi.off * r, (const uint32_t*)nullptr);
-            FH_CHECK_LAUNCH();
-        }
+        if (ypart) bucket_reduce<double>(ypart, nbk, (size_t)bi.g, r, rhs + bi.off * r, nullptr, st);
     }
     // wsq / wsum: fixed-order sum of the workgroup partials on the host
     std::vector<double> h((size_t)grid * 2 * r);

@@ -1,13 +1,14 @@
 // The reduced operator and the row passes on the GPU (operator.rs:894-1179):
-//   S*p   = segment pass (t_s = sum_c W_c u_c / n1_s, small-coordinate scatter of
-//           W_c (u_c - t_s) by wave multisplit into wave-private LDS accumulators) followed by
-//           one level pass per large block over its level-major transpose
-//           (y_l = sum_{c in l} W_c (u_c - t_{s(c)}), no float atomics);
+//   S*p   = segment pass (t_s = sum_c W_c u_c / n1_s) followed by one level pass per block
+//           over its level-major transpose (y_l = sum_{c in l} W_c (u_c - t_{s(c)})); blocks
+//           with long levels are cut into FE1 buckets, so a level's sum is a fixed-order sum of
+//           per-bucket partials (no float atomics anywhere);
 //   rhs   = segment pass over the rows (cell sums, b1, sum w v^2, sum w v; z_c to a scratch)
-//           + small scatter + level pull of z_c for the large blocks;
+//           + one level pull of z_c per block;
 //   back-substitution: alpha_s = (b1_s - sum_c W_c u_c)/n1_s, v_i -= alpha_s + u_c in place.
-// Every pass assigns L lanes to a segment (or level), L = next power of two of the mean length,
-// so cell reads are coalesced; all sums are in a fixed order (run-to-run reproducible).
+// Every pass assigns L lanes to a segment (or level item), L = next power of two of the mean
+// length over the unroll depth, so cell reads are coalesced; all sums are in a fixed order
+// (run-to-run reproducible).
 #pragma once
 #include <algorithm>
 #include <type_traits>
@@ -67,6 +68,14 @@ __device__ __forceinline__ void ld_row(const T* __restrict__ src, int r, bool v,
         if (j < r) row[j] = src[j];
 }
 
+// Unroll depth of the hoisted passes: positions per lane whose gathers are in flight at once,
+// sized so the gathered rows stay within ~64 VGPRs (no spills at 16 FP64 columns).
+template <class T, int RM>
+constexpr int gather_trips() {
+    constexpr int w = RM * (int)sizeof(T) / 4;  // VGPRs per row
+    return w <= 4 ? 8 : (w <= 8 ? 4 : (w <= 16 ? 2 : 1));
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, WAVE));
@@ -78,75 +87,6 @@ __device__ __forceinline__ T gsum(T v, int L) {
     for (int o = L >> 1; o > 0; o >>= 1) v += shfl_xor_t(v, o);
     return v;
 }
-
-// Adds z (r columns) of every valid lane to acc[key*r + j], one wave reduction per distinct key
-// (in lane order: deterministic).
-template <class T, int RM>
-__device__ __forceinline__ void multisplit_add(uint32_t key, bool valid, const T (&z)[RM], int r, T* acc,
-                                               bool global_atomic) {
-    uint64_t pending = __ballot(valid);
-    const int l = lane_id();
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lk = __shfl((int)key, leader, WAVE);
-        const uint64_t m = __ballot(valid && key == lk);
-        const bool mine = (m >> l) & 1ull;
-#pragma unroll
-        for (int j = 0; j < RM; j++) {
-            if (j < r) {
-                T v = mine ? z[j] : T(0);
-                v = wave_sum(v);
-                if (l == leader) {
-                    if (global_atomic) atomicAdd(&acc[(size_t)lk * r + j], v);
-                    else acc[(size_t)lk * r + j] += v;
-                }
-            }
-        }
-        pending &= ~m;
-    }
-}
-
-// Per-lane register accumulators of the small-coordinate scatter: a[k][j] += z[j] for key k,
-// keys < NK. One predicated add per (key, column) and cell; the wave reduction happens once,
-// at the end of the kernel (flush), not per cell.
-template <class T, int RM, int NK>
-struct SmallRegs {
-    T a[NK > 0 ? NK : 1][RM];
-    __device__ __forceinline__ void zero() {
-#pragma unroll
-        for (int k = 0; k < (NK > 0 ? NK : 1); k++)
-#pragma unroll
-            for (int j = 0; j < RM; j++) a[k][j] = T(0);
-    }
-    __device__ __forceinline__ void add(uint32_t key, bool valid, const T (&z)[RM]) {
-#pragma unroll
-        for (int k = 0; k < NK; k++) {
-            const bool hit = valid && key == (uint32_t)k;
-#pragma unroll
-            for (int j = 0; j < RM; j++) a[k][j] += hit ? z[j] : T(0);
-        }
-    }
-    // part[k*r + j] (k < ms, j < r) = sum over the workgroup (fixed order). All threads call it.
-    __device__ void flush(T* __restrict__ part, size_t ms, int r) {
-        __shared__ T red[WAVES][NK > 0 ? NK : 1][RM];
-        const int w = threadIdx.x / WAVE;
-#pragma unroll
-        for (int k = 0; k < NK; k++)
-#pragma unroll
-            for (int j = 0; j < RM; j++) {
-                const T v = wave_sum(a[k][j]);
-                if (lane_id() == 0) red[w][k][j] = v;
-            }
-        __syncthreads();
-        for (int e = threadIdx.x; e < (int)ms * r; e += WG) {
-            const int k = e / r, j = e % r;
-            T v = T(0);
-#pragma unroll
-            for (int ww = 0; ww < WAVES; ww++) v += red[ww][k][j];
-            part[e] = v;
-        }
-    }
-};
 
 template <class T, int RM>
 __device__ __forceinline__ void gather_u(const OpArgs<T>& A, const T* __restrict__ p, uint32_t c, int r, T (&u)[RM]) {
@@ -161,48 +101,27 @@ __device__ __forceinline__ void gather_u(const OpArgs<T>& A, const T* __restrict
 }
 
 // ---------------------------------------------------------------- S*p, segment pass
+// t_s = sum_{c in s} W_c u_c / n1_s for any block count (runtime loop over the blocks).
 template <class T, int RM>
-__global__ __launch_bounds__(WG) void k_seg_spmv(OpArgs<T> A, const T* __restrict__ p, int r, int L,
-                                                 T* __restrict__ t, T* __restrict__ part, int lds_acc,
-                                                 const uint32_t* __restrict__ done) {
+__global__ __launch_bounds__(WG) void k_seg_t_any(OpArgs<T> A, const T* __restrict__ p, int r, int L,
+                                                  T* __restrict__ t, const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    extern __shared__ unsigned char smem_raw[];
-    T* acc = reinterpret_cast<T*>(smem_raw);
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int spw = WAVE / L, gl = l / L, li = l % L;
-    const size_t accl = A.ms * (size_t)r;
-    T* my = lds_acc ? acc + (size_t)w * accl : part;
-    if (lds_acc)
-        for (size_t i = l; i < accl; i += WAVE) my[i] = T(0);
     const size_t nwt = (size_t)gridDim.x * WAVES;
     for (size_t base = ((size_t)blockIdx.x * WAVES + w) * spw; base < A.g1; base += nwt * spw) {
         const size_t s = base + gl;
         const bool sv = s < A.g1;
         const uint32_t c0 = sv ? A.seg_off[s] : 0, c1 = sv ? A.seg_off[s + 1] : 0;
-        const uint32_t tmax = wave_max_u32((c1 - c0 + L - 1) / L);
         T tv[RM];
 #pragma unroll
         for (int j = 0; j < RM; j++) tv[j] = T(0);
-        for (uint32_t it = 0; it < tmax; it += 2) {
-            const uint32_t ca = c0 + it * L + li, cb = ca + L;
-            const bool va = ca < c1, vb = cb < c1;
-            T ua[RM], ub[RM];
+        for (uint32_t c = c0 + li; c < c1; c += L) {
+            T u[RM];
+            gather_u<T, RM>(A, p, c, r, u);
+            const T wc = A.cw ? A.cw[c] : T(1);
 #pragma unroll
-            for (int j = 0; j < RM; j++) { ua[j] = T(0); ub[j] = T(0); }
-            for (int b = 0; b < A.nb; b++) {
-                const size_t ka = va ? (size_t)(A.boff[b] + A.cell_codes[(size_t)ca * A.nb + b]) * r : 0;
-                const size_t kb = vb ? (size_t)(A.boff[b] + A.cell_codes[(size_t)cb * A.nb + b]) * r : 0;
-#pragma unroll
-                for (int j = 0; j < RM; j++)
-                    if (j < r) {
-                        if (va) ua[j] += p[ka + j];
-                        if (vb) ub[j] += p[kb + j];
-                    }
-            }
-            const T wa = va ? (A.cw ? A.cw[ca] : T(1)) : T(0);
-            const T wb = vb ? (A.cw ? A.cw[cb] : T(1)) : T(0);
-#pragma unroll
-            for (int j = 0; j < RM; j++) tv[j] += wa * ua[j] + wb * ub[j];
+            for (int j = 0; j < RM; j++) tv[j] += wc * u[j];
         }
         const T inv = sv ? A.inv_n1[s] : T(0);
 #pragma unroll
@@ -211,47 +130,17 @@ __global__ __launch_bounds__(WG) void k_seg_spmv(OpArgs<T> A, const T* __restric
 #pragma unroll
             for (int j = 0; j < RM; j++)
                 if (j < r) t[s * r + j] = tv[j];
-        if (A.nsb > 0) {
-            for (uint32_t it = 0; it < tmax; it++) {
-                const uint32_t c = c0 + it * L + li;
-                const bool cv = c < c1;
-                T z[RM];
-#pragma unroll
-                for (int j = 0; j < RM; j++) z[j] = T(0);
-                if (cv) {
-                    T u[RM];
-                    gather_u<T, RM>(A, p, c, r, u);
-                    const T wc = A.cw ? A.cw[c] : T(1);
-#pragma unroll
-                    for (int j = 0; j < RM; j++) z[j] = wc * (u[j] - tv[j]);
-                }
-                for (int b = 0; b < A.nsb; b++) {
-                    const uint32_t key = cv ? A.boff[b] + A.cell_codes[(size_t)c * A.nb + b] : 0u;
-                    multisplit_add<T, RM>(key, cv, z, r, my, !lds_acc);
-                }
-            }
-        }
-    }
-    if (lds_acc) {
-        __syncthreads();
-        for (size_t i = threadIdx.x; i < accl; i += WG) {
-            T v = T(0);
-#pragma unroll
-            for (int ww = 0; ww < WAVES; ww++) v += acc[(size_t)ww * accl + i];
-            part[(size_t)blockIdx.x * accl + i] = v;
-        }
     }
 }
 
-// t_s = sum_{c in s} W_c u_c / n1_s only (the small scatter runs in the level pass): NB
-// (compile-time block count) and U = 8 trips per lane with every index load issued before the
+// t_s = sum_{c in s} W_c u_c / n1_s with NB (compile-time block count) and U = 8 trips per lane with every index load issued before the
 // gathers, so a lane keeps ~8 x NB independent loads in flight (the passes are latency-bound
 // with fewer: 32 resident waves per CU at most).
 template <class T, int RM, int NB>
 __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__ p, int r, int L, T* __restrict__ t,
                                               const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    constexpr int U = 8;
+    constexpr int U = gather_trips<T, RM>();
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int spw = WAVE / L, gl = l / L, li = l % L;
     const size_t nwt = (size_t)gridDim.x * WAVES;
@@ -301,26 +190,79 @@ __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__
     }
 }
 
-// y[i] = sum over the workgroup partials part[q*len + i]: one workgroup per entry, strided
-// per-thread sums and a fixed tree (deterministic, no serial chain of loads).
-template <class T>
-__global__ __launch_bounds__(WG) void k_reduce_small(const T* __restrict__ part, size_t nparts, size_t len,
-                                                     T* __restrict__ y, const uint32_t* __restrict__ done) {
+// ---------------------------------------------------------------- S*p, dense small rows
+// Partial sums over a column range of y_S[k] = sum_l C[k][l] p_L[l] - sum_s D[k][s] t_s for the
+// small levels k in chunk blockIdx.y (KC levels): columns are the g1 segments (D, t) followed by
+// the m - ms large coordinates (C, p_L). Each workgroup writes part[blockIdx.x][k][j].
+template <class T, int RM>
+constexpr int small_chunk() { return (sizeof(T) == 8 ? 32 : 64) / RM < 32 ? (sizeof(T) == 8 ? 32 : 64) / RM : 32; }
+
+template <class T, int RM>
+__global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ D, const T* __restrict__ t,
+                                                    const T* __restrict__ C, const T* __restrict__ pL, size_t g1,
+                                                    size_t mL, int ms, int r, T* __restrict__ part,
+                                                    const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    __shared__ T red[WAVES];
-    for (size_t i = blockIdx.x; i < len; i += gridDim.x) {
-        T s = T(0);
-        for (size_t q = threadIdx.x; q < nparts; q += WG) s += part[q * len + i];
-        s = wave_sum(s);
-        if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            T t = T(0);
+    constexpr int KC = small_chunk<T, RM>();
+    __shared__ T red[WAVES][KC][RM];
+    const int k0 = blockIdx.y * KC;
+    T acc[KC][RM];
 #pragma unroll
-            for (int w = 0; w < WAVES; w++) t += red[w];
-            y[i] = t;
+    for (int kk = 0; kk < KC; kk++)
+#pragma unroll
+        for (int j = 0; j < RM; j++) acc[kk][j] = T(0);
+    const size_t nc = g1 + mL;
+    for (size_t col = (size_t)blockIdx.x * WG + threadIdx.x; col < nc; col += (size_t)gridDim.x * WG) {
+        const bool seg = col < g1;
+        const size_t c = seg ? col : col - g1;
+        const T* M = seg ? D : C;
+        const size_t ld = seg ? g1 : mL;
+        T v[RM];
+        ld_row<T, RM>((seg ? t : pL) + c * r, r, true, v);
+        if (seg)
+#pragma unroll
+            for (int j = 0; j < RM; j++) v[j] = -v[j];
+#pragma unroll
+        for (int kk = 0; kk < KC; kk++) {
+            const int k = k0 + kk;
+            if (k < ms) {
+                const T mk = M[(size_t)k * ld + c];
+#pragma unroll
+                for (int j = 0; j < RM; j++) acc[kk][j] += mk * v[j];
+            }
         }
-        __syncthreads();
+    }
+    const int w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int kk = 0; kk < KC; kk++)
+#pragma unroll
+        for (int j = 0; j < RM; j++) {
+            const T sm = wave_sum(acc[kk][j]);
+            if (lane_id() == 0) red[w][kk][j] = sm;
+        }
+    __syncthreads();
+    for (int e = threadIdx.x; e < KC * r; e += WG) {
+        const int kk = e / r, j = e % r;
+        if (k0 + kk >= ms) continue;
+        T sm = T(0);
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ww++) sm += red[ww][kk][j];
+        part[(size_t)blockIdx.x * ms * r + (size_t)(k0 + kk) * r + j] = sm;
+    }
+}
+
+// y_S[k][j] = nr_k p_S[k][j] + sum over the workgroup partials (one wave per entry, fixed order).
+template <class T>
+__global__ __launch_bounds__(WG) void k_small_dense_final(const T* __restrict__ part, unsigned nparts, int ms, int r,
+                                                          const T* __restrict__ nrs, const T* __restrict__ p,
+                                                          T* __restrict__ y, const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    const size_t len = (size_t)ms * r;
+    for (size_t e = (size_t)blockIdx.x * WAVES + threadIdx.x / WAVE; e < len; e += (size_t)gridDim.x * WAVES) {
+        T sm = T(0);
+        for (unsigned q = lane_id(); q < nparts; q += WAVE) sm += part[(size_t)q * len + e];
+        sm = wave_sum(sm);
+        if (lane_id() == 0) y[e] = nrs[e / r] * p[e] + sm;
     }
 }
 
@@ -328,23 +270,21 @@ struct OtherOffs {
     uint32_t off[NB_MAX];
 };
 
-// ---------------------------------------------------------------- S*p, level pass of one large block
-// y_l = sum_{c in l} W_c (u_c - t_{s(c)}) over the level-major records; with NK > 0 the same
-// W_c (u_c - t_s) is also scattered into the nsb small blocks (keys < NK) through per-lane
-// registers, and each workgroup writes its partial small vector to part[blockIdx.x][ms*r].
-template <class T, int RM, int NK, int NO>
+// ---------------------------------------------------------------- S*p, level pass of one block
+// y_l = sum_{c in l} W_c (u_c - t_{s(c)}) over the level-major records of item
+// (bucket, level); with several buckets the item sums go to ypart and k_bucket_reduce adds
+// them in bucket order.
+template <class T, int RM, int NO>
 __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lseg,
                                                  const uint32_t* __restrict__ lother, const T* __restrict__ lw,
-                                                 size_t g, uint32_t gb, uint32_t off_b, OtherOffs oo, int nother, int nsb,
-                                                 size_t ms, const T* __restrict__ p, const T* __restrict__ t, int r,
-                                                 int L, T* __restrict__ y, T* __restrict__ ypart, T* __restrict__ part,
+                                                 size_t g, uint32_t gb, uint32_t off_b, OtherOffs oo, int nother,
+                                                 const T* __restrict__ p, const T* __restrict__ t, int r, int L,
+                                                 T* __restrict__ y, T* __restrict__ ypart,
                                                  const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    constexpr int U = 8;
+    constexpr int U = gather_trips<T, RM>();
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
-    SmallRegs<T, RM, NK> sm;
-    if (NK > 0) sm.zero();
     const size_t nwt = (size_t)gridDim.x * WAVES;
     for (size_t base = ((size_t)blockIdx.x * WAVES + w) * gpw; base < g; base += nwt * gpw) {
         const size_t it_ = base + gl;  // item = bucket * gb + level
@@ -392,12 +332,6 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
                     z[j] = (pv[h] && j < r) ? wc[h] * (u[j] - tr[j]) : T(0);
                     acc[j] += z[j];
                 }
-                if (NK > 0) {
-                    // small blocks come first among the other blocks
-#pragma unroll
-                    for (int sb = 0; sb < NO; sb++)
-                        if (sb < nsb) sm.add(oo.off[sb] + oc[h][sb], pv[h], z);
-                }
             }
         }
 #pragma unroll
@@ -409,10 +343,22 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
                 if (j < r) dst[j] = acc[j];
         }
     }
-    if (NK > 0) sm.flush(part + (size_t)blockIdx.x * ms * r, ms, r);
 }
 
-// y[l] = sum over the buckets of ypart[k*gb + l] (fixed order).
+// y[l] = sum over the buckets of ypart[k*gb + l] (fixed order): one thread per output for a few
+// buckets; one wave per output (lane-strided sums, fixed shuffle tree) for many.
+template <class T>
+__global__ __launch_bounds__(WG) void k_bucket_reduce_wave(const T* __restrict__ ypart, uint32_t nbk, size_t gb, int r,
+                                                           T* __restrict__ y, const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    const size_t nout = gb * r;
+    for (size_t e = (size_t)blockIdx.x * WAVES + threadIdx.x / WAVE; e < nout; e += (size_t)gridDim.x * WAVES) {
+        T s = T(0);
+        for (uint32_t k = lane_id(); k < nbk; k += WAVE) s += ypart[(size_t)k * nout + e];
+        s = wave_sum(s);
+        if (lane_id() == 0) y[e] = s;
+    }
+}
 template <class T>
 __global__ void k_bucket_reduce(const T* __restrict__ ypart, uint32_t nbk, size_t gb, int r, T* __restrict__ y,
                                 const uint32_t* __restrict__ done) {
@@ -448,141 +394,49 @@ __device__ __forceinline__ void cell_sums(const OpArgs<double>& A, const double*
     }
 }
 
+// rhs of a block: pull of z_c along the level-major lists of item (bucket, level).
 template <int RM>
-__global__ __launch_bounds__(WG) void k_rhs_seg(OpArgs<double> A, const double* __restrict__ rows,
-                                                const double* __restrict__ wr, int r, int L, double* __restrict__ b1,
-                                                double* __restrict__ zc, double* __restrict__ part,
-                                                double* __restrict__ spart, int lds_acc) {
-    extern __shared__ unsigned char smem_raw[];
-    double* acc = reinterpret_cast<double*>(smem_raw);
-    __shared__ double red[WAVES][2 * 16];
-    const int w = threadIdx.x / WAVE, l = lane_id();
-    const int spw = WAVE / L, gl = l / L, li = l % L;
-    const size_t accl = A.ms * (size_t)r;
-    double* my = lds_acc ? acc + (size_t)w * accl : part;
-    if (lds_acc)
-        for (size_t i = l; i < accl; i += WAVE) my[i] = 0.0;
-    double wsq[RM], wsum[RM];
-#pragma unroll
-    for (int j = 0; j < RM; j++) { wsq[j] = 0.0; wsum[j] = 0.0; }
-    const size_t nwt = (size_t)gridDim.x * WAVES;
-    for (size_t base = ((size_t)blockIdx.x * WAVES + w) * spw; base < A.g1; base += nwt * spw) {
-        const size_t s = base + gl;
-        const bool sv = s < A.g1;
-        const uint32_t c0 = sv ? A.seg_off[s] : 0, c1 = sv ? A.seg_off[s + 1] : 0;
-        const uint32_t tmax = wave_max_u32((c1 - c0 + L - 1) / L);
-        double bv[RM];
-#pragma unroll
-        for (int j = 0; j < RM; j++) bv[j] = 0.0;
-        for (uint32_t it = 0; it < tmax; it++) {
-            const uint32_t c = c0 + it * L + li;
-            if (c < c1) {
-                double sc[RM];
-                cell_sums<RM>(A, rows, wr, c, r, sc, wsq, wsum, true);
-#pragma unroll
-                for (int j = 0; j < RM; j++) bv[j] += sc[j];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < RM; j++) bv[j] = gsum(bv[j], L);
-        if (sv && li == 0)
-#pragma unroll
-            for (int j = 0; j < RM; j++)
-                if (j < r) b1[s * r + j] = bv[j];
-        const double inv = sv ? A.inv_n1[s] : 0.0;
-        for (uint32_t it = 0; it < tmax; it++) {
-            const uint32_t c = c0 + it * L + li;
-            const bool cv = c < c1;
-            double z[RM];
-#pragma unroll
-            for (int j = 0; j < RM; j++) z[j] = 0.0;
-            if (cv) {
-                double sc[RM], d0[RM], d1[RM];
-                cell_sums<RM>(A, rows, wr, c, r, sc, d0, d1, false);
-                const double f = (A.cw ? A.cw[c] : 1.0) * inv;
-#pragma unroll
-                for (int j = 0; j < RM; j++) z[j] = sc[j] - f * bv[j];
-                if (zc)
-#pragma unroll
-                    for (int j = 0; j < RM; j++)
-                        if (j < r) zc[(size_t)c * r + j] = z[j];
-            }
-            for (int b = 0; b < A.nsb; b++) {
-                const uint32_t key = cv ? A.boff[b] + A.cell_codes[(size_t)c * A.nb + b] : 0u;
-                multisplit_add<double, RM>(key, cv, z, r, my, !lds_acc);
-            }
-        }
-    }
-    // per-workgroup sums of w v^2 and w v
-#pragma unroll
-    for (int j = 0; j < RM; j++) {
-        double a = wave_sum(wsq[j]), b = wave_sum(wsum[j]);
-        if (l == 0 && j < r) { red[w][j] = a; red[w][16 + j] = b; }
-    }
-    __syncthreads();
-    if (threadIdx.x < (unsigned)r) {
-        double a = 0.0, b = 0.0;
-        for (int ww = 0; ww < WAVES; ww++) { a += red[ww][threadIdx.x]; b += red[ww][16 + threadIdx.x]; }
-        spart[(size_t)blockIdx.x * 2 * r + threadIdx.x] = a;
-        spart[(size_t)blockIdx.x * 2 * r + r + threadIdx.x] = b;
-    }
-    if (lds_acc) {
-        for (size_t i = threadIdx.x; i < accl; i += WG) {
-            double v = 0.0;
-#pragma unroll
-            for (int ww = 0; ww < WAVES; ww++) v += acc[(size_t)ww * accl + i];
-            part[(size_t)blockIdx.x * accl + i] = v;
-        }
-    }
-}
-
-// rhs of a large block: pull of z_c along the level lists; with NK > 0 z_c is also scattered
-// into the small blocks (keys from the level-major records) as in k_lvl_spmv.
-template <int RM, int NK>
 __global__ __launch_bounds__(WG) void k_lvl_pull(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lperm,
-                                                 const uint32_t* __restrict__ lother, int nother, OtherOffs oo, int nsb,
-                                                 size_t ms, size_t g, uint32_t gb, const double* __restrict__ zc, int r,
-                                                 int L, double* __restrict__ out, double* __restrict__ ypart,
-                                                 double* __restrict__ part) {
+                                                 size_t g, uint32_t gb, const double* __restrict__ zc, int r, int L,
+                                                 double* __restrict__ out, double* __restrict__ ypart) {
+    constexpr int U = gather_trips<double, RM>();
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
-    SmallRegs<double, RM, NK> sm;
-    if (NK > 0) sm.zero();
     const size_t nwt = (size_t)gridDim.x * WAVES;
     for (size_t base = ((size_t)blockIdx.x * WAVES + w) * gpw; base < g; base += nwt * gpw) {
-        const size_t lv = base + gl;  // item = bucket * gb + level
-        const bool valid = lv < g;
-        const uint32_t j0 = valid ? loff[lv] : 0, j1 = valid ? loff[lv + 1] : 0;
+        const size_t it_ = base + gl;  // item = bucket * gb + level
+        const bool valid = it_ < g;
+        const uint32_t j0 = valid ? loff[it_] : 0, j1 = valid ? loff[it_ + 1] : 0;
         const uint32_t tmax = wave_max_u32((j1 - j0 + L - 1) / L);
         double acc[RM];
 #pragma unroll
         for (int j = 0; j < RM; j++) acc[j] = 0.0;
-        for (uint32_t it = 0; it < tmax; it++) {
-            const uint32_t pos = j0 + it * L + li;
-            const bool pv = pos < j1;
-            double z[RM];
-            const size_t c = pv ? lperm[pos] : 0;
+        for (uint32_t it = 0; it < tmax; it += U) {
+            uint32_t c[U];
+            bool pv[U];
 #pragma unroll
-            for (int j = 0; j < RM; j++) {
-                z[j] = (pv && j < r) ? zc[c * r + j] : 0.0;
-                acc[j] += z[j];
+            for (int h = 0; h < U; h++) {
+                const uint32_t pos = j0 + (it + h) * L + li;
+                pv[h] = pos < j1;
+                c[h] = pv[h] ? ldnt(&lperm[pos]) : 0u;
             }
-            if (NK > 0)
-                for (int sb = 0; sb < nsb; sb++) {
-                    const uint32_t code = pv ? lother[(size_t)pos * nother + sb] : 0u;
-                    sm.add(oo.off[sb] + code, pv, z);
-                }
+#pragma unroll
+            for (int h = 0; h < U; h++) {
+                double z[RM];
+                ld_row<double, RM>(zc + (size_t)c[h] * r, r, pv[h], z);
+#pragma unroll
+                for (int j = 0; j < RM; j++) acc[j] += z[j];
+            }
         }
 #pragma unroll
         for (int j = 0; j < RM; j++) acc[j] = gsum(acc[j], L);
         if (valid && li == 0) {
-            double* dst = ypart ? ypart + lv * r : out + (lv % gb) * r;
+            double* dst = ypart ? ypart + it_ * r : out + (it_ % gb) * r;
 #pragma unroll
             for (int j = 0; j < RM; j++)
                 if (j < r) dst[j] = acc[j];
         }
     }
-    if (NK > 0) sm.flush(part + (size_t)blockIdx.x * ms * r, ms, r);
 }
 
 // ---------------------------------------------------------------- back-substitution (FP64)
@@ -645,14 +499,12 @@ __global__ __launch_bounds__(WG) void k_resid_seg_any(OpArgs<double> A, double* 
 // Back-substitution with a compile-time block count: U trips per lane with the index loads
 // issued before the gathers; u_c stays in registers between the alpha sum and the row update
 // when a segment fits one trip (the common case), so x is gathered once per cell.
-template <int RM>
-constexpr int resid_trips() { return RM <= 2 ? 8 : (RM <= 4 ? 4 : 2); }
 
 template <int RM, int NB>
 __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __restrict__ rows, int r, int L,
                                                   const double* __restrict__ x, const double* __restrict__ b1,
                                                   double* __restrict__ alpha) {
-    constexpr int U = resid_trips<RM>();
+    constexpr int U = gather_trips<double, RM>();
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int spw = WAVE / L, gl = l / L, li = l % L;
     const size_t nwt = (size_t)gridDim.x * WAVES;
@@ -733,7 +585,7 @@ template <int RM>
 __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const double* __restrict__ rows,
                                                     const double* __restrict__ wr, int r, int L, double* __restrict__ b1,
                                                     double* __restrict__ zc, double* __restrict__ spart) {
-    constexpr int U = resid_trips<RM>();
+    constexpr int U = gather_trips<double, RM>();
     __shared__ double red[WAVES][2 * 16];
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int spw = WAVE / L, gl = l / L, li = l % L;
@@ -799,12 +651,6 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
     }
 }
 
-template <class T>
-size_t lds_bytes(const Absorber& a, int r) {
-    return (size_t)WAVES * a.ms * r * sizeof(T);
-}
-constexpr size_t LDS_ACC_MAX = 96 * 1024;
-
 inline unsigned seg_grid(const Absorber& a) {
     const size_t spw = WAVE / a.seg_lanes;
     return grid_for(a.g1, (unsigned)(spw * WAVES), 1u << 16);
@@ -844,20 +690,6 @@ inline OpArgs<T> op_args_impl(const Absorber& a) {
         else { constexpr int RM = 16; __VA_ARGS__; }               \
     } while (0)
 
-// Register budget of the small-coordinate accumulators (bytes per lane).
-constexpr int SMALL_REG_BYTES = 576;
-template <class T, int RM, int NK>
-constexpr bool small_fits() { return NK * RM * (int)sizeof(T) <= SMALL_REG_BYTES; }
-
-// Calls f(std::integral_constant<int, NK>) with the smallest NK in {16, 32} >= ms whose
-// accumulators fit the register budget; returns false when none does.
-template <class T, int RM, class F>
-bool with_small_nk(size_t ms, F&& f) {
-    if constexpr (small_fits<T, RM, 16>()) if (ms <= 16) { f(std::integral_constant<int, 16>{}); return true; }
-    if constexpr (small_fits<T, RM, 32>()) if (ms <= 32) { f(std::integral_constant<int, 32>{}); return true; }
-    return false;
-}
-
 template <class F>
 void with_nb(int nb, F&& f) {
     switch (nb) {
@@ -883,15 +715,6 @@ void with_no(int no, F&& f) {
     }
 }
 
-template <class T>
-DBuf<T>& ypart_buf(Scratch<T>& s) { return s.ypart; }
-
-// Index of the first large block (-1 when every block is small).
-inline int first_large(const Absorber& a) {
-    for (size_t b = 0; b < a.nb; b++)
-        if (!a.blocks[b].small) return (int)b;
-    return -1;
-}
 
 inline OtherOffs other_offs(const Absorber& a, int b) {
     OtherOffs oo{};
@@ -901,100 +724,78 @@ inline OtherOffs other_offs(const Absorber& a, int b) {
     return oo;
 }
 
-inline unsigned lvl_grid(const Absorber& a, size_t g) {
-    const int Ll = (int)a.lvl_lanes;
-    return grid_for(g, (unsigned)((WAVE / Ll) * WAVES), 1u << 20);
+template <class T>
+void bucket_reduce(const T* ypart, uint32_t nbk, size_t gb, int r, T* y, const uint32_t* done, hipStream_t st) {
+    if (nbk <= 32)
+        hipLaunchKernelGGL(k_bucket_reduce<T>, dim3(grid_for(gb * r, WG, 8192)), dim3(WG), 0, st, ypart, nbk, gb, r, y,
+                           done);
+    else
+        hipLaunchKernelGGL(k_bucket_reduce_wave<T>, dim3(grid_for(gb * r, WAVES, 8192)), dim3(WG), 0, st, ypart, nbk,
+                           gb, r, y, done);
+    FH_CHECK_LAUNCH();
+}
+
+inline unsigned lvl_grid(int L, size_t items) {
+    return grid_for(items, (unsigned)((WAVE / L) * WAVES), 1u << 20);
 }
 
 template <class T>
 void spmv_impl(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st) {
     require(r >= 1 && r <= 16, "spmv: 1..16 columns");
     OpArgs<T> A = op_args_impl<T>(a);
-    const size_t accl = a.ms * (size_t)r;
-    const int fl = first_large(a);
     const int nb = (int)a.nb;
+    if (nb == 0) return;
     const WeightArrays<T>* W;
     if constexpr (std::is_same<T, double>::value) W = &a.w64; else W = &a.w32;
     s.t.reserve(std::max<size_t>(a.g1 * r, 1));
     const int L = (int)a.seg_lanes;
     const unsigned grid = seg_grid(a);
-    // Small scatter inside the first large block's level pass (per-lane registers) when the
-    // accumulators fit; otherwise in the segment pass (wave multisplit).
-    bool small_in_lvl = false;
-    if (accl && fl >= 0) {
-        FH_DISPATCH_R(r, small_in_lvl = with_small_nk<T, RM>(a.ms, [](auto) {}));
-    }
-    if (accl && !small_in_lvl) {
-        const bool lds = lds_bytes<T>(a, r) <= LDS_ACC_MAX;
-        s.part.reserve(std::max<size_t>(lds ? (size_t)grid * accl : accl, 1));
-        if (!lds) FH_HIP(hipMemsetAsync(s.part.p, 0, accl * sizeof(T), st));
-        const size_t shm = lds ? lds_bytes<T>(a, r) : 0;
-        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_seg_spmv<T, RM>), dim3(grid), dim3(WG), shm, st, A, p, r, L, s.t.p,
-                                            s.part.p, (int)lds, done));
-        FH_CHECK_LAUNCH();
-        if (lds)
-            hipLaunchKernelGGL(k_reduce_small<T>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
-                               s.part.p, (size_t)grid, accl, y, done);
-        else
-            FH_HIP(hipMemcpyAsync(y, s.part.p, accl * sizeof(T), hipMemcpyDeviceToDevice, st));
-        FH_CHECK_LAUNCH();
-    } else if (nb >= 1 && nb <= 4) {
+    if (nb <= 4) {
         FH_DISPATCH_R(r, with_nb(nb, [&](auto nbc) {
             constexpr int NB = decltype(nbc)::value;
             hipLaunchKernelGGL((k_seg_t<T, RM, NB>), dim3(grid), dim3(WG), 0, st, A, p, r, L, s.t.p, done);
         }));
-        FH_CHECK_LAUNCH();
     } else {
-        OpArgs<T> At = A;
-        At.nsb = 0;  // t only
-        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_seg_spmv<T, RM>), dim3(grid), dim3(WG), 0, st, At, p, r, L, s.t.p,
-                                            (T*)nullptr, 0, done));
+        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_seg_t_any<T, RM>), dim3(grid), dim3(WG), 0, st, A, p, r, L, s.t.p, done));
+    }
+    FH_CHECK_LAUNCH();
+    if (a.small_dense) {
+        const size_t ms = a.ms, mL = a.m - a.ms;
+        unsigned nwg = 0, nch = 0;
+        FH_DISPATCH_R(r, {
+            constexpr int KC = small_chunk<T, RM>();
+            nch = (unsigned)((ms + KC - 1) / KC);
+            nwg = grid_for(a.g1 + mL, WG * 4, 512);
+            s.part.reserve((size_t)nwg * ms * r);
+            hipLaunchKernelGGL((k_small_dense<T, RM>), dim3(nwg, nch), dim3(WG), 0, st, W->dsm.p, s.t.p, W->csm.p,
+                               p + ms * r, a.g1, mL, (int)ms, r, s.part.p, done);
+        });
+        FH_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_small_dense_final<T>, dim3(grid_for(ms * r, WAVES, 1024)), dim3(WG), 0, st, s.part.p, nwg,
+                           (int)ms, r, W->nrs.p, p, y, done);
         FH_CHECK_LAUNCH();
     }
-    int nsb = 0;
-    for (auto& bi : a.blocks) nsb += bi.small ? 1 : 0;
     for (int b = 0; b < nb; b++) {
         const BlockInfo& bi = a.blocks[b];
-        if (bi.small) continue;
+        if (bi.small && a.small_dense) continue;
         const OtherOffs oo = other_offs(a, b);
-        const int Ll = (int)a.lvl_lanes;
+        const int Ll = (int)a.blanes[b];
         const uint32_t nbk = a.nbk[b];
         const size_t items = (size_t)nbk * bi.g;
-        const unsigned lg = lvl_grid(a, items);
         const T* lw = a.unit() ? nullptr : W->lw[b].p;
         T* ypart = nullptr;
         if (nbk > 1) {
-            ypart_buf<T>(s).reserve(items * r);
-            ypart = ypart_buf<T>(s).p;
+            s.ypart.reserve(items * r);
+            ypart = s.ypart.p;
         }
-        if (b == fl && small_in_lvl) {
-            s.part.reserve((size_t)lg * accl);
-            FH_DISPATCH_R(r, with_small_nk<T, RM>(a.ms, [&](auto nk) {
-                constexpr int NK = decltype(nk)::value;
-                with_no(nb - 1, [&](auto noc) {
-                    constexpr int NO = decltype(noc)::value;
-                    hipLaunchKernelGGL((k_lvl_spmv<T, RM, NK, NO>), dim3(lg), dim3(WG), 0, st, a.boff[b].p,
-                                       a.lseg[b].p, a.lother[b].p, lw, items, (uint32_t)bi.g, (uint32_t)bi.off, oo,
-                                       nb - 1, nsb, (size_t)a.ms, p, s.t.p, r, Ll, y, ypart, s.part.p, done);
-                });
-            }));
-            FH_CHECK_LAUNCH();
-            hipLaunchKernelGGL(k_reduce_small<T>, dim3((unsigned)std::min<size_t>(accl, 8192)), dim3(WG), 0, st,
-                               s.part.p, (size_t)lg, accl, y, done);
-        } else {
-            FH_DISPATCH_R(r, with_no(nb - 1, [&](auto noc) {
-                constexpr int NO = decltype(noc)::value;
-                hipLaunchKernelGGL((k_lvl_spmv<T, RM, 0, NO>), dim3(lg), dim3(WG), 0, st, a.boff[b].p, a.lseg[b].p,
-                                   a.lother[b].p, lw, items, (uint32_t)bi.g, (uint32_t)bi.off, oo, nb - 1, 0, (size_t)0,
-                                   p, s.t.p, r, Ll, y, ypart, (T*)nullptr, done);
-            }));
-        }
+        FH_DISPATCH_R(r, with_no(nb - 1, [&](auto noc) {
+            constexpr int NO = decltype(noc)::value;
+            hipLaunchKernelGGL((k_lvl_spmv<T, RM, NO>), dim3(lvl_grid(Ll, items)), dim3(WG), 0, st, a.boff[b].p,
+                               a.lseg[b].p, a.lother[b].p, lw, items, (uint32_t)bi.g, (uint32_t)bi.off, oo, nb - 1, p,
+                               s.t.p, r, Ll, y, ypart, done);
+        }));
         FH_CHECK_LAUNCH();
-        if (ypart) {
-            hipLaunchKernelGGL(k_bucket_reduce<T>, dim3(grid_for(bi.g * r, WG, 8192)), dim3(WG), 0, st, ypart, nbk,
-                               (size_t)bi.g, r, y + bi.off * r, done);
-            FH_CHECK_LAUNCH();
-        }
+        if (ypart) bucket_reduce<T>(ypart, nbk, (size_t)bi.g, r, y + bi.off * r, done, st);
     }
 }
 }  // namespace fh

@@ -16,7 +16,7 @@ constexpr int RS_ITEMS = 16;
 constexpr int RS_WAVES = RS_WG / WAVE;
 constexpr int RS_TILE = RS_WG * RS_ITEMS;
 
-__global__ __launch_bounds__(RS_WG) void rs_upsweep(const uint64_t* __restrict__ keys, size_t n, int shift,
+__global__ __launch_bounds__(RS_WG) void rs_upsweep(const uint64_t* __restrict__ keys, size_t n, int shift, uint32_t dmask,
                                                     uint32_t* __restrict__ hist, uint32_t ntiles) {
     __shared__ uint32_t cnt[RS_WAVES][256];
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_WG) (&cnt[0][0])[i] = 0;
@@ -26,7 +26,7 @@ __global__ __launch_bounds__(RS_WG) void rs_upsweep(const uint64_t* __restrict__
     for (int i = 0; i < RS_ITEMS; i++) {
         size_t idx = base + (size_t)i * WAVE + l;
         bool valid = idx < n;
-        uint32_t d = valid ? (uint32_t)(keys[idx] >> shift) & 0xFFu : 0u;
+        uint32_t d = valid ? (uint32_t)(keys[idx] >> shift) & dmask : 0u;
         uint64_t act = __ballot(valid);
         uint64_t m = match_key(d, 8, act);
         if (valid && (m & lanemask_lt()) == 0) cnt[w][d] += (uint32_t)__popcll(m);
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(RS_WG) void rs_upsweep(const uint64_t* __restrict__
 
 __global__ __launch_bounds__(RS_WG) void rs_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                      size_t n, int shift, const uint32_t* __restrict__ hist,
+                                                      size_t n, int shift, uint32_t dmask, const uint32_t* __restrict__ hist,
                                                       uint32_t ntiles) {
     __shared__ uint32_t cnt[RS_WAVES][256];
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_WG) (&cnt[0][0])[i] = 0;
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(RS_WG) void rs_downsweep(const uint64_t* __restrict
     for (int i = 0; i < RS_ITEMS; i++) {
         size_t idx = base + (size_t)i * WAVE + l;
         bool valid = idx < n;
-        uint32_t d = (uint32_t)(k[i] >> shift) & 0xFFu;
+        uint32_t d = (uint32_t)(k[i] >> shift) & dmask;
         uint64_t act = __ballot(valid);
         uint64_t m = match_key(d, 8, act);
         uint32_t c = cnt[w][d];
@@ -185,7 +185,13 @@ __global__ void sorted_check(const uint64_t* __restrict__ keys, size_t n, uint32
     size_t stride = (size_t)gridDim.x * blockDim.x;
     bool bad = false;
     for (; i < n; i += stride) bad |= keys[i] < keys[i - 1];
-    if (__ballot(bad) != 0 && lane_id() == 0) atomicOr(flag, 1u);
+    // one flag write per workgroup (a write per wave contends on one address)
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (__ballot(bad) != 0 && lane_id() == 0) atomicOr(&any, 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && any) atomicOr(flag, 1u);
 }
 
 }  // namespace
@@ -199,10 +205,13 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
     uint64_t *ka = keys, *kb = keys_alt;
     uint32_t *va = vals, *vb = vals_alt;
     for (int shift = 0; shift < end_bit; shift += 8) {
-        hipLaunchKernelGGL(rs_upsweep, dim3(ntiles), dim3(RS_WG), 0, st, ka, n, shift, hist.p, ntiles);
+        // digits beyond end_bit are masked: bits above it may carry payload
+        const uint32_t dmask = end_bit - shift >= 8 ? 0xFFu : ((1u << (end_bit - shift)) - 1u);
+        hipLaunchKernelGGL(rs_upsweep, dim3(ntiles), dim3(RS_WG), 0, st, ka, n, shift, dmask, hist.p, ntiles);
         FH_CHECK_LAUNCH();
         exclusive_scan_u32(hist.p, hist.p, (size_t)256 * ntiles, nullptr, st);
-        hipLaunchKernelGGL(rs_downsweep, dim3(ntiles), dim3(RS_WG), 0, st, ka, va, kb, vb, n, shift, hist.p, ntiles);
+        hipLaunchKernelGGL(rs_downsweep, dim3(ntiles), dim3(RS_WG), 0, st, ka, va, kb, vb, n, shift, dmask, hist.p,
+                           ntiles);
         FH_CHECK_LAUNCH();
         std::swap(ka, kb);
         std::swap(va, vb);

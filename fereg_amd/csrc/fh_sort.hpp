@@ -6,8 +6,9 @@
 
 namespace fh {
 
-// Stable sort of (keys, vals) by key bits [0, end_bit). Sorted data is returned in keys/vals
-// (the alternate buffers are scratch of the same sizes).
+// Stable sort of (keys, vals) by key bits [0, end_bit); bits at and above end_bit are ignored
+// (they may carry payload that moves with the key). Sorted data is returned in keys/vals (the
+// alternate buffers are scratch of the same sizes).
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
                       int end_bit, hipStream_t st);
 

@@ -6,6 +6,7 @@
 //   coordinates with a per-wave LDS bitmap, and the level-major transpose of every large block
 //   (a second radix sort of the cells by code, structure.rs:144-153).
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 
 #include "fh_internal.hpp"
@@ -26,13 +27,26 @@ __global__ void k_hist(const uint32_t* __restrict__ codes, size_t n, uint32_t* _
         atomicAdd(&cnt[codes[i]], 1u);
 }
 
+// *out += the workgroup's sum of v (one atomic per workgroup: atomics from every wave on one
+// address serialize). Every thread calls it; blockDim.x <= 1024.
+__device__ void block_add_u64(unsigned long long v, unsigned long long* out) {
+    __shared__ unsigned long long part[1024 / WAVE];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    if (lane_id() == 0) part[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (unsigned w = 0; w < blockDim.x / WAVE; w++) t += part[w];
+        if (t) atomicAdd(out, t);
+    }
+}
+
 __global__ void k_sumsq(const uint32_t* __restrict__ v, size_t g, unsigned long long* out) {
     unsigned long long s = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < g; i += (size_t)gridDim.x * blockDim.x)
         s += (unsigned long long)v[i] * v[i];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, WAVE);
-    if (lane_id() == 0) atomicAdd(out, s);
+    block_add_u64(s, out);
 }
 
 // key[i] = packed fields of row vals[i] (fields of the word: most significant first).
@@ -92,10 +106,11 @@ __global__ void k_offsets(const uint32_t* __restrict__ key, size_t M, size_t g, 
     }
 }
 
-__global__ void k_offsets_u64(const uint64_t* __restrict__ key, size_t M, size_t g, uint32_t* __restrict__ off) {
+__global__ void k_offsets_u64(const uint64_t* __restrict__ key, size_t M, size_t g, uint32_t* __restrict__ off,
+                              uint64_t mask = ~0ull) {
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
-        uint64_t s = key[c];
-        int64_t prev = c ? (int64_t)key[c - 1] : -1;
+        uint64_t s = key[c] & mask;
+        int64_t prev = c ? (int64_t)(key[c - 1] & mask) : -1;
         for (int64_t k = prev + 1; k <= (int64_t)s; k++) off[k] = (uint32_t)c;
         if (c == M - 1)
             for (size_t k = (size_t)s + 1; k <= g; k++) off[k] = (uint32_t)M;
@@ -130,23 +145,31 @@ __global__ __launch_bounds__(SW_WG) void k_small_work(const uint32_t* __restrict
         acc += (unsigned long long)d * d;
         __builtin_amdgcn_wave_barrier();
     }
-    if (l == 0) atomicAdd(out, acc);
+    if (l != 0) acc = 0;  // every lane holds the wave's sum
+    block_add_u64(acc, out);
 }
 
 // One small block (block 0, codes ascending within a segment): distinct = runs of equal codes.
+// L lanes per segment read its cells coalesced; a cell starts a run when it is the segment's
+// first or its code differs from the previous cell's.
 __global__ void k_small_work1(const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cell_codes, size_t g1,
-                              int nb, unsigned long long* out) {
+                              int nb, int L, unsigned long long* out) {
+    const int l = lane_id();
+    const int spw = WAVE / L, gl = l / L, li = l % L;
+    const size_t nwaves = (size_t)gridDim.x * (blockDim.x / WAVE);
     unsigned long long acc = 0;
-    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t c0 = seg_off[s], c1 = seg_off[s + 1];
-        unsigned long long d = 0;
-        for (uint32_t c = c0; c < c1; c++)
-            d += (c == c0 || cell_codes[(size_t)c * nb] != cell_codes[(size_t)(c - 1) * nb]) ? 1 : 0;
-        acc += d * d;
+    for (size_t base = ((size_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * spw; base < g1;
+         base += nwaves * spw) {
+        const size_t s = base + gl;
+        const bool sv = s < g1;
+        const uint32_t c0 = sv ? seg_off[s] : 0, c1 = sv ? seg_off[s + 1] : 0;
+        uint32_t d = 0;
+        for (uint32_t c = c0 + li; c < c1; c += L)
+            d += (c == c0 || cell_codes[(size_t)c * nb] != cell_codes[(size_t)(c - 1) * nb]) ? 1u : 0u;
+        for (int o = L >> 1; o > 0; o >>= 1) d += (uint32_t)__shfl_xor((int)d, o, WAVE);
+        if (li == 0) acc += (unsigned long long)d * d;
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, WAVE);
-    if (lane_id() == 0) atomicAdd(out, acc);
+    block_add_u64(acc, out);
 }
 
 __global__ void k_level_keys(const uint32_t* __restrict__ cell_codes, size_t M, int nb, int b,
@@ -155,11 +178,70 @@ __global__ void k_level_keys(const uint32_t* __restrict__ cell_codes, size_t M, 
         keys[c] = cell_codes[c * nb + b];
 }
 
-// keys[c] = bucket(s(c)) * g + code_b(c): the (bucket, level) item of every cell.
+// keys[c] = bucket(s(c)) * g + code_b(c): the (bucket, level) item of every cell, in the low
+// `ib` bits; with `pay` set, s(c) and the codes of the other blocks ride in the bits above
+// (the sort only looks at the low ib bits), so the level records come out of the sorted keys
+// without a gather. *unsorted |= the items are not ascending in cell order.
+struct PayloadLayout {
+    int ib, sb;                 // item bits, segment bits
+    int ob[NB_MAX];             // bits of each other block's code (block order without b)
+    int pay;                    // payload present
+};
+
+__device__ __forceinline__ uint64_t bucket_key(const uint32_t* __restrict__ cell_codes,
+                                               const uint32_t* __restrict__ cell_seg, size_t c, int nb, int b,
+                                               int shift, uint64_t g, const PayloadLayout& pl) {
+    const uint32_t sg = cell_seg[c];
+    uint64_t k = (uint64_t)(sg >> shift) * g + cell_codes[c * nb + b];
+    if (pl.pay) {
+        int at = pl.ib;
+        k |= (uint64_t)sg << at;
+        at += pl.sb;
+        int o = 0;
+        for (int bb = 0; bb < nb; bb++) {
+            if (bb == b) continue;
+            k |= (uint64_t)cell_codes[c * nb + bb] << at;
+            at += pl.ob[o++];
+        }
+    }
+    return k;
+}
+
 __global__ void k_bucket_keys(const uint32_t* __restrict__ cell_codes, const uint32_t* __restrict__ cell_seg, size_t M,
-                              int nb, int b, int shift, uint64_t g, uint64_t* __restrict__ keys) {
-    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x)
-        keys[c] = (uint64_t)(cell_seg[c] >> shift) * g + cell_codes[c * nb + b];
+                              int nb, int b, int shift, uint64_t g, PayloadLayout pl, uint64_t* __restrict__ keys,
+                              uint32_t* __restrict__ unsorted) {
+    const uint64_t mask = pl.ib >= 64 ? ~0ull : ((1ull << pl.ib) - 1);
+    bool bad = false;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t k = bucket_key(cell_codes, cell_seg, c, nb, b, shift, g, pl);
+        keys[c] = k;
+        if (c > 0) {
+            const uint32_t sp = cell_seg[c - 1];
+            const uint64_t kp = (uint64_t)(sp >> shift) * g + cell_codes[(c - 1) * nb + b];
+            bad |= (k & mask) < kp;
+        }
+    }
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (__ballot(bad) != 0 && lane_id() == 0) atomicOr(&any, 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && any) atomicOr(unsorted, 1u);
+}
+
+// Level records from the sorted payload keys (coalesced, no gather).
+__global__ void k_level_records_keys(const uint64_t* __restrict__ keys, size_t M, int nother, PayloadLayout pl,
+                                     uint32_t* __restrict__ lseg, uint32_t* __restrict__ lother) {
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[j];
+        int at = pl.ib;
+        lseg[j] = (uint32_t)((k >> at) & ((1ull << pl.sb) - 1));
+        at += pl.sb;
+        for (int o = 0; o < nother; o++) {
+            lother[j * nother + o] = (uint32_t)((k >> at) & ((1ull << pl.ob[o]) - 1));
+            at += pl.ob[o];
+        }
+    }
 }
 
 __global__ void k_level_records(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ cell_seg,
@@ -188,6 +270,15 @@ __global__ void k_distinct(const uint64_t* __restrict__ keys, size_t n, int bits
 
 constexpr unsigned TPB = 256;
 inline unsigned blocks_for(size_t n) { return grid_for(n, TPB, 8192); }
+
+// Lanes per group: the next power of two of the mean length, within 1..64.
+inline uint32_t lanes(double mean) {
+    uint32_t L = 1;
+    while (L < 64 && L < mean) L <<= 1;
+    return L;
+}
+// Lanes per segment of the segment passes: about 8 cells per lane.
+inline uint32_t seg_lanes_for(size_t M, size_t g1) { return lanes(g1 ? double(M) / double(g1) / 8.0 : 1.0); }
 
 template <class T>
 T read_scalar(const T* dev, hipStream_t st) {
@@ -399,8 +490,9 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         DBuf<unsigned long long> acc(1);
         FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
         if (nsb == 1) {
-            hipLaunchKernelGGL(k_small_work1, dim3(blocks_for(a.g1)), dim3(TPB), 0, st, a.seg_off.p, a.cell_codes.p,
-                               (size_t)a.g1, nb, acc.p);
+            const int L = (int)seg_lanes_for(M, a.g1);
+            hipLaunchKernelGGL(k_small_work1, dim3(grid_for(a.g1, (TPB / WAVE) * (WAVE / L), 2048)), dim3(TPB), 0, st,
+                               a.seg_off.p, a.cell_codes.p, (size_t)a.g1, nb, L, acc.p);
         } else {
             unsigned grid = grid_for(a.g1, SW_WG / WAVE, 4096);
             hipLaunchKernelGGL(k_small_work, dim3(grid), dim3(SW_WG), words * 4 * (SW_WG / WAVE), st, a.seg_off.p,
@@ -414,18 +506,32 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         }
     }
 
-    // bucketed level-major transpose of every large block
+    // bucketed level-major transpose of every block
     a.lperm.clear(); a.loff.clear(); a.lseg.clear(); a.lother.clear(); a.bperm.clear(); a.boff.clear();
     a.lperm.resize(nb); a.loff.resize(nb); a.lseg.resize(nb); a.lother.resize(nb); a.bperm.resize(nb);
     a.boff.resize(nb);
     a.nbk.assign(nb, 1);
+    a.bshift.assign(nb, 31);
+    a.blanes.assign(nb, 1);
     for (int b = 0; b < nb; b++) {
-        if (a.blocks[b].small) continue;
         const uint64_t g = a.blocks[b].g;
-        const uint32_t nbk = (uint32_t)((a.g1 + (1ull << a.bucket_shift) - 1) >> a.bucket_shift);
+        // long levels (a few levels over many cells) are cut into FE1 buckets of about
+        // ITEM_CELLS cells per (bucket, level) item so that every item fills a wave
+        constexpr double ITEM_CELLS = 2048.0;
+        int shift = a.bucket_shift;
+        const double per_level = double(M) / double(std::max<uint64_t>(g, 1));
+        if (per_level > ITEM_CELLS && a.g1 > 1) {
+            const double want = std::ceil(per_level / ITEM_CELLS);  // buckets
+            shift = 0;
+            while (shift < 31 && double((a.g1 + (1ull << (shift + 1)) - 1) >> (shift + 1)) >= want) shift++;
+            shift = std::min(shift, a.bucket_shift);
+        }
+        const uint32_t nbk = (uint32_t)((a.g1 + (1ull << shift) - 1) >> shift);
+        a.bshift[b] = shift;
         a.nbk[b] = std::max<uint32_t>(nbk, 1);
         const uint64_t items = (uint64_t)a.nbk[b] * g;
         require(items < (1ull << 32), "too many (bucket, level) items");
+        a.blanes[b] = lanes(items ? double(M) / double(items) / 8.0 : 1.0);  // ~8 positions per lane
         a.bperm[b].alloc(std::max<size_t>(M, 1));
         a.boff[b].alloc(items + 1);
         FH_HIP(hipMemsetAsync(a.boff[b].p, 0, a.boff[b].bytes(), st));
@@ -434,33 +540,33 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         if (M > 0) {
             DBuf<uint64_t> keys(M), kalt(M);
             DBuf<uint32_t> valt(M), flag(1);
+            PayloadLayout pl{};
+            pl.ib = std::max(1, (int)bits_for(items));
+            pl.sb = std::max(1, (int)bits_for(a.g1));
+            int tot = pl.ib + pl.sb, o = 0;
+            for (int bb = 0; bb < nb; bb++)
+                if (bb != b) { pl.ob[o] = std::max(1, (int)bits_for(a.blocks[bb].g)); tot += pl.ob[o++]; }
+            pl.pay = tot <= 64;
+            FH_HIP(hipMemsetAsync(flag.p, 0, sizeof(uint32_t), st));
             hipLaunchKernelGGL(k_bucket_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, a.cell_codes.p, a.cell_seg.p, M, nb,
-                               b, a.bucket_shift, g, keys.p);
+                               b, shift, g, pl, keys.p, flag.p);
             hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, M);
             FH_CHECK_LAUNCH();
-            check_sorted_u64(keys.p, M, flag.p, st);
             if (read_scalar(flag.p, st) != 0)
-                radix_sort_pairs(keys.p, a.bperm[b].p, kalt.p, valt.p, M, (int)bits_for(items), st);
+                radix_sort_pairs(keys.p, a.bperm[b].p, kalt.p, valt.p, M, pl.ib, st);
+            const uint64_t mask = pl.ib >= 64 ? ~0ull : ((1ull << pl.ib) - 1);
             hipLaunchKernelGGL(k_offsets_u64, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, (size_t)items,
-                               a.boff[b].p);
-            hipLaunchKernelGGL(k_level_records, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, a.cell_seg.p,
-                               a.cell_codes.p, M, nb, b, a.lseg[b].p, a.lother[b].p);
+                               a.boff[b].p, mask);
+            if (pl.pay)
+                hipLaunchKernelGGL(k_level_records_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, nb - 1, pl,
+                                   a.lseg[b].p, a.lother[b].p);
+            else
+                hipLaunchKernelGGL(k_level_records, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, a.cell_seg.p,
+                                   a.cell_codes.p, M, nb, b, a.lseg[b].p, a.lother[b].p);
             FH_CHECK_LAUNCH();
         }
     }
-    // lanes per segment / per level: the next power of two of the mean length, within 1..64
-    auto lanes = [](double mean) {
-        uint32_t L = 1;
-        while (L < 64 && L < mean) L <<= 1;
-        return L;
-    };
-    a.seg_lanes = lanes(a.g1 ? double(M) / double(a.g1) / 8.0 : 1.0);
-    uint64_t items = 0;
-    int nlarge = 0;
-    for (size_t b = 0; b < a.blocks.size(); b++)
-        if (!a.blocks[b].small) { items += (uint64_t)a.nbk[b] * a.blocks[b].g; nlarge++; }
-    // about 8 positions per lane in the level passes
-    a.lvl_lanes = lanes(items ? double(M) * nlarge / double(items) / 8.0 : 1.0);
+    a.seg_lanes = seg_lanes_for(M, a.g1);
 }
 
 void ensure_level_cells(Absorber& a, size_t b) {

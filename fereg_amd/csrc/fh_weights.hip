@@ -59,41 +59,63 @@ __global__ __launch_bounds__(TPB) void k_partial_sum(const double* __restrict__ 
     }
 }
 
-// Large block: one thread per level walks its cells bucket by bucket ((bucket, level, cell)
-// order); the cells of one FE1 level are adjacent within a bucket, so c_{s,k} are run sums.
-__global__ void k_level_weights(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bperm,
-                                const uint32_t* __restrict__ lseg, uint32_t nbk, const double* __restrict__ cw,
-                                const double* __restrict__ n1, size_t g, size_t off, double* __restrict__ nr,
-                                double* __restrict__ diag, uint8_t* __restrict__ red, double* __restrict__ inv64,
-                                float* __restrict__ inv32, double* __restrict__ lw64, float* __restrict__ lw32) {
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
+// Large block: L lanes per level read its positions coalesced, bucket by bucket ((bucket,
+// level, cell) order). The cells of one FE1 level are adjacent within a bucket, so c_{s,k} are
+// run sums: the lane holding the last position of a run sums the run forward from its start
+// (each run is summed once, in position order) and adds run^2 / n1_s.
+__global__ __launch_bounds__(TPB) void k_level_weights(const uint32_t* __restrict__ boff,
+                                                       const uint32_t* __restrict__ bperm,
+                                                       const uint32_t* __restrict__ lseg, uint32_t nbk,
+                                                       const double* __restrict__ cw, const double* __restrict__ n1,
+                                                       size_t g, size_t off, int L, double* __restrict__ nr,
+                                                       double* __restrict__ diag, uint8_t* __restrict__ red,
+                                                       double* __restrict__ inv64, float* __restrict__ inv32,
+                                                       double* __restrict__ lw64, float* __restrict__ lw32) {
+    const int l = lane_id();
+    const int gpw = WAVE / L, gl = l / L, li = l % L;
+    const size_t nwaves = (size_t)gridDim.x * (TPB / WAVE);
+    for (size_t base = ((size_t)blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE) * gpw; base < g; base += nwaves * gpw) {
+        const size_t lv = base + gl;
+        const bool valid = lv < g;
         double tot = 0.0, dq = 0.0;
         for (uint32_t k = 0; k < nbk; k++) {
-            const uint32_t j0 = boff[(size_t)k * g + l], j1 = boff[(size_t)k * g + l + 1];
-            double run = 0.0;
-            uint32_t cur = 0xFFFFFFFFu;
-            for (uint32_t j = j0; j < j1; j++) {
+            const uint32_t j0 = valid ? boff[(size_t)k * g + lv] : 0, j1 = valid ? boff[(size_t)k * g + lv + 1] : 0;
+            for (uint32_t j = j0 + li; j < j1; j += L) {
                 const double w = cw ? cw[bperm[j]] : 1.0;
                 if (lw64) { lw64[j] = w; lw32[j] = (float)w; }
-                const uint32_t s = lseg[j];
-                if (s != cur) {
-                    if (cur != 0xFFFFFFFFu) dq += run * run / n1[cur];
-                    cur = s;
-                    run = 0.0;
-                }
-                run += w;
                 tot += w;
+                const uint32_t s = lseg[j];
+                if (j + 1 < j1 && lseg[j + 1] == s) continue;  // not the run's last position
+                uint32_t q = j;
+                while (q > j0 && lseg[q - 1] == s) q--;
+                double run = 0.0;
+                for (uint32_t i = q; i <= j; i++) run += cw ? cw[bperm[i]] : 1.0;
+                dq += run * run / n1[s];
             }
-            if (cur != 0xFFFFFFFFu) dq += run * run / n1[cur];
         }
+        for (int o = L >> 1; o > 0; o >>= 1) {
+            tot += __shfl_xor(tot, o, WAVE);
+            dq += __shfl_xor(dq, o, WAVE);
+        }
+        if (!valid || li != 0) continue;
         const double d = tot - dq;
         const bool r = d <= 1e-12 * tot;  // operator.rs:21, 319-333
-        nr[off + l] = tot;
-        diag[off + l] = d;
-        red[off + l] = r;
+        nr[off + lv] = tot;
+        diag[off + lv] = d;
+        red[off + lv] = r;
         const double inv = (r || !(d > 0.0)) ? 0.0 : 1.0 / d;  // jacobi.rs:7-9
-        inv64[off + l] = inv;
-        inv32[off + l] = (float)inv;
+        inv64[off + lv] = inv;
+        inv32[off + lv] = (float)inv;
+    }
+}
+
+// lw[j] = W of the cell at position j (level-major order of a small block).
+__global__ void k_gather_lw(const uint32_t* __restrict__ bperm, const double* __restrict__ cw, size_t M,
+                            double* __restrict__ lw64, float* __restrict__ lw32) {
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (size_t)gridDim.x * blockDim.x) {
+        const double w = cw[bperm[j]];
+        lw64[j] = w;
+        lw32[j] = (float)w;
     }
 }
 
@@ -149,12 +171,15 @@ __global__ __launch_bounds__(TPB) void k_small_assemble(const uint32_t* __restri
     }
 }
 
-// Row s of C = [c_{s,0..ms-1} / sqrt(n1_s), sqrt(n1_s)] for the single small block (block 0,
-// whose codes ascend within a segment). One thread per segment, the row in registers.
+// Level sums c_{s,k} of the single small block (block 0, whose codes ascend within a segment),
+// one thread per segment, the row in registers. Optional outputs: row s of
+// C = [c_{s,0..ms-1} / sqrt(n1_s), sqrt(n1_s)] (ld ms + 1) and the dense D[k][s] = c_{s,k}
+// (FP64 and FP32).
 template <int MS>
 __global__ void k_seg_level_sums(const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cell_codes,
                                  const double* __restrict__ cw, const double* __restrict__ n1, size_t g1, int nb,
-                                 uint32_t off0, int ms, double* __restrict__ C) {
+                                 uint32_t off0, int ms, double* __restrict__ C, double* __restrict__ D64,
+                                 float* __restrict__ D32) {
     for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
         double row[MS];
 #pragma unroll
@@ -166,6 +191,11 @@ __global__ void k_seg_level_sums(const uint32_t* __restrict__ seg_off, const uin
 #pragma unroll
             for (int k = 0; k < MS; k++) row[k] += key == (uint32_t)k ? w : 0.0;
         }
+        if (D64)
+#pragma unroll
+            for (int k = 0; k < MS; k++)
+                if (k < ms) { D64[(size_t)k * g1 + s] = row[k]; D32[(size_t)k * g1 + s] = (float)row[k]; }
+        if (!C) continue;
         const double sq = sqrt(n1[s]);
         const double f = sq > 0.0 ? 1.0 / sq : 0.0;
         double* out = C + s * (ms + 1);
@@ -173,6 +203,34 @@ __global__ void k_seg_level_sums(const uint32_t* __restrict__ seg_off, const uin
         for (int k = 0; k < MS; k++)
             if (k < ms) out[k] = row[k] * f;
         out[ms] = sq;
+    }
+}
+
+// C[k][coff + l] = sum of W over the positions of level l of a large block whose small code
+// (other block 0) is k: one thread per level, buckets in order, the row in registers.
+__global__ void k_level_small_sums(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ lother, int nother,
+                                   const double* __restrict__ lw, uint32_t nbk, size_t g, int ms, size_t ldc, size_t coff,
+                                   double* __restrict__ C64, float* __restrict__ C32) {
+    constexpr int MS = 32;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
+        double row[MS];
+#pragma unroll
+        for (int k = 0; k < MS; k++) row[k] = 0.0;
+        for (uint32_t q = 0; q < nbk; q++) {
+            const uint32_t j0 = boff[(size_t)q * g + l], j1 = boff[(size_t)q * g + l + 1];
+            for (uint32_t j = j0; j < j1; j++) {
+                const uint32_t key = lother[(size_t)j * nother];
+                const double w = lw ? lw[j] : 1.0;
+#pragma unroll
+                for (int k = 0; k < MS; k++) row[k] += key == (uint32_t)k ? w : 0.0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < MS; k++)
+            if (k < ms) {
+                C64[(size_t)k * ldc + coff + l] = row[k];
+                C32[(size_t)k * ldc + coff + l] = (float)row[k];
+            }
     }
 }
 
@@ -287,7 +345,7 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
     a.w32.lw.resize(nb);
     for (int b = 0; b < nb; b++) {
         const BlockInfo& bi = a.blocks[b];
-        if (bi.small) continue;
+        if (bi.small && unit) continue;
         if (!unit) {
             a.w64.lw[b].reserve(M);
             a.w32.lw[b].reserve(M);
@@ -295,10 +353,48 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             a.w64.lw[b].release();
             a.w32.lw[b].release();
         }
-        hipLaunchKernelGGL(k_level_weights, dim3(blocks_for(bi.g)), dim3(TPB), 0, st, a.boff[b].p, a.bperm[b].p,
-                           a.lseg[b].p, a.nbk[b], cw, a.n1.p, (size_t)bi.g, (size_t)bi.off, a.nr.p, a.diag.p, a.red.p,
+        if (bi.small) {  // level-order weights only
+            hipLaunchKernelGGL(k_gather_lw, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, cw, M, a.w64.lw[b].p,
+                               a.w32.lw[b].p);
+            FH_CHECK_LAUNCH();
+            continue;
+        }
+        uint32_t L = 1;  // lanes per level: ~4 positions per lane
+        while (L < 64 && L * 4.0 < double(a.M) / double(std::max<uint64_t>(bi.g, 1))) L <<= 1;
+        const unsigned grid = grid_for(bi.g, (TPB / WAVE) * (WAVE / L), 1u << 16);
+        hipLaunchKernelGGL(k_level_weights, dim3(grid), dim3(TPB), 0, st, a.boff[b].p, a.bperm[b].p,
+                           a.lseg[b].p, a.nbk[b], cw, a.n1.p, (size_t)bi.g, (size_t)bi.off, (int)L,
+                           a.nr.p, a.diag.p,
+                           a.red.p,
                            a.w64.inv_diag.p, a.w32.inv_diag.p, unit ? nullptr : a.w64.lw[b].p,
                            unit ? nullptr : a.w32.lw[b].p);
+        FH_CHECK_LAUNCH();
+    }
+    // dense small-row operator: one small block of <= 32 levels whose D and C are not much
+    // larger than the cell records a level pass would stream
+    {
+        int nsb = 0;
+        for (auto& bi : a.blocks) nsb += bi.small ? 1 : 0;
+        a.small_dense = nsb == 1 && ms >= 1 && ms <= 32 && ms * (g1 + (m - ms)) <= 4 * std::max<size_t>(M, 1);
+    }
+    if (a.small_dense) {
+        const size_t mL = m - ms;
+        a.w64.dsm.reserve(ms * g1);
+        a.w32.dsm.reserve(ms * g1);
+        a.w64.csm.reserve(std::max<size_t>(ms * mL, 1));
+        a.w32.csm.reserve(std::max<size_t>(ms * mL, 1));
+        for (int b = 0; b < nb; b++) {
+            const BlockInfo& bi = a.blocks[b];
+            if (bi.small) continue;
+            hipLaunchKernelGGL(k_level_small_sums, dim3(blocks_for(bi.g)), dim3(TPB), 0, st, a.boff[b].p,
+                               a.lother[b].p, nb - 1, unit ? nullptr : a.w64.lw[b].p, a.nbk[b], (size_t)bi.g, (int)ms,
+                               mL, (size_t)(bi.off - ms), a.w64.csm.p, a.w32.csm.p);
+            FH_CHECK_LAUNCH();
+        }
+        if (ms > 21)  // the S_SS assembly below does not produce D for these
+            hipLaunchKernelGGL(k_seg_level_sums<32>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
+                               a.cell_codes.p, cw, a.n1.p, g1, nb, (uint32_t)a.blocks[0].off, (int)ms, (double*)nullptr,
+                               a.w64.dsm.p, a.w32.dsm.p);
         FH_CHECK_LAUNCH();
     }
     // small super-block
@@ -315,12 +411,14 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         if (so.nsb == 1 && ms <= 21) {
             // S_SS = diag(n_r) - C'C over the scaled level sums; n_r from the extra column
             DBuf<double> C(g1 * (ms + 1));
+            double* D64 = a.small_dense ? a.w64.dsm.p : nullptr;
+            float* D32 = a.small_dense ? a.w32.dsm.p : nullptr;
             if (ms <= 8)
                 hipLaunchKernelGGL(k_seg_level_sums<8>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
-                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p);
+                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, D64, D32);
             else
                 hipLaunchKernelGGL(k_seg_level_sums<21>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
-                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p);
+                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, D64, D32);
             FH_CHECK_LAUNCH();
             std::vector<double> G((ms + 1) * (ms + 1));
             gram_cols(a, C.p, g1, (int)ms + 1, 0, (int)ms + 1, nullptr, nullptr, G.data());
@@ -362,6 +460,13 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         std::vector<double> zeros(ms, 0.0);
         std::vector<float> zeros32(ms, 0.0f);
         FH_HIP(hipMemcpyAsync(a.nr.p, nrs.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
+        if (a.small_dense) {
+            std::vector<float> nrs32(nrs.begin(), nrs.end());
+            a.w64.nrs.reserve(ms);
+            a.w32.nrs.reserve(ms);
+            FH_HIP(hipMemcpyAsync(a.w64.nrs.p, nrs.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
+            FH_HIP(hipMemcpyAsync(a.w32.nrs.p, nrs32.data(), ms * sizeof(float), hipMemcpyHostToDevice, st));
+        }
         FH_HIP(hipMemcpyAsync(a.diag.p, dg.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
         FH_HIP(hipMemcpyAsync(a.red.p, rd.data(), ms, hipMemcpyHostToDevice, st));
         FH_HIP(hipMemcpyAsync(a.w64.inv_diag.p, zeros.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
