@@ -138,7 +138,7 @@ def main():
         "structure": {"cells": int(info.ncells), "m": int(info.m), "m_small": int(info.m_small)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "reduced-operator SpMV (k_seg_spmv + k_reduce_small + k_lvl_spmv), FP32, r=2",
+                     "kernel": "reduced-operator SpMV S*p (k_seg_t + k_small_dense + k_small_dense_final + k_lvl_spmv), FP32, r=2",
                      "bytes_per_launch": by, "ms_per_launch": ms,
                      "fp64": {"ms": spmv["fp64"][0], "bytes": spmv["fp64"][1],
                               "achieved": spmv["fp64"][1] / (spmv["fp64"][0] * 1e-3) / 1e9}},
