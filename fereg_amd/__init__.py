@@ -11,8 +11,8 @@ from ._lib import (FH_CONVERGENCE, FH_INTERNAL, FH_INTERRUPTED, FH_INVALID, FH_M
                    FH_PLAIN_FP64, FH_TOO_FEW, FeregError, lib, library_path, memory_stats,
                    release_cached_memory)
 from .absorber import Absorber, FeCoefs, SolveInfo, SolveOpts
-from .fit import FitResult, feols, fepois
+from .fit import FitResult, WildBootResult, feols, fepois, wildboot
 
-__all__ = ["Absorber", "FeCoefs", "SolveInfo", "SolveOpts", "FitResult", "feols", "fepois", "FeregError", "lib",
+__all__ = ["Absorber", "FeCoefs", "SolveInfo", "SolveOpts", "FitResult", "WildBootResult", "feols", "fepois", "wildboot", "FeregError", "lib",
            "library_path", "FH_OK", "FH_INTERRUPTED", "FH_INVALID", "FH_CONVERGENCE", "FH_OOM", "FH_TOO_FEW",
            "FH_INTERNAL", "FH_PLAIN_FP64", "FH_MIXED", "memory_stats", "release_cached_memory"]

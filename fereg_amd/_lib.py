@@ -74,6 +74,23 @@ class FitStats(ctypes.Structure):
                 ("n_clusters", ctypes.c_uint64), ("converged", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+FH_PVAL = {"two-tailed": 0, "equal-tailed": 1, "greater": 2, "less": 3}
+
+
+class WbootSpec(ctypes.Structure):
+    _fields_ = [("reps", ctypes.c_uint64), ("seed", ctypes.c_uint64), ("r_mat", ctypes.POINTER(ctypes.c_double)),
+                ("r", ctypes.c_double), ("level", ctypes.c_double), ("tol", ctypes.c_double),
+                ("maxiter", ctypes.c_uint64), ("pval_type", ctypes.c_int32), ("conf_int", ctypes.c_int32)]
+
+
+class WbootResult(ctypes.Structure):
+    _fields_ = [("stat", ctypes.c_double), ("p_val", ctypes.c_double), ("se", ctypes.c_double),
+                ("point", ctypes.c_double), ("ci_lo", ctypes.c_double), ("ci_hi", ctypes.c_double),
+                ("ssc", ctypes.c_double), ("reps", ctypes.c_uint64), ("reps_feasible", ctypes.c_uint64),
+                ("n_clusters", ctypes.c_uint64), ("full_enumeration", ctypes.c_int32),
+                ("fe_not_projected", ctypes.c_int32), ("seconds_bootstrap", ctypes.c_double)]
+
+
 # (name, restype, argtypes) of every exported entry point of include/fereg_hip.h
 SIGNATURES = [
     ("fh_last_error", ctypes.c_char_p, []),
@@ -112,6 +129,12 @@ SIGNATURES = [
     ("fh_ppml_fit_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, ctypes.c_int32, vp, i32p, ctypes.POINTER(Ssc),
                                        ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
                                        ctypes.POINTER(FitStats)]),
+    ("fh_ols_wildboot", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, u32p, i32p, ctypes.POINTER(Ssc),
+                                       ctypes.POINTER(SolveOptsC), ctypes.POINTER(WbootSpec), f64p, f64p, i32p,
+                                       ctypes.POINTER(WbootResult), f64p]),
+    ("fh_ols_wildboot_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, vp, i32p, ctypes.POINTER(Ssc),
+                                           ctypes.POINTER(SolveOptsC), ctypes.POINTER(WbootSpec), f64p, f64p, i32p,
+                                           ctypes.POINTER(WbootResult), f64p]),
     ("fh_device_alloc", ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),
     ("fh_device_free", ctypes.c_int, [vp]),
     ("fh_memcpy_h2d", ctypes.c_int, [vp, vp, ctypes.c_size_t]),

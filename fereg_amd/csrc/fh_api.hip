@@ -494,10 +494,19 @@ uint64_t cluster_levels(Absorber& a, const uint32_t* cl_dev) {
     return (uint64_t)h + 1;
 }
 
+// What the bootstrap needs from the OLS fit.
+struct OlsState {
+    std::vector<int> kept;
+    std::vector<double> inv, coef;  // bread and beta over the kept columns
+    DBuf<double> e;                  // residuals, row order
+    double dfk = 0.0;
+    uint64_t ncl = 0;
+};
+
 // OLS on device inputs (kept order). rows scratch: n x (1+K) in row order.
 void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, const uint32_t* cl_dev,
              const int32_t* nested, const fh_ssc& ssc, const fh_solve_opts& o, double* beta, double* vcov,
-             int32_t* kept_out, fh_fit_stats* stats, const fh_ctx* ctx) {
+             int32_t* kept_out, fh_fit_stats* stats, const fh_ctx* ctx, OlsState* state = nullptr) {
     hipStream_t st = a.st;
     const size_t n = a.n, R = K + 1;
     require(K >= 1 && K <= 21, "1..21 regressors are supported");
@@ -558,6 +567,14 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
         stats->pcg_iterations = ro.iterations;
         stats->n_clusters = ncl;
         stats->converged = 1;
+    }
+    if (state) {
+        state->kept = kept;
+        state->inv = inv;
+        state->coef = coef;
+        state->e = std::move(e);
+        state->dfk = dfk;
+        state->ncl = ncl;
     }
 }
 
@@ -1096,6 +1113,77 @@ static int ols_impl(fh_absorber* ab, const double* y, const double* X, size_t K,
         }
     });
 }
+static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size_t K, const uint32_t* cluster,
+                         const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, const fh_wboot_spec* spec,
+                         double* beta, double* vcov, int32_t* kept, fh_wboot_result* res, double* t_boot, bool dev) {
+    return guard([&] {
+        require(ab && y && X && cluster && ssc && o && spec && spec->r_mat && beta && vcov && kept && res,
+                "null argument");
+        Absorber& a = ab->a;
+        AbsScope g(a);
+        const size_t n = a.n;
+        DBuf<double> dy, dx;
+        DBuf<uint32_t> dc;
+        const double *py = y, *px = X;
+        const uint32_t* pc = cluster;
+        if (!dev) {
+            dy.alloc(n);
+            dx.alloc(n * K);
+            dc.alloc(n);
+            h2d(dy.p, y, n, a.st);
+            h2d(dx.p, X, n * K, a.st);
+            h2d(dc.p, cluster, n, a.st);
+            py = dy.p;
+            px = dx.p;
+            pc = dc.p;
+        }
+        OlsState os;
+        fh_fit_stats stats{};
+        ols_fit(a, py, px, K, FH_VCOV_CLUSTER, pc, nested, *ssc, *o, beta, vcov, kept, &stats, nullptr, &os);
+        // R over the kept columns; a restriction on a dropped column is not testable
+        std::vector<double> rk;
+        for (size_t j = 0; j < K; j++) {
+            if (kept[j]) rk.push_back(spec->r_mat[j]);
+            else require(spec->r_mat[j] == 0.0, "the hypothesis involves a regressor dropped as collinear");
+        }
+        // fixed effects not nested in the clusters: the reference projects them out exactly when
+        // their levels number at most min(G, 512) (or G <= 512); beyond that it skips the
+        // projection (ols.rs:392-410, mod.rs:959-997), which is what is done here
+        const uint64_t G = cluster_levels(a, pc);
+        std::vector<int> nest = terms_nested_in(a, pc);
+        uint64_t m_nn = 0;
+        for (size_t q = 0; q < a.nterms; q++)
+            if (!nest[q]) m_nn += a.term_g[q];
+        require(m_nn == 0 || (m_nn > std::min<uint64_t>(G, 512) && G > 512),
+                "fixed effects not nested in small cluster designs need the exact non-nested projection, "
+                "which this build does not provide");
+        WbootInput in;
+        in.rows = a.rows_tmp.p;
+        in.R = (int)K + 1;
+        in.kept = os.kept;
+        in.bread = os.inv;
+        in.beta = os.coef;
+        in.e = os.e.p;
+        in.w = a.weighted ? a.w_rows.p : nullptr;
+        in.cl = pc;
+        in.G = G;
+        const double nn = (double)n, gg = (double)os.ncl;
+        in.ssc = (ssc->k_adj ? (nn - 1.0) / (nn - os.dfk) : 1.0) * (ssc->g_adj && gg > 1.0 ? gg / (gg - 1.0) : 1.0);
+        ols_wildboot(a, in, *spec, rk.data(), *res, t_boot);
+        res->fe_not_projected = m_nn > 0;
+    });
+}
+int fh_ols_wildboot(fh_absorber* ab, const double* y, const double* X, size_t K, const uint32_t* cluster,
+                    const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, const fh_wboot_spec* spec,
+                    double* beta, double* vcov, int32_t* kept, fh_wboot_result* res, double* t_boot) {
+    return wildboot_impl(ab, y, X, K, cluster, nested, ssc, o, spec, beta, vcov, kept, res, t_boot, false);
+}
+int fh_ols_wildboot_dev(fh_absorber* ab, const double* y, const double* X, size_t K, const uint32_t* cluster,
+                        const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, const fh_wboot_spec* spec,
+                        double* beta, double* vcov, int32_t* kept, fh_wboot_result* res, double* t_boot) {
+    return wildboot_impl(ab, y, X, K, cluster, nested, ssc, o, spec, beta, vcov, kept, res, t_boot, true);
+}
+
 int fh_ols_fit(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
                const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, double* beta, double* vcov,
                int32_t* kept, fh_fit_stats* stats) {

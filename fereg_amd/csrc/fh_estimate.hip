@@ -195,18 +195,28 @@ void hetero_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, 
     gram_cols(a, z, n, ld, c0, k, nullptr, v.p, meat);
 }
 
-// Returns the number of non-empty clusters.
-uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
-                      const uint32_t* cl_kept_dev, uint64_t G, double* meat) {
+void cluster_sort(Absorber& a, const uint32_t* cl_kept_dev, uint64_t G, DBuf<uint32_t>& vals, DBuf<uint32_t>& off) {
     hipStream_t st = a.st;
+    const size_t n = a.n;
     DBuf<uint64_t> keys(n), kalt(n);
-    DBuf<uint32_t> vals(n), valt(n), off(G + 1);
+    DBuf<uint32_t> valt(n);
+    vals.alloc(n);
+    off.alloc(G + 1);
     hipLaunchKernelGGL(k_gather_cluster, dim3(grid_for(n, TPB, 4096)), dim3(TPB), 0, st, cl_kept_dev, a.pi.p, n, keys.p,
                        vals.p);
     FH_CHECK_LAUNCH();
     radix_sort_pairs(keys.p, vals.p, kalt.p, valt.p, n, (int)bits_for(G), st);
     FH_HIP(hipMemsetAsync(off.p, 0, off.bytes(), st));
     hipLaunchKernelGGL(k_cl_offsets, dim3(grid_for(n, TPB, 4096)), dim3(TPB), 0, st, keys.p, n, (size_t)G, off.p);
+    FH_CHECK_LAUNCH();
+}
+
+// Returns the number of non-empty clusters.
+uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
+                      const uint32_t* cl_kept_dev, uint64_t G, double* meat) {
+    hipStream_t st = a.st;
+    DBuf<uint32_t> vals, off;
+    cluster_sort(a, cl_kept_dev, G, vals, off);
     DBuf<double> sg(G * k);
     hipLaunchKernelGGL(k_cluster_scores, dim3(grid_for(G, TPB, 4096)), dim3(TPB), 0, st, off.p, vals.p, (size_t)G, z,
                        ld, c0, k, e, w, sg.p);

@@ -152,5 +152,30 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, int r, const doub
 // ---------------------------------------------------------------- estimation (fh_estimate.hip)
 // G = sum_i w_i z_i z_i' (r x r, FP64 on the host), z n x r row-major; w may be null.
 void gram(Absorber& a, const double* z, const double* w, size_t n, int r, double* G_host);
+// Columns c0..c0+k of z (ld), optional row weights w and v: G = sum_i w_i v_i z_i z_i' (host).
+void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
+               double* G_host);
+// Solver rows grouped by cluster (codes in kept order): rows_sorted (n), off (G+1).
+void cluster_sort(Absorber& a, const uint32_t* cl_kept_dev, uint64_t G, DBuf<uint32_t>& rows_sorted,
+                  DBuf<uint32_t>& off);
+
+// ---------------------------------------------------------------- wild bootstrap (fh_wboot.hip)
+// WCR11 on the residualized OLS block (rows n x R: column 0 = y~, 1..K = X~), the kept
+// regressors `kept`, bread (mk x mk), residuals e (n), clusters cl (kept order, G levels).
+struct WbootInput {
+    const double* rows;
+    int R;
+    std::vector<int> kept;
+    std::vector<double> bread, beta;  // kept columns
+    const double* e;
+    const double* w;
+    const uint32_t* cl;
+    uint64_t G;
+    double ssc;
+};
+void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* r_mat_kept,
+                  fh_wboot_result& res, double* t_boot_host);
+// Per absorbed term: 1 when each of its levels lies in a single cluster (codes in kept order).
+std::vector<int> terms_nested_in(Absorber& a, const uint32_t* cl);
 
 }  // namespace fh

@@ -1,0 +1,632 @@
+// WCR11 wild cluster bootstrap of one linear hypothesis on the residualized OLS block
+// (crates/fereg-core/src/wildboot/ols.rs:243-649, clusters = bootstrap clusters, Rademacher
+// weights weights.rs:65-139 from SplitMix64 rng.rs:4-31, p-value and test inversion
+// mod.rs:529-749).
+//
+// Per cluster g (targets t = 0: u_hat, t = 1: -X~ M_R of the affine interval, ols.rs:274-312):
+//   S_t[g] = sum_{i in g} w_i u_t,i x~_i, B[g] = sum w_i z_i x~_i (z = x~'C, C = bread R'),
+//   Phi_t = bread S_t, A_t = C'S_t (= R Phi_t, the cell score of the cluster).
+// Per draw b with signs V_b (exact +-1):
+//   num_t = V_b.A_t, db_t = V_b.Phi_t, q[c,t] = V_bc A_t[c] - B_c.db_t, v = ssc sum_c q q'.
+// Since V^2 = 1, sum_c q_t1 q_t2 = AA[t1,t2] - db_t2.EA_t1 - db_t1.EA_t2 + db_t1' BB db_t2 with
+// EA_t = V_b.(A_t B), AA = sum A_t1 A_t2, BB = sum B B'. Every V-dependent quantity is one
+// product V (B x G) . P (G x (2 + 4k)), P = [A_0, A_1, Phi_0, Phi_1, A_0 B, A_1 B].
+//
+// That product runs on int8 MFMA: V is exact in int8; each column of P is scaled into
+// [-1/4, 1/4] and split into 7 signed 8-bit limbs (56 fixed-point bits); int32 accumulation
+// of +-limb is exact over any G < 2^24, so the contraction is exact for the 56-bit P and
+// bit-reproducible; the limbs are recombined in FP64 per draw.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "fh_internal.hpp"
+
+namespace fh {
+
+namespace {
+
+constexpr unsigned TPB = 256;
+constexpr int LIMBS = 7;
+constexpr int WB_NTC = 19;    // 16-column tiles per GEMM workgroup
+constexpr int WB_ROWS = 128;  // draws per GEMM workgroup (4 waves x 2 tiles of 16)
+constexpr int KMAX = 21;
+constexpr uint64_t SM_GAMMA = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t SM_STREAM = 0xD1B54A32D192ED03ull;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__host__ __device__ __forceinline__ uint64_t sm_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Per row: w z, w u0, w u1 (z = x~'C, u0 = e, u1 = -x~'M_R).
+__global__ void k_wb_rowscal(const double* __restrict__ rows, int R, const int* __restrict__ kept, int mk,
+                             const double* __restrict__ C, const double* __restrict__ MR, const double* __restrict__ e,
+                             const double* __restrict__ w, size_t n, double* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double z = 0.0, u1 = 0.0;
+        for (int a = 0; a < mk; a++) {
+            const double x = rows[i * R + 1 + kept[a]];
+            z += x * C[a];
+            u1 -= x * MR[a];
+        }
+        const double wi = w ? w[i] : 1.0;
+        out[i * 3] = wi * z;
+        out[i * 3 + 1] = wi * e[i];
+        out[i * 3 + 2] = wi * u1;
+    }
+}
+
+// S[g][0..2][a]: sums over the rows of cluster g of (w u0, w u1, w z) x~_a; thread per (g, a).
+__global__ void k_wb_clsums(const uint32_t* __restrict__ off, const uint32_t* __restrict__ rs, size_t G,
+                            const double* __restrict__ rows, int R, const int* __restrict__ kept, int mk,
+                            const double* __restrict__ sc, double* __restrict__ S) {
+    const size_t total = G * mk;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t g = e / mk;
+        const int a = (int)(e % mk);
+        const int col = 1 + kept[a];
+        double s0 = 0.0, s1 = 0.0, sb = 0.0;
+        for (uint32_t j = off[g]; j < off[g + 1]; j++) {
+            const size_t i = rs[j];
+            const double x = rows[i * R + col];
+            sb += sc[i * 3] * x;
+            s0 += sc[i * 3 + 1] * x;
+            s1 += sc[i * 3 + 2] * x;
+        }
+        S[(g * 3 + 0) * mk + a] = s0;
+        S[(g * 3 + 1) * mk + a] = s1;
+        S[(g * 3 + 2) * mk + a] = sb;
+    }
+}
+
+// P[g] = [A0, A1, Phi0, Phi1, A0 B, A1 B] and Q[g] = [A0, A1, B]; per-workgroup column maxima
+// of |P| in cmax[blockIdx.x][c].
+__global__ __launch_bounds__(TPB) void k_wb_prow(const double* __restrict__ S, size_t G, int mk,
+                                                 const double* __restrict__ bread, const double* __restrict__ C,
+                                                 double* __restrict__ P, double* __restrict__ Q,
+                                                 double* __restrict__ cmax) {
+    const int NC = 2 + 4 * mk;
+    __shared__ double red[TPB / WAVE][2 + 4 * KMAX];
+    double mx[2 + 4 * KMAX];
+    for (int c = 0; c < NC; c++) mx[c] = 0.0;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < G; g += (size_t)gridDim.x * blockDim.x) {
+        const double* S0 = S + (g * 3) * mk;
+        const double* S1 = S0 + mk;
+        const double* Bc = S1 + mk;
+        double* p = P + g * NC;
+        double A0 = 0.0, A1 = 0.0;
+        for (int a = 0; a < mk; a++) {
+            A0 += C[a] * S0[a];
+            A1 += C[a] * S1[a];
+        }
+        p[0] = A0;
+        p[1] = A1;
+        for (int a = 0; a < mk; a++) {
+            double f0 = 0.0, f1 = 0.0;
+            for (int b = 0; b < mk; b++) {
+                f0 += bread[a * mk + b] * S0[b];
+                f1 += bread[a * mk + b] * S1[b];
+            }
+            p[2 + a] = f0;
+            p[2 + mk + a] = f1;
+            p[2 + 2 * mk + a] = A0 * Bc[a];
+            p[2 + 3 * mk + a] = A1 * Bc[a];
+        }
+        double* q = Q + g * (2 + mk);
+        q[0] = A0;
+        q[1] = A1;
+        for (int a = 0; a < mk; a++) q[2 + a] = Bc[a];
+        for (int c = 0; c < NC; c++) mx[c] = fmax(mx[c], fabs(p[c]));
+    }
+    const int wv = threadIdx.x / WAVE;
+    for (int c = 0; c < NC; c++) {
+        double v = mx[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+        if (lane_id() == 0) red[wv][c] = v;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < NC; c += TPB) {
+        double v = 0.0;
+        for (int ww = 0; ww < (int)(TPB / WAVE); ww++) v = fmax(v, red[ww][c]);
+        cmax[(size_t)blockIdx.x * NC + c] = v;
+    }
+}
+
+// Pq[(l NC + c) Gpad + g] = limb l (most significant first) of round(P[g][c] * inv_scale[c]).
+__global__ void k_wb_quant(const double* __restrict__ P, size_t G, int NC, const double* __restrict__ inv_scale,
+                           size_t Gpad, int8_t* __restrict__ Pq) {
+    const size_t total = G * NC;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t g = e / NC;
+        const int c = (int)(e % NC);
+        long long X = llrint(P[e] * inv_scale[c]);
+        for (int l = LIMBS - 1; l >= 0; l--) {
+            const long long d = ((X + 128) & 255) - 128;  // balanced base-256 digit
+            Pq[((size_t)l * NC + c) * Gpad + g] = (int8_t)d;
+            X = (X - d) >> 8;
+        }
+    }
+}
+
+// 16 signs from 16 bits (bit 0 -> +1, bit 1 -> -1), byte j of dword d = bit 4d + j.
+__device__ __forceinline__ v4i sign_bytes(uint32_t bits16) {
+    v4i a;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        const uint32_t nib = (bits16 >> (4 * d)) & 0xFu;
+        const uint32_t spread = (nib * 0x00204081u) & 0x01010101u;  // bit j -> byte j
+        a[d] = (int)(0x01010101u | (spread * 0xFEu));
+    }
+    return a;
+}
+
+// part[split][b][col] = sum over this split's g of V[b][g] Pq[col][g] (int32, exact).
+// Workgroup: 128 draws (4 waves x 2 row tiles) x up to 19 column tiles; each 64-g step stages
+// the column slice in LDS (80-byte rows: conflict-free 16-byte reads) and generates the
+// signs of its draws from SplitMix64 (word s of draw b covers g in [64 s, 64 s + 64)).
+__global__ __launch_bounds__(TPB) void k_wb_gemm(const int8_t* __restrict__ Pq, size_t Gpad, int ntiles,
+                                                 uint64_t seed, int full, uint32_t steps_per_split, uint32_t nsteps,
+                                                 int32_t* __restrict__ part, size_t Bpad, int ncolp) {
+    __shared__ v4i lds[WB_NTC * 16][5];
+    const int lane = lane_id(), w = threadIdx.x / WAVE;
+    const int kg = lane >> 4, li = lane & 15;
+    const int tile0 = blockIdx.z * WB_NTC;
+    const int ntc = min(WB_NTC, ntiles - tile0);
+    const uint64_t bbase = (uint64_t)blockIdx.x * WB_ROWS + (uint64_t)w * 32;
+    uint64_t st0[2], bid[2];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++) {
+        bid[rt] = bbase + 16 * rt + li;
+        st0[rt] = seed ^ (bid[rt] * SM_STREAM);
+    }
+    v4i acc[2][WB_NTC];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+        for (int t = 0; t < WB_NTC; t++) acc[rt][t] = v4i{0, 0, 0, 0};
+    const uint32_t s0 = blockIdx.y * steps_per_split;
+    const uint32_t s1 = min(nsteps, s0 + steps_per_split);
+    for (uint32_t s = s0; s < s1; s++) {
+        __syncthreads();
+        for (int it = threadIdx.x; it < ntc * 64; it += TPB) {
+            const int col = it >> 2, q = it & 3;
+            lds[col][q] = *reinterpret_cast<const v4i*>(Pq + (size_t)(tile0 * 16 + col) * Gpad + (size_t)s * 64 + q * 16);
+        }
+        __syncthreads();
+        v4i a[2];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++) {
+            const uint64_t word = full ? (s == 0 ? bid[rt] : 0ull) : sm_mix(st0[rt] + (uint64_t)(s + 2) * SM_GAMMA);
+            a[rt] = sign_bytes((uint32_t)(word >> (16 * kg)) & 0xFFFFu);
+        }
+#pragma unroll
+        for (int t = 0; t < WB_NTC; t++) {
+            if (t < ntc) {
+                const v4i bf = lds[t * 16 + li][kg];
+                acc[0][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[0], bf, acc[0][t], 0, 0, 0);
+                acc[1][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[1], bf, acc[1][t], 0, 0, 0);
+            }
+        }
+    }
+    // D layout of the 16x16 tile: lane = column (lane & 15), rows 4 (lane >> 4) + r
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+        for (int t = 0; t < WB_NTC; t++) {
+            if (t >= ntc) continue;
+            const int col = (tile0 + t) * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint64_t b = bbase + 16 * rt + 4 * kg + r;
+                part[((size_t)blockIdx.y * Bpad + b) * ncolp + col] = acc[rt][t][r];
+            }
+        }
+}
+
+// Per draw: recombine the limbs (exact int64 sums over the splits, then FP64) and form the
+// affine coefficients (num0, num1, v00, v01, v11) of t(delta).
+__global__ void k_wb_post(const int32_t* __restrict__ part, int nsplit, size_t Bpad, int ncolp, int NC, int mk,
+                          uint64_t reps, const double* __restrict__ scale, const double* __restrict__ AA,
+                          const double* __restrict__ BB, double ssc, double* __restrict__ coefs) {
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < reps; b += (uint64_t)gridDim.x * blockDim.x) {
+        auto y = [&](int c) {
+            double v = 0.0;
+            for (int l = LIMBS - 1; l >= 0; l--) {
+                long long s = 0;
+                for (int k = 0; k < nsplit; k++) s += part[((size_t)k * Bpad + b) * ncolp + (size_t)l * NC + c];
+                v += ldexp((double)s, -8 * (l + 1));
+            }
+            return v * scale[c];
+        };
+        double db0[KMAX], db1[KMAX];
+        const double n0 = y(0), n1 = y(1);
+        double e00 = 0.0, e01 = 0.0, e10 = 0.0, e11 = 0.0;  // db_s . EA_t
+        for (int a = 0; a < mk; a++) {
+            db0[a] = y(2 + a);
+            db1[a] = y(2 + mk + a);
+        }
+        for (int a = 0; a < mk; a++) {
+            const double ea0 = y(2 + 2 * mk + a), ea1 = y(2 + 3 * mk + a);
+            e00 += db0[a] * ea0;
+            e01 += db0[a] * ea1;
+            e10 += db1[a] * ea0;
+            e11 += db1[a] * ea1;
+        }
+        double q00 = 0.0, q01 = 0.0, q11 = 0.0;  // db_s' BB db_t
+        for (int a = 0; a < mk; a++) {
+            double r0 = 0.0, r1 = 0.0;
+            for (int c = 0; c < mk; c++) {
+                r0 += BB[a * mk + c] * db0[c];
+                r1 += BB[a * mk + c] * db1[c];
+            }
+            q00 += db0[a] * r0;
+            q01 += db0[a] * r1;
+            q11 += db1[a] * r1;
+        }
+        double* o = coefs + b * 5;
+        o[0] = n0;
+        o[1] = n1;
+        o[2] = ssc * (AA[0] - 2.0 * e00 + q00);
+        o[3] = ssc * (AA[1] - e10 - e01 + q01);
+        o[4] = ssc * (AA[3] - 2.0 * e11 + q11);
+    }
+}
+
+// lo/hi[level offset of term + level] = min/max cluster over the rows of the level; one thread
+// per cell (every row of a cell has the same levels).
+struct TermLevels {
+    uint32_t off[NB_MAX + 1];  // level offset of FE1 then of each block
+    int nb;
+    int has_fe1;
+};
+__global__ void k_wb_level_span(const uint32_t* __restrict__ cell_seg, const uint32_t* __restrict__ cell_codes,
+                                const uint32_t* __restrict__ coo, const uint32_t* __restrict__ pi,
+                                const uint32_t* __restrict__ cl, size_t M, TermLevels tl, uint32_t* __restrict__ lo,
+                                uint32_t* __restrict__ hi) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
+        const size_t r0 = coo ? coo[c] : c, r1 = coo ? coo[c + 1] : c + 1;
+        uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+        for (size_t i = r0; i < r1; i++) {
+            const uint32_t k = cl[pi[i]];
+            mn = min(mn, k);
+            mx = max(mx, k);
+        }
+        int t = 0;
+        if (tl.has_fe1) {
+            const uint32_t l = tl.off[0] + cell_seg[c];
+            atomicMin(&lo[l], mn);
+            atomicMax(&hi[l], mx);
+            t = 1;
+        }
+        for (int b = 0; b < tl.nb; b++) {
+            const uint32_t l = tl.off[t + b] + cell_codes[c * tl.nb + b];
+            atomicMin(&lo[l], mn);
+            atomicMax(&hi[l], mx);
+        }
+    }
+}
+__global__ void k_wb_span_flags(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, size_t L,
+                                const uint32_t* __restrict__ term_of_level_off, int nt, uint32_t* __restrict__ bad) {
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < L; l += (size_t)gridDim.x * blockDim.x) {
+        if (lo[l] <= hi[l] && lo[l] != hi[l]) {
+            int t = 0;
+            while (t + 1 < nt && l >= term_of_level_off[t + 1]) t++;
+            atomicOr(&bad[t], 1u);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ host: inference
+
+// ols.rs:139-142
+double eval_affine_t(double delta, const double* c) {
+    const double v = c[2] + delta * (2.0 * c[3] + delta * c[4]);
+    if (!std::isfinite(v) || v <= 1e-300) return NAN;
+    return (c[0] + delta * c[1]) / std::sqrt(v);
+}
+
+// mod.rs:529-580 for one restriction
+std::pair<double, uint64_t> compute_pvalue(double stat, const std::vector<double>& tb, int pval_type) {
+    if (!std::isfinite(stat)) return {NAN, 0};
+    uint64_t feas = 0, hi = 0, lo = 0;
+    const double eps = 1e-12 * (1.0 + std::fabs(stat)), as = std::fabs(stat);
+    for (double t : tb) {
+        if (!std::isfinite(t)) continue;
+        feas++;
+        if (pval_type == FH_PVAL_TWO_TAILED) {
+            if (std::fabs(t) > as + eps) hi++;
+        } else {
+            if (t > stat + eps) hi++;
+            if (t < stat - eps) lo++;
+        }
+    }
+    if (feas == 0) return {NAN, 0};
+    const double f = (double)feas;
+    switch (pval_type) {
+        case FH_PVAL_TWO_TAILED:
+        case FH_PVAL_GREATER: return {hi / f, feas};
+        case FH_PVAL_LESS: return {lo / f, feas};
+        default: return {std::min(2.0 * ((double)std::min(hi, lo) / f), 1.0), feas};
+    }
+}
+
+// Standard normal quantile: Wichura (1988), algorithm AS 241 (PPND16).
+double qnorm(double p) {
+    if (std::isnan(p) || p < 0.0 || p > 1.0) return NAN;
+    if (p == 0.0) return -INFINITY;
+    if (p == 1.0) return INFINITY;
+    const double q = p - 0.5;
+    if (std::fabs(q) <= 0.425) {
+        const double r = 0.180625 - q * q;
+        const double num = (((((((2509.0809287301226727 * r + 33430.575583588128105) * r + 67265.770927008700853) * r +
+                                45921.953931549871457) * r + 13731.693765509461125) * r + 1971.5909503065514427) * r +
+                             133.14166789178437745) * r + 3.387132872796366608);
+        const double den = (((((((5226.495278852545925 * r + 28729.085735721942674) * r + 39307.89580009271061) * r +
+                                21213.794301586595867) * r + 5394.1960214247511077) * r + 687.1870074920579083) * r +
+                             42.313330701600911252) * r + 1.0);
+        return q * num / den;
+    }
+    double r = std::sqrt(-std::log(q < 0.0 ? p : 1.0 - p));
+    double v;
+    if (r <= 5.0) {
+        r -= 1.6;
+        const double num = (((((((7.7454501427834140764e-4 * r + 0.0227238449892691845833) * r + 0.24178072517745061177) * r +
+                                1.27045825245236838258) * r + 3.64784832476320460504) * r + 5.7694972214606914055) * r +
+                             4.6303378461565452959) * r + 1.42343711074968357734);
+        const double den = (((((((1.05075007164441684324e-9 * r + 5.475938084995344946e-4) * r + 0.0151986665636164571966) * r +
+                                0.14810397642748007459) * r + 0.68976733498510000455) * r + 1.6763848301838038494) * r +
+                             2.05319162663775882187) * r + 1.0);
+        v = num / den;
+    } else {
+        r -= 5.0;
+        const double num = (((((((2.01033439929228813265e-7 * r + 2.71155556874348757815e-5) * r + 0.0012426609473880784386) * r +
+                                0.026532189526576123093) * r + 0.29656057182850489123) * r + 1.7848265399172913358) * r +
+                             5.4637849111641143699) * r + 6.6579046435011037772);
+        const double den = (((((((2.04426310338993978564e-15 * r + 1.4215117583164458887e-7) * r + 1.8463183175100546818e-5) * r +
+                                7.868691311456132591e-4) * r + 0.0148753612908506148525) * r + 0.13692988092273580531) * r +
+                             0.59983220655588793769) * r + 1.0);
+        v = num / den;
+    }
+    return q < 0.0 ? -v : v;
+}
+
+// mod.rs:637-749: bracket outwards from the point, then Chandrupatla / bisection on p - alpha.
+template <class F>
+std::pair<double, double> invert_ci_search(double point, double se, double alpha, int pval_type, double tol,
+                                           uint64_t maxiter, uint64_t reps_feas, F&& eval_p) {
+    if (!std::isfinite(point) || !std::isfinite(se) || se <= 0.0 || reps_feas == 0) return {NAN, NAN};
+    const double z = std::max(std::fabs(qnorm(1.0 - alpha / 2.0)), 1.5);
+    const double hw = std::max(1.5 * z * se, 1e-6 * (1.0 + std::fabs(point)));
+    const double eq = pval_type == FH_PVAL_EQUAL_TAILED ? 2.0 : 1.0;
+    const double p_tol = (eq / (double)reps_feas) * 1.0001;
+    const double x_tol = tol * (1.0 + std::fabs(point));
+    auto find_bound = [&](double sign) -> double {
+        double x_in = point, p_in = eval_p(x_in);
+        if (!std::isfinite(p_in) || p_in < alpha) p_in = 1.0;
+        double step = hw, x_out = point + sign * step, p_out = NAN;
+        bool bracketed = false;
+        for (int i = 0; i < 16; i++) {
+            const double p = eval_p(x_out);
+            if (std::isfinite(p)) {
+                if (p < alpha) {
+                    p_out = p;
+                    bracketed = true;
+                    break;
+                }
+                x_in = x_out;
+                p_in = p;
+            }
+            step *= 2.0;
+            x_out = point + sign * step;
+        }
+        if (!bracketed) return sign < 0.0 ? -INFINITY : INFINITY;
+        double x1 = x_in, f1 = p_in - alpha, x2 = x_out, f2 = p_out - alpha, x3 = 0.0, f3 = 0.0, t = 0.5;
+        for (uint64_t it = 0; it < maxiter; it++) {
+            if (std::fabs(x2 - x1) <= x_tol || std::min(std::fabs(f1), std::fabs(f2)) <= p_tol) break;
+            const double x = x1 + t * (x2 - x1);
+            const double fx = eval_p(x) - alpha;
+            if (!std::isfinite(fx)) {
+                t = 0.5;
+                const double xm = 0.5 * (x1 + x2), fm = eval_p(xm) - alpha;
+                if (!std::isfinite(fm)) break;
+                if ((fm < 0.0) == (f1 < 0.0)) { x1 = xm; f1 = fm; } else { x2 = xm; f2 = fm; }
+                continue;
+            }
+            if ((fx < 0.0) == (f1 < 0.0)) {
+                x3 = x1; f3 = f1; x1 = x; f1 = fx;
+            } else {
+                x3 = x2; f3 = f2; x2 = x; f2 = fx;
+            }
+            if (std::fabs(x2 - x1) <= x_tol || std::min(std::fabs(f1), std::fabs(f2)) <= p_tol) break;
+            const double d32 = f3 - f2, d12 = f1 - f2, d31 = f3 - f1;
+            if (d32 == 0.0 || d12 == 0.0 || d31 == 0.0 || (x2 - x1) == 0.0 || (x3 - x2) == 0.0) {
+                t = 0.5;
+                continue;
+            }
+            const double phi1 = d12 / d32, xi1 = (x1 - x2) / (x3 - x2);
+            if (phi1 * phi1 > xi1 || xi1 > 2.0 * phi1 - phi1 * phi1) {
+                t = 0.5;
+            } else {
+                const double raw = (f3 / d12 + (x3 - x1) / ((x2 - x1) * d31) * f2) * (f1 / d32);
+                t = std::min(std::max(raw, 1e-6), 1.0 - 1e-6);
+            }
+        }
+        return std::fabs(f1) <= std::fabs(f2) ? x1 : x2;
+    };
+    const double lo = pval_type == FH_PVAL_LESS ? -INFINITY : find_bound(-1.0);
+    const double hi = pval_type == FH_PVAL_GREATER ? INFINITY : find_bound(1.0);
+    return {lo, hi};
+}
+
+}  // namespace
+
+// Per absorbed term (absorber term order): 1 when every level lies in one cluster
+// (dof.rs nested_within, by value).
+std::vector<int> terms_nested_in(Absorber& a, const uint32_t* cl) {
+    hipStream_t st = a.st;
+    TermLevels tl{};
+    std::vector<uint32_t> offs;
+    std::vector<int64_t> term_of;  // absorber term of each level range
+    uint64_t L = 0;
+    tl.has_fe1 = a.fe1 >= 0;
+    if (a.fe1 >= 0) { offs.push_back((uint32_t)L); term_of.push_back(a.fe1); L += a.g1; }
+    tl.nb = (int)a.nb;
+    for (auto& b : a.blocks) { offs.push_back((uint32_t)L); term_of.push_back(b.term); L += b.g; }
+    std::vector<int> nested(a.nterms, 1);
+    if (L == 0 || a.M == 0) return nested;
+    require(L < (1ull << 32), "too many levels");
+    for (size_t i = 0; i < offs.size(); i++) tl.off[i] = offs[i];
+    DBuf<uint32_t> lo(L), hi(L), toff(offs.size()), bad(offs.size());
+    FH_HIP(hipMemsetAsync(lo.p, 0xFF, lo.bytes(), st));
+    FH_HIP(hipMemsetAsync(hi.p, 0, hi.bytes(), st));
+    FH_HIP(hipMemsetAsync(bad.p, 0, bad.bytes(), st));
+    FH_HIP(hipMemcpyAsync(toff.p, offs.data(), offs.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_wb_level_span, dim3(grid_for(a.M, TPB, 8192)), dim3(TPB), 0, st, a.cell_seg.p, a.cell_codes.p,
+                       a.trivial() ? nullptr : a.cell_obs_off.p, a.pi.p, cl, (size_t)a.M, tl, lo.p, hi.p);
+    hipLaunchKernelGGL(k_wb_span_flags, dim3(grid_for(L, TPB, 8192)), dim3(TPB), 0, st, lo.p, hi.p, (size_t)L, toff.p,
+                       (int)offs.size(), bad.p);
+    FH_CHECK_LAUNCH();
+    std::vector<uint32_t> hb(offs.size());
+    FH_HIP(hipMemcpyAsync(hb.data(), bad.p, hb.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < offs.size(); i++)
+        if (hb[i]) nested[term_of[i]] = 0;
+    return nested;
+}
+
+void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* rk, fh_wboot_result& res,
+                  double* t_boot_host) {
+    hipStream_t st = a.st;
+    const size_t n = a.n, G = in.G;
+    const int mk = (int)in.kept.size();
+    require(mk >= 1 && mk + 2 <= KMAX, "the bootstrap supports 1..19 kept regressors");
+    require(G >= 2 && G < (1ull << 24), "the bootstrap needs 2 .. 2^24 clusters");
+    require(spec.reps >= 1, "wildboot: reps must be at least 1");
+    require(spec.level > 0.0 && spec.level < 1.0, "wildboot: confidence level must be in (0, 1)");
+    require(spec.tol > 0.0 && std::isfinite(spec.tol) && spec.maxiter > 0, "wildboot: tol and maxiter must be positive");
+    require(spec.pval_type >= 0 && spec.pval_type <= 3, "wildboot: unknown p-value type");
+    auto t0 = std::chrono::steady_clock::now();
+    // ConstraintProj (ols.rs:76-121) for one restriction
+    std::vector<double> C(mk, 0.0), MR(mk, 0.0);
+    for (int i = 0; i < mk; i++)
+        for (int j = 0; j < mk; j++) C[i] += in.bread[(size_t)i * mk + j] * rk[j];
+    double v0 = 0.0;
+    for (int j = 0; j < mk; j++) v0 += rk[j] * C[j];
+    const double v0i = std::fabs(v0) > 1e-12 * std::max(1.0, std::fabs(v0)) ? 1.0 / v0 : 0.0;
+    for (int i = 0; i < mk; i++) MR[i] = C[i] * v0i;
+    DBuf<int> dkept(mk);
+    DBuf<double> dC(mk), dMR(mk), dbread((size_t)mk * mk);
+    FH_HIP(hipMemcpyAsync(dkept.p, in.kept.data(), mk * sizeof(int), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dC.p, C.data(), mk * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dMR.p, MR.data(), mk * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dbread.p, in.bread.data(), (size_t)mk * mk * sizeof(double), hipMemcpyHostToDevice, st));
+    // per-cluster sums
+    DBuf<uint32_t> rs, off;
+    cluster_sort(a, in.cl, G, rs, off);
+    DBuf<double> sc(n * 3), S(G * 3 * mk);
+    hipLaunchKernelGGL(k_wb_rowscal, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, in.rows, in.R, dkept.p, mk, dC.p,
+                       dMR.p, in.e, in.w, n, sc.p);
+    hipLaunchKernelGGL(k_wb_clsums, dim3(grid_for(G * mk, TPB, 8192)), dim3(TPB), 0, st, off.p, rs.p, G, in.rows, in.R,
+                       dkept.p, mk, sc.p, S.p);
+    FH_CHECK_LAUNCH();
+    const int NC = 2 + 4 * mk, QC = 2 + mk;
+    DBuf<double> P(G * NC), Q(G * QC);
+    const unsigned pg = grid_for(G, TPB, 512);
+    DBuf<double> cmax((size_t)pg * NC);
+    hipLaunchKernelGGL(k_wb_prow, dim3(pg), dim3(TPB), 0, st, S.p, G, mk, dbread.p, dC.p, P.p, Q.p, cmax.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> QG((size_t)QC * QC);
+    gram_cols(a, Q.p, G, QC, 0, QC, nullptr, nullptr, QG.data());  // synchronizes
+    std::vector<double> hmax((size_t)pg * NC);
+    FH_HIP(hipMemcpyAsync(hmax.data(), cmax.p, hmax.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    std::vector<double> scale(NC), inv_scale(NC);
+    for (int c = 0; c < NC; c++) {
+        double m = 0.0;
+        for (unsigned q = 0; q < pg; q++) m = std::max(m, hmax[(size_t)q * NC + c]);
+        scale[c] = m > 0.0 ? 4.0 * m : 1.0;  // |P / scale| <= 1/4: 7 balanced limbs cover it
+        inv_scale[c] = std::ldexp(1.0 / scale[c], 56);
+    }
+    std::vector<double> AA = {QG[0], QG[1], QG[QC], QG[QC + 1]}, BB((size_t)mk * mk);
+    for (int i = 0; i < mk; i++)
+        for (int j = 0; j < mk; j++) BB[(size_t)i * mk + j] = QG[(size_t)(2 + i) * QC + 2 + j];
+    // observed statistic: q_obs = A_0 (the cluster scores of the unrestricted residuals)
+    double point = 0.0;
+    for (int j = 0; j < mk; j++) point += rk[j] * in.beta[j];
+    const double num_obs = point - spec.r;
+    const double v_obs = in.ssc * AA[0];
+    const double se = (std::isfinite(v_obs) && v_obs > 1e-300) ? std::sqrt(v_obs) : NAN;
+    const double stat = std::isfinite(se) ? num_obs / se : NAN;
+    // draws
+    uint64_t reps = spec.reps;
+    bool full = false;
+    if (G <= 20 && (1ull << G) <= reps) {  // weights.rs:39-61
+        reps = 1ull << G;
+        full = true;
+    }
+    const size_t Gpad = (G + 63) / 64 * 64;
+    const int ncol = LIMBS * NC, ntiles = (ncol + 15) / 16, ncolp = ntiles * 16;
+    DBuf<int8_t> Pq((size_t)ncolp * Gpad);
+    FH_HIP(hipMemsetAsync(Pq.p, 0, Pq.bytes(), st));
+    DBuf<double> dinv(NC), dscale(NC), dAA(4), dBB((size_t)mk * mk);
+    FH_HIP(hipMemcpyAsync(dinv.p, inv_scale.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dscale.p, scale.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dAA.p, AA.data(), 4 * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dBB.p, BB.data(), (size_t)mk * mk * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_wb_quant, dim3(grid_for(G * NC, TPB, 8192)), dim3(TPB), 0, st, P.p, G, NC, dinv.p, Gpad, Pq.p);
+    FH_CHECK_LAUNCH();
+    const uint32_t nsteps = (uint32_t)(Gpad / 64);
+    const size_t bblocks = (reps + WB_ROWS - 1) / WB_ROWS, Bpad = bblocks * WB_ROWS;
+    const int cgroups = (ntiles + WB_NTC - 1) / WB_NTC;
+    // split the G range so the grid fills the chip (~4 workgroups per CU)
+    uint32_t nsplit = (uint32_t)std::max<size_t>(1, std::min<size_t>(nsteps, (1024 + bblocks * cgroups - 1) / (bblocks * cgroups)));
+    const uint32_t per = (nsteps + nsplit - 1) / nsplit;
+    nsplit = (nsteps + per - 1) / per;
+    DBuf<int32_t> part((size_t)nsplit * Bpad * ncolp);
+    hipLaunchKernelGGL(k_wb_gemm, dim3((unsigned)bblocks, nsplit, cgroups), dim3(TPB), 0, st, Pq.p, Gpad, ntiles,
+                       (uint64_t)spec.seed, (int)full, per, nsteps, part.p, Bpad, ncolp);
+    FH_CHECK_LAUNCH();
+    DBuf<double> coefs(reps * 5);
+    hipLaunchKernelGGL(k_wb_post, dim3(grid_for(reps, TPB, 4096)), dim3(TPB), 0, st, part.p, (int)nsplit, Bpad, ncolp,
+                       NC, mk, reps, dscale.p, dAA.p, dBB.p, in.ssc, coefs.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> hc(reps * 5);
+    FH_HIP(hipMemcpyAsync(hc.data(), coefs.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    // inference (ols.rs:553-649)
+    const double delta = spec.r - point;
+    std::vector<double> tb(reps);
+    for (uint64_t b = 0; b < reps; b++) tb[b] = eval_affine_t(delta, &hc[b * 5]);
+    auto pv = compute_pvalue(stat, tb, spec.pval_type);
+    res.stat = stat;
+    res.p_val = pv.first;
+    res.reps_feasible = pv.second;
+    res.se = se;
+    res.point = point;
+    res.ssc = in.ssc;
+    res.reps = reps;
+    res.n_clusters = G;
+    res.full_enumeration = full;
+    res.ci_lo = NAN;
+    res.ci_hi = NAN;
+    if (spec.conf_int) {
+        std::vector<double> trial(reps);
+        auto ci = invert_ci_search(point, se, 1.0 - spec.level, spec.pval_type, spec.tol, spec.maxiter, pv.second,
+                                   [&](double r_trial) {
+                                       const double d = r_trial - point, st_trial = (point - r_trial) / se;
+                                       for (uint64_t b = 0; b < reps; b++) trial[b] = eval_affine_t(d, &hc[b * 5]);
+                                       return compute_pvalue(st_trial, trial, spec.pval_type).first;
+                                   });
+        res.ci_lo = ci.first;
+        res.ci_hi = ci.second;
+    }
+    if (t_boot_host) std::memcpy(t_boot_host, tb.data(), reps * sizeof(double));
+    res.seconds_bootstrap = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace fh

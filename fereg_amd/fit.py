@@ -101,3 +101,68 @@ def fepois(y, X, codes, g, vcov="hetero", cluster=None, nested=None, ssc=None, t
     """Poisson PML by IRLS (glm/irls.rs:367-913) on the GPU."""
     return _fit("fh_ppml_fit", y, X, codes, g, None, vcov, cluster, nested, ssc, opts, absorber,
                 [ctypes.c_double(tol), ctypes.c_uint64(maxiter)])
+
+
+@dataclass
+class WildBootResult:
+    stat: float
+    p_val: float
+    se: float
+    point: float
+    conf_int: tuple
+    reps: int
+    reps_feasible: int
+    n_clusters: int
+    full_enumeration: bool
+    fe_not_projected: bool
+    ssc: float
+    seconds_bootstrap: float
+    t_boot: np.ndarray
+    fit: FitResult
+
+
+def wildboot(y, X, cluster, R, r=0.0, reps=999, seed=1, level=0.95, pval_type="two-tailed", conf_int=True,
+             tol=1e-6, maxiter=50, codes=None, g=None, nested=None, ssc=None, opts=None, absorber=None):
+    """WCR11 wild cluster bootstrap of R beta = r after OLS (wildboot/ols.rs:243-649): Rademacher
+    weights from SplitMix64 streams (seed, draw), bootstrap clusters = error clusters, p-value and
+    test-inverted interval. The draws x clusters contraction runs on int8 MFMA."""
+    lib = L.lib()
+    Xa = X if _is_torch(X) else np.asarray(X, float)
+    if not _is_torch(Xa) and Xa.ndim == 1:
+        Xa = Xa[:, None]
+    K = Xa.shape[1]
+    ab = absorber or Absorber(codes, g)
+    ssc = ssc or {}
+    sscc = L.Ssc(int(ssc.get("K.adj", True)), L.FH_KFIXEF[ssc.get("K.fixef", "nonnested")], int(ssc.get("G.adj", True)), 0)
+    nt = ab.info.nterms
+    nest = np.zeros(max(nt, 1), np.int32)
+    if nested is not None:
+        nest[:nt] = np.asarray(nested, np.int32)
+    Rv = np.ascontiguousarray(np.asarray(R, float).reshape(-1))
+    assert Rv.size == K, "R needs one entry per regressor"
+    spec = L.WbootSpec(int(reps), int(seed), _ptr(Rv, L.f64p), float(r), float(level), float(tol), int(maxiter),
+                       L.FH_PVAL[pval_type], int(bool(conf_int)))
+    beta = np.zeros(K)
+    V = np.zeros(K * K)
+    kept = np.zeros(K, np.int32)
+    res = L.WbootResult()
+    tb = np.full(int(reps), np.nan)  # full enumeration never exceeds the requested reps
+    o = (opts or SolveOpts(tol=1e-10)).c()
+    if _is_torch(y):
+        fn = lib.fh_ols_wildboot_dev
+        args = [ab._h, y.contiguous().data_ptr(), Xa.contiguous().data_ptr(), K, cluster.contiguous().data_ptr()]
+    else:
+        fn = lib.fh_ols_wildboot
+        yv = np.ascontiguousarray(y, dtype=np.float64)
+        Xv = np.ascontiguousarray(Xa, dtype=np.float64)
+        cl = np.ascontiguousarray(cluster, dtype=np.uint32)
+        args = [ab._h, _ptr(yv, L.f64p), _ptr(Xv, L.f64p), K, _ptr(cl, L.u32p)]
+    L.check(fn(*args, _ptr(nest, L.i32p), ctypes.byref(sscc), ctypes.byref(o), ctypes.byref(spec), _ptr(beta, L.f64p),
+               _ptr(V, L.f64p), _ptr(kept, L.i32p), ctypes.byref(res), _ptr(tb, L.f64p)))
+    if absorber is None:
+        ab.close()
+    fit = FitResult(beta, V.reshape(K, K), kept.astype(bool), float("nan"), float("nan"), float("nan"), 0.0, 0, 0,
+                    int(res.n_clusters))
+    return WildBootResult(res.stat, res.p_val, res.se, res.point, (res.ci_lo, res.ci_hi), int(res.reps),
+                          int(res.reps_feasible), int(res.n_clusters), bool(res.full_enumeration),
+                          bool(res.fe_not_projected), res.ssc, res.seconds_bootstrap, tb[:int(res.reps)].copy(), fit)

@@ -192,6 +192,43 @@ int fh_ppml_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K,
                     uint64_t maxiter, const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept,
                     fh_fit_stats* stats);
 
+/* WCR11 wild cluster bootstrap of one linear hypothesis R beta = r on the OLS fit, Rademacher
+ * weights, clusters = bootstrap clusters (wildboot/ols.rs:243-649 run_cluster_11; weights.rs;
+ * rng.rs SplitMix64; mod.rs:529-749 p-value and test-inverted interval). The B x G sign by
+ * cluster-score contraction runs on int8 MFMA with the scores split into 7 signed 8-bit limbs
+ * (exact int32 accumulation, FP64-equivalent). Fixed effects not nested in the clusters are not
+ * projected out of the bootstrap scores (the reference's "TooLarge" plan); result flag set. */
+#define FH_PVAL_TWO_TAILED 0
+#define FH_PVAL_EQUAL_TAILED 1
+#define FH_PVAL_GREATER 2
+#define FH_PVAL_LESS 3
+typedef struct {
+    uint64_t reps;        /* replications (full enumeration when G <= 20 and 2^G <= reps) */
+    uint64_t seed;        /* SplitMix64 seed; draw b uses stream(seed, b) */
+    const double* r_mat;  /* R, K entries (host memory) */
+    double r;             /* tested value */
+    double level;         /* confidence level of the interval, e.g. 0.95 */
+    double tol;           /* interval search tolerance (1e-6) */
+    uint64_t maxiter;     /* interval search iterations (50) */
+    int32_t pval_type;    /* FH_PVAL_* */
+    int32_t conf_int;     /* 1: invert the test for an interval */
+} fh_wboot_spec;
+typedef struct {
+    double stat, p_val, se, point, ci_lo, ci_hi, ssc;
+    uint64_t reps, reps_feasible, n_clusters;
+    int32_t full_enumeration;
+    int32_t fe_not_projected; /* absorbed terms not nested in the clusters were left in the scores */
+    double seconds_bootstrap;
+} fh_wboot_result;
+/* OLS (as fh_ols_fit, one-way clustered variance by `cluster`) followed by the bootstrap;
+ * t_boot (reps entries, host) may be NULL. Codes must be the dense cluster factor 0..G-1. */
+int fh_ols_wildboot(fh_absorber* ab, const double* y, const double* X, size_t K, const uint32_t* cluster,
+                    const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, const fh_wboot_spec* spec,
+                    double* beta, double* vcov, int32_t* kept, fh_wboot_result* res, double* t_boot);
+int fh_ols_wildboot_dev(fh_absorber* ab, const double* y, const double* X, size_t K, const uint32_t* cluster,
+                        const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, const fh_wboot_spec* spec,
+                        double* beta, double* vcov, int32_t* kept, fh_wboot_result* res, double* t_boot);
+
 /* Device memory helpers for callers without another allocator. */
 int fh_device_alloc(void** p, size_t bytes);
 int fh_device_free(void* p);

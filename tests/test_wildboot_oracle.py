@@ -1,0 +1,116 @@
+"""The WCR11 bootstrap oracle pinned on the CPU: SplitMix64 known answers, the Rademacher bit
+order, AS241 against scipy, the fast cell-score formula against the brute-force refit (FE nested
+in the clusters, where the two are the same statistic), the sign symmetry of full enumeration
+and the test-inverted interval."""
+import math
+
+import numpy as np
+import pytest
+
+from fixtures import dense_residualize, dummies
+from oracle import wildboot as wb
+
+
+def test_splitmix64_known_answers():
+    # published first outputs of SplitMix64 seeded with 0
+    assert [int(x) for x in wb.splitmix64(0, 3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_stream_and_rademacher_bit_order():
+    seed, b, G = 77, 5, 150
+    # stream(seed, b): new(seed ^ b * K) with its first draw discarded (rng.rs:14-18)
+    raw = wb.splitmix64(seed ^ ((b * wb.STREAM) & wb.M64), 4)
+    words = wb.stream_words(seed, b, 3)
+    assert [int(x) for x in words] == [int(x) for x in raw[1:4]]
+    V, full = wb.rademacher(seed, b + 1, G)
+    assert not full
+    for c in range(G):
+        bit = (int(words[c // 64]) >> (c % 64)) & 1
+        assert V[b, c] == (-1.0 if bit else 1.0)
+    # full enumeration: draw b, cluster c = bit c of b
+    V, full = wb.rademacher(1, 5000, 12)
+    assert full and V.shape == (4096, 12)
+    assert V[0].tolist() == [1.0] * 12 and V[4095].tolist() == [-1.0] * 12
+    assert V[5, 0] == -1.0 and V[5, 1] == 1.0 and V[5, 2] == -1.0
+
+
+def test_qnorm_matches_scipy():
+    from scipy.stats import norm
+    for p in (1e-300, 1e-12, 1e-4, 0.025, 0.3, 0.5, 0.7, 0.975, 1 - 1e-9):
+        assert abs(wb.qnorm(p) - norm.ppf(p)) <= 1e-12 * max(1.0, abs(norm.ppf(p)))
+
+
+def nested_design(n, G, per, k, weighted, seed):
+    """Clusters 0..G-1; FE 'store' with `per` stores inside every cluster (nested) and a
+    cluster FE; k regressors."""
+    rng = np.random.default_rng(seed)
+    cl = rng.integers(0, G, n)
+    store = cl * per + rng.integers(0, per, n)
+    X = rng.standard_normal((n, k)) + 0.3 * rng.standard_normal(G)[cl][:, None]
+    y = X @ np.linspace(0.2, 1.0, k) + rng.standard_normal(G * per)[store] + rng.standard_normal(n) * (1 + 0.5 * (cl % 3))
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    return y, X, cl, store, w
+
+
+def fit_fwl(y, X, codes, gs, w):
+    """Residualize, beta, bread and residuals by the dense projection."""
+    Z = dense_residualize(np.column_stack([y, X]), codes, gs, w)
+    yt, Xt = Z[:, 0], Z[:, 1:]
+    wv = np.ones(len(y)) if w is None else w
+    A = Xt.T @ (wv[:, None] * Xt)
+    bread = np.linalg.inv(A)
+    beta = bread @ (Xt.T @ (wv * yt))
+    return Xt, yt - Xt @ beta, beta, bread
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("G,reps", [(40, 199), (8, 999)])
+def test_fast_formula_equals_brute_force_refit(weighted, G, reps):
+    n, per, k = 480, 3, 2
+    y, X, cl, store, w = nested_design(n, G, per, k, weighted, seed=11 + G)
+    Xt, u, beta, bread = fit_fwl(y, X, [store], [G * per], w)
+    R = np.array([1.0, 0.0])
+    r = 0.1
+    ssc = 1.37  # any factor: the same multiplier enters both computations
+    res = wb.wcr11_result(beta, Xt, u, w, cl, G, bread, R, r, ssc, reps, seed=2024)
+    D = dummies(store, G * per)
+    stat, se, tb = wb.brute_t_boot(X, y, w, D, cl, G, R, r, reps, 2024, ssc)
+    assert res["full"] == (G == 8)
+    assert abs(res["stat"] - stat) <= 1e-9 * abs(stat)
+    assert abs(res["se"] - se) <= 1e-9 * se
+    np.testing.assert_allclose(res["t_boot"], tb, rtol=1e-8, atol=1e-9)
+
+
+def test_full_enumeration_sign_symmetry_and_interval():
+    G = 10
+    y, X, cl, store, w = nested_design(400, G, 2, 2, False, seed=5)
+    Xt, u, beta, bread = fit_fwl(y, X, [store], [G * 2], None)
+    R = np.array([0.0, 1.0])
+    res = wb.wcr11_result(beta, Xt, u, None, cl, G, bread, R, 0.0, 1.0, 5000, seed=1)
+    tb = res["t_boot"]
+    assert res["full"] and tb.size == 1 << G
+    # draw b and its complement flip every sign: t is odd in V
+    comp = (1 << G) - 1 - np.arange(1 << G)
+    np.testing.assert_allclose(tb[comp], -tb, rtol=1e-10, atol=1e-12)
+    lo, hi = res["ci"]
+    assert lo < res["point"] < hi
+    c = res["coefs"]
+
+    def p_at(rv):
+        d = rv - res["point"]
+        st = (res["point"] - rv) / res["se"]
+        return wb.compute_pvalue(st, wb.eval_affine_t(d, *c.T))[0]
+
+    width = hi - lo
+    assert p_at(res["point"]) >= 0.05
+    assert p_at(lo - 0.05 * width) < 0.05 + 2.0 / tb.size and p_at(hi + 0.05 * width) < 0.05 + 2.0 / tb.size
+    assert p_at(lo + 0.05 * width) >= 0.05 - 2.0 / tb.size and p_at(hi - 0.05 * width) >= 0.05 - 2.0 / tb.size
+
+
+def test_pvalue_counts_with_tolerance():
+    t = np.array([1.0, -2.0, 2.0 + 1e-13, 3.0, np.nan, -3.0])
+    p, feas = wb.compute_pvalue(2.0, t, "two-tailed")
+    assert feas == 5 and p == pytest.approx(2 / 5)
+    assert wb.compute_pvalue(2.0, t, "greater")[0] == pytest.approx(1 / 5)
+    assert wb.compute_pvalue(2.0, t, "less")[0] == pytest.approx(3 / 5)
+    assert math.isnan(wb.compute_pvalue(float("nan"), t)[0])
