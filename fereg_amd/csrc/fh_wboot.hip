@@ -191,13 +191,29 @@ __global__ __launch_bounds__(TPB) void k_wb_gemm(const int8_t* __restrict__ Pq, 
         for (int t = 0; t < WB_NTC; t++) acc[rt][t] = v4i{0, 0, 0, 0};
     const uint32_t s0 = blockIdx.y * steps_per_split;
     const uint32_t s1 = min(nsteps, s0 + steps_per_split);
+    // the next step's slice is loaded into registers while the current one is multiplied
+    constexpr int PER = (WB_NTC * 64 + TPB - 1) / TPB;
+    v4i pre[PER];
+    auto fetch = [&](uint32_t s) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int it = threadIdx.x + u * TPB;
+            if (it < ntc * 64) {
+                const int col = it >> 2, q = it & 3;
+                pre[u] = *reinterpret_cast<const v4i*>(Pq + (size_t)(tile0 * 16 + col) * Gpad + (size_t)s * 64 + q * 16);
+            }
+        }
+    };
+    if (s0 < s1) fetch(s0);
     for (uint32_t s = s0; s < s1; s++) {
         __syncthreads();
-        for (int it = threadIdx.x; it < ntc * 64; it += TPB) {
-            const int col = it >> 2, q = it & 3;
-            lds[col][q] = *reinterpret_cast<const v4i*>(Pq + (size_t)(tile0 * 16 + col) * Gpad + (size_t)s * 64 + q * 16);
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int it = threadIdx.x + u * TPB;
+            if (it < ntc * 64) lds[it >> 2][it & 3] = pre[u];
         }
         __syncthreads();
+        if (s + 1 < s1) fetch(s + 1);
         v4i a[2];
 #pragma unroll
         for (int rt = 0; rt < 2; rt++) {
@@ -228,66 +244,66 @@ __global__ __launch_bounds__(TPB) void k_wb_gemm(const int8_t* __restrict__ Pq, 
         }
 }
 
-// Per draw: recombine the limbs (exact int64 sums over the splits, then FP64) and form the
-// affine coefficients (num0, num1, v00, v01, v11) of t(delta).
-__global__ void k_wb_post(const int32_t* __restrict__ part, int nsplit, size_t Bpad, int ncolp, int NC, int mk,
-                          uint64_t reps, const double* __restrict__ scale, const double* __restrict__ AA,
-                          const double* __restrict__ BB, double ssc, double* __restrict__ coefs) {
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < reps; b += (uint64_t)gridDim.x * blockDim.x) {
-        auto y = [&](int c) {
+// Per draw (one wave each): sum the splits (exact, int64) lane-strided over the limb columns,
+// recombine the limbs in FP64, then lane 0 forms the affine coefficients
+// (num0, num1, v00, v01, v11) of t(delta).
+__global__ __launch_bounds__(TPB) void k_wb_post(const int32_t* __restrict__ part, int nsplit, size_t Bpad, int ncolp,
+                                                 int NC, int mk, uint64_t reps, const double* __restrict__ scale,
+                                                 const double* __restrict__ AA, const double* __restrict__ BB,
+                                                 double ssc, double* __restrict__ coefs) {
+    __shared__ long long S[TPB / WAVE][LIMBS * (2 + 4 * KMAX)];
+    __shared__ double Y[TPB / WAVE][2 + 4 * KMAX];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    for (uint64_t b = (uint64_t)blockIdx.x * (TPB / WAVE) + w; b < reps; b += (uint64_t)gridDim.x * (TPB / WAVE)) {
+        for (int col = lane; col < LIMBS * NC; col += WAVE) {
+            long long s = 0;
+            for (int k = 0; k < nsplit; k++) s += part[((size_t)k * Bpad + b) * ncolp + col];
+            S[w][col] = s;
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int c = lane; c < NC; c += WAVE) {
             double v = 0.0;
-            for (int l = LIMBS - 1; l >= 0; l--) {
-                long long s = 0;
-                for (int k = 0; k < nsplit; k++) s += part[((size_t)k * Bpad + b) * ncolp + (size_t)l * NC + c];
-                v += ldexp((double)s, -8 * (l + 1));
+            for (int l = LIMBS - 1; l >= 0; l--) v += ldexp((double)S[w][l * NC + c], -8 * (l + 1));
+            Y[w][c] = v * scale[c];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+            const double* y = Y[w];
+            const double* db0 = y + 2;
+            const double* db1 = y + 2 + mk;
+            const double* ea0 = y + 2 + 2 * mk;
+            const double* ea1 = y + 2 + 3 * mk;
+            double e00 = 0.0, e01 = 0.0, e10 = 0.0, e11 = 0.0, q00 = 0.0, q01 = 0.0, q11 = 0.0;
+            for (int a = 0; a < mk; a++) {
+                e00 += db0[a] * ea0[a];
+                e01 += db0[a] * ea1[a];
+                e10 += db1[a] * ea0[a];
+                e11 += db1[a] * ea1[a];
+                double r0 = 0.0, r1 = 0.0;
+                for (int c = 0; c < mk; c++) {
+                    r0 += BB[a * mk + c] * db0[c];
+                    r1 += BB[a * mk + c] * db1[c];
+                }
+                q00 += db0[a] * r0;
+                q01 += db0[a] * r1;
+                q11 += db1[a] * r1;
             }
-            return v * scale[c];
-        };
-        double db0[KMAX], db1[KMAX];
-        const double n0 = y(0), n1 = y(1);
-        double e00 = 0.0, e01 = 0.0, e10 = 0.0, e11 = 0.0;  // db_s . EA_t
-        for (int a = 0; a < mk; a++) {
-            db0[a] = y(2 + a);
-            db1[a] = y(2 + mk + a);
+            double* o = coefs + b * 5;
+            o[0] = y[0];
+            o[1] = y[1];
+            o[2] = ssc * (AA[0] - 2.0 * e00 + q00);
+            o[3] = ssc * (AA[1] - e10 - e01 + q01);
+            o[4] = ssc * (AA[3] - 2.0 * e11 + q11);
         }
-        for (int a = 0; a < mk; a++) {
-            const double ea0 = y(2 + 2 * mk + a), ea1 = y(2 + 3 * mk + a);
-            e00 += db0[a] * ea0;
-            e01 += db0[a] * ea1;
-            e10 += db1[a] * ea0;
-            e11 += db1[a] * ea1;
-        }
-        double q00 = 0.0, q01 = 0.0, q11 = 0.0;  // db_s' BB db_t
-        for (int a = 0; a < mk; a++) {
-            double r0 = 0.0, r1 = 0.0;
-            for (int c = 0; c < mk; c++) {
-                r0 += BB[a * mk + c] * db0[c];
-                r1 += BB[a * mk + c] * db1[c];
-            }
-            q00 += db0[a] * r0;
-            q01 += db0[a] * r1;
-            q11 += db1[a] * r1;
-        }
-        double* o = coefs + b * 5;
-        o[0] = n0;
-        o[1] = n1;
-        o[2] = ssc * (AA[0] - 2.0 * e00 + q00);
-        o[3] = ssc * (AA[1] - e10 - e01 + q01);
-        o[4] = ssc * (AA[3] - 2.0 * e11 + q11);
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
-// lo/hi[level offset of term + level] = min/max cluster over the rows of the level; one thread
-// per cell (every row of a cell has the same levels).
-struct TermLevels {
-    uint32_t off[NB_MAX + 1];  // level offset of FE1 then of each block
-    int nb;
-    int has_fe1;
-};
-__global__ void k_wb_level_span(const uint32_t* __restrict__ cell_seg, const uint32_t* __restrict__ cell_codes,
-                                const uint32_t* __restrict__ coo, const uint32_t* __restrict__ pi,
-                                const uint32_t* __restrict__ cl, size_t M, TermLevels tl, uint32_t* __restrict__ lo,
-                                uint32_t* __restrict__ hi) {
+// Nesting of the absorbed terms in the clusters (dof.rs nested_within, by value), with no
+// atomics on hot addresses: min/max cluster per cell, then per FE1 level over its cells
+// (seg_off) and per block level over its level-major positions.
+__global__ void k_wb_cell_span(const uint32_t* __restrict__ coo, const uint32_t* __restrict__ pi,
+                               const uint32_t* __restrict__ cl, size_t M, uint2* __restrict__ span) {
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
         const size_t r0 = coo ? coo[c] : c, r1 = coo ? coo[c + 1] : c + 1;
         uint32_t mn = 0xFFFFFFFFu, mx = 0u;
@@ -296,29 +312,54 @@ __global__ void k_wb_level_span(const uint32_t* __restrict__ cell_seg, const uin
             mn = min(mn, k);
             mx = max(mx, k);
         }
-        int t = 0;
-        if (tl.has_fe1) {
-            const uint32_t l = tl.off[0] + cell_seg[c];
-            atomicMin(&lo[l], mn);
-            atomicMax(&hi[l], mx);
-            t = 1;
-        }
-        for (int b = 0; b < tl.nb; b++) {
-            const uint32_t l = tl.off[t + b] + cell_codes[c * tl.nb + b];
-            atomicMin(&lo[l], mn);
-            atomicMax(&hi[l], mx);
-        }
+        span[c] = make_uint2(mn, mx);
     }
 }
-__global__ void k_wb_span_flags(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, size_t L,
-                                const uint32_t* __restrict__ term_of_level_off, int nt, uint32_t* __restrict__ bad) {
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < L; l += (size_t)gridDim.x * blockDim.x) {
-        if (lo[l] <= hi[l] && lo[l] != hi[l]) {
-            int t = 0;
-            while (t + 1 < nt && l >= term_of_level_off[t + 1]) t++;
-            atomicOr(&bad[t], 1u);
+
+// *bad |= some group (cells off[q]..off[q+1], through perm when given) spans clusters; one
+// flag write per workgroup. Groups: FE1 levels (off = seg_off) or (bucket, level) items.
+__global__ __launch_bounds__(TPB) void k_wb_group_span(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ perm, const uint2* __restrict__ span,
+                                                       size_t ngroups, uint2* __restrict__ gspan,
+                                                       uint32_t* __restrict__ bad) {
+    bool b = false;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < ngroups; q += (size_t)gridDim.x * blockDim.x) {
+        uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+        for (uint32_t j = off[q]; j < off[q + 1]; j++) {
+            const uint2 s = span[perm ? perm[j] : j];
+            mn = min(mn, s.x);
+            mx = max(mx, s.y);
         }
+        if (gspan) gspan[q] = make_uint2(mn, mx);
+        b |= mn < mx;
     }
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (__ballot(b) != 0 && lane_id() == 0) atomicOr(&any, 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && any) atomicOr(bad, 1u);
+}
+
+// Levels cut into nbk buckets: combine the item spans of each level.
+__global__ __launch_bounds__(TPB) void k_wb_level_span(const uint2* __restrict__ gspan, uint32_t nbk, size_t g,
+                                                       uint32_t* __restrict__ bad) {
+    bool b = false;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
+        uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+        for (uint32_t k = 0; k < nbk; k++) {
+            const uint2 s = gspan[(size_t)k * g + l];
+            mn = min(mn, s.x);
+            mx = max(mx, s.y);
+        }
+        b |= mn < mx;
+    }
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (__ballot(b) != 0 && lane_id() == 0) atomicOr(&any, 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && any) atomicOr(bad, 1u);
 }
 
 // ------------------------------------------------------------------ host: inference
@@ -469,33 +510,34 @@ std::pair<double, double> invert_ci_search(double point, double se, double alpha
 // (dof.rs nested_within, by value).
 std::vector<int> terms_nested_in(Absorber& a, const uint32_t* cl) {
     hipStream_t st = a.st;
-    TermLevels tl{};
-    std::vector<uint32_t> offs;
-    std::vector<int64_t> term_of;  // absorber term of each level range
-    uint64_t L = 0;
-    tl.has_fe1 = a.fe1 >= 0;
-    if (a.fe1 >= 0) { offs.push_back((uint32_t)L); term_of.push_back(a.fe1); L += a.g1; }
-    tl.nb = (int)a.nb;
-    for (auto& b : a.blocks) { offs.push_back((uint32_t)L); term_of.push_back(b.term); L += b.g; }
     std::vector<int> nested(a.nterms, 1);
-    if (L == 0 || a.M == 0) return nested;
-    require(L < (1ull << 32), "too many levels");
-    for (size_t i = 0; i < offs.size(); i++) tl.off[i] = offs[i];
-    DBuf<uint32_t> lo(L), hi(L), toff(offs.size()), bad(offs.size());
-    FH_HIP(hipMemsetAsync(lo.p, 0xFF, lo.bytes(), st));
-    FH_HIP(hipMemsetAsync(hi.p, 0, hi.bytes(), st));
+    if (a.M == 0) return nested;
+    const size_t nflag = a.nb + 1;
+    DBuf<uint2> span(a.M);
+    DBuf<uint32_t> bad(nflag);
     FH_HIP(hipMemsetAsync(bad.p, 0, bad.bytes(), st));
-    FH_HIP(hipMemcpyAsync(toff.p, offs.data(), offs.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_wb_level_span, dim3(grid_for(a.M, TPB, 8192)), dim3(TPB), 0, st, a.cell_seg.p, a.cell_codes.p,
-                       a.trivial() ? nullptr : a.cell_obs_off.p, a.pi.p, cl, (size_t)a.M, tl, lo.p, hi.p);
-    hipLaunchKernelGGL(k_wb_span_flags, dim3(grid_for(L, TPB, 8192)), dim3(TPB), 0, st, lo.p, hi.p, (size_t)L, toff.p,
-                       (int)offs.size(), bad.p);
+    hipLaunchKernelGGL(k_wb_cell_span, dim3(grid_for(a.M, TPB, 8192)), dim3(TPB), 0, st,
+                       a.trivial() ? nullptr : a.cell_obs_off.p, a.pi.p, cl, (size_t)a.M, span.p);
+    if (a.fe1 >= 0)
+        hipLaunchKernelGGL(k_wb_group_span, dim3(grid_for(a.g1, TPB, 4096)), dim3(TPB), 0, st, a.seg_off.p,
+                           (const uint32_t*)nullptr, span.p, (size_t)a.g1, (uint2*)nullptr, bad.p);
+    DBuf<uint2> gspan;
+    for (size_t b = 0; b < a.nb; b++) {
+        const size_t items = (size_t)a.nbk[b] * a.blocks[b].g;
+        if (a.nbk[b] > 1) gspan.reserve(items);
+        hipLaunchKernelGGL(k_wb_group_span, dim3(grid_for(items, TPB, 4096)), dim3(TPB), 0, st, a.boff[b].p,
+                           a.bperm[b].p, span.p, items, a.nbk[b] > 1 ? gspan.p : nullptr, bad.p + 1 + b);
+        if (a.nbk[b] > 1)
+            hipLaunchKernelGGL(k_wb_level_span, dim3(grid_for(a.blocks[b].g, TPB, 4096)), dim3(TPB), 0, st, gspan.p,
+                               a.nbk[b], (size_t)a.blocks[b].g, bad.p + 1 + b);
+    }
     FH_CHECK_LAUNCH();
-    std::vector<uint32_t> hb(offs.size());
-    FH_HIP(hipMemcpyAsync(hb.data(), bad.p, hb.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> hb(nflag);
+    FH_HIP(hipMemcpyAsync(hb.data(), bad.p, nflag * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     FH_HIP(hipStreamSynchronize(st));
-    for (size_t i = 0; i < offs.size(); i++)
-        if (hb[i]) nested[term_of[i]] = 0;
+    if (a.fe1 >= 0 && hb[0]) nested[a.fe1] = 0;
+    for (size_t b = 0; b < a.nb; b++)
+        if (hb[1 + b]) nested[a.blocks[b].term] = 0;
     return nested;
 }
 
@@ -592,7 +634,7 @@ void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, 
                        (uint64_t)spec.seed, (int)full, per, nsteps, part.p, Bpad, ncolp);
     FH_CHECK_LAUNCH();
     DBuf<double> coefs(reps * 5);
-    hipLaunchKernelGGL(k_wb_post, dim3(grid_for(reps, TPB, 4096)), dim3(TPB), 0, st, part.p, (int)nsplit, Bpad, ncolp,
+    hipLaunchKernelGGL(k_wb_post, dim3(grid_for(reps, TPB / WAVE, 8192)), dim3(TPB), 0, st, part.p, (int)nsplit, Bpad, ncolp,
                        NC, mk, reps, dscale.p, dAA.p, dBB.p, in.ssc, coefs.p);
     FH_CHECK_LAUNCH();
     std::vector<double> hc(reps * 5);
