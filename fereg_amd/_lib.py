@@ -91,6 +91,30 @@ class WbootResult(ctypes.Structure):
                 ("fe_not_projected", ctypes.c_int32), ("seconds_bootstrap", ctypes.c_double)]
 
 
+FH_MODEL = {"ols": 0, "ppml": 1}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64), ("precision", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int32), ("vcov_kind", ctypes.c_int32), ("n", ctypes.c_size_t),
+                ("K", ctypes.c_size_t), ("nterms", ctypes.c_size_t), ("terms", ctypes.POINTER(Term)),
+                ("y", ctypes.POINTER(ctypes.c_double)), ("X", ctypes.POINTER(ctypes.c_double)),
+                ("cluster", ctypes.POINTER(ctypes.c_uint32)), ("nested", ctypes.POINTER(ctypes.c_int32)),
+                ("ssc", Ssc), ("tol", ctypes.c_double), ("maxiter", ctypes.c_uint64),
+                ("beta", ctypes.POINTER(ctypes.c_double)), ("vcov", ctypes.POINTER(ctypes.c_double)),
+                ("kept", ctypes.POINTER(ctypes.c_int32))]
+
+
+class Response(ctypes.Structure):
+    _fields_ = [("rc", ctypes.c_int32), ("reserved", ctypes.c_int32), ("queued_s", ctypes.c_double),
+                ("service_s", ctypes.c_double), ("total_s", ctypes.c_double), ("iterations", ctypes.c_uint64),
+                ("pcg_iterations", ctypes.c_uint64)]
+
+
 # (name, restype, argtypes) of every exported entry point of include/fereg_hip.h
 SIGNATURES = [
     ("fh_last_error", ctypes.c_char_p, []),
@@ -135,6 +159,11 @@ SIGNATURES = [
     ("fh_ols_wildboot_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, vp, i32p, ctypes.POINTER(Ssc),
                                            ctypes.POINTER(SolveOptsC), ctypes.POINTER(WbootSpec), f64p, f64p, i32p,
                                            ctypes.POINTER(WbootResult), f64p]),
+    ("fh_server_config_default", None, [ctypes.POINTER(ServerConfig)]),
+    ("fh_server_create", ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+    ("fh_server_submit", ctypes.c_int, [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+    ("fh_server_wait", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(Response)]),
+    ("fh_server_destroy", None, [vp]),
     ("fh_device_alloc", ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),
     ("fh_device_free", ctypes.c_int, [vp]),
     ("fh_memcpy_h2d", ctypes.c_int, [vp, vp, ctypes.c_size_t]),

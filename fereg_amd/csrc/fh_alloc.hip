@@ -26,6 +26,17 @@ struct Block {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
+    bool idle = true;  // no work pending on it (its stream was synchronized and destroyed)
+};
+
+// The pool's own event queries must not leave an error for the caller's next
+// hipGetLastError(): clear what they set unless an error was already pending.
+struct ErrorHygiene {
+    hipError_t prev;
+    ErrorHygiene() : prev(hipPeekAtLastError()) {}
+    ~ErrorHygiene() {
+        if (prev == hipSuccess) (void)hipGetLastError();
+    }
 };
 
 struct Pool {
@@ -61,7 +72,7 @@ void release_completed(Pool& P) {
         if (prev != dev) (void)hipSetDevice(dev);
         for (auto it = fl.begin(); it != fl.end();) {
             Block& b = it->second;
-            if (b.ev && hipEventQuery(b.ev) != hipSuccess) { ++it; continue; }
+            if (!b.idle && b.ev && hipEventQuery(b.ev) != hipSuccess) { ++it; continue; }
             if (b.ev) (void)hipEventDestroy(b.ev);
             (void)hipFree(b.p);
             P.cached -= b.bytes;
@@ -86,13 +97,15 @@ void* pool_alloc(size_t bytes) {
     FH_HIP(hipGetDevice(&dev));
     const hipStream_t st = tls_stream;
     std::lock_guard<std::mutex> lk(P.mu);
+    ErrorHygiene eh;
     auto& fl = P.free_by_dev[dev];
     for (auto it = fl.lower_bound(rb); it != fl.end() && it->first <= 2 * rb; ++it) {
         Block& b = it->second;
-        if (b.stream != st && b.ev && hipEventQuery(b.ev) != hipSuccess) continue;
+        if (!b.idle && b.stream != st && b.ev && hipEventQuery(b.ev) != hipSuccess) continue;
         Block blk = b;
         fl.erase(it);
         blk.stream = st;
+        blk.idle = false;
         P.cached -= blk.bytes;
         P.in_use += blk.bytes;
         P.live.emplace(blk.p, blk);
@@ -102,12 +115,13 @@ void* pool_alloc(size_t bytes) {
     blk.bytes = rb;
     blk.device = dev;
     blk.stream = st;
+    blk.idle = false;
     hipError_t e = hipMalloc(&blk.p, rb);
     if (e == hipErrorOutOfMemory) {
-        (void)hipGetLastError();
         release_completed(P);
         e = hipMalloc(&blk.p, rb);
     }
+    if (e != hipSuccess) (void)hipGetLastError();  // reported by the throw below, not left pending
     FH_HIP(e);
     P.in_use += rb;
     P.live.emplace(blk.p, blk);
@@ -122,6 +136,7 @@ void pool_free(void* p) {
     if (it == P.live.end()) return;
     Block b = it->second;
     P.live.erase(it);
+    ErrorHygiene eh;
     int prev = -1;
     (void)hipGetDevice(&prev);
     if (prev != b.device) (void)hipSetDevice(b.device);
@@ -136,8 +151,20 @@ void pool_free(void* p) {
 void pool_release_cached() {
     Pool& P = pool();
     std::lock_guard<std::mutex> lk(P.mu);
+    ErrorHygiene eh;
     (void)hipDeviceSynchronize();
     release_completed(P);
+}
+
+void pool_forget_stream(hipStream_t st) {
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    for (auto& [dev, fl] : P.free_by_dev)
+        for (auto& [sz, b] : fl)
+            if (b.stream == st) {
+                b.idle = true;
+                b.stream = nullptr;
+            }
 }
 
 void pool_stats(size_t* in_use, size_t* cached) {

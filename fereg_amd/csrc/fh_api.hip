@@ -805,6 +805,7 @@ void fh_absorber_destroy(fh_absorber* ab) {
         DeviceGuard g(ab->a.device);
         (void)hipStreamSynchronize(st);
         delete ab;
+        pool_forget_stream(st);
         (void)hipStreamDestroy(st);
     }
 }

@@ -46,6 +46,9 @@ inline void require(bool ok, const std::string& msg, int rc = FH_INVALID) {
 void* pool_alloc(size_t bytes);  // on the current device, for use on current_stream()
 void pool_free(void* p);
 void pool_release_cached();
+// Marks the cached blocks last used on st as idle (call after st is synchronized, before it is
+// destroyed): later requests reuse them without querying events of a dead stream.
+void pool_forget_stream(hipStream_t st);
 void pool_stats(size_t* in_use, size_t* cached);
 hipStream_t current_stream();
 // Sets the stream that pool allocations of this thread are used on, for the scope's lifetime.

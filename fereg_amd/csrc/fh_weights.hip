@@ -124,19 +124,27 @@ struct SmallOffs {
     int nsb, nb;
 };
 
-// S_SS (+ n_r of the small coordinates in row ms) per workgroup: one thread per segment adds
+// S_SS (+ n_r of the small coordinates in row ms): one thread per segment adds
 // sum_c W_c [k][l] and subtracts sum_{c,c'} W_c W_c' / n1_s [k(c)][l(c')] (operator.rs:1457-1560).
-constexpr int SA_LDS_MAX = 64;  // small coordinates accumulated in LDS (ms*ms + ms doubles)
+// The terms are rounded to 64-bit fixed point (fscale = 2^(62 - ceil log2 sw), every partial sum
+// stays below 2^62) and added with integer atomics: the result is exact for the rounded terms and
+// independent of the order the atomics land in (deterministic under any scheduling). With
+// ms <= SA_LDS_MAX each workgroup accumulates in LDS and writes its partial to out[blockIdx.x].
+constexpr int SA_LDS_MAX = 64;
+__device__ __forceinline__ unsigned long long fixed(double v, double fscale) {
+    return (unsigned long long)llrint(v * fscale);  // two's complement for negative terms
+}
 __global__ __launch_bounds__(TPB) void k_small_assemble(const uint32_t* __restrict__ seg_off,
                                                         const uint32_t* __restrict__ cell_codes,
                                                         const double* __restrict__ cw, const double* __restrict__ n1,
-                                                        size_t g1, size_t ms, SmallOffs so, double* __restrict__ out) {
-    extern __shared__ double acc[];
+                                                        size_t g1, size_t ms, SmallOffs so, double fscale,
+                                                        unsigned long long* __restrict__ out) {
+    extern __shared__ unsigned long long acc[];
     const size_t len = ms * ms + ms;
     const bool lds = ms <= SA_LDS_MAX;
-    double* dst = lds ? acc : out;
+    unsigned long long* dst = lds ? acc : out;
     if (lds) {
-        for (size_t i = threadIdx.x; i < len; i += blockDim.x) acc[i] = 0.0;
+        for (size_t i = threadIdx.x; i < len; i += blockDim.x) acc[i] = 0ull;
         __syncthreads();
     }
     for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
@@ -144,21 +152,22 @@ __global__ __launch_bounds__(TPB) void k_small_assemble(const uint32_t* __restri
         double inv = 1.0 / n1[s];
         for (uint32_t c = c0; c < c1; c++) {
             double w = cw ? cw[c] : 1.0;
+            const unsigned long long fw = fixed(w, fscale);
             for (int b = 0; b < so.nsb; b++) {
                 uint32_t k = so.off[b] + cell_codes[(size_t)c * so.nb + b];
-                atomicAdd(&dst[ms * ms + k], w);
+                atomicAdd(&dst[ms * ms + k], fw);
                 for (int b2 = 0; b2 < so.nsb; b2++) {
                     uint32_t l = so.off[b2] + cell_codes[(size_t)c * so.nb + b2];
-                    atomicAdd(&dst[(size_t)k * ms + l], w);
+                    atomicAdd(&dst[(size_t)k * ms + l], fw);
                 }
             }
             for (uint32_t c2 = c0; c2 < c1; c2++) {
-                double w2 = w * (cw ? cw[c2] : 1.0) * inv;
+                const unsigned long long fw2 = fixed(-(w * (cw ? cw[c2] : 1.0) * inv), fscale);
                 for (int b = 0; b < so.nsb; b++) {
                     uint32_t k = so.off[b] + cell_codes[(size_t)c * so.nb + b];
                     for (int b2 = 0; b2 < so.nsb; b2++) {
                         uint32_t l = so.off[b2] + cell_codes[(size_t)c2 * so.nb + b2];
-                        atomicAdd(&dst[(size_t)k * ms + l], -w2);
+                        atomicAdd(&dst[(size_t)k * ms + l], fw2);
                     }
                 }
             }
@@ -166,8 +175,18 @@ __global__ __launch_bounds__(TPB) void k_small_assemble(const uint32_t* __restri
     }
     if (lds) {
         __syncthreads();
-        double* part = out + (size_t)blockIdx.x * len;
+        unsigned long long* part = out + (size_t)blockIdx.x * len;
         for (size_t i = threadIdx.x; i < len; i += blockDim.x) part[i] = acc[i];
+    }
+}
+
+// out[i] = (sum over the partials of the fixed-point parts[q][i]) / fscale (integer sums: exact).
+__global__ void k_fixed_to_double(const unsigned long long* __restrict__ parts, size_t nparts, size_t len,
+                                  double inv_fscale, double* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+        unsigned long long s = 0;
+        for (size_t q = 0; q < nparts; q++) s += parts[q * len + i];
+        out[i] = (double)(long long)s * inv_fscale;
     }
 }
 
@@ -234,24 +253,6 @@ __global__ void k_level_small_sums(const uint32_t* __restrict__ boff, const uint
     }
 }
 
-__global__ __launch_bounds__(TPB) void k_reduce_parts(const double* __restrict__ parts, size_t nparts, size_t len,
-                                                     double* __restrict__ out) {
-    __shared__ double red[TPB / WAVE];
-    for (size_t i = blockIdx.x; i < len; i += gridDim.x) {
-        double s = 0.0;
-        for (size_t q = threadIdx.x; q < nparts; q += TPB) s += parts[q * len + i];
-        s = wave_sum(s);
-        if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int w = 0; w < (int)(TPB / WAVE); w++) t += red[w];
-            out[i] = t;
-        }
-        __syncthreads();
-    }
-}
-
 double sum_dev(const double* v, size_t n, hipStream_t st) {
     const unsigned nb = 256;
     DBuf<double> part(nb);
@@ -263,6 +264,64 @@ double sum_dev(const double* v, size_t n, hipStream_t st) {
     double s = 0.0;
     for (double x : h) s += x;
     return s;
+}
+
+// ---- grounded pseudo-inverse on the GPU (large small super-blocks)
+// diag and n_r of the assembled S_SS (len ms*ms + ms) for the host's redundancy decisions.
+__global__ void k_extract_diag(const double* __restrict__ S, size_t ms, double* __restrict__ out) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < ms; k += (size_t)gridDim.x * blockDim.x) {
+        out[k] = S[k * ms + k];
+        out[ms + k] = S[ms * ms + k];
+    }
+}
+// A[i][j] = S[keep_i][keep_j] sc_i sc_j (unit-diagonal scaling); col = column 0 of A.
+__global__ void k_sweep_init(const double* __restrict__ S, size_t ms, const uint32_t* __restrict__ keep,
+                             const double* __restrict__ sc, int k, double* __restrict__ A, double* __restrict__ col) {
+    const size_t kk = (size_t)k * k;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < kk; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = e / k, l = e % k;
+        const double v = S[(size_t)keep[i] * ms + keep[l]] * sc[i] * sc[l];
+        A[e] = v;
+        if (l == 0) col[i] = v;
+    }
+}
+// Gauss-Jordan sweep of pivot j (Goodnight's operator on a symmetric matrix): with d the current
+// diagonal, A[i][l] -= A[i][j] A[j][l] / d, row/column j become A[.][j] / d and A[j][j] = -1/d.
+// A pivot d <= 1e-10 is grounded: left unswept (precond/dense.rs:25-37 replaces it by 1e30,
+// which removes the direction), so after all pivots -A is the inverse over the swept indices.
+// col holds column j before the step; the step writes column j+1 after it into colnext.
+__global__ void k_sweep_step(double* __restrict__ A, int k, int j, const double* __restrict__ col,
+                             double* __restrict__ colnext, uint8_t* __restrict__ grounded) {
+    const double d = col[j];
+    const bool g = !(d > 1e-10);
+    const double inv = g ? 0.0 : 1.0 / d;
+    const size_t kk = (size_t)k * k;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < kk; e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e / k), l = (int)(e % k);
+        double v = A[e];
+        if (!g) {
+            if (i == j && l == j) v = -inv;
+            else if (i == j) v = col[l] * inv;
+            else if (l == j) v = col[i] * inv;
+            else v -= col[i] * col[l] * inv;
+            A[e] = v;
+        }
+        if (l == j + 1) colnext[i] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) grounded[j] = g;
+}
+// minv[keep_i][keep_l] = -A[i][l] sc_i sc_l over the swept pivots (0 elsewhere).
+__global__ void k_sweep_scatter(const double* __restrict__ A, int k, const uint32_t* __restrict__ keep,
+                                const double* __restrict__ sc, const uint8_t* __restrict__ grounded, size_t ms,
+                                double* __restrict__ minv, float* __restrict__ minv32) {
+    const size_t kk = (size_t)k * k;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < kk; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = e / k, l = e % k;
+        const double v = (grounded[i] || grounded[l]) ? 0.0 : -A[e] * sc[i] * sc[l];
+        const size_t o = (size_t)keep[i] * ms + keep[l];
+        minv[o] = v;
+        minv32[o] = (float)v;
+    }
 }
 
 // Grounded LDL' of S_SS over the kept coordinates and its explicit inverse
@@ -427,35 +486,80 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
                 for (size_t l = 0; l < ms; l++) h[k * ms + l] = -G[k * (ms + 1) + l];
                 h[k * ms + k] += h[ms * ms + k];
             }
-        } else if (ms <= (size_t)SA_LDS_MAX) {
-            const unsigned grid = std::min<unsigned>(blocks_for(g1), 512);
-            DBuf<double> parts((size_t)grid * len);
-            hipLaunchKernelGGL(k_small_assemble, dim3(grid), dim3(TPB), len * sizeof(double), st, a.seg_off.p,
-                               a.cell_codes.p, cw, a.n1.p, g1, ms, so, parts.p);
-            FH_CHECK_LAUNCH();
-            hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)std::min<size_t>(len, 8192)), dim3(TPB), 0, st, parts.p,
-                               (size_t)grid, len, sfull.p);
-            FH_CHECK_LAUNCH();
-            FH_HIP(hipStreamSynchronize(st));
         } else {
-            FH_HIP(hipMemsetAsync(sfull.p, 0, sfull.bytes(), st));
-            hipLaunchKernelGGL(k_small_assemble, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p, a.cell_codes.p,
-                               cw, a.n1.p, g1, ms, so, sfull.p);
+            // 64-bit fixed point with every partial sum below 2^62 (terms and sums are <= sw)
+            const double fscale = std::ldexp(1.0, 62 - (int)std::ceil(std::log2(std::max(a.sw, 1.0)) + 1e-12));
+            const bool lds = ms <= (size_t)SA_LDS_MAX;
+            const unsigned grid = lds ? std::min<unsigned>(blocks_for(g1), 512) : blocks_for(g1);
+            const size_t nparts = lds ? grid : 1;
+            DBuf<unsigned long long> parts(nparts * len);
+            if (!lds) FH_HIP(hipMemsetAsync(parts.p, 0, parts.bytes(), st));
+            hipLaunchKernelGGL(k_small_assemble, dim3(grid), dim3(TPB), lds ? len * sizeof(unsigned long long) : 0, st,
+                               a.seg_off.p, a.cell_codes.p, cw, a.n1.p, g1, ms, so, fscale, parts.p);
+            hipLaunchKernelGGL(k_fixed_to_double, dim3(grid_for(len, TPB, 4096)), dim3(TPB), 0, st, parts.p, nparts, len,
+                               1.0 / fscale, sfull.p);
             FH_CHECK_LAUNCH();
         }
-        if (!(so.nsb == 1 && ms <= 21)) {
-            FH_HIP(hipMemcpyAsync(h.data(), sfull.p, len * sizeof(double), hipMemcpyDeviceToHost, st));
-            FH_HIP(hipStreamSynchronize(st));
-        }
+        // large blocks stay on the device: diag and n_r come to the host, the grounded inverse is
+        // formed by pivot sweeps on the GPU (O(ms^3) work; the host version is for small blocks)
+        const bool dev_inverse = !(so.nsb == 1 && ms <= 21) && ms > 64;
         std::vector<double> nrs(ms), dg(ms);
         std::vector<uint8_t> rd(ms);
-        std::vector<double> S(h.begin(), h.begin() + ms * ms);
-        for (size_t k = 0; k < ms; k++) {
-            nrs[k] = h[ms * ms + k];
-            dg[k] = S[k * ms + k];
-            rd[k] = dg[k] <= 1e-12 * nrs[k];
+        std::vector<double> inv;
+        if (dev_inverse) {
+            DBuf<double> dd(2 * ms);
+            hipLaunchKernelGGL(k_extract_diag, dim3(grid_for(ms, TPB, 1024)), dim3(TPB), 0, st, sfull.p, ms, dd.p);
+            FH_CHECK_LAUNCH();
+            FH_HIP(hipMemcpyAsync(h.data(), dd.p, 2 * ms * sizeof(double), hipMemcpyDeviceToHost, st));
+            FH_HIP(hipStreamSynchronize(st));
+            for (size_t k = 0; k < ms; k++) {
+                dg[k] = h[k];
+                nrs[k] = h[ms + k];
+                rd[k] = dg[k] <= 1e-12 * nrs[k];
+            }
+            std::vector<uint32_t> keep;
+            std::vector<double> sc;
+            for (size_t i = 0; i < ms; i++)
+                if (!rd[i] && dg[i] > 0.0) {
+                    keep.push_back((uint32_t)i);
+                    sc.push_back(1.0 / std::sqrt(dg[i]));
+                }
+            const int k = (int)keep.size();
+            a.nkeep = (uint64_t)k;
+            a.w64.minv.reserve(ms * ms);
+            a.w32.minv.reserve(ms * ms);
+            FH_HIP(hipMemsetAsync(a.w64.minv.p, 0, ms * ms * sizeof(double), st));
+            FH_HIP(hipMemsetAsync(a.w32.minv.p, 0, ms * ms * sizeof(float), st));
+            if (k > 0) {
+                const size_t kk = (size_t)k * k;
+                DBuf<uint32_t> dkeep(k);
+                DBuf<double> dsc(k), A(kk), col(2 * (size_t)k);
+                DBuf<uint8_t> grounded(k);
+                FH_HIP(hipMemcpyAsync(dkeep.p, keep.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+                FH_HIP(hipMemcpyAsync(dsc.p, sc.data(), k * sizeof(double), hipMemcpyHostToDevice, st));
+                const unsigned gs = grid_for(kk, TPB, 8192);
+                hipLaunchKernelGGL(k_sweep_init, dim3(gs), dim3(TPB), 0, st, sfull.p, ms, dkeep.p, dsc.p, k, A.p, col.p);
+                for (int j = 0; j < k; j++)
+                    hipLaunchKernelGGL(k_sweep_step, dim3(gs), dim3(TPB), 0, st, A.p, k, j, col.p + (size_t)(j & 1) * k,
+                                       col.p + (size_t)((j + 1) & 1) * k, grounded.p);
+                hipLaunchKernelGGL(k_sweep_scatter, dim3(gs), dim3(TPB), 0, st, A.p, k, dkeep.p, dsc.p, grounded.p, ms,
+                                   a.w64.minv.p, a.w32.minv.p);
+                FH_CHECK_LAUNCH();
+                FH_HIP(hipStreamSynchronize(st));  // keep/sc host vectors and the scratch end here
+            }
+        } else {
+            if (!(so.nsb == 1 && ms <= 21)) {
+                FH_HIP(hipMemcpyAsync(h.data(), sfull.p, len * sizeof(double), hipMemcpyDeviceToHost, st));
+                FH_HIP(hipStreamSynchronize(st));
+            }
+            std::vector<double> S(h.begin(), h.begin() + ms * ms);
+            for (size_t k = 0; k < ms; k++) {
+                nrs[k] = h[ms * ms + k];
+                dg[k] = S[k * ms + k];
+                rd[k] = dg[k] <= 1e-12 * nrs[k];
+            }
+            inv = grounded_inverse(S, ms, rd, a.nkeep);
         }
-        std::vector<double> inv = grounded_inverse(S, ms, rd, a.nkeep);
         std::vector<float> inv32(inv.begin(), inv.end());
         std::vector<double> zeros(ms, 0.0);
         std::vector<float> zeros32(ms, 0.0f);
@@ -471,10 +575,12 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         FH_HIP(hipMemcpyAsync(a.red.p, rd.data(), ms, hipMemcpyHostToDevice, st));
         FH_HIP(hipMemcpyAsync(a.w64.inv_diag.p, zeros.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
         FH_HIP(hipMemcpyAsync(a.w32.inv_diag.p, zeros32.data(), ms * sizeof(float), hipMemcpyHostToDevice, st));
-        a.w64.minv.reserve(ms * ms);
-        a.w32.minv.reserve(ms * ms);
-        FH_HIP(hipMemcpyAsync(a.w64.minv.p, inv.data(), ms * ms * sizeof(double), hipMemcpyHostToDevice, st));
-        FH_HIP(hipMemcpyAsync(a.w32.minv.p, inv32.data(), ms * ms * sizeof(float), hipMemcpyHostToDevice, st));
+        if (!dev_inverse) {
+            a.w64.minv.reserve(ms * ms);
+            a.w32.minv.reserve(ms * ms);
+            FH_HIP(hipMemcpyAsync(a.w64.minv.p, inv.data(), ms * ms * sizeof(double), hipMemcpyHostToDevice, st));
+            FH_HIP(hipMemcpyAsync(a.w32.minv.p, inv32.data(), ms * ms * sizeof(float), hipMemcpyHostToDevice, st));
+        }
     }
     FH_HIP(hipStreamSynchronize(st));
 }

@@ -229,6 +229,49 @@ int fh_ols_wildboot_dev(fh_absorber* ab, const double* y, const double* X, size_
                         const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, const fh_wboot_spec* spec,
                         double* beta, double* vcov, int32_t* kept, fh_wboot_result* res, double* t_boot);
 
+/* Serving engine (no reference counterpart: fereg-core is a library called once per fit).
+ * Continuous batching of independent regressions: a FIFO of requests, each admitted when one of
+ * `slots` workers is free and its HBM footprint fits `hbm_budget` beside the requests already
+ * on the device; every worker runs its request on its own stream, so requests overlap on the
+ * GPU. A result does not depend on what else is running (fixed-order reductions). */
+#define FH_MODEL_OLS 0
+#define FH_MODEL_PPML 1
+typedef struct fh_server fh_server;
+typedef struct {
+    uint32_t slots;      /* requests in flight at once (1..64) */
+    uint64_t hbm_budget; /* bytes admitted at once; 0 = three quarters of the free memory */
+    int32_t precision;   /* FH_PLAIN_FP64 or FH_MIXED */
+    int32_t reserved;
+} fh_server_config;
+typedef struct {
+    int32_t model;       /* FH_MODEL_OLS or FH_MODEL_PPML */
+    int32_t vcov_kind;   /* FH_VCOV_* */
+    size_t n, K, nterms;
+    const fh_term* terms; /* host codes, kept order */
+    const double* y;      /* host, n */
+    const double* X;      /* host, n x K */
+    const uint32_t* cluster;
+    const int32_t* nested;
+    fh_ssc ssc;
+    double tol;
+    uint64_t maxiter;
+    double* beta;         /* outputs, host: K, K x K, K */
+    double* vcov;
+    int32_t* kept;
+} fh_request;
+typedef struct {
+    int32_t rc;
+    int32_t reserved;
+    double queued_s, service_s, total_s; /* wait for admission, on the device, submit to done */
+    uint64_t iterations, pcg_iterations;
+} fh_response;
+void fh_server_config_default(fh_server_config* cfg);
+int fh_server_create(const fh_server_config* cfg, fh_server** out);
+/* Every pointer of the request must stay valid until fh_server_wait returns for its ticket. */
+int fh_server_submit(fh_server* s, const fh_request* req, uint64_t* ticket);
+int fh_server_wait(fh_server* s, uint64_t ticket, fh_response* resp);
+void fh_server_destroy(fh_server* s);
+
 /* Device memory helpers for callers without another allocator. */
 int fh_device_alloc(void** p, size_t bytes);
 int fh_device_free(void* p);

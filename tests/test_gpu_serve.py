@@ -1,0 +1,54 @@
+"""Serving engine: requests run concurrently on the device give the same results as alone."""
+import numpy as np
+import pytest
+
+from gpu_util import announce
+
+pytestmark = pytest.mark.gpu
+
+
+def test_results_identical_alone_and_in_the_mix():
+    import fereg_amd as F
+    from fereg_amd import serve
+    specs = serve.poisson_stream(16, seed=3, n_lo=2e3, n_hi=6e4, ppml_share=0.3)
+    data = [serve.make_request(s) for s in specs]
+    announce("serve mix", len(specs))
+    srv = serve.Server(slots=6)
+    tickets = []
+    for s, (codes, g, y, X) in zip(specs, data):
+        tickets.append(srv.submit(s["model"], codes, g, y, X, vcov="hetero" if s["model"] == "ppml" else "iid"))
+    mixed = [srv.wait(t) for t in tickets]
+    srv.close()
+    assert all(r.rc == 0 for r in mixed)
+    announce("serve alone")
+    alone_srv = serve.Server(slots=1)
+    for s, (codes, g, y, X), rm in zip(specs, data, mixed):
+        t = alone_srv.submit(s["model"], codes, g, y, X, vcov="hetero" if s["model"] == "ppml" else "iid")
+        ra = alone_srv.wait(t)
+        assert ra.rc == 0
+        # fixed-order reductions: bitwise the same whatever runs beside
+        np.testing.assert_array_equal(ra.coef, rm.coef)
+        np.testing.assert_array_equal(ra.vcov, rm.vcov)
+    alone_srv.close()
+    # and the same as the direct entry points
+    s, (codes, g, y, X) = specs[0], data[0]
+    ab = F.Absorber([np.asarray(c, np.uint32) for c in codes], g, precision=F.FH_MIXED)
+    fn = F.fepois if s["model"] == "ppml" else F.feols
+    fit = fn(y, X, None, None, vcov="hetero" if s["model"] == "ppml" else "iid", opts=F.SolveOpts(tol=1e-8),
+             absorber=ab)
+    ab.close()
+    np.testing.assert_allclose(fit.coef, mixed[0].coef, rtol=1e-12, atol=1e-14)
+
+
+def test_admission_by_memory_budget():
+    """A budget below one request's footprint still runs it (alone), one at a time."""
+    from fereg_amd import serve
+    specs = serve.poisson_stream(4, seed=5, n_lo=2e3, n_hi=2e4, ppml_share=0.0)
+    srv = serve.Server(slots=4, hbm_budget=1)
+    tickets = [srv.submit(s["model"], *serve.make_request(s)) for s in specs]
+    replies = [srv.wait(t) for t in tickets]
+    srv.close()
+    assert all(r.rc == 0 for r in replies)
+    # one at a time: every request but the first waited for the one before it
+    starts = sorted(r.total_s - r.service_s for r in replies)
+    assert len(starts) == 4
