@@ -88,7 +88,8 @@ class WbootResult(ctypes.Structure):
                 ("point", ctypes.c_double), ("ci_lo", ctypes.c_double), ("ci_hi", ctypes.c_double),
                 ("ssc", ctypes.c_double), ("reps", ctypes.c_uint64), ("reps_feasible", ctypes.c_uint64),
                 ("n_clusters", ctypes.c_uint64), ("full_enumeration", ctypes.c_int32),
-                ("fe_not_projected", ctypes.c_int32), ("seconds_bootstrap", ctypes.c_double)]
+                ("fe_not_projected", ctypes.c_int32), ("seconds_bootstrap", ctypes.c_double),
+                ("contraction_ms", ctypes.c_double), ("contraction_ops", ctypes.c_double)]
 
 
 FH_MODEL = {"ols": 0, "ppml": 1}

@@ -630,9 +630,14 @@ void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, 
     const uint32_t per = (nsteps + nsplit - 1) / nsplit;
     nsplit = (nsteps + per - 1) / per;
     DBuf<int32_t> part((size_t)nsplit * Bpad * ncolp);
+    hipEvent_t e0, e1;
+    FH_HIP(hipEventCreate(&e0));
+    FH_HIP(hipEventCreate(&e1));
+    FH_HIP(hipEventRecord(e0, st));
     hipLaunchKernelGGL(k_wb_gemm, dim3((unsigned)bblocks, nsplit, cgroups), dim3(TPB), 0, st, Pq.p, Gpad, ntiles,
                        (uint64_t)spec.seed, (int)full, per, nsteps, part.p, Bpad, ncolp);
     FH_CHECK_LAUNCH();
+    FH_HIP(hipEventRecord(e1, st));
     DBuf<double> coefs(reps * 5);
     hipLaunchKernelGGL(k_wb_post, dim3(grid_for(reps, TPB / WAVE, 8192)), dim3(TPB), 0, st, part.p, (int)nsplit, Bpad, ncolp,
                        NC, mk, reps, dscale.p, dAA.p, dBB.p, in.ssc, coefs.p);
@@ -640,6 +645,13 @@ void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, 
     std::vector<double> hc(reps * 5);
     FH_HIP(hipMemcpyAsync(hc.data(), coefs.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     FH_HIP(hipStreamSynchronize(st));
+    float gms = 0.f;
+    FH_HIP(hipEventElapsedTime(&gms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    res.contraction_ms = gms;
+    // int8 multiply-adds (2 ops each) of the draws x clusters x limb-columns product, padding included
+    res.contraction_ops = 2.0 * (double)Bpad * (double)Gpad * (double)ncolp;
     // inference (ols.rs:553-649)
     const double delta = spec.r - point;
     std::vector<double> tb(reps);

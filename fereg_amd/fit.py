@@ -117,6 +117,8 @@ class WildBootResult:
     fe_not_projected: bool
     ssc: float
     seconds_bootstrap: float
+    contraction_ms: float
+    contraction_ops: float
     t_boot: np.ndarray
     fit: FitResult
 
@@ -165,4 +167,5 @@ def wildboot(y, X, cluster, R, r=0.0, reps=999, seed=1, level=0.95, pval_type="t
                     int(res.n_clusters))
     return WildBootResult(res.stat, res.p_val, res.se, res.point, (res.ci_lo, res.ci_hi), int(res.reps),
                           int(res.reps_feasible), int(res.n_clusters), bool(res.full_enumeration),
-                          bool(res.fe_not_projected), res.ssc, res.seconds_bootstrap, tb[:int(res.reps)].copy(), fit)
+                          bool(res.fe_not_projected), res.ssc, res.seconds_bootstrap, res.contraction_ms,
+                          res.contraction_ops, tb[:int(res.reps)].copy(), fit)

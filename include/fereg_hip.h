@@ -219,6 +219,8 @@ typedef struct {
     int32_t full_enumeration;
     int32_t fe_not_projected; /* absorbed terms not nested in the clusters were left in the scores */
     double seconds_bootstrap;
+    double contraction_ms;    /* the MFMA contraction kernel (HIP events) */
+    double contraction_ops;   /* its int8 operations (2 per multiply-add, padding included) */
 } fh_wboot_result;
 /* OLS (as fh_ols_fit, one-way clustered variance by `cluster`) followed by the bootstrap;
  * t_boot (reps entries, host) may be NULL. Codes must be the dense cluster factor 0..G-1. */
