@@ -38,7 +38,13 @@ class Term(ctypes.Structure):
 
 
 class SolverConfig(ctypes.Structure):
-    _fields_ = [("small_limit", ctypes.c_uint64), ("precision", ctypes.c_int32), ("bucket_log2", ctypes.c_int32)]
+    _fields_ = [("small_limit", ctypes.c_uint64), ("precision", ctypes.c_int32), ("bucket_log2", ctypes.c_int32),
+                ("autotune", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class SpmvPlan(ctypes.Structure):
+    _fields_ = [("dense", ctypes.c_int32), ("seg_lanes", ctypes.c_uint32), ("block_lanes", ctypes.c_uint32 * 9),
+                ("tuned", ctypes.c_int32)]
 
 
 class SolveOptsC(ctypes.Structure):
@@ -120,6 +126,9 @@ class Response(ctypes.Structure):
 SIGNATURES = [
     ("fh_last_error", ctypes.c_char_p, []),
     ("fh_config_default", None, [ctypes.POINTER(SolverConfig)]),
+    ("fh_absorber_get_plan", ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_size_t, ctypes.POINTER(SpmvPlan)]),
+    ("fh_absorber_set_plan", ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_size_t, ctypes.POINTER(SpmvPlan)]),
+    ("fh_tune_cache_clear", None, []),
     ("fh_absorber_create", ctypes.c_int, [ctypes.POINTER(Term), ctypes.c_size_t, f64p, ctypes.c_size_t,
                                           ctypes.POINTER(SolverConfig), ctypes.POINTER(Ctx), ctypes.POINTER(vp)]),
     ("fh_absorber_create_dev", ctypes.c_int, [ctypes.POINTER(Term), ctypes.c_size_t, vp, ctypes.c_size_t,

@@ -68,7 +68,8 @@ class Absorber:
     tensors), g: levels per term, weights: kept-order weights or None.
     """
 
-    def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, bucket_log2=0, verbose=0):
+    def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, bucket_log2=0,
+                 autotune=False, verbose=0):
         lib = L.lib()
         self._keep = []
         n = None
@@ -96,7 +97,7 @@ class Absorber:
                 raise L.FeregError(L.FH_INVALID, "weight length differs from the kept sample")
         if n is None:
             raise L.FeregError(L.FH_INVALID, "no terms and no weights: the sample size is unknown")
-        cfg = L.SolverConfig(small_limit, precision, bucket_log2)
+        cfg = L.SolverConfig(small_limit, precision, bucket_log2, int(bool(autotune)), 0)
         h = ctypes.c_void_p()
         if dev:
             wptr = None
@@ -116,6 +117,27 @@ class Absorber:
         info = L.StructureInfo()
         L.check(lib.fh_absorber_info(h, ctypes.byref(info)))
         self.info = info
+
+    def plan(self, precision, r):
+        """The S*p launch plan in use for this precision and column count:
+        {"dense", "seg_lanes", "block_lanes", "tuned"}."""
+        p = L.SpmvPlan()
+        L.check(L.lib().fh_absorber_get_plan(self._h, int(precision), int(r), ctypes.byref(p)))
+        return {"dense": int(p.dense), "seg_lanes": int(p.seg_lanes), "block_lanes": [int(v) for v in p.block_lanes],
+                "tuned": bool(p.tuned)}
+
+    def set_plan(self, precision, r, plan):
+        """Pins a plan (a dict as returned by plan(); lanes 0 = default)."""
+        bl = list(plan.get("block_lanes", [])) + [0] * 9
+        p = L.SpmvPlan(int(plan.get("dense", 1)), int(plan.get("seg_lanes", 0)), (ctypes.c_uint32 * 9)(*bl[:9]), 1)
+        L.check(L.lib().fh_absorber_set_plan(self._h, int(precision), int(r), ctypes.byref(p)))
+
+    def time_spmv(self, r, precision, reps=20):
+        """Milliseconds per S*p (HIP events) and the algorithmic bytes of one product."""
+        ms, by = ctypes.c_double(0), ctypes.c_double(0)
+        L.check(L.lib().fh_absorber_time_spmv(self._h, int(r), int(precision), int(reps), ctypes.byref(ms),
+                                              ctypes.byref(by)))
+        return ms.value, by.value
 
     def close(self):
         if getattr(self, "_h", None):

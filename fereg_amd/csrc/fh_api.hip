@@ -714,6 +714,8 @@ void fh_config_default(fh_solver_config* cfg) {
     cfg->small_limit = 4096;
     cfg->precision = FH_PLAIN_FP64;
     cfg->bucket_log2 = 0;
+    cfg->autotune = 0;
+    cfg->reserved = 0;
 }
 
 static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept, size_t n, const fh_solver_config* cfg,
@@ -741,6 +743,8 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         a.precision = c.precision;
         require(c.bucket_log2 >= 0 && c.bucket_log2 <= 31, "bucket_log2 must be in 0..31");
         a.bucket_shift = c.bucket_log2 > 0 ? c.bucket_log2 : 31;
+        require(c.autotune == 0 || c.autotune == 1, "autotune must be 0 or 1");
+        a.autotune = c.autotune == 1;
         std::vector<DBuf<uint32_t>> up(nterms);
         std::vector<const uint32_t*> dc(nterms);
         for (size_t q = 0; q < nterms; q++) {
@@ -985,6 +989,44 @@ int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int
         }
     });
 }
+
+int fh_absorber_get_plan(const fh_absorber* ab, int32_t precision, size_t r, fh_spmv_plan* out) {
+    return guard([&] {
+        require(ab && out && r >= 1 && r <= 16, "bad argument");
+        require(precision == FH_PLAIN_FP64 || precision == FH_MIXED, "unknown precision");
+        const Absorber& a = ab->a;
+        const SpmvPlan& pl = a.plan_for(precision == FH_PLAIN_FP64, (int)r);
+        *out = fh_spmv_plan{};
+        out->dense = pl.dense;
+        out->seg_lanes = pl.seg_lanes ? pl.seg_lanes : a.seg_lanes;
+        for (size_t b = 0; b < a.nb && b < 9; b++)
+            out->block_lanes[b] = b < pl.blanes.size() && pl.blanes[b] ? pl.blanes[b] : a.blanes[b];
+        out->tuned = pl.tuned ? 1 : 0;
+    });
+}
+
+int fh_absorber_set_plan(fh_absorber* ab, int32_t precision, size_t r, const fh_spmv_plan* in) {
+    return guard([&] {
+        require(ab && in && r >= 1 && r <= 16, "bad argument");
+        require(precision == FH_PLAIN_FP64 || precision == FH_MIXED, "unknown precision");
+        const Absorber& a = ab->a;
+        auto pow2 = [](uint32_t v) { return v == 0 || (v <= 64 && (v & (v - 1)) == 0); };
+        require(in->dense == 0 || in->dense == 1, "plan: dense must be 0 or 1");
+        require(pow2(in->seg_lanes), "plan: lanes must be 0 or a power of two up to 64");
+        SpmvPlan pl;
+        pl.dense = in->dense;
+        pl.seg_lanes = in->seg_lanes;
+        pl.blanes.assign(a.nb, 0);
+        for (size_t b = 0; b < a.nb && b < 9; b++) {
+            require(pow2(in->block_lanes[b]), "plan: lanes must be 0 or a power of two up to 64");
+            pl.blanes[b] = in->block_lanes[b];
+        }
+        pl.tuned = true;
+        a.plan_for(precision == FH_PLAIN_FP64, (int)r) = pl;
+    });
+}
+
+void fh_tune_cache_clear(void) { tune_cache_clear(); }
 
 int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t reps, double* ms, double* bytes) {
     return guard([&] {

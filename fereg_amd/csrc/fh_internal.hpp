@@ -43,16 +43,26 @@ struct WeightArrays {
     std::vector<DBuf<T>> lw;  // per block, M (empty when unit)
     DBuf<T> minv;      // ms*ms
     // dense small-block operator (Absorber::small_dense): y_S = nr_S p_S + C p_L - D t
-    DBuf<T> dsm;       // D = ms x g1, D[k][s] = sum of W_c over the cells of segment s with small level k
-    DBuf<T> csm;       // C = ms x (m - ms), C[k][l] = sum of W_c over the cells with small level k, large coord l
+    // E = ms x sm_ld: D (ms x g1, D[k][s] = sum of W_c over the cells of segment s with small
+    // level k) in columns [0, g1), C (ms x (m - ms), C[k][l] = sum of W_c over the cells with small
+    // level k and large coordinate l) in columns [sm_coff, sm_coff + m - ms), zero pads
+    DBuf<T> esm;
     DBuf<T> nrs;       // ms, n_r of the small coordinates
 };
 
 // Per-call scratch of the PCG and operator kernels (grown on demand).
 template <class T>
 struct Scratch {
-    DBuf<T> r, z, p, q, x, rhs, t, zc, part, ypart;
+    DBuf<T> r, z, p, q, x, rhs, t, zc, part, ypart, ds, dl;
     DBuf<double> dpart;  // FP64 partial sums (dots)
+};
+
+// Launch choices of S*p (route and lane widths); zero widths mean the absorber's defaults.
+struct SpmvPlan {
+    int dense = 1;           // the single small block as dense products inside the passes too
+    uint32_t seg_lanes = 0;  // lanes per segment of the segment pass
+    std::vector<uint32_t> blanes;  // lanes per item of each block's level pass
+    bool tuned = false;
 };
 
 struct Absorber {
@@ -94,6 +104,16 @@ struct Absorber {
     // one small block with few levels: its part of S p is dense products with D and C (fewer
     // bytes than a level pass over the cells), set by refresh_weights
     bool small_dense = false;
+    size_t sm_ld = 0, sm_coff = 0;  // pitch of E and its first C column (multiples of 4)
+
+    // S*p plans per precision (FP32, FP64) and column class (r <= 1, 2, 4, 8, 16); with autotune
+    // the first S*p of a class times the candidates (autotune_spmv)
+    bool autotune = false;
+    mutable SpmvPlan plans[2][5];
+    SpmvPlan& plan_for(bool fp64, int r) const {
+        const int c = r <= 1 ? 0 : r <= 2 ? 1 : r <= 4 ? 2 : r <= 8 ? 3 : 4;
+        return plans[fp64 ? 1 : 0][c];
+    }
 
     bool trivial() const { return cell_obs_off.n == 0; }
     bool unit() const { return !weighted && trivial(); }
@@ -129,6 +149,11 @@ OpArgs<T> op_args(const Absorber& a);
 // y = S p (m x r). t_buf >= g1*r, part >= spmv_part_len(a, r).
 template <class T>
 void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st);
+// Process-wide cache of tuned S*p plans keyed by the structure signature (plan_key).
+std::string plan_key(const Absorber& a, int tbytes, int rclass);
+bool tune_cache_get(const std::string& key, SpmvPlan& plan);
+void tune_cache_put(const std::string& key, const SpmvPlan& plan);
+void tune_cache_clear();
 
 // rhs pass over rows (n x r, FP64): rhs (m x r, T), b1 (g1 x r), wsq/wsum partials -> out2 (2r).
 void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, double* wsq_wsum);

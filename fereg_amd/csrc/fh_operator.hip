@@ -1,4 +1,7 @@
 // Reduced operator, FP64 instantiation, rhs pass and back-substitution (see fh_operator_impl.hpp).
+#include <map>
+#include <mutex>
+
 #include "fh_operator_impl.hpp"
 
 namespace fh {
@@ -12,9 +15,45 @@ template OpArgs<float> op_args<float>(const Absorber&);
 
 template <class T>
 void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st) {
-    spmv_impl<T>(a, p, y, r, s, done, st);
+    spmv_planned<T>(a, p, y, r, s, done, st);
 }
 template void spmv<double>(const Absorber&, const double*, double*, int, Scratch<double>&, const uint32_t*, hipStream_t);
+
+namespace {
+std::mutex tune_mu;
+std::map<std::string, SpmvPlan>& tune_cache() {
+    static auto* c = new std::map<std::string, SpmvPlan>();
+    return *c;
+}
+}  // namespace
+
+std::string plan_key(const Absorber& a, int tbytes, int rclass) {
+    std::string k = std::to_string(tbytes) + "/" + std::to_string(rclass) + "/" + std::to_string(a.M) + "/" +
+                    std::to_string(a.g1) + "/" + std::to_string(a.m) + "/" + std::to_string(a.ms) + "/" +
+                    std::to_string(a.unit() ? 0 : 1) + "/" + std::to_string(a.small_dense ? 1 : 0);
+    for (size_t b = 0; b < a.nb; b++)
+        k += "/" + std::to_string(a.blocks[b].g) + ":" + std::to_string(a.nbk[b]) + ":" +
+             std::to_string(a.blocks[b].small ? 1 : 0);
+    return k;
+}
+
+bool tune_cache_get(const std::string& key, SpmvPlan& plan) {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto it = tune_cache().find(key);
+    if (it == tune_cache().end()) return false;
+    plan = it->second;
+    return true;
+}
+
+void tune_cache_put(const std::string& key, const SpmvPlan& plan) {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_cache()[key] = plan;
+}
+
+void tune_cache_clear() {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_cache().clear();
+}
 
 void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, double* wsq_wsum) {
     require(r >= 1 && r <= 16, "rhs: 1..16 columns");

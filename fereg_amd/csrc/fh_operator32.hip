@@ -5,7 +5,7 @@ namespace fh {
 
 template <class T>
 void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st) {
-    spmv_impl<T>(a, p, y, r, s, done, st);
+    spmv_planned<T>(a, p, y, r, s, done, st);
 }
 template void spmv<float>(const Absorber&, const float*, float*, int, Scratch<float>&, const uint32_t*, hipStream_t);
 

@@ -136,9 +136,11 @@ __global__ __launch_bounds__(WG) void k_seg_t_any(OpArgs<T> A, const T* __restri
 // t_s = sum_{c in s} W_c u_c / n1_s with NB (compile-time block count) and U = 8 trips per lane with every index load issued before the
 // gathers, so a lane keeps ~8 x NB independent loads in flight (the passes are latency-bound
 // with fewer: 32 resident waves per CU at most).
-template <class T, int RM, int NB>
+// With SKIP = 1 the first block (the single small block) is not gathered: its part of the
+// segment sum, sum_c W_c p_S[k(c)] = (D' p_S)[s], comes in ds (dense, k_small_dense_t).
+template <class T, int RM, int NB, int SKIP = 0>
 __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__ p, int r, int L, T* __restrict__ t,
-                                              const uint32_t* __restrict__ done) {
+                                              const T* __restrict__ ds, const uint32_t* __restrict__ done) {
     if (done && *done) return;
     constexpr int U = gather_trips<T, RM>();
     const int w = threadIdx.x / WAVE, l = lane_id();
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__
                 const uint32_t c = c0 + (it + h) * L + li;
                 v[h] = c < c1;
 #pragma unroll
-                for (int b = 0; b < NB; b++) code[h][b] = v[h] ? ldnt(&A.cell_codes[(size_t)c * NB + b]) : 0u;
+                for (int b = SKIP; b < NB; b++) code[h][b] = v[h] ? ldnt(&A.cell_codes[(size_t)c * NB + b]) : 0u;
                 wc[h] = v[h] ? (A.cw ? ldnt(&A.cw[c]) : T(1)) : T(0);
             }
 #pragma unroll
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__
 #pragma unroll
                 for (int j = 0; j < RM; j++) u[j] = T(0);
 #pragma unroll
-                for (int b = 0; b < NB; b++) {
+                for (int b = SKIP; b < NB; b++) {
                     T row[RM];
                     ld_row<T, RM>(p + (size_t)(A.boff[b] + code[h][b]) * r, r, v[h], row);
 #pragma unroll
@@ -182,7 +184,15 @@ __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__
             }
         }
 #pragma unroll
-        for (int j = 0; j < RM; j++) tv[j] = gsum(tv[j], L) * inv;
+        for (int j = 0; j < RM; j++) tv[j] = gsum(tv[j], L);
+        if (SKIP) {
+            T dv[RM];
+            ld_row<T, RM>(ds + s * r, r, sv, dv);
+#pragma unroll
+            for (int j = 0; j < RM; j++) tv[j] += dv[j];
+        }
+#pragma unroll
+        for (int j = 0; j < RM; j++) tv[j] *= inv;
         if (sv && li == 0)
 #pragma unroll
             for (int j = 0; j < RM; j++)
@@ -191,19 +201,80 @@ __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__
 }
 
 // ---------------------------------------------------------------- S*p, dense small rows
-// Partial sums over a column range of y_S[k] = sum_l C[k][l] p_L[l] - sum_s D[k][s] t_s for the
-// small levels k in chunk blockIdx.y (KC levels): columns are the g1 segments (D, t) followed by
-// the m - ms large coordinates (C, p_L). Each workgroup writes part[blockIdx.x][k][j].
+// The dense small-block matrix E (ms rows, pitch ld, a multiple of 4) holds D in columns
+// [0, g1) and C in columns [coff, coff + m - ms) (coff a multiple of 4, pads zero):
+//   D[k][s] = sum of W_c over the cells of segment s with small level k,
+//   C[k][l] = sum of W_c over the cells with small level k and large coordinate l.
+// Threads own quads of 4 columns, so every E access is one 16-byte aligned load (two for FP64)
+// and a quad never straddles the D/C boundary.
+template <class T>
+__device__ __forceinline__ void ld_quad(const T* __restrict__ src, T (&e)[4]) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 x = *reinterpret_cast<const float4*>(src);
+        e[0] = x.x; e[1] = x.y; e[2] = x.z; e[3] = x.w;
+    } else {
+        const double2 x = reinterpret_cast<const double2*>(src)[0], y = reinterpret_cast<const double2*>(src)[1];
+        e[0] = x.x; e[1] = x.y; e[2] = y.x; e[3] = y.y;
+    }
+}
+
+// Levels per chunk (blockIdx.y) of k_small_dense: KC x RM accumulators per thread.
 template <class T, int RM>
 constexpr int small_chunk() { return (sizeof(T) == 8 ? 32 : 64) / RM < 32 ? (sizeof(T) == 8 ? 32 : 64) / RM : 32; }
 
+// ds[s] = sum_k D[k][s] p_S[k] (g1 rows) and dl[l] = sum_k C[k][l] p_S[k] (m - ms rows): the
+// small block's contributions to the segment sums and to the large rows, one column quad per
+// thread (E read once, coalesced; p_S staged in LDS).
 template <class T, int RM>
-__global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ D, const T* __restrict__ t,
-                                                    const T* __restrict__ C, const T* __restrict__ pL, size_t g1,
-                                                    size_t mL, int ms, int r, T* __restrict__ part,
+__global__ __launch_bounds__(WG) void k_small_dense_t(const T* __restrict__ E, size_t ld, size_t coff, size_t g1,
+                                                      size_t mL, const T* __restrict__ pS, int ms, int r,
+                                                      T* __restrict__ ds, T* __restrict__ dl,
+                                                      const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    __shared__ T ps[32 * RM];
+    for (int e = threadIdx.x; e < ms * RM; e += WG) {
+        const int k = e / RM, j = e % RM;
+        ps[e] = j < r ? pS[k * r + j] : T(0);
+    }
+    __syncthreads();
+    const size_t nq = ld / 4;
+    for (size_t q = (size_t)blockIdx.x * WG + threadIdx.x; q < nq; q += (size_t)gridDim.x * WG) {
+        const size_t c4 = 4 * q;
+        T acc[4][RM];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < RM; j++) acc[i][j] = T(0);
+#pragma unroll 8
+        for (int k = 0; k < ms; k++) {
+            T e[4];
+            ld_quad(E + (size_t)k * ld + c4, e);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < RM; j++) acc[i][j] += e[i] * ps[k * RM + j];
+        }
+        const bool seg = c4 < coff;
+        T* o = seg ? ds : dl;
+        const size_t base = seg ? c4 : c4 - coff, lim = seg ? g1 : mL;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (base + i < lim)
+#pragma unroll
+                for (int j = 0; j < RM; j++)
+                    if (j < r) o[(base + i) * r + j] = acc[i][j];
+    }
+}
+
+// Partial sums over a column range of y_S[k] = sum_l C[k][l] p_L[l] - sum_s D[k][s] t_s for the
+// small levels k in chunk blockIdx.y (KC levels), one column quad per thread and step. Each
+// workgroup writes part[(k r + j) nparts + blockIdx.x] (nparts = gridDim.x).
+template <class T, int RM, int KC>
+__global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ E, size_t ld, size_t coff, size_t g1,
+                                                    size_t mL, const T* __restrict__ t, const T* __restrict__ pL,
+                                                    int ms, int r, T* __restrict__ part,
                                                     const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    constexpr int KC = small_chunk<T, RM>();
     __shared__ T red[WAVES][KC][RM];
     const int k0 = blockIdx.y * KC;
     T acc[KC][RM];
@@ -211,24 +282,28 @@ __global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ D, con
     for (int kk = 0; kk < KC; kk++)
 #pragma unroll
         for (int j = 0; j < RM; j++) acc[kk][j] = T(0);
-    const size_t nc = g1 + mL;
-    for (size_t col = (size_t)blockIdx.x * WG + threadIdx.x; col < nc; col += (size_t)gridDim.x * WG) {
-        const bool seg = col < g1;
-        const size_t c = seg ? col : col - g1;
-        const T* M = seg ? D : C;
-        const size_t ld = seg ? g1 : mL;
-        T v[RM];
-        ld_row<T, RM>((seg ? t : pL) + c * r, r, true, v);
+    const size_t nq = ld / 4;
+    for (size_t q = (size_t)blockIdx.x * WG + threadIdx.x; q < nq; q += (size_t)gridDim.x * WG) {
+        const size_t c4 = 4 * q;
+        const bool seg = c4 < coff;
+        const T* src = seg ? t : pL;
+        const size_t base = seg ? c4 : c4 - coff, lim = seg ? g1 : mL;
+        T v[4][RM];
+#pragma unroll
+        for (int i = 0; i < 4; i++) ld_row<T, RM>(src + (base + i) * r, r, base + i < lim, v[i]);
         if (seg)
 #pragma unroll
-            for (int j = 0; j < RM; j++) v[j] = -v[j];
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < RM; j++) v[i][j] = -v[i][j];
 #pragma unroll
         for (int kk = 0; kk < KC; kk++) {
             const int k = k0 + kk;
             if (k < ms) {
-                const T mk = M[(size_t)k * ld + c];
+                T e[4];
+                ld_quad(E + (size_t)k * ld + c4, e);
 #pragma unroll
-                for (int j = 0; j < RM; j++) acc[kk][j] += mk * v[j];
+                for (int j = 0; j < RM; j++) acc[kk][j] += e[0] * v[0][j] + e[1] * v[1][j] + e[2] * v[2][j] + e[3] * v[3][j];
             }
         }
     }
@@ -247,22 +322,29 @@ __global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ D, con
         T sm = T(0);
 #pragma unroll
         for (int ww = 0; ww < WAVES; ww++) sm += red[ww][kk][j];
-        part[(size_t)blockIdx.x * ms * r + (size_t)(k0 + kk) * r + j] = sm;
+        part[((size_t)(k0 + kk) * r + j) * gridDim.x + blockIdx.x] = sm;
     }
 }
 
-// y_S[k][j] = nr_k p_S[k][j] + sum over the workgroup partials (one wave per entry, fixed order).
+// y_S[k][j] = nr_k p_S[k][j] + the sum of the workgroup partials of entry e = k r + j: one
+// workgroup per entry, fixed order (strided thread sums, then waves in order).
 template <class T>
-__global__ __launch_bounds__(WG) void k_small_dense_final(const T* __restrict__ part, unsigned nparts, int ms, int r,
+__global__ __launch_bounds__(WG) void k_small_dense_final(const T* __restrict__ part, unsigned nparts, int r,
                                                           const T* __restrict__ nrs, const T* __restrict__ p,
                                                           T* __restrict__ y, const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    const size_t len = (size_t)ms * r;
-    for (size_t e = (size_t)blockIdx.x * WAVES + threadIdx.x / WAVE; e < len; e += (size_t)gridDim.x * WAVES) {
-        T sm = T(0);
-        for (unsigned q = lane_id(); q < nparts; q += WAVE) sm += part[(size_t)q * len + e];
-        sm = wave_sum(sm);
-        if (lane_id() == 0) y[e] = nrs[e / r] * p[e] + sm;
+    __shared__ T red[WAVES];
+    const size_t e = blockIdx.x;
+    T sm = T(0);
+    for (unsigned q = threadIdx.x; q < nparts; q += WG) sm += part[e * nparts + q];
+    sm = wave_sum(sm);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = sm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T tot = T(0);
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) tot += red[w];
+        y[e] = nrs[e / r] * p[e] + tot;
     }
 }
 
@@ -274,13 +356,16 @@ struct OtherOffs {
 // y_l = sum_{c in l} W_c (u_c - t_{s(c)}) over the level-major records of item
 // (bucket, level); with several buckets the item sums go to ypart and k_bucket_reduce adds
 // them in bucket order.
+// With oskip > 0 the first oskip other blocks (the single small block) are not gathered: their
+// part of the row, sum_c W_c p_S[k(c)] = (C' p_S)[l], comes in dl (this block's rows, dense,
+// k_small_dense_t).
 template <class T, int RM, int NO>
 __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lseg,
                                                  const uint32_t* __restrict__ lother, const T* __restrict__ lw,
                                                  size_t g, uint32_t gb, uint32_t off_b, OtherOffs oo, int nother,
-                                                 const T* __restrict__ p, const T* __restrict__ t, int r, int L,
-                                                 T* __restrict__ y, T* __restrict__ ypart,
-                                                 const uint32_t* __restrict__ done) {
+                                                 int oskip, const T* __restrict__ dl, const T* __restrict__ p,
+                                                 const T* __restrict__ t, int r, int L, T* __restrict__ y,
+                                                 T* __restrict__ ypart, const uint32_t* __restrict__ done) {
     if (done && *done) return;
     constexpr int U = gather_trips<T, RM>();
     const int w = threadIdx.x / WAVE, l = lane_id();
@@ -310,7 +395,7 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
                 sg[h] = pv[h] ? ldnt(&lseg[pos]) : 0u;
 #pragma unroll
                 for (int o = 0; o < NO; o++)
-                    oc[h][o] = (pv[h] && o < nother) ? ldnt(&lother[(size_t)pos * nother + o]) : 0u;
+                    oc[h][o] = (pv[h] && o < nother && o >= oskip) ? ldnt(&lother[(size_t)pos * nother + o]) : 0u;
                 wc[h] = pv[h] ? (lw ? ldnt(&lw[pos]) : T(1)) : T(0);
             }
 #pragma unroll
@@ -321,7 +406,7 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
 #pragma unroll
                 for (int o = 0; o < NO; o++) {
                     T row[RM];
-                    ld_row<T, RM>(p + (size_t)(oo.off[o] + oc[h][o]) * r, r, pv[h] && o < nother, row);
+                    ld_row<T, RM>(p + (size_t)(oo.off[o] + oc[h][o]) * r, r, pv[h] && o < nother && o >= oskip, row);
 #pragma unroll
                     for (int j = 0; j < RM; j++) u[j] += row[j];
                 }
@@ -336,6 +421,12 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
         }
 #pragma unroll
         for (int j = 0; j < RM; j++) acc[j] = gsum(acc[j], L);
+        if (dl && valid && it_ < gb) {  // once per level: the first bucket's item
+            T dv[RM];
+            ld_row<T, RM>(dl + (size_t)lv * r, r, true, dv);
+#pragma unroll
+            for (int j = 0; j < RM; j++) acc[j] += dv[j];
+        }
         if (valid && li == 0) {
             T* dst = ypart ? ypart + it_ * r : y + (size_t)(off_b + lv) * r;
 #pragma unroll
@@ -739,8 +830,11 @@ inline unsigned lvl_grid(int L, size_t items) {
     return grid_for(items, (unsigned)((WAVE / L) * WAVES), 1u << 20);
 }
 
+// The route and lane widths of one S*p: plan.dense selects the dense products for the single
+// small block also inside the segment and level passes (its codes are then never read there).
 template <class T>
-void spmv_impl(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st) {
+void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r, Scratch<T>& s,
+               const uint32_t* done, hipStream_t st) {
     require(r >= 1 && r <= 16, "spmv: 1..16 columns");
     OpArgs<T> A = op_args_impl<T>(a);
     const int nb = (int)a.nb;
@@ -748,38 +842,61 @@ void spmv_impl(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const 
     const WeightArrays<T>* W;
     if constexpr (std::is_same<T, double>::value) W = &a.w64; else W = &a.w32;
     s.t.reserve(std::max<size_t>(a.g1 * r, 1));
-    const int L = (int)a.seg_lanes;
-    const unsigned grid = seg_grid(a);
+    const int L = (int)(plan.seg_lanes ? plan.seg_lanes : a.seg_lanes);
+    const unsigned grid = grid_for(a.g1, (unsigned)((WAVE / L) * WAVES), 1u << 16);
+    const size_t ms = a.ms, mL = a.m - a.ms;
+    const T* E = a.small_dense ? W->esm.p : nullptr;
+    const bool dense = a.small_dense && plan.dense && nb <= 4 && r <= 4;
+    if (dense) {
+        s.ds.reserve(a.g1 * r);
+        s.dl.reserve(std::max<size_t>(mL * r, 1));
+        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_small_dense_t<T, RM>), dim3(grid_for(a.sm_ld / 4, WG, 4096)), dim3(WG),
+                                            0, st, E, a.sm_ld, a.sm_coff, a.g1, mL, p, (int)ms, r, s.ds.p, s.dl.p,
+                                            done));
+        FH_CHECK_LAUNCH();
+    }
     if (nb <= 4) {
         FH_DISPATCH_R(r, with_nb(nb, [&](auto nbc) {
             constexpr int NB = decltype(nbc)::value;
-            hipLaunchKernelGGL((k_seg_t<T, RM, NB>), dim3(grid), dim3(WG), 0, st, A, p, r, L, s.t.p, done);
+            if (dense)
+                hipLaunchKernelGGL((k_seg_t<T, RM, NB, 1>), dim3(grid), dim3(WG), 0, st, A, p, r, L, s.t.p, s.ds.p,
+                                   done);
+            else
+                hipLaunchKernelGGL((k_seg_t<T, RM, NB, 0>), dim3(grid), dim3(WG), 0, st, A, p, r, L, s.t.p,
+                                   (const T*)nullptr, done);
         }));
     } else {
         FH_DISPATCH_R(r, hipLaunchKernelGGL((k_seg_t_any<T, RM>), dim3(grid), dim3(WG), 0, st, A, p, r, L, s.t.p, done));
     }
     FH_CHECK_LAUNCH();
     if (a.small_dense) {
-        const size_t ms = a.ms, mL = a.m - a.ms;
-        unsigned nwg = 0, nch = 0;
+        const unsigned nwg = grid_for(a.sm_ld / 4, WG, 2048);  // ~one column quad per thread
         FH_DISPATCH_R(r, {
-            constexpr int KC = small_chunk<T, RM>();
-            nch = (unsigned)((ms + KC - 1) / KC);
-            nwg = grid_for(a.g1 + mL, WG * 4, 512);
+            constexpr int KM = small_chunk<T, RM>();
             s.part.reserve((size_t)nwg * ms * r);
-            hipLaunchKernelGGL((k_small_dense<T, RM>), dim3(nwg, nch), dim3(WG), 0, st, W->dsm.p, s.t.p, W->csm.p,
-                               p + ms * r, a.g1, mL, (int)ms, r, s.part.p, done);
+            auto launch = [&](auto kc) {
+                constexpr int KC = decltype(kc)::value;
+                const unsigned nch = (unsigned)((ms + KC - 1) / KC);
+                hipLaunchKernelGGL((k_small_dense<T, RM, KC>), dim3(nwg, nch), dim3(WG), 0, st, E, a.sm_ld, a.sm_coff,
+                                   a.g1, mL, s.t.p, p + ms * r, (int)ms, r, s.part.p, done);
+            };
+            if (ms <= 4 || KM <= 4) launch(std::integral_constant<int, (KM < 4 ? KM : 4)>{});
+            else if (ms <= 8 || KM <= 8) launch(std::integral_constant<int, (KM < 8 ? KM : 8)>{});
+            else if (ms <= 16 || KM <= 16) launch(std::integral_constant<int, (KM < 16 ? KM : 16)>{});
+            else launch(std::integral_constant<int, KM>{});
         });
         FH_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_small_dense_final<T>, dim3(grid_for(ms * r, WAVES, 1024)), dim3(WG), 0, st, s.part.p, nwg,
-                           (int)ms, r, W->nrs.p, p, y, done);
+        hipLaunchKernelGGL(k_small_dense_final<T>, dim3((unsigned)(ms * r)), dim3(WG), 0, st, s.part.p, nwg, r,
+                           W->nrs.p, p, y, done);
         FH_CHECK_LAUNCH();
     }
+    int nsb = 0;
+    for (auto& bi : a.blocks) nsb += bi.small ? 1 : 0;
     for (int b = 0; b < nb; b++) {
         const BlockInfo& bi = a.blocks[b];
         if (bi.small && a.small_dense) continue;
         const OtherOffs oo = other_offs(a, b);
-        const int Ll = (int)a.blanes[b];
+        const int Ll = (int)((size_t)b < plan.blanes.size() && plan.blanes[b] ? plan.blanes[b] : a.blanes[b]);
         const uint32_t nbk = a.nbk[b];
         const size_t items = (size_t)nbk * bi.g;
         const T* lw = a.unit() ? nullptr : W->lw[b].p;
@@ -788,14 +905,90 @@ void spmv_impl(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const 
             s.ypart.reserve(items * r);
             ypart = s.ypart.p;
         }
+        const int oskip = dense ? nsb : 0;  // small blocks precede the large ones among the others
+        const T* dlb = dense ? s.dl.p + (bi.off - ms) * r : nullptr;
         FH_DISPATCH_R(r, with_no(nb - 1, [&](auto noc) {
             constexpr int NO = decltype(noc)::value;
             hipLaunchKernelGGL((k_lvl_spmv<T, RM, NO>), dim3(lvl_grid(Ll, items)), dim3(WG), 0, st, a.boff[b].p,
-                               a.lseg[b].p, a.lother[b].p, lw, items, (uint32_t)bi.g, (uint32_t)bi.off, oo, nb - 1, p,
-                               s.t.p, r, Ll, y, ypart, done);
+                               a.lseg[b].p, a.lother[b].p, lw, items, (uint32_t)bi.g, (uint32_t)bi.off, oo, nb - 1,
+                               oskip, dlb, p, s.t.p, r, Ll, y, ypart, done);
         }));
         FH_CHECK_LAUNCH();
         if (ypart) bucket_reduce<T>(ypart, nbk, (size_t)bi.g, r, y + bi.off * r, done, st);
     }
+}
+
+// Times the candidate plans of S*p for this absorber, precision and column class on the caller's
+// (p, y) (y is overwritten, p only read) and keeps the fastest: coordinate descent over the
+// route, then the segment lanes, then each large block's level lanes, halving and doubling the
+// default widths. Each candidate runs once untimed and three times timed (minimum kept). The
+// choice is cached per structure signature, so later absorbers of the same shape skip the timing.
+template <class T>
+void autotune_spmv(const Absorber& a, SpmvPlan& plan, const T* p, T* y, int r, Scratch<T>& s, hipStream_t st) {
+    const int rc = r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : r <= 8 ? 8 : 16;
+    const std::string key = plan_key(a, (int)sizeof(T), rc);
+    if (tune_cache_get(key, plan)) {
+        plan.tuned = true;
+        return;
+    }
+    SpmvPlan best;
+    best.dense = 1;
+    best.seg_lanes = a.seg_lanes;
+    best.blanes = a.blanes;
+    hipEvent_t e0, e1;
+    FH_HIP(hipEventCreate(&e0));
+    FH_HIP(hipEventCreate(&e1));
+    auto time_plan = [&](const SpmvPlan& c) {
+        spmv_impl<T>(a, c, p, y, r, s, nullptr, st);
+        float bestms = 1e30f;
+        for (int k = 0; k < 3; k++) {
+            FH_HIP(hipEventRecord(e0, st));
+            spmv_impl<T>(a, c, p, y, r, s, nullptr, st);
+            FH_HIP(hipEventRecord(e1, st));
+            FH_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            bestms = std::min(bestms, ms);
+        }
+        return bestms;
+    };
+    float tb = time_plan(best);
+    auto consider = [&](const SpmvPlan& c) {
+        const float tc = time_plan(c);
+        if (tc < tb * 0.98f) {
+            tb = tc;
+            best = c;
+        }
+    };
+    if (a.small_dense && a.nb <= 4 && r <= 4) {
+        SpmvPlan c = best;
+        c.dense = 0;
+        consider(c);
+    }
+    for (uint32_t f : {2u, 0u}) {  // double, then halve the segment lanes
+        SpmvPlan c = best;
+        c.seg_lanes = f ? best.seg_lanes * 2 : best.seg_lanes / 2;
+        if (c.seg_lanes >= 1 && c.seg_lanes <= WAVE && c.seg_lanes != best.seg_lanes) consider(c);
+    }
+    for (size_t b = 0; b < a.nb; b++) {
+        if (a.blocks[b].small && a.small_dense) continue;
+        for (uint32_t f : {2u, 0u}) {
+            SpmvPlan c = best;
+            c.blanes[b] = f ? best.blanes[b] * 2 : best.blanes[b] / 2;
+            if (c.blanes[b] >= 1 && c.blanes[b] <= WAVE && c.blanes[b] != best.blanes[b]) consider(c);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    best.tuned = true;
+    plan = best;
+    tune_cache_put(key, plan);
+}
+
+template <class T>
+void spmv_planned(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st) {
+    SpmvPlan& plan = a.plan_for(sizeof(T) == 8, r);
+    if (a.autotune && !plan.tuned) autotune_spmv<T>(a, plan, p, y, r, s, st);
+    spmv_impl<T>(a, plan, p, y, r, s, done, st);
 }
 }  // namespace fh

@@ -193,12 +193,12 @@ __global__ void k_fixed_to_double(const unsigned long long* __restrict__ parts, 
 // Level sums c_{s,k} of the single small block (block 0, whose codes ascend within a segment),
 // one thread per segment, the row in registers. Optional outputs: row s of
 // C = [c_{s,0..ms-1} / sqrt(n1_s), sqrt(n1_s)] (ld ms + 1) and the dense D[k][s] = c_{s,k}
-// (FP64 and FP32).
+// (FP64 and FP32, pitch ldd).
 template <int MS>
 __global__ void k_seg_level_sums(const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cell_codes,
                                  const double* __restrict__ cw, const double* __restrict__ n1, size_t g1, int nb,
-                                 uint32_t off0, int ms, double* __restrict__ C, double* __restrict__ D64,
-                                 float* __restrict__ D32) {
+                                 uint32_t off0, int ms, double* __restrict__ C, size_t ldd,
+                                 double* __restrict__ D64, float* __restrict__ D32) {
     for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
         double row[MS];
 #pragma unroll
@@ -213,7 +213,7 @@ __global__ void k_seg_level_sums(const uint32_t* __restrict__ seg_off, const uin
         if (D64)
 #pragma unroll
             for (int k = 0; k < MS; k++)
-                if (k < ms) { D64[(size_t)k * g1 + s] = row[k]; D32[(size_t)k * g1 + s] = (float)row[k]; }
+                if (k < ms) { D64[(size_t)k * ldd + s] = row[k]; D32[(size_t)k * ldd + s] = (float)row[k]; }
         if (!C) continue;
         const double sq = sqrt(n1[s]);
         const double f = sq > 0.0 ? 1.0 / sq : 0.0;
@@ -437,23 +437,26 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         a.small_dense = nsb == 1 && ms >= 1 && ms <= 32 && ms * (g1 + (m - ms)) <= 4 * std::max<size_t>(M, 1);
     }
     if (a.small_dense) {
+        // E = [D | pad | C | pad], ms rows of pitch sm_ld: every column quad is 16-byte aligned
         const size_t mL = m - ms;
-        a.w64.dsm.reserve(ms * g1);
-        a.w32.dsm.reserve(ms * g1);
-        a.w64.csm.reserve(std::max<size_t>(ms * mL, 1));
-        a.w32.csm.reserve(std::max<size_t>(ms * mL, 1));
+        a.sm_coff = (g1 + 3) / 4 * 4;
+        a.sm_ld = a.sm_coff + (mL + 3) / 4 * 4;
+        a.w64.esm.reserve(ms * a.sm_ld);
+        a.w32.esm.reserve(ms * a.sm_ld);
+        FH_HIP(hipMemsetAsync(a.w64.esm.p, 0, ms * a.sm_ld * sizeof(double), st));
+        FH_HIP(hipMemsetAsync(a.w32.esm.p, 0, ms * a.sm_ld * sizeof(float), st));
         for (int b = 0; b < nb; b++) {
             const BlockInfo& bi = a.blocks[b];
             if (bi.small) continue;
             hipLaunchKernelGGL(k_level_small_sums, dim3(blocks_for(bi.g)), dim3(TPB), 0, st, a.boff[b].p,
                                a.lother[b].p, nb - 1, unit ? nullptr : a.w64.lw[b].p, a.nbk[b], (size_t)bi.g, (int)ms,
-                               mL, (size_t)(bi.off - ms), a.w64.csm.p, a.w32.csm.p);
+                               a.sm_ld, a.sm_coff + (size_t)(bi.off - ms), a.w64.esm.p, a.w32.esm.p);
             FH_CHECK_LAUNCH();
         }
         if (ms > 21)  // the S_SS assembly below does not produce D for these
             hipLaunchKernelGGL(k_seg_level_sums<32>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
                                a.cell_codes.p, cw, a.n1.p, g1, nb, (uint32_t)a.blocks[0].off, (int)ms, (double*)nullptr,
-                               a.w64.dsm.p, a.w32.dsm.p);
+                               a.sm_ld, a.w64.esm.p, a.w32.esm.p);
         FH_CHECK_LAUNCH();
     }
     // small super-block
@@ -470,14 +473,14 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         if (so.nsb == 1 && ms <= 21) {
             // S_SS = diag(n_r) - C'C over the scaled level sums; n_r from the extra column
             DBuf<double> C(g1 * (ms + 1));
-            double* D64 = a.small_dense ? a.w64.dsm.p : nullptr;
-            float* D32 = a.small_dense ? a.w32.dsm.p : nullptr;
+            double* D64 = a.small_dense ? a.w64.esm.p : nullptr;
+            float* D32 = a.small_dense ? a.w32.esm.p : nullptr;
             if (ms <= 8)
                 hipLaunchKernelGGL(k_seg_level_sums<8>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
-                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, D64, D32);
+                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32);
             else
                 hipLaunchKernelGGL(k_seg_level_sums<21>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
-                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, D64, D32);
+                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32);
             FH_CHECK_LAUNCH();
             std::vector<double> G((ms + 1) * (ms + 1));
             gram_cols(a, C.p, g1, (int)ms + 1, 0, (int)ms + 1, nullptr, nullptr, G.data());

@@ -52,6 +52,11 @@ typedef struct {
     uint64_t small_limit; /* levels factored jointly and exactly (default 4096) */
     int32_t precision;    /* FH_PLAIN_FP64 or FH_MIXED */
     int32_t bucket_log2;  /* level passes in buckets of 2^k FE1 levels; 0 = one bucket */
+    int32_t autotune;     /* 1: the first S*p of each precision and column count times the
+                             candidate launch plans and keeps the fastest, cached per structure
+                             shape for the process (results may then differ in the last bits
+                             between processes); 0: fixed default plans (bit-reproducible) */
+    int32_t reserved;
 } fh_solver_config;
 
 /* SolveOpts (absorber.rs:18-24) plus the precision of this solve. */
@@ -140,6 +145,23 @@ int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int
  * 4 nb M + 12 g1 + 2 s m r + s M [weighted] (s = 8 FP64, 4 FP32). */
 int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t reps, double* ms_per_spmv,
                           double* bytes_per_spmv);
+
+/* Launch plan of S*p for one precision and column count (no counterpart in the reference, whose
+ * CPU operator has no launch shapes): the single small block as dense products inside the
+ * segment and level passes (dense = 1) or gathered there (0), lanes per segment and lanes per
+ * level of each block (powers of two 1..64, 0 = the default). get returns the plan in use (the
+ * tuned one after an autotuned S*p); set pins one (e.g. a plan tuned in another process, for
+ * bit-identical results) and marks it tuned. */
+typedef struct {
+    int32_t dense;
+    uint32_t seg_lanes;
+    uint32_t block_lanes[9];
+    int32_t tuned;
+} fh_spmv_plan;
+int fh_absorber_get_plan(const fh_absorber* ab, int32_t precision, size_t r, fh_spmv_plan* plan);
+int fh_absorber_set_plan(fh_absorber* ab, int32_t precision, size_t r, const fh_spmv_plan* plan);
+/* Forgets every tuned plan cached in this process. */
+void fh_tune_cache_clear(void);
 
 /* ------------------------------------------------------------------ estimation */
 
