@@ -10,6 +10,8 @@
 // block is reused for requests of at least half its size. Memory goes back to HIP on
 // fh_release_cached_memory() or when hipMalloc fails (completed blocks are released and the
 // allocation is tried once more).
+#include <chrono>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -165,6 +167,148 @@ void pool_forget_stream(hipStream_t st) {
                 b.idle = true;
                 b.stream = nullptr;
             }
+}
+
+// ---------------------------------------------------------------- pinned staging
+namespace {
+
+struct Stage {
+    char* p = nullptr;
+    size_t bytes = 0;
+    int device = 0;
+    hipEvent_t ev = nullptr;  // recorded after the block's last copy (null: idle)
+};
+
+struct StagePool {
+    std::mutex mu;
+    std::multimap<size_t, Stage> free_;
+};
+
+StagePool& stage_pool() {
+    static StagePool* p = new StagePool();  // never destroyed: HIP may be gone at process exit
+    return *p;
+}
+
+constexpr size_t STAGE_MAX = 4u << 20;  // larger transfers go straight from the caller's memory
+
+Stage stage_get(size_t bytes) {
+    size_t rb = 4096;
+    while (rb < bytes) rb <<= 1;
+    int dev = 0;
+    FH_HIP(hipGetDevice(&dev));
+    StagePool& P = stage_pool();
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        ErrorHygiene eh;
+        for (auto it = P.free_.lower_bound(rb); it != P.free_.end() && it->first <= 4 * rb; ++it) {
+            Stage& s = it->second;
+            if (s.device != dev) continue;
+            // (an event whose stream is gone reports an error, not "not ready": its copy is done,
+            // since the library drains a stream before destroying it)
+            if (s.ev && hipEventQuery(s.ev) == hipErrorNotReady) continue;
+            Stage out = s;
+            P.free_.erase(it);
+            return out;
+        }
+    }
+    Stage s;
+    s.bytes = rb;
+    s.device = dev;
+    FH_HIP(hipHostMalloc((void**)&s.p, rb, hipHostMallocDefault));
+    FH_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    return s;
+}
+
+void stage_put(const Stage& s) {
+    StagePool& P = stage_pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.free_.emplace(s.bytes, s);
+}
+
+}  // namespace
+
+void stream_wait(hipStream_t st) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const hipError_t prev = hipPeekAtLastError();
+    for (;;) {
+        const hipError_t e = hipStreamQuery(st);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) FH_HIP(e);
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(5)) {
+            FH_HIP(hipStreamSynchronize(st));
+            break;
+        }
+    }
+    if (prev == hipSuccess) (void)hipGetLastError();  // the not-ready answers are not errors
+}
+
+void upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return;
+    if (bytes > STAGE_MAX) {
+        FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        return;
+    }
+    Stage s = stage_get(bytes);
+    std::memcpy(s.p, src, bytes);
+    FH_HIP(hipMemcpyAsync(dst, s.p, bytes, hipMemcpyHostToDevice, st));
+    FH_HIP(hipEventRecord(s.ev, st));
+    stage_put(s);
+}
+
+void download(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) {
+        stream_wait(st);
+        return;
+    }
+    if (bytes > STAGE_MAX) {
+        FH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        stream_wait(st);
+        return;
+    }
+    Stage s = stage_get(bytes);
+    FH_HIP(hipMemcpyAsync(s.p, src, bytes, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipEventRecord(s.ev, st));
+    stream_wait(st);
+    std::memcpy(dst, s.p, bytes);
+    stage_put(s);
+}
+
+AsyncRead::AsyncRead(size_t bytes) : bytes_(bytes) {
+    Stage s = stage_get(bytes);
+    p_ = s.p;
+    cap_ = s.bytes;
+    device_ = s.device;
+    ev_ = s.ev;
+}
+
+AsyncRead::~AsyncRead() {
+    Stage s;
+    s.p = static_cast<char*>(p_);
+    s.bytes = cap_;
+    s.device = device_;
+    s.ev = ev_;
+    stage_put(s);  // reused only once the event of its last copy has completed
+}
+
+void AsyncRead::start(const void* src, hipStream_t st) {
+    FH_HIP(hipMemcpyAsync(p_, src, bytes_, hipMemcpyDeviceToHost, st));
+    FH_HIP(hipEventRecord(ev_, st));
+}
+
+const void* AsyncRead::wait() {
+    const auto t0 = std::chrono::steady_clock::now();
+    const hipError_t prev = hipPeekAtLastError();
+    for (;;) {
+        const hipError_t e = hipEventQuery(ev_);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) FH_HIP(e);
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(5)) {
+            FH_HIP(hipEventSynchronize(ev_));
+            break;
+        }
+    }
+    if (prev == hipSuccess) (void)hipGetLastError();
+    return p_;
 }
 
 void pool_stats(size_t* in_use, size_t* cached) {

@@ -190,17 +190,17 @@ __global__ void k_init_mu(const double* __restrict__ y, double ybar, size_t n, d
 
 template <class T>
 void h2d(T* dst, const T* src, size_t count, hipStream_t st) {
-    if (count) FH_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+    if (count) upload(dst, src, count * sizeof(T), st);
 }
 template <class T>
 void d2h(T* dst, const T* src, size_t count, hipStream_t st) {
-    if (count) FH_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, st));
+    if (count) download(dst, src, count * sizeof(T), st);
 }
 
 double sum_parts(const double* dev, size_t count, hipStream_t st) {
     std::vector<double> h(count);
     d2h(h.data(), dev, count, st);
-    FH_HIP(hipStreamSynchronize(st));
+    stream_wait(st);
     double s = 0.0;
     for (double v : h) s += v;
     return s;
@@ -237,7 +237,7 @@ bool weights_ok(Absorber& a, const double* w) {
     FH_CHECK_LAUNCH();
     uint32_t h = 0;
     d2h(&h, bad.p, 1, a.st);
-    FH_HIP(hipStreamSynchronize(a.st));
+    stream_wait(a.st);
     return h == 0;
 }
 
@@ -258,17 +258,20 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
     a.b1.reserve(g1 * r);
     a.s64.rhs.reserve(std::max<size_t>(m * r, 1));
     a.s64.x.reserve(std::max<size_t>(m * r, 1));
-    std::vector<double> ws(2 * r);
-    rhs_pass(a, z, r, a.s64.rhs.p, a.b1.p, ws.data());
-    check_interrupt(ctx);
-    // ss0 = wsq - wsum^2/sw (0 if <= 1e-15 wsq), ss1 = wsq - sum_s b1^2/n1 (solve.rs:1210-1217)
+    // ss0 = wsq - wsum^2/sw (0 if <= 1e-15 wsq), ss1 = wsq - sum_s b1^2/n1 (solve.rs:1210-1217);
+    // the partials of wsq / wsum (rhs pass) and of sum b1^2/n1 are read back in one transfer
     const unsigned pg = 256;
-    DBuf<double> part((size_t)pg * r);
-    hipLaunchKernelGGL(k_wsq_b1, dim3(pg), dim3(TPB), 0, st, a.b1.p, a.n1.p, g1, r, part.p);
+    const unsigned grid = rhs_pass(a, z, r, a.s64.rhs.p, a.b1.p, (size_t)pg * r);
+    const size_t nws = (size_t)grid * 2 * r;
+    hipLaunchKernelGGL(k_wsq_b1, dim3(pg), dim3(TPB), 0, st, a.b1.p, a.n1.p, g1, r, a.s64.dpart.p + nws);
     FH_CHECK_LAUNCH();
-    std::vector<double> hp((size_t)pg * r);
-    d2h(hp.data(), part.p, hp.size(), st);
-    FH_HIP(hipStreamSynchronize(st));
+    std::vector<double> hall(nws + (size_t)pg * r);
+    d2h(hall.data(), a.s64.dpart.p, hall.size(), st);
+    check_interrupt(ctx);
+    std::vector<double> ws(2 * r, 0.0);
+    for (unsigned q = 0; q < grid; q++)
+        for (int j = 0; j < 2 * r; j++) ws[j] += hall[(size_t)q * 2 * r + j];
+    const double* hp = hall.data() + nws;
     std::vector<double> ss0(r), ss1(r);
     std::vector<int32_t> zero(r);
     for (int j = 0; j < r; j++) {
@@ -288,8 +291,10 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
         h2d(dz.p, zero.data(), r, st);
         hipLaunchKernelGGL(k_zero_cols, dim3(vgrid(m * r)), dim3(TPB), 0, st, x, m, r, dz.p);
         FH_CHECK_LAUNCH();
-        PcgResult pr = precision == FH_MIXED ? pcg_mixed(a, a.s64.rhs.p, x, r, ss1.data(), ss0.data(), tol, maxiter, ctx)
-                                             : pcg_fp64(a, a.s64.rhs.p, x, r, ss1.data(), ss0.data(), tol, maxiter, ctx);
+        const bool warm = warm_dev != nullptr;
+        PcgResult pr = precision == FH_MIXED
+                           ? pcg_mixed(a, a.s64.rhs.p, x, warm, r, ss1.data(), ss0.data(), tol, maxiter, ctx)
+                           : pcg_fp64(a, a.s64.rhs.p, x, warm, r, ss1.data(), ss0.data(), tol, maxiter, ctx);
         if (pr.rc == FH_CONVERGENCE)
             throw Error(FH_CONVERGENCE, "the FE solver did not converge within " + std::to_string(maxiter) + " iterations");
         out.iterations = std::max<uint64_t>(pr.iterations, 1);
@@ -300,7 +305,6 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
     }
     residualize_rows(a, z, r, x, a.b1.p, coefs_dev ? coefs_dev + m * r : nullptr);
     if (coefs_dev && m > 0) FH_HIP(hipMemcpyAsync(coefs_dev, x, m * r * sizeof(double), hipMemcpyDeviceToDevice, st));
-    FH_HIP(hipStreamSynchronize(st));
     return out;
 }
 
@@ -328,7 +332,6 @@ ResidOut residualize_any(Absorber& a, double* z, size_t r, double tol, uint64_t 
         tot.exact = tot.exact && o.exact;
         tot.refinements = std::max(tot.refinements, o.refinements);
     }
-    FH_HIP(hipStreamSynchronize(a.st));
     return tot;
 }
 
@@ -490,7 +493,7 @@ uint64_t cluster_levels(Absorber& a, const uint32_t* cl_dev) {
     FH_CHECK_LAUNCH();
     uint32_t h = 0;
     d2h(&h, mx.p, 1, a.st);
-    FH_HIP(hipStreamSynchronize(a.st));
+    stream_wait(a.st);
     return (uint64_t)h + 1;
 }
 
@@ -766,7 +769,7 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
             FH_CHECK_LAUNCH();
             uint32_t hb = 0;
             d2h(&hb, bad.p, 1, a.st);
-            FH_HIP(hipStreamSynchronize(a.st));
+            stream_wait(a.st);
             require(!hb, "a term code is not below its level count");
         }
         DBuf<double> wd;
@@ -824,7 +827,7 @@ const uint32_t* fh_absorber_row_order(fh_absorber* ab) {
             AbsScope g(a);
             a.host_pi.resize(a.n);
             d2h(a.host_pi.data(), a.pi.p, a.n, a.st);
-            FH_HIP(hipStreamSynchronize(a.st));
+            stream_wait(a.st);
             a.host_pi_valid = true;
         });
         if (rc) return nullptr;
@@ -869,6 +872,7 @@ static int residualize_impl(fh_absorber* ab, double* z, size_t r, const fh_solve
         ResidOut ro;
         if (dev) {
             ro = residualize_any(a, z, r, o->tol, o->maxiter, warm, coefs_out, prec, ctx);
+            stream_wait(a.st);  // the caller's device block is final on return
         } else {
             DBuf<double> dz(n * r), dw, dcf;
             h2d(dz.p, z, n * r, a.st);
@@ -880,7 +884,7 @@ static int residualize_impl(fh_absorber* ab, double* z, size_t r, const fh_solve
             ro = residualize_any(a, dz.p, r, o->tol, o->maxiter, warm ? dw.p : nullptr, coefs_out ? dcf.p : nullptr, prec, ctx);
             d2h(z, dz.p, n * r, a.st);
             if (coefs_out) d2h(coefs_out, dcf.p, p * r, a.st);
-            FH_HIP(hipStreamSynchronize(a.st));
+            stream_wait(a.st);
         }
         if (info) {
             info->iterations = ro.iterations;
@@ -920,7 +924,7 @@ int fh_absorber_copy_structure(fh_absorber* ab, uint32_t* pi, uint32_t* seg_off,
         if (seg_off) d2h(seg_off, a.seg_off.p, a.g1 + 1, a.st);
         if (cc) d2h(cc, a.cell_codes.p, a.M * a.nb, a.st);
         if (coo && !a.trivial()) d2h(coo, a.cell_obs_off.p, a.M + 1, a.st);
-        FH_HIP(hipStreamSynchronize(a.st));
+        stream_wait(a.st);
     });
 }
 
@@ -946,7 +950,7 @@ int fh_absorber_copy_level_cells(fh_absorber* ab, size_t b, uint32_t* perm, uint
         ensure_level_cells(a, b);
         d2h(perm, a.lperm[b].p, a.M, a.st);
         d2h(off, a.loff[b].p, a.blocks[b].g + 1, a.st);
-        FH_HIP(hipStreamSynchronize(a.st));
+        stream_wait(a.st);
     });
 }
 
@@ -960,7 +964,7 @@ int fh_absorber_copy_weight_state(fh_absorber* ab, double* n1, double* nr, doubl
         if (diag) d2h(diag, a.diag.p, a.m, a.st);
         if (red) d2h(red, a.red.p, a.m, a.st);
         if (sw) *sw = a.sw;
-        FH_HIP(hipStreamSynchronize(a.st));
+        stream_wait(a.st);
     });
 }
 
@@ -978,14 +982,14 @@ int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int
             h2d(dp.p, hp.data(), m * r, a.st);
             spmv<float>(a, dp.p, dy.p, (int)r, a.s32, nullptr, a.st);
             d2h(hy.data(), dy.p, m * r, a.st);
-            FH_HIP(hipStreamSynchronize(a.st));
+            stream_wait(a.st);
             for (size_t i = 0; i < m * r; i++) y[i] = hy[i];
         } else {
             DBuf<double> dp(m * r), dy(m * r);
             h2d(dp.p, p, m * r, a.st);
             spmv<double>(a, dp.p, dy.p, (int)r, a.s64, nullptr, a.st);
             d2h(y, dy.p, m * r, a.st);
-            FH_HIP(hipStreamSynchronize(a.st));
+            stream_wait(a.st);
         }
     });
 }
@@ -1077,7 +1081,7 @@ int fh_absorber_fit_rows(fh_absorber* ab, const double* coefs, size_t r, size_t 
             coo.resize(a.M + 1);
             d2h(coo.data(), a.cell_obs_off.p, a.M + 1, a.st);
         }
-        FH_HIP(hipStreamSynchronize(a.st));
+        stream_wait(a.st);
         const size_t m = a.m;
         for (size_t s = 0; s < a.g1; s++)
             for (uint32_t c = seg[s]; c < seg[s + 1]; c++) {
@@ -1104,7 +1108,7 @@ int fh_absorber_term_values(fh_absorber* ab, const double* coefs, size_t r, size
         std::vector<double> n1(a.g1), nr(a.m);
         d2h(n1.data(), a.n1.p, a.g1, a.st);
         if (a.m) d2h(nr.data(), a.nr.p, a.m, a.st);
-        FH_HIP(hipStreamSynchronize(a.st));
+        stream_wait(a.st);
         std::vector<std::vector<double>> vals(nt), lw(nt);
         vals[a.fe1].resize(a.g1);
         lw[a.fe1] = n1;

@@ -51,6 +51,34 @@ void pool_release_cached();
 void pool_forget_stream(hipStream_t st);
 void pool_stats(size_t* in_use, size_t* cached);
 hipStream_t current_stream();
+
+// Small host<->device transfers through pinned staging blocks (fh_alloc.hip). A copy from
+// pageable memory is synchronous with the host, so every small upload of a parameter array would
+// otherwise stall the host until the stream drains. upload() returns at once (the staging block
+// is reused only after its copy has completed); download() waits for the stream (stream_wait).
+void upload(void* dst_dev, const void* src_host, size_t bytes, hipStream_t st);
+void download(void* dst_host, const void* src_dev, size_t bytes, hipStream_t st);
+// Waits until all work queued on st has completed, spinning on the stream's state for up to 5 ms
+// before blocking (a blocking wait costs tens of microseconds of wake-up latency per poll).
+void stream_wait(hipStream_t st);
+
+// A device->host read into a pinned staging block that does not wait: start() queues the copy
+// on a stream, wait() spins until that copy has completed and returns the block.
+class AsyncRead {
+public:
+    explicit AsyncRead(size_t bytes);
+    ~AsyncRead();
+    AsyncRead(const AsyncRead&) = delete;
+    AsyncRead& operator=(const AsyncRead&) = delete;
+    void start(const void* src_dev, hipStream_t st);
+    const void* wait();
+
+private:
+    void* p_ = nullptr;
+    size_t bytes_ = 0, cap_ = 0;
+    int device_ = 0;
+    hipEvent_t ev_ = nullptr;
+};
 // Sets the stream that pool allocations of this thread are used on, for the scope's lifetime.
 struct StreamScope {
     hipStream_t prev;

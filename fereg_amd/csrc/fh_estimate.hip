@@ -157,8 +157,8 @@ void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, co
     gram_dispatch(k, grid, a.st, z, n, ld, c0, w, v, part.p, std::make_integer_sequence<int, GRAM_KMAX>{});
     FH_CHECK_LAUNCH();
     std::vector<double> h((size_t)grid * npair);
-    FH_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, a.st));
-    FH_HIP(hipStreamSynchronize(a.st));
+    download(h.data(), part.p, h.size() * sizeof(double), a.st);
+    stream_wait(a.st);
     std::vector<double> s(npair, 0.0);
     for (unsigned q = 0; q < grid; q++)
         for (int p = 0; p < npair; p++) s[p] += h[(size_t)q * npair + p];
@@ -176,12 +176,12 @@ void gram(Absorber& a, const double* z, const double* w, size_t n, int r, double
 
 double residual_pass(Absorber& a, const double* z, size_t n, int ld, const double* c_host, const double* w, double* e) {
     DBuf<double> c(ld), part(GRID);
-    FH_HIP(hipMemcpyAsync(c.p, c_host, ld * sizeof(double), hipMemcpyHostToDevice, a.st));
+    upload(c.p, c_host, ld * sizeof(double), a.st);
     hipLaunchKernelGGL(k_resid, dim3(GRID), dim3(TPB), 0, a.st, z, n, ld, c.p, w, e, part.p);
     FH_CHECK_LAUNCH();
     std::vector<double> h(GRID);
-    FH_HIP(hipMemcpyAsync(h.data(), part.p, GRID * sizeof(double), hipMemcpyDeviceToHost, a.st));
-    FH_HIP(hipStreamSynchronize(a.st));
+    download(h.data(), part.p, GRID * sizeof(double), a.st);
+    stream_wait(a.st);
     double s = 0.0;
     for (double x : h) s += x;
     return s;
@@ -226,8 +226,8 @@ uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, in
     FH_CHECK_LAUNCH();
     gram_cols(a, sg.p, G, k, 0, k, nullptr, nullptr, meat);
     unsigned long long c = 0;
-    FH_HIP(hipMemcpyAsync(&c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
+    download(&c, cnt.p, sizeof(c), st);
+    stream_wait(st);
     return c;
 }
 

@@ -156,7 +156,9 @@ void tune_cache_put(const std::string& key, const SpmvPlan& plan);
 void tune_cache_clear();
 
 // rhs pass over rows (n x r, FP64): rhs (m x r, T), b1 (g1 x r), wsq/wsum partials -> out2 (2r).
-void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, double* wsq_wsum);
+// Leaves the per-workgroup partials of wsq / wsum (grid x 2r, returns grid) at the start of
+// a.s64.dpart, which holds room for `extra` more doubles after them.
+unsigned rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, size_t extra);
 // rows -= alpha_s + u_c; alpha (g1 x r) optional.
 void residualize_rows(Absorber& a, double* rows, int r, const double* x, const double* b1, double* alpha);
 
@@ -167,12 +169,13 @@ struct PcgResult {
     int rc = FH_OK;
     uint64_t refinements = 0;
 };
-// Solves S x = rhs for r columns; x holds the start. FP64 throughout.
-PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, int r, const double* ss1, const double* ss0,
-                   double tol, uint64_t maxiter, const fh_ctx* ctx);
+// Solves S x = rhs for r columns; x holds the start (x_maybe_nonzero false: x is zero, which
+// spares the device read that tests it). FP64 throughout.
+PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
+                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx);
 // Mixed form: FP32 inner PCG, FP64 outer residual and update.
-PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, int r, const double* ss1, const double* ss0,
-                    double tol, uint64_t maxiter, const fh_ctx* ctx);
+PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
+                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx);
 
 // ---------------------------------------------------------------- estimation (fh_estimate.hip)
 // G = sum_i w_i z_i z_i' (r x r, FP64 on the host), z n x r row-major; w may be null.

@@ -55,13 +55,13 @@ void tune_cache_clear() {
     tune_cache().clear();
 }
 
-void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, double* wsq_wsum) {
+unsigned rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, size_t extra) {
     require(r >= 1 && r <= 16, "rhs: 1..16 columns");
     hipStream_t st = a.st;
     OpArgs<double> A = op_args_impl<double>(a);
     const unsigned grid = seg_grid(a);
     Scratch<double>& s = a.s64;
-    s.dpart.reserve((size_t)grid * 2 * r);
+    s.dpart.reserve((size_t)grid * 2 * r + extra);
     if (a.nb > 0) s.zc.reserve(a.M * r);
     const int L = (int)a.seg_lanes;
     const double* wr = a.weighted ? a.w_rows.p : nullptr;
@@ -84,13 +84,7 @@ void rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, d
         FH_CHECK_LAUNCH();
         if (ypart) bucket_reduce<double>(ypart, nbk, (size_t)bi.g, r, rhs + bi.off * r, nullptr, st);
     }
-    // wsq / wsum: fixed-order sum of the workgroup partials on the host
-    std::vector<double> h((size_t)grid * 2 * r);
-    FH_HIP(hipMemcpyAsync(h.data(), s.dpart.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
-    for (int j = 0; j < 2 * r; j++) wsq_wsum[j] = 0.0;
-    for (unsigned q = 0; q < grid; q++)
-        for (int j = 0; j < 2 * r; j++) wsq_wsum[j] += h[(size_t)q * 2 * r + j];
+    return grid;
 }
 
 void residualize_rows(Absorber& a, double* rows, int r, const double* x, const double* b1, double* alpha) {

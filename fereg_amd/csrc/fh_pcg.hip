@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
+#include <initializer_list>
 
 #include "fh_internal.hpp"
 
@@ -360,7 +362,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
         h.active[j] = ss0[j] > 0.0;
         h.absthr[j] = absthr ? absthr[j] : 0.0;
     }
-    FH_HIP(hipMemcpyAsync(S, &h, sizeof(PcgState), hipMemcpyHostToDevice, st));
+    upload(S, &h, sizeof(PcgState), st);
     hipLaunchKernelGGL(k_zero_slots, dim3(8), dim3(TPB), 0, st, s.dpart.p, 0, (int)NSLOT);
     // r = rhs - S x
     if (x_nonzero) {
@@ -376,13 +378,13 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
         // M = S exactly: x += M^-1 (rhs - S x) on the active columns (cg.rs:171-198)
         DBuf<int32_t> mask(r);
         std::vector<int32_t> hm(h.active, h.active + r);
-        FH_HIP(hipMemcpyAsync(mask.p, hm.data(), r * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        upload(mask.p, hm.data(), r * sizeof(int32_t), st);
         if constexpr (std::is_same<T, double>::value)
             hipLaunchKernelGGL(k_add_to64<double>, dim3(vgrid(nv)), dim3(TPB), 0, st, x, s.z.p, nv, r, mask.p);
         else
             throw Error(FH_INTERNAL, "exact path runs in FP64");
         FH_CHECK_LAUNCH();
-        FH_HIP(hipStreamSynchronize(st));
+        stream_wait(st);
         out.iterations = 1;
         out.exact = true;
         return out;
@@ -391,8 +393,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     hipLaunchKernelGGL(k_mask_copy<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, s.z.p, s.p.p, nv, r, S);
     FH_CHECK_LAUNCH();
     WeightArrays<T>& W = warr<T>(a);
-    int batch = WINDOW;
-    for (;;) {
+    auto enqueue = [&](int batch) {
         for (int b = 0; b < batch; b++) {
             spmv<T>(a, s.p.p, s.q.p, r, s, done, st);
             hipLaunchKernelGGL(k_dot<T>, dim3(DOTG), dim3(TPB), 0, st, s.p.p, s.q.p, (size_t)0, m, r, s.dpart.p,
@@ -410,14 +411,36 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
             hipLaunchKernelGGL(k_update_p<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, s.p.p, s.z.p, nv, r, S);
             FH_CHECK_LAUNCH();
         }
-        FH_HIP(hipMemcpyAsync(&h, S, sizeof(PcgState), hipMemcpyDeviceToHost, st));
-        FH_HIP(hipStreamSynchronize(st));
-        if (h.done) break;
-        if (ctx && ctx->interrupt && ctx->interrupt(ctx->user)) throw Error(FH_INTERRUPTED, "interrupted");
-        batch = 2;
+    };
+    // Pipelined polling: the status of batch k is read while batch k+1 (two more iterations) is
+    // already queued, so the device never waits for the host's decision. Once the stopping rule
+    // has fired every kernel of a batch returns at once, so the batch queued past the end costs
+    // only its launches.
+    struct Poll {
+        uint32_t done, err;
+        uint64_t iters;
+    };
+    static_assert(offsetof(PcgState, err) == offsetof(PcgState, done) + 4 &&
+                      offsetof(PcgState, iters) == offsetof(PcgState, done) + 8,
+                  "poll fields are contiguous");
+    AsyncRead rd0(sizeof(Poll)), rd1(sizeof(Poll));
+    AsyncRead* rd[2] = {&rd0, &rd1};
+    const void* poll_src = &S->done;
+    enqueue(WINDOW);
+    rd[0]->start(poll_src, st);
+    Poll pl{};
+    for (int k = 0;; k++) {
+        enqueue(2);
+        rd[(k + 1) & 1]->start(poll_src, st);
+        pl = *static_cast<const Poll*>(rd[k & 1]->wait());
+        if (pl.done) break;
+        if (ctx && ctx->interrupt && ctx->interrupt(ctx->user)) {
+            stream_wait(st);
+            throw Error(FH_INTERRUPTED, "interrupted");
+        }
     }
-    out.iterations = h.iters;
-    out.maxiter_hit = h.err != 0;
+    out.iterations = pl.iters;
+    out.maxiter_hit = pl.err != 0;
     return out;
 }
 
@@ -449,27 +472,37 @@ __global__ void k_dot_pair(const T* __restrict__ a, const T* __restrict__ b, siz
     }
 }
 
-// Per-column dot products a'b of two m x r FP64 blocks (fixed order).
-std::vector<double> dots64(Absorber& a, const double* x, const double* y, int r) {
-    DBuf<double> part((size_t)DOTG * r);
-    hipLaunchKernelGGL(k_dot_pair<double>, dim3(DOTG), dim3(TPB), 0, a.st, x, y, (size_t)a.m, r, part.p);
-    FH_CHECK_LAUNCH();
-    std::vector<double> h((size_t)DOTG * r), out(r, 0.0);
-    FH_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, a.st));
-    FH_HIP(hipStreamSynchronize(a.st));
-    for (int q = 0; q < DOTG; q++)
-        for (int j = 0; j < r; j++) out[j] += h[(size_t)q * r + j];
+// Per-column dot products x_i'y_i of pairs of m x r FP64 blocks (fixed order), read back in one
+// transfer: out[i][j].
+std::vector<std::vector<double>> dots64(Absorber& a, std::initializer_list<std::pair<const double*, const double*>> xy,
+                                        int r) {
+    const size_t np = xy.size();
+    DBuf<double> part(np * DOTG * r);
+    size_t i = 0;
+    for (auto& pr : xy) {
+        hipLaunchKernelGGL(k_dot_pair<double>, dim3(DOTG), dim3(TPB), 0, a.st, pr.first, pr.second, (size_t)a.m, r,
+                           part.p + i * DOTG * r);
+        FH_CHECK_LAUNCH();
+        i++;
+    }
+    std::vector<double> h(np * DOTG * r);
+    download(h.data(), part.p, h.size() * sizeof(double), a.st);
+    std::vector<std::vector<double>> out(np, std::vector<double>(r, 0.0));
+    for (size_t k = 0; k < np; k++)
+        for (int q = 0; q < DOTG; q++)
+            for (int j = 0; j < r; j++) out[k][j] += h[(k * DOTG + q) * r + j];
     return out;
 }
+std::vector<double> dots64(Absorber& a, const double* x, const double* y, int r) { return dots64(a, {{x, y}}, r)[0]; }
 
 }  // namespace
 
-PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, int r, const double* ss1, const double* ss0, double tol,
-                   uint64_t maxiter, const fh_ctx* ctx) {
-    // a nonzero start (warm) is detected by the caller passing nonzero x; test it on the device
-    std::vector<double> xn = dots64(a, x, x, r);
+PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
+                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx) {
+    // a warm start may still be all zero: test it on the device
     bool nz = false;
-    for (double v : xn) nz |= v != 0.0;
+    if (x_maybe_nonzero)
+        for (double v : dots64(a, x, x, r)) nz |= v != 0.0;
     PcgRun run = run_pcg<double>(a, rhs, x, nz, r, ss1, ss0, tol, maxiter, 0.0, nullptr, ctx);
     PcgResult res;
     res.iterations = run.iterations;
@@ -478,11 +511,11 @@ PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, int r, const doubl
     return res;
 }
 
-PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, int r, const double* ss1, const double* ss0, double tol,
-                    uint64_t maxiter, const fh_ctx* ctx) {
+PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
+                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx) {
     PcgResult res;
     if (a.ms == a.m) {  // exact dense path: nothing to iterate
-        return pcg_fp64(a, rhs, x, r, ss1, ss0, tol, maxiter, ctx);
+        return pcg_fp64(a, rhs, x, x_maybe_nonzero, r, ss1, ss0, tol, maxiter, ctx);
     }
     ensure_w32(a);
     hipStream_t st = a.st;
@@ -492,9 +525,9 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, int r, const doub
     s64.r.reserve(nv); s64.z.reserve(nv); s64.q.reserve(nv);
     s64.dpart.reserve((size_t)NSLOT * DOTG * RMAX);
     s32.x.reserve(nv); s32.rhs.reserve(nv);
-    std::vector<double> xn = dots64(a, x, x, r);
     bool nz = false;
-    for (double v : xn) nz |= v != 0.0;
+    if (x_maybe_nonzero)
+        for (double v : dots64(a, x, x, r)) nz |= v != 0.0;
     std::vector<int32_t> act(r);
     for (int j = 0; j < r; j++) act[j] = ss0[j] > 0.0;
     DBuf<int32_t> mask(r);
@@ -510,8 +543,13 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, int r, const doub
         }
         FH_CHECK_LAUNCH();
         precond<double>(a, s64.r.p, s64.z.p, r, s64.dpart.p, nullptr, st);
-        std::vector<double> rz = dots64(a, s64.r.p, s64.z.p, r);
-        std::vector<double> xb = dots64(a, x, rhs, r), xr = dots64(a, x, s64.r.p, r);
+        std::vector<double> rz, xb(r, 0.0), xr(r, 0.0);
+        if (nz) {
+            auto d = dots64(a, {{s64.r.p, s64.z.p}, {x, rhs}, {x, s64.r.p}}, r);
+            rz = d[0], xb = d[1], xr = d[2];
+        } else {
+            rz = dots64(a, s64.r.p, s64.z.p, r);  // x = 0: x'b = x'r = 0
+        }
         std::vector<double> ss1c(r), ss0c(r), absthr(r);
         bool any = false;
         for (int j = 0; j < r; j++) {
@@ -536,7 +574,7 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, int r, const doub
         PcgRun run = run_pcg<float>(a, s32.rhs.p, s32.x.p, false, r, ss1c.data(), ss0c.data(), tol,
                                     maxiter > total_iters ? maxiter - total_iters : 1, rel * rel, absthr.data(), ctx);
         total_iters += run.iterations;
-        FH_HIP(hipMemcpyAsync(mask.p, act.data(), r * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        upload(mask.p, act.data(), r * sizeof(int32_t), st);
         hipLaunchKernelGGL(k_add_to64<float>, dim3(vgrid(nv)), dim3(TPB), 0, st, x, s32.x.p, nv, r, mask.p);
         FH_CHECK_LAUNCH();
         nz = true;
@@ -545,7 +583,7 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, int r, const doub
             break;
         }
     }
-    FH_HIP(hipStreamSynchronize(st));
+    stream_wait(st);
     res.iterations = std::max<uint64_t>(total_iters, 1);
     return res;
 }

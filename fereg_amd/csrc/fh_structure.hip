@@ -277,14 +277,15 @@ inline uint32_t lanes(double mean) {
     while (L < 64 && L < mean) L <<= 1;
     return L;
 }
-// Lanes per segment of the segment passes: about 8 cells per lane.
-inline uint32_t seg_lanes_for(size_t M, size_t g1) { return lanes(g1 ? double(M) / double(g1) / 8.0 : 1.0); }
+// Lanes per segment of the segment passes: about 4 cells per lane (the gathers of a lane are
+// dependent on its code loads, so shorter chains keep more of them in flight).
+inline uint32_t seg_lanes_for(size_t M, size_t g1) { return lanes(g1 ? double(M) / double(g1) / 4.0 : 1.0); }
 
 template <class T>
 T read_scalar(const T* dev, hipStream_t st) {
     T v;
-    FH_HIP(hipMemcpyAsync(&v, dev, sizeof(T), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
+    download(&v, dev, sizeof(T), st);
+    stream_wait(st);
     return v;
 }
 
@@ -470,7 +471,7 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
     }
     if (M != n) {
         uint32_t nn = (uint32_t)n;
-        FH_HIP(hipMemcpyAsync(start.p + M, &nn, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        upload(start.p + M, &nn, sizeof(uint32_t), st);
         a.cell_obs_off = std::move(start);
     } else {
         a.cell_obs_off.release();
@@ -486,7 +487,7 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         std::vector<uint32_t> boff(nsb);
         for (int b = 0; b < nsb; b++) boff[b] = (uint32_t)a.blocks[b].off;
         DBuf<uint32_t> dboff(nsb);
-        FH_HIP(hipMemcpyAsync(dboff.p, boff.data(), nsb * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        upload(dboff.p, boff.data(), nsb * sizeof(uint32_t), st);
         DBuf<unsigned long long> acc(1);
         FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
         if (nsb == 1) {

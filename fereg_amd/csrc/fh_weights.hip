@@ -259,8 +259,8 @@ double sum_dev(const double* v, size_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(TPB), 0, st, v, n, part.p);
     FH_CHECK_LAUNCH();
     std::vector<double> h(nb);
-    FH_HIP(hipMemcpyAsync(h.data(), part.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
+    download(h.data(), part.p, nb * sizeof(double), st);
+    stream_wait(st);
     double s = 0.0;
     for (double x : h) s += x;
     return s;
@@ -513,8 +513,8 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             DBuf<double> dd(2 * ms);
             hipLaunchKernelGGL(k_extract_diag, dim3(grid_for(ms, TPB, 1024)), dim3(TPB), 0, st, sfull.p, ms, dd.p);
             FH_CHECK_LAUNCH();
-            FH_HIP(hipMemcpyAsync(h.data(), dd.p, 2 * ms * sizeof(double), hipMemcpyDeviceToHost, st));
-            FH_HIP(hipStreamSynchronize(st));
+            download(h.data(), dd.p, 2 * ms * sizeof(double), st);
+            stream_wait(st);
             for (size_t k = 0; k < ms; k++) {
                 dg[k] = h[k];
                 nrs[k] = h[ms + k];
@@ -538,8 +538,8 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
                 DBuf<uint32_t> dkeep(k);
                 DBuf<double> dsc(k), A(kk), col(2 * (size_t)k);
                 DBuf<uint8_t> grounded(k);
-                FH_HIP(hipMemcpyAsync(dkeep.p, keep.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-                FH_HIP(hipMemcpyAsync(dsc.p, sc.data(), k * sizeof(double), hipMemcpyHostToDevice, st));
+                upload(dkeep.p, keep.data(), k * sizeof(uint32_t), st);
+                upload(dsc.p, sc.data(), k * sizeof(double), st);
                 const unsigned gs = grid_for(kk, TPB, 8192);
                 hipLaunchKernelGGL(k_sweep_init, dim3(gs), dim3(TPB), 0, st, sfull.p, ms, dkeep.p, dsc.p, k, A.p, col.p);
                 for (int j = 0; j < k; j++)
@@ -548,12 +548,11 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
                 hipLaunchKernelGGL(k_sweep_scatter, dim3(gs), dim3(TPB), 0, st, A.p, k, dkeep.p, dsc.p, grounded.p, ms,
                                    a.w64.minv.p, a.w32.minv.p);
                 FH_CHECK_LAUNCH();
-                FH_HIP(hipStreamSynchronize(st));  // keep/sc host vectors and the scratch end here
             }
         } else {
             if (!(so.nsb == 1 && ms <= 21)) {
-                FH_HIP(hipMemcpyAsync(h.data(), sfull.p, len * sizeof(double), hipMemcpyDeviceToHost, st));
-                FH_HIP(hipStreamSynchronize(st));
+                download(h.data(), sfull.p, len * sizeof(double), st);
+                stream_wait(st);
             }
             std::vector<double> S(h.begin(), h.begin() + ms * ms);
             for (size_t k = 0; k < ms; k++) {
@@ -566,26 +565,25 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         std::vector<float> inv32(inv.begin(), inv.end());
         std::vector<double> zeros(ms, 0.0);
         std::vector<float> zeros32(ms, 0.0f);
-        FH_HIP(hipMemcpyAsync(a.nr.p, nrs.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
+        upload(a.nr.p, nrs.data(), ms * sizeof(double), st);
         if (a.small_dense) {
             std::vector<float> nrs32(nrs.begin(), nrs.end());
             a.w64.nrs.reserve(ms);
             a.w32.nrs.reserve(ms);
-            FH_HIP(hipMemcpyAsync(a.w64.nrs.p, nrs.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
-            FH_HIP(hipMemcpyAsync(a.w32.nrs.p, nrs32.data(), ms * sizeof(float), hipMemcpyHostToDevice, st));
+            upload(a.w64.nrs.p, nrs.data(), ms * sizeof(double), st);
+            upload(a.w32.nrs.p, nrs32.data(), ms * sizeof(float), st);
         }
-        FH_HIP(hipMemcpyAsync(a.diag.p, dg.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
-        FH_HIP(hipMemcpyAsync(a.red.p, rd.data(), ms, hipMemcpyHostToDevice, st));
-        FH_HIP(hipMemcpyAsync(a.w64.inv_diag.p, zeros.data(), ms * sizeof(double), hipMemcpyHostToDevice, st));
-        FH_HIP(hipMemcpyAsync(a.w32.inv_diag.p, zeros32.data(), ms * sizeof(float), hipMemcpyHostToDevice, st));
+        upload(a.diag.p, dg.data(), ms * sizeof(double), st);
+        upload(a.red.p, rd.data(), ms, st);
+        upload(a.w64.inv_diag.p, zeros.data(), ms * sizeof(double), st);
+        upload(a.w32.inv_diag.p, zeros32.data(), ms * sizeof(float), st);
         if (!dev_inverse) {
             a.w64.minv.reserve(ms * ms);
             a.w32.minv.reserve(ms * ms);
-            FH_HIP(hipMemcpyAsync(a.w64.minv.p, inv.data(), ms * ms * sizeof(double), hipMemcpyHostToDevice, st));
-            FH_HIP(hipMemcpyAsync(a.w32.minv.p, inv32.data(), ms * ms * sizeof(float), hipMemcpyHostToDevice, st));
+            upload(a.w64.minv.p, inv.data(), ms * ms * sizeof(double), st);
+            upload(a.w32.minv.p, inv32.data(), ms * ms * sizeof(float), st);
         }
     }
-    FH_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace fh
