@@ -44,7 +44,7 @@ class SolverConfig(ctypes.Structure):
 
 class SpmvPlan(ctypes.Structure):
     _fields_ = [("dense", ctypes.c_int32), ("seg_lanes", ctypes.c_uint32), ("block_lanes", ctypes.c_uint32 * 9),
-                ("tuned", ctypes.c_int32)]
+                ("block_mode", ctypes.c_int32 * 9), ("tuned", ctypes.c_int32)]
 
 
 class SolveOptsC(ctypes.Structure):

@@ -68,7 +68,7 @@ class Absorber:
     tensors), g: levels per term, weights: kept-order weights or None.
     """
 
-    def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, bucket_log2=0,
+    def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, bucket_log2=-1,
                  autotune=False, verbose=0):
         lib = L.lib()
         self._keep = []
@@ -124,12 +124,14 @@ class Absorber:
         p = L.SpmvPlan()
         L.check(L.lib().fh_absorber_get_plan(self._h, int(precision), int(r), ctypes.byref(p)))
         return {"dense": int(p.dense), "seg_lanes": int(p.seg_lanes), "block_lanes": [int(v) for v in p.block_lanes],
-                "tuned": bool(p.tuned)}
+                "block_mode": [int(v) for v in p.block_mode], "tuned": bool(p.tuned)}
 
     def set_plan(self, precision, r, plan):
-        """Pins a plan (a dict as returned by plan(); lanes 0 = default)."""
+        """Pins a plan (a dict as returned by plan(); lanes 0 = default, block_mode -1 = default)."""
         bl = list(plan.get("block_lanes", [])) + [0] * 9
-        p = L.SpmvPlan(int(plan.get("dense", 1)), int(plan.get("seg_lanes", 0)), (ctypes.c_uint32 * 9)(*bl[:9]), 1)
+        bm = list(plan.get("block_mode", [])) + [-1] * 9
+        p = L.SpmvPlan(int(plan.get("dense", 1)), int(plan.get("seg_lanes", 0)), (ctypes.c_uint32 * 9)(*bl[:9]),
+                       (ctypes.c_int32 * 9)(*bm[:9]), 1)
         L.check(L.lib().fh_absorber_set_plan(self._h, int(precision), int(r), ctypes.byref(p)))
 
     def time_spmv(self, r, precision, reps=20):

@@ -265,18 +265,19 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
     const size_t nws = (size_t)grid * 2 * r;
     hipLaunchKernelGGL(k_wsq_b1, dim3(pg), dim3(TPB), 0, st, a.b1.p, a.n1.p, g1, r, a.s64.dpart.p + nws);
     FH_CHECK_LAUNCH();
-    std::vector<double> hall(nws + (size_t)pg * r);
-    d2h(hall.data(), a.s64.dpart.p, hall.size(), st);
+    // one fixed-order device reduction for each, one readback
+    DBuf<double> sums(3 * (size_t)r);
+    sum_parts_to(a.s64.dpart.p, grid, (size_t)(2 * r), sums.p, st);
+    sum_parts_to(a.s64.dpart.p + nws, pg, (size_t)r, sums.p + 2 * r, st);
+    std::vector<double> hs(3 * (size_t)r);
+    d2h(hs.data(), sums.p, hs.size(), st);
     check_interrupt(ctx);
-    std::vector<double> ws(2 * r, 0.0);
-    for (unsigned q = 0; q < grid; q++)
-        for (int j = 0; j < 2 * r; j++) ws[j] += hall[(size_t)q * 2 * r + j];
-    const double* hp = hall.data() + nws;
+    const std::vector<double> ws(hs.begin(), hs.begin() + 2 * r);
+    const double* hp = hs.data() + 2 * r;
     std::vector<double> ss0(r), ss1(r);
     std::vector<int32_t> zero(r);
     for (int j = 0; j < r; j++) {
-        double b1sq = 0.0;
-        for (unsigned q = 0; q < pg; q++) b1sq += hp[(size_t)q * r + j];
+        const double b1sq = hp[j];
         const double wsq = ws[j], wsum = ws[r + j];
         const double v = wsq - wsum * wsum / a.sw;
         ss0[j] = v > 1e-15 * wsq ? v : 0.0;
@@ -716,7 +717,7 @@ const char* fh_last_error(void) { return g_last_error.c_str(); }
 void fh_config_default(fh_solver_config* cfg) {
     cfg->small_limit = 4096;
     cfg->precision = FH_PLAIN_FP64;
-    cfg->bucket_log2 = 0;
+    cfg->bucket_log2 = -1;
     cfg->autotune = 0;
     cfg->reserved = 0;
 }
@@ -744,8 +745,8 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         if (cfg) c = *cfg;
         a.small_limit = c.small_limit;
         a.precision = c.precision;
-        require(c.bucket_log2 >= 0 && c.bucket_log2 <= 31, "bucket_log2 must be in 0..31");
-        a.bucket_shift = c.bucket_log2 > 0 ? c.bucket_log2 : 31;
+        require(c.bucket_log2 >= -1 && c.bucket_log2 <= 31, "bucket_log2 must be in -1..31");
+        a.bucket_log2 = c.bucket_log2;
         require(c.autotune == 0 || c.autotune == 1, "autotune must be 0 or 1");
         a.autotune = c.autotune == 1;
         std::vector<DBuf<uint32_t>> up(nterms);
@@ -1003,8 +1004,11 @@ int fh_absorber_get_plan(const fh_absorber* ab, int32_t precision, size_t r, fh_
         *out = fh_spmv_plan{};
         out->dense = pl.dense;
         out->seg_lanes = pl.seg_lanes ? pl.seg_lanes : a.seg_lanes;
-        for (size_t b = 0; b < a.nb && b < 9; b++)
-            out->block_lanes[b] = b < pl.blanes.size() && pl.blanes[b] ? pl.blanes[b] : a.blanes[b];
+        for (size_t b = 0; b < a.nb && b < 9; b++) {
+            const LvlMode md = lvl_mode(a, pl, b, (int)r, precision == FH_PLAIN_FP64 ? 8 : 4);
+            out->block_lanes[b] = (uint32_t)md.lanes;
+            out->block_mode[b] = a.nbk[b] > 1 ? (md.merge ? 0 : 1) : -1;
+        }
         out->tuned = pl.tuned ? 1 : 0;
     });
 }
@@ -1021,9 +1025,12 @@ int fh_absorber_set_plan(fh_absorber* ab, int32_t precision, size_t r, const fh_
         pl.dense = in->dense;
         pl.seg_lanes = in->seg_lanes;
         pl.blanes.assign(a.nb, 0);
+        pl.xcd.assign(a.nb, -1);
         for (size_t b = 0; b < a.nb && b < 9; b++) {
             require(pow2(in->block_lanes[b]), "plan: lanes must be 0 or a power of two up to 64");
+            require(in->block_mode[b] >= -1 && in->block_mode[b] <= 1, "plan: block mode must be -1, 0 or 1");
             pl.blanes[b] = in->block_lanes[b];
+            pl.xcd[b] = in->block_mode[b];
         }
         pl.tuned = true;
         a.plan_for(precision == FH_PLAIN_FP64, (int)r) = pl;

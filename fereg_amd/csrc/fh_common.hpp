@@ -58,8 +58,7 @@ hipStream_t current_stream();
 // is reused only after its copy has completed); download() waits for the stream (stream_wait).
 void upload(void* dst_dev, const void* src_host, size_t bytes, hipStream_t st);
 void download(void* dst_host, const void* src_dev, size_t bytes, hipStream_t st);
-// Waits until all work queued on st has completed, spinning on the stream's state for up to 5 ms
-// before blocking (a blocking wait costs tens of microseconds of wake-up latency per poll).
+// Waits until all work queued on st has completed (spinning on an event, then blocking).
 void stream_wait(hipStream_t st);
 
 // A device->host read into a pinned staging block that does not wait: start() queues the copy

@@ -148,6 +148,42 @@ __global__ void k_count_nonempty(const uint32_t* __restrict__ off, size_t g, uns
 
 }  // namespace
 
+// out[e] = sum over q < nparts of part[q * len + e], one workgroup per entry: strided thread sums,
+// then the waves in order (a fixed order independent of the device's scheduling).
+__global__ __launch_bounds__(TPB) void k_sum_parts(const double* __restrict__ part, size_t nparts, size_t len,
+                                                   double* __restrict__ out) {
+    __shared__ double red[TPB / WAVE];
+    for (size_t e = blockIdx.x; e < len; e += gridDim.x) {
+        double sm = 0.0;
+        for (size_t q = threadIdx.x; q < nparts; q += TPB) sm += part[q * len + e];
+        sm = wave_sum(sm);
+        if (lane_id() == 0) red[threadIdx.x / WAVE] = sm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < TPB / WAVE; w++) t += red[w];
+            out[e] = t;
+        }
+        __syncthreads();
+    }
+}
+
+void sum_parts_to(const double* part, size_t nparts, size_t len, double* out, hipStream_t st) {
+    if (len == 0) return;
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>(len, 4096)), dim3(TPB), 0, st, part, nparts, len,
+                       out);
+    FH_CHECK_LAUNCH();
+}
+
+std::vector<double> sum_parts_dev(const double* part, size_t nparts, size_t len, hipStream_t st) {
+    std::vector<double> h(len, 0.0);
+    if (len == 0) return h;
+    DBuf<double> out(len);
+    sum_parts_to(part, nparts, len, out.p, st);
+    download(h.data(), out.p, len * sizeof(double), st);
+    return h;
+}
+
 void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
                double* G_host) {
     require(k >= 1 && k <= GRAM_KMAX, "at most 21 regressors are supported");
@@ -156,12 +192,7 @@ void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, co
     DBuf<double> part((size_t)grid * npair);
     gram_dispatch(k, grid, a.st, z, n, ld, c0, w, v, part.p, std::make_integer_sequence<int, GRAM_KMAX>{});
     FH_CHECK_LAUNCH();
-    std::vector<double> h((size_t)grid * npair);
-    download(h.data(), part.p, h.size() * sizeof(double), a.st);
-    stream_wait(a.st);
-    std::vector<double> s(npair, 0.0);
-    for (unsigned q = 0; q < grid; q++)
-        for (int p = 0; p < npair; p++) s[p] += h[(size_t)q * npair + p];
+    const std::vector<double> s = sum_parts_dev(part.p, grid, (size_t)npair, a.st);
     int p = 0;
     for (int x = 0; x < k; x++)
         for (int y = x; y < k; y++, p++) {
@@ -179,12 +210,7 @@ double residual_pass(Absorber& a, const double* z, size_t n, int ld, const doubl
     upload(c.p, c_host, ld * sizeof(double), a.st);
     hipLaunchKernelGGL(k_resid, dim3(GRID), dim3(TPB), 0, a.st, z, n, ld, c.p, w, e, part.p);
     FH_CHECK_LAUNCH();
-    std::vector<double> h(GRID);
-    download(h.data(), part.p, GRID * sizeof(double), a.st);
-    stream_wait(a.st);
-    double s = 0.0;
-    for (double x : h) s += x;
-    return s;
+    return sum_parts_dev(part.p, GRID, 1, a.st)[0];
 }
 
 void hetero_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,

@@ -7,9 +7,9 @@
 //   cell_obs_off  u32[M+1]    first row of every cell (absent when every cell is one row)
 //   cell_seg      u32[M]      FE1 level of every cell
 //   per block b, the level-major transpose of the cells in segment buckets (built once):
-//     the FE1 levels are cut into nbk[b] buckets of 2^bshift[b] consecutive segments (one
-//     bucket unless the block's levels are long: ~2048 cells per item); cells are ordered by
-//     (bucket, level, cell). Item i = bucket * g_b + level:
+//     the FE1 levels are cut into nbk[b] buckets of bwidth[b] consecutive segments (one per XCD
+//     once g1 is large, more when the block's levels are long: ~2048 cells per item); cells are
+//     ordered by (bucket, level, cell). Item i = bucket * g_b + level:
 //     boff        u32[nbk*g_b+1] item offsets;  bperm u32[M] cells in (bucket, level, cell) order
 //     lseg        u32[M]      FE1 level of the cell at each position
 //     lother      u32[M*(nb-1)] codes of the other blocks at each position
@@ -27,6 +27,13 @@
 namespace fh {
 
 constexpr int NB_MAX = 8;  // non-FE1 blocks supported by the kernels
+// MI355X: 8 accelerator complex dies, each with its own L2; the hardware hands consecutive
+// workgroups of a launch to the XCDs in turn.
+constexpr int XCD_COUNT = 8;
+// Working set of gathered rows that a level pass keeps per XCD (half of its 4 MB L2).
+constexpr size_t L2_TARGET = 2u << 20;
+// Automatic FE1 buckets (one per XCD) from this many FE1 levels on.
+constexpr uint64_t AUTO_BUCKET_G1 = 1u << 19;
 
 struct BlockInfo {
     int64_t term;
@@ -62,6 +69,8 @@ struct SpmvPlan {
     int dense = 1;           // the single small block as dense products inside the passes too
     uint32_t seg_lanes = 0;  // lanes per segment of the segment pass
     std::vector<uint32_t> blanes;  // lanes per item of each block's level pass
+    std::vector<int> xcd;          // per block: 1 (bucket, level) items spread over the XCDs, 0 one
+                                   // item per level over its buckets, -1 the default rule
     bool tuned = false;
 };
 
@@ -79,9 +88,10 @@ struct Absorber {
     std::vector<DBuf<uint32_t>> lperm, loff;                // plain level lists, on demand
     std::vector<DBuf<uint32_t>> bperm, boff, lseg, lother;  // bucketed transpose (large blocks)
     std::vector<uint32_t> nbk;                              // buckets per block
-    std::vector<int> bshift;                                // FE1 levels per bucket = 2^bshift
+    std::vector<uint32_t> bwidth;                           // FE1 levels per bucket
     std::vector<uint32_t> blanes;                           // lanes per item in the level passes
-    int bucket_shift = 31;  // configured bucket size of blocks with short levels (31 = one bucket)
+    std::vector<uint32_t> llanes;                           // lanes per level (one item per level)
+    int bucket_log2 = -1;  // configured FE1 bucket size 2^k of the level records (0 one bucket, -1 auto)
     std::vector<uint32_t> host_pi;
     bool host_pi_valid = false;
     // weights
@@ -147,8 +157,31 @@ template <class T>
 OpArgs<T> op_args(const Absorber& a);
 
 // y = S p (m x r). t_buf >= g1*r, part >= spmv_part_len(a, r).
+// pq (optional): SPMV_DOTG x 16 per-workgroup partials of p'y per column (fixed order).
+constexpr int SPMV_DOTG = 256;
 template <class T>
-void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st);
+void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st,
+          double* pq = nullptr);
+// Level pass of block b under a plan: XCD buckets ((bucket, level) items, partial rows reduced in
+// bucket order) while a bucket's share of t fits comfortably in an XCD's L2, else one item per
+// level over its buckets (merge); and the lanes per item.
+struct LvlMode {
+    bool merge;
+    int lanes;
+};
+inline LvlMode lvl_mode(const Absorber& a, const SpmvPlan& plan, size_t b, int r, int tbytes) {
+    const uint32_t nbk = a.nbk[b];
+    const int x = b < plan.xcd.size() && plan.xcd[b] >= 0 ? plan.xcd[b]
+                  : nbk < (uint32_t)XCD_COUNT                               ? 1
+                  : (a.g1 / nbk + 1) * (size_t)r * (size_t)tbytes <= L2_TARGET ? 1
+                                                                             : 0;
+    LvlMode m;
+    m.merge = nbk > 1 && x == 0;
+    const uint32_t def = m.merge ? a.llanes[b] : a.blanes[b];
+    m.lanes = (int)(b < plan.blanes.size() && plan.blanes[b] ? plan.blanes[b] : def);
+    return m;
+}
+
 // Process-wide cache of tuned S*p plans keyed by the structure signature (plan_key).
 std::string plan_key(const Absorber& a, int tbytes, int rclass);
 bool tune_cache_get(const std::string& key, SpmvPlan& plan);
@@ -161,6 +194,11 @@ void tune_cache_clear();
 unsigned rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, size_t extra);
 // rows -= alpha_s + u_c; alpha (g1 x r) optional.
 void residualize_rows(Absorber& a, double* rows, int r, const double* x, const double* b1, double* alpha);
+
+// Fixed-order device reduction of per-workgroup partials part[q * len + e] (q < nparts) read
+// back as len values (fh_estimate.hip).
+std::vector<double> sum_parts_dev(const double* part, size_t nparts, size_t len, hipStream_t st);
+void sum_parts_to(const double* part, size_t nparts, size_t len, double* out_dev, hipStream_t st);
 
 // ---------------------------------------------------------------- PCG (fh_pcg.hip)
 struct PcgResult {

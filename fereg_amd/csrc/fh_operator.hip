@@ -14,10 +14,12 @@ template OpArgs<double> op_args<double>(const Absorber&);
 template OpArgs<float> op_args<float>(const Absorber&);
 
 template <class T>
-void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st) {
-    spmv_planned<T>(a, p, y, r, s, done, st);
+void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st,
+          double* pq) {
+    spmv_planned<T>(a, p, y, r, s, done, st, pq);
 }
-template void spmv<double>(const Absorber&, const double*, double*, int, Scratch<double>&, const uint32_t*, hipStream_t);
+template void spmv<double>(const Absorber&, const double*, double*, int, Scratch<double>&, const uint32_t*, hipStream_t,
+                           double*);
 
 namespace {
 std::mutex tune_mu;

@@ -267,23 +267,21 @@ __global__ __launch_bounds__(WG) void k_small_dense_t(const T* __restrict__ E, s
 }
 
 // Partial sums over a column range of y_S[k] = sum_l C[k][l] p_L[l] - sum_s D[k][s] t_s for the
-// small levels k in chunk blockIdx.y (KC levels), one column quad per thread and step. Each
-// workgroup writes part[(k r + j) nparts + blockIdx.x] (nparts = gridDim.x).
+// small levels k in chunk ch (KC levels), one column quad per thread and step. Workgroup bx of nwg
+// writes part[(k r + j) nwg + bx].
 template <class T, int RM, int KC>
-__global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ E, size_t ld, size_t coff, size_t g1,
-                                                    size_t mL, const T* __restrict__ t, const T* __restrict__ pL,
-                                                    int ms, int r, T* __restrict__ part,
-                                                    const uint32_t* __restrict__ done) {
-    if (done && *done) return;
+__device__ __forceinline__ void small_dense_body(const T* __restrict__ E, size_t ld, size_t coff, size_t g1, size_t mL,
+                                                 const T* __restrict__ t, const T* __restrict__ pL, int ms, int r,
+                                                 T* __restrict__ part, unsigned bx, unsigned nwg, unsigned ch) {
     __shared__ T red[WAVES][KC][RM];
-    const int k0 = blockIdx.y * KC;
+    const int k0 = (int)ch * KC;
     T acc[KC][RM];
 #pragma unroll
     for (int kk = 0; kk < KC; kk++)
 #pragma unroll
         for (int j = 0; j < RM; j++) acc[kk][j] = T(0);
     const size_t nq = ld / 4;
-    for (size_t q = (size_t)blockIdx.x * WG + threadIdx.x; q < nq; q += (size_t)gridDim.x * WG) {
+    for (size_t q = (size_t)bx * WG + threadIdx.x; q < nq; q += (size_t)nwg * WG) {
         const size_t c4 = 4 * q;
         const bool seg = c4 < coff;
         const T* src = seg ? t : pL;
@@ -322,29 +320,77 @@ __global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ E, siz
         T sm = T(0);
 #pragma unroll
         for (int ww = 0; ww < WAVES; ww++) sm += red[ww][kk][j];
-        part[((size_t)(k0 + kk) * r + j) * gridDim.x + blockIdx.x] = sm;
+        part[((size_t)(k0 + kk) * r + j) * nwg + bx] = sm;
     }
 }
 
-// y_S[k][j] = nr_k p_S[k][j] + the sum of the workgroup partials of entry e = k r + j: one
-// workgroup per entry, fixed order (strided thread sums, then waves in order).
-template <class T>
-__global__ __launch_bounds__(WG) void k_small_dense_final(const T* __restrict__ part, unsigned nparts, int r,
-                                                          const T* __restrict__ nrs, const T* __restrict__ p,
-                                                          T* __restrict__ y, const uint32_t* __restrict__ done) {
+template <class T, int RM, int KC>
+__global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ E, size_t ld, size_t coff, size_t g1,
+                                                    size_t mL, const T* __restrict__ t, const T* __restrict__ pL,
+                                                    int ms, int r, T* __restrict__ part,
+                                                    const uint32_t* __restrict__ done) {
     if (done && *done) return;
+    small_dense_body<T, RM, KC>(E, ld, coff, g1, mL, t, pL, ms, r, part, blockIdx.x, gridDim.x, blockIdx.y);
+}
+
+// Last step of S*p: the small rows y_S[k][j] = nr_k p_S[k][j] + the sum of the partials of entry
+// e = k r + j (when part is set; one workgroup per entry, fixed order), then, when pq is set, the
+// per-workgroup partials pq[wg * 16 + j] of p'y over all m rows (the PCG's p'q, fixed order).
+template <class T>
+__global__ __launch_bounds__(WG) void k_spmv_finish(const T* __restrict__ part, unsigned nparts, size_t ms, int r,
+                                                    const T* __restrict__ nrs, const T* __restrict__ p,
+                                                    T* __restrict__ y, size_t m, double* __restrict__ pq,
+                                                    const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    constexpr int RQ = 16;
     __shared__ T red[WAVES];
-    const size_t e = blockIdx.x;
-    T sm = T(0);
-    for (unsigned q = threadIdx.x; q < nparts; q += WG) sm += part[e * nparts + q];
-    sm = wave_sum(sm);
-    if (lane_id() == 0) red[threadIdx.x / WAVE] = sm;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        T tot = T(0);
+    __shared__ double dred[WAVES][RQ];
+    double acc[RQ];
 #pragma unroll
-        for (int w = 0; w < WAVES; w++) tot += red[w];
-        y[e] = nrs[e / r] * p[e] + tot;
+    for (int j = 0; j < RQ; j++) acc[j] = 0.0;
+    if (part) {
+        for (size_t e = blockIdx.x; e < ms * r; e += gridDim.x) {
+            T sm = T(0);
+            for (unsigned q = threadIdx.x; q < nparts; q += WG) sm += part[e * nparts + q];
+            sm = wave_sum(sm);
+            if (lane_id() == 0) red[threadIdx.x / WAVE] = sm;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                T tot = T(0);
+#pragma unroll
+                for (int w = 0; w < WAVES; w++) tot += red[w];
+                const T ye = nrs[e / r] * p[e] + tot;
+                y[e] = ye;
+                const int j = (int)(e % r);
+#pragma unroll
+                for (int jj = 0; jj < RQ; jj++)
+                    if (jj == j) acc[jj] += (double)p[e] * (double)ye;
+            }
+            __syncthreads();
+        }
+    }
+    if (!pq) return;
+    const size_t e0 = part ? ms * r : 0;
+    for (size_t i = e0 + (size_t)blockIdx.x * WG + threadIdx.x; i < m * r; i += (size_t)gridDim.x * WG) {
+        const int j = (int)(i % r);
+        const double v = (double)p[i] * (double)y[i];
+#pragma unroll
+        for (int jj = 0; jj < RQ; jj++)
+            if (jj == j) acc[jj] += v;
+    }
+    const int w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int j = 0; j < RQ; j++)
+        if (j < r) {
+            const double sm = wave_sum(acc[j]);
+            if (lane_id() == 0) dred[w][j] = sm;
+        }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)r) {
+        double sm = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ww++) sm += dred[ww][threadIdx.x];
+        pq[(size_t)blockIdx.x * RQ + threadIdx.x] = sm;
     }
 }
 
@@ -359,30 +405,72 @@ struct OtherOffs {
 // With oskip > 0 the first oskip other blocks (the single small block) are not gathered: their
 // part of the row, sum_c W_c p_S[k(c)] = (C' p_S)[l], comes in dl (this block's rows, dense,
 // k_small_dense_t).
+template <class T>
+struct LvlArgs {
+    const uint32_t* loff;
+    const uint32_t* lseg;
+    const uint32_t* lother;
+    const T* lw;
+    size_t g;
+    uint32_t gb, off_b;
+    OtherOffs oo;
+    int nother, oskip;
+    const T* dl;
+    const T* p;
+    const T* t;
+    int r, L;
+    T* y;
+    T* ypart;
+    uint32_t nbk;    // buckets (items = nbk x gb, bucket-major)
+    uint32_t X;      // 1, or XCD_COUNT: bucket q runs only on the workgroups of XCD q mod X
+    uint32_t merge;  // 1: one item per level over all its buckets' ranges, y written directly (X = 1)
+};
+
 template <class T, int RM, int NO>
-__global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ lseg,
-                                                 const uint32_t* __restrict__ lother, const T* __restrict__ lw,
-                                                 size_t g, uint32_t gb, uint32_t off_b, OtherOffs oo, int nother,
-                                                 int oskip, const T* __restrict__ dl, const T* __restrict__ p,
-                                                 const T* __restrict__ t, int r, int L, T* __restrict__ y,
-                                                 T* __restrict__ ypart, const uint32_t* __restrict__ done) {
-    if (done && *done) return;
+__device__ __forceinline__ void lvl_spmv_body(const LvlArgs<T>& A, unsigned bid, unsigned nblk) {
+    const uint32_t* __restrict__ loff = A.loff;
+    const uint32_t* __restrict__ lseg = A.lseg;
+    const uint32_t* __restrict__ lother = A.lother;
+    const T* __restrict__ lw = A.lw;
+    const T* __restrict__ dl = A.dl;
+    const T* __restrict__ p = A.p;
+    const T* __restrict__ t = A.t;
+    const size_t g = A.g;
+    const uint32_t gb = A.gb, off_b = A.off_b;
+    const int nother = A.nother, oskip = A.oskip, r = A.r, L = A.L;
+    T* __restrict__ y = A.y;
+    T* __restrict__ ypart = A.ypart;
+    const OtherOffs& oo = A.oo;
     constexpr int U = gather_trips<T, RM>();
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
-    const size_t nwt = (size_t)gridDim.x * WAVES;
-    for (size_t base = ((size_t)blockIdx.x * WAVES + w) * gpw; base < g; base += nwt * gpw) {
-        const size_t it_ = base + gl;  // item = bucket * gb + level
-        const bool valid = it_ < g;
+    // Workgroups are dispatched round-robin over the XCDs (workgroup b runs on XCD b mod X), so
+    // with X = XCD_COUNT every bucket's items, and with them the t rows of its FE1 range, stay in
+    // one XCD's L2. The XCD's items are its buckets' items in order.
+    // With merge, an item is a level and its positions are the ranges of that level in every
+    // bucket, summed in bucket order.
+    const uint32_t X = A.X, xq = bid % X, merge = A.merge;
+    const size_t nl = nblk / X, lb = bid / X;
+    const size_t nbx = A.nbk > xq ? (A.nbk - xq + X - 1) / X : 0, nxi = merge ? gb : nbx * gb;
+    (void)g;
+    const size_t nwt = nl * WAVES;
+    for (size_t base = (lb * WAVES + w) * gpw; base < nxi; base += nwt * gpw) {
+        const size_t kx = base + gl;
+        const bool valid = kx < nxi;
+        // item = bucket * gb + level
+        const size_t it_ = !valid ? 0 : merge ? kx : (xq + (size_t)X * (kx / gb)) * gb + kx % gb;
         const size_t lv = valid ? it_ % gb : 0;
-        const uint32_t j0 = valid ? loff[it_] : 0, j1 = valid ? loff[it_ + 1] : 0;
-        const uint32_t tmax = wave_max_u32((j1 - j0 + L - 1) / L);
         T pb[RM], acc[RM];
 #pragma unroll
         for (int j = 0; j < RM; j++) {
             pb[j] = (valid && j < r) ? p[(size_t)(off_b + lv) * r + j] : T(0);
             acc[j] = T(0);
         }
+        const uint32_t nq = merge ? A.nbk : 1;
+        for (uint32_t bq = 0; bq < nq; bq++) {
+        const size_t itq = merge ? (size_t)bq * gb + lv : it_;
+        const uint32_t j0 = valid ? loff[itq] : 0, j1 = valid ? loff[itq + 1] : 0;
+        const uint32_t tmax = wave_max_u32((j1 - j0 + L - 1) / L);
         for (uint32_t it = 0; it < tmax; it += U) {
             // indices of U positions per lane first, then the gathers
             uint32_t sg[U], oc[U][NO > 0 ? NO : 1];
@@ -419,6 +507,7 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
                 }
             }
         }
+        }
 #pragma unroll
         for (int j = 0; j < RM; j++) acc[j] = gsum(acc[j], L);
         if (dl && valid && it_ < gb) {  // once per level: the first bucket's item
@@ -428,11 +517,34 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(const uint32_t* __restrict__ lo
             for (int j = 0; j < RM; j++) acc[j] += dv[j];
         }
         if (valid && li == 0) {
-            T* dst = ypart ? ypart + it_ * r : y + (size_t)(off_b + lv) * r;
+            T* dst = (ypart && !merge) ? ypart + it_ * r : y + (size_t)(off_b + lv) * r;
 #pragma unroll
             for (int j = 0; j < RM; j++)
                 if (j < r) dst[j] = acc[j];
         }
+    }
+}
+
+template <class T, int RM, int NO>
+__global__ __launch_bounds__(WG) void k_lvl_spmv(LvlArgs<T> A, const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    lvl_spmv_body<T, RM, NO>(A, blockIdx.x, gridDim.x);
+}
+
+// One launch for a level pass and the dense small-row partials: both read only p and t, so the
+// bandwidth-bound dense product runs beside the latency-bound gathers. Workgroups [0, nlvl) run
+// the level pass, the rest the dense chunks (nwg workgroups per chunk of KC levels).
+template <class T, int RM, int NO, int KC>
+__global__ __launch_bounds__(WG) void k_lvl_small(LvlArgs<T> A, unsigned nlvl, const T* __restrict__ E, size_t ld,
+                                                  size_t coff, size_t g1, size_t mL, const T* __restrict__ pL, int ms,
+                                                  T* __restrict__ part, unsigned nwg,
+                                                  const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    if (blockIdx.x < nlvl) {
+        lvl_spmv_body<T, RM, NO>(A, blockIdx.x, nlvl);
+    } else {
+        const unsigned b = blockIdx.x - nlvl;
+        small_dense_body<T, RM, KC>(E, ld, coff, g1, mL, A.t, pL, ms, A.r, part, b % nwg, nwg, b / nwg);
     }
 }
 
@@ -829,12 +941,24 @@ void bucket_reduce(const T* ypart, uint32_t nbk, size_t gb, int r, T* y, const u
 inline unsigned lvl_grid(int L, size_t items) {
     return grid_for(items, (unsigned)((WAVE / L) * WAVES), 1u << 20);
 }
+// The level-pass grid and XCD spread for nbk buckets: with at least XCD_COUNT buckets each XCD
+// takes the buckets q = x mod XCD_COUNT (the grid a multiple of XCD_COUNT).
+inline unsigned lvl_grid_x(int L, size_t items, uint32_t nbk, uint32_t* X) {
+    const unsigned g = lvl_grid(L, items);
+    if (nbk < (uint32_t)XCD_COUNT) {
+        *X = 1;
+        return g;
+    }
+    *X = XCD_COUNT;
+    return (g + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
+}
 
 // The route and lane widths of one S*p: plan.dense selects the dense products for the single
 // small block also inside the segment and level passes (its codes are then never read there).
+// pq (optional, SPMV_DOTG x 16 doubles): per-workgroup partials of p'y, for the PCG.
 template <class T>
 void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r, Scratch<T>& s,
-               const uint32_t* done, hipStream_t st) {
+               const uint32_t* done, hipStream_t st, double* pq = nullptr) {
     require(r >= 1 && r <= 16, "spmv: 1..16 columns");
     OpArgs<T> A = op_args_impl<T>(a);
     const int nb = (int)a.nb;
@@ -869,25 +993,29 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
         FH_DISPATCH_R(r, hipLaunchKernelGGL((k_seg_t_any<T, RM>), dim3(grid), dim3(WG), 0, st, A, p, r, L, s.t.p, done));
     }
     FH_CHECK_LAUNCH();
-    if (a.small_dense) {
-        const unsigned nwg = grid_for(a.sm_ld / 4, WG, 2048);  // ~one column quad per thread
-        FH_DISPATCH_R(r, {
-            constexpr int KM = small_chunk<T, RM>();
-            s.part.reserve((size_t)nwg * ms * r);
-            auto launch = [&](auto kc) {
-                constexpr int KC = decltype(kc)::value;
-                const unsigned nch = (unsigned)((ms + KC - 1) / KC);
-                hipLaunchKernelGGL((k_small_dense<T, RM, KC>), dim3(nwg, nch), dim3(WG), 0, st, E, a.sm_ld, a.sm_coff,
-                                   a.g1, mL, s.t.p, p + ms * r, (int)ms, r, s.part.p, done);
-            };
-            if (ms <= 4 || KM <= 4) launch(std::integral_constant<int, (KM < 4 ? KM : 4)>{});
-            else if (ms <= 8 || KM <= 8) launch(std::integral_constant<int, (KM < 8 ? KM : 8)>{});
-            else if (ms <= 16 || KM <= 16) launch(std::integral_constant<int, (KM < 16 ? KM : 16)>{});
-            else launch(std::integral_constant<int, KM>{});
-        });
-        FH_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_small_dense_final<T>, dim3((unsigned)(ms * r)), dim3(WG), 0, st, s.part.p, nwg, r,
-                           W->nrs.p, p, y, done);
+    // dense small-row partials: one launch with the first large block's level pass (or alone)
+    const unsigned nwg = a.small_dense ? grid_for(a.sm_ld / 4, WG, 2048) : 0;  // ~one column quad per thread
+    if (a.small_dense) s.part.reserve((size_t)nwg * ms * r);
+    int first_large = -1;
+    for (int b = 0; b < nb; b++)
+        if (!(a.blocks[b].small && a.small_dense)) {
+            first_large = b;
+            break;
+        }
+    auto with_kc = [&](auto rmc, auto&& f) {
+        constexpr int KM = small_chunk<T, decltype(rmc)::value>();
+        if (ms <= 4 || KM <= 4) f(std::integral_constant<int, (KM < 4 ? KM : 4)>{});
+        else if (ms <= 8 || KM <= 8) f(std::integral_constant<int, (KM < 8 ? KM : 8)>{});
+        else if (ms <= 16 || KM <= 16) f(std::integral_constant<int, (KM < 16 ? KM : 16)>{});
+        else f(std::integral_constant<int, KM>{});
+    };
+    if (a.small_dense && first_large < 0) {
+        FH_DISPATCH_R(r, with_kc(std::integral_constant<int, RM>{}, [&](auto kc) {
+            constexpr int KC = decltype(kc)::value;
+            const unsigned nch = (unsigned)((ms + KC - 1) / KC);
+            hipLaunchKernelGGL((k_small_dense<T, RM, KC>), dim3(nwg, nch), dim3(WG), 0, st, E, a.sm_ld, a.sm_coff,
+                               a.g1, mL, s.t.p, p + ms * r, (int)ms, r, s.part.p, done);
+        }));
         FH_CHECK_LAUNCH();
     }
     int nsb = 0;
@@ -895,26 +1023,58 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
     for (int b = 0; b < nb; b++) {
         const BlockInfo& bi = a.blocks[b];
         if (bi.small && a.small_dense) continue;
-        const OtherOffs oo = other_offs(a, b);
-        const int Ll = (int)((size_t)b < plan.blanes.size() && plan.blanes[b] ? plan.blanes[b] : a.blanes[b]);
         const uint32_t nbk = a.nbk[b];
-        const size_t items = (size_t)nbk * bi.g;
-        const T* lw = a.unit() ? nullptr : W->lw[b].p;
+        const LvlMode md = lvl_mode(a, plan, b, r, (int)sizeof(T));
+        const bool merge = md.merge;
+        const int Ll = md.lanes;
+        const size_t items = merge ? (size_t)bi.g : (size_t)nbk * bi.g;
         T* ypart = nullptr;
-        if (nbk > 1) {
+        if (nbk > 1 && !merge) {
             s.ypart.reserve(items * r);
             ypart = s.ypart.p;
         }
-        const int oskip = dense ? nsb : 0;  // small blocks precede the large ones among the others
-        const T* dlb = dense ? s.dl.p + (bi.off - ms) * r : nullptr;
+        LvlArgs<T> LA;
+        LA.loff = a.boff[b].p;
+        LA.lseg = a.lseg[b].p;
+        LA.lother = a.lother[b].p;
+        LA.lw = a.unit() ? nullptr : W->lw[b].p;
+        LA.g = items;
+        LA.gb = (uint32_t)bi.g;
+        LA.off_b = (uint32_t)bi.off;
+        LA.oo = other_offs(a, b);
+        LA.nother = nb - 1;
+        LA.oskip = dense ? nsb : 0;  // small blocks precede the large ones among the others
+        LA.dl = dense ? s.dl.p + (bi.off - ms) * r : nullptr;
+        LA.p = p;
+        LA.t = s.t.p;
+        LA.r = r;
+        LA.L = Ll;
+        LA.y = y;
+        LA.ypart = ypart;
+        LA.nbk = nbk;
+        LA.merge = merge ? 1u : 0u;
+        const unsigned nlvl = lvl_grid_x(Ll, items, merge ? 1u : nbk, &LA.X);
         FH_DISPATCH_R(r, with_no(nb - 1, [&](auto noc) {
             constexpr int NO = decltype(noc)::value;
-            hipLaunchKernelGGL((k_lvl_spmv<T, RM, NO>), dim3(lvl_grid(Ll, items)), dim3(WG), 0, st, a.boff[b].p,
-                               a.lseg[b].p, a.lother[b].p, lw, items, (uint32_t)bi.g, (uint32_t)bi.off, oo, nb - 1,
-                               oskip, dlb, p, s.t.p, r, Ll, y, ypart, done);
+            if (a.small_dense && b == first_large) {
+                with_kc(std::integral_constant<int, RM>{}, [&](auto kc) {
+                    constexpr int KC = decltype(kc)::value;
+                    const unsigned nch = (unsigned)((ms + KC - 1) / KC);
+                    hipLaunchKernelGGL((k_lvl_small<T, RM, NO, KC>), dim3(nlvl + nwg * nch), dim3(WG), 0, st, LA, nlvl,
+                                       E, a.sm_ld, a.sm_coff, a.g1, mL, p + ms * r, (int)ms, s.part.p, nwg, done);
+                });
+            } else {
+                hipLaunchKernelGGL((k_lvl_spmv<T, RM, NO>), dim3(nlvl), dim3(WG), 0, st, LA, done);
+            }
         }));
         FH_CHECK_LAUNCH();
         if (ypart) bucket_reduce<T>(ypart, nbk, (size_t)bi.g, r, y + bi.off * r, done, st);
+    }
+    if (a.small_dense || pq) {
+        const unsigned fg = pq ? (unsigned)SPMV_DOTG : (unsigned)(ms * r);
+        hipLaunchKernelGGL(k_spmv_finish<T>, dim3(fg), dim3(WG), 0, st, a.small_dense ? s.part.p : (const T*)nullptr,
+                           nwg, ms, r, a.small_dense ? W->nrs.p : (const T*)nullptr, p, y, a.m, pq, done);
+        FH_CHECK_LAUNCH();
     }
 }
 
@@ -934,7 +1094,8 @@ void autotune_spmv(const Absorber& a, SpmvPlan& plan, const T* p, T* y, int r, S
     SpmvPlan best;
     best.dense = 1;
     best.seg_lanes = a.seg_lanes;
-    best.blanes = a.blanes;
+    best.blanes.assign(a.nb, 0);
+    best.xcd.assign(a.nb, -1);
     hipEvent_t e0, e1;
     FH_HIP(hipEventCreate(&e0));
     FH_HIP(hipEventCreate(&e1));
@@ -972,10 +1133,18 @@ void autotune_spmv(const Absorber& a, SpmvPlan& plan, const T* p, T* y, int r, S
     }
     for (size_t b = 0; b < a.nb; b++) {
         if (a.blocks[b].small && a.small_dense) continue;
+        if (a.nbk[b] >= (uint32_t)XCD_COUNT) {  // the other bucket mode, at its own default lanes
+            const bool merged = lvl_mode(a, best, b, r, (int)sizeof(T)).merge;
+            SpmvPlan c = best;
+            c.xcd[b] = merged ? 1 : 0;
+            c.blanes[b] = 0;
+            consider(c);
+        }
+        const uint32_t cur = (uint32_t)lvl_mode(a, best, b, r, (int)sizeof(T)).lanes;
         for (uint32_t f : {2u, 0u}) {
             SpmvPlan c = best;
-            c.blanes[b] = f ? best.blanes[b] * 2 : best.blanes[b] / 2;
-            if (c.blanes[b] >= 1 && c.blanes[b] <= WAVE && c.blanes[b] != best.blanes[b]) consider(c);
+            c.blanes[b] = f ? cur * 2 : cur / 2;
+            if (c.blanes[b] >= 1 && c.blanes[b] <= WAVE && c.blanes[b] != cur) consider(c);
         }
     }
     (void)hipEventDestroy(e0);
@@ -986,9 +1155,10 @@ void autotune_spmv(const Absorber& a, SpmvPlan& plan, const T* p, T* y, int r, S
 }
 
 template <class T>
-void spmv_planned(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st) {
+void spmv_planned(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st,
+                  double* pq) {
     SpmvPlan& plan = a.plan_for(sizeof(T) == 8, r);
     if (a.autotune && !plan.tuned) autotune_spmv<T>(a, plan, p, y, r, s, st);
-    spmv_impl<T>(a, plan, p, y, r, s, done, st);
+    spmv_impl<T>(a, plan, p, y, r, s, done, st, pq);
 }
 }  // namespace fh

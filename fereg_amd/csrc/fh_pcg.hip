@@ -173,41 +173,50 @@ __global__ void k_mask_copy(const T* __restrict__ z, T* __restrict__ p, size_t n
         p[i] = S->active[i % r] ? z[i] : T(0);
 }
 
-// gamma = rho / p'q (cg.rs:224-234).
-__global__ __launch_bounds__(TPB) void k_gamma(PcgState* S, const double* __restrict__ dpart) {
-    if (S->done) return;
-    __shared__ double sums[NSLOT_MAX][RMAX];
-    const int sl[1] = {S_PQ};
-    slot_sums(dpart, sl, 1, S->r, sums);
-    const int j = threadIdx.x;
-    if (j < S->r) {
-        double g = 0.0;
-        if (S->active[j]) {
-            const double pq = sums[0][j];
-            if (pq > 0.0 && isfinite(pq)) g = S->rho[j] / pq;
-            else S->active[j] = 0;
-        }
-        S->gamma[j] = g;
-    }
-}
+// gamma = rho / p'q (cg.rs:224-234) from the p'q partials, in every workgroup (same fixed order,
+// so every workgroup holds the same values; workgroup 0 records them), then x += gamma p,
+// r -= gamma q, partial x'rhs and x'r (cg.rs:349-454), Jacobi z on the large rows and its
+// partial r'z. With SMALL (ms r <= TPB, ms <= SMALL_FUSE_MAX) workgroup 0 owns every small-row
+// element and also applies the small block's explicit inverse, z_S = M_S r_S
+// (precond/dense.rs:117-160), writing the partial r_S'z_S; the other workgroups write zeros there.
+constexpr int SMALL_FUSE_MAX = 32;
 
-// x += gamma p, r -= gamma q, partial x'rhs and x'r (cg.rs:349-454); Jacobi z on the large rows
-// and its partial r'z.
-template <class T>
+template <class T, bool SMALL>
 __global__ __launch_bounds__(TPB) void k_update(T* __restrict__ x, T* __restrict__ rr, const T* __restrict__ p,
                                                 const T* __restrict__ q, const T* __restrict__ rhs,
-                                                const T* __restrict__ inv_diag, T* __restrict__ z, size_t ms,
-                                                size_t m, int r, const PcgState* __restrict__ S,
+                                                const T* __restrict__ inv_diag, const T* __restrict__ minv,
+                                                T* __restrict__ z, size_t ms, size_t m, int r, PcgState* __restrict__ S,
                                                 double* __restrict__ dpart) {
     if (S->done) return;
-    __shared__ double red[TPB / WAVE][3][RMAX];
-    double xb[RMAX], xr[RMAX], rz[RMAX];
+    __shared__ double sums[NSLOT_MAX][RMAX];
+    __shared__ double red[TPB / WAVE][4][RMAX];
+    __shared__ T rs[SMALL ? TPB : 1];
+    {
+        const int sl[1] = {S_PQ};
+        slot_sums(dpart, sl, 1, r, sums);
+    }
     double g[RMAX];
 #pragma unroll
     for (int j = 0; j < RMAX; j++) {
-        xb[j] = 0.0; xr[j] = 0.0; rz[j] = 0.0;
-        g[j] = j < r ? S->gamma[j] : 0.0;
+        g[j] = 0.0;
+        if (j < r && S->active[j]) {
+            const double pq = sums[0][j];
+            if (pq > 0.0 && isfinite(pq)) g[j] = S->rho[j] / pq;
+        }
     }
+    if (blockIdx.x == 0 && threadIdx.x < (unsigned)r) {
+        const int j = threadIdx.x;
+        const double pq = sums[0][j];
+        double gj = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < RMAX; jj++)
+            if (jj == j) gj = g[jj];
+        if (S->active[j] && !(pq > 0.0 && isfinite(pq))) S->active[j] = 0;
+        S->gamma[j] = gj;
+    }
+    double xb[RMAX], xr[RMAX], rz[RMAX], rzs[RMAX];
+#pragma unroll
+    for (int j = 0; j < RMAX; j++) { xb[j] = 0.0; xr[j] = 0.0; rz[j] = 0.0; rzs[j] = 0.0; }
     const size_t n = m * r;
     for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
         const int j = (int)(i % r);
@@ -226,27 +235,45 @@ __global__ __launch_bounds__(TPB) void k_update(T* __restrict__ x, T* __restrict
             const T zv = inv_diag[k] * rv;
             z[i] = zv;
             c = (double)rv * (double)zv;
+        } else if (SMALL) {
+            rs[i] = rv;  // workgroup 0, first step: i < ms r <= TPB
         }
 #pragma unroll
         for (int jj = 0; jj < RMAX; jj++)
             if (jj == j) { xb[jj] += a; xr[jj] += b; rz[jj] += c; }
     }
+    if (SMALL && blockIdx.x == 0) {
+        __syncthreads();
+        const size_t e = threadIdx.x;
+        if (e < ms * r) {
+            const size_t k = e / r;
+            const int j = (int)(e % r);
+            T sm = T(0);
+            for (size_t l = 0; l < ms; l++) sm += minv[k * ms + l] * rs[l * r + j];
+            z[e] = sm;
+            const double c = (double)rs[e] * (double)sm;
+#pragma unroll
+            for (int jj = 0; jj < RMAX; jj++)
+                if (jj == j) rzs[jj] += c;
+        }
+    }
     const int w = threadIdx.x / WAVE;
 #pragma unroll
     for (int j = 0; j < RMAX; j++) {
         if (j < r) {
-            double a = wave_sum(xb[j]), b = wave_sum(xr[j]), c = wave_sum(rz[j]);
-            if (lane_id() == 0) { red[w][0][j] = a; red[w][1][j] = b; red[w][2][j] = c; }
+            double a = wave_sum(xb[j]), b = wave_sum(xr[j]), c = wave_sum(rz[j]), d = SMALL ? wave_sum(rzs[j]) : 0.0;
+            if (lane_id() == 0) { red[w][0][j] = a; red[w][1][j] = b; red[w][2][j] = c; red[w][3][j] = d; }
         }
     }
     __syncthreads();
     if (threadIdx.x < (unsigned)r) {
         const int j = threadIdx.x;
-        double a = 0.0, b = 0.0, c = 0.0;
-        for (int ww = 0; ww < TPB / WAVE; ww++) { a += red[ww][0][j]; b += red[ww][1][j]; c += red[ww][2][j]; }
+        double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+        for (int ww = 0; ww < TPB / WAVE; ww++) { a += red[ww][0][j]; b += red[ww][1][j]; c += red[ww][2][j]; d += red[ww][3][j]; }
         dpart[((size_t)S_XB * DOTG + blockIdx.x) * RMAX + j] = a;
         dpart[((size_t)S_XR * DOTG + blockIdx.x) * RMAX + j] = b;
         dpart[((size_t)S_RZL * DOTG + blockIdx.x) * RMAX + j] = c;
+        if (SMALL) dpart[((size_t)S_RZS * DOTG + blockIdx.x) * RMAX + j] = d;
     }
 }
 
@@ -316,11 +343,65 @@ Scratch<T>& scr(Absorber& a) {
     if constexpr (std::is_same<T, double>::value) return a.s64; else return a.s32;
 }
 
+// z = M^-1 r in one launch: Jacobi on the large rows with the partial r'z (slot S_RZL), and in
+// workgroup 0 the small block's explicit inverse with the partial r_S'z_S (slot S_RZS; zeros from
+// the other workgroups). Used when ms r <= TPB and ms <= SMALL_FUSE_MAX.
+template <class T>
+__global__ __launch_bounds__(TPB) void k_precond(const T* __restrict__ inv_diag, const T* __restrict__ minv,
+                                                 const T* __restrict__ rr, T* __restrict__ z, size_t ms, size_t m,
+                                                 int r, double* __restrict__ dpart) {
+    __shared__ double red[TPB / WAVE][2][RMAX];
+    double rzl[RMAX], rzs[RMAX];
+#pragma unroll
+    for (int j = 0; j < RMAX; j++) { rzl[j] = 0.0; rzs[j] = 0.0; }
+    const size_t n = m * r;
+    for (size_t i = ms * r + (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
+        const int j = (int)(i % r);
+        const T zv = inv_diag[i / r] * rr[i];
+        z[i] = zv;
+        const double c = (double)rr[i] * (double)zv;
+#pragma unroll
+        for (int jj = 0; jj < RMAX; jj++)
+            if (jj == j) rzl[jj] += c;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < ms * r) {
+        const size_t e = threadIdx.x, k = e / r;
+        const int j = (int)(e % r);
+        T sm = T(0);
+        for (size_t l = 0; l < ms; l++) sm += minv[k * ms + l] * rr[l * r + j];
+        z[e] = sm;
+        const double c = (double)rr[e] * (double)sm;
+#pragma unroll
+        for (int jj = 0; jj < RMAX; jj++)
+            if (jj == j) rzs[jj] += c;
+    }
+    const int w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int j = 0; j < RMAX; j++)
+        if (j < r) {
+            const double a = wave_sum(rzl[j]), b = wave_sum(rzs[j]);
+            if (lane_id() == 0) { red[w][0][j] = a; red[w][1][j] = b; }
+        }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)r) {
+        const int j = threadIdx.x;
+        double a = 0.0, b = 0.0;
+        for (int ww = 0; ww < TPB / WAVE; ww++) { a += red[ww][0][j]; b += red[ww][1][j]; }
+        dpart[((size_t)S_RZL * DOTG + blockIdx.x) * RMAX + j] = a;
+        dpart[((size_t)S_RZS * DOTG + blockIdx.x) * RMAX + j] = b;
+    }
+}
+
 // z = M^-1 r and the partial r'z of both parts.
 template <class T>
 void precond(Absorber& a, const T* rr, T* z, int r, double* dpart, const uint32_t* done, hipStream_t st) {
     WeightArrays<T>& W = warr<T>(a);
     const size_t m = a.m, ms = a.ms;
+    if (!done && ms <= (size_t)SMALL_FUSE_MAX && ms * (size_t)r <= (size_t)TPB) {
+        hipLaunchKernelGGL(k_precond<T>, dim3(DOTG), dim3(TPB), 0, st, W.inv_diag.p, W.minv.p, rr, z, ms, m, r, dpart);
+        FH_CHECK_LAUNCH();
+        return;
+    }
     if (m > ms) {
         hipLaunchKernelGGL(k_jacobi<T>, dim3(vgrid((m - ms) * r)), dim3(TPB), 0, st, W.inv_diag.p, rr, z, ms, m, r, done);
         hipLaunchKernelGGL(k_dot<T>, dim3(DOTG), dim3(TPB), 0, st, rr, z, ms, m, r, dpart, (int)S_RZL, done);
@@ -393,28 +474,32 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     hipLaunchKernelGGL(k_mask_copy<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, s.z.p, s.p.p, nv, r, S);
     FH_CHECK_LAUNCH();
     WeightArrays<T>& W = warr<T>(a);
+    static_assert(SPMV_DOTG == DOTG && RMAX == 16, "the S*p p'q partials use the PCG slot layout");
+    const bool small_fused = ms > 0 && ms <= (size_t)SMALL_FUSE_MAX && ms * (size_t)r <= (size_t)TPB;
     auto enqueue = [&](int batch) {
         for (int b = 0; b < batch; b++) {
-            spmv<T>(a, s.p.p, s.q.p, r, s, done, st);
-            hipLaunchKernelGGL(k_dot<T>, dim3(DOTG), dim3(TPB), 0, st, s.p.p, s.q.p, (size_t)0, m, r, s.dpart.p,
-                               (int)S_PQ, done);
-            hipLaunchKernelGGL(k_gamma, dim3(1), dim3(TPB), 0, st, S, s.dpart.p);
-            hipLaunchKernelGGL(k_update<T>, dim3(DOTG), dim3(TPB), 0, st, x, s.r.p, s.p.p, s.q.p, rhs, W.inv_diag.p,
-                               s.z.p, ms, m, r, S, s.dpart.p);
-            if (ms > 0) {
-                hipLaunchKernelGGL(k_small_apply<T>, dim3(vgrid(ms * r)), dim3(TPB), 0, st, W.minv.p, s.r.p, s.z.p, ms,
-                                   r, done);
-                hipLaunchKernelGGL(k_dot<T>, dim3(DOTG), dim3(TPB), 0, st, s.r.p, s.z.p, (size_t)0, ms, r, s.dpart.p,
-                                   (int)S_RZS, done);
+            spmv<T>(a, s.p.p, s.q.p, r, s, done, st, s.dpart.p + (size_t)S_PQ * DOTG * RMAX);
+            if (small_fused) {
+                hipLaunchKernelGGL((k_update<T, true>), dim3(DOTG), dim3(TPB), 0, st, x, s.r.p, s.p.p, s.q.p, rhs,
+                                   W.inv_diag.p, W.minv.p, s.z.p, ms, m, r, S, s.dpart.p);
+            } else {
+                hipLaunchKernelGGL((k_update<T, false>), dim3(DOTG), dim3(TPB), 0, st, x, s.r.p, s.p.p, s.q.p, rhs,
+                                   W.inv_diag.p, W.minv.p, s.z.p, ms, m, r, S, s.dpart.p);
+                if (ms > 0) {
+                    hipLaunchKernelGGL(k_small_apply<T>, dim3(vgrid(ms * r)), dim3(TPB), 0, st, W.minv.p, s.r.p, s.z.p,
+                                       ms, r, done);
+                    hipLaunchKernelGGL(k_dot<T>, dim3(DOTG), dim3(TPB), 0, st, s.r.p, s.z.p, (size_t)0, ms, r,
+                                       s.dpart.p, (int)S_RZS, done);
+                }
             }
             hipLaunchKernelGGL(k_stop, dim3(1), dim3(TPB), 0, st, S, s.dpart.p);
             hipLaunchKernelGGL(k_update_p<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, s.p.p, s.z.p, nv, r, S);
             FH_CHECK_LAUNCH();
         }
     };
-    // Pipelined polling: the status of batch k is read while batch k+1 (two more iterations) is
-    // already queued, so the device never waits for the host's decision. Once the stopping rule
-    // has fired every kernel of a batch returns at once, so the batch queued past the end costs
+    // Pipelined polling: the status after iteration k is read while iteration k+1 is already
+    // queued, so the device never waits for the host's decision. Once the stopping rule has fired
+    // every kernel of an iteration returns at once, so the one iteration queued past the end costs
     // only its launches.
     struct Poll {
         uint32_t done, err;
@@ -426,11 +511,11 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     AsyncRead rd0(sizeof(Poll)), rd1(sizeof(Poll));
     AsyncRead* rd[2] = {&rd0, &rd1};
     const void* poll_src = &S->done;
-    enqueue(WINDOW);
+    enqueue(1);
     rd[0]->start(poll_src, st);
     Poll pl{};
     for (int k = 0;; k++) {
-        enqueue(2);
+        enqueue(1);
         rd[(k + 1) & 1]->start(poll_src, st);
         pl = *static_cast<const Poll*>(rd[k & 1]->wait());
         if (pl.done) break;

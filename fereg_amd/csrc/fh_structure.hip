@@ -190,9 +190,9 @@ struct PayloadLayout {
 
 __device__ __forceinline__ uint64_t bucket_key(const uint32_t* __restrict__ cell_codes,
                                                const uint32_t* __restrict__ cell_seg, size_t c, int nb, int b,
-                                               int shift, uint64_t g, const PayloadLayout& pl) {
+                                               uint32_t bw, uint64_t g, const PayloadLayout& pl) {
     const uint32_t sg = cell_seg[c];
-    uint64_t k = (uint64_t)(sg >> shift) * g + cell_codes[c * nb + b];
+    uint64_t k = (uint64_t)(sg / bw) * g + cell_codes[c * nb + b];
     if (pl.pay) {
         int at = pl.ib;
         k |= (uint64_t)sg << at;
@@ -208,16 +208,16 @@ __device__ __forceinline__ uint64_t bucket_key(const uint32_t* __restrict__ cell
 }
 
 __global__ void k_bucket_keys(const uint32_t* __restrict__ cell_codes, const uint32_t* __restrict__ cell_seg, size_t M,
-                              int nb, int b, int shift, uint64_t g, PayloadLayout pl, uint64_t* __restrict__ keys,
+                              int nb, int b, uint32_t bw, uint64_t g, PayloadLayout pl, uint64_t* __restrict__ keys,
                               uint32_t* __restrict__ unsorted) {
     const uint64_t mask = pl.ib >= 64 ? ~0ull : ((1ull << pl.ib) - 1);
     bool bad = false;
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t k = bucket_key(cell_codes, cell_seg, c, nb, b, shift, g, pl);
+        const uint64_t k = bucket_key(cell_codes, cell_seg, c, nb, b, bw, g, pl);
         keys[c] = k;
         if (c > 0) {
             const uint32_t sp = cell_seg[c - 1];
-            const uint64_t kp = (uint64_t)(sp >> shift) * g + cell_codes[(c - 1) * nb + b];
+            const uint64_t kp = (uint64_t)(sp / bw) * g + cell_codes[(c - 1) * nb + b];
             bad |= (k & mask) < kp;
         }
     }
@@ -512,27 +512,36 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
     a.lperm.resize(nb); a.loff.resize(nb); a.lseg.resize(nb); a.lother.resize(nb); a.bperm.resize(nb);
     a.boff.resize(nb);
     a.nbk.assign(nb, 1);
-    a.bshift.assign(nb, 31);
+    a.bwidth.assign(nb, 0);
     a.blanes.assign(nb, 1);
+    a.llanes.assign(nb, 1);
     for (int b = 0; b < nb; b++) {
         const uint64_t g = a.blocks[b].g;
-        // long levels (a few levels over many cells) are cut into FE1 buckets of about
-        // ITEM_CELLS cells per (bucket, level) item so that every item fills a wave
+        // FE1 buckets of bw consecutive segments. Configured: 2^bucket_log2. Automatic: one
+        // bucket per XCD once the FE1 range is large (the level passes then keep each XCD's
+        // gathers of segment sums within an eighth of them, see the level pass). Long levels (a
+        // few levels over many cells) are cut further, to about ITEM_CELLS cells per (bucket,
+        // level) item, so that every item fills a wave.
         constexpr double ITEM_CELLS = 2048.0;
-        int shift = a.bucket_shift;
+        const uint64_t g1 = std::max<uint64_t>(a.g1, 1);
+        uint64_t bw = g1;
+        if (a.bucket_log2 > 0) bw = std::min<uint64_t>(g1, 1ull << a.bucket_log2);
+        else if (a.bucket_log2 < 0 && a.g1 >= AUTO_BUCKET_G1) bw = (g1 + XCD_COUNT - 1) / XCD_COUNT;
         const double per_level = double(M) / double(std::max<uint64_t>(g, 1));
         if (per_level > ITEM_CELLS && a.g1 > 1) {
             const double want = std::ceil(per_level / ITEM_CELLS);  // buckets
-            shift = 0;
-            while (shift < 31 && double((a.g1 + (1ull << (shift + 1)) - 1) >> (shift + 1)) >= want) shift++;
-            shift = std::min(shift, a.bucket_shift);
+            bw = std::min<uint64_t>(bw, std::max<uint64_t>(1, (uint64_t)std::ceil(double(g1) / want)));
         }
-        const uint32_t nbk = (uint32_t)((a.g1 + (1ull << shift) - 1) >> shift);
-        a.bshift[b] = shift;
+        const uint32_t nbk = (uint32_t)((g1 + bw - 1) / bw);
+        a.bwidth[b] = (uint32_t)bw;
         a.nbk[b] = std::max<uint32_t>(nbk, 1);
         const uint64_t items = (uint64_t)a.nbk[b] * g;
         require(items < (1ull << 32), "too many (bucket, level) items");
-        a.blanes[b] = lanes(items ? double(M) / double(items) / 8.0 : 1.0);  // ~8 positions per lane
+        // lanes per item: ~8 positions per lane, ~4 for the short items of a bucketed block; one
+        // item per level over its buckets: ~2 per lane of each bucket's range
+        const double ppi = items ? double(M) / double(items) : 1.0;
+        a.blanes[b] = lanes(a.nbk[b] > 1 ? ppi / 4.0 : ppi / 8.0);
+        a.llanes[b] = lanes(ppi / 2.0);
         a.bperm[b].alloc(std::max<size_t>(M, 1));
         a.boff[b].alloc(items + 1);
         FH_HIP(hipMemsetAsync(a.boff[b].p, 0, a.boff[b].bytes(), st));
@@ -550,7 +559,7 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
             pl.pay = tot <= 64;
             FH_HIP(hipMemsetAsync(flag.p, 0, sizeof(uint32_t), st));
             hipLaunchKernelGGL(k_bucket_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, a.cell_codes.p, a.cell_seg.p, M, nb,
-                               b, shift, g, pl, keys.p, flag.p);
+                               b, (uint32_t)bw, g, pl, keys.p, flag.p);
             hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, M);
             FH_CHECK_LAUNCH();
             if (read_scalar(flag.p, st) != 0)

@@ -253,6 +253,29 @@ __global__ void k_level_small_sums(const uint32_t* __restrict__ boff, const uint
     }
 }
 
+// Spreads the packed small-block results [n_r | diag | redundant | inverse] into their arrays
+// (FP64 and FP32 copies; the small coordinates get no Jacobi scale).
+__global__ void k_small_spread(const double* __restrict__ pk, size_t ms, int with_inv, double* __restrict__ nr,
+                               double* __restrict__ nrs64, float* __restrict__ nrs32, double* __restrict__ diag,
+                               uint8_t* __restrict__ red, double* __restrict__ inv_diag64, float* __restrict__ inv_diag32,
+                               double* __restrict__ minv64, float* __restrict__ minv32) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ms + (with_inv ? ms * ms : 0);
+         i += (size_t)gridDim.x * blockDim.x) {
+        if (i < ms) {
+            nr[i] = pk[i];
+            if (nrs64) { nrs64[i] = pk[i]; nrs32[i] = (float)pk[i]; }
+            diag[i] = pk[ms + i];
+            red[i] = pk[2 * ms + i] != 0.0 ? 1 : 0;
+            inv_diag64[i] = 0.0;
+            inv_diag32[i] = 0.0f;
+        } else {
+            const double v = pk[3 * ms + (i - ms)];
+            minv64[i - ms] = v;
+            minv32[i - ms] = (float)v;
+        }
+    }
+}
+
 double sum_dev(const double* v, size_t n, hipStream_t st) {
     const unsigned nb = 256;
     DBuf<double> part(nb);
@@ -562,27 +585,30 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             }
             inv = grounded_inverse(S, ms, rd, a.nkeep);
         }
-        std::vector<float> inv32(inv.begin(), inv.end());
-        std::vector<double> zeros(ms, 0.0);
-        std::vector<float> zeros32(ms, 0.0f);
-        upload(a.nr.p, nrs.data(), ms * sizeof(double), st);
+        // the host results in one transfer ([n_r | diag | redundant | inverse]), spread on the device
+        const bool with_inv = !dev_inverse;
+        std::vector<double> pk(3 * ms + (with_inv ? ms * ms : 0));
+        for (size_t k = 0; k < ms; k++) {
+            pk[k] = nrs[k];
+            pk[ms + k] = dg[k];
+            pk[2 * ms + k] = rd[k] ? 1.0 : 0.0;
+        }
+        if (with_inv) std::copy(inv.begin(), inv.end(), pk.begin() + 3 * ms);
+        DBuf<double> dpk(pk.size());
+        upload(dpk.p, pk.data(), pk.size() * sizeof(double), st);
         if (a.small_dense) {
-            std::vector<float> nrs32(nrs.begin(), nrs.end());
             a.w64.nrs.reserve(ms);
             a.w32.nrs.reserve(ms);
-            upload(a.w64.nrs.p, nrs.data(), ms * sizeof(double), st);
-            upload(a.w32.nrs.p, nrs32.data(), ms * sizeof(float), st);
         }
-        upload(a.diag.p, dg.data(), ms * sizeof(double), st);
-        upload(a.red.p, rd.data(), ms, st);
-        upload(a.w64.inv_diag.p, zeros.data(), ms * sizeof(double), st);
-        upload(a.w32.inv_diag.p, zeros32.data(), ms * sizeof(float), st);
-        if (!dev_inverse) {
+        if (with_inv) {
             a.w64.minv.reserve(ms * ms);
             a.w32.minv.reserve(ms * ms);
-            upload(a.w64.minv.p, inv.data(), ms * ms * sizeof(double), st);
-            upload(a.w32.minv.p, inv32.data(), ms * ms * sizeof(float), st);
         }
+        hipLaunchKernelGGL(k_small_spread, dim3(grid_for(ms * ms + ms, TPB, 1024)), dim3(TPB), 0, st, dpk.p, ms,
+                           with_inv ? 1 : 0, a.nr.p, a.small_dense ? a.w64.nrs.p : nullptr,
+                           a.small_dense ? a.w32.nrs.p : nullptr, a.diag.p, a.red.p, a.w64.inv_diag.p,
+                           a.w32.inv_diag.p, with_inv ? a.w64.minv.p : nullptr, with_inv ? a.w32.minv.p : nullptr);
+        FH_CHECK_LAUNCH();
     }
 }
 

@@ -51,7 +51,8 @@ typedef struct {
 typedef struct {
     uint64_t small_limit; /* levels factored jointly and exactly (default 4096) */
     int32_t precision;    /* FH_PLAIN_FP64 or FH_MIXED */
-    int32_t bucket_log2;  /* level passes in buckets of 2^k FE1 levels; 0 = one bucket */
+    int32_t bucket_log2;  /* level records in buckets of 2^k FE1 levels; 0 = one bucket;
+                             -1 (default) = one bucket per XCD from 2^19 FE1 levels on */
     int32_t autotune;     /* 1: the first S*p of each precision and column count times the
                              candidate launch plans and keeps the fastest, cached per structure
                              shape for the process (results may then differ in the last bits
@@ -149,13 +150,18 @@ int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t
 /* Launch plan of S*p for one precision and column count (no counterpart in the reference, whose
  * CPU operator has no launch shapes): the single small block as dense products inside the
  * segment and level passes (dense = 1) or gathered there (0), lanes per segment and lanes per
- * level of each block (powers of two 1..64, 0 = the default). get returns the plan in use (the
- * tuned one after an autotuned S*p); set pins one (e.g. a plan tuned in another process, for
- * bit-identical results) and marks it tuned. */
+ * item of each block's level pass (powers of two 1..64, 0 = the default), and for a block whose
+ * level records are cut into FE1 buckets the item shape (block_mode 1: one item per (bucket,
+ * level), buckets spread over the XCDs; 0: one item per level over its buckets; -1: the default
+ * rule, XCD buckets while a bucket's share of the segment sums fits in half an XCD's L2). get
+ * returns the plan in use (the tuned one after an autotuned S*p; block_mode -1 for unbucketed
+ * blocks); set pins one (e.g. a plan tuned in another process, for bit-identical results) and
+ * marks it tuned. */
 typedef struct {
     int32_t dense;
     uint32_t seg_lanes;
     uint32_t block_lanes[9];
+    int32_t block_mode[9];
     int32_t tuned;
 } fh_spmv_plan;
 int fh_absorber_get_plan(const fh_absorber* ab, int32_t precision, size_t r, fh_spmv_plan* plan);

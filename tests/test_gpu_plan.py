@@ -18,31 +18,37 @@ def _design(rng, n=40_000, weighted=False):
     return codes, gs, w
 
 
+@pytest.mark.parametrize("bucket_log2", [0, 8])
 @pytest.mark.parametrize("weighted", [False, True])
-def test_every_plan_matches_oracle(weighted):
+def test_every_plan_matches_oracle(weighted, bucket_log2):
+    # bucket_log2 = 8 cuts the 3000 FE1 levels into 12 buckets: both item shapes of the large
+    # block's level pass (XCD buckets, one item per level) are exercised
     import fereg_amd as F
     rng = np.random.default_rng(11)
     codes, gs, w = _design(rng, weighted=weighted)
     o = OracleSolver(codes, gs, weights=w)
-    a = F.Absorber(codes, gs, weights=w)
+    a = F.Absorber(codes, gs, weights=w, bucket_log2=bucket_log2)
     base = a.plan(F.FH_PLAIN_FP64, 2)
     assert base["dense"] == 1 and not base["tuned"]
     nb = sum(1 for v in base["block_lanes"] if v)
+    modes = (-1, 0, 1) if bucket_log2 else (-1,)
     for r in (1, 2, 3, 5):
         p = rng.standard_normal(o.info.m * r)
         yo = o.apply(p, r)
         scale = np.abs(yo).max()
         for dense in (1, 0):
             for f in (1, 2, 0.5):
-                plan = {"dense": dense, "seg_lanes": max(1, int(base["seg_lanes"] * f)),
-                        "block_lanes": [max(1, min(64, int(v * f))) if v else 0 for v in base["block_lanes"][:nb]]}
-                for prec, tol in ((F.FH_PLAIN_FP64, 1e-12), (F.FH_MIXED, 3e-5)):
-                    announce("plan", weighted, r, dense, f, prec)
-                    a.set_plan(prec, r, plan)
-                    got = a.plan(prec, r)
-                    assert got["tuned"] and got["dense"] == dense and got["seg_lanes"] == plan["seg_lanes"]
-                    y = a.apply(p, r, precision=prec)
-                    np.testing.assert_allclose(y, yo, rtol=0, atol=tol * scale)
+                for mode in modes:
+                    plan = {"dense": dense, "seg_lanes": max(1, int(base["seg_lanes"] * f)),
+                            "block_lanes": [max(1, min(64, int(v * f))) if v else 0 for v in base["block_lanes"][:nb]],
+                            "block_mode": [mode] * nb}
+                    for prec, tol in ((F.FH_PLAIN_FP64, 1e-12), (F.FH_MIXED, 3e-5)):
+                        announce("plan", weighted, bucket_log2, r, dense, f, mode, prec)
+                        a.set_plan(prec, r, plan)
+                        got = a.plan(prec, r)
+                        assert got["tuned"] and got["dense"] == dense and got["seg_lanes"] == plan["seg_lanes"]
+                        y = a.apply(p, r, precision=prec)
+                        np.testing.assert_allclose(y, yo, rtol=0, atol=tol * scale)
 
 
 def test_autotune_caches_and_solves():
@@ -90,3 +96,5 @@ def test_invalid_plans_are_rejected():
         a.set_plan(F.FH_PLAIN_FP64, 2, {"dense": 2})
     with pytest.raises(Exception):
         a.set_plan(F.FH_PLAIN_FP64, 2, {"dense": 1, "block_lanes": [128]})
+    with pytest.raises(Exception):
+        a.set_plan(F.FH_PLAIN_FP64, 2, {"dense": 1, "block_mode": [2]})

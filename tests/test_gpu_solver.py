@@ -149,20 +149,22 @@ def test_device_resident_block():
     np.testing.assert_allclose(z.cpu().numpy(), dense_residualize(v, codes, gs)[pi], atol=1e-9)
 
 
+@pytest.mark.parametrize("g1,bucket_log2", [(300_000, 17), (600_000, -1)])
 @pytest.mark.parametrize("weighted", [False, True])
-def test_bucketed_level_lists_match_oracle(weighted):
-    """FE1 with more than 2^17 levels: the level passes run over several segment buckets."""
+def test_bucketed_level_lists_match_oracle(weighted, g1, bucket_log2):
+    """FE1 with more than 2^17 levels: the level passes run over several segment buckets (2^17
+    configured: 3 buckets; automatic from 2^19 levels: one per XCD, 75000 levels each)."""
     import fereg_amd as F
     rng = np.random.default_rng(77)
-    n = 600_000
+    n = 2 * g1
     i = np.arange(n)
     codes = [(i // 2).astype(np.uint32), rng.integers(0, 5000, n).astype(np.uint32), (i % 2).astype(np.uint32),
              rng.integers(0, 3, n).astype(np.uint32)]
-    gs = [300_000, 5000, 2, 3]
+    gs = [g1, 5000, 2, 3]
     w = rng.uniform(0.5, 2.0, n) if weighted else None
     o = OracleSolver(codes, gs, weights=w)
-    a = F.Absorber(codes, gs, weights=w, bucket_log2=17)
-    assert a.info.g1 == 300_000 and a.info.m_small == 5
+    a = F.Absorber(codes, gs, weights=w, bucket_log2=bucket_log2)
+    assert a.info.g1 == g1 and a.info.m_small == 5
     for r in (1, 3):
         p = rng.standard_normal(o.info.m * r)
         announce("bucketed apply", weighted, r)
