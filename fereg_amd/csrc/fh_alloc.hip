@@ -1,20 +1,26 @@
-// Stream-ordered caching pool for the library's device buffers.
+// Stream-ordered caching pool for the library's device buffers, and a pool of streams.
 //
 // hipMalloc is slow and hipFree waits for the whole device, so a fit that allocates and frees
 // tens of scratch arrays would spend most of its time in the allocator. Freed blocks are kept
-// per device in a size-ordered free list together with the stream they were last used on and
-// an event recorded on that stream at the free:
+// per device in a size-ordered free list together with the stream they were last used on:
 //   - a request on the same stream reuses a block at once (stream order makes it safe);
-//   - a request on another stream reuses a block only once its event has completed.
+//   - a request on another stream reuses a block once the work queued on the block's stream up
+//     to the first such request has completed (an event recorded then, queried without waiting);
+//   - blocks of a stream that was drained (returned to the stream pool) are idle: reusable at once.
 // Blocks are rounded to 256 B (below 1 MiB: to the next power of two) or 2 MiB multiples and a
 // block is reused for requests of at least half its size. Memory goes back to HIP on
 // fh_release_cached_memory() or when hipMalloc fails (completed blocks are released and the
 // allocation is tried once more).
+//
+// Streams: creating and destroying a HIP stream costs ~100 us each, so the streams of destroyed
+// absorbers are drained and kept for the next ones (never destroyed: the device pool keys its
+// blocks by stream).
 #include <chrono>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "fh_common.hpp"
 
@@ -28,7 +34,8 @@ struct Block {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
-    bool idle = true;  // no work pending on it (its stream was synchronized and destroyed)
+    bool ev_pending = false;  // ev recorded on `stream` after the free
+    bool idle = true;         // no work pending on it (its stream was drained)
 };
 
 // The pool's own event queries must not leave an error for the caller's next
@@ -66,6 +73,18 @@ size_t round_bytes(size_t b) {
     return (b + q - 1) / q * q;
 }
 
+// Whether the work that used a free block has completed (caller holds the lock; the block's
+// device is current or the event API does not need it).
+bool block_done(Block& b) {
+    if (b.idle) return true;
+    if (!b.ev_pending) {
+        if (!b.ev && hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) return false;
+        if (hipEventRecord(b.ev, b.stream) != hipSuccess) return false;
+        b.ev_pending = true;
+    }
+    return hipEventQuery(b.ev) == hipSuccess;
+}
+
 // Releases every cached block whose last use has completed (caller holds the lock).
 void release_completed(Pool& P) {
     for (auto& [dev, fl] : P.free_by_dev) {
@@ -74,7 +93,7 @@ void release_completed(Pool& P) {
         if (prev != dev) (void)hipSetDevice(dev);
         for (auto it = fl.begin(); it != fl.end();) {
             Block& b = it->second;
-            if (!b.idle && b.ev && hipEventQuery(b.ev) != hipSuccess) { ++it; continue; }
+            if (!block_done(b)) { ++it; continue; }
             if (b.ev) (void)hipEventDestroy(b.ev);
             (void)hipFree(b.p);
             P.cached -= b.bytes;
@@ -82,6 +101,16 @@ void release_completed(Pool& P) {
         }
         if (prev != dev && prev >= 0) (void)hipSetDevice(prev);
     }
+}
+
+struct StreamPool {
+    std::mutex mu;
+    std::map<int, std::vector<hipStream_t>> free_by_dev;
+};
+
+StreamPool& stream_pool() {
+    static StreamPool* p = new StreamPool();
+    return *p;
 }
 
 }  // namespace
@@ -103,11 +132,12 @@ void* pool_alloc(size_t bytes) {
     auto& fl = P.free_by_dev[dev];
     for (auto it = fl.lower_bound(rb); it != fl.end() && it->first <= 2 * rb; ++it) {
         Block& b = it->second;
-        if (!b.idle && b.stream != st && b.ev && hipEventQuery(b.ev) != hipSuccess) continue;
+        if (b.stream != st && !block_done(b)) continue;
         Block blk = b;
         fl.erase(it);
         blk.stream = st;
         blk.idle = false;
+        blk.ev_pending = false;
         P.cached -= blk.bytes;
         P.in_use += blk.bytes;
         P.live.emplace(blk.p, blk);
@@ -138,13 +168,7 @@ void pool_free(void* p) {
     if (it == P.live.end()) return;
     Block b = it->second;
     P.live.erase(it);
-    ErrorHygiene eh;
-    int prev = -1;
-    (void)hipGetDevice(&prev);
-    if (prev != b.device) (void)hipSetDevice(b.device);
-    if (!b.ev) (void)hipEventCreateWithFlags(&b.ev, hipEventDisableTiming);
-    if (b.ev) (void)hipEventRecord(b.ev, b.stream);
-    if (prev != b.device && prev >= 0) (void)hipSetDevice(prev);
+    b.ev_pending = false;  // the completion event is recorded when another stream asks for it
     P.in_use -= b.bytes;
     P.cached += b.bytes;
     P.free_by_dev[b.device].emplace(b.bytes, b);
@@ -154,7 +178,9 @@ void pool_release_cached() {
     Pool& P = pool();
     std::lock_guard<std::mutex> lk(P.mu);
     ErrorHygiene eh;
-    (void)hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();  // every cached block's last use has completed
+    for (auto& [dev, fl] : P.free_by_dev)
+        for (auto& [sz, b] : fl) b.idle = true;
     release_completed(P);
 }
 
@@ -163,10 +189,34 @@ void pool_forget_stream(hipStream_t st) {
     std::lock_guard<std::mutex> lk(P.mu);
     for (auto& [dev, fl] : P.free_by_dev)
         for (auto& [sz, b] : fl)
-            if (b.stream == st) {
-                b.idle = true;
-                b.stream = nullptr;
-            }
+            if (b.stream == st) b.idle = true;
+}
+
+hipStream_t stream_acquire() {
+    int dev = 0;
+    FH_HIP(hipGetDevice(&dev));
+    StreamPool& S = stream_pool();
+    {
+        std::lock_guard<std::mutex> lk(S.mu);
+        auto& v = S.free_by_dev[dev];
+        if (!v.empty()) {
+            hipStream_t st = v.back();
+            v.pop_back();
+            return st;
+        }
+    }
+    hipStream_t st = nullptr;
+    FH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    return st;
+}
+
+void stream_release(hipStream_t st, int device) {
+    if (!st) return;
+    (void)hipStreamSynchronize(st);
+    pool_forget_stream(st);  // everything queued on it has completed
+    StreamPool& S = stream_pool();
+    std::lock_guard<std::mutex> lk(S.mu);
+    S.free_by_dev[device].push_back(st);
 }
 
 // ---------------------------------------------------------------- pinned staging

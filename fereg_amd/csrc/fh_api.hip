@@ -736,7 +736,17 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         auto h = std::make_unique<fh_absorber>();
         Absorber& a = h->a;
         FH_HIP(hipGetDevice(&a.device));
-        FH_HIP(hipStreamCreateWithFlags(&a.st, hipStreamNonBlocking));
+        a.st = stream_acquire();
+        // on a failure below the stream goes back to the pool after the absorber's buffers
+        struct StreamBack {
+            hipStream_t st;
+            int dev;
+            bool armed = true;
+            ~StreamBack() {
+                if (armed) stream_release(st, dev);
+            }
+        } back{a.st, a.device};
+        auto hold = std::move(h);  // destroyed before `back`: buffers first, then the stream
         StreamScope ss(a.st);
         a.n = n;
         a.nterms = nterms;
@@ -793,7 +803,8 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
             FH_CHECK_LAUNCH();
         }
         refresh_weights(a, wk ? wrow.p : nullptr);
-        *out = h.release();
+        back.armed = false;
+        *out = hold.release();
     });
 }
 
@@ -810,11 +821,11 @@ void fh_absorber_destroy(fh_absorber* ab) {
     if (!ab) return;
     hipStream_t st = ab->a.st;
     {
-        DeviceGuard g(ab->a.device);
+        const int dev = ab->a.device;
+        DeviceGuard g(dev);
         (void)hipStreamSynchronize(st);
         delete ab;
-        pool_forget_stream(st);
-        (void)hipStreamDestroy(st);
+        stream_release(st, dev);
     }
 }
 

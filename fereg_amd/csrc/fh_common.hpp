@@ -49,6 +49,10 @@ void pool_release_cached();
 // Marks the cached blocks last used on st as idle (call after st is synchronized, before it is
 // destroyed): later requests reuse them without querying events of a dead stream.
 void pool_forget_stream(hipStream_t st);
+// Streams from the library's pool (non-blocking, default priority); release drains the stream
+// and keeps it for the next acquire on that device.
+hipStream_t stream_acquire();
+void stream_release(hipStream_t st, int device);
 void pool_stats(size_t* in_use, size_t* cached);
 hipStream_t current_stream();
 
