@@ -57,7 +57,21 @@ def timed(args, fn):
     return (time.perf_counter() - t0) / args.steps, res
 
 
-def spmv_roofline(F, ab, r, reps):
+def measured_traffic(key):
+    """HBM bytes per launch measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarized in
+    profiles/spmv_ols_n1e7_traffic.md (line "<key>: <bytes>"), or None."""
+    path = os.path.join(ROOT, "profiles", "spmv_ols_n1e7_traffic.md")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith(key + ":"):
+                    return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def spmv_roofline(F, ab, r, reps, traffic_key=None):
     spmv = {}
     for name, prec in (("fp64", F.FH_PLAIN_FP64), ("fp32", F.FH_MIXED)):
         ms, by = ctypes.c_double(), ctypes.c_double()
@@ -67,8 +81,11 @@ def spmv_roofline(F, ab, r, reps):
     ms, by = spmv["fp32"]
     achieved = by / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": f"reduced-operator SpMV S*p (k_seg_t + k_small_dense + k_small_dense_final + k_lvl_spmv), FP32, r={r}",
+            "traffic": measured_traffic(traffic_key) if traffic_key else None,
+            "traffic_source": "profiles/spmv_ols_n1e7_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE, calibrated "
+                              "on a 1 GiB device copy)" if traffic_key else None,
+            "kernel": f"reduced-operator SpMV S*p (k_small_dense_t + k_seg_t + k_lvl_small + k_bucket_reduce + "
+                      f"k_spmv_finish), FP32, r={r}",
             "bytes_per_launch": by, "ms_per_launch": ms,
             "fp64": {"ms": spmv["fp64"][0], "bytes": spmv["fp64"][1],
                      "achieved": spmv["fp64"][1] / (spmv["fp64"][0] * 1e-3) / 1e9}}
@@ -103,7 +120,8 @@ def run_ols(args, np, torch, F, synth):
         timings[name], results[name] = timed(args, lambda: call(prec))
         log(name, "s/call", timings[name], "beta", results[name].coef, "iters", results[name].iterations)
     ab = F.Absorber(d["codes"], d["g"])
-    roof = spmv_roofline(F, ab, 2, args.spmv_reps)
+    roof = spmv_roofline(F, ab, 2, args.spmv_reps,
+                         "traffic_bytes_fp32_r2" if (args.n == 10_000_000 and args.design == "simple") else None)
     info = ab.info
     ab.close()
     cpu, dbeta = None, {}
@@ -185,7 +203,8 @@ def run_ppml(args, np, torch, F, synth):
         out[name] = timed(args, lambda: call(prec))
         log(name, "s/call", out[name][0], "beta", out[name][1].coef, "irls", out[name][1].iterations)
     ab = F.Absorber(d["codes"], d["g"])
-    roof = spmv_roofline(F, ab, 2, args.spmv_reps)
+    roof = spmv_roofline(F, ab, 2, args.spmv_reps,
+                         "traffic_bytes_fp32_r2" if (args.n == 10_000_000 and args.design == "simple") else None)
     ab.close()
     bm, bf = out["mixed"][1], out["fp64"][1]
     config = {"workload": f"poisson count~x1 | indiv+firm+year, n={n}, robust SE, tol 1e-8", "n": n}
@@ -219,6 +238,19 @@ def run_serve(args, np, torch, F, synth):
          {"serve": res, "value_is": "p99 latency (s) of the batched engine"}, unit="s (p99 latency)")
 
 
+def run_copy(args, np, torch, F, synth):
+    """HBM calibration: a plain device copy of a 1 GiB buffer (reads 1 GiB, writes 1 GiB), the
+    reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    nbytes = 1 << 30
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    dst = torch.empty_like(src)
+    s, _ = timed(args, lambda: dst.copy_(src))
+    gbs = 2.0 * nbytes / s / 1e9
+    log("copy 1 GiB:", s * 1e3, "ms", gbs, "GB/s")
+    line(args, gbs, s * 1e3, "u8", {"workload": "device copy, 1 GiB read + 1 GiB written"}, None, None,
+         {"bytes_read": nbytes, "bytes_written": nbytes}, unit="GB/s", higher=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,7 +258,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--design", default="simple", choices=["simple", "difficult"])
-    ap.add_argument("--workload", default="ols", choices=["ols", "wcr", "ppml", "serve"])
+    ap.add_argument("--workload", default="ols", choices=["ols", "wcr", "ppml", "serve", "copy"])
     ap.add_argument("--reps", type=int, default=9999, help="bootstrap draws (wcr)")
     ap.add_argument("--requests", type=int, default=300, help="requests in the stream (serve)")
     ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second (serve)")
@@ -245,7 +277,8 @@ def main():
     from fereg_amd import synth
 
     torch.cuda.set_device(0)
-    {"ols": run_ols, "wcr": run_wcr, "ppml": run_ppml, "serve": run_serve}[args.workload](args, np, torch, F, synth)
+    {"ols": run_ols, "wcr": run_wcr, "ppml": run_ppml, "serve": run_serve,
+     "copy": run_copy}[args.workload](args, np, torch, F, synth)
 
 
 if __name__ == "__main__":
