@@ -17,9 +17,11 @@
 //   lperm/loff: the plain (level, cell) lists of structure.rs:144-153, built on demand.
 //   weight state: cw T[M] (null = unit), inv_n1 T[g1], inv_diag T[m], minv T[ms*ms] (dense
 //   grounded pseudo-inverse of the small super-block); with one small block of <= 32 levels,
-//   the dense small-row operator D T[ms*g1], C T[ms*(m-ms)], nrs T[ms].
+//   the dense small-row operator E T[ms*sm_ld] (D and C side by side), nrs T[ms]; that block
+//   then has no level-major transpose unless one is requested.
 #pragma once
 
+#include <algorithm>
 #include <memory>
 
 #include "fh_common.hpp"
@@ -135,6 +137,20 @@ struct Absorber {
 void build_structure(Absorber& a, const std::vector<const uint32_t*>& dcodes);
 // Level-sorted cells of block b (built on demand for small blocks).
 void ensure_level_cells(Absorber& a, size_t b);
+// The bucketed level-major transpose of block b (every block but the dense small one is built
+// with the structure; ensure builds a missing one).
+void build_block_transpose(Absorber& a, int b);
+void ensure_block_transpose(Absorber& a, int b);
+
+// The dense small-row operator applies: a single small block of <= 32 levels whose D and C are
+// not much larger than the cell records a level pass over it would stream. A function of the
+// structure only.
+inline bool small_dense_rule(const Absorber& a) {
+    int nsb = 0;
+    for (auto& bi : a.blocks) nsb += bi.small ? 1 : 0;
+    const size_t ms = a.ms, m = a.m, g1 = a.g1, M = a.M;
+    return nsb == 1 && ms >= 1 && ms <= 32 && ms * (g1 + (m - ms)) <= 4 * std::max<size_t>(M, 1);
+}
 
 // ---------------------------------------------------------------- weights (fh_weights.hip)
 void refresh_weights(Absorber& a, const double* w_rows_dev /* null = unit */);

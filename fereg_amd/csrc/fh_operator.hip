@@ -57,6 +57,68 @@ void tune_cache_clear() {
     tune_cache().clear();
 }
 
+namespace {
+
+// rhs of the dense small block: part[wg][k][j] = sum of z_c[j] over the cells c of this
+// workgroup's range whose small code is k (chunk of KC levels in blockIdx.y). RMP >= r threads
+// share a cell (one column each), so every z_c element is read once; lanes with the same column
+// are reduced by shuffles, then the waves in order.
+template <int RMP, int KC>
+__global__ __launch_bounds__(WG) void k_small_rhs(const uint32_t* __restrict__ cell_codes, int nb, int bs,
+                                                  const double* __restrict__ zc, size_t M, int r, int ms,
+                                                  double* __restrict__ part) {
+    constexpr int CPB = WG / RMP;
+    __shared__ double red[WAVES][KC][RMP];
+    const int j = threadIdx.x % RMP, ci = threadIdx.x / RMP, k0 = blockIdx.y * KC;
+    double acc[KC];
+#pragma unroll
+    for (int kk = 0; kk < KC; kk++) acc[kk] = 0.0;
+    if (j < r)
+        for (size_t c = (size_t)blockIdx.x * CPB + ci; c < M; c += (size_t)gridDim.x * CPB) {
+            const int k = (int)cell_codes[c * nb + bs] - k0;
+            const double z = zc[c * r + j];
+#pragma unroll
+            for (int kk = 0; kk < KC; kk++) acc[kk] += k == kk ? z : 0.0;
+        }
+    const int w = threadIdx.x / WAVE, l = lane_id();
+#pragma unroll
+    for (int kk = 0; kk < KC; kk++) {
+        double v = acc[kk];
+#pragma unroll
+        for (int o = RMP; o < WAVE; o <<= 1) v += __shfl_xor(v, o, WAVE);
+        if (l < RMP) red[w][kk][l] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < KC * r; e += WG) {
+        const int kk = e / r, jj = e % r;
+        if (k0 + kk >= ms) continue;
+        double sm = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ww++) sm += red[ww][kk][jj];
+        part[(size_t)blockIdx.x * ms * r + (size_t)(k0 + kk) * r + jj] = sm;
+    }
+}
+
+void small_rhs(const Absorber& a, int b, const double* zc, int r, double* rhs_b, hipStream_t st) {
+    const int ms = (int)a.blocks[b].g, nb = (int)a.nb;
+    auto go = [&](auto rmp, auto kc) {
+        constexpr int RMP = decltype(rmp)::value, KC = decltype(kc)::value;
+        const unsigned nwg = grid_for(a.M, WG / RMP, 1024), nch = (unsigned)((ms + KC - 1) / KC);
+        DBuf<double> part((size_t)nwg * ms * r);
+        hipLaunchKernelGGL((k_small_rhs<RMP, KC>), dim3(nwg, nch), dim3(WG), 0, st, a.cell_codes.p, nb, b, zc,
+                           (size_t)a.M, r, ms, part.p);
+        FH_CHECK_LAUNCH();
+        sum_parts_to(part.p, nwg, (size_t)ms * r, rhs_b, st);
+    };
+    if (r <= 1) go(std::integral_constant<int, 1>{}, std::integral_constant<int, 32>{});
+    else if (r <= 2) go(std::integral_constant<int, 2>{}, std::integral_constant<int, 32>{});
+    else if (r <= 4) go(std::integral_constant<int, 4>{}, std::integral_constant<int, 32>{});
+    else if (r <= 8) go(std::integral_constant<int, 8>{}, std::integral_constant<int, 32>{});
+    else go(std::integral_constant<int, 16>{}, std::integral_constant<int, 32>{});
+}
+
+}  // namespace
+
 unsigned rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, size_t extra) {
     require(r >= 1 && r <= 16, "rhs: 1..16 columns");
     hipStream_t st = a.st;
@@ -72,6 +134,10 @@ unsigned rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b
     FH_CHECK_LAUNCH();
     for (size_t b = 0; b < a.nb; b++) {
         const BlockInfo& bi = a.blocks[b];
+        if (a.boff[b].n == 0) {  // the dense small block: binned by code, no level pass
+            small_rhs(a, (int)b, s.zc.p, r, rhs + bi.off * r, st);
+            continue;
+        }
         const int Ll = (int)a.blanes[b];
         const uint32_t nbk = a.nbk[b];
         const size_t items = (size_t)nbk * bi.g;

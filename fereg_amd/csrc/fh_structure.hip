@@ -515,68 +515,85 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
     a.bwidth.assign(nb, 0);
     a.blanes.assign(nb, 1);
     a.llanes.assign(nb, 1);
+    // the single small block of the dense small-row operator never runs a level pass; its
+    // transpose is built only on request (ensure_block_transpose)
+    const bool skip_small = small_dense_rule(a);
     for (int b = 0; b < nb; b++) {
-        const uint64_t g = a.blocks[b].g;
-        // FE1 buckets of bw consecutive segments. Configured: 2^bucket_log2. Automatic: one
-        // bucket per XCD once the FE1 range is large (the level passes then keep each XCD's
-        // gathers of segment sums within an eighth of them, see the level pass). Long levels (a
-        // few levels over many cells) are cut further, to about ITEM_CELLS cells per (bucket,
-        // level) item, so that every item fills a wave.
-        constexpr double ITEM_CELLS = 2048.0;
-        const uint64_t g1 = std::max<uint64_t>(a.g1, 1);
-        uint64_t bw = g1;
-        if (a.bucket_log2 > 0) bw = std::min<uint64_t>(g1, 1ull << a.bucket_log2);
-        else if (a.bucket_log2 < 0 && a.g1 >= AUTO_BUCKET_G1) bw = (g1 + XCD_COUNT - 1) / XCD_COUNT;
-        const double per_level = double(M) / double(std::max<uint64_t>(g, 1));
-        if (per_level > ITEM_CELLS && a.g1 > 1) {
-            const double want = std::ceil(per_level / ITEM_CELLS);  // buckets
-            bw = std::min<uint64_t>(bw, std::max<uint64_t>(1, (uint64_t)std::ceil(double(g1) / want)));
-        }
-        const uint32_t nbk = (uint32_t)((g1 + bw - 1) / bw);
-        a.bwidth[b] = (uint32_t)bw;
-        a.nbk[b] = std::max<uint32_t>(nbk, 1);
-        const uint64_t items = (uint64_t)a.nbk[b] * g;
-        require(items < (1ull << 32), "too many (bucket, level) items");
-        // lanes per item: ~8 positions per lane, ~4 for the short items of a bucketed block; one
-        // item per level over its buckets: ~2 per lane of each bucket's range
-        const double ppi = items ? double(M) / double(items) : 1.0;
-        a.blanes[b] = lanes(a.nbk[b] > 1 ? ppi / 4.0 : ppi / 8.0);
-        a.llanes[b] = lanes(ppi / 2.0);
-        a.bperm[b].alloc(std::max<size_t>(M, 1));
-        a.boff[b].alloc(items + 1);
-        FH_HIP(hipMemsetAsync(a.boff[b].p, 0, a.boff[b].bytes(), st));
-        a.lseg[b].alloc(std::max<size_t>(M, 1));
-        a.lother[b].alloc(std::max<size_t>(M * (nb - 1), 1));
-        if (M > 0) {
-            DBuf<uint64_t> keys(M), kalt(M);
-            DBuf<uint32_t> valt(M), flag(1);
-            PayloadLayout pl{};
-            pl.ib = std::max(1, (int)bits_for(items));
-            pl.sb = std::max(1, (int)bits_for(a.g1));
-            int tot = pl.ib + pl.sb, o = 0;
-            for (int bb = 0; bb < nb; bb++)
-                if (bb != b) { pl.ob[o] = std::max(1, (int)bits_for(a.blocks[bb].g)); tot += pl.ob[o++]; }
-            pl.pay = tot <= 64;
-            FH_HIP(hipMemsetAsync(flag.p, 0, sizeof(uint32_t), st));
-            hipLaunchKernelGGL(k_bucket_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, a.cell_codes.p, a.cell_seg.p, M, nb,
-                               b, (uint32_t)bw, g, pl, keys.p, flag.p);
-            hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, M);
-            FH_CHECK_LAUNCH();
-            if (read_scalar(flag.p, st) != 0)
-                radix_sort_pairs(keys.p, a.bperm[b].p, kalt.p, valt.p, M, pl.ib, st);
-            const uint64_t mask = pl.ib >= 64 ? ~0ull : ((1ull << pl.ib) - 1);
-            hipLaunchKernelGGL(k_offsets_u64, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, (size_t)items,
-                               a.boff[b].p, mask);
-            if (pl.pay)
-                hipLaunchKernelGGL(k_level_records_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, nb - 1, pl,
-                                   a.lseg[b].p, a.lother[b].p);
-            else
-                hipLaunchKernelGGL(k_level_records, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, a.cell_seg.p,
-                                   a.cell_codes.p, M, nb, b, a.lseg[b].p, a.lother[b].p);
-            FH_CHECK_LAUNCH();
-        }
+        if (skip_small && a.blocks[b].small) continue;
+        build_block_transpose(a, b);
     }
     a.seg_lanes = seg_lanes_for(M, a.g1);
+}
+
+// Bucketed level-major transpose of block b: bperm, boff, lseg, lother (and nbk, bwidth, lanes).
+void build_block_transpose(Absorber& a, int b) {
+    hipStream_t st = a.st;
+    StreamScope ss(st);
+    const size_t M = a.M;
+    const int nb = (int)a.nb;
+    const uint64_t g = a.blocks[b].g;
+    // FE1 buckets of bw consecutive segments. Configured: 2^bucket_log2. Automatic: one
+    // bucket per XCD once the FE1 range is large (the level passes then keep each XCD's
+    // gathers of segment sums within an eighth of them, see the level pass). Long levels (a
+    // few levels over many cells) are cut further, to about ITEM_CELLS cells per (bucket,
+    // level) item, so that every item fills a wave.
+    constexpr double ITEM_CELLS = 2048.0;
+    const uint64_t g1 = std::max<uint64_t>(a.g1, 1);
+    uint64_t bw = g1;
+    if (a.bucket_log2 > 0) bw = std::min<uint64_t>(g1, 1ull << a.bucket_log2);
+    else if (a.bucket_log2 < 0 && a.g1 >= AUTO_BUCKET_G1) bw = (g1 + XCD_COUNT - 1) / XCD_COUNT;
+    const double per_level = double(M) / double(std::max<uint64_t>(g, 1));
+    if (per_level > ITEM_CELLS && a.g1 > 1) {
+        const double want = std::ceil(per_level / ITEM_CELLS);  // buckets
+        bw = std::min<uint64_t>(bw, std::max<uint64_t>(1, (uint64_t)std::ceil(double(g1) / want)));
+    }
+    const uint32_t nbk = (uint32_t)((g1 + bw - 1) / bw);
+    a.bwidth[b] = (uint32_t)bw;
+    a.nbk[b] = std::max<uint32_t>(nbk, 1);
+    const uint64_t items = (uint64_t)a.nbk[b] * g;
+    require(items < (1ull << 32), "too many (bucket, level) items");
+    // lanes per item: ~8 positions per lane, ~4 for the short items of a bucketed block; one
+    // item per level over its buckets: ~2 per lane of each bucket's range
+    const double ppi = items ? double(M) / double(items) : 1.0;
+    a.blanes[b] = lanes(a.nbk[b] > 1 ? ppi / 4.0 : ppi / 8.0);
+    a.llanes[b] = lanes(ppi / 2.0);
+    a.bperm[b].alloc(std::max<size_t>(M, 1));
+    a.boff[b].alloc(items + 1);
+    FH_HIP(hipMemsetAsync(a.boff[b].p, 0, a.boff[b].bytes(), st));
+    a.lseg[b].alloc(std::max<size_t>(M, 1));
+    a.lother[b].alloc(std::max<size_t>(M * (nb - 1), 1));
+    if (M > 0) {
+        DBuf<uint64_t> keys(M), kalt(M);
+        DBuf<uint32_t> valt(M), flag(1);
+        PayloadLayout pl{};
+        pl.ib = std::max(1, (int)bits_for(items));
+        pl.sb = std::max(1, (int)bits_for(a.g1));
+        int tot = pl.ib + pl.sb, o = 0;
+        for (int bb = 0; bb < nb; bb++)
+            if (bb != b) { pl.ob[o] = std::max(1, (int)bits_for(a.blocks[bb].g)); tot += pl.ob[o++]; }
+        pl.pay = tot <= 64;
+        FH_HIP(hipMemsetAsync(flag.p, 0, sizeof(uint32_t), st));
+        hipLaunchKernelGGL(k_bucket_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, a.cell_codes.p, a.cell_seg.p, M, nb,
+                           b, (uint32_t)bw, g, pl, keys.p, flag.p);
+        hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, M);
+        FH_CHECK_LAUNCH();
+        if (read_scalar(flag.p, st) != 0)
+            radix_sort_pairs(keys.p, a.bperm[b].p, kalt.p, valt.p, M, pl.ib, st);
+        const uint64_t mask = pl.ib >= 64 ? ~0ull : ((1ull << pl.ib) - 1);
+        hipLaunchKernelGGL(k_offsets_u64, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, (size_t)items,
+                           a.boff[b].p, mask);
+        if (pl.pay)
+            hipLaunchKernelGGL(k_level_records_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, nb - 1, pl,
+                               a.lseg[b].p, a.lother[b].p);
+        else
+            hipLaunchKernelGGL(k_level_records, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, a.cell_seg.p,
+                               a.cell_codes.p, M, nb, b, a.lseg[b].p, a.lother[b].p);
+        FH_CHECK_LAUNCH();
+    }
+}
+
+void ensure_block_transpose(Absorber& a, int b) {
+    if (a.boff[b].n == 0) build_block_transpose(a, b);
 }
 
 void ensure_level_cells(Absorber& a, size_t b) {

@@ -523,6 +523,7 @@ std::vector<int> terms_nested_in(Absorber& a, const uint32_t* cl) {
                            (const uint32_t*)nullptr, span.p, (size_t)a.g1, (uint2*)nullptr, bad.p);
     DBuf<uint2> gspan;
     for (size_t b = 0; b < a.nb; b++) {
+        ensure_block_transpose(a, (int)b);
         const size_t items = (size_t)a.nbk[b] * a.blocks[b].g;
         if (a.nbk[b] > 1) gspan.reserve(items);
         hipLaunchKernelGGL(k_wb_group_span, dim3(grid_for(items, TPB, 4096)), dim3(TPB), 0, st, a.boff[b].p,

@@ -425,9 +425,10 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
     const int nb = (int)a.nb;
     a.w64.lw.resize(nb);
     a.w32.lw.resize(nb);
+    const bool dense_small = small_dense_rule(a);
     for (int b = 0; b < nb; b++) {
         const BlockInfo& bi = a.blocks[b];
-        if (bi.small && unit) continue;
+        if (bi.small && (unit || dense_small)) continue;  // no level pass over the dense small block
         if (!unit) {
             a.w64.lw[b].reserve(M);
             a.w32.lw[b].reserve(M);
@@ -454,11 +455,7 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
     }
     // dense small-row operator: one small block of <= 32 levels whose D and C are not much
     // larger than the cell records a level pass would stream
-    {
-        int nsb = 0;
-        for (auto& bi : a.blocks) nsb += bi.small ? 1 : 0;
-        a.small_dense = nsb == 1 && ms >= 1 && ms <= 32 && ms * (g1 + (m - ms)) <= 4 * std::max<size_t>(M, 1);
-    }
+    a.small_dense = dense_small;
     if (a.small_dense) {
         // E = [D | pad | C | pad], ms rows of pitch sm_ld: every column quad is 16-byte aligned
         const size_t mL = m - ms;
