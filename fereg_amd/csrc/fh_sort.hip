@@ -39,15 +39,23 @@ __global__ __launch_bounds__(RS_WG) void rs_upsweep(const uint64_t* __restrict__
     hist[(size_t)t * ntiles + blockIdx.x] = s;
 }
 
+// Scatter pass. The keys of a tile are first placed in LDS in their sorted-within-tile order
+// (digit, then input order), then written out by consecutive threads: a digit's run of the tile
+// goes to consecutive global addresses, so the writes coalesce instead of landing as single
+// 8- and 4-byte stores on 256 different lines.
 __global__ __launch_bounds__(RS_WG) void rs_downsweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                       size_t n, int shift, uint32_t dmask, const uint32_t* __restrict__ hist,
                                                       uint32_t ntiles) {
     __shared__ uint32_t cnt[RS_WAVES][256];
+    __shared__ uint32_t lstart[256], gstart[256];
+    __shared__ uint64_t lk[RS_TILE];
+    __shared__ uint32_t lv[RS_TILE];
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_WG) (&cnt[0][0])[i] = 0;
     __syncthreads();
     const int w = threadIdx.x / WAVE, l = lane_id();
-    const size_t base = (size_t)blockIdx.x * RS_TILE + (size_t)w * WAVE * RS_ITEMS;
+    const size_t tile0 = (size_t)blockIdx.x * RS_TILE;
+    const size_t base = tile0 + (size_t)w * WAVE * RS_ITEMS;
     uint64_t k[RS_ITEMS];
     uint32_t v[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
@@ -73,24 +81,50 @@ __global__ __launch_bounds__(RS_WG) void rs_downsweep(const uint64_t* __restrict
     }
     __syncthreads();
     {
+        // per digit t: the tile's total, its global base (hist) and the per-wave local bases
         const int t = threadIdx.x;
-        uint32_t run = hist[(size_t)t * ntiles + blockIdx.x];
+        uint32_t tot = 0;
 #pragma unroll
         for (int ww = 0; ww < RS_WAVES; ww++) {
-            uint32_t c = cnt[ww][t];
-            cnt[ww][t] = run;
-            run += c;
+            const uint32_t c = cnt[ww][t];
+            cnt[ww][t] = tot;  // wave's offset inside the digit's run of this tile
+            tot += c;
         }
+        gstart[t] = hist[(size_t)t * ntiles + blockIdx.x];
+        // exclusive scan of the totals over the 256 digits (one per thread)
+        uint32_t x = tot;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, WAVE);
+            if (l >= o) x += y;
+        }
+        __shared__ uint32_t wtot[RS_WAVES];
+        if (l == WAVE - 1) wtot[w] = x;
+        __syncthreads();
+        uint32_t off = 0;
+#pragma unroll
+        for (int ww = 0; ww < RS_WAVES; ww++)
+            if (ww < w) off += wtot[ww];
+        lstart[t] = off + x - tot;
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; i++) {
         size_t idx = base + (size_t)i * WAVE + l;
         if (idx < n) {
-            uint32_t pos = cnt[w][dig[i]] + rank[i];
-            kout[pos] = k[i];
-            vout[pos] = v[i];
+            const uint32_t li = lstart[dig[i]] + cnt[w][dig[i]] + rank[i];
+            lk[li] = k[i];
+            lv[li] = v[i];
         }
+    }
+    __syncthreads();
+    const size_t cnt_tile = n - tile0 < (size_t)RS_TILE ? n - tile0 : (size_t)RS_TILE;
+    for (uint32_t li = threadIdx.x; li < cnt_tile; li += RS_WG) {
+        const uint64_t kk = lk[li];
+        const uint32_t d = (uint32_t)(kk >> shift) & dmask;
+        const uint32_t pos = gstart[d] + (li - lstart[d]);
+        kout[pos] = kk;
+        vout[pos] = lv[li];
     }
 }
 
