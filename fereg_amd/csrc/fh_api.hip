@@ -620,14 +620,18 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
     uint64_t it = 0, pcg_total = 0;
     bool converged = false;
     const int prec = o.precision < 0 ? a.precision : o.precision;
+    // inner tolerance schedule of the fixest flavour (glm/irls.rs:893-901, glm/mod.rs:826-831):
+    // start at max(1e-4, tol), then eps^{3/2} of the relative deviance change (an inexact Newton
+    // step), never below highest = max(1e-12, min(itol, 0.1 tol)); convergence needs the last
+    // FE solve at that final tolerance
+    const double highest = std::max(1e-12, std::min(o.tol, 0.1 * tol));
+    double tol_in = std::max(1e-4, tol);
     for (it = 1; it <= maxiter; it++) {
         check_interrupt(ctx);
         hipLaunchKernelGGL(k_irls_fill, dim3(vgrid(n)), dim3(TPB), 0, st, yr.p, mu.p, eta.p, (const double*)nullptr,
                            (const double*)nullptr, xr.p, n, K, rows, zraw.p, w.p);
         FH_CHECK_LAUNCH();
         refresh_weights(a, w.p);
-        // inner tolerance: loose while the deviance still moves, tight at the end
-        double tol_in = o.tol;
         ResidOut ro = residualize_any(a, rows, R, tol_in, o.maxiter, have_warm ? warm.p : nullptr, coefs.p, prec, ctx);
         pcg_total += ro.iterations;
         FH_HIP(hipMemcpyAsync(warm.p, coefs.p, p * R * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -655,6 +659,7 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
         residual_pass(a, rows, n, (int)R, c.data(), nullptr, e.p);
         FH_HIP(hipMemcpyAsync(eta_old.p, eta.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         double step = 1.0, dev = NAN;
+        bool halved = false;
         for (int half = 0; half < 30; half++) {
             hipLaunchKernelGGL(k_irls_eta, dim3(1024), dim3(TPB), 0, st, zraw.p, e.p, (const double*)nullptr, eta_old.p,
                                step, yr.p, (const double*)nullptr, n, eta.p, mu.p, part.p);
@@ -662,11 +667,22 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
             dev = 2.0 * sum_parts(part.p, 1024, st);
             if (std::isfinite(dev) && (std::isnan(dev_old) || dev <= dev_old * (1.0 + 1e-12) + 1e-300)) break;
             step *= 0.5;
+            halved = true;
         }
-        const double crit = std::isnan(dev_old) ? INFINITY : std::fabs(dev - dev_old) / (0.1 + std::fabs(dev));
-        log_line(ctx, 2, "fereg_hip: irls iteration " + std::to_string(it) + " deviance " + std::to_string(dev));
+        // an iteration with step halving never converges (glm/irls.rs:828-834)
+        const double crit =
+            std::isnan(dev_old) || halved ? INFINITY : std::fabs(dev - dev_old) / (0.1 + std::fabs(dev));
+        log_line(ctx, 2, "fereg_hip: irls iteration " + std::to_string(it) + " deviance " + std::to_string(dev) +
+                             " inner tol " + std::to_string(tol_in));
+        const bool at_final_tol = tol_in <= 1.1 * highest;
+        if (!std::isnan(dev_old)) {  // glm/irls.rs:746-749, 824-827, 899-901
+            double delta = dev_old - dev;
+            if (!std::isnan(delta) && dev < 0.1 * delta) delta = dev;
+            const double eps = std::fabs(delta) / std::max(std::min(dev, dev_old), 0.1);
+            tol_in = std::max(std::min(eps * std::sqrt(eps), tol_in), highest);
+        }
         dev_old = dev;
-        if (crit < tol) {
+        if (crit < tol && (ro.exact || at_final_tol)) {
             converged = true;
             break;
         }

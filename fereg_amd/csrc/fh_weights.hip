@@ -64,13 +64,11 @@ __global__ __launch_bounds__(TPB) void k_partial_sum(const double* __restrict__ 
 // run sums: the lane holding the last position of a run sums the run forward from its start
 // (each run is summed once, in position order) and adds run^2 / n1_s.
 __global__ __launch_bounds__(TPB) void k_level_weights(const uint32_t* __restrict__ boff,
-                                                       const uint32_t* __restrict__ bperm,
                                                        const uint32_t* __restrict__ lseg, uint32_t nbk,
-                                                       const double* __restrict__ cw, const double* __restrict__ n1,
+                                                       const double* __restrict__ lw, const double* __restrict__ n1,
                                                        size_t g, size_t off, int L, double* __restrict__ nr,
                                                        double* __restrict__ diag, uint8_t* __restrict__ red,
-                                                       double* __restrict__ inv64, float* __restrict__ inv32,
-                                                       double* __restrict__ lw64, float* __restrict__ lw32) {
+                                                       double* __restrict__ inv64, float* __restrict__ inv32) {
     const int l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
     const size_t nwaves = (size_t)gridDim.x * (TPB / WAVE);
@@ -81,15 +79,14 @@ __global__ __launch_bounds__(TPB) void k_level_weights(const uint32_t* __restric
         for (uint32_t k = 0; k < nbk; k++) {
             const uint32_t j0 = valid ? boff[(size_t)k * g + lv] : 0, j1 = valid ? boff[(size_t)k * g + lv + 1] : 0;
             for (uint32_t j = j0 + li; j < j1; j += L) {
-                const double w = cw ? cw[bperm[j]] : 1.0;
-                if (lw64) { lw64[j] = w; lw32[j] = (float)w; }
+                const double w = lw ? lw[j] : 1.0;
                 tot += w;
                 const uint32_t s = lseg[j];
                 if (j + 1 < j1 && lseg[j + 1] == s) continue;  // not the run's last position
                 uint32_t q = j;
                 while (q > j0 && lseg[q - 1] == s) q--;
                 double run = 0.0;
-                for (uint32_t i = q; i <= j; i++) run += cw ? cw[bperm[i]] : 1.0;
+                for (uint32_t i = q; i <= j; i++) run += lw ? lw[i] : 1.0;
                 dq += run * run / n1[s];
             }
         }
@@ -106,6 +103,24 @@ __global__ __launch_bounds__(TPB) void k_level_weights(const uint32_t* __restric
         const double inv = (r || !(d > 0.0)) ? 0.0 : 1.0 / d;  // jacobi.rs:7-9
         inv64[off + lv] = inv;
         inv32[off + lv] = (float)inv;
+    }
+}
+
+// lw[j] = W of the cell at position j of a bucketed block (bucket-major order). Bucket q's
+// positions only reference cells of its FE1 range; with X = XCD_COUNT they are gathered by the
+// workgroups of XCD q mod X, whose L2 then holds that slice of the cell weights.
+__global__ __launch_bounds__(TPB) void k_gather_lw_x(const uint32_t* __restrict__ boff, uint32_t nbk, size_t g,
+                                                     const uint32_t* __restrict__ bperm, const double* __restrict__ cw,
+                                                     uint32_t X, double* __restrict__ lw64, float* __restrict__ lw32) {
+    const uint32_t xq = blockIdx.x % X;
+    const size_t lb = blockIdx.x / X, nl = gridDim.x / X;
+    for (uint32_t q = xq; q < nbk; q += X) {
+        const size_t j0 = boff[(size_t)q * g], j1 = boff[(size_t)(q + 1) * g];
+        for (size_t j = j0 + lb * TPB + threadIdx.x; j < j1; j += nl * TPB) {
+            const double w = cw[bperm[j]];
+            lw64[j] = w;
+            lw32[j] = (float)w;
+        }
     }
 }
 
@@ -195,21 +210,33 @@ __global__ void k_fixed_to_double(const unsigned long long* __restrict__ parts, 
 // C = [c_{s,0..ms-1} / sqrt(n1_s), sqrt(n1_s)] (ld ms + 1) and the dense D[k][s] = c_{s,k}
 // (FP64 and FP32, pitch ldd).
 template <int MS>
-__global__ void k_seg_level_sums(const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ cell_codes,
-                                 const double* __restrict__ cw, const double* __restrict__ n1, size_t g1, int nb,
-                                 uint32_t off0, int ms, double* __restrict__ C, size_t ldd,
-                                 double* __restrict__ D64, float* __restrict__ D32) {
-    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(TPB) void k_seg_level_sums(const uint32_t* __restrict__ seg_off,
+                                                        const uint32_t* __restrict__ cell_codes,
+                                                        const double* __restrict__ cw, const double* __restrict__ n1,
+                                                        size_t g1, int nb, uint32_t off0, int ms,
+                                                        double* __restrict__ C, size_t ldd, double* __restrict__ D64,
+                                                        float* __restrict__ D32, int L) {
+    // L lanes per segment (its cells strided over the lanes, register bins, shuffle reduction)
+    const int l = lane_id();
+    const int gpw = WAVE / L, gl = l / L, li = l % L;
+    const size_t nwaves = (size_t)gridDim.x * (TPB / WAVE);
+    for (size_t base = ((size_t)blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE) * gpw; base < g1; base += nwaves * gpw) {
+        const size_t s = base + gl;
+        const bool valid = s < g1;
         double row[MS];
 #pragma unroll
         for (int k = 0; k < MS; k++) row[k] = 0.0;
-        const uint32_t c0 = seg_off[s], c1 = seg_off[s + 1];
-        for (uint32_t c = c0; c < c1; c++) {
+        const uint32_t c0 = valid ? seg_off[s] : 0, c1 = valid ? seg_off[s + 1] : 0;
+        for (uint32_t c = c0 + li; c < c1; c += L) {
             const uint32_t key = off0 + cell_codes[(size_t)c * nb];
             const double w = cw ? cw[c] : 1.0;
 #pragma unroll
             for (int k = 0; k < MS; k++) row[k] += key == (uint32_t)k ? w : 0.0;
         }
+#pragma unroll
+        for (int k = 0; k < MS; k++)
+            for (int o = L >> 1; o > 0; o >>= 1) row[k] += __shfl_xor(row[k], o, WAVE);
+        if (!valid || li != 0) continue;
         if (D64)
 #pragma unroll
             for (int k = 0; k < MS; k++)
@@ -226,18 +253,26 @@ __global__ void k_seg_level_sums(const uint32_t* __restrict__ seg_off, const uin
 }
 
 // C[k][coff + l] = sum of W over the positions of level l of a large block whose small code
-// (other block 0) is k: one thread per level, buckets in order, the row in registers.
-__global__ void k_level_small_sums(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ lother, int nother,
-                                   const double* __restrict__ lw, uint32_t nbk, size_t g, int ms, size_t ldc, size_t coff,
-                                   double* __restrict__ C64, float* __restrict__ C32) {
-    constexpr int MS = 32;
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
+// (other block 0) is k: L lanes per level over its buckets' ranges in order, MS register bins
+// per lane, then the lanes reduced by shuffles (fixed order).
+template <int MS>
+__global__ __launch_bounds__(TPB) void k_level_small_sums(const uint32_t* __restrict__ boff,
+                                                          const uint32_t* __restrict__ lother, int nother,
+                                                          const double* __restrict__ lw, uint32_t nbk, size_t g,
+                                                          int ms, size_t ldc, size_t coff, int L,
+                                                          double* __restrict__ C64, float* __restrict__ C32) {
+    const int l = lane_id();
+    const int gpw = WAVE / L, gl = l / L, li = l % L;
+    const size_t nwaves = (size_t)gridDim.x * (TPB / WAVE);
+    for (size_t base = ((size_t)blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE) * gpw; base < g; base += nwaves * gpw) {
+        const size_t lv = base + gl;
+        const bool valid = lv < g;
         double row[MS];
 #pragma unroll
         for (int k = 0; k < MS; k++) row[k] = 0.0;
         for (uint32_t q = 0; q < nbk; q++) {
-            const uint32_t j0 = boff[(size_t)q * g + l], j1 = boff[(size_t)q * g + l + 1];
-            for (uint32_t j = j0; j < j1; j++) {
+            const uint32_t j0 = valid ? boff[(size_t)q * g + lv] : 0, j1 = valid ? boff[(size_t)q * g + lv + 1] : 0;
+            for (uint32_t j = j0 + li; j < j1; j += L) {
                 const uint32_t key = lother[(size_t)j * nother];
                 const double w = lw ? lw[j] : 1.0;
 #pragma unroll
@@ -246,9 +281,13 @@ __global__ void k_level_small_sums(const uint32_t* __restrict__ boff, const uint
         }
 #pragma unroll
         for (int k = 0; k < MS; k++)
+            for (int o = L >> 1; o > 0; o >>= 1) row[k] += __shfl_xor(row[k], o, WAVE);
+        if (!valid || li != 0) continue;
+#pragma unroll
+        for (int k = 0; k < MS; k++)
             if (k < ms) {
-                C64[(size_t)k * ldc + coff + l] = row[k];
-                C32[(size_t)k * ldc + coff + l] = (float)row[k];
+                C64[(size_t)k * ldc + coff + lv] = row[k];
+                C32[(size_t)k * ldc + coff + lv] = (float)row[k];
             }
     }
 }
@@ -274,6 +313,10 @@ __global__ void k_small_spread(const double* __restrict__ pk, size_t ms, int wit
             minv32[i - ms] = (float)v;
         }
     }
+}
+
+inline unsigned seg_sums_grid(const Absorber& a) {
+    return grid_for(a.g1, (TPB / WAVE) * (WAVE / a.seg_lanes), 1u << 16);
 }
 
 double sum_dev(const double* v, size_t n, hipStream_t st) {
@@ -442,15 +485,19 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             FH_CHECK_LAUNCH();
             continue;
         }
+        if (!unit) {  // level-order weights first: the level sums then read them in order
+            const uint32_t X = a.nbk[b] >= (uint32_t)XCD_COUNT ? (uint32_t)XCD_COUNT : 1u;
+            const unsigned gg = (unsigned)(grid_for(M, TPB, 4096) / X * X + X);
+            hipLaunchKernelGGL(k_gather_lw_x, dim3(gg), dim3(TPB), 0, st, a.boff[b].p, a.nbk[b], (size_t)bi.g,
+                               a.bperm[b].p, cw, X, a.w64.lw[b].p, a.w32.lw[b].p);
+            FH_CHECK_LAUNCH();
+        }
         uint32_t L = 1;  // lanes per level: ~4 positions per lane
         while (L < 64 && L * 4.0 < double(a.M) / double(std::max<uint64_t>(bi.g, 1))) L <<= 1;
         const unsigned grid = grid_for(bi.g, (TPB / WAVE) * (WAVE / L), 1u << 16);
-        hipLaunchKernelGGL(k_level_weights, dim3(grid), dim3(TPB), 0, st, a.boff[b].p, a.bperm[b].p,
-                           a.lseg[b].p, a.nbk[b], cw, a.n1.p, (size_t)bi.g, (size_t)bi.off, (int)L,
-                           a.nr.p, a.diag.p,
-                           a.red.p,
-                           a.w64.inv_diag.p, a.w32.inv_diag.p, unit ? nullptr : a.w64.lw[b].p,
-                           unit ? nullptr : a.w32.lw[b].p);
+        hipLaunchKernelGGL(k_level_weights, dim3(grid), dim3(TPB), 0, st, a.boff[b].p, a.lseg[b].p, a.nbk[b],
+                           unit ? nullptr : a.w64.lw[b].p, a.n1.p, (size_t)bi.g, (size_t)bi.off, (int)L, a.nr.p,
+                           a.diag.p, a.red.p, a.w64.inv_diag.p, a.w32.inv_diag.p);
         FH_CHECK_LAUNCH();
     }
     // dense small-row operator: one small block of <= 32 levels whose D and C are not much
@@ -468,15 +515,23 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         for (int b = 0; b < nb; b++) {
             const BlockInfo& bi = a.blocks[b];
             if (bi.small) continue;
-            hipLaunchKernelGGL(k_level_small_sums, dim3(blocks_for(bi.g)), dim3(TPB), 0, st, a.boff[b].p,
-                               a.lother[b].p, nb - 1, unit ? nullptr : a.w64.lw[b].p, a.nbk[b], (size_t)bi.g, (int)ms,
-                               a.sm_ld, a.sm_coff + (size_t)(bi.off - ms), a.w64.esm.p, a.w32.esm.p);
+            uint32_t L = 1;  // lanes per level: ~8 positions per lane
+            while (L < 64 && L * 8.0 < double(a.M) / double(std::max<uint64_t>(bi.g, 1))) L <<= 1;
+            const unsigned grid = grid_for(bi.g, (TPB / WAVE) * (WAVE / L), 1u << 16);
+            if (ms <= 16)
+                hipLaunchKernelGGL(k_level_small_sums<16>, dim3(grid), dim3(TPB), 0, st, a.boff[b].p, a.lother[b].p,
+                                   nb - 1, unit ? nullptr : a.w64.lw[b].p, a.nbk[b], (size_t)bi.g, (int)ms, a.sm_ld,
+                                   a.sm_coff + (size_t)(bi.off - ms), (int)L, a.w64.esm.p, a.w32.esm.p);
+            else
+                hipLaunchKernelGGL(k_level_small_sums<32>, dim3(grid), dim3(TPB), 0, st, a.boff[b].p, a.lother[b].p,
+                                   nb - 1, unit ? nullptr : a.w64.lw[b].p, a.nbk[b], (size_t)bi.g, (int)ms, a.sm_ld,
+                                   a.sm_coff + (size_t)(bi.off - ms), (int)L, a.w64.esm.p, a.w32.esm.p);
             FH_CHECK_LAUNCH();
         }
         if (ms > 21)  // the S_SS assembly below does not produce D for these
-            hipLaunchKernelGGL(k_seg_level_sums<32>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
+            hipLaunchKernelGGL(k_seg_level_sums<32>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                a.cell_codes.p, cw, a.n1.p, g1, nb, (uint32_t)a.blocks[0].off, (int)ms, (double*)nullptr,
-                               a.sm_ld, a.w64.esm.p, a.w32.esm.p);
+                               a.sm_ld, a.w64.esm.p, a.w32.esm.p, (int)a.seg_lanes);
         FH_CHECK_LAUNCH();
     }
     // small super-block
@@ -496,11 +551,13 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             double* D64 = a.small_dense ? a.w64.esm.p : nullptr;
             float* D32 = a.small_dense ? a.w32.esm.p : nullptr;
             if (ms <= 8)
-                hipLaunchKernelGGL(k_seg_level_sums<8>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
-                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32);
+                hipLaunchKernelGGL(k_seg_level_sums<8>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
+                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
+                                   (int)a.seg_lanes);
             else
-                hipLaunchKernelGGL(k_seg_level_sums<21>, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p,
-                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32);
+                hipLaunchKernelGGL(k_seg_level_sums<21>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
+                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
+                                   (int)a.seg_lanes);
             FH_CHECK_LAUNCH();
             std::vector<double> G((ms + 1) * (ms + 1));
             gram_cols(a, C.p, g1, (int)ms + 1, 0, (int)ms + 1, nullptr, nullptr, G.data());
