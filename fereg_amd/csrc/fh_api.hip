@@ -32,13 +32,19 @@ namespace {
 constexpr int TPB = 256;
 inline unsigned vgrid(size_t n) { return grid_for(n, TPB, 8192); }
 
+// rows[i][0] = y[pi[i]], rows[i][1 + k] = X[pi[i]][k]: one thread per output element, so the
+// writes (and, for near-identity pi, the reads of X) are coalesced.
 __global__ void k_gather_rows(const double* __restrict__ y, const double* __restrict__ X, size_t K,
                               const uint32_t* __restrict__ pi, size_t n, double* __restrict__ rows) {
     const size_t R = K + 1;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * R; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = e / R, c = e - i * R;
         const size_t o = pi[i];
-        if (y) rows[i * R] = y[o];
-        for (size_t k = 0; k < K; k++) rows[i * R + 1 + k] = X[o * K + k];
+        if (c == 0) {
+            if (y) rows[e] = y[o];
+        } else {
+            rows[e] = X[o * K + c - 1];
+        }
     }
 }
 
@@ -63,8 +69,16 @@ __global__ void k_check_weights(const double* __restrict__ w, size_t n, uint32_t
 }
 __global__ void k_max_u32(const uint32_t* __restrict__ c, size_t n, uint32_t* out) {
     uint32_t m = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        m = max(m, c[i]);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 7 * stride < n; i += 8 * stride) {  // 8 independent loads in flight per lane
+        uint32_t v[8];
+#pragma unroll
+        for (int h = 0; h < 8; h++) v[h] = c[i + h * stride];
+#pragma unroll
+        for (int h = 0; h < 8; h++) m = max(m, v[h]);
+    }
+    for (; i < n; i += stride) m = max(m, c[i]);
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, WAVE));
     if (lane_id() == 0) atomicMax(out, m);
 }
@@ -490,7 +504,7 @@ uint64_t cluster_levels(Absorber& a, const uint32_t* cl_dev) {
     require(cl_dev != nullptr, "a clustered variance matrix needs cluster codes");
     DBuf<uint32_t> mx(1);
     FH_HIP(hipMemsetAsync(mx.p, 0, sizeof(uint32_t), a.st));
-    hipLaunchKernelGGL(k_max_u32, dim3(1024), dim3(TPB), 0, a.st, cl_dev, a.n, mx.p);
+    hipLaunchKernelGGL(k_max_u32, dim3(grid_for(a.n, TPB * 8, 4096)), dim3(TPB), 0, a.st, cl_dev, a.n, mx.p);
     FH_CHECK_LAUNCH();
     uint32_t h = 0;
     d2h(&h, mx.p, 1, a.st);
@@ -516,7 +530,8 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     require(K >= 1 && K <= 21, "1..21 regressors are supported");
     a.rows_tmp.reserve(n * R);
     double* rows = a.rows_tmp.p;
-    hipLaunchKernelGGL(k_gather_rows, dim3(vgrid(n)), dim3(TPB), 0, st, y, X, K, a.pi.p, n, rows);
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(n * (K + 1), TPB, 1u << 16)), dim3(TPB), 0, st, y, X, K, a.pi.p,
+                       n, rows);
     FH_CHECK_LAUNCH();
     const int prec = o.precision < 0 ? a.precision : o.precision;
     ResidOut ro = residualize_any(a, rows, R, o.tol, o.maxiter, nullptr, nullptr, prec, ctx);

@@ -126,18 +126,39 @@ __global__ void k_cl_offsets(const uint64_t* __restrict__ key, size_t n, size_t 
 }
 
 // s_g = sum over the rows of cluster g (sorted order) of x_i e_i w_i; one thread per cluster.
-__global__ void k_cluster_scores(const uint32_t* __restrict__ off, const uint32_t* __restrict__ rows_sorted, size_t g,
-                                 const double* __restrict__ z, int ld, int c0, int k, const double* __restrict__ e,
-                                 const double* __restrict__ w, double* __restrict__ sg) {
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < g; q += (size_t)gridDim.x * blockDim.x) {
-        double acc[GRAM_KMAX];
-        for (int a = 0; a < k; a++) acc[a] = 0.0;
-        for (uint32_t j = off[q]; j < off[q + 1]; j++) {
+// sg[q] = sum over the rows i of cluster q of z[i][c0 .. c0+k) e_i w_i: L lanes per cluster
+// (its rows strided over the lanes, each a row gather of k values), then the lanes reduced by
+// shuffles in a fixed order.
+template <int KM>
+__global__ __launch_bounds__(TPB) void k_cluster_scores(const uint32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ rows_sorted, size_t g,
+                                                        const double* __restrict__ z, int ld, int c0, int k,
+                                                        const double* __restrict__ e, const double* __restrict__ w,
+                                                        int L, double* __restrict__ sg) {
+    const int l = lane_id();
+    const int gpw = WAVE / L, gl = l / L, li = l % L;
+    const size_t nwaves = (size_t)gridDim.x * (TPB / WAVE);
+    for (size_t base = ((size_t)blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE) * gpw; base < g; base += nwaves * gpw) {
+        const size_t q = base + gl;
+        const bool valid = q < g;
+        double acc[KM];
+#pragma unroll
+        for (int a = 0; a < KM; a++) acc[a] = 0.0;
+        const uint32_t j0 = valid ? off[q] : 0, j1 = valid ? off[q + 1] : 0;
+        for (uint32_t j = j0 + li; j < j1; j += L) {
             const size_t i = rows_sorted[j];
             const double s = e[i] * (w ? w[i] : 1.0);
-            for (int a = 0; a < k; a++) acc[a] += z[i * ld + c0 + a] * s;
+#pragma unroll
+            for (int a = 0; a < KM; a++)
+                if (a < k) acc[a] += z[i * ld + c0 + a] * s;
         }
-        for (int a = 0; a < k; a++) sg[q * k + a] = acc[a];
+#pragma unroll
+        for (int a = 0; a < KM; a++)
+            for (int o = L >> 1; o > 0; o >>= 1) acc[a] += __shfl_xor(acc[a], o, WAVE);
+        if (valid && li == 0)
+#pragma unroll
+            for (int a = 0; a < KM; a++)
+                if (a < k) sg[q * k + a] = acc[a];
     }
 }
 
@@ -244,8 +265,17 @@ uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, in
     DBuf<uint32_t> vals, off;
     cluster_sort(a, cl_kept_dev, G, vals, off);
     DBuf<double> sg(G * k);
-    hipLaunchKernelGGL(k_cluster_scores, dim3(grid_for(G, TPB, 4096)), dim3(TPB), 0, st, off.p, vals.p, (size_t)G, z,
-                       ld, c0, k, e, w, sg.p);
+    uint32_t L = 1;  // lanes per cluster: ~4 rows per lane
+    while (L < 64 && L * 4.0 < double(n) / double(std::max<uint64_t>(G, 1))) L <<= 1;
+    const unsigned cg = grid_for(G, (TPB / WAVE) * (WAVE / L), 1u << 16);
+    auto launch = [&](auto km) {
+        constexpr int KM = decltype(km)::value;
+        hipLaunchKernelGGL((k_cluster_scores<KM>), dim3(cg), dim3(TPB), 0, st, off.p, vals.p, (size_t)G, z, ld, c0, k,
+                           e, w, (int)L, sg.p);
+    };
+    if (k <= 4) launch(std::integral_constant<int, 4>{});
+    else if (k <= 12) launch(std::integral_constant<int, 12>{});
+    else launch(std::integral_constant<int, GRAM_KMAX>{});
     DBuf<unsigned long long> cnt(1);
     FH_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), st));
     hipLaunchKernelGGL(k_count_nonempty, dim3(grid_for(G, TPB, 4096)), dim3(TPB), 0, st, off.p, (size_t)G, cnt.p);

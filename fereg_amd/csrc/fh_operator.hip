@@ -110,11 +110,16 @@ void small_rhs(const Absorber& a, int b, const double* zc, int r, double* rhs_b,
         FH_CHECK_LAUNCH();
         sum_parts_to(part.p, nwg, (size_t)ms * r, rhs_b, st);
     };
-    if (r <= 1) go(std::integral_constant<int, 1>{}, std::integral_constant<int, 32>{});
-    else if (r <= 2) go(std::integral_constant<int, 2>{}, std::integral_constant<int, 32>{});
-    else if (r <= 4) go(std::integral_constant<int, 4>{}, std::integral_constant<int, 32>{});
-    else if (r <= 8) go(std::integral_constant<int, 8>{}, std::integral_constant<int, 32>{});
-    else go(std::integral_constant<int, 16>{}, std::integral_constant<int, 32>{});
+    auto by_r = [&](auto kc) {
+        if (r <= 1) go(std::integral_constant<int, 1>{}, kc);
+        else if (r <= 2) go(std::integral_constant<int, 2>{}, kc);
+        else if (r <= 4) go(std::integral_constant<int, 4>{}, kc);
+        else if (r <= 8) go(std::integral_constant<int, 8>{}, kc);
+        else go(std::integral_constant<int, 16>{}, kc);
+    };
+    if (ms <= 8) by_r(std::integral_constant<int, 8>{});
+    else if (ms <= 16) by_r(std::integral_constant<int, 16>{});
+    else by_r(std::integral_constant<int, 32>{});
 }
 
 }  // namespace

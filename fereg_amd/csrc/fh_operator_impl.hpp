@@ -63,9 +63,79 @@ __device__ __forceinline__ void ld_row(const T* __restrict__ src, int r, bool v,
             return;
         }
     }
+    // r < RM: the row start is only element-aligned, so the vector loads are 4-byte (FP32) or
+    // 8-byte (FP64) aligned; the hardware serves dword-aligned vector loads from global memory,
+    // so a row of 11 FP32 values is 4 loads (4 + 4 + 2 + 1) rather than 11
+    if constexpr (std::is_same<T, float>::value && RM % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < RM / 4; q++) {
+            const int j = 4 * q;
+            if (j + 4 <= r) {
+                float4 x;
+                __builtin_memcpy(&x, src + j, sizeof(float4));
+                row[j] = x.x; row[j + 1] = x.y; row[j + 2] = x.z; row[j + 3] = x.w;
+            } else if (j + 2 <= r) {
+                float2 x;
+                __builtin_memcpy(&x, src + j, sizeof(float2));
+                row[j] = x.x; row[j + 1] = x.y;
+                if (j + 3 <= r) row[j + 2] = src[j + 2];
+            } else if (j + 1 <= r) {
+                row[j] = src[j];
+            }
+        }
+        return;
+    } else if constexpr (std::is_same<T, double>::value && RM % 2 == 0) {
+#pragma unroll
+        for (int q = 0; q < RM / 2; q++) {
+            const int j = 2 * q;
+            if (j + 2 <= r) {
+                double2 x;
+                __builtin_memcpy(&x, src + j, sizeof(double2));
+                row[j] = x.x; row[j + 1] = x.y;
+            } else if (j + 1 <= r) {
+                row[j] = src[j];
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < RM; j++)
         if (j < r) row[j] = src[j];
+}
+
+// dst[j] = row[j] (j < r): the store counterpart of ld_row (16/8-byte vector stores when r == RM,
+// element-aligned vector chunks otherwise).
+template <class T, int RM>
+__device__ __forceinline__ void st_row(T* __restrict__ dst, int r, const T (&row)[RM]) {
+    if constexpr (std::is_same<T, double>::value && RM % 2 == 0) {
+#pragma unroll
+        for (int q = 0; q < RM / 2; q++) {
+            const int j = 2 * q;
+            if (j + 2 <= r) {
+                const double2 x = make_double2(row[j], row[j + 1]);
+                __builtin_memcpy(dst + j, &x, sizeof(double2));
+            } else if (j + 1 <= r) {
+                dst[j] = row[j];
+            }
+        }
+    } else if constexpr (std::is_same<T, float>::value && RM % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < RM / 4; q++) {
+            const int j = 4 * q;
+            if (j + 4 <= r) {
+                const float4 x = make_float4(row[j], row[j + 1], row[j + 2], row[j + 3]);
+                __builtin_memcpy(dst + j, &x, sizeof(float4));
+            } else {
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    if (j + i < r) dst[j + i] = row[j + i];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < RM; j++)
+            if (j < r) dst[j] = row[j];
+    }
 }
 
 // Unroll depth of the hoisted passes: positions per lane whose gathers are in flight at once,
@@ -583,14 +653,15 @@ __device__ __forceinline__ void cell_sums(const OpArgs<double>& A, const double*
     const size_t r0 = A.cell_obs_off ? A.cell_obs_off[c] : c, r1 = A.cell_obs_off ? A.cell_obs_off[c + 1] : c + 1;
     for (size_t i = r0; i < r1; i++) {
         const double w = wr ? wr[i] : 1.0;
+        double v[RM];
+        ld_row<double, RM>(rows + i * r, r, true, v);
 #pragma unroll
         for (int j = 0; j < RM; j++)
             if (j < r) {
-                const double v = rows[i * r + j];
-                const double wv = w * v;
+                const double wv = w * v[j];
                 sc[j] += wv;
                 if (stats) {
-                    wsq[j] += wv * v;
+                    wsq[j] += wv * v[j];
                     wsum[j] += wv;
                 }
             }
@@ -771,10 +842,13 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
                 if (c < c1) {
                     const size_t r0 = A.cell_obs_off ? A.cell_obs_off[c] : c;
                     const size_t r1 = A.cell_obs_off ? A.cell_obs_off[c + 1] : c + 1;
-                    for (size_t i = r0; i < r1; i++)
+                    for (size_t i = r0; i < r1; i++) {
+                        double v[RM];
+                        ld_row<double, RM>(rows + i * r, r, true, v);
 #pragma unroll
-                        for (int j = 0; j < RM; j++)
-                            if (j < r) rows[i * r + j] -= uk[h][j] + av[j];
+                        for (int j = 0; j < RM; j++) v[j] -= uk[h][j] + av[j];
+                        st_row<double, RM>(rows + i * r, r, v);
+                    }
                 }
             }
         }
@@ -833,9 +907,10 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
                         cell_sums<RM>(A, rows, wr, c, r, sk[h], d0, d1, false);
                     }
                     const double f = (A.cw ? A.cw[c] : 1.0) * inv;
+                    double zv[RM];
 #pragma unroll
-                    for (int j = 0; j < RM; j++)
-                        if (j < r) zc[(size_t)c * r + j] = sk[h][j] - f * bv[j];
+                    for (int j = 0; j < RM; j++) zv[j] = sk[h][j] - f * bv[j];
+                    st_row<double, RM>(zc + (size_t)c * r, r, zv);
                 }
             }
         }

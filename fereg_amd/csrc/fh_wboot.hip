@@ -317,7 +317,7 @@ __global__ void k_wb_cell_span(const uint32_t* __restrict__ coo, const uint32_t*
 }
 
 // *bad |= some group (cells off[q]..off[q+1], through perm when given) spans clusters; one
-// flag write per workgroup. Groups: FE1 levels (off = seg_off) or (bucket, level) items.
+// flag write per workgroup. Used for the FE1 levels (off = seg_off, contiguous cells).
 __global__ __launch_bounds__(TPB) void k_wb_group_span(const uint32_t* __restrict__ off,
                                                        const uint32_t* __restrict__ perm, const uint2* __restrict__ span,
                                                        size_t ngroups, uint2* __restrict__ gspan,
@@ -341,19 +341,57 @@ __global__ __launch_bounds__(TPB) void k_wb_group_span(const uint32_t* __restric
     if (threadIdx.x == 0 && any) atomicOr(bad, 1u);
 }
 
-// Levels cut into nbk buckets: combine the item spans of each level.
-__global__ __launch_bounds__(TPB) void k_wb_level_span(const uint2* __restrict__ gspan, uint32_t nbk, size_t g,
-                                                       uint32_t* __restrict__ bad) {
-    bool b = false;
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
-        uint32_t mn = 0xFFFFFFFFu, mx = 0u;
-        for (uint32_t k = 0; k < nbk; k++) {
-            const uint2 s = gspan[(size_t)k * g + l];
-            mn = min(mn, s.x);
-            mx = max(mx, s.y);
+// Per level of block b: the smallest and largest cluster over its cells (integer atomics: the
+// result does not depend on their order). Blocks of up to SPAN_LDS_G levels are reduced in LDS
+// per workgroup first (a small block's few levels would otherwise take every atomic).
+constexpr size_t SPAN_LDS_G = 4096;
+__global__ __launch_bounds__(TPB) void k_code_span(const uint32_t* __restrict__ cell_codes, int nb, int b,
+                                                   const uint2* __restrict__ span, size_t M, size_t g,
+                                                   uint32_t* __restrict__ lmin, uint32_t* __restrict__ lmax) {
+    __shared__ uint32_t smin[SPAN_LDS_G], smax[SPAN_LDS_G];
+    const bool lds = g <= SPAN_LDS_G;
+    if (lds) {
+        for (size_t l = threadIdx.x; l < g; l += TPB) {
+            smin[l] = 0xFFFFFFFFu;
+            smax[l] = 0u;
         }
-        b |= mn < mx;
+        __syncthreads();
     }
+    for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < M; c += (size_t)gridDim.x * TPB) {
+        const uint32_t lv = cell_codes[c * nb + b];
+        const uint2 sp = span[c];
+        if (lds) {
+            atomicMin(&smin[lv], sp.x);
+            atomicMax(&smax[lv], sp.y);
+        } else {
+            atomicMin(&lmin[lv], sp.x);
+            atomicMax(&lmax[lv], sp.y);
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (size_t l = threadIdx.x; l < g; l += TPB)
+            if (smin[l] != 0xFFFFFFFFu) {
+                atomicMin(&lmin[l], smin[l]);
+                atomicMax(&lmax[l], smax[l]);
+            }
+    }
+}
+
+__global__ void k_fill_span(uint32_t* __restrict__ lmin, uint32_t* __restrict__ lmax, size_t g) {
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
+        lmin[l] = 0xFFFFFFFFu;
+        lmax[l] = 0u;
+    }
+}
+
+// *bad |= some level spans clusters (levels without cells have min > max).
+__global__ __launch_bounds__(TPB) void k_span_check(const uint32_t* __restrict__ lmin,
+                                                    const uint32_t* __restrict__ lmax, size_t g,
+                                                    uint32_t* __restrict__ bad) {
+    bool b = false;
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x)
+        b |= lmin[l] < lmax[l];
     __shared__ int any;
     if (threadIdx.x == 0) any = 0;
     __syncthreads();
@@ -521,16 +559,14 @@ std::vector<int> terms_nested_in(Absorber& a, const uint32_t* cl) {
     if (a.fe1 >= 0)
         hipLaunchKernelGGL(k_wb_group_span, dim3(grid_for(a.g1, TPB, 4096)), dim3(TPB), 0, st, a.seg_off.p,
                            (const uint32_t*)nullptr, span.p, (size_t)a.g1, (uint2*)nullptr, bad.p);
-    DBuf<uint2> gspan;
     for (size_t b = 0; b < a.nb; b++) {
-        ensure_block_transpose(a, (int)b);
-        const size_t items = (size_t)a.nbk[b] * a.blocks[b].g;
-        if (a.nbk[b] > 1) gspan.reserve(items);
-        hipLaunchKernelGGL(k_wb_group_span, dim3(grid_for(items, TPB, 4096)), dim3(TPB), 0, st, a.boff[b].p,
-                           a.bperm[b].p, span.p, items, a.nbk[b] > 1 ? gspan.p : nullptr, bad.p + 1 + b);
-        if (a.nbk[b] > 1)
-            hipLaunchKernelGGL(k_wb_level_span, dim3(grid_for(a.blocks[b].g, TPB, 4096)), dim3(TPB), 0, st, gspan.p,
-                               a.nbk[b], (size_t)a.blocks[b].g, bad.p + 1 + b);
+        const size_t g = a.blocks[b].g;
+        DBuf<uint32_t> lmin(g), lmax(g);
+        hipLaunchKernelGGL(k_fill_span, dim3(grid_for(g, TPB, 4096)), dim3(TPB), 0, st, lmin.p, lmax.p, g);
+        hipLaunchKernelGGL(k_code_span, dim3(grid_for(a.M, TPB, g <= SPAN_LDS_G ? 1024 : 8192)), dim3(TPB), 0, st,
+                           a.cell_codes.p, (int)a.nb, (int)b, span.p, (size_t)a.M, g, lmin.p, lmax.p);
+        hipLaunchKernelGGL(k_span_check, dim3(grid_for(g, TPB, 4096)), dim3(TPB), 0, st, lmin.p, lmax.p, g,
+                           bad.p + 1 + b);
     }
     FH_CHECK_LAUNCH();
     std::vector<uint32_t> hb(nflag);
