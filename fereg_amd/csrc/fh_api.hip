@@ -80,7 +80,14 @@ __global__ void k_max_u32(const uint32_t* __restrict__ c, size_t n, uint32_t* ou
     }
     for (; i < n; i += stride) m = max(m, c[i]);
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, WAVE));
-    if (lane_id() == 0) atomicMax(out, m);
+    __shared__ uint32_t wm[TPB / WAVE];
+    if (lane_id() == 0) wm[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // one atomic per workgroup
+        uint32_t b = 0;
+        for (int w = 0; w < TPB / WAVE; w++) b = max(b, wm[w]);
+        atomicMax(out, b);
+    }
 }
 __global__ void k_check_y(const double* __restrict__ y, size_t n, uint32_t* flag, double* part) {
     bool bad = false;
@@ -263,36 +270,39 @@ struct ResidOut {
     uint64_t refinements = 0;
 };
 
-// One residualization of a device block of r <= 16 columns (solve.rs:1141-1311).
+// One residualization of a device block of r <= 16 columns (solve.rs:1141-1311). The solver's
+// vectors carry rp = padded_cols(r) columns (zero, inactive pads) so that their rows are whole
+// vectors; the caller's block, warm start and coefficients keep r columns.
 ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t maxiter, const double* warm_dev,
                            double* coefs_dev, int precision, const fh_ctx* ctx) {
     hipStream_t st = a.st;
     const size_t m = a.m, g1 = a.g1;
+    const int rp = padded_cols(r);
     ResidOut out;
-    a.b1.reserve(g1 * r);
-    a.s64.rhs.reserve(std::max<size_t>(m * r, 1));
-    a.s64.x.reserve(std::max<size_t>(m * r, 1));
+    a.b1.reserve(g1 * rp);
+    a.s64.rhs.reserve(std::max<size_t>(m * rp, 1));
+    a.s64.x.reserve(std::max<size_t>(m * rp, 1));
     // ss0 = wsq - wsum^2/sw (0 if <= 1e-15 wsq), ss1 = wsq - sum_s b1^2/n1 (solve.rs:1210-1217);
     // the partials of wsq / wsum (rhs pass) and of sum b1^2/n1 are read back in one transfer
     const unsigned pg = 256;
-    const unsigned grid = rhs_pass(a, z, r, a.s64.rhs.p, a.b1.p, (size_t)pg * r);
-    const size_t nws = (size_t)grid * 2 * r;
-    hipLaunchKernelGGL(k_wsq_b1, dim3(pg), dim3(TPB), 0, st, a.b1.p, a.n1.p, g1, r, a.s64.dpart.p + nws);
+    const unsigned grid = rhs_pass(a, z, r, rp, a.s64.rhs.p, a.b1.p, (size_t)pg * rp);
+    const size_t nws = (size_t)grid * 2 * rp;
+    hipLaunchKernelGGL(k_wsq_b1, dim3(pg), dim3(TPB), 0, st, a.b1.p, a.n1.p, g1, rp, a.s64.dpart.p + nws);
     FH_CHECK_LAUNCH();
     // one fixed-order device reduction for each, one readback
-    DBuf<double> sums(3 * (size_t)r);
-    sum_parts_to(a.s64.dpart.p, grid, (size_t)(2 * r), sums.p, st);
-    sum_parts_to(a.s64.dpart.p + nws, pg, (size_t)r, sums.p + 2 * r, st);
-    std::vector<double> hs(3 * (size_t)r);
+    DBuf<double> sums(3 * (size_t)rp);
+    sum_parts_to(a.s64.dpart.p, grid, (size_t)(2 * rp), sums.p, st);
+    sum_parts_to(a.s64.dpart.p + nws, pg, (size_t)rp, sums.p + 2 * rp, st);
+    std::vector<double> hs(3 * (size_t)rp);
     d2h(hs.data(), sums.p, hs.size(), st);
     check_interrupt(ctx);
-    const std::vector<double> ws(hs.begin(), hs.begin() + 2 * r);
-    const double* hp = hs.data() + 2 * r;
-    std::vector<double> ss0(r), ss1(r);
-    std::vector<int32_t> zero(r);
-    for (int j = 0; j < r; j++) {
+    const std::vector<double> ws(hs.begin(), hs.begin() + 2 * rp);
+    const double* hp = hs.data() + 2 * rp;
+    std::vector<double> ss0(rp), ss1(rp);
+    std::vector<int32_t> zero(rp);
+    for (int j = 0; j < rp; j++) {  // pad columns: all sums zero, so ss0 = 0 (inactive)
         const double b1sq = hp[j];
-        const double wsq = ws[j], wsum = ws[r + j];
+        const double wsq = ws[j], wsum = ws[rp + j];
         const double v = wsq - wsum * wsum / a.sw;
         ss0[j] = v > 1e-15 * wsq ? v : 0.0;
         ss1[j] = wsq - b1sq;
@@ -300,16 +310,23 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
     }
     double* x = a.s64.x.p;
     if (m > 0) {
-        if (warm_dev) FH_HIP(hipMemcpyAsync(x, warm_dev, m * r * sizeof(double), hipMemcpyDeviceToDevice, st));
-        else FH_HIP(hipMemsetAsync(x, 0, m * r * sizeof(double), st));
-        DBuf<int32_t> dz(r);
-        h2d(dz.p, zero.data(), r, st);
-        hipLaunchKernelGGL(k_zero_cols, dim3(vgrid(m * r)), dim3(TPB), 0, st, x, m, r, dz.p);
+        if (warm_dev && rp == r) {
+            FH_HIP(hipMemcpyAsync(x, warm_dev, m * r * sizeof(double), hipMemcpyDeviceToDevice, st));
+        } else {
+            FH_HIP(hipMemsetAsync(x, 0, m * rp * sizeof(double), st));
+            if (warm_dev) {
+                hipLaunchKernelGGL(k_cols_in, dim3(vgrid(m * r)), dim3(TPB), 0, st, x, m, rp, 0, r, warm_dev);
+                FH_CHECK_LAUNCH();
+            }
+        }
+        DBuf<int32_t> dz(rp);
+        h2d(dz.p, zero.data(), rp, st);
+        hipLaunchKernelGGL(k_zero_cols, dim3(vgrid(m * rp)), dim3(TPB), 0, st, x, m, rp, dz.p);
         FH_CHECK_LAUNCH();
         const bool warm = warm_dev != nullptr;
         PcgResult pr = precision == FH_MIXED
-                           ? pcg_mixed(a, a.s64.rhs.p, x, warm, r, ss1.data(), ss0.data(), tol, maxiter, ctx)
-                           : pcg_fp64(a, a.s64.rhs.p, x, warm, r, ss1.data(), ss0.data(), tol, maxiter, ctx);
+                           ? pcg_mixed(a, a.s64.rhs.p, x, warm, rp, ss1.data(), ss0.data(), tol, maxiter, ctx)
+                           : pcg_fp64(a, a.s64.rhs.p, x, warm, rp, ss1.data(), ss0.data(), tol, maxiter, ctx);
         if (pr.rc == FH_CONVERGENCE)
             throw Error(FH_CONVERGENCE, "the FE solver did not converge within " + std::to_string(maxiter) + " iterations");
         out.iterations = std::max<uint64_t>(pr.iterations, 1);
@@ -318,8 +335,21 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
         out.exact = pr.exact || allconst;
         out.refinements = pr.refinements;
     }
-    residualize_rows(a, z, r, x, a.b1.p, coefs_dev ? coefs_dev + m * r : nullptr);
-    if (coefs_dev && m > 0) FH_HIP(hipMemcpyAsync(coefs_dev, x, m * r * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (rp == r) {
+        residualize_rows(a, z, r, rp, x, a.b1.p, coefs_dev ? coefs_dev + m * r : nullptr);
+        if (coefs_dev && m > 0) FH_HIP(hipMemcpyAsync(coefs_dev, x, m * r * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+        DBuf<double> alpha;
+        if (coefs_dev) alpha.alloc(std::max<size_t>(g1 * rp, 1));
+        residualize_rows(a, z, r, rp, x, a.b1.p, coefs_dev ? alpha.p : nullptr);
+        if (coefs_dev) {  // FeCoefs [x; alpha1] with the caller's r columns
+            if (m > 0)
+                hipLaunchKernelGGL(k_cols_out, dim3(vgrid(m * r)), dim3(TPB), 0, st, x, m, rp, 0, r, coefs_dev);
+            hipLaunchKernelGGL(k_cols_out, dim3(vgrid(g1 * r)), dim3(TPB), 0, st, alpha.p, g1, rp, 0, r,
+                               coefs_dev + m * r);
+            FH_CHECK_LAUNCH();
+        }
+    }
     return out;
 }
 
@@ -504,7 +534,7 @@ uint64_t cluster_levels(Absorber& a, const uint32_t* cl_dev) {
     require(cl_dev != nullptr, "a clustered variance matrix needs cluster codes");
     DBuf<uint32_t> mx(1);
     FH_HIP(hipMemsetAsync(mx.p, 0, sizeof(uint32_t), a.st));
-    hipLaunchKernelGGL(k_max_u32, dim3(grid_for(a.n, TPB * 8, 4096)), dim3(TPB), 0, a.st, cl_dev, a.n, mx.p);
+    hipLaunchKernelGGL(k_max_u32, dim3(grid_for(a.n, TPB * 8, 1024)), dim3(TPB), 0, a.st, cl_dev, a.n, mx.p);
     FH_CHECK_LAUNCH();
     uint32_t h = 0;
     d2h(&h, mx.p, 1, a.st);

@@ -118,12 +118,12 @@ struct Absorber {
     bool small_dense = false;
     size_t sm_ld = 0, sm_coff = 0;  // pitch of E and its first C column (multiples of 4)
 
-    // S*p plans per precision (FP32, FP64) and column class (r <= 1, 2, 4, 8, 16); with autotune
-    // the first S*p of a class times the candidates (autotune_spmv)
+    // S*p plans per precision (FP32, FP64) and column class (r <= 1, 2, 4, 8, 12, 16); with
+    // autotune the first S*p of a class times the candidates (autotune_spmv)
     bool autotune = false;
-    mutable SpmvPlan plans[2][5];
+    mutable SpmvPlan plans[2][6];
     SpmvPlan& plan_for(bool fp64, int r) const {
-        const int c = r <= 1 ? 0 : r <= 2 ? 1 : r <= 4 ? 2 : r <= 8 ? 3 : 4;
+        const int c = r <= 1 ? 0 : r <= 2 ? 1 : r <= 4 ? 2 : r <= 8 ? 3 : r <= 12 ? 4 : 5;
         return plans[fp64 ? 1 : 0][c];
     }
 
@@ -207,14 +207,20 @@ void tune_cache_clear();
 // rhs pass over rows (n x r, FP64): rhs (m x r, T), b1 (g1 x r), wsq/wsum partials -> out2 (2r).
 // Leaves the per-workgroup partials of wsq / wsum (grid x 2r, returns grid) at the start of
 // a.s64.dpart, which holds room for `extra` more doubles after them.
-unsigned rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, size_t extra);
-// rows -= alpha_s + u_c; alpha (g1 x r) optional.
-void residualize_rows(Absorber& a, double* rows, int r, const double* x, const double* b1, double* alpha);
+// rows: n x r; rhs (m x rp) and b1 (g1 x rp) with zero pad columns r..rp-1.
+unsigned rhs_pass(Absorber& a, const double* rows, int r, int rp, double* rhs, double* b1, size_t extra);
+// rows (n x r) -= alpha_s + u_c, with x (m x rp) and b1, alpha (g1 x rp; alpha optional).
+void residualize_rows(Absorber& a, double* rows, int r, int rp, const double* x, const double* b1, double* alpha);
 
 // Fixed-order device reduction of per-workgroup partials part[q * len + e] (q < nparts) read
 // back as len values (fh_estimate.hip).
 std::vector<double> sum_parts_dev(const double* part, size_t nparts, size_t len, hipStream_t st);
 void sum_parts_to(const double* part, size_t nparts, size_t len, double* out_dev, hipStream_t st);
+
+// Column count the solver's m x r vectors are padded to: the register classes of the kernels,
+// so every row of p, t and x is whole 16-byte vectors (r <= 2 rows are 8 or 16 bytes already).
+// The pad columns are zero and inactive.
+inline int padded_cols(int r) { return r <= 2 ? r : r <= 4 ? 4 : r <= 8 ? 8 : r <= 12 ? 12 : 16; }
 
 // ---------------------------------------------------------------- PCG (fh_pcg.hip)
 struct PcgResult {

@@ -124,23 +124,23 @@ void small_rhs(const Absorber& a, int b, const double* zc, int r, double* rhs_b,
 
 }  // namespace
 
-unsigned rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b1, size_t extra) {
-    require(r >= 1 && r <= 16, "rhs: 1..16 columns");
+unsigned rhs_pass(Absorber& a, const double* rows, int r, int rp, double* rhs, double* b1, size_t extra) {
+    require(r >= 1 && r <= rp && rp <= 16, "rhs: 1..16 columns");
     hipStream_t st = a.st;
     OpArgs<double> A = op_args_impl<double>(a);
     const unsigned grid = seg_grid(a);
     Scratch<double>& s = a.s64;
-    s.dpart.reserve((size_t)grid * 2 * r + extra);
-    if (a.nb > 0) s.zc.reserve(a.M * r);
+    s.dpart.reserve((size_t)grid * 2 * rp + extra);
+    if (a.nb > 0) s.zc.reserve(a.M * rp);
     const int L = (int)a.seg_lanes;
     const double* wr = a.weighted ? a.w_rows.p : nullptr;
-    FH_DISPATCH_R(r, hipLaunchKernelGGL((k_rhs_seg_reg<RM>), dim3(grid), dim3(WG), 0, st, A, rows, wr, r, L, b1,
-                                        a.nb > 0 ? s.zc.p : nullptr, s.dpart.p));
+    FH_DISPATCH_R(rp, hipLaunchKernelGGL((k_rhs_seg_reg<RM>), dim3(grid), dim3(WG), 0, st, A, rows, wr, r, rp, L, b1,
+                                         a.nb > 0 ? s.zc.p : nullptr, s.dpart.p));
     FH_CHECK_LAUNCH();
     for (size_t b = 0; b < a.nb; b++) {
         const BlockInfo& bi = a.blocks[b];
         if (a.boff[b].n == 0) {  // the dense small block: binned by code, no level pass
-            small_rhs(a, (int)b, s.zc.p, r, rhs + bi.off * r, st);
+            small_rhs(a, (int)b, s.zc.p, rp, rhs + bi.off * rp, st);
             continue;
         }
         const int Ll = (int)a.blanes[b];
@@ -148,31 +148,31 @@ unsigned rhs_pass(Absorber& a, const double* rows, int r, double* rhs, double* b
         const size_t items = (size_t)nbk * bi.g;
         double* ypart = nullptr;
         if (nbk > 1) {
-            s.ypart.reserve(items * r);
+            s.ypart.reserve(items * rp);
             ypart = s.ypart.p;
         }
-        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_lvl_pull<RM>), dim3(lvl_grid(Ll, items)), dim3(WG), 0, st, a.boff[b].p,
-                                            a.bperm[b].p, items, (uint32_t)bi.g, s.zc.p, r, Ll, rhs + bi.off * r,
-                                            ypart));
+        FH_DISPATCH_R(rp, hipLaunchKernelGGL((k_lvl_pull<RM>), dim3(lvl_grid(Ll, items)), dim3(WG), 0, st,
+                                             a.boff[b].p, a.bperm[b].p, items, (uint32_t)bi.g, s.zc.p, rp, Ll,
+                                             rhs + bi.off * rp, ypart));
         FH_CHECK_LAUNCH();
-        if (ypart) bucket_reduce<double>(ypart, nbk, (size_t)bi.g, r, rhs + bi.off * r, nullptr, st);
+        if (ypart) bucket_reduce<double>(ypart, nbk, (size_t)bi.g, rp, rhs + bi.off * rp, nullptr, st);
     }
     return grid;
 }
 
-void residualize_rows(Absorber& a, double* rows, int r, const double* x, const double* b1, double* alpha) {
-    require(r >= 1 && r <= 16, "residualize: 1..16 columns");
+void residualize_rows(Absorber& a, double* rows, int r, int rp, const double* x, const double* b1, double* alpha) {
+    require(r >= 1 && r <= rp && rp <= 16, "residualize: 1..16 columns");
     OpArgs<double> A = op_args_impl<double>(a);
     const unsigned grid = seg_grid(a);
     const int L = (int)a.seg_lanes;
     if (a.nb <= 4) {
-        FH_DISPATCH_R(r, with_nb0((int)a.nb, [&](auto nbc) {
+        FH_DISPATCH_R(rp, with_nb0((int)a.nb, [&](auto nbc) {
             constexpr int NB = decltype(nbc)::value;
-            hipLaunchKernelGGL((k_resid_seg<RM, NB>), dim3(grid), dim3(WG), 0, a.st, A, rows, r, L, x, b1, alpha);
+            hipLaunchKernelGGL((k_resid_seg<RM, NB>), dim3(grid), dim3(WG), 0, a.st, A, rows, r, rp, L, x, b1, alpha);
         }));
     } else {
-        FH_DISPATCH_R(r, hipLaunchKernelGGL((k_resid_seg_any<RM>), dim3(grid), dim3(WG), 0, a.st, A, rows, r, L, x, b1,
-                                            alpha));
+        FH_DISPATCH_R(rp, hipLaunchKernelGGL((k_resid_seg_any<RM>), dim3(grid), dim3(WG), 0, a.st, A, rows, r, rp, L, x,
+                                             b1, alpha));
     }
     FH_CHECK_LAUNCH();
 }

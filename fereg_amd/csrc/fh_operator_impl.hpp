@@ -33,7 +33,10 @@ __device__ __forceinline__ T ldnt(const T* p) { return __builtin_nontemporal_loa
 // row[j] = src[j] (j < r) for a valid lane, else 0. When r == RM the row is read with 16-byte
 // (or 8-byte) vector loads: one gather instruction per row instead of one per column, which
 // matters because random gathers are bounded by instructions x distinct lines, not by bytes.
-template <class T, int RM>
+// UA: rows with r < RM are read in element-aligned vector chunks. That pays for rows read
+// in sequence (each line is used whole) and costs for random row gathers (a chunk that crosses a
+// line boundary turns into two requests), so the gathers keep per-element loads.
+template <class T, int RM, bool UA = false>
 __device__ __forceinline__ void ld_row(const T* __restrict__ src, int r, bool v, T (&row)[RM]) {
 #pragma unroll
     for (int j = 0; j < RM; j++) row[j] = T(0);
@@ -66,7 +69,7 @@ __device__ __forceinline__ void ld_row(const T* __restrict__ src, int r, bool v,
     // r < RM: the row start is only element-aligned, so the vector loads are 4-byte (FP32) or
     // 8-byte (FP64) aligned; the hardware serves dword-aligned vector loads from global memory,
     // so a row of 11 FP32 values is 4 loads (4 + 4 + 2 + 1) rather than 11
-    if constexpr (std::is_same<T, float>::value && RM % 4 == 0) {
+    if constexpr (UA && std::is_same<T, float>::value && RM % 4 == 0) {
 #pragma unroll
         for (int q = 0; q < RM / 4; q++) {
             const int j = 4 * q;
@@ -84,7 +87,7 @@ __device__ __forceinline__ void ld_row(const T* __restrict__ src, int r, bool v,
             }
         }
         return;
-    } else if constexpr (std::is_same<T, double>::value && RM % 2 == 0) {
+    } else if constexpr (UA && std::is_same<T, double>::value && RM % 2 == 0) {
 #pragma unroll
         for (int q = 0; q < RM / 2; q++) {
             const int j = 2 * q;
@@ -654,7 +657,7 @@ __device__ __forceinline__ void cell_sums(const OpArgs<double>& A, const double*
     for (size_t i = r0; i < r1; i++) {
         const double w = wr ? wr[i] : 1.0;
         double v[RM];
-        ld_row<double, RM>(rows + i * r, r, true, v);
+        ld_row<double, RM, true>(rows + i * r, r, true, v);
 #pragma unroll
         for (int j = 0; j < RM; j++)
             if (j < r) {
@@ -716,8 +719,8 @@ __global__ __launch_bounds__(WG) void k_lvl_pull(const uint32_t* __restrict__ lo
 // ---------------------------------------------------------------- back-substitution (FP64)
 // Any block count (runtime loop over the blocks, u gathered twice).
 template <int RM>
-__global__ __launch_bounds__(WG) void k_resid_seg_any(OpArgs<double> A, double* __restrict__ rows, int r, int L,
-                                                  const double* __restrict__ x, const double* __restrict__ b1,
+__global__ __launch_bounds__(WG) void k_resid_seg_any(OpArgs<double> A, double* __restrict__ rows, int r, int rp,
+                                                  int L, const double* __restrict__ x, const double* __restrict__ b1,
                                                   double* __restrict__ alpha) {
     const int w = threadIdx.x / WAVE, l = lane_id();
     const int spw = WAVE / L, gl = l / L, li = l % L;
@@ -734,7 +737,7 @@ __global__ __launch_bounds__(WG) void k_resid_seg_any(OpArgs<double> A, double* 
             const uint32_t c = c0 + it * L + li;
             if (c < c1 && A.nb > 0) {
                 double u[RM];
-                gather_u<double, RM>(A, x, c, r, u);
+                gather_u<double, RM>(A, x, c, rp, u);
                 const double wc = A.cw ? A.cw[c] : 1.0;
 #pragma unroll
                 for (int j = 0; j < RM; j++) av[j] += wc * u[j];
@@ -744,16 +747,16 @@ __global__ __launch_bounds__(WG) void k_resid_seg_any(OpArgs<double> A, double* 
         for (int j = 0; j < RM; j++) av[j] = gsum(av[j], L);
         const double inv = sv ? A.inv_n1[s] : 0.0;
 #pragma unroll
-        for (int j = 0; j < RM; j++) av[j] = sv && j < r ? (b1[s * r + j] - av[j]) * inv : 0.0;
+        for (int j = 0; j < RM; j++) av[j] = sv && j < rp ? (b1[s * rp + j] - av[j]) * inv : 0.0;
         if (alpha && sv && li == 0)
 #pragma unroll
             for (int j = 0; j < RM; j++)
-                if (j < r) alpha[s * r + j] = av[j];
+                if (j < rp) alpha[s * rp + j] = av[j];
         for (uint32_t it = 0; it < tmax; it++) {
             const uint32_t c = c0 + it * L + li;
             if (c < c1) {
                 double u[RM];
-                if (A.nb > 0) gather_u<double, RM>(A, x, c, r, u);
+                if (A.nb > 0) gather_u<double, RM>(A, x, c, rp, u);
                 else
 #pragma unroll
                     for (int j = 0; j < RM; j++) u[j] = 0.0;
@@ -775,7 +778,7 @@ __global__ __launch_bounds__(WG) void k_resid_seg_any(OpArgs<double> A, double* 
 // when a segment fits one trip (the common case), so x is gathered once per cell.
 
 template <int RM, int NB>
-__global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __restrict__ rows, int r, int L,
+__global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __restrict__ rows, int r, int rp, int L,
                                                   const double* __restrict__ x, const double* __restrict__ b1,
                                                   double* __restrict__ alpha) {
     constexpr int U = gather_trips<double, RM>();
@@ -808,7 +811,7 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
                     double row[RM];
-                    ld_row<double, RM>(x + (size_t)(A.boff[b] + code[h][b]) * r, r, v, row);
+                    ld_row<double, RM>(x + (size_t)(A.boff[b] + code[h][b]) * rp, rp, v, row);
 #pragma unroll
                     for (int j = 0; j < RM; j++) u[h][j] += row[j];
                 }
@@ -826,11 +829,11 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
         for (int j = 0; j < RM; j++) av[j] = gsum(av[j], L);
         const double inv = sv ? A.inv_n1[s] : 0.0;
 #pragma unroll
-        for (int j = 0; j < RM; j++) av[j] = sv && j < r ? (b1[s * r + j] - av[j]) * inv : 0.0;
+        for (int j = 0; j < RM; j++) av[j] = sv && j < rp ? (b1[s * rp + j] - av[j]) * inv : 0.0;
         if (alpha && sv && li == 0)
 #pragma unroll
             for (int j = 0; j < RM; j++)
-                if (j < r) alpha[s * r + j] = av[j];
+                if (j < rp) alpha[s * rp + j] = av[j];
         for (uint32_t it = 0; it < tmax; it += U) {
             if (tmax > U) {
                 double wc[U];
@@ -844,7 +847,7 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
                     const size_t r1 = A.cell_obs_off ? A.cell_obs_off[c + 1] : c + 1;
                     for (size_t i = r0; i < r1; i++) {
                         double v[RM];
-                        ld_row<double, RM>(rows + i * r, r, true, v);
+                        ld_row<double, RM, true>(rows + i * r, r, true, v);
 #pragma unroll
                         for (int j = 0; j < RM; j++) v[j] -= uk[h][j] + av[j];
                         st_row<double, RM>(rows + i * r, r, v);
@@ -860,8 +863,9 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
 // one trip, so every row is read once.
 template <int RM>
 __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const double* __restrict__ rows,
-                                                    const double* __restrict__ wr, int r, int L, double* __restrict__ b1,
-                                                    double* __restrict__ zc, double* __restrict__ spart) {
+                                                    const double* __restrict__ wr, int r, int rp, int L,
+                                                    double* __restrict__ b1, double* __restrict__ zc,
+                                                    double* __restrict__ spart) {
     constexpr int U = gather_trips<double, RM>();
     __shared__ double red[WAVES][2 * 16];
     const int w = threadIdx.x / WAVE, l = lane_id();
@@ -894,7 +898,7 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
         if (sv && li == 0)
 #pragma unroll
             for (int j = 0; j < RM; j++)
-                if (j < r) b1[s * r + j] = bv[j];
+                if (j < rp) b1[s * rp + j] = bv[j];
         if (!zc) continue;
         const double inv = sv ? A.inv_n1[s] : 0.0;
         for (uint32_t it = 0; it < tmax; it += U) {
@@ -910,7 +914,7 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
                     double zv[RM];
 #pragma unroll
                     for (int j = 0; j < RM; j++) zv[j] = sk[h][j] - f * bv[j];
-                    st_row<double, RM>(zc + (size_t)c * r, r, zv);
+                    st_row<double, RM>(zc + (size_t)c * rp, rp, zv);
                 }
             }
         }
@@ -918,14 +922,14 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
 #pragma unroll
     for (int j = 0; j < RM; j++) {
         double a = wave_sum(wsq[j]), b = wave_sum(wsum[j]);
-        if (l == 0 && j < r) { red[w][j] = a; red[w][16 + j] = b; }
+        if (l == 0 && j < rp) { red[w][j] = a; red[w][16 + j] = b; }
     }
     __syncthreads();
-    if (threadIdx.x < (unsigned)r) {
+    if (threadIdx.x < (unsigned)rp) {
         double a = 0.0, b = 0.0;
         for (int ww = 0; ww < WAVES; ww++) { a += red[ww][threadIdx.x]; b += red[ww][16 + threadIdx.x]; }
-        spart[(size_t)blockIdx.x * 2 * r + threadIdx.x] = a;
-        spart[(size_t)blockIdx.x * 2 * r + r + threadIdx.x] = b;
+        spart[(size_t)blockIdx.x * 2 * rp + threadIdx.x] = a;
+        spart[(size_t)blockIdx.x * 2 * rp + rp + threadIdx.x] = b;
     }
 }
 
@@ -965,6 +969,7 @@ inline OpArgs<T> op_args_impl(const Absorber& a) {
         else if ((r) <= 2) { constexpr int RM = 2; __VA_ARGS__; }   \
         else if ((r) <= 4) { constexpr int RM = 4; __VA_ARGS__; }   \
         else if ((r) <= 8) { constexpr int RM = 8; __VA_ARGS__; }   \
+        else if ((r) <= 12) { constexpr int RM = 12; __VA_ARGS__; } \
         else { constexpr int RM = 16; __VA_ARGS__; }               \
     } while (0)
 
@@ -1160,7 +1165,7 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
 // choice is cached per structure signature, so later absorbers of the same shape skip the timing.
 template <class T>
 void autotune_spmv(const Absorber& a, SpmvPlan& plan, const T* p, T* y, int r, Scratch<T>& s, hipStream_t st) {
-    const int rc = r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : r <= 8 ? 8 : 16;
+    const int rc = r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : r <= 8 ? 8 : r <= 12 ? 12 : 16;
     const std::string key = plan_key(a, (int)sizeof(T), rc);
     if (tune_cache_get(key, plan)) {
         plan.tuned = true;

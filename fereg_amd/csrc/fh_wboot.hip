@@ -317,7 +317,7 @@ __global__ void k_wb_cell_span(const uint32_t* __restrict__ coo, const uint32_t*
 }
 
 // *bad |= some group (cells off[q]..off[q+1], through perm when given) spans clusters; one
-// flag write per workgroup. Used for the FE1 levels (off = seg_off, contiguous cells).
+// flag write per workgroup. Groups: FE1 levels (off = seg_off) or (bucket, level) items.
 __global__ __launch_bounds__(TPB) void k_wb_group_span(const uint32_t* __restrict__ off,
                                                        const uint32_t* __restrict__ perm, const uint2* __restrict__ span,
                                                        size_t ngroups, uint2* __restrict__ gspan,
@@ -341,63 +341,74 @@ __global__ __launch_bounds__(TPB) void k_wb_group_span(const uint32_t* __restric
     if (threadIdx.x == 0 && any) atomicOr(bad, 1u);
 }
 
-// Per level of block b: the smallest and largest cluster over its cells (integer atomics: the
-// result does not depend on their order). Blocks of up to SPAN_LDS_G levels are reduced in LDS
-// per workgroup first (a small block's few levels would otherwise take every atomic).
-constexpr size_t SPAN_LDS_G = 4096;
-__global__ __launch_bounds__(TPB) void k_code_span(const uint32_t* __restrict__ cell_codes, int nb, int b,
-                                                   const uint2* __restrict__ span, size_t M, size_t g,
-                                                   uint32_t* __restrict__ lmin, uint32_t* __restrict__ lmax) {
-    __shared__ uint32_t smin[SPAN_LDS_G], smax[SPAN_LDS_G];
-    const bool lds = g <= SPAN_LDS_G;
-    if (lds) {
-        for (size_t l = threadIdx.x; l < g; l += TPB) {
-            smin[l] = 0xFFFFFFFFu;
-            smax[l] = 0u;
-        }
-        __syncthreads();
-    }
-    for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < M; c += (size_t)gridDim.x * TPB) {
-        const uint32_t lv = cell_codes[c * nb + b];
-        const uint2 sp = span[c];
-        if (lds) {
-            atomicMin(&smin[lv], sp.x);
-            atomicMax(&smax[lv], sp.y);
-        } else {
-            atomicMin(&lmin[lv], sp.x);
-            atomicMax(&lmax[lv], sp.y);
-        }
-    }
-    if (lds) {
-        __syncthreads();
-        for (size_t l = threadIdx.x; l < g; l += TPB)
-            if (smin[l] != 0xFFFFFFFFu) {
-                atomicMin(&lmin[l], smin[l]);
-                atomicMax(&lmax[l], smax[l]);
-            }
-    }
-}
-
-__global__ void k_fill_span(uint32_t* __restrict__ lmin, uint32_t* __restrict__ lmax, size_t g) {
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
-        lmin[l] = 0xFFFFFFFFu;
-        lmax[l] = 0u;
-    }
-}
-
-// *bad |= some level spans clusters (levels without cells have min > max).
-__global__ __launch_bounds__(TPB) void k_span_check(const uint32_t* __restrict__ lmin,
-                                                    const uint32_t* __restrict__ lmax, size_t g,
-                                                    uint32_t* __restrict__ bad) {
+// Levels cut into nbk buckets: combine the item spans of each level.
+__global__ __launch_bounds__(TPB) void k_wb_level_span(const uint2* __restrict__ gspan, uint32_t nbk, size_t g,
+                                                       uint32_t* __restrict__ bad) {
     bool b = false;
-    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x)
-        b |= lmin[l] < lmax[l];
+    for (size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x; l < g; l += (size_t)gridDim.x * blockDim.x) {
+        uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+        for (uint32_t k = 0; k < nbk; k++) {
+            const uint2 s = gspan[(size_t)k * g + l];
+            mn = min(mn, s.x);
+            mx = max(mx, s.y);
+        }
+        b |= mn < mx;
+    }
     __shared__ int any;
     if (threadIdx.x == 0) any = 0;
     __syncthreads();
     if (__ballot(b) != 0 && lane_id() == 0) atomicOr(&any, 1);
     __syncthreads();
     if (threadIdx.x == 0 && any) atomicOr(bad, 1u);
+}
+
+// Per level of a block of at most MS levels without level-major records (the dense small block):
+// the smallest and largest cluster over its cells, register bins per lane, reduced over the wave
+// by shuffles and over the workgroup in LDS; part[wg] holds each workgroup's (min, max) per level.
+template <int MS>
+__global__ __launch_bounds__(TPB) void k_small_code_span(const uint32_t* __restrict__ cell_codes, int nb, int b,
+                                                         const uint2* __restrict__ span, size_t M,
+                                                         uint2* __restrict__ part) {
+    __shared__ uint2 red[TPB / WAVE][MS];
+    uint32_t mn[MS], mx[MS];
+#pragma unroll
+    for (int k = 0; k < MS; k++) { mn[k] = 0xFFFFFFFFu; mx[k] = 0u; }
+    for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < M; c += (size_t)gridDim.x * TPB) {
+        const uint32_t lv = cell_codes[c * nb + b];
+        const uint2 sp = span[c];
+#pragma unroll
+        for (int k = 0; k < MS; k++)
+            if (lv == (uint32_t)k) { mn[k] = min(mn[k], sp.x); mx[k] = max(mx[k], sp.y); }
+    }
+    const int w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < MS; k++) {
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], o, WAVE));
+            mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], o, WAVE));
+        }
+        if (lane_id() == 0) red[w][k] = make_uint2(mn[k], mx[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < MS) {
+        uint32_t a = 0xFFFFFFFFu, z = 0u;
+        for (int ww = 0; ww < TPB / WAVE; ww++) { a = min(a, red[ww][threadIdx.x].x); z = max(z, red[ww][threadIdx.x].y); }
+        part[(size_t)blockIdx.x * MS + threadIdx.x] = make_uint2(a, z);
+    }
+}
+
+// *bad |= some level (columns of the nparts x MS partials) spans clusters.
+template <int MS>
+__global__ __launch_bounds__(TPB) void k_small_span_check(const uint2* __restrict__ part, unsigned nparts, int g,
+                                                          uint32_t* __restrict__ bad) {
+    if (threadIdx.x >= (unsigned)g) return;
+    uint32_t a = 0xFFFFFFFFu, z = 0u;
+    for (unsigned q = 0; q < nparts; q++) {
+        const uint2 v = part[(size_t)q * MS + threadIdx.x];
+        a = min(a, v.x);
+        z = max(z, v.y);
+    }
+    if (a != 0xFFFFFFFFu && a < z) atomicOr(bad, 1u);
 }
 
 // ------------------------------------------------------------------ host: inference
@@ -559,14 +570,25 @@ std::vector<int> terms_nested_in(Absorber& a, const uint32_t* cl) {
     if (a.fe1 >= 0)
         hipLaunchKernelGGL(k_wb_group_span, dim3(grid_for(a.g1, TPB, 4096)), dim3(TPB), 0, st, a.seg_off.p,
                            (const uint32_t*)nullptr, span.p, (size_t)a.g1, (uint2*)nullptr, bad.p);
+    DBuf<uint2> gspan;
     for (size_t b = 0; b < a.nb; b++) {
-        const size_t g = a.blocks[b].g;
-        DBuf<uint32_t> lmin(g), lmax(g);
-        hipLaunchKernelGGL(k_fill_span, dim3(grid_for(g, TPB, 4096)), dim3(TPB), 0, st, lmin.p, lmax.p, g);
-        hipLaunchKernelGGL(k_code_span, dim3(grid_for(a.M, TPB, g <= SPAN_LDS_G ? 1024 : 8192)), dim3(TPB), 0, st,
-                           a.cell_codes.p, (int)a.nb, (int)b, span.p, (size_t)a.M, g, lmin.p, lmax.p);
-        hipLaunchKernelGGL(k_span_check, dim3(grid_for(g, TPB, 4096)), dim3(TPB), 0, st, lmin.p, lmax.p, g,
-                           bad.p + 1 + b);
+        if (a.boff[b].n == 0 && a.blocks[b].g <= 32) {  // the dense small block: no level records
+            const unsigned np = grid_for(a.M, TPB * 16, 1024);
+            DBuf<uint2> part((size_t)np * 32);
+            hipLaunchKernelGGL(k_small_code_span<32>, dim3(np), dim3(TPB), 0, st, a.cell_codes.p, (int)a.nb, (int)b,
+                               span.p, (size_t)a.M, part.p);
+            hipLaunchKernelGGL(k_small_span_check<32>, dim3(1), dim3(TPB), 0, st, part.p, np, (int)a.blocks[b].g,
+                               bad.p + 1 + b);
+            continue;
+        }
+        ensure_block_transpose(a, (int)b);
+        const size_t items = (size_t)a.nbk[b] * a.blocks[b].g;
+        if (a.nbk[b] > 1) gspan.reserve(items);
+        hipLaunchKernelGGL(k_wb_group_span, dim3(grid_for(items, TPB, 4096)), dim3(TPB), 0, st, a.boff[b].p,
+                           a.bperm[b].p, span.p, items, a.nbk[b] > 1 ? gspan.p : nullptr, bad.p + 1 + b);
+        if (a.nbk[b] > 1)
+            hipLaunchKernelGGL(k_wb_level_span, dim3(grid_for(a.blocks[b].g, TPB, 4096)), dim3(TPB), 0, st, gspan.p,
+                               a.nbk[b], (size_t)a.blocks[b].g, bad.p + 1 + b);
     }
     FH_CHECK_LAUNCH();
     std::vector<uint32_t> hb(nflag);
