@@ -1124,16 +1124,19 @@ int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t
         hipEvent_t e0, e1;
         FH_HIP(hipEventCreate(&e0));
         FH_HIP(hipEventCreate(&e1));
+        // the solvers run S*p on columns padded to the kernels' class (zero pads): time that width
+        const size_t rq = padded_cols(r);
         auto run = [&](auto* tag) {
             using T = std::remove_pointer_t<decltype(tag)>;
-            DBuf<T> dp(m * r), dy(m * r);
-            std::vector<T> hp(m * r);
-            for (size_t i = 0; i < m * r; i++) hp[i] = T(0.001 * (double)(i % 997));
-            h2d(dp.p, hp.data(), m * r, a.st);
+            DBuf<T> dp(m * rq), dy(m * rq);
+            std::vector<T> hp(m * rq, T(0));
+            for (size_t i = 0; i < m; i++)
+                for (size_t j = 0; j < r; j++) hp[i * rq + j] = T(0.001 * (double)((i * r + j) % 997));
+            h2d(dp.p, hp.data(), m * rq, a.st);
             Scratch<T>& sc = precision == FH_MIXED ? reinterpret_cast<Scratch<T>&>(a.s32) : reinterpret_cast<Scratch<T>&>(a.s64);
-            spmv<T>(a, dp.p, dy.p, (int)r, sc, nullptr, a.st);  // warm-up (scratch allocation)
+            spmv<T>(a, dp.p, dy.p, (int)rq, sc, nullptr, a.st);  // warm-up (scratch allocation)
             FH_HIP(hipEventRecord(e0, a.st));
-            for (uint32_t k = 0; k < reps; k++) spmv<T>(a, dp.p, dy.p, (int)r, sc, nullptr, a.st);
+            for (uint32_t k = 0; k < reps; k++) spmv<T>(a, dp.p, dy.p, (int)rq, sc, nullptr, a.st);
             FH_HIP(hipEventRecord(e1, a.st));
             FH_HIP(hipEventSynchronize(e1));
             float t = 0.f;

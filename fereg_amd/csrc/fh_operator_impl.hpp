@@ -397,13 +397,98 @@ __device__ __forceinline__ void small_dense_body(const T* __restrict__ E, size_t
     }
 }
 
+// The same partials for wide blocks (RM >= 8): four threads share a column quad and own RM/4
+// columns each, so one chunk covers up to 12..32 small levels and t, p_L are read once (KC x RM
+// accumulators no longer fit one thread); the quad's E loads coincide across the four threads.
+// Lanes with the same column slice are reduced by shuffles, then the waves in order.
+template <class T, int RM, int KC>
+__device__ __forceinline__ void small_dense_cols(const T* __restrict__ E, size_t ld, size_t coff, size_t g1, size_t mL,
+                                                 const T* __restrict__ t, const T* __restrict__ pL, int ms, int r,
+                                                 T* __restrict__ part, unsigned bx, unsigned nwg, unsigned ch) {
+    constexpr int TPQ = 4, CW = RM / TPQ, QPB = WG / TPQ;
+    __shared__ T red[WAVES][KC][RM];
+    const int sl = threadIdx.x % TPQ, qi = threadIdx.x / TPQ, c0 = sl * CW, k0 = (int)ch * KC;
+    T acc[KC][CW];
+#pragma unroll
+    for (int kk = 0; kk < KC; kk++)
+#pragma unroll
+        for (int c = 0; c < CW; c++) acc[kk][c] = T(0);
+    const size_t nq = ld / 4;
+    for (size_t q = (size_t)bx * QPB + qi; q < nq; q += (size_t)nwg * QPB) {
+        const size_t c4 = 4 * q;
+        const bool seg = c4 < coff;
+        const T* src = seg ? t : pL;
+        const size_t base = seg ? c4 : c4 - coff, lim = seg ? g1 : mL;
+        T v[4][CW];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                const T x = (c0 + c < r && base + i < lim) ? src[(base + i) * r + c0 + c] : T(0);
+                v[i][c] = seg ? -x : x;
+            }
+#pragma unroll
+        for (int kk = 0; kk < KC; kk++) {
+            const int k = k0 + kk;
+            if (k < ms) {
+                T e[4];
+                ld_quad(E + (size_t)k * ld + c4, e);
+#pragma unroll
+                for (int c = 0; c < CW; c++) acc[kk][c] += e[0] * v[0][c] + e[1] * v[1][c] + e[2] * v[2][c] + e[3] * v[3][c];
+            }
+        }
+    }
+    const int w = threadIdx.x / WAVE, l = lane_id();
+#pragma unroll
+    for (int kk = 0; kk < KC; kk++)
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+            T x = acc[kk][c];
+#pragma unroll
+            for (int o = TPQ; o < WAVE; o <<= 1) x += __shfl_xor(x, o, WAVE);
+            if (l < TPQ) red[w][kk][l * CW + c] = x;
+        }
+    __syncthreads();
+    for (int e = threadIdx.x; e < KC * r; e += WG) {
+        const int kk = e / r, jj = e % r;
+        if (k0 + kk >= ms) continue;
+        T sm = T(0);
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ww++) sm += red[ww][kk][jj];
+        part[((size_t)(k0 + kk) * r + jj) * nwg + bx] = sm;
+    }
+}
+
+// Dense small-row partials, picking the layout by the column class.
+template <class T, int RM, int KC>
+__device__ __forceinline__ void small_dense_any(const T* __restrict__ E, size_t ld, size_t coff, size_t g1, size_t mL,
+                                                const T* __restrict__ t, const T* __restrict__ pL, int ms, int r,
+                                                T* __restrict__ part, unsigned bx, unsigned nwg, unsigned ch) {
+    if constexpr (RM >= 8)
+        small_dense_cols<T, RM, KC>(E, ld, coff, g1, mL, t, pL, ms, r, part, bx, nwg, ch);
+    else
+        small_dense_body<T, RM, KC>(E, ld, coff, g1, mL, t, pL, ms, r, part, bx, nwg, ch);
+}
+
+// Levels per chunk of the dense small-row partials: KC x RM accumulators per thread (rows layout)
+// or KC x RM/4 (column layout of the wide blocks), at most 32.
+template <class T, int RM>
+constexpr int small_chunk_any() {
+    if constexpr (RM >= 8) {
+        constexpr int k = (sizeof(T) == 8 ? 48 : 64) / (RM / 4);
+        return k < 32 ? k : 32;
+    } else {
+        return small_chunk<T, RM>();
+    }
+}
+
 template <class T, int RM, int KC>
 __global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ E, size_t ld, size_t coff, size_t g1,
                                                     size_t mL, const T* __restrict__ t, const T* __restrict__ pL,
                                                     int ms, int r, T* __restrict__ part,
                                                     const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    small_dense_body<T, RM, KC>(E, ld, coff, g1, mL, t, pL, ms, r, part, blockIdx.x, gridDim.x, blockIdx.y);
+    small_dense_any<T, RM, KC>(E, ld, coff, g1, mL, t, pL, ms, r, part, blockIdx.x, gridDim.x, blockIdx.y);
 }
 
 // Last step of S*p: the small rows y_S[k][j] = nr_k p_S[k][j] + the sum of the partials of entry
@@ -617,7 +702,7 @@ __global__ __launch_bounds__(WG) void k_lvl_small(LvlArgs<T> A, unsigned nlvl, c
         lvl_spmv_body<T, RM, NO>(A, blockIdx.x, nlvl);
     } else {
         const unsigned b = blockIdx.x - nlvl;
-        small_dense_body<T, RM, KC>(E, ld, coff, g1, mL, A.t, pL, ms, A.r, part, b % nwg, nwg, b / nwg);
+        small_dense_any<T, RM, KC>(E, ld, coff, g1, mL, A.t, pL, ms, A.r, part, b % nwg, nwg, b / nwg);
     }
 }
 
@@ -1083,7 +1168,7 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
             break;
         }
     auto with_kc = [&](auto rmc, auto&& f) {
-        constexpr int KM = small_chunk<T, decltype(rmc)::value>();
+        constexpr int KM = small_chunk_any<T, decltype(rmc)::value>();
         if (ms <= 4 || KM <= 4) f(std::integral_constant<int, (KM < 4 ? KM : 4)>{});
         else if (ms <= 8 || KM <= 8) f(std::integral_constant<int, (KM < 8 ? KM : 8)>{});
         else if (ms <= 16 || KM <= 16) f(std::integral_constant<int, (KM < 16 ? KM : 16)>{});

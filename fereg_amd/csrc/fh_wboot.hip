@@ -397,18 +397,29 @@ __global__ __launch_bounds__(TPB) void k_small_code_span(const uint32_t* __restr
     }
 }
 
-// *bad |= some level (columns of the nparts x MS partials) spans clusters.
+// *bad |= some level (columns of the nparts x MS partials) spans clusters: workgroup k reduces
+// level k's column over the partials.
 template <int MS>
-__global__ __launch_bounds__(TPB) void k_small_span_check(const uint2* __restrict__ part, unsigned nparts, int g,
+__global__ __launch_bounds__(TPB) void k_small_span_check(const uint2* __restrict__ part, unsigned nparts,
                                                           uint32_t* __restrict__ bad) {
-    if (threadIdx.x >= (unsigned)g) return;
+    __shared__ uint2 red[TPB / WAVE];
+    const unsigned k = blockIdx.x;
     uint32_t a = 0xFFFFFFFFu, z = 0u;
-    for (unsigned q = 0; q < nparts; q++) {
-        const uint2 v = part[(size_t)q * MS + threadIdx.x];
+    for (unsigned q = threadIdx.x; q < nparts; q += TPB) {
+        const uint2 v = part[(size_t)q * MS + k];
         a = min(a, v.x);
         z = max(z, v.y);
     }
-    if (a != 0xFFFFFFFFu && a < z) atomicOr(bad, 1u);
+    for (int o = 32; o > 0; o >>= 1) {
+        a = min(a, (uint32_t)__shfl_xor((int)a, o, WAVE));
+        z = max(z, (uint32_t)__shfl_xor((int)z, o, WAVE));
+    }
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = make_uint2(a, z);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < TPB / WAVE; w++) { a = min(a, red[w].x); z = max(z, red[w].y); }
+        if (a != 0xFFFFFFFFu && a < z) atomicOr(bad, 1u);
+    }
 }
 
 // ------------------------------------------------------------------ host: inference
@@ -577,7 +588,7 @@ std::vector<int> terms_nested_in(Absorber& a, const uint32_t* cl) {
             DBuf<uint2> part((size_t)np * 32);
             hipLaunchKernelGGL(k_small_code_span<32>, dim3(np), dim3(TPB), 0, st, a.cell_codes.p, (int)a.nb, (int)b,
                                span.p, (size_t)a.M, part.p);
-            hipLaunchKernelGGL(k_small_span_check<32>, dim3(1), dim3(TPB), 0, st, part.p, np, (int)a.blocks[b].g,
+            hipLaunchKernelGGL(k_small_span_check<32>, dim3(a.blocks[b].g), dim3(TPB), 0, st, part.p, np,
                                bad.p + 1 + b);
             continue;
         }

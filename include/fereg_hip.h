@@ -143,7 +143,9 @@ int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int
 /* SpMV timing probe (no counterpart in the reference; measurement only): runs `reps` products
  * y = S p on the absorber's stream and returns the mean milliseconds per product (HIP events on
  * that stream) and the compulsory HBM bytes of one product,
- * 4 nb M + 12 g1 + 2 s m r + s M [weighted] (s = 8 FP64, 4 FP32). */
+ * 4 nb M + 12 g1 + 2 s m r + s M [weighted] (s = 8 FP64, 4 FP32). The product runs on rows
+ * padded as the solvers pad them (r = 3 -> 4, 5..8 -> 8, 9..12 -> 12, 13..16 -> 16 columns, the
+ * pads zero); the bytes count r columns. */
 int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t reps, double* ms_per_spmv,
                           double* bytes_per_spmv);
 
