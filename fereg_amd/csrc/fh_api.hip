@@ -16,11 +16,14 @@ namespace fh {
 // fh_estimate.hip
 void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
                double* G_host);
-double residual_pass(Absorber& a, const double* z, size_t n, int ld, const double* c_host, const double* w, double* e);
+// e = z[:, :k] c (n), also written to column ecol of z when ecol >= 0; returns sum w e^2.
+double residual_pass(Absorber& a, double* z, size_t n, int ld, int k, const double* c_host, const double* w, double* e,
+                     int ecol);
 void hetero_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
                  double* meat);
-uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
-                      const uint32_t* cl_kept_dev, uint64_t G, double* meat);
+// e_i at e[i * es].
+uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, size_t es,
+                      const double* w, const uint32_t* cl_kept_dev, uint64_t G, double* meat);
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
@@ -32,20 +35,33 @@ namespace {
 constexpr int TPB = 256;
 inline unsigned vgrid(size_t n) { return grid_for(n, TPB, 8192); }
 
-// rows[i][0] = y[pi[i]], rows[i][1 + k] = X[pi[i]][k]: one thread per output element, so the
-// writes (and, for near-identity pi, the reads of X) are coalesced.
-__global__ void k_gather_rows(const double* __restrict__ y, const double* __restrict__ X, size_t K,
-                              const uint32_t* __restrict__ pi, size_t n, double* __restrict__ rows) {
-    const size_t R = K + 1;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * R; e += (size_t)gridDim.x * blockDim.x) {
-        const size_t i = e / R, c = e - i * R;
+// rows[i][0] = y[pi[i]], rows[i][1 + k] = X[pi[i]][k], rows[i][K+1 .. ld) = 0: one thread per
+// output element (index type I), so the writes (and, for near-identity pi, the reads of X) are
+// coalesced.
+template <class I>
+__global__ void k_gather_rows(const double* __restrict__ y, const double* __restrict__ X, I K,
+                              const uint32_t* __restrict__ pi, I n, I ld, double* __restrict__ rows) {
+    const I total = n * ld, stride = (I)gridDim.x * blockDim.x;
+    for (I e = (I)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const I i = e / ld, c = e - i * ld;
         const size_t o = pi[i];
         if (c == 0) {
             if (y) rows[e] = y[o];
         } else {
-            rows[e] = X[o * K + c - 1];
+            rows[e] = c <= K ? X[o * K + c - 1] : 0.0;
         }
     }
+}
+
+void gather_rows(const double* y, const double* X, size_t K, const uint32_t* pi, size_t n, size_t ld, double* rows,
+                 hipStream_t st) {
+    const unsigned grid = grid_for(n * ld, TPB, 1u << 16);
+    if (n * ld < (1ull << 32) - (size_t)grid * TPB)
+        hipLaunchKernelGGL(k_gather_rows<uint32_t>, dim3(grid), dim3(TPB), 0, st, y, X, (uint32_t)K, pi, (uint32_t)n,
+                           (uint32_t)ld, rows);
+    else
+        hipLaunchKernelGGL(k_gather_rows<size_t>, dim3(grid), dim3(TPB), 0, st, y, X, K, pi, n, ld, rows);
+    FH_CHECK_LAUNCH();
 }
 
 __global__ void k_gather_x(const double* __restrict__ X, size_t K, const uint32_t* __restrict__ pi, size_t n,
@@ -481,9 +497,10 @@ struct FitCommon {
 
 // V for a residualized block rows (n x R: [target, X...]) with e the residual combo applied
 // (fit/vcov.rs:316-382). bread over kept columns; scores x_i e_i w_i.
-void vcov_from(Absorber& a, double* rows, int R, const std::vector<int>& excluded, const std::vector<double>& bread,
-               const double* e, const double* w, int kind, const uint32_t* cl_dev, uint64_t G, const fh_ssc& ssc,
-               double dfk, double iid_scale, std::vector<double>& V, uint64_t* ncl) {
+void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
+               const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
+               const uint32_t* cl_dev, uint64_t G, const fh_ssc& ssc, double dfk, double iid_scale,
+               std::vector<double>& V, uint64_t* ncl) {
     const int K = R - 1;
     std::vector<int> kept;
     for (int j = 0; j < K; j++)
@@ -500,11 +517,13 @@ void vcov_from(Absorber& a, double* rows, int R, const std::vector<int>& exclude
         std::vector<double> meat((size_t)K * K);
         double factor = 1.0;
         if (kind == FH_VCOV_HETERO) {
-            hetero_meat(a, rows, a.n, R, 1, K, e, w, meat.data());
+            hetero_meat(a, rows, a.n, ld, 1, K, e, w, meat.data());
             if (ssc.k_adj) factor = n / (n - dfk);
         } else {
             require(cl_dev != nullptr, "a clustered variance matrix needs cluster codes");
-            uint64_t g = cluster_meat(a, rows, a.n, R, 1, K, e, w, cl_dev, G, meat.data());
+            // residuals from the rows' pad column when they are there: one row read per observation
+            uint64_t g = ecol >= 0 ? cluster_meat(a, rows, a.n, ld, 1, K, rows + ecol, (size_t)ld, w, cl_dev, G, meat.data())
+                                   : cluster_meat(a, rows, a.n, ld, 1, K, e, 1, w, cl_dev, G, meat.data());
             if (ncl) *ncl = g;
             if (ssc.g_adj) factor *= (double)g / ((double)g - 1.0);
             if (ssc.k_adj) factor *= (n - 1.0) / (n - dfk);
@@ -532,6 +551,7 @@ void vcov_from(Absorber& a, double* rows, int R, const std::vector<int>& exclude
 // Cluster levels: max code + 1.
 uint64_t cluster_levels(Absorber& a, const uint32_t* cl_dev) {
     require(cl_dev != nullptr, "a clustered variance matrix needs cluster codes");
+    if (a.cl_runs && a.cl_runs->cl == cl_dev) return a.cl_runs->G;
     DBuf<uint32_t> mx(1);
     FH_HIP(hipMemsetAsync(mx.p, 0, sizeof(uint32_t), a.st));
     hipLaunchKernelGGL(k_max_u32, dim3(grid_for(a.n, TPB * 8, 1024)), dim3(TPB), 0, a.st, cl_dev, a.n, mx.p);
@@ -549,6 +569,7 @@ struct OlsState {
     DBuf<double> e;                  // residuals, row order
     double dfk = 0.0;
     uint64_t ncl = 0;
+    int ld = 0, ecol = -1;  // stride of the solver rows, their residual column (-1: none)
 };
 
 // OLS on device inputs (kept order). rows scratch: n x (1+K) in row order.
@@ -558,16 +579,19 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     hipStream_t st = a.st;
     const size_t n = a.n, R = K + 1;
     require(K >= 1 && K <= 21, "1..21 regressors are supported");
-    a.rows_tmp.reserve(n * R);
+    // solver rows padded to the solver's column class (zero pads, residualized as inactive
+    // columns): whole 16-byte vectors per row; a pad column takes the residuals, so the cluster
+    // sums read one aligned row per observation
+    const size_t ld = R <= 16 ? (size_t)padded_cols((int)R) : R;
+    const int ecol = ld > R ? (int)R : -1;
+    a.rows_tmp.reserve(n * ld);
     double* rows = a.rows_tmp.p;
-    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(n * (K + 1), TPB, 1u << 16)), dim3(TPB), 0, st, y, X, K, a.pi.p,
-                       n, rows);
-    FH_CHECK_LAUNCH();
+    gather_rows(y, X, K, a.pi.p, n, ld, rows, st);
     const int prec = o.precision < 0 ? a.precision : o.precision;
-    ResidOut ro = residualize_any(a, rows, R, o.tol, o.maxiter, nullptr, nullptr, prec, ctx);
+    ResidOut ro = residualize_any(a, rows, ld, o.tol, o.maxiter, nullptr, nullptr, prec, ctx);
     const double* w = a.weighted ? a.w_rows.p : nullptr;
     std::vector<double> G(R * R);
-    gram_cols(a, rows, n, (int)R, 0, (int)R, w, nullptr, G.data());
+    gram_cols(a, rows, n, (int)ld, 0, (int)R, w, nullptr, G.data());
     // regress column 0 on 1..K (fit/design.rs:64-95)
     std::vector<double> Gxx(K * K);
     for (size_t i = 0; i < K; i++)
@@ -590,7 +614,7 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     c[0] = 1.0;
     for (int i = 0; i < mk; i++) c[kept[i] + 1] = -coef[i];
     DBuf<double> e(n);
-    const double ssr = residual_pass(a, rows, n, (int)R, c.data(), w, e.p);
+    const double ssr = residual_pass(a, rows, n, (int)ld, (int)R, c.data(), w, e.p, ecol);
     const double np = (double)mk + [&] {
         double s = 0.0;
         for (uint64_t g : a.term_g) s += (double)g;
@@ -600,7 +624,8 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     std::vector<double> V;
     uint64_t ncl = 0;
     const uint64_t G_cl = kind == FH_VCOV_CLUSTER ? cluster_levels(a, cl_dev) : 0;
-    vcov_from(a, rows, (int)R, excluded, inv, e.p, w, kind, cl_dev, G_cl, ssc, dfk, ssr / ((double)n - 1.0), V, &ncl);
+    vcov_from(a, rows, (int)ld, (int)R, excluded, inv, e.p, ecol, w, kind, cl_dev, G_cl, ssc, dfk,
+              ssr / ((double)n - 1.0), V, &ncl);
     for (size_t j = 0; j < K; j++) {
         beta[j] = 0.0;
         kept_out[j] = !excluded[j];
@@ -624,6 +649,8 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
         state->e = std::move(e);
         state->dfk = dfk;
         state->ncl = ncl;
+        state->ld = (int)ld;
+        state->ecol = ecol;
     }
 }
 
@@ -701,7 +728,7 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
         std::vector<double> c(R, 0.0);
         c[0] = 1.0;
         for (int i = 0; i < mk; i++) c[kept[i] + 1] = -coef[i];
-        residual_pass(a, rows, n, (int)R, c.data(), nullptr, e.p);
+        residual_pass(a, rows, n, (int)R, (int)R, c.data(), nullptr, e.p, -1);
         FH_HIP(hipMemcpyAsync(eta_old.p, eta.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         double step = 1.0, dev = NAN;
         bool halved = false;
@@ -743,7 +770,7 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
     uint64_t ncl = 0;
     const uint64_t G_cl = kind == FH_VCOV_CLUSTER ? cluster_levels(a, cl_dev) : 0;
     std::vector<double> V;
-    vcov_from(a, rows, (int)R, excluded, inv, e.p, w.p, kind, cl_dev, G_cl, ssc, dfk, 1.0, V, &ncl);
+    vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, w.p, kind, cl_dev, G_cl, ssc, dfk, 1.0, V, &ncl);
     for (size_t j = 0; j < K; j++) {
         beta[j] = 0.0;
         kept_out[j] = !excluded[j];
@@ -1266,6 +1293,15 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
             px = dx.p;
             pc = dc.p;
         }
+        // the fit's CRV1 scores and the bootstrap's cluster sums share one set of cluster runs
+        const uint64_t G = cluster_levels(a, pc);
+        ClusterRuns runs;
+        cluster_runs(a, pc, G, runs);
+        struct RunsScope {
+            Absorber& a;
+            ~RunsScope() { a.cl_runs = nullptr; }
+        } runs_scope{a};
+        a.cl_runs = &runs;
         OlsState os;
         fh_fit_stats stats{};
         ols_fit(a, py, px, K, FH_VCOV_CLUSTER, pc, nested, *ssc, *o, beta, vcov, kept, &stats, nullptr, &os);
@@ -1278,7 +1314,6 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
         // fixed effects not nested in the clusters: the reference projects them out exactly when
         // their levels number at most min(G, 512) (or G <= 512); beyond that it skips the
         // projection (ols.rs:392-410, mod.rs:959-997), which is what is done here
-        const uint64_t G = cluster_levels(a, pc);
         std::vector<int> nest = terms_nested_in(a, pc);
         uint64_t m_nn = 0;
         for (size_t q = 0; q < a.nterms; q++)
@@ -1288,11 +1323,12 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
                 "which this build does not provide");
         WbootInput in;
         in.rows = a.rows_tmp.p;
-        in.R = (int)K + 1;
+        in.R = os.ld;
         in.kept = os.kept;
         in.bread = os.inv;
         in.beta = os.coef;
-        in.e = os.e.p;
+        in.e = os.ecol >= 0 ? a.rows_tmp.p + os.ecol : os.e.p;
+        in.es = os.ecol >= 0 ? (size_t)os.ld : 1;
         in.w = a.weighted ? a.w_rows.p : nullptr;
         in.cl = pc;
         in.G = G;

@@ -75,17 +75,28 @@ void gram_dispatch(int k, unsigned grid, hipStream_t st, const double* z, size_t
 }
 
 // e_i = sum_j c_j z_ij; partial sums of w e^2 per workgroup.
-__global__ __launch_bounds__(TPB) void k_resid(const double* __restrict__ z, size_t n, int ld,
+// e_i = sum_{j < k} c_j z[i][j] (and z[i][ecol] = e_i when ecol >= 0); part[wg] = partial
+// sums of w_i e_i^2.
+__global__ __launch_bounds__(TPB) void k_resid(double* __restrict__ z, size_t n, int ld, int k,
                                                const double* __restrict__ c, const double* __restrict__ w,
-                                               double* __restrict__ e, double* __restrict__ part) {
+                                               double* __restrict__ e, int ecol, double* __restrict__ part) {
     __shared__ double cc[32];
-    if (threadIdx.x < (unsigned)ld) cc[threadIdx.x] = c[threadIdx.x];
+    if (threadIdx.x < 32u) cc[threadIdx.x] = threadIdx.x < (unsigned)k ? c[threadIdx.x] : 0.0;
     __syncthreads();
     double s = 0.0;
     for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
         double v = 0.0;
-        for (int j = 0; j < ld; j++) v += cc[j] * z[i * ld + j];
+        double* zi = z + i * ld;
+        int j = 0;
+        if ((ld & 1) == 0)  // 16-byte aligned rows: paired loads
+            for (; j + 1 < k; j += 2) {
+                const double2 q = *reinterpret_cast<const double2*>(zi + j);
+                v += cc[j] * q.x;
+                v += cc[j + 1] * q.y;
+            }
+        for (; j < k; j++) v += cc[j] * zi[j];
         e[i] = v;
+        if (ecol >= 0) zi[ecol] = v;
         s += (w ? w[i] : 1.0) * v * v;
     }
     __shared__ double red[TPB / WAVE];
@@ -107,11 +118,26 @@ __global__ void k_sq_we(const double* __restrict__ e, const double* __restrict__
     }
 }
 
-__global__ void k_gather_cluster(const uint32_t* __restrict__ cl_kept, const uint32_t* __restrict__ pi, size_t n,
-                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+// flag[i] = 1 where a run of adjacent solver rows with one cluster code starts.
+__global__ void k_run_flags(const uint32_t* __restrict__ cl_kept, const uint32_t* __restrict__ pi, size_t n,
+                            uint32_t* __restrict__ flag) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        flag[i] = (i == 0 || cl_kept[pi[i]] != cl_kept[pi[i - 1]]) ? 1u : 0u;
+}
+
+// At each run start i (run q = pos[i]): run_off[q] = i, key[q] = its cluster, val[q] = q; and
+// run_off[nrun] = n.
+__global__ void k_run_emit(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                           const uint32_t* __restrict__ cl_kept, const uint32_t* __restrict__ pi, size_t n,
+                           uint32_t* __restrict__ run_off, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        keys[i] = cl_kept[pi[i]];
-        vals[i] = (uint32_t)i;
+        const uint32_t q = pos[i];
+        if (flag[i]) {
+            run_off[q] = (uint32_t)i;
+            key[q] = cl_kept[pi[i]];
+            val[q] = q;
+        }
+        if (i == n - 1) run_off[q + flag[i]] = (uint32_t)n;
     }
 }
 
@@ -125,16 +151,16 @@ __global__ void k_cl_offsets(const uint64_t* __restrict__ key, size_t n, size_t 
     }
 }
 
-// s_g = sum over the rows of cluster g (sorted order) of x_i e_i w_i; one thread per cluster.
-// sg[q] = sum over the rows i of cluster q of z[i][c0 .. c0+k) e_i w_i: L lanes per cluster
-// (its rows strided over the lanes, each a row gather of k values), then the lanes reduced by
-// shuffles in a fixed order.
+// sg[q] = sum over the rows i of cluster q of z[i][c0 .. c0+k) e_i w_i: L lanes per cluster,
+// each summing whole runs of adjacent rows (the cluster's runs strided over the lanes), then the
+// lanes reduced by shuffles in a fixed order.
 template <int KM>
 __global__ __launch_bounds__(TPB) void k_cluster_scores(const uint32_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ rows_sorted, size_t g,
+                                                        const uint32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ run_off, size_t g,
                                                         const double* __restrict__ z, int ld, int c0, int k,
-                                                        const double* __restrict__ e, const double* __restrict__ w,
-                                                        int L, double* __restrict__ sg) {
+                                                        const double* __restrict__ e, size_t es,
+                                                        const double* __restrict__ w, int L, double* __restrict__ sg) {
     const int l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
     const size_t nwaves = (size_t)gridDim.x * (TPB / WAVE);
@@ -146,11 +172,15 @@ __global__ __launch_bounds__(TPB) void k_cluster_scores(const uint32_t* __restri
         for (int a = 0; a < KM; a++) acc[a] = 0.0;
         const uint32_t j0 = valid ? off[q] : 0, j1 = valid ? off[q + 1] : 0;
         for (uint32_t j = j0 + li; j < j1; j += L) {
-            const size_t i = rows_sorted[j];
-            const double s = e[i] * (w ? w[i] : 1.0);
+            const uint32_t run = order[j];
+            const uint32_t i1 = run_off[run + 1];
+            for (uint32_t i = run_off[run]; i < i1; i++) {
+                const double s = e[(size_t)i * es] * (w ? w[i] : 1.0);
+                const double* zi = z + (size_t)i * ld + c0;
 #pragma unroll
-            for (int a = 0; a < KM; a++)
-                if (a < k) acc[a] += z[i * ld + c0 + a] * s;
+                for (int a = 0; a < KM; a++)
+                    if (a < k) acc[a] += zi[a] * s;
+            }
         }
 #pragma unroll
         for (int a = 0; a < KM; a++)
@@ -226,10 +256,12 @@ void gram(Absorber& a, const double* z, const double* w, size_t n, int r, double
     gram_cols(a, z, n, r, 0, r, w, nullptr, G_host);
 }
 
-double residual_pass(Absorber& a, const double* z, size_t n, int ld, const double* c_host, const double* w, double* e) {
-    DBuf<double> c(ld), part(GRID);
-    upload(c.p, c_host, ld * sizeof(double), a.st);
-    hipLaunchKernelGGL(k_resid, dim3(GRID), dim3(TPB), 0, a.st, z, n, ld, c.p, w, e, part.p);
+double residual_pass(Absorber& a, double* z, size_t n, int ld, int k, const double* c_host, const double* w, double* e,
+                     int ecol) {
+    require(k >= 1 && k <= 32 && k <= ld && ecol < ld, "residual pass: bad column layout");
+    DBuf<double> c(k), part(GRID);
+    upload(c.p, c_host, k * sizeof(double), a.st);
+    hipLaunchKernelGGL(k_resid, dim3(GRID), dim3(TPB), 0, a.st, z, n, ld, k, c.p, w, e, ecol, part.p);
     FH_CHECK_LAUNCH();
     return sum_parts_dev(part.p, GRID, 1, a.st)[0];
 }
@@ -242,43 +274,69 @@ void hetero_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, 
     gram_cols(a, z, n, ld, c0, k, nullptr, v.p, meat);
 }
 
-void cluster_sort(Absorber& a, const uint32_t* cl_kept_dev, uint64_t G, DBuf<uint32_t>& vals, DBuf<uint32_t>& off) {
+void cluster_runs(Absorber& a, const uint32_t* cl_kept_dev, uint64_t G, ClusterRuns& cr) {
     hipStream_t st = a.st;
     const size_t n = a.n;
-    DBuf<uint64_t> keys(n), kalt(n);
-    DBuf<uint32_t> valt(n);
-    vals.alloc(n);
-    off.alloc(G + 1);
-    hipLaunchKernelGGL(k_gather_cluster, dim3(grid_for(n, TPB, 4096)), dim3(TPB), 0, st, cl_kept_dev, a.pi.p, n, keys.p,
-                       vals.p);
+    cr.cl = cl_kept_dev;
+    cr.G = G;
+    cr.off.alloc(G + 1);
+    FH_HIP(hipMemsetAsync(cr.off.p, 0, cr.off.bytes(), st));
+    cr.nrun = 0;
+    if (n == 0) return;
+    DBuf<uint32_t> flag(n), pos(n), total(1);
+    hipLaunchKernelGGL(k_run_flags, dim3(grid_for(n, TPB, 4096)), dim3(TPB), 0, st, cl_kept_dev, a.pi.p, n, flag.p);
     FH_CHECK_LAUNCH();
-    radix_sort_pairs(keys.p, vals.p, kalt.p, valt.p, n, (int)bits_for(G), st);
-    FH_HIP(hipMemsetAsync(off.p, 0, off.bytes(), st));
-    hipLaunchKernelGGL(k_cl_offsets, dim3(grid_for(n, TPB, 4096)), dim3(TPB), 0, st, keys.p, n, (size_t)G, off.p);
+    exclusive_scan_u32(flag.p, pos.p, n, total.p, st);
+    uint32_t nr = 0;
+    download(&nr, total.p, sizeof(nr), st);
+    cr.nrun = nr;
+    cr.run_off.alloc(nr + 1);
+    cr.order.alloc(nr);
+    DBuf<uint64_t> keys(nr), kalt(nr);
+    DBuf<uint32_t> valt(nr);
+    hipLaunchKernelGGL(k_run_emit, dim3(grid_for(n, TPB, 4096)), dim3(TPB), 0, st, flag.p, pos.p, cl_kept_dev,
+                       a.pi.p, n, cr.run_off.p, keys.p, cr.order.p);
+    FH_CHECK_LAUNCH();
+    // stable: a cluster's runs stay in row order
+    radix_sort_pairs(keys.p, cr.order.p, kalt.p, valt.p, nr, (int)bits_for(G), st);
+    hipLaunchKernelGGL(k_cl_offsets, dim3(grid_for(nr, TPB, 4096)), dim3(TPB), 0, st, keys.p, (size_t)nr, (size_t)G,
+                       cr.off.p);
     FH_CHECK_LAUNCH();
 }
 
+const ClusterRuns& runs_for(Absorber& a, const uint32_t* cl_kept_dev, uint64_t G, ClusterRuns& local) {
+    if (a.cl_runs && a.cl_runs->cl == cl_kept_dev && a.cl_runs->G == G) return *a.cl_runs;
+    cluster_runs(a, cl_kept_dev, G, local);
+    return local;
+}
+
+uint32_t lanes_per_cluster(const ClusterRuns& cr) {
+    uint32_t L = 1;
+    while (L < 64 && L * 2.0 < double(cr.nrun) / double(std::max<uint64_t>(cr.G, 1))) L <<= 1;
+    return L;
+}
+
 // Returns the number of non-empty clusters.
-uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
-                      const uint32_t* cl_kept_dev, uint64_t G, double* meat) {
+uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, size_t es,
+                      const double* w, const uint32_t* cl_kept_dev, uint64_t G, double* meat) {
     hipStream_t st = a.st;
-    DBuf<uint32_t> vals, off;
-    cluster_sort(a, cl_kept_dev, G, vals, off);
+    ClusterRuns local;
+    const ClusterRuns& cr = runs_for(a, cl_kept_dev, G, local);
+    const uint32_t* off = cr.off.p;
     DBuf<double> sg(G * k);
-    uint32_t L = 1;  // lanes per cluster: ~4 rows per lane
-    while (L < 64 && L * 4.0 < double(n) / double(std::max<uint64_t>(G, 1))) L <<= 1;
+    const uint32_t L = lanes_per_cluster(cr);
     const unsigned cg = grid_for(G, (TPB / WAVE) * (WAVE / L), 1u << 16);
     auto launch = [&](auto km) {
         constexpr int KM = decltype(km)::value;
-        hipLaunchKernelGGL((k_cluster_scores<KM>), dim3(cg), dim3(TPB), 0, st, off.p, vals.p, (size_t)G, z, ld, c0, k,
-                           e, w, (int)L, sg.p);
+        hipLaunchKernelGGL((k_cluster_scores<KM>), dim3(cg), dim3(TPB), 0, st, off, cr.order.p, cr.run_off.p,
+                           (size_t)G, z, ld, c0, k, e, es, w, (int)L, sg.p);
     };
     if (k <= 4) launch(std::integral_constant<int, 4>{});
     else if (k <= 12) launch(std::integral_constant<int, 12>{});
     else launch(std::integral_constant<int, GRAM_KMAX>{});
     DBuf<unsigned long long> cnt(1);
     FH_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(k_count_nonempty, dim3(grid_for(G, TPB, 4096)), dim3(TPB), 0, st, off.p, (size_t)G, cnt.p);
+    hipLaunchKernelGGL(k_count_nonempty, dim3(grid_for(G, TPB, 4096)), dim3(TPB), 0, st, off, (size_t)G, cnt.p);
     FH_CHECK_LAUNCH();
     gram_cols(a, sg.p, G, k, 0, k, nullptr, nullptr, meat);
     unsigned long long c = 0;

@@ -76,6 +76,8 @@ struct SpmvPlan {
     bool tuned = false;
 };
 
+struct ClusterRuns;
+
 struct Absorber {
     int device = 0;
     hipStream_t st = nullptr;
@@ -126,6 +128,10 @@ struct Absorber {
         const int c = r <= 1 ? 0 : r <= 2 ? 1 : r <= 4 ? 2 : r <= 8 ? 3 : r <= 12 ? 4 : 5;
         return plans[fp64 ? 1 : 0][c];
     }
+
+    // runs of the clusters of the current call's codes (set for the duration of one API call
+    // that uses one clustering more than once)
+    const ClusterRuns* cl_runs = nullptr;
 
     bool trivial() const { return cell_obs_off.n == 0; }
     bool unit() const { return !weighted && trivial(); }
@@ -243,19 +249,35 @@ void gram(Absorber& a, const double* z, const double* w, size_t n, int r, double
 // Columns c0..c0+k of z (ld), optional row weights w and v: G = sum_i w_i v_i z_i z_i' (host).
 void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
                double* G_host);
-// Solver rows grouped by cluster (codes in kept order): rows_sorted (n), off (G+1).
-void cluster_sort(Absorber& a, const uint32_t* cl_kept_dev, uint64_t G, DBuf<uint32_t>& rows_sorted,
-                  DBuf<uint32_t>& off);
+// Clusters as runs of adjacent solver rows with one cluster code: the rows of cluster g are the
+// runs order[off[g] .. off[g+1]) (in row order), run q spanning rows run_off[q] .. run_off[q+1].
+// Rows in solver order follow the absorbed terms, so a cluster that is (or nests) a term comes in
+// runs of many rows and the runs, not the rows, are sorted.
+struct ClusterRuns {
+    const uint32_t* cl = nullptr;  // codes described (kept order)
+    uint64_t G = 0;
+    size_t nrun = 0;
+    DBuf<uint32_t> run_off;  // nrun + 1
+    DBuf<uint32_t> order;    // nrun, stable by cluster
+    DBuf<uint32_t> off;      // G + 1
+};
+void cluster_runs(Absorber& a, const uint32_t* cl_kept_dev, uint64_t G, ClusterRuns& cr);
+// The runs of these codes: the ones set on the absorber for the current call when they match,
+// else built into `local`.
+const ClusterRuns& runs_for(Absorber& a, const uint32_t* cl_kept_dev, uint64_t G, ClusterRuns& local);
+// L lanes per cluster for ~2 runs per lane.
+uint32_t lanes_per_cluster(const ClusterRuns& cr);
 
 // ---------------------------------------------------------------- wild bootstrap (fh_wboot.hip)
-// WCR11 on the residualized OLS block (rows n x R: column 0 = y~, 1..K = X~), the kept
+// WCR11 on the residualized OLS block (rows n x K+1 with stride R: column 0 = y~, 1..K = X~), the kept
 // regressors `kept`, bread (mk x mk), residuals e (n), clusters cl (kept order, G levels).
 struct WbootInput {
     const double* rows;
     int R;
     std::vector<int> kept;
     std::vector<double> bread, beta;  // kept columns
-    const double* e;
+    const double* e;  // e_i at e[i * es]
+    size_t es = 1;
     const double* w;
     const uint32_t* cl;
     uint64_t G;

@@ -42,44 +42,66 @@ __host__ __device__ __forceinline__ uint64_t sm_mix(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-// Per row: w z, w u0, w u1 (z = x~'C, u0 = e, u1 = -x~'M_R).
-__global__ void k_wb_rowscal(const double* __restrict__ rows, int R, const int* __restrict__ kept, int mk,
-                             const double* __restrict__ C, const double* __restrict__ MR, const double* __restrict__ e,
-                             const double* __restrict__ w, size_t n, double* __restrict__ out) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        double z = 0.0, u1 = 0.0;
-        for (int a = 0; a < mk; a++) {
-            const double x = rows[i * R + 1 + kept[a]];
-            z += x * C[a];
-            u1 -= x * MR[a];
+// S[g][0..2][a]: sums over the rows i of cluster g of (w u0, w u1, w z)_i x~_ia, with z = x~_i'C,
+// u0 = e_i and u1 = -x~_i'M_R (x~ the kept regressors): L lanes per cluster, each summing whole
+// runs of adjacent rows (the cluster's runs strided over the lanes), then the lanes reduced by
+// shuffles in a fixed order.
+template <int MK>
+__global__ __launch_bounds__(TPB) void k_wb_clsums(const uint32_t* __restrict__ off, const uint32_t* __restrict__ order,
+                                                   const uint32_t* __restrict__ run_off, size_t G,
+                                                   const double* __restrict__ rows, int R,
+                                                   const int* __restrict__ kept, int mk, const double* __restrict__ C,
+                                                   const double* __restrict__ MR, const double* __restrict__ e,
+                                                   size_t es, const double* __restrict__ w, int L,
+                                                   double* __restrict__ S) {
+    const int l = lane_id();
+    const int gpw = WAVE / L, gl = l / L, li = l % L;
+    const size_t nwaves = (size_t)gridDim.x * (TPB / WAVE);
+    for (size_t base = ((size_t)blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE) * gpw; base < G; base += nwaves * gpw) {
+        const size_t q = base + gl;
+        const bool valid = q < G;
+        double s0[MK], s1[MK], sb[MK];
+#pragma unroll
+        for (int a = 0; a < MK; a++) s0[a] = s1[a] = sb[a] = 0.0;
+        const uint32_t j0 = valid ? off[q] : 0, j1 = valid ? off[q + 1] : 0;
+        for (uint32_t j = j0 + li; j < j1; j += L) {
+            const uint32_t run = order[j];
+            const uint32_t i1 = run_off[run + 1];
+            for (uint32_t i = run_off[run]; i < i1; i++) {
+                double x[MK], z = 0.0, u1 = 0.0;
+#pragma unroll
+                for (int a = 0; a < MK; a++) {
+                    x[a] = a < mk ? rows[(size_t)i * R + 1 + kept[a]] : 0.0;
+                    if (a < mk) {
+                        z += x[a] * C[a];
+                        u1 -= x[a] * MR[a];
+                    }
+                }
+                const double wi = w ? w[i] : 1.0;
+                const double wz = wi * z, w0 = wi * e[(size_t)i * es], w1 = wi * u1;
+#pragma unroll
+                for (int a = 0; a < MK; a++) {
+                    s0[a] += w0 * x[a];
+                    s1[a] += w1 * x[a];
+                    sb[a] += wz * x[a];
+                }
+            }
         }
-        const double wi = w ? w[i] : 1.0;
-        out[i * 3] = wi * z;
-        out[i * 3 + 1] = wi * e[i];
-        out[i * 3 + 2] = wi * u1;
-    }
-}
-
-// S[g][0..2][a]: sums over the rows of cluster g of (w u0, w u1, w z) x~_a; thread per (g, a).
-__global__ void k_wb_clsums(const uint32_t* __restrict__ off, const uint32_t* __restrict__ rs, size_t G,
-                            const double* __restrict__ rows, int R, const int* __restrict__ kept, int mk,
-                            const double* __restrict__ sc, double* __restrict__ S) {
-    const size_t total = G * mk;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const size_t g = e / mk;
-        const int a = (int)(e % mk);
-        const int col = 1 + kept[a];
-        double s0 = 0.0, s1 = 0.0, sb = 0.0;
-        for (uint32_t j = off[g]; j < off[g + 1]; j++) {
-            const size_t i = rs[j];
-            const double x = rows[i * R + col];
-            sb += sc[i * 3] * x;
-            s0 += sc[i * 3 + 1] * x;
-            s1 += sc[i * 3 + 2] * x;
-        }
-        S[(g * 3 + 0) * mk + a] = s0;
-        S[(g * 3 + 1) * mk + a] = s1;
-        S[(g * 3 + 2) * mk + a] = sb;
+#pragma unroll
+        for (int a = 0; a < MK; a++)
+            for (int o = L >> 1; o > 0; o >>= 1) {
+                s0[a] += __shfl_xor(s0[a], o, WAVE);
+                s1[a] += __shfl_xor(s1[a], o, WAVE);
+                sb[a] += __shfl_xor(sb[a], o, WAVE);
+            }
+        if (valid && li == 0)
+#pragma unroll
+            for (int a = 0; a < MK; a++)
+                if (a < mk) {
+                    S[(q * 3 + 0) * mk + a] = s0[a];
+                    S[(q * 3 + 1) * mk + a] = s1[a];
+                    S[(q * 3 + 2) * mk + a] = sb[a];
+                }
     }
 }
 
@@ -638,13 +660,21 @@ void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, 
     FH_HIP(hipMemcpyAsync(dMR.p, MR.data(), mk * sizeof(double), hipMemcpyHostToDevice, st));
     FH_HIP(hipMemcpyAsync(dbread.p, in.bread.data(), (size_t)mk * mk * sizeof(double), hipMemcpyHostToDevice, st));
     // per-cluster sums
-    DBuf<uint32_t> rs, off;
-    cluster_sort(a, in.cl, G, rs, off);
-    DBuf<double> sc(n * 3), S(G * 3 * mk);
-    hipLaunchKernelGGL(k_wb_rowscal, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, in.rows, in.R, dkept.p, mk, dC.p,
-                       dMR.p, in.e, in.w, n, sc.p);
-    hipLaunchKernelGGL(k_wb_clsums, dim3(grid_for(G * mk, TPB, 8192)), dim3(TPB), 0, st, off.p, rs.p, G, in.rows, in.R,
-                       dkept.p, mk, sc.p, S.p);
+    ClusterRuns local;
+    const ClusterRuns& cr = runs_for(a, in.cl, G, local);
+    DBuf<double> S(G * 3 * mk);
+    {
+        const uint32_t L = lanes_per_cluster(cr);
+        const unsigned cg = grid_for(G, (TPB / WAVE) * (WAVE / L), 1u << 16);
+        auto launch = [&](auto mkc) {
+            hipLaunchKernelGGL((k_wb_clsums<decltype(mkc)::value>), dim3(cg), dim3(TPB), 0, st, cr.off.p, cr.order.p,
+                               cr.run_off.p, (size_t)G, in.rows, in.R, dkept.p, mk, dC.p, dMR.p, in.e, in.es, in.w, (int)L,
+                               S.p);
+        };
+        if (mk <= 4) launch(std::integral_constant<int, 4>{});
+        else if (mk <= 12) launch(std::integral_constant<int, 12>{});
+        else launch(std::integral_constant<int, 20>{});
+    }
     FH_CHECK_LAUNCH();
     const int NC = 2 + 4 * mk, QC = 2 + mk;
     DBuf<double> P(G * NC), Q(G * QC);
