@@ -4,6 +4,7 @@
 //   pi            u32[n]      kept-observation index of every row (row order = sorted order)
 //   seg_off       u32[g1+1]   first cell of every FE1 level
 //   cell_codes    u32[M*nb]   block codes per cell, row-major (cell, block)
+//   large_codes   u32[M*(nb-1)] the same without block 0, when block 0 is the dense small block
 //   cell_obs_off  u32[M+1]    first row of every cell (absent when every cell is one row)
 //   cell_seg      u32[M]      FE1 level of every cell
 //   per block b, the level-major transpose of the cells in segment buckets (built once):
@@ -89,6 +90,9 @@ struct Absorber {
     uint64_t small_limit = 4096;
     int precision = FH_PLAIN_FP64;
     DBuf<uint32_t> pi, seg_off, cell_codes, cell_obs_off, cell_seg;
+    // codes of the large blocks only (M x (nb - 1), row-major), for the segment pass when the one
+    // small block (block 0) goes through the dense products: it then streams only these
+    DBuf<uint32_t> large_codes;
     std::vector<DBuf<uint32_t>> lperm, loff;                // plain level lists, on demand
     std::vector<DBuf<uint32_t>> bperm, boff, lseg, lother;  // bucketed transpose (large blocks)
     std::vector<uint32_t> nbk;                              // buckets per block
@@ -166,6 +170,7 @@ void ensure_w32(Absorber& a);
 template <class T>
 struct OpArgs {
     const uint32_t* cell_codes;
+    const uint32_t* large_codes;  // cell_codes without block 0 (dense small block), or null
     const uint32_t* seg_off;
     const uint32_t* cell_obs_off;
     const T* cw;

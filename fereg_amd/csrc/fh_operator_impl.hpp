@@ -239,7 +239,11 @@ __global__ __launch_bounds__(WG) void k_seg_t(OpArgs<T> A, const T* __restrict__
                 const uint32_t c = c0 + (it + h) * L + li;
                 v[h] = c < c1;
 #pragma unroll
-                for (int b = SKIP; b < NB; b++) code[h][b] = v[h] ? ld_idx(&A.cell_codes[(size_t)c * NB + b]) : 0u;
+                // with the small block skipped, the large blocks' own code array (no block-0 column)
+                for (int b = SKIP; b < NB; b++)
+                    code[h][b] = !v[h] ? 0u
+                                 : SKIP ? ld_idx(&A.large_codes[(size_t)c * (NB - 1) + b - 1])
+                                        : ld_idx(&A.cell_codes[(size_t)c * NB + b]);
                 wc[h] = v[h] ? (A.cw ? ld_idx(&A.cw[c]) : T(1)) : T(0);
             }
 #pragma unroll
@@ -1031,6 +1035,7 @@ template <class T>
 inline OpArgs<T> op_args_impl(const Absorber& a) {
     OpArgs<T> A{};
     A.cell_codes = a.cell_codes.p;
+    A.large_codes = a.small_dense && a.nb > 1 ? a.large_codes.p : nullptr;
     A.seg_off = a.seg_off.p;
     A.cell_obs_off = a.trivial() ? nullptr : a.cell_obs_off.p;
     const WeightArrays<T>* W;

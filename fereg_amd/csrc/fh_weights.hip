@@ -18,6 +18,15 @@ namespace {
 constexpr unsigned TPB = 256;
 inline unsigned blocks_for(size_t n) { return grid_for(n, TPB, 8192); }
 
+// out[c][b - 1] = codes[c][b] for b = 1 .. nb-1.
+__global__ void k_drop_block0(const uint32_t* __restrict__ codes, size_t M, int nb, uint32_t* __restrict__ out) {
+    const size_t w = (size_t)(nb - 1), total = M * w;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = e / w, b = e - c * w;
+        out[e] = codes[c * nb + 1 + b];
+    }
+}
+
 __global__ void k_cell_weights(const uint32_t* __restrict__ coo, const double* __restrict__ w, size_t M,
                                double* __restrict__ cw, float* __restrict__ cw32) {
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
@@ -503,6 +512,12 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
     // dense small-row operator: one small block of <= 32 levels whose D and C are not much
     // larger than the cell records a level pass would stream
     a.small_dense = dense_small;
+    if (a.small_dense && nb > 1 && a.large_codes.n == 0) {
+        a.large_codes.alloc(M * (nb - 1));
+        hipLaunchKernelGGL(k_drop_block0, dim3(blocks_for(M * (nb - 1))), dim3(TPB), 0, st, a.cell_codes.p, M, nb,
+                           a.large_codes.p);
+        FH_CHECK_LAUNCH();
+    }
     if (a.small_dense) {
         // E = [D | pad | C | pad], ms rows of pitch sm_ld: every column quad is 16-byte aligned
         const size_t mL = m - ms;
