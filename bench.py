@@ -145,8 +145,11 @@ def run_ols(args, np, torch, F, synth):
         log("cpu oracle s/call", cpu_s, "dbeta", dbeta)
     config = {"workload": f"ols y~x1 | indiv+firm+year, n={n}, I={I}, T=10, F={Fn}, {args.design} graph, iid",
               "n": n, "fe": [I, Fn, 10], "tol": 1e-8}
+    rm, rf = results["mixed"], results["fp64"]
     line(args, timings["mixed"], timings["mixed"] * 1e3, "fp32+fp64", config, roof, cpu,
          {"plain_fp64_s_per_call": timings["fp64"],
+          "mixed_vs_fp64": {"rel_dbeta": float(np.max(np.abs(rm.coef - rf.coef) / np.abs(rf.coef))),
+                            "rel_dse": float(np.max(np.abs(rm.se - rf.se) / rf.se))},
           "pcg_iterations": {k: int(v.iterations) for k, v in results.items()}, "parity_vs_oracle": dbeta,
           "structure": {"cells": int(info.ncells), "m": int(info.m), "m_small": int(info.m_small)}})
 

@@ -239,6 +239,7 @@ struct PcgResult {
     bool exact = false;
     int rc = FH_OK;
     uint64_t refinements = 0;
+    bool accepted_fp32 = false;  // mixed form: the last correction passed without an FP64 residual
 };
 // Solves S x = rhs for r columns; x holds the start (x_maybe_nonzero false: x is zero, which
 // spares the device read that tests it). FP64 throughout.

@@ -205,3 +205,32 @@ def test_pool_reuses_and_releases_scratch():
     assert peaks[2] == peaks[1]
     F.release_cached_memory()
     assert F.memory_stats()[1] == 0
+
+
+@pytest.mark.parametrize("design,weighted", [("simple", False), ("difficult", False), ("simple", True)])
+def test_mixed_form_meets_the_fp64_accuracy(design, weighted):
+    """At tol 1e-8 the mixed form's last correction is usually accepted from its FP32 recursive
+    residual (a rounding bound instead of an FP64 S*p): its residualized columns must stay as
+    close to a tight FP64 solve as the FP64 form at the same tolerance."""
+    import fereg_amd as F
+    from fereg_amd import synth
+    d = synth.panel(200_000, T=10, F=2_000, design=design, seed=31)
+    codes = [d["indiv"], d["firm"], d["year"]]
+    n = d["n"]
+    rng = np.random.default_rng(32)
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    v = np.column_stack([d["y"], d["X"][:, 0], rng.standard_normal(n)])
+    out = {}
+    for name, prec, tol in (("ref", F.FH_PLAIN_FP64, 1e-14), ("fp64", F.FH_PLAIN_FP64, 1e-8),
+                            ("mixed", F.FH_MIXED, 1e-8)):
+        a = F.Absorber(codes, d["g"], weights=w, precision=prec)
+        pi = a.row_order()
+        rows = np.ascontiguousarray(v[pi])
+        announce("mixed accuracy", design, weighted, name)
+        a.residualize(rows, 3, F.SolveOpts(tol=tol, maxiter=10_000))
+        out[name] = rows
+        a.close()
+    scale = np.sqrt((out["ref"] ** 2).sum(0))
+    err = {k: np.max(np.abs(out[k] - out["ref"]) / scale) for k in ("fp64", "mixed")}
+    assert err["mixed"] <= max(10 * err["fp64"], 1e-10), err
+    assert err["mixed"] <= 1e-7, err
