@@ -306,11 +306,18 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
     hipLaunchKernelGGL(k_wsq_b1, dim3(pg), dim3(TPB), 0, st, a.b1.p, a.n1.p, g1, rp, a.s64.dpart.p + nws);
     FH_CHECK_LAUNCH();
     // one fixed-order device reduction for each, one readback
-    DBuf<double> sums(3 * (size_t)rp);
+    DBuf<double> sums(3 * (size_t)rp + 1);
     sum_parts_to(a.s64.dpart.p, grid, (size_t)(2 * rp), sums.p, st);
     sum_parts_to(a.s64.dpart.p + nws, pg, (size_t)rp, sums.p + 2 * rp, st);
-    std::vector<double> hs(3 * (size_t)rp);
+    const bool with_sw = !a.sw_ready;  // the weight sum rides along when it is still on the device
+    if (with_sw)
+        FH_HIP(hipMemcpyAsync(sums.p + 3 * rp, a.sw_dev.p, sizeof(double), hipMemcpyDeviceToDevice, st));
+    std::vector<double> hs(3 * (size_t)rp + (with_sw ? 1 : 0));
     d2h(hs.data(), sums.p, hs.size(), st);
+    if (with_sw) {
+        a.sw = hs[3 * rp];
+        a.sw_ready = true;
+    }
     check_interrupt(ctx);
     const std::vector<double> ws(hs.begin(), hs.begin() + 2 * rp);
     const double* hp = hs.data() + 2 * rp;
@@ -319,7 +326,7 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
     for (int j = 0; j < rp; j++) {  // pad columns: all sums zero, so ss0 = 0 (inactive)
         const double b1sq = hp[j];
         const double wsq = ws[j], wsum = ws[rp + j];
-        const double v = wsq - wsum * wsum / a.sw;
+        const double v = wsq - wsum * wsum / absorber_sw(a);
         ss0[j] = v > 1e-15 * wsq ? v : 0.0;
         ss1[j] = wsq - b1sq;
         zero[j] = ss0[j] == 0.0;
@@ -1063,7 +1070,7 @@ int fh_absorber_copy_weight_state(fh_absorber* ab, double* n1, double* nr, doubl
         if (nr) d2h(nr, a.nr.p, a.m, a.st);
         if (diag) d2h(diag, a.diag.p, a.m, a.st);
         if (red) d2h(red, a.red.p, a.m, a.st);
-        if (sw) *sw = a.sw;
+        if (sw) *sw = absorber_sw(a);
         stream_wait(a.st);
     });
 }
@@ -1232,7 +1239,7 @@ int fh_absorber_term_values(fh_absorber* ab, const double* coefs, size_t r, size
             if ((int64_t)t == a.norm_fe1) continue;
             double mean = 0.0;
             for (size_t l = 0; l < vals[t].size(); l++) mean += vals[t][l] * lw[t][l];
-            mean /= a.sw;
+            mean /= absorber_sw(a);
             for (double& v : vals[t]) v -= mean;
             shift += mean;
         }

@@ -105,7 +105,11 @@ struct Absorber {
     // weights
     bool weighted = false;
     DBuf<double> w_rows;  // n (row order) when weighted
+    // sum of the weights: reduced on the device at each weights refresh (sw_dev) and read with
+    // the next readback that needs it (absorber_sw, or the rhs sums of a residualization)
     double sw = 0.0;
+    bool sw_ready = false;
+    DBuf<double> sw_dev;
     DBuf<double> n1, nr, diag;
     DBuf<uint8_t> red;
     uint64_t nkeep = 0;
@@ -163,6 +167,9 @@ inline bool small_dense_rule(const Absorber& a) {
 }
 
 // ---------------------------------------------------------------- weights (fh_weights.hip)
+// The sum of the weights on the host (one readback after a refresh, none when a residualization
+// has already brought it back).
+double absorber_sw(Absorber& a);
 void refresh_weights(Absorber& a, const double* w_rows_dev /* null = unit */);
 void ensure_w32(Absorber& a);
 

@@ -577,8 +577,9 @@ void build_block_transpose(Absorber& a, int b) {
                            b, (uint32_t)bw, g, pl, keys.p, flag.p);
         hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, M);
         FH_CHECK_LAUNCH();
-        if (read_scalar(flag.p, st) != 0)
-            radix_sort_pairs(keys.p, a.bperm[b].p, kalt.p, valt.p, M, pl.ib, st);
+        // a large block's (bucket, level) keys are practically never in cell order already: sort
+        // without reading the kernel's sortedness flag back (a host round trip)
+        radix_sort_pairs(keys.p, a.bperm[b].p, kalt.p, valt.p, M, pl.ib, st);
         const uint64_t mask = pl.ib >= 64 ? ~0ull : ((1ull << pl.ib) - 1);
         hipLaunchKernelGGL(k_offsets_u64, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, (size_t)items,
                            a.boff[b].p, mask);

@@ -441,6 +441,14 @@ std::vector<double> grounded_inverse(const std::vector<double>& a, size_t ms, co
 
 }  // namespace
 
+double absorber_sw(Absorber& a) {
+    if (!a.sw_ready) {
+        download(&a.sw, a.sw_dev.p, sizeof(double), a.st);
+        a.sw_ready = true;
+    }
+    return a.sw;
+}
+
 void refresh_weights(Absorber& a, const double* w_rows_dev) {
     hipStream_t st = a.st;
     const size_t M = a.M, g1 = a.g1, m = a.m, ms = a.ms;
@@ -468,7 +476,14 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
     hipLaunchKernelGGL(k_n1, dim3(blocks_for(g1)), dim3(TPB), 0, st, a.seg_off.p, cw, g1, a.n1.p, a.w64.inv_n1.p,
                        a.w32.inv_n1.p);
     FH_CHECK_LAUNCH();
-    a.sw = sum_dev(a.n1.p, g1, st);
+    {  // the weight sum stays on the device until a readback needs it
+        DBuf<double> part(256);
+        a.sw_dev.reserve(1);
+        hipLaunchKernelGGL(k_partial_sum, dim3(256), dim3(TPB), 0, st, a.n1.p, g1, part.p);
+        FH_CHECK_LAUNCH();
+        sum_parts_to(part.p, 256, 1, a.sw_dev.p, st);
+        a.sw_ready = false;
+    }
     a.nr.reserve(std::max<size_t>(m, 1));
     a.diag.reserve(std::max<size_t>(m, 1));
     a.red.reserve(std::max<size_t>(m, 1));
@@ -583,7 +598,7 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             }
         } else {
             // 64-bit fixed point with every partial sum below 2^62 (terms and sums are <= sw)
-            const double fscale = std::ldexp(1.0, 62 - (int)std::ceil(std::log2(std::max(a.sw, 1.0)) + 1e-12));
+            const double fscale = std::ldexp(1.0, 62 - (int)std::ceil(std::log2(std::max(absorber_sw(a), 1.0)) + 1e-12));
             const bool lds = ms <= (size_t)SA_LDS_MAX;
             const unsigned grid = lds ? std::min<unsigned>(blocks_for(g1), 512) : blocks_for(g1);
             const size_t nparts = lds ? grid : 1;
