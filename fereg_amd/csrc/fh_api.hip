@@ -71,10 +71,28 @@ __global__ void k_gather_x(const double* __restrict__ X, size_t K, const uint32_
 }
 
 // flag = 1 when some code >= g (structure.rs:483-489) or some weight is not positive and finite.
-__global__ void k_check_codes(const uint32_t* __restrict__ c, size_t n, uint64_t g, uint32_t* flag) {
+// The same for up to NB_MAX + 1 terms in one launch (blockIdx.y = term), 16-byte loads where the
+// array is 16-byte aligned.
+struct CodeCheck {
+    const uint32_t* c[NB_MAX + 1];
+    uint64_t g[NB_MAX + 1];
+};
+__global__ void k_check_codes_all(CodeCheck cc, size_t n, uint32_t* flag) {
+    const uint32_t* c = cc.c[blockIdx.y];
+    const uint64_t g = cc.g[blockIdx.y];
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
     bool bad = false;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        bad |= c[i] >= g;
+    size_t head = 0;
+    if ((reinterpret_cast<uintptr_t>(c) & 15) == 0) {
+        const size_t nq = n / 4;
+        const uint4* c4 = reinterpret_cast<const uint4*>(c);
+        for (size_t i = t0; i < nq; i += stride) {
+            const uint4 v = c4[i];
+            bad |= v.x >= g || v.y >= g || v.z >= g || v.w >= g;
+        }
+        head = nq * 4;
+    }
+    for (size_t i = head + t0; i < n; i += stride) bad |= c[i] >= g;
     if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1u);
 }
 __global__ void k_check_weights(const double* __restrict__ w, size_t n, uint32_t* flag) {
@@ -870,8 +888,16 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         {
             DBuf<uint32_t> bad(1);
             FH_HIP(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), a.st));
-            for (size_t q = 0; q < nterms; q++)
-                hipLaunchKernelGGL(k_check_codes, dim3(1024), dim3(TPB), 0, a.st, dc[q], n, terms[q].g, bad.p);
+            for (size_t q0 = 0; q0 < nterms; q0 += NB_MAX + 1) {
+                CodeCheck cc{};
+                const size_t nq = std::min<size_t>(nterms - q0, NB_MAX + 1);
+                for (size_t q = 0; q < nq; q++) {
+                    cc.c[q] = dc[q0 + q];
+                    cc.g[q] = terms[q0 + q].g;
+                }
+                hipLaunchKernelGGL(k_check_codes_all, dim3(grid_for(n / 4 + 1, TPB, 2048), (unsigned)nq), dim3(TPB), 0,
+                                   a.st, cc, n, bad.p);
+            }
             FH_CHECK_LAUNCH();
             uint32_t hb = 0;
             d2h(&hb, bad.p, 1, a.st);

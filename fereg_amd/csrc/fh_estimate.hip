@@ -298,7 +298,7 @@ void cluster_runs(Absorber& a, const uint32_t* cl_kept_dev, uint64_t G, ClusterR
                        a.pi.p, n, cr.run_off.p, keys.p, cr.order.p);
     FH_CHECK_LAUNCH();
     // stable: a cluster's runs stay in row order
-    radix_sort_pairs(keys.p, cr.order.p, kalt.p, valt.p, nr, (int)bits_for(G), st);
+    radix_sort_pairs(keys, cr.order, kalt, valt, nr, (int)bits_for(G), st);
     hipLaunchKernelGGL(k_cl_offsets, dim3(grid_for(nr, TPB, 4096)), dim3(TPB), 0, st, keys.p, (size_t)nr, (size_t)G,
                        cr.off.p);
     FH_CHECK_LAUNCH();

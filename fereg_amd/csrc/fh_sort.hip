@@ -230,9 +230,10 @@ __global__ void sorted_check(const uint64_t* __restrict__ keys, size_t n, uint32
 
 }  // namespace
 
-void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
-                      int end_bit, hipStream_t st) {
-    if (n <= 1 || end_bit <= 0) return;
+// Runs the passes; returns true when the sorted data ended in the alternate buffers.
+static bool radix_passes(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
+                         int end_bit, hipStream_t st) {
+    if (n <= 1 || end_bit <= 0) return false;
     StreamScope ss(st);  // scratch is stream-ordered on st
     const uint32_t ntiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
     DBuf<uint32_t> hist((size_t)256 * ntiles);
@@ -250,9 +251,23 @@ void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32
         std::swap(ka, kb);
         std::swap(va, vb);
     }
-    if (ka != keys) {
-        FH_HIP(hipMemcpyAsync(keys, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
-        FH_HIP(hipMemcpyAsync(vals, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    return ka != keys;
+}
+
+void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
+                      int end_bit, hipStream_t st) {
+    if (radix_passes(keys, vals, keys_alt, vals_alt, n, end_bit, st)) {
+        FH_HIP(hipMemcpyAsync(keys, keys_alt, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+        FH_HIP(hipMemcpyAsync(vals, vals_alt, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    }
+}
+
+void radix_sort_pairs(DBuf<uint64_t>& keys, DBuf<uint32_t>& vals, DBuf<uint64_t>& keys_alt, DBuf<uint32_t>& vals_alt,
+                      size_t n, int end_bit, hipStream_t st) {
+    require(keys.n >= n && vals.n >= n && keys_alt.n >= n && vals_alt.n >= n, "radix sort: buffers shorter than n");
+    if (radix_passes(keys.p, vals.p, keys_alt.p, vals_alt.p, n, end_bit, st)) {
+        std::swap(keys, keys_alt);
+        std::swap(vals, vals_alt);
     }
 }
 

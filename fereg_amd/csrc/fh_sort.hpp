@@ -12,6 +12,12 @@ namespace fh {
 void radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
                       int end_bit, hipStream_t st);
 
+// The same on owned buffers: when the passes leave the result in the alternate buffers, the
+// buffers are swapped instead of copied back (keys/vals then own the sorted data, the alternates
+// the scratch). All four hold at least n elements.
+void radix_sort_pairs(DBuf<uint64_t>& keys, DBuf<uint32_t>& vals, DBuf<uint64_t>& keys_alt, DBuf<uint32_t>& vals_alt,
+                      size_t n, int end_bit, hipStream_t st);
+
 // out[i] = sum_{j<i} in[i]; *total_dev (may be null) = sum of all. in and out may alias.
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total_dev, hipStream_t st);
 

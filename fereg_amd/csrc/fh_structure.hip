@@ -207,6 +207,8 @@ __device__ __forceinline__ uint64_t bucket_key(const uint32_t* __restrict__ cell
     return k;
 }
 
+// keys[c] = (bucket, level) item of cell c with its payload; *unsorted (when given) |= the items
+// are not in cell order.
 __global__ void k_bucket_keys(const uint32_t* __restrict__ cell_codes, const uint32_t* __restrict__ cell_seg, size_t M,
                               int nb, int b, uint32_t bw, uint64_t g, PayloadLayout pl, uint64_t* __restrict__ keys,
                               uint32_t* __restrict__ unsorted) {
@@ -215,12 +217,13 @@ __global__ void k_bucket_keys(const uint32_t* __restrict__ cell_codes, const uin
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
         const uint64_t k = bucket_key(cell_codes, cell_seg, c, nb, b, bw, g, pl);
         keys[c] = k;
-        if (c > 0) {
+        if (unsorted && c > 0) {
             const uint32_t sp = cell_seg[c - 1];
             const uint64_t kp = (uint64_t)(sp / bw) * g + cell_codes[(c - 1) * nb + b];
             bad |= (k & mask) < kp;
         }
     }
+    if (!unsorted) return;
     __shared__ int any;
     if (threadIdx.x == 0) any = 0;
     __syncthreads();
@@ -327,7 +330,7 @@ static int64_t choose_fe1(Absorber& a, const std::vector<const uint32_t*>& dc) {
             hipLaunchKernelGGL(k_pair_keys, dim3(blocks_for(n)), dim3(TPB), 0, st, dc[f], dc[o], bo, n, keys.p);
             hipLaunchKernelGGL(k_iota, dim3(blocks_for(n)), dim3(TPB), 0, st, vals.p, n);
             FH_CHECK_LAUNCH();
-            radix_sort_pairs(keys.p, vals.p, kalt.p, valt.p, n, (int)(bits_for(g[f]) + bo), st);
+            radix_sort_pairs(keys, vals, kalt, valt, n, (int)(bits_for(g[f]) + bo), st);
             FH_HIP(hipMemsetAsync(d.p, 0, d.bytes(), st));
             FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
             hipLaunchKernelGGL(k_distinct, dim3(blocks_for(n)), dim3(TPB), 0, st, keys.p, n, bo, d.p);
@@ -434,7 +437,7 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
                 check_sorted_u64(keys.p, n, flag.p, st);
                 sorted = read_scalar(flag.p, st) == 0;
             }
-            if (!sorted) radix_sort_pairs(keys.p, a.pi.p, kalt.p, valt.p, n, (int)width, st);
+            if (!sorted) radix_sort_pairs(keys, a.pi, kalt, valt, n, (int)width, st);
             first = false;
         }
     }
@@ -564,7 +567,7 @@ void build_block_transpose(Absorber& a, int b) {
     a.lother[b].alloc(std::max<size_t>(M * (nb - 1), 1));
     if (M > 0) {
         DBuf<uint64_t> keys(M), kalt(M);
-        DBuf<uint32_t> valt(M), flag(1);
+        DBuf<uint32_t> valt(M);
         PayloadLayout pl{};
         pl.ib = std::max(1, (int)bits_for(items));
         pl.sb = std::max(1, (int)bits_for(a.g1));
@@ -572,14 +575,13 @@ void build_block_transpose(Absorber& a, int b) {
         for (int bb = 0; bb < nb; bb++)
             if (bb != b) { pl.ob[o] = std::max(1, (int)bits_for(a.blocks[bb].g)); tot += pl.ob[o++]; }
         pl.pay = tot <= 64;
-        FH_HIP(hipMemsetAsync(flag.p, 0, sizeof(uint32_t), st));
         hipLaunchKernelGGL(k_bucket_keys, dim3(blocks_for(M)), dim3(TPB), 0, st, a.cell_codes.p, a.cell_seg.p, M, nb,
-                           b, (uint32_t)bw, g, pl, keys.p, flag.p);
+                           b, (uint32_t)bw, g, pl, keys.p, (uint32_t*)nullptr);
         hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.bperm[b].p, M);
         FH_CHECK_LAUNCH();
         // a large block's (bucket, level) keys are practically never in cell order already: sort
-        // without reading the kernel's sortedness flag back (a host round trip)
-        radix_sort_pairs(keys.p, a.bperm[b].p, kalt.p, valt.p, M, pl.ib, st);
+        // without testing that (the test costs a host round trip and a second read of the codes)
+        radix_sort_pairs(keys, a.bperm[b], kalt, valt, M, pl.ib, st);
         const uint64_t mask = pl.ib >= 64 ? ~0ull : ((1ull << pl.ib) - 1);
         hipLaunchKernelGGL(k_offsets_u64, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, (size_t)items,
                            a.boff[b].p, mask);
@@ -615,7 +617,7 @@ void ensure_level_cells(Absorber& a, size_t b) {
     hipLaunchKernelGGL(k_iota, dim3(blocks_for(M)), dim3(TPB), 0, st, a.lperm[b].p, M);
     FH_CHECK_LAUNCH();
     check_sorted_u64(keys.p, M, flag.p, st);
-    if (read_scalar(flag.p, st) != 0) radix_sort_pairs(keys.p, a.lperm[b].p, kalt.p, valt.p, M, (int)bits_for(g), st);
+    if (read_scalar(flag.p, st) != 0) radix_sort_pairs(keys, a.lperm[b], kalt, valt, M, (int)bits_for(g), st);
     hipLaunchKernelGGL(k_offsets_u64, dim3(blocks_for(M)), dim3(TPB), 0, st, keys.p, M, (size_t)g, a.loff[b].p);
     FH_CHECK_LAUNCH();
 }
