@@ -127,6 +127,9 @@ struct Absorber {
     // bytes than a level pass over the cells), set by refresh_weights
     bool small_dense = false;
     size_t sm_ld = 0, sm_coff = 0;  // pitch of E and its first C column (multiples of 4)
+    // E as u8 counts (no prior weights), valid while *esm8_bad == 0 on the device
+    DBuf<uint8_t> esm8;
+    DBuf<uint32_t> esm8_bad;
 
     // S*p plans per precision (FP32, FP64) and column class (r <= 1, 2, 4, 8, 12, 16); with
     // autotune the first S*p of a class times the candidates (autotune_spmv)

@@ -297,6 +297,23 @@ __device__ __forceinline__ void ld_quad(const T* __restrict__ src, T (&e)[4]) {
     }
 }
 
+// The same from the u8 copy of E (cell counts without prior weights) when use8. Only the
+// standalone D'p_S / C'p_S product reads it: in the fused level pass the second load path costs
+// registers that the latency-bound gathers need (measured slower there).
+template <class T>
+__device__ __forceinline__ void ld_quad_e(const T* __restrict__ E, const uint8_t* __restrict__ E8, bool use8,
+                                          size_t idx, T (&e)[4]) {
+    if (use8) {
+        const uint32_t v = *reinterpret_cast<const uint32_t*>(E8 + idx);
+        e[0] = T(v & 0xFFu);
+        e[1] = T((v >> 8) & 0xFFu);
+        e[2] = T((v >> 16) & 0xFFu);
+        e[3] = T(v >> 24);
+    } else {
+        ld_quad(E + idx, e);
+    }
+}
+
 // Levels per chunk (blockIdx.y) of k_small_dense: KC x RM accumulators per thread.
 template <class T, int RM>
 constexpr int small_chunk() { return (sizeof(T) == 8 ? 32 : 64) / RM < 32 ? (sizeof(T) == 8 ? 32 : 64) / RM : 32; }
@@ -305,11 +322,13 @@ constexpr int small_chunk() { return (sizeof(T) == 8 ? 32 : 64) / RM < 32 ? (siz
 // small block's contributions to the segment sums and to the large rows, one column quad per
 // thread (E read once, coalesced; p_S staged in LDS).
 template <class T, int RM>
-__global__ __launch_bounds__(WG) void k_small_dense_t(const T* __restrict__ E, size_t ld, size_t coff, size_t g1,
-                                                      size_t mL, const T* __restrict__ pS, int ms, int r,
+__global__ __launch_bounds__(WG) void k_small_dense_t(const T* __restrict__ E, const uint8_t* __restrict__ E8,
+                                                      const uint32_t* __restrict__ bad8, size_t ld, size_t coff,
+                                                      size_t g1, size_t mL, const T* __restrict__ pS, int ms, int r,
                                                       T* __restrict__ ds, T* __restrict__ dl,
                                                       const uint32_t* __restrict__ done) {
     if (done && *done) return;
+    const bool use8 = E8 && *bad8 == 0;
     __shared__ T ps[32 * RM];
     for (int e = threadIdx.x; e < ms * RM; e += WG) {
         const int k = e / RM, j = e % RM;
@@ -327,7 +346,7 @@ __global__ __launch_bounds__(WG) void k_small_dense_t(const T* __restrict__ E, s
 #pragma unroll 8
         for (int k = 0; k < ms; k++) {
             T e[4];
-            ld_quad(E + (size_t)k * ld + c4, e);
+            ld_quad_e(E, E8, use8, (size_t)k * ld + c4, e);
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -1142,13 +1161,15 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
     const unsigned grid = grid_for(a.g1, (unsigned)((WAVE / L) * WAVES), 1u << 16);
     const size_t ms = a.ms, mL = a.m - a.ms;
     const T* E = a.small_dense ? W->esm.p : nullptr;
+    const uint8_t* E8 = a.small_dense && !a.weighted ? a.esm8.p : nullptr;  // u8 counts when they fit
+    const uint32_t* bad8 = a.small_dense ? a.esm8_bad.p : nullptr;
     const bool dense = a.small_dense && plan.dense && nb <= 4 && r <= 4;
     if (dense) {
         s.ds.reserve(a.g1 * r);
         s.dl.reserve(std::max<size_t>(mL * r, 1));
         FH_DISPATCH_R(r, hipLaunchKernelGGL((k_small_dense_t<T, RM>), dim3(grid_for(a.sm_ld / 4, WG, 4096)), dim3(WG),
-                                            0, st, E, a.sm_ld, a.sm_coff, a.g1, mL, p, (int)ms, r, s.ds.p, s.dl.p,
-                                            done));
+                                            0, st, E, E8, bad8, a.sm_ld, a.sm_coff, a.g1, mL, p, (int)ms, r, s.ds.p,
+                                            s.dl.p, done));
         FH_CHECK_LAUNCH();
     }
     if (nb <= 4) {

@@ -214,6 +214,13 @@ __global__ void k_fixed_to_double(const unsigned long long* __restrict__ parts, 
     }
 }
 
+// E8[i] = v when v is an integer in [0, 255], else *bad = 1 (the u8 copy of E is then unused).
+__device__ __forceinline__ void put_e8(uint8_t* __restrict__ E8, uint32_t* __restrict__ bad, size_t i, double v) {
+    if (!E8) return;
+    if (v >= 0.0 && v <= 255.0 && v == rint(v)) E8[i] = (uint8_t)v;
+    else atomicOr(bad, 1u);
+}
+
 // Level sums c_{s,k} of the single small block (block 0, whose codes ascend within a segment),
 // one thread per segment, the row in registers. Optional outputs: row s of
 // C = [c_{s,0..ms-1} / sqrt(n1_s), sqrt(n1_s)] (ld ms + 1) and the dense D[k][s] = c_{s,k}
@@ -224,7 +231,8 @@ __global__ __launch_bounds__(TPB) void k_seg_level_sums(const uint32_t* __restri
                                                         const double* __restrict__ cw, const double* __restrict__ n1,
                                                         size_t g1, int nb, uint32_t off0, int ms,
                                                         double* __restrict__ C, size_t ldd, double* __restrict__ D64,
-                                                        float* __restrict__ D32, int L) {
+                                                        float* __restrict__ D32, uint8_t* __restrict__ D8,
+                                                        uint32_t* __restrict__ bad8, int L) {
     // L lanes per segment (its cells strided over the lanes, register bins, shuffle reduction)
     const int l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
@@ -249,7 +257,11 @@ __global__ __launch_bounds__(TPB) void k_seg_level_sums(const uint32_t* __restri
         if (D64)
 #pragma unroll
             for (int k = 0; k < MS; k++)
-                if (k < ms) { D64[(size_t)k * ldd + s] = row[k]; D32[(size_t)k * ldd + s] = (float)row[k]; }
+                if (k < ms) {
+                    D64[(size_t)k * ldd + s] = row[k];
+                    D32[(size_t)k * ldd + s] = (float)row[k];
+                    put_e8(D8, bad8, (size_t)k * ldd + s, row[k]);
+                }
         if (!C) continue;
         const double sq = sqrt(n1[s]);
         const double f = sq > 0.0 ? 1.0 / sq : 0.0;
@@ -269,7 +281,8 @@ __global__ __launch_bounds__(TPB) void k_level_small_sums(const uint32_t* __rest
                                                           const uint32_t* __restrict__ lother, int nother,
                                                           const double* __restrict__ lw, uint32_t nbk, size_t g,
                                                           int ms, size_t ldc, size_t coff, int L,
-                                                          double* __restrict__ C64, float* __restrict__ C32) {
+                                                          double* __restrict__ C64, float* __restrict__ C32,
+                                                          uint8_t* __restrict__ C8, uint32_t* __restrict__ bad8) {
     const int l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
     const size_t nwaves = (size_t)gridDim.x * (TPB / WAVE);
@@ -297,6 +310,7 @@ __global__ __launch_bounds__(TPB) void k_level_small_sums(const uint32_t* __rest
             if (k < ms) {
                 C64[(size_t)k * ldc + coff + lv] = row[k];
                 C32[(size_t)k * ldc + coff + lv] = (float)row[k];
+                put_e8(C8, bad8, (size_t)k * ldc + coff + lv, row[k]);
             }
     }
 }
@@ -542,6 +556,16 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         a.w32.esm.reserve(ms * a.sm_ld);
         FH_HIP(hipMemsetAsync(a.w64.esm.p, 0, ms * a.sm_ld * sizeof(double), st));
         FH_HIP(hipMemsetAsync(a.w32.esm.p, 0, ms * a.sm_ld * sizeof(float), st));
+        // without prior weights E holds cell counts: a u8 copy when every count fits (the S*p
+        // kernels test the device flag and fall back to E64/E32 otherwise)
+        a.esm8_bad.reserve(1);
+        FH_HIP(hipMemsetAsync(a.esm8_bad.p, 0, sizeof(uint32_t), st));
+        if (!a.weighted) {
+            a.esm8.reserve(ms * a.sm_ld);
+            FH_HIP(hipMemsetAsync(a.esm8.p, 0, ms * a.sm_ld, st));
+        } else {
+            a.esm8.release();
+        }
         for (int b = 0; b < nb; b++) {
             const BlockInfo& bi = a.blocks[b];
             if (bi.small) continue;
@@ -551,17 +575,20 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             if (ms <= 16)
                 hipLaunchKernelGGL(k_level_small_sums<16>, dim3(grid), dim3(TPB), 0, st, a.boff[b].p, a.lother[b].p,
                                    nb - 1, unit ? nullptr : a.w64.lw[b].p, a.nbk[b], (size_t)bi.g, (int)ms, a.sm_ld,
-                                   a.sm_coff + (size_t)(bi.off - ms), (int)L, a.w64.esm.p, a.w32.esm.p);
+                                   a.sm_coff + (size_t)(bi.off - ms), (int)L, a.w64.esm.p, a.w32.esm.p,
+                                   a.weighted ? (uint8_t*)nullptr : a.esm8.p, a.esm8_bad.p);
             else
                 hipLaunchKernelGGL(k_level_small_sums<32>, dim3(grid), dim3(TPB), 0, st, a.boff[b].p, a.lother[b].p,
                                    nb - 1, unit ? nullptr : a.w64.lw[b].p, a.nbk[b], (size_t)bi.g, (int)ms, a.sm_ld,
-                                   a.sm_coff + (size_t)(bi.off - ms), (int)L, a.w64.esm.p, a.w32.esm.p);
+                                   a.sm_coff + (size_t)(bi.off - ms), (int)L, a.w64.esm.p, a.w32.esm.p,
+                                   a.weighted ? (uint8_t*)nullptr : a.esm8.p, a.esm8_bad.p);
             FH_CHECK_LAUNCH();
         }
         if (ms > 21)  // the S_SS assembly below does not produce D for these
             hipLaunchKernelGGL(k_seg_level_sums<32>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                a.cell_codes.p, cw, a.n1.p, g1, nb, (uint32_t)a.blocks[0].off, (int)ms, (double*)nullptr,
-                               a.sm_ld, a.w64.esm.p, a.w32.esm.p, (int)a.seg_lanes);
+                               a.sm_ld, a.w64.esm.p, a.w32.esm.p, a.weighted ? (uint8_t*)nullptr : a.esm8.p,
+                               a.esm8_bad.p, (int)a.seg_lanes);
         FH_CHECK_LAUNCH();
     }
     // small super-block
@@ -580,14 +607,15 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             DBuf<double> C(g1 * (ms + 1));
             double* D64 = a.small_dense ? a.w64.esm.p : nullptr;
             float* D32 = a.small_dense ? a.w32.esm.p : nullptr;
+            uint8_t* D8 = a.small_dense && !a.weighted ? a.esm8.p : nullptr;
             if (ms <= 8)
                 hipLaunchKernelGGL(k_seg_level_sums<8>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                    a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
-                                   (int)a.seg_lanes);
+                                   D8, a.esm8_bad.p, (int)a.seg_lanes);
             else
                 hipLaunchKernelGGL(k_seg_level_sums<21>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                    a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
-                                   (int)a.seg_lanes);
+                                   D8, a.esm8_bad.p, (int)a.seg_lanes);
             FH_CHECK_LAUNCH();
             std::vector<double> G((ms + 1) * (ms + 1));
             gram_cols(a, C.p, g1, (int)ms + 1, 0, (int)ms + 1, nullptr, nullptr, G.data());

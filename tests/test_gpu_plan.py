@@ -98,3 +98,31 @@ def test_invalid_plans_are_rejected():
         a.set_plan(F.FH_PLAIN_FP64, 2, {"dense": 1, "block_lanes": [128]})
     with pytest.raises(Exception):
         a.set_plan(F.FH_PLAIN_FP64, 2, {"dense": 1, "block_mode": [2]})
+
+
+@pytest.mark.parametrize("n,g1", [(100_000, 100), (40_000, 3000)])
+def test_dense_small_block_counts_beyond_u8(n, g1):
+    """Without prior weights E holds cell counts, read from a u8 copy when every count fits; with
+    100 FE1 levels, 5 small and 50 large levels most (small, large) counts exceed 255 and S*p
+    must fall back to E in the solve precision (the second design stays within u8)."""
+    import fereg_amd as F
+    rng = np.random.default_rng(14)
+    gs = [g1, 5, 50]
+    codes = [rng.integers(0, g, n).astype(np.uint32) for g in gs]
+    o = OracleSolver(codes, gs)
+    a = F.Absorber(codes, gs)
+    assert a.plan(F.FH_PLAIN_FP64, 2)["dense"] == 1
+    for r in (1, 2, 3):
+        p = rng.standard_normal(o.info.m * r)
+        yo = o.apply(p, r)
+        scale = np.abs(yo).max()
+        for prec, tol in ((F.FH_PLAIN_FP64, 1e-12), (F.FH_MIXED, 3e-5)):
+            announce("u8 counts", n, g1, r, prec)
+            np.testing.assert_allclose(a.apply(p, r, precision=prec), yo, rtol=0, atol=tol * scale)
+    v = rng.standard_normal((n, 2))
+    pi = a.row_order()
+    rows = np.ascontiguousarray(v[pi])
+    ro = rows.copy()
+    a.residualize(rows, 2, F.SolveOpts(tol=1e-10))
+    o.residualize(ro, 2, tol=1e-10)
+    np.testing.assert_allclose(rows, ro, atol=1e-8)
