@@ -350,7 +350,10 @@ __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restr
             const bool done_w = k >= WINDOW && sum <= thr;
             const bool at_roundoff = rn <= S->floor_[j];
             const bool rel = S->rel2 > 0.0 && rn <= S->rel2 * S->rho0[j];
-            if (done_w || at_roundoff || rel || !(rn > 0.0)) S->active[j] = 0;
+            // a correction solve of the mixed form also stops once its r'M^-1 r meets the outer
+            // test (absthr is that test's threshold; the caller then checks the rounding bound)
+            const bool at_abs = S->absthr[j] > 0.0 && rn <= S->absthr[j];
+            if (done_w || at_roundoff || rel || at_abs || !(rn > 0.0)) S->active[j] = 0;
             else beta = rn / S->rho[j];
         }
         S->beta[j] = beta;
@@ -661,7 +664,12 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
     for (int j = 0; j < r; j++) act[j] = ss0[j] > 0.0;
     DBuf<int32_t> mask(r);
     uint64_t total_iters = 0;
-    const double rel = 1e-5;  // inner FP32 solve: reduce r'M^-1 r by rel^2
+    // Inner FP32 solve: stops at its column thresholds (the window estimate, or r'M^-1 r below
+    // the outer test's threshold) or when r'M^-1 r has dropped by rel^2. When every column's
+    // threshold lies within 1e-10 of its r'M^-1 r, one sweep can finish (rel 1e-5); beyond that
+    // two sweeps are needed anyway, and a first stop at 1e-8 costs fewer iterations in total than
+    // one at 1e-10 (measured: 8 vs 9 on a 1e7-row panel).
+    const double rel_one = 1e-5, rel_split = 1e-4;
     for (int sweep = 0; sweep < 40; sweep++) {
         // FP64 outer residual r = rhs - S x and its preconditioned norm
         if (nz) {
@@ -695,6 +703,10 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
             absthr[j] = 0.01 * thr;
         }
         if (!any) break;
+        double need = 1.0;
+        for (int j = 0; j < r; j++)
+            if (act[j]) need = std::min(need, absthr[j] / rz[j]);
+        const double rel = need >= rel_one * rel_one ? rel_one : rel_split;
         res.refinements++;
         // FP32 correction solve S d = r from d = 0
         hipLaunchKernelGGL((k_convert<float, double>), dim3(vgrid(nv)), dim3(TPB), 0, st, s64.r.p, s32.rhs.p, nv);
