@@ -522,10 +522,18 @@ struct FitCommon {
 
 // V for a residualized block rows (n x R: [target, X...]) with e the residual combo applied
 // (fit/vcov.rs:316-382). bread over kept columns; scores x_i e_i w_i.
+// A fit that feeds a wild cluster bootstrap: the bootstrap's per-cluster sums are formed in the
+// CRV1 step (their first block is the CRV1 score), so the rows are gathered by cluster once.
+struct WbPre {
+    std::vector<double> rmat;  // restriction over all K regressors
+    DBuf<double> S;            // G x 3 x kept, set by vcov_from
+    bool ready = false;
+};
+
 void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
                const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
                const uint32_t* cl_dev, uint64_t G, const fh_ssc& ssc, double dfk, double iid_scale,
-               std::vector<double>& V, uint64_t* ncl) {
+               std::vector<double>& V, uint64_t* ncl, WbPre* wb = nullptr) {
     const int K = R - 1;
     std::vector<int> kept;
     for (int j = 0; j < K; j++)
@@ -547,8 +555,26 @@ void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>&
         } else {
             require(cl_dev != nullptr, "a clustered variance matrix needs cluster codes");
             // residuals from the rows' pad column when they are there: one row read per observation
-            uint64_t g = ecol >= 0 ? cluster_meat(a, rows, a.n, ld, 1, K, rows + ecol, (size_t)ld, w, cl_dev, G, meat.data())
-                                   : cluster_meat(a, rows, a.n, ld, 1, K, e, 1, w, cl_dev, G, meat.data());
+            const double* ep = ecol >= 0 ? rows + ecol : e;
+            const size_t es = ecol >= 0 ? (size_t)ld : 1;
+            uint64_t g = 0;
+            if (wb && mk + 2 <= 21) {
+                // the bootstrap's cluster sums; their first block, sum w e x~ over the kept
+                // regressors, is the CRV1 score (same runs, lanes and products: the same values)
+                std::vector<double> rk(mk), C, MR;
+                for (int i = 0; i < mk; i++) rk[i] = wb->rmat[kept[i]];
+                wb_constraint(bread, mk, rk.data(), C, MR);
+                wb_cluster_sums(a, rows, ld, kept, C, MR, ep, es, w, cl_dev, G, wb->S);
+                wb->ready = true;
+                std::vector<double> mkk0((size_t)mk * mk);
+                gram_cols(a, wb->S.p, G, 3 * mk, 0, mk, nullptr, nullptr, mkk0.data());
+                std::fill(meat.begin(), meat.end(), 0.0);
+                for (int i = 0; i < mk; i++)
+                    for (int j = 0; j < mk; j++) meat[(size_t)kept[i] * K + kept[j]] = mkk0[(size_t)i * mk + j];
+                g = count_nonempty_clusters(a, cl_dev, G);
+            } else {
+                g = cluster_meat(a, rows, a.n, ld, 1, K, ep, es, w, cl_dev, G, meat.data());
+            }
             if (ncl) *ncl = g;
             if (ssc.g_adj) factor *= (double)g / ((double)g - 1.0);
             if (ssc.k_adj) factor *= (n - 1.0) / (n - dfk);
@@ -595,6 +621,7 @@ struct OlsState {
     double dfk = 0.0;
     uint64_t ncl = 0;
     int ld = 0, ecol = -1;  // stride of the solver rows, their residual column (-1: none)
+    WbPre* wb = nullptr;    // set by the bootstrap: form its cluster sums in the CRV1 step
 };
 
 // OLS on device inputs (kept order). rows scratch: n x (1+K) in row order.
@@ -650,7 +677,7 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     uint64_t ncl = 0;
     const uint64_t G_cl = kind == FH_VCOV_CLUSTER ? cluster_levels(a, cl_dev) : 0;
     vcov_from(a, rows, (int)ld, (int)R, excluded, inv, e.p, ecol, w, kind, cl_dev, G_cl, ssc, dfk,
-              ssr / ((double)n - 1.0), V, &ncl);
+              ssr / ((double)n - 1.0), V, &ncl, state ? state->wb : nullptr);
     for (size_t j = 0; j < K; j++) {
         beta[j] = 0.0;
         kept_out[j] = !excluded[j];
@@ -1336,6 +1363,9 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
         } runs_scope{a};
         a.cl_runs = &runs;
         OlsState os;
+        WbPre pre;
+        pre.rmat.assign(spec->r_mat, spec->r_mat + K);
+        os.wb = &pre;
         fh_fit_stats stats{};
         ols_fit(a, py, px, K, FH_VCOV_CLUSTER, pc, nested, *ssc, *o, beta, vcov, kept, &stats, nullptr, &os);
         // R over the kept columns; a restriction on a dropped column is not testable
@@ -1367,6 +1397,7 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
         in.G = G;
         const double nn = (double)n, gg = (double)os.ncl;
         in.ssc = (ssc->k_adj ? (nn - 1.0) / (nn - os.dfk) : 1.0) * (ssc->g_adj && gg > 1.0 ? gg / (gg - 1.0) : 1.0);
+        in.S = pre.ready ? pre.S.p : nullptr;
         ols_wildboot(a, in, *spec, rk.data(), *res, t_boot);
         res->fe_not_projected = m_nn > 0;
     });

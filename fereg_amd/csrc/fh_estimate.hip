@@ -316,6 +316,18 @@ uint32_t lanes_per_cluster(const ClusterRuns& cr) {
     return L;
 }
 
+uint64_t count_nonempty_clusters(Absorber& a, const uint32_t* cl, uint64_t G) {
+    ClusterRuns local;
+    const ClusterRuns& cr = runs_for(a, cl, G, local);
+    DBuf<unsigned long long> cnt(1);
+    FH_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), a.st));
+    hipLaunchKernelGGL(k_count_nonempty, dim3(grid_for(G, TPB, 4096)), dim3(TPB), 0, a.st, cr.off.p, (size_t)G, cnt.p);
+    FH_CHECK_LAUNCH();
+    unsigned long long c = 0;
+    download(&c, cnt.p, sizeof(c), a.st);
+    return c;
+}
+
 // Returns the number of non-empty clusters.
 uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, size_t es,
                       const double* w, const uint32_t* cl_kept_dev, uint64_t G, double* meat) {

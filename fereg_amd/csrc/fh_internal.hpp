@@ -298,7 +298,18 @@ struct WbootInput {
     const uint32_t* cl;
     uint64_t G;
     double ssc;
+    const double* S = nullptr;  // per-cluster sums already formed by the fit (G x 3 x kept), or null
 };
+// ConstraintProj for one restriction rk over the kept regressors: C = bread rk, M_R = C / (rk'C).
+void wb_constraint(const std::vector<double>& bread, int mk, const double* rk, std::vector<double>& C,
+                   std::vector<double>& MR);
+// S[g][0..2][a] = sums over the rows of cluster g of (w e, w u1, w z) x~_a over the kept
+// regressors (z = x~'C, u1 = -x~'M_R), on the call's cluster runs; S[g][0] is the CRV1 score.
+void wb_cluster_sums(Absorber& a, const double* rows, int R, const std::vector<int>& kept, const std::vector<double>& C,
+                     const std::vector<double>& MR, const double* e, size_t es, const double* w, const uint32_t* cl,
+                     uint64_t G, DBuf<double>& S);
+// Non-empty clusters of the call's runs.
+uint64_t count_nonempty_clusters(Absorber& a, const uint32_t* cl, uint64_t G);
 void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* r_mat_kept,
                   fh_wboot_result& res, double* t_boot_host);
 // Per absorbed term: 1 when each of its levels lies in a single cluster (codes in kept order).

@@ -633,6 +633,45 @@ std::vector<int> terms_nested_in(Absorber& a, const uint32_t* cl) {
     return nested;
 }
 
+void wb_constraint(const std::vector<double>& bread, int mk, const double* rk, std::vector<double>& C,
+                   std::vector<double>& MR) {
+    // ConstraintProj (ols.rs:76-121) for one restriction
+    C.assign(mk, 0.0);
+    MR.assign(mk, 0.0);
+    for (int i = 0; i < mk; i++)
+        for (int j = 0; j < mk; j++) C[i] += bread[(size_t)i * mk + j] * rk[j];
+    double v0 = 0.0;
+    for (int j = 0; j < mk; j++) v0 += rk[j] * C[j];
+    const double v0i = std::fabs(v0) > 1e-12 * std::max(1.0, std::fabs(v0)) ? 1.0 / v0 : 0.0;
+    for (int i = 0; i < mk; i++) MR[i] = C[i] * v0i;
+}
+
+void wb_cluster_sums(Absorber& a, const double* rows, int R, const std::vector<int>& kept, const std::vector<double>& C,
+                     const std::vector<double>& MR, const double* e, size_t es, const double* w, const uint32_t* cl,
+                     uint64_t G, DBuf<double>& S) {
+    hipStream_t st = a.st;
+    const int mk = (int)kept.size();
+    require(mk >= 1 && mk + 2 <= KMAX, "the bootstrap supports 1..19 kept regressors");
+    DBuf<int> dkept(mk);
+    DBuf<double> dC(mk), dMR(mk);
+    FH_HIP(hipMemcpyAsync(dkept.p, kept.data(), mk * sizeof(int), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dC.p, C.data(), mk * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dMR.p, MR.data(), mk * sizeof(double), hipMemcpyHostToDevice, st));
+    ClusterRuns local;
+    const ClusterRuns& cr = runs_for(a, cl, G, local);
+    S.alloc(G * 3 * mk);
+    const uint32_t L = lanes_per_cluster(cr);
+    const unsigned cg = grid_for(G, (TPB / WAVE) * (WAVE / L), 1u << 16);
+    auto launch = [&](auto mkc) {
+        hipLaunchKernelGGL((k_wb_clsums<decltype(mkc)::value>), dim3(cg), dim3(TPB), 0, st, cr.off.p, cr.order.p,
+                           cr.run_off.p, (size_t)G, rows, R, dkept.p, mk, dC.p, dMR.p, e, es, w, (int)L, S.p);
+    };
+    if (mk <= 4) launch(std::integral_constant<int, 4>{});
+    else if (mk <= 12) launch(std::integral_constant<int, 12>{});
+    else launch(std::integral_constant<int, 20>{});
+    FH_CHECK_LAUNCH();
+}
+
 void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* rk, fh_wboot_result& res,
                   double* t_boot_host) {
     hipStream_t st = a.st;
@@ -645,42 +684,23 @@ void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, 
     require(spec.tol > 0.0 && std::isfinite(spec.tol) && spec.maxiter > 0, "wildboot: tol and maxiter must be positive");
     require(spec.pval_type >= 0 && spec.pval_type <= 3, "wildboot: unknown p-value type");
     auto t0 = std::chrono::steady_clock::now();
-    // ConstraintProj (ols.rs:76-121) for one restriction
-    std::vector<double> C(mk, 0.0), MR(mk, 0.0);
-    for (int i = 0; i < mk; i++)
-        for (int j = 0; j < mk; j++) C[i] += in.bread[(size_t)i * mk + j] * rk[j];
-    double v0 = 0.0;
-    for (int j = 0; j < mk; j++) v0 += rk[j] * C[j];
-    const double v0i = std::fabs(v0) > 1e-12 * std::max(1.0, std::fabs(v0)) ? 1.0 / v0 : 0.0;
-    for (int i = 0; i < mk; i++) MR[i] = C[i] * v0i;
-    DBuf<int> dkept(mk);
-    DBuf<double> dC(mk), dMR(mk), dbread((size_t)mk * mk);
-    FH_HIP(hipMemcpyAsync(dkept.p, in.kept.data(), mk * sizeof(int), hipMemcpyHostToDevice, st));
+    std::vector<double> C, MR;
+    wb_constraint(in.bread, mk, rk, C, MR);
+    DBuf<double> dC(mk), dbread((size_t)mk * mk);
     FH_HIP(hipMemcpyAsync(dC.p, C.data(), mk * sizeof(double), hipMemcpyHostToDevice, st));
-    FH_HIP(hipMemcpyAsync(dMR.p, MR.data(), mk * sizeof(double), hipMemcpyHostToDevice, st));
     FH_HIP(hipMemcpyAsync(dbread.p, in.bread.data(), (size_t)mk * mk * sizeof(double), hipMemcpyHostToDevice, st));
-    // per-cluster sums
-    ClusterRuns local;
-    const ClusterRuns& cr = runs_for(a, in.cl, G, local);
-    DBuf<double> S(G * 3 * mk);
-    {
-        const uint32_t L = lanes_per_cluster(cr);
-        const unsigned cg = grid_for(G, (TPB / WAVE) * (WAVE / L), 1u << 16);
-        auto launch = [&](auto mkc) {
-            hipLaunchKernelGGL((k_wb_clsums<decltype(mkc)::value>), dim3(cg), dim3(TPB), 0, st, cr.off.p, cr.order.p,
-                               cr.run_off.p, (size_t)G, in.rows, in.R, dkept.p, mk, dC.p, dMR.p, in.e, in.es, in.w, (int)L,
-                               S.p);
-        };
-        if (mk <= 4) launch(std::integral_constant<int, 4>{});
-        else if (mk <= 12) launch(std::integral_constant<int, 12>{});
-        else launch(std::integral_constant<int, 20>{});
+    // per-cluster sums (formed by the fit's CRV1 step when it ran for this bootstrap)
+    DBuf<double> Sown;
+    const double* Sp = in.S;
+    if (!Sp) {
+        wb_cluster_sums(a, in.rows, in.R, in.kept, C, MR, in.e, in.es, in.w, in.cl, G, Sown);
+        Sp = Sown.p;
     }
-    FH_CHECK_LAUNCH();
     const int NC = 2 + 4 * mk, QC = 2 + mk;
     DBuf<double> P(G * NC), Q(G * QC);
     const unsigned pg = grid_for(G, TPB, 512);
     DBuf<double> cmax((size_t)pg * NC);
-    hipLaunchKernelGGL(k_wb_prow, dim3(pg), dim3(TPB), 0, st, S.p, G, mk, dbread.p, dC.p, P.p, Q.p, cmax.p);
+    hipLaunchKernelGGL(k_wb_prow, dim3(pg), dim3(TPB), 0, st, Sp, G, mk, dbread.p, dC.p, P.p, Q.p, cmax.p);
     FH_CHECK_LAUNCH();
     std::vector<double> QG((size_t)QC * QC);
     gram_cols(a, Q.p, G, QC, 0, QC, nullptr, nullptr, QG.data());  // synchronizes
