@@ -758,7 +758,7 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
         refresh_weights(a, w.p);
         ResidOut ro = residualize_any(a, rows, R, tol_in, o.maxiter, have_warm ? warm.p : nullptr, coefs.p, prec, ctx);
         pcg_total += ro.iterations;
-        FH_HIP(hipMemcpyAsync(warm.p, coefs.p, p * R * sizeof(double), hipMemcpyDeviceToDevice, st));
+        std::swap(warm, coefs);  // this solve's coefficients start the next one (coefs is rewritten)
         have_warm = true;
         std::vector<double> G(R * R);
         gram_cols(a, rows, n, (int)R, 0, (int)R, w.p, nullptr, G.data());
@@ -781,7 +781,7 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
         c[0] = 1.0;
         for (int i = 0; i < mk; i++) c[kept[i] + 1] = -coef[i];
         residual_pass(a, rows, n, (int)R, (int)R, c.data(), nullptr, e.p, -1);
-        FH_HIP(hipMemcpyAsync(eta_old.p, eta.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        std::swap(eta_old, eta);  // k_irls_eta rewrites eta in full and reads eta_old only when halving
         double step = 1.0, dev = NAN;
         bool halved = false;
         for (int half = 0; half < 30; half++) {
