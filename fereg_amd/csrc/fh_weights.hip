@@ -552,17 +552,23 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         const size_t mL = m - ms;
         a.sm_coff = (g1 + 3) / 4 * 4;
         a.sm_ld = a.sm_coff + (mL + 3) / 4 * 4;
-        a.w64.esm.reserve(ms * a.sm_ld);
-        a.w32.esm.reserve(ms * a.sm_ld);
-        FH_HIP(hipMemsetAsync(a.w64.esm.p, 0, ms * a.sm_ld * sizeof(double), st));
-        FH_HIP(hipMemsetAsync(a.w32.esm.p, 0, ms * a.sm_ld * sizeof(float), st));
+        // every refresh rewrites all entries of E but the pads, so a buffer is zeroed when allocated
+        const size_t ne = ms * a.sm_ld;
+        if (a.w64.esm.n < ne) {
+            a.w64.esm.reserve(ne);
+            a.w32.esm.reserve(ne);
+            FH_HIP(hipMemsetAsync(a.w64.esm.p, 0, ne * sizeof(double), st));
+            FH_HIP(hipMemsetAsync(a.w32.esm.p, 0, ne * sizeof(float), st));
+        }
         // without prior weights E holds cell counts: a u8 copy when every count fits (the S*p
         // kernels test the device flag and fall back to E64/E32 otherwise)
         a.esm8_bad.reserve(1);
         FH_HIP(hipMemsetAsync(a.esm8_bad.p, 0, sizeof(uint32_t), st));
         if (!a.weighted) {
-            a.esm8.reserve(ms * a.sm_ld);
-            FH_HIP(hipMemsetAsync(a.esm8.p, 0, ms * a.sm_ld, st));
+            if (a.esm8.n < ne) {
+                a.esm8.reserve(ne);
+                FH_HIP(hipMemsetAsync(a.esm8.p, 0, ne, st));
+            }
         } else {
             a.esm8.release();
         }
