@@ -302,6 +302,7 @@ struct ResidOut {
     uint64_t iterations = 1;
     bool exact = true;
     uint64_t refinements = 0;
+    bool escalated = false;
 };
 
 // One residualization of a device block of r <= 16 columns (solve.rs:1141-1311). The solver's
@@ -365,9 +366,21 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
         hipLaunchKernelGGL(k_zero_cols, dim3(vgrid(m * rp)), dim3(TPB), 0, st, x, m, rp, dz.p);
         FH_CHECK_LAUNCH();
         const bool warm = warm_dev != nullptr;
-        PcgResult pr = precision == FH_MIXED
-                           ? pcg_mixed(a, a.s64.rhs.p, x, warm, rp, ss1.data(), ss0.data(), tol, maxiter, ctx)
-                           : pcg_fp64(a, a.s64.rhs.p, x, warm, rp, ss1.data(), ss0.data(), tol, maxiter, ctx);
+        auto solve = [&](bool from_x, bool can_esc, uint64_t mi) {
+            return precision == FH_MIXED
+                       ? pcg_mixed(a, a.s64.rhs.p, x, from_x, rp, ss1.data(), ss0.data(), tol, mi, ctx, can_esc)
+                       : pcg_fp64(a, a.s64.rhs.p, x, from_x, rp, ss1.data(), ss0.data(), tol, mi, ctx, can_esc);
+        };
+        PcgResult pr = solve(warm, escalation_possible(a), maxiter);
+        if (pr.escalate) {
+            // slow convergence at the probe: the first large block gets the exact block
+            // preconditioner and the solve restarts from the current iterate (cg.rs:264-274)
+            escalate(a);
+            const uint64_t used = pr.iterations;
+            pr = solve(true, false, maxiter > used ? maxiter - used : 1);
+            pr.iterations += used;
+            out.escalated = true;
+        }
         if (pr.rc == FH_CONVERGENCE)
             throw Error(FH_CONVERGENCE, "the FE solver did not converge within " + std::to_string(maxiter) + " iterations");
         out.iterations = std::max<uint64_t>(pr.iterations, 1);
@@ -417,6 +430,7 @@ ResidOut residualize_any(Absorber& a, double* z, size_t r, double tol, uint64_t 
         tot.iterations = std::max(tot.iterations, o.iterations);
         tot.exact = tot.exact && o.exact;
         tot.refinements = std::max(tot.refinements, o.refinements);
+        tot.escalated |= o.escalated;
     }
     return tot;
 }
@@ -1049,7 +1063,7 @@ static int residualize_impl(fh_absorber* ab, double* z, size_t r, const fh_solve
         if (info) {
             info->iterations = ro.iterations;
             info->exact = ro.exact;
-            info->escalated = 0;
+            info->escalated = ro.escalated ? 1 : 0;
             info->refinements = ro.refinements;
             info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }

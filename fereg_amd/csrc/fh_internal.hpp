@@ -144,6 +144,8 @@ struct Absorber {
     // that uses one clustering more than once)
     const ClusterRuns* cl_runs = nullptr;
 
+    bool escalated = false;  // a large block joined the small super-block (escalation)
+
     bool trivial() const { return cell_obs_off.n == 0; }
     bool unit() const { return !weighted && trivial(); }
     uint32_t seg_lanes = 8;  // lanes per segment in the segment passes
@@ -170,6 +172,14 @@ inline bool small_dense_rule(const Absorber& a) {
 }
 
 // ---------------------------------------------------------------- weights (fh_weights.hip)
+// Escalation (solve.rs:838-891, cg.rs:264-274): when the PCG converges slowly (rho_hat > 0.6 at
+// its probe) and one large block remains, it joins the small super-block, whose preconditioner is
+// then the exact grounded inverse of S (built on the GPU by pivot sweeps, O(ms) passes over ms^2
+// values), while the super-block stays within ESC_MAX coordinates; once per absorber. Other
+// designs keep Jacobi (the reference's sparse Cholesky of a large block is not built).
+constexpr uint64_t ESC_MAX = 8192;
+bool escalation_possible(const Absorber& a);
+void escalate(Absorber& a);
 // The sum of the weights on the host (one readback after a refresh, none when a residualization
 // has already brought it back).
 double absorber_sw(Absorber& a);
@@ -250,14 +260,15 @@ struct PcgResult {
     int rc = FH_OK;
     uint64_t refinements = 0;
     bool accepted_fp32 = false;  // mixed form: the last correction passed without an FP64 residual
+    bool escalate = false;       // stopped at the escalation probe (x holds the progress so far)
 };
 // Solves S x = rhs for r columns; x holds the start (x_maybe_nonzero false: x is zero, which
 // spares the device read that tests it). FP64 throughout.
 PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
-                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx);
+                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate = false);
 // Mixed form: FP32 inner PCG, FP64 outer residual and update.
 PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
-                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx);
+                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate = false);
 
 // ---------------------------------------------------------------- estimation (fh_estimate.hip)
 // G = sum_i w_i z_i z_i' (r x r, FP64 on the host), z n x r row-major; w may be null.

@@ -1216,6 +1216,7 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
     for (int b = 0; b < nb; b++) {
         const BlockInfo& bi = a.blocks[b];
         if (bi.small && a.small_dense) continue;
+        require(a.boff[b].n > 0 && a.lseg[b].n > 0, "S*p: a block without level records");
         const uint32_t nbk = a.nbk[b];
         const LvlMode md = lvl_mode(a, plan, b, r, (int)sizeof(T));
         const bool merge = md.merge;

@@ -22,14 +22,21 @@ constexpr int WINDOW = 5;  // cg.rs:23
 constexpr int DOTG = 256;  // workgroups of the dot kernels (fixed: fixed summation order)
 constexpr int TPB = 256;
 
+// Escalation probe (cg.rs:29-37, 254-274): r'z after PROBE_FROM iterations of a solve and the
+// largest rate rho_hat = (r'z_k / r'z_from)^(1 / (k - from)) at k = PROBE_AT over the active
+// columns; rho_hat > PROBE_RATIO asks for a stronger preconditioner (err = 2).
+constexpr int PROBE_FROM = 7, PROBE_AT = 12;
+constexpr double PROBE_RATIO = 0.6;
+
 struct PcgState {
     double rho[RMAX], rho_new[RMAX], gamma[RMAX], beta[RMAX], ss1[RMAX], ss0[RMAX], floor_[RMAX], thr[RMAX];
     double ring[RMAX][WINDOW];
-    double xb[RMAX], xr[RMAX], rho0[RMAX], absthr[RMAX];
+    double xb[RMAX], xr[RMAX], rho0[RMAX], absthr[RMAX], rho_from[RMAX];
     int32_t active[RMAX];
-    uint32_t done, err;
+    uint32_t done, err;  // err: 1 = maxiter reached, 2 = escalation requested
     uint64_t iters, k, maxiter;
     int32_t r;
+    int32_t can_escalate;
     double tol, rel2;
 };
 
@@ -360,13 +367,22 @@ __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restr
         S->rho_new[j] = rn;
         S->rho[j] = rn;
         if (S->active[j]) atomicOr(&any, 1);
+        if (k == PROBE_FROM) S->rho_from[j] = rn;
     }
+    __shared__ double rh[RMAX];
+    if (j < RMAX) rh[j] = 0.0;
+    __syncthreads();
+    if (k == PROBE_AT && S->can_escalate && j < S->r && S->active[j] && S->rho_from[j] > 0.0)
+        rh[j] = pow(S->rho_new[j] / S->rho_from[j], 1.0 / (double)(PROBE_AT - PROBE_FROM));
     __syncthreads();
     if (j == 0) {
         S->k = k;
         S->iters += 1;
+        double rho_hat = 0.0;
+        for (int q = 0; q < RMAX; q++) rho_hat = fmax(rho_hat, rh[q]);
         if (!any) S->done = 1;
         else if (S->iters >= S->maxiter) { S->err = 1; S->done = 1; }
+        else if (k == PROBE_AT && S->can_escalate && rho_hat > PROBE_RATIO) { S->err = 2; S->done = 1; }
     }
 }
 
@@ -464,12 +480,14 @@ struct PcgRun {
     uint64_t iterations = 0;
     bool exact = false;
     bool maxiter_hit = false;
+    bool escalate = false;  // stopped at the probe: the caller strengthens the preconditioner
 };
 
 // One PCG solve in precision T from x (x_nonzero: start from x, else from 0).
 template <class T>
 PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const double* ss1, const double* ss0,
-               double tol, uint64_t maxiter, double rel2, const double* absthr, const fh_ctx* ctx) {
+               double tol, uint64_t maxiter, double rel2, const double* absthr, const fh_ctx* ctx,
+               bool can_escalate = false) {
     hipStream_t st = a.st;
     Scratch<T>& s = scr<T>(a);
     const size_t m = a.m, ms = a.ms, nv = m * r;
@@ -483,6 +501,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     h.tol = tol;
     h.rel2 = rel2;
     h.maxiter = maxiter;
+    h.can_escalate = can_escalate ? 1 : 0;
     for (int j = 0; j < r; j++) {
         h.ss1[j] = ss1[j];
         h.ss0[j] = ss0[j];
@@ -572,7 +591,8 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
         }
     }
     out.iterations = pl.iters;
-    out.maxiter_hit = pl.err != 0;
+    out.maxiter_hit = pl.err == 1;
+    out.escalate = pl.err == 2;
     return out;
 }
 
@@ -630,25 +650,26 @@ std::vector<double> dots64(Absorber& a, const double* x, const double* y, int r)
 }  // namespace
 
 PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
-                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx) {
+                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate) {
     // a warm start may still be all zero: test it on the device
     bool nz = false;
     if (x_maybe_nonzero)
         for (double v : dots64(a, x, x, r)) nz |= v != 0.0;
-    PcgRun run = run_pcg<double>(a, rhs, x, nz, r, ss1, ss0, tol, maxiter, 0.0, nullptr, ctx);
+    PcgRun run = run_pcg<double>(a, rhs, x, nz, r, ss1, ss0, tol, maxiter, 0.0, nullptr, ctx, can_escalate);
     PcgResult res;
     res.iterations = run.iterations;
     res.exact = run.exact;
     res.rc = run.maxiter_hit ? FH_CONVERGENCE : FH_OK;
+    res.escalate = run.escalate;
     return res;
 }
 
 PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
-                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx) {
+                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate) {
     PcgResult res;
-    if (a.ms == a.m) {  // exact dense path: nothing to iterate
-        return pcg_fp64(a, rhs, x, x_maybe_nonzero, r, ss1, ss0, tol, maxiter, ctx);
-    }
+    // exact dense path (nothing to iterate), or an escalated exact block preconditioner, whose
+    // grounded inverse of a slowly mixing block is too ill-conditioned for FP32 sweeps
+    if (a.ms == a.m || a.escalated) return pcg_fp64(a, rhs, x, x_maybe_nonzero, r, ss1, ss0, tol, maxiter, ctx, false);
     ensure_w32(a);
     hipStream_t st = a.st;
     const size_t m = a.m, nv = m * r;
@@ -664,6 +685,7 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
     for (int j = 0; j < r; j++) act[j] = ss0[j] > 0.0;
     DBuf<int32_t> mask(r);
     uint64_t total_iters = 0;
+    bool finished = false;
     // Inner FP32 solve: stops at its column thresholds (the window estimate, or r'M^-1 r below
     // the outer test's threshold) or when r'M^-1 r has dropped by rel^2. When every column's
     // threshold lies within 1e-10 of its r'M^-1 r, one sweep can finish (rel 1e-5); beyond that
@@ -702,7 +724,10 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
             ss1c[j] = ss1[j];
             absthr[j] = 0.01 * thr;
         }
-        if (!any) break;
+        if (!any) {
+            finished = true;
+            break;
+        }
         double need = 1.0;
         for (int j = 0; j < r; j++)
             if (act[j]) need = std::min(need, absthr[j] / rz[j]);
@@ -713,14 +738,21 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
         FH_HIP(hipMemsetAsync(s32.x.p, 0, nv * sizeof(float), st));
         FH_CHECK_LAUNCH();
         PcgRun run = run_pcg<float>(a, s32.rhs.p, s32.x.p, false, r, ss1c.data(), ss0c.data(), tol,
-                                    maxiter > total_iters ? maxiter - total_iters : 1, rel * rel, absthr.data(), ctx);
+                                    maxiter > total_iters ? maxiter - total_iters : 1, rel * rel, absthr.data(), ctx,
+                                    can_escalate && sweep == 0);
         total_iters += run.iterations;
         upload(mask.p, act.data(), r * sizeof(int32_t), st);
         hipLaunchKernelGGL(k_add_to64<float>, dim3(vgrid(nv)), dim3(TPB), 0, st, x, s32.x.p, nv, r, mask.p);
         FH_CHECK_LAUNCH();
         nz = true;
+        if (run.escalate) {  // x holds the progress so far; the caller restarts after escalating
+            res.escalate = true;
+            finished = true;
+            break;
+        }
         if (run.maxiter_hit || total_iters >= maxiter) {
             res.rc = FH_CONVERGENCE;
+            finished = true;
             break;
         }
         // Acceptance without the FP64 residual. The correction solve's recursive residual rho
@@ -750,9 +782,19 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
             }
             if (accept) {
                 res.accepted_fp32 = true;
+                finished = true;
                 break;
             }
         }
+    }
+    if (!finished) {
+        // the refinement sweeps stopped gaining (an FP32-limited preconditioner): finish in FP64
+        // from the current iterate
+        PcgResult f = pcg_fp64(a, rhs, x, true, r, ss1, ss0, tol, maxiter > total_iters ? maxiter - total_iters : 1,
+                               ctx, false);
+        total_iters += f.iterations;
+        res.rc = f.rc;
+        res.exact = f.exact;
     }
     stream_wait(st);
     res.iterations = std::max<uint64_t>(total_iters, 1);

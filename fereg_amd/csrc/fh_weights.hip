@@ -463,6 +463,40 @@ double absorber_sw(Absorber& a) {
     return a.sw;
 }
 
+bool escalation_possible(const Absorber& a) {
+    if (a.escalated || a.m <= a.ms || a.boff.size() < a.nb) return false;
+    // the escalated block must be the only large one: the super-block then covers S and the
+    // restarted solve is exact (a dense block next to Jacobi blocks stagnated on a
+    // near-singular ring block in testing and is not enabled)
+    int nlarge = 0;
+    uint64_t g = 0;
+    for (const BlockInfo& bi : a.blocks)
+        if (!bi.small) {
+            nlarge++;
+            g = bi.g;
+        }
+    return nlarge == 1 && a.ms + g <= ESC_MAX;
+}
+
+void escalate(Absorber& a) {
+    require(escalation_possible(a), "escalation: no block to escalate");
+    a.escalated = true;
+    // blocks are ordered small first, so the first large block is adjacent to the small
+    // super-block in the coordinates and in the cell codes
+    for (size_t b = 0; b < a.nb; b++) {
+        BlockInfo& bi = a.blocks[b];
+        if (bi.small) continue;
+        bi.small = true;
+        a.ms += bi.g;
+        break;
+    }
+    // with two or more small blocks the dense small-row products no longer apply: every block
+    // then runs a level pass, so the level records skipped for a dense small block are built
+    for (size_t b = 0; b < a.nb; b++)
+        if (a.boff[b].n == 0) build_block_transpose(a, (int)b);
+    refresh_weights(a, a.weighted ? a.w_rows.p : nullptr);
+}
+
 void refresh_weights(Absorber& a, const double* w_rows_dev) {
     hipStream_t st = a.st;
     const size_t M = a.M, g1 = a.g1, m = a.m, ms = a.ms;
