@@ -68,7 +68,9 @@ typedef struct {
     int32_t precision; /* -1 = the absorber's configured precision */
 } fh_solve_opts;
 
-/* SolveInfo (absorber.rs:27-37); refinements = FP64 outer sweeps of the mixed form. */
+/* SolveInfo (absorber.rs:27-37); refinements = FP64 outer sweeps of the mixed form; escalated = 1
+ * when the solve's rate probe (iteration 12) gave the remaining large block the exact block
+ * preconditioner and restarted (kept by the absorber for later solves). */
 typedef struct {
     uint64_t iterations;
     int32_t escalated;
