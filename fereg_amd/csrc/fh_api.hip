@@ -53,8 +53,39 @@ __global__ void k_gather_rows(const double* __restrict__ y, const double* __rest
     }
 }
 
+// The same with one thread per row for ld in {2, 4, 8, 12, 16}: the row's loads are issued
+// together (element threads each wait on pi, then on X) and the row is stored as 16-byte pairs.
+template <int LD>
+__global__ void k_gather_rows_row(const double* __restrict__ y, const double* __restrict__ X, int K,
+                                  const uint32_t* __restrict__ pi, size_t n, double* __restrict__ rows) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t o = pi[i];
+        double v[LD];
+        v[0] = y[o];
+#pragma unroll
+        for (int c = 1; c < LD; c++) v[c] = c <= K ? X[o * K + c - 1] : 0.0;
+        double2* dst = reinterpret_cast<double2*>(rows + i * LD);
+#pragma unroll
+        for (int q = 0; q < LD / 2; q++) dst[q] = make_double2(v[2 * q], v[2 * q + 1]);
+    }
+}
+
 void gather_rows(const double* y, const double* X, size_t K, const uint32_t* pi, size_t n, size_t ld, double* rows,
                  hipStream_t st) {
+    if (y && (ld == 2 || ld == 4 || ld == 8 || ld == 12 || ld == 16) && K + 1 <= ld) {
+        const unsigned g = grid_for(n, TPB, 1u << 16);
+        auto go = [&](auto ldc) {
+            hipLaunchKernelGGL((k_gather_rows_row<decltype(ldc)::value>), dim3(g), dim3(TPB), 0, st, y, X, (int)K, pi,
+                               n, rows);
+        };
+        if (ld == 2) go(std::integral_constant<int, 2>{});
+        else if (ld == 4) go(std::integral_constant<int, 4>{});
+        else if (ld == 8) go(std::integral_constant<int, 8>{});
+        else if (ld == 12) go(std::integral_constant<int, 12>{});
+        else go(std::integral_constant<int, 16>{});
+        FH_CHECK_LAUNCH();
+        return;
+    }
     const unsigned grid = grid_for(n * ld, TPB, 1u << 16);
     if (n * ld < (1ull << 32) - (size_t)grid * TPB)
         hipLaunchKernelGGL(k_gather_rows<uint32_t>, dim3(grid), dim3(TPB), 0, st, y, X, (uint32_t)K, pi, (uint32_t)n,

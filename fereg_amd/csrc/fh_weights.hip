@@ -652,6 +652,10 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
                 hipLaunchKernelGGL(k_seg_level_sums<8>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                    a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
                                    D8, a.esm8_bad.p, (int)a.seg_lanes);
+            else if (ms <= 16)
+                hipLaunchKernelGGL(k_seg_level_sums<16>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
+                                   a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
+                                   D8, a.esm8_bad.p, (int)a.seg_lanes);
             else
                 hipLaunchKernelGGL(k_seg_level_sums<21>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                    a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
