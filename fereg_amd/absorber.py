@@ -61,6 +61,19 @@ def _ptr(a, t):
     return a.ctypes.data_as(t)
 
 
+def _dev_ready(*ts):
+    """Contiguous forms of the torch device tensors handed to a `_dev` entry point, with torch's
+    current stream drained. The library runs on the absorber's own stream, which does not wait for
+    torch's: a copy into an input, a `.contiguous()` copy or a `torch.zeros` output queued there
+    could otherwise still be running when the library reads or writes the memory. The caller keeps
+    the returned tensors alive until the call returns (the library is synchronous to the host)."""
+    import torch
+    out = [None if t is None else t.contiguous() for t in ts]
+    for d in {t.device for t in out if t is not None and t.is_cuda}:
+        torch.cuda.current_stream(d).synchronize()
+    return out
+
+
 class Absorber:
     """solve::build / FeSolver::new (solve.rs:338-465) on the GPU.
 
@@ -105,6 +118,7 @@ class Absorber:
                 weights = weights.contiguous().double()
                 self._keep.append(weights)
                 wptr = weights.data_ptr()
+            _dev_ready(*self._keep)
             L.check(lib.fh_absorber_create_dev(terms, len(codes), wptr, n, ctypes.byref(cfg), None, ctypes.byref(h)))
         else:
             w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
@@ -173,7 +187,7 @@ class Absorber:
         if w_rows is None:
             L.check(lib.fh_absorber_set_weights(self._h, None, None))
         elif _is_torch(w_rows):
-            w_rows = w_rows.contiguous().double()
+            (w_rows,) = _dev_ready(w_rows.double())
             L.check(lib.fh_absorber_set_weights_dev(self._h, w_rows.data_ptr(), None))
         else:
             w = np.ascontiguousarray(w_rows, dtype=np.float64)
@@ -201,6 +215,7 @@ class Absorber:
                 wm = warm.data if _is_torch(warm.data) else torch.as_tensor(warm.data, device=z.device)
                 if warm.r != r or warm.p != p:
                     raise L.FeregError(L.FH_INVALID, "residualize: warm start has a different shape")
+            _, wm, cf = _dev_ready(z, wm, cf)
             L.check(lib.fh_absorber_residualize_dev(self._h, z.data_ptr(), r, ctypes.byref(o.c()),
                                                     None if wm is None else wm.data_ptr(),
                                                     None if cf is None else cf.data_ptr(), ctypes.byref(info), None))

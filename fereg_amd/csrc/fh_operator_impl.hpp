@@ -885,7 +885,10 @@ __global__ __launch_bounds__(WG) void k_resid_seg_any(OpArgs<double> A, double* 
 
 // Back-substitution with a compile-time block count: U trips per lane with the index loads
 // issued before the gathers; u_c stays in registers between the alpha sum and the row update
-// when a segment fits one trip (the common case), so x is gathered once per cell.
+// when a segment fits one trip (the common case), so x is gathered once per cell. When every
+// cell is one row and the segment fits one trip, the rows, b1 and 1/n are loaded at the start
+// with the cell codes, so their latency overlaps the gathers instead of following the alpha
+// store (which the compiler cannot move them past).
 
 template <int RM, int NB>
 __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __restrict__ rows, int r, int rp, int L,
@@ -900,7 +903,18 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
         const bool sv = s < A.g1;
         const uint32_t c0 = sv ? A.seg_off[s] : 0, c1 = sv ? A.seg_off[s + 1] : 0;
         const uint32_t tmax = wave_max_u32((c1 - c0 + L - 1) / L);
-        double av[RM], uk[U][RM];
+        const bool pre = !A.cell_obs_off && tmax <= U;  // wave-uniform
+        double av[RM], uk[U][RM], rv[U][RM], bs[RM];
+        if (pre) {
+#pragma unroll
+            for (int h = 0; h < U; h++) {
+                const uint32_t c = c0 + h * L + li;
+                ld_row<double, RM, true>(rows + (size_t)c * r, r, c < c1, rv[h]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RM; j++) bs[j] = sv && j < rp ? b1[s * rp + j] : 0.0;
+        const double inv = sv ? A.inv_n1[s] : 0.0;
 #pragma unroll
         for (int j = 0; j < RM; j++) av[j] = 0.0;
         auto gather = [&](uint32_t it, double (&u)[U][RM], double (&wc)[U]) {
@@ -937,13 +951,24 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
         }
 #pragma unroll
         for (int j = 0; j < RM; j++) av[j] = gsum(av[j], L);
-        const double inv = sv ? A.inv_n1[s] : 0.0;
 #pragma unroll
-        for (int j = 0; j < RM; j++) av[j] = sv && j < rp ? (b1[s * rp + j] - av[j]) * inv : 0.0;
+        for (int j = 0; j < RM; j++) av[j] = sv && j < rp ? (bs[j] - av[j]) * inv : 0.0;
         if (alpha && sv && li == 0)
 #pragma unroll
             for (int j = 0; j < RM; j++)
                 if (j < rp) alpha[s * rp + j] = av[j];
+        if (pre) {
+#pragma unroll
+            for (int h = 0; h < U; h++) {
+                const uint32_t c = c0 + h * L + li;
+                if (c < c1) {
+#pragma unroll
+                    for (int j = 0; j < RM; j++) rv[h][j] -= uk[h][j] + av[j];
+                    st_row<double, RM>(rows + (size_t)c * r, r, rv[h]);
+                }
+            }
+            continue;
+        }
         for (uint32_t it = 0; it < tmax; it += U) {
             if (tmax > U) {
                 double wc[U];
@@ -989,10 +1014,36 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
         const bool sv = s < A.g1;
         const uint32_t c0 = sv ? A.seg_off[s] : 0, c1 = sv ? A.seg_off[s + 1] : 0;
         const uint32_t tmax = wave_max_u32((c1 - c0 + L - 1) / L);
-        double bv[RM], sk[U][RM];
+        double bv[RM], sk[U][RM], cwk[U];
 #pragma unroll
         for (int j = 0; j < RM; j++) bv[j] = 0.0;
+        // 1/n and (one row per cell) the cell weights are loaded with the rows, ahead of the b1
+        // store that the compiler cannot move them past
+        const double inv = zc && sv ? A.inv_n1[s] : 0.0;
         for (uint32_t it = 0; it < tmax; it += U) {
+            if (!A.cell_obs_off) {
+                // one row per cell: the U rows (and weights) are loaded together
+                double wk[U];
+#pragma unroll
+                for (int h = 0; h < U; h++) {
+                    const uint32_t c = c0 + (it + h) * L + li;
+                    ld_row<double, RM, true>(rows + (size_t)c * r, r, c < c1, sk[h]);
+                    wk[h] = c < c1 ? (wr ? wr[c] : 1.0) : 0.0;
+                    cwk[h] = zc && c < c1 && A.cw ? A.cw[c] : 1.0;
+                }
+#pragma unroll
+                for (int h = 0; h < U; h++)
+#pragma unroll
+                    for (int j = 0; j < RM; j++)
+                        if (j < r) {
+                            const double wv = wk[h] * sk[h][j];
+                            wsq[j] += wv * sk[h][j];
+                            wsum[j] += wv;
+                            sk[h][j] = wv;
+                            bv[j] += wv;
+                        }
+                continue;
+            }
 #pragma unroll
             for (int h = 0; h < U; h++) {
                 const uint32_t c = c0 + (it + h) * L + li;
@@ -1010,7 +1061,7 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
             for (int j = 0; j < RM; j++)
                 if (j < rp) b1[s * rp + j] = bv[j];
         if (!zc) continue;
-        const double inv = sv ? A.inv_n1[s] : 0.0;
+        const bool one = !A.cell_obs_off && tmax <= U;  // cwk holds this segment's cell weights
         for (uint32_t it = 0; it < tmax; it += U) {
 #pragma unroll
             for (int h = 0; h < U; h++) {
@@ -1020,7 +1071,7 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
                         double d0[RM], d1[RM];
                         cell_sums<RM>(A, rows, wr, c, r, sk[h], d0, d1, false);
                     }
-                    const double f = (A.cw ? A.cw[c] : 1.0) * inv;
+                    const double f = (one ? cwk[h] : (A.cw ? A.cw[c] : 1.0)) * inv;
                     double zv[RM];
 #pragma unroll
                     for (int j = 0; j < RM; j++) zv[j] = sk[h][j] - f * bv[j];

@@ -11,7 +11,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import _lib as L
-from .absorber import Absorber, SolveOpts, _is_torch, _ptr
+from .absorber import Absorber, SolveOpts, _dev_ready, _is_torch, _ptr
 
 
 @dataclass
@@ -74,9 +74,10 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
     st = L.FitStats()
     o = (opts or SolveOpts(tol=1e-10)).c()
     if _is_torch(y):
-        cl = None if cluster is None else cluster.contiguous().data_ptr()
+        held = _dev_ready(y, X, cluster)  # kept alive through the call
+        cl = None if cluster is None else held[2].data_ptr()
         fn = getattr(lib, kind_fn + "_dev")
-        args = [ab._h, y.contiguous().data_ptr(), X.contiguous().data_ptr(), K, kind, cl, _ptr(nest, L.i32p),
+        args = [ab._h, held[0].data_ptr(), held[1].data_ptr(), K, kind, cl, _ptr(nest, L.i32p),
                 ctypes.byref(sscc)]
     else:
         yv = np.ascontiguousarray(y, dtype=np.float64)
@@ -152,7 +153,8 @@ def wildboot(y, X, cluster, R, r=0.0, reps=999, seed=1, level=0.95, pval_type="t
     o = (opts or SolveOpts(tol=1e-10)).c()
     if _is_torch(y):
         fn = lib.fh_ols_wildboot_dev
-        args = [ab._h, y.contiguous().data_ptr(), Xa.contiguous().data_ptr(), K, cluster.contiguous().data_ptr()]
+        held = _dev_ready(y, Xa, cluster)  # kept alive through the call
+        args = [ab._h, held[0].data_ptr(), held[1].data_ptr(), K, held[2].data_ptr()]
     else:
         fn = lib.fh_ols_wildboot
         yv = np.ascontiguousarray(y, dtype=np.float64)

@@ -10,7 +10,10 @@
  *
  * Threading: an absorber is Send-not-Sync (absorber.rs:62): one caller at a time, movable
  * between threads. Each absorber owns one HIP stream of default priority on the device that was
- * current when it was created; every call is synchronous with respect to the host.
+ * current when it was created; every call is synchronous with respect to the host. That stream
+ * does not wait for other streams: device inputs (and outputs) of a _dev call must have no writes
+ * still queued on another stream when the call is made (synchronize the producing stream first);
+ * device outputs are complete when the call returns.
  *
  * Memory: functions ending in _dev take device pointers (HBM-resident inputs/outputs on the
  * absorber's device); the others take host pointers.
