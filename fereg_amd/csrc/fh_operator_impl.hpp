@@ -886,7 +886,8 @@ __global__ __launch_bounds__(WG) void k_resid_seg_any(OpArgs<double> A, double* 
 // Back-substitution with a compile-time block count: U trips per lane with the index loads
 // issued before the gathers; u_c stays in registers between the alpha sum and the row update
 // when a segment fits one trip (the common case), so x is gathered once per cell. When every
-// cell is one row and the segment fits one trip, the rows, b1 and 1/n are loaded at the start
+// cell is one row, the segment fits one trip and a lane has four or more trips (rows of up to
+// four columns), the rows, b1 and 1/n are loaded at the start
 // with the cell codes, so their latency overlaps the gathers instead of following the alpha
 // store (which the compiler cannot move them past).
 
@@ -903,7 +904,8 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
         const bool sv = s < A.g1;
         const uint32_t c0 = sv ? A.seg_off[s] : 0, c1 = sv ? A.seg_off[s + 1] : 0;
         const uint32_t tmax = wave_max_u32((c1 - c0 + L - 1) / L);
-        const bool pre = !A.cell_obs_off && tmax <= U;  // wave-uniform
+        // wave-uniform; four or more trips per lane only (8 columns, U = 2: 688 -> 714 us)
+        const bool pre = U >= 4 && !A.cell_obs_off && tmax <= U;
         double av[RM], uk[U][RM], rv[U][RM], bs[RM];
         if (pre) {
 #pragma unroll
@@ -912,9 +914,10 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
                 ld_row<double, RM, true>(rows + (size_t)c * r, r, c < c1, rv[h]);
             }
         }
+        if (U >= 4)
 #pragma unroll
-        for (int j = 0; j < RM; j++) bs[j] = sv && j < rp ? b1[s * rp + j] : 0.0;
-        const double inv = sv ? A.inv_n1[s] : 0.0;
+            for (int j = 0; j < RM; j++) bs[j] = sv && j < rp ? b1[s * rp + j] : 0.0;
+        const double inv_early = U >= 4 && sv ? A.inv_n1[s] : 0.0;
 #pragma unroll
         for (int j = 0; j < RM; j++) av[j] = 0.0;
         auto gather = [&](uint32_t it, double (&u)[U][RM], double (&wc)[U]) {
@@ -951,6 +954,10 @@ __global__ __launch_bounds__(WG) void k_resid_seg(OpArgs<double> A, double* __re
         }
 #pragma unroll
         for (int j = 0; j < RM; j++) av[j] = gsum(av[j], L);
+        const double inv = U >= 4 ? inv_early : (sv ? A.inv_n1[s] : 0.0);
+        if (U < 4)
+#pragma unroll
+            for (int j = 0; j < RM; j++) bs[j] = sv && j < rp ? b1[s * rp + j] : 0.0;
 #pragma unroll
         for (int j = 0; j < RM; j++) av[j] = sv && j < rp ? (bs[j] - av[j]) * inv : 0.0;
         if (alpha && sv && li == 0)
@@ -1018,10 +1025,12 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
 #pragma unroll
         for (int j = 0; j < RM; j++) bv[j] = 0.0;
         // 1/n and (one row per cell) the cell weights are loaded with the rows, ahead of the b1
-        // store that the compiler cannot move them past
-        const double inv = zc && sv ? A.inv_n1[s] : 0.0;
+        // store that the compiler cannot move them past. Only with four or more trips per lane:
+        // with fewer the extra code path costs occupancy (8 columns: 503 -> 530 us, 12 columns:
+        // 709 -> 1130 us).
+        const double inv_early = U >= 4 && zc && sv ? A.inv_n1[s] : 0.0;
         for (uint32_t it = 0; it < tmax; it += U) {
-            if (!A.cell_obs_off) {
+            if (U >= 4 && !A.cell_obs_off) {
                 // one row per cell: the U rows (and weights) are loaded together
                 double wk[U];
 #pragma unroll
@@ -1061,7 +1070,8 @@ __global__ __launch_bounds__(WG) void k_rhs_seg_reg(OpArgs<double> A, const doub
             for (int j = 0; j < RM; j++)
                 if (j < rp) b1[s * rp + j] = bv[j];
         if (!zc) continue;
-        const bool one = !A.cell_obs_off && tmax <= U;  // cwk holds this segment's cell weights
+        const double inv = U >= 4 ? inv_early : (sv ? A.inv_n1[s] : 0.0);
+        const bool one = U >= 4 && !A.cell_obs_off && tmax <= U;  // cwk holds this segment's cell weights
         for (uint32_t it = 0; it < tmax; it += U) {
 #pragma unroll
             for (int h = 0; h < U; h++) {
