@@ -42,18 +42,32 @@ __host__ __device__ __forceinline__ uint64_t sm_mix(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+// row[q] = src[q], q < LD (LD even, src 16-byte aligned): LD / 2 loads of 16 bytes.
+template <int LD>
+__device__ __forceinline__ void ld_row16(const double* __restrict__ src, double (&row)[LD]) {
+#pragma unroll
+    for (int q = 0; q < LD / 2; q++) {
+        const double2 v = reinterpret_cast<const double2*>(src)[q];
+        row[2 * q] = v.x;
+        row[2 * q + 1] = v.y;
+    }
+}
+
 // S[g][0..2][a]: sums over the rows i of cluster g of (w u0, w u1, w z)_i x~_ia, with z = x~_i'C,
 // u0 = e_i and u1 = -x~_i'M_R (x~ the kept regressors): L lanes per cluster, each summing whole
 // runs of adjacent rows (the cluster's runs strided over the lanes), then the lanes reduced by
 // shuffles in a fixed order.
-template <int MK>
+// LD > 0: every regressor is kept (x~_ia in column 1 + a) and the row stride is LD, so a row is
+// read whole with 16-byte loads (LD / 2 requests for a random row instead of one per column);
+// with ecol >= 0 the residual is column ecol of the same row.
+template <int MK, int LD = 0>
 __global__ __launch_bounds__(TPB) void k_wb_clsums(const uint32_t* __restrict__ off, const uint32_t* __restrict__ order,
                                                    const uint32_t* __restrict__ run_off, size_t G,
                                                    const double* __restrict__ rows, int R,
                                                    const int* __restrict__ kept, int mk, const double* __restrict__ C,
                                                    const double* __restrict__ MR, const double* __restrict__ e,
                                                    size_t es, const double* __restrict__ w, int L,
-                                                   double* __restrict__ S) {
+                                                   double* __restrict__ S, int ecol = -1) {
     const int l = lane_id();
     const int gpw = WAVE / L, gl = l / L, li = l % L;
     const size_t nwaves = (size_t)gridDim.x * (TPB / WAVE);
@@ -68,17 +82,32 @@ __global__ __launch_bounds__(TPB) void k_wb_clsums(const uint32_t* __restrict__ 
             const uint32_t run = order[j];
             const uint32_t i1 = run_off[run + 1];
             for (uint32_t i = run_off[run]; i < i1; i++) {
-                double x[MK], z = 0.0, u1 = 0.0;
+                double x[MK], z = 0.0, u1 = 0.0, ev;
+                if constexpr (LD > 0) {
+                    double row[LD];
+                    ld_row16<LD>(rows + (size_t)i * LD, row);
 #pragma unroll
-                for (int a = 0; a < MK; a++) {
-                    x[a] = a < mk ? rows[(size_t)i * R + 1 + kept[a]] : 0.0;
+                    for (int a = 0; a < MK; a++) x[a] = a + 1 < LD && a < mk ? row[a + 1 < LD ? a + 1 : 0] : 0.0;
+                    if (ecol >= 0) {
+                        ev = 0.0;
+#pragma unroll
+                        for (int q = 0; q < LD; q++) ev = q == ecol ? row[q] : ev;
+                    } else {
+                        ev = e[(size_t)i * es];
+                    }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < MK; a++) x[a] = a < mk ? rows[(size_t)i * R + 1 + kept[a]] : 0.0;
+                    ev = e[(size_t)i * es];
+                }
+#pragma unroll
+                for (int a = 0; a < MK; a++)
                     if (a < mk) {
                         z += x[a] * C[a];
                         u1 -= x[a] * MR[a];
                     }
-                }
                 const double wi = w ? w[i] : 1.0;
-                const double wz = wi * z, w0 = wi * e[(size_t)i * es], w1 = wi * u1;
+                const double wz = wi * z, w0 = wi * ev, w1 = wi * u1;
 #pragma unroll
                 for (int a = 0; a < MK; a++) {
                     s0[a] += w0 * x[a];
@@ -664,11 +693,31 @@ void wb_cluster_sums(Absorber& a, const double* rows, int R, const std::vector<i
     const unsigned cg = grid_for(G, (TPB / WAVE) * (WAVE / L), 1u << 16);
     auto launch = [&](auto mkc) {
         hipLaunchKernelGGL((k_wb_clsums<decltype(mkc)::value>), dim3(cg), dim3(TPB), 0, st, cr.off.p, cr.order.p,
-                           cr.run_off.p, (size_t)G, rows, R, dkept.p, mk, dC.p, dMR.p, e, es, w, (int)L, S.p);
+                           cr.run_off.p, (size_t)G, rows, R, dkept.p, mk, dC.p, dMR.p, e, es, w, (int)L, S.p, -1);
     };
-    if (mk <= 4) launch(std::integral_constant<int, 4>{});
-    else if (mk <= 12) launch(std::integral_constant<int, 12>{});
-    else launch(std::integral_constant<int, 20>{});
+    // whole-row loads when every regressor is kept and the stride is a padded width
+    bool ident = 1 + mk <= R && (reinterpret_cast<uintptr_t>(rows) & 15) == 0;
+    for (int q = 0; q < mk; q++) ident &= kept[q] == q;
+    const int ecol = es == (size_t)R && e >= rows && e < rows + R ? (int)(e - rows) : -1;
+    auto launch_rows = [&](auto mkc, auto ldc) {
+        hipLaunchKernelGGL((k_wb_clsums<decltype(mkc)::value, decltype(ldc)::value>), dim3(cg), dim3(TPB), 0, st,
+                           cr.off.p, cr.order.p, cr.run_off.p, (size_t)G, rows, R, dkept.p, mk, dC.p, dMR.p, e, es, w,
+                           (int)L, S.p, ecol);
+    };
+    using I4 = std::integral_constant<int, 4>;
+    using I8 = std::integral_constant<int, 8>;
+    using I12 = std::integral_constant<int, 12>;
+    using I16 = std::integral_constant<int, 16>;
+    using I20 = std::integral_constant<int, 20>;
+    if (ident && mk <= 4 && R == 4) launch_rows(I4{}, I4{});
+    else if (ident && mk <= 4 && R == 8) launch_rows(I4{}, I8{});
+    else if (ident && mk <= 12 && R == 8) launch_rows(I12{}, I8{});
+    else if (ident && mk <= 12 && R == 12) launch_rows(I12{}, I12{});
+    else if (ident && mk <= 12 && R == 16) launch_rows(I12{}, I16{});
+    else if (ident && mk <= 20 && R == 16) launch_rows(I20{}, I16{});
+    else if (mk <= 4) launch(I4{});
+    else if (mk <= 12) launch(I12{});
+    else launch(I20{});
     FH_CHECK_LAUNCH();
 }
 
