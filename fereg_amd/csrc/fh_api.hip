@@ -55,15 +55,28 @@ __global__ void k_gather_rows(const double* __restrict__ y, const double* __rest
 
 // The same with one thread per row for ld in {2, 4, 8, 12, 16}: the row's loads are issued
 // together (element threads each wait on pi, then on X) and the row is stored as 16-byte pairs.
-template <int LD>
+// X2: K even and X 16-byte aligned, so the random X row is read as K/2 loads of 16 bytes (one
+// request each) instead of K.
+template <int LD, bool X2>
 __global__ void k_gather_rows_row(const double* __restrict__ y, const double* __restrict__ X, int K,
                                   const uint32_t* __restrict__ pi, size_t n, double* __restrict__ rows) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const size_t o = pi[i];
         double v[LD];
         v[0] = y[o];
+        if constexpr (X2) {
+            const double2* xr = reinterpret_cast<const double2*>(X + o * K);
 #pragma unroll
-        for (int c = 1; c < LD; c++) v[c] = c <= K ? X[o * K + c - 1] : 0.0;
+            for (int q = 0; q < (LD - 1) / 2; q++) {
+                const double2 x2 = 2 * q + 1 < K ? xr[q] : make_double2(0.0, 0.0);
+                v[1 + 2 * q] = x2.x;
+                v[2 + 2 * q] = x2.y;
+            }
+            if constexpr (LD % 2 == 0) v[LD - 1] = 0.0;  // the pad column (K <= LD - 2 here)
+        } else {
+#pragma unroll
+            for (int c = 1; c < LD; c++) v[c] = c <= K ? X[o * K + c - 1] : 0.0;
+        }
         double2* dst = reinterpret_cast<double2*>(rows + i * LD);
 #pragma unroll
         for (int q = 0; q < LD / 2; q++) dst[q] = make_double2(v[2 * q], v[2 * q + 1]);
@@ -74,9 +87,13 @@ void gather_rows(const double* y, const double* X, size_t K, const uint32_t* pi,
                  hipStream_t st) {
     if (y && (ld == 2 || ld == 4 || ld == 8 || ld == 12 || ld == 16) && K + 1 <= ld) {
         const unsigned g = grid_for(n, TPB, 1u << 16);
+        const bool x2 = K % 2 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0;
         auto go = [&](auto ldc) {
-            hipLaunchKernelGGL((k_gather_rows_row<decltype(ldc)::value>), dim3(g), dim3(TPB), 0, st, y, X, (int)K, pi,
-                               n, rows);
+            constexpr int LDC = decltype(ldc)::value;
+            if (x2)
+                hipLaunchKernelGGL((k_gather_rows_row<LDC, true>), dim3(g), dim3(TPB), 0, st, y, X, (int)K, pi, n, rows);
+            else
+                hipLaunchKernelGGL((k_gather_rows_row<LDC, false>), dim3(g), dim3(TPB), 0, st, y, X, (int)K, pi, n, rows);
         };
         if (ld == 2) go(std::integral_constant<int, 2>{});
         else if (ld == 4) go(std::integral_constant<int, 4>{});

@@ -110,6 +110,45 @@ __global__ __launch_bounds__(TPB) void k_resid(double* __restrict__ z, size_t n,
     }
 }
 
+// The same with a compile-time stride LD (even, 16-byte aligned rows): the row is loaded whole
+// (LD / 2 loads of 16 bytes issued together) before the products, which the runtime column loop
+// above waits for one by one. Same products in the same order as k_resid.
+template <int LD>
+__global__ __launch_bounds__(TPB) void k_resid_ld(double* __restrict__ z, size_t n, int k,
+                                                  const double* __restrict__ c, const double* __restrict__ w,
+                                                  double* __restrict__ e, int ecol, double* __restrict__ part) {
+    __shared__ double cc[32];
+    if (threadIdx.x < 32u) cc[threadIdx.x] = threadIdx.x < (unsigned)k ? c[threadIdx.x] : 0.0;
+    __syncthreads();
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
+        double* zi = z + i * LD;
+        double row[LD];
+#pragma unroll
+        for (int q = 0; q < LD / 2; q++) {
+            const double2 v2 = reinterpret_cast<const double2*>(zi)[q];
+            row[2 * q] = v2.x;
+            row[2 * q + 1] = v2.y;
+        }
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < LD; j++)
+            if (j < k) v += cc[j] * row[j];
+        e[i] = v;
+        if (ecol >= 0) zi[ecol] = v;
+        s += (w ? w[i] : 1.0) * v * v;
+    }
+    __shared__ double red[TPB / WAVE];
+    s = wave_sum(s);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < TPB / WAVE; q++) t += red[q];
+        part[blockIdx.x] = t;
+    }
+}
+
 // v_i = (w_i e_i)^2 (hetero meat weights, fit/vcov.rs:322-332).
 __global__ void k_sq_we(const double* __restrict__ e, const double* __restrict__ w, size_t n, double* __restrict__ v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -261,7 +300,12 @@ double residual_pass(Absorber& a, double* z, size_t n, int ld, int k, const doub
     require(k >= 1 && k <= 32 && k <= ld && ecol < ld, "residual pass: bad column layout");
     DBuf<double> c(k), part(GRID);
     upload(c.p, c_host, k * sizeof(double), a.st);
-    hipLaunchKernelGGL(k_resid, dim3(GRID), dim3(TPB), 0, a.st, z, n, ld, k, c.p, w, e, ecol, part.p);
+    const bool al = (reinterpret_cast<uintptr_t>(z) & 15) == 0;
+    if (al && ld == 4) hipLaunchKernelGGL(k_resid_ld<4>, dim3(GRID), dim3(TPB), 0, a.st, z, n, k, c.p, w, e, ecol, part.p);
+    else if (al && ld == 8) hipLaunchKernelGGL(k_resid_ld<8>, dim3(GRID), dim3(TPB), 0, a.st, z, n, k, c.p, w, e, ecol, part.p);
+    else if (al && ld == 12) hipLaunchKernelGGL(k_resid_ld<12>, dim3(GRID), dim3(TPB), 0, a.st, z, n, k, c.p, w, e, ecol, part.p);
+    else if (al && ld == 16) hipLaunchKernelGGL(k_resid_ld<16>, dim3(GRID), dim3(TPB), 0, a.st, z, n, k, c.p, w, e, ecol, part.p);
+    else hipLaunchKernelGGL(k_resid, dim3(GRID), dim3(TPB), 0, a.st, z, n, ld, k, c.p, w, e, ecol, part.p);
     FH_CHECK_LAUNCH();
     return sum_parts_dev(part.p, GRID, 1, a.st)[0];
 }
