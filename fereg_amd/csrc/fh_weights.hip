@@ -564,8 +564,11 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
                                a.bperm[b].p, cw, X, a.w64.lw[b].p, a.w32.lw[b].p);
             FH_CHECK_LAUNCH();
         }
-        uint32_t L = 1;  // lanes per level: ~4 positions per lane
-        while (L < 64 && L * 4.0 < double(a.M) / double(std::max<uint64_t>(bi.g, 1))) L <<= 1;
+        // lanes per level: ~4 positions per lane of each (bucket, level) item, since the lanes
+        // walk a level's buckets one after another
+        uint32_t L = 1;
+        while (L < 64 && L * 4.0 < double(a.M) / double(std::max<uint64_t>(bi.g, 1) * std::max<uint32_t>(a.nbk[b], 1)))
+            L <<= 1;
         const unsigned grid = grid_for(bi.g, (TPB / WAVE) * (WAVE / L), 1u << 16);
         hipLaunchKernelGGL(k_level_weights, dim3(grid), dim3(TPB), 0, st, a.boff[b].p, a.lseg[b].p, a.nbk[b],
                            unit ? nullptr : a.w64.lw[b].p, a.n1.p, (size_t)bi.g, (size_t)bi.off, (int)L, a.nr.p,
