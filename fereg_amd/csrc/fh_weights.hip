@@ -612,8 +612,13 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         for (int b = 0; b < nb; b++) {
             const BlockInfo& bi = a.blocks[b];
             if (bi.small) continue;
-            uint32_t L = 1;  // lanes per level: ~8 positions per lane
-            while (L < 64 && L * 8.0 < double(a.M) / double(std::max<uint64_t>(bi.g, 1))) L <<= 1;
+            // lanes per level: ~4 positions per lane of each (bucket, level) item (the lanes walk
+            // a level's buckets one after another; C2: 86 -> 63 us, a sweep of 1..16 lanes gave
+            // 98, 74, 63, 65, 86 us)
+            uint32_t L = 1;
+            while (L < 64 &&
+                   L * 4.0 < double(a.M) / double(std::max<uint64_t>(bi.g, 1) * std::max<uint32_t>(a.nbk[b], 1)))
+                L <<= 1;
             const unsigned grid = grid_for(bi.g, (TPB / WAVE) * (WAVE / L), 1u << 16);
             if (ms <= 16)
                 hipLaunchKernelGGL(k_level_small_sums<16>, dim3(grid), dim3(TPB), 0, st, a.boff[b].p, a.lother[b].p,
