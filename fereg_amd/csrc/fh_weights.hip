@@ -338,8 +338,13 @@ __global__ void k_small_spread(const double* __restrict__ pk, size_t ms, int wit
     }
 }
 
+// Lanes per segment of the segment level sums: half the segment pass's, since every lane's MS
+// register bins are reduced across the segment's lanes (C2, 10 cells per segment: 1, 2, 4, 8
+// lanes gave 106, 106, 127, 201 us).
+inline uint32_t seg_sums_lanes(const Absorber& a) { return a.seg_lanes > 1 ? a.seg_lanes / 2 : 1; }
+
 inline unsigned seg_sums_grid(const Absorber& a) {
-    return grid_for(a.g1, (TPB / WAVE) * (WAVE / a.seg_lanes), 1u << 16);
+    return grid_for(a.g1, (TPB / WAVE) * (WAVE / seg_sums_lanes(a)), 1u << 16);
 }
 
 double sum_dev(const double* v, size_t n, hipStream_t st) {
@@ -636,7 +641,7 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             hipLaunchKernelGGL(k_seg_level_sums<32>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                a.cell_codes.p, cw, a.n1.p, g1, nb, (uint32_t)a.blocks[0].off, (int)ms, (double*)nullptr,
                                a.sm_ld, a.w64.esm.p, a.w32.esm.p, a.weighted ? (uint8_t*)nullptr : a.esm8.p,
-                               a.esm8_bad.p, (int)a.seg_lanes);
+                               a.esm8_bad.p, (int)seg_sums_lanes(a));
         FH_CHECK_LAUNCH();
     }
     // small super-block
@@ -659,15 +664,15 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
             if (ms <= 8)
                 hipLaunchKernelGGL(k_seg_level_sums<8>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                    a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
-                                   D8, a.esm8_bad.p, (int)a.seg_lanes);
+                                   D8, a.esm8_bad.p, (int)seg_sums_lanes(a));
             else if (ms <= 16)
                 hipLaunchKernelGGL(k_seg_level_sums<16>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                    a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
-                                   D8, a.esm8_bad.p, (int)a.seg_lanes);
+                                   D8, a.esm8_bad.p, (int)seg_sums_lanes(a));
             else
                 hipLaunchKernelGGL(k_seg_level_sums<21>, dim3(seg_sums_grid(a)), dim3(TPB), 0, st, a.seg_off.p,
                                    a.cell_codes.p, cw, a.n1.p, g1, nb, so.off[0], (int)ms, C.p, a.sm_ld, D64, D32,
-                                   D8, a.esm8_bad.p, (int)a.seg_lanes);
+                                   D8, a.esm8_bad.p, (int)seg_sums_lanes(a));
             FH_CHECK_LAUNCH();
             std::vector<double> G((ms + 1) * (ms + 1));
             gram_cols(a, C.p, g1, (int)ms + 1, 0, (int)ms + 1, nullptr, nullptr, G.data());
