@@ -164,7 +164,10 @@ void residualize_rows(Absorber& a, double* rows, int r, int rp, const double* x,
     require(r >= 1 && r <= rp && rp <= 16, "residualize: 1..16 columns");
     OpArgs<double> A = op_args_impl<double>(a);
     const unsigned grid = seg_grid(a);
-    const int L = (int)a.seg_lanes;
+    // rows of one or two columns: half the segment pass's lanes, so a lane's eight trips cover
+    // more of its segment (C2, 10 cells per segment: 2 lanes 142 us, 4 lanes 157 us; wider rows
+    // keep the segment pass's count, 12 columns: 1298 against 1178 us)
+    const int L = rp <= 2 && a.seg_lanes > 1 ? (int)a.seg_lanes / 2 : (int)a.seg_lanes;
     if (a.nb <= 4) {
         FH_DISPATCH_R(rp, with_nb0((int)a.nb, [&](auto nbc) {
             constexpr int NB = decltype(nbc)::value;
