@@ -1224,7 +1224,8 @@ int fh_absorber_get_plan(const fh_absorber* ab, int32_t precision, size_t r, fh_
         const SpmvPlan& pl = a.plan_for(precision == FH_PLAIN_FP64, (int)r);
         *out = fh_spmv_plan{};
         out->dense = pl.dense;
-        out->seg_lanes = pl.seg_lanes ? pl.seg_lanes : a.seg_lanes;
+        out->seg_lanes = pl.seg_lanes ? pl.seg_lanes
+                                      : spmv_seg_lanes(a, padded_cols((int)r), precision == FH_PLAIN_FP64 ? 8 : 4);
         for (size_t b = 0; b < a.nb && b < 9; b++) {
             const LvlMode md = lvl_mode(a, pl, b, (int)r, precision == FH_PLAIN_FP64 ? 8 : 4);
             out->block_lanes[b] = (uint32_t)md.lanes;

@@ -216,6 +216,13 @@ struct LvlMode {
     bool merge;
     int lanes;
 };
+// Default lanes per segment of the S*p segment pass: the absorber's (~4 cells per lane), halved
+// for FP32 rows of more than 8 columns (12 columns at C2: 2 lanes 388 us, 4 lanes 412 us; FP64
+// and narrower rows are fastest at the absorber's count).
+inline uint32_t spmv_seg_lanes(const Absorber& a, int r, int tbytes) {
+    return tbytes == 4 && r > 8 && a.seg_lanes > 1 ? a.seg_lanes / 2 : a.seg_lanes;
+}
+
 inline LvlMode lvl_mode(const Absorber& a, const SpmvPlan& plan, size_t b, int r, int tbytes) {
     const uint32_t nbk = a.nbk[b];
     const int x = b < plan.xcd.size() && plan.xcd[b] >= 0 ? plan.xcd[b]

@@ -1218,7 +1218,7 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
     const WeightArrays<T>* W;
     if constexpr (std::is_same<T, double>::value) W = &a.w64; else W = &a.w32;
     s.t.reserve(std::max<size_t>(a.g1 * r, 1));
-    const int L = (int)(plan.seg_lanes ? plan.seg_lanes : a.seg_lanes);
+    const int L = (int)(plan.seg_lanes ? plan.seg_lanes : spmv_seg_lanes(a, r, (int)sizeof(T)));
     const unsigned grid = grid_for(a.g1, (unsigned)((WAVE / L) * WAVES), 1u << 16);
     const size_t ms = a.ms, mL = a.m - a.ms;
     const T* E = a.small_dense ? W->esm.p : nullptr;
@@ -1348,7 +1348,7 @@ void autotune_spmv(const Absorber& a, SpmvPlan& plan, const T* p, T* y, int r, S
     }
     SpmvPlan best;
     best.dense = 1;
-    best.seg_lanes = a.seg_lanes;
+    best.seg_lanes = spmv_seg_lanes(a, r, (int)sizeof(T));
     best.blanes.assign(a.nb, 0);
     best.xcd.assign(a.nb, -1);
     hipEvent_t e0, e1;
