@@ -231,7 +231,12 @@ inline LvlMode lvl_mode(const Absorber& a, const SpmvPlan& plan, size_t b, int r
                                                                              : 0;
     LvlMode m;
     m.merge = nbk > 1 && x == 0;
-    const uint32_t def = m.merge ? a.llanes[b] : a.blanes[b];
+    // unweighted FP64 rows of one or two columns: half the item lanes (C2: 196 us against 202 us
+    // at 4 lanes); FP32, and weighted passes that also stream the level weights (Poisson panel:
+    // 134 against 127 us), are fastest at the default
+    const uint32_t def = m.merge ? a.llanes[b]
+                         : tbytes == 8 && r <= 2 && !a.weighted && a.blanes[b] > 1 ? a.blanes[b] / 2
+                                                                                   : a.blanes[b];
     m.lanes = (int)(b < plan.blanes.size() && plan.blanes[b] ? plan.blanes[b] : def);
     return m;
 }
