@@ -253,16 +253,25 @@ __global__ void k_irls_fill(const double* __restrict__ y, const double* __restri
 }
 
 // eta = (z - e) + off (step s in (0,1] toward it from eta_old), mu = max(exp eta, 100 eps),
-// deviance partials (irls.rs:254-270; family.rs:103-112).
-__global__ __launch_bounds__(TPB) void k_irls_eta(const double* __restrict__ zraw, const double* __restrict__ e,
-                                                  const double* __restrict__ off, const double* __restrict__ eta_old,
-                                                  double step, const double* __restrict__ y,
-                                                  const double* __restrict__ tw, size_t n, double* __restrict__ eta,
-                                                  double* __restrict__ mu, double* __restrict__ part) {
+// deviance partials (irls.rs:254-270; family.rs:103-112); also writes the next iteration's IRLS
+// fill (k_irls_fill of this eta and mu) into separate buffers, which the caller swaps in at the
+// top of the next iteration, so the fill pass does not read y, mu and eta back. A relaunch with a
+// halved step rewrites them; the current rows and weights (read by the variance after the last
+// iteration) are left alone.
+__global__ __launch_bounds__(TPB) void k_irls_eta_fill(const double* __restrict__ zraw, const double* __restrict__ e,
+                                                       const double* __restrict__ off,
+                                                       const double* __restrict__ eta_old, double step,
+                                                       const double* __restrict__ y, const double* __restrict__ tw,
+                                                       size_t n, double* __restrict__ eta, double* __restrict__ mu,
+                                                       double* __restrict__ part, const double* __restrict__ xr,
+                                                       size_t K, double* __restrict__ rows_n,
+                                                       double* __restrict__ zraw_n, double* __restrict__ w_n) {
     double acc = 0.0;
     const double mu_min = 100.0 * 2.220446049250313e-16;
+    const size_t R = K + 1;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        double et = zraw[i] - e[i] + (off ? off[i] : 0.0);
+        const double oi = off ? off[i] : 0.0;
+        double et = zraw[i] - e[i] + oi;
         if (step < 1.0) et = eta_old[i] + step * (et - eta_old[i]);
         double m = exp(et);
         if (!(m >= mu_min)) m = mu_min;
@@ -270,6 +279,11 @@ __global__ __launch_bounds__(TPB) void k_irls_eta(const double* __restrict__ zra
         mu[i] = m;
         const double yi = y[i], wi = tw ? tw[i] : 1.0;
         acc += wi * (m - yi) + (yi > 0.0 ? wi * yi * (log(yi) - et) : 0.0);
+        const double z = et - oi + (yi - m) / m;
+        rows_n[i * R] = z;
+        zraw_n[i] = z;
+        for (size_t k = 0; k < K; k++) rows_n[i * R + 1 + k] = xr[i * K + k];
+        w_n[i] = wi * m;
     }
     __shared__ double red[TPB / WAVE];
     acc = wave_sum(acc);
@@ -796,6 +810,9 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
     FH_CHECK_LAUNCH();
     a.rows_tmp.reserve(n * R);
     double* rows = a.rows_tmp.p;
+    // the next iteration's fill, written by the eta pass (swapped in at the top of the loop)
+    DBuf<double> rows_n(n * R), zraw_n(n), w_n(n);
+    bool have_next = false;
     const size_t p = a.m + a.g1;
     DBuf<double> warm(p * R), coefs(p * R);
     bool have_warm = false;
@@ -814,9 +831,16 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
     double tol_in = std::max(1e-4, tol);
     for (it = 1; it <= maxiter; it++) {
         check_interrupt(ctx);
-        hipLaunchKernelGGL(k_irls_fill, dim3(vgrid(n)), dim3(TPB), 0, st, yr.p, mu.p, eta.p, (const double*)nullptr,
-                           (const double*)nullptr, xr.p, n, K, rows, zraw.p, w.p);
-        FH_CHECK_LAUNCH();
+        if (have_next) {
+            std::swap(a.rows_tmp, rows_n);
+            rows = a.rows_tmp.p;
+            std::swap(zraw, zraw_n);
+            std::swap(w, w_n);
+        } else {
+            hipLaunchKernelGGL(k_irls_fill, dim3(vgrid(n)), dim3(TPB), 0, st, yr.p, mu.p, eta.p,
+                               (const double*)nullptr, (const double*)nullptr, xr.p, n, K, rows, zraw.p, w.p);
+            FH_CHECK_LAUNCH();
+        }
         refresh_weights(a, w.p);
         ResidOut ro = residualize_any(a, rows, R, tol_in, o.maxiter, have_warm ? warm.p : nullptr, coefs.p, prec, ctx);
         pcg_total += ro.iterations;
@@ -843,13 +867,15 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
         c[0] = 1.0;
         for (int i = 0; i < mk; i++) c[kept[i] + 1] = -coef[i];
         residual_pass(a, rows, n, (int)R, (int)R, c.data(), nullptr, e.p, -1);
-        std::swap(eta_old, eta);  // k_irls_eta rewrites eta in full and reads eta_old only when halving
+        std::swap(eta_old, eta);  // k_irls_eta_fill rewrites eta in full and reads eta_old only when halving
         double step = 1.0, dev = NAN;
         bool halved = false;
         for (int half = 0; half < 30; half++) {
-            hipLaunchKernelGGL(k_irls_eta, dim3(1024), dim3(TPB), 0, st, zraw.p, e.p, (const double*)nullptr, eta_old.p,
-                               step, yr.p, (const double*)nullptr, n, eta.p, mu.p, part.p);
+            hipLaunchKernelGGL(k_irls_eta_fill, dim3(1024), dim3(TPB), 0, st, zraw.p, e.p, (const double*)nullptr,
+                               eta_old.p, step, yr.p, (const double*)nullptr, n, eta.p, mu.p, part.p, xr.p, K,
+                               rows_n.p, zraw_n.p, w_n.p);
             FH_CHECK_LAUNCH();
+            have_next = true;
             dev = 2.0 * sum_parts(part.p, 1024, st);
             if (std::isfinite(dev) && (std::isnan(dev_old) || dev <= dev_old * (1.0 + 1e-12) + 1e-300)) break;
             step *= 0.5;
