@@ -841,13 +841,17 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
                                (const double*)nullptr, (const double*)nullptr, xr.p, n, K, rows, zraw.p, w.p);
             FH_CHECK_LAUNCH();
         }
-        refresh_weights(a, w.p);
+        // the weights move into the absorber instead of being copied there; the buffer they
+        // replace (n entries) becomes a spare for the eta pass's next weights
+        a.w_rows.reserve(n);
+        std::swap(a.w_rows, w);
+        refresh_weights(a, a.w_rows.p);
         ResidOut ro = residualize_any(a, rows, R, tol_in, o.maxiter, have_warm ? warm.p : nullptr, coefs.p, prec, ctx);
         pcg_total += ro.iterations;
         std::swap(warm, coefs);  // this solve's coefficients start the next one (coefs is rewritten)
         have_warm = true;
         std::vector<double> G(R * R);
-        gram_cols(a, rows, n, (int)R, 0, (int)R, w.p, nullptr, G.data());
+        gram_cols(a, rows, n, (int)R, 0, (int)R, a.w_rows.p, nullptr, G.data());
         std::vector<double> Gxx(K * K);
         for (size_t i = 0; i < K; i++)
             for (size_t j = 0; j < K; j++) Gxx[i * K + j] = G[(i + 1) * R + (j + 1)];
@@ -910,7 +914,7 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind,
     uint64_t ncl = 0;
     const uint64_t G_cl = kind == FH_VCOV_CLUSTER ? cluster_levels(a, cl_dev) : 0;
     std::vector<double> V;
-    vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, w.p, kind, cl_dev, G_cl, ssc, dfk, 1.0, V, &ncl);
+    vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, a.w_rows.p, kind, cl_dev, G_cl, ssc, dfk, 1.0, V, &ncl);
     for (size_t j = 0; j < K; j++) {
         beta[j] = 0.0;
         kept_out[j] = !excluded[j];
