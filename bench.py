@@ -270,8 +270,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
-              "(the solver is single-GPU; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the solver works on one "
+              "GPU's share of the design; sharding the FE1 segments over GPUs is not built)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
