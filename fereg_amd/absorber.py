@@ -181,9 +181,19 @@ class Absorber:
         return self._weights_rows
 
     _weights_rows = None
+    # set by a Poisson fit, which leaves its last IRLS weights in the absorber (glm/irls.rs sets
+    # the solver's weights every iteration); the next call through this object restores unit
+    # weights first
+    _irls_weights = False
+
+    def _unit_if_irls(self):
+        if self._irls_weights:
+            self._irls_weights = False
+            self.set_weights(None)
 
     def set_weights(self, w_rows):
         lib = L.lib()
+        self._irls_weights = False
         if w_rows is None:
             L.check(lib.fh_absorber_set_weights(self._h, None, None))
         elif _is_torch(w_rows):
@@ -201,6 +211,7 @@ class Absorber:
 
     def residualize(self, z, r, opts=None, warm=None):
         """Residualizes the n x r row-major block z in place (solve.rs:1141-1311)."""
+        self._unit_if_irls()
         o = opts or SolveOpts()
         lib = L.lib()
         p = self.coef_rows()

@@ -61,6 +61,7 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
         X = X[:, None]
     K = X.shape[1]
     ab = absorber or Absorber(codes, g, weights=weights)
+    ab._unit_if_irls()
     kind = L.FH_VCOV[vcov]
     ssc = ssc or {}
     sscc = L.Ssc(int(ssc.get("K.adj", True)), L.FH_KFIXEF[ssc.get("K.fixef", "nonnested")], int(ssc.get("G.adj", True)), 0)
@@ -87,7 +88,11 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
         args = [ab._h, _ptr(yv, L.f64p), _ptr(Xv, L.f64p), K, kind, None if cl is None else _ptr(cl, L.u32p),
                 _ptr(nest, L.i32p), ctypes.byref(sscc)]
     args += extra + [ctypes.byref(o), _ptr(beta, L.f64p), _ptr(V, L.f64p), _ptr(kept, L.i32p), ctypes.byref(st)]
-    L.check(fn(*args))
+    try:
+        L.check(fn(*args))
+    finally:
+        if kind_fn == "fh_ppml_fit":
+            ab._irls_weights = True
     return FitResult(beta, V.reshape(K, K), kept.astype(bool), st.df_k, st.nparams, st.ssr, st.deviance,
                      st.iterations, st.pcg_iterations, st.n_clusters)
 
@@ -135,6 +140,7 @@ def wildboot(y, X, cluster, R, r=0.0, reps=999, seed=1, level=0.95, pval_type="t
         Xa = Xa[:, None]
     K = Xa.shape[1]
     ab = absorber or Absorber(codes, g)
+    ab._unit_if_irls()
     ssc = ssc or {}
     sscc = L.Ssc(int(ssc.get("K.adj", True)), L.FH_KFIXEF[ssc.get("K.fixef", "nonnested")], int(ssc.get("G.adj", True)), 0)
     nt = ab.info.nterms

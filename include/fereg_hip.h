@@ -218,7 +218,10 @@ int fh_ols_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, 
                    double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
 
 /* Poisson PML by IRLS with warm-started weighted residualization (glm/irls.rs:367-913, fixest
- * flavour; family.rs:103-145), robust (hetero) or iid or cluster variance. y, X kept order. */
+ * flavour; family.rs:103-145), robust (hetero) or iid or cluster variance. y, X kept order.
+ * The absorber must be unweighted on entry and holds the last IRLS weights on return (the
+ * solver's weights are set every iteration); fh_absorber_set_weights(ab, NULL) restores unit
+ * weights before the absorber is reused. */
 int fh_ppml_fit(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t vcov_kind,
                 const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
                 const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
