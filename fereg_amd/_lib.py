@@ -119,7 +119,7 @@ class Request(ctypes.Structure):
 class Response(ctypes.Structure):
     _fields_ = [("rc", ctypes.c_int32), ("reserved", ctypes.c_int32), ("queued_s", ctypes.c_double),
                 ("service_s", ctypes.c_double), ("total_s", ctypes.c_double), ("iterations", ctypes.c_uint64),
-                ("pcg_iterations", ctypes.c_uint64)]
+                ("pcg_iterations", ctypes.c_uint64), ("start_s", ctypes.c_double), ("end_s", ctypes.c_double)]
 
 
 # (name, restype, argtypes) of every exported entry point of include/fereg_hip.h
@@ -157,12 +157,12 @@ SIGNATURES = [
                                   ctypes.POINTER(SolveOptsC), f64p, f64p, i32p, ctypes.POINTER(FitStats)]),
     ("fh_ols_fit_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, ctypes.c_int32, vp, i32p, ctypes.POINTER(Ssc),
                                       ctypes.POINTER(SolveOptsC), f64p, f64p, i32p, ctypes.POINTER(FitStats)]),
-    ("fh_ppml_fit", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, ctypes.c_int32, u32p, i32p, ctypes.POINTER(Ssc),
-                                   ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
-                                   ctypes.POINTER(FitStats)]),
-    ("fh_ppml_fit_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, ctypes.c_int32, vp, i32p, ctypes.POINTER(Ssc),
+    ("fh_ppml_fit", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, f64p, ctypes.c_int32, u32p, i32p,
+                                   ctypes.POINTER(Ssc), ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(SolveOptsC),
+                                   f64p, f64p, i32p, ctypes.POINTER(FitStats), ctypes.POINTER(Ctx)]),
+    ("fh_ppml_fit_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_int32, vp, i32p, ctypes.POINTER(Ssc),
                                        ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
-                                       ctypes.POINTER(FitStats)]),
+                                       ctypes.POINTER(FitStats), ctypes.POINTER(Ctx)]),
     ("fh_ols_wildboot", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, u32p, i32p, ctypes.POINTER(Ssc),
                                        ctypes.POINTER(SolveOptsC), ctypes.POINTER(WbootSpec), f64p, f64p, i32p,
                                        ctypes.POINTER(WbootResult), f64p]),

@@ -74,6 +74,38 @@ def _dev_ready(*ts):
     return out
 
 
+def _on_gpu(t, what):
+    if not t.is_cuda:
+        raise L.FeregError(L.FH_INVALID, f"{what}: a torch tensor must live on the GPU (device pointer)")
+
+
+def _dev_f64(t, what):
+    """A float64 device tensor of the values of `t` (a float tensor on the GPU)."""
+    import torch
+    if t is None:
+        return None
+    _on_gpu(t, what)
+    if not t.is_floating_point():
+        raise L.FeregError(L.FH_INVALID, f"{what}: floating-point values expected, got {t.dtype}")
+    return t.to(torch.float64)
+
+
+def _dev_codes(t, what):
+    """A 32-bit device tensor of integer codes (int32 and uint32 pass as they are; other integer
+    types are converted after checking that every code lies in 0 .. 2^31 - 1)."""
+    import torch
+    if t is None:
+        return None
+    _on_gpu(t, what)
+    if t.dtype in (torch.int32, torch.uint32):
+        return t
+    if t.is_floating_point() or t.dtype == torch.bool or t.is_complex():
+        raise L.FeregError(L.FH_INVALID, f"{what}: integer codes expected, got {t.dtype}")
+    if t.numel() and (int(t.min()) < 0 or int(t.max()) >= 2 ** 31):
+        raise L.FeregError(L.FH_INVALID, f"{what}: codes must lie in 0 .. 2^31 - 1")
+    return t.to(torch.int32)
+
+
 class Absorber:
     """solve::build / FeSolver::new (solve.rs:338-465) on the GPU.
 
@@ -90,7 +122,9 @@ class Absorber:
         terms = (L.Term * max(1, len(codes)))()
         for q, (c, gq) in enumerate(zip(codes, g)):
             if dev:
-                c = c.contiguous()
+                if not _is_torch(c):
+                    raise L.FeregError(L.FH_INVALID, "codes: mix of torch device tensors and host arrays")
+                c = _dev_codes(c, "codes").contiguous()
                 self._keep.append(c)
                 terms[q] = L.Term(ctypes.cast(c.data_ptr(), L.u32p), int(gq), 1)
                 nq = c.numel()
@@ -115,7 +149,9 @@ class Absorber:
         if dev:
             wptr = None
             if weights is not None:
-                weights = weights.contiguous().double()
+                if not _is_torch(weights):
+                    raise L.FeregError(L.FH_INVALID, "weights: mix of torch device tensors and host arrays")
+                weights = _dev_f64(weights, "weights").contiguous()
                 self._keep.append(weights)
                 wptr = weights.data_ptr()
             _dev_ready(*self._keep)
@@ -181,23 +217,15 @@ class Absorber:
         return self._weights_rows
 
     _weights_rows = None
-    # set by a Poisson fit, which leaves its last IRLS weights in the absorber (glm/irls.rs sets
-    # the solver's weights every iteration); the next call through this object restores unit
-    # weights first
-    _irls_weights = False
-
-    def _unit_if_irls(self):
-        if self._irls_weights:
-            self._irls_weights = False
-            self.set_weights(None)
 
     def set_weights(self, w_rows):
         lib = L.lib()
-        self._irls_weights = False
         if w_rows is None:
             L.check(lib.fh_absorber_set_weights(self._h, None, None))
         elif _is_torch(w_rows):
-            (w_rows,) = _dev_ready(w_rows.double())
+            (w_rows,) = _dev_ready(_dev_f64(w_rows, "weights"))
+            if w_rows.numel() != self._n:
+                raise L.FeregError(L.FH_INVALID, "weights: one per row expected")
             L.check(lib.fh_absorber_set_weights_dev(self._h, w_rows.data_ptr(), None))
         else:
             w = np.ascontiguousarray(w_rows, dtype=np.float64)
@@ -211,7 +239,6 @@ class Absorber:
 
     def residualize(self, z, r, opts=None, warm=None):
         """Residualizes the n x r row-major block z in place (solve.rs:1141-1311)."""
-        self._unit_if_irls()
         o = opts or SolveOpts()
         lib = L.lib()
         p = self.coef_rows()
@@ -219,7 +246,10 @@ class Absorber:
         coefs = None
         if _is_torch(z):
             import torch
-            assert z.dtype == torch.float64 and z.is_contiguous() and z.numel() == self._n * r
+            _on_gpu(z, "residualize")
+            if z.dtype != torch.float64 or not z.is_contiguous() or z.numel() != self._n * r:
+                raise L.FeregError(L.FH_INVALID, "residualize: a contiguous float64 n x r block is residualized "
+                                   "in place")
             cf = torch.zeros(p * r, dtype=torch.float64, device=z.device) if o.want_coefs else None
             wm = None
             if warm is not None:
@@ -233,7 +263,8 @@ class Absorber:
             if cf is not None:
                 coefs = FeCoefs(p, r, cf)
         else:
-            assert z.dtype == np.float64 and z.flags.c_contiguous
+            if z.dtype != np.float64 or not z.flags.c_contiguous:
+                raise L.FeregError(L.FH_INVALID, "residualize: a C-contiguous float64 block is residualized in place")
             if z.size != self._n * r:
                 raise L.FeregError(L.FH_INVALID, "residualize: block size differs from n*R")
             cf = np.zeros(p * r) if o.want_coefs else None

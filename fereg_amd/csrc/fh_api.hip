@@ -1,6 +1,6 @@
 // C-ABI entry points of fereg_hip (include/fereg_hip.h) and the host orchestration of a
-// residualization (solve.rs:1141-1311), an OLS fit (fit/ols.rs:213-330, fit/vcov.rs:63-382) and
-// a Poisson PML fit by IRLS (glm/irls.rs:367-913, fixest flavour).
+// residualization (solve.rs:1141-1311) and an OLS fit (fit/ols.rs:213-330, fit/vcov.rs:63-382);
+// the Poisson IRLS is in fh_irls.hip.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -8,22 +8,10 @@
 #include <functional>
 #include <memory>
 
-#include "fh_internal.hpp"
+#include "fh_fit.hpp"
 #include "fh_sort.hpp"
 
 namespace fh {
-
-// fh_estimate.hip
-void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
-               double* G_host);
-// e = z[:, :k] c (n), also written to column ecol of z when ecol >= 0; returns sum w e^2.
-double residual_pass(Absorber& a, double* z, size_t n, int ld, int k, const double* c_host, const double* w, double* e,
-                     int ecol);
-void hetero_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
-                 double* meat);
-// e_i at e[i * es].
-uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, size_t es,
-                      const double* w, const uint32_t* cl_kept_dev, uint64_t G, double* meat);
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
@@ -112,12 +100,6 @@ void gather_rows(const double* y, const double* X, size_t K, const uint32_t* pi,
     FH_CHECK_LAUNCH();
 }
 
-__global__ void k_gather_x(const double* __restrict__ X, size_t K, const uint32_t* __restrict__ pi, size_t n,
-                           double* __restrict__ out) {
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * K; e += (size_t)gridDim.x * blockDim.x)
-        out[e] = X[(size_t)pi[e / K] * K + e % K];
-}
-
 // flag = 1 when some code >= g (structure.rs:483-489) or some weight is not positive and finite.
 // The same for up to NB_MAX + 1 terms in one launch (blockIdx.y = term), 16-byte loads where the
 // array is 16-byte aligned.
@@ -171,25 +153,6 @@ __global__ void k_max_u32(const uint32_t* __restrict__ c, size_t n, uint32_t* ou
         atomicMax(out, b);
     }
 }
-__global__ void k_check_y(const double* __restrict__ y, size_t n, uint32_t* flag, double* part) {
-    bool bad = false;
-    double s = 0.0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        bad |= !(y[i] >= 0.0) || !isfinite(y[i]);
-        s += y[i];
-    }
-    if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1u);
-    __shared__ double red[TPB / WAVE];
-    s = wave_sum(s);
-    if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int q = 0; q < TPB / WAVE; q++) t += red[q];
-        part[blockIdx.x] = t;
-    }
-}
-
 __global__ void k_gather_vec(const double* __restrict__ v, const uint32_t* __restrict__ pi, size_t n,
                              double* __restrict__ out) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -235,84 +198,7 @@ __global__ void k_wsq_b1(const double* __restrict__ b1, const double* __restrict
     }
 }
 
-// IRLS fill (glm/irls.rs:467-493): w = mu, working variable z = eta - off + (y - mu)/mu in column 0
-// of the block, raw X (row order) in columns 1..K.
-__global__ void k_irls_fill(const double* __restrict__ y, const double* __restrict__ mu, const double* __restrict__ eta,
-                            const double* __restrict__ off, const double* __restrict__ tw, const double* __restrict__ xr,
-                            size_t n, size_t K, double* __restrict__ rows, double* __restrict__ zraw,
-                            double* __restrict__ w) {
-    const size_t R = K + 1;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double m = mu[i];
-        const double z = eta[i] - (off ? off[i] : 0.0) + (y[i] - m) / m;
-        rows[i * R] = z;
-        zraw[i] = z;
-        for (size_t k = 0; k < K; k++) rows[i * R + 1 + k] = xr[i * K + k];
-        w[i] = (tw ? tw[i] : 1.0) * m;
-    }
-}
-
-// eta = (z - e) + off (step s in (0,1] toward it from eta_old), mu = max(exp eta, 100 eps),
-// deviance partials (irls.rs:254-270; family.rs:103-112); also writes the next iteration's IRLS
-// fill (k_irls_fill of this eta and mu) into separate buffers, which the caller swaps in at the
-// top of the next iteration, so the fill pass does not read y, mu and eta back. A relaunch with a
-// halved step rewrites them; the current rows and weights (read by the variance after the last
-// iteration) are left alone.
-__global__ __launch_bounds__(TPB) void k_irls_eta_fill(const double* __restrict__ zraw, const double* __restrict__ e,
-                                                       const double* __restrict__ off,
-                                                       const double* __restrict__ eta_old, double step,
-                                                       const double* __restrict__ y, const double* __restrict__ tw,
-                                                       size_t n, double* __restrict__ eta, double* __restrict__ mu,
-                                                       double* __restrict__ part, const double* __restrict__ xr,
-                                                       size_t K, double* __restrict__ rows_n,
-                                                       double* __restrict__ zraw_n, double* __restrict__ w_n) {
-    double acc = 0.0;
-    const double mu_min = 100.0 * 2.220446049250313e-16;
-    const size_t R = K + 1;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double oi = off ? off[i] : 0.0;
-        double et = zraw[i] - e[i] + oi;
-        if (step < 1.0) et = eta_old[i] + step * (et - eta_old[i]);
-        double m = exp(et);
-        if (!(m >= mu_min)) m = mu_min;
-        eta[i] = et;
-        mu[i] = m;
-        const double yi = y[i], wi = tw ? tw[i] : 1.0;
-        acc += wi * (m - yi) + (yi > 0.0 ? wi * yi * (log(yi) - et) : 0.0);
-        const double z = et - oi + (yi - m) / m;
-        rows_n[i * R] = z;
-        zraw_n[i] = z;
-        for (size_t k = 0; k < K; k++) rows_n[i * R + 1 + k] = xr[i * K + k];
-        w_n[i] = wi * m;
-    }
-    __shared__ double red[TPB / WAVE];
-    acc = wave_sum(acc);
-    if (lane_id() == 0) red[threadIdx.x / WAVE] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int q = 0; q < TPB / WAVE; q++) t += red[q];
-        part[blockIdx.x] = t;
-    }
-}
-
-__global__ void k_init_mu(const double* __restrict__ y, double ybar, size_t n, double* __restrict__ mu,
-                          double* __restrict__ eta) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double m = 0.5 * (y[i] + ybar);
-        mu[i] = m;
-        eta[i] = log(m);
-    }
-}
-
-template <class T>
-void h2d(T* dst, const T* src, size_t count, hipStream_t st) {
-    if (count) upload(dst, src, count * sizeof(T), st);
-}
-template <class T>
-void d2h(T* dst, const T* src, size_t count, hipStream_t st) {
-    if (count) download(dst, src, count * sizeof(T), st);
-}
+}  // namespace
 
 double sum_parts(const double* dev, size_t count, hipStream_t st) {
     std::vector<double> h(count);
@@ -358,14 +244,19 @@ bool weights_ok(Absorber& a, const double* w) {
     return h == 0;
 }
 
+void restore_user_weights(Absorber& a) {
+    if (!a.irls_weights) return;
+    a.irls_weights = false;
+    if (a.user_w.n) {
+        a.w_rows = std::move(a.user_w);
+        refresh_weights(a, a.w_rows.p);
+    } else {
+        refresh_weights(a, nullptr);
+    }
+}
+
 // ------------------------------------------------------------------ residualize
 
-struct ResidOut {
-    uint64_t iterations = 1;
-    bool exact = true;
-    uint64_t refinements = 0;
-    bool escalated = false;
-};
 
 // One residualization of a device block of r <= 16 columns (solve.rs:1141-1311). The solver's
 // vectors carry rp = padded_cols(r) columns (zero, inactive pads) so that their rows are whole
@@ -518,67 +409,25 @@ struct AbsScope {
 
 // ------------------------------------------------------------------ estimation helpers (host, K x K)
 
-// Column-order Cholesky with exclusion of pivots below tol and the inverse over the kept
-// columns (vcov.rs:124-128, 163-211).
-void chol_collin(const std::vector<double>& A, int k, double tol, std::vector<int>& excluded, std::vector<double>& inv) {
-    std::vector<double> R((size_t)k * k, 0.0);
-    excluded.assign(k, 0);
-    for (int j = 0; j < k; j++) {
-        double rjj = A[(size_t)j * k + j];
-        for (int i = 0; i < j; i++)
-            if (!excluded[i]) rjj -= R[(size_t)i * k + j] * R[(size_t)i * k + j];
-        if (!(rjj >= tol)) {
-            excluded[j] = 1;
-            continue;
-        }
-        const double d = std::sqrt(rjj);
-        R[(size_t)j * k + j] = d;
-        for (int c = j + 1; c < k; c++) {
-            double v = A[(size_t)c * k + j];
-            for (int i = 0; i < j; i++)
-                if (!excluded[i]) v -= R[(size_t)i * k + c] * R[(size_t)i * k + j];
-            R[(size_t)j * k + c] = v / d;
-        }
-    }
-    std::vector<int> kept;
-    for (int j = 0; j < k; j++)
-        if (!excluded[j]) kept.push_back(j);
-    const int mk = (int)kept.size();
-    std::vector<double> ri((size_t)mk * mk, 0.0);
-    for (int c = 0; c < mk; c++) {
-        ri[(size_t)c * mk + c] = 1.0 / R[(size_t)kept[c] * k + kept[c]];
-        for (int row = c - 1; row >= 0; row--) {
-            double s = 0.0;
-            for (int l = row + 1; l <= c; l++) s += R[(size_t)kept[row] * k + kept[l]] * ri[(size_t)l * mk + c];
-            ri[(size_t)row * mk + c] = -s / R[(size_t)kept[row] * k + kept[row]];
-        }
-    }
-    inv.assign((size_t)mk * mk, 0.0);
-    for (int i = 0; i < mk; i++)
-        for (int j = 0; j < mk; j++) {
-            double s = 0.0;
-            for (int l = std::max(i, j); l < mk; l++) s += ri[(size_t)i * mk + l] * ri[(size_t)j * mk + l];
-            inv[(size_t)i * mk + j] = s;
-        }
+double fe_params(const Absorber& a) {
+    double s = 0.0;
+    for (uint64_t g : a.term_g) s += (double)g;
+    return a.nterms > 0 ? s - ((double)a.nterms - 1.0) : 0.0;
 }
 
 // fixest df.K (fit/vcov.rs:63-112, K.exact = FALSE).
 double df_k(const Absorber& a, int kind, const fh_ssc& ssc, const int32_t* nested, double np, int kdesign) {
     const size_t nt = a.nterms;
-    double fe_params = 0.0;
-    if (nt > 0) {
-        for (uint64_t g : a.term_g) fe_params += (double)g;
-        fe_params -= (double)nt - 1.0;
-    }
+    const double fe_par = fe_params(a);
     int kf = ssc.k_fixef;
     if (kind != FH_VCOV_CLUSTER && kf == FH_KFIXEF_NONNESTED) kf = FH_KFIXEF_FULL;
-    if (kf == FH_KFIXEF_NONE) return np - fe_params;
+    if (kf == FH_KFIXEF_NONE) return np - fe_par;
     if (kf == FH_KFIXEF_FULL || nt == 0) return np;
     int n_nested = 0;
     for (size_t q = 0; q < nt; q++) n_nested += nested && nested[q] ? 1 : 0;
     double k = np;
     if ((size_t)n_nested == nt) {
-        k = k - fe_params + 1.0;
+        k = k - fe_par + 1.0;
     } else {
         double s = 0.0, c = 0.0;
         for (size_t q = 0; q < nt; q++)
@@ -609,7 +458,7 @@ struct WbPre {
 void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
                const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
                const uint32_t* cl_dev, uint64_t G, const fh_ssc& ssc, double dfk, double iid_scale,
-               std::vector<double>& V, uint64_t* ncl, WbPre* wb = nullptr) {
+               std::vector<double>& V, uint64_t* ncl, WbPre* wb) {
     const int K = R - 1;
     std::vector<int> kept;
     for (int j = 0; j < K; j++)
@@ -782,158 +631,6 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     }
 }
 
-void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, const uint32_t* cl_dev,
-              const int32_t* nested, const fh_ssc& ssc, double tol, uint64_t maxiter, const fh_solve_opts& o,
-              double* beta, double* vcov, int32_t* kept_out, fh_fit_stats* stats, const fh_ctx* ctx) {
-    hipStream_t st = a.st;
-    const size_t n = a.n, R = K + 1;
-    require(K >= 1 && K <= 21, "1..21 regressors are supported");
-    require(!a.weighted, "prior weights of a Poisson fit are not supported here");
-    // row-order copies of y and X
-    DBuf<double> yr(n), xr(n * K), mu(n), eta(n), eta_old(n), zraw(n), w(n), e(n), part(1024);
-    hipLaunchKernelGGL(k_gather_vec, dim3(vgrid(n)), dim3(TPB), 0, st, y, a.pi.p, n, yr.p);
-    hipLaunchKernelGGL(k_gather_x, dim3(vgrid(n * K)), dim3(TPB), 0, st, X, K, a.pi.p, n, xr.p);
-    FH_CHECK_LAUNCH();
-    // y must be >= 0 and finite
-    double ybar = 0.0;
-    {
-        DBuf<uint32_t> flag(1);
-        FH_HIP(hipMemsetAsync(flag.p, 0, sizeof(uint32_t), st));
-        hipLaunchKernelGGL(k_check_y, dim3(1024), dim3(TPB), 0, st, yr.p, n, flag.p, part.p);
-        FH_CHECK_LAUNCH();
-        uint32_t bad = 0;
-        d2h(&bad, flag.p, 1, st);
-        ybar = sum_parts(part.p, 1024, st) / (double)n;
-        require(!bad, "Poisson outcomes must be non-negative and finite");
-    }
-    hipLaunchKernelGGL(k_init_mu, dim3(vgrid(n)), dim3(TPB), 0, st, yr.p, ybar, n, mu.p, eta.p);
-    FH_CHECK_LAUNCH();
-    a.rows_tmp.reserve(n * R);
-    double* rows = a.rows_tmp.p;
-    // the next iteration's fill, written by the eta pass (swapped in at the top of the loop)
-    DBuf<double> rows_n(n * R), zraw_n(n), w_n(n);
-    bool have_next = false;
-    const size_t p = a.m + a.g1;
-    DBuf<double> warm(p * R), coefs(p * R);
-    bool have_warm = false;
-    double dev_old = NAN;
-    std::vector<int> excluded;
-    std::vector<double> inv, coef;
-    std::vector<int> kept;
-    uint64_t it = 0, pcg_total = 0;
-    bool converged = false;
-    const int prec = o.precision < 0 ? a.precision : o.precision;
-    // inner tolerance schedule of the fixest flavour (glm/irls.rs:893-901, glm/mod.rs:826-831):
-    // start at max(1e-4, tol), then eps^{3/2} of the relative deviance change (an inexact Newton
-    // step), never below highest = max(1e-12, min(itol, 0.1 tol)); convergence needs the last
-    // FE solve at that final tolerance
-    const double highest = std::max(1e-12, std::min(o.tol, 0.1 * tol));
-    double tol_in = std::max(1e-4, tol);
-    for (it = 1; it <= maxiter; it++) {
-        check_interrupt(ctx);
-        if (have_next) {
-            std::swap(a.rows_tmp, rows_n);
-            rows = a.rows_tmp.p;
-            std::swap(zraw, zraw_n);
-            std::swap(w, w_n);
-        } else {
-            hipLaunchKernelGGL(k_irls_fill, dim3(vgrid(n)), dim3(TPB), 0, st, yr.p, mu.p, eta.p,
-                               (const double*)nullptr, (const double*)nullptr, xr.p, n, K, rows, zraw.p, w.p);
-            FH_CHECK_LAUNCH();
-        }
-        // the weights move into the absorber instead of being copied there; the buffer they
-        // replace (n entries) becomes a spare for the eta pass's next weights
-        a.w_rows.reserve(n);
-        std::swap(a.w_rows, w);
-        refresh_weights(a, a.w_rows.p);
-        ResidOut ro = residualize_any(a, rows, R, tol_in, o.maxiter, have_warm ? warm.p : nullptr, coefs.p, prec, ctx);
-        pcg_total += ro.iterations;
-        std::swap(warm, coefs);  // this solve's coefficients start the next one (coefs is rewritten)
-        have_warm = true;
-        std::vector<double> G(R * R);
-        gram_cols(a, rows, n, (int)R, 0, (int)R, a.w_rows.p, nullptr, G.data());
-        std::vector<double> Gxx(K * K);
-        for (size_t i = 0; i < K; i++)
-            for (size_t j = 0; j < K; j++) Gxx[i * K + j] = G[(i + 1) * R + (j + 1)];
-        chol_collin(Gxx, (int)K, 1e-9, excluded, inv);
-        kept.clear();
-        for (size_t j = 0; j < K; j++)
-            if (!excluded[j]) kept.push_back((int)j);
-        const int mk = (int)kept.size();
-        require(mk > 0, "all the regressors are collinear (with the fixed effects)");
-        coef.assign(mk, 0.0);
-        for (int i = 0; i < mk; i++) {
-            double s = 0.0;
-            for (int j = 0; j < mk; j++) s += inv[(size_t)i * mk + j] * G[(size_t)(kept[j] + 1) * R];
-            coef[i] = s;
-        }
-        std::vector<double> c(R, 0.0);
-        c[0] = 1.0;
-        for (int i = 0; i < mk; i++) c[kept[i] + 1] = -coef[i];
-        residual_pass(a, rows, n, (int)R, (int)R, c.data(), nullptr, e.p, -1);
-        std::swap(eta_old, eta);  // k_irls_eta_fill rewrites eta in full and reads eta_old only when halving
-        double step = 1.0, dev = NAN;
-        bool halved = false;
-        for (int half = 0; half < 30; half++) {
-            hipLaunchKernelGGL(k_irls_eta_fill, dim3(1024), dim3(TPB), 0, st, zraw.p, e.p, (const double*)nullptr,
-                               eta_old.p, step, yr.p, (const double*)nullptr, n, eta.p, mu.p, part.p, xr.p, K,
-                               rows_n.p, zraw_n.p, w_n.p);
-            FH_CHECK_LAUNCH();
-            have_next = true;
-            dev = 2.0 * sum_parts(part.p, 1024, st);
-            if (std::isfinite(dev) && (std::isnan(dev_old) || dev <= dev_old * (1.0 + 1e-12) + 1e-300)) break;
-            step *= 0.5;
-            halved = true;
-        }
-        // an iteration with step halving never converges (glm/irls.rs:828-834)
-        const double crit =
-            std::isnan(dev_old) || halved ? INFINITY : std::fabs(dev - dev_old) / (0.1 + std::fabs(dev));
-        log_line(ctx, 2, "fereg_hip: irls iteration " + std::to_string(it) + " deviance " + std::to_string(dev) +
-                             " inner tol " + std::to_string(tol_in));
-        const bool at_final_tol = tol_in <= 1.1 * highest;
-        if (!std::isnan(dev_old)) {  // glm/irls.rs:746-749, 824-827, 899-901
-            double delta = dev_old - dev;
-            if (!std::isnan(delta) && dev < 0.1 * delta) delta = dev;
-            const double eps = std::fabs(delta) / std::max(std::min(dev, dev_old), 0.1);
-            tol_in = std::max(std::min(eps * std::sqrt(eps), tol_in), highest);
-        }
-        dev_old = dev;
-        if (crit < tol && (ro.exact || at_final_tol)) {
-            converged = true;
-            break;
-        }
-    }
-    if (!converged) throw Error(FH_CONVERGENCE, "IRLS did not converge within " + std::to_string(maxiter) + " iterations");
-    // variance at the last IRLS weights: bread (X~'WX~)^-1, scores x~ e w (fit/glm.rs:560-600)
-    const double np = (double)kept.size() + [&] {
-        double s = 0.0;
-        for (uint64_t g : a.term_g) s += (double)g;
-        return s - (a.nterms > 1 ? (double)a.nterms - 1.0 : 0.0);
-    }();
-    const double dfk = df_k(a, kind, ssc, nested, np, (int)kept.size());
-    uint64_t ncl = 0;
-    const uint64_t G_cl = kind == FH_VCOV_CLUSTER ? cluster_levels(a, cl_dev) : 0;
-    std::vector<double> V;
-    vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, a.w_rows.p, kind, cl_dev, G_cl, ssc, dfk, 1.0, V, &ncl);
-    for (size_t j = 0; j < K; j++) {
-        beta[j] = 0.0;
-        kept_out[j] = !excluded[j];
-    }
-    for (size_t i = 0; i < kept.size(); i++) beta[kept[i]] = coef[i];
-    std::memcpy(vcov, V.data(), K * K * sizeof(double));
-    if (stats) {
-        stats->ssr = 0.0;
-        stats->df_k = dfk;
-        stats->nparams = np;
-        stats->deviance = dev_old;
-        stats->iterations = it;
-        stats->pcg_iterations = pcg_total;
-        stats->n_clusters = ncl;
-        stats->converged = 1;
-    }
-}
-
-}  // namespace
 }  // namespace fh
 
 using namespace fh;
@@ -1095,6 +792,8 @@ static int set_weights_impl(fh_absorber* ab, const double* w, const fh_ctx* ctx,
         AbsScope g(a);
         check_interrupt(ctx);
         if (!w) {
+            a.irls_weights = false;
+            a.user_w.release();
             refresh_weights(a, nullptr);
             return;
         }
@@ -1102,6 +801,8 @@ static int set_weights_impl(fh_absorber* ab, const double* w, const fh_ctx* ctx,
         if (dev) FH_HIP(hipMemcpyAsync(d.p, w, a.n * sizeof(double), hipMemcpyDeviceToDevice, a.st));
         else h2d(d.p, w, a.n, a.st);
         require(weights_ok(a, d.p), "weights must be positive and finite");
+        a.irls_weights = false;
+        a.user_w.release();
         refresh_weights(a, d.p);
     });
 }
@@ -1114,6 +815,7 @@ static int residualize_impl(fh_absorber* ab, double* z, size_t r, const fh_solve
         require(ab != nullptr && o != nullptr, "null argument");
         Absorber& a = ab->a;
         AbsScope g(a);
+        restore_user_weights(a);
         auto t0 = std::chrono::steady_clock::now();
         const size_t n = a.n, p = a.m + a.g1;
         if (info) *info = fh_solve_info{1, 0, 1, 0, 0.0};
@@ -1211,6 +913,7 @@ int fh_absorber_copy_weight_state(fh_absorber* ab, double* n1, double* nr, doubl
         require(ab != nullptr, "null absorber");
         Absorber& a = ab->a;
         AbsScope g(a);
+        restore_user_weights(a);
         if (n1) d2h(n1, a.n1.p, a.g1, a.st);
         if (nr) d2h(nr, a.nr.p, a.m, a.st);
         if (diag) d2h(diag, a.diag.p, a.m, a.st);
@@ -1226,6 +929,7 @@ int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int
         require(r >= 1 && r <= 16, "1..16 columns");
         Absorber& a = ab->a;
         AbsScope g(a);
+        restore_user_weights(a);
         const size_t m = a.m;
         if (m == 0) return;
         if (precision == FH_MIXED) {
@@ -1296,6 +1000,7 @@ int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t
         require(ab && ms && bytes && r >= 1 && r <= 16 && reps >= 1, "bad argument");
         Absorber& a = ab->a;
         AbsScope g(a);
+        restore_user_weights(a);
         const size_t m = a.m;
         const double s = precision == FH_MIXED ? 4.0 : 8.0;
         *bytes = 4.0 * a.nb * a.M + 12.0 * a.g1 + 2.0 * s * m * r + (a.unit() ? 0.0 : s * a.M);
@@ -1362,6 +1067,7 @@ int fh_absorber_term_values(fh_absorber* ab, const double* coefs, size_t r, size
         require(ab && coefs && out && lens && col < r, "bad argument");
         Absorber& a = ab->a;
         AbsScope g(a);
+        restore_user_weights(a);
         const size_t nt = a.nterms;
         if (a.fe1 < 0 || a.norm_fe1 < 0) {
             for (size_t q = 0; q < nt; q++) lens[q] = 0;
@@ -1406,6 +1112,7 @@ static int ols_impl(fh_absorber* ab, const double* y, const double* X, size_t K,
         require(ab && y && X && ssc && o && beta && vcov && kept, "null argument");
         Absorber& a = ab->a;
         AbsScope g(a);
+        restore_user_weights(a);
         const size_t n = a.n;
         if (dev) {
             ols_fit(a, y, X, K, kind, cluster, nested, *ssc, *o, beta, vcov, kept, stats, nullptr);
@@ -1430,6 +1137,7 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
                 "null argument");
         Absorber& a = ab->a;
         AbsScope g(a);
+        restore_user_weights(a);
         const size_t n = a.n;
         DBuf<double> dy, dx;
         DBuf<uint32_t> dc;
@@ -1517,39 +1225,48 @@ int fh_ols_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, 
     return ols_impl(ab, y, X, K, kind, cluster, nested, ssc, o, beta, vcov, kept, stats, true);
 }
 
-static int ppml_impl(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
-                     const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter, const fh_solve_opts* o,
-                     double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats, bool dev) {
+static int ppml_impl(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset, int32_t kind,
+                     const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
+                     const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats,
+                     const fh_ctx* ctx, bool dev) {
     return guard([&] {
         require(ab && y && X && ssc && o && beta && vcov && kept, "null argument");
         Absorber& a = ab->a;
         AbsScope g(a);
         const size_t n = a.n;
         if (dev) {
-            ppml_fit(a, y, X, K, kind, cluster, nested, *ssc, tol, maxiter, *o, beta, vcov, kept, stats, nullptr);
+            ppml_fit(a, y, X, K, offset, kind, cluster, nested, *ssc, tol, maxiter, *o, beta, vcov, kept, stats, ctx);
         } else {
-            DBuf<double> dy(n), dx(n * K);
+            DBuf<double> dy(n), dx(n * K), doff;
             DBuf<uint32_t> dc;
             h2d(dy.p, y, n, a.st);
             h2d(dx.p, X, n * K, a.st);
+            if (offset) {
+                doff.alloc(n);
+                h2d(doff.p, offset, n, a.st);
+            }
             if (cluster) {
                 dc.alloc(n);
                 h2d(dc.p, cluster, n, a.st);
             }
-            ppml_fit(a, dy.p, dx.p, K, kind, cluster ? dc.p : nullptr, nested, *ssc, tol, maxiter, *o, beta, vcov, kept,
-                     stats, nullptr);
+            ppml_fit(a, dy.p, dx.p, K, offset ? doff.p : nullptr, kind, cluster ? dc.p : nullptr, nested, *ssc, tol,
+                     maxiter, *o, beta, vcov, kept, stats, ctx);
         }
     });
 }
-int fh_ppml_fit(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
-                const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter, const fh_solve_opts* o,
-                double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats) {
-    return ppml_impl(ab, y, X, K, kind, cluster, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, false);
+int fh_ppml_fit(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset, int32_t kind,
+                const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
+                const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats,
+                const fh_ctx* ctx) {
+    return ppml_impl(ab, y, X, K, offset, kind, cluster, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, ctx,
+                     false);
 }
-int fh_ppml_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
-                    const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter, const fh_solve_opts* o,
-                    double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats) {
-    return ppml_impl(ab, y, X, K, kind, cluster, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, true);
+int fh_ppml_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset, int32_t kind,
+                    const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
+                    const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats,
+                    const fh_ctx* ctx) {
+    return ppml_impl(ab, y, X, K, offset, kind, cluster, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, ctx,
+                     true);
 }
 
 int fh_device_alloc(void** p, size_t bytes) {

@@ -3,9 +3,10 @@
 // per-cluster score sums s_g = sum_{i in g} x_i e_i w_i of the CRV1 meat (vcov.rs:320-448), with
 // the rows sorted by cluster once (radix sort) so every sum is in a fixed order.
 #include <algorithm>
+#include <cmath>
 #include <utility>
 
-#include "fh_internal.hpp"
+#include "fh_fit.hpp"
 #include "fh_sort.hpp"
 
 namespace fh {
@@ -23,9 +24,9 @@ constexpr int GRID = 1024;
 // split into NG groups (thread t takes group t % NG and every (256/NG)-th row); the block adds
 // the threads of each group in a fixed order.
 template <int K>
-__global__ __launch_bounds__(TPB) void k_gram(const double* __restrict__ z, size_t n, int ld, int c0,
+__global__ __launch_bounds__(TPB) void k_gram(double* __restrict__ z, size_t n, int ld, int c0,
                                               const double* __restrict__ w, const double* __restrict__ v,
-                                              double* __restrict__ part) {
+                                              double ztol, double* __restrict__ part) {
     constexpr int NP = K * (K + 1) / 2;
     constexpr int NG = NP <= 36 ? 1 : (NP + 35) / 36;
     constexpr int GPS = (NP + NG - 1) / NG;  // pairs per group
@@ -38,9 +39,16 @@ __global__ __launch_bounds__(TPB) void k_gram(const double* __restrict__ z, size
     if (rl < nrl) {
         for (size_t i = (size_t)blockIdx.x * nrl + rl; i < n; i += (size_t)gridDim.x * nrl) {
             double x[K];
-            const double* row = z + i * ld + c0;
+            double* row = z + i * ld + c0;
 #pragma unroll
             for (int c = 0; c < K; c++) x[c] = row[c];
+            if (ztol >= 0.0)  // Mata _edittozerotol: tiny entries become 0 in the block too
+#pragma unroll
+                for (int c = 0; c < K; c++)
+                    if (x[c] != 0.0 && fabs(x[c]) <= ztol) {
+                        x[c] = 0.0;
+                        row[c] = 0.0;
+                    }
             double wt = w ? w[i] : 1.0;
             if (v) wt *= v[i];
             int p = 0;
@@ -63,15 +71,77 @@ __global__ __launch_bounds__(TPB) void k_gram(const double* __restrict__ z, size
 }
 
 template <int K>
-void launch_gram(unsigned grid, hipStream_t st, const double* z, size_t n, int ld, int c0, const double* w,
-                 const double* v, double* part) {
-    hipLaunchKernelGGL((k_gram<K>), dim3(grid), dim3(TPB), 0, st, z, n, ld, c0, w, v, part);
+void launch_gram(unsigned grid, hipStream_t st, double* z, size_t n, int ld, int c0, const double* w,
+                 const double* v, double ztol, double* part) {
+    hipLaunchKernelGGL((k_gram<K>), dim3(grid), dim3(TPB), 0, st, z, n, ld, c0, w, v, ztol, part);
 }
 
 template <int... Ks>
-void gram_dispatch(int k, unsigned grid, hipStream_t st, const double* z, size_t n, int ld, int c0, const double* w,
-                   const double* v, double* part, std::integer_sequence<int, Ks...>) {
-    ((k == Ks + 1 ? launch_gram<Ks + 1>(grid, st, z, n, ld, c0, w, v, part) : void()), ...);
+void gram_dispatch(int k, unsigned grid, hipStream_t st, double* z, size_t n, int ld, int c0, const double* w,
+                   const double* v, double ztol, double* part, std::integer_sequence<int, Ks...>) {
+    ((k == Ks + 1 ? launch_gram<Ks + 1>(grid, st, z, n, ld, c0, w, v, ztol, part) : void()), ...);
+}
+
+// Per-workgroup partial Gram of transformed rows y_i = z_i[0..R) T (T: R x M, staged in LDS),
+// weighted by w: the passes of the Cholesky QR factor, which form the products with Q = A R^-1
+// row by row. Pair layout and fixed-order block sums as in k_gram.
+template <int M>
+__global__ __launch_bounds__(TPB) void k_gram_t(const double* __restrict__ z, size_t n, int ld, int R,
+                                                const double* __restrict__ T, const double* __restrict__ w,
+                                                double* __restrict__ part) {
+    constexpr int NP = M * (M + 1) / 2;
+    constexpr int NG = NP <= 36 ? 1 : (NP + 35) / 36;
+    constexpr int GPS = (NP + NG - 1) / NG;
+    __shared__ double Ts[GRAM_KMAX * GRAM_KMAX];
+    __shared__ double red[TPB][GPS];
+    for (int q = threadIdx.x; q < R * M; q += TPB) Ts[q] = T[q];
+    __syncthreads();
+    const int grp = threadIdx.x % NG;
+    const int rl = threadIdx.x / NG, nrl = TPB / NG;
+    double acc[GPS];
+#pragma unroll
+    for (int q = 0; q < GPS; q++) acc[q] = 0.0;
+    if (rl < nrl) {
+        for (size_t i = (size_t)blockIdx.x * nrl + rl; i < n; i += (size_t)gridDim.x * nrl) {
+            double y[M];
+#pragma unroll
+            for (int c = 0; c < M; c++) y[c] = 0.0;
+            const double* row = z + i * ld;
+            for (int a = 0; a < R; a++) {
+                const double xa = row[a];
+#pragma unroll
+                for (int c = 0; c < M; c++) y[c] += xa * Ts[a * M + c];
+            }
+            const double wt = w ? w[i] : 1.0;
+            int p = 0;
+#pragma unroll
+            for (int a = 0; a < M; a++)
+#pragma unroll
+                for (int b = a; b < M; b++, p++)
+                    if (NG == 1 || p / GPS == grp) acc[p % GPS] += wt * y[a] * y[b];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < GPS; q++) red[threadIdx.x][q] = acc[q];
+    __syncthreads();
+    for (int p = threadIdx.x; p < NP; p += TPB) {
+        const int g = p / GPS, q = p % GPS;
+        double sm = 0.0;
+        for (int t = g; t < nrl * NG; t += NG) sm += red[t][q];
+        part[(size_t)blockIdx.x * NP + p] = sm;
+    }
+}
+
+template <int M>
+void launch_gram_t(unsigned grid, hipStream_t st, const double* z, size_t n, int ld, int R, const double* T,
+                   const double* w, double* part) {
+    hipLaunchKernelGGL((k_gram_t<M>), dim3(grid), dim3(TPB), 0, st, z, n, ld, R, T, w, part);
+}
+
+template <int... Ms>
+void gram_t_dispatch(int m, unsigned grid, hipStream_t st, const double* z, size_t n, int ld, int R, const double* T,
+                     const double* w, double* part, std::integer_sequence<int, Ms...>) {
+    ((m == Ms + 1 ? launch_gram_t<Ms + 1>(grid, st, z, n, ld, R, T, w, part) : void()), ...);
 }
 
 // e_i = sum_j c_j z_ij; partial sums of w e^2 per workgroup.
@@ -275,12 +345,14 @@ std::vector<double> sum_parts_dev(const double* part, size_t nparts, size_t len,
 }
 
 void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
-               double* G_host) {
+               double* G_host, double ztol) {
     require(k >= 1 && k <= GRAM_KMAX, "at most 21 regressors are supported");
     const int npair = k * (k + 1) / 2;
     const unsigned grid = std::max(1u, std::min<unsigned>(GRID, (unsigned)((n + 255) / 256)));
     DBuf<double> part((size_t)grid * npair);
-    gram_dispatch(k, grid, a.st, z, n, ld, c0, w, v, part.p, std::make_integer_sequence<int, GRAM_KMAX>{});
+    // the block is written only where ztol edits an entry
+    gram_dispatch(k, grid, a.st, const_cast<double*>(z), n, ld, c0, w, v, ztol, part.p,
+                  std::make_integer_sequence<int, GRAM_KMAX>{});
     FH_CHECK_LAUNCH();
     const std::vector<double> s = sum_parts_dev(part.p, grid, (size_t)npair, a.st);
     int p = 0;
@@ -289,6 +361,58 @@ void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, co
             G_host[x * k + y] = s[p];
             G_host[y * k + x] = s[p];
         }
+}
+
+void gram_transformed(Absorber& a, const double* z, size_t n, int ld, int R, const std::vector<double>& T, int m,
+                      const double* w, double* G_host) {
+    require(m >= 1 && m <= GRAM_KMAX && R >= 1 && R <= GRAM_KMAX && (int)T.size() == R * m,
+            "transformed Gram: bad shape");
+    const int npair = m * (m + 1) / 2;
+    const unsigned grid = std::max(1u, std::min<unsigned>(GRID, (unsigned)((n + 255) / 256)));
+    DBuf<double> part((size_t)grid * npair), Td(T.size());
+    upload(Td.p, T.data(), T.size() * sizeof(double), a.st);
+    gram_t_dispatch(m, grid, a.st, z, n, ld, R, Td.p, w, part.p, std::make_integer_sequence<int, GRAM_KMAX>{});
+    FH_CHECK_LAUNCH();
+    const std::vector<double> s = sum_parts_dev(part.p, grid, (size_t)npair, a.st);
+    int p = 0;
+    for (int x = 0; x < m; x++)
+        for (int y = x; y < m; y++, p++) {
+            G_host[x * m + y] = s[p];
+            G_host[y * m + x] = s[p];
+        }
+}
+
+bool qr_factor(Absorber& a, const double* z, size_t n, int ld, int R, const std::vector<double>& T0, int m,
+               const double* w, const std::vector<double>& G0, std::vector<double>& Rf) {
+    // Shifted Cholesky QR with two refinement passes (Fukaya, Kannan, Nakatsukasa, Yamamoto and
+    // Yanagisawa, SIAM J. Sci. Comput. 42(1), 2020): R1 = chol(A'A + s I) with
+    // s = 11 (m n + m (m + 1)) u ||A||_F^2 exists for any A with cond(A) < 1/u; each further
+    // pass forms Q = A R^-1 row by row, factors Q'Q and folds its factor into R. Two passes bring
+    // Q to orthogonality at the working precision, so R is the triangular factor of A's QR
+    // decomposition up to rounding (positive diagonal).
+    double tr = 0.0;
+    for (int j = 0; j < m; j++) tr += G0[(size_t)j * m + j];
+    if (!(tr > 0.0) || !std::isfinite(tr)) return false;
+    const double s = 11.0 * ((double)m * (double)n + (double)m * (m + 1)) * 1.1102230246251565e-16 * tr;
+    std::vector<double> G1(G0);
+    for (int j = 0; j < m; j++) G1[(size_t)j * m + j] += s;
+    std::vector<double> Racc, Rp, G((size_t)m * m);
+    if (!chol_upper(G1, m, Racc)) return false;
+    for (int pass = 0; pass < 2; pass++) {
+        const std::vector<double> Ri = upper_inverse(Racc, m);
+        std::vector<double> T((size_t)R * m, 0.0);  // T0 Racc^-1
+        for (int r0 = 0; r0 < R; r0++)
+            for (int c = 0; c < m; c++) {
+                double v = 0.0;
+                for (int l = 0; l <= c; l++) v += T0[(size_t)r0 * m + l] * Ri[(size_t)l * m + c];
+                T[(size_t)r0 * m + c] = v;
+            }
+        gram_transformed(a, z, n, ld, R, T, m, w, G.data());
+        if (!chol_upper(G, m, Rp)) return false;
+        Racc = upper_mul(Rp, Racc, m);
+    }
+    Rf = Racc;
+    return true;
 }
 
 void gram(Absorber& a, const double* z, const double* w, size_t n, int r, double* G_host) {

@@ -1,0 +1,104 @@
+// Host-side pieces shared by the estimation entry points: the residualization driver, the
+// variance assembly, the small dense algebra of the K x K systems, and the transfer helpers.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "fh_internal.hpp"
+
+namespace fh {
+
+// ------------------------------------------------------------------ transfers and control
+
+template <class T>
+inline void h2d(T* dst, const T* src, size_t count, hipStream_t st) {
+    if (count) upload(dst, src, count * sizeof(T), st);
+}
+template <class T>
+inline void d2h(T* dst, const T* src, size_t count, hipStream_t st) {
+    if (count) download(dst, src, count * sizeof(T), st);
+}
+// Sum of `count` device partials, read back (waits for the stream).
+double sum_parts(const double* dev, size_t count, hipStream_t st);
+// Throws FH_INTERRUPTED when the caller's interrupt callback asks for it.
+void check_interrupt(const fh_ctx* ctx);
+void log_line(const fh_ctx* ctx, int level, const std::string& s);
+
+// ------------------------------------------------------------------ small dense algebra (fh_linalg.hip)
+
+struct CholPivots {
+    std::vector<double> R, pivot;  // k x k upper factor over the kept columns, every column's pivot
+    std::vector<int> excluded;
+};
+CholPivots chol_pivots(const std::vector<double>& A, int k, double tol);
+// Column-order Cholesky excluding pivots below tol; inv = inverse of the kept block.
+void chol_collin(const std::vector<double>& A, int k, double tol, std::vector<int>& excluded, std::vector<double>& inv);
+// The exclusions of chol_collin, when columns carrying errors up to err could not change them.
+bool chol_collin_certain(const std::vector<double>& A, int k, double tol, const std::vector<double>& err,
+                         std::vector<int>& excluded);
+std::vector<double> upper_inverse(const std::vector<double>& R, int k);
+// Sweep generalized inverse (Mata invsym in column order), relative pivot tolerance.
+std::vector<double> invsym(const std::vector<double>& A, int k, double tol, std::vector<int>* dropped = nullptr);
+constexpr double INVSYM_TOL = 1e-13;
+bool chol_upper(const std::vector<double>& A, int k, std::vector<double>& R);
+std::vector<double> upper_mul(const std::vector<double>& A, const std::vector<double>& B, int k);
+// b solving R_xx b = R_xy for the triangular factor of [X, y] (y last).
+std::vector<double> qr_solve_upper(const std::vector<double>& R, int m);
+
+// ------------------------------------------------------------------ device estimation passes (fh_estimate.hip)
+
+// G = sum_i w_i (z_i T)'(z_i T) over the first R columns of the rows z (stride ld), T R x m
+// (row-major, host): the Gram of transformed rows.
+void gram_transformed(Absorber& a, const double* z, size_t n, int ld, int R, const std::vector<double>& T, int m,
+                      const double* w, double* G_host);
+// e = z[:, :k] c (n), also written to column ecol of z when ecol >= 0; returns sum w e^2.
+double residual_pass(Absorber& a, double* z, size_t n, int ld, int k, const double* c_host, const double* w, double* e,
+                     int ecol);
+void hetero_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, const double* w,
+                 double* meat);
+// e_i at e[i * es]; returns the non-empty clusters.
+uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* e, size_t es,
+                      const double* w, const uint32_t* cl_kept_dev, uint64_t G, double* meat);
+
+// R factor (m x m, upper) of the QR decomposition of sqrt(w) [z_i T0] by shifted Cholesky QR
+// with two refinement passes; T0 (R x m) selects and orders the columns. False when a
+// refinement pass meets a non-positive pivot (numerically rank-deficient columns).
+bool qr_factor(Absorber& a, const double* z, size_t n, int ld, int R, const std::vector<double>& T0, int m,
+               const double* w, const std::vector<double>& G0, std::vector<double>& Rf);
+
+// ------------------------------------------------------------------ residualization (fh_api.hip)
+
+struct ResidOut {
+    uint64_t iterations = 1;
+    bool exact = true;
+    uint64_t refinements = 0;
+    bool escalated = false;
+};
+// Residualizes the n x r device block z (row order) in place; warm / coefs: (m + g1) x r.
+ResidOut residualize_any(Absorber& a, double* z, size_t r, double tol, uint64_t maxiter, const double* warm_dev,
+                         double* coefs_dev, int precision, const fh_ctx* ctx);
+// Puts the caller's own weights back when a Poisson fit left its IRLS weights in the absorber.
+void restore_user_weights(Absorber& a);
+
+// ------------------------------------------------------------------ variance (fh_api.hip)
+
+// fixest df.K (fit/vcov.rs:63-112).
+double df_k(const Absorber& a, int kind, const fh_ssc& ssc, const int32_t* nested, double np, int kdesign);
+// Number of fixed-effect parameters: sum of the term levels minus (terms - 1).
+double fe_params(const Absorber& a);
+struct WbPre;
+void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
+               const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
+               const uint32_t* cl_dev, uint64_t G, const fh_ssc& ssc, double dfk, double iid_scale,
+               std::vector<double>& V, uint64_t* ncl, WbPre* wb = nullptr);
+uint64_t cluster_levels(Absorber& a, const uint32_t* cl_dev);
+
+// ------------------------------------------------------------------ Poisson PML (fh_irls.hip)
+
+void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, const double* offset, int kind,
+              const uint32_t* cl_dev, const int32_t* nested, const fh_ssc& ssc, double tol, uint64_t maxiter,
+              const fh_solve_opts& o, double* beta, double* vcov, int32_t* kept_out, fh_fit_stats* stats,
+              const fh_ctx* ctx);
+
+}  // namespace fh

@@ -105,6 +105,11 @@ struct Absorber {
     // weights
     bool weighted = false;
     DBuf<double> w_rows;  // n (row order) when weighted
+    // A Poisson fit sets the solver's weights every IRLS iteration and leaves its last ones in
+    // place (irls_weights); the caller's own weights (row order, empty = unit) wait in user_w and
+    // come back before the next call that is not a Poisson fit (restore_user_weights).
+    bool irls_weights = false;
+    DBuf<double> user_w;
     // sum of the weights: reduced on the device at each weights refresh (sw_dev) and read with
     // the next readback that needs it (absorber_sw, or the rhs sums of a residualization)
     double sw = 0.0;
@@ -286,8 +291,10 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
 // G = sum_i w_i z_i z_i' (r x r, FP64 on the host), z n x r row-major; w may be null.
 void gram(Absorber& a, const double* z, const double* w, size_t n, int r, double* G_host);
 // Columns c0..c0+k of z (ld), optional row weights w and v: G = sum_i w_i v_i z_i z_i' (host).
+// ztol >= 0: entries of magnitude <= ztol are set to 0 in z as they are read (Mata
+// _edittozerotol), and the Gram is of the edited block.
 void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
-               double* G_host);
+               double* G_host, double ztol = -1.0);
 // Clusters as runs of adjacent solver rows with one cluster code: the rows of cluster g are the
 // runs order[off[g] .. off[g+1]) (in row order), run q spanning rows run_off[q] .. run_off[q+1].
 // Rows in solver order follow the absorbed terms, so a cluster that is (or nests) a term comes in

@@ -50,6 +50,7 @@ struct fh_server {
     uint64_t next_ticket = 1, inflight_bytes = 0, running = 0;
     bool stop = false;
     std::vector<std::thread> workers;
+    Clock::time_point created = Clock::now();
 
     void run_job(Job* j) {
         fh_response& out = j->resp;
@@ -64,8 +65,8 @@ struct fh_server {
         if (rc == FH_OK) {
             fh_solve_opts o{r.tol, r.maxiter, 0, -1};
             if (r.model == FH_MODEL_PPML)
-                rc = fh_ppml_fit(ab, r.y, r.X, r.K, r.vcov_kind, r.cluster, r.nested, &r.ssc, r.tol, r.maxiter, &o,
-                                 r.beta, r.vcov, r.kept, &st);
+                rc = fh_ppml_fit(ab, r.y, r.X, r.K, nullptr, r.vcov_kind, r.cluster, r.nested, &r.ssc, r.tol, r.maxiter,
+                                 &o, r.beta, r.vcov, r.kept, &st, nullptr);
             else
                 rc = fh_ols_fit(ab, r.y, r.X, r.K, r.vcov_kind, r.cluster, r.nested, &r.ssc, &o, r.beta, r.vcov,
                                 r.kept, &st);
@@ -103,6 +104,8 @@ struct fh_server {
                 j->resp.queued_s = std::chrono::duration<double>(j->started - j->submitted).count();
                 j->resp.service_s = std::chrono::duration<double>(j->finished - j->started).count();
                 j->resp.total_s = std::chrono::duration<double>(j->finished - j->submitted).count();
+                j->resp.start_s = std::chrono::duration<double>(j->started - created).count();
+                j->resp.end_s = std::chrono::duration<double>(j->finished - created).count();
                 j->done = true;
                 inflight_bytes -= j->bytes;
                 running--;

@@ -547,71 +547,90 @@ double qnorm(double p) {
     return q < 0.0 ? -v : v;
 }
 
-// mod.rs:637-749: bracket outwards from the point, then Chandrupatla / bisection on p - alpha.
+// The test-inverted confidence interval (mod.rs:637-749): each endpoint is where the bootstrap
+// p-value of H0: R beta = x falls below alpha moving away from the point estimate. The settings
+// that decide the result follow the reference: the first step hw = max(1.5 z se, 1e-6 (1 + |point|))
+// with z = max(|qnorm(1 - alpha/2)|, 1.5), up to 16 doublings of the step to find a value below
+// alpha, and a search that ends once the bracket is x_tol = tol (1 + |point|) wide or a bracket end
+// lies within p_tol = 1.0001 (1 or 2 for equal tails) / (feasible draws) of alpha, i.e. within one
+// draw of it. p(x) is a step function, so every point meeting that rule is a valid endpoint.
+//
+// The root finder is Chandrupatla's (T. R. Chandrupatla, "A new hybrid quadratic/bisection
+// algorithm for finding the zero of a nonlinear function without using derivatives", Advances in
+// Engineering Software 28 (1997) 145-149): it keeps a bracket [a, b] with the newest point a,
+// the previous point c, and takes inverse quadratic interpolation through (a, b, c) when the
+// paper's validity test on xi = (a - b)/(c - b) and phi = (fa - fb)/(fc - fb) holds, else
+// bisects.
+struct CiRoot {
+    double x_tol, f_tol;
+    uint64_t maxiter;
+
+    // f(x_in) >= 0 > f(x_out); returns the end of the final bracket with the smaller |f|.
+    template <class F>
+    double solve(F&& f, double x_in, double f_in, double x_out, double f_out) const {
+        double a = x_in, fa = f_in;  // newest point
+        double b = x_out, fb = f_out;  // other end of the bracket (opposite sign)
+        double c = b, fc = fb;         // point dropped last
+        double t = 0.5;
+        auto done = [&] { return std::fabs(b - a) <= x_tol || std::min(std::fabs(fa), std::fabs(fb)) <= f_tol; };
+        for (uint64_t it = 0; it < maxiter && !done(); it++) {
+            const double width = std::fabs(b - a);
+            const double tlim = std::min(0.5, 0.5 * x_tol / width);
+            t = std::min(std::max(t, tlim), 1.0 - tlim);
+            double xt = a + t * (b - a), ft = f(xt);
+            if (!std::isfinite(ft)) {  // an infeasible trial: halve the bracket instead
+                xt = 0.5 * (a + b);
+                ft = f(xt);
+                if (!std::isfinite(ft)) break;
+            }
+            if ((ft >= 0.0) == (fa >= 0.0)) {
+                c = a;
+                fc = fa;
+            } else {
+                c = b;
+                fc = fb;
+                b = a;
+                fb = fa;
+            }
+            a = xt;
+            fa = ft;
+            t = 0.5;
+            const double dcb = c - b, dfcb = fc - fb;
+            if (dcb != 0.0 && dfcb != 0.0 && fb != fa && fc != fa && b != a) {
+                const double xi = (a - b) / dcb, phi = (fa - fb) / dfcb;
+                if (phi * phi < xi && (1.0 - phi) * (1.0 - phi) < 1.0 - xi)
+                    t = fa / (fb - fa) * fc / (fb - fc) + (c - a) / (b - a) * fa / (fc - fa) * fb / (fc - fb);
+            }
+        }
+        return std::fabs(fa) <= std::fabs(fb) ? a : b;
+    }
+};
+
 template <class F>
 std::pair<double, double> invert_ci_search(double point, double se, double alpha, int pval_type, double tol,
                                            uint64_t maxiter, uint64_t reps_feas, F&& eval_p) {
     if (!std::isfinite(point) || !std::isfinite(se) || se <= 0.0 || reps_feas == 0) return {NAN, NAN};
     const double z = std::max(std::fabs(qnorm(1.0 - alpha / 2.0)), 1.5);
     const double hw = std::max(1.5 * z * se, 1e-6 * (1.0 + std::fabs(point)));
-    const double eq = pval_type == FH_PVAL_EQUAL_TAILED ? 2.0 : 1.0;
-    const double p_tol = (eq / (double)reps_feas) * 1.0001;
-    const double x_tol = tol * (1.0 + std::fabs(point));
-    auto find_bound = [&](double sign) -> double {
-        double x_in = point, p_in = eval_p(x_in);
-        if (!std::isfinite(p_in) || p_in < alpha) p_in = 1.0;
-        double step = hw, x_out = point + sign * step, p_out = NAN;
-        bool bracketed = false;
-        for (int i = 0; i < 16; i++) {
-            const double p = eval_p(x_out);
-            if (std::isfinite(p)) {
-                if (p < alpha) {
-                    p_out = p;
-                    bracketed = true;
-                    break;
-                }
-                x_in = x_out;
-                p_in = p;
-            }
-            step *= 2.0;
-            x_out = point + sign * step;
+    const double tails = pval_type == FH_PVAL_EQUAL_TAILED ? 2.0 : 1.0;
+    const CiRoot root{tol * (1.0 + std::fabs(point)), 1.0001 * tails / (double)reps_feas, maxiter};
+    auto f = [&](double x) { return eval_p(x) - alpha; };
+    auto endpoint = [&](double dir) -> double {
+        // the last point with p >= alpha (the estimate itself counts as one) and the first below
+        double inside = point, f_inside = f(point);
+        if (!std::isfinite(f_inside) || f_inside < 0.0) f_inside = 1.0 - alpha;
+        double step = hw;
+        for (int k = 0; k < 16; k++, step *= 2.0) {
+            const double x = point + dir * step, fx = f(x);
+            if (!std::isfinite(fx)) continue;
+            if (fx < 0.0) return root.solve(f, inside, f_inside, x, fx);
+            inside = x;
+            f_inside = fx;
         }
-        if (!bracketed) return sign < 0.0 ? -INFINITY : INFINITY;
-        double x1 = x_in, f1 = p_in - alpha, x2 = x_out, f2 = p_out - alpha, x3 = 0.0, f3 = 0.0, t = 0.5;
-        for (uint64_t it = 0; it < maxiter; it++) {
-            if (std::fabs(x2 - x1) <= x_tol || std::min(std::fabs(f1), std::fabs(f2)) <= p_tol) break;
-            const double x = x1 + t * (x2 - x1);
-            const double fx = eval_p(x) - alpha;
-            if (!std::isfinite(fx)) {
-                t = 0.5;
-                const double xm = 0.5 * (x1 + x2), fm = eval_p(xm) - alpha;
-                if (!std::isfinite(fm)) break;
-                if ((fm < 0.0) == (f1 < 0.0)) { x1 = xm; f1 = fm; } else { x2 = xm; f2 = fm; }
-                continue;
-            }
-            if ((fx < 0.0) == (f1 < 0.0)) {
-                x3 = x1; f3 = f1; x1 = x; f1 = fx;
-            } else {
-                x3 = x2; f3 = f2; x2 = x; f2 = fx;
-            }
-            if (std::fabs(x2 - x1) <= x_tol || std::min(std::fabs(f1), std::fabs(f2)) <= p_tol) break;
-            const double d32 = f3 - f2, d12 = f1 - f2, d31 = f3 - f1;
-            if (d32 == 0.0 || d12 == 0.0 || d31 == 0.0 || (x2 - x1) == 0.0 || (x3 - x2) == 0.0) {
-                t = 0.5;
-                continue;
-            }
-            const double phi1 = d12 / d32, xi1 = (x1 - x2) / (x3 - x2);
-            if (phi1 * phi1 > xi1 || xi1 > 2.0 * phi1 - phi1 * phi1) {
-                t = 0.5;
-            } else {
-                const double raw = (f3 / d12 + (x3 - x1) / ((x2 - x1) * d31) * f2) * (f1 / d32);
-                t = std::min(std::max(raw, 1e-6), 1.0 - 1e-6);
-            }
-        }
-        return std::fabs(f1) <= std::fabs(f2) ? x1 : x2;
+        return dir * INFINITY;  // p stays above alpha: the interval is open on this side
     };
-    const double lo = pval_type == FH_PVAL_LESS ? -INFINITY : find_bound(-1.0);
-    const double hi = pval_type == FH_PVAL_GREATER ? INFINITY : find_bound(1.0);
+    const double lo = pval_type == FH_PVAL_LESS ? -INFINITY : endpoint(-1.0);
+    const double hi = pval_type == FH_PVAL_GREATER ? INFINITY : endpoint(1.0);
     return {lo, hi};
 }
 

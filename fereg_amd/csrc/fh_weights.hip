@@ -10,9 +10,6 @@
 
 namespace fh {
 
-void gram_cols(Absorber& a, const double* z, size_t n, int ld, int c0, int k, const double* w, const double* v,
-               double* G_host);
-
 namespace {
 
 constexpr unsigned TPB = 256;

@@ -11,7 +11,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import _lib as L
-from .absorber import Absorber, SolveOpts, _dev_ready, _is_torch, _ptr
+from .absorber import Absorber, SolveOpts, _dev_codes, _dev_f64, _dev_ready, _is_torch, _ptr
 
 
 @dataclass
@@ -27,6 +27,7 @@ class FitResult:
     pcg_iterations: int
     n_clusters: int
     se: np.ndarray = field(init=False)
+    converged: bool = True
 
     def __post_init__(self):
         self.se = np.sqrt(np.clip(np.diag(self.vcov), 0.0, None))
@@ -54,14 +55,29 @@ def nested_fixest(fe_codes, cluster_codes, fe_names=None, cluster_names=None):
     return nested
 
 
-def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, extra):
+def _host_f64(a, what, n=None):
+    v = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and v.shape[0] != n:
+        raise L.FeregError(L.FH_INVALID, f"{what}: {v.shape[0]} rows, the sample has {n}")
+    return v
+
+
+def _host_codes(a, what):
+    v = np.asarray(a)
+    if v.size and (not np.issubdtype(v.dtype, np.integer) or v.min() < 0 or v.max() >= 2 ** 32):
+        raise L.FeregError(L.FH_INVALID, f"{what}: integer codes in 0 .. 2^32 - 1 expected")
+    return np.ascontiguousarray(v, dtype=np.uint32)
+
+
+def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, extra_args, offset=None):
     lib = L.lib()
     X = np.asarray(X, float) if not _is_torch(X) else X
     if not _is_torch(X) and X.ndim == 1:
         X = X[:, None]
     K = X.shape[1]
+    if absorber is not None and weights is not None:
+        raise L.FeregError(L.FH_INVALID, "weights belong to the absorber: pass them when it is built")
     ab = absorber or Absorber(codes, g, weights=weights)
-    ab._unit_if_irls()
     kind = L.FH_VCOV[vcov]
     ssc = ssc or {}
     sscc = L.Ssc(int(ssc.get("K.adj", True)), L.FH_KFIXEF[ssc.get("K.fixef", "nonnested")], int(ssc.get("G.adj", True)), 0)
@@ -74,27 +90,36 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
     kept = np.zeros(K, np.int32)
     st = L.FitStats()
     o = (opts or SolveOpts(tol=1e-10)).c()
+    poisson = kind_fn == "fh_ppml_fit"
     if _is_torch(y):
-        held = _dev_ready(y, X, cluster)  # kept alive through the call
-        cl = None if cluster is None else held[2].data_ptr()
+        held = _dev_ready(_dev_f64(y, "y"), _dev_f64(X, "X"), _dev_codes(cluster, "cluster"),
+                          _dev_f64(offset, "offset"))  # kept alive through the call
         fn = getattr(lib, kind_fn + "_dev")
-        args = [ab._h, held[0].data_ptr(), held[1].data_ptr(), K, kind, cl, _ptr(nest, L.i32p),
-                ctypes.byref(sscc)]
+        args = [ab._h, held[0].data_ptr(), held[1].data_ptr(), K]
+        if poisson:
+            args.append(None if offset is None else held[3].data_ptr())
+        args += [kind, None if cluster is None else held[2].data_ptr(), _ptr(nest, L.i32p), ctypes.byref(sscc)]
     else:
-        yv = np.ascontiguousarray(y, dtype=np.float64)
-        Xv = np.ascontiguousarray(X, dtype=np.float64)
-        cl = None if cluster is None else np.ascontiguousarray(cluster, dtype=np.uint32)
+        n = ab.n()
+        yv = _host_f64(y, "y", n)
+        Xv = _host_f64(X, "X", n)
+        cl = None if cluster is None else _host_codes(cluster, "cluster")
         fn = getattr(lib, kind_fn)
-        args = [ab._h, _ptr(yv, L.f64p), _ptr(Xv, L.f64p), K, kind, None if cl is None else _ptr(cl, L.u32p),
-                _ptr(nest, L.i32p), ctypes.byref(sscc)]
-    args += extra + [ctypes.byref(o), _ptr(beta, L.f64p), _ptr(V, L.f64p), _ptr(kept, L.i32p), ctypes.byref(st)]
-    try:
-        L.check(fn(*args))
-    finally:
-        if kind_fn == "fh_ppml_fit":
-            ab._irls_weights = True
-    return FitResult(beta, V.reshape(K, K), kept.astype(bool), st.df_k, st.nparams, st.ssr, st.deviance,
-                     st.iterations, st.pcg_iterations, st.n_clusters)
+        args = [ab._h, _ptr(yv, L.f64p), _ptr(Xv, L.f64p), K]
+        if poisson:
+            ov = None if offset is None else _host_f64(offset, "offset", n)
+            args.append(None if ov is None else _ptr(ov, L.f64p))
+        args += [kind, None if cl is None else _ptr(cl, L.u32p), _ptr(nest, L.i32p), ctypes.byref(sscc)]
+    args += extra_args + [ctypes.byref(o), _ptr(beta, L.f64p), _ptr(V, L.f64p), _ptr(kept, L.i32p), ctypes.byref(st)]
+    if poisson:
+        args.append(None)
+    L.check(fn(*args))
+    res = FitResult(beta, V.reshape(K, K), kept.astype(bool), st.df_k, st.nparams, st.ssr, st.deviance,
+                    st.iterations, st.pcg_iterations, st.n_clusters)
+    res.converged = bool(st.converged)
+    if absorber is None:
+        ab.close()
+    return res
 
 
 def feols(y, X, codes, g, weights=None, vcov="iid", cluster=None, nested=None, ssc=None, opts=None, absorber=None):
@@ -102,11 +127,12 @@ def feols(y, X, codes, g, weights=None, vcov="iid", cluster=None, nested=None, s
     return _fit("fh_ols_fit", y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, [])
 
 
-def fepois(y, X, codes, g, vcov="hetero", cluster=None, nested=None, ssc=None, tol=1e-8, maxiter=100, opts=None,
-           absorber=None):
-    """Poisson PML by IRLS (glm/irls.rs:367-913) on the GPU."""
-    return _fit("fh_ppml_fit", y, X, codes, g, None, vcov, cluster, nested, ssc, opts, absorber,
-                [ctypes.c_double(tol), ctypes.c_uint64(maxiter)])
+def fepois(y, X, codes, g, weights=None, offset=None, vcov="hetero", cluster=None, nested=None, ssc=None, tol=1e-8,
+           maxiter=100, opts=None, absorber=None):
+    """Poisson PML by IRLS (glm/irls.rs:367-913, fixest flavour) on the GPU. `weights` are prior
+    weights (or the absorber's, when one is passed), `offset` enters the linear predictor."""
+    return _fit("fh_ppml_fit", y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber,
+                [ctypes.c_double(tol), ctypes.c_uint64(maxiter)], offset=offset)
 
 
 @dataclass
@@ -140,7 +166,6 @@ def wildboot(y, X, cluster, R, r=0.0, reps=999, seed=1, level=0.95, pval_type="t
         Xa = Xa[:, None]
     K = Xa.shape[1]
     ab = absorber or Absorber(codes, g)
-    ab._unit_if_irls()
     ssc = ssc or {}
     sscc = L.Ssc(int(ssc.get("K.adj", True)), L.FH_KFIXEF[ssc.get("K.fixef", "nonnested")], int(ssc.get("G.adj", True)), 0)
     nt = ab.info.nterms
@@ -159,13 +184,13 @@ def wildboot(y, X, cluster, R, r=0.0, reps=999, seed=1, level=0.95, pval_type="t
     o = (opts or SolveOpts(tol=1e-10)).c()
     if _is_torch(y):
         fn = lib.fh_ols_wildboot_dev
-        held = _dev_ready(y, Xa, cluster)  # kept alive through the call
+        held = _dev_ready(_dev_f64(y, "y"), _dev_f64(Xa, "X"), _dev_codes(cluster, "cluster"))  # kept alive
         args = [ab._h, held[0].data_ptr(), held[1].data_ptr(), K, held[2].data_ptr()]
     else:
         fn = lib.fh_ols_wildboot
         yv = np.ascontiguousarray(y, dtype=np.float64)
         Xv = np.ascontiguousarray(Xa, dtype=np.float64)
-        cl = np.ascontiguousarray(cluster, dtype=np.uint32)
+        cl = _host_codes(cluster, "cluster")
         args = [ab._h, _ptr(yv, L.f64p), _ptr(Xv, L.f64p), K, _ptr(cl, L.u32p)]
     L.check(fn(*args, _ptr(nest, L.i32p), ctypes.byref(sscc), ctypes.byref(o), ctypes.byref(spec), _ptr(beta, L.f64p),
                _ptr(V, L.f64p), _ptr(kept, L.i32p), ctypes.byref(res), _ptr(tb, L.f64p)))

@@ -29,6 +29,8 @@ class Reply:
     service_s: float
     total_s: float
     iterations: int
+    start_s: float = 0.0  # admission and completion, seconds since the server was created
+    end_s: float = 0.0
 
 
 class _Pending:
@@ -75,7 +77,7 @@ class Server:
             p = self._pending.pop(ticket)
         K = p.beta.size
         return Reply(p.beta, p.vcov.reshape(K, K), p.kept.astype(bool), r.rc, r.queued_s, r.service_s, r.total_s,
-                     int(r.iterations))
+                     int(r.iterations), r.start_s, r.end_s)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -218,17 +218,19 @@ int fh_ols_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, 
                    double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
 
 /* Poisson PML by IRLS with warm-started weighted residualization (glm/irls.rs:367-913, fixest
- * flavour; family.rs:103-145), robust (hetero) or iid or cluster variance. y, X kept order.
- * The absorber must be unweighted on entry and holds the last IRLS weights on return (the
- * solver's weights are set every iteration); fh_absorber_set_weights(ab, NULL) restores unit
- * weights before the absorber is reused. */
-int fh_ppml_fit(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t vcov_kind,
+ * flavour: fit/glm.rs; family.rs:103-145), robust (hetero), iid or cluster variance. y, X and
+ * offset (NULL = none) in KEPT order. The absorber's weights (fh_absorber_create / set_weights)
+ * are the prior weights. The solver's weights are set every IRLS iteration; the absorber puts
+ * the prior weights back before its next call that is not a Poisson fit. ctx may be NULL; its
+ * interrupt callback is polled between IRLS iterations and between PCG iterations (rc 1). */
+int fh_ppml_fit(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset, int32_t vcov_kind,
                 const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
-                const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
-int fh_ppml_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t vcov_kind,
-                    const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol,
+                const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats,
+                const fh_ctx* ctx);
+int fh_ppml_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset,
+                    int32_t vcov_kind, const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol,
                     uint64_t maxiter, const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept,
-                    fh_fit_stats* stats);
+                    fh_fit_stats* stats, const fh_ctx* ctx);
 
 /* WCR11 wild cluster bootstrap of one linear hypothesis R beta = r on the OLS fit, Rademacher
  * weights, clusters = bootstrap clusters (wildboot/ols.rs:243-649 run_cluster_11; weights.rs;
@@ -304,6 +306,7 @@ typedef struct {
     int32_t reserved;
     double queued_s, service_s, total_s; /* wait for admission, on the device, submit to done */
     uint64_t iterations, pcg_iterations;
+    double start_s, end_s;               /* admission and completion, seconds since fh_server_create */
 } fh_response;
 void fh_server_config_default(fh_server_config* cfg);
 int fh_server_create(const fh_server_config* cfg, fh_server** out);

@@ -291,8 +291,26 @@ def wcr11_result(beta, xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95
             return compute_pvalue(st, tb, pval_type)[0]
 
         ci = invert_ci_search(point, se, alpha, pval_type, tol, maxiter, feas, eval_p)
+    else:
+        eval_p = None
+    eq = 2.0 if pval_type == "equal-tailed" else 1.0
     return {"stat": stat, "se": se, "point": point, "p_val": p, "feasible": feas, "ci": ci, "t_boot": t_boot,
-            "coefs": c, "full": core["full"], "reps": c.shape[0]}
+            "coefs": c, "full": core["full"], "reps": c.shape[0], "eval_p": eval_p, "alpha": 1.0 - level,
+            "p_tol": 1.0001 * eq / feas if feas else float("nan"), "x_tol": tol * (1.0 + abs(point))}
+
+
+def ci_endpoint_ok(x, ref):
+    """mod.rs:673-747's stopping rule as a property of an endpoint x: p(x) lies within one draw
+    (p_tol) of alpha, or p - alpha changes sign within x_tol of x (an infinite endpoint needs
+    p >= alpha at every bracketing step, i.e. the reference's own infinite endpoint)."""
+    f = lambda v: ref["eval_p"](v) - ref["alpha"]
+    if not math.isfinite(x):
+        return True
+    fx = f(x)
+    if math.isfinite(fx) and abs(fx) <= ref["p_tol"]:
+        return True
+    a, b = f(x - ref["x_tol"]), f(x + ref["x_tol"])
+    return (a >= 0) != (b >= 0) or min(abs(a), abs(b)) <= ref["p_tol"]
 
 
 def brute_t_boot(X, y, w, D, cl, G, R, r, reps, seed, ssc):

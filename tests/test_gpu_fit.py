@@ -18,13 +18,14 @@ def _run(case, model):
     cl = inp["clusters"][0] if inp["clusters"] else None
     codes, gs = inp["codes"], inp["g"]
     w = inp["w"]
-    if not codes and w is None:
-        w = np.ones(len(inp["y"]))
     if model == "ols":
+        if not codes and w is None:
+            w = np.ones(len(inp["y"]))
         return F.feols(inp["y"], inp["X"], codes, gs, weights=w, vcov=kind, cluster=cl, nested=nested,
                        ssc=case["ssc"], opts=F.SolveOpts(tol=1e-12))
-    return F.fepois(inp["y"], inp["X"], codes, gs, vcov=kind, cluster=cl, nested=nested, ssc=case["ssc"],
-                    tol=1e-12, opts=F.SolveOpts(tol=1e-12))
+    # the fixtures were written with glm.tol = 1e-12 (tests/fit_glm.rs:1-2)
+    return F.fepois(inp["y"], inp["X"], codes, gs, weights=w, offset=inp["offset"], vcov=kind, cluster=cl,
+                    nested=nested, ssc=case["ssc"], tol=1e-12, opts=F.SolveOpts(tol=1e-12))
 
 
 @pytest.mark.parametrize("case", [c for c in cases("fixest_ols.json") if len(c["cluster"]) <= 1],
@@ -40,15 +41,67 @@ def test_fixest_ols(case):
     assert abs(res.df_k - e["df_k"]) < 1e-9
 
 
-@pytest.mark.parametrize("case", [c for c in cases("fixest_pois.json") if not c.get("offset") and not c.get("weights")
-                                  and len(c["cluster"]) <= 1], ids=lambda c: c["name"])
+@pytest.mark.parametrize("case", [c for c in cases("fixest_pois.json") if len(c["cluster"]) <= 1],
+                         ids=lambda c: c["name"])
 def test_fixest_poisson(case):
+    """fixest's IRLS path: iteration count, deviance, coefficients and variance at the
+    reference's own tolerance (tests/fit_glm.rs:12)."""
     announce("fixest poisson", case["name"])
     res = _run(case, "pois")
     e = case["expect"]
-    np.testing.assert_allclose(res.coef[res.kept], np.atleast_1d(e["coef"]), rtol=1e-7)
+    assert res.converged and res.iterations == e["iterations"]
+    np.testing.assert_allclose(res.coef[res.kept], np.atleast_1d(e["coef"]), rtol=1e-8)
     V = res.vcov[np.ix_(res.kept, res.kept)]
-    np.testing.assert_allclose(V.reshape(-1), np.array(e["vcov"], float).reshape(-1), rtol=1e-6)
+    np.testing.assert_allclose(V.reshape(-1), np.array(e["vcov"], float).reshape(-1), rtol=1e-8)
+    assert abs(res.deviance - e["deviance"]) <= 1e-8 * e["deviance"]
+
+
+@pytest.mark.parametrize("vcov", ["iid", "hetero", "cluster"])
+def test_poisson_matches_oracle_weighted_with_offset(vcov):
+    """A three-way design with prior weights and an offset against the numpy IRLS oracle."""
+    import fereg_amd as F
+    from oracle import poisson
+    rng = np.random.default_rng(21)
+    n = 40_000
+    gs = [4000, 300, 9]
+    codes = [rng.integers(0, g, n).astype(np.uint32) for g in gs]
+    X = rng.standard_normal((n, 2))
+    off = 0.2 * rng.standard_normal(n)
+    w = rng.uniform(0.5, 2.0, n)
+    fe = sum(0.3 * rng.standard_normal(g)[c] for c, g in zip(codes, gs))
+    y = rng.poisson(np.exp(0.1 * X[:, 0] - 0.2 * X[:, 1] + fe + off)).astype(float)
+    cl = codes[1] if vcov == "cluster" else None
+    nested = [False, True, False] if vcov == "cluster" else None
+    announce("poisson vs oracle", vcov)
+    res = F.fepois(y, X, codes, gs, weights=w, offset=off, vcov=vcov, cluster=cl, nested=nested, tol=1e-10,
+                   opts=F.SolveOpts(tol=1e-12))
+    ref = poisson.ppml(codes, gs, y, X, weights=w, offset=off, tol=1e-10, itol=1e-12, vcov=vcov, cluster=cl,
+                       nested=nested)
+    assert res.iterations == ref["iterations"]
+    np.testing.assert_allclose(res.coef, ref["coef"], rtol=1e-8)
+    np.testing.assert_allclose(res.se, ref["se"], rtol=1e-7)
+    assert abs(res.deviance - ref["deviance"]) <= 1e-9 * ref["deviance"]
+
+
+def test_prior_weights_survive_a_poisson_fit():
+    """A weighted absorber through fepois, then feols: the OLS fit is the weighted one (the
+    absorber restores the caller's weights after the IRLS), bit for bit."""
+    import fereg_amd as F
+    rng = np.random.default_rng(5)
+    n = 30_000
+    gs = [2000, 50]
+    codes = [rng.integers(0, g, n).astype(np.uint32) for g in gs]
+    w = rng.uniform(0.5, 2.0, n)
+    x = rng.standard_normal((n, 1))
+    y = x[:, 0] + rng.standard_normal(n)
+    yc = rng.poisson(np.exp(0.2 * x[:, 0])).astype(float)
+    announce("prior weights after poisson")
+    ab = F.Absorber(codes, gs, weights=w)
+    F.fepois(yc, x, None, None, vcov="hetero", tol=1e-10, opts=F.SolveOpts(tol=1e-12), absorber=ab)
+    r1 = F.feols(y, x, None, None, vcov="hetero", opts=F.SolveOpts(tol=1e-12), absorber=ab)
+    ab.close()
+    r0 = F.feols(y, x, codes, gs, weights=w, vcov="hetero", opts=F.SolveOpts(tol=1e-12))
+    assert np.array_equal(r0.coef, r1.coef) and np.array_equal(r0.vcov, r1.vcov)
 
 
 def test_poisson_refit_on_one_absorber_is_bitwise_stable():

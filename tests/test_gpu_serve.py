@@ -41,7 +41,8 @@ def test_results_identical_alone_and_in_the_mix():
 
 
 def test_admission_by_memory_budget():
-    """A budget below one request's footprint still runs it (alone), one at a time."""
+    """A budget below one request's footprint still runs each request, alone: with four slots free
+    the engine admits one request at a time, so no two service intervals overlap."""
     from fereg_amd import serve
     specs = serve.poisson_stream(4, seed=5, n_lo=2e3, n_hi=2e4, ppml_share=0.0)
     srv = serve.Server(slots=4, hbm_budget=1)
@@ -49,6 +50,22 @@ def test_admission_by_memory_budget():
     replies = [srv.wait(t) for t in tickets]
     srv.close()
     assert all(r.rc == 0 for r in replies)
-    # one at a time: every request but the first waited for the one before it
-    starts = sorted(r.total_s - r.service_s for r in replies)
-    assert len(starts) == 4
+    spans = sorted((r.start_s, r.end_s) for r in replies)
+    for (s0, e0), (s1, e1) in zip(spans, spans[1:]):
+        assert s0 <= e0 <= s1 <= e1, spans
+    # FIFO admission: the service order is the submission order
+    assert [r.start_s for r in replies] == sorted(r.start_s for r in replies)
+
+
+def test_admission_overlaps_with_room():
+    """With room for every request, requests submitted together are in flight together."""
+    from fereg_amd import serve
+    specs = serve.poisson_stream(4, seed=6, n_lo=2e4, n_hi=4e4, ppml_share=0.0)
+    data = [serve.make_request(s) for s in specs]
+    srv = serve.Server(slots=4)
+    tickets = [srv.submit(s["model"], *d) for s, d in zip(specs, data)]
+    replies = [srv.wait(t) for t in tickets]
+    srv.close()
+    assert all(r.rc == 0 for r in replies)
+    first_end = min(r.end_s for r in replies)
+    assert sum(r.start_s < first_end for r in replies) >= 2

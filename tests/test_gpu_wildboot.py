@@ -1,5 +1,7 @@
 """WCR11 bootstrap on the GPU (int8 MFMA contraction) against the oracle's cell-score formula
 (pinned on the CPU against the brute-force refit) on the same residualized design."""
+import math
+
 import numpy as np
 import pytest
 
@@ -34,8 +36,12 @@ def check(res, ref, rtol=1e-7):
     assert abs(res.stat - ref["stat"]) <= 1e-8 * (1 + abs(ref["stat"]))
     np.testing.assert_allclose(res.t_boot, ref["t_boot"], rtol=rtol, atol=rtol)
     assert res.p_val == ref["p_val"] and res.reps_feasible == ref["feasible"]
+    # p(x) is a step function: every point meeting the search's stopping rule is an endpoint, and
+    # two searches may stop at different such points (within about one draw's step of p)
     for a, b in zip(res.conf_int, ref["ci"]):
-        assert a == b or abs(a - b) <= 1e-6 * (1 + abs(b)), (res.conf_int, ref["ci"])
+        assert math.isfinite(a) == math.isfinite(b), (res.conf_int, ref["ci"])
+        assert wb.ci_endpoint_ok(a, ref), (a, res.conf_int, ref["ci"])
+        assert a == b or abs(a - b) <= 0.25 * ref["se"], (res.conf_int, ref["ci"])
 
 
 @pytest.mark.parametrize("weighted", [False, True])
