@@ -103,8 +103,7 @@ def vcov(kind, bread, X, e, w, n, dfk, cluster=None, k_adj=True, g_adj=True):
     if kind == "cluster":
         codes = np.asarray(cluster)
         G = int(codes.max()) + 1
-        sg = np.zeros((G, X.shape[1]))
-        np.add.at(sg, codes, s)
+        sg = np.stack([np.bincount(codes, weights=s[:, j], minlength=G) for j in range(X.shape[1])], axis=1)
         Geff = len(np.unique(codes))
         meat = sg.T @ sg
         factor = 1.0

@@ -17,6 +17,7 @@
 #include "fereg_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -133,15 +134,6 @@ static int cmp_rows(const void* a, const void* b) {
     return x < y ? -1 : (x > y);
 }
 
-static const uint32_t* g_lvl_codes;
-static uint64_t g_lvl_nb, g_lvl_b;
-static int cmp_cells(const void* a, const void* b) {
-    uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
-    uint32_t cx = g_lvl_codes[(uint64_t)x * g_lvl_nb + g_lvl_b], cy = g_lvl_codes[(uint64_t)y * g_lvl_nb + g_lvl_b];
-    if (cx != cy) return cx < cy ? -1 : 1;
-    return x < y ? -1 : (x > y);
-}
-
 static void weight_refresh(or_solver* s);
 
 int or_build(const uint32_t* const* codes, const uint64_t* g, uint64_t nt, uint64_t n, uint64_t small_limit,
@@ -210,9 +202,24 @@ int or_build(const uint32_t* const* codes, const uint64_t* g, uint64_t nt, uint6
     else { zeros = calloc(n ? n : 1, sizeof(uint32_t)); cols[nc++] = zeros; }
     for (uint64_t b = 0; b < nb; b++) cols[nc++] = codes[s->bterm[b]];
     s->pi = malloc((n ? n : 1) * sizeof(uint32_t));
-    for (uint64_t i = 0; i < n; i++) s->pi[i] = (uint32_t)i;
     g_sort_cols = cols; g_sort_ncols = nc;
-    qsort(s->pi, n, sizeof(uint32_t), cmp_rows);
+    {
+        /* The sort by (FE1 code, tuple, obs) as a stable counting sort by the FE1 code (the
+         * primary key; obs ascending within each code) followed by a sort of each FE1 group by
+         * (tuple, obs): the same order as one sort, the groups sorted in parallel. */
+        uint64_t g1c = s->fe1 >= 0 ? g[s->fe1] : 1;
+        uint64_t* goff = calloc(g1c + 1, sizeof(uint64_t));
+        for (uint64_t i = 0; i < n; i++) goff[cols[0][i] + 1]++;
+        for (uint64_t k = 0; k < g1c; k++) goff[k + 1] += goff[k];
+        uint64_t* cur = malloc((g1c + 1) * sizeof(uint64_t));
+        memcpy(cur, goff, (g1c + 1) * sizeof(uint64_t));
+        for (uint64_t i = 0; i < n; i++) s->pi[cur[cols[0][i]]++] = (uint32_t)i;
+        free(cur);
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (uint64_t k = 0; k < g1c; k++)
+            if (goff[k + 1] - goff[k] > 1) qsort(s->pi + goff[k], goff[k + 1] - goff[k], sizeof(uint32_t), cmp_rows);
+        free(goff);
+    }
 
     /* cells = runs of equal tuples; seg_off by FE1 level (structure.rs:1103-1120) */
     uint32_t* starts = malloc((n + 1) * sizeof(uint32_t));
@@ -270,13 +277,15 @@ int or_build(const uint32_t* const* codes, const uint64_t* g, uint64_t nt, uint6
     s->lperm = calloc(nb + 1, sizeof(uint32_t*));
     s->loff = calloc(nb + 1, sizeof(uint32_t*));
     for (uint64_t b = 0; b < nb; b++) {
+        /* cells by (level, cell): a stable counting sort by the level */
         uint32_t* perm = malloc((M ? M : 1) * sizeof(uint32_t));
-        for (uint64_t c = 0; c < M; c++) perm[c] = (uint32_t)c;
-        g_lvl_codes = s->cell_codes; g_lvl_nb = nb; g_lvl_b = b;
-        qsort(perm, M, sizeof(uint32_t), cmp_cells);
         uint32_t* lo = calloc(s->bg[b] + 1, sizeof(uint32_t));
         for (uint64_t c = 0; c < M; c++) lo[s->cell_codes[c * nb + b] + 1]++;
         for (uint64_t k = 0; k < s->bg[b]; k++) lo[k + 1] += lo[k];
+        uint32_t* cur = malloc((s->bg[b] + 1) * sizeof(uint32_t));
+        memcpy(cur, lo, (s->bg[b] + 1) * sizeof(uint32_t));
+        for (uint64_t c = 0; c < M; c++) perm[cur[s->cell_codes[c * nb + b]]++] = (uint32_t)c;
+        free(cur);
         s->lperm[b] = perm;
         s->loff[b] = lo;
     }
@@ -491,88 +500,127 @@ static void precond_apply(const or_solver* s, const double* rin, uint64_t r, dou
     if (s->ms > 0) or_small_apply(s, rin, r, z);
 }
 
+/* out[e] = sum over threads t = 0..nt-1 (in order) of part[t * len + e] */
+static void sum_threads(const double* part, int nt, uint64_t len, double* out) {
+#pragma omp parallel for schedule(static)
+    for (uint64_t e = 0; e < len; e++) {
+        double v = 0.0;
+        for (int t = 0; t < nt; t++) v += part[(size_t)t * len + e];
+        out[e] = v;
+    }
+}
+
+int or_threads(void) { return omp_get_max_threads(); }
+
 /* ---------------------------------------------------------------- operator */
 
 /* S*P (operator.rs:894-948): u_c = sum_b p[off_b + code_b(c)], t_s = sum_c W_c u_c / n1_s,
  * y[off_b + code_b(c)] += W_c (u_c - t_s). */
 void or_apply(const or_solver* s, const double* p, uint64_t r, double* y) {
-    uint64_t nb = s->nb;
-    for (uint64_t i = 0; i < s->m * r; i++) y[i] = 0.0;
-    double* u = malloc((r ? r : 1) * sizeof(double));
-    double* t = malloc((r ? r : 1) * sizeof(double));
-    for (uint64_t sg = 0; sg < s->g1; sg++) {
-        uint32_t c0 = s->seg_off[sg], c1 = s->seg_off[sg + 1];
-        for (uint64_t j = 0; j < r; j++) t[j] = 0.0;
-        for (uint32_t c = c0; c < c1; c++)
-            for (uint64_t j = 0; j < r; j++) {
-                double v = 0.0;
-                for (uint64_t b = 0; b < nb; b++) v += p[(s->boff[b] + s->cell_codes[(uint64_t)c * nb + b]) * r + j];
-                t[j] += s->cw[c] * v;
-            }
-        for (uint64_t j = 0; j < r; j++) t[j] /= s->n1[sg];
-        for (uint32_t c = c0; c < c1; c++) {
-            for (uint64_t j = 0; j < r; j++) {
-                double v = 0.0;
-                for (uint64_t b = 0; b < nb; b++) v += p[(s->boff[b] + s->cell_codes[(uint64_t)c * nb + b]) * r + j];
-                u[j] = s->cw[c] * (v - t[j]);
-            }
-            for (uint64_t b = 0; b < nb; b++) {
-                uint64_t k = s->boff[b] + s->cell_codes[(uint64_t)c * nb + b];
-                for (uint64_t j = 0; j < r; j++) y[k * r + j] += u[j];
+    uint64_t nb = s->nb, my = s->m * r;
+    int nt = omp_get_max_threads();
+    /* per-thread partial y over a static split of the segments, summed in thread order */
+    double* ys = calloc((size_t)nt * my + 1, sizeof(double));
+#pragma omp parallel num_threads(nt)
+    {
+        double* yt = ys + (size_t)omp_get_thread_num() * my;
+        double* u = malloc((r ? r : 1) * sizeof(double));
+        double* t = malloc((r ? r : 1) * sizeof(double));
+#pragma omp for schedule(static)
+        for (uint64_t sg = 0; sg < s->g1; sg++) {
+            uint32_t c0 = s->seg_off[sg], c1 = s->seg_off[sg + 1];
+            for (uint64_t j = 0; j < r; j++) t[j] = 0.0;
+            for (uint32_t c = c0; c < c1; c++)
+                for (uint64_t j = 0; j < r; j++) {
+                    double v = 0.0;
+                    for (uint64_t b = 0; b < nb; b++) v += p[(s->boff[b] + s->cell_codes[(uint64_t)c * nb + b]) * r + j];
+                    t[j] += s->cw[c] * v;
+                }
+            for (uint64_t j = 0; j < r; j++) t[j] /= s->n1[sg];
+            for (uint32_t c = c0; c < c1; c++) {
+                for (uint64_t j = 0; j < r; j++) {
+                    double v = 0.0;
+                    for (uint64_t b = 0; b < nb; b++) v += p[(s->boff[b] + s->cell_codes[(uint64_t)c * nb + b]) * r + j];
+                    u[j] = s->cw[c] * (v - t[j]);
+                }
+                for (uint64_t b = 0; b < nb; b++) {
+                    uint64_t k = s->boff[b] + s->cell_codes[(uint64_t)c * nb + b];
+                    for (uint64_t j = 0; j < r; j++) yt[k * r + j] += u[j];
+                }
             }
         }
+        free(u); free(t);
     }
-    free(u); free(t);
+    sum_threads(ys, nt, my, y);
+    free(ys);
 }
 
 /* Ops::rhs (operator.rs:969-1066): s_c = sum w v, b1_s, z_c = s_c - W_c b1_s / n1_s,
  * rhs[k] = sum z_c, wsq = sum w v^2, wsum = sum w v. */
 void or_rhs(const or_solver* s, const double* rows, uint64_t r, double* rhs, double* b1, double* wsq, double* wsum) {
-    uint64_t nb = s->nb;
-    for (uint64_t i = 0; i < s->m * r; i++) rhs[i] = 0.0;
-    for (uint64_t j = 0; j < r; j++) { wsq[j] = 0.0; wsum[j] = 0.0; }
+    uint64_t nb = s->nb, mr = s->m * r;
     uint64_t maxc = 0;
     for (uint64_t sg = 0; sg < s->g1; sg++) {
         uint64_t k = s->seg_off[sg + 1] - s->seg_off[sg];
         if (k > maxc) maxc = k;
     }
-    double* sc = malloc((maxc * r + 1) * sizeof(double));
-    for (uint64_t sg = 0; sg < s->g1; sg++) {
-        uint32_t c0 = s->seg_off[sg], c1 = s->seg_off[sg + 1];
-        for (uint64_t j = 0; j < r; j++) b1[sg * r + j] = 0.0;
-        for (uint32_t c = c0; c < c1; c++) {
-            double* v = sc + (c - c0) * r;
-            for (uint64_t j = 0; j < r; j++) v[j] = 0.0;
-            for (uint64_t i = cell_row0(s, c); i < cell_row1(s, c); i++) {
-                double w = s->w_rows ? s->w_rows[i] : 1.0;
-                for (uint64_t j = 0; j < r; j++) {
-                    double wv = w * rows[i * r + j];
-                    v[j] += wv;
-                    wsq[j] += wv * rows[i * r + j];
-                    wsum[j] += wv;
+    int nt = omp_get_max_threads();
+    /* per thread: rhs (m r), wsq (r), wsum (r); summed in thread order */
+    uint64_t stride = mr + 2 * r;
+    double* acc = calloc((size_t)nt * stride + 1, sizeof(double));
+#pragma omp parallel num_threads(nt)
+    {
+        double* at = acc + (size_t)omp_get_thread_num() * stride;
+        double *rt = at, *sq = at + mr, *sm = at + mr + r;
+        double* sc = malloc((maxc * r + 1) * sizeof(double));
+#pragma omp for schedule(static)
+        for (uint64_t sg = 0; sg < s->g1; sg++) {
+            uint32_t c0 = s->seg_off[sg], c1 = s->seg_off[sg + 1];
+            for (uint64_t j = 0; j < r; j++) b1[sg * r + j] = 0.0;
+            for (uint32_t c = c0; c < c1; c++) {
+                double* v = sc + (c - c0) * r;
+                for (uint64_t j = 0; j < r; j++) v[j] = 0.0;
+                for (uint64_t i = cell_row0(s, c); i < cell_row1(s, c); i++) {
+                    double w = s->w_rows ? s->w_rows[i] : 1.0;
+                    for (uint64_t j = 0; j < r; j++) {
+                        double wv = w * rows[i * r + j];
+                        v[j] += wv;
+                        sq[j] += wv * rows[i * r + j];
+                        sm[j] += wv;
+                    }
+                }
+                for (uint64_t j = 0; j < r; j++) b1[sg * r + j] += v[j];
+            }
+            for (uint32_t c = c0; c < c1; c++) {
+                double* v = sc + (c - c0) * r;
+                double f = s->cw[c] / s->n1[sg];
+                for (uint64_t j = 0; j < r; j++) v[j] -= f * b1[sg * r + j];
+                for (uint64_t b = 0; b < nb; b++) {
+                    uint64_t k = s->boff[b] + s->cell_codes[(uint64_t)c * nb + b];
+                    for (uint64_t j = 0; j < r; j++) rt[k * r + j] += v[j];
                 }
             }
-            for (uint64_t j = 0; j < r; j++) b1[sg * r + j] += v[j];
         }
-        for (uint32_t c = c0; c < c1; c++) {
-            double* v = sc + (c - c0) * r;
-            double f = s->cw[c] / s->n1[sg];
-            for (uint64_t j = 0; j < r; j++) v[j] -= f * b1[sg * r + j];
-            for (uint64_t b = 0; b < nb; b++) {
-                uint64_t k = s->boff[b] + s->cell_codes[(uint64_t)c * nb + b];
-                for (uint64_t j = 0; j < r; j++) rhs[k * r + j] += v[j];
-            }
-        }
+        free(sc);
     }
-    free(sc);
+    double* tot = malloc((stride + 1) * sizeof(double));
+    sum_threads(acc, nt, stride, tot);
+    memcpy(rhs, tot, mr * sizeof(double));
+    memcpy(wsq, tot + mr, r * sizeof(double));
+    memcpy(wsum, tot + mr + r, r * sizeof(double));
+    free(tot);
+    free(acc);
 }
 
 /* Ops::residualize_rows (operator.rs:1107-1179): alpha_s = (b1_s - sum W_c u_c) / n1_s,
  * v_i -= alpha_s + u_c. */
 void or_residualize_rows(const or_solver* s, double* rows, uint64_t r, const double* x, const double* b1, double* alpha) {
     uint64_t nb = s->nb;
+#pragma omp parallel
+    {
     double* a = malloc((r ? r : 1) * sizeof(double));
     double* u = malloc((r ? r : 1) * sizeof(double));
+#pragma omp for schedule(static)
     for (uint64_t sg = 0; sg < s->g1; sg++) {
         uint32_t c0 = s->seg_off[sg], c1 = s->seg_off[sg + 1];
         for (uint64_t j = 0; j < r; j++) a[j] = b1[sg * r + j];
@@ -596,6 +644,7 @@ void or_residualize_rows(const or_solver* s, double* rows, uint64_t r, const dou
         }
     }
     free(a); free(u);
+    }
 }
 
 /* ---------------------------------------------------------------- PCG */

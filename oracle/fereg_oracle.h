@@ -1,4 +1,4 @@
-/* CPU oracle of the fixed-effect demeaning path (FP64, single-threaded).
+/* CPU oracle of the fixed-effect demeaning path (FP64, OpenMP threads; OMP_NUM_THREADS sets them).
  *
  * TEST INFRASTRUCTURE ONLY. Only tests/, the smoke check and bench.py's cpu_baseline leg may
  * load this library, and only as the checker: the product path never calls it.
@@ -30,6 +30,8 @@ typedef struct {
 int or_build(const uint32_t* const* codes, const uint64_t* g, uint64_t nterms, uint64_t n,
              uint64_t small_limit, const double* w_kept, or_solver** out);
 void or_free(or_solver* s);
+/* Threads the oracle's parallel loops use (omp_get_max_threads). */
+int or_threads(void);
 void or_get_info(const or_solver* s, or_info* out);
 /* Block table: term, levels, offset, small flag for each of the nb blocks. */
 void or_get_blocks(const or_solver* s, int64_t* term, uint64_t* g, uint64_t* off, int32_t* small);

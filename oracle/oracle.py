@@ -39,6 +39,7 @@ def lib():
         L.or_build.argtypes = [ctypes.POINTER(u32p), u64p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                f64p, ctypes.POINTER(ctypes.c_void_p)]
         L.or_free.argtypes = [ctypes.c_void_p]
+        L.or_threads.restype = ctypes.c_int
         L.or_get_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrInfo)]
         L.or_get_blocks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), u64p, u64p,
                                     ctypes.POINTER(ctypes.c_int32)]
@@ -54,6 +55,11 @@ def lib():
                                      f64p, f64p, u64p, ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
+
+
+def threads():
+    """Threads of the oracle's OpenMP loops."""
+    return lib().or_threads()
 
 
 def _p(a, t):

@@ -75,6 +75,11 @@ def rademacher(seed, reps, G):
     return V, False
 
 
+def cluster_sums(cl, G, v):
+    """Per-cluster column sums of the n x k array v (codes cl in 0..G-1)."""
+    return np.stack([np.bincount(cl, weights=v[:, j], minlength=G) for j in range(v.shape[1])], axis=1)
+
+
 def eval_affine_t(delta, n0, n1, d00, d01, d11):
     """ols.rs:139-142 (vectorized)."""
     v = d00 + delta * (2.0 * d01 + delta * d11)
@@ -237,18 +242,14 @@ def wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="
     z = xt @ C
     u0 = u
     u1 = -(xt @ MR)
-    S0 = np.zeros((G, k))
-    S1 = np.zeros((G, k))
-    Bc = np.zeros((G, k))
-    np.add.at(S0, cl, (wv * u0)[:, None] * xt)
-    np.add.at(S1, cl, (wv * u1)[:, None] * xt)
-    np.add.at(Bc, cl, (wv * z)[:, None] * xt)
+    S0 = cluster_sums(cl, G, (wv * u0)[:, None] * xt)
+    S1 = cluster_sums(cl, G, (wv * u1)[:, None] * xt)
+    Bc = cluster_sums(cl, G, (wv * z)[:, None] * xt)
     Phi0 = S0 @ bread.T
     Phi1 = S1 @ bread.T
     A0 = Phi0 @ R
     A1 = Phi1 @ R
-    q_obs = np.zeros(G)
-    np.add.at(q_obs, cl, wv * z * u0)
+    q_obs = np.bincount(cl, weights=wv * z * u0, minlength=G)
     V, full = rademacher(seed, reps, G)
     # per draw: explicit cell scores (no expansion)
     num0 = V @ A0
