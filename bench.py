@@ -4,15 +4,18 @@ Default workload (the bench line): seconds per estimation call of the 3-FE OLS o
 panel (individual, firm, year; I = 1e6, T = 10, F = 1e5, firm drawn uniformly: the "simple"
 graph), inputs already in HBM. A step = one call: structure build (sort, cells, weights) +
 residualize [y, x1] + Gram + beta + iid SE, through the C ABI (fh_absorber_create_dev +
-fh_ols_fit_dev). value = s/call of the chosen form (FP32 PCG + FP64 refinement); the plain FP64
-form is timed in the same run. The roofline object is the reduced-operator SpMV S*p, timed live
-with HIP events on the absorber's stream. cpu_baseline = the single-threaded C oracle on the
-same full-size input (the reference itself is Rust and is not built here).
+fh_ols_fit_dev). value = s/call of the chosen form (FP64 lead + FP32 PCG sweeps refined on FP64
+residuals); the plain FP64 form is timed in the same run. The same line carries the difficult
+graph (firm = row mod F) and the Poisson PML panel (robust SE), both forms each, with beta / SE
+against the oracle. The roofline object is the reduced-operator SpMV S*p, timed live with HIP
+events on the absorber's stream. cpu_baseline = the C oracle (OpenMP, every host core) on the
+same full-size input, 1 warm-up + median of 3 (the reference itself is Rust and is not built
+here).
 
 Other workloads (--workload): "wcr" (K = 10 regressors, CRV1 by firm, WCR11 bootstrap of x1 = 0
-with 9999 Rademacher draws; roofline of the int8 MFMA contraction), "ppml" (Poisson PML, robust
-SE), "serve" (the mixed request stream with Poisson arrivals: p50/p99 latency and solves/s,
-against one request at a time in plain FP64).
+with 9999 Rademacher draws; roofline of the int8 MFMA contraction), "ppml" (Poisson PML alone),
+"serve" (the mixed request stream with Poisson arrivals: p50/p99 latency and solves/s, against
+one request at a time in plain FP64).
 """
 import argparse
 import ctypes
@@ -103,55 +106,137 @@ def panel_dev(np, torch, synth, n, design, K=1, model="ols", seed=2024):
     return dd
 
 
+def host_cpu():
+    """The host's CPU as the CPU baseline reports it: nproc and lscpu's model and topology."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def median_of(fn, reps=3, warmup=1):
+    """The reference's timing method (README.md:31-33): warm-up runs, then the median of `reps`."""
+    import statistics
+    for _ in range(warmup):
+        fn()
+    ts, out = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), out
+
+
+def oracle_ols_call(np, d, K=1, tol=1e-8):
+    """One oracle estimation call: structure, residualize, OLS with iid SE (the CPU baseline's
+    unit of work, and the parity reference)."""
+    from oracle import estimate
+    from oracle.oracle import OracleSolver
+    s = OracleSolver(d["codes_h"], d["g"])
+    pi = s.structure()["pi"]
+    rows = np.ascontiguousarray(np.concatenate([d["y_h"][:, None], d["X_h"]], 1)[pi])
+    s.residualize(rows, rows.shape[1], tol=tol)
+    coef, bread, e, kept = estimate.ols_from_block(rows)
+    n = rows.shape[0]
+    dfk = estimate.df_k("iid", estimate.nparams(K, d["g"]), d["g"], [False] * 3, n_design=K)
+    V = estimate.vcov("iid", bread, rows[:, 1:], e, None, n, dfk)
+    return coef, np.sqrt(np.diag(V))
+
+
+def parity(np, res, coef, se):
+    return {"rel_dbeta": float(np.max(np.abs(res.coef - coef) / np.abs(coef))),
+            "rel_dse": float(np.max(np.abs(res.se - se) / se))}
+
+
+def time_forms(args, F, d, fit):
+    """s/call of the mixed and the plain FP64 form (new absorber per call, inputs in HBM)."""
+    out = {}
+    for name, prec in (("mixed", F.FH_MIXED), ("fp64", F.FH_PLAIN_FP64)):
+        def call(prec=prec):
+            ab = F.Absorber(d["codes"], d["g"], precision=prec)
+            r = fit(ab)
+            ab.close()
+            return r
+        out[name] = timed(args, call)
+    return out
+
+
 def run_ols(args, np, torch, F, synth):
     n = args.n
     I, Fn = n // 10, max(1, n // 100)
-    log("generating panel n", n, "F", Fn, args.design)
-    d = panel_dev(np, torch, synth, n, args.design)
-
-    def call(precision):
-        ab = F.Absorber(d["codes"], d["g"], precision=precision)
-        res = F.feols(d["y"], d["X"], None, None, vcov="iid", opts=F.SolveOpts(tol=1e-8), absorber=ab)
-        ab.close()
-        return res
-
-    results, timings = {}, {}
-    for name, prec in (("mixed", F.FH_MIXED), ("fp64", F.FH_PLAIN_FP64)):
-        timings[name], results[name] = timed(args, lambda: call(prec))
-        log(name, "s/call", timings[name], "beta", results[name].coef, "iters", results[name].iterations)
-    ab = F.Absorber(d["codes"], d["g"])
+    ols = lambda d: (lambda ab: F.feols(d["y"], d["X"], None, None, vcov="iid", opts=F.SolveOpts(tol=1e-8),
+                                        absorber=ab))
+    designs = {}
+    for design in (("simple", "difficult") if args.design == "simple" else (args.design,)):
+        log("generating panel n", n, "F", Fn, design)
+        d = panel_dev(np, torch, synth, n, design)
+        tf = time_forms(args, F, d, ols(d))
+        for name, (t, r) in tf.items():
+            log(design, name, "s/call", t, "beta", r.coef, "iters", r.iterations)
+        par, cpu = {}, None
+        if not args.no_cpu:
+            from oracle import oracle as orc
+            t_cpu, (coef, se) = median_of(lambda: oracle_ols_call(np, d))
+            par = {k: parity(np, r, coef, se) for k, (t, r) in tf.items()}
+            cpu = {"value": t_cpu, "unit": "s/call", "cores": orc.threads(), "kind": "port",
+                   "sample": f"full workload (n={n}, {design} graph): the C oracle's structure + residualize "
+                             f"(tol 1e-8) + OLS iid, OpenMP on {orc.threads()} threads, 1 warm-up + median of 3",
+                   "host": host_cpu()}
+            log(design, "cpu oracle s/call", t_cpu, "parity", par)
+        designs[design] = {"tf": tf, "parity": par, "cpu": cpu, "d": d}
+    main_d = designs[args.design]
+    ab = F.Absorber(main_d["d"]["codes"], main_d["d"]["g"])
     roof = spmv_roofline(F, ab, 2, args.spmv_reps,
                          "traffic_bytes_fp32_r2" if (args.n == 10_000_000 and args.design == "simple") else None)
     info = ab.info
     ab.close()
-    cpu, dbeta = None, {}
-    if not args.no_cpu:
-        from oracle import estimate
-        from oracle.oracle import OracleSolver
-        t0 = time.perf_counter()
-        s = OracleSolver(d["codes_h"], d["g"])
-        pi = s.structure()["pi"]
-        rows = np.ascontiguousarray(np.concatenate([d["y_h"][:, None], d["X_h"]], 1)[pi])
-        s.residualize(rows, 2, tol=1e-8)
-        coef, bread, e, kept = estimate.ols_from_block(rows)
-        V = estimate.vcov("iid", bread, rows[:, 1:], e, None, n, estimate.nparams(1, d["g"]))
-        cpu_s = time.perf_counter() - t0
-        se_o = np.sqrt(np.diag(V))
-        for name in results:
-            dbeta[name] = {"rel_dbeta": float(np.max(np.abs(results[name].coef - coef) / np.abs(coef))),
-                           "rel_dse": float(np.max(np.abs(results[name].se - se_o) / se_o))}
-        cpu = {"value": cpu_s, "unit": "s/call", "cores": 1, "kind": "port",
-               "sample": f"full workload (n={n}): oracle structure + residualize (tol 1e-8) + OLS iid, C, one thread"}
-        log("cpu oracle s/call", cpu_s, "dbeta", dbeta)
+    extra = {}
+    for design, v in designs.items():
+        rm, rf = v["tf"]["mixed"][1], v["tf"]["fp64"][1]
+        extra[f"c2_{design}"] = {
+            "mixed_s_per_call": v["tf"]["mixed"][0], "fp64_s_per_call": v["tf"]["fp64"][0],
+            "pcg_iterations": {k: int(r.iterations) for k, (t, r) in v["tf"].items()},
+            "parity_vs_oracle": v["parity"],
+            "mixed_vs_fp64": {"rel_dbeta": float(np.max(np.abs(rm.coef - rf.coef) / np.abs(rf.coef))),
+                              "rel_dse": float(np.max(np.abs(rm.se - rf.se) / rf.se))}}
+    if args.design == "simple" and not args.no_ppml:
+        extra["c4_ppml"] = ppml_section(args, np, torch, F, synth)
     config = {"workload": f"ols y~x1 | indiv+firm+year, n={n}, I={I}, T=10, F={Fn}, {args.design} graph, iid",
               "n": n, "fe": [I, Fn, 10], "tol": 1e-8}
-    rm, rf = results["mixed"], results["fp64"]
-    line(args, timings["mixed"], timings["mixed"] * 1e3, "fp32+fp64", config, roof, cpu,
-         {"plain_fp64_s_per_call": timings["fp64"],
-          "mixed_vs_fp64": {"rel_dbeta": float(np.max(np.abs(rm.coef - rf.coef) / np.abs(rf.coef))),
-                            "rel_dse": float(np.max(np.abs(rm.se - rf.se) / rf.se))},
-          "pcg_iterations": {k: int(v.iterations) for k, v in results.items()}, "parity_vs_oracle": dbeta,
-          "structure": {"cells": int(info.ncells), "m": int(info.m), "m_small": int(info.m_small)}})
+    tm, tfp = main_d["tf"]["mixed"], main_d["tf"]["fp64"]
+    extra.update({"plain_fp64_s_per_call": tfp[0], "parity_vs_oracle": main_d["parity"],
+                  "structure": {"cells": int(info.ncells), "m": int(info.m), "m_small": int(info.m_small)}})
+    line(args, tm[0], tm[0] * 1e3, "fp32+fp64", config, roof, main_d["cpu"], extra)
+
+
+def ppml_section(args, np, torch, F, synth):
+    """C4: Poisson PML, robust SE, n = args.n, both forms, parity against the IRLS oracle."""
+    n = args.n
+    log("generating poisson panel n", n)
+    d = panel_dev(np, torch, synth, n, "simple", model="poisson", seed=2026)
+    fit = lambda ab: F.fepois(d["y"], d["X"], None, None, vcov="hetero", tol=1e-8, opts=F.SolveOpts(tol=1e-8),
+                              absorber=ab)
+    tf = time_forms(args, F, d, fit)
+    for name, (t, r) in tf.items():
+        log("ppml", name, "s/call", t, "beta", r.coef, "irls", r.iterations, "pcg", r.pcg_iterations)
+    out = {"mixed_s_per_call": tf["mixed"][0], "fp64_s_per_call": tf["fp64"][0],
+           "irls_iterations": {k: int(r.iterations) for k, (t, r) in tf.items()},
+           "pcg_iterations": {k: int(r.pcg_iterations) for k, (t, r) in tf.items()}}
+    if not args.no_cpu:
+        from oracle import poisson
+        t0 = time.perf_counter()
+        ref = poisson.ppml(d["codes_h"], d["g"], d["y_h"], d["X_h"], tol=1e-8, itol=1e-8, vcov="hetero")
+        log("ppml oracle s", time.perf_counter() - t0, "iterations", ref["iterations"])
+        out["parity_vs_oracle"] = {k: parity(np, r, ref["coef"], ref["se"]) for k, (t, r) in tf.items()}
+        out["oracle_irls_iterations"] = int(ref["iterations"])
+    return out
 
 
 def run_wcr(args, np, torch, F, synth):
@@ -191,30 +276,14 @@ def run_wcr(args, np, torch, F, synth):
 
 
 def run_ppml(args, np, torch, F, synth):
-    n = args.n
-    log("generating poisson panel n", n)
-    d = panel_dev(np, torch, synth, n, "simple", model="poisson", seed=2026)
-    out = {}
-
-    def call(precision):
-        ab = F.Absorber(d["codes"], d["g"], precision=precision)
-        res = F.fepois(d["y"], d["X"], None, None, vcov="hetero", tol=1e-8, opts=F.SolveOpts(tol=1e-8), absorber=ab)
-        ab.close()
-        return res
-
-    for name, prec in (("mixed", F.FH_MIXED), ("fp64", F.FH_PLAIN_FP64)):
-        out[name] = timed(args, lambda: call(prec))
-        log(name, "s/call", out[name][0], "beta", out[name][1].coef, "irls", out[name][1].iterations)
+    sec = ppml_section(args, np, torch, F, synth)
+    d = panel_dev(np, torch, synth, args.n, "simple", model="poisson", seed=2026)
     ab = F.Absorber(d["codes"], d["g"])
-    roof = spmv_roofline(F, ab, 2, args.spmv_reps,
-                         "traffic_bytes_fp32_r2" if (args.n == 10_000_000 and args.design == "simple") else None)
+    roof = spmv_roofline(F, ab, 2, args.spmv_reps)
     ab.close()
-    bm, bf = out["mixed"][1], out["fp64"][1]
-    config = {"workload": f"poisson count~x1 | indiv+firm+year, n={n}, robust SE, tol 1e-8", "n": n}
-    line(args, out["mixed"][0], out["mixed"][0] * 1e3, "fp32+fp64", config, roof, None,
-         {"plain_fp64_s_per_call": out["fp64"][0], "irls_iterations": int(bm.iterations),
-          "mixed_vs_fp64": {"rel_dbeta": float(np.max(np.abs(bm.coef - bf.coef) / np.abs(bf.coef))),
-                            "rel_dse": float(np.max(np.abs(bm.se - bf.se) / bf.se))}})
+    config = {"workload": f"poisson count~x1 | indiv+firm+year, n={args.n}, robust SE, tol 1e-8", "n": args.n}
+    line(args, sec["mixed_s_per_call"], sec["mixed_s_per_call"] * 1e3, "fp32+fp64", config, roof, None,
+         {"plain_fp64_s_per_call": sec["fp64_s_per_call"], "c4_ppml": sec})
 
 
 def run_serve(args, np, torch, F, synth):
@@ -267,6 +336,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second (serve)")
     ap.add_argument("--slots", type=int, default=8, help="requests in flight (serve)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ppml", action="store_true", help="leave the Poisson section out of the ols line")
     ap.add_argument("--spmv-reps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -39,7 +39,7 @@ class Term(ctypes.Structure):
 
 class SolverConfig(ctypes.Structure):
     _fields_ = [("small_limit", ctypes.c_uint64), ("precision", ctypes.c_int32), ("bucket_log2", ctypes.c_int32),
-                ("autotune", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("autotune", ctypes.c_int32), ("mixed_lead", ctypes.c_int32)]
 
 
 class SpmvPlan(ctypes.Structure):

@@ -648,7 +648,7 @@ void fh_config_default(fh_solver_config* cfg) {
     cfg->precision = FH_PLAIN_FP64;
     cfg->bucket_log2 = -1;
     cfg->autotune = 0;
-    cfg->reserved = 0;
+    cfg->mixed_lead = -1;
 }
 
 static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept, size_t n, const fh_solver_config* cfg,
@@ -688,6 +688,8 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         a.bucket_log2 = c.bucket_log2;
         require(c.autotune == 0 || c.autotune == 1, "autotune must be 0 or 1");
         a.autotune = c.autotune == 1;
+        require(c.mixed_lead >= -1 && c.mixed_lead <= 8, "mixed_lead must be in -1..8");
+        a.mixed_lead = c.mixed_lead < 0 ? MIXED_LEAD_DEFAULT : (uint32_t)c.mixed_lead;
         std::vector<DBuf<uint32_t>> up(nterms);
         std::vector<const uint32_t*> dc(nterms);
         for (size_t q = 0; q < nterms; q++) {

@@ -30,6 +30,8 @@
 namespace fh {
 
 constexpr int NB_MAX = 8;  // non-FE1 blocks supported by the kernels
+// Most FP64 iterations a mixed-precision solve runs before its FP32 correction sweeps (default).
+constexpr uint32_t MIXED_LEAD_DEFAULT = 8;
 // MI355X: 8 accelerator complex dies, each with its own L2; the hardware hands consecutive
 // workgroups of a launch to the XCDs in turn.
 constexpr int XCD_COUNT = 8;
@@ -150,6 +152,7 @@ struct Absorber {
     const ClusterRuns* cl_runs = nullptr;
 
     bool escalated = false;  // a large block joined the small super-block (escalation)
+    uint32_t mixed_lead = MIXED_LEAD_DEFAULT;  // most FP64 iterations of a mixed solve before its FP32 sweeps
 
     bool trivial() const { return cell_obs_off.n == 0; }
     bool unit() const { return !weighted && trivial(); }
@@ -276,7 +279,6 @@ struct PcgResult {
     bool exact = false;
     int rc = FH_OK;
     uint64_t refinements = 0;
-    bool accepted_fp32 = false;  // mixed form: the last correction passed without an FP64 residual
     bool escalate = false;       // stopped at the escalation probe (x holds the progress so far)
 };
 // Solves S x = rhs for r columns; x holds the start (x_maybe_nonzero false: x is zero, which

@@ -3,8 +3,9 @@
 // Hestenes-Stiefel window of 5, thr = max(tol^2 (ss1 - x'rhs - x'r), 1e-26 ss0), the roundoff
 // stop, beta, freezing of converged columns), so the host only polls a status word every few
 // iterations; every kernel of an iteration returns at once after the solve has finished.
-// The mixed form runs the same PCG in FP32 on the correction system and refines in FP64
-// (residual rhs - S x recomputed with the FP64 operator after every inner solve).
+// The mixed form runs the first iterations in FP64, then the same PCG in FP32 on the correction
+// system, refining in FP64 (residual rhs - S x recomputed with the FP64 operator after every
+// inner solve; a column is accepted only on that FP64 residual).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -22,6 +23,9 @@ constexpr int WINDOW = 5;  // cg.rs:23
 constexpr int DOTG = 256;  // workgroups of the dot kernels (fixed: fixed summation order)
 constexpr int TPB = 256;
 
+// Largest drop of r'M^-1 r one FP32 correction sweep of the mixed form is asked for (rel_one^2).
+constexpr double HANDOVER_GAIN = 1e10;
+
 // Escalation probe (cg.rs:29-37, 254-274): r'z after PROBE_FROM iterations of a solve and the
 // largest rate rho_hat = (r'z_k / r'z_from)^(1 / (k - from)) at k = PROBE_AT over the active
 // columns; rho_hat > PROBE_RATIO asks for a stronger preconditioner (err = 2).
@@ -33,11 +37,12 @@ struct PcgState {
     double ring[RMAX][WINDOW];
     double xb[RMAX], xr[RMAX], rho0[RMAX], absthr[RMAX], rho_from[RMAX];
     int32_t active[RMAX];
-    uint32_t done, err;  // err: 1 = maxiter reached, 2 = escalation requested
+    uint32_t done, err;  // err: 1 = maxiter reached, 2 = escalation requested, 3 = hand-over
     uint64_t iters, k, maxiter;
     int32_t r;
     int32_t can_escalate;
     double tol, rel2;
+    uint64_t handover_at;  // > 0: stop with err = 3 after this iteration while columns remain
 };
 
 enum Slot { S_PQ = 0, S_XB = 1, S_XR = 2, S_RZL = 3, S_RZS = 4, NSLOT = 5 };
@@ -150,50 +155,6 @@ __global__ void k_add_to64(double* __restrict__ x, const TI* __restrict__ d, siz
                            const int32_t* __restrict__ mask) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         if (!mask || mask[i % r]) x[i] += (double)d[i];
-}
-
-// Per-workgroup partials of d'Dd per column (D = diag(S), d an m x r correction) in `slot`.
-template <class T>
-__global__ __launch_bounds__(TPB) void k_dnorm(const T* __restrict__ d, const double* __restrict__ diag, size_t m,
-                                               int r, double* __restrict__ dpart, int slot) {
-    __shared__ double red[TPB / WAVE][RMAX];
-    double acc[RMAX];
-#pragma unroll
-    for (int j = 0; j < RMAX; j++) acc[j] = 0.0;
-    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < m * r; i += (size_t)gridDim.x * TPB) {
-        const int j = (int)(i % r);
-        const double v = (double)d[i];
-        const double x = diag[i / r] * v * v;
-#pragma unroll
-        for (int jj = 0; jj < RMAX; jj++)
-            if (jj == j) acc[jj] += x;
-    }
-#pragma unroll
-    for (int j = 0; j < RMAX; j++)
-        if (j < r) {
-            const double sm = wave_sum(acc[j]);
-            if (lane_id() == 0) red[threadIdx.x / WAVE][j] = sm;
-        }
-    __syncthreads();
-    if (threadIdx.x < (unsigned)RMAX) {
-        double sm = 0.0;
-        if ((int)threadIdx.x < r)
-            for (int ww = 0; ww < TPB / WAVE; ww++) sm += red[ww][threadIdx.x];
-        dpart[((size_t)slot * DOTG + blockIdx.x) * RMAX + threadIdx.x] = sm;
-    }
-}
-
-// out[j] = d'Dd (slot sums), out[RMAX + j] = the solve's final r'z.
-__global__ __launch_bounds__(TPB) void k_accept_stats(const PcgState* __restrict__ S, const double* __restrict__ dpart,
-                                                      int slot, double* __restrict__ out) {
-    __shared__ double sums[NSLOT_MAX][RMAX];
-    const int sl[1] = {slot};
-    slot_sums(dpart, sl, 1, S->r, sums);
-    const int j = threadIdx.x;
-    if (j < S->r) {
-        out[j] = sums[0][j];
-        out[RMAX + j] = S->rho[j];
-    }
 }
 
 // Start of a (re)start: rho = r'z, inactive when rho <= 0 (cg.rs:199-205).
@@ -335,12 +296,17 @@ __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restr
     const int sl[4] = {S_RZL, S_RZS, S_XB, S_XR};
     slot_sums(dpart, sl, 4, S->r, sums);
     const int j = threadIdx.x;
-    __shared__ int any;
-    if (j == 0) any = 0;
+    __shared__ int any, not_ready, soon;
+    if (j == 0) {
+        any = 0;
+        not_ready = 0;
+        soon = 1;
+    }
     __syncthreads();
     const uint64_t k = S->k + 1;
     if (j < S->r) {
         const double rn = sums[0][j] + sums[1][j];
+        const double rho_prev = S->rho[j];
         double beta = 0.0;
         if (S->active[j]) {
             const double xb = sums[2][j], xr = sums[3][j];
@@ -358,7 +324,7 @@ __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restr
             const bool at_roundoff = rn <= S->floor_[j];
             const bool rel = S->rel2 > 0.0 && rn <= S->rel2 * S->rho0[j];
             // a correction solve of the mixed form also stops once its r'M^-1 r meets the outer
-            // test (absthr is that test's threshold; the caller then checks the rounding bound)
+            // test (absthr is that test's threshold; the caller decides on the FP64 residual)
             const bool at_abs = S->absthr[j] > 0.0 && rn <= S->absthr[j];
             if (done_w || at_roundoff || rel || at_abs || !(rn > 0.0)) S->active[j] = 0;
             else beta = rn / S->rho[j];
@@ -366,7 +332,18 @@ __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restr
         S->beta[j] = beta;
         S->rho_new[j] = rn;
         S->rho[j] = rn;
-        if (S->active[j]) atomicOr(&any, 1);
+        if (S->active[j]) {
+            atomicOr(&any, 1);
+            if (S->handover_at) {
+                // hand-over test of a mixed solve's FP64 lead: one FP32 sweep (a drop of r'M^-1 r by
+                // at most HANDOVER_GAIN) reaches the sweep target 0.01 thr; the next FP64 step is
+                // predicted to reach the roundoff floor at the current rate
+                const double target = fmax(0.01 * S->thr[j], S->floor_[j]);
+                if (!(rn <= HANDOVER_GAIN * target)) atomicOr(&not_ready, 1);
+                const double next = rho_prev > 0.0 ? rn * (rn / rho_prev) : INFINITY;
+                if (!(next <= S->floor_[j])) atomicAnd(&soon, 0);
+            }
+        }
         if (k == PROBE_FROM) S->rho_from[j] = rn;
     }
     __shared__ double rh[RMAX];
@@ -383,6 +360,7 @@ __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restr
         if (!any) S->done = 1;
         else if (S->iters >= S->maxiter) { S->err = 1; S->done = 1; }
         else if (k == PROBE_AT && S->can_escalate && rho_hat > PROBE_RATIO) { S->err = 2; S->done = 1; }
+        else if (S->handover_at && (k >= S->handover_at || (!not_ready && !soon))) { S->err = 3; S->done = 1; }
     }
 }
 
@@ -481,13 +459,15 @@ struct PcgRun {
     bool exact = false;
     bool maxiter_hit = false;
     bool escalate = false;  // stopped at the probe: the caller strengthens the preconditioner
+    bool handover = false;  // stopped at handover_at with columns still active (active[] below)
+    int32_t active[RMAX] = {};
 };
 
 // One PCG solve in precision T from x (x_nonzero: start from x, else from 0).
 template <class T>
 PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const double* ss1, const double* ss0,
                double tol, uint64_t maxiter, double rel2, const double* absthr, const fh_ctx* ctx,
-               bool can_escalate = false) {
+               bool can_escalate = false, uint64_t handover_at = 0, bool r_ready = false) {
     hipStream_t st = a.st;
     Scratch<T>& s = scr<T>(a);
     const size_t m = a.m, ms = a.ms, nv = m * r;
@@ -502,6 +482,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     h.rel2 = rel2;
     h.maxiter = maxiter;
     h.can_escalate = can_escalate ? 1 : 0;
+    h.handover_at = handover_at;
     for (int j = 0; j < r; j++) {
         h.ss1[j] = ss1[j];
         h.ss0[j] = ss0[j];
@@ -511,8 +492,9 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     }
     upload(S, &h, sizeof(PcgState), st);
     hipLaunchKernelGGL(k_zero_slots, dim3(8), dim3(TPB), 0, st, s.dpart.p, 0, (int)NSLOT);
-    // r = rhs - S x
-    if (x_nonzero) {
+    // r = rhs - S x (already in s.r when r_ready)
+    if (r_ready) {
+    } else if (x_nonzero) {
         spmv<T>(a, x, s.q.p, r, s, nullptr, st);
         hipLaunchKernelGGL(k_sub<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, rhs, s.q.p, s.r.p, nv);
     } else {
@@ -593,6 +575,8 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     out.iterations = pl.iters;
     out.maxiter_hit = pl.err == 1;
     out.escalate = pl.err == 2;
+    out.handover = pl.err == 3;
+    if (out.handover) download(out.active, S->active, sizeof(out.active), st);
     return out;
 }
 
@@ -683,40 +667,85 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
         for (double v : dots64(a, x, x, r)) nz |= v != 0.0;
     std::vector<int32_t> act(r);
     for (int j = 0; j < r; j++) act[j] = ss0[j] > 0.0;
-    DBuf<int32_t> mask(r);
     uint64_t total_iters = 0;
+    // The FP64 outer residual of the start and its preconditioned norm decide whether an FP64 lead
+    // runs. A lead keeps the reference's FP64 iteration, with its stopping rule, while the next
+    // step is predicted to reach the roundoff floor (the difficult panel: the preconditioner nearly
+    // solves it, two FP64 steps against two FP32 sweeps), and hands over to the FP32 sweeps once
+    // one sweep can reach the target, at the latest after mixed_lead iterations. A warm start that
+    // is already that close (IRLS) goes straight to the sweeps.
+    auto outer_residual = [&] {
+        if (nz) spmv<double>(a, x, s64.q.p, r, s64, nullptr, st);
+        hipLaunchKernelGGL(k_sub<double>, dim3(vgrid(nv)), dim3(TPB), 0, st, rhs, nz ? s64.q.p : nullptr, s64.r.p, nv);
+        FH_CHECK_LAUNCH();
+        precond<double>(a, s64.r.p, s64.z.p, r, s64.dpart.p, nullptr, st);
+        return dots64(a, {{s64.r.p, s64.z.p}, {x, rhs}, {x, s64.r.p}}, r);
+    };
+    auto thr_of = [&](int j, const std::vector<std::vector<double>>& d) {
+        return std::max(tol * tol * (ss1[j] - (d[1][j] + d[2][j])), 1e-26 * ss0[j]);
+    };
+    bool have_r = false;
+    std::vector<std::vector<double>> dd;
+    if (a.mixed_lead > 0) {
+        // from x = 0 the residual is the right-hand side, whose r'M^-1 r is of the order of the
+        // column's sum of squares: never within one sweep of the target at a tolerance the
+        // reference would use, so a cold start goes to the lead without the readback
+        bool ready = false;
+        if (nz) {
+            dd = outer_residual();
+            ready = true;
+            for (int j = 0; j < r; j++)
+                if (act[j] && dd[0][j] > 0.0) ready &= dd[0][j] <= HANDOVER_GAIN * 0.01 * thr_of(j, dd);
+        }
+        if (ready) {
+            have_r = true;
+        } else {
+            PcgRun lead = run_pcg<double>(a, rhs, x, nz, r, ss1, ss0, tol, maxiter, 0.0, nullptr, ctx, false,
+                                          a.mixed_lead, nz);
+            total_iters += lead.iterations;
+            if (!lead.handover) {
+                res.iterations = std::max<uint64_t>(total_iters, 1);
+                res.exact = lead.exact;
+                res.rc = lead.maxiter_hit ? FH_CONVERGENCE : FH_OK;
+                return res;
+            }
+            for (int j = 0; j < r; j++) act[j] = act[j] && lead.active[j];
+            nz = true;
+            dd.clear();  // s64.r holds the lead's FP64 residual (the PCG's own recursion, as in cg.rs)
+            have_r = true;
+        }
+    }
+    DBuf<int32_t> mask(r);
     bool finished = false;
     // Inner FP32 solve: stops at its column thresholds (the window estimate, or r'M^-1 r below
     // the outer test's threshold) or when r'M^-1 r has dropped by rel^2. When every column's
     // threshold lies within 1e-10 of its r'M^-1 r, one sweep can finish (rel 1e-5); beyond that
     // two sweeps are needed anyway, and a first stop at 1e-8 costs fewer iterations in total than
-    // one at 1e-10 (measured: 8 vs 9 on a 1e7-row panel).
+    // one at 1e-10.
     const double rel_one = 1e-5, rel_split = 1e-4;
     for (int sweep = 0; sweep < 40; sweep++) {
-        // FP64 outer residual r = rhs - S x and its preconditioned norm
-        if (nz) {
-            spmv<double>(a, x, s64.q.p, r, s64, nullptr, st);
-            hipLaunchKernelGGL(k_sub<double>, dim3(vgrid(nv)), dim3(TPB), 0, st, rhs, s64.q.p, s64.r.p, nv);
+        // FP64 outer residual r = rhs - S x (from the FP64 operator; the lead's residual at the
+        // first sweep) and its preconditioned norm
+        std::vector<std::vector<double>> d;
+        if (!have_r) {
+            d = outer_residual();
+        } else if (!dd.empty()) {
+            d = std::move(dd);  // the start's residual, already reduced
+            dd.clear();
         } else {
-            hipLaunchKernelGGL(k_sub<double>, dim3(vgrid(nv)), dim3(TPB), 0, st, rhs, (const double*)nullptr, s64.r.p, nv);
+            precond<double>(a, s64.r.p, s64.z.p, r, s64.dpart.p, nullptr, st);
+            d = dots64(a, {{s64.r.p, s64.z.p}, {x, rhs}, {x, s64.r.p}}, r);
         }
-        FH_CHECK_LAUNCH();
-        precond<double>(a, s64.r.p, s64.z.p, r, s64.dpart.p, nullptr, st);
-        std::vector<double> rz, xb(r, 0.0), xr(r, 0.0);
-        if (nz) {
-            auto d = dots64(a, {{s64.r.p, s64.z.p}, {x, rhs}, {x, s64.r.p}}, r);
-            rz = d[0], xb = d[1], xr = d[2];
-        } else {
-            rz = dots64(a, s64.r.p, s64.z.p, r);  // x = 0: x'b = x'r = 0
-        }
+        have_r = false;
+        const std::vector<double>&rz = d[0], &xb = d[1], &xr = d[2];
         std::vector<double> ss1c(r), ss0c(r), absthr(r);
         bool any = false;
         for (int j = 0; j < r; j++) {
             if (!act[j]) { ss0c[j] = 0.0; ss1c[j] = 0.0; continue; }
             const double ssk = ss1[j] - (xb[j] + xr[j]);
             const double thr = std::max(tol * tol * ssk, 1e-26 * ss0[j]);
-            // the error ||x* - x||_S^2 is estimated by r'M^-1 r; a margin of 100 covers the
-            // preconditioner's misestimate at the last sweep
+            // the error ||x* - x||_S^2 is estimated by r'M^-1 r on the FP64 residual; a margin
+            // of 100 covers the preconditioner's misestimate
             if (sweep > 0 && rz[j] <= 0.01 * thr) { act[j] = 0; continue; }
             if (!(rz[j] > 0.0)) { act[j] = 0; continue; }
             any = true;
@@ -754,37 +783,6 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
             res.rc = FH_CONVERGENCE;
             finished = true;
             break;
-        }
-        // Acceptance without the FP64 residual. The correction solve's recursive residual rho
-        // (FP32) departs from the true residual r - S d by rounding that grows with the
-        // iterations and the sizes of r and S d: ||S d||_{D^-1} <= lambda ||d||_D with lambda <=
-        // the number of absorbed terms under the diagonal scaling D (doubled for the block-diagonal
-        // small part of M). When sqrt(rho) plus 16 eps32 (its + 1)(||r|| + 2 lambda ||d||_D) stays
-        // below the outer threshold's square root for every active column, the next FP64 residual
-        // would pass the outer test (measured departures are ~1e-3 of that bound), so the FP64 S*p
-        // and its dots are skipped. Otherwise the loop checks in FP64 as before.
-        {
-            hipLaunchKernelGGL(k_dnorm<float>, dim3(DOTG), dim3(TPB), 0, st, s32.x.p, a.diag.p, m, r, s32.dpart.p,
-                               (int)S_XB);
-            hipLaunchKernelGGL(k_accept_stats, dim3(1), dim3(TPB), 0, st,
-                               reinterpret_cast<const PcgState*>(a.status.p), s32.dpart.p, (int)S_XB, s64.dpart.p);
-            FH_CHECK_LAUNCH();
-            double hst[2 * RMAX];
-            download(hst, s64.dpart.p, sizeof(hst), st);
-            const double eps32 = 5.960464477539063e-08, lam = 2.0 * (double)std::max<uint64_t>(a.nterms, 1);
-            bool accept = true;
-            for (int j = 0; j < r && accept; j++) {
-                if (!act[j]) continue;
-                const double dmd = std::max(hst[j], 0.0), rho = std::max(hst[RMAX + j], 0.0);
-                const double drift =
-                    16.0 * eps32 * (double)(run.iterations + 1) * (std::sqrt(rz[j]) + lam * std::sqrt(dmd));
-                accept = std::isfinite(drift) && std::sqrt(rho) + drift <= std::sqrt(absthr[j]);
-            }
-            if (accept) {
-                res.accepted_fp32 = true;
-                finished = true;
-                break;
-            }
         }
     }
     if (!finished) {

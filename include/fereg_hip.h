@@ -60,7 +60,9 @@ typedef struct {
                              candidate launch plans and keeps the fastest, cached per structure
                              shape for the process (results may then differ in the last bits
                              between processes); 0: fixed default plans (bit-reproducible) */
-    int32_t reserved;
+    int32_t mixed_lead;   /* FH_MIXED: most FP64 PCG iterations before the FP32 correction
+                             sweeps (0..8; -1 = the default, 8); the lead hands over earlier
+                             once one FP32 sweep can reach the tolerance */
 } fh_solver_config;
 
 /* SolveOpts (absorber.rs:18-24) plus the precision of this solve. */
