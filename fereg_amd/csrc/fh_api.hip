@@ -388,25 +388,6 @@ ResidOut residualize_any(Absorber& a, double* z, size_t r, double tol, uint64_t 
     return tot;
 }
 
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        FH_HIP(hipGetDevice(&prev));
-        if (prev != dev) FH_HIP(hipSetDevice(dev));
-    }
-    ~DeviceGuard() {
-        int cur = -1;
-        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
-// The absorber's device is current and pool buffers are ordered on its stream.
-struct AbsScope {
-    DeviceGuard dg;
-    StreamScope ss;
-    explicit AbsScope(const Absorber& a) : dg(a.device), ss(a.st) {}
-};
-
 // ------------------------------------------------------------------ estimation helpers (host, K x K)
 
 double fe_params(const Absorber& a) {
@@ -634,10 +615,6 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
 }  // namespace fh
 
 using namespace fh;
-
-struct fh_absorber {
-    Absorber a;
-};
 
 extern "C" {
 

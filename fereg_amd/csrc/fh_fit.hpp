@@ -2,6 +2,7 @@
 // variance assembly, the small dense algebra of the K x K systems, and the transfer helpers.
 #pragma once
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,27 @@ template <class T>
 inline void d2h(T* dst, const T* src, size_t count, hipStream_t st) {
     if (count) download(dst, src, count * sizeof(T), st);
 }
+// Runs f and maps an exception to the reference's error code (fh_last_error holds the message).
+int guard(const std::function<void()>& f);
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        FH_HIP(hipGetDevice(&prev));
+        if (prev != dev) FH_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// The absorber's device is current and pool buffers are ordered on its stream.
+struct AbsScope {
+    DeviceGuard dg;
+    StreamScope ss;
+    explicit AbsScope(const Absorber& a) : dg(a.device), ss(a.st) {}
+};
+
 // Sum of `count` device partials, read back (waits for the stream).
 double sum_parts(const double* dev, size_t count, hipStream_t st);
 // Throws FH_INTERRUPTED when the caller's interrupt callback asks for it.
@@ -102,3 +124,8 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, const dou
               const fh_ctx* ctx);
 
 }  // namespace fh
+
+// The handle behind the C ABI's opaque fh_absorber.
+struct fh_absorber {
+    fh::Absorber a;
+};

@@ -217,6 +217,15 @@ constexpr int SPMV_DOTG = 256;
 template <class T>
 void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st,
           double* pq = nullptr);
+// Scratch elements of one S*p (t, ds, dl, part, ypart of Scratch<T>) under a plan, and an S*p with
+// the plan in use that never tunes (for callers that must not allocate).
+struct SpmvNeed {
+    size_t t = 0, ds = 0, dl = 0, part = 0, ypart = 0;
+};
+template <class T>
+SpmvNeed spmv_scratch_need(const Absorber& a, const SpmvPlan& plan, int r);
+template <class T>
+void spmv_fixed(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, hipStream_t st);
 // Level pass of block b under a plan: XCD buckets ((bucket, level) items, partial rows reduced in
 // bucket order) while a bucket's share of t fits comfortably in an XCD's L2, else one item per
 // level over its buckets (merge); and the lanes per item.

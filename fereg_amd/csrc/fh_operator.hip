@@ -20,6 +20,8 @@ void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint3
 }
 template void spmv<double>(const Absorber&, const double*, double*, int, Scratch<double>&, const uint32_t*, hipStream_t,
                            double*);
+template SpmvNeed spmv_scratch_need<double>(const Absorber&, const SpmvPlan&, int);
+template void spmv_fixed<double>(const Absorber&, const double*, double*, int, Scratch<double>&, hipStream_t);
 
 namespace {
 std::mutex tune_mu;

@@ -10,5 +10,7 @@ void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint3
 }
 template void spmv<float>(const Absorber&, const float*, float*, int, Scratch<float>&, const uint32_t*, hipStream_t,
                           double*);
+template SpmvNeed spmv_scratch_need<float>(const Absorber&, const SpmvPlan&, int);
+template void spmv_fixed<float>(const Absorber&, const float*, float*, int, Scratch<float>&, hipStream_t);
 
 }  // namespace fh

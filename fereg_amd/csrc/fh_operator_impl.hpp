@@ -1333,6 +1333,29 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
     }
 }
 
+// Element counts of the scratch arrays spmv_impl reserves for this plan and column count (the
+// same rules as its reserve calls).
+template <class T>
+SpmvNeed spmv_scratch_need(const Absorber& a, const SpmvPlan& plan, int r) {
+    SpmvNeed n;
+    const int nb = (int)a.nb;
+    if (nb == 0) return n;
+    const size_t ms = a.ms, mL = a.m - a.ms;
+    n.t = std::max<size_t>(a.g1 * r, 1);
+    if (a.small_dense && plan.dense && nb <= 4 && r <= 4) {
+        n.ds = a.g1 * r;
+        n.dl = std::max<size_t>(mL * r, 1);
+    }
+    if (a.small_dense) n.part = (size_t)grid_for(a.sm_ld / 4, WG, 2048) * ms * r;
+    for (int b = 0; b < nb; b++) {
+        const BlockInfo& bi = a.blocks[b];
+        if (bi.small && a.small_dense) continue;
+        const LvlMode md = lvl_mode(a, plan, b, r, (int)sizeof(T));
+        if (a.nbk[b] > 1 && !md.merge) n.ypart = std::max(n.ypart, (size_t)a.nbk[b] * bi.g * r);
+    }
+    return n;
+}
+
 // Times the candidate plans of S*p for this absorber, precision and column class on the caller's
 // (p, y) (y is overwritten, p only read) and keeps the fastest: coordinate descent over the
 // route, then the segment lanes, then each large block's level lanes, halving and doubling the
@@ -1407,6 +1430,12 @@ void autotune_spmv(const Absorber& a, SpmvPlan& plan, const T* p, T* y, int r, S
     best.tuned = true;
     plan = best;
     tune_cache_put(key, plan);
+}
+
+// S*p with the plan in use (the default one when none has been tuned), never tuning.
+template <class T>
+void spmv_fixed(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, hipStream_t st) {
+    spmv_impl<T>(a, a.plan_for(sizeof(T) == 8, r), p, y, r, s, nullptr, st);
 }
 
 template <class T>

@@ -147,6 +147,18 @@ int fh_absorber_copy_weight_state(fh_absorber* ab, double* n1, double* nr, doubl
 /* Reduced operator y = S p (operator.rs:1375), host buffers m x r: the SpMV parity check. */
 int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int32_t precision);
 
+/* y = S p as fh_absorber_apply, on DEVICE buffers p and y (m x r on the absorber's device), with
+ * no allocation of any kind: no counterpart in the reference; for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Every
+ * scratch array comes from one static device array of fh_noalloc_scratch_bytes() (64 MiB) per
+ * GPU, held from the library's load to the process's end; a product that needs more is refused
+ * with FH_INVALID (n = 1e7 with 1e6 / 1e5 / 10 levels fits in FP64 up to 2 columns, in FP32 up to
+ * 4). Calls on one GPU share that array: calls that overlap on one GPU race. The absorber must
+ * hold the caller's weights (not a Poisson fit's IRLS weights, whose restoring allocates).
+ * Same kernels and launch plan as fh_absorber_apply: the same values, bit for bit. */
+int fh_absorber_apply_noalloc(fh_absorber* ab, const double* p, size_t r, double* y, int32_t precision);
+size_t fh_noalloc_scratch_bytes(void);
+
 /* SpMV timing probe (no counterpart in the reference; measurement only): runs `reps` products
  * y = S p on the absorber's stream and returns the mean milliseconds per product (HIP events on
  * that stream) and the compulsory HBM bytes of one product,
