@@ -62,6 +62,26 @@ class Ctx(ctypes.Structure):
                 ("user", vp), ("verbose", ctypes.c_int32)]
 
 
+class CallContext:
+    """An fh_ctx (api.rs:198-229) from Python callables: interrupt() -> bool is polled between
+    kernels and iterations (True stops the call with FH_INTERRUPTED); log(line) receives the
+    library's progress lines at verbosity >= their level. Keeps the ctypes callbacks alive."""
+
+    def __init__(self, interrupt=None, log=None, verbose=0):
+        IFN = Ctx._fields_[0][1]
+        LFN = Ctx._fields_[1][1]
+        self._i = IFN((lambda _u: 1 if interrupt() else 0) if interrupt else 0)
+        self._l = LFN((lambda _u, s: log(s.decode())) if log else 0)
+        self.c = Ctx(self._i, self._l, None, int(verbose))
+
+    def ref(self):
+        return ctypes.byref(self.c)
+
+
+def ctx_ref(ctx):
+    return None if ctx is None else ctx.ref()
+
+
 class StructureInfo(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint64), ("nterms", ctypes.c_uint64), ("g1", ctypes.c_uint64),
                 ("ncells", ctypes.c_uint64), ("nb", ctypes.c_uint64), ("m", ctypes.c_uint64),

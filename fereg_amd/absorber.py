@@ -237,8 +237,9 @@ class Absorber:
     def coef_rows(self):
         return L.lib().fh_absorber_coef_rows(self._h)
 
-    def residualize(self, z, r, opts=None, warm=None):
-        """Residualizes the n x r row-major block z in place (solve.rs:1141-1311)."""
+    def residualize(self, z, r, opts=None, warm=None, ctx=None):
+        """Residualizes the n x r row-major block z in place (solve.rs:1141-1311); ctx: an
+        optional _lib.CallContext (interrupt and log callbacks)."""
         o = opts or SolveOpts()
         lib = L.lib()
         p = self.coef_rows()
@@ -259,7 +260,8 @@ class Absorber:
             _, wm, cf = _dev_ready(z, wm, cf)
             L.check(lib.fh_absorber_residualize_dev(self._h, z.data_ptr(), r, ctypes.byref(o.c()),
                                                     None if wm is None else wm.data_ptr(),
-                                                    None if cf is None else cf.data_ptr(), ctypes.byref(info), None))
+                                                    None if cf is None else cf.data_ptr(), ctypes.byref(info),
+                                                    L.ctx_ref(ctx)))
             if cf is not None:
                 coefs = FeCoefs(p, r, cf)
         else:
@@ -275,7 +277,8 @@ class Absorber:
                 wm = np.ascontiguousarray(warm.data, dtype=np.float64)
             L.check(lib.fh_absorber_residualize(self._h, _ptr(z, L.f64p), r, ctypes.byref(o.c()),
                                                 None if wm is None else _ptr(wm, L.f64p),
-                                                None if cf is None else _ptr(cf, L.f64p), ctypes.byref(info), None))
+                                                None if cf is None else _ptr(cf, L.f64p), ctypes.byref(info),
+                                                L.ctx_ref(ctx)))
             if cf is not None:
                 coefs = FeCoefs(p, r, cf)
         return SolveInfo(info.iterations, bool(info.escalated), coefs, bool(info.exact), info.refinements,

@@ -183,7 +183,7 @@ __global__ void k_wsq_b1(const double* __restrict__ b1, const double* __restrict
     for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < g1; s += (size_t)gridDim.x * blockDim.x)
         for (int j = 0; j < r; j++) {
             const double v = b1[s * r + j];
-            acc[j] += v * v / n1[s];
+            if (n1[s] > 0.0) acc[j] += v * v / n1[s];  // empty FE1 levels: b1 = 0, no term
         }
     for (int j = 0; j < 16; j++)
         if (j < r) {
@@ -195,6 +195,22 @@ __global__ void k_wsq_b1(const double* __restrict__ b1, const double* __restrict
         double v = 0.0;
         for (int q = 0; q < TPB / WAVE; q++) v += red[q][threadIdx.x];
         part[(size_t)blockIdx.x * r + threadIdx.x] = v;
+    }
+}
+
+// fit_rows (solve.rs:1313-1317): out[i] = alpha1[s(c)] + sum_b x[off_b + code_b(c)] for every row i
+// of cell c, column col of the (m + g1) x r coefficients; one thread per cell.
+struct BlockOffs {
+    uint32_t off[NB_MAX];
+};
+__global__ void k_fit_rows(const double* __restrict__ coefs, int r, int col, const uint32_t* __restrict__ cell_codes,
+                           const uint32_t* __restrict__ cell_seg, const uint32_t* __restrict__ cell_obs_off,
+                           size_t M, int nb, BlockOffs bo, size_t m, double* __restrict__ out) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
+        double v = coefs[(m + cell_seg[c]) * r + col];
+        for (int b = 0; b < nb; b++) v += coefs[((size_t)bo.off[b] + cell_codes[c * nb + b]) * r + col];
+        const size_t r0 = cell_obs_off ? cell_obs_off[c] : c, r1 = cell_obs_off ? cell_obs_off[c + 1] : c + 1;
+        for (size_t i = r0; i < r1; i++) out[i] = v;
     }
 }
 
@@ -1020,22 +1036,18 @@ int fh_absorber_fit_rows(fh_absorber* ab, const double* coefs, size_t r, size_t 
         require(ab && coefs && out && col < r, "bad argument");
         Absorber& a = ab->a;
         AbsScope g(a);
-        std::vector<uint32_t> seg(a.g1 + 1), cc(a.M * a.nb), coo;
-        d2h(seg.data(), a.seg_off.p, a.g1 + 1, a.st);
-        d2h(cc.data(), a.cell_codes.p, a.M * a.nb, a.st);
-        if (!a.trivial()) {
-            coo.resize(a.M + 1);
-            d2h(coo.data(), a.cell_obs_off.p, a.M + 1, a.st);
-        }
+        require(a.nb <= (size_t)NB_MAX, "fit_rows: too many absorbed terms");
+        const size_t p = a.m + a.g1;
+        DBuf<double> dc(p * r), dout(a.n);
+        h2d(dc.p, coefs, p * r, a.st);
+        BlockOffs bo{};
+        for (size_t b = 0; b < a.nb; b++) bo.off[b] = (uint32_t)a.blocks[b].off;
+        hipLaunchKernelGGL(k_fit_rows, dim3(grid_for(a.M, TPB, 8192)), dim3(TPB), 0, a.st, dc.p, (int)r, (int)col,
+                           a.cell_codes.p, a.cell_seg.p, a.trivial() ? nullptr : a.cell_obs_off.p, (size_t)a.M,
+                           (int)a.nb, bo, (size_t)a.m, dout.p);
+        FH_CHECK_LAUNCH();
+        d2h(out, dout.p, a.n, a.st);
         stream_wait(a.st);
-        const size_t m = a.m;
-        for (size_t s = 0; s < a.g1; s++)
-            for (uint32_t c = seg[s]; c < seg[s + 1]; c++) {
-                double v = coefs[(m + s) * r + col];
-                for (size_t b = 0; b < a.nb; b++) v += coefs[(a.blocks[b].off + cc[(size_t)c * a.nb + b]) * r + col];
-                const size_t r0 = coo.empty() ? c : coo[c], r1 = coo.empty() ? c + 1 : coo[c + 1];
-                for (size_t i = r0; i < r1; i++) out[i] = v;
-            }
     });
 }
 

@@ -45,8 +45,11 @@ __global__ void k_n1(const uint32_t* __restrict__ seg_off, const double* __restr
         else
             v = double(c1 - c0);
         n1[s] = v;
-        inv64[s] = 1.0 / v;
-        inv32[s] = (float)(1.0 / v);
+        // an FE1 level without rows (codes need not use every level) contributes nothing: its
+        // segment sums are 0, not 0/0, so the dense small-block products stay finite
+        const double inv = v > 0.0 ? 1.0 / v : 0.0;
+        inv64[s] = inv;
+        inv32[s] = (float)inv;
     }
 }
 

@@ -69,7 +69,8 @@ def _host_codes(a, what):
     return np.ascontiguousarray(v, dtype=np.uint32)
 
 
-def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, extra_args, offset=None):
+def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, extra_args, offset=None,
+         ctx=None):
     lib = L.lib()
     X = np.asarray(X, float) if not _is_torch(X) else X
     if not _is_torch(X) and X.ndim == 1:
@@ -112,7 +113,7 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
         args += [kind, None if cl is None else _ptr(cl, L.u32p), _ptr(nest, L.i32p), ctypes.byref(sscc)]
     args += extra_args + [ctypes.byref(o), _ptr(beta, L.f64p), _ptr(V, L.f64p), _ptr(kept, L.i32p), ctypes.byref(st)]
     if poisson:
-        args.append(None)
+        args.append(L.ctx_ref(ctx))
     L.check(fn(*args))
     res = FitResult(beta, V.reshape(K, K), kept.astype(bool), st.df_k, st.nparams, st.ssr, st.deviance,
                     st.iterations, st.pcg_iterations, st.n_clusters)
@@ -128,11 +129,11 @@ def feols(y, X, codes, g, weights=None, vcov="iid", cluster=None, nested=None, s
 
 
 def fepois(y, X, codes, g, weights=None, offset=None, vcov="hetero", cluster=None, nested=None, ssc=None, tol=1e-8,
-           maxiter=100, opts=None, absorber=None):
+           maxiter=100, opts=None, absorber=None, ctx=None):
     """Poisson PML by IRLS (glm/irls.rs:367-913, fixest flavour) on the GPU. `weights` are prior
     weights (or the absorber's, when one is passed), `offset` enters the linear predictor."""
     return _fit("fh_ppml_fit", y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber,
-                [ctypes.c_double(tol), ctypes.c_uint64(maxiter)], offset=offset)
+                [ctypes.c_double(tol), ctypes.c_uint64(maxiter)], offset=offset, ctx=ctx)
 
 
 @dataclass

@@ -536,7 +536,7 @@ void or_apply(const or_solver* s, const double* p, uint64_t r, double* y) {
                     for (uint64_t b = 0; b < nb; b++) v += p[(s->boff[b] + s->cell_codes[(uint64_t)c * nb + b]) * r + j];
                     t[j] += s->cw[c] * v;
                 }
-            for (uint64_t j = 0; j < r; j++) t[j] /= s->n1[sg];
+            for (uint64_t j = 0; j < r; j++) t[j] = s->n1[sg] > 0.0 ? t[j] / s->n1[sg] : 0.0;
             for (uint32_t c = c0; c < c1; c++) {
                 for (uint64_t j = 0; j < r; j++) {
                     double v = 0.0;
@@ -630,7 +630,7 @@ void or_residualize_rows(const or_solver* s, double* rows, uint64_t r, const dou
                 for (uint64_t b = 0; b < nb; b++) v += x[(s->boff[b] + s->cell_codes[(uint64_t)c * nb + b]) * r + j];
                 a[j] -= s->cw[c] * v;
             }
-        for (uint64_t j = 0; j < r; j++) a[j] /= s->n1[sg];
+        for (uint64_t j = 0; j < r; j++) a[j] = s->n1[sg] > 0.0 ? a[j] / s->n1[sg] : 0.0;
         if (alpha)
             for (uint64_t j = 0; j < r; j++) alpha[sg * r + j] = a[j];
         for (uint32_t c = c0; c < c1; c++) {
@@ -774,7 +774,8 @@ int or_residualize(or_solver* s, double* zrows, uint64_t r, double tol, uint64_t
         double v = wsq[j] - wsum[j] * wsum[j] / s->sw;
         ss0[j] = v > 1e-15 * wsq[j] ? v : 0.0;
         double b1sq = 0.0;
-        for (uint64_t sg = 0; sg < g1; sg++) b1sq += b1[sg * r + j] * b1[sg * r + j] / s->n1[sg];
+        for (uint64_t sg = 0; sg < g1; sg++)
+            if (s->n1[sg] > 0.0) b1sq += b1[sg * r + j] * b1[sg * r + j] / s->n1[sg]; /* empty levels: no term */
         ss1[j] = wsq[j] - b1sq;
     }
     if (warm) memcpy(x, warm, m * r * sizeof(double));

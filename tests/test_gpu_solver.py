@@ -234,3 +234,41 @@ def test_mixed_form_meets_the_fp64_accuracy(design, weighted):
     err = {k: np.max(np.abs(out[k] - out["ref"]) / scale) for k in ("fp64", "mixed")}
     assert err["mixed"] <= max(10 * err["fp64"], 1e-10), err
     assert err["mixed"] <= 1e-7, err
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_empty_levels_with_the_dense_small_block(precision):
+    """Codes need not use every level: FE1 levels without rows (their segments are empty, so
+    n1 = 0), unused large-block levels and an unused small level, with one small block of 12
+    levels (the dense small-row route). S*p and the residuals match the oracle, and the
+    coefficients of the empty FE1 levels are 0."""
+    import fereg_amd as F
+    rng = np.random.default_rng(44)
+    n = 200_000
+    gs = [20000, 5000, 12]
+    codes = [rng.integers(0, 19000, n).astype(np.uint32), rng.integers(0, 4900, n).astype(np.uint32),
+             rng.integers(0, 11, n).astype(np.uint32)]
+    o = OracleSolver(codes, gs)
+    a = F.Absorber(codes, gs, precision=precision)
+    m = o.info.m
+    for r in (1, 2):
+        p = rng.standard_normal(m * r)
+        announce("empty levels apply", precision, r)
+        yo = o.apply(p, r)
+        ya = a.apply(p, r, precision=precision)
+        scale = np.abs(yo).max()
+        assert np.all(np.isfinite(ya))
+        np.testing.assert_allclose(ya, yo, rtol=0, atol=(1e-12 if precision == 0 else 3e-5) * scale)
+    v = rng.standard_normal((n, 2)) + rng.standard_normal(20000)[codes[0]][:, None]
+    pi = a.row_order()
+    ro = np.ascontiguousarray(v[pi])
+    rg = ro.copy()
+    o.residualize(ro, 2, tol=1e-12)
+    announce("empty levels residualize", precision)
+    info = a.residualize(rg, 2, F.SolveOpts(tol=1e-12, want_coefs=True))
+    np.testing.assert_allclose(rg, ro, rtol=0, atol=1e-9 * np.abs(ro).max())
+    alpha = info.coefs.data.reshape(-1, 2)[m:]
+    fe1 = int(a.info.fe1)
+    empty = np.setdiff1d(np.arange(gs[fe1]), codes[fe1])
+    assert len(empty) > 0 and np.all(alpha[empty] == 0.0) and np.all(np.isfinite(alpha))
+    a.close()
