@@ -142,6 +142,31 @@ class Response(ctypes.Structure):
                 ("pcg_iterations", ctypes.c_uint64), ("start_s", ctypes.c_double), ("end_s", ctypes.c_double)]
 
 
+FH_COL = {"f64": 0, "i32": 1, "i64": 2, "codes": 3}
+FH_FAMILY = {"gaussian": 0, "poisson": 1, "logit": 2}
+FH_WEIGHT = {None: 0, "aweight": 1, "fweight": 2, "pweight": 3}
+
+
+class FactorCol(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("data", vp), ("base", ctypes.c_int64),
+                ("nlevels", ctypes.c_uint64)]
+
+
+class SampleSpec(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_size_t), ("y", vp), ("x", ctypes.POINTER(vp)), ("nx", ctypes.c_size_t),
+                ("weights", vp), ("offset", vp), ("drop", vp), ("factors", ctypes.POINTER(FactorCol)),
+                ("nfactors", ctypes.c_size_t), ("term_factors", u32p), ("term_off", u32p), ("nterms", ctypes.c_size_t),
+                ("cluster_factors", u32p), ("cluster_off", u32p), ("nclusters", ctypes.c_size_t),
+                ("weight_kind", ctypes.c_int32), ("family", ctypes.c_int32), ("missing", ctypes.c_int32),
+                ("drop_singletons", ctypes.c_int32), ("separation", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class SampleCounts(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("n_input", "nobs", "na", "zero_weight", "singletons", "perfect_fit",
+                                               "na_y", "na_x", "na_fe", "na_cluster", "na_weights", "na_offset",
+                                               "na_extra")]
+
+
 # (name, restype, argtypes) of every exported entry point of include/fereg_hip.h
 SIGNATURES = [
     ("fh_last_error", ctypes.c_char_p, []),
@@ -191,6 +216,29 @@ SIGNATURES = [
     ("fh_ols_wildboot_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, vp, i32p, ctypes.POINTER(Ssc),
                                            ctypes.POINTER(SolveOptsC), ctypes.POINTER(WbootSpec), f64p, f64p, i32p,
                                            ctypes.POINTER(WbootResult), f64p]),
+    ("fh_factorize", ctypes.c_int, [ctypes.POINTER(FactorCol), ctypes.c_size_t, ctypes.c_int32, u32p, u64p, u64p]),
+    ("fh_factorize_dev", ctypes.c_int, [ctypes.POINTER(FactorCol), ctypes.c_size_t, ctypes.c_int32, vp, u64p, u64p]),
+    ("fh_combine", ctypes.c_int, [ctypes.POINTER(vp), u64p, ctypes.c_size_t, ctypes.c_size_t, u32p, u64p]),
+    ("fh_combine_dev", ctypes.c_int, [ctypes.POINTER(vp), u64p, ctypes.c_size_t, ctypes.c_size_t, vp, u64p]),
+    ("fh_restrict", ctypes.c_int, [u32p, ctypes.c_uint64, ctypes.c_size_t, u32p, ctypes.c_size_t, u32p, u64p]),
+    ("fh_restrict_dev", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_size_t, vp, ctypes.c_size_t, vp, u64p]),
+    ("fh_drop_singletons", ctypes.c_int, [ctypes.POINTER(Term), ctypes.c_size_t, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_uint8), f64p, f64p, ctypes.c_int32, ctypes.c_int32,
+                                          u64p, u64p]),
+    ("fh_drop_singletons_dev", ctypes.c_int, [ctypes.POINTER(Term), ctypes.c_size_t, ctypes.c_size_t, vp, vp, vp,
+                                              ctypes.c_int32, ctypes.c_int32, u64p, u64p]),
+    ("fh_sample_build", ctypes.c_int, [ctypes.POINTER(SampleSpec), ctypes.POINTER(Ctx), ctypes.POINTER(vp)]),
+    ("fh_sample_build_dev", ctypes.c_int, [ctypes.POINTER(SampleSpec), ctypes.POINTER(Ctx), ctypes.POINTER(vp)]),
+    ("fh_sample_destroy", None, [vp]),
+    ("fh_sample_get_counts", ctypes.c_int, [vp, ctypes.POINTER(SampleCounts)]),
+    ("fh_sample_kept_dev", vp, [vp]),
+    ("fh_sample_kept", ctypes.c_int, [vp, u32p]),
+    ("fh_sample_term_dev", vp, [vp, ctypes.c_size_t, u64p]),
+    ("fh_sample_term", ctypes.c_int, [vp, ctypes.c_size_t, u32p, u64p]),
+    ("fh_sample_cluster_dev", vp, [vp, ctypes.c_size_t, u64p]),
+    ("fh_sample_cluster", ctypes.c_int, [vp, ctypes.c_size_t, u32p, u64p]),
+    ("fh_sample_gather", ctypes.c_int, [vp, f64p, f64p]),
+    ("fh_sample_gather_dev", ctypes.c_int, [vp, vp, vp]),
     ("fh_server_config_default", None, [ctypes.POINTER(ServerConfig)]),
     ("fh_server_create", ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
     ("fh_server_submit", ctypes.c_int, [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),

@@ -285,6 +285,117 @@ int fh_ols_wildboot_dev(fh_absorber* ab, const double* y, const double* X, size_
                         const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, const fh_wboot_spec* spec,
                         double* beta, double* vcov, int32_t* kept, fh_wboot_result* res, double* t_boot);
 
+/* ------------------------------------------------------------------ factors and the sample */
+
+/* A factor column (fit::FactorCol, fit/mod.rs; factor::FactorValues::Num, api.rs). Missing
+ * entries: non-finite doubles, INT32_MIN / INT64_MIN, codes outside base .. base + nlevels - 1. */
+#define FH_COL_F64 0
+#define FH_COL_I32 1
+#define FH_COL_I64 2
+#define FH_COL_CODES 3 /* uint32 codes */
+typedef struct {
+    int32_t kind;
+    int32_t reserved;
+    const void* data; /* n values of the kind */
+    int64_t base;     /* FH_COL_CODES */
+    uint64_t nlevels; /* FH_COL_CODES */
+} fh_factor_col;
+
+/* factor::factorize_col (factor.rs:307-345): dense codes over the n rows in ascending value
+ * order (numeric order; -0.0 and 0.0 are one level); missing entries share one extra level, code
+ * 0, present only when some row is missing (*n_missing rows; may be NULL). strict = 1 is
+ * factor::factorize (factor.rs:44-76) of a double column: infinities are levels and a NaN is an
+ * error (FH_INVALID "missing values in factor"). Codes are n uint32, host or device (_dev). */
+int fh_factorize(const fh_factor_col* col, size_t n, int32_t strict, uint32_t* codes, uint64_t* nlevels,
+                 uint64_t* n_missing);
+int fh_factorize_dev(const fh_factor_col* col, size_t n, int32_t strict, uint32_t* codes, uint64_t* nlevels,
+                     uint64_t* n_missing);
+/* factor::combine (factor.rs:259-281): the interaction a#b#..., codes in lexicographic order of
+ * (a, b, ...) over the combinations that occur. FH_INVALID when the product of the level counts
+ * reaches 2^63. parts: nparts arrays of n codes, each below its nlevels. */
+int fh_combine(const uint32_t* const* parts, const uint64_t* nlevels, size_t nparts, size_t n, uint32_t* codes,
+               uint64_t* out_nlevels);
+int fh_combine_dev(const uint32_t* const* parts, const uint64_t* nlevels, size_t nparts, size_t n,
+                   uint32_t* codes, uint64_t* out_nlevels);
+/* factor::restrict (factor.rs:285-305): the codes of the rows kept (ascending row indices),
+ * renumbered densely in their order. */
+int fh_restrict(const uint32_t* codes, uint64_t nlevels, size_t n, const uint32_t* kept, size_t nkept, uint32_t* out,
+                uint64_t* out_nlevels);
+int fh_restrict_dev(const uint32_t* codes, uint64_t nlevels, size_t n, const uint32_t* kept, size_t nkept,
+                    uint32_t* out, uint64_t* out_nlevels);
+
+/* sample::drop_singletons (sample.rs:44-74): cycles over the terms in order, dropping the live
+ * rows (alive[i] = 1) of levels with one live row (singletons = 1) and, with y_sep, of levels whose
+ * outcome sum is 0 (Poisson) or 0 or the count (logit: nonzero outcomes count 1), until no term
+ * changes. fw: frequency weights (counts are weight sums) or NULL. Terms' intercept is ignored. */
+#define FH_FAMILY_GAUSSIAN 0
+#define FH_FAMILY_POISSON 1
+#define FH_FAMILY_LOGIT 2
+int fh_drop_singletons(const fh_term* terms, size_t nterms, size_t n, uint8_t* alive, const double* fw,
+                       const double* y_sep, int32_t family, int32_t singletons, uint64_t* n_singletons,
+                       uint64_t* n_separated);
+int fh_drop_singletons_dev(const fh_term* terms, size_t nterms, size_t n, uint8_t* alive, const double* fw,
+                           const double* y_sep, int32_t family, int32_t singletons, uint64_t* n_singletons,
+                           uint64_t* n_separated);
+
+/* The estimation sample (fit/input.rs:193-407 scan + build; session.rs:458-490 prepare): factor
+ * codes, the missing-value and zero-weight scan, interactions, singleton and fe-separation
+ * removal, then the kept rows and the term and cluster codes over them. */
+#define FH_WEIGHT_NONE 0
+#define FH_WEIGHT_AWEIGHT 1
+#define FH_WEIGHT_FWEIGHT 2
+#define FH_WEIGHT_PWEIGHT 3
+typedef struct {
+    size_t n;
+    const double* y;              /* n, or NULL */
+    const double* const* x;       /* nx columns of n values */
+    size_t nx;
+    const double* weights;        /* n, or NULL */
+    const double* offset;         /* n, or NULL */
+    const uint8_t* drop;          /* n, or NULL: 1 = a row the caller excludes (FitInput::drop) */
+    const fh_factor_col* factors; /* nfactors columns of n values */
+    size_t nfactors;
+    const uint32_t* term_factors; /* factor indices of every absorbed term, concatenated; term q */
+    const uint32_t* term_off;     /*   is the interaction of term_factors[term_off[q] .. term_off[q+1]) */
+    size_t nterms;
+    const uint32_t* cluster_factors; /* the same for cluster dimensions (at most 4) */
+    const uint32_t* cluster_off;
+    size_t nclusters;
+    int32_t weight_kind;     /* FH_WEIGHT_*: frequency weights are summed in the singleton counts */
+    int32_t family;          /* FH_FAMILY_* */
+    int32_t missing;         /* 1: fit layer: rows with a missing value (y, x, offset, weights, a
+                                used factor, drop) or a zero weight are dropped and counted; a
+                                negative weight is an error. 0: session: every row is used and a
+                                missing factor value is an error */
+    int32_t drop_singletons; /* 1: cyclic singleton removal */
+    int32_t separation;      /* 1: fixest's fe separation rule for Poisson and logit */
+    int32_t reserved;
+} fh_sample_spec;
+/* fit::Dropped (fit/mod.rs:473-500). */
+typedef struct {
+    uint64_t n_input, nobs;
+    uint64_t na, zero_weight, singletons, perfect_fit;
+    uint64_t na_y, na_x, na_fe, na_cluster, na_weights, na_offset, na_extra;
+} fh_sample_counts;
+typedef struct fh_sample fh_sample;
+/* Builds on the current device; spec arrays on the host (fh_sample_build) or the device (_dev).
+ * FH_TOO_FEW when fewer than 2 rows remain. */
+int fh_sample_build(const fh_sample_spec* spec, const fh_ctx* ctx, fh_sample** out);
+int fh_sample_build_dev(const fh_sample_spec* spec, const fh_ctx* ctx, fh_sample** out);
+void fh_sample_destroy(fh_sample* s);
+int fh_sample_get_counts(const fh_sample* s, fh_sample_counts* out);
+/* Kept input rows (nobs, ascending); term / cluster codes over them (nobs, dense) with their
+ * level counts. Device pointers stay valid until fh_sample_destroy. */
+const uint32_t* fh_sample_kept_dev(const fh_sample* s);
+int fh_sample_kept(const fh_sample* s, uint32_t* rows);
+const uint32_t* fh_sample_term_dev(const fh_sample* s, size_t q, uint64_t* g);
+int fh_sample_term(const fh_sample* s, size_t q, uint32_t* codes, uint64_t* g);
+const uint32_t* fh_sample_cluster_dev(const fh_sample* s, size_t q, uint64_t* g);
+int fh_sample_cluster(const fh_sample* s, size_t q, uint32_t* codes, uint64_t* g);
+/* out[j] = col[kept[j]]: an input column (n) on the kept rows (nobs). */
+int fh_sample_gather(const fh_sample* s, const double* col, double* out);
+int fh_sample_gather_dev(const fh_sample* s, const double* col, double* out);
+
 /* Serving engine (no reference counterpart: fereg-core is a library called once per fit).
  * Continuous batching of independent regressions: a FIFO of requests, each admitted when one of
  * `slots` workers is free and its HBM footprint fits `hbm_budget` beside the requests already

@@ -26,6 +26,22 @@ def cases(name):
     return json.load(open(os.path.join(GOLDEN, name)))
 
 
+def fixest_ols_cases(multiway=False):
+    """fixest OLS fixtures the absorber path covers: at least one absorbed term; K.exact (fixest's
+    reference-level search, fit/fixef.rs:300-430) only where it cannot change df_k (one iid
+    case); with multiway=False, at most one cluster dimension."""
+    out = []
+    for c in cases("fixest_ols.json"):
+        if not c["fe"]:
+            continue
+        if c["ssc"].get("K.exact") and c["name"] != "fe3_iid_exact":
+            continue
+        if len(c["cluster"]) > 1 and not multiway:
+            continue
+        out.append(c)
+    return out
+
+
 def dense_codes(*cols):
     """Dense level codes in ascending value order; several columns = their interaction
     (lexicographic), as factor.rs:38-76 and :255."""

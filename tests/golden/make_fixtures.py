@@ -7,6 +7,10 @@ Sources (fixtures of crates/fereg-core/tests, written there by fixest 0.14.2):
   tests/data/fit_d1.csv, fit_toy.csv          the input panels
   tests/data/fit_fixest.json                  OLS cases (tests/fit_ols.rs:10-67, RTOL 1e-8)
   tests/data/fit_fixest_glm.json              Poisson cases (tests/fit_glm.rs)
+  tests/data/ols_data.csv, ols_s*.json        reghdfe 6.13.1 ground truth (tests/ols.rs:765-935,
+                                              written by tests/data/gen_ols_fixtures.do); the
+                                              model of each fixture restates the table of
+                                              tests/ols.rs:803-838
 """
 import json
 import os
@@ -15,12 +19,36 @@ import shutil
 SRC = "/root/reference/crates/fereg-core/tests/data"
 DST = os.path.dirname(os.path.abspath(__file__))
 
-OLS_CASES = ["fe2_iid", "fe2_hetero", "fe2_cl_g", "fe2_cl_c1", "fe2_cl_c3", "fe2_iid_ssc_none",
-             "fe2_iid_ssc_full", "fe2_iid_ssc_noadj", "fe2_c1_c3_ssc_nog", "fe3_iid_exact",
-             "fe1_w_iid", "fe1_w_hetero", "fe1_w_cl", "fe_g_c1_cl_c1", "collin",
-             "toy_iid", "toy_hetero", "toy_cl_g", "toy_cl_g_full", "toy_cl_g_none"]
+OLS_CASES = None  # every OLS case
 POIS_CASES = ["pois_fe2_iid", "pois_fe2_hetero", "pois_collin", "pois_inter_fx", "pois_fe2_off_w"]
 KEEP = ["coef", "se", "vcov", "df_k", "nobs", "nparams", "collin", "obs", "G", "resid", "ssr", "coef_names"]
+
+# reghdfe fixtures: (name, y, x, absorb terms, cluster dims, robust, weight (var, kind), singletons,
+# noconstant, dof options (firstpair, pairwise, clusters, continuous), expected status)
+_X3, _X2 = ["x1", "x2", "x3"], ["x1", "x2"]
+_IDYEAR = [["id"], ["year"]]
+_DOF = [False, True, True, True]
+REGHDFE = [
+    ("s01_unadj", "y", _X3, _IDYEAR, [], False, None, True, False, _DOF, []),
+    ("s02_robust_3fe", "y", _X3, [["id"], ["year"], ["firm"]], [], True, None, True, False, _DOF, []),
+    ("s03_cluster_eq_fe", "y", _X2, _IDYEAR, [["id"]], False, None, True, False, _DOF, []),
+    ("s04_fe_nested_in_cluster", "y", _X3, _IDYEAR, [["region"]], False, None, True, False, _DOF, []),
+    ("s05_twoway_psd", "y", _X3, [["firm"], ["year"]], [["c1"], ["c2"]], False, None, True, False, _DOF, []),
+    ("s06_aw_cluster", "y", _X3, _IDYEAR, [["firm"]], False, ["w", "aweight"], True, False, _DOF, []),
+    ("s07_fw_robust", "y", _X3, _IDYEAR, [], True, ["fw", "fweight"], True, False, _DOF, []),
+    ("s08_pw", "y", _X3, [["id"], ["year"], ["firm"]], [], False, ["w", "pweight"], True, False, _DOF, []),
+    ("s09_absorbed_collinear", "y", ["x1", "x2", "xa", "xc"], _IDYEAR, [], False, None, True, False, _DOF, [0, 0, 2, 3]),
+    ("s10_interaction", "y", _X2, [["id"], ["firm", "year"]], [], False, None, True, False, _DOF, []),
+    ("s11_noabsorb_cluster", "y", _X3, [], [["firm"]], False, None, True, False, _DOF, []),
+    ("s12_noconstant", "y", _X2, _IDYEAR, [], False, None, True, True, _DOF, []),
+    ("s13_keepsing_twoway", "y", _X2, _IDYEAR, [["id"], ["year"]], False, None, False, False, _DOF, []),
+    ("s14_groupvar", "y", _X2, [["id"], ["firm"], ["year"]], [], False, None, True, False, _DOF, []),
+    ("s15_firstpair_cluster", "y", _X2, [["id"], ["year"], ["firm"]], [["firm"]], False, None, True, False,
+     [True, False, False, False], []),
+    ("s17_y_absorbed", "y2", _X2, _IDYEAR, [], False, None, True, False, _DOF, []),
+    ("s18_fw_twoway", "y", _X3, [["firm"], ["year"]], [["c1"], ["c2"]], False, ["fw", "fweight"], True, False, _DOF, []),
+]
+REGHDFE_KEEP = ["cmdline", "colnames", "b", "V", "dof_table", "sample", "scalars", "gv14"]
 
 
 def main():
@@ -29,7 +57,7 @@ def main():
     cases = json.load(open(os.path.join(SRC, "fit_fixest.json")))
     out = []
     for c in cases:
-        if c["name"] not in OLS_CASES:
+        if OLS_CASES is not None and c["name"] not in OLS_CASES:
             continue
         e = {k: c["expect"][k] for k in KEEP if k in c["expect"]}
         out.append({k: c[k] for k in ["name", "data", "y", "x", "fe", "weights", "vcov", "cluster", "ssc"]} | {"expect": e})
@@ -42,7 +70,15 @@ def main():
         e = {k: c["expect"][k] for k in KEEP + ["deviance", "iterations", "eta", "irls_weights"] if k in c["expect"]}
         keep.append({k: c[k] for k in ["name", "data", "y", "x", "fe", "weights", "offset", "vcov", "cluster", "ssc"]} | {"expect": e})
     json.dump(keep, open(os.path.join(DST, "fixest_pois.json"), "w"), indent=0)
-    print(len(out), "ols cases;", len(keep), "poisson cases")
+    shutil.copyfile(os.path.join(SRC, "ols_data.csv"), os.path.join(DST, "ols_data.csv"))
+    rg = []
+    for name, y, x, absorb, vce, robust, weight, sing, nocons, dof, status in REGHDFE:
+        j = json.load(open(os.path.join(SRC, f"ols_{name}.json")))
+        rg.append({"name": name, "y": y, "x": x, "absorb": absorb, "cluster": vce, "robust": robust,
+                   "weight": weight, "singletons": sing, "noconstant": nocons, "dof": dof, "status": status,
+                   "expect": {k: j[k] for k in REGHDFE_KEEP if k in j}})
+    json.dump(rg, open(os.path.join(DST, "reghdfe_ols.json"), "w"))
+    print(len(out), "ols cases;", len(keep), "poisson cases;", len(rg), "reghdfe cases")
 
 
 if __name__ == "__main__":

@@ -3,7 +3,7 @@ tests/fit_ols.rs:10; GLM b 1e-7 / V 1e-6, tests/glm.rs:3-4)."""
 import numpy as np
 import pytest
 
-from fixtures import case_inputs, cases
+from fixtures import case_inputs, cases, fixest_ols_cases
 from gpu_util import announce
 from test_oracle import fixest_nested
 
@@ -28,7 +28,7 @@ def _run(case, model):
                     nested=nested, ssc=case["ssc"], tol=1e-12, opts=F.SolveOpts(tol=1e-12))
 
 
-@pytest.mark.parametrize("case", [c for c in cases("fixest_ols.json") if len(c["cluster"]) <= 1],
+@pytest.mark.parametrize("case", fixest_ols_cases(),
                          ids=lambda c: c["name"])
 def test_fixest_ols(case):
     announce("fixest ols", case["name"])

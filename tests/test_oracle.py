@@ -3,7 +3,7 @@ dense dummy-variable projection and the fixest golden fixtures (no GPU)."""
 import numpy as np
 import pytest
 
-from fixtures import case_inputs, cases, dense_codes, dense_residualize, dummies, nested_by_value
+from fixtures import case_inputs, cases, dense_codes, dense_residualize, dummies, fixest_ols_cases, nested_by_value
 from oracle import estimate
 from oracle.oracle import OracleSolver
 
@@ -277,7 +277,7 @@ def _fit_ols_oracle(case):
     return coef, V, dfk, kept
 
 
-@pytest.mark.parametrize("case", [c for c in cases("fixest_ols.json") if len(c["cluster"]) <= 1], ids=lambda c: c["name"])
+@pytest.mark.parametrize("case", fixest_ols_cases(), ids=lambda c: c["name"])
 def test_fixest_ols_fixtures(case):
     coef, V, dfk, kept = _fit_ols_oracle(case)
     e = case["expect"]
