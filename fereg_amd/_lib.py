@@ -94,6 +94,14 @@ class Ssc(ctypes.Structure):
                 ("reserved", ctypes.c_int32)]
 
 
+FH_GDF = {"min": 0, "conventional": 1}
+
+
+class Clusters(ctypes.Structure):
+    _fields_ = [("codes", ctypes.POINTER(vp)), ("ndim", ctypes.c_size_t), ("g_df", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class FitStats(ctypes.Structure):
     _fields_ = [("ssr", ctypes.c_double), ("df_k", ctypes.c_double), ("nparams", ctypes.c_double),
                 ("deviance", ctypes.c_double), ("iterations", ctypes.c_uint64), ("pcg_iterations", ctypes.c_uint64),
@@ -210,6 +218,20 @@ SIGNATURES = [
     ("fh_ppml_fit_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_int32, vp, i32p, ctypes.POINTER(Ssc),
                                        ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
                                        ctypes.POINTER(FitStats), ctypes.POINTER(Ctx)]),
+    ("fh_ols_fit_multi", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(Clusters), i32p,
+                                        ctypes.POINTER(Ssc), ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
+                                        ctypes.POINTER(FitStats)]),
+    ("fh_ols_fit_multi_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(Clusters),
+                                            i32p, ctypes.POINTER(Ssc), ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
+                                            ctypes.POINTER(FitStats)]),
+    ("fh_ppml_fit_multi", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, f64p, ctypes.c_int32,
+                                         ctypes.POINTER(Clusters), i32p, ctypes.POINTER(Ssc), ctypes.c_double,
+                                         ctypes.c_uint64, ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
+                                         ctypes.POINTER(FitStats), ctypes.POINTER(Ctx)]),
+    ("fh_ppml_fit_multi_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, vp, ctypes.c_int32,
+                                             ctypes.POINTER(Clusters), i32p, ctypes.POINTER(Ssc), ctypes.c_double,
+                                             ctypes.c_uint64, ctypes.POINTER(SolveOptsC), f64p, f64p, i32p,
+                                             ctypes.POINTER(FitStats), ctypes.POINTER(Ctx)]),
     ("fh_ols_wildboot", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, u32p, i32p, ctypes.POINTER(Ssc),
                                        ctypes.POINTER(SolveOptsC), ctypes.POINTER(WbootSpec), f64p, f64p, i32p,
                                        ctypes.POINTER(WbootResult), f64p]),

@@ -454,7 +454,7 @@ struct WbPre {
 
 void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
                const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
-               const uint32_t* cl_dev, uint64_t G, const fh_ssc& ssc, double dfk, double iid_scale,
+               const ClusterSet& cs, const fh_ssc& ssc, double dfk, double iid_scale,
                std::vector<double>& V, uint64_t* ncl, WbPre* wb) {
     const int K = R - 1;
     std::vector<int> kept;
@@ -474,8 +474,20 @@ void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>&
         if (kind == FH_VCOV_HETERO) {
             hetero_meat(a, rows, a.n, ld, 1, K, e, w, meat.data());
             if (ssc.k_adj) factor = n / (n - dfk);
+        } else if (cs.dims.size() > 1) {
+            // several dimensions (fit/vcov.rs:366-380): G.df "min" scales by the smallest G once,
+            // "conventional" every subset meat by its own G/(G-1)
+            const double* ep = ecol >= 0 ? rows + ecol : e;
+            const size_t es = ecol >= 0 ? (size_t)ld : 1;
+            const bool conventional = ssc.g_adj && cs.g_df == FH_GDF_CONVENTIONAL;
+            const uint64_t gmin = multiway_meat(a, rows, ld, 1, K, ep, es, w, cs, conventional, meat.data());
+            if (ncl) *ncl = gmin;
+            if (ssc.g_adj && cs.g_df == FH_GDF_MIN) factor *= (double)gmin / ((double)gmin - 1.0);
+            if (ssc.k_adj) factor *= (n - 1.0) / (n - dfk);
         } else {
-            require(cl_dev != nullptr, "a clustered variance matrix needs cluster codes");
+            require(!cs.dims.empty() && cs.dims[0] != nullptr, "a clustered variance matrix needs cluster codes");
+            const uint32_t* cl_dev = cs.dims[0];
+            const uint64_t G = cs.G[0];
             // residuals from the rows' pad column when they are there: one row read per observation
             const double* ep = ecol >= 0 ? rows + ecol : e;
             const size_t es = ecol >= 0 ? (size_t)ld : 1;
@@ -517,8 +529,25 @@ void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>&
                 vk[(size_t)i * mk + j] = s * factor;
             }
     }
+    // fixest repairs a matrix with an eigenvalue <= 0 (fit/vcov.rs:487-491, every kind)
+    bool finite = true;
+    for (double v : vk) finite &= std::isfinite(v);
+    if (finite) posdef_fix(vk, mk);
     for (int i = 0; i < mk; i++)
         for (int j = 0; j < mk; j++) V[(size_t)kept[i] * K + kept[j]] = vk[(size_t)i * mk + j];
+}
+
+ClusterSet cluster_set(Absorber& a, const uint32_t* const* dims, size_t ndim, int g_df) {
+    require(ndim <= 4, "at most four cluster dimensions are supported");
+    require(g_df == FH_GDF_MIN || g_df == FH_GDF_CONVENTIONAL, "unknown G.df rule");
+    ClusterSet cs;
+    cs.g_df = g_df;
+    for (size_t d = 0; d < ndim; d++) {
+        require(dims[d] != nullptr, "null cluster codes");
+        cs.dims.push_back(dims[d]);
+        cs.G.push_back(cluster_levels(a, dims[d]));
+    }
+    return cs;
 }
 
 // Cluster levels: max code + 1.
@@ -547,12 +576,13 @@ struct OlsState {
 };
 
 // OLS on device inputs (kept order). rows scratch: n x (1+K) in row order.
-void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, const uint32_t* cl_dev,
+void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, const ClusterSet& cs,
              const int32_t* nested, const fh_ssc& ssc, const fh_solve_opts& o, double* beta, double* vcov,
              int32_t* kept_out, fh_fit_stats* stats, const fh_ctx* ctx, OlsState* state = nullptr) {
     hipStream_t st = a.st;
     const size_t n = a.n, R = K + 1;
     require(K >= 1 && K <= 21, "1..21 regressors are supported");
+    require(kind != FH_VCOV_CLUSTER || !cs.dims.empty(), "a clustered variance matrix needs cluster codes");
     // solver rows padded to the solver's column class (zero pads, residualized as inactive
     // columns): whole 16-byte vectors per row; a pad column takes the residuals, so the cluster
     // sums read one aligned row per observation
@@ -597,8 +627,7 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     const double dfk = df_k(a, kind, ssc, nested, np, mk);
     std::vector<double> V;
     uint64_t ncl = 0;
-    const uint64_t G_cl = kind == FH_VCOV_CLUSTER ? cluster_levels(a, cl_dev) : 0;
-    vcov_from(a, rows, (int)ld, (int)R, excluded, inv, e.p, ecol, w, kind, cl_dev, G_cl, ssc, dfk,
+    vcov_from(a, rows, (int)ld, (int)R, excluded, inv, e.p, ecol, w, kind, cs, ssc, dfk,
               ssr / ((double)n - 1.0), V, &ncl, state ? state->wb : nullptr);
     for (size_t j = 0; j < K; j++) {
         beta[j] = 0.0;
@@ -1096,27 +1125,46 @@ int fh_absorber_term_values(fh_absorber* ab, const double* coefs, size_t r, size
     });
 }
 
-static int ols_impl(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
-                    const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, double* beta, double* vcov,
-                    int32_t* kept, fh_fit_stats* stats, bool dev) {
+// Device codes of the cluster dimensions (host codes are uploaded into hold) and their set.
+struct ClusterArgs {
+    std::vector<DBuf<uint32_t>> up;
+    std::vector<const uint32_t*> ptr;
+};
+static ClusterSet clusters_in(Absorber& a, const uint32_t* const* dims, size_t ndim, int g_df, bool dev,
+                              ClusterArgs& hold) {
+    require(ndim == 0 || dims != nullptr, "null cluster dimensions");
+    hold.ptr.assign(ndim, nullptr);
+    for (size_t d = 0; d < ndim; d++) {
+        require(dims[d] != nullptr, "null cluster codes");
+        if (dev) {
+            hold.ptr[d] = dims[d];
+        } else {
+            hold.up.emplace_back(a.n);
+            h2d(hold.up.back().p, dims[d], a.n, a.st);
+            hold.ptr[d] = hold.up.back().p;
+        }
+    }
+    return cluster_set(a, hold.ptr.data(), ndim, g_df);
+}
+
+static int ols_impl(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind,
+                    const uint32_t* const* dims, size_t ndim, int32_t g_df, const int32_t* nested, const fh_ssc* ssc,
+                    const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats, bool dev) {
     return guard([&] {
         require(ab && y && X && ssc && o && beta && vcov && kept, "null argument");
         Absorber& a = ab->a;
         AbsScope g(a);
         restore_user_weights(a);
         const size_t n = a.n;
+        ClusterArgs hold;
+        const ClusterSet cs = clusters_in(a, dims, ndim, g_df, dev, hold);
         if (dev) {
-            ols_fit(a, y, X, K, kind, cluster, nested, *ssc, *o, beta, vcov, kept, stats, nullptr);
+            ols_fit(a, y, X, K, kind, cs, nested, *ssc, *o, beta, vcov, kept, stats, nullptr);
         } else {
             DBuf<double> dy(n), dx(n * K);
-            DBuf<uint32_t> dc;
             h2d(dy.p, y, n, a.st);
             h2d(dx.p, X, n * K, a.st);
-            if (cluster) {
-                dc.alloc(n);
-                h2d(dc.p, cluster, n, a.st);
-            }
-            ols_fit(a, dy.p, dx.p, K, kind, cluster ? dc.p : nullptr, nested, *ssc, *o, beta, vcov, kept, stats, nullptr);
+            ols_fit(a, dy.p, dx.p, K, kind, cs, nested, *ssc, *o, beta, vcov, kept, stats, nullptr);
         }
     });
 }
@@ -1159,7 +1207,10 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
         pre.rmat.assign(spec->r_mat, spec->r_mat + K);
         os.wb = &pre;
         fh_fit_stats stats{};
-        ols_fit(a, py, px, K, FH_VCOV_CLUSTER, pc, nested, *ssc, *o, beta, vcov, kept, &stats, nullptr, &os);
+        ClusterSet cs;
+        cs.dims = {pc};
+        cs.G = {G};
+        ols_fit(a, py, px, K, FH_VCOV_CLUSTER, cs, nested, *ssc, *o, beta, vcov, kept, &stats, nullptr, &os);
         // R over the kept columns; a restriction on a dropped column is not testable
         std::vector<double> rk;
         for (size_t j = 0; j < K; j++) {
@@ -1208,40 +1259,51 @@ int fh_ols_wildboot_dev(fh_absorber* ab, const double* y, const double* X, size_
 int fh_ols_fit(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
                const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, double* beta, double* vcov,
                int32_t* kept, fh_fit_stats* stats) {
-    return ols_impl(ab, y, X, K, kind, cluster, nested, ssc, o, beta, vcov, kept, stats, false);
+    return ols_impl(ab, y, X, K, kind, &cluster, cluster ? 1 : 0, FH_GDF_MIN, nested, ssc, o, beta, vcov, kept, stats,
+                    false);
 }
 int fh_ols_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const uint32_t* cluster,
                    const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, double* beta, double* vcov,
                    int32_t* kept, fh_fit_stats* stats) {
-    return ols_impl(ab, y, X, K, kind, cluster, nested, ssc, o, beta, vcov, kept, stats, true);
+    return ols_impl(ab, y, X, K, kind, &cluster, cluster ? 1 : 0, FH_GDF_MIN, nested, ssc, o, beta, vcov, kept, stats,
+                    true);
+}
+int fh_ols_fit_multi(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind, const fh_clusters* cl,
+                     const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, double* beta, double* vcov,
+                     int32_t* kept, fh_fit_stats* stats) {
+    return ols_impl(ab, y, X, K, kind, cl ? cl->codes : nullptr, cl ? cl->ndim : 0, cl ? cl->g_df : FH_GDF_MIN, nested,
+                    ssc, o, beta, vcov, kept, stats, false);
+}
+int fh_ols_fit_multi_dev(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t kind,
+                         const fh_clusters* cl, const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o,
+                         double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats) {
+    return ols_impl(ab, y, X, K, kind, cl ? cl->codes : nullptr, cl ? cl->ndim : 0, cl ? cl->g_df : FH_GDF_MIN, nested,
+                    ssc, o, beta, vcov, kept, stats, true);
 }
 
 static int ppml_impl(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset, int32_t kind,
-                     const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
-                     const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats,
-                     const fh_ctx* ctx, bool dev) {
+                     const uint32_t* const* dims, size_t ndim, int32_t g_df, const int32_t* nested, const fh_ssc* ssc,
+                     double tol, uint64_t maxiter, const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept,
+                     fh_fit_stats* stats, const fh_ctx* ctx, bool dev) {
     return guard([&] {
         require(ab && y && X && ssc && o && beta && vcov && kept, "null argument");
         Absorber& a = ab->a;
         AbsScope g(a);
         const size_t n = a.n;
+        ClusterArgs hold;
+        const ClusterSet cs = clusters_in(a, dims, ndim, g_df, dev, hold);
         if (dev) {
-            ppml_fit(a, y, X, K, offset, kind, cluster, nested, *ssc, tol, maxiter, *o, beta, vcov, kept, stats, ctx);
+            ppml_fit(a, y, X, K, offset, kind, cs, nested, *ssc, tol, maxiter, *o, beta, vcov, kept, stats, ctx);
         } else {
             DBuf<double> dy(n), dx(n * K), doff;
-            DBuf<uint32_t> dc;
             h2d(dy.p, y, n, a.st);
             h2d(dx.p, X, n * K, a.st);
             if (offset) {
                 doff.alloc(n);
                 h2d(doff.p, offset, n, a.st);
             }
-            if (cluster) {
-                dc.alloc(n);
-                h2d(dc.p, cluster, n, a.st);
-            }
-            ppml_fit(a, dy.p, dx.p, K, offset ? doff.p : nullptr, kind, cluster ? dc.p : nullptr, nested, *ssc, tol,
-                     maxiter, *o, beta, vcov, kept, stats, ctx);
+            ppml_fit(a, dy.p, dx.p, K, offset ? doff.p : nullptr, kind, cs, nested, *ssc, tol, maxiter, *o, beta, vcov,
+                     kept, stats, ctx);
         }
     });
 }
@@ -1249,15 +1311,29 @@ int fh_ppml_fit(fh_absorber* ab, const double* y, const double* X, size_t K, con
                 const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
                 const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats,
                 const fh_ctx* ctx) {
-    return ppml_impl(ab, y, X, K, offset, kind, cluster, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, ctx,
-                     false);
+    return ppml_impl(ab, y, X, K, offset, kind, &cluster, cluster ? 1 : 0, FH_GDF_MIN, nested, ssc, tol, maxiter, o,
+                     beta, vcov, kept, stats, ctx, false);
 }
 int fh_ppml_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset, int32_t kind,
                     const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
                     const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats,
                     const fh_ctx* ctx) {
-    return ppml_impl(ab, y, X, K, offset, kind, cluster, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, ctx,
-                     true);
+    return ppml_impl(ab, y, X, K, offset, kind, &cluster, cluster ? 1 : 0, FH_GDF_MIN, nested, ssc, tol, maxiter, o,
+                     beta, vcov, kept, stats, ctx, true);
+}
+int fh_ppml_fit_multi(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset, int32_t kind,
+                      const fh_clusters* cl, const int32_t* nested, const fh_ssc* ssc, double tol, uint64_t maxiter,
+                      const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats,
+                      const fh_ctx* ctx) {
+    return ppml_impl(ab, y, X, K, offset, kind, cl ? cl->codes : nullptr, cl ? cl->ndim : 0,
+                     cl ? cl->g_df : FH_GDF_MIN, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, ctx, false);
+}
+int fh_ppml_fit_multi_dev(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset,
+                          int32_t kind, const fh_clusters* cl, const int32_t* nested, const fh_ssc* ssc, double tol,
+                          uint64_t maxiter, const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept,
+                          fh_fit_stats* stats, const fh_ctx* ctx) {
+    return ppml_impl(ab, y, X, K, offset, kind, cl ? cl->codes : nullptr, cl ? cl->ndim : 0,
+                     cl ? cl->g_df : FH_GDF_MIN, nested, ssc, tol, maxiter, o, beta, vcov, kept, stats, ctx, true);
 }
 
 int fh_device_alloc(void** p, size_t bytes) {

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <utility>
 
+#include <functional>
+
 #include "fh_fit.hpp"
 #include "fh_sort.hpp"
 
@@ -523,6 +525,57 @@ uint64_t cluster_meat(Absorber& a, const double* z, size_t n, int ld, int c0, in
     download(&c, cnt.p, sizeof(c), st);
     stream_wait(st);
     return c;
+}
+
+uint64_t multiway_meat(Absorber& a, const double* z, int ld, int c0, int k, const double* e, size_t es,
+                       const double* w, const ClusterSet& cs, bool conventional, double* meat) {
+    const int q = (int)cs.dims.size();
+    require(q >= 1 && q <= 4, "1..4 cluster dimensions are supported");
+    const size_t kk = (size_t)k * k;
+    std::fill(meat, meat + kk, 0.0);
+    std::vector<double> part(kk);
+    DBuf<uint32_t> inter(q > 1 ? a.n : 0);
+    uint64_t gmin = UINT64_MAX;
+    // subsets by size, each size in lexicographic order (fit/vcov.rs:196-214)
+    for (int size = 1; size <= q; size++) {
+        const double sign = (size % 2) ? 1.0 : -1.0;
+        std::vector<std::vector<int>> subs;
+        std::vector<int> cur;
+        std::function<void(int)> rec = [&](int start) {
+            if ((int)cur.size() == size) {
+                subs.push_back(cur);
+                return;
+            }
+            for (int d = start; d < q; d++) {
+                cur.push_back(d);
+                rec(d + 1);
+                cur.pop_back();
+            }
+        };
+        rec(0);
+        for (const auto& sub : subs) {
+            const uint32_t* codes;
+            uint64_t G;
+            if (size == 1) {
+                codes = cs.dims[sub[0]];
+                G = cs.G[sub[0]];
+            } else {
+                std::vector<const uint32_t*> parts;
+                std::vector<uint64_t> pg;
+                for (int d : sub) {
+                    parts.push_back(cs.dims[d]);
+                    pg.push_back(cs.G[d]);
+                }
+                G = combine_dev(parts, pg, a.n, inter.p, a.st);
+                codes = inter.p;
+            }
+            const uint64_t g = cluster_meat(a, z, a.n, ld, c0, k, e, es, w, codes, G, part.data());
+            if (size == 1) gmin = std::min(gmin, g);
+            const double f = sign * (conventional ? (double)g / ((double)g - 1.0) : 1.0);
+            for (size_t i = 0; i < kk; i++) meat[i] += f * part[i];
+        }
+    }
+    return gmin;
 }
 
 }  // namespace fh

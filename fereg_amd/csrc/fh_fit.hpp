@@ -63,6 +63,14 @@ std::vector<double> upper_inverse(const std::vector<double>& R, int k);
 // Sweep generalized inverse (Mata invsym in column order), relative pivot tolerance.
 std::vector<double> invsym(const std::vector<double>& A, int k, double tol, std::vector<int>* dropped = nullptr);
 constexpr double INVSYM_TOL = 1e-13;
+// Stata invsym(A) with the pivot order of the largest remaining diagonal.
+std::vector<double> invsym_greedy(const std::vector<double>& A, int k, double tol, std::vector<int>* dropped = nullptr);
+// Symmetric eigendecomposition by cyclic Jacobi rotations: U's columns are the eigenvectors.
+void sym_eigen(const std::vector<double>& A, int k, std::vector<double>& lambda, std::vector<double>& U);
+// fixest's PSD repair (eigenvalues clipped at 1e-16); true when an entry moved by more than 1e-8.
+bool posdef_fix(std::vector<double>& V, int k);
+// reghdfe's PSD repair (Cameron-Gelbach-Miller); true when applied.
+bool fix_psd(std::vector<double>& V, int k, bool report_constant);
 bool chol_upper(const std::vector<double>& A, int k, std::vector<double>& R);
 std::vector<double> upper_mul(const std::vector<double>& A, const std::vector<double>& B, int k);
 // b solving R_xx b = R_xy for the triangular factor of [X, y] (y last).
@@ -110,16 +118,36 @@ double df_k(const Absorber& a, int kind, const fh_ssc& ssc, const int32_t* neste
 // Number of fixed-effect parameters: sum of the term levels minus (terms - 1).
 double fe_params(const Absorber& a);
 struct WbPre;
+// Cluster dimensions of a variance request: kept-order device codes with their level counts
+// (max code + 1), and fixest's ssc(G.df) rule for several dimensions.
+struct ClusterSet {
+    std::vector<const uint32_t*> dims;
+    std::vector<uint64_t> G;
+    int g_df = FH_GDF_MIN;
+};
+uint64_t cluster_levels(Absorber& a, const uint32_t* cl_dev);
+ClusterSet cluster_set(Absorber& a, const uint32_t* const* dims, size_t ndim, int g_df);
+// Multiway meat (vcov.rs:480-558): sum over the non-empty subsets S of the dimensions of
+// (-1)^(|S|+1) scale(G_S) meat_S, meat_S clustered by the intersection of S; scale(G) = G/(G-1)
+// when `conventional`, else 1. Scores z[i][c0..c0+k) e_i w_i. Returns the smallest G of a single
+// dimension; G_S counts non-empty clusters.
+uint64_t multiway_meat(Absorber& a, const double* z, int ld, int c0, int k, const double* e, size_t es,
+                       const double* w, const ClusterSet& cs, bool conventional, double* meat);
 void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
                const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
-               const uint32_t* cl_dev, uint64_t G, const fh_ssc& ssc, double dfk, double iid_scale,
+               const ClusterSet& cs, const fh_ssc& ssc, double dfk, double iid_scale,
                std::vector<double>& V, uint64_t* ncl, WbPre* wb = nullptr);
-uint64_t cluster_levels(Absorber& a, const uint32_t* cl_dev);
+
+// ------------------------------------------------------------------ factors (fh_factor.hip)
+
+// Dense codes of the interaction of device code arrays (n rows) into out; returns the levels.
+uint64_t combine_dev(const std::vector<const uint32_t*>& parts, const std::vector<uint64_t>& g, size_t n,
+                     uint32_t* out, hipStream_t st);
 
 // ------------------------------------------------------------------ Poisson PML (fh_irls.hip)
 
 void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, const double* offset, int kind,
-              const uint32_t* cl_dev, const int32_t* nested, const fh_ssc& ssc, double tol, uint64_t maxiter,
+              const ClusterSet& cs, const int32_t* nested, const fh_ssc& ssc, double tol, uint64_t maxiter,
               const fh_solve_opts& o, double* beta, double* vcov, int32_t* kept_out, fh_fit_stats* stats,
               const fh_ctx* ctx);
 

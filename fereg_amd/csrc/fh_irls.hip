@@ -240,13 +240,14 @@ std::vector<double> normal_coefs(const std::vector<double>& G, int R, const std:
 }  // namespace
 
 void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, const double* offset, int kind,
-              const uint32_t* cl_dev, const int32_t* nested, const fh_ssc& ssc, double tol, uint64_t maxiter,
+              const ClusterSet& cs, const int32_t* nested, const fh_ssc& ssc, double tol, uint64_t maxiter,
               const fh_solve_opts& o, double* beta, double* vcov, int32_t* kept_out, fh_fit_stats* stats,
               const fh_ctx* ctx) {
     hipStream_t st = a.st;
     const size_t n = a.n, R = K + 1;
     require(K >= 1 && K <= 21, "1..21 regressors are supported");
     require(maxiter >= 1, "the IRLS needs at least one iteration");
+    require(kind != FH_VCOV_CLUSTER || !cs.dims.empty(), "a clustered variance matrix needs cluster codes");
     // The caller's weights are the prior weights; the absorber holds the IRLS weights from here on
     // and gets the caller's back before its next non-Poisson call.
     if (!a.irls_weights) {
@@ -427,9 +428,8 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, const dou
     const double np = (double)mk + fe_params(a);
     const double dfk = df_k(a, kind, ssc, nested, np, mk);
     uint64_t ncl = 0;
-    const uint64_t G_cl = kind == FH_VCOV_CLUSTER ? cluster_levels(a, cl_dev) : 0;
     std::vector<double> V;
-    vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, a.w_rows.p, kind, cl_dev, G_cl, ssc, dfk, 1.0, V, &ncl);
+    vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, a.w_rows.p, kind, cs, ssc, dfk, 1.0, V, &ncl);
     for (size_t j = 0; j < K; j++) {
         beta[j] = 0.0;
         kept_out[j] = !excluded[j];

@@ -3,6 +3,7 @@
 // Cholesky and triangular helpers for the QR factor, and the triangular least-squares solve.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 #include "fh_fit.hpp"
@@ -182,6 +183,160 @@ std::vector<double> qr_solve_upper(const std::vector<double>& R, int m) {
         b[j] = s / d;
     }
     return b;
+}
+
+std::vector<double> invsym_greedy(const std::vector<double>& A, int k, double tol, std::vector<int>* dropped_out) {
+    // Stata invsym(A) without an order (vcov.rs:70-116): the next pivot is the unvisited one with
+    // the largest remaining diagonal (lowest index on ties); a pivot <= tol times its original
+    // diagonal (or <= 0, NaN) is dropped with its row and column.
+    std::vector<double> m(A);
+    std::vector<int> visited(k, 0), dropped(k, 0);
+    for (int it = 0; it < k; it++) {
+        int p = -1;
+        for (int i = 0; i < k; i++)
+            if (!visited[i] && (p < 0 || m[(size_t)i * k + i] > m[(size_t)p * k + p])) p = i;
+        visited[p] = 1;
+        const double d = m[(size_t)p * k + p];
+        if (std::isnan(d) || d <= tol * A[(size_t)p * k + p] || d <= 0.0) {
+            dropped[p] = 1;
+            for (int j = 0; j < k; j++) m[(size_t)p * k + j] = m[(size_t)j * k + p] = 0.0;
+            continue;
+        }
+        double* row = &m[(size_t)p * k];
+        for (int j = 0; j < k; j++) row[j] /= d;
+        for (int i = 0; i < k; i++) {
+            if (i == p) continue;
+            const double f = m[(size_t)i * k + p];
+            if (f == 0.0) continue;
+            for (int j = 0; j < k; j++) m[(size_t)i * k + j] -= f * row[j];
+            m[(size_t)i * k + p] = -f / d;
+        }
+        m[(size_t)p * k + p] = 1.0 / d;
+    }
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < i; j++) {
+            const double v = 0.5 * (m[(size_t)i * k + j] + m[(size_t)j * k + i]);
+            m[(size_t)i * k + j] = m[(size_t)j * k + i] = v;
+        }
+    if (dropped_out) *dropped_out = dropped;
+    return m;
+}
+
+void sym_eigen(const std::vector<double>& A, int k, std::vector<double>& lambda, std::vector<double>& U) {
+    // Cyclic Jacobi rotations (Golub and Van Loan, Matrix Computations, 4th ed., Alg. 8.5.3):
+    // each rotation zeroes one off-diagonal pair with the smaller of the two angles that do; the
+    // sweeps end when the off-diagonal mass is below (1e-2 eps)^2 of the total. U holds the
+    // eigenvectors as columns (row-major), lambda the eigenvalues (unsorted).
+    std::vector<double> m(A);
+    U.assign((size_t)k * k, 0.0);
+    for (int i = 0; i < k; i++) U[(size_t)i * k + i] = 1.0;
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < k; i++)
+            for (int j = 0; j < k; j++) {
+                const double v = m[(size_t)i * k + j] * m[(size_t)i * k + j];
+                tot += v;
+                if (i != j) off += v;
+            }
+        if (off == 0.0 || off <= (1e-2 * eps) * (1e-2 * eps) * tot) break;
+        for (int p = 0; p < k; p++)
+            for (int q = p + 1; q < k; q++) {
+                const double apq = m[(size_t)p * k + q];
+                if (apq == 0.0) continue;
+                const double tau = (m[(size_t)q * k + q] - m[(size_t)p * k + p]) / (2.0 * apq);
+                const double t = tau >= 0.0 ? 1.0 / (tau + std::sqrt(1.0 + tau * tau))
+                                            : -1.0 / (-tau + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int r = 0; r < k; r++) {  // columns p, q of m
+                    const double xp = m[(size_t)r * k + p], xq = m[(size_t)r * k + q];
+                    m[(size_t)r * k + p] = c * xp - s * xq;
+                    m[(size_t)r * k + q] = s * xp + c * xq;
+                }
+                for (int r = 0; r < k; r++) {  // rows p, q of m
+                    const double xp = m[(size_t)p * k + r], xq = m[(size_t)q * k + r];
+                    m[(size_t)p * k + r] = c * xp - s * xq;
+                    m[(size_t)q * k + r] = s * xp + c * xq;
+                }
+                for (int r = 0; r < k; r++) {
+                    const double up = U[(size_t)r * k + p], uq = U[(size_t)r * k + q];
+                    U[(size_t)r * k + p] = c * up - s * uq;
+                    U[(size_t)r * k + q] = s * up + c * uq;
+                }
+            }
+    }
+    lambda.resize(k);
+    for (int i = 0; i < k; i++) lambda[i] = m[(size_t)i * k + i];
+}
+
+bool posdef_fix(std::vector<double>& V, int k) {
+    // fixest mat_posdef_fix (fit/vcov.rs:514-534): when an eigenvalue is <= 0, V is rebuilt with
+    // the eigenvalues clipped at 1e-16; true when that moved an entry by more than 1e-8.
+    if (k == 0) return false;
+    std::vector<double> lam, U;
+    sym_eigen(V, k, lam, U);
+    bool any = false;
+    for (double l : lam) any |= !(l > 0.0);
+    if (!any) return false;
+    std::vector<double> f((size_t)k * k, 0.0);
+    for (int e = 0; e < k; e++) {
+        const double l = std::max(lam[e], 1e-16);
+        for (int i = 0; i < k; i++) {
+            const double ui = U[(size_t)i * k + e] * l;
+            for (int j = 0; j < k; j++) f[(size_t)i * k + j] += ui * U[(size_t)j * k + e];
+        }
+    }
+    bool changed = false;
+    for (size_t i = 0; i < f.size(); i++) changed |= std::fabs(V[i] - f[i]) > 1e-8;
+    V = f;
+    return changed;
+}
+
+namespace {
+// U diag(max(lambda, 0)) U' from the lower triangle, mirrored; false when no eigenvalue is negative.
+bool clip_psd(const std::vector<double>& V, int k, std::vector<double>& out) {
+    std::vector<double> lam, U;
+    sym_eigen(V, k, lam, U);
+    bool neg = false;
+    for (double l : lam) neg |= l < 0.0;
+    if (!neg) return false;
+    out.assign((size_t)k * k, 0.0);
+    for (int e = 0; e < k; e++) {
+        if (lam[e] <= 0.0) continue;
+        for (int i = 0; i < k; i++) {
+            const double ui = U[(size_t)i * k + e] * lam[e];
+            for (int j = 0; j <= i; j++) out[(size_t)i * k + j] += ui * U[(size_t)j * k + e];
+        }
+    }
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < i; j++) out[(size_t)j * k + i] = out[(size_t)i * k + j];
+    return true;
+}
+}  // namespace
+
+bool fix_psd(std::vector<double>& V, int k, bool report_constant) {
+    // reghdfe_fix_psd (Cameron, Gelbach and Miller 2011; vcov.rs:689-712): symmetrize; with a
+    // negative eigenvalue, replace V by its clipped form, and with a reported constant (last
+    // row and column) put back the separately clipped regressor block.
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < i; j++) {
+            const double s = 0.5 * (V[(size_t)i * k + j] + V[(size_t)j * k + i]);
+            V[(size_t)i * k + j] = V[(size_t)j * k + i] = s;
+        }
+    const int kb = report_constant ? k - 1 : k;
+    std::vector<double> backup((size_t)kb * kb);
+    for (int i = 0; i < kb; i++)
+        for (int j = 0; j < kb; j++) backup[(size_t)i * kb + j] = V[(size_t)i * k + j];
+    std::vector<double> fixed;
+    if (!clip_psd(V, k, fixed)) return false;
+    V = fixed;
+    if (report_constant) {
+        std::vector<double> sub;
+        if (!clip_psd(backup, kb, sub)) sub = backup;
+        for (int i = 0; i < kb; i++)
+            for (int j = 0; j < kb; j++) V[(size_t)i * k + j] = sub[(size_t)i * kb + j];
+    }
+    return true;
 }
 
 }  // namespace fh

@@ -69,6 +69,15 @@ def _host_codes(a, what):
     return np.ascontiguousarray(v, dtype=np.uint32)
 
 
+def _cluster_dims(cluster):
+    """A cluster argument as a list of dimensions (None, one array, or a list / tuple of arrays)."""
+    if cluster is None:
+        return []
+    if isinstance(cluster, (list, tuple)):
+        return list(cluster)
+    return [cluster]
+
+
 def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, extra_args, offset=None,
          ctx=None):
     lib = L.lib()
@@ -78,6 +87,9 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
     K = X.shape[1]
     if absorber is not None and weights is not None:
         raise L.FeregError(L.FH_INVALID, "weights belong to the absorber: pass them when it is built")
+    dims = _cluster_dims(cluster)
+    if len(dims) > 4:
+        raise L.FeregError(L.FH_INVALID, "at most four cluster dimensions are supported")
     ab = absorber or Absorber(codes, g, weights=weights)
     kind = L.FH_VCOV[vcov]
     ssc = ssc or {}
@@ -93,24 +105,30 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
     o = (opts or SolveOpts(tol=1e-10)).c()
     poisson = kind_fn == "fh_ppml_fit"
     if _is_torch(y):
-        held = _dev_ready(_dev_f64(y, "y"), _dev_f64(X, "X"), _dev_codes(cluster, "cluster"),
-                          _dev_f64(offset, "offset"))  # kept alive through the call
-        fn = getattr(lib, kind_fn + "_dev")
+        held = _dev_ready(_dev_f64(y, "y"), _dev_f64(X, "X"), _dev_f64(offset, "offset"),
+                          *[_dev_codes(c, "cluster") for c in dims])  # kept alive through the call
+        cptr = [t.data_ptr() for t in held[3:]]
+        fn = getattr(lib, kind_fn + "_multi_dev")
         args = [ab._h, held[0].data_ptr(), held[1].data_ptr(), K]
         if poisson:
-            args.append(None if offset is None else held[3].data_ptr())
-        args += [kind, None if cluster is None else held[2].data_ptr(), _ptr(nest, L.i32p), ctypes.byref(sscc)]
+            args.append(None if offset is None else held[2].data_ptr())
     else:
         n = ab.n()
         yv = _host_f64(y, "y", n)
         Xv = _host_f64(X, "X", n)
-        cl = None if cluster is None else _host_codes(cluster, "cluster")
-        fn = getattr(lib, kind_fn)
+        cls = [_host_codes(c, "cluster") for c in dims]
+        for c in cls:
+            if c.shape[0] != n:
+                raise L.FeregError(L.FH_INVALID, f"cluster: {c.shape[0]} rows, the sample has {n}")
+        cptr = [c.ctypes.data for c in cls]
+        fn = getattr(lib, kind_fn + "_multi")
         args = [ab._h, _ptr(yv, L.f64p), _ptr(Xv, L.f64p), K]
         if poisson:
             ov = None if offset is None else _host_f64(offset, "offset", n)
             args.append(None if ov is None else _ptr(ov, L.f64p))
-        args += [kind, None if cl is None else _ptr(cl, L.u32p), _ptr(nest, L.i32p), ctypes.byref(sscc)]
+    carr = (L.vp * max(len(cptr), 1))(*cptr)
+    clc = L.Clusters(carr, len(cptr), L.FH_GDF[ssc.get("G.df", "min")], 0)
+    args += [kind, ctypes.byref(clc) if cptr else None, _ptr(nest, L.i32p), ctypes.byref(sscc)]
     args += extra_args + [ctypes.byref(o), _ptr(beta, L.f64p), _ptr(V, L.f64p), _ptr(kept, L.i32p), ctypes.byref(st)]
     if poisson:
         args.append(L.ctx_ref(ctx))
@@ -124,7 +142,8 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
 
 
 def feols(y, X, codes, g, weights=None, vcov="iid", cluster=None, nested=None, ssc=None, opts=None, absorber=None):
-    """OLS with absorbed FEs (fit/ols.rs:213-330) on the GPU."""
+    """OLS with absorbed FEs (fit/ols.rs:213-330) on the GPU. cluster: one code array or a list
+    of up to four (multiway, fixest's ssc "G.df" rule)."""
     return _fit("fh_ols_fit", y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, [])
 
 

@@ -231,6 +231,27 @@ int fh_ols_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, 
                    const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o,
                    double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
 
+/* Several cluster dimensions (fit/vcov.rs:366-380; vcov.rs:480-558): the meat is the
+ * inclusion-exclusion sum over the non-empty subsets of the dimensions of meats clustered by each
+ * subset's intersection. g_df: FH_GDF_MIN (fixest's default ssc(G.df = "min"): the matrix is scaled
+ * once by G/(G-1) with the smallest G) or FH_GDF_CONVENTIONAL (every subset meat by its own G/(G-1)).
+ * As fixest does, a variance matrix with an eigenvalue <= 0 is rebuilt with the eigenvalues clipped
+ * at 1e-16 (fit/vcov.rs:487-534; every fit entry point). stats->n_clusters = the smallest G. */
+#define FH_GDF_MIN 0
+#define FH_GDF_CONVENTIONAL 1
+typedef struct {
+    const uint32_t* const* codes; /* ndim arrays of n kept-order codes (host, or device for _dev) */
+    size_t ndim;                  /* 1..4 */
+    int32_t g_df;
+    int32_t reserved;
+} fh_clusters;
+int fh_ols_fit_multi(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t vcov_kind,
+                     const fh_clusters* clusters, const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o,
+                     double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
+int fh_ols_fit_multi_dev(fh_absorber* ab, const double* y, const double* X, size_t K, int32_t vcov_kind,
+                         const fh_clusters* clusters, const int32_t* nested, const fh_ssc* ssc,
+                         const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept, fh_fit_stats* stats);
+
 /* Poisson PML by IRLS with warm-started weighted residualization (glm/irls.rs:367-913, fixest
  * flavour: fit/glm.rs; family.rs:103-145), robust (hetero), iid or cluster variance. y, X and
  * offset (NULL = none) in KEPT order. The absorber's weights (fh_absorber_create / set_weights)
@@ -245,6 +266,16 @@ int fh_ppml_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K,
                     int32_t vcov_kind, const uint32_t* cluster, const int32_t* nested, const fh_ssc* ssc, double tol,
                     uint64_t maxiter, const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept,
                     fh_fit_stats* stats, const fh_ctx* ctx);
+
+/* fh_ppml_fit with several cluster dimensions (see fh_clusters). */
+int fh_ppml_fit_multi(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset,
+                      int32_t vcov_kind, const fh_clusters* clusters, const int32_t* nested, const fh_ssc* ssc,
+                      double tol, uint64_t maxiter, const fh_solve_opts* o, double* beta, double* vcov, int32_t* kept,
+                      fh_fit_stats* stats, const fh_ctx* ctx);
+int fh_ppml_fit_multi_dev(fh_absorber* ab, const double* y, const double* X, size_t K, const double* offset,
+                          int32_t vcov_kind, const fh_clusters* clusters, const int32_t* nested, const fh_ssc* ssc,
+                          double tol, uint64_t maxiter, const fh_solve_opts* o, double* beta, double* vcov,
+                          int32_t* kept, fh_fit_stats* stats, const fh_ctx* ctx);
 
 /* WCR11 wild cluster bootstrap of one linear hypothesis R beta = r on the OLS fit, Rademacher
  * weights, clusters = bootstrap clusters (wildboot/ols.rs:243-649 run_cluster_11; weights.rs;

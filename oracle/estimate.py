@@ -10,6 +10,8 @@ checker. Follows fereg-core's fit layer (fixest conventions):
   vcov          fit/vcov.rs:316-382 (iid, hetero, one-way cluster)
   poisson IRLS  glm/irls.rs:367-913 with glm/family.rs:103-145 (fixest flavour)
 """
+import itertools
+
 import numpy as np
 
 
@@ -89,29 +91,63 @@ def df_k(kind, np_, term_sizes, nested, k_adj=True, k_fixef="nonnested", n_desig
     return max(k, n_design + 1)
 
 
-def vcov(kind, bread, X, e, w, n, dfk, cluster=None, k_adj=True, g_adj=True):
-    """fit/vcov.rs:316-382: iid, hetero, one-way CRV1 (G.df = "min")."""
+def _cluster_meat(s, codes):
+    codes = np.asarray(codes)
+    _, dense = np.unique(codes, return_inverse=True)
+    dense = dense.reshape(-1)
+    G = int(dense.max()) + 1
+    sg = np.stack([np.bincount(dense, weights=s[:, j], minlength=G) for j in range(s.shape[1])], axis=1)
+    return sg.T @ sg, G
+
+
+def _intersection(dims):
+    return np.unique(np.stack([np.asarray(d) for d in dims], axis=1), axis=0, return_inverse=True)[1].reshape(-1)
+
+
+def posdef_fix(V):
+    """fixest mat_posdef_fix (fit/vcov.rs:514-534): eigenvalues <= 0 clipped at 1e-16."""
+    if V.size == 0:
+        return V
+    lam, U = np.linalg.eigh(V)
+    if np.all(lam > 0):
+        return V
+    return (U * np.maximum(lam, 1e-16)) @ U.T
+
+
+def vcov(kind, bread, X, e, w, n, dfk, cluster=None, k_adj=True, g_adj=True, g_df="min"):
+    """fit/vcov.rs:316-382: iid, hetero, CRV1 over one or several cluster dimensions
+    (inclusion-exclusion over the subsets of the dimensions, vcov.rs:480-558; G.df "min" or
+    "conventional"), then fixest's PSD repair."""
     if kind == "iid":
         ssr = float(np.sum((w if w is not None else 1.0) * e * e))
         factor = (n - 1) / (n - dfk) if k_adj else 1.0
-        return bread * (ssr / (n - 1)) * factor
+        return posdef_fix(bread * (ssr / (n - 1)) * factor)
     s = X * (e * (w if w is not None else 1.0))[:, None]
     if kind == "hetero":
         meat = s.T @ s
         factor = n / (n - dfk) if k_adj else 1.0
-        return factor * bread @ meat @ bread
+        return posdef_fix(factor * bread @ meat @ bread)
     if kind == "cluster":
-        codes = np.asarray(cluster)
-        G = int(codes.max()) + 1
-        sg = np.stack([np.bincount(codes, weights=s[:, j], minlength=G) for j in range(X.shape[1])], axis=1)
-        Geff = len(np.unique(codes))
-        meat = sg.T @ sg
+        dims = cluster if isinstance(cluster, (list, tuple)) else [cluster]
+        q = len(dims)
+        conventional = g_adj and g_df == "conventional"
+        meat = np.zeros((X.shape[1], X.shape[1]))
+        gs = []
+        for size in range(1, q + 1):
+            sign = 1.0 if size % 2 else -1.0
+            for sub in itertools.combinations(range(q), size):
+                codes = dims[sub[0]] if size == 1 else _intersection([dims[d] for d in sub])
+                m, G = _cluster_meat(s, codes)
+                if size == 1:
+                    gs.append(G)
+                meat += sign * (G / (G - 1) if conventional else 1.0) * m
         factor = 1.0
-        if g_adj:
-            factor *= Geff / (Geff - 1)
+        gmin = min(gs)
+        if g_adj and g_df == "min":
+            factor *= gmin / (gmin - 1)
         if k_adj:
             factor *= (n - 1) / (n - dfk)
-        return factor * bread @ meat @ bread
+        return posdef_fix(factor * bread @ meat @ bread)
     raise ValueError(kind)
 
 

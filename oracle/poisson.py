@@ -237,7 +237,8 @@ def ppml(codes, g, y, X, weights=None, offset=None, tol=1e-8, itol=1e-10, maxite
     if vcov == "iid":
         V = bread * ((n - 1) / (n - dfk) if k_adj else 1.0)
     else:
-        cl = None if cluster is None else np.asarray(cluster)[pi]
+        dims = [] if cluster is None else (cluster if isinstance(cluster, (list, tuple)) else [cluster])
+        cl = [np.asarray(c)[pi] for c in dims] or None
         V = estimate.vcov(vcov, bread, Xt, e, w, n, dfk, cluster=cl, k_adj=k_adj, g_adj=g_adj)
     Vf = np.zeros((K, K))
     Vf[np.ix_(kept, kept)] = V

@@ -20,7 +20,7 @@ SRC = "/root/reference/crates/fereg-core/tests/data"
 DST = os.path.dirname(os.path.abspath(__file__))
 
 OLS_CASES = None  # every OLS case
-POIS_CASES = ["pois_fe2_iid", "pois_fe2_hetero", "pois_collin", "pois_inter_fx", "pois_fe2_off_w"]
+POIS_CASES = ["pois_fe2_iid", "pois_fe2_hetero", "pois_collin", "pois_inter_fx", "pois_fe2_off_w", "pois_fe2_cl_c3_h"]
 KEEP = ["coef", "se", "vcov", "df_k", "nobs", "nparams", "collin", "obs", "G", "resid", "ssr", "coef_names"]
 
 # reghdfe fixtures: (name, y, x, absorb terms, cluster dims, robust, weight (var, kind), singletons,

@@ -15,7 +15,7 @@ def _run(case, model):
     inp = case_inputs(case)
     kind = case["vcov"]
     nested = fixest_nested(case, inp) if kind == "cluster" else None
-    cl = inp["clusters"][0] if inp["clusters"] else None
+    cl = inp["clusters"] if inp["clusters"] else None
     codes, gs = inp["codes"], inp["g"]
     w = inp["w"]
     if model == "ols":
@@ -28,7 +28,7 @@ def _run(case, model):
                     nested=nested, ssc=case["ssc"], tol=1e-12, opts=F.SolveOpts(tol=1e-12))
 
 
-@pytest.mark.parametrize("case", fixest_ols_cases(),
+@pytest.mark.parametrize("case", fixest_ols_cases(multiway=True),
                          ids=lambda c: c["name"])
 def test_fixest_ols(case):
     announce("fixest ols", case["name"])

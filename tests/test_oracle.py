@@ -272,12 +272,12 @@ def _fit_ols_oracle(case):
     kfix = ssc["K.fixef"]
     dfk = estimate.df_k(kind, np_, gs, nested, k_adj=ssc["K.adj"], k_fixef=kfix, n_design=len(kept))
     Xk = rows[:, 1:][:, kept]
-    V = estimate.vcov(kind, bread, Xk, e, wr, n, dfk, cluster=cl[0] if cl else None, k_adj=ssc["K.adj"],
-                      g_adj=ssc["G.adj"])
+    V = estimate.vcov(kind, bread, Xk, e, wr, n, dfk, cluster=cl if cl else None, k_adj=ssc["K.adj"],
+                      g_adj=ssc["G.adj"], g_df=ssc.get("G.df", "min"))
     return coef, V, dfk, kept
 
 
-@pytest.mark.parametrize("case", fixest_ols_cases(), ids=lambda c: c["name"])
+@pytest.mark.parametrize("case", fixest_ols_cases(multiway=True), ids=lambda c: c["name"])
 def test_fixest_ols_fixtures(case):
     coef, V, dfk, kept = _fit_ols_oracle(case)
     e = case["expect"]

@@ -15,7 +15,7 @@ RTOL = 1e-8
 def _run(case, **kw):
     inp = case_inputs(case)
     nested = fixest_nested(case, inp) if case["vcov"] == "cluster" else None
-    cl = inp["clusters"][0] if inp["clusters"] else None
+    cl = inp["clusters"] if inp["clusters"] else None
     return inp, poisson.ppml(inp["codes"], inp["g"], inp["y"], inp["X"], weights=inp["w"], offset=inp["offset"],
                              tol=1e-12, itol=1e-12, vcov=case["vcov"], cluster=cl, nested=nested, **kw)
 
