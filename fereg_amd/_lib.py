@@ -175,6 +175,30 @@ class SampleCounts(ctypes.Structure):
                                                "na_extra")]
 
 
+FH_VCE = {"unadjusted": 0, "robust": 1, "cluster": 2}
+
+
+class ReghdfeSpec(ctypes.Structure):
+    _fields_ = [("vce", ctypes.c_int32), ("weight_kind", ctypes.c_int32), ("terms", ctypes.POINTER(Term)),
+                ("nterms", ctypes.c_size_t), ("clusters", ctypes.POINTER(vp)), ("nclusters", ctypes.c_size_t),
+                ("noconstant", ctypes.c_int32), ("dof_firstpair", ctypes.c_int32), ("dof_pairwise", ctypes.c_int32),
+                ("dof_clusters", ctypes.c_int32), ("dof_continuous", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("tol", ctypes.c_double), ("maxiter", ctypes.c_uint64), ("num_singletons", ctypes.c_uint64)]
+
+
+REGHDFE_SCALARS = ["N", "rank", "df_r", "df_m", "df_a", "df_a_initial", "df_a_redundant", "df_a_nested", "N_clust",
+                   "N_clustervars"]
+REGHDFE_FIT = ["tss", "tss_within", "rss", "mss", "rmse", "F", "ll", "ll_0", "r2", "r2_within", "r2_a",
+               "r2_a_within", "sumweights", "num_singletons", "N_full", "report_constant", "converged", "y_absorbed"]
+
+
+class ReghdfeStats(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_double) for k in REGHDFE_SCALARS] + [("N_clust_dim", ctypes.c_double * 4)]
+                + [(k, ctypes.c_double) for k in REGHDFE_FIT]
+                + [("iterations", ctypes.c_uint64), ("nslots", ctypes.c_uint64), ("psd_fixed", ctypes.c_int32),
+                   ("reserved", ctypes.c_int32)])
+
+
 # (name, restype, argtypes) of every exported entry point of include/fereg_hip.h
 SIGNATURES = [
     ("fh_last_error", ctypes.c_char_p, []),
@@ -261,6 +285,11 @@ SIGNATURES = [
     ("fh_sample_cluster", ctypes.c_int, [vp, ctypes.c_size_t, u32p, u64p]),
     ("fh_sample_gather", ctypes.c_int, [vp, f64p, f64p]),
     ("fh_sample_gather_dev", ctypes.c_int, [vp, vp, vp]),
+    ("fh_reghdfe_spec_default", None, [ctypes.POINTER(ReghdfeSpec)]),
+    ("fh_reghdfe_fit", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, ctypes.POINTER(ReghdfeSpec), f64p, f64p, i32p,
+                                      f64p, ctypes.POINTER(ReghdfeStats), ctypes.POINTER(Ctx)]),
+    ("fh_reghdfe_fit_dev", ctypes.c_int, [vp, vp, vp, ctypes.c_size_t, ctypes.POINTER(ReghdfeSpec), f64p, f64p, i32p,
+                                          f64p, ctypes.POINTER(ReghdfeStats), ctypes.POINTER(Ctx)]),
     ("fh_server_config_default", None, [ctypes.POINTER(ServerConfig)]),
     ("fh_server_create", ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
     ("fh_server_submit", ctypes.c_int, [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),

@@ -216,6 +216,12 @@ __global__ void k_fit_rows(const double* __restrict__ coefs, int r, int col, con
 
 }  // namespace
 
+// Rows [y, X] of the kept-order inputs in solver row order, padded to ld columns with zeros.
+void gather_block(const double* y, const double* X, size_t K, const uint32_t* pi, size_t n, size_t ld, double* rows,
+                  hipStream_t st) {
+    gather_rows(y, X, K, pi, n, ld, rows, st);
+}
+
 double sum_parts(const double* dev, size_t count, hipStream_t st) {
     std::vector<double> h(count);
     d2h(h.data(), dev, count, st);

@@ -108,6 +108,9 @@ struct ResidOut {
 // Residualizes the n x r device block z (row order) in place; warm / coefs: (m + g1) x r.
 ResidOut residualize_any(Absorber& a, double* z, size_t r, double tol, uint64_t maxiter, const double* warm_dev,
                          double* coefs_dev, int precision, const fh_ctx* ctx);
+// Rows [y, X] of kept-order device inputs in solver row order (pi), padded to ld columns.
+void gather_block(const double* y, const double* X, size_t K, const uint32_t* pi, size_t n, size_t ld, double* rows,
+                  hipStream_t st);
 // Puts the caller's own weights back when a Poisson fit left its IRLS weights in the absorber.
 void restore_user_weights(Absorber& a);
 

@@ -316,6 +316,57 @@ int fh_ols_wildboot_dev(fh_absorber* ab, const double* y, const double* X, size_
                         const int32_t* nested, const fh_ssc* ssc, const fh_solve_opts* o, const fh_wboot_spec* spec,
                         double* beta, double* vcov, int32_t* kept, fh_wboot_result* res, double* t_boot);
 
+/* OLS with reghdfe 6.13.1's conventions (Session::ols, ols.rs:234-706; dof.rs:64-182): the
+ * variables are standardized by their unweighted sd; stage 1 drops regressors explained by the
+ * fixed effects (sum x~^2 / sum x^2 <= min(1e-6, tol / 10)), stage 2 sweeps the rest in order
+ * (Mata invsym on the standardized Gram); the constant is the regression through the weighted
+ * means; unadjusted, robust (pweight + unadjusted = robust) or multiway cluster variance with
+ * reghdfe's degrees of freedom (nested terms, redundant levels, pairwise mobility groups) and
+ * small-sample factors, the CGM PSD repair for several cluster dimensions; the Wald F test.
+ * The absorber holds the model's raw weights (kept order at creation) and weight_kind says what
+ * they are; a model without absorbed terms uses an absorber with one one-level term and nterms =
+ * 0 here. Outputs: b (K + 1: the regressors, then _cons when reported), V ((K + 1) x (K + 1)
+ * row-major over the same k_all = K + report_constant entries), status (K: 0 kept, 2 absorbed by
+ * the fixed effects, 3 collinear), dof_table (5 per slot, max(nterms, 1) slots: K, M, K - M,
+ * inexact, nested). Omitted coefficients are 0 with zero variance. */
+#define FH_VCE_UNADJUSTED 0
+#define FH_VCE_ROBUST 1
+#define FH_VCE_CLUSTER 2
+typedef struct {
+    int32_t vce;                     /* FH_VCE_* */
+    int32_t weight_kind;             /* FH_WEIGHT_* of the absorber's weights */
+    const fh_term* terms;            /* kept-order codes of the absorbed terms (degrees of freedom) */
+    size_t nterms;
+    const uint32_t* const* clusters; /* FH_VCE_CLUSTER: kept-order codes of 1..4 dimensions */
+    size_t nclusters;
+    int32_t noconstant;
+    int32_t dof_firstpair;           /* DofOptions (api.rs): dofadjustments() */
+    int32_t dof_pairwise;
+    int32_t dof_clusters;
+    int32_t dof_continuous;
+    int32_t precision;               /* -1 = the absorber's */
+    double tol;                      /* solver tolerance (reghdfe tolerance()) */
+    uint64_t maxiter;
+    uint64_t num_singletons;         /* reported in N_full */
+} fh_reghdfe_spec;
+/* e() scalars (NaN = missing). */
+typedef struct {
+    double N, rank, df_r, df_m, df_a, df_a_initial, df_a_redundant, df_a_nested;
+    double N_clust, N_clustervars, N_clust_dim[4];
+    double tss, tss_within, rss, mss, rmse, F, ll, ll_0, r2, r2_within, r2_a, r2_a_within, sumweights;
+    double num_singletons, N_full, report_constant, converged, y_absorbed;
+    uint64_t iterations, nslots;
+    int32_t psd_fixed;
+    int32_t reserved;
+} fh_reghdfe_stats;
+void fh_reghdfe_spec_default(fh_reghdfe_spec* spec);
+int fh_reghdfe_fit(fh_absorber* ab, const double* y, const double* X, size_t K, const fh_reghdfe_spec* spec,
+                   double* b, double* V, int32_t* status, double* dof_table, fh_reghdfe_stats* stats,
+                   const fh_ctx* ctx);
+int fh_reghdfe_fit_dev(fh_absorber* ab, const double* y, const double* X, size_t K, const fh_reghdfe_spec* spec,
+                       double* b, double* V, int32_t* status, double* dof_table, fh_reghdfe_stats* stats,
+                       const fh_ctx* ctx);
+
 /* ------------------------------------------------------------------ factors and the sample */
 
 /* A factor column (fit::FactorCol, fit/mod.rs; factor::FactorValues::Num, api.rs). Missing
