@@ -1,8 +1,9 @@
 """Estimation entry points over the C ABI (fit/mod.rs:591; fit/ols.rs:213; fit/glm.rs:362).
 
-`feols` and `fepois` take kept-sample arrays (sample accounting -- missing values, singleton
-removal, factor encoding -- happens upstream, as in the reference's fit/input.rs), FE codes per
-term, and a variance request in fixest's conventions (fit/vcov.rs:278-382).
+`fit` is the fit layer on raw columns: the estimation sample (missing values, zero weights,
+factor encoding, interactions, singleton and fe-separation removal; fit/input.rs) is built on the
+GPU, then `feols` / `fepois` run on the kept rows. `feols` and `fepois` take kept-sample arrays,
+FE codes per term, and a variance request in fixest's conventions (fit/vcov.rs:278-382).
 """
 import ctypes
 from dataclasses import dataclass, field
@@ -28,6 +29,9 @@ class FitResult:
     n_clusters: int
     se: np.ndarray = field(init=False)
     converged: bool = True
+    nobs: Optional[int] = None
+    obs: Optional[np.ndarray] = None
+    dropped: Optional[dict] = None
 
     def __post_init__(self):
         self.se = np.sqrt(np.clip(np.diag(self.vcov), 0.0, None))
@@ -153,6 +157,72 @@ def fepois(y, X, codes, g, weights=None, offset=None, vcov="hetero", cluster=Non
     weights (or the absorber's, when one is passed), `offset` enters the linear predictor."""
     return _fit("fh_ppml_fit", y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber,
                 [ctypes.c_double(tol), ctypes.c_uint64(maxiter)], offset=offset, ctx=ctx)
+
+
+def nested_by_value(fe_codes, cluster_codes):
+    """Every level of each FE term within one cluster of some dimension (fit/vcov.rs:115-140 by
+    value), on the host."""
+    out = []
+    for c in fe_codes:
+        c = np.asarray(c, np.int64)
+        g = int(c.max()) + 1 if len(c) else 0
+        ok = False
+        for k in cluster_codes:
+            k = np.asarray(k, np.int64)
+            lo = np.full(g, np.iinfo(np.int64).max)
+            hi = np.full(g, np.iinfo(np.int64).min)
+            np.minimum.at(lo, c, k)
+            np.maximum.at(hi, c, k)
+            if np.all(lo == hi):
+                ok = True
+                break
+        out.append(ok)
+    return out
+
+
+def fit(y, X, fe, model="ols", vcov="iid", cluster=(), weights=None, offset=None, ssc=None, fe_names=None,
+        cluster_names=None, tol=1e-8, maxiter=100, opts=None, ctx=None):
+    """fit::fit (fit/mod.rs:591) in fixest's flavour on raw columns: fe and cluster are lists of
+    terms (a factor column, or a tuple of columns for an interaction). Returns the FitResult of
+    the kept sample with `nobs`, `obs` (kept input rows) and `dropped` (fit::Dropped counts)."""
+    from .sample import Sample
+    X = np.asarray(X, float)
+    if X.ndim == 1:
+        X = X[:, None]
+    y = np.asarray(y, float)
+    fam = {"ols": "gaussian", "poisson": "poisson"}[model]
+    cl_terms = list(cluster) if vcov == "cluster" else []
+    with Sample(y=y, x=list(X.T), fe=fe, cluster=cl_terms, weights=weights, offset=offset, family=fam,
+                missing=True, drop_singletons=True, separation=fam != "gaussian", ctx=ctx) as smp:
+        kept = smp.kept()
+        terms = [smp.term(q) for q in range(smp.nterms)]
+        clusters = [smp.cluster(q)[0] for q in range(smp.nclusters)]
+        counts = dict(smp.counts)
+    codes, g = [c for c, _ in terms], [gq for _, gq in terms]
+    w = None if weights is None else np.asarray(weights, float)[kept]
+    nested = None
+    if vcov == "cluster":
+        fe_names = fe_names or [None] * len(codes)
+        cl_names = cluster_names or []
+        by_name = [nm is not None and (nm in cl_names or ("^" in nm and any(f in cl_names for f in nm.split("^"))))
+                   for nm in fe_names]
+        if all(by_name) or (cl_names and all(c in fe_names for c in cl_names)):
+            nested = by_name
+        else:
+            by_value = nested_by_value(codes, clusters)
+            nested = [a or b for a, b in zip(by_name, by_value)]
+    yk, Xk = y[kept], np.ascontiguousarray(X[kept])
+    if model == "ols":
+        res = feols(yk, Xk, codes, g, weights=w, vcov=vcov, cluster=clusters or None, nested=nested, ssc=ssc,
+                    opts=opts)
+    else:
+        off = None if offset is None else np.asarray(offset, float)[kept]
+        res = fepois(yk, Xk, codes, g, weights=w, offset=off, vcov=vcov, cluster=clusters or None, nested=nested,
+                     ssc=ssc, tol=tol, maxiter=maxiter, opts=opts, ctx=ctx)
+    res.nobs = int(counts["nobs"])
+    res.obs = kept
+    res.dropped = counts
+    return res
 
 
 @dataclass

@@ -124,3 +124,31 @@ def test_poisson_refit_on_one_absorber_is_bitwise_stable():
     r3 = F.fepois(y, x[:, None], codes, gs, vcov="hetero", tol=1e-10, opts=F.SolveOpts(tol=1e-12))
     np.testing.assert_allclose(r3.coef, r1.coef, rtol=1e-12, atol=0)
     ab.close()
+
+
+FIT_LAYER_CASES = [(c, "ols") for c in fixest_ols_cases(multiway=True)] + [(c, "poisson") for c in cases("fixest_pois.json")]
+
+
+@pytest.mark.parametrize("case,model", FIT_LAYER_CASES, ids=lambda c: c["name"] if isinstance(c, dict) else c)
+def test_fit_layer_from_raw_columns(case, model):
+    """fit::fit on the fixture's raw columns: the GPU sample (missing values, singletons,
+    separation) gives fixest's rows, then the fit gives its estimates."""
+    import fereg_amd as F
+    from fixtures import as_list, read_csv
+    d = read_csv(case["data"])
+    announce("fit layer", case["name"])
+    kw = dict(tol=1e-12) if model == "poisson" else {}
+    res = F.fit(d[case["y"]], np.stack([d[x] for x in as_list(case["x"])], 1),
+                fe=[tuple(d[f] for f in t["factors"]) for t in case["fe"]], model=model, vcov=case["vcov"],
+                cluster=[tuple(d[f] for f in c) for c in case["cluster"]],
+                weights=d[case["weights"]] if case.get("weights") else None,
+                offset=d[case["offset"]] if case.get("offset") else None, ssc=case["ssc"],
+                fe_names=["^".join(t["factors"]) for t in case["fe"]],
+                cluster_names=["^".join(c) for c in case["cluster"]], opts=F.SolveOpts(tol=1e-12), **kw)
+    e = case["expect"]
+    np.testing.assert_array_equal(res.obs, np.array(e["obs"], np.int64) - 1)
+    assert res.nobs == e["nobs"]
+    np.testing.assert_allclose(res.coef[res.kept], np.atleast_1d(e["coef"]), rtol=1e-8 if model == "ols" else 1e-7)
+    V = res.vcov[np.ix_(res.kept, res.kept)]
+    np.testing.assert_allclose(V.reshape(-1), np.array(e["vcov"], float).reshape(-1),
+                               rtol=1e-8 if model == "ols" else 1e-6)
