@@ -137,6 +137,7 @@ struct Absorber {
     // E as u8 counts (no prior weights), valid while *esm8_bad == 0 on the device
     DBuf<uint8_t> esm8;
     DBuf<uint32_t> esm8_bad;
+    bool esm8_ok = false;  // host copy of *esm8_bad == 0 after the last weight refresh
 
     // S*p plans per precision (FP32, FP64) and column class (r <= 1, 2, 4, 8, 12, 16); with
     // autotune the first S*p of a class times the candidates (autotune_spmv)

@@ -297,13 +297,29 @@ __device__ __forceinline__ void ld_quad(const T* __restrict__ src, T (&e)[4]) {
     }
 }
 
-// The same from the u8 copy of E (cell counts without prior weights) when use8. Only the
-// standalone D'p_S / C'p_S product reads it: in the fused level pass the second load path costs
-// registers that the latency-bound gathers need (measured slower there).
+// The same from the u8 copy of E (cell counts without prior weights) when use8 (a device flag:
+// the standalone D'p_S / C'p_S product); the fused partials pick their load path at compile time
+// (ld_quad_sel) from the host copy of that flag.
 template <class T>
 __device__ __forceinline__ void ld_quad_e(const T* __restrict__ E, const uint8_t* __restrict__ E8, bool use8,
                                           size_t idx, T (&e)[4]) {
     if (use8) {
+        const uint32_t v = *reinterpret_cast<const uint32_t*>(E8 + idx);
+        e[0] = T(v & 0xFFu);
+        e[1] = T((v >> 8) & 0xFFu);
+        e[2] = T((v >> 16) & 0xFFu);
+        e[3] = T(v >> 24);
+    } else {
+        ld_quad(E + idx, e);
+    }
+}
+
+// A quad of E from the u8 counts (U8) or from E itself, chosen at compile time so that the fused
+// level pass carries one load path only.
+template <class T, bool U8>
+__device__ __forceinline__ void ld_quad_sel(const T* __restrict__ E, const uint8_t* __restrict__ E8, size_t idx,
+                                            T (&e)[4]) {
+    if constexpr (U8) {
         const uint32_t v = *reinterpret_cast<const uint32_t*>(E8 + idx);
         e[0] = T(v & 0xFFu);
         e[1] = T((v >> 8) & 0xFFu);
@@ -367,8 +383,9 @@ __global__ __launch_bounds__(WG) void k_small_dense_t(const T* __restrict__ E, c
 // Partial sums over a column range of y_S[k] = sum_l C[k][l] p_L[l] - sum_s D[k][s] t_s for the
 // small levels k in chunk ch (KC levels), one column quad per thread and step. Workgroup bx of nwg
 // writes part[(k r + j) nwg + bx].
-template <class T, int RM, int KC>
-__device__ __forceinline__ void small_dense_body(const T* __restrict__ E, size_t ld, size_t coff, size_t g1, size_t mL,
+template <class T, int RM, int KC, bool U8>
+__device__ __forceinline__ void small_dense_body(const T* __restrict__ E, const uint8_t* __restrict__ E8, size_t ld,
+                                                 size_t coff, size_t g1, size_t mL,
                                                  const T* __restrict__ t, const T* __restrict__ pL, int ms, int r,
                                                  T* __restrict__ part, unsigned bx, unsigned nwg, unsigned ch) {
     __shared__ T red[WAVES][KC][RM];
@@ -397,7 +414,7 @@ __device__ __forceinline__ void small_dense_body(const T* __restrict__ E, size_t
             const int k = k0 + kk;
             if (k < ms) {
                 T e[4];
-                ld_quad(E + (size_t)k * ld + c4, e);
+                ld_quad_sel<T, U8>(E, E8, (size_t)k * ld + c4, e);
 #pragma unroll
                 for (int j = 0; j < RM; j++) acc[kk][j] += e[0] * v[0][j] + e[1] * v[1][j] + e[2] * v[2][j] + e[3] * v[3][j];
             }
@@ -426,8 +443,9 @@ __device__ __forceinline__ void small_dense_body(const T* __restrict__ E, size_t
 // columns each, so one chunk covers up to 12..32 small levels and t, p_L are read once (KC x RM
 // accumulators no longer fit one thread); the quad's E loads coincide across the four threads.
 // Lanes with the same column slice are reduced by shuffles, then the waves in order.
-template <class T, int RM, int KC>
-__device__ __forceinline__ void small_dense_cols(const T* __restrict__ E, size_t ld, size_t coff, size_t g1, size_t mL,
+template <class T, int RM, int KC, bool U8>
+__device__ __forceinline__ void small_dense_cols(const T* __restrict__ E, const uint8_t* __restrict__ E8, size_t ld,
+                                                 size_t coff, size_t g1, size_t mL,
                                                  const T* __restrict__ t, const T* __restrict__ pL, int ms, int r,
                                                  T* __restrict__ part, unsigned bx, unsigned nwg, unsigned ch) {
     constexpr int TPQ = 4, CW = RM / TPQ, QPB = WG / TPQ;
@@ -457,7 +475,7 @@ __device__ __forceinline__ void small_dense_cols(const T* __restrict__ E, size_t
             const int k = k0 + kk;
             if (k < ms) {
                 T e[4];
-                ld_quad(E + (size_t)k * ld + c4, e);
+                ld_quad_sel<T, U8>(E, E8, (size_t)k * ld + c4, e);
 #pragma unroll
                 for (int c = 0; c < CW; c++) acc[kk][c] += e[0] * v[0][c] + e[1] * v[1][c] + e[2] * v[2][c] + e[3] * v[3][c];
             }
@@ -485,14 +503,15 @@ __device__ __forceinline__ void small_dense_cols(const T* __restrict__ E, size_t
 }
 
 // Dense small-row partials, picking the layout by the column class.
-template <class T, int RM, int KC>
-__device__ __forceinline__ void small_dense_any(const T* __restrict__ E, size_t ld, size_t coff, size_t g1, size_t mL,
-                                                const T* __restrict__ t, const T* __restrict__ pL, int ms, int r,
-                                                T* __restrict__ part, unsigned bx, unsigned nwg, unsigned ch) {
+template <class T, int RM, int KC, bool U8>
+__device__ __forceinline__ void small_dense_any(const T* __restrict__ E, const uint8_t* __restrict__ E8, size_t ld,
+                                                size_t coff, size_t g1, size_t mL, const T* __restrict__ t,
+                                                const T* __restrict__ pL, int ms, int r, T* __restrict__ part,
+                                                unsigned bx, unsigned nwg, unsigned ch) {
     if constexpr (RM >= 8)
-        small_dense_cols<T, RM, KC>(E, ld, coff, g1, mL, t, pL, ms, r, part, bx, nwg, ch);
+        small_dense_cols<T, RM, KC, U8>(E, E8, ld, coff, g1, mL, t, pL, ms, r, part, bx, nwg, ch);
     else
-        small_dense_body<T, RM, KC>(E, ld, coff, g1, mL, t, pL, ms, r, part, bx, nwg, ch);
+        small_dense_body<T, RM, KC, U8>(E, E8, ld, coff, g1, mL, t, pL, ms, r, part, bx, nwg, ch);
 }
 
 // Levels per chunk of the dense small-row partials: KC x RM accumulators per thread (rows layout)
@@ -507,13 +526,13 @@ constexpr int small_chunk_any() {
     }
 }
 
-template <class T, int RM, int KC>
-__global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ E, size_t ld, size_t coff, size_t g1,
-                                                    size_t mL, const T* __restrict__ t, const T* __restrict__ pL,
-                                                    int ms, int r, T* __restrict__ part,
+template <class T, int RM, int KC, bool U8>
+__global__ __launch_bounds__(WG) void k_small_dense(const T* __restrict__ E, const uint8_t* __restrict__ E8, size_t ld,
+                                                    size_t coff, size_t g1, size_t mL, const T* __restrict__ t,
+                                                    const T* __restrict__ pL, int ms, int r, T* __restrict__ part,
                                                     const uint32_t* __restrict__ done) {
     if (done && *done) return;
-    small_dense_any<T, RM, KC>(E, ld, coff, g1, mL, t, pL, ms, r, part, blockIdx.x, gridDim.x, blockIdx.y);
+    small_dense_any<T, RM, KC, U8>(E, E8, ld, coff, g1, mL, t, pL, ms, r, part, blockIdx.x, gridDim.x, blockIdx.y);
 }
 
 // Last step of S*p: the small rows y_S[k][j] = nr_k p_S[k][j] + the sum of the partials of entry
@@ -717,17 +736,17 @@ __global__ __launch_bounds__(WG) void k_lvl_spmv(LvlArgs<T> A, const uint32_t* _
 // One launch for a level pass and the dense small-row partials: both read only p and t, so the
 // bandwidth-bound dense product runs beside the latency-bound gathers. Workgroups [0, nlvl) run
 // the level pass, the rest the dense chunks (nwg workgroups per chunk of KC levels).
-template <class T, int RM, int NO, int KC>
-__global__ __launch_bounds__(WG) void k_lvl_small(LvlArgs<T> A, unsigned nlvl, const T* __restrict__ E, size_t ld,
-                                                  size_t coff, size_t g1, size_t mL, const T* __restrict__ pL, int ms,
-                                                  T* __restrict__ part, unsigned nwg,
-                                                  const uint32_t* __restrict__ done) {
+template <class T, int RM, int NO, int KC, bool U8>
+__global__ __launch_bounds__(WG) void k_lvl_small(LvlArgs<T> A, unsigned nlvl, const T* __restrict__ E,
+                                                  const uint8_t* __restrict__ E8, size_t ld, size_t coff, size_t g1,
+                                                  size_t mL, const T* __restrict__ pL, int ms, T* __restrict__ part,
+                                                  unsigned nwg, const uint32_t* __restrict__ done) {
     if (done && *done) return;
     if (blockIdx.x < nlvl) {
         lvl_spmv_body<T, RM, NO>(A, blockIdx.x, nlvl);
     } else {
         const unsigned b = blockIdx.x - nlvl;
-        small_dense_any<T, RM, KC>(E, ld, coff, g1, mL, A.t, pL, ms, A.r, part, b % nwg, nwg, b / nwg);
+        small_dense_any<T, RM, KC, U8>(E, E8, ld, coff, g1, mL, A.t, pL, ms, A.r, part, b % nwg, nwg, b / nwg);
     }
 }
 
@@ -1224,6 +1243,7 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
     const T* E = a.small_dense ? W->esm.p : nullptr;
     const uint8_t* E8 = a.small_dense && !a.weighted ? a.esm8.p : nullptr;  // u8 counts when they fit
     const uint32_t* bad8 = a.small_dense ? a.esm8_bad.p : nullptr;
+    const bool u8 = E8 && a.esm8_ok;  // the fused partials read the u8 counts
     const bool dense = a.small_dense && plan.dense && nb <= 4 && r <= 4;
     if (dense) {
         s.ds.reserve(a.g1 * r);
@@ -1267,8 +1287,12 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
         FH_DISPATCH_R(r, with_kc(std::integral_constant<int, RM>{}, [&](auto kc) {
             constexpr int KC = decltype(kc)::value;
             const unsigned nch = (unsigned)((ms + KC - 1) / KC);
-            hipLaunchKernelGGL((k_small_dense<T, RM, KC>), dim3(nwg, nch), dim3(WG), 0, st, E, a.sm_ld, a.sm_coff,
-                               a.g1, mL, s.t.p, p + ms * r, (int)ms, r, s.part.p, done);
+            if (u8)
+                hipLaunchKernelGGL((k_small_dense<T, RM, KC, true>), dim3(nwg, nch), dim3(WG), 0, st, E, E8, a.sm_ld,
+                                   a.sm_coff, a.g1, mL, s.t.p, p + ms * r, (int)ms, r, s.part.p, done);
+            else
+                hipLaunchKernelGGL((k_small_dense<T, RM, KC, false>), dim3(nwg, nch), dim3(WG), 0, st, E, E8, a.sm_ld,
+                                   a.sm_coff, a.g1, mL, s.t.p, p + ms * r, (int)ms, r, s.part.p, done);
         }));
         FH_CHECK_LAUNCH();
     }
@@ -1315,8 +1339,14 @@ void spmv_impl(const Absorber& a, const SpmvPlan& plan, const T* p, T* y, int r,
                 with_kc(std::integral_constant<int, RM>{}, [&](auto kc) {
                     constexpr int KC = decltype(kc)::value;
                     const unsigned nch = (unsigned)((ms + KC - 1) / KC);
-                    hipLaunchKernelGGL((k_lvl_small<T, RM, NO, KC>), dim3(nlvl + nwg * nch), dim3(WG), 0, st, LA, nlvl,
-                                       E, a.sm_ld, a.sm_coff, a.g1, mL, p + ms * r, (int)ms, s.part.p, nwg, done);
+                    if (u8)
+                        hipLaunchKernelGGL((k_lvl_small<T, RM, NO, KC, true>), dim3(nlvl + nwg * nch), dim3(WG), 0, st,
+                                           LA, nlvl, E, E8, a.sm_ld, a.sm_coff, a.g1, mL, p + ms * r, (int)ms,
+                                           s.part.p, nwg, done);
+                    else
+                        hipLaunchKernelGGL((k_lvl_small<T, RM, NO, KC, false>), dim3(nlvl + nwg * nch), dim3(WG), 0,
+                                           st, LA, nlvl, E, E8, a.sm_ld, a.sm_coff, a.g1, mL, p + ms * r, (int)ms,
+                                           s.part.p, nwg, done);
                 });
             } else {
                 hipLaunchKernelGGL((k_lvl_spmv<T, RM, NO>), dim3(nlvl), dim3(WG), 0, st, LA, done);

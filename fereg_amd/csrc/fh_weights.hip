@@ -779,6 +779,13 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
                            a.w32.inv_diag.p, with_inv ? a.w64.minv.p : nullptr, with_inv ? a.w32.minv.p : nullptr);
         FH_CHECK_LAUNCH();
     }
+    // the u8 counts serve S*p when every count fit (one flag read per refresh, not per product)
+    a.esm8_ok = false;
+    if (a.small_dense && !a.weighted && a.esm8.n) {
+        uint32_t bad = 1;
+        download(&bad, a.esm8_bad.p, sizeof(bad), st);
+        a.esm8_ok = bad == 0;
+    }
 }
 
 }  // namespace fh
