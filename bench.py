@@ -83,7 +83,23 @@ def spmv_roofline(F, ab, r, reps, traffic_key=None):
         log("spmv r", r, name, "ms", ms.value, "MB", by.value / 1e6, "GB/s", by.value / ms.value / 1e6)
     ms, by = spmv["fp32"]
     achieved = by / (ms * 1e-3) / 1e9
+    # the passes' measured ceiling: one L2 gather per cell and pass (the large levels' p in the
+    # segment pass, an XCD bucket's segment sums in the level pass), timed by the calibration
+    # kernel at the same table sizes and entry width
+    info = ab.info
+    cells, large = int(info.ncells), int(info.m - info.m_small)
+    g1_bucket = max(1, int(info.g1) // 8)
+    cal = []
+    for levels in (large, g1_bucket):
+        t = ctypes.c_double()
+        F._lib.check(F.lib().fh_time_gather(max(levels, 1), cells, 4 * r, 20, ctypes.byref(t)))
+        cal.append(t.value)
+    l2 = {"ceiling_ms": sum(cal), "frac": sum(cal) / ms, "gathers_per_launch": 2 * cells,
+          "tables": [large, g1_bucket], "entry_bytes": 4 * r,
+          "source": "fh_time_gather: streamed random codes gathering an L2-resident table, one pass per S*p pass"}
+    log("spmv l2 gather ceiling ms", sum(cal), "frac", sum(cal) / ms)
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "measured_bound": "l2_requests", "l2": l2,
             "traffic": measured_traffic(traffic_key) if traffic_key else None,
             "traffic_source": "profiles/spmv_ols_n1e7_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE, calibrated "
                               "on a 1 GiB device copy)" if traffic_key else None,

@@ -231,6 +231,7 @@ SIGNATURES = [
     ("fh_absorber_apply", ctypes.c_int, [vp, f64p, ctypes.c_size_t, f64p, ctypes.c_int32]),
     ("fh_absorber_apply_noalloc", ctypes.c_int, [vp, vp, ctypes.c_size_t, vp, ctypes.c_int32]),
     ("fh_noalloc_scratch_bytes", ctypes.c_size_t, []),
+    ("fh_time_gather", ctypes.c_int, [ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int32, ctypes.c_uint32, f64p]),
     ("fh_absorber_time_spmv", ctypes.c_int, [vp, ctypes.c_size_t, ctypes.c_int32, ctypes.c_uint32, f64p, f64p]),
     ("fh_ols_fit", ctypes.c_int, [vp, f64p, f64p, ctypes.c_size_t, ctypes.c_int32, u32p, i32p, ctypes.POINTER(Ssc),
                                   ctypes.POINTER(SolveOptsC), f64p, f64p, i32p, ctypes.POINTER(FitStats)]),

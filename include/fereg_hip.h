@@ -168,6 +168,12 @@ size_t fh_noalloc_scratch_bytes(void);
 int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t reps, double* ms_per_spmv,
                           double* bytes_per_spmv);
 
+/* Calibration of the S*p passes' ceiling (no counterpart in the reference): the time of one pass
+ * that streams `count` random 32-bit codes below `levels` and gathers a table entry of entry_bytes
+ * (4, 8 or 16) per code from an L2-resident table -- the access pattern of the segment and level
+ * passes, whose limit is the L2's request rate, not HBM bandwidth. */
+int fh_time_gather(size_t levels, size_t count, int32_t entry_bytes, uint32_t reps, double* ms_per_pass);
+
 /* Launch plan of S*p for one precision and column count (no counterpart in the reference, whose
  * CPU operator has no launch shapes): the single small block as dense products inside the
  * segment and level passes (dense = 1) or gathered there (0), lanes per segment and lanes per
