@@ -759,6 +759,17 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
             require(weights_ok(a, wd.p), "weights must be positive and finite");
             wk = wd.p;
         }
+        // the terms' codes stay with the absorber (the bootstrap's non-nested projection)
+        a.codes_kept.resize(nterms);
+        for (size_t q = 0; q < nterms; q++) {
+            if (dev) {
+                a.codes_kept[q].alloc(n);
+                FH_HIP(hipMemcpyAsync(a.codes_kept[q].p, dc[q], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, a.st));
+            } else {
+                a.codes_kept[q] = std::move(up[q]);
+            }
+            dc[q] = a.codes_kept[q].p;
+        }
         auto t0 = std::chrono::steady_clock::now();
         build_structure(a, dc);
         log_line(ctx, 2, "fereg_hip: structure " +
@@ -1223,16 +1234,18 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
             if (kept[j]) rk.push_back(spec->r_mat[j]);
             else require(spec->r_mat[j] == 0.0, "the hypothesis involves a regressor dropped as collinear");
         }
-        // fixed effects not nested in the clusters: the reference projects them out exactly when
-        // their levels number at most min(G, 512) (or G <= 512); beyond that it skips the
-        // projection (ols.rs:392-410, mod.rs:959-997), which is what is done here
+        // fixed effects not nested in the clusters (wildboot/mod.rs:959-997): projected exactly
+        // when their levels number at most min(G, 512) (Path 1); the reference's other exact form
+        // (Path 2: G <= 512 clusters and more levels, one solve per cluster) is not provided;
+        // with more than 512 of both the projection is skipped, as the reference does
         std::vector<int> nest = terms_nested_in(a, pc);
         uint64_t m_nn = 0;
         for (size_t q = 0; q < a.nterms; q++)
             if (!nest[q]) m_nn += a.term_g[q];
-        require(m_nn == 0 || (m_nn > std::min<uint64_t>(G, 512) && G > 512),
-                "fixed effects not nested in small cluster designs need the exact non-nested projection, "
-                "which this build does not provide");
+        const bool path1 = m_nn > 0 && m_nn <= std::min<uint64_t>(G, 512);
+        require(m_nn == 0 || path1 || G > 512,
+                "fixed effects not nested in the clusters with more levels than clusters (<= 512) need the "
+                "per-cluster projection, which this build does not provide");
         WbootInput in;
         in.rows = a.rows_tmp.p;
         in.R = os.ld;
@@ -1247,8 +1260,10 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
         const double nn = (double)n, gg = (double)os.ncl;
         in.ssc = (ssc->k_adj ? (nn - 1.0) / (nn - os.dfk) : 1.0) * (ssc->g_adj && gg > 1.0 ? gg / (gg - 1.0) : 1.0);
         in.S = pre.ready ? pre.S.p : nullptr;
+        WbExt ext;
+        if (path1) wb_path1(a, nest, m_nn, in, rk, ext);
         ols_wildboot(a, in, *spec, rk.data(), *res, t_boot);
-        res->fe_not_projected = m_nn > 0;
+        res->fe_not_projected = m_nn > 0 && !path1;
     });
 }
 int fh_ols_wildboot(fh_absorber* ab, const double* y, const double* X, size_t K, const uint32_t* cluster,

@@ -88,6 +88,7 @@ struct Absorber {
     uint64_t n = 0, nterms = 0, g1 = 0, M = 0, nb = 0, m = 0, ms = 0, small_work = 0;
     int64_t fe1 = -1, norm_fe1 = -1;
     std::vector<uint64_t> term_g;
+    std::vector<DBuf<uint32_t>> codes_kept;  // the terms' codes, kept order (n each)
     std::vector<BlockInfo> blocks;
     uint64_t small_limit = 4096;
     int precision = FH_PLAIN_FP64;
@@ -350,6 +351,14 @@ void wb_constraint(const std::vector<double>& bread, int mk, const double* rk, s
 void wb_cluster_sums(Absorber& a, const double* rows, int R, const std::vector<int>& kept, const std::vector<double>& C,
                      const std::vector<double>& MR, const double* e, size_t es, const double* w, const uint32_t* cl,
                      uint64_t G, DBuf<double>& S);
+// Buffers of the bootstrap's non-nested projection (kept alive for the contraction).
+struct WbExt {
+    DBuf<double> rows, S;
+};
+// Fixed effects not nested in the clusters (m_nn levels): extends in (and rk) to the nested form
+// with the projection's pseudo-regressors appended.
+void wb_path1(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, WbootInput& in, std::vector<double>& rk,
+              WbExt& ext);
 // Non-empty clusters of the call's runs.
 uint64_t count_nonempty_clusters(Absorber& a, const uint32_t* cl, uint64_t G);
 void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* r_mat_kept,

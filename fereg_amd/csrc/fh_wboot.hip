@@ -20,7 +20,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "fh_internal.hpp"
+#include "fh_fit.hpp"
 
 namespace fh {
 
@@ -634,6 +634,41 @@ std::pair<double, double> invert_ci_search(double point, double se, double alpha
     return {lo, hi};
 }
 
+// rows_ext[i] = [rows[i] (ld columns), D_nn[i] (m columns)]: the indicator of row i's level in
+// each non-nested term q at column ld + off[q] + code (the level of kept row pi[i]).
+struct NnTerms {
+    const uint32_t* codes[8];
+    uint32_t off[8];
+    int nq;
+};
+__global__ void k_nn_rows(const double* __restrict__ rows, size_t n, int ld, const uint32_t* __restrict__ pi,
+                          NnTerms t, int m, double* __restrict__ out) {
+    const int le = ld + m;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double* o = out + i * le;
+        for (int c = 0; c < ld; c++) o[c] = rows[i * ld + c];
+        for (int c = 0; c < m; c++) o[ld + c] = 0.0;
+        const uint32_t k = pi[i];
+        for (int q = 0; q < t.nq; q++) o[ld + t.off[q] + t.codes[q][k]] = 1.0;
+    }
+}
+
+// The same indicator block alone, in another absorber's row order pi2 (n x m).
+__global__ void k_nn_dummies(size_t n, const uint32_t* __restrict__ pi2, NnTerms t, int m, double* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double* o = out + i * m;
+        for (int c = 0; c < m; c++) o[c] = 0.0;
+        const uint32_t k = pi2[i];
+        for (int q = 0; q < t.nq; q++) o[t.off[q] + t.codes[q][k]] = 1.0;
+    }
+}
+
+__global__ void k_to_kept(const double* __restrict__ v_rows, const uint32_t* __restrict__ pi, size_t n,
+                          double* __restrict__ v_kept) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        v_kept[pi[i]] = v_rows[i];
+}
+
 }  // namespace
 
 // Per absorbed term (absorber term order): 1 when every level lies in one cluster
@@ -743,7 +778,7 @@ void wb_cluster_sums(Absorber& a, const double* rows, int R, const std::vector<i
 void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* rk, fh_wboot_result& res,
                   double* t_boot_host) {
     hipStream_t st = a.st;
-    const size_t n = a.n, G = in.G;
+    const size_t G = in.G;
     const int mk = (int)in.kept.size();
     require(mk >= 1 && mk + 2 <= KMAX, "the bootstrap supports 1..19 kept regressors");
     require(G >= 2 && G < (1ull << 24), "the bootstrap needs 2 .. 2^24 clusters");
@@ -869,6 +904,120 @@ void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, 
     }
     if (t_boot_host) std::memcpy(t_boot_host, tb.data(), reps * sizeof(double));
     res.seconds_bootstrap = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void wb_path1(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, WbootInput& in, std::vector<double>& rk,
+              WbExt& ext) {
+    // Fixed effects not nested in the clusters (Path 1 of the reference, wildboot/mod.rs:959-1180;
+    // ols.rs:413-448). With D_nn the non-nested terms' dummies, D~ = M_nest D_nn their residuals
+    // on the nested terms and G_nn = D~' W D~, the FE-projected bootstrap residual of a draw is
+    // u* - D~ G_nn^+ D_nn' W u* (u* = V_g u~ by cluster), so the cell score of cluster c becomes
+    //   q[c, t] = V_c A_t[c] - B_c.(V Phi_t) - f_c.(V psi_t),
+    // f_c = sum_{i in c} w_i z_i D_nn[i] (= the same sum over D~: z is orthogonal to every FE and
+    // the nested terms' levels lie within clusters), psi_t[g] = G_nn^+ F_t[g],
+    // F_t[g] = sum_{i in g} w_i u_t,i D_nn[i]. That is the nested form with m_nn pseudo-regressors
+    // appended: S_t gets F_t, B gets f, the bread becomes blockdiag(bread, G_nn^+) and C, M_R,
+    // beta and R get zeros -- one more cluster-sum pass over [rows, D_nn] and the same contraction.
+    hipStream_t st = a.st;
+    const size_t n = a.n;
+    const int mk = (int)in.kept.size(), m = (int)m_nn;
+    require(mk + m + 2 <= KMAX,
+            "the bootstrap's non-nested projection supports kept regressors + non-nested levels <= 19");
+    NnTerms nt{};
+    uint32_t off = 0;
+    for (size_t q = 0; q < a.nterms; q++) {
+        if (nested[q]) continue;
+        require(nt.nq < 8, "at most 8 non-nested terms");
+        require(a.codes_kept.size() == a.nterms && a.codes_kept[q].p, "the absorber has no term codes", FH_INTERNAL);
+        nt.codes[nt.nq] = a.codes_kept[q].p;
+        nt.off[nt.nq++] = off;
+        off += (uint32_t)a.term_g[q];
+    }
+    // G_nn = D~' W D~ (weights: the absorber's, row order)
+    std::vector<double> Gnn((size_t)m * m);
+    bool any_nested = false;
+    for (size_t q = 0; q < a.nterms; q++) any_nested |= nested[q] != 0;
+    const int ld = in.R, le = ld + m;
+    ext.rows.alloc(n * le);
+    hipLaunchKernelGGL(k_nn_rows, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, in.rows, n, ld, a.pi.p, nt, m,
+                       ext.rows.p);
+    FH_CHECK_LAUNCH();
+    if (!any_nested) {
+        gram_cols(a, ext.rows.p, n, le, ld, m, in.w, nullptr, Gnn.data());
+    } else {
+        // residualize D_nn on the nested terms through an absorber of those terms alone
+        std::vector<fh_term> tt;
+        for (size_t q = 0; q < a.nterms; q++)
+            if (nested[q]) tt.push_back(fh_term{a.codes_kept[q].p, a.term_g[q], 1});
+        DBuf<double> wk;
+        if (in.w) {
+            wk.alloc(n);
+            hipLaunchKernelGGL(k_to_kept, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, in.w, a.pi.p, n, wk.p);
+            FH_CHECK_LAUNCH();
+        }
+        stream_wait(st);
+        fh_solver_config cfg;
+        fh_config_default(&cfg);
+        fh_absorber* tmp = nullptr;
+        if (fh_absorber_create_dev(tt.data(), tt.size(), in.w ? wk.p : nullptr, n, &cfg, nullptr, &tmp) != FH_OK)
+            throw Error(FH_INTERNAL, std::string("non-nested projection: ") + fh_last_error());
+        struct Tmp {
+            fh_absorber* p;
+            ~Tmp() { fh_absorber_destroy(p); }
+        } hold{tmp};
+        DBuf<double> dn(n * m);
+        hipLaunchKernelGGL(k_nn_dummies, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, n, tmp->a.pi.p, nt, m, dn.p);
+        FH_CHECK_LAUNCH();
+        stream_wait(st);
+        fh_solve_opts o{1e-12, 100000, 0, FH_PLAIN_FP64};
+        fh_solve_info info{};
+        if (fh_absorber_residualize_dev(tmp, dn.p, (size_t)m, &o, nullptr, nullptr, &info, nullptr) != FH_OK)
+            throw Error(FH_INTERNAL, std::string("non-nested projection: ") + fh_last_error());
+        {
+            AbsScope sc(tmp->a);
+            gram_cols(tmp->a, dn.p, n, m, 0, m, tmp->a.weighted ? tmp->a.w_rows.p : nullptr, nullptr, Gnn.data());
+        }
+    }
+    // G_nn^+ (wildboot/mod.rs:434-470: eigenvalues <= 1e-10 of the largest dropped)
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < i; j++) Gnn[(size_t)i * m + j] = Gnn[(size_t)j * m + i] = 0.5 * (Gnn[(size_t)i * m + j] + Gnn[(size_t)j * m + i]);
+    std::vector<double> lam, U, Gp((size_t)m * m, 0.0);
+    sym_eigen(Gnn, m, lam, U);
+    double lmax = 0.0;
+    for (double l : lam) lmax = std::max(lmax, std::fabs(l));
+    for (int e = 0; e < m; e++) {
+        if (!(lam[e] > 1e-10 * lmax) || lmax == 0.0) continue;
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) Gp[(size_t)i * m + j] += U[(size_t)i * m + e] * U[(size_t)j * m + e] / lam[e];
+    }
+    // the extended problem
+    const int mk2 = mk + m;
+    std::vector<double> bread2((size_t)mk2 * mk2, 0.0);
+    for (int i = 0; i < mk; i++)
+        for (int j = 0; j < mk; j++) bread2[(size_t)i * mk2 + j] = in.bread[(size_t)i * mk + j];
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) bread2[(size_t)(mk + i) * mk2 + mk + j] = Gp[(size_t)i * m + j];
+    std::vector<int> kept2(in.kept);
+    for (int j = 0; j < m; j++) kept2.push_back(ld - 1 + j);
+    rk.resize(mk2, 0.0);
+    std::vector<double> C2, MR2;
+    wb_constraint(bread2, mk2, rk.data(), C2, MR2);
+    // the residual: its own array, or the same column of the extended rows
+    const double* e2 = in.e;
+    size_t es2 = in.es;
+    if (in.es == (size_t)ld && in.e >= in.rows && in.e < in.rows + ld) {
+        e2 = ext.rows.p + (in.e - in.rows);
+        es2 = (size_t)le;
+    }
+    wb_cluster_sums(a, ext.rows.p, le, kept2, C2, MR2, e2, es2, in.w, in.cl, in.G, ext.S);
+    in.rows = ext.rows.p;
+    in.R = le;
+    in.kept = kept2;
+    in.bread = bread2;
+    in.beta.resize(mk2, 0.0);
+    in.e = e2;
+    in.es = es2;
+    in.S = ext.S.p;
 }
 
 }  // namespace fh

@@ -8,8 +8,8 @@ Restates crates/fereg-core/src/wildboot/:
   weights.rs:65-80      Rademacher signs: bit j of the successive 64-bit draws, 0 -> +1, 1 -> -1
   weights.rs:39-61,118  full enumeration when G <= 20 and 2^G <= reps (draw b: bit c of b)
   ols.rs:76-121         ConstraintProj: C = bread R', M_R = C (R bread R')^+
-  ols.rs:243-649        run_cluster_11, cells = clusters (err_equals_boot), no non-nested
-                        FE projection (NonNestedPlan::None / TooLarge), affine CI targets
+  ols.rs:243-649        run_cluster_11, cells = clusters (err_equals_boot), the non-nested FE
+                        projection of Path 1 (mod.rs:959-1180) or none (None / TooLarge), affine CI targets
                         {u_hat, -X M_R}: per draw num = V.R Phi, db = V.Phi,
                         q_cell[c, t] = V_c A_c[t] - B_c . db_t, v = ssc sum_c q q'
   ols.rs:139-142        eval_affine_t
@@ -225,11 +225,16 @@ def invert_ci_search(point, se, alpha, pval_type, tol, maxiter, reps_feas, eval_
 
 
 def wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="two-tailed", conf_int=True,
-          tol=1e-6, maxiter=50):
-    """run_cluster_11 (ols.rs:243-649) with cells = clusters and no non-nested FE projection.
+          tol=1e-6, maxiter=50, dnn=None, dtilde=None):
+    """run_cluster_11 (ols.rs:243-649) with cells = clusters.
 
     xt: n x k residualized regressors (kept columns), u: residuals, w: weights or None, cl: cluster
     codes 0..G-1 (row order), bread = (X~'WX~)^-1, R: k-vector, r: tested value, ssc: CRV1 factor.
+    dnn / dtilde: the dummies of the fixed effects not nested in the clusters and their residuals
+    on the nested ones (n x m): the exact projection of Path 1 (wildboot/mod.rs:959-1180,
+    ols.rs:413-448): the FE-projected bootstrap residual is u* - D~ G^+ D_nn' W u*, G = D~'WD~
+    (pseudo-inverse: eigenvalues <= 1e-10 of the largest dropped, mod.rs:434-470), so each cell
+    score loses f_c . (V psi), f_c = sum_c w z D_nn, psi[g] = G^+ sum_g w u_t D_nn.
     Returns a dict with stat, se, point, p_val, ci, t_boot, coefs (reps x 5), full.
     """
     n, k = xt.shape
@@ -250,6 +255,19 @@ def wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="
     A0 = Phi0 @ R
     A1 = Phi1 @ R
     q_obs = np.bincount(cl, weights=wv * z * u0, minlength=G)
+    if dnn is not None:
+        Gn = dtilde.T @ (wv[:, None] * dtilde)
+        Gn = 0.5 * (Gn + Gn.T)
+        lam, U = np.linalg.eigh(Gn)
+        lmax = np.max(np.abs(lam)) if lam.size else 0.0
+        keep = lam > 1e-10 * lmax
+        Gp = (U[:, keep] / lam[keep]) @ U[:, keep].T
+        F0 = cluster_sums(cl, G, (wv * u0)[:, None] * dnn)
+        F1 = cluster_sums(cl, G, (wv * u1)[:, None] * dnn)
+        fz = cluster_sums(cl, G, (wv * z)[:, None] * dnn)
+        Phi0 = np.concatenate([Phi0, F0 @ Gp], axis=1)
+        Phi1 = np.concatenate([Phi1, F1 @ Gp], axis=1)
+        Bc = np.concatenate([Bc, fz], axis=1)
     V, full = rademacher(seed, reps, G)
     # per draw: explicit cell scores (no expansion)
     num0 = V @ A0
@@ -266,9 +284,9 @@ def wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="
 
 
 def wcr11_result(beta, xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="two-tailed",
-                 conf_int=True, tol=1e-6, maxiter=50):
+                 conf_int=True, tol=1e-6, maxiter=50, dnn=None, dtilde=None):
     """The observed statistic, the bootstrap t's, the p-value and the test-inverted interval."""
-    core = wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed)
+    core = wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, dnn=dnn, dtilde=dtilde)
     num_obs = float(R @ beta) - r
     point = num_obs + r
     v_obs = ssc * float(core["q_obs"] @ core["q_obs"])

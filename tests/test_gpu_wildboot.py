@@ -8,7 +8,8 @@ import pytest
 from fixtures import dense_residualize
 from gpu_util import announce
 from oracle import wildboot as wb
-from test_wildboot_oracle import fit_fwl, nested_design
+from fixtures import dummies
+from test_wildboot_oracle import fit_fwl, nested_design, non_nested_design, path1_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -80,14 +81,61 @@ def test_wcr11_large_non_nested_fe_skips_projection():
     check(res, ref)
 
 
-def test_wcr11_small_non_nested_design_is_refused():
+@pytest.mark.parametrize("with_firm", [True, False])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_wcr11_non_nested_projection(with_firm, weighted):
+    """Path 1 (wildboot/mod.rs:959-1180): a year FE crossing the firm clusters is projected out of
+    every draw exactly -- against the oracle's Path 1 and the brute-force refits."""
+    import fereg_amd as F
+    G, T, reps, r = 30, 6, 199, 0.2
+    R = np.array([1.0, 0.0])
+    y, X, cl, year, codes, gs, w = non_nested_design(900, G, T, 2, weighted, seed=5 + weighted, with_firm=with_firm)
+    announce("wcr11 non-nested path 1", with_firm, weighted)
+    ab = F.Absorber(codes, gs, weights=w)
+    pi = ab.row_order()
+    res = F.wildboot(y, X, cl.astype(np.uint32), R, r=r, reps=reps, seed=99, opts=F.SolveOpts(tol=1e-13),
+                     absorber=ab)
+    ab.close()
+    assert not res.fe_not_projected
+    yo, Xo, clo = y[pi], X[pi], cl[pi]
+    wo = None if w is None else w[pi]
+    Xt, u, beta, bread = fit_fwl(yo, Xo, [c[pi] for c in codes], gs, wo)
+    dnn, dt = path1_inputs(year[pi], T, clo, G, wo, with_firm)
+    ref = wb.wcr11_result(beta, Xt, u, wo, clo, G, bread, R, r, res.ssc, reps, 99, dnn=dnn, dtilde=dt)
+    check(res, ref)
+    D = np.concatenate([dummies(c, g) for c, g in zip(codes, gs)], axis=1)
+    stat, se, tb = wb.brute_t_boot(X, y, w, D, cl, G, R, r, reps, 99, res.ssc)
+    np.testing.assert_allclose(res.t_boot, tb, rtol=1e-7, atol=1e-8)
+
+
+def test_wcr11_non_nested_projection_many_clusters():
+    """firm + year, cluster(firm) with 3000 firms: Path 1 at a realistic shape."""
+    import fereg_amd as F
+    G, T, reps, r = 3000, 10, 999, 0.0
+    R = np.array([0.0, 1.0])
+    y, X, cl, year, codes, gs, w = non_nested_design(60_000, G, T, 2, False, seed=31)
+    announce("wcr11 non-nested path 1, G 3000")
+    ab = F.Absorber(codes, gs)
+    pi = ab.row_order()
+    res = F.wildboot(y, X, cl.astype(np.uint32), R, r=r, reps=reps, seed=5, opts=F.SolveOpts(tol=1e-13), absorber=ab)
+    ab.close()
+    assert not res.fe_not_projected
+    yo, Xo, clo = y[pi], X[pi], cl[pi]
+    Xt, u, beta, bread = fit_fwl(yo, Xo, [c[pi] for c in codes], gs, None)
+    dnn, dt = path1_inputs(year[pi], T, clo, G, None, True)
+    ref = wb.wcr11_result(beta, Xt, u, None, clo, G, bread, R, r, res.ssc, reps, 5, dnn=dnn, dtilde=dt)
+    check(res, ref)
+
+
+def test_wcr11_per_cluster_projection_is_refused():
+    """More non-nested levels than clusters (<= 512): the reference's Path 2, not provided."""
     import fereg_amd as F
     rng = np.random.default_rng(2)
-    n, G = 500, 30
+    n, G, T = 800, 30, 40
     cl = rng.integers(0, G, n).astype(np.uint32)
-    year = rng.integers(0, 5, n).astype(np.uint32)
+    year = rng.integers(0, T, n).astype(np.uint32)
     X = rng.standard_normal((n, 1))
     y = X[:, 0] + rng.standard_normal(n)
     with pytest.raises(F.FeregError) as e:
-        F.wildboot(y, X, cl, [1.0], reps=99, codes=[year], g=[5])
+        F.wildboot(y, X, cl, [1.0], reps=99, codes=[year], g=[T])
     assert e.value.rc == 198
