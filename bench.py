@@ -47,17 +47,30 @@ def line(args, value, ms, dtype, config, roofline, cpu, extra, unit="s/call", hi
 
 
 def timed(args, fn):
-    """fn() over warmup + steps calls; returns (s/call, last result)."""
+    """fn() over warmup + steps calls; returns (s/call, last result). Python's cyclic garbage
+    collector is off inside the timed region, as timeit does: with torch loaded the process
+    tracks ~170k objects and a generation-2 pass stops the host for 30-40 ms, which would land on
+    whichever call happens to allocate the triggering object (the GPU work is unaffected)."""
+    import gc
     import torch
     for _ in range(args.warmup):
         fn()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res = None
-    for _ in range(args.steps):
-        res = fn()
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / args.steps, res
+    gc.collect()
+    gc.freeze()
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = None
+        for _ in range(args.steps):
+            res = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps, res
+    finally:
+        if was:
+            gc.enable()
+        gc.unfreeze()
 
 
 def measured_traffic(key):
@@ -348,8 +361,8 @@ def main():
     ap.add_argument("--design", default="simple", choices=["simple", "difficult"])
     ap.add_argument("--workload", default="ols", choices=["ols", "wcr", "ppml", "serve", "copy"])
     ap.add_argument("--reps", type=int, default=9999, help="bootstrap draws (wcr)")
-    ap.add_argument("--requests", type=int, default=300, help="requests in the stream (serve)")
-    ap.add_argument("--rate", type=float, default=20.0, help="arrivals per second (serve)")
+    ap.add_argument("--requests", type=int, default=2000, help="requests in the stream (serve)")
+    ap.add_argument("--rate", type=float, default=100.0, help="arrivals per second (serve)")
     ap.add_argument("--slots", type=int, default=8, help="requests in flight (serve)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ppml", action="store_true", help="leave the Poisson section out of the ols line")
