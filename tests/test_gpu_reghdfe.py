@@ -18,11 +18,11 @@ DOF_NAMES = ["firstpair", "pairwise", "clusters", "continuous"]
 
 @pytest.mark.parametrize("case", cases("reghdfe_ols.json"), ids=lambda c: c["name"])
 def test_reghdfe_fixtures(case):
-    import fereg_amd.reghdfe as R
+    from fereg_amd import reghdfe
     d = read_csv("ols_data.csv")
     w = case["weight"]
     announce("reghdfe", case["name"])
-    res = R.reghdfe(d[case["y"]], np.stack([d[x] for x in case["x"]], 1),
+    res = reghdfe(d[case["y"]], np.stack([d[x] for x in case["x"]], 1),
                     absorb=[tuple(d[f] for f in t) for t in case["absorb"]], vce=vce_of(case),
                     cluster=[tuple(d[f] for f in t) for t in case["cluster"]], weights=d[w[0]] if w else None,
                     weight_kind=w[1] if w else None, noconstant=case["noconstant"], singletons=case["singletons"],
@@ -36,7 +36,7 @@ def test_reghdfe_fixtures(case):
 @pytest.mark.parametrize("kind,vce", [(None, "cluster"), ("aweight", "robust"), ("fweight", "cluster"),
                                       ("pweight", "unadjusted")])
 def test_reghdfe_matches_the_oracle_at_scale(kind, vce):
-    import fereg_amd.reghdfe as R
+    from fereg_amd import reghdfe
     rng = np.random.default_rng(17)
     n = 200_000
     gi, gf, gt = 30_000, 2_000, 15
@@ -50,7 +50,7 @@ def test_reghdfe_matches_the_oracle_at_scale(kind, vce):
     w = None if kind is None else (rng.integers(1, 4, n).astype(float) if kind == "fweight" else rng.uniform(0.5, 2, n))
     cl = [(region,), (t,)] if vce == "cluster" else []
     announce("reghdfe at scale", kind, vce)
-    res = R.reghdfe(y, X, absorb=[i.astype(float), f.astype(float), t.astype(float)], vce=vce,
+    res = reghdfe(y, X, absorb=[i.astype(float), f.astype(float), t.astype(float)], vce=vce,
                     cluster=[tuple(c.astype(float) for c in dim) for dim in cl], weights=w, weight_kind=kind,
                     tol=1e-12)
     smp = S.build(n, y=y, x=list(X.T), weights=w, weight_kind=kind,
