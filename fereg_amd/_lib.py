@@ -39,7 +39,8 @@ class Term(ctypes.Structure):
 
 class SolverConfig(ctypes.Structure):
     _fields_ = [("small_limit", ctypes.c_uint64), ("precision", ctypes.c_int32), ("bucket_log2", ctypes.c_int32),
-                ("autotune", ctypes.c_int32), ("mixed_lead", ctypes.c_int32)]
+                ("autotune", ctypes.c_int32), ("mixed_lead", ctypes.c_int32), ("sparse_chol", ctypes.c_int32),
+                ("joint", ctypes.c_int32)]
 
 
 class SpmvPlan(ctypes.Structure):
@@ -296,6 +297,10 @@ SIGNATURES = [
     ("fh_server_submit", ctypes.c_int, [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
     ("fh_server_wait", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(Response)]),
     ("fh_server_destroy", None, [vp]),
+    ("fh_sparse_chol_factor", ctypes.c_int, [ctypes.c_size_t, u64p, u32p, f64p, ctypes.c_size_t, ctypes.POINTER(vp)]),
+    ("fh_sparse_chol_info", ctypes.c_int, [vp, u64p, f64p, u64p]),
+    ("fh_sparse_chol_solve", ctypes.c_int, [vp, f64p, ctypes.c_size_t]),
+    ("fh_sparse_chol_destroy", None, [vp]),
     ("fh_device_alloc", ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),
     ("fh_device_free", ctypes.c_int, [vp]),
     ("fh_memcpy_h2d", ctypes.c_int, [vp, vp, ctypes.c_size_t]),

@@ -114,7 +114,7 @@ class Absorber:
     """
 
     def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, bucket_log2=-1,
-                 autotune=False, verbose=0, mixed_lead=-1):
+                 autotune=False, verbose=0, mixed_lead=-1, sparse_chol=True, joint=True):
         lib = L.lib()
         self._keep = []
         n = None
@@ -144,7 +144,8 @@ class Absorber:
                 raise L.FeregError(L.FH_INVALID, "weight length differs from the kept sample")
         if n is None:
             raise L.FeregError(L.FH_INVALID, "no terms and no weights: the sample size is unknown")
-        cfg = L.SolverConfig(small_limit, precision, bucket_log2, int(bool(autotune)), int(mixed_lead))
+        cfg = L.SolverConfig(small_limit, precision, bucket_log2, int(bool(autotune)), int(mixed_lead),
+                             int(bool(sparse_chol)), int(bool(joint)))
         h = ctypes.c_void_p()
         if dev:
             wptr = None

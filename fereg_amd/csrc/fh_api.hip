@@ -348,13 +348,13 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
         };
         PcgResult pr = solve(warm, escalation_possible(a), maxiter);
         if (pr.escalate) {
-            // slow convergence at the probe: the first large block gets the exact block
-            // preconditioner and the solve restarts from the current iterate (cg.rs:264-274)
-            escalate(a);
+            // slow convergence at the probe: the reduced matrix (joint, else per large block) is
+            // factored exactly and the solve restarts from the current iterate (cg.rs:264-274);
+            // with no factor within the budgets it restarts on Jacobi
             const uint64_t used = pr.iterations;
+            out.escalated = escalate(a, ctx);
             pr = solve(true, false, maxiter > used ? maxiter - used : 1);
             pr.iterations += used;
-            out.escalated = true;
         }
         if (pr.rc == FH_CONVERGENCE)
             throw Error(FH_CONVERGENCE, "the FE solver did not converge within " + std::to_string(maxiter) + " iterations");
@@ -677,6 +677,8 @@ void fh_config_default(fh_solver_config* cfg) {
     cfg->bucket_log2 = -1;
     cfg->autotune = 0;
     cfg->mixed_lead = -1;
+    cfg->sparse_chol = 1;
+    cfg->joint = 1;
 }
 
 static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept, size_t n, const fh_solver_config* cfg,
@@ -718,6 +720,10 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         a.autotune = c.autotune == 1;
         require(c.mixed_lead >= -1 && c.mixed_lead <= 8, "mixed_lead must be in -1..8");
         a.mixed_lead = c.mixed_lead < 0 ? MIXED_LEAD_DEFAULT : (uint32_t)c.mixed_lead;
+        require(c.sparse_chol == 0 || c.sparse_chol == 1, "sparse_chol must be 0 or 1");
+        require(c.joint == 0 || c.joint == 1, "joint must be 0 or 1");
+        a.sparse_chol = c.sparse_chol == 1;
+        a.joint_policy = c.joint;
         std::vector<DBuf<uint32_t>> up(nterms);
         std::vector<const uint32_t*> dc(nterms);
         for (size_t q = 0; q < nterms; q++) {

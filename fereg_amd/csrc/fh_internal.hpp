@@ -80,6 +80,7 @@ struct SpmvPlan {
 };
 
 struct ClusterRuns;
+struct FacGpu;  // fh_escalate.hip
 
 struct Absorber {
     int device = 0;
@@ -153,7 +154,14 @@ struct Absorber {
     // that uses one clustering more than once)
     const ClusterRuns* cl_runs = nullptr;
 
-    bool escalated = false;  // a large block joined the small super-block (escalation)
+    // escalated preconditioner (fh_escalate.hip): exact sparse factors of the joint S or of the
+    // large blocks, with the solver knobs of SolverConfig::{sparse_chol, joint}
+    std::vector<std::shared_ptr<FacGpu>> facs;
+    bool fac_joint = false;   // one factor over every coordinate
+    bool fac_fresh = false;   // built for the current weights
+    bool fac_failed = false;  // no factor fit the budgets: Jacobi for good
+    bool sparse_chol = true;
+    int joint_policy = 1;     // 0 off, 1 on escalation
     uint32_t mixed_lead = MIXED_LEAD_DEFAULT;  // most FP64 iterations of a mixed solve before its FP32 sweeps
 
     bool trivial() const { return cell_obs_off.n == 0; }
@@ -182,14 +190,19 @@ inline bool small_dense_rule(const Absorber& a) {
 }
 
 // ---------------------------------------------------------------- weights (fh_weights.hip)
-// Escalation (solve.rs:838-891, cg.rs:264-274): when the PCG converges slowly (rho_hat > 0.6 at
-// its probe) and one large block remains, it joins the small super-block, whose preconditioner is
-// then the exact grounded inverse of S (built on the GPU by pivot sweeps, O(ms) passes over ms^2
-// values), while the super-block stays within ESC_MAX coordinates; once per absorber. Other
-// designs keep Jacobi (the reference's sparse Cholesky of a large block is not built).
-constexpr uint64_t ESC_MAX = 8192;
+// Escalation (solve.rs:685-891, cg.rs:254-274; fh_escalate.hip): when the PCG converges slowly
+// (rho_hat > 0.6 at its probe) the reduced matrix is factored exactly: the joint S over every
+// coordinate within the budgets (M = S: the restart is exact), else each large block S_bb. The
+// factors stay with the absorber; after a weight refresh they remain the preconditioner (stale,
+// so not exact) while their kept coordinates are unchanged, and the next escalation rebuilds them.
 bool escalation_possible(const Absorber& a);
-void escalate(Absorber& a);
+bool escalate(Absorber& a, const fh_ctx* ctx);  // false: no factor within the budgets (Jacobi stays)
+void facs_after_refresh(Absorber& a);
+bool facs_exact(const Absorber& a);  // a joint factor for the current weights: M = S
+// z = M^-1 r on the factored coordinates (FP64, m x r rows) and the partial r'z per factor in
+// rzf_part (SPMV_DOTG x 16 entries, each factor its own share of them; zeroed by the caller).
+void facs_apply(Absorber& a, const double* rr, double* z, int r, double* rzf_part, const uint32_t* done,
+                hipStream_t st);
 // The sum of the weights on the host (one readback after a refresh, none when a residualization
 // has already brought it back).
 double absorber_sw(Absorber& a);

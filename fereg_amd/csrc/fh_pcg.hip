@@ -41,12 +41,13 @@ struct PcgState {
     uint64_t iters, k, maxiter;
     int32_t r;
     int32_t can_escalate;
+    int32_t fac;  // escalated factors write their r'z partials to slot S_RZF
     double tol, rel2;
     uint64_t handover_at;  // > 0: stop with err = 3 after this iteration while columns remain
 };
 
-enum Slot { S_PQ = 0, S_XB = 1, S_XR = 2, S_RZL = 3, S_RZS = 4, NSLOT = 5 };
-constexpr int NSLOT_MAX = 4;
+enum Slot { S_PQ = 0, S_XB = 1, S_XR = 2, S_RZL = 3, S_RZS = 4, S_RZF = 5, NSLOT = 6 };
+constexpr int NSLOT_MAX = 5;
 
 // Per-workgroup partial dots: dpart[(slot*DOTG + wg)*RMAX + j] over rows [row0, row1).
 template <class T>
@@ -160,11 +161,11 @@ __global__ void k_add_to64(double* __restrict__ x, const TI* __restrict__ d, siz
 // Start of a (re)start: rho = r'z, inactive when rho <= 0 (cg.rs:199-205).
 __global__ __launch_bounds__(TPB) void k_start(PcgState* S, const double* __restrict__ dpart) {
     __shared__ double sums[NSLOT_MAX][RMAX];
-    const int sl[2] = {S_RZL, S_RZS};
-    slot_sums(dpart, sl, 2, S->r, sums);
+    const int sl[3] = {S_RZL, S_RZS, S_RZF};
+    slot_sums(dpart, sl, S->fac ? 3 : 2, S->r, sums);
     const int j = threadIdx.x;
     if (j < S->r) {
-        double rho = sums[0][j] + sums[1][j];
+        double rho = sums[0][j] + sums[1][j] + (S->fac ? sums[2][j] : 0.0);
         S->rho[j] = rho;
         S->rho0[j] = rho;
         if (!(rho > 0.0)) S->active[j] = 0;
@@ -293,8 +294,8 @@ __global__ __launch_bounds__(TPB) void k_update(T* __restrict__ x, T* __restrict
 __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restrict__ dpart) {
     if (S->done) return;
     __shared__ double sums[NSLOT_MAX][RMAX];
-    const int sl[4] = {S_RZL, S_RZS, S_XB, S_XR};
-    slot_sums(dpart, sl, 4, S->r, sums);
+    const int sl[5] = {S_RZL, S_RZS, S_XB, S_XR, S_RZF};
+    slot_sums(dpart, sl, S->fac ? 5 : 4, S->r, sums);
     const int j = threadIdx.x;
     __shared__ int any, not_ready, soon;
     if (j == 0) {
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restr
     __syncthreads();
     const uint64_t k = S->k + 1;
     if (j < S->r) {
-        const double rn = sums[0][j] + sums[1][j];
+        const double rn = sums[0][j] + sums[1][j] + (S->fac ? sums[4][j] : 0.0);
         const double rho_prev = S->rho[j];
         double beta = 0.0;
         if (S->active[j]) {
@@ -433,6 +434,16 @@ __global__ __launch_bounds__(TPB) void k_precond(const T* __restrict__ inv_diag,
     }
 }
 
+// The escalated factors' part of z = M^-1 r (FP64 only: an absorber with factors solves in FP64).
+template <class T>
+void precond_factors(Absorber& a, const T* rr, T* z, int r, double* dpart, const uint32_t* done, hipStream_t st) {
+    if (a.facs.empty()) return;
+    if constexpr (std::is_same<T, double>::value)
+        facs_apply(a, rr, z, r, dpart + (size_t)S_RZF * DOTG * RMAX, done, st);
+    else
+        throw Error(FH_INTERNAL, "escalated factors are applied in FP64");
+}
+
 // z = M^-1 r and the partial r'z of both parts.
 template <class T>
 void precond(Absorber& a, const T* rr, T* z, int r, double* dpart, const uint32_t* done, hipStream_t st) {
@@ -441,6 +452,7 @@ void precond(Absorber& a, const T* rr, T* z, int r, double* dpart, const uint32_
     if (!done && ms <= (size_t)SMALL_FUSE_MAX && ms * (size_t)r <= (size_t)TPB) {
         hipLaunchKernelGGL(k_precond<T>, dim3(DOTG), dim3(TPB), 0, st, W.inv_diag.p, W.minv.p, rr, z, ms, m, r, dpart);
         FH_CHECK_LAUNCH();
+        precond_factors(a, rr, z, r, dpart, done, st);
         return;
     }
     if (m > ms) {
@@ -452,6 +464,7 @@ void precond(Absorber& a, const T* rr, T* z, int r, double* dpart, const uint32_
         hipLaunchKernelGGL(k_dot<T>, dim3(DOTG), dim3(TPB), 0, st, rr, z, (size_t)0, ms, r, dpart, (int)S_RZS, done);
     }
     FH_CHECK_LAUNCH();
+    precond_factors(a, rr, z, r, dpart, done, st);
 }
 
 struct PcgRun {
@@ -482,6 +495,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     h.rel2 = rel2;
     h.maxiter = maxiter;
     h.can_escalate = can_escalate ? 1 : 0;
+    h.fac = a.facs.empty() ? 0 : 1;
     h.handover_at = handover_at;
     for (int j = 0; j < r; j++) {
         h.ss1[j] = ss1[j];
@@ -503,8 +517,9 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     FH_CHECK_LAUNCH();
     precond<T>(a, s.r.p, s.z.p, r, s.dpart.p, nullptr, st);
     PcgRun out;
-    if (ms == m && a.nkeep > 0) {
-        // M = S exactly: x += M^-1 (rhs - S x) on the active columns (cg.rs:171-198)
+    if ((ms == m && a.nkeep > 0) || facs_exact(a)) {
+        // M = S exactly (the dense small factor covers S, or a joint factor for the current
+        // weights): x += M^-1 (rhs - S x) on the active columns (cg.rs:171-198)
         DBuf<int32_t> mask(r);
         std::vector<int32_t> hm(h.active, h.active + r);
         upload(mask.p, hm.data(), r * sizeof(int32_t), st);
@@ -540,6 +555,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
                                        s.dpart.p, (int)S_RZS, done);
                 }
             }
+            precond_factors(a, (const T*)s.r.p, s.z.p, r, s.dpart.p, done, st);
             hipLaunchKernelGGL(k_stop, dim3(1), dim3(TPB), 0, st, S, s.dpart.p);
             hipLaunchKernelGGL(k_update_p<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, s.p.p, s.z.p, nv, r, S);
             FH_CHECK_LAUNCH();
@@ -651,9 +667,10 @@ PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, bool x_maybe_nonze
 PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
                     const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate) {
     PcgResult res;
-    // exact dense path (nothing to iterate), or an escalated exact block preconditioner, whose
-    // grounded inverse of a slowly mixing block is too ill-conditioned for FP32 sweeps
-    if (a.ms == a.m || a.escalated) return pcg_fp64(a, rhs, x, x_maybe_nonzero, r, ss1, ss0, tol, maxiter, ctx, false);
+    // exact dense path (nothing to iterate), or escalated sparse factors, whose grounded inverse of
+    // a slowly mixing system is too ill-conditioned for FP32 sweeps (and is applied in FP64 only)
+    if (a.ms == a.m || !a.facs.empty())
+        return pcg_fp64(a, rhs, x, x_maybe_nonzero, r, ss1, ss0, tol, maxiter, ctx, false);
     ensure_w32(a);
     hipStream_t st = a.st;
     const size_t m = a.m, nv = m * r;

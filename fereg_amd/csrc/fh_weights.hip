@@ -347,18 +347,6 @@ inline unsigned seg_sums_grid(const Absorber& a) {
     return grid_for(a.g1, (TPB / WAVE) * (WAVE / seg_sums_lanes(a)), 1u << 16);
 }
 
-double sum_dev(const double* v, size_t n, hipStream_t st) {
-    const unsigned nb = 256;
-    DBuf<double> part(nb);
-    hipLaunchKernelGGL(k_partial_sum, dim3(nb), dim3(TPB), 0, st, v, n, part.p);
-    FH_CHECK_LAUNCH();
-    std::vector<double> h(nb);
-    download(h.data(), part.p, nb * sizeof(double), st);
-    stream_wait(st);
-    double s = 0.0;
-    for (double x : h) s += x;
-    return s;
-}
 
 // ---- grounded pseudo-inverse on the GPU (large small super-blocks)
 // diag and n_r of the assembled S_SS (len ms*ms + ms) for the host's redundancy decisions.
@@ -466,40 +454,6 @@ double absorber_sw(Absorber& a) {
         a.sw_ready = true;
     }
     return a.sw;
-}
-
-bool escalation_possible(const Absorber& a) {
-    if (a.escalated || a.m <= a.ms || a.boff.size() < a.nb) return false;
-    // the escalated block must be the only large one: the super-block then covers S and the
-    // restarted solve is exact (a dense block next to Jacobi blocks stagnated on a
-    // near-singular ring block in testing and is not enabled)
-    int nlarge = 0;
-    uint64_t g = 0;
-    for (const BlockInfo& bi : a.blocks)
-        if (!bi.small) {
-            nlarge++;
-            g = bi.g;
-        }
-    return nlarge == 1 && a.ms + g <= ESC_MAX;
-}
-
-void escalate(Absorber& a) {
-    require(escalation_possible(a), "escalation: no block to escalate");
-    a.escalated = true;
-    // blocks are ordered small first, so the first large block is adjacent to the small
-    // super-block in the coordinates and in the cell codes
-    for (size_t b = 0; b < a.nb; b++) {
-        BlockInfo& bi = a.blocks[b];
-        if (bi.small) continue;
-        bi.small = true;
-        a.ms += bi.g;
-        break;
-    }
-    // with two or more small blocks the dense small-row products no longer apply: every block
-    // then runs a level pass, so the level records skipped for a dense small block are built
-    for (size_t b = 0; b < a.nb; b++)
-        if (a.boff[b].n == 0) build_block_transpose(a, (int)b);
-    refresh_weights(a, a.weighted ? a.w_rows.p : nullptr);
 }
 
 void refresh_weights(Absorber& a, const double* w_rows_dev) {
@@ -786,6 +740,7 @@ void refresh_weights(Absorber& a, const double* w_rows_dev) {
         download(&bad, a.esm8_bad.p, sizeof(bad), st);
         a.esm8_ok = bad == 0;
     }
+    facs_after_refresh(a);
 }
 
 }  // namespace fh

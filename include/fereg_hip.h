@@ -63,6 +63,11 @@ typedef struct {
     int32_t mixed_lead;   /* FH_MIXED: most FP64 PCG iterations before the FP32 correction
                              sweeps (0..8; -1 = the default, 8); the lead hands over earlier
                              once one FP32 sweep can reach the tolerance */
+    int32_t sparse_chol;  /* 1 (default): a slowly converging solve escalates to exact sparse
+                             factors of the reduced matrix; 0: Jacobi throughout */
+    int32_t joint;        /* 1 (default): the escalation factors the joint matrix over every
+                             coordinate when its budgets allow (the restart is then exact),
+                             else each large block; 0: each large block only */
 } fh_solver_config;
 
 /* SolveOpts (absorber.rs:18-24) plus the precision of this solve. */
@@ -74,8 +79,8 @@ typedef struct {
 } fh_solve_opts;
 
 /* SolveInfo (absorber.rs:27-37); refinements = FP64 outer sweeps of the mixed form; escalated = 1
- * when the solve's rate probe (iteration 12) gave the remaining large block the exact block
- * preconditioner and restarted (kept by the absorber for later solves). */
+ * when the solve's rate probe (iteration 12) replaced Jacobi by exact sparse factors (the joint
+ * reduced matrix, or each large block) and restarted; the absorber keeps them for later solves. */
 typedef struct {
     uint64_t iterations;
     int32_t escalated;
@@ -527,6 +532,19 @@ int fh_server_create(const fh_server_config* cfg, fh_server** out);
 int fh_server_submit(fh_server* s, const fh_request* req, uint64_t* ticket);
 int fh_server_wait(fh_server* s, uint64_t ticket, fh_response* resp);
 void fh_server_destroy(fh_server* s);
+
+/* SparseChol::try_factor / solve (crates/fereg-core/src/precond/sparse_chol.rs:216-449): the exact
+ * LDL^T of a symmetric positive semidefinite matrix given by its lower triangle (CSR: row i holds
+ * columns <= i, ascending), scaled to unit diagonal, with every pivot <= 1e-10 grounded (its
+ * coordinate's solution component becomes ~0: a reflexive generalized inverse, as the solver's
+ * escalated preconditioner uses it). Matrix indices [0, nlast) are eliminated last. Host memory
+ * and host compute; solve works in place on an n x nrhs row-major block. */
+typedef struct fh_sparse_chol fh_sparse_chol;
+int fh_sparse_chol_factor(size_t n, const uint64_t* ptr, const uint32_t* col, const double* val, size_t nlast,
+                          fh_sparse_chol** out);
+int fh_sparse_chol_info(const fh_sparse_chol* f, uint64_t* len, double* flops, uint64_t* grounded);
+int fh_sparse_chol_solve(const fh_sparse_chol* f, double* b, size_t nrhs);
+void fh_sparse_chol_destroy(fh_sparse_chol* f);
 
 /* Device memory helpers for callers without another allocator. */
 int fh_device_alloc(void** p, size_t bytes);

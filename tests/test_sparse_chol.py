@@ -1,0 +1,80 @@
+"""The host sparse LDL^T (precond/sparse_chol.rs) through the C ABI against dense linear algebra:
+exact solves of SPD systems, the grounded generalized inverse of singular Laplacians (one
+grounded pivot per connected component, A M A = A), and the reduced-system matrices of FE
+designs assembled from dummies (operator.rs:2292-2315's explicit S)."""
+import numpy as np
+import pytest
+
+from fereg_amd.sparse_chol import SparseChol
+
+
+def _laplacian(n, edges, w):
+    a = np.zeros((n, n))
+    for (i, j), x in zip(edges, w):
+        a[i, i] += x
+        a[j, j] += x
+        a[i, j] -= x
+        a[j, i] -= x
+    return a
+
+
+def test_spd_solve_matches_dense():
+    rng = np.random.default_rng(0)
+    n = 60
+    b = rng.standard_normal((n, n)) * (rng.random((n, n)) < 0.08)
+    a = b @ b.T + n * np.eye(n)
+    f = SparseChol.from_dense(a)
+    assert f.grounded == 0
+    rhs = rng.standard_normal((n, 3))
+    np.testing.assert_allclose(f.solve(rhs), np.linalg.solve(a, rhs), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(f.solve(rhs[:, 0]), np.linalg.solve(a, rhs[:, 0]), rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("nlast", [0, 5])
+def test_singular_laplacian_is_grounded_once_per_component(nlast):
+    # three components: a ring, a path and a random graph; the solve is a reflexive generalized
+    # inverse (A M A = A) and solves every consistent system exactly
+    rng = np.random.default_rng(1)
+    edges = [(i, (i + 1) % 40) for i in range(40)]
+    edges += [(40 + i, 41 + i) for i in range(24)]
+    for _ in range(80):
+        i, j = rng.choice(np.arange(65, 95), 2, replace=False)
+        edges.append((i, j))
+    n = 95
+    a = _laplacian(n, edges, rng.uniform(0.5, 2.0, len(edges)))
+    f = SparseChol.from_dense(a, nlast=nlast)
+    assert f.grounded == 3
+    m = f.solve(np.eye(n))
+    np.testing.assert_allclose(a @ m @ a, a, atol=1e-9 * np.abs(a).max())
+    v = rng.standard_normal((n, 2))
+    x = f.solve(a @ v)
+    np.testing.assert_allclose(a @ x, a @ v, atol=1e-9 * np.abs(a @ v).max())
+
+
+def test_reduced_fe_matrix_from_dummies():
+    # S = A_rr - A_r1 A_11^-1 A_1r for an (indiv, firm, year) design: the Schur complement of the
+    # eliminated FE1, with small coordinates (years) ordered last
+    rng = np.random.default_rng(2)
+    n, g1, F, T = 3000, 300, 120, 6
+    ind = rng.integers(0, g1, n)
+    firm = rng.integers(0, F, n)
+    yr = rng.integers(0, T, n)
+    w = rng.uniform(0.5, 2.0, n)
+    D1 = np.zeros((n, g1)); D1[np.arange(n), ind] = 1
+    Dr = np.zeros((n, T + F)); Dr[np.arange(n), yr] = 1; Dr[np.arange(n), T + firm] = 1
+    n1 = D1.T @ w
+    Arr = Dr.T @ (w[:, None] * Dr)
+    Ar1 = Dr.T @ (w[:, None] * D1)
+    S = Arr - (Ar1 / n1) @ Ar1.T
+    f = SparseChol.from_dense(S, nlast=T)
+    # year and firm each carry a constant that FE1 absorbs: two null directions
+    assert f.grounded == S.shape[0] - np.linalg.matrix_rank(S) == 2
+    v = rng.standard_normal((T + F, 2))
+    x = f.solve(S @ v)
+    np.testing.assert_allclose(S @ x, S @ v, atol=1e-9 * np.abs(S @ v).max())
+
+
+def test_rejects_entries_above_the_diagonal():
+    import fereg_amd as F
+    with pytest.raises(F.FeregError):
+        SparseChol(2, [0, 2, 3], [0, 1, 1], [1.0, 0.5, 1.0])
