@@ -298,7 +298,7 @@ SIGNATURES = [
     ("fh_server_wait", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(Response)]),
     ("fh_server_destroy", None, [vp]),
     ("fh_sparse_chol_factor", ctypes.c_int, [ctypes.c_size_t, u64p, u32p, f64p, ctypes.c_size_t, ctypes.POINTER(vp)]),
-    ("fh_sparse_chol_info", ctypes.c_int, [vp, u64p, f64p, u64p]),
+    ("fh_sparse_chol_info", ctypes.c_int, [vp, u64p, f64p, u64p, u64p]),
     ("fh_sparse_chol_solve", ctypes.c_int, [vp, f64p, ctypes.c_size_t]),
     ("fh_sparse_chol_destroy", None, [vp]),
     ("fh_device_alloc", ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),

@@ -341,21 +341,31 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
         hipLaunchKernelGGL(k_zero_cols, dim3(vgrid(m * rp)), dim3(TPB), 0, st, x, m, rp, dz.p);
         FH_CHECK_LAUNCH();
         const bool warm = warm_dev != nullptr;
-        auto solve = [&](bool from_x, bool can_esc, uint64_t mi) {
+        auto solve = [&](bool from_x, bool can_esc, uint64_t mi, JointProbe* jp) {
             return precision == FH_MIXED
-                       ? pcg_mixed(a, a.s64.rhs.p, x, from_x, rp, ss1.data(), ss0.data(), tol, mi, ctx, can_esc)
-                       : pcg_fp64(a, a.s64.rhs.p, x, from_x, rp, ss1.data(), ss0.data(), tol, mi, ctx, can_esc);
+                       ? pcg_mixed(a, a.s64.rhs.p, x, from_x, rp, ss1.data(), ss0.data(), tol, mi, ctx, can_esc, jp)
+                       : pcg_fp64(a, a.s64.rhs.p, x, from_x, rp, ss1.data(), ss0.data(), tol, mi, ctx, can_esc, jp);
         };
-        PcgResult pr = solve(warm, escalation_possible(a), maxiter);
-        if (pr.escalate) {
-            // slow convergence at the probe: the reduced matrix (joint, else per large block) is
-            // factored exactly and the solve restarts from the current iterate (cg.rs:264-274);
-            // with no factor within the budgets it restarts on Jacobi
-            const uint64_t used = pr.iterations;
-            out.escalated = escalate(a, ctx);
-            pr = solve(true, false, maxiter > used ? maxiter - used : 1);
-            pr.iterations += used;
+        // the early probe (iteration 6) may build the joint factor when the projected PCG time
+        // exceeds its estimated cost; the escalation probe (iteration 12) factors the reduced
+        // matrix when the rate is slow (cg.rs:254-274, solve.rs:932-975); either way the solve
+        // restarts from the current iterate
+        JointProbe jp;
+        jp.build_est = escalation_possible(a) ? joint_build_estimate(a) : INFINITY;
+        uint64_t used = 0;
+        PcgResult pr = solve(warm, escalation_possible(a), maxiter, jp.build_est < INFINITY ? &jp : nullptr);
+        if (jp.fired) {
+            used += pr.iterations;
+            out.escalated = escalate_joint(a, ctx, jp.without);
+            pr = solve(true, escalation_possible(a), maxiter > used ? maxiter - used : 1, nullptr);
         }
+        if (pr.escalate) {
+            // with no factor within the budgets the solve restarts on Jacobi
+            used += pr.iterations;
+            out.escalated = escalate(a, ctx) || out.escalated;
+            pr = solve(true, false, maxiter > used ? maxiter - used : 1, nullptr);
+        }
+        pr.iterations += used;
         if (pr.rc == FH_CONVERGENCE)
             throw Error(FH_CONVERGENCE, "the FE solver did not converge within " + std::to_string(maxiter) + " iterations");
         out.iterations = std::max<uint64_t>(pr.iterations, 1);

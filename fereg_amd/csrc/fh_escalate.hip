@@ -41,6 +41,18 @@ constexpr uint64_t ESC_MAX_NNZ = 30'000'000;     // structural nonzeros of the a
 constexpr uint64_t ESC_MAX_LEN = 100'000'000;    // factor entries
 constexpr double ESC_MAX_FLOPS = 2e10;           // multiply-adds of the host factorization
 
+// Cost model of a factor build here (seconds; the reference's is solve.rs:82-121 on its CPU):
+// GPU assembly per emitted item pair (count, emit, radix sort, run sums), per structural nonzero
+// (download, the host CSR, the ordering and the symbolic pass), per multiply-add of the host
+// numeric factorization and per factor entry (transposes, level schedules, upload).
+// Measured on MI355X with its host (rings and coupled rings of 1.5e3 to 1.5e5 coordinates): the
+// assembly 27 ms for 1e6 pairs, the analysis 12-30 ms for 0.44-0.93 M nonzeros, the numeric pass
+// and the upload 3-10 ms each for 0.5-1.1 M entries.
+constexpr double C_PAIR = 3e-8;
+constexpr double C_NNZ = 4e-8;
+constexpr double C_FLOP = 1.5e-9;
+constexpr double C_ENTRY = 2e-8;
+
 struct AsmArgs {
     const uint32_t* seg_off;
     const uint32_t* codes;  // M x nb
@@ -304,6 +316,7 @@ struct FacGpu {
     LdlFactor f;
     size_t nf = 0;
     uint32_t nlf = 0, nlb = 0;    // forward / backward levels
+    uint32_t maxw = 1;            // most rows in one level
     std::vector<uint32_t> hfofs, hbofs;
     DBuf<uint32_t> gmap, fptr, fidx, bptr, bidx, fsched, fofs, bsched, bofs;
     DBuf<double> sc, dinv, fval, bval, y;
@@ -376,6 +389,9 @@ void upload_factor(FacGpu& F, hipStream_t st) {
     schedule(levb, nlb, bs, F.hbofs);
     F.nlf = nlf;
     F.nlb = nlb;
+    F.maxw = 1;
+    for (uint32_t l = 0; l < nlf; l++) F.maxw = std::max(F.maxw, F.hfofs[l + 1] - F.hfofs[l]);
+    for (uint32_t l = 0; l < nlb; l++) F.maxw = std::max(F.maxw, F.hbofs[l + 1] - F.hbofs[l]);
     std::vector<uint32_t> li32(f.li.begin(), f.li.end());
     put(F.gmap, gmap, st);
     put(F.sc, f.scale, st);
@@ -490,35 +506,81 @@ bool assemble(Absorber& a, uint64_t lo, uint64_t range, SymLower& out, std::vect
     return out.n > 0;
 }
 
-// Builds the factor of [lo, lo + range) (nlast: kept coordinates below a.ms ordered last).
+double secs_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
+
+// Builds the factor of [lo, lo + range) (small_last: kept coordinates below a.ms ordered last).
+// budget (seconds): the build stops before a stage whose estimated remaining cost exceeds it
+// (costs already spent are sunk). joint: the joint factor's statistics are kept on the absorber.
 std::shared_ptr<FacGpu> build_factor(Absorber& a, uint64_t lo, uint64_t range, bool small_last, const fh_ctx* ctx,
-                                     const char* what) {
+                                     const char* what, double budget, bool joint) {
     const auto t0 = std::chrono::steady_clock::now();
     auto F = std::make_shared<FacGpu>();
     F->lo = lo;
     F->range = range;
     SymLower S;
+    const std::string tag = std::string("fereg: escalation: ") + what;
     if (!assemble(a, lo, range, S, F->keep)) {
-        log_line(ctx, 2, std::string("fereg: escalation: ") + what + " over the assembly budgets");
+        log_line(ctx, 2, tag + " over the assembly budgets");
+        if (joint) a.fac_failed_joint = true;
+        return nullptr;
+    }
+    const double t_asm = secs_since(t0);
+    const double nnz = (double)S.ptr[S.n];
+    if (joint) a.joint_nnz = nnz;
+    if (C_NNZ * nnz > budget) {
+        log_line(ctx, 2, tag + " rejected (cost) after assembly: nnz=" + std::to_string((uint64_t)nnz));
         return nullptr;
     }
     size_t nlast = 0;
     if (small_last)
         while (nlast < F->keep.size() && F->keep[nlast] < a.ms) nlast++;
-    if (!ldl_analyze(S, nlast, ESC_MAX_LEN, ESC_MAX_FLOPS, F->f)) {
-        log_line(ctx, 2, std::string("fereg: escalation: ") + what + " over the fill budget (L=" +
-                             std::to_string(F->f.len()) + ")");
+    const bool fits = ldl_analyze(S, nlast, ESC_MAX_LEN, ESC_MAX_FLOPS, F->f);
+    const double t_an = secs_since(t0) - t_asm;
+    if (joint) {
+        a.joint_len = (double)F->f.len();
+        a.joint_flops = F->f.flops;
+    }
+    if (!fits) {
+        log_line(ctx, 2, tag + " over the fill budget (L=" + std::to_string(F->f.len()) + ")");
+        if (joint) a.fac_failed_joint = true;
+        return nullptr;
+    }
+    if (C_FLOP * F->f.flops + C_ENTRY * (double)F->f.len() > budget) {
+        log_line(ctx, 2, tag + " rejected (cost) after analysis: L=" + std::to_string(F->f.len()));
         return nullptr;
     }
     check_interrupt(ctx);
-    if (!ldl_numeric(S, F->f)) return nullptr;
+    if (!ldl_numeric(S, F->f)) {
+        if (joint) a.fac_failed_joint = true;
+        return nullptr;
+    }
+    const double t_num = secs_since(t0) - t_asm - t_an;
     upload_factor(*F, a.st);
-    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    log_line(ctx, 2, std::string("fereg: escalation: ") + what + " factor n=" + std::to_string(F->nf) +
-                         " nnz=" + std::to_string(S.ptr[S.n]) + " L=" + std::to_string(F->f.len()) +
+    const double secs = secs_since(t0);
+    log_line(ctx, 2, tag + " factor n=" + std::to_string(F->nf) + " nnz=" + std::to_string((uint64_t)nnz) +
+                         " L=" + std::to_string(F->f.len()) + " flops=" + std::to_string(F->f.flops) +
                          " grounded=" + std::to_string(F->f.grounded) + " levels=" + std::to_string(F->nlf) + "/" +
-                         std::to_string(F->nlb) + " in " + std::to_string(secs) + "s");
+                         std::to_string(F->nlb) + " in " + std::to_string(secs) + "s (assembly " +
+                         std::to_string(t_asm) + ", analysis " + std::to_string(t_an) + ", numeric " +
+                         std::to_string(t_num) + ", upload " + std::to_string(secs - t_asm - t_an - t_num) + ")");
     return F;
+}
+
+// Item pairs the joint assembly emits at most (structure only; one download of seg_off).
+double joint_pairs(Absorber& a) {
+    if (a.joint_pairs < 0.0) {
+        std::vector<uint32_t> so(a.g1 + 1);
+        download(so.data(), a.seg_off.p, so.size() * sizeof(uint32_t), a.st);
+        double work = 0.0;
+        for (size_t s = 0; s < a.g1; s++) {
+            const double it = (double)(so[s + 1] - so[s]) * (double)a.nb;
+            work += it * (it + 1) / 2;
+        }
+        a.joint_pairs = work;
+    }
+    return a.joint_pairs;
 }
 
 // Jacobi and the dense small factor leave the factored coordinates to the factors: their inverse
@@ -553,8 +615,8 @@ bool escalate(Absorber& a, const fh_ctx* ctx) {
         a.fac_joint = false;
         refresh_weights(a, a.weighted ? a.w_rows.p : nullptr);
     }
-    if (a.joint_policy != 0) {
-        auto J = build_factor(a, 0, a.m, true, ctx, "joint");
+    if (a.joint_policy != 0 && !a.fac_failed_joint) {
+        auto J = build_factor(a, 0, a.m, true, ctx, "joint", INFINITY, true);
         if (J) {
             a.facs.push_back(J);
             a.fac_joint = true;
@@ -563,7 +625,7 @@ bool escalate(Absorber& a, const fh_ctx* ctx) {
     if (a.facs.empty()) {
         for (const BlockInfo& bi : a.blocks) {
             if (bi.small) continue;
-            auto B = build_factor(a, bi.off, bi.g, false, ctx, "block");
+            auto B = build_factor(a, bi.off, bi.g, false, ctx, "block", INFINITY, false);
             if (B) a.facs.push_back(B);
         }
     }
@@ -604,6 +666,29 @@ void facs_after_refresh(Absorber& a) {
 
 bool facs_exact(const Absorber& a) { return a.fac_joint && a.fac_fresh && !a.facs.empty(); }
 
+double joint_build_estimate(Absorber& a) {
+    if (a.joint_policy == 0 || !a.sparse_chol || a.fac_failed || a.fac_failed_joint || !a.facs.empty() ||
+        a.m <= a.ms)
+        return INFINITY;
+    double c = C_PAIR * joint_pairs(a);
+    if (a.joint_nnz >= 0.0) c += C_NNZ * a.joint_nnz;
+    if (a.joint_flops >= 0.0) c += C_FLOP * a.joint_flops + C_ENTRY * a.joint_len;
+    return c;
+}
+
+bool escalate_joint(Absorber& a, const fh_ctx* ctx, double without) {
+    if (joint_build_estimate(a) == INFINITY) return false;
+    log_line(ctx, 2, "fereg: early probe: joint factor est " + std::to_string(joint_build_estimate(a)) +
+                         "s vs pcg est " + std::to_string(without) + "s");
+    auto J = build_factor(a, 0, a.m, true, ctx, "joint (probe)", 0.8 * without, true);
+    if (!J) return false;
+    a.facs.push_back(J);
+    a.fac_joint = true;
+    a.fac_fresh = true;
+    mask_factored(a);
+    return true;
+}
+
 void facs_apply(Absorber& a, const double* rr, double* z, int r, double* rzf_part, const uint32_t* done,
                 hipStream_t st) {
     const int nfac = (int)a.facs.size();
@@ -615,7 +700,9 @@ void facs_apply(Absorber& a, const double* rr, double* z, int r, double* rzf_par
         const FacDev D = F.dev();
         const size_t vbytes = F.nf * (size_t)r * sizeof(double);
         if (vbytes <= FAC_LDS_BYTES) {
-            hipLaunchKernelGGL(k_fac_lds, dim3(1), dim3(FAC_LDS_THREADS), vbytes, st, D, F.nf, F.nlf, F.nlb, rr, z, r,
+            // one wave per row of the widest level, up to 16 (a barrier costs every wave of the group)
+            const unsigned threads = WAVE * std::min<unsigned>(FAC_LDS_THREADS / WAVE, F.maxw);
+            hipLaunchKernelGGL(k_fac_lds, dim3(1), dim3(threads), vbytes, st, D, F.nf, F.nlf, F.nlb, rr, z, r,
                                rzf_part, q * span, done);
             FH_CHECK_LAUNCH();
             continue;
@@ -630,10 +717,14 @@ void facs_apply(Absorber& a, const double* rr, double* z, int r, double* rzf_par
                 uint32_t l1 = l + 1;
                 unsigned grid = grid_for(rows, TPB / WAVE, 1u << 14);
                 unsigned threads = TPB;
-                if (rows < FAC_WIDE) {  // a run of narrow levels in one workgroup
-                    while (l1 < nl && ofs[l1 + 1] - ofs[l1] < FAC_WIDE) l1++;
+                if (rows < FAC_WIDE) {  // a run of narrow levels in one workgroup, a wave per row
+                    uint32_t w = rows;
+                    while (l1 < nl && ofs[l1 + 1] - ofs[l1] < FAC_WIDE) {
+                        w = std::max(w, ofs[l1 + 1] - ofs[l1]);
+                        l1++;
+                    }
                     grid = 1;
-                    threads = FAC_LDS_THREADS;
+                    threads = WAVE * std::min<unsigned>(FAC_LDS_THREADS / WAVE, w);
                 }
                 if (dir == 0)
                     hipLaunchKernelGGL(k_fac_level<true>, dim3(grid), dim3(threads), 0, st, D, l, l1, rr, F.y.p, z, r, done);

@@ -23,6 +23,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <memory>
 
 #include "fh_common.hpp"
@@ -160,8 +161,12 @@ struct Absorber {
     bool fac_joint = false;   // one factor over every coordinate
     bool fac_fresh = false;   // built for the current weights
     bool fac_failed = false;  // no factor fit the budgets: Jacobi for good
+    bool fac_failed_joint = false;  // the joint factor exceeded a hard budget (or failed)
     bool sparse_chol = true;
-    int joint_policy = 1;     // 0 off, 1 on escalation
+    int joint_policy = 1;     // 0 off, 1 on escalation and at the early probe's cost rule
+    // what is known of the joint factor's cost (structure only): the assembly's item pairs, then
+    // the nonzeros, factor entries and multiply-adds once it was assembled and analyzed
+    double joint_pairs = -1.0, joint_nnz = -1.0, joint_len = -1.0, joint_flops = -1.0;
     uint32_t mixed_lead = MIXED_LEAD_DEFAULT;  // most FP64 iterations of a mixed solve before its FP32 sweeps
 
     bool trivial() const { return cell_obs_off.n == 0; }
@@ -199,6 +204,12 @@ bool escalation_possible(const Absorber& a);
 bool escalate(Absorber& a, const fh_ctx* ctx);  // false: no factor within the budgets (Jacobi stays)
 void facs_after_refresh(Absorber& a);
 bool facs_exact(const Absorber& a);  // a joint factor for the current weights: M = S
+// The joint factor's estimated cost in seconds from what is known of it so far (a lower bound
+// before it was first assembled), or INFINITY when the joint factor is not available.
+double joint_build_estimate(Absorber& a);
+// Builds the joint factor when every stage's estimated remaining cost stays within the margin of
+// `without` (the projected PCG seconds): true when built (M = S for the restart).
+bool escalate_joint(Absorber& a, const fh_ctx* ctx, double without);
 // z = M^-1 r on the factored coordinates (FP64, m x r rows) and the partial r'z per factor in
 // rzf_part (SPMV_DOTG x 16 entries, each factor its own share of them; zeroed by the caller).
 void facs_apply(Absorber& a, const double* rr, double* z, int r, double* rzf_part, const uint32_t* done,
@@ -305,13 +316,23 @@ struct PcgResult {
     uint64_t refinements = 0;
     bool escalate = false;       // stopped at the escalation probe (x holds the progress so far)
 };
+// The early probe's joint-factor rule (solve.rs:932-975): a solve that may escalate stops after its
+// iteration-6 probe when the joint factor's estimated cost (build_est, seconds) is within the
+// margin of the projected remaining PCG time (fired; x holds the progress so far).
+struct JointProbe {
+    double build_est = INFINITY;
+    bool fired = false;
+    double without = 0.0, rho = 0.0, remaining = 0.0;
+};
 // Solves S x = rhs for r columns; x holds the start (x_maybe_nonzero false: x is zero, which
 // spares the device read that tests it). FP64 throughout.
 PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
-                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate = false);
+                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate = false,
+                   JointProbe* jp = nullptr);
 // Mixed form: FP32 inner PCG, FP64 outer residual and update.
 PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
-                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate = false);
+                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate = false,
+                    JointProbe* jp = nullptr);
 
 // ---------------------------------------------------------------- estimation (fh_estimate.hip)
 // G = sum_i w_i z_i z_i' (r x r, FP64 on the host), z n x r row-major; w may be null.

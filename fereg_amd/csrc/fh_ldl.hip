@@ -153,6 +153,211 @@ std::vector<uint32_t> md_order(size_t n, const std::vector<uint64_t>& xadj, cons
     return order;
 }
 
+// Nested dissection by BFS level structures (George 1973, in the form of George and Liu's
+// "automatic nested dissection"): a part is cut at the middle BFS level from a pseudo-peripheral
+// node; both sides are ordered first (recursively), the separating level last. Parts of at most
+// ND_LEAF nodes, and parts a level cut cannot split, are ordered by minimum degree. The
+// elimination tree is then about log2(n) separators deep instead of the long chains minimum
+// degree leaves on path- and ring-like graphs, which matters for the level-scheduled GPU solves.
+constexpr size_t ND_LEAF = 64;
+
+std::vector<uint32_t> nd_order(size_t n, const std::vector<uint64_t>& xadj, const std::vector<uint32_t>& adj,
+                               size_t nlast) {
+    const double dense = std::max(16.0, 10.0 * std::sqrt((double)(n - nlast)));
+    std::vector<uint8_t> out(n, 0);  // 1: not part of the dissection (last group or dense row)
+    std::vector<uint32_t> dense_nodes;
+    for (size_t i = 0; i < nlast; i++) out[i] = 1;
+    for (size_t i = nlast; i < n; i++)
+        if ((double)(xadj[i + 1] - xadj[i]) > dense) {
+            out[i] = 1;
+            dense_nodes.push_back((uint32_t)i);
+        }
+    std::vector<uint32_t> order;
+    order.reserve(n);
+    std::vector<int64_t> part(n, -1);  // current part id of each node
+    std::vector<int64_t> lev(n, -1);
+    std::vector<uint32_t> q;
+    // BFS over the nodes of part `pid` from `s`; returns the visit order (levels in lev)
+    auto bfs = [&](uint32_t s0, int64_t pid, std::vector<uint32_t>& vis) {
+        vis.clear();
+        vis.push_back(s0);
+        lev[s0] = 0;
+        for (size_t h = 0; h < vis.size(); h++) {
+            const uint32_t u = vis[h];
+            for (uint64_t e = xadj[u]; e < xadj[u + 1]; e++) {
+                const uint32_t v = adj[e];
+                if (part[v] == pid && lev[v] < 0) {
+                    lev[v] = lev[u] + 1;
+                    vis.push_back(v);
+                }
+            }
+        }
+    };
+    // minimum degree on a small induced subgraph, appended to the order
+    auto leaf = [&](const std::vector<uint32_t>& nodes, int64_t pid) {
+        std::vector<uint32_t> loc(nodes.size());
+        std::vector<uint64_t> sx(nodes.size() + 1, 0);
+        std::vector<uint32_t> sa;
+        std::vector<int64_t> idx(0);
+        for (size_t k = 0; k < nodes.size(); k++) lev[nodes[k]] = (int64_t)k;  // local index
+        for (size_t k = 0; k < nodes.size(); k++) {
+            const uint32_t u = nodes[k];
+            for (uint64_t e = xadj[u]; e < xadj[u + 1]; e++)
+                if (part[adj[e]] == pid) sa.push_back((uint32_t)lev[adj[e]]);
+            sx[k + 1] = sa.size();
+        }
+        for (uint32_t k : md_order(nodes.size(), sx, sa, 0)) order.push_back(nodes[k]);
+        for (uint32_t u : nodes) {
+            lev[u] = -1;
+            part[u] = -2;  // ordered
+        }
+    };
+    // work stack of parts (node lists); a part is ordered after everything pushed above it, so the
+    // separator is emitted once both sides are done: (nodes, is_separator)
+    struct Item {
+        std::vector<uint32_t> nodes;
+        bool sep;
+    };
+    std::vector<Item> stack;
+    int64_t next_pid = 0;
+    {
+        std::vector<uint32_t> all;
+        for (size_t i = 0; i < n; i++)
+            if (!out[i]) all.push_back((uint32_t)i);
+        stack.push_back({std::move(all), false});
+    }
+    std::vector<uint32_t> vis;
+    while (!stack.empty()) {
+        Item it = std::move(stack.back());
+        stack.pop_back();
+        if (it.sep) {
+            for (uint32_t u : it.nodes) {
+                order.push_back(u);
+                part[u] = -2;
+            }
+            continue;
+        }
+        const int64_t pid = next_pid++;
+        for (uint32_t u : it.nodes) part[u] = pid;
+        if (it.nodes.size() <= ND_LEAF) {
+            leaf(it.nodes, pid);
+            continue;
+        }
+        // one connected component from the first node; the rest of the part is its own part
+        bfs(it.nodes[0], pid, vis);
+        if (vis.size() < it.nodes.size()) {
+            std::vector<uint32_t> rest;
+            for (uint32_t u : it.nodes)
+                if (lev[u] < 0) rest.push_back(u);
+            for (uint32_t u : vis) lev[u] = -1;
+            for (uint32_t u : rest) part[u] = -1;
+            stack.push_back({std::move(rest), false});
+            it.nodes = vis;
+            if (it.nodes.size() <= ND_LEAF) {
+                leaf(it.nodes, pid);
+                continue;
+            }
+        }
+        // pseudo-peripheral start: BFS again from the last node reached while the depth grows
+        uint32_t s0 = vis.back();
+        int64_t depth = lev[vis.back()];
+        for (int rep = 0; rep < 4; rep++) {
+            for (uint32_t u : vis) lev[u] = -1;
+            bfs(s0, pid, vis);
+            const int64_t d = lev[vis.back()];
+            if (d <= depth && rep > 0) break;
+            depth = d;
+            s0 = vis.back();
+        }
+        depth = lev[vis.back()];  // the last BFS's depth (its levels are in lev)
+        if (depth < 2) {  // no level separates two non-empty sides
+            for (uint32_t u : vis) lev[u] = -1;
+            leaf(vis, pid);
+            continue;
+        }
+        // the level where half of the part has been reached
+        std::vector<size_t> cnt(depth + 1, 0);
+        for (uint32_t u : vis) cnt[lev[u]]++;
+        size_t acc = 0;
+        int64_t L = 1;
+        for (int64_t l = 0; l <= depth; l++) {
+            acc += cnt[l];
+            if (2 * acc >= vis.size()) {
+                L = std::min<int64_t>(std::max<int64_t>(l, 1), depth - 1);
+                break;
+            }
+        }
+        std::vector<uint32_t> A, B, Sp;
+        for (uint32_t u : vis) (lev[u] < L ? A : lev[u] > L ? B : Sp).push_back(u);
+        for (uint32_t u : vis) {
+            lev[u] = -1;
+            part[u] = -1;
+        }
+        stack.push_back({std::move(Sp), true});
+        stack.push_back({std::move(B), false});
+        stack.push_back({std::move(A), false});
+    }
+    for (uint32_t i : dense_nodes) order.push_back(i);
+    for (size_t i = 0; i < nlast; i++) order.push_back((uint32_t)i);
+    return order;
+}
+
+// Symbolic analysis of the ordering perm: the permuted upper triangle by columns, the elimination
+// tree and the column counts (Liu: walking from each entry of column k up the tree until a node
+// already visited for k). Returns the elimination tree's height.
+uint64_t symbolic(const SymLower& a, const std::vector<uint32_t>& perm, LdlFactor& f) {
+    const size_t n = a.n;
+    std::vector<uint32_t> inv(n);
+    for (size_t k = 0; k < n; k++) inv[perm[k]] = (uint32_t)k;
+    f.perm = perm;
+    // entry (i, j), j <= i, goes to column max(inv i, inv j) at row min(...)
+    f.up.assign(n + 1, 0);
+    for (size_t i = 0; i < n; i++)
+        for (uint64_t e = a.ptr[i]; e < a.ptr[i + 1]; e++) f.up[std::max(inv[i], inv[a.col[e]]) + 1]++;
+    for (size_t k = 0; k < n; k++) f.up[k + 1] += f.up[k];
+    f.uri.resize(f.up[n]);
+    f.upos.resize(f.up[n]);
+    {
+        std::vector<uint64_t> fill(f.up.begin(), f.up.end() - 1);
+        for (size_t i = 0; i < n; i++)
+            for (uint64_t e = a.ptr[i]; e < a.ptr[i + 1]; e++) {
+                const uint32_t pi = inv[i], pj = inv[a.col[e]];
+                const uint32_t c = std::max(pi, pj);
+                f.uri[fill[c]] = std::min(pi, pj);
+                f.upos[fill[c]++] = e;
+            }
+    }
+    f.parent.assign(n, -1);
+    std::vector<int64_t> flag(n, -1);
+    std::vector<uint64_t> lnz(n, 0);
+    for (size_t k = 0; k < n; k++) {
+        flag[k] = (int64_t)k;
+        for (uint64_t e = f.up[k]; e < f.up[k + 1]; e++) {
+            int64_t i = f.uri[e];
+            while (flag[i] != (int64_t)k) {
+                if (f.parent[i] < 0) f.parent[i] = (int64_t)k;
+                lnz[i]++;
+                flag[i] = (int64_t)k;
+                i = f.parent[i];
+            }
+        }
+    }
+    f.lp.assign(n + 1, 0);
+    double flops = 0.0;
+    for (size_t k = 0; k < n; k++) {
+        f.lp[k + 1] = f.lp[k] + lnz[k];
+        flops += (double)lnz[k] * (double)lnz[k];
+    }
+    f.flops = flops;
+    std::vector<uint64_t> h(n, 1);
+    uint64_t height = n ? 1 : 0;
+    for (size_t i = 0; i < n; i++) {  // parent[i] > i: a node's height is final when it is reached
+        height = std::max(height, h[i]);
+        if (f.parent[i] >= 0) h[f.parent[i]] = std::max(h[f.parent[i]], h[i] + 1);
+    }
+    return height;
+}
+
 }  // namespace
 
 bool ldl_analyze(const SymLower& a, size_t nlast, uint64_t max_len, double max_flops, LdlFactor& f) {
@@ -184,52 +389,24 @@ bool ldl_analyze(const SymLower& a, size_t nlast, uint64_t max_len, double max_f
                 adj[fill[j]++] = (uint32_t)i;
             }
     }
-    f.perm = md_order(n, xadj, adj, nlast);
-    std::vector<uint32_t> inv(n);
-    for (size_t k = 0; k < n; k++) inv[f.perm[k]] = (uint32_t)k;
-    // the permuted upper triangle by columns: entry (i, j), j <= i, goes to column max(inv i,
-    // inv j) at row min(...)
-    f.up.assign(n + 1, 0);
-    for (size_t i = 0; i < n; i++)
-        for (uint64_t e = a.ptr[i]; e < a.ptr[i + 1]; e++)
-            f.up[std::max(inv[i], inv[a.col[e]]) + 1]++;
-    for (size_t k = 0; k < n; k++) f.up[k + 1] += f.up[k];
-    f.uri.resize(f.up[n]);
-    f.upos.resize(f.up[n]);
-    {
-        std::vector<uint64_t> fill(f.up.begin(), f.up.end() - 1);
-        for (size_t i = 0; i < n; i++)
-            for (uint64_t e = a.ptr[i]; e < a.ptr[i + 1]; e++) {
-                const uint32_t pi = inv[i], pj = inv[a.col[e]];
-                const uint32_t c = std::max(pi, pj);
-                f.uri[fill[c]] = std::min(pi, pj);
-                f.upos[fill[c]++] = e;
-            }
+    // minimum degree (least fill); when its elimination tree is tall, nested dissection too, and
+    // the ordering whose level-scheduled solve is estimated cheaper within the budgets is kept:
+    // a level costs about one dependent round of loads (~2.5 us per direction), an entry a few
+    // bytes of streaming
+    auto apply_cost = [](uint64_t height, uint64_t len) { return 5e-6 * (double)height + 2e-10 * (double)len; };
+    const uint64_t h_md = symbolic(a, md_order(n, xadj, adj, nlast), f);
+    f.height = h_md;
+    if (h_md > 2 * ND_LEAF) {
+        LdlFactor g;
+        g.n = n;
+        g.diag_pos = f.diag_pos;
+        const uint64_t h_nd = symbolic(a, nd_order(n, xadj, adj, nlast), g);
+        g.height = h_nd;
+        const bool md_fits = f.lp[n] <= max_len && f.flops <= max_flops;
+        const bool nd_fits = g.lp[n] <= max_len && g.flops <= max_flops;
+        if (nd_fits && (!md_fits || apply_cost(h_nd, g.lp[n]) < apply_cost(h_md, f.lp[n]))) f = std::move(g);
     }
-    // elimination tree and column counts (Liu): walking from each entry of column k up the tree
-    // until a node already visited for k
-    f.parent.assign(n, -1);
-    std::vector<int64_t> flag(n, -1);
-    std::vector<uint64_t> lnz(n, 0);
-    for (size_t k = 0; k < n; k++) {
-        flag[k] = (int64_t)k;
-        for (uint64_t e = f.up[k]; e < f.up[k + 1]; e++) {
-            int64_t i = f.uri[e];
-            while (flag[i] != (int64_t)k) {
-                if (f.parent[i] < 0) f.parent[i] = (int64_t)k;
-                lnz[i]++;
-                flag[i] = (int64_t)k;
-                i = f.parent[i];
-            }
-        }
-    }
-    f.lp.assign(n + 1, 0);
-    double flops = 0.0;
-    for (size_t k = 0; k < n; k++) {
-        f.lp[k + 1] = f.lp[k] + lnz[k];
-        flops += (double)lnz[k] * (double)lnz[k];
-    }
-    f.flops = flops;
+    const double flops = f.flops;
     if (f.lp[n] > max_len || flops > max_flops) return false;
     return true;
 }
@@ -360,7 +537,7 @@ int fh_sparse_chol_factor(size_t n, const uint64_t* ptr, const uint32_t* col, co
     }
 }
 
-int fh_sparse_chol_info(const fh_sparse_chol* f, uint64_t* len, double* flops, uint64_t* grounded) {
+int fh_sparse_chol_info(const fh_sparse_chol* f, uint64_t* len, double* flops, uint64_t* grounded, uint64_t* height) {
     if (!f) {
         fh::set_last_error("fh_sparse_chol_info: null handle");
         return FH_INVALID;
@@ -368,6 +545,7 @@ int fh_sparse_chol_info(const fh_sparse_chol* f, uint64_t* len, double* flops, u
     if (len) *len = f->f.len();
     if (flops) *flops = f->f.flops;
     if (grounded) *grounded = f->f.grounded;
+    if (height) *height = f->f.height;
     return FH_OK;
 }
 

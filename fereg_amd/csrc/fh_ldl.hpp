@@ -43,13 +43,16 @@ struct LdlFactor {
     std::vector<uint64_t> upos;
     std::vector<uint64_t> diag_pos;  // A.val index of each matrix index's diagonal (UINT64_MAX: none)
     uint64_t grounded = 0;
+    uint64_t height = 0;            // elimination-tree height: the levels of each triangular solve
     double flops = 0.0;             // multiply-adds of the numeric factorization (sum of squared column counts)
     uint64_t len() const { return lp.empty() ? 0 : lp[n]; }
 };
 
 // Ordering and symbolic analysis. Matrix indices [0, nlast) are eliminated last, in their order
-// (the reference orders the small coordinates last: solve.rs:776-789). Returns false when the
-// factor would exceed max_len entries or max_flops multiply-adds (f then holds the counts only).
+// (the reference orders the small coordinates last: solve.rs:776-789). The ordering is minimum
+// degree, or nested dissection when that gives a much shorter elimination tree (a cheaper
+// level-scheduled solve). Returns false when the factor would exceed max_len entries or
+// max_flops multiply-adds (f then holds the counts only).
 bool ldl_analyze(const SymLower& a, size_t nlast, uint64_t max_len, double max_flops, LdlFactor& f);
 // Numeric factorization of a (same pattern as at analysis). Returns false on a non-finite pivot.
 bool ldl_numeric(const SymLower& a, LdlFactor& f);

@@ -31,14 +31,19 @@ constexpr double HANDOVER_GAIN = 1e10;
 // columns; rho_hat > PROBE_RATIO asks for a stronger preconditioner (err = 2).
 constexpr int PROBE_FROM = 7, PROBE_AT = 12;
 constexpr double PROBE_RATIO = 0.6;
+// Early probe (cg.rs:38-42): rho_hat = (r'z_6 / r'z_3)^(1/3) and the projected remaining
+// iterations max_j ln(r'z_j / thr_j) / ln(1 / rho_hat_j), for the joint factor's cost rule.
+constexpr int EARLY_FROM = 3, EARLY_AT = 6;
 
 struct PcgState {
     double rho[RMAX], rho_new[RMAX], gamma[RMAX], beta[RMAX], ss1[RMAX], ss0[RMAX], floor_[RMAX], thr[RMAX];
     double ring[RMAX][WINDOW];
-    double xb[RMAX], xr[RMAX], rho0[RMAX], absthr[RMAX], rho_from[RMAX];
+    double xb[RMAX], xr[RMAX], rho0[RMAX], absthr[RMAX], rho_from[RMAX], rho_early[RMAX];
     int32_t active[RMAX];
     uint32_t done, err;  // err: 1 = maxiter reached, 2 = escalation requested, 3 = hand-over
-    uint64_t iters, k, maxiter;
+    uint64_t iters;
+    double probe_rho, probe_rem;  // the early probe's rate and remaining iterations (-1: not yet)
+    uint64_t k, maxiter;
     int32_t r;
     int32_t can_escalate;
     int32_t fac;  // escalated factors write their r'z partials to slot S_RZF
@@ -346,18 +351,36 @@ __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restr
             }
         }
         if (k == PROBE_FROM) S->rho_from[j] = rn;
+        if (k == EARLY_FROM) S->rho_early[j] = rn;
     }
-    __shared__ double rh[RMAX];
-    if (j < RMAX) rh[j] = 0.0;
+    __shared__ double rh[RMAX], rem[RMAX];
+    if (j < RMAX) {
+        rh[j] = 0.0;
+        rem[j] = 0.0;
+    }
     __syncthreads();
     if (k == PROBE_AT && S->can_escalate && j < S->r && S->active[j] && S->rho_from[j] > 0.0)
         rh[j] = pow(S->rho_new[j] / S->rho_from[j], 1.0 / (double)(PROBE_AT - PROBE_FROM));
+    if (k == EARLY_AT && S->can_escalate && j < S->r && S->active[j] && S->rho_early[j] > 0.0) {
+        const double rr = pow(S->rho_new[j] / S->rho_early[j], 1.0 / (double)(EARLY_AT - EARLY_FROM));
+        const double left = S->maxiter > S->iters + 1 ? (double)(S->maxiter - S->iters - 1) : 0.0;
+        const double rm = rr < 1.0 ? fmax(log(S->rho_new[j] / S->thr[j]), 0.0) / log(1.0 / rr) : left;
+        rh[j] = rr;
+        rem[j] = fmin(rm, left);
+    }
     __syncthreads();
     if (j == 0) {
         S->k = k;
         S->iters += 1;
-        double rho_hat = 0.0;
-        for (int q = 0; q < RMAX; q++) rho_hat = fmax(rho_hat, rh[q]);
+        double rho_hat = 0.0, remaining = 0.0;
+        for (int q = 0; q < RMAX; q++) {
+            rho_hat = fmax(rho_hat, rh[q]);
+            remaining = fmax(remaining, rem[q]);
+        }
+        if (k == EARLY_AT && S->can_escalate) {
+            S->probe_rho = rho_hat;
+            S->probe_rem = remaining;
+        }
         if (!any) S->done = 1;
         else if (S->iters >= S->maxiter) { S->err = 1; S->done = 1; }
         else if (k == PROBE_AT && S->can_escalate && rho_hat > PROBE_RATIO) { S->err = 2; S->done = 1; }
@@ -467,6 +490,10 @@ void precond(Absorber& a, const T* rr, T* z, int r, double* dpart, const uint32_
     precond_factors(a, rr, z, r, dpart, done, st);
 }
 
+// Accept the joint factor when its estimated cost is below this share of the projected PCG cost
+// (solve.rs:115-117: the cost model's margin).
+constexpr double JOINT_MARGIN = 0.8;
+
 struct PcgRun {
     uint64_t iterations = 0;
     bool exact = false;
@@ -480,7 +507,7 @@ struct PcgRun {
 template <class T>
 PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const double* ss1, const double* ss0,
                double tol, uint64_t maxiter, double rel2, const double* absthr, const fh_ctx* ctx,
-               bool can_escalate = false, uint64_t handover_at = 0, bool r_ready = false) {
+               bool can_escalate = false, uint64_t handover_at = 0, bool r_ready = false, JointProbe* jp = nullptr) {
     hipStream_t st = a.st;
     Scratch<T>& s = scr<T>(a);
     const size_t m = a.m, ms = a.ms, nv = m * r;
@@ -497,6 +524,8 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     h.can_escalate = can_escalate ? 1 : 0;
     h.fac = a.facs.empty() ? 0 : 1;
     h.handover_at = handover_at;
+    h.probe_rho = -1.0;
+    h.probe_rem = -1.0;
     for (int j = 0; j < r; j++) {
         h.ss1[j] = ss1[j];
         h.ss0[j] = ss0[j];
@@ -568,13 +597,17 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     struct Poll {
         uint32_t done, err;
         uint64_t iters;
+        double probe_rho, probe_rem;
     };
     static_assert(offsetof(PcgState, err) == offsetof(PcgState, done) + 4 &&
-                      offsetof(PcgState, iters) == offsetof(PcgState, done) + 8,
+                      offsetof(PcgState, iters) == offsetof(PcgState, done) + 8 &&
+                      offsetof(PcgState, probe_rem) == offsetof(PcgState, done) + 24,
                   "poll fields are contiguous");
     AsyncRead rd0(sizeof(Poll)), rd1(sizeof(Poll));
     AsyncRead* rd[2] = {&rd0, &rd1};
     const void* poll_src = &S->done;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool probed = jp == nullptr;
     enqueue(1);
     rd[0]->start(poll_src, st);
     Poll pl{};
@@ -586,6 +619,26 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
         if (ctx && ctx->interrupt && ctx->interrupt(ctx->user)) {
             stream_wait(st);
             throw Error(FH_INTERRUPTED, "interrupted");
+        }
+        if (!probed && pl.probe_rem >= 0.0) {
+            // the early probe: the projected remaining iterations at the measured time per
+            // iteration against the joint factor's estimated cost (solve.rs:932-975); the
+            // fast-converging rule of the reference first (rho < 0.5 and < 20 iterations)
+            probed = true;
+            if (!(pl.probe_rho < 0.5 && pl.probe_rem < 20.0)) {
+                const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                const double without = pl.probe_rem * el / (double)std::max<uint64_t>(pl.iters, 1);
+                if (jp->build_est <= JOINT_MARGIN * without) {
+                    jp->fired = true;
+                    jp->without = without;
+                    jp->rho = pl.probe_rho;
+                    jp->remaining = pl.probe_rem;
+                    // the iteration queued after this one completes too; x holds every finished step
+                    download(&pl, poll_src, sizeof(Poll), st);
+                    if (pl.done) jp->fired = false;
+                    break;
+                }
+            }
         }
     }
     out.iterations = pl.iters;
@@ -650,12 +703,14 @@ std::vector<double> dots64(Absorber& a, const double* x, const double* y, int r)
 }  // namespace
 
 PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
-                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate) {
+                   const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate,
+                   JointProbe* jp) {
     // a warm start may still be all zero: test it on the device
     bool nz = false;
     if (x_maybe_nonzero)
         for (double v : dots64(a, x, x, r)) nz |= v != 0.0;
-    PcgRun run = run_pcg<double>(a, rhs, x, nz, r, ss1, ss0, tol, maxiter, 0.0, nullptr, ctx, can_escalate);
+    PcgRun run = run_pcg<double>(a, rhs, x, nz, r, ss1, ss0, tol, maxiter, 0.0, nullptr, ctx, can_escalate, 0, false,
+                                 can_escalate ? jp : nullptr);
     PcgResult res;
     res.iterations = run.iterations;
     res.exact = run.exact;
@@ -665,12 +720,13 @@ PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, bool x_maybe_nonze
 }
 
 PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
-                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate) {
+                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate,
+                    JointProbe* jp) {
     PcgResult res;
     // exact dense path (nothing to iterate), or escalated sparse factors, whose grounded inverse of
     // a slowly mixing system is too ill-conditioned for FP32 sweeps (and is applied in FP64 only)
     if (a.ms == a.m || !a.facs.empty())
-        return pcg_fp64(a, rhs, x, x_maybe_nonzero, r, ss1, ss0, tol, maxiter, ctx, false);
+        return pcg_fp64(a, rhs, x, x_maybe_nonzero, r, ss1, ss0, tol, maxiter, ctx, can_escalate && a.ms < a.m, jp);
     ensure_w32(a);
     hipStream_t st = a.st;
     const size_t m = a.m, nv = m * r;
@@ -785,14 +841,14 @@ PcgResult pcg_mixed(Absorber& a, const double* rhs, double* x, bool x_maybe_nonz
         FH_CHECK_LAUNCH();
         PcgRun run = run_pcg<float>(a, s32.rhs.p, s32.x.p, false, r, ss1c.data(), ss0c.data(), tol,
                                     maxiter > total_iters ? maxiter - total_iters : 1, rel * rel, absthr.data(), ctx,
-                                    can_escalate && sweep == 0);
+                                    can_escalate && sweep == 0, 0, false, can_escalate && sweep == 0 ? jp : nullptr);
         total_iters += run.iterations;
         upload(mask.p, act.data(), r * sizeof(int32_t), st);
         hipLaunchKernelGGL(k_add_to64<float>, dim3(vgrid(nv)), dim3(TPB), 0, st, x, s32.x.p, nv, r, mask.p);
         FH_CHECK_LAUNCH();
         nz = true;
-        if (run.escalate) {  // x holds the progress so far; the caller restarts after escalating
-            res.escalate = true;
+        if (run.escalate || (jp && jp->fired)) {  // x holds the progress so far; the caller restarts
+            res.escalate = run.escalate;
             finished = true;
             break;
         }

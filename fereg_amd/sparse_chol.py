@@ -26,10 +26,10 @@ class SparseChol:
         check(L.fh_sparse_chol_factor(self.n, ptr.ctypes.data_as(_lib.u64p), col.ctypes.data_as(_lib.u32p),
                                       val.ctypes.data_as(_lib.f64p), int(nlast), ctypes.byref(h)))
         self._h = h
-        ln, gr = ctypes.c_uint64(), ctypes.c_uint64()
+        ln, gr, ht = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         fl = ctypes.c_double()
-        check(L.fh_sparse_chol_info(h, ctypes.byref(ln), ctypes.byref(fl), ctypes.byref(gr)))
-        self.len, self.flops, self.grounded = int(ln.value), float(fl.value), int(gr.value)
+        check(L.fh_sparse_chol_info(h, ctypes.byref(ln), ctypes.byref(fl), ctypes.byref(gr), ctypes.byref(ht)))
+        self.len, self.flops, self.grounded, self.height = int(ln.value), float(fl.value), int(gr.value), int(ht.value)
 
     @classmethod
     def from_dense(cls, a, nlast=0):

@@ -542,7 +542,10 @@ void fh_server_destroy(fh_server* s);
 typedef struct fh_sparse_chol fh_sparse_chol;
 int fh_sparse_chol_factor(size_t n, const uint64_t* ptr, const uint32_t* col, const double* val, size_t nlast,
                           fh_sparse_chol** out);
-int fh_sparse_chol_info(const fh_sparse_chol* f, uint64_t* len, double* flops, uint64_t* grounded);
+/* len: factor entries; flops: multiply-adds; grounded: grounded pivots; height: the elimination
+ * tree's height (levels of a triangular solve; nested dissection replaces minimum degree when it
+ * shortens the tree enough). */
+int fh_sparse_chol_info(const fh_sparse_chol* f, uint64_t* len, double* flops, uint64_t* grounded, uint64_t* height);
 int fh_sparse_chol_solve(const fh_sparse_chol* f, double* b, size_t nrhs);
 void fh_sparse_chol_destroy(fh_sparse_chol* f);
 

@@ -190,3 +190,28 @@ def test_fast_designs_do_not_escalate():
     rows = np.ascontiguousarray(v[a.row_order()])
     info = a.residualize(rows, 2, F.SolveOpts(tol=1e-10))
     assert not info.escalated and info.iterations < 12
+
+
+def test_early_probe_builds_the_joint_factor():
+    """The ring's projected PCG time at the iteration-6 probe exceeds the joint factor's estimated
+    cost (solve.rs:932-975), so the joint factor is built there and the restart is exact: fewer
+    iterations in total than an escalation at iteration 12 could give."""
+    import fereg_amd as F
+    from fereg_amd._lib import CallContext
+    codes, gs = _ring(1500)
+    n = len(codes[0])
+    rng = np.random.default_rng(13)
+    a = F.Absorber(codes, gs, small_limit=64)
+    v = rng.standard_normal((n, 2))
+    pi = a.row_order()
+    rows = np.ascontiguousarray(v[pi])
+    lines = []
+    announce("early probe")
+    info = a.residualize(rows, 2, F.SolveOpts(tol=1e-12, maxiter=100_000),
+                         ctx=CallContext(log=lines.append, verbose=2))
+    print("\n".join(lines))
+    assert info.escalated and info.iterations < 12, (info, lines)
+    assert any("early probe" in s for s in lines), lines
+    ref = dense_residualize(v, codes, gs)[pi]
+    scale = np.sqrt((v ** 2).sum(0))
+    assert np.max(np.abs(rows - ref) / scale) <= 1e-9, info
