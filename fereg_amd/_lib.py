@@ -14,7 +14,7 @@ FH_TOO_FEW = 2001
 FH_INTERNAL = 9999
 FH_PLAIN_FP64 = 0
 FH_MIXED = 1
-FH_VCOV = {"iid": 0, "hetero": 1, "cluster": 2}
+FH_VCOV = {"iid": 0, "hetero": 1, "cluster": 2, "hc2": 3, "hc3": 4}
 FH_KFIXEF = {"none": 0, "full": 1, "nonnested": 2}
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -300,6 +300,7 @@ SIGNATURES = [
     ("fh_sparse_chol_factor", ctypes.c_int, [ctypes.c_size_t, u64p, u32p, f64p, ctypes.c_size_t, ctypes.POINTER(vp)]),
     ("fh_sparse_chol_info", ctypes.c_int, [vp, u64p, f64p, u64p, u64p]),
     ("fh_sparse_chol_solve", ctypes.c_int, [vp, f64p, ctypes.c_size_t]),
+    ("fh_sparse_chol_inverse_entries", ctypes.c_int, [vp, ctypes.c_size_t, u32p, u32p, f64p]),
     ("fh_sparse_chol_destroy", None, [vp]),
     ("fh_device_alloc", ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),
     ("fh_device_free", ctypes.c_int, [vp]),

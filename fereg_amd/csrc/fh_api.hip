@@ -351,9 +351,9 @@ ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t m
         // matrix when the rate is slow (cg.rs:254-274, solve.rs:932-975); either way the solve
         // restarts from the current iterate
         JointProbe jp;
-        jp.build_est = escalation_possible(a) ? joint_build_estimate(a) : INFINITY;
         uint64_t used = 0;
-        PcgResult pr = solve(warm, escalation_possible(a), maxiter, jp.build_est < INFINITY ? &jp : nullptr);
+        const bool joint_ok = escalation_possible(a) && a.joint_policy != 0 && !a.fac_failed_joint;
+        PcgResult pr = solve(warm, escalation_possible(a), maxiter, joint_ok ? &jp : nullptr);
         if (jp.fired) {
             used += pr.iterations;
             out.escalated = escalate_joint(a, ctx, jp.without);
@@ -471,7 +471,8 @@ struct WbPre {
 void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
                const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
                const ClusterSet& cs, const fh_ssc& ssc, double dfk, double iid_scale,
-               std::vector<double>& V, uint64_t* ncl, WbPre* wb) {
+               std::vector<double>& V, uint64_t* ncl, WbPre* wb, bool glm) {
+    require(kind >= FH_VCOV_IID && kind <= FH_VCOV_HC3, "unknown variance kind " + std::to_string(kind));
     const int K = R - 1;
     std::vector<int> kept;
     for (int j = 0; j < K; j++)
@@ -490,6 +491,14 @@ void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>&
         if (kind == FH_VCOV_HETERO) {
             hetero_meat(a, rows, a.n, ld, 1, K, e, w, meat.data());
             if (ssc.k_adj) factor = n / (n - dfk);
+        } else if (kind == FH_VCOV_HC2 || kind == FH_VCOV_HC3) {
+            // fit/vcov.rs:333-364, fixest flavour: the scores over sqrt(1 - h) or (1 - h) with the
+            // full model's leverage, no small-sample factor; the regressor part of the leverage
+            // carries the weight except in an OLS without absorbed terms (fit/vcov.rs:235)
+            DBuf<double> es(a.n);
+            hc_scores(a, rows, ld, kept, bread, ecol >= 0 ? rows + ecol : e, ecol >= 0 ? (size_t)ld : 1, w,
+                      glm || a.nterms > 0, kind == FH_VCOV_HC3, es.p);
+            hetero_meat(a, rows, a.n, ld, 1, K, es.p, w, meat.data());
         } else if (cs.dims.size() > 1) {
             // several dimensions (fit/vcov.rs:366-380): G.df "min" scales by the smallest G once,
             // "conventional" every subset meat by its own G/(G-1)

@@ -602,6 +602,25 @@ void mask_factored(Absorber& a) {
 
 }  // namespace
 
+bool factor_joint_host(Absorber& a, LdlFactor& f, std::vector<uint32_t>& keep, std::string& why) {
+    SymLower S;
+    if (!assemble(a, 0, a.m, S, keep)) {
+        why = "the reduced matrix exceeds the assembly budgets";
+        return false;
+    }
+    size_t nlast = 0;
+    while (nlast < keep.size() && keep[nlast] < a.ms) nlast++;
+    if (!ldl_analyze(S, nlast, ESC_MAX_LEN, ESC_MAX_FLOPS, f)) {
+        why = "the factor exceeds its fill budget (" + std::to_string(f.len()) + " entries)";
+        return false;
+    }
+    if (!ldl_numeric(S, f)) {
+        why = "a non-finite pivot";
+        return false;
+    }
+    return true;
+}
+
 bool escalation_possible(const Absorber& a) {
     if (!a.sparse_chol || a.fac_failed || a.m <= a.ms) return false;
     return a.facs.empty() || !a.fac_fresh;

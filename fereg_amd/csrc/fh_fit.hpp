@@ -139,7 +139,7 @@ uint64_t multiway_meat(Absorber& a, const double* z, int ld, int c0, int k, cons
 void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
                const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
                const ClusterSet& cs, const fh_ssc& ssc, double dfk, double iid_scale,
-               std::vector<double>& V, uint64_t* ncl, WbPre* wb = nullptr);
+               std::vector<double>& V, uint64_t* ncl, WbPre* wb = nullptr, bool glm = false);
 
 // ------------------------------------------------------------------ factors (fh_factor.hip)
 

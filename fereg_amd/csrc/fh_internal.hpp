@@ -83,6 +83,25 @@ struct SpmvPlan {
 struct ClusterRuns;
 struct FacGpu;  // fh_escalate.hip
 
+// The PCG's status as the device publishes it to pinned host memory after every iteration
+// (zero-copy: the host polls it without a copy kernel); iters_done (the iteration count, the done
+// flag in bit 63) is written last.
+struct PollMirror {
+    uint32_t err, pad;
+    uint64_t iters_done;
+    double probe_rho, probe_rem;
+};
+// Its pinned host allocation (fh_pcg.hip), one per absorber.
+struct HostPoll {
+    PollMirror* h = nullptr;  // host view
+    PollMirror* d = nullptr;  // device view of the same memory
+    HostPoll() = default;
+    HostPoll(const HostPoll&) = delete;
+    HostPoll& operator=(const HostPoll&) = delete;
+    ~HostPoll();
+    void ensure();
+};
+
 struct Absorber {
     int device = 0;
     hipStream_t st = nullptr;
@@ -168,6 +187,7 @@ struct Absorber {
     // the nonzeros, factor entries and multiply-adds once it was assembled and analyzed
     double joint_pairs = -1.0, joint_nnz = -1.0, joint_len = -1.0, joint_flops = -1.0;
     uint32_t mixed_lead = MIXED_LEAD_DEFAULT;  // most FP64 iterations of a mixed solve before its FP32 sweeps
+    HostPoll poll;  // the PCG status mirror
 
     bool trivial() const { return cell_obs_off.n == 0; }
     bool unit() const { return !weighted && trivial(); }
@@ -203,6 +223,21 @@ inline bool small_dense_rule(const Absorber& a) {
 bool escalation_possible(const Absorber& a);
 bool escalate(Absorber& a, const fh_ctx* ctx);  // false: no factor within the budgets (Jacobi stays)
 void facs_after_refresh(Absorber& a);
+// The exact LDL^T of the joint reduced matrix for the current weights on the host (assembled on the
+// GPU, small coordinates last, the escalation's budgets); keep: the factored (kept) coordinates.
+// False with the reason when a budget is exceeded.
+struct LdlFactor;
+bool factor_joint_host(Absorber& a, LdlFactor& f, std::vector<uint32_t>& keep, std::string& why);
+// Leverage of the fixed effects per cell: hcell[c] = d'(D'WD)^+ d / w for the FE indicators d of
+// cell c's rows (leverage.rs:1-30), as 1/n1_s + u_c' S^+ u_c with u_c the FE1-demeaned reduced
+// coordinates of cell c, S^+ by selected inversion of the joint factor (fh_leverage.hip).
+void fe_leverage_cells(Absorber& a, double* hcell_dev, const fh_ctx* ctx);
+// HC2 / HC3 scores: e_out = e / sqrt(1 - h) (HC2) or e / (1 - h) (HC3) with the full model's
+// leverage h = c (hcell + x~' B x~), c = w (row_weight) or 1, over the kept regressors (columns
+// 1 + kept[j] of the rows, stride ld; B = bread over them). Throws when a leverage is 1
+// (fit/vcov.rs:337-351: fixest's check at 1 - sqrt(eps)).
+void hc_scores(Absorber& a, const double* rows, int ld, const std::vector<int>& kept, const std::vector<double>& bread,
+               const double* e, size_t es, const double* w, bool row_weight, bool hc3, double* e_out);
 bool facs_exact(const Absorber& a);  // a joint factor for the current weights: M = S
 // The joint factor's estimated cost in seconds from what is known of it so far (a lower bound
 // before it was first assembled), or INFINITY when the joint factor is not available.
@@ -317,10 +352,9 @@ struct PcgResult {
     bool escalate = false;       // stopped at the escalation probe (x holds the progress so far)
 };
 // The early probe's joint-factor rule (solve.rs:932-975): a solve that may escalate stops after its
-// iteration-6 probe when the joint factor's estimated cost (build_est, seconds) is within the
-// margin of the projected remaining PCG time (fired; x holds the progress so far).
+// iteration-6 probe when the joint factor's estimated cost (joint_build_estimate, seconds) is
+// within the margin of the projected remaining PCG time (fired; x holds the progress so far).
 struct JointProbe {
-    double build_est = INFINITY;
     bool fired = false;
     double without = 0.0, rho = 0.0, remaining = 0.0;
 };

@@ -429,7 +429,8 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, const dou
     const double dfk = df_k(a, kind, ssc, nested, np, mk);
     uint64_t ncl = 0;
     std::vector<double> V;
-    vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, a.w_rows.p, kind, cs, ssc, dfk, 1.0, V, &ncl);
+    vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, a.w_rows.p, kind, cs, ssc, dfk, 1.0, V, &ncl, nullptr,
+              true);
     for (size_t j = 0; j < K; j++) {
         beta[j] = 0.0;
         kept_out[j] = !excluded[j];

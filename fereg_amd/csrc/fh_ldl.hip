@@ -487,6 +487,41 @@ void ldl_solve_host(const LdlFactor& f, double* b, size_t nrhs, size_t ld) {
         for (size_t c = 0; c < nrhs; c++) b[(size_t)f.perm[k] * ld + c] = f.scale[k] * y[k * nrhs + c];
 }
 
+double selinv_at(const LdlFactor& f, const std::vector<double>& zd, const std::vector<double>& zx, uint32_t i,
+                 uint32_t j) {
+    if (i == j) return zd[i];
+    if (i < j) std::swap(i, j);  // lower triangle: column j, row i > j
+    const uint32_t* b = f.li.data() + f.lp[j];
+    const uint32_t* e = f.li.data() + f.lp[j + 1];
+    const uint32_t* p = std::lower_bound(b, e, i);
+    return (p != e && *p == i) ? zx[f.lp[j] + (p - b)] : 0.0;
+}
+
+void ldl_selected_inverse(const LdlFactor& f, std::vector<double>& zd, std::vector<double>& zx) {
+    const size_t n = f.n;
+    zd.assign(n, 0.0);
+    zx.assign(f.len(), 0.0);
+    // columns from the last: Z_ij = -sum_k L_kj Z_ik and Z_jj = 1/d_j - sum_k L_kj Z_kj over the
+    // rows k of column j; every Z_ik needed lies in columns > j, already done
+    std::vector<double> acc;
+    for (size_t j = n; j-- > 0;) {
+        const uint64_t p0 = f.lp[j], p1 = f.lp[j + 1];
+        acc.assign(p1 - p0, 0.0);
+        for (uint64_t a = p0; a < p1; a++) {
+            const uint32_t i = f.li[a];
+            double s = 0.0;
+            for (uint64_t b = p0; b < p1; b++) s += f.lx[b] * selinv_at(f, zd, zx, i, f.li[b]);
+            acc[a - p0] = -s;
+        }
+        double dj = 1.0 / f.d[j];
+        for (uint64_t a = p0; a < p1; a++) {
+            zx[a] = acc[a - p0];
+            dj -= f.lx[a] * zx[a];
+        }
+        zd[j] = dj;
+    }
+}
+
 }  // namespace fh
 
 // ------------------------------------------------------------------ C ABI (host only)
@@ -556,6 +591,29 @@ int fh_sparse_chol_solve(const fh_sparse_chol* f, double* b, size_t nrhs) {
     }
     fh::ldl_solve_host(f->f, b, nrhs, nrhs);
     return FH_OK;
+}
+
+int fh_sparse_chol_inverse_entries(const fh_sparse_chol* f, size_t npairs, const uint32_t* rows,
+                                   const uint32_t* cols, double* out) {
+    try {
+        fh::require(f != nullptr && (npairs == 0 || (rows && cols && out)), "fh_sparse_chol_inverse_entries: null argument");
+        std::vector<double> zd, zx;
+        fh::ldl_selected_inverse(f->f, zd, zx);
+        std::vector<uint32_t> pos(f->f.n);
+        for (size_t k = 0; k < f->f.n; k++) pos[f->f.perm[k]] = (uint32_t)k;
+        for (size_t q = 0; q < npairs; q++) {
+            fh::require(rows[q] < f->f.n && cols[q] < f->f.n, "fh_sparse_chol_inverse_entries: index out of range");
+            const uint32_t i = pos[rows[q]], j = pos[cols[q]];
+            out[q] = f->f.scale[i] * f->f.scale[j] * fh::selinv_at(f->f, zd, zx, i, j);
+        }
+        return FH_OK;
+    } catch (const fh::Error& e) {
+        fh::set_last_error(e.what());
+        return e.rc;
+    } catch (const std::exception& e) {
+        fh::set_last_error(e.what());
+        return FH_INTERNAL;
+    }
 }
 
 void fh_sparse_chol_destroy(fh_sparse_chol* f) { delete f; }

@@ -59,4 +59,13 @@ bool ldl_numeric(const SymLower& a, LdlFactor& f);
 // x = A^+ b in place on the host (matrix index space), nrhs columns with row stride ld.
 void ldl_solve_host(const LdlFactor& f, double* b, size_t nrhs, size_t ld);
 
+// Selected inverse (Takahashi, Yamamoto and Kanenaga's recurrences, the form leverage.rs:22-26
+// uses): Z = L^-T D^-1 L^-1 of the scaled permuted matrix on the pattern of L, zd its diagonal
+// and zx its strictly lower entries at L's positions (column j, rows li[lp[j]..lp[j+1])). Every
+// pair of indices that share a structural entry of A is in that pattern.
+void ldl_selected_inverse(const LdlFactor& f, std::vector<double>& zd, std::vector<double>& zx);
+// Z at positions (i, j) (any order) of the selected inverse; 0 outside the pattern.
+double selinv_at(const LdlFactor& f, const std::vector<double>& zd, const std::vector<double>& zx, uint32_t i,
+                 uint32_t j);
+
 }  // namespace fh

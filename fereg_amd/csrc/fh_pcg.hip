@@ -7,6 +7,8 @@
 // system, refining in FP64 (residual rhs - S x recomputed with the FP64 operator after every
 // inner solve; a column is accepted only on that FP64 residual).
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -164,7 +166,18 @@ __global__ void k_add_to64(double* __restrict__ x, const TI* __restrict__ d, siz
 }
 
 // Start of a (re)start: rho = r'z, inactive when rho <= 0 (cg.rs:199-205).
-__global__ __launch_bounds__(TPB) void k_start(PcgState* S, const double* __restrict__ dpart) {
+// The poll fields to the host's pinned mirror: the data, then after a system-scope fence the word
+// the host waits on (the iteration count with the done flag in its top bit), so a host that sees
+// that word sees the data.
+__device__ void publish(const PcgState* S, PollMirror* m) {
+    m->err = S->err;
+    m->probe_rho = S->probe_rho;
+    m->probe_rem = S->probe_rem;
+    __threadfence_system();
+    *(volatile uint64_t*)&m->iters_done = S->iters | ((uint64_t)(S->done != 0) << 63);
+}
+
+__global__ __launch_bounds__(TPB) void k_start(PcgState* S, const double* __restrict__ dpart, PollMirror* mirror) {
     __shared__ double sums[NSLOT_MAX][RMAX];
     const int sl[3] = {S_RZL, S_RZS, S_RZF};
     slot_sums(dpart, sl, S->fac ? 3 : 2, S->r, sums);
@@ -182,6 +195,7 @@ __global__ __launch_bounds__(TPB) void k_start(PcgState* S, const double* __rest
         for (int c = 0; c < S->r; c++) any |= S->active[c];
         S->done = any ? 0u : 1u;
         S->k = 0;
+        if (!any && mirror) publish(S, mirror);
     }
 }
 
@@ -296,8 +310,8 @@ __global__ __launch_bounds__(TPB) void k_update(T* __restrict__ x, T* __restrict
 }
 
 // Stopping rule and beta (cg.rs:237-289).
-__global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restrict__ dpart) {
-    if (S->done) return;
+// Called by every thread of one 256-thread workgroup.
+__device__ void stop_rule(PcgState* S, const double* __restrict__ dpart, PollMirror* mirror) {
     __shared__ double sums[NSLOT_MAX][RMAX];
     const int sl[5] = {S_RZL, S_RZS, S_XB, S_XR, S_RZF};
     slot_sums(dpart, sl, S->fac ? 5 : 4, S->r, sums);
@@ -385,7 +399,13 @@ __global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restr
         else if (S->iters >= S->maxiter) { S->err = 1; S->done = 1; }
         else if (k == PROBE_AT && S->can_escalate && rho_hat > PROBE_RATIO) { S->err = 2; S->done = 1; }
         else if (S->handover_at && (k >= S->handover_at || (!not_ready && !soon))) { S->err = 3; S->done = 1; }
+        if (mirror) publish(S, mirror);
     }
+}
+
+__global__ __launch_bounds__(TPB) void k_stop(PcgState* S, const double* __restrict__ dpart, PollMirror* mirror) {
+    if (S->done) return;
+    stop_rule(S, dpart, mirror);
 }
 
 template <class T>
@@ -562,7 +582,15 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
         out.exact = true;
         return out;
     }
-    hipLaunchKernelGGL(k_start, dim3(1), dim3(TPB), 0, st, S, s.dpart.p);
+    a.poll.ensure();
+    volatile PollMirror* mh = a.poll.h;
+    mh->err = 0;
+    mh->probe_rho = -1.0;
+    mh->probe_rem = -1.0;
+    mh->iters_done = 0;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    PollMirror* md = a.poll.d;
+    hipLaunchKernelGGL(k_start, dim3(1), dim3(TPB), 0, st, S, s.dpart.p, md);
     hipLaunchKernelGGL(k_mask_copy<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, s.z.p, s.p.p, nv, r, S);
     FH_CHECK_LAUNCH();
     WeightArrays<T>& W = warr<T>(a);
@@ -585,15 +613,16 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
                 }
             }
             precond_factors(a, (const T*)s.r.p, s.z.p, r, s.dpart.p, done, st);
-            hipLaunchKernelGGL(k_stop, dim3(1), dim3(TPB), 0, st, S, s.dpart.p);
+            hipLaunchKernelGGL(k_stop, dim3(1), dim3(TPB), 0, st, S, s.dpart.p, md);
             hipLaunchKernelGGL(k_update_p<T>, dim3(vgrid(nv)), dim3(TPB), 0, st, s.p.p, s.z.p, nv, r, S);
             FH_CHECK_LAUNCH();
         }
     };
     // Pipelined polling: the status after iteration k is read while iteration k+1 is already
-    // queued, so the device never waits for the host's decision. Once the stopping rule has fired
-    // every kernel of an iteration returns at once, so the one iteration queued past the end costs
-    // only its launches.
+    // queued, so the device never waits for the host's decision. The device publishes it to pinned
+    // host memory (no copy kernel between iterations). Once the stopping rule has fired every
+    // kernel of an iteration returns at once, so the one iteration queued past the end costs only
+    // its launches.
     struct Poll {
         uint32_t done, err;
         uint64_t iters;
@@ -603,18 +632,40 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
                       offsetof(PcgState, iters) == offsetof(PcgState, done) + 8 &&
                       offsetof(PcgState, probe_rem) == offsetof(PcgState, done) + 24,
                   "poll fields are contiguous");
-    AsyncRead rd0(sizeof(Poll)), rd1(sizeof(Poll));
-    AsyncRead* rd[2] = {&rd0, &rd1};
-    const void* poll_src = &S->done;
+    static_assert(sizeof(Poll) == sizeof(PollMirror), "the mirror holds the poll fields");
+    // Waits until the device has published iteration `target` (or the end). A status that does not
+    // come while the stream has drained (nothing left that could publish it) is read from the
+    // device state instead.
+    auto wait_for = [&](uint64_t target) {
+        Poll p{};
+        auto t_spin = std::chrono::steady_clock::now();
+        constexpr uint64_t DONE = 1ull << 63;
+        for (uint64_t spins = 0;; spins++) {
+            const uint64_t w = mh->iters_done;
+            if ((w & ~DONE) >= target || (w & DONE)) {
+                std::atomic_thread_fence(std::memory_order_acquire);
+                p.done = (w & DONE) ? 1u : 0u;
+                p.err = mh->err;
+                p.iters = w & ~DONE;
+                p.probe_rho = mh->probe_rho;
+                p.probe_rem = mh->probe_rem;
+                return p;
+            }
+            if ((spins & 1023) == 1023 &&
+                std::chrono::steady_clock::now() - t_spin > std::chrono::milliseconds(20) &&
+                hipStreamQuery(st) == hipSuccess) {
+                download(&p, &S->done, sizeof(Poll), st);
+                return p;
+            }
+        }
+    };
     const auto t0 = std::chrono::steady_clock::now();
     bool probed = jp == nullptr;
     enqueue(1);
-    rd[0]->start(poll_src, st);
     Poll pl{};
-    for (int k = 0;; k++) {
+    for (uint64_t k = 1;; k++) {
         enqueue(1);
-        rd[(k + 1) & 1]->start(poll_src, st);
-        pl = *static_cast<const Poll*>(rd[k & 1]->wait());
+        pl = wait_for(k);
         if (pl.done) break;
         if (ctx && ctx->interrupt && ctx->interrupt(ctx->user)) {
             stream_wait(st);
@@ -628,13 +679,13 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
             if (!(pl.probe_rho < 0.5 && pl.probe_rem < 20.0)) {
                 const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 const double without = pl.probe_rem * el / (double)std::max<uint64_t>(pl.iters, 1);
-                if (jp->build_est <= JOINT_MARGIN * without) {
+                if (joint_build_estimate(a) <= JOINT_MARGIN * without) {
                     jp->fired = true;
                     jp->without = without;
                     jp->rho = pl.probe_rho;
                     jp->remaining = pl.probe_rem;
                     // the iteration queued after this one completes too; x holds every finished step
-                    download(&pl, poll_src, sizeof(Poll), st);
+                    download(&pl, &S->done, sizeof(Poll), st);
                     if (pl.done) jp->fired = false;
                     break;
                 }
@@ -701,6 +752,37 @@ std::vector<std::vector<double>> dots64(Absorber& a, std::initializer_list<std::
 std::vector<double> dots64(Absorber& a, const double* x, const double* y, int r) { return dots64(a, {{x, y}}, r)[0]; }
 
 }  // namespace
+
+// Mirrors come from pinned pages shared by the process (a pinned allocation costs far more than a
+// PCG iteration, and absorbers are often created per call): a slot is taken on an absorber's first
+// solve and returned when it is destroyed.
+namespace {
+std::mutex g_poll_mu;
+std::vector<std::pair<PollMirror*, PollMirror*>> g_poll_free;  // (host, device) views
+}  // namespace
+
+HostPoll::~HostPoll() {
+    if (!h) return;
+    std::lock_guard<std::mutex> lk(g_poll_mu);
+    g_poll_free.emplace_back(h, d);
+}
+
+void HostPoll::ensure() {
+    if (h) return;
+    std::lock_guard<std::mutex> lk(g_poll_mu);
+    if (g_poll_free.empty()) {
+        constexpr size_t SLOTS = 64;
+        void* p = nullptr;
+        FH_HIP(hipHostMalloc(&p, SLOTS * sizeof(PollMirror), hipHostMallocCoherent | hipHostMallocMapped));
+        void* dp = nullptr;
+        FH_HIP(hipHostGetDevicePointer(&dp, p, 0));
+        for (size_t i = SLOTS; i-- > 0;)
+            g_poll_free.emplace_back(static_cast<PollMirror*>(p) + i, static_cast<PollMirror*>(dp) + i);
+    }
+    h = g_poll_free.back().first;
+    d = g_poll_free.back().second;
+    g_poll_free.pop_back();
+}
 
 PcgResult pcg_fp64(Absorber& a, const double* rhs, double* x, bool x_maybe_nonzero, int r, const double* ss1,
                    const double* ss0, double tol, uint64_t maxiter, const fh_ctx* ctx, bool can_escalate,

@@ -49,6 +49,16 @@ class SparseChol:
         check(lib().fh_sparse_chol_solve(self._h, x.ctypes.data_as(_lib.f64p), nrhs))
         return x
 
+    def inverse_entries(self, rows, cols):
+        """Entries of the generalized inverse at (rows[q], cols[q]) by selected inversion (exact on
+        every pair that shares a structural entry of the matrix; 0 elsewhere)."""
+        rows = np.ascontiguousarray(rows, dtype=np.uint32)
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        out = np.zeros(len(rows))
+        check(lib().fh_sparse_chol_inverse_entries(self._h, len(rows), rows.ctypes.data_as(_lib.u32p),
+                                                   cols.ctypes.data_as(_lib.u32p), out.ctypes.data_as(_lib.f64p)))
+        return out
+
     def close(self):
         if getattr(self, "_h", None):
             lib().fh_sparse_chol_destroy(self._h)

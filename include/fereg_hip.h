@@ -206,6 +206,8 @@ void fh_tune_cache_clear(void);
 #define FH_VCOV_IID 0
 #define FH_VCOV_HETERO 1
 #define FH_VCOV_CLUSTER 2
+#define FH_VCOV_HC2 3 /* fit/mod.rs:108-118: scores / sqrt(1 - h), the full model's exact leverage */
+#define FH_VCOV_HC3 4 /* scores / (1 - h) */
 
 /* fixest small-sample rules (fit/vcov.rs:63-112, ssc of fit/mod.rs). */
 #define FH_KFIXEF_NONE 0
@@ -547,6 +549,11 @@ int fh_sparse_chol_factor(size_t n, const uint64_t* ptr, const uint32_t* col, co
  * shortens the tree enough). */
 int fh_sparse_chol_info(const fh_sparse_chol* f, uint64_t* len, double* flops, uint64_t* grounded, uint64_t* height);
 int fh_sparse_chol_solve(const fh_sparse_chol* f, double* b, size_t nrhs);
+/* Entries (rows[q], cols[q]) of the same generalized inverse by selected inversion on the factor's
+ * pattern (leverage.rs:22-26: Takahashi recurrences; every pair sharing a structural entry of the
+ * matrix is covered, other pairs read 0). */
+int fh_sparse_chol_inverse_entries(const fh_sparse_chol* f, size_t npairs, const uint32_t* rows,
+                                   const uint32_t* cols, double* out);
 void fh_sparse_chol_destroy(fh_sparse_chol* f);
 
 /* Device memory helpers for callers without another allocator. */

@@ -7,7 +7,9 @@ checker. Follows fereg-core's fit layer (fixest conventions):
   regress       fit/design.rs:64-95 (coefficients from the Gram matrix)
   nparams       fit/ols.rs:27-37
   df_k          fit/vcov.rs:63-112 (K.adj, K.fixef none/full/nonnested)
-  vcov          fit/vcov.rs:316-382 (iid, hetero, one-way cluster)
+  vcov          fit/vcov.rs:316-382 (iid, hetero, HC2/HC3, one- to multiway cluster)
+  hat_values    leverage.rs:1-30 (the exact leverage of the full model, here by brute force on the
+                explicit design: diag of W^1/2 X (X'WX)^+ X' W^1/2 over [X, every FE dummy])
   poisson IRLS  glm/irls.rs:367-913 with glm/family.rs:103-145 (fixest flavour)
 """
 import itertools
@@ -114,10 +116,33 @@ def posdef_fix(V):
     return (U * np.maximum(lam, 1e-16)) @ U.T
 
 
-def vcov(kind, bread, X, e, w, n, dfk, cluster=None, k_adj=True, g_adj=True, g_df="min"):
-    """fit/vcov.rs:316-382: iid, hetero, CRV1 over one or several cluster dimensions
-    (inclusion-exclusion over the subsets of the dimensions, vcov.rs:480-558; G.df "min" or
-    "conventional"), then fixest's PSD repair."""
+def fe_design(codes, gs, n, intercept=False):
+    """The explicit dummies of every absorbed term (and a constant when asked), n x sum(g)."""
+    cols = [np.ones((n, 1))] if intercept else []
+    for c, g in zip(codes, gs):
+        d = np.zeros((n, g))
+        d[np.arange(n), np.asarray(c)] = 1.0
+        cols.append(d)
+    return np.concatenate(cols, axis=1) if cols else np.zeros((n, 0))
+
+
+def hat_values(design, w=None, row_weight=True):
+    """h_i = w_i x_i' (X'WX)^+ x_i for the full design X (leverage.rs:1-30 defines the FE part the
+    same way; with the regressors the FWL split h = h_FE + w x~'(X~'WX~)^-1 x~ holds): the
+    diagonal of the weighted projection, by a pseudo-inverse of the explicit design.
+    row_weight=False leaves the leading w_i out: fixest's OLS without absorbed terms
+    (fit/vcov.rs:235, `weighted` is false for a t-inference fit with no fixed effect)."""
+    sw = np.sqrt(w) if w is not None else np.ones(design.shape[0])
+    a = design * sw[:, None]
+    h = np.einsum("ij,ji->i", a, np.linalg.pinv(a, rcond=1e-12))
+    return h if row_weight or w is None else h / w
+
+
+def vcov(kind, bread, X, e, w, n, dfk, cluster=None, k_adj=True, g_adj=True, g_df="min", h=None):
+    """fit/vcov.rs:316-382: iid, hetero, HC2 / HC3 (scores over sqrt(1 - h) or (1 - h) with the
+    full model's leverage h, no small-sample factor: fit/vcov.rs:333-364, fixest flavour), CRV1
+    over one or several cluster dimensions (inclusion-exclusion over the subsets of the
+    dimensions, vcov.rs:480-558; G.df "min" or "conventional"), then fixest's PSD repair."""
     if kind == "iid":
         ssr = float(np.sum((w if w is not None else 1.0) * e * e))
         factor = (n - 1) / (n - dfk) if k_adj else 1.0
@@ -127,6 +152,12 @@ def vcov(kind, bread, X, e, w, n, dfk, cluster=None, k_adj=True, g_adj=True, g_d
         meat = s.T @ s
         factor = n / (n - dfk) if k_adj else 1.0
         return posdef_fix(factor * bread @ meat @ bread)
+    if kind in ("hc2", "hc3"):
+        if np.any(h > 1.0 - np.sqrt(np.finfo(float).eps)):
+            raise ValueError("a leverage equals 1")
+        d = (1.0 - h) if kind == "hc3" else np.sqrt(1.0 - h)
+        sh = s / d[:, None]
+        return posdef_fix(bread @ (sh.T @ sh) @ bread)
     if kind == "cluster":
         dims = cluster if isinstance(cluster, (list, tuple)) else [cluster]
         q = len(dims)

@@ -239,7 +239,13 @@ def ppml(codes, g, y, X, weights=None, offset=None, tol=1e-8, itol=1e-10, maxite
     else:
         dims = [] if cluster is None else (cluster if isinstance(cluster, (list, tuple)) else [cluster])
         cl = [np.asarray(c)[pi] for c in dims] or None
-        V = estimate.vcov(vcov, bread, Xt, e, w, n, dfk, cluster=cl, k_adj=k_adj, g_adj=g_adj)
+        h = None
+        if vcov in ("hc2", "hc3"):
+            # the full model's leverage at the last IRLS weights (fit/glm.rs:588-599; a GLM keeps
+            # the weight in the regressor part even without absorbed terms)
+            D = estimate.fe_design([np.asarray(c)[pi] for c in codes], g, n, intercept=not len(codes))
+            h = estimate.hat_values(np.concatenate([xr[:, kept], D], axis=1), w)
+        V = estimate.vcov(vcov, bread, Xt, e, w, n, dfk, cluster=cl, k_adj=k_adj, g_adj=g_adj, h=h)
     Vf = np.zeros((K, K))
     Vf[np.ix_(kept, kept)] = V
     return {"coef": beta, "vcov": Vf, "se": np.sqrt(np.maximum(np.diag(Vf), 0.0)), "deviance": dev,

@@ -7,6 +7,9 @@ Sources (fixtures of crates/fereg-core/tests, written there by fixest 0.14.2):
   tests/data/fit_d1.csv, fit_toy.csv          the input panels
   tests/data/fit_fixest.json                  OLS cases (tests/fit_ols.rs:10-67, RTOL 1e-8)
   tests/data/fit_fixest_glm.json              Poisson cases (tests/fit_glm.rs)
+  tests/data/fit_se.csv, fit_se_fixest.json   HC2 / HC3 / CRV3 cases (fixest 0.14.2 through
+                                              tests/data/gen_se_fixtures.R; OLS and Poisson,
+                                              intercept-only absorbed terms, no IV)
   tests/data/ols_data.csv, ols_s*.json        reghdfe 6.13.1 ground truth (tests/ols.rs:765-935,
                                               written by tests/data/gen_ols_fixtures.do); the
                                               model of each fixture restates the table of
@@ -78,6 +81,22 @@ def main():
                    "weight": weight, "singletons": sing, "noconstant": nocons, "dof": dof, "status": status,
                    "expect": {k: j[k] for k in REGHDFE_KEEP if k in j}})
     json.dump(rg, open(os.path.join(DST, "reghdfe_ols.json"), "w"))
+    shutil.copyfile(os.path.join(SRC, "fit_se.csv"), os.path.join(DST, "fit_se.csv"))
+    nrows = sum(1 for _ in open(os.path.join(SRC, "fit_se.csv"))) - 1
+    se = []
+    for c in json.load(open(os.path.join(SRC, "fit_se_fixest.json"))):
+        if c["vcov"] not in ("hc2", "hc3", "crv3") or c["model"] not in ("ols", "poisson") or c["endog"]:
+            continue
+        if any(t["slopes"] or not t["intercept"] for t in c["fe"]) or c["name"].startswith("iv_"):
+            continue
+        e = {k: c["expect"][k] for k in ["coef", "se", "vcov", "df_k", "nobs", "collin", "G", "coef_names"]
+             if k in c["expect"]}
+        if e["nobs"] != nrows:
+            continue  # every case here keeps all rows; the fixture carries no row list
+        e["obs"] = list(range(1, nrows + 1))
+        se.append({k: c[k] for k in ["name", "data", "model", "y", "x", "fe", "intercept", "weights", "vcov",
+                                      "cluster", "ssc"]} | {"expect": e})
+    json.dump(se, open(os.path.join(DST, "fixest_se.json"), "w"), indent=0)
     print(len(out), "ols cases;", len(keep), "poisson cases;", len(rg), "reghdfe cases")
 
 

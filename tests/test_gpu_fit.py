@@ -152,3 +152,80 @@ def test_fit_layer_from_raw_columns(case, model):
     V = res.vcov[np.ix_(res.kept, res.kept)]
     np.testing.assert_allclose(V.reshape(-1), np.array(e["vcov"], float).reshape(-1),
                                rtol=1e-8 if model == "ols" else 1e-6)
+
+
+SE_HC = [c for c in cases("fixest_se.json") if c["vcov"] in ("hc2", "hc3")]
+
+
+@pytest.mark.parametrize("case", SE_HC, ids=lambda c: c["name"])
+def test_fixest_hc2_hc3(case):
+    """HC2 / HC3 with the exact leverage of the full model (selected inversion of the joint
+    reduced matrix on the GPU path) against fixest 0.14.2 (fit_se_fixest.json); without absorbed
+    terms the regressors' block of fixest's matrix (its intercept is the absorber's constant)."""
+    import fereg_amd as F
+    inp = case_inputs(case)
+    codes, gs = inp["codes"], inp["g"]
+    announce("hc", case["name"])
+    n = len(inp["y"])
+    # without absorbed terms the absorber's one level is the constant (it needs the sample size)
+    ab = F.Absorber(codes, gs, weights=inp["w"] if inp["w"] is not None or codes else np.ones(n))
+    if case["model"] == "ols":
+        res = F.feols(inp["y"], inp["X"], None, None, vcov=case["vcov"], opts=F.SolveOpts(tol=1e-13), absorber=ab)
+    else:
+        res = F.fepois(inp["y"], inp["X"], None, None, vcov=case["vcov"], tol=1e-10, opts=F.SolveOpts(tol=1e-13),
+                       absorber=ab)
+    ab.close()
+    e = case["expect"]
+    E = np.array(e["vcov"], float).reshape(len(e["coef"]), -1)
+    ec = np.array(e["coef"], float)
+    if not codes:
+        E, ec = E[1:, 1:], ec[1:]
+    np.testing.assert_allclose(res.coef, ec, rtol=1e-8)
+    rtol = 1e-8 if (codes or case["model"] == "ols") else 2e-5  # fixest's no-FE IRLS stops elsewhere
+    np.testing.assert_allclose(res.vcov[np.ix_(res.kept, res.kept)], E, rtol=rtol, atol=1e-14)
+
+
+@pytest.mark.parametrize("kind", ["hc2", "hc3"])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_hc_matches_the_oracle_on_three_terms(kind, weighted):
+    """A larger three-term design with repeated cells and a small block: the GPU leverage and
+    HC variance against the oracle's brute-force hat matrix."""
+    import fereg_amd as F
+    from oracle import estimate
+    from oracle.oracle import OracleSolver
+    rng = np.random.default_rng(21)
+    n, gs = 3000, [300, 60, 5]
+    codes = [rng.integers(0, g, n).astype(np.uint32) for g in gs]
+    codes[0] = rng.permutation(np.repeat(np.arange(300), 10)).astype(np.uint32)  # no singleton level
+    X = rng.standard_normal((n, 3))
+    y = X @ np.array([1.0, -0.5, 0.2]) + rng.standard_normal(n)
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    announce("hc oracle", kind, weighted)
+    res = F.feols(y, X, codes, gs, weights=w, vcov=kind, opts=F.SolveOpts(tol=1e-13))
+    s = OracleSolver(codes, gs, weights=w)
+    pi = s.structure()["pi"]
+    rows = np.ascontiguousarray(np.concatenate([y[:, None], X], axis=1)[pi])
+    s.residualize(rows, 4, tol=1e-13)
+    wr = w[pi] if w is not None else None
+    coef, bread, e, kept = estimate.ols_from_block(rows, wr)
+    D = estimate.fe_design([c[pi] for c in codes], gs, n)
+    h = estimate.hat_values(np.concatenate([X[pi], D], axis=1), wr)
+    V = estimate.vcov(kind, bread, rows[:, 1:], e, wr, n, 0.0, h=h)
+    np.testing.assert_allclose(res.coef, coef, rtol=1e-9)
+    np.testing.assert_allclose(res.vcov, V, rtol=1e-8, atol=1e-16)
+
+
+def test_hc_with_a_singleton_level_is_an_error():
+    """A level with one observation has leverage 1: fixest refuses HC2/HC3 (fit/vcov.rs:337-351)."""
+    import fereg_amd as F
+    rng = np.random.default_rng(22)
+    n = 397  # levels 0..98 with four rows each, level 99 with one
+    codes = [np.concatenate([np.repeat(np.arange(99), 4), [99]]).astype(np.uint32),
+             rng.integers(0, 7, n).astype(np.uint32)]
+    X = rng.standard_normal((n, 2))
+    y = X.sum(1) + rng.standard_normal(n)
+    announce("hc singleton")
+    with pytest.raises(F.FeregError, match="leverage"):
+        F.feols(y, X, codes, [100, 7], vcov="hc2", opts=F.SolveOpts(tol=1e-12))
+    r = F.feols(y, X, codes, [100, 7], vcov="hetero", opts=F.SolveOpts(tol=1e-12))
+    assert np.all(np.isfinite(r.vcov))

@@ -122,3 +122,27 @@ def test_disconnected_rings_with_rows_ordered_last():
     v = rng.standard_normal((n, 3))
     x = f.solve(a @ v)
     np.testing.assert_allclose(a @ x, a @ v, atol=1e-9 * np.abs(a @ v).max())
+
+
+def test_selected_inverse_matches_the_dense_inverse_on_the_pattern():
+    """Takahashi's recurrences give the inverse's entries on every structural pair (and the
+    diagonal) of an SPD matrix; for a grounded Laplacian, u'Z u on range vectors is invariant."""
+    rng = np.random.default_rng(5)
+    n = 80
+    b = rng.standard_normal((n, n)) * (rng.random((n, n)) < 0.06)
+    a = b @ b.T + 0.5 * np.eye(n)
+    f = SparseChol.from_dense(a)
+    inv = np.linalg.inv(a)
+    r, c = np.nonzero(np.tril(a))
+    np.testing.assert_allclose(f.inverse_entries(r, c), inv[r, c], rtol=1e-10, atol=1e-13)
+    # a connected ring (one null direction): for u = e_i - e_j with i, j adjacent (a range
+    # vector), u'Zu = u'A^+u
+    m = 200
+    edges = [(i, (i + 1) % m) for i in range(m)]
+    L = _sparse_laplacian(m, edges, rng.uniform(0.5, 2.0, m)).toarray()
+    g = SparseChol.from_dense(L)
+    P = np.linalg.pinv(L)
+    for i in range(0, m, 17):
+        j = (i + 1) % m
+        z = g.inverse_entries([i, j, i], [i, j, j])
+        assert abs((z[0] + z[1] - 2 * z[2]) - (P[i, i] + P[j, j] - 2 * P[i, j])) < 1e-9 * abs(P[i, i])
