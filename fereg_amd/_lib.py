@@ -14,7 +14,7 @@ FH_TOO_FEW = 2001
 FH_INTERNAL = 9999
 FH_PLAIN_FP64 = 0
 FH_MIXED = 1
-FH_VCOV = {"iid": 0, "hetero": 1, "cluster": 2, "hc2": 3, "hc3": 4}
+FH_VCOV = {"iid": 0, "hetero": 1, "cluster": 2, "hc2": 3, "hc3": 4, "crv3": 5}
 FH_KFIXEF = {"none": 0, "full": 1, "nonnested": 2}
 
 u32p = ctypes.POINTER(ctypes.c_uint32)

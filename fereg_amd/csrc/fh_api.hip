@@ -433,7 +433,7 @@ double df_k(const Absorber& a, int kind, const fh_ssc& ssc, const int32_t* neste
     const size_t nt = a.nterms;
     const double fe_par = fe_params(a);
     int kf = ssc.k_fixef;
-    if (kind != FH_VCOV_CLUSTER && kf == FH_KFIXEF_NONNESTED) kf = FH_KFIXEF_FULL;
+    if (kind != FH_VCOV_CLUSTER && kind != FH_VCOV_CRV3 && kf == FH_KFIXEF_NONNESTED) kf = FH_KFIXEF_FULL;
     if (kf == FH_KFIXEF_NONE) return np - fe_par;
     if (kf == FH_KFIXEF_FULL || nt == 0) return np;
     int n_nested = 0;
@@ -607,7 +607,9 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     hipStream_t st = a.st;
     const size_t n = a.n, R = K + 1;
     require(K >= 1 && K <= 21, "1..21 regressors are supported");
-    require(kind != FH_VCOV_CLUSTER || !cs.dims.empty(), "a clustered variance matrix needs cluster codes");
+    require((kind != FH_VCOV_CLUSTER && kind != FH_VCOV_CRV3) || !cs.dims.empty(),
+            "a clustered variance matrix needs cluster codes");
+    require(kind >= FH_VCOV_IID && kind <= FH_VCOV_CRV3, "unknown variance kind " + std::to_string(kind));
     // solver rows padded to the solver's column class (zero pads, residualized as inactive
     // columns): whole 16-byte vectors per row; a pad column takes the residuals, so the cluster
     // sums read one aligned row per observation
@@ -617,6 +619,11 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     double* rows = a.rows_tmp.p;
     gather_rows(y, X, K, a.pi.p, n, ld, rows, st);
     const int prec = o.precision < 0 ? a.precision : o.precision;
+    DBuf<double> raw;  // CRV3 refits start from the raw rows
+    if (kind == FH_VCOV_CRV3) {
+        raw.alloc(n * ld);
+        FH_HIP(hipMemcpyAsync(raw.p, rows, n * ld * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
     ResidOut ro = residualize_any(a, rows, ld, o.tol, o.maxiter, nullptr, nullptr, prec, ctx);
     const double* w = a.weighted ? a.w_rows.p : nullptr;
     std::vector<double> G(R * R);
@@ -644,16 +651,50 @@ void ols_fit(Absorber& a, const double* y, const double* X, size_t K, int kind, 
     for (int i = 0; i < mk; i++) c[kept[i] + 1] = -coef[i];
     DBuf<double> e(n);
     const double ssr = residual_pass(a, rows, n, (int)ld, (int)R, c.data(), w, e.p, ecol);
-    const double np = (double)mk + [&] {
-        double s = 0.0;
-        for (uint64_t g : a.term_g) s += (double)g;
-        return s - (a.nterms > 1 ? (double)a.nterms - 1.0 : 0.0);
-    }();
+    // fixed-effect parameters; without absorbed terms the absorber's constant is fixest's
+    // intercept, one parameter
+    const double np = (double)mk + (a.nterms > 0 ? fe_params(a) : 1.0);
     const double dfk = df_k(a, kind, ssc, nested, np, mk);
     std::vector<double> V;
     uint64_t ncl = 0;
-    vcov_from(a, rows, (int)ld, (int)R, excluded, inv, e.p, ecol, w, kind, cs, ssc, dfk,
-              ssr / ((double)n - 1.0), V, &ncl, state ? state->wb : nullptr);
+    if (kind == FH_VCOV_CRV3) {
+        // each refit: the cluster's rows get weight zero, the raw rows are residualized again and
+        // the kept regressors' normal equations solved (crv3.rs: every FE re-estimated)
+        DBuf<double> w0, tmp(n * ld);
+        const bool had_w = a.weighted;
+        if (had_w) {
+            w0.alloc(n);
+            FH_HIP(hipMemcpyAsync(w0.p, a.w_rows.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        CrvRefit refit = [&](const double* wm, uint64_t g) {
+            refresh_weights(a, wm);
+            FH_HIP(hipMemcpyAsync(tmp.p, raw.p, n * ld * sizeof(double), hipMemcpyDeviceToDevice, st));
+            residualize_any(a, tmp.p, ld, o.tol, o.maxiter, nullptr, nullptr, prec, ctx);
+            std::vector<double> Gg(R * R);
+            gram_cols(a, tmp.p, n, (int)ld, 0, (int)R, a.w_rows.p, nullptr, Gg.data());
+            std::vector<double> sub((size_t)mk * mk), ginv;
+            for (int i = 0; i < mk; i++)
+                for (int j = 0; j < mk; j++) sub[(size_t)i * mk + j] = Gg[(size_t)(kept[i] + 1) * R + kept[j] + 1];
+            std::vector<int> ex;
+            chol_collin(sub, mk, 1e-9, ex, ginv);
+            for (int i = 0; i < mk; i++)
+                require(!ex[i], "CRV3: coefficient " + std::to_string(kept[i]) +
+                                    " is not estimable when cluster " + std::to_string(g) + " is left out");
+            std::vector<double> b(mk, 0.0);
+            for (int i = 0; i < mk; i++)
+                for (int j = 0; j < mk; j++) b[i] += ginv[(size_t)i * mk + j] * Gg[(size_t)(kept[j] + 1) * R];
+            return b;
+        };
+        std::vector<double> Vk;
+        crv3_vcov(a, cs, ssc, dfk, had_w ? w0.p : nullptr, coef, refit, Vk, &ncl);
+        refresh_weights(a, had_w ? w0.p : nullptr);
+        V.assign(K * K, 0.0);
+        for (int i = 0; i < mk; i++)
+            for (int j = 0; j < mk; j++) V[(size_t)kept[i] * K + kept[j]] = Vk[(size_t)i * mk + j];
+    } else {
+        vcov_from(a, rows, (int)ld, (int)R, excluded, inv, e.p, ecol, w, kind, cs, ssc, dfk,
+                  ssr / ((double)n - 1.0), V, &ncl, state ? state->wb : nullptr);
+    }
     for (size_t j = 0; j < K; j++) {
         beta[j] = 0.0;
         kept_out[j] = !excluded[j];

@@ -141,6 +141,17 @@ void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>&
                const ClusterSet& cs, const fh_ssc& ssc, double dfk, double iid_scale,
                std::vector<double>& V, uint64_t* ncl, WbPre* wb = nullptr, bool glm = false);
 
+// ------------------------------------------------------------------ CRV3 (fh_crv3.hip)
+
+// A refit with row weights wm (solver order, zero on the rows of the left-out cluster g): the
+// coefficients over the full fit's kept regressors.
+using CrvRefit = std::function<std::vector<double>(const double* wm_row, uint64_t g)>;
+// V = factor sum_S sign scale(G_S) jackknife meat_S over the subsets of the cluster dimensions
+// (crv3.rs, fit/vcov.rs:369-397), factor = G_min/(G_min - 1) (G.df min) x (n - 1)/(n - df_k), then
+// fixest's PSD repair; beta over the kept regressors, w_row the fit's own row weights (or null).
+void crv3_vcov(Absorber& a, const ClusterSet& cs, const fh_ssc& ssc, double dfk, const double* w_row,
+               const std::vector<double>& beta, const CrvRefit& refit, std::vector<double>& V, uint64_t* ncl);
+
 // ------------------------------------------------------------------ factors (fh_factor.hip)
 
 // Dense codes of the interaction of device code arrays (n rows) into out; returns the levels.

@@ -247,7 +247,8 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, const dou
     const size_t n = a.n, R = K + 1;
     require(K >= 1 && K <= 21, "1..21 regressors are supported");
     require(maxiter >= 1, "the IRLS needs at least one iteration");
-    require(kind != FH_VCOV_CLUSTER || !cs.dims.empty(), "a clustered variance matrix needs cluster codes");
+    require((kind != FH_VCOV_CLUSTER && kind != FH_VCOV_CRV3) || !cs.dims.empty(),
+            "a clustered variance matrix needs cluster codes");
     // The caller's weights are the prior weights; the absorber holds the IRLS weights from here on
     // and gets the caller's back before its next non-Poisson call.
     if (!a.irls_weights) {
@@ -425,12 +426,45 @@ void ppml_fit(Absorber& a, const double* y, const double* X, size_t K, const dou
     std::vector<int> singular;
     chol_collin(sub, mk, DBL_MIN, singular, inv);
     for (int s : singular) require(!s, "the Poisson information matrix is singular", FH_INTERNAL);
-    const double np = (double)mk + fe_params(a);
+    const double np = (double)mk + (a.nterms > 0 ? fe_params(a) : 1.0);  // the constant without absorbed terms
     const double dfk = df_k(a, kind, ssc, nested, np, mk);
     uint64_t ncl = 0;
     std::vector<double> V;
-    vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, a.w_rows.p, kind, cs, ssc, dfk, 1.0, V, &ncl, nullptr,
-              true);
+    if (kind == FH_VCOV_CRV3) {
+        // each refit: the IRLS again with the left-out cluster's prior weights at zero
+        // (crv3.rs: GLMs always refit); the prior weights come back afterwards
+        DBuf<double> prior;
+        const bool had_tw = a.user_w.n > 0;
+        if (had_tw) {
+            prior.alloc(n);
+            FH_HIP(hipMemcpyAsync(prior.p, a.user_w.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        CrvRefit refit = [&](const double* wm, uint64_t g) {
+            DBuf<double> saved = std::move(a.user_w);
+            a.user_w.alloc(n);
+            FH_HIP(hipMemcpyAsync(a.user_w.p, wm, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            std::vector<double> b(K), Vt(K * K);
+            std::vector<int32_t> kp(K);
+            ppml_fit(a, y, X, K, offset, FH_VCOV_IID, ClusterSet{}, nested, ssc, tol, maxiter, o, b.data(), Vt.data(),
+                     kp.data(), nullptr, ctx);
+            a.user_w = std::move(saved);
+            std::vector<double> bk(mk);
+            for (int i = 0; i < mk; i++) {
+                require(kp[kept[i]], "CRV3: coefficient " + std::to_string(kept[i]) +
+                                         " is not estimable when cluster " + std::to_string(g) + " is left out");
+                bk[i] = b[kept[i]];
+            }
+            return bk;
+        };
+        std::vector<double> Vk;
+        crv3_vcov(a, cs, ssc, dfk, had_tw ? prior.p : nullptr, coef, refit, Vk, &ncl);
+        V.assign(K * K, 0.0);
+        for (int i = 0; i < mk; i++)
+            for (int j = 0; j < mk; j++) V[(size_t)kept[i] * K + kept[j]] = Vk[(size_t)i * mk + j];
+    } else {
+        vcov_from(a, rows, (int)R, (int)R, excluded, inv, e.p, -1, a.w_rows.p, kind, cs, ssc, dfk, 1.0, V, &ncl,
+                  nullptr, true);
+    }
     for (size_t j = 0; j < K; j++) {
         beta[j] = 0.0;
         kept_out[j] = !excluded[j];

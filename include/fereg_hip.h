@@ -208,6 +208,7 @@ void fh_tune_cache_clear(void);
 #define FH_VCOV_CLUSTER 2
 #define FH_VCOV_HC2 3 /* fit/mod.rs:108-118: scores / sqrt(1 - h), the full model's exact leverage */
 #define FH_VCOV_HC3 4 /* scores / (1 - h) */
+#define FH_VCOV_CRV3 5 /* crv3.rs: leave-one-cluster-out jackknife (every FE re-estimated), 1-4 dimensions */
 
 /* fixest small-sample rules (fit/vcov.rs:63-112, ssc of fit/mod.rs). */
 #define FH_KFIXEF_NONE 0

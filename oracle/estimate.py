@@ -10,6 +10,8 @@ checker. Follows fereg-core's fit layer (fixest conventions):
   vcov          fit/vcov.rs:316-382 (iid, hetero, HC2/HC3, one- to multiway cluster)
   hat_values    leverage.rs:1-30 (the exact leverage of the full model, here by brute force on the
                 explicit design: diag of W^1/2 X (X'WX)^+ X' W^1/2 over [X, every FE dummy])
+  crv3          crv3.rs:1-104, fit/vcov.rs:369-397 (leave-one-cluster-out refits by dropping the
+                cluster's rows, inclusion-exclusion over the cluster dimensions)
   poisson IRLS  glm/irls.rs:367-913 with glm/family.rs:103-145 (fixest flavour)
 """
 import itertools
@@ -190,3 +192,52 @@ def ols_from_block(rows, w=None, tol=1e-9):
     excluded, kept, inv, coef, resid = regress(g, np.eye(r)[0], design, tol)
     e = rows @ resid
     return coef, inv, e, kept
+
+
+def crv3(fit_beta, codes, y, X, w, dims, dfk, k_adj=True, g_adj=True, g_df="min", offset=None):
+    """The CRV3 variance: for every non-empty subset of the cluster dimensions (clustered by the
+    intersection) and every cluster g of it, beta_(-g) = fit_beta(codes, y, X, w, offset) on the
+    rows outside g (codes re-densified, as a fresh fit sees them); V = factor sum sign
+    scale(G) sum_g (beta_(-g) - beta)(beta_(-g) - beta)' with factor G_min/(G_min - 1)
+    (n - 1)/(n - df_k) (crv3.rs:89-104, fit/vcov.rs:369-397), then fixest's PSD repair."""
+    n = len(y)
+
+    def sub(keep):
+        cs = []
+        for c in codes:
+            _, inv = np.unique(np.asarray(c)[keep], return_inverse=True)
+            cs.append(inv.astype(np.uint32))
+        return cs
+
+    def fit(keep):
+        return fit_beta(sub(keep), y[keep], X[keep], None if w is None else w[keep],
+                        None if offset is None else offset[keep])
+
+    beta = fit(np.ones(n, bool))
+    k = len(beta)
+    q = len(dims)
+    conventional = g_adj and g_df == "conventional"
+    meat = np.zeros((k, k))
+    gs = []
+    for size in range(1, q + 1):
+        sign = 1.0 if size % 2 else -1.0
+        for s in itertools.combinations(range(q), size):
+            cl = np.asarray(dims[s[0]]) if size == 1 else _intersection([dims[d] for d in s])
+            G = int(cl.max()) + 1
+            m = np.zeros((k, k))
+            for g in range(G):
+                keep = cl != g
+                if keep.all():
+                    continue
+                d = fit(keep) - beta
+                m += np.outer(d, d)
+            ng = len(np.unique(cl))
+            if size == 1:
+                gs.append(ng)
+            meat += sign * (ng / (ng - 1) if conventional else 1.0) * m
+    factor = 1.0
+    if g_adj and g_df == "min":
+        factor *= min(gs) / (min(gs) - 1)
+    if k_adj:
+        factor *= (n - 1) / (n - dfk)
+    return beta, posdef_fix(factor * meat)

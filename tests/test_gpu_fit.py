@@ -229,3 +229,34 @@ def test_hc_with_a_singleton_level_is_an_error():
         F.feols(y, X, codes, [100, 7], vcov="hc2", opts=F.SolveOpts(tol=1e-12))
     r = F.feols(y, X, codes, [100, 7], vcov="hetero", opts=F.SolveOpts(tol=1e-12))
     assert np.all(np.isfinite(r.vcov))
+
+
+SE_CRV3 = [c for c in cases("fixest_se.json") if c["vcov"] == "crv3"]
+
+
+@pytest.mark.parametrize("case", SE_CRV3, ids=lambda c: c["name"])
+def test_fixest_crv3(case):
+    """CRV3: every cluster (of every subset of the dimensions) left out by a zero weight, the
+    model re-estimated on the GPU (OLS: one residualization; Poisson: the IRLS), against fixest
+    0.14.2 (fit_se_fixest.json)."""
+    import fereg_amd as F
+    inp = case_inputs(case)
+    codes, gs, w = inp["codes"], inp["g"], inp["w"]
+    if not codes and w is None:
+        w = np.ones(len(inp["y"]))
+    nested = fixest_nested(case, inp)
+    announce("crv3", case["name"])
+    if case["model"] == "ols":
+        res = F.feols(inp["y"], inp["X"], codes, gs, weights=w, vcov="crv3", cluster=inp["clusters"], nested=nested,
+                      ssc=case["ssc"], opts=F.SolveOpts(tol=1e-13))
+    else:
+        res = F.fepois(inp["y"], inp["X"], codes, gs, weights=w, vcov="crv3", cluster=inp["clusters"],
+                       nested=nested, ssc=case["ssc"], tol=1e-10, opts=F.SolveOpts(tol=1e-13))
+    e = case["expect"]
+    E = np.array(e["vcov"], float).reshape(len(e["coef"]), -1)
+    ec = np.array(e["coef"], float)
+    if not codes:
+        E, ec = E[1:, 1:], ec[1:]
+    np.testing.assert_allclose(res.coef, ec, rtol=1e-8)
+    np.testing.assert_allclose(res.vcov[np.ix_(res.kept, res.kept)], E, rtol=1e-7, atol=1e-14)
+    assert abs(res.df_k - e["df_k"]) < 1e-9

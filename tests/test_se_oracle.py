@@ -76,3 +76,40 @@ def test_leverage_sums_to_the_rank():
     h = estimate.hat_values(full, w)
     assert abs(h.sum() - np.linalg.matrix_rank(full)) < 1e-9
     assert np.all(h > 0) and np.all(h < 1)
+
+
+CRV3 = [c for c in cases("fixest_se.json") if c["vcov"] == "crv3"]
+
+
+def ols_beta(codes, y, X, w, offset=None):
+    """OLS coefficients of y on X with the absorbed terms of codes (the constant without any)."""
+    n = len(y)
+    gs = [int(c.max()) + 1 for c in codes]
+    if codes:
+        s = OracleSolver(codes, gs, weights=w)
+        pi = s.structure()["pi"]
+    else:
+        s = OracleSolver([], [], weights=w if w is not None else np.ones(n))
+        pi = np.arange(n)
+    rows = np.ascontiguousarray(np.concatenate([y[:, None], X], axis=1)[pi])
+    s.residualize(rows, rows.shape[1], tol=1e-14)
+    return estimate.ols_from_block(rows, None if w is None else w[pi])[0]
+
+
+def pois_beta(codes, y, X, w, offset=None):
+    n = len(y)
+    if not codes:
+        codes = [np.zeros(n, np.uint32)]
+    return poisson.ppml(codes, [int(c.max()) + 1 for c in codes], y, X, weights=w, offset=offset, tol=1e-10,
+                        itol=1e-12, vcov="iid")["coef"]
+
+
+@pytest.mark.parametrize("case", CRV3, ids=lambda c: c["name"])
+def test_crv3_fixtures(case):
+    inp = case_inputs(case)
+    fb = ols_beta if case["model"] == "ols" else pois_beta
+    beta, V = estimate.crv3(fb, inp["codes"], inp["y"], inp["X"], inp["w"], inp["clusters"], case["expect"]["df_k"],
+                            k_adj=case["ssc"]["K.adj"], g_adj=case["ssc"]["G.adj"], g_df=case["ssc"]["G.df"])
+    ec, E = _expect(case, bool(case["fe"]))
+    np.testing.assert_allclose(beta, ec, rtol=1e-8)
+    np.testing.assert_allclose(V, E, rtol=1e-8, atol=1e-14)
