@@ -320,8 +320,9 @@ def run_serve(args, np, torch, F, synth):
     specs = serve.poisson_stream(args.requests, seed=7)
     log("requests", len(specs), "mean n", float(np.mean([s["n"] for s in specs])), "rate", args.rate)
     res = {}
-    for label, slots, prec in (("batched", args.slots, F.FH_MIXED), ("one_at_a_time_fp64", 1, F.FH_PLAIN_FP64)):
-        srv = serve.Server(slots=slots, precision=prec)
+    for label, slots, prec, pack in (("batched", args.slots, F.FH_MIXED, args.pack),
+                                     ("one_at_a_time_fp64", 1, F.FH_PLAIN_FP64, 1)):
+        srv = serve.Server(slots=slots, precision=prec, pack=pack)
         w = serve.make_request(specs[0])
         srv.wait(srv.submit(specs[0]["model"], *w))
         replies, wall, late = serve.run_stream(srv, specs, args.rate,
@@ -330,7 +331,9 @@ def run_serve(args, np, torch, F, synth):
         tot = np.array([r.total_s for r in replies])
         res[label] = {"p50_s": float(np.percentile(tot, 50)), "p99_s": float(np.percentile(tot, 99)),
                       "solves_per_s": len(replies) / wall, "errors": int(sum(r.rc != 0 for r in replies)),
-                      "max_submit_lateness_s": late}
+                      "max_submit_lateness_s": late,
+                      "requests_in_packed_solves": int(sum(r.packed > 1 for r in replies)),
+                      "mean_pack": float(np.mean([r.packed for r in replies if r.packed > 1] or [1]))}
         log(label, res[label])
     b = res["batched"]
     config = {"workload": f"serving stream: {len(specs)} requests, Poisson arrivals at {args.rate}/s, n log-uniform "
@@ -364,6 +367,7 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="requests in the stream (serve)")
     ap.add_argument("--rate", type=float, default=100.0, help="arrivals per second (serve)")
     ap.add_argument("--slots", type=int, default=8, help="requests in flight (serve)")
+    ap.add_argument("--pack", type=int, default=32, help="most small OLS requests packed into one solve (serve)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ppml", action="store_true", help="leave the Poisson section out of the ols line")
     ap.add_argument("--spmv-reps", type=int, default=50)

@@ -40,7 +40,7 @@ class Term(ctypes.Structure):
 class SolverConfig(ctypes.Structure):
     _fields_ = [("small_limit", ctypes.c_uint64), ("precision", ctypes.c_int32), ("bucket_log2", ctypes.c_int32),
                 ("autotune", ctypes.c_int32), ("mixed_lead", ctypes.c_int32), ("sparse_chol", ctypes.c_int32),
-                ("joint", ctypes.c_int32)]
+                ("joint", ctypes.c_int32), ("dense_limit", ctypes.c_int32)]
 
 
 class SpmvPlan(ctypes.Structure):
@@ -87,7 +87,7 @@ class StructureInfo(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint64), ("nterms", ctypes.c_uint64), ("g1", ctypes.c_uint64),
                 ("ncells", ctypes.c_uint64), ("nb", ctypes.c_uint64), ("m", ctypes.c_uint64),
                 ("m_small", ctypes.c_uint64), ("small_work", ctypes.c_uint64), ("fe1", ctypes.c_int64),
-                ("norm_fe1", ctypes.c_int64), ("has_cell_obs_off", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("norm_fe1", ctypes.c_int64), ("has_cell_obs_off", ctypes.c_int32), ("dense_small", ctypes.c_int32)]
 
 
 class Ssc(ctypes.Structure):
@@ -132,7 +132,7 @@ FH_MODEL = {"ols": 0, "ppml": 1}
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64), ("precision", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("pack", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
@@ -146,7 +146,7 @@ class Request(ctypes.Structure):
 
 
 class Response(ctypes.Structure):
-    _fields_ = [("rc", ctypes.c_int32), ("reserved", ctypes.c_int32), ("queued_s", ctypes.c_double),
+    _fields_ = [("rc", ctypes.c_int32), ("packed", ctypes.c_int32), ("queued_s", ctypes.c_double),
                 ("service_s", ctypes.c_double), ("total_s", ctypes.c_double), ("iterations", ctypes.c_uint64),
                 ("pcg_iterations", ctypes.c_uint64), ("start_s", ctypes.c_double), ("end_s", ctypes.c_double)]
 

@@ -114,7 +114,7 @@ class Absorber:
     """
 
     def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, bucket_log2=-1,
-                 autotune=False, verbose=0, mixed_lead=-1, sparse_chol=True, joint=True):
+                 autotune=False, verbose=0, mixed_lead=-1, sparse_chol=True, joint=True, dense_limit=-1):
         lib = L.lib()
         self._keep = []
         n = None
@@ -145,7 +145,7 @@ class Absorber:
         if n is None:
             raise L.FeregError(L.FH_INVALID, "no terms and no weights: the sample size is unknown")
         cfg = L.SolverConfig(small_limit, precision, bucket_log2, int(bool(autotune)), int(mixed_lead),
-                             int(bool(sparse_chol)), int(bool(joint)))
+                             int(bool(sparse_chol)), int(bool(joint)), int(dense_limit))
         h = ctypes.c_void_p()
         if dev:
             wptr = None

@@ -420,6 +420,165 @@ ResidOut residualize_any(Absorber& a, double* z, size_t r, double tol, uint64_t 
     return tot;
 }
 
+// ------------------------------------------------------------------ packed requests
+
+namespace {
+
+// rows[i][c] *= f[req(i)][c] for the solver rows of a packed solve (requests contiguous).
+__global__ void k_scale_ranges(double* __restrict__ rows, size_t n, int ld, int R, const uint32_t* __restrict__ off,
+                               int nreq, const double* __restrict__ f) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nreq;  // the request of row i: off[j] <= i < off[j + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (off[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        for (int c = 0; c < R; c++) rows[i * ld + c] *= f[lo * R + c];
+    }
+}
+
+}  // namespace
+
+void ols_fit_packed(const std::vector<PackedOls*>& reqs, int precision) {
+    const size_t nreq = reqs.size();
+    size_t Q = 0, Kmax = 0, ntot = 0;
+    double tol = INFINITY;
+    uint64_t maxiter = 0;
+    for (const PackedOls* r : reqs) {
+        Q = std::max(Q, r->codes.size());
+        Kmax = std::max(Kmax, r->K);
+        ntot += r->n;
+        tol = std::min(tol, r->tol);
+        maxiter = std::max(maxiter, r->maxiter);
+    }
+    const size_t R = Kmax + 1;
+    require(R <= 16 && Q >= 1 && Q <= NB_MAX + 1, "packed solve: 1..15 regressors and 1..9 terms");
+    // packed codes (disjoint level ranges per request) and the scaled [y, X] block, kept order
+    std::vector<std::vector<uint32_t>> pc(Q, std::vector<uint32_t>(ntot));
+    std::vector<uint64_t> pg(Q, 0);
+    std::vector<double> yp(ntot), xp(ntot * Kmax, 0.0), scale(nreq * R, 1.0);
+    std::vector<uint32_t> off(nreq + 1, 0);
+    size_t r0 = 0;
+    for (size_t j = 0; j < nreq; j++) {
+        const PackedOls& r = *reqs[j];
+        off[j] = (uint32_t)r0;
+        for (size_t q = 0; q < Q; q++) {
+            if (q < r.codes.size()) {
+                for (size_t i = 0; i < r.n; i++) pc[q][r0 + i] = (uint32_t)(r.codes[q][i] + pg[q]);
+                pg[q] += r.g[q];
+            } else {
+                for (size_t i = 0; i < r.n; i++) pc[q][r0 + i] = (uint32_t)pg[q];
+                pg[q] += 1;
+            }
+        }
+        for (size_t c = 0; c <= r.K; c++) {
+            double ss = 0.0;
+            for (size_t i = 0; i < r.n; i++) {
+                const double v = c == 0 ? r.y[i] : r.X[i * r.K + c - 1];
+                ss += v * v;
+            }
+            const double sc = ss > 0.0 ? 1.0 / std::sqrt(ss / (double)r.n) : 1.0;
+            scale[j * R + c] = sc;
+            for (size_t i = 0; i < r.n; i++) {
+                if (c == 0) yp[r0 + i] = r.y[i] * sc;
+                else xp[(r0 + i) * Kmax + c - 1] = r.X[i * r.K + c - 1] * sc;
+            }
+        }
+        r0 += r.n;
+    }
+    off[nreq] = (uint32_t)ntot;
+    for (uint64_t gq : pg) require(gq < (1ull << 32), "packed solve: too many levels");
+    std::vector<fh_term> terms(Q);
+    for (size_t q = 0; q < Q; q++) terms[q] = fh_term{pc[q].data(), pg[q], 1};
+    fh_solver_config cfg;
+    fh_config_default(&cfg);
+    cfg.precision = precision;
+    fh_absorber* ab = nullptr;
+    int rc = fh_absorber_create(terms.data(), Q, nullptr, ntot, &cfg, nullptr, &ab);
+    if (rc != FH_OK) {
+        for (PackedOls* r : reqs) r->rc = rc;
+        return;
+    }
+    std::unique_ptr<fh_absorber, void (*)(fh_absorber*)> hold(ab, fh_absorber_destroy);
+    Absorber& a = ab->a;
+    AbsScope g(a);
+    hipStream_t st = a.st;
+    const size_t ld = padded_cols((int)R);
+    DBuf<double> dy(ntot), dx(std::max<size_t>(ntot * Kmax, 1)), rows(ntot * ld), dscale(nreq * R), inv(nreq * R);
+    DBuf<uint32_t> doff(nreq + 1);
+    h2d(dy.p, yp.data(), ntot, st);
+    h2d(dx.p, xp.data(), ntot * Kmax, st);
+    gather_rows(dy.p, dx.p, Kmax, a.pi.p, ntot, ld, rows.p, st);
+    // each request's FE1 levels come after the previous request's, so its rows stay contiguous
+    // in the solver order (the structure sorts by FE1 first): the kept offsets are row offsets
+    std::vector<double> unscale(nreq * R);
+    for (size_t i = 0; i < nreq * R; i++) unscale[i] = 1.0 / scale[i];
+    h2d(doff.p, off.data(), nreq + 1, st);
+    h2d(inv.p, unscale.data(), nreq * R, st);
+    const double tol_p = tol / std::sqrt((double)nreq);  // every request within its own tolerance
+    ResidOut ro;
+    try {
+        ro = residualize_any(a, rows.p, ld, tol_p, maxiter, nullptr, nullptr, precision, nullptr);
+    } catch (const Error& e) {
+        for (PackedOls* r : reqs) r->rc = e.rc;
+        return;
+    }
+    hipLaunchKernelGGL(k_scale_ranges, dim3(grid_for(ntot, TPB, 4096)), dim3(TPB), 0, st, rows.p, ntot, (int)ld, (int)R,
+                       doff.p, (int)nreq, inv.p);
+    FH_CHECK_LAUNCH();
+    for (size_t j = 0; j < nreq; j++) {
+        PackedOls& r = *reqs[j];
+        r.iterations = ro.iterations;
+        try {
+            double* rj = rows.p + (size_t)off[j] * ld;
+            const size_t K = r.K, Rj = K + 1, n = r.n;
+            std::vector<double> G(Rj * Rj);
+            gram_cols(a, rj, n, (int)ld, 0, (int)Rj, nullptr, nullptr, G.data());
+            std::vector<double> Gxx(K * K);
+            for (size_t i = 0; i < K; i++)
+                for (size_t l = 0; l < K; l++) Gxx[i * K + l] = G[(i + 1) * Rj + l + 1];
+            std::vector<int> excluded;
+            std::vector<double> binv;
+            chol_collin(Gxx, (int)K, 1e-9, excluded, binv);
+            std::vector<int> kept;
+            for (size_t i = 0; i < K; i++)
+                if (!excluded[i]) kept.push_back((int)i);
+            const int mk = (int)kept.size();
+            require(mk > 0, "all the regressors are collinear (with the fixed effects)");
+            std::vector<double> coef(mk, 0.0), c(Rj, 0.0);
+            for (int i = 0; i < mk; i++)
+                for (int l = 0; l < mk; l++) coef[i] += binv[(size_t)i * mk + l] * G[(size_t)(kept[l] + 1) * Rj];
+            c[0] = 1.0;
+            for (int i = 0; i < mk; i++) c[kept[i] + 1] = -coef[i];
+            DBuf<double> e(n);
+            const double ssr = residual_pass(a, rj, n, (int)ld, (int)Rj, c.data(), nullptr, e.p, -1);
+            double fe_par = 1.0;  // the constant without absorbed terms
+            if (!r.codes.empty()) {
+                fe_par = 0.0;
+                for (uint64_t gq : r.g) fe_par += (double)gq;
+                fe_par -= (double)r.codes.size() - 1.0;
+            }
+            const double np = (double)mk + fe_par;
+            const double dfk = r.ssc.k_fixef == FH_KFIXEF_NONE ? np - fe_par : np;
+            std::vector<double> V;
+            vcov_from(a, rj, (int)ld, (int)Rj, excluded, binv, e.p, -1, nullptr, r.kind, ClusterSet{}, r.ssc, dfk,
+                      ssr / ((double)n - 1.0), V, nullptr, nullptr, false, n);
+            for (size_t i = 0; i < K; i++) {
+                r.beta[i] = 0.0;
+                r.kept[i] = !excluded[i];
+            }
+            for (int i = 0; i < mk; i++) r.beta[kept[i]] = coef[i];
+            std::memcpy(r.vcov, V.data(), K * K * sizeof(double));
+            r.rc = FH_OK;
+        } catch (const Error& e) {
+            set_last_error(e.what());
+            r.rc = e.rc;
+        }
+    }
+    stream_wait(st);
+}
+
 // ------------------------------------------------------------------ estimation helpers (host, K x K)
 
 double fe_params(const Absorber& a) {
@@ -471,14 +630,16 @@ struct WbPre {
 void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
                const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
                const ClusterSet& cs, const fh_ssc& ssc, double dfk, double iid_scale,
-               std::vector<double>& V, uint64_t* ncl, WbPre* wb, bool glm) {
+               std::vector<double>& V, uint64_t* ncl, WbPre* wb, bool glm, size_t nrows) {
     require(kind >= FH_VCOV_IID && kind <= FH_VCOV_HC3, "unknown variance kind " + std::to_string(kind));
+    require(nrows == 0 || kind <= FH_VCOV_HETERO, "a row range supports iid and hetero variances");
+    const size_t nr = nrows ? nrows : a.n;
     const int K = R - 1;
     std::vector<int> kept;
     for (int j = 0; j < K; j++)
         if (!excluded[j]) kept.push_back(j);
     const int mk = (int)kept.size();
-    const double n = (double)a.n;
+    const double n = (double)nr;
     V.assign((size_t)K * K, 0.0);
     std::vector<double> vk((size_t)mk * mk, 0.0);
     if (mk == 0) return;
@@ -489,7 +650,7 @@ void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>&
         std::vector<double> meat((size_t)K * K);
         double factor = 1.0;
         if (kind == FH_VCOV_HETERO) {
-            hetero_meat(a, rows, a.n, ld, 1, K, e, w, meat.data());
+            hetero_meat(a, rows, nr, ld, 1, K, e, w, meat.data());
             if (ssc.k_adj) factor = n / (n - dfk);
         } else if (kind == FH_VCOV_HC2 || kind == FH_VCOV_HC3) {
             // fit/vcov.rs:333-364, fixest flavour: the scores over sqrt(1 - h) or (1 - h) with the
@@ -739,6 +900,7 @@ void fh_config_default(fh_solver_config* cfg) {
     cfg->mixed_lead = -1;
     cfg->sparse_chol = 1;
     cfg->joint = 1;
+    cfg->dense_limit = -1;
 }
 
 static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept, size_t n, const fh_solver_config* cfg,
@@ -784,6 +946,8 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         require(c.joint == 0 || c.joint == 1, "joint must be 0 or 1");
         a.sparse_chol = c.sparse_chol == 1;
         a.joint_policy = c.joint;
+        require(c.dense_limit >= -1, "dense_limit must be -1 (the default) or at least 0");
+        a.dense_limit = c.dense_limit < 0 ? DENSE_LIMIT_DEFAULT : (uint64_t)c.dense_limit;
         std::vector<DBuf<uint32_t>> up(nterms);
         std::vector<const uint32_t*> dc(nterms);
         for (size_t q = 0; q < nterms; q++) {
@@ -840,6 +1004,20 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         build_structure(a, dc);
         log_line(ctx, 2, "fereg_hip: structure " +
                              std::to_string(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()) + "s");
+        a.ms_struct = a.ms;
+        a.small_struct.clear();
+        for (const BlockInfo& bi : a.blocks) a.small_struct.push_back(bi.small ? 1 : 0);
+        if (a.ms > a.dense_limit) {
+            // The structure keeps the reference's small/large order (the row order and cell codes
+            // depend on it); the small blocks are preconditioned like large ones (Jacobi, level
+            // passes) when their dense factor would cost more than the iterations it saves
+            for (size_t b = 0; b < a.nb; b++)
+                if (a.blocks[b].small) {
+                    a.blocks[b].small = false;
+                    ensure_block_transpose(a, (int)b);
+                }
+            a.ms = 0;
+        }
         DBuf<double> wrow;
         if (wk) {
             wrow.alloc(n);
@@ -971,8 +1149,10 @@ int fh_absorber_info(const fh_absorber* ab, fh_structure_info* o) {
     return guard([&] {
         require(ab && o, "null argument");
         const Absorber& a = ab->a;
-        *o = fh_structure_info{a.n, a.nterms, a.g1, a.M, a.nb, a.m, a.ms, a.small_work, a.fe1, a.norm_fe1,
-                               (int32_t)!a.trivial(), 0};
+        *o = fh_structure_info{a.n,        a.nterms,  a.g1,
+                               a.M,        a.nb,      a.m,
+                               a.ms_struct, a.small_work, a.fe1,
+                               a.norm_fe1, (int32_t)!a.trivial(), (int32_t)(a.ms > 0)};
     });
 }
 
@@ -997,7 +1177,7 @@ int fh_absorber_copy_blocks(const fh_absorber* ab, int64_t* term, uint64_t* g, u
             term[b] = a.blocks[b].term;
             g[b] = a.blocks[b].g;
             off[b] = a.blocks[b].off;
-            small[b] = a.blocks[b].small;
+            small[b] = a.small_struct[b];
         }
     });
 }

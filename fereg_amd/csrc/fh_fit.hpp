@@ -139,7 +139,34 @@ uint64_t multiway_meat(Absorber& a, const double* z, int ld, int c0, int k, cons
 void vcov_from(Absorber& a, double* rows, int ld, int R, const std::vector<int>& excluded,
                const std::vector<double>& bread, const double* e, int ecol, const double* w, int kind,
                const ClusterSet& cs, const fh_ssc& ssc, double dfk, double iid_scale,
-               std::vector<double>& V, uint64_t* ncl, WbPre* wb = nullptr, bool glm = false);
+               std::vector<double>& V, uint64_t* ncl, WbPre* wb = nullptr, bool glm = false, size_t nrows = 0);
+
+// ------------------------------------------------------------------ packed requests (fh_api.hip)
+
+// One request of a packed solve: an unweighted OLS with iid or hetero variance (host inputs in
+// kept order), its outputs and return code.
+struct PackedOls {
+    size_t n = 0, K = 0;
+    std::vector<const uint32_t*> codes;
+    std::vector<uint64_t> g;
+    const double* y = nullptr;
+    const double* X = nullptr;
+    int kind = FH_VCOV_IID;
+    fh_ssc ssc{};
+    double tol = 1e-8;
+    uint64_t maxiter = 16000;
+    double* beta = nullptr;
+    double* vcov = nullptr;
+    int32_t* kept = nullptr;
+    int rc = FH_OK;
+    uint64_t iterations = 0;
+};
+// Several independent requests in one absorber: their fixed effects take disjoint level ranges
+// (a request with fewer terms gets one constant level in the missing ones, absorbed by its own
+// first term), so the packed projection is block diagonal and each request's residuals are its
+// own; every column is scaled to unit mean square per request for the solve (the tolerance is
+// relative per request) and unscaled before the fits. One structure build, one multi-RHS PCG.
+void ols_fit_packed(const std::vector<PackedOls*>& reqs, int precision);
 
 // ------------------------------------------------------------------ CRV3 (fh_crv3.hip)
 

@@ -31,6 +31,12 @@
 namespace fh {
 
 constexpr int NB_MAX = 8;  // non-FE1 blocks supported by the kernels
+// Largest small super-block (levels) preconditioned by its dense grounded inverse; above it the
+// small blocks take Jacobi and level passes like large ones. Measured on MI355X (three-term panels,
+// n = 1.2e5 - 4e5): a dense block of 1203 / 2753 / 4010 levels cost 9.9 / 78 / 194 ms per fit
+// against 1.6 / 2.8 / 2.5 ms with Jacobi (8 PCG iterations); up to 64 levels the inverse is a host
+// computation of microseconds.
+constexpr uint64_t DENSE_LIMIT_DEFAULT = 64;
 // Most FP64 iterations a mixed-precision solve runs before its FP32 correction sweeps (default).
 constexpr uint32_t MIXED_LEAD_DEFAULT = 8;
 // MI355X: 8 accelerator complex dies, each with its own L2; the hardware hands consecutive
@@ -112,6 +118,11 @@ struct Absorber {
     std::vector<DBuf<uint32_t>> codes_kept;  // the terms' codes, kept order (n each)
     std::vector<BlockInfo> blocks;
     uint64_t small_limit = 4096;
+    uint64_t dense_limit = DENSE_LIMIT_DEFAULT;  // largest small super-block factored densely
+    // the structure's own small/large split (as the reference classifies): ms and the blocks'
+    // small flags describe the solver's dense region, which is empty when it exceeds dense_limit
+    uint64_t ms_struct = 0;
+    std::vector<uint8_t> small_struct;
     int precision = FH_PLAIN_FP64;
     DBuf<uint32_t> pi, seg_off, cell_codes, cell_obs_off, cell_seg;
     // codes of the large blocks only (M x (nb - 1), row-major), for the segment pass when the one

@@ -4,8 +4,14 @@
 // fits the budget next to the requests already on the device (the head waits otherwise, unless
 // nothing is running). Each slot is a worker thread with its own absorber stream, so the
 // kernels of the requests in flight interleave on the device and a request that arrives while
-// others run starts as soon as a slot and memory free up. Results do not depend on the mix:
-// every reduction in the library is in a fixed order, independent of concurrent work.
+// others run starts as soon as a slot and memory free up.
+//
+// Packing: a slot that admits a small unweighted OLS request (iid or hetero variance) also
+// admits the consecutive queued requests of that kind, up to `pack` of them, and solves them as
+// ONE absorber: their fixed effects on disjoint level ranges, one structure build, one multi-RHS
+// PCG launch sequence over all their columns (ols_fit_packed). Each packed result equals its
+// solve alone within the request's tolerance; with pack = 1 every request runs alone, and a
+// result then does not depend on the mix at all (every reduction is in a fixed order).
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -14,6 +20,7 @@
 #include <thread>
 #include <unordered_map>
 
+#include "fh_fit.hpp"
 #include "fh_internal.hpp"
 
 namespace {
@@ -28,6 +35,14 @@ struct Job {
     fh_response resp{};
     bool done = false;
 };
+
+// Packable: small unweighted OLS with iid or hetero variance (the packed solve's range).
+constexpr size_t PACK_ROWS = 200'000;      // largest request packed
+constexpr size_t PACK_TOTAL = 2'000'000;   // rows of one packed solve
+bool packable(const fh_request& r) {
+    return r.model == FH_MODEL_OLS && (r.vcov_kind == FH_VCOV_IID || r.vcov_kind == FH_VCOV_HETERO) && r.K <= 15 &&
+           r.nterms <= 8 && r.n <= PACK_ROWS && !r.cluster;
+}
 
 uint64_t footprint(const fh_request& r) {
     // inputs (codes, y, X, cluster), the structure (pi, cells, level transposes), the row
@@ -73,14 +88,54 @@ struct fh_server {
         }
         if (ab) fh_absorber_destroy(ab);
         out.rc = rc;
+        out.packed = 1;
         out.iterations = st.iterations;
         out.pcg_iterations = st.pcg_iterations;
+    }
+
+    // One packed solve of several OLS requests.
+    void run_batch(const std::vector<Job*>& js) {
+        std::vector<fh::PackedOls> reqs(js.size());
+        std::vector<fh::PackedOls*> ptrs;
+        for (size_t i = 0; i < js.size(); i++) {
+            const fh_request& r = js[i]->req;
+            fh::PackedOls& p = reqs[i];
+            p.n = r.n;
+            p.K = r.K;
+            for (size_t q = 0; q < r.nterms; q++) {
+                p.codes.push_back(r.terms[q].codes);
+                p.g.push_back(r.terms[q].g);
+            }
+            p.y = r.y;
+            p.X = r.X;
+            p.kind = r.vcov_kind;
+            p.ssc = r.ssc;
+            p.tol = r.tol;
+            p.maxiter = r.maxiter;
+            p.beta = r.beta;
+            p.vcov = r.vcov;
+            p.kept = r.kept;
+            ptrs.push_back(&p);
+        }
+        try {
+            fh::ols_fit_packed(ptrs, cfg.precision);
+        } catch (const fh::Error& e) {
+            for (auto& p : reqs) p.rc = e.rc;
+        } catch (...) {
+            for (auto& p : reqs) p.rc = FH_INTERNAL;
+        }
+        for (size_t i = 0; i < js.size(); i++) {
+            js[i]->resp.rc = reqs[i].rc;
+            js[i]->resp.iterations = reqs[i].iterations;
+            js[i]->resp.pcg_iterations = reqs[i].iterations;
+            js[i]->resp.packed = (int32_t)js.size();
+        }
     }
 
     void worker() {
         (void)hipSetDevice(device);
         for (;;) {
-            Job* j = nullptr;
+            std::vector<Job*> js;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] {
@@ -91,23 +146,43 @@ struct fh_server {
                 });
                 if (stop && queue.empty()) return;
                 if (queue.empty()) continue;
-                j = queue.front();
+                js.push_back(queue.front());
                 queue.pop_front();
-                inflight_bytes += j->bytes;
+                uint64_t bytes = js[0]->bytes;
+                size_t rows = js[0]->req.n;
+                // the consecutive packable requests behind a packable head join its solve
+                if (cfg.pack > 1 && packable(js[0]->req)) {
+                    while (!queue.empty() && js.size() < (size_t)cfg.pack) {
+                        const Job* h = queue.front();
+                        if (!packable(h->req) || rows + h->req.n > PACK_TOTAL ||
+                            inflight_bytes + bytes + h->bytes > cfg.hbm_budget)
+                            break;
+                        js.push_back(queue.front());
+                        queue.pop_front();
+                        bytes += h->bytes;
+                        rows += h->req.n;
+                    }
+                }
+                inflight_bytes += bytes;
                 running++;
-                j->started = Clock::now();
+                const auto now = Clock::now();
+                for (Job* j : js) j->started = now;
             }
-            run_job(j);
+            if (js.size() == 1) run_job(js[0]);
+            else run_batch(js);
             {
                 std::lock_guard<std::mutex> lk(mu);
-                j->finished = Clock::now();
-                j->resp.queued_s = std::chrono::duration<double>(j->started - j->submitted).count();
-                j->resp.service_s = std::chrono::duration<double>(j->finished - j->started).count();
-                j->resp.total_s = std::chrono::duration<double>(j->finished - j->submitted).count();
-                j->resp.start_s = std::chrono::duration<double>(j->started - created).count();
-                j->resp.end_s = std::chrono::duration<double>(j->finished - created).count();
-                j->done = true;
-                inflight_bytes -= j->bytes;
+                const auto fin = Clock::now();
+                for (Job* j : js) {
+                    j->finished = fin;
+                    j->resp.queued_s = std::chrono::duration<double>(j->started - j->submitted).count();
+                    j->resp.service_s = std::chrono::duration<double>(j->finished - j->started).count();
+                    j->resp.total_s = std::chrono::duration<double>(j->finished - j->submitted).count();
+                    j->resp.start_s = std::chrono::duration<double>(j->started - created).count();
+                    j->resp.end_s = std::chrono::duration<double>(j->finished - created).count();
+                    j->done = true;
+                    inflight_bytes -= j->bytes;
+                }
                 running--;
             }
             cv_done.notify_all();
@@ -122,7 +197,7 @@ void fh_server_config_default(fh_server_config* c) {
     c->slots = 8;
     c->hbm_budget = 0;
     c->precision = FH_MIXED;
-    c->reserved = 0;
+    c->pack = 32;
 }
 
 int fh_server_create(const fh_server_config* cfg, fh_server** out) {
@@ -132,7 +207,7 @@ int fh_server_create(const fh_server_config* cfg, fh_server** out) {
         auto s = std::make_unique<fh_server>();
         if (cfg) s->cfg = *cfg;
         else fh_server_config_default(&s->cfg);
-        if (s->cfg.slots < 1 || s->cfg.slots > 64) return FH_INVALID;
+        if (s->cfg.slots < 1 || s->cfg.slots > 64 || s->cfg.pack < 0 || s->cfg.pack > 256) return FH_INVALID;
         if (hipGetDevice(&s->device) != hipSuccess) return FH_INTERNAL;
         if (s->cfg.hbm_budget == 0) {
             size_t fr = 0, tot = 0;

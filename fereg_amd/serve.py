@@ -1,7 +1,8 @@
 """Serving over the library's continuous-batching engine (fh_server_* in include/fereg_hip.h).
 
 `Server` admits regressions (OLS or Poisson PML) into a FIFO; the native engine runs up to
-`slots` of them at once, each on its own stream, admitted by HBM footprint. `poisson_stream`
+`slots` solves at once, each on its own stream, admitted by HBM footprint, and packs consecutive
+small OLS requests into one solve (one absorber, one multi-RHS PCG over all of them). `poisson_stream`
 draws the mixed request stream (log-uniform n, 2 or 3 FEs, K uniform, OLS/PPML mix) and
 `run_stream` replays it with Poisson arrivals at a fixed rate and reports latency percentiles
 and throughput.
@@ -31,6 +32,7 @@ class Reply:
     iterations: int
     start_s: float = 0.0  # admission and completion, seconds since the server was created
     end_s: float = 0.0
+    packed: int = 1  # requests in the solve this one ran in
 
 
 class _Pending:
@@ -41,8 +43,10 @@ class _Pending:
 
 
 class Server:
-    def __init__(self, slots=8, hbm_budget=0, precision=L.FH_MIXED):
-        cfg = L.ServerConfig(int(slots), int(hbm_budget), int(precision), 0)
+    def __init__(self, slots=8, hbm_budget=0, precision=L.FH_MIXED, pack=32):
+        """pack: most consecutive small OLS requests (iid / hetero, no weights) solved as one packed
+        absorber (1: every request alone)."""
+        cfg = L.ServerConfig(int(slots), int(hbm_budget), int(precision), int(pack))
         h = ctypes.c_void_p()
         L.check(L.lib().fh_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
@@ -77,7 +81,7 @@ class Server:
             p = self._pending.pop(ticket)
         K = p.beta.size
         return Reply(p.beta, p.vcov.reshape(K, K), p.kept.astype(bool), r.rc, r.queued_s, r.service_s, r.total_s,
-                     int(r.iterations), r.start_s, r.end_s)
+                     int(r.iterations), r.start_s, r.end_s, int(r.packed))
 
     def close(self):
         if getattr(self, "_h", None):

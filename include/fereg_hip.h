@@ -68,6 +68,9 @@ typedef struct {
     int32_t joint;        /* 1 (default): the escalation factors the joint matrix over every
                              coordinate when its budgets allow (the restart is then exact),
                              else each large block; 0: each large block only */
+    int32_t dense_limit;  /* largest small super-block (levels) preconditioned by its dense
+                             inverse (-1: the default, 64); a larger one keeps small_limit's
+                             structure (row order) but is preconditioned like large blocks */
 } fh_solver_config;
 
 /* SolveOpts (absorber.rs:18-24) plus the precision of this solve. */
@@ -102,7 +105,8 @@ typedef struct {
     uint64_t n, nterms, g1, ncells, nb, m, m_small, small_work;
     int64_t fe1, norm_fe1;
     int32_t has_cell_obs_off;
-    int32_t reserved;
+    int32_t dense_small; /* 1: the small super-block is preconditioned by its dense inverse (m_small
+                            <= dense_limit); 0: its blocks take Jacobi like the large ones */
 } fh_structure_info;
 
 const char* fh_last_error(void);
@@ -504,7 +508,10 @@ typedef struct {
     uint32_t slots;      /* requests in flight at once (1..64) */
     uint64_t hbm_budget; /* bytes admitted at once; 0 = three quarters of the free memory */
     int32_t precision;   /* FH_PLAIN_FP64 or FH_MIXED */
-    int32_t reserved;
+    int32_t pack;        /* most requests packed into one solve (default 32; 0 or 1: each alone):
+                            consecutive small unweighted OLS requests with iid or hetero variance
+                            share one absorber on disjoint level ranges and one PCG launch
+                            sequence, each result within its own tolerance of its solve alone */
 } fh_server_config;
 typedef struct {
     int32_t model;       /* FH_MODEL_OLS or FH_MODEL_PPML */
@@ -524,7 +531,7 @@ typedef struct {
 } fh_request;
 typedef struct {
     int32_t rc;
-    int32_t reserved;
+    int32_t packed;                      /* requests in the solve this one ran in (1: alone) */
     double queued_s, service_s, total_s; /* wait for admission, on the device, submit to done */
     uint64_t iterations, pcg_iterations;
     double start_s, end_s;               /* admission and completion, seconds since fh_server_create */

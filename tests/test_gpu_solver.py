@@ -277,3 +277,26 @@ def test_empty_levels_with_the_dense_small_block(precision):
     empty = np.setdiff1d(np.arange(gs[fe1]), codes[fe1])
     assert len(empty) > 0 and np.all(alpha[empty] == 0.0) and np.all(np.isfinite(alpha))
     a.close()
+
+
+@pytest.mark.parametrize("di", [2, 3])
+@pytest.mark.parametrize("dense_limit", [-1, 4096])
+def test_small_blocks_dense_or_jacobi(di, dense_limit):
+    """Small blocks of more than 64 levels in all (256 and 107 here): by default they keep the
+    structure's order but are preconditioned like large blocks; with a higher dense limit their
+    super-block is inverted densely on the GPU (pivot sweeps). Both reach the projection."""
+    import fereg_amd as F
+    codes, gs = residualize_designs()[di]
+    n = len(codes[0])
+    rng = np.random.default_rng(200 + di)
+    w = rng.uniform(0.2, 3.0, n)
+    a = F.Absorber(codes, gs, weights=w, dense_limit=dense_limit)
+    assert a.info.m_small == sum(g for g in gs[1:])  # the structure's split is the reference's
+    assert a.info.dense_small == (0 if dense_limit < 0 else 1)
+    v = rng.standard_normal((n, 2))
+    pi = a.row_order()
+    rows = np.ascontiguousarray(v[pi])
+    announce("dense limit", di, dense_limit)
+    a.residualize(rows, 2, F.SolveOpts(tol=1e-12))
+    ref = dense_residualize(v, codes, gs, w)[pi]
+    np.testing.assert_allclose(rows, ref, atol=1e-9)
