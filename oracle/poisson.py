@@ -127,12 +127,17 @@ def qr_coefs(block, kept, w):
 
 
 def ppml(codes, g, y, X, weights=None, offset=None, tol=1e-8, itol=1e-10, maxiter=100, vcov="hetero", cluster=None,
-         nested=None, k_adj=True, g_adj=True, k_fixef="nonnested"):
+         nested=None, k_adj=True, g_adj=True, k_fixef="nonnested", slope_terms=None):
     """Poisson PML of y on X (kept order) with the FEs of `codes`; returns a dict with coef, vcov,
     se, deviance, iterations, converged, eta, mu, w (IRLS weights), kept, df_k, and pi (the row
-    order of the internal arrays)."""
+    order of the internal arrays). slope_terms (codes, g, intercept, slope columns) replace
+    codes / g by terms with varying slopes, absorbed by the explicit-design projection."""
     n, K = X.shape
-    s = OracleSolver(codes, g)
+    if slope_terms is not None:
+        from . import slopes as _sl
+        s = _sl.DenseSlopeSolver(slope_terms, n)
+    else:
+        s = OracleSolver(codes, g)
     pi = s.structure()["pi"]
     yr = np.asarray(y, float)[pi]
     xr = np.ascontiguousarray(np.asarray(X, float)[pi])
@@ -230,10 +235,15 @@ def ppml(codes, g, y, X, weights=None, offset=None, tol=1e-8, itol=1e-10, maxite
     Xt = block[:, [1 + j for j in kept]]
     sub = G[np.ix_([1 + j for j in kept], [1 + j for j in kept])]
     _, bread = estimate.chol_collin(sub, np.finfo(float).tiny)
-    sizes = list(g)
-    np_ = len(kept) + (sum(sizes) - (len(sizes) - 1) if sizes else 0)
-    nest = nested if nested is not None else [False] * len(sizes)
-    dfk = estimate.df_k(vcov, np_, sizes, nest, k_adj=k_adj, k_fixef=k_fixef, n_design=len(kept))
+    if slope_terms is not None:
+        np_ = _sl.nparams(len(kept), slope_terms)
+        nest = nested if nested is not None else [False] * len(slope_terms)
+        dfk = _sl.df_k(vcov, np_, slope_terms, nest, k_fixef=k_fixef, n_design=len(kept))
+    else:
+        sizes = list(g)
+        np_ = len(kept) + (sum(sizes) - (len(sizes) - 1) if sizes else 0)
+        nest = nested if nested is not None else [False] * len(sizes)
+        dfk = estimate.df_k(vcov, np_, sizes, nest, k_adj=k_adj, k_fixef=k_fixef, n_design=len(kept))
     if vcov == "iid":
         V = bread * ((n - 1) / (n - dfk) if k_adj else 1.0)
     else:

@@ -116,3 +116,18 @@ def dense_residualize(block, codes, gs, w=None):
     sw = np.sqrt(w) if w is not None else np.ones(n)
     coef, *_ = np.linalg.lstsq(D * sw[:, None], block * sw[:, None], rcond=None)
     return block - D @ coef
+
+
+def slope_inputs(case):
+    """case_inputs plus the absorbed terms with their slopes: terms = [(codes, g, intercept,
+    [slope columns])] in the fixture's term order; X gains fixest's constant column when no term
+    has level intercepts."""
+    inp = case_inputs(case)
+    d = read_csv(case["data"])
+    obs = np.array(case["expect"]["obs"], dtype=np.int64) - 1
+    inp["terms"] = [(c, g, bool(t["intercept"]), [d[s][obs] for s in t["slopes"]])
+                    for c, g, t in zip(inp["codes"], inp["g"], case["fe"])]
+    if not any(t[2] for t in inp["terms"]):
+        # no absorbed intercept: fixest's model keeps its constant as the first regressor
+        inp["X"] = np.concatenate([np.ones((len(inp["y"]), 1)), inp["X"]], axis=1)
+    return inp

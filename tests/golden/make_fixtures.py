@@ -10,6 +10,8 @@ Sources (fixtures of crates/fereg-core/tests, written there by fixest 0.14.2):
   tests/data/fit_se.csv, fit_se_fixest.json   HC2 / HC3 / CRV3 cases (fixest 0.14.2 through
                                               tests/data/gen_se_fixtures.R; OLS and Poisson,
                                               intercept-only absorbed terms, no IV)
+  tests/data/fit_fixest_fe.json (sl_*),       varying slopes (fe[x] terms, with and without the
+  fit_fixest_glm.json (pois_sl*)              level intercepts): OLS and Poisson on fit_d1.csv
   tests/data/ols_data.csv, ols_s*.json        reghdfe 6.13.1 ground truth (tests/ols.rs:765-935,
                                               written by tests/data/gen_ols_fixtures.do); the
                                               model of each fixture restates the table of
@@ -97,7 +99,16 @@ def main():
         se.append({k: c[k] for k in ["name", "data", "model", "y", "x", "fe", "intercept", "weights", "vcov",
                                       "cluster", "ssc"]} | {"expect": e})
     json.dump(se, open(os.path.join(DST, "fixest_se.json"), "w"), indent=0)
-    print(len(out), "ols cases;", len(keep), "poisson cases;", len(rg), "reghdfe cases")
+    sl = []
+    for src, model in (("fit_fixest_fe.json", "ols"), ("fit_fixest_glm.json", "poisson")):
+        for c in json.load(open(os.path.join(SRC, src))):
+            if not any(t["slopes"] or not t["intercept"] for t in c["fe"]) or c.get("endog"):
+                continue
+            e = {k: c["expect"][k] for k in KEEP + ["deviance", "iterations", "fitted"] if k in c["expect"]}
+            sl.append({k: c[k] for k in ["name", "data", "y", "x", "fe", "weights", "vcov", "cluster", "ssc"]}
+                      | {"model": model, "offset": c.get("offset"), "expect": e})
+    json.dump(sl, open(os.path.join(DST, "fixest_slopes.json"), "w"), indent=0)
+    print(len(out), "ols cases;", len(sl), "slope cases;", len(keep), "poisson cases;", len(rg), "reghdfe cases")
 
 
 if __name__ == "__main__":
