@@ -34,7 +34,8 @@ class FeregError(RuntimeError):
 
 
 class Term(ctypes.Structure):
-    _fields_ = [("codes", u32p), ("g", ctypes.c_uint64), ("intercept", ctypes.c_int32)]
+    _fields_ = [("codes", u32p), ("g", ctypes.c_uint64), ("intercept", ctypes.c_int32), ("nslopes", ctypes.c_int32),
+                ("slopes", ctypes.POINTER(f64p))]
 
 
 class SolverConfig(ctypes.Structure):
@@ -222,6 +223,7 @@ SIGNATURES = [
     ("fh_absorber_residualize_dev", ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(SolveOptsC), vp, vp,
                                                    ctypes.POINTER(SolveInfoC), ctypes.POINTER(Ctx)]),
     ("fh_absorber_coef_rows", ctypes.c_size_t, [vp]),
+    ("fh_absorber_term_slots", ctypes.c_size_t, [vp]),
     ("fh_absorber_fit_rows", ctypes.c_int, [vp, f64p, ctypes.c_size_t, ctypes.c_size_t, f64p]),
     ("fh_absorber_term_values", ctypes.c_int, [vp, f64p, ctypes.c_size_t, ctypes.c_size_t, f64p, u64p]),
     ("fh_absorber_info", ctypes.c_int, [vp, ctypes.POINTER(StructureInfo)]),

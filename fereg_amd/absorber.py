@@ -111,28 +111,52 @@ class Absorber:
 
     codes: list of per-term kept-order codes (uint32 numpy arrays, or torch uint32/int32 device
     tensors), g: levels per term, weights: kept-order weights or None.
+    intercepts (per term, default all True) and slopes (per term, a list of kept-order slope columns,
+    default none) describe varying slopes (fe#c.x, fe##c.x): any slope or a term without intercept
+    makes this the varying-slopes solver (slopes.rs), which solves in FP64.
     """
 
     def __init__(self, codes, g, weights=None, small_limit=4096, precision=L.FH_PLAIN_FP64, bucket_log2=-1,
-                 autotune=False, verbose=0, mixed_lead=-1, sparse_chol=True, joint=True, dense_limit=-1):
+                 autotune=False, verbose=0, mixed_lead=-1, sparse_chol=True, joint=True, dense_limit=-1,
+                 intercepts=None, slopes=None):
         lib = L.lib()
         self._keep = []
+        ptr_arrays = []  # the per-term slope pointer arrays, alive through the create call
         n = None
         dev = any(_is_torch(c) for c in codes) or (weights is not None and _is_torch(weights))
         terms = (L.Term * max(1, len(codes)))()
+        intercepts = [True] * len(codes) if intercepts is None else list(intercepts)
+        slopes = [[] for _ in codes] if slopes is None else [list(s) for s in slopes]
+        if len(intercepts) != len(codes) or len(slopes) != len(codes):
+            raise L.FeregError(L.FH_INVALID, "intercepts and slopes need one entry per term")
         for q, (c, gq) in enumerate(zip(codes, g)):
+            sl, xs = [], []
+            for x in slopes[q]:
+                if dev:
+                    x = _dev_f64(x, "slopes").contiguous()
+                    sl.append(ctypes.cast(x.data_ptr(), L.f64p))
+                else:
+                    x = np.ascontiguousarray(x, dtype=np.float64)
+                    sl.append(_ptr(x, L.f64p))
+                xs.append(x)
+            self._keep.extend(xs)
+            sarr = (L.f64p * max(1, len(sl)))(*sl)
+            ptr_arrays.append(sarr)
+            icpt = 1 if intercepts[q] else 0
             if dev:
                 if not _is_torch(c):
                     raise L.FeregError(L.FH_INVALID, "codes: mix of torch device tensors and host arrays")
                 c = _dev_codes(c, "codes").contiguous()
                 self._keep.append(c)
-                terms[q] = L.Term(ctypes.cast(c.data_ptr(), L.u32p), int(gq), 1)
+                terms[q] = L.Term(ctypes.cast(c.data_ptr(), L.u32p), int(gq), icpt, len(sl), sarr)
                 nq = c.numel()
             else:
                 c = np.ascontiguousarray(c, dtype=np.uint32)
                 self._keep.append(c)
-                terms[q] = L.Term(_ptr(c, L.u32p), int(gq), 1)
+                terms[q] = L.Term(_ptr(c, L.u32p), int(gq), icpt, len(sl), sarr)
                 nq = len(c)
+            if any((x.numel() if dev else len(x)) != nq for x in xs):
+                raise L.FeregError(L.FH_INVALID, "slope column length differs from the kept sample")
             if n is None:
                 n = nq
             elif n != nq:
@@ -292,9 +316,11 @@ class Absorber:
         return out
 
     def term_values(self, c: FeCoefs, col: int) -> List[np.ndarray]:
-        nt = self.info.nterms
+        """Per term (per slot with varying slopes: each term's intercept, then its slopes) the
+        level values (solve.rs:1319-1343, slopes.rs:2435-2464)."""
+        nt = int(L.lib().fh_absorber_term_slots(self._h))
         lens = np.zeros(max(nt, 1), np.uint64)
-        tot = self.info.g1 + self.info.m
+        tot = self.coef_rows()
         out = np.zeros(max(tot, 1))
         data = np.ascontiguousarray(c.data, dtype=np.float64)
         L.check(L.lib().fh_absorber_term_values(self._h, _ptr(data, L.f64p), c.r, col, _ptr(out, L.f64p),

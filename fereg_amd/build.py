@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfereg_hip.so")
 SOURCES = ["fh_alloc.hip", "fh_sort.hip", "fh_structure.hip", "fh_weights.hip", "fh_operator.hip", "fh_operator32.hip", "fh_pcg.hip",
-           "fh_estimate.hip", "fh_linalg.hip", "fh_irls.hip", "fh_wboot.hip", "fh_serve.hip", "fh_noalloc.hip", "fh_factor.hip", "fh_reghdfe.hip", "fh_probe.hip", "fh_ldl.hip", "fh_escalate.hip", "fh_leverage.hip", "fh_crv3.hip", "fh_api.hip"]
+           "fh_estimate.hip", "fh_linalg.hip", "fh_irls.hip", "fh_wboot.hip", "fh_serve.hip", "fh_noalloc.hip", "fh_factor.hip", "fh_reghdfe.hip", "fh_probe.hip", "fh_ldl.hip", "fh_escalate.hip", "fh_leverage.hip", "fh_crv3.hip", "fh_slopes.hip", "fh_api.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
 

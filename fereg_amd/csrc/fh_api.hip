@@ -285,6 +285,7 @@ void restore_user_weights(Absorber& a) {
 // vectors; the caller's block, warm start and coefficients keep r columns.
 ResidOut residualize_block(Absorber& a, double* z, int r, double tol, uint64_t maxiter, const double* warm_dev,
                            double* coefs_dev, int precision, const fh_ctx* ctx) {
+    if (a.slope) return slope_residualize(a, z, r, tol, maxiter, warm_dev, coefs_dev, ctx);
     hipStream_t st = a.st;
     const size_t m = a.m, g1 = a.g1;
     const int rp = padded_cols(r);
@@ -581,20 +582,45 @@ void ols_fit_packed(const std::vector<PackedOls*>& reqs, int precision) {
 
 // ------------------------------------------------------------------ estimation helpers (host, K x K)
 
+static bool term_has_icpt(const Absorber& a, size_t q) { return q >= a.term_icpt.size() || a.term_icpt[q]; }
+static double term_ns(const Absorber& a, size_t q) { return q < a.term_ns.size() ? (double)a.term_ns[q] : 0.0; }
+
+// fixest nparams' FE part (fit/ols.rs:25-37): every term G (intercept + slopes), minus one per
+// intercept term beyond the first.
 double fe_params(const Absorber& a) {
     double s = 0.0;
-    for (uint64_t g : a.term_g) s += (double)g;
-    return a.nterms > 0 ? s - ((double)a.nterms - 1.0) : 0.0;
+    int nic = 0;
+    for (size_t q = 0; q < a.nterms; q++) {
+        const bool ic = term_has_icpt(a, q);
+        s += (double)a.term_g[q] * ((ic ? 1.0 : 0.0) + term_ns(a, q));
+        nic += ic ? 1 : 0;
+    }
+    return s - (double)std::max(nic - 1, 0);
+}
+
+// The FE parameters df.K removes (fit/vcov.rs:63-69): the intercept terms' levels (slope-only
+// terms count 0) minus one per such term beyond the first.
+static double df_size(const Absorber& a, size_t q) { return term_has_icpt(a, q) ? (double)a.term_g[q] : 0.0; }
+static double fe_params_df(const Absorber& a, int* n_ok_out) {
+    double s = 0.0;
+    int n_ok = 0;
+    for (size_t q = 0; q < a.nterms; q++) {
+        s += df_size(a, q);
+        n_ok += df_size(a, q) > 0.0 ? 1 : 0;
+    }
+    if (n_ok_out) *n_ok_out = n_ok;
+    return n_ok > 0 ? s - ((double)n_ok - 1.0) : 0.0;
 }
 
 // fixest df.K (fit/vcov.rs:63-112, K.exact = FALSE).
 double df_k(const Absorber& a, int kind, const fh_ssc& ssc, const int32_t* nested, double np, int kdesign) {
     const size_t nt = a.nterms;
-    const double fe_par = fe_params(a);
+    int n_ok = 0;
+    const double fe_par = fe_params_df(a, &n_ok);
     int kf = ssc.k_fixef;
     if (kind != FH_VCOV_CLUSTER && kind != FH_VCOV_CRV3 && kf == FH_KFIXEF_NONNESTED) kf = FH_KFIXEF_FULL;
     if (kf == FH_KFIXEF_NONE) return np - fe_par;
-    if (kf == FH_KFIXEF_FULL || nt == 0) return np;
+    if (kf == FH_KFIXEF_FULL || nt == 0 || n_ok == 0) return np;
     int n_nested = 0;
     for (size_t q = 0; q < nt; q++) n_nested += nested && nested[q] ? 1 : 0;
     double k = np;
@@ -604,8 +630,8 @@ double df_k(const Absorber& a, int kind, const fh_ssc& ssc, const int32_t* neste
         double s = 0.0, c = 0.0;
         for (size_t q = 0; q < nt; q++)
             if (nested && nested[q]) {
-                s += (double)a.term_g[q];
-                c += a.term_g[q] > 0 ? 1.0 : 0.0;
+                s += df_size(a, q);
+                c += df_size(a, q) > 0.0 ? 1.0 : 0.0;
             }
         k -= s - c;
     }
@@ -909,9 +935,14 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
         require(out != nullptr, "null output handle");
         *out = nullptr;
         require(nterms == 0 || terms != nullptr, "null terms");
+        bool slopes = false;
         for (size_t q = 0; q < nterms; q++) {
-            require(terms[q].intercept == 1, "slope terms are not supported by this solver");
+            require(terms[q].intercept == 0 || terms[q].intercept == 1, "intercept must be 0 or 1");
+            require(terms[q].nslopes >= 0 && (terms[q].nslopes == 0 || terms[q].slopes != nullptr),
+                    "null slope columns");
+            require(terms[q].intercept == 1 || terms[q].nslopes > 0, "absorb term without intercept or slopes");
             require(terms[q].codes != nullptr || n == 0, "null term codes");
+            slopes |= terms[q].intercept != 1 || terms[q].nslopes > 0;
         }
         require(n >= 1, "no observations", FH_TOO_FEW);
         auto h = std::make_unique<fh_absorber>();
@@ -999,6 +1030,35 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
                 a.codes_kept[q] = std::move(up[q]);
             }
             dc[q] = a.codes_kept[q].p;
+        }
+        for (size_t q = 0; q < nterms; q++) {
+            a.term_icpt.push_back(terms[q].intercept == 1 ? 1 : 0);
+            a.term_ns.push_back((uint32_t)terms[q].nslopes);
+        }
+        if (slopes) {
+            // the varying-slopes solver (solve.rs:338-349 chooses it the same way)
+            std::vector<SlopeTerm> sts(nterms);
+            std::vector<DBuf<double>> sup;
+            for (size_t q = 0; q < nterms; q++) {
+                sts[q].codes = dc[q];
+                sts[q].g = terms[q].g;
+                sts[q].icpt = terms[q].intercept == 1;
+                for (int k = 0; k < terms[q].nslopes; k++) {
+                    require(terms[q].slopes[k] != nullptr, "null slope column");
+                    if (dev) {
+                        sts[q].slopes.push_back(terms[q].slopes[k]);
+                    } else {
+                        sup.emplace_back(n);
+                        h2d(sup.back().p, terms[q].slopes[k], n, a.st);
+                        sts[q].slopes.push_back(sup.back().p);
+                    }
+                }
+            }
+            slope_create(a, sts, wk);
+            stream_wait(a.st);
+            back.armed = false;
+            *out = hold.release();
+            return;
         }
         auto t0 = std::chrono::steady_clock::now();
         build_structure(a, dc);
@@ -1145,10 +1205,27 @@ int fh_absorber_residualize_dev(fh_absorber* ab, double* z, size_t r, const fh_s
 
 size_t fh_absorber_coef_rows(const fh_absorber* ab) { return ab ? ab->a.m + ab->a.g1 : 0; }
 
+size_t fh_absorber_term_slots(const fh_absorber* ab) {
+    if (!ab) return 0;
+    const Absorber& a = ab->a;
+    size_t s = 0;
+    for (size_t q = 0; q < a.nterms; q++)
+        s += (q >= a.term_icpt.size() || a.term_icpt[q] ? 1 : 0) + (q < a.term_ns.size() ? a.term_ns[q] : 0);
+    return s;
+}
+
 int fh_absorber_info(const fh_absorber* ab, fh_structure_info* o) {
     return guard([&] {
         require(ab && o, "null argument");
         const Absorber& a = ab->a;
+        if (a.slope) {
+            // E's levels as the segments, one cell per row, the non-E blocks
+            int64_t e = 0;
+            uint64_t ge = 0, be = 0, msm = 0, nb = 0;
+            slope_summary(a, &e, &ge, &be, &msm, &nb);
+            *o = fh_structure_info{a.n, a.nterms, ge, a.n, nb, a.m, msm, 0, e, a.norm_fe1, 0, (int32_t)(a.ms > 0)};
+            return;
+        }
         *o = fh_structure_info{a.n,        a.nterms,  a.g1,
                                a.M,        a.nb,      a.m,
                                a.ms_struct, a.small_work, a.fe1,
@@ -1161,6 +1238,7 @@ int fh_absorber_copy_structure(fh_absorber* ab, uint32_t* pi, uint32_t* seg_off,
         require(ab != nullptr, "null absorber");
         Absorber& a = ab->a;
         AbsScope g(a);
+        require(!a.slope, "not available for an absorber with varying slopes (no cell structure)");
         if (pi) d2h(pi, a.pi.p, a.n, a.st);
         if (seg_off) d2h(seg_off, a.seg_off.p, a.g1 + 1, a.st);
         if (cc) d2h(cc, a.cell_codes.p, a.M * a.nb, a.st);
@@ -1173,6 +1251,10 @@ int fh_absorber_copy_blocks(const fh_absorber* ab, int64_t* term, uint64_t* g, u
     return guard([&] {
         require(ab != nullptr, "null absorber");
         const Absorber& a = ab->a;
+        if (a.slope) {
+            slope_blocks(a, term, g, off, small);
+            return;
+        }
         for (size_t b = 0; b < a.nb; b++) {
             term[b] = a.blocks[b].term;
             g[b] = a.blocks[b].g;
@@ -1186,6 +1268,7 @@ int fh_absorber_copy_level_cells(fh_absorber* ab, size_t b, uint32_t* perm, uint
     return guard([&] {
         require(ab != nullptr, "null absorber");
         Absorber& a = ab->a;
+        require(!a.slope, "not available for an absorber with varying slopes (no cell structure)");
         require(b < a.nb, "block index out of range");
         AbsScope g(a);
         ensure_level_cells(a, b);
@@ -1200,6 +1283,7 @@ int fh_absorber_copy_weight_state(fh_absorber* ab, double* n1, double* nr, doubl
         require(ab != nullptr, "null absorber");
         Absorber& a = ab->a;
         AbsScope g(a);
+        require(!a.slope, "not available for an absorber with varying slopes (no cell structure)");
         restore_user_weights(a);
         if (n1) d2h(n1, a.n1.p, a.g1, a.st);
         if (nr) d2h(nr, a.nr.p, a.m, a.st);
@@ -1219,6 +1303,7 @@ int fh_absorber_apply(fh_absorber* ab, const double* p, size_t r, double* y, int
         restore_user_weights(a);
         const size_t m = a.m;
         if (m == 0) return;
+        require(!a.slope || precision != FH_MIXED, "an absorber with varying slopes applies S in FP64");
         if (precision == FH_MIXED) {
             std::vector<float> hp(p, p + m * r), hy(m * r);
             DBuf<float> dp(m * r), dy(m * r);
@@ -1242,6 +1327,7 @@ int fh_absorber_get_plan(const fh_absorber* ab, int32_t precision, size_t r, fh_
         require(ab && out && r >= 1 && r <= 16, "bad argument");
         require(precision == FH_PLAIN_FP64 || precision == FH_MIXED, "unknown precision");
         const Absorber& a = ab->a;
+        require(!a.slope, "not available for an absorber with varying slopes (no cell structure)");
         const SpmvPlan& pl = a.plan_for(precision == FH_PLAIN_FP64, (int)r);
         *out = fh_spmv_plan{};
         out->dense = pl.dense;
@@ -1261,6 +1347,7 @@ int fh_absorber_set_plan(fh_absorber* ab, int32_t precision, size_t r, const fh_
         require(ab && in && r >= 1 && r <= 16, "bad argument");
         require(precision == FH_PLAIN_FP64 || precision == FH_MIXED, "unknown precision");
         const Absorber& a = ab->a;
+        require(!a.slope, "not available for an absorber with varying slopes (no cell structure)");
         auto pow2 = [](uint32_t v) { return v == 0 || (v <= 64 && (v & (v - 1)) == 0); };
         require(in->dense == 0 || in->dense == 1, "plan: dense must be 0 or 1");
         require(pow2(in->seg_lanes), "plan: lanes must be 0 or a power of two up to 64");
@@ -1286,6 +1373,7 @@ int fh_absorber_time_spmv(fh_absorber* ab, size_t r, int32_t precision, uint32_t
     return guard([&] {
         require(ab && ms && bytes && r >= 1 && r <= 16 && reps >= 1, "bad argument");
         Absorber& a = ab->a;
+        require(!a.slope, "not available for an absorber with varying slopes (no cell structure)");
         AbsScope g(a);
         restore_user_weights(a);
         const size_t m = a.m;
@@ -1328,6 +1416,15 @@ int fh_absorber_fit_rows(fh_absorber* ab, const double* coefs, size_t r, size_t 
         require(ab && coefs && out && col < r, "bad argument");
         Absorber& a = ab->a;
         AbsScope g(a);
+        if (a.slope) {
+            const size_t p = a.m + a.g1;
+            DBuf<double> dc(p * r), dout(a.n);
+            h2d(dc.p, coefs, p * r, a.st);
+            slope_fit_rows(a, dc.p, (int)r, (int)col, dout.p);
+            d2h(out, dout.p, a.n, a.st);
+            stream_wait(a.st);
+            return;
+        }
         require(a.nb <= (size_t)NB_MAX, "fit_rows: too many absorbed terms");
         const size_t p = a.m + a.g1;
         DBuf<double> dc(p * r), dout(a.n);
@@ -1352,6 +1449,15 @@ int fh_absorber_term_values(fh_absorber* ab, const double* coefs, size_t r, size
         AbsScope g(a);
         restore_user_weights(a);
         const size_t nt = a.nterms;
+        if (a.slope) {
+            size_t pos = 0, k = 0;
+            for (auto& v : slope_term_values(a, coefs, r, col)) {
+                lens[k++] = v.size();
+                std::memcpy(out + pos, v.data(), v.size() * sizeof(double));
+                pos += v.size();
+            }
+            return;
+        }
         if (a.fe1 < 0 || a.norm_fe1 < 0) {
             for (size_t q = 0; q < nt; q++) lens[q] = 0;
             return;
@@ -1439,6 +1545,7 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
                 "null argument");
         Absorber& a = ab->a;
         AbsScope g(a);
+        require(!a.slope, "the wild bootstrap with varying slopes is not supported");
         restore_user_weights(a);
         const size_t n = a.n;
         DBuf<double> dy, dx;

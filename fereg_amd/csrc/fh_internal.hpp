@@ -87,7 +87,8 @@ struct SpmvPlan {
 };
 
 struct ClusterRuns;
-struct FacGpu;  // fh_escalate.hip
+struct FacGpu;      // fh_escalate.hip
+struct SlopeState;  // fh_slopes.hip
 
 // The PCG's status as the device publishes it to pinned host memory after every iteration
 // (zero-copy: the host polls it without a copy kernel); iters_done (the iteration count, the done
@@ -199,6 +200,14 @@ struct Absorber {
     double joint_pairs = -1.0, joint_nnz = -1.0, joint_len = -1.0, joint_flops = -1.0;
     uint32_t mixed_lead = MIXED_LEAD_DEFAULT;  // most FP64 iterations of a mixed solve before its FP32 sweeps
     HostPoll poll;  // the PCG status mirror
+
+    // Varying slopes (fh_slopes.hip): set when a term has slopes or no intercept. The absorber then
+    // has an observation-level structure of its own; pi, n, m, ms, g1 (= E's levels x its width,
+    // the FeCoefs tail), the weights, w64.inv_diag (zero: the block-Jacobi factors cover the large
+    // coordinates) and w64.minv (the dense small inverse) keep their meaning for the PCG.
+    std::shared_ptr<SlopeState> slope;
+    std::vector<uint8_t> term_icpt;  // per term: level intercepts (1 for every intercept-only term)
+    std::vector<uint32_t> term_ns;   // per term: slope columns
 
     bool trivial() const { return cell_obs_off.n == 0; }
     bool unit() const { return !weighted && trivial(); }
@@ -353,6 +362,45 @@ void sum_parts_to(const double* part, size_t nparts, size_t len, double* out_dev
 // so every row of p, t and x is whole 16-byte vectors (r <= 2 rows are 8 or 16 bytes already).
 // The pad columns are zero and inactive.
 inline int padded_cols(int r) { return r <= 2 ? r : r <= 4 ? 4 : r <= 8 ? 8 : r <= 12 ? 12 : 16; }
+
+// ---------------------------------------------------------------- varying slopes (fh_slopes.hip)
+// One absorbed term of a slope absorber: device codes and slope columns, kept order.
+struct SlopeTerm {
+    const uint32_t* codes = nullptr;
+    uint64_t g = 0;
+    bool icpt = true;
+    std::vector<const double*> slopes;
+};
+constexpr int SL_BMAX = 8;  // widest term: intercept + slopes
+// Builds the observation-level structure (slopes.rs:354-527) and the weight state for kept-order
+// weights w_kept (device, null = unit).
+void slope_create(Absorber& a, const std::vector<SlopeTerm>& terms, const double* w_kept);
+// Weight state for row-order weights (device, null = unit; zeros allowed: rows left out).
+void slope_refresh(Absorber& a, const double* w_rows);
+// y = S p (m x r, FP64) with optional p'y partials in the PCG's slot layout (SPMV_DOTG x 16).
+void slope_spmv(const Absorber& a, const double* p, double* y, int r, const uint32_t* done, hipStream_t st,
+                double* pq);
+// z = M^-1 r on the block-Jacobi coordinates [ms, m) and their r'z partials (SPMV_DOTG x 16).
+void slope_precond(Absorber& a, const double* rr, double* z, int r, double* rzf_part, const uint32_t* done,
+                   hipStream_t st);
+struct ResidOut;
+// One residualization of the n x r device block z (row order, r <= 16), solve.rs:1141-1311 with the
+// slope operator; warm / coefs (m + g1) x r.
+ResidOut slope_residualize(Absorber& a, double* z, int r, double tol, uint64_t maxiter, const double* warm,
+                           double* coefs, const fh_ctx* ctx);
+// Fit of column col of coefs ((m + g1) x r, device) per row (row order).
+void slope_fit_rows(Absorber& a, const double* coefs, int r, int col, double* out);
+// term_values (slopes.rs:2435-2464): one vector per slot (each term's intercept, then its slopes),
+// from host coefs ((m + g1) x r).
+std::vector<std::vector<double>> slope_term_values(Absorber& a, const double* coefs, size_t r, size_t col);
+// Block layout (term, levels, first coordinate, small flag) of a slope absorber.
+void slope_blocks(const Absorber& a, int64_t* term, uint64_t* g, uint64_t* off, int32_t* small);
+// The E term, its levels and width, and the small coordinates.
+void slope_summary(const Absorber& a, int64_t* e, uint64_t* ge, uint64_t* be, uint64_t* m_small, uint64_t* nb);
+// The grounded explicit inverse of a dense symmetric block (precond/dense.rs), redundant
+// coordinates left out; nkeep receives the coordinates kept (fh_weights.hip).
+std::vector<double> grounded_inverse(const std::vector<double>& a, size_t ms, const std::vector<uint8_t>& red,
+                                     uint64_t& nkeep);
 
 // ---------------------------------------------------------------- PCG (fh_pcg.hip)
 struct PcgResult {

@@ -183,6 +183,7 @@ void fe_leverage_cells(Absorber& a, double* hcell, const fh_ctx* ctx) {
 void hc_scores(Absorber& a, const double* rows, int ld, const std::vector<int>& kept, const std::vector<double>& bread,
                const double* e, size_t es, const double* w, bool row_weight, bool hc3, double* e_out) {
     hipStream_t st = a.st;
+    require(!a.slope, "HC2/HC3 with varying slopes are not supported (the leverage of slope terms is not built)");
     const int nk = (int)kept.size();
     require(nk <= HC_KMAX, "HC2/HC3: at most 32 kept regressors");
     DBuf<double> hcell(std::max<uint64_t>(a.M, 1)), B(std::max(nk * nk, 1));

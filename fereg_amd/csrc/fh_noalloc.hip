@@ -96,6 +96,7 @@ int fh_absorber_apply_noalloc(fh_absorber* ab, const double* p, size_t r, double
         require(r >= 1 && r <= 16, "1..16 columns");
         require(precision == FH_PLAIN_FP64 || precision == FH_MIXED, "unknown precision");
         Absorber& a = ab->a;
+        require(!a.slope, "apply_noalloc: not available for an absorber with varying slopes");
         require(!a.irls_weights,
                 "apply_noalloc: the absorber holds a Poisson fit's IRLS weights; restoring the caller's weights "
                 "allocates, so call another entry point (or set_weights) first");

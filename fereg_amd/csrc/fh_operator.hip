@@ -16,6 +16,10 @@ template OpArgs<float> op_args<float>(const Absorber&);
 template <class T>
 void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st,
           double* pq) {
+    if (a.slope) {
+        slope_spmv(a, p, y, r, done, st, pq);
+        return;
+    }
     spmv_planned<T>(a, p, y, r, s, done, st, pq);
 }
 template void spmv<double>(const Absorber&, const double*, double*, int, Scratch<double>&, const uint32_t*, hipStream_t,

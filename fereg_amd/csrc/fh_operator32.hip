@@ -6,6 +6,7 @@ namespace fh {
 template <class T>
 void spmv(const Absorber& a, const T* p, T* y, int r, Scratch<T>& s, const uint32_t* done, hipStream_t st,
           double* pq) {
+    require(!a.slope, "slope absorbers solve in FP64", FH_INTERNAL);
     spmv_planned<T>(a, p, y, r, s, done, st, pq);
 }
 template void spmv<float>(const Absorber&, const float*, float*, int, Scratch<float>&, const uint32_t*, hipStream_t,

@@ -480,6 +480,13 @@ __global__ __launch_bounds__(TPB) void k_precond(const T* __restrict__ inv_diag,
 // The escalated factors' part of z = M^-1 r (FP64 only: an absorber with factors solves in FP64).
 template <class T>
 void precond_factors(Absorber& a, const T* rr, T* z, int r, double* dpart, const uint32_t* done, hipStream_t st) {
+    if (a.slope) {
+        if constexpr (std::is_same<T, double>::value)
+            slope_precond(a, rr, z, r, dpart + (size_t)S_RZF * DOTG * RMAX, done, st);
+        else
+            throw Error(FH_INTERNAL, "slope absorbers solve in FP64");
+        return;
+    }
     if (a.facs.empty()) return;
     if constexpr (std::is_same<T, double>::value)
         facs_apply(a, rr, z, r, dpart + (size_t)S_RZF * DOTG * RMAX, done, st);
@@ -542,7 +549,7 @@ PcgRun run_pcg(Absorber& a, const T* rhs, T* x, bool x_nonzero, int r, const dou
     h.rel2 = rel2;
     h.maxiter = maxiter;
     h.can_escalate = can_escalate ? 1 : 0;
-    h.fac = a.facs.empty() ? 0 : 1;
+    h.fac = (a.facs.empty() && !a.slope) ? 0 : 1;  // the slope block Jacobi writes the factor slot
     h.handover_at = handover_at;
     h.probe_rho = -1.0;
     h.probe_rem = -1.0;

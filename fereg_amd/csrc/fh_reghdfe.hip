@@ -633,6 +633,7 @@ static int reghdfe_impl(fh_absorber* ab, const double* y, const double* X, size_
         require(ab && y && X && sp && b && V && status && dof_table, "null argument");
         Absorber& a = ab->a;
         AbsScope g(a);
+        require(!a.slope, "the reghdfe convention with varying slopes is not supported");
         restore_user_weights(a);
         const size_t n = a.n;
         require(sp->nterms == 0 || sp->terms, "null terms");

@@ -40,8 +40,12 @@ struct Job {
 constexpr size_t PACK_ROWS = 200'000;      // largest request packed
 constexpr size_t PACK_TOTAL = 2'000'000;   // rows of one packed solve
 bool packable(const fh_request& r) {
-    return r.model == FH_MODEL_OLS && (r.vcov_kind == FH_VCOV_IID || r.vcov_kind == FH_VCOV_HETERO) && r.K <= 15 &&
-           r.nterms <= 8 && r.n <= PACK_ROWS && !r.cluster;
+    if (!(r.model == FH_MODEL_OLS && (r.vcov_kind == FH_VCOV_IID || r.vcov_kind == FH_VCOV_HETERO) && r.K <= 15 &&
+          r.nterms <= 8 && r.n <= PACK_ROWS && !r.cluster))
+        return false;
+    for (size_t q = 0; q < r.nterms; q++)  // intercept-only terms: their cell structures pack
+        if (r.terms[q].intercept != 1 || r.terms[q].nslopes != 0) return false;
+    return true;
 }
 
 uint64_t footprint(const fh_request& r) {

@@ -406,6 +406,8 @@ __global__ void k_sweep_scatter(const double* __restrict__ A, int k, const uint3
     }
 }
 
+}  // namespace
+
 // Grounded LDL' of S_SS over the kept coordinates and its explicit inverse
 // (precond/dense.rs:25-37, 79-160): unit-diagonal scaling, pivots <= 1e-10 replaced by 1e30.
 std::vector<double> grounded_inverse(const std::vector<double>& a, size_t ms, const std::vector<uint8_t>& red,
@@ -446,7 +448,6 @@ std::vector<double> grounded_inverse(const std::vector<double>& a, size_t ms, co
     return inv;
 }
 
-}  // namespace
 
 double absorber_sw(Absorber& a) {
     if (!a.sw_ready) {
@@ -457,6 +458,10 @@ double absorber_sw(Absorber& a) {
 }
 
 void refresh_weights(Absorber& a, const double* w_rows_dev) {
+    if (a.slope) {
+        slope_refresh(a, w_rows_dev);
+        return;
+    }
     hipStream_t st = a.st;
     const size_t M = a.M, g1 = a.g1, m = a.m, ms = a.ms;
     a.weighted = w_rows_dev != nullptr;

@@ -83,7 +83,7 @@ def _cluster_dims(cluster):
 
 
 def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, extra_args, offset=None,
-         ctx=None):
+         ctx=None, intercepts=None, slopes=None):
     lib = L.lib()
     X = np.asarray(X, float) if not _is_torch(X) else X
     if not _is_torch(X) and X.ndim == 1:
@@ -94,7 +94,7 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
     dims = _cluster_dims(cluster)
     if len(dims) > 4:
         raise L.FeregError(L.FH_INVALID, "at most four cluster dimensions are supported")
-    ab = absorber or Absorber(codes, g, weights=weights)
+    ab = absorber or Absorber(codes, g, weights=weights, intercepts=intercepts, slopes=slopes)
     kind = L.FH_VCOV[vcov]
     ssc = ssc or {}
     sscc = L.Ssc(int(ssc.get("K.adj", True)), L.FH_KFIXEF[ssc.get("K.fixef", "nonnested")], int(ssc.get("G.adj", True)), 0)
@@ -145,18 +145,23 @@ def _fit(kind_fn, y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, abs
     return res
 
 
-def feols(y, X, codes, g, weights=None, vcov="iid", cluster=None, nested=None, ssc=None, opts=None, absorber=None):
+def feols(y, X, codes, g, weights=None, vcov="iid", cluster=None, nested=None, ssc=None, opts=None, absorber=None,
+          intercepts=None, slopes=None):
     """OLS with absorbed FEs (fit/ols.rs:213-330) on the GPU. cluster: one code array or a list
-    of up to four (multiway, fixest's ssc "G.df" rule)."""
-    return _fit("fh_ols_fit", y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, [])
+    of up to four (multiway, fixest's ssc "G.df" rule). intercepts / slopes: per term, varying
+    slopes as for Absorber (a model whose terms have no intercept keeps its constant among X,
+    as fixest does)."""
+    return _fit("fh_ols_fit", y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber, [],
+                intercepts=intercepts, slopes=slopes)
 
 
 def fepois(y, X, codes, g, weights=None, offset=None, vcov="hetero", cluster=None, nested=None, ssc=None, tol=1e-8,
-           maxiter=100, opts=None, absorber=None, ctx=None):
+           maxiter=100, opts=None, absorber=None, ctx=None, intercepts=None, slopes=None):
     """Poisson PML by IRLS (glm/irls.rs:367-913, fixest flavour) on the GPU. `weights` are prior
     weights (or the absorber's, when one is passed), `offset` enters the linear predictor."""
     return _fit("fh_ppml_fit", y, X, codes, g, weights, vcov, cluster, nested, ssc, opts, absorber,
-                [ctypes.c_double(tol), ctypes.c_uint64(maxiter)], offset=offset, ctx=ctx)
+                [ctypes.c_double(tol), ctypes.c_uint64(maxiter)], offset=offset, ctx=ctx, intercepts=intercepts,
+                slopes=slopes)
 
 
 def nested_by_value(fe_codes, cluster_codes):

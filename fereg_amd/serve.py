@@ -164,3 +164,14 @@ def run_stream(server, specs, rate, gen_threads=8, ahead=16, progress=None):
     wall = time.perf_counter() - t0
     pool.shutdown()
     return replies, wall, late
+
+
+def run_burst(server, specs, reqs):
+    """Submits already generated requests (make_request of each spec) all at once and waits for
+    every reply: the engine's throughput with a full queue. Returns the replies and the wall time
+    from the first submission to the last reply."""
+    t0 = time.perf_counter()
+    tickets = [server.submit(s["model"], *r, vcov="hetero" if s["model"] == "ppml" else "iid")
+               for s, r in zip(specs, reqs)]
+    replies = [server.wait(t) for t in tickets]
+    return replies, time.perf_counter() - t0

@@ -42,12 +42,18 @@ extern "C" {
 
 typedef struct fh_absorber fh_absorber;
 
-/* One absorbed term (session::Term, intercept only): dense codes 0..g-1 in kept order.
- * Replaces `Term { codes, g, intercept, .. }` (crates/fereg-core/src/session.rs). */
+/* One absorbed term: dense codes 0..g-1 in kept order, with level intercepts (intercept = 1) and/or
+ * varying slopes (fe#c.x: nslopes kept-order columns, the term's slope variables). Replaces
+ * `Term { codes, g, intercept, slopes }` (crates/fereg-core/src/session.rs). An absorber with a
+ * slope term or a term without intercept is the varying-slopes solver (slopes.rs: SlopeSolver,
+ * solve.rs:338-349 picks it the same way); at most 3 slopes per term (4 with the intercept).
+ * Zero-initialized trailing fields give an intercept-only term. */
 typedef struct {
     const uint32_t* codes;
     uint64_t g;
-    int32_t intercept; /* must be 1: slope terms are out of scope */
+    int32_t intercept;            /* 1: level intercepts; 0: slopes only (nslopes >= 1) */
+    int32_t nslopes;              /* slope columns */
+    const double* const* slopes;  /* nslopes arrays of n doubles (host, or device for _dev) */
 } fh_term;
 
 /* SolverConfig (solve.rs:53-80): the knobs this solver honours. */
@@ -136,12 +142,18 @@ int fh_absorber_residualize_dev(fh_absorber* ab, double* z, size_t r, const fh_s
                                 const double* warm, double* coefs_out, fh_solve_info* info,
                                 const fh_ctx* ctx);
 
-/* FeCoefs rows p = m + g1 (solve.rs:7-9). */
+/* FeCoefs rows p = m + g1 (solve.rs:7-9); with varying slopes p = m + G_E b_E
+ * ([x (m x r); alpha_E], E's levels level-major: intercept, then slopes; slopes.rs:30-31). */
 size_t fh_absorber_coef_rows(const fh_absorber* ab);
+/* Value vectors term_values returns: one per term for intercept-only absorbers; with varying
+ * slopes one per slot (each term's intercept, then each slope; slopes.rs:2435-2464). */
+size_t fh_absorber_term_slots(const fh_absorber* ab);
 /* Absorber::fit_rows (absorber.rs:92; solve.rs:1313-1317): FE fit of column col per row. */
 int fh_absorber_fit_rows(fh_absorber* ab, const double* coefs, size_t r, size_t col, double* out_rows);
 /* Absorber::term_values (absorber.rs:94-96; solve.rs:1319-1343): out holds sum_q g_q values in
- * term order; lens (nterms) receives g_q. */
+ * term order; lens (nterms) receives g_q. With varying slopes out holds one vector per slot
+ * (fh_absorber_term_slots entries in lens): every intercept-only term but FE1 has weighted mean 0
+ * and FE1's intercept takes the shifts; slope values are reported as solved. */
 int fh_absorber_term_values(fh_absorber* ab, const double* coefs, size_t r, size_t col, double* out,
                             uint64_t* lens);
 

@@ -194,12 +194,14 @@ def ols_from_block(rows, w=None, tol=1e-9):
     return coef, inv, e, kept
 
 
-def crv3(fit_beta, codes, y, X, w, dims, dfk, k_adj=True, g_adj=True, g_df="min", offset=None):
+def crv3(fit_beta, codes, y, X, w, dims, dfk, k_adj=True, g_adj=True, g_df="min", offset=None, slopes=None):
     """The CRV3 variance: for every non-empty subset of the cluster dimensions (clustered by the
     intersection) and every cluster g of it, beta_(-g) = fit_beta(codes, y, X, w, offset) on the
     rows outside g (codes re-densified, as a fresh fit sees them); V = factor sum sign
     scale(G) sum_g (beta_(-g) - beta)(beta_(-g) - beta)' with factor G_min/(G_min - 1)
-    (n - 1)/(n - df_k) (crv3.rs:89-104, fit/vcov.rs:369-397), then fixest's PSD repair."""
+    (n - 1)/(n - df_k) (crv3.rs:89-104, fit/vcov.rs:369-397), then fixest's PSD repair.
+    slopes (per term, its slope columns): fit_beta then takes them, restricted to the same rows,
+    after the codes."""
     n = len(y)
 
     def sub(keep):
@@ -210,8 +212,10 @@ def crv3(fit_beta, codes, y, X, w, dims, dfk, k_adj=True, g_adj=True, g_df="min"
         return cs
 
     def fit(keep):
-        return fit_beta(sub(keep), y[keep], X[keep], None if w is None else w[keep],
-                        None if offset is None else offset[keep])
+        rest = (y[keep], X[keep], None if w is None else w[keep], None if offset is None else offset[keep])
+        if slopes is not None:
+            return fit_beta(sub(keep), [[np.asarray(x)[keep] for x in xs] for xs in slopes], *rest)
+        return fit_beta(sub(keep), *rest)
 
     beta = fit(np.ones(n, bool))
     k = len(beta)

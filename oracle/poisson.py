@@ -253,7 +253,10 @@ def ppml(codes, g, y, X, weights=None, offset=None, tol=1e-8, itol=1e-10, maxite
         if vcov in ("hc2", "hc3"):
             # the full model's leverage at the last IRLS weights (fit/glm.rs:588-599; a GLM keeps
             # the weight in the regressor part even without absorbed terms)
-            D = estimate.fe_design([np.asarray(c)[pi] for c in codes], g, n, intercept=not len(codes))
+            if slope_terms is not None:
+                D = _sl.slope_design(slope_terms, n)
+            else:
+                D = estimate.fe_design([np.asarray(c)[pi] for c in codes], g, n, intercept=not len(codes))
             h = estimate.hat_values(np.concatenate([xr[:, kept], D], axis=1), w)
         V = estimate.vcov(vcov, bread, Xt, e, w, n, dfk, cluster=cl, k_adj=k_adj, g_adj=g_adj, h=h)
     Vf = np.zeros((K, K))

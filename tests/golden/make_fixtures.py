@@ -11,7 +11,8 @@ Sources (fixtures of crates/fereg-core/tests, written there by fixest 0.14.2):
                                               tests/data/gen_se_fixtures.R; OLS and Poisson,
                                               intercept-only absorbed terms, no IV)
   tests/data/fit_fixest_fe.json (sl_*),       varying slopes (fe[x] terms, with and without the
-  fit_fixest_glm.json (pois_sl*)              level intercepts): OLS and Poisson on fit_d1.csv
+  fit_fixest_glm.json (pois_sl*),             level intercepts): OLS and Poisson on fit_d1.csv,
+  fit_se_fixest.json (*_sl)                   HC2 / HC3 / CRV3 / hetero on fit_se.csv
   tests/data/ols_data.csv, ols_s*.json        reghdfe 6.13.1 ground truth (tests/ols.rs:765-935,
                                               written by tests/data/gen_ols_fixtures.do); the
                                               model of each fixture restates the table of
@@ -107,6 +108,19 @@ def main():
             e = {k: c["expect"][k] for k in KEEP + ["deviance", "iterations", "fitted"] if k in c["expect"]}
             sl.append({k: c[k] for k in ["name", "data", "y", "x", "fe", "weights", "vcov", "cluster", "ssc"]}
                       | {"model": model, "offset": c.get("offset"), "expect": e})
+    nse = sum(1 for _ in open(os.path.join(SRC, "fit_se.csv"))) - 1
+    for c in json.load(open(os.path.join(SRC, "fit_se_fixest.json"))):
+        if not any(t["slopes"] or not t["intercept"] for t in c["fe"]) or c.get("endog"):
+            continue
+        if c["vcov"] not in ("hc2", "hc3", "crv3", "hetero") or c["model"] not in ("ols", "poisson"):
+            continue
+        e = {k: c["expect"][k] for k in ["coef", "se", "vcov", "df_k", "nobs", "collin", "G", "coef_names"]
+             if k in c["expect"]}
+        if e["nobs"] != nse:
+            continue  # every case here keeps all rows; the fixture carries no row list
+        e["obs"] = list(range(1, nse + 1))
+        sl.append({k: c[k] for k in ["name", "data", "y", "x", "fe", "weights", "vcov", "cluster", "ssc"]}
+                  | {"model": c["model"], "offset": None, "expect": e})
     json.dump(sl, open(os.path.join(DST, "fixest_slopes.json"), "w"), indent=0)
     print(len(out), "ols cases;", len(sl), "slope cases;", len(keep), "poisson cases;", len(rg), "reghdfe cases")
 
