@@ -43,6 +43,9 @@ namespace {
 constexpr int TPB = 256;
 constexpr int WPB = TPB / WAVE;
 constexpr int RC = 4;                   // columns per sweep of the row kernels
+// A block with a level longer than this many rows runs its level passes over items of at most CH
+// rows (fixed boundaries) whose partial sums are added per level in item order.
+constexpr uint32_t CH = 1024;
 constexpr int SL_SMAX = NB_MAX * SL_BMAX;  // slope columns of the non-E blocks
 constexpr int DOTG = SPMV_DOTG;
 constexpr double LEVEL_TOL = 1e-12;     // slopes.rs:58: per-level inverses of E
@@ -63,6 +66,13 @@ struct SlBlock {
     bool small;
     uint64_t joff;  // first entry of its block-Jacobi inverses (jacobi blocks)
     DBuf<uint32_t> perm, poff;  // rows by (level, row); offsets per level (g + 1)
+    // long levels: items of <= CH rows of the level lists (first / end position, level) and the
+    // items of each level (g + 1)
+    uint64_t poff_n = 0;  // rows over its levels (n)
+    DBuf<double> svp;     // its slope values in level-list order (n x ns)
+    bool chunked = false;
+    uint64_t nitems = 0;
+    DBuf<uint32_t> it_first, it_end, it_lvl, lvl_it;
 };
 
 struct SlopeState {
@@ -82,35 +92,48 @@ struct SlopeState {
     DBuf<double> gdiag;  // m: diagonal of sum w h h' per coordinate
     DBuf<uint8_t> red;   // m: redundant coordinates
     DBuf<double> zbuf;   // n x r scratch of the row kernels
+    DBuf<double> ipart;  // item partials of the chunked level passes
+    // E's segments in items of <= CH rows when one is longer (first / end row, segment; items of
+    // each segment, ge + 1), their partials and t per segment (ge x be x r)
+    bool e_chunked = false;
+    uint64_t e_nitems = 0;
+    DBuf<uint32_t> e_first, e_end, e_lvl, seg_it;
+    DBuf<double> epart, tbuf, segpart;
 };
 
 namespace {
 
 // ------------------------------------------------------------------ device helpers
 
-// Generalized inverse of the symmetric k x k row-major a in place by the symmetric sweep
+// Generalized inverse of the symmetric K x K row-major a in place by the symmetric sweep
 // (Goodnight); pivot p is dropped (row and column p zero) unless its Schur complement given the
-// earlier kept pivots exceeds tol * refd[p] (slopes.rs:99-132).
-__device__ void sweep_ginv(double* a, int k, const double* refd, double tol) {
-    for (int p = 0; p < k; p++) {
-        const double d = a[p * k + p];
+// earlier kept pivots exceeds tol * refd[p] (slopes.rs:99-132). K is a compile-time constant so
+// that a stays in registers.
+template <int K>
+__device__ void sweep_ginv(double (&a)[K * K], const double (&refd)[K], double tol) {
+#pragma unroll
+    for (int p = 0; p < K; p++) {
+        const double d = a[p * K + p];
         if (!(d > tol * refd[p] && d > 0.0 && isfinite(d))) {
-            for (int j = 0; j < k; j++) {
-                a[p * k + j] = 0.0;
-                a[j * k + p] = 0.0;
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                a[p * K + j] = 0.0;
+                a[j * K + p] = 0.0;
             }
             continue;
         }
         const double inv = 1.0 / d;
-        for (int j = 0; j < k; j++) a[p * k + j] *= inv;
-        for (int i = 0; i < k; i++) {
+#pragma unroll
+        for (int j = 0; j < K; j++) a[p * K + j] *= inv;
+#pragma unroll
+        for (int i = 0; i < K; i++) {
             if (i == p) continue;
-            const double c = a[i * k + p];
-            if (c == 0.0) continue;
-            for (int j = 0; j < k; j++) a[i * k + j] -= c * a[p * k + j];
-            a[i * k + p] = -c * inv;
+            const double c = a[i * K + p];
+#pragma unroll
+            for (int j = 0; j < K; j++) a[i * K + j] -= c * a[p * K + j];
+            a[i * K + p] = -c * inv;
         }
-        a[p * k + p] = inv;
+        a[p * K + p] = inv;
     }
 }
 
@@ -146,25 +169,181 @@ __device__ __forceinline__ double slot_val(const SegArgs& A, int b, int d, size_
 
 enum { M_APPLY = 0, M_RHS = 1, M_RESID = 2 };
 
-// One wave per segment of E.
+// Groups of L lanes per segment of E (BE = E's width, a compile-time constant so that the
+// accumulators stay in registers; a group's lanes stride the segment's rows and its sums are
+// reduced by xor shuffles within the group, in fixed order).
 //   M_APPLY: src = P (m x r); u = D_R P; dst = Z = w (u - h_E' t) (n x r).
 //   M_RHS:   src = V (n x ld); u = v; dst = Z; part: per-wave [wsq | wsum | ss1] (3 r).
 //   M_RESID: src = dst = V (n x ld); d = v - D_R x; v = d - h_E' t; alpha (ge be x r) = t.
-template <int MODE>
+template <int MODE, int L, int BE>
 __global__ __launch_bounds__(TPB) void k_sl_seg(SegArgs A, const double* src, int ld, const double* __restrict__ x,
                                                 int r, double* dst, double* __restrict__ alpha,
                                                 double* __restrict__ part, const uint32_t* done) {
     if (done && *done) return;
     __shared__ double wsums[WPB][3][16];
-    const int lane = lane_id(), wv = threadIdx.x / WAVE;
+    const int lane = lane_id(), wv = threadIdx.x / WAVE, gl = lane % L;
     if (MODE == M_RHS) {
         for (int e = lane; e < 48; e += WAVE) wsums[wv][e / 16][e % 16] = 0.0;
         __syncthreads();
     }
-    const int be = A.be;
-    for (uint64_t s = (uint64_t)blockIdx.x * WPB + wv; s < A.ge; s += (uint64_t)gridDim.x * WPB) {
-        const size_t i0 = A.seg_off[s], i1 = A.seg_off[s + 1];
-        const double* mv = A.minv + s * be * be;
+    constexpr int GPW = WAVE / L;  // groups per wave
+    const uint64_t ngroups = (uint64_t)gridDim.x * WPB * GPW;
+    const uint64_t g0 = ((uint64_t)blockIdx.x * WPB + wv) * GPW + lane / L;
+    // every lane of a wave runs the same number of segment rounds (the shuffles of the RHS sums
+    // and the group sums need their whole group)
+    const uint64_t rounds = (A.ge + ngroups - 1) / ngroups;
+    for (uint64_t rd = 0; rd < rounds; rd++) {
+        const uint64_t s = g0 + rd * ngroups;
+        const bool live = s < A.ge;
+        const size_t i0 = live ? A.seg_off[s] : 0, i1 = live ? A.seg_off[s + 1] : 0;
+        const double* mv = A.minv + (live ? s : 0) * BE * BE;
+        for (int c0 = 0; c0 < r; c0 += RC) {
+            const int nc = min(RC, r - c0);
+            double acc[BE][RC], wsq[RC], wsm[RC];
+#pragma unroll
+            for (int c = 0; c < RC; c++) {
+                wsq[c] = 0.0;
+                wsm[c] = 0.0;
+#pragma unroll
+                for (int e = 0; e < BE; e++) acc[e][c] = 0.0;
+            }
+            for (size_t i = i0 + gl; i < i1; i += L) {
+                const double wi = A.w ? A.w[i] : 1.0;
+                double u[RC];
+#pragma unroll
+                for (int c = 0; c < RC; c++) u[c] = 0.0;
+                if (MODE == M_APPLY || MODE == M_RESID) {
+                    const double* P = MODE == M_APPLY ? src : x;
+                    for (int b = 0; b < A.nb; b++) {
+                        const size_t k0 = A.coord[i * A.nb + b];
+                        for (int d = 0; d < A.bw[b]; d++) {
+                            const double h = slot_val(A, b, d, i);
+                            const double* pr = P + (k0 + d) * r + c0;
+#pragma unroll
+                            for (int c = 0; c < RC; c++)
+                                if (c < nc) u[c] += h * pr[c];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < RC; c++) {
+                    if (c >= nc) continue;
+                    if (MODE == M_RESID) {
+                        u[c] = src[i * ld + c0 + c] - u[c];
+                        dst[i * ld + c0 + c] = u[c];  // d, read back by this lane below
+                    } else if (MODE == M_RHS) {
+                        u[c] = src[i * ld + c0 + c];
+                        wsq[c] += wi * u[c] * u[c];
+                        wsm[c] += wi * u[c];
+                    } else {
+                        dst[i * r + c0 + c] = u[c];  // u, read back by this lane below
+                    }
+                }
+                double h[BE];
+#pragma unroll
+                for (int e = 0; e < BE; e++)
+                    h[e] = (A.e_icpt && e == 0) ? 1.0 : A.he[i * A.e_ns + e - A.e_icpt];
+#pragma unroll
+                for (int e = 0; e < BE; e++)
+#pragma unroll
+                    for (int c = 0; c < RC; c++) acc[e][c] += wi * h[e] * u[c];
+            }
+            // t = M_s^+ acc (every lane of the group holds the group's sums)
+#pragma unroll
+            for (int e = 0; e < BE; e++)
+#pragma unroll
+                for (int c = 0; c < RC; c++) acc[e][c] = group_sum<L>(acc[e][c]);
+            double t[BE][RC];
+#pragma unroll
+            for (int e = 0; e < BE; e++) {
+                double me[BE];
+#pragma unroll
+                for (int f = 0; f < BE; f++) me[f] = live ? mv[e * BE + f] : 0.0;
+#pragma unroll
+                for (int c = 0; c < RC; c++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int f = 0; f < BE; f++) v += me[f] * acc[f][c];
+                    t[e][c] = v;
+                }
+            }
+            if (MODE == M_RHS) {
+#pragma unroll
+                for (int c = 0; c < RC; c++) {
+                    const double q = group_sum<L>(wsq[c]), sm = group_sum<L>(wsm[c]);
+                    double at = 0.0;
+#pragma unroll
+                    for (int e = 0; e < BE; e++) at += acc[e][c] * t[e][c];
+                    // the wave's groups of this round, added by the wave reduction (fixed order)
+                    const bool own = live && gl == 0;
+                    const double wq = wave_sum(own ? q : 0.0), wsm2 = wave_sum(own ? sm : 0.0),
+                                 ws1 = wave_sum(own ? q - at : 0.0);
+                    if (lane == 0 && c < nc) {
+                        wsums[wv][0][c0 + c] += wq;
+                        wsums[wv][1][c0 + c] += wsm2;
+                        wsums[wv][2][c0 + c] += ws1;
+                    }
+                }
+            }
+            if (MODE == M_RESID && alpha && live && gl == 0) {
+#pragma unroll
+                for (int e = 0; e < BE; e++)
+#pragma unroll
+                    for (int c = 0; c < RC; c++)
+                        if (c < nc) alpha[(s * BE + e) * r + c0 + c] = t[e][c];
+            }
+            for (size_t i = i0 + gl; i < i1; i += L) {
+                const double wi = A.w ? A.w[i] : 1.0;
+                double h[BE];
+#pragma unroll
+                for (int e = 0; e < BE; e++)
+                    h[e] = (A.e_icpt && e == 0) ? 1.0 : A.he[i * A.e_ns + e - A.e_icpt];
+#pragma unroll
+                for (int c = 0; c < RC; c++) {
+                    if (c >= nc) continue;
+                    double ht = 0.0;
+#pragma unroll
+                    for (int e = 0; e < BE; e++) ht += h[e] * t[e][c];
+                    if (MODE == M_APPLY) {
+                        const size_t o = i * r + c0 + c;
+                        dst[o] = wi * (dst[o] - ht);
+                    } else if (MODE == M_RHS) {
+                        dst[i * r + c0 + c] = wi * (src[i * ld + c0 + c] - ht);
+                    } else {
+                        const size_t o = i * ld + c0 + c;
+                        dst[o] = dst[o] - ht;
+                    }
+                }
+            }
+        }
+    }
+    if (MODE == M_RHS) {
+        __syncthreads();
+        const size_t gw = (size_t)blockIdx.x * WPB + wv;
+        for (int e = lane; e < 3 * r; e += WAVE) part[gw * 3 * r + e] = wsums[wv][e / r][e % r];
+    }
+}
+
+struct ItemArgs {
+    const uint32_t* first;
+    const uint32_t* end;
+    const uint32_t* lvl;
+    const uint32_t* lvl_it;
+    uint64_t nitems;
+};
+
+// E's long segments, phase A: one wave per item of a segment. u as in k_sl_seg (stashed the same
+// way); part[it][e][c] = sum over the item's rows of w h_E,e u_c, and for M_RHS also sum w v^2 and
+// sum w v (rows be and be + 1 of the item's (be + 2) x r partials).
+template <int MODE>
+__global__ __launch_bounds__(TPB) void k_sl_segA(SegArgs A, ItemArgs I, const double* src, int ld,
+                                                 const double* __restrict__ x, int r, double* dst,
+                                                 double* __restrict__ part, const uint32_t* done) {
+    if (done && *done) return;
+    const int lane = lane_id(), be = A.be, pw = be + 2;
+    for (uint64_t it = ((uint64_t)blockIdx.x * TPB + threadIdx.x) / WAVE; it < I.nitems;
+         it += (uint64_t)gridDim.x * WPB) {
+        const size_t i0 = I.first[it], i1 = I.end[it];
         for (int c0 = 0; c0 < r; c0 += RC) {
             const int nc = min(RC, r - c0);
             double acc[SL_BMAX][RC], wsq[RC], wsm[RC];
@@ -192,25 +371,19 @@ __global__ __launch_bounds__(TPB) void k_sl_seg(SegArgs A, const double* src, in
                         }
                     }
                 }
-                if (MODE == M_RESID) {
 #pragma unroll
-                    for (int c = 0; c < RC; c++)
-                        if (c < nc) {
-                            u[c] = src[i * ld + c0 + c] - u[c];
-                            dst[i * ld + c0 + c] = u[c];  // d, read back by this lane below
-                        }
-                } else if (MODE == M_RHS) {
-#pragma unroll
-                    for (int c = 0; c < RC; c++)
-                        if (c < nc) {
-                            u[c] = src[i * ld + c0 + c];
-                            wsq[c] += wi * u[c] * u[c];
-                            wsm[c] += wi * u[c];
-                        }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < RC; c++)
-                        if (c < nc) dst[i * r + c0 + c] = u[c];  // u, read back by this lane below
+                for (int c = 0; c < RC; c++) {
+                    if (c >= nc) continue;
+                    if (MODE == M_RESID) {
+                        u[c] = src[i * ld + c0 + c] - u[c];
+                        dst[i * ld + c0 + c] = u[c];
+                    } else if (MODE == M_RHS) {
+                        u[c] = src[i * ld + c0 + c];
+                        wsq[c] += wi * u[c] * u[c];
+                        wsm[c] += wi * u[c];
+                    } else {
+                        dst[i * r + c0 + c] = u[c];
+                    }
                 }
                 double h[SL_BMAX];
                 e_row(A.he, A.e_ns, A.e_icpt, i, h);
@@ -220,73 +393,82 @@ __global__ __launch_bounds__(TPB) void k_sl_seg(SegArgs A, const double* src, in
 #pragma unroll
                         for (int c = 0; c < RC; c++) acc[e][c] += wi * h[e] * u[c];
             }
-            // t = M_s^+ acc (every lane holds the wave's sums)
-            double t[SL_BMAX][RC];
 #pragma unroll
-            for (int e = 0; e < SL_BMAX; e++)
-#pragma unroll
-                for (int c = 0; c < RC; c++) acc[e][c] = e < be ? wave_sum(acc[e][c]) : 0.0;
-#pragma unroll
-            for (int e = 0; e < SL_BMAX; e++)
+            for (int e = 0; e < SL_BMAX + 2; e++)
 #pragma unroll
                 for (int c = 0; c < RC; c++) {
+                    if (e >= be && !(MODE == M_RHS && e < be + 2)) continue;
                     double v = 0.0;
-                    if (e < be)
-                        for (int f = 0; f < be; f++) v += mv[e * be + f] * acc[f][c];
-                    t[e][c] = v;
-                }
-            if (MODE == M_RHS) {
+                    if (e < be) {
 #pragma unroll
-                for (int c = 0; c < RC; c++) {
-                    const double q = wave_sum(wsq[c]), sm = wave_sum(wsm[c]);
-                    double at = 0.0;
-#pragma unroll
-                    for (int e = 0; e < SL_BMAX; e++) at += acc[e][c] * t[e][c];
-                    if (lane == 0 && c < nc) {
-                        wsums[wv][0][c0 + c] += q;
-                        wsums[wv][1][c0 + c] += sm;
-                        wsums[wv][2][c0 + c] += q - at;
-                    }
-                }
-            }
-            if (MODE == M_RESID && alpha && lane < be * RC) {
-                const int e = lane / RC, c = lane % RC;
-                double v = 0.0;
-#pragma unroll
-                for (int ee = 0; ee < SL_BMAX; ee++)
-#pragma unroll
-                    for (int cc = 0; cc < RC; cc++)
-                        if (ee == e && cc == c) v = t[ee][cc];
-                if (c < nc) alpha[(s * be + e) * r + c0 + c] = v;
-            }
-            for (size_t i = i0 + lane; i < i1; i += WAVE) {
-                const double wi = A.w ? A.w[i] : 1.0;
-                double h[SL_BMAX];
-                e_row(A.he, A.e_ns, A.e_icpt, i, h);
-#pragma unroll
-                for (int c = 0; c < RC; c++) {
-                    if (c >= nc) continue;
-                    double ht = 0.0;
-#pragma unroll
-                    for (int e = 0; e < SL_BMAX; e++)
-                        if (e < be) ht += h[e] * t[e][c];
-                    if (MODE == M_APPLY) {
-                        const size_t o = i * r + c0 + c;
-                        dst[o] = wi * (dst[o] - ht);
-                    } else if (MODE == M_RHS) {
-                        dst[i * r + c0 + c] = wi * (src[i * ld + c0 + c] - ht);
+                        for (int ee = 0; ee < SL_BMAX; ee++)
+                            if (ee == e) v = acc[ee][c];
                     } else {
-                        const size_t o = i * ld + c0 + c;
-                        dst[o] = dst[o] - ht;
+                        v = e == be ? wsq[c] : wsm[c];
                     }
+                    v = wave_sum(v);
+                    if (lane == 0 && c < nc) part[(it * pw + e) * r + c0 + c] = v;
                 }
-            }
         }
     }
-    if (MODE == M_RHS) {
-        __syncthreads();
-        const size_t gw = (size_t)blockIdx.x * WPB + wv;
-        for (int e = lane; e < 3 * r; e += WAVE) part[gw * 3 * r + e] = wsums[wv][e / r][e % r];
+}
+
+// Phase B: per (segment, column) the items' partials in order, t = M_s^+ acc into T (and alpha),
+// and for M_RHS the segment's [wsq | wsum | ss1] (3 r per segment).
+template <int MODE>
+__global__ void k_sl_segB(SegArgs A, ItemArgs I, const double* __restrict__ part, int r, double* __restrict__ T,
+                          double* __restrict__ alpha, double* __restrict__ segpart, const uint32_t* done) {
+    if (done && *done) return;
+    const int be = A.be, pw = be + 2;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < A.ge * (uint64_t)r;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = e / r;
+        const int c = (int)(e % r);
+        double acc[SL_BMAX], q = 0.0, sm = 0.0;
+        for (int f = 0; f < SL_BMAX; f++) acc[f] = 0.0;
+        for (uint32_t it = I.lvl_it[s]; it < I.lvl_it[s + 1]; it++) {
+            for (int f = 0; f < be; f++) acc[f] += part[((uint64_t)it * pw + f) * r + c];
+            if (MODE == M_RHS) {
+                q += part[((uint64_t)it * pw + be) * r + c];
+                sm += part[((uint64_t)it * pw + be + 1) * r + c];
+            }
+        }
+        const double* mv = A.minv + s * be * be;
+        double at = 0.0;
+        for (int f = 0; f < be; f++) {
+            double t = 0.0;
+            for (int g2 = 0; g2 < be; g2++) t += mv[f * be + g2] * acc[g2];
+            T[(s * be + f) * r + c] = t;
+            if (MODE == M_RESID && alpha) alpha[(s * be + f) * r + c] = t;
+            at += acc[f] * t;
+        }
+        if (MODE == M_RHS) {
+            segpart[s * 3 * r + c] = q;
+            segpart[s * 3 * r + r + c] = sm;
+            segpart[s * 3 * r + 2 * r + c] = q - at;
+        }
+    }
+}
+
+// Phase C, one thread per row: z = w (u - h_E' t_s) (M_APPLY, M_RHS into dst), v = d - h_E' t_s
+// (M_RESID in place).
+template <int MODE>
+__global__ void k_sl_segC(SegArgs A, const uint32_t* __restrict__ row_seg, size_t n, const double* src, int ld,
+                          int r, const double* __restrict__ T, double* dst, const uint32_t* done) {
+    if (done && *done) return;
+    const int be = A.be;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t s = row_seg[i];
+        const double wi = A.w ? A.w[i] : 1.0;
+        double h[SL_BMAX];
+        e_row(A.he, A.e_ns, A.e_icpt, i, h);
+        for (int c = 0; c < r; c++) {
+            double ht = 0.0;
+            for (int f = 0; f < be; f++) ht += h[f] * T[(s * be + f) * r + c];
+            if (MODE == M_APPLY) dst[i * r + c] = wi * (dst[i * r + c] - ht);
+            else if (MODE == M_RHS) dst[i * r + c] = wi * (src[i * ld + c] - ht);
+            else dst[i * ld + c] = dst[i * ld + c] - ht;
+        }
     }
 }
 
@@ -295,18 +477,60 @@ struct PullArgs {
     const uint32_t* poff;
     uint64_t g;
     uint64_t off;
-    int b, icpt, s0, nsv;
-    const double* sv;
+    int b, icpt, ns;
+    const double* svp;  // the block's slope values in level-list order (n x ns)
 };
 
-// Y[off + l b + d][c] = sum over the rows i of level l of h_d(i) Z[i][c], one wave per level.
+// Y[off + l b + d][c] = sum over the rows i of level l of h_d(i) Z[i][c]: groups of L lanes per
+// level, B = the block's width (compile time), group sums by xor shuffles (fixed order).
+template <int L, int B>
 __global__ __launch_bounds__(TPB) void k_sl_pull(PullArgs P, const double* __restrict__ Z, int r,
                                                  double* __restrict__ Y, const uint32_t* done) {
     if (done && *done) return;
-    const int lane = lane_id();
-    for (uint64_t l = ((uint64_t)blockIdx.x * TPB + threadIdx.x) / WAVE; l < P.g;
-         l += (uint64_t)gridDim.x * WPB) {
+    constexpr int GPW = WAVE / L;
+    const int lane = lane_id(), gl = lane % L;
+    const uint64_t ngroups = (uint64_t)gridDim.x * WPB * GPW;
+    const uint64_t g0 = ((uint64_t)blockIdx.x * WPB + threadIdx.x / WAVE) * GPW + lane / L;
+    for (uint64_t l = g0; l < P.g; l += ngroups) {
         const size_t k0 = P.poff[l], k1 = P.poff[l + 1];
+        for (int c0 = 0; c0 < r; c0 += RC) {
+            const int nc = min(RC, r - c0);
+            double acc[B][RC];
+#pragma unroll
+            for (int d = 0; d < B; d++)
+#pragma unroll
+                for (int c = 0; c < RC; c++) acc[d][c] = 0.0;
+            for (size_t k = k0 + gl; k < k1; k += L) {
+                const size_t i = P.perm[k];
+                double z[RC];
+#pragma unroll
+                for (int c = 0; c < RC; c++) z[c] = c < nc ? Z[i * r + c0 + c] : 0.0;
+#pragma unroll
+                for (int d = 0; d < B; d++) {
+                    const double h = P.icpt ? (d == 0 ? 1.0 : P.svp[k * P.ns + d - 1]) : P.svp[k * P.ns + d];
+#pragma unroll
+                    for (int c = 0; c < RC; c++) acc[d][c] += h * z[c];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < B; d++)
+#pragma unroll
+                for (int c = 0; c < RC; c++) {
+                    const double v = group_sum<L>(acc[d][c]);
+                    if (gl == 0 && c < nc) Y[(P.off + l * B + d) * r + c0 + c] = v;
+                }
+        }
+    }
+}
+
+// The pull over items: part[it][d][c] = sum over the item's rows of h_d(i) Z[i][c] (b x r per item).
+__global__ __launch_bounds__(TPB) void k_sl_pull_items(PullArgs P, ItemArgs I, const double* __restrict__ Z, int r,
+                                                       double* __restrict__ part, const uint32_t* done) {
+    if (done && *done) return;
+    const int lane = lane_id();
+    for (uint64_t it = ((uint64_t)blockIdx.x * TPB + threadIdx.x) / WAVE; it < I.nitems;
+         it += (uint64_t)gridDim.x * WPB) {
+        const size_t k0 = I.first[it], k1 = I.end[it];
         for (int c0 = 0; c0 < r; c0 += RC) {
             const int nc = min(RC, r - c0);
             double acc[SL_BMAX][RC];
@@ -322,7 +546,7 @@ __global__ __launch_bounds__(TPB) void k_sl_pull(PullArgs P, const double* __res
 #pragma unroll
                 for (int d = 0; d < SL_BMAX; d++) {
                     if (d >= P.b) break;
-                    const double h = P.icpt ? (d == 0 ? 1.0 : P.sv[i * P.nsv + P.s0 + d - 1]) : P.sv[i * P.nsv + P.s0 + d];
+                    const double h = P.icpt ? (d == 0 ? 1.0 : P.svp[k * P.ns + d - 1]) : P.svp[k * P.ns + d];
 #pragma unroll
                     for (int c = 0; c < RC; c++) acc[d][c] += h * z[c];
                 }
@@ -333,8 +557,33 @@ __global__ __launch_bounds__(TPB) void k_sl_pull(PullArgs P, const double* __res
                 for (int c = 0; c < RC; c++) {
                     if (d >= P.b) continue;
                     const double v = wave_sum(acc[d][c]);
-                    if (lane == 0 && c < nc) Y[(P.off + l * P.b + d) * r + c0 + c] = v;
+                    if (lane == 0 && c < nc) part[(it * P.b + d) * r + c0 + c] = v;
                 }
+        }
+    }
+}
+
+// out[l * len + e] = sum of part[it * len + e] over the items of level l: one workgroup per level,
+// its threads over the items (it = first + t, t + TPB, ...), then the waves in order (fixed).
+__global__ __launch_bounds__(TPB) void k_sl_item_sum(const uint32_t* __restrict__ lvl_it, uint64_t g,
+                                                     const double* __restrict__ part, int len,
+                                                     double* __restrict__ out, const uint32_t* done) {
+    if (done && *done) return;
+    __shared__ double red[WPB];
+    for (uint64_t l = blockIdx.x; l < g; l += gridDim.x) {
+        const uint32_t a0 = lvl_it[l], a1 = lvl_it[l + 1];
+        for (int q = 0; q < len; q++) {
+            double s = 0.0;
+            for (uint32_t it = a0 + threadIdx.x; it < a1; it += TPB) s += part[(uint64_t)it * len + q];
+            s = wave_sum(s);
+            if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double t = 0.0;
+                for (int w = 0; w < WPB; w++) t += red[w];
+                out[l * len + q] = t;
+            }
+            __syncthreads();
         }
     }
 }
@@ -474,68 +723,91 @@ __global__ void k_sl_rows(RowSrc S, const uint32_t* __restrict__ pi, size_t n, u
     }
 }
 
+// svp[k * ns + j] = sv[perm[k] * nsv + s0 + j]: a block's slope values in its level-list order.
+__global__ void k_sl_svp(const uint32_t* __restrict__ perm, size_t n, const double* __restrict__ sv, int nsv, int s0,
+                         int ns, double* __restrict__ svp) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * ns; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = e / ns;
+        svp[e] = sv[(size_t)perm[k] * nsv + s0 + e % ns];
+    }
+}
+
 // ------------------------------------------------------------------ weight kernels
 
 // M_s^+ of every level of E (slopes.rs:551-622): with an intercept, the centred slope block is
 // swept (pivots <= 1e-12 of sum w x^2 dropped, so a level without within variation gets slope 0
 // and the mean as intercept) and bordered; without, M_s itself is swept. Levels without weight
-// get 0 (their rows contribute nothing).
-__global__ void k_sl_minv(const uint32_t* __restrict__ seg_off, uint64_t ge, const double* __restrict__ he, int e_ns,
+// get 0 (their rows contribute nothing). NS = E's slopes.
+template <int NS>
+__global__ void k_sl_minv(const uint32_t* __restrict__ seg_off, uint64_t ge, const double* __restrict__ he,
                           int e_icpt, const double* __restrict__ w, double* __restrict__ minv) {
-    const int be = e_icpt + e_ns;
+    constexpr int G2 = NS > 0 ? NS : 1;
+    const int be = e_icpt + NS;
     for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ge; s += (uint64_t)gridDim.x * blockDim.x) {
         const size_t i0 = seg_off[s], i1 = seg_off[s + 1];
         double* out = minv + s * be * be;
-        double G[SL_BMAX * SL_BMAX], xbar[SL_BMAX], refd[SL_BMAX];
-        for (int k = 0; k < SL_BMAX * SL_BMAX; k++) G[k] = 0.0;
-        for (int k = 0; k < SL_BMAX; k++) { xbar[k] = 0.0; refd[k] = 0.0; }
+        double G[G2 * G2], xbar[G2], refd[G2];
+#pragma unroll
+        for (int k = 0; k < G2 * G2; k++) G[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < G2; k++) {
+            xbar[k] = 0.0;
+            refd[k] = 0.0;
+        }
         for (int k = 0; k < be * be; k++) out[k] = 0.0;
-        const int ns = e_ns;
         if (e_icpt) {
             double sw = 0.0;
             for (size_t i = i0; i < i1; i++) {
                 const double wi = w ? w[i] : 1.0;
                 sw += wi;
-                for (int k = 0; k < ns; k++) {
-                    const double x = he[i * ns + k];
+#pragma unroll
+                for (int k = 0; k < NS; k++) {
+                    const double x = he[i * NS + k];
                     xbar[k] += wi * x;
                     refd[k] += wi * x * x;
                 }
             }
             if (!(sw > 0.0)) continue;
-            for (int k = 0; k < ns; k++) xbar[k] /= sw;
+#pragma unroll
+            for (int k = 0; k < NS; k++) xbar[k] /= sw;
             for (size_t i = i0; i < i1; i++) {
                 const double wi = w ? w[i] : 1.0;
-                for (int k = 0; k < ns; k++) {
-                    const double dk = wi * (he[i * ns + k] - xbar[k]);
-                    for (int l = 0; l <= k; l++) G[k * ns + l] += dk * (he[i * ns + l] - xbar[l]);
+#pragma unroll
+                for (int k = 0; k < NS; k++) {
+                    const double dk = wi * (he[i * NS + k] - xbar[k]);
+#pragma unroll
+                    for (int l = 0; l < NS; l++) G[k * G2 + l] += dk * (he[i * NS + l] - xbar[l]);
                 }
             }
-            for (int k = 0; k < ns; k++)
-                for (int l = 0; l < k; l++) G[l * ns + k] = G[k * ns + l];
-            sweep_ginv(G, ns, refd, LEVEL_TOL);
+            if (NS > 0) sweep_ginv<G2>(G, refd, LEVEL_TOL);
             double q = 0.0;
-            for (int k = 0; k < ns; k++) {
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
                 double gx = 0.0;
-                for (int l = 0; l < ns; l++) gx += G[k * ns + l] * xbar[l];
+#pragma unroll
+                for (int l = 0; l < NS; l++) gx += G[k * G2 + l] * xbar[l];
                 q += xbar[k] * gx;
                 out[1 + k] = -gx;
                 out[(1 + k) * be] = -gx;
-                for (int l = 0; l < ns; l++) out[(1 + k) * be + 1 + l] = G[k * ns + l];
+#pragma unroll
+                for (int l = 0; l < NS; l++) out[(1 + k) * be + 1 + l] = G[k * G2 + l];
             }
             out[0] = 1.0 / sw + q;
         } else {
             for (size_t i = i0; i < i1; i++) {
                 const double wi = w ? w[i] : 1.0;
-                for (int k = 0; k < ns; k++)
-                    for (int l = 0; l <= k; l++) G[k * ns + l] += wi * he[i * ns + k] * he[i * ns + l];
+#pragma unroll
+                for (int k = 0; k < NS; k++)
+#pragma unroll
+                    for (int l = 0; l < NS; l++) G[k * G2 + l] += wi * he[i * NS + k] * he[i * NS + l];
             }
-            for (int k = 0; k < ns; k++) {
-                refd[k] = G[k * ns + k];
-                for (int l = 0; l < k; l++) G[l * ns + k] = G[k * ns + l];
-            }
-            sweep_ginv(G, ns, refd, LEVEL_TOL);
-            for (int k = 0; k < be * be; k++) out[k] = G[k];
+#pragma unroll
+            for (int k = 0; k < NS; k++) refd[k] = G[k * G2 + k];
+            if (NS > 0) sweep_ginv<G2>(G, refd, LEVEL_TOL);
+#pragma unroll
+            for (int k = 0; k < NS; k++)
+#pragma unroll
+                for (int l = 0; l < NS; l++) out[k * be + l] = G[k * G2 + l];
         }
     }
 }
@@ -544,8 +816,8 @@ struct DiagArgs {
     const uint32_t* perm;
     const uint32_t* poff;
     uint64_t g, off;
-    int b, icpt, s0, nsv;
-    const double* sv;
+    int b, icpt, ns;
+    const double* svp;  // the block's slope values in level-list order
     const double* he;
     int e_ns, e_icpt, be;
     const uint32_t* row_seg;
@@ -555,62 +827,157 @@ struct DiagArgs {
     double* jinv;  // this block's inverses (g x b^2)
 };
 
-// Per level of one block (slopes.rs:624-686, 787-834): the Gram sum w h h' and the Schur diagonal
-// block D_l = Gram - sum_s C_s' M_s^+ C_s over the runs of equal E level in the level's row list
-// (C_s = sum w h_E h'); the redundancy flags, the Gram diagonal and (jacobi) the generalized
-// inverse of D_l with the redundant coordinates left out.
-__global__ void k_sl_diag(DiagArgs A, double* __restrict__ gdiag, uint8_t* __restrict__ red) {
-    const int b = A.b, be = A.be;
-    for (uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; l < A.g; l += (uint64_t)gridDim.x * blockDim.x) {
-        double gl[SL_BMAX * SL_BMAX], dl[SL_BMAX * SL_BMAX], C[SL_BMAX * SL_BMAX];
-        for (int k = 0; k < SL_BMAX * SL_BMAX; k++) { gl[k] = 0.0; dl[k] = 0.0; }
-        size_t k = A.poff[l];
-        const size_t k1 = A.poff[l + 1];
-        while (k < k1) {
-            const uint32_t s = A.row_seg[A.perm[k]];
-            for (int q = 0; q < SL_BMAX * SL_BMAX; q++) C[q] = 0.0;
-            for (; k < k1 && A.row_seg[A.perm[k]] == s; k++) {
-                const size_t i = A.perm[k];
-                const double wi = A.w ? A.w[i] : 1.0;
-                double hq[SL_BMAX], he[SL_BMAX];
-                for (int d = 0; d < b; d++)
-                    hq[d] = A.icpt ? (d == 0 ? 1.0 : A.sv[i * A.nsv + A.s0 + d - 1]) : A.sv[i * A.nsv + A.s0 + d];
-                e_row(A.he, A.e_ns, A.e_icpt, i, he);
-                for (int p = 0; p < b; p++)
-                    for (int q = 0; q < b; q++) gl[p * b + q] += wi * hq[p] * hq[q];
-                for (int e = 0; e < be; e++)
-                    for (int p = 0; p < b; p++) C[e * b + p] += wi * he[e] * hq[p];
+// Over the level-list positions [k, k1): gl += sum w h h' and cmc += sum_s C_s' M_s^+ C_s over the
+// runs of equal E level (C_s = sum w h_E h'), runs taken whole (the caller's range holds whole runs).
+// B, BE: the block's and E's widths.
+template <int B, int BE>
+__device__ void diag_accumulate(const DiagArgs& A, size_t k, size_t k1, double (&gl)[B * B], double (&dl)[B * B]) {
+    while (k < k1) {
+        const uint32_t s = A.row_seg[A.perm[k]];
+        double C[BE][B];
+#pragma unroll
+        for (int e = 0; e < BE; e++)
+#pragma unroll
+            for (int p = 0; p < B; p++) C[e][p] = 0.0;
+        for (; k < k1 && A.row_seg[A.perm[k]] == s; k++) {
+            const size_t i = A.perm[k];
+            const double wi = A.w ? A.w[i] : 1.0;
+            double hq[B], he[BE];
+#pragma unroll
+            for (int d = 0; d < B; d++)
+                hq[d] = A.icpt ? (d == 0 ? 1.0 : A.svp[k * A.ns + d - 1]) : A.svp[k * A.ns + d];
+#pragma unroll
+            for (int e = 0; e < BE; e++) he[e] = (A.e_icpt && e == 0) ? 1.0 : A.he[i * A.e_ns + e - A.e_icpt];
+#pragma unroll
+            for (int p = 0; p < B; p++)
+#pragma unroll
+                for (int q = 0; q < B; q++) gl[p * B + q] += wi * hq[p] * hq[q];
+#pragma unroll
+            for (int e = 0; e < BE; e++)
+#pragma unroll
+                for (int p = 0; p < B; p++) C[e][p] += wi * he[e] * hq[p];
+        }
+        const double* mv = A.minv + (size_t)s * BE * BE;
+        double M[BE][BE];
+#pragma unroll
+        for (int e = 0; e < BE; e++)
+#pragma unroll
+            for (int f = 0; f < BE; f++) M[e][f] = mv[e * BE + f];
+#pragma unroll
+        for (int p = 0; p < B; p++)
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int e = 0; e < BE; e++) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int f = 0; f < BE; f++) t += M[e][f] * C[f][q];
+                    acc += C[e][p] * t;
+                }
+                dl[p * B + q] += acc;
             }
-            const double* mv = A.minv + (size_t)s * be * be;
-            for (int p = 0; p < b; p++)
-                for (int q = 0; q < b; q++) {
-                    double acc = 0.0;
-                    for (int e = 0; e < be; e++) {
-                        double t = 0.0;
-                        for (int f = 0; f < be; f++) t += mv[e * be + f] * C[f * b + q];
-                        acc += C[e * b + p] * t;
-                    }
-                    dl[p * b + q] += acc;
-                }
+    }
+}
+
+// The diagonal block of level l from its Gram gl and its sum_s C_s' M_s^+ C_s (in dl): D_l =
+// gl - dl, the redundancy flags, the Gram diagonal and (jacobi) the generalized inverse of D_l
+// with the redundant coordinates left out.
+template <int B>
+__device__ void diag_finish(const DiagArgs& A, uint64_t l, const double (&gl)[B * B], double (&dl)[B * B],
+                            double* __restrict__ gdiag, uint8_t* __restrict__ red) {
+    bool rd[B];
+    double refd[B];
+#pragma unroll
+    for (int q = 0; q < B * B; q++) dl[q] = gl[q] - dl[q];
+#pragma unroll
+    for (int p = 0; p < B; p++) {
+        rd[p] = !(dl[p * B + p] > REDUNDANT_REL * gl[p * B + p]);
+        gdiag[A.off + l * B + p] = gl[p * B + p];
+        red[A.off + l * B + p] = rd[p] ? 1 : 0;
+        refd[p] = dl[p * B + p];
+    }
+    if (!A.jacobi) return;
+#pragma unroll
+    for (int p = 0; p < B; p++)
+        if (rd[p])
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                dl[p * B + q] = 0.0;
+                dl[q * B + p] = 0.0;
+            }
+    sweep_ginv<B>(dl, refd, JACOBI_TOL);
+#pragma unroll
+    for (int q = 0; q < B * B; q++) A.jinv[l * B * B + q] = dl[q];
+}
+
+// Per level of one block with short levels (slopes.rs:624-686, 787-834): one thread per level.
+template <int B, int BE>
+__global__ void k_sl_diag(DiagArgs A, double* __restrict__ gdiag, uint8_t* __restrict__ red) {
+    for (uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; l < A.g; l += (uint64_t)gridDim.x * blockDim.x) {
+        double gl[B * B], dl[B * B];
+#pragma unroll
+        for (int q = 0; q < B * B; q++) {
+            gl[q] = 0.0;
+            dl[q] = 0.0;
         }
-        bool rd[SL_BMAX];
-        double refd[SL_BMAX];
-        for (int q = 0; q < b * b; q++) dl[q] = gl[q] - dl[q];
-        for (int p = 0; p < b; p++) {
-            rd[p] = !(dl[p * b + p] > REDUNDANT_REL * gl[p * b + p]);
-            gdiag[A.off + l * b + p] = gl[p * b + p];
-            red[A.off + l * b + p] = rd[p] ? 1 : 0;
-            refd[p] = dl[p * b + p];
+        diag_accumulate<B, BE>(A, A.poff[l], A.poff[l + 1], gl, dl);
+        diag_finish<B>(A, l, gl, dl, gdiag, red);
+    }
+}
+
+// Long levels: one thread per item. An item owns the runs of equal E level that start inside its
+// nominal range: it skips the run continuing from the previous item and finishes its last run past
+// its nominal end. part[it] = [gl | cmc] (2 B^2).
+template <int B, int BE>
+__global__ void k_sl_diag_items(DiagArgs A, ItemArgs I, double* __restrict__ part) {
+    constexpr int BB = B * B;
+    for (uint64_t it = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; it < I.nitems;
+         it += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t l = I.lvl[it];
+        const size_t lo = A.poff[l], hi = A.poff[l + 1];
+        size_t k0 = I.first[it], k1 = I.end[it];
+        if (k0 > lo) {
+            const uint32_t sp = A.row_seg[A.perm[k0 - 1]];
+            while (k0 < hi && A.row_seg[A.perm[k0]] == sp) k0++;
         }
-        if (!A.jacobi) continue;
-        for (int p = 0; p < b; p++)
-            if (rd[p])
-                for (int q = 0; q < b; q++) {
-                    dl[p * b + q] = 0.0;
-                    dl[q * b + p] = 0.0;
-                }
-        sweep_ginv(dl, b, refd, JACOBI_TOL);
-        for (int q = 0; q < b * b; q++) A.jinv[l * b * b + q] = dl[q];
+        if (k1 < hi && k1 > lo) {
+            const uint32_t sl = A.row_seg[A.perm[k1 - 1]];
+            while (k1 < hi && A.row_seg[A.perm[k1]] == sl) k1++;
+        }
+        double gl[BB], dl[BB];
+#pragma unroll
+        for (int q = 0; q < BB; q++) {
+            gl[q] = 0.0;
+            dl[q] = 0.0;
+        }
+        if (k0 < k1) diag_accumulate<B, BE>(A, k0, k1, gl, dl);
+#pragma unroll
+        for (int q = 0; q < BB; q++) {
+            part[it * 2 * BB + q] = gl[q];
+            part[it * 2 * BB + BB + q] = dl[q];
+        }
+    }
+}
+
+template <int B>
+__global__ void k_sl_diag_finish(DiagArgs A, ItemArgs I, const double* __restrict__ part, double* __restrict__ gdiag,
+                                 uint8_t* __restrict__ red) {
+    constexpr int BB = B * B;
+    for (uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; l < A.g; l += (uint64_t)gridDim.x * blockDim.x) {
+        double gl[BB], dl[BB];
+#pragma unroll
+        for (int q = 0; q < BB; q++) {
+            gl[q] = 0.0;
+            dl[q] = 0.0;
+        }
+        for (uint32_t it = I.lvl_it[l]; it < I.lvl_it[l + 1]; it++)
+#pragma unroll
+            for (int q = 0; q < BB; q++) {
+                gl[q] += part[(uint64_t)it * 2 * BB + q];
+                dl[q] += part[(uint64_t)it * 2 * BB + BB + q];
+            }
+        diag_finish<B>(A, l, gl, dl, gdiag, red);
     }
 }
 
@@ -707,13 +1074,160 @@ PullArgs pull_args(const SlopeState& S, const SlBlock& B) {
     P.off = B.off;
     P.b = B.b;
     P.icpt = B.icpt;
-    P.s0 = B.s0;
-    P.nsv = S.nsv;
-    P.sv = S.sv.p;
+    P.ns = B.ns;
+    P.svp = B.svp.p;
     return P;
 }
 
-unsigned seg_grid(const SlopeState& S) { return grid_for(std::max<uint64_t>(S.ge, 1), WPB, 1u << 15); }
+// Calls f(integral_constant L, integral_constant W) for lanes L in {4, 16, 64} and width W in 1..4.
+template <class F>
+void with_lanes_width(int L, int w, F&& f) {
+    auto by_w = [&](auto lc) {
+        switch (w) {
+            case 1: f(lc, std::integral_constant<int, 1>{}); break;
+            case 2: f(lc, std::integral_constant<int, 2>{}); break;
+            case 3: f(lc, std::integral_constant<int, 3>{}); break;
+            default: f(lc, std::integral_constant<int, 4>{}); break;
+        }
+    };
+    if (L == 4) by_w(std::integral_constant<int, 4>{});
+    else if (L == 16) by_w(std::integral_constant<int, 16>{});
+    else by_w(std::integral_constant<int, 64>{});
+}
+
+ItemArgs item_args(const SlBlock& B) {
+    return ItemArgs{B.it_first.p, B.it_end.p, B.it_lvl.p, B.lvl_it.p, B.nitems};
+}
+
+// The pass over E's segments in one mode: one wave per segment, or, when a segment is longer than
+// CH rows, phases A-C over items. M_RHS leaves its per-column [wsq | wsum | ss1] partials in
+// *part_out (count *nparts_out, 3 r each) for a fixed-order sum.
+template <int MODE>
+void seg_pass(const Absorber& a, SlopeState& S, const double* src, int ld, const double* x, int r, double* dst,
+              double* alpha, const uint32_t* done, hipStream_t st, DBuf<double>* rhs_part = nullptr,
+              size_t* nparts_out = nullptr) {
+    const SegArgs A = seg_args(a);
+    if (!S.e_chunked) {
+        const double avg = (double)a.n / (double)std::max<uint64_t>(S.ge, 1);
+        const int L = avg <= 8.0 ? 4 : avg <= 32.0 ? 16 : 64;
+        // the right-hand side's per-wave partials are summed afterwards: fewer waves there
+        const unsigned grid = grid_for(std::max<uint64_t>(S.ge, 1), WPB * (WAVE / L), MODE == M_RHS ? 4096u : 1u << 15);
+        double* part = nullptr;
+        if (MODE == M_RHS) {
+            const size_t nw = (size_t)grid * WPB;
+            rhs_part->reserve(nw * 3 * r);
+            part = rhs_part->p;
+            *nparts_out = nw;
+        }
+        with_lanes_width(L, S.be, [&](auto lc, auto bc) {
+            constexpr int LL = decltype(lc)::value, BE = decltype(bc)::value;
+            hipLaunchKernelGGL((k_sl_seg<MODE, LL, BE>), dim3(grid), dim3(TPB), 0, st, A, src, ld, x, r, dst, alpha,
+                               part, done);
+        });
+        FH_CHECK_LAUNCH();
+        return;
+    }
+    const ItemArgs I{S.e_first.p, S.e_end.p, S.e_lvl.p, S.seg_it.p, S.e_nitems};
+    S.epart.reserve(S.e_nitems * (S.be + 2) * r);
+    S.tbuf.reserve(S.ge * S.be * r);
+    double* segpart = nullptr;
+    if (MODE == M_RHS) {
+        rhs_part->reserve(S.ge * 3 * r);
+        segpart = rhs_part->p;
+        *nparts_out = S.ge;
+    }
+    hipLaunchKernelGGL((k_sl_segA<MODE>), dim3(grid_for(S.e_nitems, WPB, 1u << 15)), dim3(TPB), 0, st, A, I, src, ld,
+                       x, r, dst, S.epart.p, done);
+    hipLaunchKernelGGL((k_sl_segB<MODE>), dim3(grid_for(S.ge * r, TPB, 8192)), dim3(TPB), 0, st, A, I, S.epart.p, r,
+                       S.tbuf.p, alpha, segpart, done);
+    hipLaunchKernelGGL((k_sl_segC<MODE>), dim3(grid_for(a.n, TPB, 8192)), dim3(TPB), 0, st, A, S.row_seg.p, a.n, src,
+                       ld, r, S.tbuf.p, dst, done);
+    FH_CHECK_LAUNCH();
+}
+
+// E's segments in items of <= CH rows when the longest exceeds CH.
+void build_e_items(SlopeState& S, const std::vector<uint32_t>& hseg, hipStream_t st) {
+    uint32_t maxlen = 0;
+    for (uint64_t s = 0; s < S.ge; s++) maxlen = std::max(maxlen, hseg[s + 1] - hseg[s]);
+    S.e_chunked = maxlen > CH;
+    if (!S.e_chunked) return;
+    std::vector<uint32_t> first, end, lvl, seg_it(S.ge + 1);
+    for (uint64_t s = 0; s < S.ge; s++) {
+        seg_it[s] = (uint32_t)first.size();
+        const uint32_t len = hseg[s + 1] - hseg[s];
+        const uint32_t nch = std::max<uint32_t>(1, (len + CH - 1) / CH);
+        for (uint32_t c = 0; c < nch; c++) {
+            first.push_back(hseg[s] + c * CH);
+            end.push_back(std::min(hseg[s + 1], hseg[s] + (c + 1) * CH));
+            lvl.push_back((uint32_t)s);
+        }
+    }
+    seg_it[S.ge] = (uint32_t)first.size();
+    S.e_nitems = first.size();
+    S.e_first.alloc(S.e_nitems);
+    S.e_end.alloc(S.e_nitems);
+    S.e_lvl.alloc(S.e_nitems);
+    S.seg_it.alloc(S.ge + 1);
+    h2d(S.e_first.p, first.data(), S.e_nitems, st);
+    h2d(S.e_end.p, end.data(), S.e_nitems, st);
+    h2d(S.e_lvl.p, lvl.data(), S.e_nitems, st);
+    h2d(S.seg_it.p, seg_it.data(), S.ge + 1, st);
+    stream_wait(st);
+}
+
+// Y rows of block B (b x r per level) = the pull of Z over its levels.
+void pull_block(SlopeState& S, const SlBlock& B, const double* Z, int r, double* Y, const uint32_t* done,
+                hipStream_t st) {
+    if (!B.chunked) {
+        // lanes per level from the mean level length
+        const double avg = (double)B.poff_n / (double)std::max<uint64_t>(B.g, 1);
+        const int L = avg <= 8.0 ? 4 : avg <= 32.0 ? 16 : 64;
+        const unsigned grid = grid_for(std::max<uint64_t>(B.g, 1), WPB * (WAVE / L), 1u << 15);
+        with_lanes_width(L, B.b, [&](auto lc, auto bc) {
+            constexpr int LL = decltype(lc)::value, BB = decltype(bc)::value;
+            hipLaunchKernelGGL((k_sl_pull<LL, BB>), dim3(grid), dim3(TPB), 0, st, pull_args(S, B), Z, r, Y, done);
+        });
+        return;
+    }
+    S.ipart.reserve(B.nitems * B.b * r);
+    hipLaunchKernelGGL(k_sl_pull_items, dim3(grid_for(B.nitems, WPB, 1u << 15)), dim3(TPB), 0, st, pull_args(S, B),
+                       item_args(B), Z, r, S.ipart.p, done);
+    hipLaunchKernelGGL(k_sl_item_sum, dim3((unsigned)std::min<uint64_t>(std::max<uint64_t>(B.g, 1), 8192)), dim3(TPB), 0,
+                       st, B.lvl_it.p, B.g, S.ipart.p, B.b * r, Y + B.off * r, done);
+}
+
+// Items of <= CH rows over the level lists of a block whose longest level exceeds CH.
+void build_items(SlBlock& B, hipStream_t st) {
+    std::vector<uint32_t> off(B.g + 1);
+    d2h(off.data(), B.poff.p, B.g + 1, st);
+    stream_wait(st);
+    B.poff_n = off[B.g];
+    uint32_t maxlen = 0;
+    for (uint64_t l = 0; l < B.g; l++) maxlen = std::max(maxlen, off[l + 1] - off[l]);
+    B.chunked = maxlen > CH;
+    if (!B.chunked) return;
+    std::vector<uint32_t> first, end, lvl, lvl_it(B.g + 1);
+    for (uint64_t l = 0; l < B.g; l++) {
+        lvl_it[l] = (uint32_t)first.size();
+        const uint32_t len = off[l + 1] - off[l];
+        const uint32_t nch = std::max<uint32_t>(1, (len + CH - 1) / CH);
+        for (uint32_t c = 0; c < nch; c++) {
+            first.push_back(off[l] + c * CH);
+            end.push_back(std::min(off[l + 1], off[l] + (c + 1) * CH));
+            lvl.push_back((uint32_t)l);
+        }
+    }
+    lvl_it[B.g] = (uint32_t)first.size();
+    B.nitems = first.size();
+    B.it_first.alloc(B.nitems);
+    B.it_end.alloc(B.nitems);
+    B.it_lvl.alloc(B.nitems);
+    B.lvl_it.alloc(B.g + 1);
+    h2d(B.it_first.p, first.data(), B.nitems, st);
+    h2d(B.it_end.p, end.data(), B.nitems, st);
+    h2d(B.it_lvl.p, lvl.data(), B.nitems, st);
+    h2d(B.lvl_it.p, lvl_it.data(), B.g + 1, st);
+}
 
 // Rows (of the index space 0..n-1, codes[idx[i]] when idx) sorted by (code, index): the sorted
 // indices and the offsets of every level (g + 1).
@@ -741,11 +1255,8 @@ void slope_spmv(const Absorber& a, const double* p, double* y, int r, const uint
     SlopeState& S = *a.slope;
     require(r >= 1 && r <= 16, "slopes: 1..16 columns");
     S.zbuf.reserve(a.n * r);
-    hipLaunchKernelGGL((k_sl_seg<M_APPLY>), dim3(seg_grid(S)), dim3(TPB), 0, st, seg_args(a), p, 0, nullptr, r,
-                       S.zbuf.p, nullptr, nullptr, done);
-    for (const SlBlock& B : S.blocks)
-        hipLaunchKernelGGL(k_sl_pull, dim3(grid_for(std::max<uint64_t>(B.g, 1), WPB, 1u << 15)), dim3(TPB), 0, st,
-                           pull_args(S, B), S.zbuf.p, r, y, done);
+    seg_pass<M_APPLY>(a, S, p, 0, nullptr, r, S.zbuf.p, nullptr, done, st);
+    for (const SlBlock& B : S.blocks) pull_block(S, B, S.zbuf.p, r, y, done, st);
     if (pq) hipLaunchKernelGGL(k_sl_dot, dim3(DOTG), dim3(TPB), 0, st, p, y, a.m * r, r, pq, done);
     FH_CHECK_LAUNCH();
 }
@@ -793,8 +1304,16 @@ void slope_refresh(Absorber& a, const double* w_rows) {
     a.sw_ready = true;
     // M_s^+ per level of E
     S.minv.reserve(std::max<uint64_t>(S.ge * S.be * S.be, 1));
-    hipLaunchKernelGGL(k_sl_minv, dim3(grid_for(std::max<uint64_t>(S.ge, 1), TPB, 8192)), dim3(TPB), 0, st,
-                       S.seg_off.p, S.ge, S.he.p, S.e_ns, S.e_icpt, w, S.minv.p);
+    {
+        const unsigned grid = grid_for(std::max<uint64_t>(S.ge, 1), TPB, 8192);
+        switch (S.e_ns) {
+            case 0: hipLaunchKernelGGL(k_sl_minv<0>, dim3(grid), dim3(TPB), 0, st, S.seg_off.p, S.ge, S.he.p, S.e_icpt, w, S.minv.p); break;
+            case 1: hipLaunchKernelGGL(k_sl_minv<1>, dim3(grid), dim3(TPB), 0, st, S.seg_off.p, S.ge, S.he.p, S.e_icpt, w, S.minv.p); break;
+            case 2: hipLaunchKernelGGL(k_sl_minv<2>, dim3(grid), dim3(TPB), 0, st, S.seg_off.p, S.ge, S.he.p, S.e_icpt, w, S.minv.p); break;
+            case 3: hipLaunchKernelGGL(k_sl_minv<3>, dim3(grid), dim3(TPB), 0, st, S.seg_off.p, S.ge, S.he.p, S.e_icpt, w, S.minv.p); break;
+            default: hipLaunchKernelGGL(k_sl_minv<4>, dim3(grid), dim3(TPB), 0, st, S.seg_off.p, S.ge, S.he.p, S.e_icpt, w, S.minv.p); break;
+        }
+    }
     FH_CHECK_LAUNCH();
     // diagonal blocks, redundancy, block-Jacobi inverses
     S.gdiag.reserve(std::max<size_t>(m, 1));
@@ -814,9 +1333,8 @@ void slope_refresh(Absorber& a, const double* w_rows) {
         D.off = B.off;
         D.b = B.b;
         D.icpt = B.icpt;
-        D.s0 = B.s0;
-        D.nsv = S.nsv;
-        D.sv = S.sv.p;
+        D.ns = B.ns;
+        D.svp = B.svp.p;
         D.he = S.he.p;
         D.e_ns = S.e_ns;
         D.e_icpt = S.e_icpt;
@@ -826,8 +1344,29 @@ void slope_refresh(Absorber& a, const double* w_rows) {
         D.w = w;
         D.jacobi = !(S.dense && B.small);
         D.jinv = D.jacobi ? S.jinv.p + B.joff : nullptr;
-        hipLaunchKernelGGL(k_sl_diag, dim3(grid_for(std::max<uint64_t>(B.g, 1), TPB, 8192)), dim3(TPB), 0, st, D,
-                           S.gdiag.p, S.red.p);
+        // widths as compile-time constants (the lanes argument of the helper is unused here)
+        with_lanes_width(4, B.b, [&](auto, auto bc) {
+            constexpr int BW = decltype(bc)::value;
+            auto by_be = [&](auto ec) {
+                constexpr int BE = decltype(ec)::value;
+                if (B.chunked) {
+                    DBuf<double> part(B.nitems * 2 * BW * BW);
+                    hipLaunchKernelGGL((k_sl_diag_items<BW, BE>), dim3(grid_for(B.nitems, TPB, 8192)), dim3(TPB), 0,
+                                       st, D, item_args(B), part.p);
+                    hipLaunchKernelGGL((k_sl_diag_finish<BW>), dim3(grid_for(std::max<uint64_t>(B.g, 1), TPB, 8192)),
+                                       dim3(TPB), 0, st, D, item_args(B), part.p, S.gdiag.p, S.red.p);
+                } else {
+                    hipLaunchKernelGGL((k_sl_diag<BW, BE>), dim3(grid_for(std::max<uint64_t>(B.g, 1), TPB, 8192)),
+                                       dim3(TPB), 0, st, D, S.gdiag.p, S.red.p);
+                }
+            };
+            switch (S.be) {
+                case 1: by_be(std::integral_constant<int, 1>{}); break;
+                case 2: by_be(std::integral_constant<int, 2>{}); break;
+                case 3: by_be(std::integral_constant<int, 3>{}); break;
+                default: by_be(std::integral_constant<int, 4>{}); break;
+            }
+        });
         FH_CHECK_LAUNCH();
     }
     // Jacobi leaves every coordinate to the block inverses (zero inverse diagonal)
@@ -922,6 +1461,7 @@ void slope_create(Absorber& a, const std::vector<SlopeTerm>& terms, const double
     std::vector<uint32_t> hseg(S->ge + 1);
     d2h(hseg.data(), S->seg_off.p, S->ge + 1, st);
     stream_wait(st);
+    build_e_items(*S, hseg, st);
     auto width = [&](size_t q) { return (size_t)(terms[q].icpt ? 1 : 0) + terms[q].slopes.size(); };
     std::vector<size_t> others;
     for (size_t q = 0; q < nt; q++)
@@ -1005,7 +1545,16 @@ void slope_create(Absorber& a, const std::vector<SlopeTerm>& terms, const double
                        S->he.p);
     FH_CHECK_LAUNCH();
     // per block: rows by (level, row)
-    for (SlBlock& B : S->blocks) sort_by_code(terms[B.term].codes, a.pi.p, n, B.g, B.perm, B.poff, st);
+    for (SlBlock& B : S->blocks) {
+        sort_by_code(terms[B.term].codes, a.pi.p, n, B.g, B.perm, B.poff, st);
+        build_items(B, st);
+        B.svp.alloc(std::max<size_t>(n * B.ns, 1));
+        if (B.ns > 0) {
+            hipLaunchKernelGGL(k_sl_svp, dim3(grid_for(n * B.ns, TPB, 8192)), dim3(TPB), 0, st, B.perm.p, n, S->sv.p,
+                               S->nsv, B.s0, B.ns, B.svp.p);
+            FH_CHECK_LAUNCH();
+        }
+    }
     // the kept-order slope columns are no longer needed once in row order
     S->slope_cols.clear();
     a.fe1 = fe1;
@@ -1031,17 +1580,12 @@ ResidOut slope_residualize(Absorber& a, double* z, int r, double tol, uint64_t m
     S.zbuf.reserve(n * r);
     a.s64.rhs.reserve(std::max<size_t>(m * r, 1));
     a.s64.x.reserve(std::max<size_t>(m * r, 1));
-    const unsigned grid = seg_grid(S);
-    const size_t nw = (size_t)grid * WPB;
-    DBuf<double> part(nw * 3 * r), sums(3 * (size_t)r);
-    const SegArgs A = seg_args(a);
-    hipLaunchKernelGGL((k_sl_seg<M_RHS>), dim3(grid), dim3(TPB), 0, st, A, z, r, nullptr, r, S.zbuf.p, nullptr, part.p,
-                       nullptr);
-    for (const SlBlock& B : S.blocks)
-        hipLaunchKernelGGL(k_sl_pull, dim3(grid_for(std::max<uint64_t>(B.g, 1), WPB, 1u << 15)), dim3(TPB), 0, st,
-                           pull_args(S, B), S.zbuf.p, r, a.s64.rhs.p, nullptr);
+    DBuf<double> part, sums(3 * (size_t)r);
+    size_t nparts = 0;
+    seg_pass<M_RHS>(a, S, z, r, nullptr, r, S.zbuf.p, nullptr, nullptr, st, &part, &nparts);
+    for (const SlBlock& B : S.blocks) pull_block(S, B, S.zbuf.p, r, a.s64.rhs.p, nullptr, st);
     FH_CHECK_LAUNCH();
-    sum_parts_to(part.p, nw, 3 * (size_t)r, sums.p, st);
+    sum_parts_to(part.p, nparts, 3 * (size_t)r, sums.p, st);
     std::vector<double> hs(3 * (size_t)r);
     d2h(hs.data(), sums.p, hs.size(), st);
     stream_wait(st);
@@ -1072,9 +1616,7 @@ ResidOut slope_residualize(Absorber& a, double* z, int r, double tol, uint64_t m
         out.iterations = std::max<uint64_t>(pr.iterations, 1);
     }
     (void)allconst;
-    hipLaunchKernelGGL((k_sl_seg<M_RESID>), dim3(grid), dim3(TPB), 0, st, A, z, r, m > 0 ? x : nullptr, r, z,
-                       coefs ? coefs + m * r : nullptr, nullptr, nullptr);
-    FH_CHECK_LAUNCH();
+    seg_pass<M_RESID>(a, S, z, r, m > 0 ? x : nullptr, r, z, coefs ? coefs + m * r : nullptr, nullptr, st);
     if (coefs && m > 0) FH_HIP(hipMemcpyAsync(coefs, x, m * r * sizeof(double), hipMemcpyDeviceToDevice, st));
     return out;
 }

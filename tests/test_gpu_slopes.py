@@ -212,3 +212,50 @@ def test_what_slope_absorbers_refuse():
                 intercepts=[t[2] for t in terms], slopes=[t[3] for t in terms])
     with pytest.raises(F.FeregError, match="without intercept or slopes"):
         F.Absorber([terms[0][0]], [terms[0][1]], intercepts=[False])
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_long_levels_run_in_chunks(weighted):
+    """Levels longer than the level passes' item size (1024 rows): two year levels of 15 000 rows
+    with a slope and a 7-level term; E (50 levels of 600 rows) runs cross the item boundaries."""
+    rng = np.random.default_rng(8)
+    n = 30_000
+    e = np.repeat(np.arange(50), 600).astype(np.uint32)
+    year = rng.integers(0, 2, n).astype(np.uint32)
+    k7 = rng.integers(0, 7, n).astype(np.uint32)
+    x = rng.standard_normal(n)
+    w = 0.5 + rng.uniform(size=n) if weighted else None
+    terms = [(e, 50, True, []), (year, 2, True, [x]), (k7, 7, True, [])]
+    V = _columns(terms, n, 2, 13)
+    ab = _absorber(terms, w)
+    try:
+        E, info = _resid(ab, V)
+    finally:
+        ab.close()
+    ref = osl.residualize(V, osl.slope_design(terms, n), w)
+    assert _max_rel(E, ref) <= 1e-9, info.iterations
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_long_segments_of_e_run_in_chunks(weighted):
+    """indiv[x] + year: E is year (the only intercept-only term) with two segments of 2000 rows,
+    longer than the item size, so its pass runs over items; residuals and coefficients (fit
+    reconstruction) against the projection."""
+    rng = np.random.default_rng(9)
+    n = 4000
+    indiv = (np.arange(n) // 10).astype(np.uint32)
+    year = (np.arange(n) % 2).astype(np.uint32)
+    x = rng.standard_normal(n)
+    w = 0.5 + rng.uniform(size=n) if weighted else None
+    terms = [(indiv, 400, True, [x]), (year, 2, True, [])]
+    V = _columns(terms, n, 3, 17)
+    ab = _absorber(terms, w)
+    try:
+        E, info = _resid(ab, V)
+        fit = np.empty(n)
+        fit[ab.row_order()] = ab.fit_rows(info.coefs, 1)
+    finally:
+        ab.close()
+    ref = osl.residualize(V, osl.slope_design(terms, n), w)
+    assert _max_rel(E, ref) <= 1e-9, info.iterations
+    np.testing.assert_allclose(V[:, 1] - E[:, 1], fit, atol=1e-9 * np.abs(V[:, 1]).max())
