@@ -116,7 +116,8 @@ FH_PVAL = {"two-tailed": 0, "equal-tailed": 1, "greater": 2, "less": 3}
 class WbootSpec(ctypes.Structure):
     _fields_ = [("reps", ctypes.c_uint64), ("seed", ctypes.c_uint64), ("r_mat", ctypes.POINTER(ctypes.c_double)),
                 ("r", ctypes.c_double), ("level", ctypes.c_double), ("tol", ctypes.c_double),
-                ("maxiter", ctypes.c_uint64), ("pval_type", ctypes.c_int32), ("conf_int", ctypes.c_int32)]
+                ("maxiter", ctypes.c_uint64), ("pval_type", ctypes.c_int32), ("conf_int", ctypes.c_int32),
+                ("boot_type", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class WbootResult(ctypes.Structure):

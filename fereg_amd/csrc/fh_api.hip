@@ -1596,7 +1596,9 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
         for (size_t q = 0; q < a.nterms; q++)
             if (!nest[q]) m_nn += a.term_g[q];
         const bool path1 = m_nn > 0 && m_nn <= std::min<uint64_t>(G, 512);
-        require(m_nn == 0 || path1 || G > 512,
+        const int bt = spec->boot_type == 0 ? 11 : spec->boot_type;
+        require(bt == 11 || bt == 13 || bt == 31 || bt == 33, "wildboot: boot_type must be 11, 13, 31 or 33");
+        require(bt != 11 || m_nn == 0 || path1 || G > 512,
                 "fixed effects not nested in the clusters with more levels than clusters (<= 512) need the "
                 "per-cluster projection, which this build does not provide");
         WbootInput in;
@@ -1613,6 +1615,18 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
         const double nn = (double)n, gg = (double)os.ncl;
         in.ssc = (ssc->k_adj ? (nn - 1.0) / (nn - os.dfk) : 1.0) * (ssc->g_adj && gg > 1.0 ? gg / (gg - 1.0) : 1.0);
         in.S = pre.ready ? pre.S.p : nullptr;
+        if (bt != 11) {
+            // MNW: the non-nested terms' orthonormal basis joins the regressors when it fits; else
+            // their leave-cluster-out refitting is left out, as the reference does with a warning
+            DBuf<double> Qb;
+            int r_nn = 0;
+            const bool use_q = path1 && in.kept.size() + m_nn + 2 <= 21;
+            if (use_q) mnw_basis(a, nest, m_nn, in, Qb, r_nn);
+            in.S = nullptr;
+            ols_wildboot_mnw(a, in, *spec, bt, rk.data(), Qb.p, r_nn, *res, t_boot);
+            res->fe_not_projected = m_nn > 0 && !use_q;
+            return;
+        }
         WbExt ext;
         if (path1) wb_path1(a, nest, m_nn, in, rk, ext);
         ols_wildboot(a, in, *spec, rk.data(), *res, t_boot);

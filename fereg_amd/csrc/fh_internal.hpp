@@ -490,6 +490,14 @@ void wb_path1(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, WbootI
 uint64_t count_nonempty_clusters(Absorber& a, const uint32_t* cl, uint64_t G);
 void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* r_mat_kept,
                   fh_wboot_result& res, double* t_boot_host);
+// MNW 13 / 31 / 33 (wildboot/ols.rs:651-1082) on the same input: Q (n x r_nn, row order, or null)
+// the orthonormal basis of the non-nested terms' dummies residualized on the nested ones.
+void ols_wildboot_mnw(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, int boot_type,
+                      const double* r_mat_kept, const double* Q, int r_nn, fh_wboot_result& res, double* t_boot_host);
+// That basis for the non-nested terms (m_nn levels): Q'WQ = I over the eigenvalues of the
+// residualized dummies' Gram above 1e-10 of the largest (r_nn of them).
+void mnw_basis(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, const WbootInput& in, DBuf<double>& Q,
+               int& r_nn);
 // Per absorbed term: 1 when each of its levels lies in a single cluster (codes in kept order).
 std::vector<int> terms_nested_in(Absorber& a, const uint32_t* cl);
 

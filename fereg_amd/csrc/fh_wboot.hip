@@ -775,6 +775,98 @@ void wb_cluster_sums(Absorber& a, const double* rows, int R, const std::vector<i
     FH_CHECK_LAUNCH();
 }
 
+// The draws and the inference shared by WCR11 and the MNW variants: the per-cluster rows P
+// (device, G x NC with NC = 2 + 4 mk: [num_0, num_1, Phi_0, Phi_1, A_0 B, A_1 B] and their column
+// scales), AA (the per-cluster scalars' Gram, 2 x 2) and BB (sum B B', mk x mk) give every draw's
+// affine coefficients (num0, num1, v00, v01, v11) of t(delta) by the int8 MFMA contraction;
+// then the observed statistic (point, v_obs), the p-value and the test-inverted interval.
+void wb_draws(Absorber& a, const double* P, size_t G, int NC, int mk, const std::vector<double>& scale,
+              const std::vector<double>& inv_scale, const std::vector<double>& AA, const std::vector<double>& BB,
+              double ssc, double point, double v_obs, const fh_wboot_spec& spec, fh_wboot_result& res,
+              double* t_boot_host, std::chrono::steady_clock::time_point t0) {
+    hipStream_t st = a.st;
+    const double num_obs = point - spec.r;
+    const double se = (std::isfinite(v_obs) && v_obs > 1e-300) ? std::sqrt(v_obs) : NAN;
+    const double stat = std::isfinite(se) ? num_obs / se : NAN;
+    // draws
+    uint64_t reps = spec.reps;
+    bool full = false;
+    if (G <= 20 && (1ull << G) <= reps) {  // weights.rs:39-61
+        reps = 1ull << G;
+        full = true;
+    }
+    const size_t Gpad = (G + 63) / 64 * 64;
+    const int ncol = LIMBS * NC, ntiles = (ncol + 15) / 16, ncolp = ntiles * 16;
+    DBuf<int8_t> Pq((size_t)ncolp * Gpad);
+    FH_HIP(hipMemsetAsync(Pq.p, 0, Pq.bytes(), st));
+    DBuf<double> dinv(NC), dscale(NC), dAA(4), dBB((size_t)mk * mk);
+    FH_HIP(hipMemcpyAsync(dinv.p, inv_scale.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dscale.p, scale.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dAA.p, AA.data(), 4 * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dBB.p, BB.data(), (size_t)mk * mk * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_wb_quant, dim3(grid_for(G * NC, TPB, 8192)), dim3(TPB), 0, st, P, G, NC, dinv.p, Gpad, Pq.p);
+    FH_CHECK_LAUNCH();
+    const uint32_t nsteps = (uint32_t)(Gpad / 64);
+    const size_t bblocks = (reps + WB_ROWS - 1) / WB_ROWS, Bpad = bblocks * WB_ROWS;
+    const int cgroups = (ntiles + WB_NTC - 1) / WB_NTC;
+    // split the G range so the grid fills the chip (~4 workgroups per CU)
+    uint32_t nsplit = (uint32_t)std::max<size_t>(1, std::min<size_t>(nsteps, (1024 + bblocks * cgroups - 1) / (bblocks * cgroups)));
+    const uint32_t per = (nsteps + nsplit - 1) / nsplit;
+    nsplit = (nsteps + per - 1) / per;
+    DBuf<int32_t> part((size_t)nsplit * Bpad * ncolp);
+    hipEvent_t e0, e1;
+    FH_HIP(hipEventCreate(&e0));
+    FH_HIP(hipEventCreate(&e1));
+    FH_HIP(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_wb_gemm, dim3((unsigned)bblocks, nsplit, cgroups), dim3(TPB), 0, st, Pq.p, Gpad, ntiles,
+                       (uint64_t)spec.seed, (int)full, per, nsteps, part.p, Bpad, ncolp);
+    FH_CHECK_LAUNCH();
+    FH_HIP(hipEventRecord(e1, st));
+    DBuf<double> coefs(reps * 5);
+    hipLaunchKernelGGL(k_wb_post, dim3(grid_for(reps, TPB / WAVE, 8192)), dim3(TPB), 0, st, part.p, (int)nsplit, Bpad, ncolp,
+                       NC, mk, reps, dscale.p, dAA.p, dBB.p, ssc, coefs.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> hc(reps * 5);
+    FH_HIP(hipMemcpyAsync(hc.data(), coefs.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    float gms = 0.f;
+    FH_HIP(hipEventElapsedTime(&gms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    res.contraction_ms = gms;
+    // int8 multiply-adds (2 ops each) of the draws x clusters x limb-columns product, padding included
+    res.contraction_ops = 2.0 * (double)Bpad * (double)Gpad * (double)ncolp;
+    // inference (ols.rs:553-649)
+    const double delta = spec.r - point;
+    std::vector<double> tb(reps);
+    for (uint64_t b = 0; b < reps; b++) tb[b] = eval_affine_t(delta, &hc[b * 5]);
+    auto pv = compute_pvalue(stat, tb, spec.pval_type);
+    res.stat = stat;
+    res.p_val = pv.first;
+    res.reps_feasible = pv.second;
+    res.se = se;
+    res.point = point;
+    res.ssc = ssc;
+    res.reps = reps;
+    res.n_clusters = G;
+    res.full_enumeration = full;
+    res.ci_lo = NAN;
+    res.ci_hi = NAN;
+    if (spec.conf_int) {
+        std::vector<double> trial(reps);
+        auto ci = invert_ci_search(point, se, 1.0 - spec.level, spec.pval_type, spec.tol, spec.maxiter, pv.second,
+                                   [&](double r_trial) {
+                                       const double d = r_trial - point, st_trial = (point - r_trial) / se;
+                                       for (uint64_t b = 0; b < reps; b++) trial[b] = eval_affine_t(d, &hc[b * 5]);
+                                       return compute_pvalue(st_trial, trial, spec.pval_type).first;
+                                   });
+        res.ci_lo = ci.first;
+        res.ci_hi = ci.second;
+    }
+    if (t_boot_host) std::memcpy(t_boot_host, tb.data(), reps * sizeof(double));
+    res.seconds_bootstrap = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* rk, fh_wboot_result& res,
                   double* t_boot_host) {
     hipStream_t st = a.st;
@@ -823,87 +915,8 @@ void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, 
     // observed statistic: q_obs = A_0 (the cluster scores of the unrestricted residuals)
     double point = 0.0;
     for (int j = 0; j < mk; j++) point += rk[j] * in.beta[j];
-    const double num_obs = point - spec.r;
     const double v_obs = in.ssc * AA[0];
-    const double se = (std::isfinite(v_obs) && v_obs > 1e-300) ? std::sqrt(v_obs) : NAN;
-    const double stat = std::isfinite(se) ? num_obs / se : NAN;
-    // draws
-    uint64_t reps = spec.reps;
-    bool full = false;
-    if (G <= 20 && (1ull << G) <= reps) {  // weights.rs:39-61
-        reps = 1ull << G;
-        full = true;
-    }
-    const size_t Gpad = (G + 63) / 64 * 64;
-    const int ncol = LIMBS * NC, ntiles = (ncol + 15) / 16, ncolp = ntiles * 16;
-    DBuf<int8_t> Pq((size_t)ncolp * Gpad);
-    FH_HIP(hipMemsetAsync(Pq.p, 0, Pq.bytes(), st));
-    DBuf<double> dinv(NC), dscale(NC), dAA(4), dBB((size_t)mk * mk);
-    FH_HIP(hipMemcpyAsync(dinv.p, inv_scale.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
-    FH_HIP(hipMemcpyAsync(dscale.p, scale.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
-    FH_HIP(hipMemcpyAsync(dAA.p, AA.data(), 4 * sizeof(double), hipMemcpyHostToDevice, st));
-    FH_HIP(hipMemcpyAsync(dBB.p, BB.data(), (size_t)mk * mk * sizeof(double), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_wb_quant, dim3(grid_for(G * NC, TPB, 8192)), dim3(TPB), 0, st, P.p, G, NC, dinv.p, Gpad, Pq.p);
-    FH_CHECK_LAUNCH();
-    const uint32_t nsteps = (uint32_t)(Gpad / 64);
-    const size_t bblocks = (reps + WB_ROWS - 1) / WB_ROWS, Bpad = bblocks * WB_ROWS;
-    const int cgroups = (ntiles + WB_NTC - 1) / WB_NTC;
-    // split the G range so the grid fills the chip (~4 workgroups per CU)
-    uint32_t nsplit = (uint32_t)std::max<size_t>(1, std::min<size_t>(nsteps, (1024 + bblocks * cgroups - 1) / (bblocks * cgroups)));
-    const uint32_t per = (nsteps + nsplit - 1) / nsplit;
-    nsplit = (nsteps + per - 1) / per;
-    DBuf<int32_t> part((size_t)nsplit * Bpad * ncolp);
-    hipEvent_t e0, e1;
-    FH_HIP(hipEventCreate(&e0));
-    FH_HIP(hipEventCreate(&e1));
-    FH_HIP(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_wb_gemm, dim3((unsigned)bblocks, nsplit, cgroups), dim3(TPB), 0, st, Pq.p, Gpad, ntiles,
-                       (uint64_t)spec.seed, (int)full, per, nsteps, part.p, Bpad, ncolp);
-    FH_CHECK_LAUNCH();
-    FH_HIP(hipEventRecord(e1, st));
-    DBuf<double> coefs(reps * 5);
-    hipLaunchKernelGGL(k_wb_post, dim3(grid_for(reps, TPB / WAVE, 8192)), dim3(TPB), 0, st, part.p, (int)nsplit, Bpad, ncolp,
-                       NC, mk, reps, dscale.p, dAA.p, dBB.p, in.ssc, coefs.p);
-    FH_CHECK_LAUNCH();
-    std::vector<double> hc(reps * 5);
-    FH_HIP(hipMemcpyAsync(hc.data(), coefs.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    FH_HIP(hipStreamSynchronize(st));
-    float gms = 0.f;
-    FH_HIP(hipEventElapsedTime(&gms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    res.contraction_ms = gms;
-    // int8 multiply-adds (2 ops each) of the draws x clusters x limb-columns product, padding included
-    res.contraction_ops = 2.0 * (double)Bpad * (double)Gpad * (double)ncolp;
-    // inference (ols.rs:553-649)
-    const double delta = spec.r - point;
-    std::vector<double> tb(reps);
-    for (uint64_t b = 0; b < reps; b++) tb[b] = eval_affine_t(delta, &hc[b * 5]);
-    auto pv = compute_pvalue(stat, tb, spec.pval_type);
-    res.stat = stat;
-    res.p_val = pv.first;
-    res.reps_feasible = pv.second;
-    res.se = se;
-    res.point = point;
-    res.ssc = in.ssc;
-    res.reps = reps;
-    res.n_clusters = G;
-    res.full_enumeration = full;
-    res.ci_lo = NAN;
-    res.ci_hi = NAN;
-    if (spec.conf_int) {
-        std::vector<double> trial(reps);
-        auto ci = invert_ci_search(point, se, 1.0 - spec.level, spec.pval_type, spec.tol, spec.maxiter, pv.second,
-                                   [&](double r_trial) {
-                                       const double d = r_trial - point, st_trial = (point - r_trial) / se;
-                                       for (uint64_t b = 0; b < reps; b++) trial[b] = eval_affine_t(d, &hc[b * 5]);
-                                       return compute_pvalue(st_trial, trial, spec.pval_type).first;
-                                   });
-        res.ci_lo = ci.first;
-        res.ci_hi = ci.second;
-    }
-    if (t_boot_host) std::memcpy(t_boot_host, tb.data(), reps * sizeof(double));
-    res.seconds_bootstrap = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    wb_draws(a, P.p, G, NC, mk, scale, inv_scale, AA, BB, in.ssc, point, v_obs, spec, res, t_boot_host, t0);
 }
 
 void wb_path1(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, WbootInput& in, std::vector<double>& rk,
@@ -1018,6 +1031,345 @@ void wb_path1(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, WbootI
     in.e = e2;
     in.es = es2;
     in.S = ext.S.p;
+}
+
+
+// ------------------------------------------------------------------ MNW 13 / 31 / 33
+// MacKinnon, Nielsen and Webb (2023): the WCR bootstrap with the CRV3 jackknife in the data
+// generating process (31), in the variance of the statistic (13), or both (33), for one
+// restriction with the bootstrap clusters as the error clusters (wildboot/ols.rs:651-1082).
+// Per cluster g, with A_g = X_g'W X_g and s_t,g = X_g'W u_t (t = 0: u_hat, t = 1: -X~ M_R, the
+// affine interval's two targets), G = sum A_g and G_(-g) = G - A_g:
+//   D_t = s_t                                     (DGP 1)
+//   D_t = s_t + A_g (y1 - G_(-g)^+ R'h^+ R y1),   y1 = G_(-g)^+ s_t, h = R G_(-g)^+ R'   (DGP 3:
+//         the score of the restricted leave-cluster-out residuals)
+//   J = R G^+ (CRV1 variance) or R G_(-g)^+ (CRV3), k_t = J D_t, JA = J A_g,
+//   phi_t = G^+ D_t, num_t = R phi_t.
+// A draw's variance is mult sum_g (V_bg k_t - JA db_t)^2 with db_t = V_b phi: the form of the
+// WCR11 contraction with P = [num_0, num_1, phi_0, phi_1, k_0 JA, k_1 JA], AA = sum k k',
+// BB = sum JA JA'. mult = (G - 1)/G for the CRV3 variance, the CRV1 factor otherwise. Fixed
+// effects not nested in the clusters enter as the reference's Path-1 pseudo-regressors: an
+// orthonormal basis Q of their dummies residualized on the nested terms (Q'WQ = I).
+
+namespace {
+
+// out[g] = [A_g (p x p), s_0 (p), s_1 (p)] with x = [x~ (kept regressors), Q]: one wave per cluster,
+// each lane summing its entries over the cluster's runs in row order (fixed order).
+__global__ __launch_bounds__(TPB) void k_mnw_sums(const uint32_t* __restrict__ off, const uint32_t* __restrict__ order,
+                                                  const uint32_t* __restrict__ run_off, size_t G,
+                                                  const double* __restrict__ rows, int R, const int* __restrict__ kept,
+                                                  int k, const double* __restrict__ Q, int r,
+                                                  const double* __restrict__ MR, const double* __restrict__ e,
+                                                  size_t es, const double* __restrict__ w, double* __restrict__ out) {
+    const int p = k + r, E = p * p + 2 * p;
+    for (size_t g = (size_t)blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE; g < G;
+         g += (size_t)gridDim.x * (TPB / WAVE)) {
+        for (int ent = lane_id(); ent < E; ent += WAVE) {
+            int a, b = 0, kind;
+            if (ent < p * p) {
+                kind = 0;
+                a = ent / p;
+                b = ent % p;
+            } else if (ent < p * p + p) {
+                kind = 1;
+                a = ent - p * p;
+            } else {
+                kind = 2;
+                a = ent - p * p - p;
+            }
+            double s = 0.0;
+            for (uint32_t j = off[g]; j < off[g + 1]; j++) {
+                const uint32_t run = order[j];
+                for (uint32_t i = run_off[run]; i < run_off[run + 1]; i++) {
+                    const double wi = w ? w[i] : 1.0;
+                    const double xa = a < k ? rows[(size_t)i * R + 1 + kept[a]] : Q[(size_t)i * r + a - k];
+                    double f;
+                    if (kind == 0) {
+                        f = b < k ? rows[(size_t)i * R + 1 + kept[b]] : Q[(size_t)i * r + b - k];
+                    } else if (kind == 1) {
+                        f = e[(size_t)i * es];
+                    } else {
+                        double z = 0.0;
+                        for (int c = 0; c < k; c++) z += rows[(size_t)i * R + 1 + kept[c]] * MR[c];
+                        f = -z;
+                    }
+                    s += wi * xa * f;
+                }
+            }
+            out[g * E + ent] = s;
+        }
+    }
+}
+
+// dst[i] = src[inv[pi[i]]] (rows of m values): another absorber's row order into this one's.
+__global__ void k_mnw_perm(const double* __restrict__ src, int m, const uint32_t* __restrict__ inv,
+                           const uint32_t* __restrict__ pi, size_t n, double* __restrict__ dst) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t j = inv[pi[i]];
+        for (int c = 0; c < m; c++) dst[i * m + c] = src[j * m + c];
+    }
+}
+
+__global__ void k_mnw_inv(const uint32_t* __restrict__ pi, size_t n, uint32_t* __restrict__ inv) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        inv[pi[i]] = (uint32_t)i;
+}
+
+// Q[i] = D[i] T (n x m times m x r).
+__global__ void k_mnw_basis(const double* __restrict__ D, size_t n, int m, const double* __restrict__ T, int r,
+                            double* __restrict__ Q) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < r; c++) {
+            double s = 0.0;
+            for (int l = 0; l < m; l++) s += D[i * m + l] * T[l * r + c];
+            Q[i * r + c] = s;
+        }
+}
+
+// Symmetric pseudo-inverse: eigenvalues <= tol * (largest |eigenvalue|) dropped (wildboot/mod.rs:434-470).
+std::vector<double> sym_pinv(std::vector<double> A, int n, double tol) {
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < i; j++) A[(size_t)i * n + j] = A[(size_t)j * n + i] = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
+    std::vector<double> lam, U, out((size_t)n * n, 0.0);
+    if (n == 0) return out;
+    sym_eigen(A, n, lam, U);
+    double lmax = 0.0;
+    for (double l : lam) lmax = std::max(lmax, std::fabs(l));
+    if (lmax == 0.0) return out;
+    for (int e = 0; e < n; e++) {
+        if (!(lam[e] > tol * lmax)) continue;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) out[(size_t)i * n + j] += U[(size_t)i * n + e] * U[(size_t)j * n + e] / lam[e];
+    }
+    return out;
+}
+
+}  // namespace
+
+void mnw_basis(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, const WbootInput& in, DBuf<double>& Q,
+               int& r_nn) {
+    hipStream_t st = a.st;
+    const size_t n = a.n;
+    const int m = (int)m_nn;
+    NnTerms nt{};
+    uint32_t off = 0;
+    bool any_nested = false;
+    for (size_t q = 0; q < a.nterms; q++) {
+        if (nested[q]) {
+            any_nested = true;
+            continue;
+        }
+        require(nt.nq < 8, "at most 8 non-nested terms");
+        require(a.codes_kept.size() == a.nterms && a.codes_kept[q].p, "the absorber has no term codes", FH_INTERNAL);
+        nt.codes[nt.nq] = a.codes_kept[q].p;
+        nt.off[nt.nq++] = off;
+        off += (uint32_t)a.term_g[q];
+    }
+    DBuf<double> D(n * m);
+    std::vector<double> Gnn((size_t)m * m);
+    if (!any_nested) {
+        hipLaunchKernelGGL(k_nn_dummies, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, n, a.pi.p, nt, m, D.p);
+        FH_CHECK_LAUNCH();
+        gram_cols(a, D.p, n, m, 0, m, in.w, nullptr, Gnn.data());
+    } else {
+        // the dummies residualized on the nested terms, in an absorber of those terms alone
+        std::vector<fh_term> tt;
+        for (size_t q = 0; q < a.nterms; q++)
+            if (nested[q]) tt.push_back(fh_term{a.codes_kept[q].p, a.term_g[q], 1});
+        DBuf<double> wk;
+        if (in.w) {
+            wk.alloc(n);
+            hipLaunchKernelGGL(k_to_kept, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, in.w, a.pi.p, n, wk.p);
+            FH_CHECK_LAUNCH();
+        }
+        stream_wait(st);
+        fh_solver_config cfg;
+        fh_config_default(&cfg);
+        fh_absorber* tmp = nullptr;
+        if (fh_absorber_create_dev(tt.data(), tt.size(), in.w ? wk.p : nullptr, n, &cfg, nullptr, &tmp) != FH_OK)
+            throw Error(FH_INTERNAL, std::string("non-nested projection: ") + fh_last_error());
+        struct Tmp {
+            fh_absorber* p;
+            ~Tmp() { fh_absorber_destroy(p); }
+        } hold{tmp};
+        DBuf<double> dn(n * m);
+        hipLaunchKernelGGL(k_nn_dummies, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, n, tmp->a.pi.p, nt, m, dn.p);
+        FH_CHECK_LAUNCH();
+        stream_wait(st);
+        fh_solve_opts o{1e-12, 100000, 0, FH_PLAIN_FP64};
+        fh_solve_info info{};
+        if (fh_absorber_residualize_dev(tmp, dn.p, (size_t)m, &o, nullptr, nullptr, &info, nullptr) != FH_OK)
+            throw Error(FH_INTERNAL, std::string("non-nested projection: ") + fh_last_error());
+        {
+            AbsScope sc(tmp->a);
+            gram_cols(tmp->a, dn.p, n, m, 0, m, tmp->a.weighted ? tmp->a.w_rows.p : nullptr, nullptr, Gnn.data());
+            stream_wait(tmp->a.st);
+        }
+        DBuf<uint32_t> inv(n);
+        hipLaunchKernelGGL(k_mnw_inv, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, tmp->a.pi.p, n, inv.p);
+        hipLaunchKernelGGL(k_mnw_perm, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, dn.p, m, inv.p, a.pi.p, n, D.p);
+        FH_CHECK_LAUNCH();
+        stream_wait(st);
+    }
+    // Q = D~ U Lambda^-1/2 over the eigenvalues above 1e-10 of the largest (wildboot/mod.rs:1036-1076)
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < i; j++) Gnn[(size_t)i * m + j] = Gnn[(size_t)j * m + i] = 0.5 * (Gnn[(size_t)i * m + j] + Gnn[(size_t)j * m + i]);
+    std::vector<double> lam, U;
+    sym_eigen(Gnn, m, lam, U);
+    double lmax = 0.0;
+    for (double l : lam) lmax = std::max(lmax, std::fabs(l));
+    std::vector<int> keep;
+    for (int e = 0; e < m; e++)
+        if (lmax > 0.0 && lam[e] > 1e-10 * lmax) keep.push_back(e);
+    r_nn = (int)keep.size();
+    if (r_nn == 0) return;
+    std::vector<double> T((size_t)m * r_nn);
+    for (int l = 0; l < m; l++)
+        for (int c = 0; c < r_nn; c++) T[(size_t)l * r_nn + c] = U[(size_t)l * m + keep[c]] / std::sqrt(lam[keep[c]]);
+    DBuf<double> dT(T.size());
+    h2d(dT.p, T.data(), T.size(), st);
+    Q.alloc(n * r_nn);
+    hipLaunchKernelGGL(k_mnw_basis, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, D.p, n, m, dT.p, r_nn, Q.p);
+    FH_CHECK_LAUNCH();
+    stream_wait(st);
+}
+
+void ols_wildboot_mnw(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, int boot_type, const double* rk,
+                      const double* Q, int r_nn, fh_wboot_result& res, double* t_boot_host) {
+    hipStream_t st = a.st;
+    const size_t G = in.G;
+    const int k = (int)in.kept.size(), p = k + r_nn;
+    require(k >= 1 && p + 2 <= KMAX, "the MNW bootstrap supports kept regressors + non-nested levels <= 19");
+    require(G >= 2 && G < (1ull << 24), "the bootstrap needs 2 .. 2^24 clusters");
+    require(spec.reps >= 1, "wildboot: reps must be at least 1");
+    require(spec.level > 0.0 && spec.level < 1.0, "wildboot: confidence level must be in (0, 1)");
+    require(spec.tol > 0.0 && std::isfinite(spec.tol) && spec.maxiter > 0, "wildboot: tol and maxiter must be positive");
+    require(spec.pval_type >= 0 && spec.pval_type <= 3, "wildboot: unknown p-value type");
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> C, MR;
+    wb_constraint(in.bread, k, rk, C, MR);
+    DBuf<int> dkept(k);
+    DBuf<double> dMR(k);
+    h2d(dkept.p, in.kept.data(), k, st);
+    h2d(dMR.p, MR.data(), k, st);
+    ClusterRuns local;
+    const ClusterRuns& cr = runs_for(a, in.cl, G, local);
+    const int E = p * p + 2 * p;
+    DBuf<double> S(G * E);
+    hipLaunchKernelGGL(k_mnw_sums, dim3(grid_for(G, TPB / WAVE, 1u << 16)), dim3(TPB), 0, st, cr.off.p, cr.order.p,
+                       cr.run_off.p, G, in.rows, in.R, dkept.p, k, Q, r_nn, dMR.p, in.e, in.es, in.w, S.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> hs(G * E);
+    d2h(hs.data(), S.p, hs.size(), st);
+    stream_wait(st);
+    // G, its pseudo-inverse (eigenvalues <= 1e-12 of the largest dropped) and R over [x~, Q]
+    std::vector<double> Gt((size_t)p * p, 0.0), Ra(p, 0.0);
+    for (size_t g = 0; g < G; g++)
+        for (int e = 0; e < p * p; e++) Gt[e] += hs[g * E + e];
+    for (int j = 0; j < k; j++) Ra[j] = rk[j];
+    const std::vector<double> Gi = sym_pinv(Gt, p, 1e-12);
+    std::vector<double> cR(p, 0.0);  // G^+ R'
+    for (int i = 0; i < p; i++)
+        for (int j = 0; j < p; j++) cR[i] += Gi[(size_t)i * p + j] * Ra[j];
+    const bool dgp3 = boot_type == 31 || boot_type == 33, den3 = boot_type == 13 || boot_type == 33;
+    const double mult = den3 ? ((double)G - 1.0) / (double)G : in.ssc;
+    const int NC = 2 + 4 * p;
+    std::vector<double> P(G * NC), AA(4, 0.0), BB((size_t)p * p, 0.0), cmax(NC, 0.0);
+    double v_obs = 0.0;
+    std::vector<double> A((size_t)p * p), J(p), JA(p), D(p), y1(p), gnr(p), Gm((size_t)p * p), Gmi;
+    for (size_t g = 0; g < G; g++) {
+        const double* hg = hs.data() + g * E;
+        for (int e = 0; e < p * p; e++) A[e] = hg[e];
+        const double* s0 = hg + p * p;
+        const double* s1 = s0 + p;
+        if (dgp3 || den3) {
+            for (int e = 0; e < p * p; e++) Gm[e] = Gt[e] - A[e];
+            Gmi = sym_pinv(Gm, p, 1e-10);
+        }
+        // J = R G_(-g)^+ (CRV3) or R G^+ (CRV1)
+        for (int c = 0; c < p; c++) {
+            double s = 0.0;
+            if (den3)
+                for (int j = 0; j < p; j++) s += Ra[j] * Gmi[(size_t)j * p + c];
+            else
+                s = cR[c];
+            J[c] = s;
+        }
+        for (int c = 0; c < p; c++) {
+            double s = 0.0;
+            for (int j = 0; j < p; j++) s += J[j] * A[(size_t)j * p + c];
+            JA[c] = s;
+        }
+        double jo = 0.0;
+        for (int c = 0; c < p; c++) jo += J[c] * s0[c];
+        v_obs += jo * jo;
+        double hinv = 0.0;
+        if (dgp3) {
+            double h = 0.0;
+            for (int i = 0; i < p; i++) {
+                double s = 0.0;
+                for (int j = 0; j < p; j++) s += Gmi[(size_t)i * p + j] * Ra[j];
+                gnr[i] = s;
+                h += Ra[i] * s;
+            }
+            hinv = h > 0.0 ? 1.0 / h : 0.0;  // the 1 x 1 pseudo-inverse
+        }
+        double kt[2], rphi[2];
+        double* row = P.data() + g * NC;
+        for (int t = 0; t < 2; t++) {
+            const double* sin = t == 0 ? s0 : s1;
+            if (!dgp3) {
+                for (int c = 0; c < p; c++) D[c] = sin[c];
+            } else {
+                for (int i = 0; i < p; i++) {
+                    double s = 0.0;
+                    for (int j = 0; j < p; j++) s += Gmi[(size_t)i * p + j] * sin[j];
+                    y1[i] = s;
+                }
+                double ry = 0.0;
+                for (int i = 0; i < p; i++) ry += Ra[i] * y1[i];
+                for (int i = 0; i < p; i++) y1[i] -= gnr[i] * hinv * ry;  // the restricted step
+                for (int i = 0; i < p; i++) {
+                    double s = 0.0;
+                    for (int j = 0; j < p; j++) s += A[(size_t)i * p + j] * y1[j];
+                    D[i] = sin[i] + s;
+                }
+            }
+            double kk = 0.0, rp = 0.0;
+            for (int c = 0; c < p; c++) kk += J[c] * D[c];
+            for (int i = 0; i < p; i++) {
+                double ph = 0.0;
+                for (int j = 0; j < p; j++) ph += Gi[(size_t)i * p + j] * D[j];
+                row[2 + t * p + i] = ph;
+                rp += Ra[i] * ph;
+            }
+            kt[t] = kk;
+            rphi[t] = rp;
+        }
+        row[0] = rphi[0];
+        row[1] = rphi[1];
+        for (int c = 0; c < p; c++) {
+            row[2 + 2 * p + c] = kt[0] * JA[c];
+            row[2 + 3 * p + c] = kt[1] * JA[c];
+        }
+        AA[0] += kt[0] * kt[0];
+        AA[1] += kt[0] * kt[1];
+        AA[3] += kt[1] * kt[1];
+        for (int i = 0; i < p; i++)
+            for (int j = 0; j < p; j++) BB[(size_t)i * p + j] += JA[i] * JA[j];
+        for (int c = 0; c < NC; c++) cmax[c] = std::max(cmax[c], std::fabs(row[c]));
+    }
+    AA[2] = AA[1];
+    std::vector<double> scale(NC), inv_scale(NC);
+    for (int c = 0; c < NC; c++) {
+        scale[c] = cmax[c] > 0.0 ? 4.0 * cmax[c] : 1.0;
+        inv_scale[c] = std::ldexp(1.0 / scale[c], 56);
+    }
+    DBuf<double> dP(P.size());
+    h2d(dP.p, P.data(), P.size(), st);
+    double point = 0.0;
+    for (int j = 0; j < k; j++) point += rk[j] * in.beta[j];
+    wb_draws(a, dP.p, G, NC, p, scale, inv_scale, AA, BB, mult, point, mult * v_obs, spec, res, t_boot_host, t0);
 }
 
 }  // namespace fh

@@ -251,10 +251,13 @@ class WildBootResult:
 
 
 def wildboot(y, X, cluster, R, r=0.0, reps=999, seed=1, level=0.95, pval_type="two-tailed", conf_int=True,
-             tol=1e-6, maxiter=50, codes=None, g=None, nested=None, ssc=None, opts=None, absorber=None):
-    """WCR11 wild cluster bootstrap of R beta = r after OLS (wildboot/ols.rs:243-649): Rademacher
+             tol=1e-6, maxiter=50, codes=None, g=None, nested=None, ssc=None, opts=None, absorber=None,
+             boot_type=11):
+    """Wild cluster bootstrap of R beta = r after OLS (wildboot/ols.rs:243-649): Rademacher
     weights from SplitMix64 streams (seed, draw), bootstrap clusters = error clusters, p-value and
-    test-inverted interval. The draws x clusters contraction runs on int8 MFMA."""
+    test-inverted interval. The draws x clusters contraction runs on int8 MFMA. boot_type 11 is
+    WCR11; 13, 31 and 33 are MacKinnon, Nielsen and Webb's (2023) variants with the CRV3
+    jackknife in the variance, the data generating process, or both (wildboot/ols.rs:651-1082)."""
     lib = L.lib()
     Xa = X if _is_torch(X) else np.asarray(X, float)
     if not _is_torch(Xa) and Xa.ndim == 1:
@@ -270,7 +273,7 @@ def wildboot(y, X, cluster, R, r=0.0, reps=999, seed=1, level=0.95, pval_type="t
     Rv = np.ascontiguousarray(np.asarray(R, float).reshape(-1))
     assert Rv.size == K, "R needs one entry per regressor"
     spec = L.WbootSpec(int(reps), int(seed), _ptr(Rv, L.f64p), float(r), float(level), float(tol), int(maxiter),
-                       L.FH_PVAL[pval_type], int(bool(conf_int)))
+                       L.FH_PVAL[pval_type], int(bool(conf_int)), int(boot_type), 0)
     beta = np.zeros(K)
     V = np.zeros(K * K)
     kept = np.zeros(K, np.int32)

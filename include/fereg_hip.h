@@ -327,6 +327,10 @@ typedef struct {
     uint64_t maxiter;     /* interval search iterations (50) */
     int32_t pval_type;    /* FH_PVAL_* */
     int32_t conf_int;     /* 1: invert the test for an interval */
+    int32_t boot_type;    /* 11 (or 0): WCR11; 13, 31, 33: MacKinnon-Nielsen-Webb (2023) with the
+                             CRV3 jackknife in the variance (13), the data generating process (31)
+                             or both (33) (wildboot/ols.rs:651-1082) */
+    int32_t reserved;
 } fh_wboot_spec;
 typedef struct {
     double stat, p_val, se, point, ci_lo, ci_hi, ssc;

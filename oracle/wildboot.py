@@ -16,8 +16,10 @@ Restates crates/fereg-core/src/wildboot/:
   mod.rs:529-580        compute_pvalue
   mod.rs:637-749        invert_ci_search (bracketing + Chandrupatla)
   fit/dist.rs:32        qnorm (Wichura's AS241)
-plus a brute-force refit oracle (tests/wildboot.rs:538-708): y* = X b_R + V e_R, dense OLS with
-explicit FE dummies, CRV1 t per draw.
+plus brute-force refit oracles (tests/wildboot.rs:538-708): y* = X b_R + V e_R, dense OLS with
+explicit FE dummies, CRV1 t per draw (WCR11), and the MNW (2023) 13 / 31 / 33 variants with
+restricted leave-one-cluster-out jackknife residuals in the DGP and / or the CRV3 variance
+(ols.rs:651-1082 is what they check).
 """
 import math
 
@@ -374,3 +376,175 @@ def brute_t_boot(X, y, w, D, cl, G, R, r, reps, seed, ssc):
         tb[b] = (Rf @ bb - target) / crv1_se(eb, bb)
     return stat, se, tb
 
+
+
+def brute_mnw(X, y, w, D, cl, G, R, r, reps, seed, ssc, boot_type):
+    """Brute-force MNW (2023) wild cluster bootstrap, restating the reference's own check
+    (tests/wildboot.rs:538-708): a dense design F = [X, a basis of the FE dummies D], the
+    restricted fit imposing R b = r, and for boot types 31 / 33 the jackknife residuals
+    y_g - F_g b_R^(-g) from restricted leave-one-cluster-out refits (the FE refitted on the other
+    clusters: a basis of the dummies over the remaining rows, so FE levels only in cluster g drop
+    out); y* = F b_R + V_g u_g; the statistic's variance is CRV1 (11, 31; times ssc) or CRV3
+    (13, 33: (G-1)/G sum_g (b^(-g) - b)^2 over leave-one-cluster-out refits of the same sample).
+    Returns (stat, se, t_boot) over the draws of rademacher(seed, reps, G)."""
+    n, k = X.shape
+    wv = np.ones(n) if w is None else np.asarray(w, float)
+    Dm = D if D is not None else np.zeros((n, 0))
+
+    def basis(rows_w):
+        if Dm.shape[1] == 0:
+            return np.zeros((n, 0))
+        sub = Dm * np.sqrt(rows_w)[:, None]
+        _, sv, vt = np.linalg.svd(sub, full_matrices=False)
+        if sv.size == 0 or sv[0] == 0:
+            return np.zeros((n, 0))
+        rank = int(np.sum(sv > 1e-10 * sv[0]))
+        return Dm @ vt[:rank].T
+
+    def fit(cols, yv, wts):
+        A = cols.T @ (wts[:, None] * cols)
+        Ai = np.linalg.inv(A)
+        return Ai @ (cols.T @ (wts * yv)), Ai
+
+    def restricted(b, Ai, Rf):
+        return b - Ai @ Rf * ((Rf @ b - r) / (Rf @ Ai @ Rf))
+
+    F = np.column_stack([X, basis(wv)])
+    Rf = np.zeros(F.shape[1])
+    Rf[:k] = R
+    bu, Ai = fit(F, y, wv)
+    eu = y - F @ bu
+    br = restricted(bu, Ai, Rf)
+    er = y - F @ br
+
+    def loo(yv, g, restrict):
+        wg = np.where(cl == g, 0.0, wv)
+        cols = np.column_stack([X, basis(wg)])
+        b, Aig = fit(cols, yv, wg)
+        if restrict:
+            Rg = np.zeros(cols.shape[1])
+            Rg[:k] = R
+            b = restricted(b, Aig, Rg)
+        return b, cols
+
+    def se_crv1(e):
+        sc = np.zeros((G, F.shape[1]))
+        np.add.at(sc, cl, (wv * e)[:, None] * F)
+        return math.sqrt(Rf @ (Ai @ (sc.T @ sc) @ Ai * ssc) @ Rf)
+
+    def se_crv3(yv, b):
+        d = np.array([R @ (loo(yv, g, False)[0][:k] - b[:k]) for g in range(G)])
+        return math.sqrt((G - 1) / G * np.sum(d * d))
+
+    den3 = boot_type in (13, 33)
+    if boot_type in (31, 33):
+        u = np.empty(n)
+        for g in range(G):
+            bg, cols = loo(y, g, True)
+            m = cl == g
+            u[m] = y[m] - cols[m] @ bg
+    else:
+        u = er
+    se = se_crv3(y, bu) if den3 else se_crv1(eu)
+    stat = (Rf @ bu - r) / se
+    V, _ = rademacher(seed, reps, G)
+    yhat = F @ br
+    target = Rf @ br
+    tb = np.empty(V.shape[0])
+    for b in range(V.shape[0]):
+        yb = yhat + V[b][cl] * u
+        bb = Ai @ (F.T @ (wv * yb))
+        s = se_crv3(yb, bb) if den3 else se_crv1(yb - F @ bb)
+        tb[b] = (Rf @ bb - target) / s
+    return stat, se, tb
+
+
+def _sym_pinv(A, tol):
+    """Symmetric pseudo-inverse with eigenvalues <= tol * max |eigenvalue| dropped (mod.rs:434-470)."""
+    A = 0.5 * (A + A.T)
+    lam, U = np.linalg.eigh(A)
+    lmax = np.max(np.abs(lam)) if lam.size else 0.0
+    if lmax == 0.0:
+        return np.zeros_like(A)
+    keep = lam > tol * lmax
+    return (U[:, keep] / lam[keep]) @ U[:, keep].T
+
+
+def mnw_basis(Dnn, Dnest, w):
+    """Path 1's basis for MNW (mod.rs:1036-1076): the non-nested dummies residualized on the nested
+    terms' dummies, times U Lambda^-1/2 over the eigenvalues of their weighted Gram above 1e-10 of the
+    largest, so Q'WQ = I."""
+    n = Dnn.shape[0]
+    wv = np.ones(n) if w is None else np.asarray(w, float)
+    Dt = Dnn
+    if Dnest is not None and Dnest.shape[1]:
+        sw = np.sqrt(wv)
+        coef, *_ = np.linalg.lstsq(Dnest * sw[:, None], Dnn * sw[:, None], rcond=1e-12)
+        Dt = Dnn - Dnest @ coef
+    Gm = Dt.T @ (wv[:, None] * Dt)
+    lam, U = np.linalg.eigh(0.5 * (Gm + Gm.T))
+    lmax = np.max(np.abs(lam))
+    keep = lam > 1e-10 * lmax
+    return Dt @ (U[:, keep] / np.sqrt(lam[keep]))
+
+
+def mnw_fast(Xt, u, w, cl, G, bread, R, r, beta, ssc, reps, seed, boot_type, Q=None):
+    """run_cluster_mnw (ols.rs:651-1082) for one restriction with the affine targets {u_hat, -X M_R}:
+    Xt the FWL-residualized kept regressors, u their residuals, Q the non-nested basis (or None).
+    Returns (stat, se, t_boot)."""
+    n, k = Xt.shape
+    wv = np.ones(n) if w is None else np.asarray(w, float)
+    xa = Xt if Q is None else np.column_stack([Xt, Q])
+    p = xa.shape[1]
+    C = bread @ R
+    MR = C / (R @ C)
+    targets = np.column_stack([u, -(Xt @ MR)])
+    A = np.zeros((G, p, p))
+    S = np.zeros((G, 2, p))
+    for g in range(G):
+        m = cl == g
+        A[g] = xa[m].T @ (wv[m][:, None] * xa[m])
+        S[g] = (wv[m][:, None] * targets[m]).T @ xa[m]
+    Gt = A.sum(axis=0)
+    Gi = _sym_pinv(Gt, 1e-12)
+    Ra = np.zeros(p)
+    Ra[:k] = R
+    dgp3, den3 = boot_type in (31, 33), boot_type in (13, 33)
+    mult = (G - 1) / G if den3 else ssc
+    rphi = np.zeros((G, 2))
+    phi = np.zeros((G, 2, p))
+    kk = np.zeros((G, 2))
+    JA = np.zeros((G, p))
+    jobs = np.zeros(G)
+    for g in range(G):
+        Gmi = _sym_pinv(Gt - A[g], 1e-10) if (dgp3 or den3) else None
+        J = Ra @ Gmi if den3 else Gi @ Ra
+        JA[g] = J @ A[g]
+        jobs[g] = J @ S[g, 0]
+        for t in range(2):
+            s = S[g, t]
+            if dgp3:
+                y1 = Gmi @ s
+                gnr = Gmi @ Ra
+                h = Ra @ gnr
+                y1 = y1 - gnr * ((Ra @ y1) / h if h > 0 else 0.0)
+                d = s + A[g] @ y1
+            else:
+                d = s
+            kk[g, t] = J @ d
+            phi[g, t] = Gi @ d
+            rphi[g, t] = Ra @ phi[g, t]
+    point = R @ beta
+    v_obs = mult * np.sum(jobs * jobs)
+    se = math.sqrt(v_obs)
+    stat = (point - r) / se
+    V, _ = rademacher(seed, reps, G)
+    delta = r - point
+    tb = np.empty(V.shape[0])
+    for b in range(V.shape[0]):
+        v = V[b]
+        num = v @ rphi[:, 0] + delta * (v @ rphi[:, 1])
+        db = v @ phi[:, 0] + delta * (v @ phi[:, 1])
+        j = v * (kk[:, 0] + delta * kk[:, 1]) - JA @ db
+        tb[b] = num / math.sqrt(mult * np.sum(j * j))
+    return stat, se, tb

@@ -139,3 +139,35 @@ def test_wcr11_per_cluster_projection_is_refused():
     with pytest.raises(F.FeregError) as e:
         F.wildboot(y, X, cl, [1.0], reps=99, codes=[year], g=[T])
     assert e.value.rc == 198
+
+
+@pytest.mark.parametrize("boot_type", [13, 31, 33])
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("cross", [False, True])
+def test_mnw_matches_brute_force_refits(boot_type, weighted, cross):
+    """MNW 13 / 31 / 33 on the GPU (per-cluster Gram blocks and scores on the device, the jackknife
+    algebra per cluster, the WCR11 contraction) against refits by definition: restricted
+    leave-one-cluster-out jackknife residuals and per-sample CRV3 variances (tests/wildboot.rs:
+    538-708); full enumeration of 2^8 draws, FE nested in the clusters and (cross) a year term
+    crossing them (the Path-1 basis)."""
+    import fereg_amd as F
+    from test_wildboot_oracle import _year_design
+    G = 8
+    y, X, cl, store, year, w = _year_design(240, G, 3 + boot_type, weighted)
+    codes = [store.astype(np.uint32), year.astype(np.uint32)] if cross else [store.astype(np.uint32)]
+    gs = [2 * G, 4] if cross else [2 * G]
+    announce("mnw", boot_type, weighted, cross)
+    ab = F.Absorber(codes, gs, weights=w)
+    R = np.array([1.0, 0.0])
+    res = F.wildboot(y, X, cl.astype(np.uint32), R, r=0.3, reps=999, seed=7, opts=F.SolveOpts(tol=1e-13),
+                     absorber=ab, boot_type=boot_type)
+    ab.close()
+    assert res.full_enumeration and res.reps == 1 << G and not res.fe_not_projected
+    ssc = res.ssc
+    if boot_type in (13, 33):
+        assert abs(ssc - (G - 1) / G) < 1e-15
+    D = np.column_stack([dummies(c, g) for c, g in zip(codes, gs)])
+    stat, se, tb = wb.brute_mnw(X, y, w, D, cl, G, R, 0.3, 999, 7, ssc, boot_type)
+    assert abs(res.stat - stat) <= 1e-8 * abs(stat) and abs(res.se - se) <= 1e-8 * se
+    np.testing.assert_allclose(res.t_boot, tb, rtol=1e-7, atol=1e-8)
+    assert 0.0 <= res.p_val <= 1.0 and res.conf_int[0] < res.point < res.conf_int[1]

@@ -154,3 +154,50 @@ def test_non_nested_projection_equals_brute_force_refit(with_firm, weighted):
     # without the projection the draws differ: the projection is doing work
     plain = wb.wcr11_result(beta, Xt, u, w, cl, G, bread, R, r, ssc, reps, seed=99)
     assert np.max(np.abs(plain["t_boot"] - tb)) > 1e-3
+
+
+def _year_design(n, G, seed, weighted):
+    """Clusters of stores (nested) and a year term crossing the clusters."""
+    rng = np.random.default_rng(seed)
+    cl = rng.integers(0, G, n)
+    store = cl * 2 + rng.integers(0, 2, n)
+    year = rng.integers(0, 4, n)
+    X = rng.standard_normal((n, 2))
+    y = X @ np.array([0.5, -0.2]) + rng.standard_normal(2 * G)[store] + rng.standard_normal(4)[year] + \
+        rng.standard_normal(n) * (1.0 + 0.5 * (cl % 3))
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    return y, X, cl, store, year, w
+
+
+@pytest.mark.parametrize("boot_type", [13, 31, 33])
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("cross", [False, True])
+def test_mnw_fast_formula_equals_brute_force(boot_type, weighted, cross):
+    """The MNW algebra (ols.rs:651-1082) against refits by definition: the restricted
+    leave-one-cluster-out jackknife residuals and the CRV3 variance of every bootstrap sample, with
+    FE nested in the clusters and (cross) a year term crossing them through the Path-1 basis."""
+    G, n = 8, 240
+    y, X, cl, store, year, w = _year_design(n, G, 3 + boot_type, weighted)
+    terms = [store, year] if cross else [store]
+    gs = [2 * G, 4] if cross else [2 * G]
+    Xt, u, beta, bread = fit_fwl(y, X, terms, gs, w)
+    R = np.array([1.0, 0.0])
+    r = 0.3
+    ssc = 1.21
+    Q = wb.mnw_basis(dummies(year, 4), dummies(store, 2 * G), w) if cross else None
+    stat, se, tb = wb.mnw_fast(Xt, u, w, cl, G, bread, R, r, beta, ssc, 999, 7, boot_type, Q)
+    D = np.column_stack([dummies(t, g) for t, g in zip(terms, gs)])
+    bstat, bse, btb = wb.brute_mnw(X, y, w, D, cl, G, R, r, 999, 7, ssc, boot_type)
+    assert abs(stat - bstat) <= 1e-9 * abs(bstat) and abs(se - bse) <= 1e-9 * bse
+    np.testing.assert_allclose(tb, btb, rtol=1e-8, atol=1e-9)
+
+
+def test_mnw_brute_force_with_dgp1_and_crv1_is_wcr11():
+    """Boot type 11 of the brute force is the WCR11 refit oracle (the shared machinery)."""
+    G = 8
+    y, X, cl, store, year, w = _year_design(200, G, 4, True)
+    D = dummies(store, 2 * G)
+    a = wb.brute_mnw(X, y, w, D, cl, G, np.array([0.0, 1.0]), 0.0, 999, 3, 1.3, 11)
+    b = wb.brute_t_boot(X, y, w, D, cl, G, np.array([0.0, 1.0]), 0.0, 999, 3, 1.3)
+    assert abs(a[0] - b[0]) <= 1e-10 * abs(b[0])
+    np.testing.assert_allclose(a[2], b[2], rtol=1e-10, atol=1e-12)
