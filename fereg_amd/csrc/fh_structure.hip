@@ -59,6 +59,74 @@ __global__ void k_pack(CodeSet cs, const uint32_t* __restrict__ vals, size_t n, 
     }
 }
 
+// FE1 in kept order: st[0] |= some code below its predecessor's, or more than 64 above it (the
+// fast path is off); otherwise off[l] = the first row of level l (n + 1 threads, empty levels
+// take the next level's first row, off[g1] = n), each offset written by the row where the code
+// steps past it -- at most 64 writes per row, whatever the input.
+__global__ void k_fe1_offsets(const uint32_t* __restrict__ fe1, size_t n, uint64_t g1, uint32_t* __restrict__ off,
+                              uint32_t* __restrict__ st) {
+    uint32_t bad = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t prev = i == 0 ? -1 : (int64_t)fe1[i - 1];
+        const int64_t cur = i == n ? (int64_t)g1 : (int64_t)fe1[i];
+        if (cur < prev || cur - prev > 64) {
+            bad = 1;
+            continue;
+        }
+        for (int64_t l = prev + 1; l <= cur; l++) off[l] = (uint32_t)i;
+    }
+    if (__any(bad) && lane_id() == 0) atomicOr(&st[0], 1u);
+}
+
+// st[1] = the longest level, off[s + 1] - off[s] (one atomic per workgroup).
+__global__ __launch_bounds__(256) void k_max_len(const uint32_t* __restrict__ off, size_t g1, uint32_t* __restrict__ st) {
+    __shared__ uint32_t red[256 / WAVE];
+    uint32_t m = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < g1; i += (size_t)gridDim.x * blockDim.x)
+        m = max(m, off[i + 1] - off[i]);
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, WAVE));
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned w = 1; w < blockDim.x / WAVE; w++) m = max(m, red[w]);
+        atomicMax(&st[1], m);
+    }
+}
+
+// Rows sorted by (FE1, block codes, row) when FE1 ascends in kept order and no level has more than
+// L rows: L lanes per FE1 level, lane j holding the level's row off[s] + j with key = its packed
+// block codes; its position is its rank among the level's rows by (key, row), the order the
+// stable radix sort of the packed (FE1, blocks) word gives.
+template <int L>
+__global__ __launch_bounds__(256) void k_level_rank_sort(CodeSet cs, const uint32_t* __restrict__ off, uint64_t g1,
+                                                         uint32_t* __restrict__ pi) {
+    constexpr int GPW = WAVE / L;
+    const int lane = lane_id(), j = lane % L, base = lane - j;
+    const uint64_t ngroups = (uint64_t)gridDim.x * (blockDim.x / WAVE) * GPW;
+    const uint64_t g0 = ((uint64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * GPW + lane / L;
+    const uint64_t rounds = (g1 + ngroups - 1) / ngroups;
+    for (uint64_t rd = 0; rd < rounds; rd++) {  // the same rounds in every lane (the shuffles)
+        const uint64_t s = g0 + rd * ngroups;
+        const uint32_t lo = s < g1 ? off[s] : 0, len = s < g1 ? off[s + 1] - lo : 0;
+        uint64_t key = ~0ull;
+        if ((uint32_t)j < len) {
+            key = 0;
+            for (int q = 0; q < cs.nq; q++) key |= (uint64_t)cs.c[q][lo + j] << cs.shift[q];
+        }
+        const uint32_t khi = (uint32_t)(key >> 32), klo = (uint32_t)key;
+        uint32_t rank = 0;
+#pragma unroll
+        for (int l = 0; l < L; l++) {
+            const uint32_t ohi = (uint32_t)__shfl((int)khi, base + l, WAVE);
+            const uint32_t olo = (uint32_t)__shfl((int)klo, base + l, WAVE);
+            const uint64_t ok = ((uint64_t)ohi << 32) | olo;
+            rank += ((uint32_t)l < len) && (ok < key || (ok == key && l < j)) ? 1u : 0u;
+        }
+        if ((uint32_t)j < len) pi[lo + rank] = lo + j;
+    }
+}
+
 __global__ void k_iota(uint32_t* __restrict__ v, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         v[i] = (uint32_t)i;
@@ -347,6 +415,44 @@ static int64_t choose_fe1(Absorber& a, const std::vector<const uint32_t*>& dc) {
     return best;
 }
 
+// Rows not yet in (FE1, blocks) order but with FE1 ascending in kept order and no FE1 level
+// longer than 64 rows (panels stored unit by unit, periods unordered): each level's rows are
+// sorted in registers by their block codes instead of a global radix sort. False when the input
+// does not qualify (the caller sorts globally).
+static bool level_rank_sort(Absorber& a, const std::vector<const uint32_t*>& dc,
+                            const std::vector<const uint32_t*>& fields, const std::vector<unsigned>& fbits,
+                            hipStream_t st) {
+    const size_t n = a.n;
+    if (a.fe1 < 0 || fields.size() < 2) return false;
+    DBuf<uint32_t> off(a.g1 + 1), st2(2);
+    FH_HIP(hipMemsetAsync(st2.p, 0, st2.bytes(), st));
+    hipLaunchKernelGGL(k_fe1_offsets, dim3(blocks_for(n + 1)), dim3(TPB), 0, st, dc[a.fe1], n, (uint64_t)a.g1, off.p,
+                       st2.p);
+    hipLaunchKernelGGL(k_max_len, dim3(grid_for(a.g1, TPB, 256)), dim3(TPB), 0, st, off.p, (size_t)a.g1, st2.p);
+    FH_CHECK_LAUNCH();
+    uint32_t hs[2] = {1, 0};
+    download(hs, st2.p, sizeof(hs), st);
+    if (hs[0] != 0 || hs[1] > 64) return false;
+    CodeSet cb{};
+    unsigned sh = 0;
+    cb.nq = (int)fields.size() - 1;
+    for (size_t k = fields.size(); k-- > 1;) {
+        cb.c[k - 1] = fields[k];
+        cb.shift[k - 1] = sh;
+        sh += fbits[k];
+    }
+    auto go = [&](auto lc) {
+        constexpr int L = decltype(lc)::value;
+        hipLaunchKernelGGL((k_level_rank_sort<L>), dim3(grid_for(a.g1, (TPB / WAVE) * (WAVE / L), 1u << 15)), dim3(TPB),
+                           0, st, cb, off.p, (uint64_t)a.g1, a.pi.p);
+    };
+    if (hs[1] <= 16) go(std::integral_constant<int, 16>{});
+    else if (hs[1] <= 32) go(std::integral_constant<int, 32>{});
+    else go(std::integral_constant<int, 64>{});
+    FH_CHECK_LAUNCH();
+    return true;
+}
+
 void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
     hipStream_t st = a.st;
     const size_t n = a.n, nt = a.nterms;
@@ -436,6 +542,7 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
             if (first && words.size() == 1) {
                 check_sorted_u64(keys.p, n, flag.p, st);
                 sorted = read_scalar(flag.p, st) == 0;
+                if (!sorted && level_rank_sort(a, dc, fields, fbits, st)) sorted = true;
             }
             if (!sorted) radix_sort_pairs(keys, a.pi, kalt, valt, n, (int)width, st);
             first = false;

@@ -61,6 +61,24 @@ def test_structure_large_random(sort):
     _compare(codes, gs, 4096)
 
 
+@pytest.mark.parametrize("maxlen,gap", [(16, 0), (32, 0), (64, 0), (65, 0), (16, 3), (16, 100)])
+def test_structure_ascending_fe1_levels(maxlen, gap):
+    """FE1 ascending in kept order: levels of up to 64 rows are sorted in registers by their block
+    codes (ties broken by row, as the stable radix sort does); a 65-row level, or a run of more
+    than 64 empty FE1 levels, takes the radix sort."""
+    rng = np.random.default_rng(maxlen + gap)
+    lens = rng.integers(1, maxlen + 1, 3000)
+    lens[7] = maxlen
+    fe1 = np.repeat(np.arange(len(lens)), lens)
+    fe1 = np.where(fe1 >= 500, fe1 + gap, fe1).astype(np.uint32)  # `gap` empty levels after level 499
+    g1 = len(lens) + gap
+    n = len(fe1)
+    firm = rng.integers(0, 50, n).astype(np.uint32)
+    year = rng.integers(0, 4, n).astype(np.uint32)
+    announce("structure ascending fe1", maxlen, gap)
+    _compare([fe1, firm, year], [g1, 50, 4], 4096)
+
+
 def test_structure_panel_and_repeated_cells():
     rng = np.random.default_rng(5)
     i = np.arange(50_000)
