@@ -490,6 +490,11 @@ void wb_path1(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, WbootI
 uint64_t count_nonempty_clusters(Absorber& a, const uint32_t* cl, uint64_t G);
 void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* r_mat_kept,
                   fh_wboot_result& res, double* t_boot_host);
+// WCR11 with every fixed effect projected out of the draws one cluster at a time (2 .. 2048
+// clusters): one residualization (tolerance min(tol, 1e-10), at most maxiter iterations) per
+// cluster, then the same contraction over G x (2 + 2G) per-cluster columns.
+void ols_wildboot_cells(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* r_mat_kept,
+                        double tol, uint64_t maxiter, fh_wboot_result& res, double* t_boot_host);
 // MNW 13 / 31 / 33 (wildboot/ols.rs:651-1082) on the same input: Q (n x r_nn, row order, or null)
 // the orthonormal basis of the non-nested terms' dummies residualized on the nested ones.
 void ols_wildboot_mnw(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, int boot_type,

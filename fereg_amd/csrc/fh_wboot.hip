@@ -31,6 +31,7 @@ constexpr int LIMBS = 7;
 constexpr int WB_NTC = 19;    // 16-column tiles per GEMM workgroup
 constexpr int WB_ROWS = 128;  // draws per GEMM workgroup (4 waves x 2 tiles of 16)
 constexpr int KMAX = 21;
+constexpr size_t PC_GMAX = 2048;  // clusters of the per-cluster projection
 constexpr uint64_t SM_GAMMA = 0x9E3779B97F4A7C15ull;
 constexpr uint64_t SM_STREAM = 0xD1B54A32D192ED03ull;
 
@@ -347,6 +348,50 @@ __global__ __launch_bounds__(TPB) void k_wb_post(const int32_t* __restrict__ par
             o[4] = ssc * (AA[3] - 2.0 * e11 + q11);
         }
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// The per-cluster form (one wave per draw): column c of part holds limb l of V_b . P[:, c] at
+// l NC + c; num_t = column t, q_t[c] = column 2 + t nc + c, v_t1t2 = ssc sum_c q_t1[c] q_t2[c],
+// summed per lane over c = lane, lane + 64, ... and reduced over the wave in a fixed order.
+__global__ __launch_bounds__(TPB) void k_wb_post_cells(const int32_t* __restrict__ part, int nsplit, size_t Bpad,
+                                                       int ncolp, int NC, int nc, uint64_t reps,
+                                                       const double* __restrict__ scale, double ssc,
+                                                       double* __restrict__ coefs) {
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    for (uint64_t b = (uint64_t)blockIdx.x * (TPB / WAVE) + w; b < reps; b += (uint64_t)gridDim.x * (TPB / WAVE)) {
+        const int32_t* pb = part + b * (size_t)ncolp;
+        auto col = [&](int c) {
+            double v = 0.0;
+            for (int l = LIMBS - 1; l >= 0; l--) {
+                long long s = 0;
+                for (int k = 0; k < nsplit; k++) s += pb[(size_t)k * Bpad * ncolp + (size_t)l * NC + c];
+                v += ldexp((double)s, -8 * (l + 1));
+            }
+            return v * scale[c];
+        };
+        double s00 = 0.0, s01 = 0.0, s11 = 0.0;
+        for (int c = lane; c < nc; c += WAVE) {
+            const double q0 = col(2 + c), q1 = col(2 + nc + c);
+            s00 += q0 * q0;
+            s01 += q0 * q1;
+            s11 += q1 * q1;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            s00 += __shfl_xor(s00, o, WAVE);
+            s01 += __shfl_xor(s01, o, WAVE);
+            s11 += __shfl_xor(s11, o, WAVE);
+        }
+        const double n0 = col(0), n1 = col(1);
+        if (lane == 0) {
+            double* o = coefs + b * 5;
+            o[0] = n0;
+            o[1] = n1;
+            o[2] = ssc * s00;
+            o[3] = ssc * s01;
+            o[4] = ssc * s11;
+        }
     }
 }
 
@@ -669,6 +714,125 @@ __global__ void k_to_kept(const double* __restrict__ v_rows, const uint32_t* __r
         v_kept[pi[i]] = v_rows[i];
 }
 
+// Per row (row order): cluster code, z = x~'C and the weighted targets w u_0 = w e, w u_1 = -w x~'M_R.
+__global__ void k_pc_rows(const double* __restrict__ rows, int R, const int* __restrict__ kept, int mk,
+                          const double* __restrict__ C, const double* __restrict__ MR, const double* __restrict__ e,
+                          size_t es, const double* __restrict__ w, const uint32_t* __restrict__ cl,
+                          const uint32_t* __restrict__ pi, size_t n, uint32_t* __restrict__ clr,
+                          double* __restrict__ z, double2* __restrict__ wu) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double zi = 0.0, u1 = 0.0;
+        for (int a = 0; a < mk; a++) {
+            const double x = rows[i * R + 1 + kept[a]];
+            zi += x * C[a];
+            u1 -= x * MR[a];
+        }
+        const double wi = w ? w[i] : 1.0;
+        clr[i] = cl[pi[i]];
+        z[i] = zi;
+        wu[i] = make_double2(wi * e[i * es], wi * u1);
+    }
+}
+
+// H[i][j] = z_i when row i lies in cluster c0 + j, else 0 (n x nb, j < nb).
+__global__ void k_pc_cols(const uint32_t* __restrict__ clr, const double* __restrict__ z, size_t n, uint32_t c0,
+                          int nb, double* __restrict__ H) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * nb; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = e / nb;
+        const uint32_t j = (uint32_t)(e % nb);
+        H[e] = clr[i] == c0 + j ? z[i] : 0.0;
+    }
+}
+
+// P[g][2 + t nc + c0 + j] = sum over the rows i of cluster g of (w u_t)_i H[i][j]: one wave per
+// cluster, lanes strided over its runs, each summing whole runs in row order, then a fixed-order
+// reduction over the wave.
+__global__ __launch_bounds__(TPB) void k_pc_sums(const uint32_t* __restrict__ off, const uint32_t* __restrict__ order,
+                                                 const uint32_t* __restrict__ run_off, size_t G,
+                                                 const double2* __restrict__ wu, const double* __restrict__ H, int nb,
+                                                 uint32_t c0, size_t nc, int NC, double* __restrict__ P) {
+    const int lane = lane_id();
+    for (size_t g = (size_t)blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE; g < G;
+         g += (size_t)gridDim.x * (TPB / WAVE)) {
+        double s0[16], s1[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) s0[j] = s1[j] = 0.0;
+        for (uint32_t k = off[g] + lane; k < off[g + 1]; k += WAVE) {
+            const uint32_t run = order[k];
+            for (uint32_t i = run_off[run]; i < run_off[run + 1]; i++) {
+                const double2 u = wu[i];
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    if (j < nb) {
+                        const double h = H[(size_t)i * nb + j];
+                        s0[j] += u.x * h;
+                        s1[j] += u.y * h;
+                    }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                s0[j] += __shfl_xor(s0[j], o, WAVE);
+                s1[j] += __shfl_xor(s1[j], o, WAVE);
+            }
+        if (lane == 0) {
+            double* p = P + g * NC + 2 + c0;
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if (j < nb) {
+                    p[j] = s0[j];
+                    p[nc + j] = s1[j];
+                }
+        }
+    }
+}
+
+// Phi[g][t] = bread S_t[g] (mk each) and P[g][t] = C.S_t[g] (the draw's numerators).
+__global__ void k_pc_phi(const double* __restrict__ S, size_t G, int mk, const double* __restrict__ bread,
+                         const double* __restrict__ C, int NC, double* __restrict__ Phi, double* __restrict__ P) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < G * 2; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t g = e / 2;
+        const int t = (int)(e % 2);
+        const double* St = S + (g * 3 + t) * mk;
+        double num = 0.0;
+        for (int a = 0; a < mk; a++) {
+            double f = 0.0;
+            for (int b = 0; b < mk; b++) f += bread[a * mk + b] * St[b];
+            Phi[e * mk + a] = f;
+            num += C[a] * St[a];
+        }
+        P[g * NC + t] = num;
+    }
+}
+
+// P[g][2 + t nc + c] -= B_c . Phi_t[g] (B_c = S[c][2], the cluster's sum of w z x~): the regressors'
+// part of the projection.
+__global__ void k_pc_fix(const double* __restrict__ S, const double* __restrict__ Phi, size_t G, int mk, int NC,
+                         double* __restrict__ P) {
+    const size_t nc = G;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < G * 2 * nc; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t g = e / (2 * nc), r = e % (2 * nc);
+        const int t = (int)(r / nc);
+        const size_t c = r % nc;
+        const double* Bc = S + (c * 3 + 2) * mk;
+        const double* ph = Phi + (g * 2 + t) * mk;
+        double d = 0.0;
+        for (int a = 0; a < mk; a++) d += Bc[a] * ph[a];
+        P[g * NC + 2 + r] -= d;
+    }
+}
+
+// cmax[c] = max over g of |P[g][c]| (one thread per column, G rows).
+__global__ void k_pc_cmax(const double* __restrict__ P, size_t G, int NC, double* __restrict__ cmax) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < NC; c += gridDim.x * blockDim.x) {
+        double m = 0.0;
+        for (size_t g = 0; g < G; g++) m = fmax(m, fabs(P[g * NC + c]));
+        cmax[c] = m;
+    }
+}
+
 }  // namespace
 
 // Per absorbed term (absorber term order): 1 when every level lies in one cluster
@@ -780,10 +944,12 @@ void wb_cluster_sums(Absorber& a, const double* rows, int R, const std::vector<i
 // scales), AA (the per-cluster scalars' Gram, 2 x 2) and BB (sum B B', mk x mk) give every draw's
 // affine coefficients (num0, num1, v00, v01, v11) of t(delta) by the int8 MFMA contraction;
 // then the observed statistic (point, v_obs), the p-value and the test-inverted interval.
+// cells > 0: the per-cluster form instead, P = [num_0, num_1, q_0 (cells), q_1 (cells)] with the cell
+// scores of a draw V_b . q_t[c] (AA and BB unused).
 void wb_draws(Absorber& a, const double* P, size_t G, int NC, int mk, const std::vector<double>& scale,
               const std::vector<double>& inv_scale, const std::vector<double>& AA, const std::vector<double>& BB,
               double ssc, double point, double v_obs, const fh_wboot_spec& spec, fh_wboot_result& res,
-              double* t_boot_host, std::chrono::steady_clock::time_point t0) {
+              double* t_boot_host, std::chrono::steady_clock::time_point t0, size_t cells = 0) {
     hipStream_t st = a.st;
     const double num_obs = point - spec.r;
     const double se = (std::isfinite(v_obs) && v_obs > 1e-300) ? std::sqrt(v_obs) : NAN;
@@ -802,8 +968,10 @@ void wb_draws(Absorber& a, const double* P, size_t G, int NC, int mk, const std:
     DBuf<double> dinv(NC), dscale(NC), dAA(4), dBB((size_t)mk * mk);
     FH_HIP(hipMemcpyAsync(dinv.p, inv_scale.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
     FH_HIP(hipMemcpyAsync(dscale.p, scale.data(), NC * sizeof(double), hipMemcpyHostToDevice, st));
-    FH_HIP(hipMemcpyAsync(dAA.p, AA.data(), 4 * sizeof(double), hipMemcpyHostToDevice, st));
-    FH_HIP(hipMemcpyAsync(dBB.p, BB.data(), (size_t)mk * mk * sizeof(double), hipMemcpyHostToDevice, st));
+    if (cells == 0) {
+        FH_HIP(hipMemcpyAsync(dAA.p, AA.data(), 4 * sizeof(double), hipMemcpyHostToDevice, st));
+        FH_HIP(hipMemcpyAsync(dBB.p, BB.data(), (size_t)mk * mk * sizeof(double), hipMemcpyHostToDevice, st));
+    }
     hipLaunchKernelGGL(k_wb_quant, dim3(grid_for(G * NC, TPB, 8192)), dim3(TPB), 0, st, P, G, NC, dinv.p, Gpad, Pq.p);
     FH_CHECK_LAUNCH();
     const uint32_t nsteps = (uint32_t)(Gpad / 64);
@@ -823,8 +991,12 @@ void wb_draws(Absorber& a, const double* P, size_t G, int NC, int mk, const std:
     FH_CHECK_LAUNCH();
     FH_HIP(hipEventRecord(e1, st));
     DBuf<double> coefs(reps * 5);
-    hipLaunchKernelGGL(k_wb_post, dim3(grid_for(reps, TPB / WAVE, 8192)), dim3(TPB), 0, st, part.p, (int)nsplit, Bpad, ncolp,
-                       NC, mk, reps, dscale.p, dAA.p, dBB.p, ssc, coefs.p);
+    if (cells > 0)
+        hipLaunchKernelGGL(k_wb_post_cells, dim3(grid_for(reps, TPB / WAVE, 8192)), dim3(TPB), 0, st, part.p,
+                           (int)nsplit, Bpad, ncolp, NC, (int)cells, reps, dscale.p, ssc, coefs.p);
+    else
+        hipLaunchKernelGGL(k_wb_post, dim3(grid_for(reps, TPB / WAVE, 8192)), dim3(TPB), 0, st, part.p, (int)nsplit,
+                           Bpad, ncolp, NC, mk, reps, dscale.p, dAA.p, dBB.p, ssc, coefs.p);
     FH_CHECK_LAUNCH();
     std::vector<double> hc(reps * 5);
     FH_HIP(hipMemcpyAsync(hc.data(), coefs.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1033,6 +1205,91 @@ void wb_path1(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, WbootI
     in.S = ext.S.p;
 }
 
+
+void ols_wildboot_cells(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* rk, double tol,
+                        uint64_t maxiter, fh_wboot_result& res, double* t_boot_host) {
+    // The exact projection of every fixed effect, one cluster at a time (Path 2 of the reference,
+    // wildboot/mod.rs:1290-1363, ols.rs:456-548). With the FE projector P_D, X~ the residualized
+    // kept regressors, z = X~ C and u_t the draws' targets, a draw's cell score of cluster c is
+    //   q_t[c] = sum_g V_g M_t[c, g],
+    //   M_t[c, g] = <1_c z, M_X~ M_D (1_g u_t)>_W = <M_D (1_c z), 1_g u_t>_W - B_c . Phi_t[g],
+    // the second form by the W-symmetry of M_D and X~'W P_D = 0. So one residualization per
+    // cluster (the column 1_c z) serves both targets, where the reference projects 1_g u_t for
+    // every target and cluster. The G x (2 + 2G) matrix P = [C.S_0, C.S_1, M_0', M_1'] goes through
+    // the same int8 contraction as the other forms; each draw then sums its cells' squares.
+    hipStream_t st = a.st;
+    const size_t G = in.G, n = a.n;
+    const int mk = (int)in.kept.size();
+    require(mk >= 1 && mk + 2 <= KMAX, "the bootstrap supports 1..19 kept regressors");
+    require(G >= 2 && G <= PC_GMAX, "the per-cluster projection supports 2 .. 2048 clusters");
+    require(spec.reps >= 1, "wildboot: reps must be at least 1");
+    require(spec.level > 0.0 && spec.level < 1.0, "wildboot: confidence level must be in (0, 1)");
+    require(spec.tol > 0.0 && std::isfinite(spec.tol) && spec.maxiter > 0, "wildboot: tol and maxiter must be positive");
+    require(spec.pval_type >= 0 && spec.pval_type <= 3, "wildboot: unknown p-value type");
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> C, MR;
+    wb_constraint(in.bread, mk, rk, C, MR);
+    DBuf<double> Sown;
+    const double* Sp = in.S;
+    if (!Sp) {
+        wb_cluster_sums(a, in.rows, in.R, in.kept, C, MR, in.e, in.es, in.w, in.cl, G, Sown);
+        Sp = Sown.p;
+    }
+    DBuf<int> dkept(mk);
+    DBuf<double> dC(mk), dMR(mk), dbread((size_t)mk * mk);
+    FH_HIP(hipMemcpyAsync(dkept.p, in.kept.data(), mk * sizeof(int), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dC.p, C.data(), mk * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dMR.p, MR.data(), mk * sizeof(double), hipMemcpyHostToDevice, st));
+    FH_HIP(hipMemcpyAsync(dbread.p, in.bread.data(), (size_t)mk * mk * sizeof(double), hipMemcpyHostToDevice, st));
+    DBuf<uint32_t> clr(n);
+    DBuf<double> z(n);
+    DBuf<double2> wu(n);
+    hipLaunchKernelGGL(k_pc_rows, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, in.rows, in.R, dkept.p, mk, dC.p,
+                       dMR.p, in.e, in.es, in.w, in.cl, a.pi.p, n, clr.p, z.p, wu.p);
+    FH_CHECK_LAUNCH();
+    ClusterRuns local;
+    const ClusterRuns& cr = runs_for(a, in.cl, G, local);
+    const int NC = (int)(2 + 2 * G);
+    DBuf<double> P(G * NC), H(n * 16);
+    // the columns 1_c z, 16 clusters per residualization (FP64, the reference's tolerance cap)
+    for (size_t c0 = 0; c0 < G; c0 += 16) {
+        const int nb = (int)std::min<size_t>(16, G - c0);
+        hipLaunchKernelGGL(k_pc_cols, dim3(grid_for(n * nb, TPB, 8192)), dim3(TPB), 0, st, clr.p, z.p, n,
+                           (uint32_t)c0, nb, H.p);
+        FH_CHECK_LAUNCH();
+        residualize_any(a, H.p, (size_t)nb, std::min(tol, 1e-10), maxiter, nullptr, nullptr, FH_PLAIN_FP64, nullptr);
+        hipLaunchKernelGGL(k_pc_sums, dim3(grid_for(G, TPB / WAVE, 4096)), dim3(TPB), 0, st, cr.off.p, cr.order.p,
+                           cr.run_off.p, G, wu.p, H.p, nb, (uint32_t)c0, G, NC, P.p);
+        FH_CHECK_LAUNCH();
+    }
+    DBuf<double> Phi(G * 2 * mk), cmax(NC);
+    hipLaunchKernelGGL(k_pc_phi, dim3(grid_for(G * 2, TPB, 1024)), dim3(TPB), 0, st, Sp, G, mk, dbread.p, dC.p, NC,
+                       Phi.p, P.p);
+    hipLaunchKernelGGL(k_pc_fix, dim3(grid_for(G * 2 * G, TPB, 8192)), dim3(TPB), 0, st, Sp, Phi.p, G, mk, NC, P.p);
+    hipLaunchKernelGGL(k_pc_cmax, dim3(grid_for(NC, TPB, 64)), dim3(TPB), 0, st, P.p, G, NC, cmax.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> hmax(NC), hS(G * 3 * mk);
+    FH_HIP(hipMemcpyAsync(hmax.data(), cmax.p, NC * sizeof(double), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(hS.data(), Sp, hS.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    std::vector<double> scale(NC), inv_scale(NC);
+    for (int c = 0; c < NC; c++) {
+        scale[c] = hmax[c] > 0.0 ? 4.0 * hmax[c] : 1.0;  // |P / scale| <= 1/4: 7 balanced limbs cover it
+        inv_scale[c] = std::ldexp(1.0 / scale[c], 56);
+    }
+    // observed statistic: the cluster scores of the unrestricted residuals, A_0[g] = C.S_0[g]
+    double aa = 0.0;
+    for (size_t g = 0; g < G; g++) {
+        double a0 = 0.0;
+        for (int j = 0; j < mk; j++) a0 += C[j] * hS[(g * 3) * mk + j];
+        aa += a0 * a0;
+    }
+    double point = 0.0;
+    for (int j = 0; j < mk; j++) point += rk[j] * in.beta[j];
+    const std::vector<double> none;
+    wb_draws(a, P.p, G, NC, mk, scale, inv_scale, none, none, in.ssc, point, in.ssc * aa, spec, res, t_boot_host, t0,
+             G);
+}
 
 // ------------------------------------------------------------------ MNW 13 / 31 / 33
 // MacKinnon, Nielsen and Webb (2023): the WCR bootstrap with the CRV3 jackknife in the data

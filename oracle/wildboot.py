@@ -9,7 +9,8 @@ Restates crates/fereg-core/src/wildboot/:
   weights.rs:39-61,118  full enumeration when G <= 20 and 2^G <= reps (draw b: bit c of b)
   ols.rs:76-121         ConstraintProj: C = bread R', M_R = C (R bread R')^+
   ols.rs:243-649        run_cluster_11, cells = clusters (err_equals_boot), the non-nested FE
-                        projection of Path 1 (mod.rs:959-1180) or none (None / TooLarge), affine CI targets
+                        projection of Path 1 (mod.rs:959-1180) or of Path 2 (mod.rs:1290-1363,
+                        ols.rs:456-548) or none (None / TooLarge), affine CI targets
                         {u_hat, -X M_R}: per draw num = V.R Phi, db = V.Phi,
                         q_cell[c, t] = V_c A_c[t] - B_c . db_t, v = ssc sum_c q q'
   ols.rs:139-142        eval_affine_t
@@ -227,7 +228,7 @@ def invert_ci_search(point, se, alpha, pval_type, tol, maxiter, reps_feas, eval_
 
 
 def wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="two-tailed", conf_int=True,
-          tol=1e-6, maxiter=50, dnn=None, dtilde=None):
+          tol=1e-6, maxiter=50, dnn=None, dtilde=None, p2_design=None):
     """run_cluster_11 (ols.rs:243-649) with cells = clusters.
 
     xt: n x k residualized regressors (kept columns), u: residuals, w: weights or None, cl: cluster
@@ -237,6 +238,10 @@ def wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="
     ols.rs:413-448): the FE-projected bootstrap residual is u* - D~ G^+ D_nn' W u*, G = D~'WD~
     (pseudo-inverse: eigenvalues <= 1e-10 of the largest dropped, mod.rs:434-470), so each cell
     score loses f_c . (V psi), f_c = sum_c w z D_nn, psi[g] = G^+ sum_g w u_t D_nn.
+    p2_design: the explicit dummies D of every absorbed term (n x p): the exact projection of
+    Path 2 (mod.rs:1290-1363, ols.rs:456-548): for every target t and cluster g the column 1_g u_t
+    is projected on D, and the cell score of cluster c loses sum_g V_g p2_t[c, g] with
+    p2_t[c, g] = sum_{i in c} w_i z_i P_D(1_g u_t)_i.
     Returns a dict with stat, se, point, p_val, ci, t_boot, coefs (reps x 5), full.
     """
     n, k = xt.shape
@@ -270,6 +275,17 @@ def wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="
         Phi0 = np.concatenate([Phi0, F0 @ Gp], axis=1)
         Phi1 = np.concatenate([Phi1, F1 @ Gp], axis=1)
         Bc = np.concatenate([Bc, fz], axis=1)
+    p2 = None
+    if p2_design is not None:
+        sw = np.sqrt(wv)
+        Dw = p2_design * sw[:, None]
+        p2 = []
+        for ut in (u0, u1):
+            cols = np.zeros((n, G))
+            cols[np.arange(n), cl] = ut  # column g: 1_g u_t
+            coef, *_ = np.linalg.lstsq(Dw, cols * sw[:, None], rcond=1e-12)
+            proj = p2_design @ coef  # P_D(1_g u_t), column g
+            p2.append(cluster_sums(cl, G, (wv * z)[:, None] * proj))  # [c, g]
     V, full = rademacher(seed, reps, G)
     # per draw: explicit cell scores (no expansion)
     num0 = V @ A0
@@ -278,6 +294,9 @@ def wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="
     db1 = V @ Phi1
     q0 = V * A0[None, :] - db0 @ Bc.T
     q1 = V * A1[None, :] - db1 @ Bc.T
+    if p2 is not None:
+        q0 = q0 - V @ p2[0].T
+        q1 = q1 - V @ p2[1].T
     d00 = ssc * np.sum(q0 * q0, axis=1)
     d01 = ssc * np.sum(q0 * q1, axis=1)
     d11 = ssc * np.sum(q1 * q1, axis=1)
@@ -286,9 +305,9 @@ def wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="
 
 
 def wcr11_result(beta, xt, u, w, cl, G, bread, R, r, ssc, reps, seed, level=0.95, pval_type="two-tailed",
-                 conf_int=True, tol=1e-6, maxiter=50, dnn=None, dtilde=None):
+                 conf_int=True, tol=1e-6, maxiter=50, dnn=None, dtilde=None, p2_design=None):
     """The observed statistic, the bootstrap t's, the p-value and the test-inverted interval."""
-    core = wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, dnn=dnn, dtilde=dtilde)
+    core = wcr11(xt, u, w, cl, G, bread, R, r, ssc, reps, seed, dnn=dnn, dtilde=dtilde, p2_design=p2_design)
     num_obs = float(R @ beta) - r
     point = num_obs + r
     v_obs = ssc * float(core["q_obs"] @ core["q_obs"])

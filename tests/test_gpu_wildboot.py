@@ -127,11 +127,94 @@ def test_wcr11_non_nested_projection_many_clusters():
     check(res, ref)
 
 
-def test_wcr11_per_cluster_projection_is_refused():
-    """More non-nested levels than clusters (<= 512): the reference's Path 2, not provided."""
+@pytest.mark.parametrize("with_firm", [True, False])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_wcr11_per_cluster_projection(with_firm, weighted):
+    """Path 2 (wildboot/mod.rs:1290-1363): a year term with more levels (40) than clusters (12)
+    crossing them, projected out of every draw one cluster at a time -- against the oracle's
+    Path 2 (one projection per target and cluster) and the brute-force refits."""
+    import fereg_amd as F
+    G, T, reps, r = 12, 40, 199, 0.5
+    R = np.array([0.0, 1.0])
+    y, X, cl, year, codes, gs, w = non_nested_design(700, G, T, 2, weighted, seed=17 + weighted, with_firm=with_firm)
+    announce("wcr11 per-cluster projection", with_firm, weighted)
+    ab = F.Absorber(codes, gs, weights=w)
+    pi = ab.row_order()
+    res = F.wildboot(y, X, cl.astype(np.uint32), R, r=r, reps=reps, seed=5, opts=F.SolveOpts(tol=1e-13),
+                     absorber=ab)
+    ab.close()
+    assert not res.fe_not_projected
+    yo, Xo, clo = y[pi], X[pi], cl[pi]
+    wo = None if w is None else w[pi]
+    Xt, u, beta, bread = fit_fwl(yo, Xo, [c[pi] for c in codes], gs, wo)
+    D = np.concatenate([dummies(c[pi], g) for c, g in zip(codes, gs)], axis=1)
+    ref = wb.wcr11_result(beta, Xt, u, wo, clo, G, bread, R, r, res.ssc, reps, 5, p2_design=D)
+    check(res, ref)
+    Dn = np.concatenate([dummies(c, g) for c, g in zip(codes, gs)], axis=1)
+    stat, se, tb = wb.brute_t_boot(X, y, w, Dn, cl, G, R, r, reps, 5, res.ssc)
+    np.testing.assert_allclose(res.t_boot, tb, rtol=1e-7, atol=1e-8)
+
+
+def test_wcr11_per_cluster_projection_many_levels():
+    """Path 1's case with more non-nested levels (30) than fit beside the regressors: the same
+    exact projection taken one cluster at a time, against the oracle's Path 1 and the refits."""
+    import fereg_amd as F
+    G, T, reps, r = 60, 30, 499, 0.0
+    R = np.array([1.0, -1.0, 0.0])
+    y, X, cl, year, codes, gs, w = non_nested_design(3000, G, T, 3, True, seed=41)
+    announce("wcr11 per-cluster projection, 30 crossing levels")
+    ab = F.Absorber(codes, gs, weights=w)
+    pi = ab.row_order()
+    res = F.wildboot(y, X, cl.astype(np.uint32), R, r=r, reps=reps, seed=9, opts=F.SolveOpts(tol=1e-13),
+                     absorber=ab)
+    ab.close()
+    assert not res.fe_not_projected
+    yo, Xo, clo, wo = y[pi], X[pi], cl[pi], w[pi]
+    Xt, u, beta, bread = fit_fwl(yo, Xo, [c[pi] for c in codes], gs, wo)
+    dnn, dt = path1_inputs(year[pi], T, clo, G, wo, True)
+    ref = wb.wcr11_result(beta, Xt, u, wo, clo, G, bread, R, r, res.ssc, reps, 9, dnn=dnn, dtilde=dt)
+    check(res, ref)
+    Dn = np.concatenate([dummies(c, g) for c, g in zip(codes, gs)], axis=1)
+    stat, se, tb = wb.brute_t_boot(X, y, w, Dn, cl, G, R, r, reps, 9, res.ssc)
+    np.testing.assert_allclose(res.t_boot, tb, rtol=1e-7, atol=1e-8)
+
+
+def test_wcr11_per_cluster_projection_at_scale():
+    """300 clusters, 400 crossing levels, 100k rows (the reference's Path 2 near its cluster
+    limit): against the oracle's Path 1 algebra on the same design (the same projection), with
+    the residualization by the C oracle."""
+    import fereg_amd as F
+    from oracle import estimate
+    from oracle.oracle import OracleSolver
+    G, T, reps, r = 300, 400, 999, 0.1
+    R = np.array([0.0, 1.0])
+    y, X, cl, year, codes, gs, w = non_nested_design(100_000, G, T, 2, False, seed=43)
+    announce("wcr11 per-cluster projection, G 300, 400 levels")
+    ab = F.Absorber(codes, gs)
+    pi = ab.row_order()
+    res = F.wildboot(y, X, cl.astype(np.uint32), R, r=r, reps=reps, seed=21, opts=F.SolveOpts(tol=1e-13),
+                     absorber=ab)
+    ab.close()
+    assert not res.fe_not_projected
+    s = OracleSolver(codes, gs)
+    assert np.array_equal(s.structure()["pi"], pi)
+    rows = np.ascontiguousarray(np.column_stack([y, X])[pi])
+    s.residualize(rows, rows.shape[1], tol=1e-13)
+    beta, bread, u, kept = estimate.ols_from_block(rows)
+    Xt, clo = rows[:, 1:], cl[pi]
+    dnn = dummies(year[pi], T)
+    cnt = np.bincount(clo, minlength=G).astype(float)
+    dt = dnn - (wb.cluster_sums(clo, G, dnn) / cnt[:, None])[clo]  # residuals on the nested firm term
+    ref = wb.wcr11_result(beta, Xt, u, None, clo, G, bread, R, r, res.ssc, reps, 21, dnn=dnn, dtilde=dt)
+    check(res, ref, rtol=1e-6)
+
+
+def test_wcr11_projection_beyond_both_forms_is_refused():
+    """More than 2048 clusters with 30 crossing levels beside the regressors: neither the appended
+    form (kept + levels + 2 > 21) nor the per-cluster one (G > 2048) is built for it."""
     import fereg_amd as F
     rng = np.random.default_rng(2)
-    n, G, T = 800, 30, 40
+    n, G, T = 8000, 2100, 30
     cl = rng.integers(0, G, n).astype(np.uint32)
     year = rng.integers(0, T, n).astype(np.uint32)
     X = rng.standard_normal((n, 1))

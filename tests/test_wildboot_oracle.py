@@ -156,6 +156,29 @@ def test_non_nested_projection_equals_brute_force_refit(with_firm, weighted):
     assert np.max(np.abs(plain["t_boot"] - tb)) > 1e-3
 
 
+@pytest.mark.parametrize("with_firm", [True, False])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_per_cluster_projection_equals_brute_force_refit(with_firm, weighted):
+    """Path 2: a year term with more levels (40) than clusters (12) crossing them, projected out of
+    every draw by one projection per (target, cluster); the t's equal the brute-force refits."""
+    G, T, k, reps = 12, 40, 2, 199
+    y, X, cl, year, codes, gs, w = non_nested_design(700, G, T, k, weighted, seed=17 + weighted, with_firm=with_firm)
+    Xt, u, beta, bread = fit_fwl(y, X, codes, gs, w)
+    R, r, ssc = np.array([0.0, 1.0]), 0.5, 1.07
+    D = np.concatenate([dummies(c, g) for c, g in zip(codes, gs)], axis=1)
+    res = wb.wcr11_result(beta, Xt, u, w, cl, G, bread, R, r, ssc, reps, seed=5, p2_design=D)
+    stat, se, tb = wb.brute_t_boot(X, y, w, D, cl, G, R, r, reps, 5, ssc)
+    assert abs(res["stat"] - stat) <= 1e-9 * abs(stat)
+    np.testing.assert_allclose(res["t_boot"], tb, rtol=1e-8, atol=1e-10)
+    # Path 1 on the same design (the year dummies residualized on the nested firm term) is the
+    # same projection
+    dnn, dt = path1_inputs(year, T, cl, G, w, with_firm)
+    p1 = wb.wcr11_result(beta, Xt, u, w, cl, G, bread, R, r, ssc, reps, seed=5, dnn=dnn, dtilde=dt)
+    np.testing.assert_allclose(p1["t_boot"], tb, rtol=1e-8, atol=1e-10)
+    plain = wb.wcr11_result(beta, Xt, u, w, cl, G, bread, R, r, ssc, reps, seed=5)
+    assert np.max(np.abs(plain["t_boot"] - tb)) > 1e-3
+
+
 def _year_design(n, G, seed, weighted):
     """Clusters of stores (nested) and a year term crossing the clusters."""
     rng = np.random.default_rng(seed)
