@@ -991,9 +991,11 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
                 dc[q] = up[q].p;
             }
         }
-        // code range check (structure.rs:483-489): every code below its level count
+        // code range check (structure.rs:483-489): every code below its level count; its flag is
+        // read while the codes are copied (no code indexes memory before the check has passed)
+        DBuf<uint32_t> bad(1);
+        AsyncRead bad_read(sizeof(uint32_t));
         {
-            DBuf<uint32_t> bad(1);
             FH_HIP(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), a.st));
             for (size_t q0 = 0; q0 < nterms; q0 += NB_MAX + 1) {
                 CodeCheck cc{};
@@ -1006,19 +1008,7 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
                                    a.st, cc, n, bad.p);
             }
             FH_CHECK_LAUNCH();
-            uint32_t hb = 0;
-            d2h(&hb, bad.p, 1, a.st);
-            stream_wait(a.st);
-            require(!hb, "a term code is not below its level count");
-        }
-        DBuf<double> wd;
-        const double* wk = nullptr;
-        if (w_kept) {
-            wd.alloc(n);
-            if (dev) FH_HIP(hipMemcpyAsync(wd.p, w_kept, n * sizeof(double), hipMemcpyDeviceToDevice, a.st));
-            else h2d(wd.p, w_kept, n, a.st);
-            require(weights_ok(a, wd.p), "weights must be positive and finite");
-            wk = wd.p;
+            bad_read.start(bad.p, a.st);
         }
         // the terms' codes stay with the absorber (the bootstrap's non-nested projection)
         a.codes_kept.resize(nterms);
@@ -1030,6 +1020,16 @@ static int create_impl(const fh_term* terms, size_t nterms, const double* w_kept
                 a.codes_kept[q] = std::move(up[q]);
             }
             dc[q] = a.codes_kept[q].p;
+        }
+        require(*static_cast<const uint32_t*>(bad_read.wait()) == 0, "a term code is not below its level count");
+        DBuf<double> wd;
+        const double* wk = nullptr;
+        if (w_kept) {
+            wd.alloc(n);
+            if (dev) FH_HIP(hipMemcpyAsync(wd.p, w_kept, n * sizeof(double), hipMemcpyDeviceToDevice, a.st));
+            else h2d(wd.p, w_kept, n, a.st);
+            require(weights_ok(a, wd.p), "weights must be positive and finite");
+            wk = wd.p;
         }
         for (size_t q = 0; q < nterms; q++) {
             a.term_icpt.push_back(terms[q].intercept == 1 ? 1 : 0);

@@ -219,13 +219,14 @@ __global__ void sorted_check(const uint64_t* __restrict__ keys, size_t n, uint32
     size_t stride = (size_t)gridDim.x * blockDim.x;
     bool bad = false;
     for (; i < n; i += stride) bad |= keys[i] < keys[i - 1];
-    // one flag write per workgroup (a write per wave contends on one address)
+    // one plain store of 1 per workgroup that saw a descent: every writer stores the same value,
+    // so no atomic is needed (atomics from every workgroup on one address serialize)
     __shared__ int any;
     if (threadIdx.x == 0) any = 0;
     __syncthreads();
-    if (__ballot(bad) != 0 && lane_id() == 0) atomicOr(&any, 1);
+    if (__ballot(bad) != 0 && lane_id() == 0) any = 1;
     __syncthreads();
-    if (threadIdx.x == 0 && any) atomicOr(flag, 1u);
+    if (threadIdx.x == 0 && any) *reinterpret_cast<volatile uint32_t*>(flag) = 1u;
 }
 
 }  // namespace

@@ -27,26 +27,26 @@ __global__ void k_hist(const uint32_t* __restrict__ codes, size_t n, uint32_t* _
         atomicAdd(&cnt[codes[i]], 1u);
 }
 
-// *out += the workgroup's sum of v (one atomic per workgroup: atomics from every wave on one
-// address serialize). Every thread calls it; blockDim.x <= 1024.
-__device__ void block_add_u64(unsigned long long v, unsigned long long* out) {
-    __shared__ unsigned long long part[1024 / WAVE];
+// part[blockIdx.x] = the workgroup's sum of v (no atomics: one atomic per workgroup on a single
+// address serializes over thousands of workgroups). Every thread calls it; blockDim.x <= 1024.
+__device__ void block_sum_u64(unsigned long long v, unsigned long long* part) {
+    __shared__ unsigned long long wpart[1024 / WAVE];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    if (lane_id() == 0) part[threadIdx.x / WAVE] = v;
+    if (lane_id() == 0) wpart[threadIdx.x / WAVE] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
-        for (unsigned w = 0; w < blockDim.x / WAVE; w++) t += part[w];
-        if (t) atomicAdd(out, t);
+        for (unsigned w = 0; w < blockDim.x / WAVE; w++) t += wpart[w];
+        part[blockIdx.x] = t;
     }
 }
 
-__global__ void k_sumsq(const uint32_t* __restrict__ v, size_t g, unsigned long long* out) {
+__global__ void k_sumsq(const uint32_t* __restrict__ v, size_t g, unsigned long long* part) {
     unsigned long long s = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < g; i += (size_t)gridDim.x * blockDim.x)
         s += (unsigned long long)v[i] * v[i];
-    block_add_u64(s, out);
+    block_sum_u64(s, part);
 }
 
 // key[i] = packed fields of row vals[i] (fields of the word: most significant first).
@@ -78,8 +78,10 @@ __global__ void k_fe1_offsets(const uint32_t* __restrict__ fe1, size_t n, uint64
     if (__any(bad) && lane_id() == 0) atomicOr(&st[0], 1u);
 }
 
-// st[1] = the longest level, off[s + 1] - off[s] (one atomic per workgroup).
-__global__ __launch_bounds__(256) void k_max_len(const uint32_t* __restrict__ off, size_t g1, uint32_t* __restrict__ st) {
+// part[blockIdx.x] = the longest level, off[s + 1] - off[s], among the workgroup's levels (no
+// atomics: one contended atomic per workgroup costs more than the whole read).
+__global__ __launch_bounds__(256) void k_max_len(const uint32_t* __restrict__ off, size_t g1,
+                                                 uint32_t* __restrict__ part) {
     __shared__ uint32_t red[256 / WAVE];
     uint32_t m = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < g1; i += (size_t)gridDim.x * blockDim.x)
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(256) void k_max_len(const uint32_t* __restrict__ of
     __syncthreads();
     if (threadIdx.x == 0) {
         for (unsigned w = 1; w < blockDim.x / WAVE; w++) m = max(m, red[w]);
-        atomicMax(&st[1], m);
+        part[blockIdx.x] = m;
     }
 }
 
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(SW_WG) void k_small_work(const uint32_t* __restrict
         __builtin_amdgcn_wave_barrier();
     }
     if (l != 0) acc = 0;  // every lane holds the wave's sum
-    block_add_u64(acc, out);
+    block_sum_u64(acc, out);
 }
 
 // One small block (block 0, codes ascending within a segment): distinct = runs of equal codes.
@@ -237,7 +239,7 @@ __global__ void k_small_work1(const uint32_t* __restrict__ seg_off, const uint32
         for (int o = L >> 1; o > 0; o >>= 1) d += (uint32_t)__shfl_xor((int)d, o, WAVE);
         if (li == 0) acc += (unsigned long long)d * d;
     }
-    block_add_u64(acc, out);
+    block_sum_u64(acc, out);
 }
 
 __global__ void k_level_keys(const uint32_t* __restrict__ cell_codes, size_t M, int nb, int b,
@@ -360,6 +362,15 @@ T read_scalar(const T* dev, hipStream_t st) {
     return v;
 }
 
+// The sum of n per-workgroup partials (exact: integers), read back.
+static unsigned long long read_part_sum(const DBuf<unsigned long long>& part, size_t n, hipStream_t st) {
+    std::vector<unsigned long long> h(n);
+    download(h.data(), part.p, n * sizeof(unsigned long long), st);
+    unsigned long long t = 0;
+    for (unsigned long long v : h) t += v;
+    return t;
+}
+
 }  // namespace
 
 // choose_elimination (structure.rs:256-313), with the histogram and E sums on the GPU.
@@ -377,13 +388,14 @@ static int64_t choose_fe1(Absorber& a, const std::vector<const uint32_t*>& dc) {
     if (cands.size() < 2) return f0;
     hipStream_t st = a.st;
     DBuf<uint32_t> cnt(std::max<uint64_t>(g[f0], 1));
-    DBuf<unsigned long long> acc(1);
+    uint64_t gmax = g[f0];
+    for (size_t q : cands) gmax = std::max<uint64_t>(gmax, g[q]);
+    DBuf<unsigned long long> acc(blocks_for(gmax));
     FH_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), st));
-    FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
     hipLaunchKernelGGL(k_hist, dim3(blocks_for(n)), dim3(TPB), 0, st, dc[f0], n, cnt.p);
     hipLaunchKernelGGL(k_sumsq, dim3(blocks_for(g[f0])), dim3(TPB), 0, st, cnt.p, (size_t)g[f0], acc.p);
     FH_CHECK_LAUNCH();
-    unsigned long long ssq = read_scalar(acc.p, st);
+    unsigned long long ssq = read_part_sum(acc, blocks_for(g[f0]), st);
     if ((nt - 1) * ssq <= 64ull * n) return f0;
     DBuf<uint64_t> keys(n), kalt(n);
     DBuf<uint32_t> vals(n), valt(n);
@@ -400,11 +412,10 @@ static int64_t choose_fe1(Absorber& a, const std::vector<const uint32_t*>& dc) {
             FH_CHECK_LAUNCH();
             radix_sort_pairs(keys, vals, kalt, valt, n, (int)(bits_for(g[f]) + bo), st);
             FH_HIP(hipMemsetAsync(d.p, 0, d.bytes(), st));
-            FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
             hipLaunchKernelGGL(k_distinct, dim3(blocks_for(n)), dim3(TPB), 0, st, keys.p, n, bo, d.p);
             hipLaunchKernelGGL(k_sumsq, dim3(blocks_for(g[f])), dim3(TPB), 0, st, d.p, (size_t)g[f], acc.p);
             FH_CHECK_LAUNCH();
-            e += read_scalar(acc.p, st);
+            e += read_part_sum(acc, blocks_for(g[f]), st);
         }
         // key (E, Reverse(g), f): smaller E, then more levels, then lower index
         if (best < 0 || e < bestE || (e == bestE && g[f] > g[best])) {
@@ -417,22 +428,32 @@ static int64_t choose_fe1(Absorber& a, const std::vector<const uint32_t*>& dc) {
 
 // Rows not yet in (FE1, blocks) order but with FE1 ascending in kept order and no FE1 level
 // longer than 64 rows (panels stored unit by unit, periods unordered): each level's rows are
-// sorted in registers by their block codes instead of a global radix sort. False when the input
-// does not qualify (the caller sorts globally).
-static bool level_rank_sort(Absorber& a, const std::vector<const uint32_t*>& dc,
-                            const std::vector<const uint32_t*>& fields, const std::vector<unsigned>& fbits,
-                            hipStream_t st) {
+// The row order of a single-word key: true when the packed (FE1, blocks) keys are already sorted
+// (one check and readback); else, when FE1 ascends in kept order with no level above 64 rows, the
+// rows are sorted per level in registers by their block codes (k_level_rank_sort), which replaces
+// the global radix sort (one more pass and readback). Returns true when pi holds the sorted order.
+static bool order_single_word(Absorber& a, const std::vector<const uint32_t*>& dc, const uint64_t* keys,
+                              const std::vector<const uint32_t*>& fields, const std::vector<unsigned>& fbits,
+                              hipStream_t st) {
     const size_t n = a.n;
+    constexpr unsigned ML_BLOCKS = 256;
+    DBuf<uint32_t> info(2 + ML_BLOCKS);
+    check_sorted_u64(keys, n, info.p, st);
+    if (read_scalar(info.p, st) == 0) return true;  // already in order
     if (a.fe1 < 0 || fields.size() < 2) return false;
-    DBuf<uint32_t> off(a.g1 + 1), st2(2);
-    FH_HIP(hipMemsetAsync(st2.p, 0, st2.bytes(), st));
+    DBuf<uint32_t> off(a.g1 + 1);
+    FH_HIP(hipMemsetAsync(info.p + 1, 0, sizeof(uint32_t), st));
+    const unsigned nml = grid_for(a.g1, TPB, ML_BLOCKS);
     hipLaunchKernelGGL(k_fe1_offsets, dim3(blocks_for(n + 1)), dim3(TPB), 0, st, dc[a.fe1], n, (uint64_t)a.g1, off.p,
-                       st2.p);
-    hipLaunchKernelGGL(k_max_len, dim3(grid_for(a.g1, TPB, 256)), dim3(TPB), 0, st, off.p, (size_t)a.g1, st2.p);
+                       info.p + 1);
+    hipLaunchKernelGGL(k_max_len, dim3(nml), dim3(TPB), 0, st, off.p, (size_t)a.g1, info.p + 2);
     FH_CHECK_LAUNCH();
-    uint32_t hs[2] = {1, 0};
-    download(hs, st2.p, sizeof(hs), st);
-    if (hs[0] != 0 || hs[1] > 64) return false;
+    std::vector<uint32_t> hs(1 + nml);
+    download(hs.data(), info.p + 1, hs.size() * sizeof(uint32_t), st);
+    if (hs[0] != 0) return false;
+    uint32_t maxlen = 0;
+    for (unsigned b = 0; b < nml; b++) maxlen = std::max(maxlen, hs[1 + b]);
+    if (maxlen > 64) return false;
     CodeSet cb{};
     unsigned sh = 0;
     cb.nq = (int)fields.size() - 1;
@@ -446,8 +467,8 @@ static bool level_rank_sort(Absorber& a, const std::vector<const uint32_t*>& dc,
         hipLaunchKernelGGL((k_level_rank_sort<L>), dim3(grid_for(a.g1, (TPB / WAVE) * (WAVE / L), 1u << 15)), dim3(TPB),
                            0, st, cb, off.p, (uint64_t)a.g1, a.pi.p);
     };
-    if (hs[1] <= 16) go(std::integral_constant<int, 16>{});
-    else if (hs[1] <= 32) go(std::integral_constant<int, 32>{});
+    if (maxlen <= 16) go(std::integral_constant<int, 16>{});
+    else if (maxlen <= 32) go(std::integral_constant<int, 32>{});
     else go(std::integral_constant<int, 64>{});
     FH_CHECK_LAUNCH();
     return true;
@@ -523,7 +544,6 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         }
         DBuf<uint64_t> keys(n), kalt(n);
         DBuf<uint32_t> valt(n);
-        DBuf<uint32_t> flag(1);
         bool first = true;
         for (auto [b, e2] : words) {
             CodeSet cs{};
@@ -539,11 +559,7 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
             hipLaunchKernelGGL(k_pack, dim3(blocks_for(n)), dim3(TPB), 0, st, cs, first ? nullptr : a.pi.p, n, keys.p);
             FH_CHECK_LAUNCH();
             bool sorted = false;
-            if (first && words.size() == 1) {
-                check_sorted_u64(keys.p, n, flag.p, st);
-                sorted = read_scalar(flag.p, st) == 0;
-                if (!sorted && level_rank_sort(a, dc, fields, fbits, st)) sorted = true;
-            }
+            if (first && words.size() == 1) sorted = order_single_word(a, dc, keys.p, fields, fbits, st);
             if (!sorted) radix_sort_pairs(keys, a.pi, kalt, valt, n, (int)width, st);
             first = false;
         }
@@ -587,37 +603,41 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         a.cell_obs_off.release();
     }
 
-    // sum_seg s^2 of the small coordinates, then the demotion rule (structure.rs:521-525)
+    // sum_seg s^2 of the small coordinates, then the demotion rule (structure.rs:521-525); the
+    // partials are read back while the large blocks' transposes (which the rule does not change)
+    // run on the stream
     a.small_work = 0;
     int nsb = 0;
     for (auto& b : a.blocks) nsb += b.small ? 1 : 0;
-    if (a.ms > 0 && M > 0) {
+    const bool want_work = a.ms > 0 && M > 0;
+    unsigned work_grid = 0;
+    DBuf<unsigned long long> work_part;
+    DBuf<uint32_t> dboff;
+    if (want_work) {
         int words = (int)((a.ms + 31) / 32);
         require(words * 4 * (SW_WG / WAVE) <= 64 * 1024, "small_limit above 131072 levels is not supported");
         std::vector<uint32_t> boff(nsb);
         for (int b = 0; b < nsb; b++) boff[b] = (uint32_t)a.blocks[b].off;
-        DBuf<uint32_t> dboff(nsb);
+        dboff.alloc(nsb);
         upload(dboff.p, boff.data(), nsb * sizeof(uint32_t), st);
-        DBuf<unsigned long long> acc(1);
-        FH_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long), st));
+        work_part.alloc(4096);
         if (nsb == 1) {
             const int L = (int)seg_lanes_for(M, a.g1);
-            hipLaunchKernelGGL(k_small_work1, dim3(grid_for(a.g1, (TPB / WAVE) * (WAVE / L), 2048)), dim3(TPB), 0, st,
-                               a.seg_off.p, a.cell_codes.p, (size_t)a.g1, nb, L, acc.p);
+            work_grid = grid_for(a.g1, (TPB / WAVE) * (WAVE / L), 2048);
+            hipLaunchKernelGGL(k_small_work1, dim3(work_grid), dim3(TPB), 0, st, a.seg_off.p, a.cell_codes.p,
+                               (size_t)a.g1, nb, L, work_part.p);
         } else {
-            unsigned grid = grid_for(a.g1, SW_WG / WAVE, 4096);
-            hipLaunchKernelGGL(k_small_work, dim3(grid), dim3(SW_WG), words * 4 * (SW_WG / WAVE), st, a.seg_off.p,
-                               a.cell_codes.p, (size_t)a.g1, nb, nsb, dboff.p, words, acc.p);
+            work_grid = grid_for(a.g1, SW_WG / WAVE, 4096);
+            hipLaunchKernelGGL(k_small_work, dim3(work_grid), dim3(SW_WG), words * 4 * (SW_WG / WAVE), st,
+                               a.seg_off.p, a.cell_codes.p, (size_t)a.g1, nb, nsb, dboff.p, words, work_part.p);
         }
         FH_CHECK_LAUNCH();
-        a.small_work = read_scalar(acc.p, st);
-        if (a.small_work > 64ull * M) {
-            for (auto& b : a.blocks) b.small = false;
-            a.ms = 0;
-        }
     }
+    AsyncRead work_read(std::max<size_t>(work_grid, 1) * sizeof(unsigned long long));
+    if (want_work) work_read.start(work_part.p, st);
 
-    // bucketed level-major transpose of every block
+    // bucketed level-major transpose of every block: the large blocks now, the small ones once the
+    // rule has decided whether they stay small
     a.lperm.clear(); a.loff.clear(); a.lseg.clear(); a.lother.clear(); a.bperm.clear(); a.boff.clear();
     a.lperm.resize(nb); a.loff.resize(nb); a.lseg.resize(nb); a.lother.resize(nb); a.bperm.resize(nb);
     a.boff.resize(nb);
@@ -625,12 +645,23 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
     a.bwidth.assign(nb, 0);
     a.blanes.assign(nb, 1);
     a.llanes.assign(nb, 1);
-    // the single small block of the dense small-row operator never runs a level pass; its
-    // transpose is built only on request (ensure_block_transpose)
-    const bool skip_small = small_dense_rule(a);
-    for (int b = 0; b < nb; b++) {
-        if (skip_small && a.blocks[b].small) continue;
-        build_block_transpose(a, b);
+    for (int b = 0; b < nb; b++)
+        if (!a.blocks[b].small) build_block_transpose(a, b);
+    if (want_work) {
+        const unsigned long long* hp = static_cast<const unsigned long long*>(work_read.wait());
+        for (unsigned i = 0; i < work_grid; i++) a.small_work += hp[i];
+    }
+    if (want_work && a.small_work > 64ull * M) {
+        for (auto& b : a.blocks) b.small = false;
+        a.ms = 0;
+        for (int b = 0; b < nb; b++)
+            if (a.bperm[b].n == 0) build_block_transpose(a, b);  // the demoted blocks
+    } else {
+        // the single small block of the dense small-row operator never runs a level pass; its
+        // transpose is built only on request (ensure_block_transpose)
+        const bool skip_small = small_dense_rule(a);
+        for (int b = 0; b < nb; b++)
+            if (a.blocks[b].small && !skip_small) build_block_transpose(a, b);
     }
     a.seg_lanes = seg_lanes_for(M, a.g1);
 }
