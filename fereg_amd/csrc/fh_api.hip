@@ -1588,10 +1588,10 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
             else require(spec->r_mat[j] == 0.0, "the hypothesis involves a regressor dropped as collinear");
         }
         // fixed effects not nested in the clusters (wildboot/mod.rs:959-997) are projected out of
-        // the draws exactly: as appended pseudo-regressors when their levels number at most
-        // min(G, 512) and fit beside the kept regressors (Path 1), else one cluster at a time
-        // (Path 2's projection) when G <= 512, or G <= 2048 for the levels Path 1 would take;
-        // with more than 512 of both the projection is skipped, as the reference does
+        // the draws exactly. At most min(G, 512) levels (Path 1): as appended pseudo-regressors
+        // when they fit beside the kept regressors, else through Path 1's algebra with the draws as
+        // FP64 products (m_nn / 16 solves). More levels with G <= 512 (Path 2): one cluster at a
+        // time (G / 16 solves). More than 512 of both: skipped, as the reference does.
         std::vector<int> nest = terms_nested_in(a, pc);
         uint64_t m_nn = 0;
         for (size_t q = 0; q < a.nterms; q++)
@@ -1599,12 +1599,10 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
         const bool ref_path1 = m_nn > 0 && m_nn <= std::min<uint64_t>(G, 512);
         const bool ref_path2 = m_nn > 0 && !ref_path1 && G <= 512;
         const bool path1 = ref_path1 && os.kept.size() + m_nn + 2 <= 21;
-        const bool cells = (ref_path1 && !path1 && G <= 2048) || ref_path2;
+        const bool cells = ref_path2;
+        const bool wide = ref_path1 && !path1;
         const int bt = spec->boot_type == 0 ? 11 : spec->boot_type;
         require(bt == 11 || bt == 13 || bt == 31 || bt == 33, "wildboot: boot_type must be 11, 13, 31 or 33");
-        require(bt != 11 || !ref_path1 || path1 || cells,
-                "fixed effects not nested in more than 2048 clusters with more levels than fit beside the "
-                "regressors (kept + levels + 2 > 21) are not projected by this build");
         WbootInput in;
         in.rows = a.rows_tmp.p;
         in.R = os.ld;
@@ -1633,6 +1631,11 @@ static int wildboot_impl(fh_absorber* ab, const double* y, const double* X, size
         }
         if (cells) {
             ols_wildboot_cells(a, in, *spec, rk.data(), o->tol, o->maxiter, *res, t_boot);
+            res->fe_not_projected = false;
+            return;
+        }
+        if (wide) {
+            ols_wildboot_wide(a, nest, m_nn, in, *spec, rk.data(), *res, t_boot);
             res->fe_not_projected = false;
             return;
         }

@@ -490,7 +490,12 @@ void wb_path1(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, WbootI
 uint64_t count_nonempty_clusters(Absorber& a, const uint32_t* cl, uint64_t G);
 void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* r_mat_kept,
                   fh_wboot_result& res, double* t_boot_host);
-// WCR11 with every fixed effect projected out of the draws one cluster at a time (2 .. 2048
+// WCR11 with the non-nested terms (m_nn <= 512 levels) projected out through Path 1's algebra with
+// the draws as two FP64 matrix products (any number of clusters).
+void ols_wildboot_wide(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, const WbootInput& in,
+                       const fh_wboot_spec& spec, const double* r_mat_kept, fh_wboot_result& res,
+                       double* t_boot_host);
+// WCR11 with every fixed effect projected out of the draws one cluster at a time (2 .. 512
 // clusters): one residualization (tolerance min(tol, 1e-10), at most maxiter iterations) per
 // cluster, then the same contraction over G x (2 + 2G) per-cluster columns.
 void ols_wildboot_cells(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* r_mat_kept,

@@ -22,6 +22,8 @@
 
 #include "fh_fit.hpp"
 
+#include <rocblas/rocblas.h>
+
 namespace fh {
 
 namespace {
@@ -31,7 +33,7 @@ constexpr int LIMBS = 7;
 constexpr int WB_NTC = 19;    // 16-column tiles per GEMM workgroup
 constexpr int WB_ROWS = 128;  // draws per GEMM workgroup (4 waves x 2 tiles of 16)
 constexpr int KMAX = 21;
-constexpr size_t PC_GMAX = 2048;  // clusters of the per-cluster projection
+constexpr size_t PC_GMAX = 512;  // clusters of the per-cluster projection (the reference's Path 2 limit)
 constexpr uint64_t SM_GAMMA = 0x9E3779B97F4A7C15ull;
 constexpr uint64_t SM_STREAM = 0xD1B54A32D192ED03ull;
 
@@ -939,6 +941,44 @@ void wb_cluster_sums(Absorber& a, const double* rows, int R, const std::vector<i
     FH_CHECK_LAUNCH();
 }
 
+// The inference from the draws' affine coefficients (reps x 5, host): the observed statistic,
+// the bootstrap t's, the p-value and the test-inverted interval (ols.rs:553-649).
+void wb_infer(const std::vector<double>& hc, uint64_t reps, bool full, size_t G, double point, double v_obs,
+              double ssc, const fh_wboot_spec& spec, fh_wboot_result& res, double* t_boot_host,
+              std::chrono::steady_clock::time_point t0) {
+    const double num_obs = point - spec.r;
+    const double se = (std::isfinite(v_obs) && v_obs > 1e-300) ? std::sqrt(v_obs) : NAN;
+    const double stat = std::isfinite(se) ? num_obs / se : NAN;
+    const double delta = spec.r - point;
+    std::vector<double> tb(reps);
+    for (uint64_t b = 0; b < reps; b++) tb[b] = eval_affine_t(delta, &hc[b * 5]);
+    auto pv = compute_pvalue(stat, tb, spec.pval_type);
+    res.stat = stat;
+    res.p_val = pv.first;
+    res.reps_feasible = pv.second;
+    res.se = se;
+    res.point = point;
+    res.ssc = ssc;
+    res.reps = reps;
+    res.n_clusters = G;
+    res.full_enumeration = full;
+    res.ci_lo = NAN;
+    res.ci_hi = NAN;
+    if (spec.conf_int) {
+        std::vector<double> trial(reps);
+        auto ci = invert_ci_search(point, se, 1.0 - spec.level, spec.pval_type, spec.tol, spec.maxiter, pv.second,
+                                   [&](double r_trial) {
+                                       const double d = r_trial - point, st_trial = (point - r_trial) / se;
+                                       for (uint64_t b = 0; b < reps; b++) trial[b] = eval_affine_t(d, &hc[b * 5]);
+                                       return compute_pvalue(st_trial, trial, spec.pval_type).first;
+                                   });
+        res.ci_lo = ci.first;
+        res.ci_hi = ci.second;
+    }
+    if (t_boot_host) std::memcpy(t_boot_host, tb.data(), reps * sizeof(double));
+    res.seconds_bootstrap = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // The draws and the inference shared by WCR11 and the MNW variants: the per-cluster rows P
 // (device, G x NC with NC = 2 + 4 mk: [num_0, num_1, Phi_0, Phi_1, A_0 B, A_1 B] and their column
 // scales), AA (the per-cluster scalars' Gram, 2 x 2) and BB (sum B B', mk x mk) give every draw's
@@ -951,9 +991,6 @@ void wb_draws(Absorber& a, const double* P, size_t G, int NC, int mk, const std:
               double ssc, double point, double v_obs, const fh_wboot_spec& spec, fh_wboot_result& res,
               double* t_boot_host, std::chrono::steady_clock::time_point t0, size_t cells = 0) {
     hipStream_t st = a.st;
-    const double num_obs = point - spec.r;
-    const double se = (std::isfinite(v_obs) && v_obs > 1e-300) ? std::sqrt(v_obs) : NAN;
-    const double stat = std::isfinite(se) ? num_obs / se : NAN;
     // draws
     uint64_t reps = spec.reps;
     bool full = false;
@@ -1008,35 +1045,7 @@ void wb_draws(Absorber& a, const double* P, size_t G, int NC, int mk, const std:
     res.contraction_ms = gms;
     // int8 multiply-adds (2 ops each) of the draws x clusters x limb-columns product, padding included
     res.contraction_ops = 2.0 * (double)Bpad * (double)Gpad * (double)ncolp;
-    // inference (ols.rs:553-649)
-    const double delta = spec.r - point;
-    std::vector<double> tb(reps);
-    for (uint64_t b = 0; b < reps; b++) tb[b] = eval_affine_t(delta, &hc[b * 5]);
-    auto pv = compute_pvalue(stat, tb, spec.pval_type);
-    res.stat = stat;
-    res.p_val = pv.first;
-    res.reps_feasible = pv.second;
-    res.se = se;
-    res.point = point;
-    res.ssc = ssc;
-    res.reps = reps;
-    res.n_clusters = G;
-    res.full_enumeration = full;
-    res.ci_lo = NAN;
-    res.ci_hi = NAN;
-    if (spec.conf_int) {
-        std::vector<double> trial(reps);
-        auto ci = invert_ci_search(point, se, 1.0 - spec.level, spec.pval_type, spec.tol, spec.maxiter, pv.second,
-                                   [&](double r_trial) {
-                                       const double d = r_trial - point, st_trial = (point - r_trial) / se;
-                                       for (uint64_t b = 0; b < reps; b++) trial[b] = eval_affine_t(d, &hc[b * 5]);
-                                       return compute_pvalue(st_trial, trial, spec.pval_type).first;
-                                   });
-        res.ci_lo = ci.first;
-        res.ci_hi = ci.second;
-    }
-    if (t_boot_host) std::memcpy(t_boot_host, tb.data(), reps * sizeof(double));
-    res.seconds_bootstrap = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    wb_infer(hc, reps, full, G, point, v_obs, ssc, spec, res, t_boot_host, t0);
 }
 
 void ols_wildboot(Absorber& a, const WbootInput& in, const fh_wboot_spec& spec, const double* rk, fh_wboot_result& res,
@@ -1221,7 +1230,7 @@ void ols_wildboot_cells(Absorber& a, const WbootInput& in, const fh_wboot_spec& 
     const size_t G = in.G, n = a.n;
     const int mk = (int)in.kept.size();
     require(mk >= 1 && mk + 2 <= KMAX, "the bootstrap supports 1..19 kept regressors");
-    require(G >= 2 && G <= PC_GMAX, "the per-cluster projection supports 2 .. 2048 clusters");
+    require(G >= 2 && G <= PC_GMAX, "the per-cluster projection supports 2 .. 512 clusters");
     require(spec.reps >= 1, "wildboot: reps must be at least 1");
     require(spec.level > 0.0 && spec.level < 1.0, "wildboot: confidence level must be in (0, 1)");
     require(spec.tol > 0.0 && std::isfinite(spec.tol) && spec.maxiter > 0, "wildboot: tol and maxiter must be positive");
@@ -1289,6 +1298,325 @@ void ols_wildboot_cells(Absorber& a, const WbootInput& in, const fh_wboot_spec& 
     const std::vector<double> none;
     wb_draws(a, P.p, G, NC, mk, scale, inv_scale, none, none, in.ssc, point, in.ssc * aa, spec, res, t_boot_host, t0,
              G);
+}
+
+// ------------------------------------------------------------------ Path 1 with many levels
+// Path 1 (wildboot/mod.rs:959-1180, ols.rs:413-448) when the non-nested levels do not fit beside the
+// regressors (any number of clusters). With p = kept + m_nn,
+// Phi~_t[g] = [bread S_t[g], G_nn^- F_t[g]] and B~_g = [B_g, f_g], a draw's cell scores are
+//   q_t[c] = V_c A_t[c] - B~_c . d_t,  d_t = sum_g V_g Phi~_t[g],
+// so the draws are two plain FP64 products: Y = V [A_0, A_1, Phi~_0, Phi~_1] (reps x G times
+// G x (2 + 2p)) and Z_t = d_t B~' (reps x p times p x G), by rocBLAS with atomics off (fixed
+// order), the signs V materialized from the same SplitMix64 stream; a wave per draw then sums
+// its cells' squares. Any generalized inverse G_nn^- gives the same projection (F_t and f lie in
+// the range of G_nn); the grounded LDL' inverse is used (O(m^3), no eigen-decomposition).
+
+namespace {
+
+// WS[g][t][j], t = 0, 1: sum of w u_t, t = 2: sum of w z over the rows of cluster g whose level in
+// a non-nested term is column j (= the term's offset + its code): one wave per (cluster, chunk of
+// 64 columns), lanes over the columns, the cluster's runs in row order.
+__global__ __launch_bounds__(TPB) void k_wide_sums(const uint32_t* __restrict__ off, const uint32_t* __restrict__ order,
+                                                   const uint32_t* __restrict__ run_off, size_t G,
+                                                   const double2* __restrict__ wu, const double* __restrict__ z,
+                                                   const double* __restrict__ w, const uint32_t* __restrict__ pi,
+                                                   NnTerms nt, int m, double* __restrict__ WS) {
+    const int lane = lane_id();
+    const size_t nch = (size_t)(m + WAVE - 1) / WAVE, tasks = G * nch;
+    for (size_t task = (size_t)blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE; task < tasks;
+         task += (size_t)gridDim.x * (TPB / WAVE)) {
+        const size_t g = task / nch;
+        const int j = (int)(task % nch) * WAVE + lane;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (uint32_t k = off[g]; k < off[g + 1]; k++) {
+            const uint32_t run = order[k];
+            for (uint32_t i = run_off[run]; i < run_off[run + 1]; i++) {
+                const uint32_t kr = pi[i];
+                bool hit = false;
+                for (int q = 0; q < nt.nq; q++) hit |= (int)(nt.off[q] + nt.codes[q][kr]) == j;
+                if (hit) {
+                    const double2 u = wu[i];
+                    s0 += u.x;
+                    s1 += u.y;
+                    s2 += (w ? w[i] : 1.0) * z[i];
+                }
+            }
+        }
+        if (j < m) {
+            WS[(g * 3 + 0) * m + j] = s0;
+            WS[(g * 3 + 1) * m + j] = s1;
+            WS[(g * 3 + 2) * m + j] = s2;
+        }
+    }
+}
+
+// Pm[g] = [C.S_0, C.S_1, bread S_0 (mk, then m left for G^- F_0), bread S_1 (mk, then m)] (row
+// stride NC1 = 2 + 2p) and Bt[g] = [B_g, f_g] (p).
+__global__ void k_wide_rows(const double* __restrict__ S, const double* __restrict__ WS, size_t G, int mk, int m,
+                            const double* __restrict__ bread, const double* __restrict__ C, double* __restrict__ Pm,
+                            double* __restrict__ Bt) {
+    const int p = mk + m, NC1 = 2 + 2 * p;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < G; g += (size_t)gridDim.x * blockDim.x) {
+        double* row = Pm + g * NC1;
+        for (int t = 0; t < 2; t++) {
+            const double* St = S + (g * 3 + t) * mk;
+            double num = 0.0;
+            for (int a = 0; a < mk; a++) {
+                double f = 0.0;
+                for (int b = 0; b < mk; b++) f += bread[a * mk + b] * St[b];
+                row[2 + t * p + a] = f;
+                num += C[a] * St[a];
+            }
+            row[t] = num;
+        }
+        double* bt = Bt + g * p;
+        for (int a = 0; a < mk; a++) bt[a] = S[(g * 3 + 2) * mk + a];
+        for (int j = 0; j < m; j++) bt[mk + j] = WS[(g * 3 + 2) * m + j];
+    }
+}
+
+// V[b][g] = the Rademacher sign of cluster g in draw b (weights.rs:65-80): bit g mod 64 of word
+// g / 64 of the draw's SplitMix64 stream, 0 -> +1 (full enumeration: bit g of b).
+__global__ void k_wide_signs(uint64_t seed, int full, uint64_t reps, size_t G, double* __restrict__ V) {
+    const size_t words = (G + 63) / 64;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < reps * words; e += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t b = e / words;
+        const size_t s = e % words;
+        const uint64_t word = full ? (s == 0 ? b : 0ull) : sm_mix((seed ^ (b * SM_STREAM)) + (uint64_t)(s + 2) * SM_GAMMA);
+        const size_t g1 = min(G, s * 64 + 64);
+        for (size_t g = s * 64; g < g1; g++) V[b * G + g] = ((word >> (g - s * 64)) & 1ull) ? -1.0 : 1.0;
+    }
+}
+
+// Per draw (one wave): num_t = Y[b][t], q_t[c] = V[b][c] A_t[c] - Z_t[b][c], v = ssc sum_c q q'.
+__global__ __launch_bounds__(TPB) void k_wide_post(const double* __restrict__ Y, int NC1, const double* __restrict__ V,
+                                                   const double* __restrict__ Pm, const double* __restrict__ Z0,
+                                                   const double* __restrict__ Z1, size_t G, uint64_t reps, double ssc,
+                                                   double* __restrict__ coefs) {
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    for (uint64_t b = (uint64_t)blockIdx.x * (TPB / WAVE) + w; b < reps; b += (uint64_t)gridDim.x * (TPB / WAVE)) {
+        double s00 = 0.0, s01 = 0.0, s11 = 0.0;
+        for (size_t c = lane; c < G; c += WAVE) {
+            const double v = V[b * G + c];
+            const double q0 = v * Pm[c * NC1] - Z0[b * G + c];
+            const double q1 = v * Pm[c * NC1 + 1] - Z1[b * G + c];
+            s00 += q0 * q0;
+            s01 += q0 * q1;
+            s11 += q1 * q1;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            s00 += __shfl_xor(s00, o, WAVE);
+            s01 += __shfl_xor(s01, o, WAVE);
+            s11 += __shfl_xor(s11, o, WAVE);
+        }
+        if (lane == 0) {
+            double* o = coefs + b * 5;
+            o[0] = Y[b * NC1];
+            o[1] = Y[b * NC1 + 1];
+            o[2] = ssc * s00;
+            o[3] = ssc * s01;
+            o[4] = ssc * s11;
+        }
+    }
+}
+
+__global__ void k_scale_rows_sqrt(double* __restrict__ D, size_t n, int m, const double* __restrict__ w) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * m; e += (size_t)gridDim.x * blockDim.x)
+        D[e] *= sqrt(w[e / m]);
+}
+
+void blas_ok(rocblas_status s, const char* what) {
+    if (s != rocblas_status_success)
+        throw Error(FH_INTERNAL, std::string("rocBLAS ") + what + " failed (status " + std::to_string((int)s) + ")");
+}
+
+// A rocBLAS handle on the absorber's stream, atomics off (results independent of scheduling).
+struct Blas {
+    rocblas_handle h = nullptr;
+    explicit Blas(hipStream_t st) {
+        blas_ok(rocblas_create_handle(&h), "create_handle");
+        blas_ok(rocblas_set_stream(h, st), "set_stream");
+        blas_ok(rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed), "set_atomics_mode");
+    }
+    ~Blas() {
+        if (h) (void)rocblas_destroy_handle(h);
+    }
+    Blas(const Blas&) = delete;
+    Blas& operator=(const Blas&) = delete;
+    // row-major C (rows x cols, ldc) = op(A) op(B) through the column-major identity C' = B' A'
+    void gemm(rocblas_operation ta, rocblas_operation tb, int m, int n, int k, const double* A, int lda,
+              const double* B, int ldb, double* C, int ldc) {
+        const double one = 1.0, zero = 0.0;
+        blas_ok(rocblas_dgemm(h, ta, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc), "dgemm");
+    }
+};
+
+}  // namespace
+
+void ols_wildboot_wide(Absorber& a, const std::vector<int>& nested, uint64_t m_nn, const WbootInput& in,
+                       const fh_wboot_spec& spec, const double* rk, fh_wboot_result& res, double* t_boot_host) {
+    hipStream_t st = a.st;
+    const size_t G = in.G, n = a.n;
+    const int mk = (int)in.kept.size(), m = (int)m_nn, p = mk + m, NC1 = 2 + 2 * p;
+    require(mk >= 1 && mk + 2 <= KMAX, "the bootstrap supports 1..19 kept regressors");
+    require(m >= 1 && m <= 512, "the non-nested projection supports 1..512 levels");
+    require(G >= 2 && G < (1ull << 24), "the bootstrap needs 2 .. 2^24 clusters");
+    require(n < (1ull << 31), "the non-nested projection with many levels supports < 2^31 rows");
+    require(spec.reps >= 1, "wildboot: reps must be at least 1");
+    require(spec.level > 0.0 && spec.level < 1.0, "wildboot: confidence level must be in (0, 1)");
+    require(spec.tol > 0.0 && std::isfinite(spec.tol) && spec.maxiter > 0, "wildboot: tol and maxiter must be positive");
+    require(spec.pval_type >= 0 && spec.pval_type <= 3, "wildboot: unknown p-value type");
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> C, MR;
+    wb_constraint(in.bread, mk, rk, C, MR);
+    DBuf<double> Sown;
+    const double* Sp = in.S;
+    if (!Sp) {
+        wb_cluster_sums(a, in.rows, in.R, in.kept, C, MR, in.e, in.es, in.w, in.cl, G, Sown);
+        Sp = Sown.p;
+    }
+    NnTerms nt{};
+    uint32_t noff = 0;
+    bool any_nested = false;
+    for (size_t q = 0; q < a.nterms; q++) {
+        if (nested[q]) {
+            any_nested = true;
+            continue;
+        }
+        require(nt.nq < 8, "at most 8 non-nested terms");
+        require(a.codes_kept.size() == a.nterms && a.codes_kept[q].p, "the absorber has no term codes", FH_INTERNAL);
+        nt.codes[nt.nq] = a.codes_kept[q].p;
+        nt.off[nt.nq++] = noff;
+        noff += (uint32_t)a.term_g[q];
+    }
+    Blas blas(st);
+    // G_nn = D~' W D~ over the dummies, residualized on the nested terms when there are any
+    std::vector<double> Gnn((size_t)m * m);
+    {
+        DBuf<double> D(n * m), dG((size_t)m * m);
+        const double* wg = in.w;
+        fh_absorber* tmp = nullptr;
+        struct Tmp {
+            fh_absorber* p;
+            ~Tmp() {
+                if (p) fh_absorber_destroy(p);
+            }
+        } hold{nullptr};
+        if (!any_nested) {
+            hipLaunchKernelGGL(k_nn_dummies, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, n, a.pi.p, nt, m, D.p);
+            FH_CHECK_LAUNCH();
+        } else {
+            std::vector<fh_term> tt;
+            for (size_t q = 0; q < a.nterms; q++)
+                if (nested[q]) tt.push_back(fh_term{a.codes_kept[q].p, a.term_g[q], 1});
+            DBuf<double> wk;
+            if (in.w) {
+                wk.alloc(n);
+                hipLaunchKernelGGL(k_to_kept, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, in.w, a.pi.p, n, wk.p);
+                FH_CHECK_LAUNCH();
+            }
+            stream_wait(st);
+            fh_solver_config cfg;
+            fh_config_default(&cfg);
+            if (fh_absorber_create_dev(tt.data(), tt.size(), in.w ? wk.p : nullptr, n, &cfg, nullptr, &tmp) != FH_OK)
+                throw Error(FH_INTERNAL, std::string("non-nested projection: ") + fh_last_error());
+            hold.p = tmp;
+            hipLaunchKernelGGL(k_nn_dummies, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, n, tmp->a.pi.p, nt, m, D.p);
+            FH_CHECK_LAUNCH();
+            stream_wait(st);
+            fh_solve_opts o{1e-12, 100000, 0, FH_PLAIN_FP64};
+            fh_solve_info info{};
+            if (fh_absorber_residualize_dev(tmp, D.p, (size_t)m, &o, nullptr, nullptr, &info, nullptr) != FH_OK)
+                throw Error(FH_INTERNAL, std::string("non-nested projection: ") + fh_last_error());
+            stream_wait(tmp->a.st);
+            wg = tmp->a.weighted ? tmp->a.w_rows.p : nullptr;
+        }
+        if (wg) {
+            hipLaunchKernelGGL(k_scale_rows_sqrt, dim3(grid_for(n * m, TPB, 8192)), dim3(TPB), 0, st, D.p, n, m, wg);
+            FH_CHECK_LAUNCH();
+        }
+        // row-major D (n x m) is column-major D' (m x n): G = D' D
+        blas.gemm(rocblas_operation_none, rocblas_operation_transpose, m, m, (int)n, D.p, m, D.p, m, dG.p, m);
+        download(Gnn.data(), dG.p, Gnn.size() * sizeof(double), st);
+    }
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < i; j++)
+            Gnn[(size_t)i * m + j] = Gnn[(size_t)j * m + i] = 0.5 * (Gnn[(size_t)i * m + j] + Gnn[(size_t)j * m + i]);
+    uint64_t nkeep = 0;
+    const std::vector<double> Gm = grounded_inverse(Gnn, (size_t)m, std::vector<uint8_t>(m, 0), nkeep);
+    // per-cluster sums of the dummies, the rows of the two products
+    DBuf<uint32_t> clr(n);
+    DBuf<double> z(n), dC(mk), dMR(mk), dbread((size_t)mk * mk), dGm((size_t)m * m);
+    DBuf<double2> wu(n);
+    DBuf<int> dkept(mk);
+    upload(dkept.p, in.kept.data(), mk * sizeof(int), st);
+    upload(dC.p, C.data(), mk * sizeof(double), st);
+    upload(dMR.p, MR.data(), mk * sizeof(double), st);
+    upload(dbread.p, in.bread.data(), (size_t)mk * mk * sizeof(double), st);
+    upload(dGm.p, Gm.data(), Gm.size() * sizeof(double), st);
+    hipLaunchKernelGGL(k_pc_rows, dim3(grid_for(n, TPB, 8192)), dim3(TPB), 0, st, in.rows, in.R, dkept.p, mk, dC.p,
+                       dMR.p, in.e, in.es, in.w, in.cl, a.pi.p, n, clr.p, z.p, wu.p);
+    FH_CHECK_LAUNCH();
+    ClusterRuns local;
+    const ClusterRuns& cr = runs_for(a, in.cl, G, local);
+    DBuf<double> WS(G * 3 * m), Pm(G * NC1), Bt(G * p);
+    const size_t nch = (size_t)(m + WAVE - 1) / WAVE;
+    hipLaunchKernelGGL(k_wide_sums, dim3(grid_for(G * nch, TPB / WAVE, 1u << 16)), dim3(TPB), 0, st, cr.off.p,
+                       cr.order.p, cr.run_off.p, G, wu.p, z.p, in.w, a.pi.p, nt, m, WS.p);
+    hipLaunchKernelGGL(k_wide_rows, dim3(grid_for(G, TPB, 4096)), dim3(TPB), 0, st, Sp, WS.p, G, mk, m, dbread.p, dC.p,
+                       Pm.p, Bt.p);
+    FH_CHECK_LAUNCH();
+    // G^- F_t into Pm: column-major (m x G) = G^- (m x m, symmetric) times F_t' (m x G, stride 3m)
+    for (int t = 0; t < 2; t++)
+        blas.gemm(rocblas_operation_none, rocblas_operation_none, m, (int)G, m, dGm.p, m, WS.p + (size_t)t * m, 3 * m,
+                  Pm.p + 2 + (size_t)t * p + mk, NC1);
+    // draws
+    uint64_t reps = spec.reps;
+    bool full = false;
+    if (G <= 20 && (1ull << G) <= reps) {  // weights.rs:39-61
+        reps = 1ull << G;
+        full = true;
+    }
+    require(reps < (1ull << 31), "the non-nested projection with many levels supports < 2^31 draws");
+    DBuf<double> V(reps * G), Y(reps * NC1), Z0(reps * G), Z1(reps * G), coefs(reps * 5);
+    hipLaunchKernelGGL(k_wide_signs, dim3(grid_for(reps * ((G + 63) / 64), TPB, 8192)), dim3(TPB), 0, st,
+                       (uint64_t)spec.seed, (int)full, reps, G, V.p);
+    FH_CHECK_LAUNCH();
+    hipEvent_t e0, e1;
+    FH_HIP(hipEventCreate(&e0));
+    FH_HIP(hipEventCreate(&e1));
+    FH_HIP(hipEventRecord(e0, st));
+    // Y' (NC1 x reps) = Pm' (NC1 x G) V' (G x reps)
+    blas.gemm(rocblas_operation_none, rocblas_operation_none, NC1, (int)reps, (int)G, Pm.p, NC1, V.p, (int)G, Y.p, NC1);
+    // Z_t' (G x reps) = Bt (G x p) d_t' (p x reps), d_t = columns 2 + t p .. of Y
+    blas.gemm(rocblas_operation_transpose, rocblas_operation_none, (int)G, (int)reps, p, Bt.p, p, Y.p + 2, NC1, Z0.p,
+              (int)G);
+    blas.gemm(rocblas_operation_transpose, rocblas_operation_none, (int)G, (int)reps, p, Bt.p, p, Y.p + 2 + p, NC1,
+              Z1.p, (int)G);
+    FH_HIP(hipEventRecord(e1, st));
+    hipLaunchKernelGGL(k_wide_post, dim3(grid_for(reps, TPB / WAVE, 8192)), dim3(TPB), 0, st, Y.p, NC1, V.p, Pm.p,
+                       Z0.p, Z1.p, G, reps, in.ssc, coefs.p);
+    FH_CHECK_LAUNCH();
+    std::vector<double> hc(reps * 5), hS(G * 3 * mk);
+    FH_HIP(hipMemcpyAsync(hc.data(), coefs.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipMemcpyAsync(hS.data(), Sp, hS.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    FH_HIP(hipStreamSynchronize(st));
+    float gms = 0.f;
+    FH_HIP(hipEventElapsedTime(&gms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    res.contraction_ms = gms;
+    res.contraction_ops = 2.0 * (double)reps * (double)G * (double)NC1 + 4.0 * (double)reps * (double)G * (double)p;
+    double aa = 0.0;
+    for (size_t g = 0; g < G; g++) {
+        double a0 = 0.0;
+        for (int j = 0; j < mk; j++) a0 += C[j] * hS[(g * 3) * mk + j];
+        aa += a0 * a0;
+    }
+    double point = 0.0;
+    for (int j = 0; j < mk; j++) point += rk[j] * in.beta[j];
+    wb_infer(hc, reps, full, G, point, in.ssc * aa, in.ssc, spec, res, t_boot_host, t0);
 }
 
 // ------------------------------------------------------------------ MNW 13 / 31 / 33

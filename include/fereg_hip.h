@@ -312,12 +312,12 @@ int fh_ppml_fit_multi_dev(fh_absorber* ab, const double* y, const double* X, siz
  * rng.rs SplitMix64; mod.rs:529-749 p-value and test-inverted interval). The B x G sign by
  * cluster-score contraction runs on int8 MFMA with the scores split into 7 signed 8-bit limbs
  * (exact int32 accumulation, FP64-equivalent). Fixed effects not nested in the clusters
- * (wildboot/mod.rs:959-997) are projected out of the draws exactly: as appended pseudo-regressors
- * (Path 1: m_nn <= min(G, 512) levels and kept + m_nn <= 19) or one cluster at a time (Path 2's
- * projection: G <= 512 and more levels, or Path 1's levels with G <= 2048; one FP64
- * residualization per cluster at tolerance min(o->tol, 1e-10)). Path 1's levels beyond 2048
- * clusters that do not fit beside the regressors: FH_INVALID. More than 512 levels and clusters:
- * not projected (the reference's "TooLarge" plan), result flag set. */
+ * (wildboot/mod.rs:959-997) are projected out of the draws exactly. Path 1 (m_nn <= min(G, 512)
+ * levels): as appended pseudo-regressors when kept + m_nn <= 19, else by Path 1's algebra with
+ * the draws as FP64 matrix products (rocBLAS, atomics off). Path 2 (G <= 512 and more levels):
+ * one cluster at a time, one FP64 residualization per cluster at tolerance min(o->tol, 1e-10).
+ * More than 512 levels and clusters: not projected (the reference's "TooLarge" plan), result
+ * flag set. */
 #define FH_PVAL_TWO_TAILED 0
 #define FH_PVAL_EQUAL_TAILED 1
 #define FH_PVAL_GREATER 2

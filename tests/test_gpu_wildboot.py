@@ -156,8 +156,8 @@ def test_wcr11_per_cluster_projection(with_firm, weighted):
 
 
 def test_wcr11_per_cluster_projection_many_levels():
-    """Path 1's case with more non-nested levels (30) than fit beside the regressors: the same
-    exact projection taken one cluster at a time, against the oracle's Path 1 and the refits."""
+    """Path 1 with more non-nested levels (30) than fit beside the regressors (60 clusters): Path
+    1's algebra with the draws as FP64 products, against the oracle's Path 1 and the refits."""
     import fereg_amd as F
     G, T, reps, r = 60, 30, 499, 0.0
     R = np.array([1.0, -1.0, 0.0])
@@ -209,19 +209,57 @@ def test_wcr11_per_cluster_projection_at_scale():
     check(res, ref, rtol=1e-6)
 
 
-def test_wcr11_projection_beyond_both_forms_is_refused():
-    """More than 2048 clusters with 30 crossing levels beside the regressors: neither the appended
-    form (kept + levels + 2 > 21) nor the per-cluster one (G > 2048) is built for it."""
+@pytest.mark.parametrize("weighted", [False, True])
+def test_wcr11_many_levels_many_clusters(weighted):
+    """Path 1 with 2100 clusters and 30 crossing levels (more than fit beside the regressors):
+    Path 1's algebra with the draws as FP64 products -- against the oracle's Path 1 and the
+    brute-force refits."""
     import fereg_amd as F
-    rng = np.random.default_rng(2)
-    n, G, T = 8000, 2100, 30
-    cl = rng.integers(0, G, n).astype(np.uint32)
-    year = rng.integers(0, T, n).astype(np.uint32)
-    X = rng.standard_normal((n, 1))
-    y = X[:, 0] + rng.standard_normal(n)
-    with pytest.raises(F.FeregError) as e:
-        F.wildboot(y, X, cl, [1.0], reps=99, codes=[year], g=[T])
-    assert e.value.rc == 198
+    rng = np.random.default_rng(2 + weighted)
+    n, G, T, reps, r = 8000, 2100, 30, 199, 0.25
+    cl = rng.integers(0, G, n)
+    year = rng.integers(0, T, n)
+    X = rng.standard_normal((n, 2)) + 0.3 * rng.standard_normal(T)[year][:, None]
+    y = X @ np.array([0.5, -0.3]) + rng.standard_normal(T)[year] + rng.standard_normal(n) * (1 + 0.5 * (cl % 3))
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    R = np.array([1.0, 0.0])
+    announce("wcr11 many levels, G 2100", weighted)
+    codes, gs = [year.astype(np.uint32)], [T]
+    ab = F.Absorber(codes, gs, weights=w)
+    pi = ab.row_order()
+    res = F.wildboot(y, X, cl.astype(np.uint32), R, r=r, reps=reps, seed=13, opts=F.SolveOpts(tol=1e-13),
+                     absorber=ab)
+    ab.close()
+    assert not res.fe_not_projected and res.n_clusters == G
+    yo, Xo, clo = y[pi], X[pi], cl[pi]
+    wo = None if w is None else w[pi]
+    Xt, u, beta, bread = fit_fwl(yo, Xo, [year[pi].astype(np.uint32)], gs, wo)
+    dnn, dt = path1_inputs(year[pi], T, clo, G, wo, False)
+    ref = wb.wcr11_result(beta, Xt, u, wo, clo, G, bread, R, r, res.ssc, reps, 13, dnn=dnn, dtilde=dt)
+    check(res, ref)
+    stat, se, tb = wb.brute_t_boot(X, y, w, dummies(year, T), cl, G, R, r, reps, 13, res.ssc)
+    np.testing.assert_allclose(res.t_boot, tb, rtol=1e-7, atol=1e-8)
+
+
+def test_wcr11_many_levels_many_clusters_nested_firm():
+    """The same with a firm term nested in the 2100 firm clusters: the crossing year dummies are
+    residualized on it before their Gram -- against the oracle's Path 1."""
+    import fereg_amd as F
+    G, T, reps, r = 2100, 30, 499, 0.0
+    R = np.array([0.0, 1.0])
+    y, X, cl, year, codes, gs, w = non_nested_design(9000, G, T, 2, True, seed=61)
+    announce("wcr11 many levels, G 2100, nested firm")
+    ab = F.Absorber(codes, gs, weights=w)
+    pi = ab.row_order()
+    res = F.wildboot(y, X, cl.astype(np.uint32), R, r=r, reps=reps, seed=3, opts=F.SolveOpts(tol=1e-13),
+                     absorber=ab)
+    ab.close()
+    assert not res.fe_not_projected
+    yo, Xo, clo, wo = y[pi], X[pi], cl[pi], w[pi]
+    Xt, u, beta, bread = fit_fwl(yo, Xo, [c[pi] for c in codes], gs, wo)
+    dnn, dt = path1_inputs(year[pi], T, clo, G, wo, True)
+    ref = wb.wcr11_result(beta, Xt, u, wo, clo, G, bread, R, r, res.ssc, reps, 3, dnn=dnn, dtilde=dt)
+    check(res, ref)
 
 
 @pytest.mark.parametrize("boot_type", [13, 31, 33])
