@@ -30,7 +30,10 @@ namespace {
 
 constexpr unsigned TPB = 256;
 constexpr int LIMBS = 7;
-constexpr int WB_NTC = 19;    // 16-column tiles per GEMM workgroup
+// 16-column tiles per GEMM workgroup: 8 keeps the accumulators small enough for several
+// workgroups per CU, whose waves hide each step's HBM fetch (C3, 294 limb columns: 19 tiles per
+// workgroup 0.97 ms, 8 tiles 0.60 ms; 4, 6, 10 and 12 tiles 0.64-0.68 ms)
+constexpr int WB_NTC = 8;
 constexpr int WB_ROWS = 128;  // draws per GEMM workgroup (4 waves x 2 tiles of 16)
 constexpr int KMAX = 21;
 constexpr size_t PC_GMAX = 512;  // clusters of the per-cluster projection (the reference's Path 2 limit)
@@ -220,7 +223,7 @@ __device__ __forceinline__ v4i sign_bytes(uint32_t bits16) {
 }
 
 // part[split][b][col] = sum over this split's g of V[b][g] Pq[col][g] (int32, exact).
-// Workgroup: 128 draws (4 waves x 2 row tiles) x up to 19 column tiles; each 64-g step stages
+// Workgroup: 128 draws (4 waves x 2 row tiles) x up to WB_NTC column tiles; each 64-g step stages
 // the column slice in LDS (80-byte rows: conflict-free 16-byte reads) and generates the
 // signs of its draws from SplitMix64 (word s of draw b covers g in [64 s, 64 s + 64)).
 __global__ __launch_bounds__(TPB) void k_wb_gemm(const int8_t* __restrict__ Pq, size_t Gpad, int ntiles,
@@ -1014,8 +1017,8 @@ void wb_draws(Absorber& a, const double* P, size_t G, int NC, int mk, const std:
     const uint32_t nsteps = (uint32_t)(Gpad / 64);
     const size_t bblocks = (reps + WB_ROWS - 1) / WB_ROWS, Bpad = bblocks * WB_ROWS;
     const int cgroups = (ntiles + WB_NTC - 1) / WB_NTC;
-    // split the G range so the grid fills the chip (~4 workgroups per CU)
-    uint32_t nsplit = (uint32_t)std::max<size_t>(1, std::min<size_t>(nsteps, (1024 + bblocks * cgroups - 1) / (bblocks * cgroups)));
+    // split the G range so the grid fills the chip (~8 workgroups per CU; 4 per CU: 0.67 ms at C3)
+    uint32_t nsplit = (uint32_t)std::max<size_t>(1, std::min<size_t>(nsteps, (2048 + bblocks * cgroups - 1) / (bblocks * cgroups)));
     const uint32_t per = (nsteps + nsplit - 1) / nsplit;
     nsplit = (nsteps + per - 1) / per;
     DBuf<int32_t> part((size_t)nsplit * Bpad * ncolp);
