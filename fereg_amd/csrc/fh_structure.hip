@@ -135,7 +135,11 @@ __global__ void k_iota(uint32_t* __restrict__ v, size_t n) {
 }
 
 // Row i starts a cell when its (FE1, block codes) tuple differs from row i-1's.
-__global__ void k_cell_flags(CodeSet cs, const uint32_t* __restrict__ pi, size_t n, uint32_t* __restrict__ flags) {
+// flags[i] = 1 where sorted row i starts a cell; part[blockIdx.x] = the workgroup's count of them
+// (their sum is the cell count M; M = n means every row is its own cell).
+__global__ __launch_bounds__(256) void k_cell_flags(CodeSet cs, const uint32_t* __restrict__ pi, size_t n,
+                                                    uint32_t* __restrict__ flags, uint32_t* __restrict__ part) {
+    uint32_t cnt = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         uint32_t f = 1;
         if (i > 0) {
@@ -144,6 +148,17 @@ __global__ void k_cell_flags(CodeSet cs, const uint32_t* __restrict__ pi, size_t
             for (int q = 0; q < cs.nq; q++) f |= cs.c[q][o] != cs.c[q][p];
         }
         flags[i] = f;
+        cnt += f;
+    }
+    __shared__ uint32_t red[256 / WAVE];
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, o, WAVE);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (unsigned w = 0; w < blockDim.x / WAVE; w++) t += red[w];
+        part[blockIdx.x] = t;
     }
 }
 
@@ -153,13 +168,13 @@ __global__ void k_cell_start(const uint32_t* __restrict__ flags, const uint32_t*
         if (flags[i]) start[scan[i]] = (uint32_t)i;
 }
 
-// Per cell: FE1 level and block codes of its first row.
+// Per cell: FE1 level and block codes of its first row (row c when start is null).
 __global__ void k_cell_codes(CodeSet cs, int has_fe1, const uint32_t* __restrict__ start,
                              const uint32_t* __restrict__ pi, size_t M, uint32_t* __restrict__ cell_seg,
                              uint32_t* __restrict__ cell_codes) {
     const int nb = cs.nq - 1;
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < M; c += (size_t)gridDim.x * blockDim.x) {
-        uint32_t o = pi[start[c]];
+        uint32_t o = pi[start ? start[c] : (uint32_t)c];  // no start: every row its own cell
         cell_seg[c] = has_fe1 ? cs.c[0][o] : 0u;
         for (int b = 0; b < nb; b++) cell_codes[c * nb + b] = cs.c[1 + b][o];
     }
@@ -575,19 +590,35 @@ void build_structure(Absorber& a, const std::vector<const uint32_t*>& dc) {
         all.c[all.nq++] = zeros.p;
     }
     for (auto& b : a.blocks) all.c[all.nq++] = dc[b.term];
-    DBuf<uint32_t> flags(std::max<size_t>(n, 1)), scan(std::max<size_t>(n, 1)), mdev(1);
-    hipLaunchKernelGGL(k_cell_flags, dim3(blocks_for(n)), dim3(TPB), 0, st, all, a.pi.p, n, flags.p);
-    FH_CHECK_LAUNCH();
-    exclusive_scan_u32(flags.p, scan.p, n, mdev.p, st);
-    a.M = n ? read_scalar(mdev.p, st) : 0;
+    // the cell count from per-workgroup flag counts; when every row is its own cell (distinct key
+    // tuples, C2: M = n = 1e7) the scan and the start pass are skipped (cell c = sorted row c)
+    DBuf<uint32_t> flags(std::max<size_t>(n, 1));
+    const unsigned fgrid = n ? blocks_for(n) : 0;
+    DBuf<uint32_t> fpart(std::max<unsigned>(fgrid, 1));
+    if (n) {
+        hipLaunchKernelGGL(k_cell_flags, dim3(fgrid), dim3(TPB), 0, st, all, a.pi.p, n, flags.p, fpart.p);
+        FH_CHECK_LAUNCH();
+        std::vector<uint32_t> hp(fgrid);
+        download(hp.data(), fpart.p, fgrid * sizeof(uint32_t), st);
+        uint64_t tot = 0;
+        for (uint32_t v : hp) tot += v;
+        a.M = tot;
+    } else {
+        a.M = 0;
+    }
     const size_t M = a.M;
-    DBuf<uint32_t> start(M + 1);
-    hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(n)), dim3(TPB), 0, st, flags.p, scan.p, n, start.p);
-    FH_CHECK_LAUNCH();
+    DBuf<uint32_t> start;
+    if (M != n) {
+        DBuf<uint32_t> scan(n);
+        start.alloc(M + 1);
+        exclusive_scan_u32(flags.p, scan.p, n, nullptr, st);
+        hipLaunchKernelGGL(k_cell_start, dim3(blocks_for(n)), dim3(TPB), 0, st, flags.p, scan.p, n, start.p);
+        FH_CHECK_LAUNCH();
+    }
     a.cell_seg.alloc(std::max<size_t>(M, 1));
     a.cell_codes.alloc(std::max<size_t>(M * nb, 1));
-    hipLaunchKernelGGL(k_cell_codes, dim3(blocks_for(M)), dim3(TPB), 0, st, all, (int)(a.fe1 >= 0), start.p, a.pi.p,
-                       M, a.cell_seg.p, a.cell_codes.p);
+    hipLaunchKernelGGL(k_cell_codes, dim3(blocks_for(M)), dim3(TPB), 0, st, all, (int)(a.fe1 >= 0),
+                       M != n ? (const uint32_t*)start.p : nullptr, a.pi.p, M, a.cell_seg.p, a.cell_codes.p);
     FH_CHECK_LAUNCH();
     a.seg_off.alloc(a.g1 + 1);
     FH_HIP(hipMemsetAsync(a.seg_off.p, 0, a.seg_off.bytes(), st));
